@@ -1,0 +1,165 @@
+"""Benchmark of the coupled FP64 RK4 hot path on one MI355X.
+
+Default workload: ferromagnet LLG + RK4 with exchange, cubic anisotropy,
+uniform bias and zero-padding-free FFT magnetostatics on a 256x256x128 grid
+(YIG-like parameters, dt 1e-14 s), state resident in HBM.  A step is one
+full RK4 step (4 fused stage kernels + 4 magnetostatic convolutions).
+
+Prints one JSON line: throughput in cell-RK4-steps/s, the roofline of the
+dominant kernel from HIP events recorded around its launches on the
+library's stream, the whole-step fraction of the HBM roofline at the
+algorithmic bytes per cell-step, and the CPU oracle timed on a bounded
+sample of the same physics.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "go-ferro_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "coupled RK4 cell-steps/s + % HBM roofline; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(cls: str, n: int, nh: int) -> float:
+    """Algorithmic bytes of one launch of a kernel class (each input read
+    once, each output written once).  n cells, nh = nx*ny*(nz/2+1) complex
+    points per component."""
+    c16 = 16.0
+    if cls == "mag_stage":
+        return 130.0 * n          # average over the four stages, see bytes table
+    if cls == "fft_z_fwd":
+        return 28.0 * n + 3 * nh * c16
+    if cls in ("fft_y_fwd", "fft_y_inv"):
+        return 6 * nh * c16
+    if cls == "fft_x_conv":
+        return 15 * nh * c16
+    if cls == "fft_z_inv":
+        return 3 * nh * c16 + 24.0 * n
+    return 0.0
+
+
+def b_alg_fm(shape) -> float:
+    """Algorithmic FP64 bytes per cell-step of FM LLG + FFT magnetostatics:
+    fused-RK4 state traffic 136*3, magnetostatic field reads 4*24, material
+    id 4*4, and per convolution 24 in + 24 out + 4 id + 6 real tensor
+    spectra at 48*(nz/2+1)/nz."""
+    nz = shape[2]
+    conv = 24 + 24 + 4 + 48.0 * (nz // 2 + 1) / nz
+    return 136 * 3 + 4 * 24 + 4 * 4 + 4 * conv
+
+
+def cpu_baseline(seconds_hint: float = 15.0) -> dict:
+    """The oracle (single-threaded C port of the reference algorithm) on a
+    bounded sample of the same physics."""
+    import numpy as np  # noqa: F401
+
+    from goferro import FIELD_M, workloads
+    from oracle.oracle import Oracle
+
+    shape = (128, 128, 32)
+    prob, m = workloads.fm_film(shape)
+    o = Oracle(prob)
+    o.set(FIELD_M, m)
+    o.demag()
+    o.step(1)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds_hint or steps >= 50:
+            break
+    n = shape[0] * shape[1] * shape[2]
+    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FM LLG + FFT magnetostatics {shape[0]}x{shape[1]}x{shape[2]}, {steps} RK4 steps, "
+                      f"{el:.1f} s, oracle/gf_oracle.c single thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", default="256x256x128")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import goferro
+    from goferro import FIELD_M, workloads
+
+    shape = tuple(int(v) for v in args.grid.split("x"))
+    n = shape[0] * shape[1] * shape[2]
+    nh = shape[0] * shape[1] * (shape[2] // 2 + 1)
+    prob, m = workloads.fm_film(shape)
+    t_setup = time.perf_counter()
+    ctx = goferro.Context(prob)
+    ctx.upload(FIELD_M, m)
+    ctx.demag()
+    ctx.sync()
+    t_setup = time.perf_counter() - t_setup
+    print(f"bench: setup {t_setup:.1f} s for {args.grid}", file=sys.stderr, flush=True)
+
+    ctx.step(args.warmup)
+    ctx.sync()
+    timing = not args.no_kernel_timing
+    ctx.set_kernel_timing(timing)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    kt = ctx.kernel_times() if timing else {}
+    ctx.set_kernel_timing(False)
+    print(f"bench: {args.steps} steps in {el:.3f} s", file=sys.stderr, flush=True)
+
+    value = n * args.steps / el
+    balg = b_alg_fm(shape)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FM LLG+RK4, exchange + cubic anisotropy + bias + FFT magnetostatics, "
+                               f"{args.grid}, dx 2 nm, YIG-like, dt 1e-14 s",
+                   "grid": list(shape), "cells": n},
+        "step_roofline": {"bytes_per_cell_step": balg,
+                          "achieved_GBs": balg * value / 1e9,
+                          "frac": balg * value / (HBM_PEAK_GBS * 1e9)},
+    }
+    if kt:
+        per = {}
+        for cls, (ms, cnt) in kt.items():
+            if cnt:
+                b = kernel_bytes(cls, n, nh)
+                per[cls] = {"launches": cnt, "avg_us": 1e3 * ms / cnt,
+                            "share": ms / max(1e-12, sum(v[0] for v in kt.values())),
+                            "achieved_GBs": (b / (ms / cnt * 1e-3)) / 1e9 if b else None}
+        dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
+        d = per[dom]
+        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                           "bytes_per_launch": kernel_bytes(dom, n, nh), "avg_launch_us": d["avg_us"]}
+        out["kernels"] = per
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

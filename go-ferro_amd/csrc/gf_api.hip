@@ -1,0 +1,620 @@
+// gf_api.hip — the C ABI (include/goferro.h): context set-up, transfers,
+// the RK4 protocol and the drives.  Host side of the gfx950 library.
+//
+// Set-up mirrors magnetic_system::initialize_host / initialize_host_supple /
+// initialize_device (magnetic_initialize.cpp:4-1129): interface masks become
+// bits of a per-cell code, per-material constants are hoisted into a device
+// table, the Newell tensor spectra are built once.  The step protocol
+// mirrors main.cpp:243-483 for the magnetic physics.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gf_internal.h"
+
+namespace gf {
+
+gf_status set_error(gf_ctx* c, gf_status code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+gf_status hip_check(gf_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return GF_OK;
+    return set_error(c, GF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static hipEvent_t take_event(gf_ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void timer_begin(gf_ctx* c, int cls, hipEvent_t* ev) {
+    (void)cls;
+    *ev = nullptr;
+    if (!c->timing) return;
+    *ev = take_event(c);
+    if (*ev) hipEventRecord(*ev, c->stream);
+}
+
+void timer_end(gf_ctx* c, int cls, hipEvent_t start) {
+    if (!c->timing || !start) return;
+    hipEvent_t stop = take_event(c);
+    if (!stop) return;
+    hipEventRecord(stop, c->stream);
+    c->timers.push_back(TimerSlot{start, stop, cls});
+}
+
+static void drain_timers(gf_ctx* c) {
+    if (c->timers.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& t : c->timers) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+            c->kernel_ms[t.cls] += ms;
+            c->kernel_launches[t.cls] += 1;
+        }
+        c->event_pool.push_back(t.start);
+        c->event_pool.push_back(t.stop);
+    }
+    c->timers.clear();
+}
+
+}  // namespace gf
+
+using namespace gf;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+bool is_fm(const gf_ctx* c, int64_t id) {
+    const uint32_t t = c->host_code[id] & kMatMask;
+    return t && c->host_mats[t - 1].if_FM;
+}
+bool is_afm(const gf_ctx* c, int64_t id) {
+    const uint32_t t = c->host_code[id] & kMatMask;
+    return t && c->host_mats[t - 1].if_AFM;
+}
+
+// Face masks of one magnetic kind as code bits (magnetic_initialize.cpp:
+// 136-379 for FM, 635-~870 for AFM): a face is a surface when exactly one
+// side is of the kind, or, at a non-periodic wall, when the cell is.
+void build_surface_bits(gf_ctx* c, bool (*kind)(const gf_ctx*, int64_t), int shift) {
+    const int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    auto id = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k < nz; k++) {
+                const int64_t me = id(i, j, k);
+                if (!kind(c, me)) continue;
+                uint32_t bits = 0;
+                // neighbour on each side, or the wall rule
+                const bool per[3] = {c->per[0] != 0, c->per[1] != 0, c->per[2] != 0};
+                const int64_t lo[3] = {i, j, k};
+                const int64_t n3[3] = {nx, ny, nz};
+                for (int ax = 0; ax < 3; ax++) {
+                    for (int side = 0; side < 2; side++) {   // 0 = backward, 1 = forward
+                        int64_t q[3] = {lo[0], lo[1], lo[2]};
+                        const bool at_wall = side == 0 ? lo[ax] == 0 : lo[ax] == n3[ax] - 1;
+                        bool surf;
+                        if (at_wall && !per[ax]) {
+                            surf = true;   // wall face next to a cell of the kind
+                        } else {
+                            q[ax] = side == 0 ? (lo[ax] == 0 ? n3[ax] - 1 : lo[ax] - 1)
+                                              : (lo[ax] == n3[ax] - 1 ? 0 : lo[ax] + 1);
+                            surf = !kind(c, id(q[0], q[1], q[2]));
+                        }
+                        if (surf) bits |= 1u << (shift + 2 * ax + side);
+                    }
+                }
+                c->host_code[me] |= bits;
+            }
+}
+
+DevMaterial make_dev_material(const gf_material& m, double dt) {
+    DevMaterial d{};
+    d.is_fm = m.if_FM ? 1 : 0;
+    d.is_afm = m.if_AFM ? 1 : 0;
+    d.anis_type = m.anisotropy_type;
+    d.Ms = m.Ms;
+    if (m.if_FM) {
+        const double Ms = m.Ms;
+        d.K1n = m.K1 / kMu0 / Ms;
+        d.K2n = m.K2 / kMu0 / Ms;
+        d.B1n = m.B1 / kMu0 / Ms;
+        d.B2n = m.B2 / kMu0 / Ms;
+        d.Aex = m.Aex;
+        d.Aex_self = 2. * m.Aex * m.Aex / (m.Aex + m.Aex);
+        d.dmi_slope = m.iDMI / 2. / m.Aex;
+        d.exch = 2. / kMu0 / Ms;
+        d.dmi_pos = 2. * m.iDMI / kMu0 / Ms;
+        d.dmi_neg = -2. * m.iDMI / kMu0 / Ms;
+        d.precess = -m.gyro / (1 + m.FM_damping * m.FM_damping) * dt;
+        d.alpha_precess = m.FM_damping * d.precess;
+    } else {
+        d.Aex = m.Aex;
+    }
+    const double ms[2] = {m.Ms_AFM1, m.Ms_AFM2};
+    const double k1[2] = {m.K1_AFM1, m.K1_AFM2};
+    const double b1[2] = {m.B1_AFM1, m.B1_AFM2};
+    const double b2[2] = {m.B2_AFM1, m.B2_AFM2};
+    const double ae[2] = {m.Aex_AFM1, m.Aex_AFM2};
+    const double gy[2] = {m.gyro_AFM1, m.gyro_AFM2};
+    const double al[2] = {m.damping_AFM1, m.damping_AFM2};
+    for (int s = 0; s < 2; s++) {
+        d.aMs[s] = ms[s];
+        d.aAex[s] = ae[s];
+        if (!m.if_AFM) continue;
+        d.aK1n[s] = k1[s] / kMu0 / ms[s];
+        d.aB1n[s] = b1[s] / kMu0 / ms[s];
+        d.aB2n[s] = b2[s] / kMu0 / ms[s];
+        d.aJ[s] = -1. * m.J_AFM / kMu0 / ms[s];
+        d.aAex_self[s] = 2. * ae[s] * ae[s] / (ae[s] + ae[s]);
+        d.aExch[s] = 2. / kMu0 / ms[s];
+        d.aPrecess[s] = -gy[s] / (1. + al[s] * al[s]) * dt;
+        d.aAlphaPrecess[s] = al[s] * d.aPrecess[s];
+    }
+    for (int a = 0; a < 3; a++) d.u[a] = m.uniaxis_AFM[a];
+    return d;
+}
+
+gf_status alloc3(gf_ctx* c, double* p[3], const char* what) {
+    for (int a = 0; a < 3; a++) {
+        gf_status st = hip_check(c, hipMalloc(&p[a], sizeof(double) * c->n), what);
+        if (st) return st;
+        st = hip_check(c, hipMemsetAsync(p[a], 0, sizeof(double) * c->n, c->stream), what);
+        if (st) return st;
+    }
+    return GF_OK;
+}
+
+void free3(double* p[3]) {
+    for (int a = 0; a < 3; a++) {
+        if (p[a]) hipFree(p[a]);
+        p[a] = nullptr;
+    }
+}
+
+// magnetic_solver.cpp:1834-1861 (drive time per stage)
+void stage_drive(gf_ctx* c, int stage, double out[3]) {
+    if (c->flags & GF_FLAG_PRESCRIBE_HEXT) {
+        for (int a = 0; a < 3; a++) out[a] = c->hext[a];
+        return;
+    }
+    const double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    for (int a = 0; a < 3; a++) out[a] = c->hext_stat[a] + c->hext_altr[a] * std::sin(2. * kPi * c->h_altr_freq[a] * t);
+}
+
+double** fm_store(gf_ctx* c, int sel) { return sel == 0 ? c->m : (sel == 1 ? c->st_a : c->st_b); }
+double** afm1_store(gf_ctx* c, int sel) { return sel == 0 ? c->m1 : (sel == 1 ? c->st1_a : c->st1_b); }
+double** afm2_store(gf_ctx* c, int sel) { return sel == 0 ? c->m2 : (sel == 1 ? c->st2_a : c->st2_b); }
+
+// statics after an update phase (main.cpp:299-306 and the 1D field inside
+// update_m*, magnetic_solver.cpp:1533-1538, magnetic_AFM_solver.cpp:1746-1751)
+gf_status statics_after_update(gf_ctx* c) {
+    const bool em = c->flags & GF_FLAG_EMDYNAMIC;
+    const bool ms = c->flags & GF_FLAG_MAGNETOSTATICS;
+    const bool one_d = c->flags & GF_FLAG_MAG_1D;
+    if (em) return GF_OK;
+    if (one_d) {
+        if ((c->any_fm && ms) || c->any_afm) return gf_demag(c);
+        return GF_OK;
+    }
+    if ((c->any_fm || c->any_afm) && ms) return gf_demag(c);
+    return GF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gf_last_error(const gf_ctx* ctx) {
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+gf_status gf_create(const gf_desc* d, gf_ctx** out) {
+    g_create_error.clear();
+    if (!out) { g_create_error = "out is null"; return GF_ERR_ARG; }
+    *out = nullptr;
+    if (!d) { g_create_error = "desc is null"; return GF_ERR_ARG; }
+    if (d->struct_size != sizeof(gf_desc) || d->abi_version != GF_ABI_VERSION) {
+        g_create_error = "gf_desc size/version mismatch (header and library differ)";
+        return GF_ERR_ARG;
+    }
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1 || !(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0)) {
+        g_create_error = "grid sizes and spacings must be positive";
+        return GF_ERR_ARG;
+    }
+    if (d->n_materials < 0 || (d->n_materials > 0 && !d->materials) || !d->material_cell) {
+        g_create_error = "materials / material_cell missing";
+        return GF_ERR_ARG;
+    }
+    if (d->n_materials > static_cast<int32_t>(kMatMask)) {
+        g_create_error = "too many materials";
+        return GF_ERR_ARG;
+    }
+    if ((d->flags & GF_FLAG_PRESCRIBE_HEXT) && d->n_prescribe_Hext > 0 && !d->prescribe_Hext) {
+        g_create_error = "prescribed drive table missing";
+        return GF_ERR_ARG;
+    }
+    gf_ctx* c = nullptr;
+    try {
+        c = new gf_ctx();
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host allocation failed";
+        return GF_ERR_ALLOC;
+    }
+    gf_status st = GF_OK;
+    try {
+        c->nx = d->nx; c->ny = d->ny; c->nz = d->nz; c->n = d->nx * d->ny * d->nz;
+        c->dx = d->dx; c->dy = d->dy; c->dz = d->dz;
+        for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
+        c->flags = d->flags; c->dt = d->dt; c->time = 0.;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) { c->Tg2c[a][b] = d->Tg2c[a][b]; c->Tc2g[b][a] = d->Tg2c[a][b]; }
+        for (int a = 0; a < 3; a++) {
+            c->hext_stat[a] = d->Hext_stat[a]; c->hext_altr[a] = d->Hext_altr[a];
+            c->h_altr_freq[a] = d->H_altr_freq[a]; c->demag_fac[a] = d->demag_fac[a]; c->hext[a] = 0.;
+        }
+        if ((d->flags & GF_FLAG_PRESCRIBE_HEXT) && d->n_prescribe_Hext > 0)
+            c->prescribe.assign(d->prescribe_Hext, d->prescribe_Hext + 4 * d->n_prescribe_Hext);
+        c->host_mats.assign(d->materials, d->materials + d->n_materials);
+        c->host_code.resize(static_cast<size_t>(c->n));
+        for (int64_t i = 0; i < c->n; i++) {
+            const uint32_t t = d->material_cell[i];
+            if (t > static_cast<uint32_t>(d->n_materials)) {
+                c->err = "material_cell id out of range at cell " + std::to_string(i);
+                g_create_error = c->err;
+                delete c;
+                return GF_ERR_ARG;
+            }
+            uint32_t code = t;
+            if (t && d->materials[t - 1].if_FM) { code |= kIsFM; c->nfm++; }
+            if (t && d->materials[t - 1].if_AFM) { code |= kIsAFM; c->nafm++; }
+            c->host_code[static_cast<size_t>(i)] = code;
+        }
+        for (const auto& m : c->host_mats) {
+            if (m.if_FM) c->any_fm = true;
+            if (m.if_AFM) c->any_afm = true;
+        }
+        if (c->any_fm) build_surface_bits(c, is_fm, kFmSurfShift);
+        if (c->any_afm) build_surface_bits(c, is_afm, kAfmSurfShift);
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host allocation failed";
+        delete c;
+        return GF_ERR_ALLOC;
+    }
+
+    st = hip_check(c, hipGetDevice(&c->device), "hipGetDevice");
+    if (!st) st = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!st) st = hip_check(c, hipMalloc(&c->d_code, sizeof(uint32_t) * c->n), "hipMalloc code");
+    if (!st)
+        st = hip_check(c, hipMemcpy(c->d_code, c->host_code.data(), sizeof(uint32_t) * c->n, hipMemcpyHostToDevice),
+                       "copy code");
+    if (!st) {
+        std::vector<DevMaterial> dm;
+        for (const auto& m : c->host_mats) dm.push_back(make_dev_material(m, c->dt));
+        if (dm.empty()) dm.push_back(DevMaterial{});
+        st = hip_check(c, hipMalloc(&c->d_mats, sizeof(DevMaterial) * dm.size()), "hipMalloc materials");
+        if (!st)
+            st = hip_check(c, hipMemcpy(c->d_mats, dm.data(), sizeof(DevMaterial) * dm.size(), hipMemcpyHostToDevice),
+                           "copy materials");
+    }
+    if (!st) st = alloc3(c, c->hstat, "hipMalloc hstat");
+    if (!st && c->any_fm) {
+        st = alloc3(c, c->m, "hipMalloc m");
+        if (!st) st = alloc3(c, c->acc, "hipMalloc acc");
+        if (!st) st = alloc3(c, c->st_a, "hipMalloc store a");
+        if (!st) st = alloc3(c, c->st_b, "hipMalloc store b");
+    }
+    if (!st && c->any_afm) {
+        st = alloc3(c, c->m1, "hipMalloc m1");
+        if (!st) st = alloc3(c, c->m2, "hipMalloc m2");
+        if (!st) st = alloc3(c, c->acc1, "hipMalloc acc1");
+        if (!st) st = alloc3(c, c->acc2, "hipMalloc acc2");
+        if (!st) st = alloc3(c, c->st1_a, "hipMalloc store1 a");
+        if (!st) st = alloc3(c, c->st1_b, "hipMalloc store1 b");
+        if (!st) st = alloc3(c, c->st2_a, "hipMalloc store2 a");
+        if (!st) st = alloc3(c, c->st2_b, "hipMalloc store2 b");
+    }
+    if (!st) {
+        c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
+        st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
+    }
+    if (!st && (c->flags & GF_FLAG_MAGNETOSTATICS) && !(c->flags & GF_FLAG_MAG_1D) &&
+        !(c->flags & GF_FLAG_DEMAG_FACTOR) && (c->any_fm || c->any_afm))
+        st = demag_setup(c);
+    if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "create sync");
+    if (st) {
+        g_create_error = c->err;
+        gf_destroy(c);
+        return st;
+    }
+    *out = c;
+    return GF_OK;
+}
+
+gf_status gf_destroy(gf_ctx* c) {
+    if (!c) return GF_ERR_ARG;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& t : c->timers) { hipEventDestroy(t.start); hipEventDestroy(t.stop); }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    free3(c->m); free3(c->acc); free3(c->st_a); free3(c->st_b); free3(c->dm_last);
+    free3(c->m1); free3(c->m2); free3(c->acc1); free3(c->acc2);
+    free3(c->st1_a); free3(c->st1_b); free3(c->st2_a); free3(c->st2_b);
+    free3(c->hstat); free3(c->hnu);
+    if (c->red_partial) hipFree(c->red_partial);
+    if (c->d_code) hipFree(c->d_code);
+    if (c->d_mats) hipFree(c->d_mats);
+    demag_free(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return GF_OK;
+}
+
+static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
+    *st = GF_OK;
+    switch (id) {
+    case GF_FIELD_M:
+        if (!c->any_fm) break;
+        return c->m;
+    case GF_FIELD_M_STORE:
+        if (!c->any_fm) break;
+        if (for_write && c->mag_store_sel == 0) {
+            // a store that differs from m lives in a free buffer
+            c->mag_store_sel = 2;
+        }
+        return fm_store(c, c->mag_store_sel);
+    case GF_FIELD_HSTAT:
+        return c->hstat;
+    case GF_FIELD_HEXT_NONUNIF:
+        if (for_write && !c->hnu[0]) {
+            if ((*st = alloc3(c, c->hnu, "hipMalloc hext_nonunif"))) return nullptr;
+        }
+        if (!c->hnu[0]) break;
+        return c->hnu;
+    case GF_FIELD_M_AFM1:
+        if (!c->any_afm) break;
+        return c->m1;
+    case GF_FIELD_M_AFM2:
+        if (!c->any_afm) break;
+        return c->m2;
+    default:
+        break;
+    }
+    *st = set_error(c, GF_ERR_ARG, "field " + std::to_string(id) + " is not available in this context");
+    return nullptr;
+}
+
+gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
+    if (!c || !host) return set_error(c, GF_ERR_ARG, "null argument");
+    if (count != static_cast<size_t>(3 * c->n)) return set_error(c, GF_ERR_ARG, "count must be 3*nx*ny*nz");
+    gf_status st;
+    double** f = field_for(c, id, true, &st);
+    if (!f) return st;
+    const size_t bytes = sizeof(double) * c->n;
+    for (int a = 0; a < 3; a++) {
+        st = hip_check(c, hipMemcpyAsync(f[a], host + a * c->n, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+        if (st) return st;
+    }
+    // initialize_device: the stores start equal to m (magnetic_initialize.cpp:1081-1097)
+    auto reset = [&](double** m, double** sa, double** sb) -> gf_status {
+        for (int a = 0; a < 3; a++) {
+            gf_status s2 = hip_check(c, hipMemcpyAsync(sa[a], m[a], bytes, hipMemcpyDeviceToDevice, c->stream), "store");
+            if (s2) return s2;
+            s2 = hip_check(c, hipMemcpyAsync(sb[a], m[a], bytes, hipMemcpyDeviceToDevice, c->stream), "store");
+            if (s2) return s2;
+        }
+        return GF_OK;
+    };
+    if (id == GF_FIELD_M) { st = reset(c->m, c->st_a, c->st_b); c->mag_store_sel = 0; c->mag_next_stage = 1; }
+    if (id == GF_FIELD_M_AFM1) { st = reset(c->m1, c->st1_a, c->st1_b); c->afm_store_sel = 0; c->afm_next_stage = 1; }
+    if (id == GF_FIELD_M_AFM2) { st = reset(c->m2, c->st2_a, c->st2_b); c->afm_store_sel = 0; c->afm_next_stage = 1; }
+    if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
+    if (st) return st;
+    return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
+}
+
+gf_status gf_download(gf_ctx* c, int id, double* host, size_t count) {
+    if (!c || !host) return set_error(c, GF_ERR_ARG, "null argument");
+    if (count != static_cast<size_t>(3 * c->n)) return set_error(c, GF_ERR_ARG, "count must be 3*nx*ny*nz");
+    gf_status st;
+    double** f = field_for(c, id, false, &st);
+    if (!f) return st;
+    for (int a = 0; a < 3; a++) {
+        st = hip_check(c, hipMemcpyAsync(host + a * c->n, f[a], sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream),
+                       "download");
+        if (st) return st;
+    }
+    return hip_check(c, hipStreamSynchronize(c->stream), "download sync");
+}
+
+// global.cpp:513-549 (H drive table; t advances every call)
+gf_status gf_time_advance(gf_ctx* c) {
+    if (!c) return GF_ERR_ARG;
+    c->time = c->time + c->dt;
+    if (c->flags & GF_FLAG_PRESCRIBE_HEXT) {
+        const int np = static_cast<int>(c->prescribe.size() / 4);
+        if (c->prescribe_idx < np && (c->time - c->dt) > c->prescribe[4 * c->prescribe_idx]) c->prescribe_idx++;
+        if (c->prescribe_idx < np) {
+            for (int a = 0; a < 3; a++) c->hext[a] = c->prescribe[4 * c->prescribe_idx + 1 + a];
+        } else {
+            for (int a = 0; a < 3; a++) c->hext[a] = 0.;
+        }
+    }
+    return GF_OK;
+}
+
+gf_status gf_mag_stage(gf_ctx* c, int stage) {
+    if (!c) return GF_ERR_ARG;
+    if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
+    if (!c->any_fm) return set_error(c, GF_ERR_STATE, "no ferromagnetic material in this context");
+    if (stage != c->mag_next_stage)
+        return set_error(c, GF_ERR_STATE, "FM stages must run in the order 1,2,3,4 (expected " +
+                                              std::to_string(c->mag_next_stage) + ")");
+    StageArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** s = fm_store(c, c->mag_store_sel);
+    const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
+    double** o = fm_store(c, out_sel);
+    for (int i = 0; i < 3; i++) {
+        a.store[i] = s[i]; a.base[i] = c->m[i]; a.acc[i] = c->acc[i]; a.out[i] = o[i];
+        a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = nullptr;
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    stage_drive(c, stage, a.hext);
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    a.stage = stage;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_MAG_STAGE, &ev);
+    gf_status st = hip_check(c, launch_mag_stage(a, c->n, c->stream), "mag stage");
+    timer_end(c, GF_K_MAG_STAGE, ev);
+    if (st) return st;
+    c->mag_store_sel = out_sel;
+    c->mag_next_stage = stage == 4 ? 1 : stage + 1;
+    return GF_OK;
+}
+
+gf_status gf_afm_stage(gf_ctx* c, int stage) {
+    if (!c) return GF_ERR_ARG;
+    if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
+    if (!c->any_afm) return set_error(c, GF_ERR_STATE, "no antiferromagnetic material in this context");
+    if (stage != c->afm_next_stage)
+        return set_error(c, GF_ERR_STATE, "AFM stages must run in the order 1,2,3,4");
+    AfmStageArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** s1 = afm1_store(c, c->afm_store_sel);
+    double** s2 = afm2_store(c, c->afm_store_sel);
+    const int out_sel = stage == 4 ? 0 : (c->afm_store_sel == 1 ? 2 : 1);
+    double** o1 = afm1_store(c, out_sel);
+    double** o2 = afm2_store(c, out_sel);
+    for (int i = 0; i < 3; i++) {
+        a.store1[i] = s1[i]; a.store2[i] = s2[i];
+        a.base1[i] = c->m1[i]; a.base2[i] = c->m2[i];
+        a.acc1[i] = c->acc1[i]; a.acc2[i] = c->acc2[i];
+        a.out1[i] = o1[i]; a.out2[i] = o2[i];
+        a.hstat[i] = c->hstat[i];
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    // the AFM path refreshes the sinusoidal drive unconditionally
+    // (magnetic_AFM_solver.cpp:61-64)
+    const double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    for (int i = 0; i < 3; i++) {
+        c->hext[i] = c->hext_stat[i] + c->hext_altr[i] * std::sin(2. * kPi * c->h_altr_freq[i] * t);
+        a.hext[i] = c->hext[i];
+    }
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    a.stage = stage;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_AFM_STAGE, &ev);
+    gf_status st = hip_check(c, launch_afm_stage(a, c->n, c->stream), "afm stage");
+    timer_end(c, GF_K_AFM_STAGE, ev);
+    if (st) return st;
+    c->afm_store_sel = out_sel;
+    c->afm_next_stage = stage == 4 ? 1 : stage + 1;
+    return GF_OK;
+}
+
+gf_status gf_demag(gf_ctx* c) {
+    if (!c) return GF_ERR_ARG;
+    if (!(c->any_fm || c->any_afm)) return GF_OK;
+    double* zero[3] = {nullptr, nullptr, nullptr};
+    double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : zero;
+    double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : zero;
+    double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : zero;
+    if (c->flags & GF_FLAG_MAG_1D) return hip_check(c, launch_demag_local(c, fm, s1, s2, 1), "demag 1d");
+    if (c->flags & GF_FLAG_DEMAG_FACTOR) return hip_check(c, launch_demag_local(c, fm, s1, s2, 0), "demag factor");
+    if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT magnetostatics was not set up (flags)");
+    return demag_run(c, fm, s1, s2);
+}
+
+// main.cpp:243-483 for the magnetic physics
+gf_status gf_step(gf_ctx* c, int64_t nsteps) {
+    if (!c) return GF_ERR_ARG;
+    if (nsteps < 0) return set_error(c, GF_ERR_ARG, "nsteps must be >= 0");
+    if ((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1))
+        return set_error(c, GF_ERR_STATE, "gf_step called in the middle of a step");
+    for (int64_t s = 0; s < nsteps; s++) {
+        gf_status st = gf_time_advance(c);
+        for (int stage = 1; stage <= 4 && !st; stage++) {
+            if (c->any_fm) st = gf_mag_stage(c, stage);
+            if (!st && c->any_afm) st = gf_afm_stage(c, stage);
+            if (!st) st = statics_after_update(c);
+        }
+        if (st) return st;
+    }
+    return GF_OK;
+}
+
+gf_status gf_average(gf_ctx* c, int id, double out[3]) {
+    if (!c || !out) return GF_ERR_ARG;
+    gf_status st;
+    if (id != GF_FIELD_M && id != GF_FIELD_M_AFM1 && id != GF_FIELD_M_AFM2)
+        return set_error(c, GF_ERR_ARG, "average is defined for m, m_AFM1, m_AFM2");
+    double** f = field_for(c, id, false, &st);
+    if (!f) return st;
+    const double* ff[3] = {f[0], f[1], f[2]};
+    st = hip_check(c, launch_sum3(ff, c->n, c->red_partial, c->red_blocks, c->stream), "sum");
+    if (st) return st;
+    std::vector<double> part(static_cast<size_t>(3 * c->red_blocks));
+    st = hip_check(c, hipMemcpyAsync(part.data(), c->red_partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                                     c->stream), "sum copy");
+    if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "sum sync");
+    if (st) return st;
+    const double cnt = static_cast<double>(id == GF_FIELD_M ? c->nfm : c->nafm);
+    for (int a = 0; a < 3; a++) {
+        double s = 0.;
+        for (int64_t b = 0; b < c->red_blocks; b++) s += part[static_cast<size_t>(3 * b + a)];
+        out[a] = s / cnt;
+    }
+    return GF_OK;
+}
+
+gf_status gf_get_time(const gf_ctx* c, double* t) {
+    if (!c || !t) return GF_ERR_ARG;
+    *t = c->time;
+    return GF_OK;
+}
+
+gf_status gf_sync(gf_ctx* c) {
+    if (!c) return GF_ERR_ARG;
+    return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+gf_status gf_get_stream(gf_ctx* c, void** stream) {
+    if (!c || !stream) return GF_ERR_ARG;
+    *stream = c->stream;
+    return GF_OK;
+}
+
+gf_status gf_set_kernel_timing(gf_ctx* c, int enable) {
+    if (!c) return GF_ERR_ARG;
+    drain_timers(c);
+    c->timing = enable != 0;
+    for (int k = 0; k < GF_K_COUNT; k++) { c->kernel_ms[k] = 0.; c->kernel_launches[k] = 0; }
+    return GF_OK;
+}
+
+gf_status gf_kernel_times(gf_ctx* c, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]) {
+    if (!c || !ms || !launches) return GF_ERR_ARG;
+    drain_timers(c);
+    for (int k = 0; k < GF_K_COUNT; k++) { ms[k] = c->kernel_ms[k]; launches[k] = c->kernel_launches[k]; }
+    return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,585 @@
+// gf_fft.hip — FP64 FFT magnetostatic convolution for gfx950.
+//
+// Replaces magnetic_system::get_H_static (magnetostatic_solver.cpp:114-219):
+// M = Ms m (FM) or Ms1 m1 + Ms2 m2 (AFM) -> 3 real-to-complex 3D transforms
+// -> H_k = (sum of three M_k * N_k) / (-dx dy dz) with the nine tensor
+// spectra -> 3 complex-to-real transforms.  Transforms are unnormalised; the
+// kernel is pre-divided by n (as cuFFT D2Z/Z2D with the reference's kernel).
+//
+// Five launches, each a set of 1D transforms held in LDS, each reading and
+// writing global memory in runs of >= 16 consecutive complex values:
+//   zfwd   [x][y][z] reals      -> z transform -> [x][kz][y]     (spec_a)
+//   yfwd   [x][kz][y]           -> y transform -> [kz][y][x]     (spec_b)
+//   xconv  [kz][y][x]           -> x fwd, tensor product, x inv  (spec_b, in place)
+//   yinv   [kz][y][x]           -> y inverse   -> [x][kz][y]     (spec_a)
+//   zinv   [x][kz][y]           -> z inverse (Hermitian) -> H [x][y][z]
+// Each layout puts the next transform's axis or its store run innermost,
+// so the transposes ride on the LDS staging the transforms need anyway.
+// Transforms are radix-2: forward decimation in time (bit-reversed load,
+// natural result), inverse decimation in frequency (natural load,
+// bit-reversed result read back at the store), twiddles staged in LDS.
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "gf_internal.h"
+
+namespace gf {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ int bitrev(int j, int logn) {
+    return logn ? static_cast<int>(__brev(static_cast<unsigned>(j)) >> (32 - logn)) : 0;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// P pencils of length N = 2^logn at buf[p*N], bit-reversed in, natural out.
+__device__ void fft_dit(double2* buf, int pencils, int logn, const double2* tw) {
+    const int N = 1 << logn;
+    const int half = N >> 1;
+    const int total = pencils * half;
+    for (int s = 0; s < logn; s++) {
+        const int h = 1 << s;
+        const int tstep = N >> (s + 1);
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            const int p = b >> (logn - 1);
+            const int q = b & (half - 1);
+            const int j = q & (h - 1);
+            const int i0 = p * N + ((q >> s) << (s + 1)) + j;
+            const int i1 = i0 + h;
+            const double2 w = tw[j * tstep];
+            const double2 u = buf[i0];
+            const double2 v = buf[i1];
+            const double2 t = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+            buf[i0] = make_double2(u.x + t.x, u.y + t.y);
+            buf[i1] = make_double2(u.x - t.x, u.y - t.y);
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse transform (conjugate twiddles), natural in, bit-reversed out.
+__device__ void ifft_dif(double2* buf, int pencils, int logn, const double2* tw) {
+    const int N = 1 << logn;
+    const int half = N >> 1;
+    const int total = pencils * half;
+    for (int s = logn - 1; s >= 0; s--) {
+        const int h = 1 << s;
+        const int tstep = N >> (s + 1);
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            const int p = b >> (logn - 1);
+            const int q = b & (half - 1);
+            const int j = q & (h - 1);
+            const int i0 = p * N + ((q >> s) << (s + 1)) + j;
+            const int i1 = i0 + h;
+            const double2 w = make_double2(tw[j * tstep].x, -tw[j * tstep].y);
+            const double2 u = buf[i0];
+            const double2 v = buf[i1];
+            const double2 d = make_double2(u.x - v.x, u.y - v.y);
+            buf[i0] = make_double2(u.x + v.x, u.y + v.y);
+            buf[i1] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void load_twiddles(double2* dst, const double2* src, int n) {
+    for (int j = threadIdx.x; j < (n >> 1); j += blockDim.x) dst[j] = src[j];
+}
+
+struct ZArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* fm[3];
+    const double* a1[3];
+    const double* a2[3];
+    const double* raw;          // mode 1 input
+    double2* out;               // component c at out + c*nh, layout [x][kz][y]
+    const double2* tw;
+    int64_t nh;
+    int nx, ny, nz, nzh;
+    int logny, lognz, pencils, logp;
+};
+
+// Magnetisation assembly (magnetostatic_solver.cpp:121-145) fused with the
+// z real-to-complex transform.  kRaw = true transforms a plain real array
+// (used once, for the tensor kernels).
+template <bool kRaw>
+__global__ void __launch_bounds__(kThreads) zfwd_kernel(ZArgs a) {
+    extern __shared__ double2 smem[];
+    const int c = blockIdx.z;
+    const int x = blockIdx.y;
+    const int y0 = blockIdx.x * a.pencils;
+    const double* fmc = c == 0 ? a.fm[0] : (c == 1 ? a.fm[1] : a.fm[2]);
+    const double* a1c = c == 0 ? a.a1[0] : (c == 1 ? a.a1[1] : a.a1[2]);
+    const double* a2c = c == 0 ? a.a2[0] : (c == 1 ? a.a2[1] : a.a2[2]);
+    double2* tw = smem;
+    double2* buf = smem + (a.nz >> 1);
+    load_twiddles(tw, a.tw, a.nz);
+    const int total = a.pencils * a.nz;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int r = idx >> a.lognz;
+        const int k = idx & (a.nz - 1);
+        const int y = y0 + r;
+        double v = 0.;
+        if (y < a.ny) {
+            const int64_t cell = (static_cast<int64_t>(x) * a.ny + y) * a.nz + k;
+            if (kRaw) {
+                v = a.raw[cell];
+            } else {
+                const uint32_t code = a.code[cell];
+                if (code & kIsFM) {
+                    v = a.mats[(code & kMatMask) - 1].Ms * fmc[cell];
+                } else if (code & kIsAFM) {
+                    const DevMaterial& M = a.mats[(code & kMatMask) - 1];
+                    v = M.aMs[0] * a1c[cell] + M.aMs[1] * a2c[cell];
+                }
+            }
+        }
+        buf[r * a.nz + bitrev(k, a.lognz)] = make_double2(v, 0.);
+    }
+    __syncthreads();
+    fft_dit(buf, a.pencils, a.lognz, tw);
+    double2* out = a.out + c * a.nh;
+    const int ztotal = a.pencils * a.nzh;
+    for (int idx = threadIdx.x; idx < ztotal; idx += blockDim.x) {
+        const int r = idx & (a.pencils - 1);
+        const int kz = idx >> a.logp;
+        const int y = y0 + r;
+        if (y < a.ny) out[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y] = buf[r * a.nz + kz];
+    }
+}
+
+struct YArgs {
+    const double2* in;
+    double2* out;
+    const double2* tw;
+    int64_t nh;
+    int nx, ny, nzh;
+    int logny, pencils, logp;
+};
+
+// y forward transform: [x][kz][y] -> [kz][y][x]
+__global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    const int c = blockIdx.z;
+    const int kz = blockIdx.y;
+    const int x0 = blockIdx.x * a.pencils;
+    double2* tw = smem;
+    double2* buf = smem + (a.ny >> 1);
+    load_twiddles(tw, a.tw, a.ny);
+    const double2* in = a.in + c * a.nh;
+    const int total = a.pencils * a.ny;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int r = idx >> a.logny;
+        const int y = idx & (a.ny - 1);
+        const int x = x0 + r;
+        double2 v = make_double2(0., 0.);
+        if (x < a.nx) v = in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y];
+        buf[r * a.ny + bitrev(y, a.logny)] = v;
+    }
+    __syncthreads();
+    fft_dit(buf, a.pencils, a.logny, tw);
+    double2* out = a.out + c * a.nh;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int r = idx & (a.pencils - 1);
+        const int y = idx >> a.logp;
+        const int x = x0 + r;
+        if (x < a.nx) out[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[r * a.ny + y];
+    }
+}
+
+// y inverse transform: [kz][y][x] -> [x][kz][y]
+__global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
+    extern __shared__ double2 smem[];
+    const int c = blockIdx.z;
+    const int kz = blockIdx.y;
+    const int x0 = blockIdx.x * a.pencils;
+    double2* tw = smem;
+    double2* buf = smem + (a.ny >> 1);
+    load_twiddles(tw, a.tw, a.ny);
+    const double2* in = a.in + c * a.nh;
+    const int total = a.pencils * a.ny;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int r = idx & (a.pencils - 1);
+        const int y = idx >> a.logp;
+        const int x = x0 + r;
+        double2 v = make_double2(0., 0.);
+        if (x < a.nx) v = in[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x];
+        buf[r * a.ny + y] = v;
+    }
+    __syncthreads();
+    ifft_dif(buf, a.pencils, a.logny, tw);
+    double2* out = a.out + c * a.nh;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int r = idx >> a.logny;
+        const int y = idx & (a.ny - 1);
+        const int x = x0 + r;
+        if (x < a.nx) out[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y] = buf[r * a.ny + bitrev(y, a.logny)];
+    }
+}
+
+struct XArgs {
+    double2* data;              // component c at data + c*nh, layout [kz][y][x]
+    const double2* kspec;       // spectrum s at kspec + s*nh
+    const double2* tw;
+    int64_t nh;
+    double den;                 // -dx*dy*dz
+    int nx, ny, nzh;
+    int lognx, pencils, logp;
+};
+
+// x forward transform, tensor product and x inverse transform, in place.
+// kConv = false: forward transform of component 0 only (tensor set-up).
+template <bool kConv>
+__global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
+    extern __shared__ double2 smem[];
+    const int kz = blockIdx.y;
+    const int y0 = blockIdx.x * a.pencils;
+    const int comps = kConv ? 3 : 1;
+    double2* tw = smem;
+    double2* buf = smem + (a.nx >> 1);
+    load_twiddles(tw, a.tw, a.nx);
+    const int total = comps * a.pencils * a.nx;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int x = idx & (a.nx - 1);
+        const int t = idx >> a.lognx;
+        const int r = t & (a.pencils - 1);
+        const int c = t >> a.logp;
+        const int y = y0 + r;
+        double2 v = make_double2(0., 0.);
+        if (y < a.ny) v = a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x];
+        buf[t * a.nx + bitrev(x, a.lognx)] = v;
+    }
+    __syncthreads();
+    fft_dit(buf, comps * a.pencils, a.lognx, tw);
+    if (kConv) {
+        // magnetostatic_solver.cpp:163-177; spectra order
+        // xx, yy, zz, xy, yz, zx, yx, zy, xz
+        const int pts = a.pencils * a.nx;
+        for (int idx = threadIdx.x; idx < pts; idx += blockDim.x) {
+            const int x = idx & (a.nx - 1);
+            const int r = idx >> a.lognx;
+            const int y = y0 + r;
+            if (y >= a.ny) continue;
+            const int64_t g = (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x;
+            const int ix = (0 * a.pencils + r) * a.nx + x;
+            const int iy = (1 * a.pencils + r) * a.nx + x;
+            const int iz = (2 * a.pencils + r) * a.nx + x;
+            const double2 mx = buf[ix], my = buf[iy], mz = buf[iz];
+            const double2* K = a.kspec + g;
+            const int64_t s = a.nh;
+            double2 t0 = cmul(mx, K[0 * s]), t1 = cmul(my, K[3 * s]), t2 = cmul(mz, K[8 * s]);
+            const double2 hx = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
+            t0 = cmul(my, K[1 * s]); t1 = cmul(mz, K[4 * s]); t2 = cmul(mx, K[6 * s]);
+            const double2 hy = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
+            t0 = cmul(mz, K[2 * s]); t1 = cmul(mx, K[5 * s]); t2 = cmul(my, K[7 * s]);
+            const double2 hz = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
+            buf[ix] = hx; buf[iy] = hy; buf[iz] = hz;
+        }
+        __syncthreads();
+        ifft_dif(buf, comps * a.pencils, a.lognx, tw);
+    }
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int x = idx & (a.nx - 1);
+        const int t = idx >> a.lognx;
+        const int r = t & (a.pencils - 1);
+        const int c = t >> a.logp;
+        const int y = y0 + r;
+        if (y < a.ny) {
+            const int src = kConv ? bitrev(x, a.lognx) : x;
+            a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[t * a.nx + src];
+        }
+    }
+}
+
+struct ZInvArgs {
+    const double2* in;          // layout [x][kz][y]
+    double* out[3];
+    const double2* tw;
+    int64_t nh;
+    int nx, ny, nz, nzh;
+    int lognz, pencils, logp;
+};
+
+// z complex-to-real inverse: Hermitian extension of the half spectrum, the
+// imaginary parts of the self-conjugate bins left as they are (they only
+// reach the discarded imaginary output).
+__global__ void __launch_bounds__(kThreads) zinv_kernel(ZInvArgs a) {
+    extern __shared__ double2 smem[];
+    const int c = blockIdx.z;
+    const int x = blockIdx.y;
+    const int y0 = blockIdx.x * a.pencils;
+    double2* tw = smem;
+    double2* buf = smem + (a.nz >> 1);
+    load_twiddles(tw, a.tw, a.nz);
+    const double2* in = a.in + c * a.nh;
+    const int ztotal = a.pencils * a.nzh;
+    for (int idx = threadIdx.x; idx < ztotal; idx += blockDim.x) {
+        const int r = idx & (a.pencils - 1);
+        const int kz = idx >> a.logp;
+        const int y = y0 + r;
+        double2 v = make_double2(0., 0.);
+        if (y < a.ny) v = in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y];
+        buf[r * a.nz + kz] = v;
+        const int mirror = a.nz - kz;
+        if (kz >= 1 && mirror > kz && mirror < a.nz) buf[r * a.nz + mirror] = make_double2(v.x, -v.y);
+    }
+    __syncthreads();
+    ifft_dif(buf, a.pencils, a.lognz, tw);
+    double* out = c == 0 ? a.out[0] : (c == 1 ? a.out[1] : a.out[2]);
+    const int total = a.pencils * a.nz;
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int r = idx >> a.lognz;
+        const int k = idx & (a.nz - 1);
+        const int y = y0 + r;
+        if (y < a.ny) out[(static_cast<int64_t>(x) * a.ny + y) * a.nz + k] = buf[r * a.nz + bitrev(k, a.lognz)].x;
+    }
+}
+
+struct LocalArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* fm[3];
+    const double* a1[3];
+    const double* a2[3];
+    double* h[3];
+    double fac[3];
+    int64_t n;
+    int one_d;
+};
+
+// Demag-factor field (magnetostatic_solver.cpp:192-218) or the 1D model
+// Hz = -Ms mz (:222-249); cells that are neither FM nor AFM keep H.
+__global__ void __launch_bounds__(kThreads) demag_local_kernel(LocalArgs a) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t code = a.code[i];
+    if (code & kIsFM) {
+        const double Ms = a.mats[(code & kMatMask) - 1].Ms;
+        if (a.one_d) {
+            a.h[0][i] = 0.; a.h[1][i] = 0.; a.h[2][i] = -Ms * a.fm[2][i];
+        } else {
+            for (int c = 0; c < 3; c++) a.h[c][i] = Ms * a.fm[c][i] * a.fac[c];
+        }
+    } else if (code & kIsAFM) {
+        const DevMaterial& M = a.mats[(code & kMatMask) - 1];
+        if (a.one_d) {
+            a.h[0][i] = 0.; a.h[1][i] = 0.; a.h[2][i] = -M.aMs[0] * a.a1[2][i] - M.aMs[1] * a.a2[2][i];
+        } else {
+            for (int c = 0; c < 3; c++) a.h[c][i] = (M.aMs[0] * a.a1[c][i] + M.aMs[1] * a.a2[c][i]) * a.fac[c];
+        }
+    }
+}
+
+int ilog2(int64_t n) {
+    int l = 0;
+    while ((int64_t{1} << l) < n) l++;
+    return l;
+}
+
+int pick_pencils(int64_t len, int64_t budget, int64_t cap, int64_t other) {
+    int64_t p = budget / len;
+    if (p < 1) p = 1;
+    if (p > cap) p = cap;
+    int64_t pow = 1;
+    while (pow * 2 <= p) pow *= 2;
+    while (pow > 1 && pow > other) pow /= 2;
+    return static_cast<int>(pow);
+}
+
+gf_status make_twiddles(gf_ctx* c, Twiddles& t, int64_t n) {
+    t.n = n;
+    const int64_t cnt = n / 2 > 0 ? n / 2 : 1;
+    std::vector<double2> h(static_cast<size_t>(cnt));
+    for (int64_t j = 0; j < cnt; j++) {
+        // same angle expression as a radix-2 stage of span n
+        const double ang = 2.0 * kPi * static_cast<double>(j) / static_cast<double>(n);
+        h[static_cast<size_t>(j)] = make_double2(std::cos(ang), -std::sin(ang));
+    }
+    gf_status st = hip_check(c, hipMalloc(&t.dev, sizeof(double2) * cnt), "hipMalloc twiddles");
+    if (st) return st;
+    return hip_check(c, hipMemcpy(t.dev, h.data(), sizeof(double2) * cnt, hipMemcpyHostToDevice),
+                     "copy twiddles");
+}
+
+struct Shapes {
+    int pz, py, px;
+    size_t lds_z, lds_y, lds_x;
+};
+
+Shapes shapes(const gf_ctx* c) {
+    Shapes s;
+    s.pz = pick_pencils(c->nz, 2048, 16, c->ny);
+    s.py = pick_pencils(c->ny, 2048, 16, c->nx);
+    s.px = pick_pencils(c->nx, 1024, 8, c->ny);
+    s.lds_z = sizeof(double2) * (c->nz / 2 + s.pz * c->nz);
+    s.lds_y = sizeof(double2) * (c->ny / 2 + s.py * c->ny);
+    s.lds_x = sizeof(double2) * (c->nx / 2 + 3 * s.px * c->nx);
+    return s;
+}
+
+unsigned blocks_for(int64_t n, int p) { return static_cast<unsigned>((n + p - 1) / p); }
+
+}  // namespace
+
+// Forward pipeline helpers shared by the tensor set-up and the convolution.
+static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* rawp, const double* const* fm, const double* const* s1,
+                           const double* const* s2, int comps) {
+    const Shapes sh = shapes(c);
+    ZArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    for (int i = 0; i < 3; i++) {
+        a.fm[i] = fm ? fm[i] : nullptr;
+        a.a1[i] = s1 ? s1[i] : nullptr;
+        a.a2[i] = s2 ? s2[i] : nullptr;
+    }
+    a.raw = rawp; a.out = c->demag.spec_a; a.tw = c->demag.twz.dev;
+    a.nh = c->nx * c->ny * c->demag.nzh;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
+    a.nzh = static_cast<int>(c->demag.nzh);
+    a.logny = ilog2(c->ny); a.lognz = ilog2(c->nz); a.pencils = sh.pz; a.logp = ilog2(sh.pz);
+    dim3 grid(blocks_for(c->ny, sh.pz), static_cast<unsigned>(c->nx), static_cast<unsigned>(comps));
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_FFT_Z_FWD, &ev);
+    if (raw) hipLaunchKernelGGL(zfwd_kernel<true>, grid, dim3(kThreads), sh.lds_z, c->stream, a);
+    else hipLaunchKernelGGL(zfwd_kernel<false>, grid, dim3(kThreads), sh.lds_z, c->stream, a);
+    timer_end(c, GF_K_FFT_Z_FWD, ev);
+    return hipGetLastError();
+}
+
+static hipError_t run_y(gf_ctx* c, bool inverse, int comps) {
+    const Shapes sh = shapes(c);
+    YArgs a{};
+    a.in = inverse ? c->demag.spec_b : c->demag.spec_a;
+    a.out = inverse ? c->demag.spec_a : c->demag.spec_b;
+    a.tw = c->demag.twy.dev;
+    a.nh = c->nx * c->ny * c->demag.nzh;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
+    a.logny = ilog2(c->ny); a.pencils = sh.py; a.logp = ilog2(sh.py);
+    dim3 grid(blocks_for(c->nx, sh.py), static_cast<unsigned>(c->demag.nzh), static_cast<unsigned>(comps));
+    hipEvent_t ev = nullptr;
+    const int cls = inverse ? GF_K_FFT_Y_INV : GF_K_FFT_Y_FWD;
+    timer_begin(c, cls, &ev);
+    if (inverse) hipLaunchKernelGGL(yinv_kernel, grid, dim3(kThreads), sh.lds_y, c->stream, a);
+    else hipLaunchKernelGGL(yfwd_kernel, grid, dim3(kThreads), sh.lds_y, c->stream, a);
+    timer_end(c, cls, ev);
+    return hipGetLastError();
+}
+
+static hipError_t run_x(gf_ctx* c, bool conv) {
+    const Shapes sh = shapes(c);
+    XArgs a{};
+    a.data = c->demag.spec_b; a.kspec = c->demag.kspec; a.tw = c->demag.twx.dev;
+    a.nh = c->nx * c->ny * c->demag.nzh;
+    a.den = -c->dx * c->dy * c->dz;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
+    a.lognx = ilog2(c->nx); a.pencils = sh.px; a.logp = ilog2(sh.px);
+    dim3 grid(blocks_for(c->ny, sh.px), static_cast<unsigned>(c->demag.nzh));
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_FFT_X_CONV, &ev);
+    if (conv) hipLaunchKernelGGL(xconv_kernel<true>, grid, dim3(kThreads), sh.lds_x, c->stream, a);
+    else hipLaunchKernelGGL(xconv_kernel<false>, grid, dim3(kThreads), sh.lds_x, c->stream, a);
+    timer_end(c, GF_K_FFT_X_CONV, ev);
+    return hipGetLastError();
+}
+
+static hipError_t run_zinv(gf_ctx* c) {
+    const Shapes sh = shapes(c);
+    ZInvArgs a{};
+    a.in = c->demag.spec_a;
+    for (int i = 0; i < 3; i++) a.out[i] = c->hstat[i];
+    a.tw = c->demag.twz.dev;
+    a.nh = c->nx * c->ny * c->demag.nzh;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
+    a.nzh = static_cast<int>(c->demag.nzh);
+    a.lognz = ilog2(c->nz); a.pencils = sh.pz; a.logp = ilog2(sh.pz);
+    dim3 grid(blocks_for(c->ny, sh.pz), static_cast<unsigned>(c->nx), 3);
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_FFT_Z_INV, &ev);
+    hipLaunchKernelGGL(zinv_kernel, grid, dim3(kThreads), sh.lds_z, c->stream, a);
+    timer_end(c, GF_K_FFT_Z_INV, ev);
+    return hipGetLastError();
+}
+
+static bool pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+gf_status demag_setup(gf_ctx* c) {
+    if (!pow2(c->nx) || !pow2(c->ny) || !pow2(c->nz))
+        return set_error(c, GF_ERR_UNSUPPORTED, "FFT magnetostatics needs power-of-two nx, ny, nz");
+    if (c->nx > 1024 || c->ny > 2048 || c->nz > 2048)
+        return set_error(c, GF_ERR_UNSUPPORTED, "FFT magnetostatics supports nx <= 1024, ny, nz <= 2048");
+    DemagPlan& d = c->demag;
+    d.nzh = c->nz / 2 + 1;
+    const int64_t nh = c->nx * c->ny * d.nzh;
+    gf_status st;
+    if ((st = hip_check(c, hipMalloc(&d.spec_a, sizeof(double2) * 3 * nh), "hipMalloc spec_a"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.spec_b, sizeof(double2) * 3 * nh), "hipMalloc spec_b"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.kspec, sizeof(double2) * 9 * nh), "hipMalloc kspec"))) return st;
+    if ((st = make_twiddles(c, d.twx, c->nx))) return st;
+    if ((st = make_twiddles(c, d.twy, c->ny))) return st;
+    if ((st = make_twiddles(c, d.twz, c->nz))) return st;
+
+    // Real-space tensor kernels on the host (bit-exact with the reference's
+    // evaluation), then forward transforms on the device.
+    std::vector<double> real9(static_cast<size_t>(9 * c->n));
+    unsigned hw = std::thread::hardware_concurrency();
+    int threads = hw ? static_cast<int>(hw) : 1;
+    if (threads > 64) threads = 64;
+    newell_real_kernels(c->nx, c->ny, c->nz, c->dx, c->dy, c->dz, real9.data(), threads);
+    double* draw = nullptr;
+    if ((st = hip_check(c, hipMalloc(&draw, sizeof(double) * c->n), "hipMalloc kernel staging"))) return st;
+    for (int s = 0; s < 9; s++) {
+        st = hip_check(c, hipMemcpyAsync(draw, real9.data() + s * c->n, sizeof(double) * c->n, hipMemcpyHostToDevice,
+                                         c->stream), "copy kernel");
+        if (!st) st = hip_check(c, run_zfwd(c, true, draw, nullptr, nullptr, nullptr, 1), "zfwd kernel");
+        if (!st) st = hip_check(c, run_y(c, false, 1), "yfwd kernel");
+        if (!st) st = hip_check(c, run_x(c, false), "xfwd kernel");
+        if (!st)
+            st = hip_check(c, hipMemcpyAsync(d.kspec + s * nh, d.spec_b, sizeof(double2) * nh,
+                                             hipMemcpyDeviceToDevice, c->stream), "copy spectrum");
+        if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "kernel spectra");
+        if (st) { hipFree(draw); return st; }
+    }
+    hipFree(draw);
+    d.ready = true;
+    return GF_OK;
+}
+
+gf_status demag_run(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2) {
+    gf_status st;
+    if ((st = hip_check(c, run_zfwd(c, false, nullptr, fm, s1, s2, 3), "zfwd"))) return st;
+    if ((st = hip_check(c, run_y(c, false, 3), "yfwd"))) return st;
+    if ((st = hip_check(c, run_x(c, true), "xconv"))) return st;
+    if ((st = hip_check(c, run_y(c, true, 3), "yinv"))) return st;
+    return hip_check(c, run_zinv(c), "zinv");
+}
+
+hipError_t launch_demag_local(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2,
+                              int one_d) {
+    LocalArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    for (int i = 0; i < 3; i++) {
+        a.fm[i] = fm[i]; a.a1[i] = s1[i]; a.a2[i] = s2[i]; a.h[i] = c->hstat[i]; a.fac[i] = c->demag_fac[i];
+    }
+    a.n = c->n; a.one_d = one_d;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_OTHER, &ev);
+    hipLaunchKernelGGL(demag_local_kernel, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, a);
+    timer_end(c, GF_K_OTHER, ev);
+    return hipGetLastError();
+}
+
+void demag_free(gf_ctx* c) {
+    DemagPlan& d = c->demag;
+    hipFree(d.spec_a); hipFree(d.spec_b); hipFree(d.kspec);
+    hipFree(d.twx.dev); hipFree(d.twy.dev); hipFree(d.twz.dev);
+    d = DemagPlan{};
+}
+
+}  // namespace gf

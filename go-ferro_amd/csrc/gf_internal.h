@@ -1,0 +1,220 @@
+// gf_internal.h — shared definitions of the gfx950 hot-path library.
+//
+// Device data layout (all FP64, SoA, z fastest, index (i*ny + j)*nz + k):
+//   m[3], acc[3], store_a[3], store_b[3]   FM state, RK accumulator, stage stores
+//   m1/m2 + their acc/stores                AFM sublattices
+//   hstat[3]                                magnetostatic field
+//   code[n] (uint32)                        material id + interface bits (below)
+//   spec_a/spec_b                           complex FFT work buffers, 3 components
+//   kspec[9]                                demag tensor spectra
+// A cell code packs what the reference keeps in material_cell plus the six
+// FM_surf* / AFM_surf* face masks (magnetic_initialize.cpp:136-379, 635-870).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/goferro.h"
+
+namespace gf {
+
+constexpr double kMu0 = 1.256637061e-6;               // magnetic_system.h:14
+constexpr double kPi = 3.14159265358979323846;        // magnetic_system.h:16
+
+// cell code bits
+constexpr uint32_t kMatMask = 0xFFFFu;
+constexpr uint32_t kIsFM = 1u << 16;
+constexpr uint32_t kIsAFM = 1u << 17;
+constexpr int kFmSurfShift = 18;    // 6 bits: xb, xf, yb, yf, zb, zf
+constexpr int kAfmSurfShift = 24;   // 6 bits, same order
+enum Face { kXB = 0, kXF = 1, kYB = 2, kYF = 3, kZB = 4, kZF = 5 };
+
+// Per-material constants as the stage kernels need them.  Every entry is
+// the value the reference computes per cell from the material record, in
+// the same operation order, so hoisting it to set-up changes no bit.
+struct DevMaterial {
+    // FM (magnetic_solver.cpp:84-87, 164-220)
+    double Ms;
+    double K1n, K2n;        // K / mu0 / Ms
+    double B1n, B2n;        // B / mu0 / Ms
+    double Aex;
+    double Aex_self;        // 2*Aex*Aex/(Aex+Aex): harmonic mean with itself
+    double dmi_slope;       // iDMI / 2. / Aex
+    double exch;            // 2. / mu0 / Ms
+    double dmi_pos;         // 2. * iDMI / mu0 / Ms
+    double dmi_neg;         // -2. * iDMI / mu0 / Ms
+    double precess;         // -gyro / (1 + alpha^2) * dt
+    double alpha_precess;   // alpha * precess
+    int32_t anis_type;
+    int32_t is_fm;
+    // AFM (magnetic_AFM_solver.cpp:108-200), index 0/1 = sublattice 1/2
+    double aMs[2];
+    double aK1n[2];
+    double aB1n[2], aB2n[2];
+    double aJ[2];           // -1. * J_AFM / mu0 / Ms_sub
+    double aAex[2];
+    double aAex_self[2];
+    double aExch[2];        // 2. / mu0 / Ms_sub
+    double aPrecess[2];
+    double aAlphaPrecess[2];
+    double u[3];
+    int32_t is_afm;
+    int32_t pad;
+};
+
+// Everything a stage kernel reads, passed by value as kernel arguments.
+struct StageArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* store[3];     // stage store read by the stencil
+    const double* base[3];      // m at the start of the step
+    double* acc[3];             // RK accumulator
+    double* out[3];             // next store, or m at stage 4
+    const double* hstat[3];
+    const double* hnu[3];       // may be null
+    double* dm_out[3];          // k of this stage for inspection, may be null
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
+    double hext[3];
+    double Tg2c[3][3];
+    double Tc2g[3][3];
+    int32_t rotate;
+    int32_t stage;
+};
+
+struct AfmStageArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* store1[3];
+    const double* store2[3];
+    const double* base1[3];
+    const double* base2[3];
+    double* acc1[3];
+    double* acc2[3];
+    double* out1[3];
+    double* out2[3];
+    const double* hstat[3];
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
+    double hext[3];
+    double Tg2c[3][3];
+    double Tc2g[3][3];
+    int32_t rotate;
+    int32_t stage;
+};
+
+// Twiddle table of one transform length: w[j] = exp(-2 pi i j / N), j < N/2.
+struct Twiddles {
+    int64_t n = 0;
+    double2* dev = nullptr;
+};
+
+struct DemagPlan {
+    bool ready = false;
+    int64_t nzh = 0;                 // nz/2 + 1
+    double2* spec_a = nullptr;       // 3 components, layout [x][kz][y]
+    double2* spec_b = nullptr;       // 3 components, layout [kz][y][x]
+    double2* kspec = nullptr;        // 9 spectra, layout [kz][y][x]
+    Twiddles twx, twy, twz;
+};
+
+struct TimerSlot {
+    hipEvent_t start, stop;
+    int cls;
+};
+
+}  // namespace gf
+
+struct gf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    int64_t nx = 0, ny = 0, nz = 0, n = 0;
+    double dx = 0, dy = 0, dz = 0;
+    int per[3] = {0, 0, 0};
+    uint32_t flags = 0;
+    double dt = 0, time = 0;
+    double Tg2c[3][3] = {}, Tc2g[3][3] = {};
+    double hext_stat[3] = {}, hext_altr[3] = {}, h_altr_freq[3] = {}, hext[3] = {};
+    double demag_fac[3] = {};
+    std::vector<double> prescribe;   // rows of (t, Hx, Hy, Hz)
+    int prescribe_idx = 0;
+
+    std::vector<gf_material> host_mats;
+    std::vector<uint32_t> host_code;
+    bool any_fm = false, any_afm = false;
+    int64_t nfm = 0, nafm = 0;
+    bool has_hnu = false;
+
+    uint32_t* d_code = nullptr;
+    gf::DevMaterial* d_mats = nullptr;
+
+    // FM
+    double* m[3] = {};
+    double* acc[3] = {};
+    double* st_a[3] = {};
+    double* st_b[3] = {};
+    double* dm_last[3] = {};
+    // AFM
+    double* m1[3] = {};
+    double* m2[3] = {};
+    double* acc1[3] = {};
+    double* acc2[3] = {};
+    double* st1_a[3] = {};
+    double* st1_b[3] = {};
+    double* st2_a[3] = {};
+    double* st2_b[3] = {};
+    // statics
+    double* hstat[3] = {};
+    double* hnu[3] = {};
+    // reduction scratch
+    double* red_partial = nullptr;
+    int64_t red_blocks = 0;
+
+    // Which buffer holds the store the next stage reads: 0 = m, 1 = a, 2 = b.
+    int mag_store_sel = 0;
+    int afm_store_sel = 0;
+    int mag_next_stage = 1;
+    int afm_next_stage = 1;
+
+    gf::DemagPlan demag;
+
+    bool timing = false;
+    std::vector<gf::TimerSlot> timers;
+    std::vector<hipEvent_t> event_pool;
+    double kernel_ms[GF_K_COUNT] = {};
+    int64_t kernel_launches[GF_K_COUNT] = {};
+};
+
+namespace gf {
+
+// error helpers (gf_api.hip)
+gf_status set_error(gf_ctx* c, gf_status code, const std::string& msg);
+gf_status hip_check(gf_ctx* c, hipError_t e, const char* what);
+void timer_begin(gf_ctx* c, int cls, hipEvent_t* ev);
+void timer_end(gf_ctx* c, int cls, hipEvent_t start);
+
+// launchers (gf_llg.hip)
+hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s);
+hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s);
+
+// demag (gf_fft.hip)
+gf_status demag_setup(gf_ctx* c);
+gf_status demag_run(gf_ctx* c, const double* const* fm_store, const double* const* s1,
+                    const double* const* s2);
+void demag_free(gf_ctx* c);
+hipError_t launch_demag_local(gf_ctx* c, const double* const* fm_store, const double* const* s1,
+                              const double* const* s2, int one_d);
+
+// reductions (gf_llg.hip)
+hipError_t launch_sum3(const double* const* f, int64_t n, double* partial, int64_t blocks, hipStream_t s);
+
+// host Newell kernel (gf_newell.cpp)
+void newell_real_kernels(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                         double* out9, int threads);
+
+}  // namespace gf

@@ -1,0 +1,319 @@
+// gf_llg.hip — fused LLG RK4 stage kernels for gfx950 (FP64).
+//
+// One launch per RK stage does what the reference spreads over get_dm_RKs,
+// get_dm (at stage 4) and update_m_RKs / update_m
+// (magnetic_solver.cpp:3-229, 1449-1457, 1515-1636;
+//  magnetic_AFM_solver.cpp:3-430, 1715-1834):
+//   stage 1:  k = f(m);      next = m + k/2;  acc = k/6
+//   stage 2:  k = f(s2);     next = m + k/2;  acc = acc + k/3
+//   stage 3:  k = f(s3);     next = m + k;    acc = acc + k/3
+//   stage 4:  k = f(s4);     m = m + (acc + k/6)
+// The accumulator is summed in the reference's order
+// (((k1/6 + k2/3) + k3/3) + k4/6), so m(t+dt) keeps its rounding; the four
+// k-arrays and the separate combine/update/copy launches disappear.  The
+// stencil store is read-only within a stage and the next store goes to the
+// other buffer, so physics that share a stage still see the old store.
+#include "gf_internal.h"
+
+namespace gf {
+
+namespace {
+
+__device__ __forceinline__ void rot(const double T[3][3], double ix, double iy, double iz, double& ox, double& oy,
+                                    double& oz) {
+    ox = T[0][0] * ix + T[0][1] * iy + T[0][2] * iz;
+    oy = T[1][0] * ix + T[1][1] * iy + T[1][2] * iz;
+    oz = T[2][0] * ix + T[2][1] * iy + T[2][2] * iz;
+}
+
+// Exchange-weighted neighbour differences of one store for one cell
+// (get_m_spatial_derivative_local, magnetic_solver.cpp:1639-1831).
+// Returns hx[3] = sum_faces Aij * d2m (before the 2/mu0/Ms factor) and the
+// DMI first derivatives d[4] = dmxdx, dmydy, dmzdx, dmzdy.
+struct Stencil {
+    double ex[3];
+    double d[4];
+};
+
+struct P3 {
+    const double* p[3];
+};
+
+template <bool kDmi>
+__device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, const DevMaterial* mats,
+                                           uint32_t own, int surf_shift, int sub, int64_t i, int64_t j, int64_t k,
+                                           int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                                           double A, double A_self, double slope) {
+    const int64_t id = (i * ny + j) * nz + k;
+    const double* s[3] = {s3.p[0], s3.p[1], s3.p[2]};
+    const double c0 = s[0][id], c1 = s[1][id], c2 = s[2][id];
+    const uint32_t mat = own & kMatMask;
+    Stencil r;
+    double bw[3], fw[3], ab, af;
+    // x faces
+    {
+        int64_t nb;
+        if ((own >> (surf_shift + kXB)) & 1u) {
+            if (kDmi) { bw[0] = -slope * c2; bw[1] = 0.; bw[2] = slope * c0; }
+            else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+            ab = A;
+        } else {
+            nb = ((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k;
+            bw[0] = (c0 - s[0][nb]) / dx; bw[1] = (c1 - s[1][nb]) / dx; bw[2] = (c2 - s[2][nb]) / dx;
+            const uint32_t nm = code[nb] & kMatMask;
+            if (nm == mat) ab = A_self;
+            else {
+                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+                ab = 2. * A * a2 / (A + a2);
+            }
+        }
+        if ((own >> (surf_shift + kXF)) & 1u) {
+            if (kDmi) { fw[0] = -slope * c2; fw[1] = 0.; fw[2] = slope * c0; }
+            else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+            af = A;
+        } else {
+            nb = ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k;
+            fw[0] = (s[0][nb] - c0) / dx; fw[1] = (s[1][nb] - c1) / dx; fw[2] = (s[2][nb] - c2) / dx;
+            const uint32_t nm = code[nb] & kMatMask;
+            if (nm == mat) af = A_self;
+            else {
+                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+                af = 2. * A * a2 / (A + a2);
+            }
+        }
+        r.d[0] = (bw[0] + fw[0]) * 0.5;
+        r.d[2] = (bw[2] + fw[2]) * 0.5;
+        for (int a = 0; a < 3; a++) r.ex[a] = af * (fw[a] / dx) + ab * (-bw[a] / dx);
+    }
+    // y faces
+    {
+        int64_t nb;
+        if ((own >> (surf_shift + kYB)) & 1u) {
+            if (kDmi) { bw[0] = 0.; bw[1] = -slope * c2; bw[2] = slope * c1; }
+            else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+            ab = A;
+        } else {
+            nb = (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k;
+            bw[0] = (c0 - s[0][nb]) / dy; bw[1] = (c1 - s[1][nb]) / dy; bw[2] = (c2 - s[2][nb]) / dy;
+            const uint32_t nm = code[nb] & kMatMask;
+            if (nm == mat) ab = A_self;
+            else {
+                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+                ab = 2. * A * a2 / (A + a2);
+            }
+        }
+        if ((own >> (surf_shift + kYF)) & 1u) {
+            if (kDmi) { fw[0] = 0.; fw[1] = -slope * c2; fw[2] = slope * c1; }
+            else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+            af = A;
+        } else {
+            nb = (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k;
+            fw[0] = (s[0][nb] - c0) / dy; fw[1] = (s[1][nb] - c1) / dy; fw[2] = (s[2][nb] - c2) / dy;
+            const uint32_t nm = code[nb] & kMatMask;
+            if (nm == mat) af = A_self;
+            else {
+                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+                af = 2. * A * a2 / (A + a2);
+            }
+        }
+        r.d[1] = (bw[1] + fw[1]) * 0.5;
+        r.d[3] = (bw[2] + fw[2]) * 0.5;
+        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] / dy) + ab * (-bw[a] / dy);
+    }
+    // z faces: zero slope at surfaces for both kinds
+    {
+        int64_t nb;
+        if ((own >> (surf_shift + kZB)) & 1u) {
+            bw[0] = 0.; bw[1] = 0.; bw[2] = 0.;
+            ab = A;
+        } else {
+            nb = (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1);
+            bw[0] = (c0 - s[0][nb]) / dz; bw[1] = (c1 - s[1][nb]) / dz; bw[2] = (c2 - s[2][nb]) / dz;
+            const uint32_t nm = code[nb] & kMatMask;
+            if (nm == mat) ab = A_self;
+            else {
+                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+                ab = 2. * A * a2 / (A + a2);
+            }
+        }
+        if ((own >> (surf_shift + kZF)) & 1u) {
+            fw[0] = 0.; fw[1] = 0.; fw[2] = 0.;
+            af = A;
+        } else {
+            nb = (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1);
+            fw[0] = (s[0][nb] - c0) / dz; fw[1] = (s[1][nb] - c1) / dz; fw[2] = (s[2][nb] - c2) / dz;
+            const uint32_t nm = code[nb] & kMatMask;
+            if (nm == mat) af = A_self;
+            else {
+                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+                af = 2. * A * a2 / (A + a2);
+            }
+        }
+        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] / dz) + ab * (-bw[a] / dz);
+    }
+    return r;
+}
+
+// RK update of one component, see the file comment.
+__device__ __forceinline__ void rk_update(int stage, int64_t id, int a, double k, const double* const* base,
+                                          double* const* acc, double* const* out) {
+    const double m = base[a][id];
+    if (stage == 1) {
+        out[a][id] = m + k * 0.5;
+        acc[a][id] = k / 6.;
+    } else if (stage == 2) {
+        out[a][id] = m + k * 0.5;
+        acc[a][id] = acc[a][id] + k / 3.;
+    } else if (stage == 3) {
+        out[a][id] = m + k;
+        acc[a][id] = acc[a][id] + k / 3.;
+    } else {
+        out[a][id] = m + (acc[a][id] + k / 6.);
+    }
+}
+
+__global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
+    const int64_t n = a.nx * a.ny * a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const uint32_t own = a.code[id];
+    if (!(own & kIsFM)) return;
+    const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+    const int64_t nyz = a.ny * a.nz;
+    const int64_t x = id / nyz, y = (id - x * nyz) / a.nz, z = id - x * nyz - y * a.nz;
+
+    const double mgx = a.store[0][id], mgy = a.store[1][id], mgz = a.store[2][id];
+    double mx = mgx, my = mgy, mz = mgz;
+    if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
+
+    const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
+    const Stencil st = stencil<true>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, a.nx, a.ny, a.nz, a.dx,
+                                     a.dy, a.dz, M.Aex, M.Aex_self, M.dmi_slope);
+
+    // magnetocrystalline anisotropy (magnetic_solver.cpp:124-141)
+    double hac0 = 0., hac1 = 0., hac2 = 0.;
+    if (M.anis_type == 1) {
+        hac0 = -2. * (M.K1n * (my * my + mz * mz) + M.K2n * my * my * mz * mz) * mx;
+        hac1 = -2. * (M.K1n * (mx * mx + mz * mz) + M.K2n * mx * mx * mz * mz) * my;
+        hac2 = -2. * (M.K1n * (mx * mx + my * my) + M.K2n * mx * mx * my * my) * mz;
+    } else if (M.anis_type == 2) {
+        hac2 = 2. * M.K1n * mz;
+    }
+    double ha0 = hac0, ha1 = hac1, ha2 = hac2;
+    if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
+
+    // H_eff (magnetic_solver.cpp:164-198); magnetoelastic term is zero
+    // without a strain input and is left out of the sum.
+    double H0 = a.hstat[0][id] + ha0 + M.exch * st.ex[0] + M.dmi_pos * st.d[2] + a.hext[0];
+    double H1 = a.hstat[1][id] + ha1 + M.exch * st.ex[1] + M.dmi_pos * st.d[3] + a.hext[1];
+    double H2 = a.hstat[2][id] + ha2 + M.exch * st.ex[2] + M.dmi_neg * (st.d[0] + st.d[1]) + a.hext[2];
+    if (a.hnu[0]) {
+        H0 = H0 + a.hnu[0][id];
+        H1 = H1 + a.hnu[1][id];
+        H2 = H2 + a.hnu[2][id];
+    }
+
+    // LLG right-hand side times dt (magnetic_solver.cpp:206-220)
+    const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
+    const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
+    const double k0 = M.precess * px + M.alpha_precess * qx;
+    const double k1 = M.precess * py + M.alpha_precess * qy;
+    const double k2 = M.precess * pz + M.alpha_precess * qz;
+    if (a.dm_out[0]) {
+        a.dm_out[0][id] = k0; a.dm_out[1][id] = k1; a.dm_out[2][id] = k2;
+    }
+    rk_update(a.stage, id, 0, k0, a.base, a.acc, a.out);
+    rk_update(a.stage, id, 1, k1, a.base, a.acc, a.out);
+    rk_update(a.stage, id, 2, k2, a.base, a.acc, a.out);
+}
+
+__global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
+    const int64_t n = a.nx * a.ny * a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const uint32_t own = a.code[id];
+    if (!(own & kIsAFM)) return;
+    const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+    const int64_t nyz = a.ny * a.nz;
+    const int64_t x = id / nyz, y = (id - x * nyz) / a.nz, z = id - x * nyz - y * a.nz;
+
+    for (int sub = 0; sub < 2; sub++) {
+        const P3 st = sub == 0 ? P3{{a.store1[0], a.store1[1], a.store1[2]}} : P3{{a.store2[0], a.store2[1], a.store2[2]}};
+        const P3 other = sub == 0 ? P3{{a.store2[0], a.store2[1], a.store2[2]}} : P3{{a.store1[0], a.store1[1], a.store1[2]}};
+        const double mgx = st.p[0][id], mgy = st.p[1][id], mgz = st.p[2][id];
+        double mx = mgx, my = mgy, mz = mgz;
+        if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
+        const Stencil s = stencil<false>(st, a.code, a.mats, own, kAfmSurfShift, sub, x, y, z, a.nx, a.ny, a.nz,
+                                         a.dx, a.dy, a.dz, M.aAex[sub], M.aAex_self[sub], 0.);
+        // uniaxial anisotropy 2K1 (u.m) u (magnetic_AFM_solver.cpp:148-154)
+        const double udotm = mx * M.u[0] + my * M.u[1] + mz * M.u[2];
+        const double hac0 = 2. * M.aK1n[sub] * udotm * M.u[0];
+        const double hac1 = 2. * M.aK1n[sub] * udotm * M.u[1];
+        const double hac2 = 2. * M.aK1n[sub] * udotm * M.u[2];
+        double ha0 = hac0, ha1 = hac1, ha2 = hac2;
+        if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
+        // exchange + inter-sublattice + drive (magnetic_AFM_solver.cpp:170-200)
+        const double H0 = a.hstat[0][id] + ha0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * other.p[0][id] + a.hext[0];
+        const double H1 = a.hstat[1][id] + ha1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * other.p[1][id] + a.hext[1];
+        const double H2 = a.hstat[2][id] + ha2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * other.p[2][id] + a.hext[2];
+        const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
+        const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
+        const double k0 = M.aPrecess[sub] * px + M.aAlphaPrecess[sub] * qx;
+        const double k1 = M.aPrecess[sub] * py + M.aAlphaPrecess[sub] * qy;
+        const double k2 = M.aPrecess[sub] * pz + M.aAlphaPrecess[sub] * qz;
+        const double* const* base = sub == 0 ? a.base1 : a.base2;
+        double* const* acc = sub == 0 ? a.acc1 : a.acc2;
+        double* const* out = sub == 0 ? a.out1 : a.out2;
+        rk_update(a.stage, id, 0, k0, base, acc, out);
+        rk_update(a.stage, id, 1, k1, base, acc, out);
+        rk_update(a.stage, id, 2, k2, base, acc, out);
+    }
+}
+
+// Per-block partial sums of three fields; block b writes partial[3*b + c].
+__global__ void __launch_bounds__(256) sum3_kernel(const double* f0, const double* f1, const double* f2, int64_t n,
+                                                    double* partial) {
+    __shared__ double red[3][256];
+    double s0 = 0., s1 = 0., s2 = 0.;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        s0 += f0[i]; s1 += f1[i]; s2 += f2[i];
+    }
+    red[0][threadIdx.x] = s0; red[1][threadIdx.x] = s1; red[2][threadIdx.x] = s2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = red[0][0];
+        partial[3 * blockIdx.x + 1] = red[1][0];
+        partial[3 * blockIdx.x + 2] = red[2][0];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s) {
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(mag_stage_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s) {
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(afm_stage_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum3(const double* const* f, int64_t n, double* partial, int64_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(sum3_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, f[0], f[1], f[2], n,
+                       partial);
+    return hipGetLastError();
+}
+
+}  // namespace gf

@@ -1,0 +1,293 @@
+"""goferro — Python binding of the gfx950 coupled-RK4 library (libgoferro.so).
+
+The binding is a thin ctypes layer over the C ABI declared in
+``include/goferro.h``; no torch types cross it.  ``Problem`` describes one
+simulation the way the reference's input files do (grid, material table,
+per-cell material ids, drive, flags); ``Context`` owns the device state.
+
+The library is built in-tree by ``goferro._build.build()``.  Loading fails
+loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+__all__ = [
+    "Material", "Problem", "Context", "GoFerroError", "load_library", "LIB_PATH",
+    "MU0", "FLAG_MAGNETOSTATICS", "FLAG_MAG_1D", "FLAG_DEMAG_FACTOR", "FLAG_PRESCRIBE_HEXT",
+    "FLAG_ORIENT_ROTATE", "FLAG_HEXT_NONUNIF", "FLAG_EMDYNAMIC",
+    "FIELD_M", "FIELD_M_STORE", "FIELD_HSTAT", "FIELD_HEXT_NONUNIF", "FIELD_M_AFM1", "FIELD_M_AFM2",
+    "KERNEL_CLASSES", "EXPORTED_SYMBOLS",
+]
+
+LIB_PATH = Path(__file__).resolve().parent / "libgoferro.so"
+MU0 = 1.256637061e-6  # magnetic_system.h:14
+
+ABI_VERSION = 1
+FLAG_MAGNETOSTATICS = 1 << 0
+FLAG_MAG_1D = 1 << 1
+FLAG_DEMAG_FACTOR = 1 << 2
+FLAG_PRESCRIBE_HEXT = 1 << 3
+FLAG_ORIENT_ROTATE = 1 << 4
+FLAG_HEXT_NONUNIF = 1 << 5
+FLAG_EMDYNAMIC = 1 << 6
+
+FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2 = range(6)
+
+KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_conv", "fft_y_inv",
+                  "fft_z_inv", "other"]
+
+# every entry point declared in include/goferro.h
+EXPORTED_SYMBOLS = [
+    "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
+    "gf_mag_stage", "gf_afm_stage", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
+    "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times",
+]
+
+
+class GoFerroError(RuntimeError):
+    """A gf_* call returned a non-zero status."""
+
+
+class Material(ctypes.Structure):
+    """gf_material (include/goferro.h); field meaning per material.h:3-58."""
+
+    _fields_ = [
+        ("if_FM", ctypes.c_int32), ("anisotropy_type", ctypes.c_int32),
+        ("Ms", ctypes.c_double), ("gyro", ctypes.c_double), ("FM_damping", ctypes.c_double),
+        ("K1", ctypes.c_double), ("K2", ctypes.c_double), ("Aex", ctypes.c_double), ("iDMI", ctypes.c_double),
+        ("B1", ctypes.c_double), ("B2", ctypes.c_double),
+        ("if_AFM", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("Ms_AFM1", ctypes.c_double), ("Ms_AFM2", ctypes.c_double),
+        ("gyro_AFM1", ctypes.c_double), ("gyro_AFM2", ctypes.c_double),
+        ("damping_AFM1", ctypes.c_double), ("damping_AFM2", ctypes.c_double),
+        ("K1_AFM1", ctypes.c_double), ("K1_AFM2", ctypes.c_double),
+        ("uniaxis_AFM", ctypes.c_double * 3),
+        ("Aex_AFM1", ctypes.c_double), ("Aex_AFM2", ctypes.c_double),
+        ("J_AFM", ctypes.c_double),
+        ("B1_AFM1", ctypes.c_double), ("B1_AFM2", ctypes.c_double),
+        ("B2_AFM1", ctypes.c_double), ("B2_AFM2", ctypes.c_double),
+    ]
+
+    @classmethod
+    def make(cls, **kw) -> "Material":
+        m = cls()
+        for k, v in kw.items():
+            if k == "uniaxis_AFM":
+                u = np.asarray(v, dtype=float)
+                u = u / np.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2])  # read.cpp:151-155
+                m.uniaxis_AFM[:] = list(u)
+            else:
+                setattr(m, k, v)
+        return m
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("abi_version", ctypes.c_uint32),
+        ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("periodic", ctypes.c_int32 * 3), ("flags", ctypes.c_uint32),
+        ("dt", ctypes.c_double),
+        ("n_materials", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("materials", ctypes.POINTER(Material)),
+        ("material_cell", ctypes.POINTER(ctypes.c_uint32)),
+        ("Tg2c", (ctypes.c_double * 3) * 3),
+        ("Hext_stat", ctypes.c_double * 3), ("Hext_altr", ctypes.c_double * 3),
+        ("H_altr_freq", ctypes.c_double * 3), ("demag_fac", ctypes.c_double * 3),
+        ("n_prescribe_Hext", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("prescribe_Hext", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+@dataclasses.dataclass
+class Problem:
+    """One simulation, in the reference's terms (read.cpp:4-237)."""
+
+    shape: tuple[int, int, int]
+    spacing: tuple[float, float, float]
+    materials: Sequence[Material]
+    material_cell: np.ndarray                    # uint32, shape `shape`, 0 = vacuum
+    dt: float
+    periodic: tuple[bool, bool, bool] = (True, True, True)
+    flags: int = 0
+    Hext_stat: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    Hext_altr: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    H_altr_freq: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    demag_fac: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    Tg2c: Optional[np.ndarray] = None
+    prescribe_Hext: Optional[np.ndarray] = None  # rows (t, Hx, Hy, Hz)
+
+    @property
+    def n(self) -> int:
+        return int(self.shape[0] * self.shape[1] * self.shape[2])
+
+    def desc(self):
+        """Return (gf_desc, keep-alive list) for this problem."""
+        cell = np.ascontiguousarray(self.material_cell, dtype=np.uint32).reshape(-1)
+        if cell.size != self.n:
+            raise ValueError("material_cell has the wrong size")
+        mats = (Material * max(1, len(self.materials)))(*self.materials)
+        d = _Desc()
+        d.struct_size = ctypes.sizeof(_Desc)
+        d.abi_version = ABI_VERSION
+        d.nx, d.ny, d.nz = (int(v) for v in self.shape)
+        d.dx, d.dy, d.dz = (float(v) for v in self.spacing)
+        d.periodic[:] = [int(bool(p)) for p in self.periodic]
+        d.flags = int(self.flags)
+        d.dt = float(self.dt)
+        d.n_materials = len(self.materials)
+        d.materials = ctypes.cast(mats, ctypes.POINTER(Material))
+        d.material_cell = cell.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        T = np.eye(3) if self.Tg2c is None else np.asarray(self.Tg2c, dtype=float)
+        for a in range(3):
+            for b in range(3):
+                d.Tg2c[a][b] = float(T[a, b])
+        d.Hext_stat[:] = [float(v) for v in self.Hext_stat]
+        d.Hext_altr[:] = [float(v) for v in self.Hext_altr]
+        d.H_altr_freq[:] = [float(v) for v in self.H_altr_freq]
+        d.demag_fac[:] = [float(v) for v in self.demag_fac]
+        keep = [cell, mats]
+        if self.prescribe_Hext is not None:
+            pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)
+            d.n_prescribe_Hext = pre.shape[0]
+            d.prescribe_Hext = pre.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            keep.append(pre)
+        return d, keep
+
+
+_lib = None
+
+
+def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
+    """Load libgoferro.so (built in-tree); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not Path(path).exists():
+        raise GoFerroError(f"{path} is missing: build it with goferro._build.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(str(path))
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "gf_create": ([ctypes.POINTER(_Desc), ctypes.POINTER(P)], ctypes.c_int),
+        "gf_destroy": ([P], ctypes.c_int),
+        "gf_last_error": ([P], ctypes.c_char_p),
+        "gf_upload": ([P, ctypes.c_int, dp, ctypes.c_size_t], ctypes.c_int),
+        "gf_download": ([P, ctypes.c_int, dp, ctypes.c_size_t], ctypes.c_int),
+        "gf_time_advance": ([P], ctypes.c_int),
+        "gf_mag_stage": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_afm_stage": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_demag": ([P], ctypes.c_int),
+        "gf_step": ([P, ctypes.c_int64], ctypes.c_int),
+        "gf_average": ([P, ctypes.c_int, dp], ctypes.c_int),
+        "gf_get_time": ([P, dp], ctypes.c_int),
+        "gf_sync": ([P], ctypes.c_int),
+        "gf_get_stream": ([P, ctypes.POINTER(P)], ctypes.c_int),
+        "gf_set_kernel_timing": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_kernel_times": ([P, dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """Device-resident simulation (one gf_ctx)."""
+
+    def __init__(self, problem: Problem):
+        self.lib = load_library()
+        self.problem = problem
+        self.shape = tuple(int(v) for v in problem.shape)
+        self.n = problem.n
+        d, self._keep = problem.desc()
+        h = ctypes.c_void_p()
+        st = self.lib.gf_create(ctypes.byref(d), ctypes.byref(h))
+        if st != 0:
+            raise GoFerroError(f"gf_create failed ({st}): {self.lib.gf_last_error(None).decode()}")
+        self._h = h
+
+    # -- plumbing
+    def _check(self, st: int, what: str) -> None:
+        if st != 0:
+            raise GoFerroError(f"{what} failed ({st}): {self.lib.gf_last_error(self._h).decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.gf_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- fields
+    def upload(self, field: int, value: np.ndarray) -> None:
+        a = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
+        if a.size != 3 * self.n:
+            raise ValueError("field must hold 3*nx*ny*nz values")
+        self._check(self.lib.gf_upload(self._h, field, _dptr(a), a.size), "gf_upload")
+
+    def download(self, field: int) -> np.ndarray:
+        a = np.empty(3 * self.n, dtype=np.float64)
+        self._check(self.lib.gf_download(self._h, field, _dptr(a), a.size), "gf_download")
+        return a.reshape((3,) + self.shape)
+
+    # -- protocol
+    def time_advance(self) -> None:
+        self._check(self.lib.gf_time_advance(self._h), "gf_time_advance")
+
+    def mag_stage(self, stage: int) -> None:
+        self._check(self.lib.gf_mag_stage(self._h, stage), "gf_mag_stage")
+
+    def afm_stage(self, stage: int) -> None:
+        self._check(self.lib.gf_afm_stage(self._h, stage), "gf_afm_stage")
+
+    def demag(self) -> None:
+        self._check(self.lib.gf_demag(self._h), "gf_demag")
+
+    def step(self, nsteps: int = 1) -> None:
+        self._check(self.lib.gf_step(self._h, int(nsteps)), "gf_step")
+
+    def average(self, field: int = FIELD_M) -> np.ndarray:
+        out = np.zeros(3)
+        self._check(self.lib.gf_average(self._h, field, _dptr(out)), "gf_average")
+        return out
+
+    @property
+    def time(self) -> float:
+        t = ctypes.c_double()
+        self._check(self.lib.gf_get_time(self._h, ctypes.byref(t)), "gf_get_time")
+        return t.value
+
+    def sync(self) -> None:
+        self._check(self.lib.gf_sync(self._h), "gf_sync")
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        self._check(self.lib.gf_set_kernel_timing(self._h, int(bool(enable))), "gf_set_kernel_timing")
+
+    def kernel_times(self) -> dict:
+        ms = (ctypes.c_double * len(KERNEL_CLASSES))()
+        cnt = (ctypes.c_int64 * len(KERNEL_CLASSES))()
+        self._check(self.lib.gf_kernel_times(self._h, ms, cnt), "gf_kernel_times")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_CLASSES)}

@@ -1,0 +1,190 @@
+/*
+ * goferro.h — C ABI of the MI355X (gfx950) coupled RK4 hot path.
+ *
+ * The reference drives its physics through argument-less member functions
+ * that act on OpenACC-resident arrays (magnetic_system.h:160-210,
+ * EMdynamic_system.h:132-179, ferroelectric_system.h:96-131,
+ * elastic_system.h:172-224) in the fixed protocol of main.cpp:243-483.
+ * This header replaces that surface with plain C: a context owns the device
+ * state, the caller owns host buffers, every call is ordered on the context's
+ * HIP stream, and every call returns a status (0 = OK, < 0 = error).
+ *
+ * Layout contract (matrix.h:46-52): every cell field is an SoA array of
+ * nx*ny*nz doubles indexed (i*ny + j)*nz + k, z fastest.  material_cell is
+ * uint32 with 0 = vacuum and m >= 1 selecting materials[m-1] (global.h:91-92).
+ *
+ * No torch types appear here; the PyTorch extension and the ctypes wrapper
+ * call the same entry points.
+ */
+#ifndef GOFERRO_H
+#define GOFERRO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum gf_status_code {
+    GF_OK = 0,
+    GF_ERR_ARG = -1,          /* invalid argument (null pointer, bad id, size) */
+    GF_ERR_UNSUPPORTED = -2,  /* valid input the build does not implement */
+    GF_ERR_HIP = -3,          /* HIP runtime error; text in gf_last_error */
+    GF_ERR_ALLOC = -4,        /* host or device allocation failed */
+    GF_ERR_STATE = -5         /* call out of protocol order */
+};
+typedef int gf_status;
+
+/* ---- physics flags (gf_desc.flags) -------------------------------------- */
+#define GF_FLAG_MAGNETOSTATICS   (1u << 0)  /* if_magnetostatics */
+#define GF_FLAG_MAG_1D           (1u << 1)  /* if_mag_1Dmodel: Hz = -Ms mz */
+#define GF_FLAG_DEMAG_FACTOR     (1u << 2)  /* if_demag_factor */
+#define GF_FLAG_PRESCRIBE_HEXT   (1u << 3)  /* if_prescribe_Hext (table drive) */
+#define GF_FLAG_ORIENT_ROTATE    (1u << 4)  /* if_OrientRotate */
+#define GF_FLAG_HEXT_NONUNIF     (1u << 5)  /* Hext_nonunif field uploaded */
+#define GF_FLAG_EMDYNAMIC        (1u << 6)  /* if_EMdynamic (statics frozen) */
+
+/*
+ * One material record.  Field meaning and units follow the reference's
+ * material class (material.h:3-58) as filled by read_materials
+ * (read.cpp:113-215), including the derived B1/B2 (read.cpp:141-142,
+ * 162-165).  Booleans are int32 (0/1).
+ */
+typedef struct gf_material {
+    /* ferromagnet (material.h:12-18) */
+    int32_t if_FM;
+    int32_t anisotropy_type;   /* 1 = cubic, 2 = uniaxial along crystal z */
+    double Ms, gyro, FM_damping;
+    double K1, K2, Aex, iDMI;
+    double B1, B2;
+    /* two-sublattice antiferromagnet (material.h:24-35) */
+    int32_t if_AFM;
+    int32_t reserved0;
+    double Ms_AFM1, Ms_AFM2;
+    double gyro_AFM1, gyro_AFM2;
+    double damping_AFM1, damping_AFM2;
+    double K1_AFM1, K1_AFM2;
+    double uniaxis_AFM[3];     /* already normalised, read.cpp:151-155 */
+    double Aex_AFM1, Aex_AFM2;
+    double J_AFM;
+    double B1_AFM1, B1_AFM2, B2_AFM1, B2_AFM2;
+} gf_material;
+
+/*
+ * Simulation descriptor: the subset of geometry_parameters and
+ * global_parameters (read.cpp:4-111, 217-237) that the hot path reads.
+ */
+typedef struct gf_desc {
+    uint32_t struct_size;      /* sizeof(gf_desc), ABI check */
+    uint32_t abi_version;      /* GF_ABI_VERSION */
+    int64_t nx, ny, nz;        /* nx_system, ny_system, nz_system */
+    double dx, dy, dz;
+    int32_t periodic[3];       /* periodicX/Y/Z */
+    uint32_t flags;            /* GF_FLAG_* */
+    double dt;
+    int32_t n_materials;
+    int32_t reserved0;
+    const gf_material* materials;   /* n_materials records, copied at create */
+    const uint32_t* material_cell;  /* nx*ny*nz ids, copied at create */
+    double Tg2c[3][3];         /* global -> crystal rotation (read.cpp:27-29) */
+    double Hext_stat[3];       /* A/m (read.cpp:35) */
+    double Hext_altr[3];       /* A/m (read.cpp:36) */
+    double H_altr_freq[3];     /* Hz  (read.cpp:37) */
+    double demag_fac[3];       /* read.cpp:46 */
+    int32_t n_prescribe_Hext;  /* rows of the prescribed drive table */
+    int32_t reserved1;
+    const double* prescribe_Hext;   /* rows of (t, Hx, Hy, Hz), copied at create */
+} gf_desc;
+
+/* ---- field identifiers for upload/download ------------------------------ */
+enum gf_field_id {
+    GF_FIELD_M = 0,         /* FM m: 3 arrays (mx,my,mz), 3n doubles       */
+    GF_FIELD_M_STORE = 1,   /* FM m at the current RK stage, 3n            */
+    GF_FIELD_HSTAT = 2,     /* magnetostatic field Hx,Hy,Hz, 3n            */
+    GF_FIELD_HEXT_NONUNIF = 3, /* non-uniform external field, 3n          */
+    GF_FIELD_M_AFM1 = 4,    /* AFM sublattice 1, 3n                        */
+    GF_FIELD_M_AFM2 = 5,    /* AFM sublattice 2, 3n                        */
+    GF_FIELD_COUNT
+};
+
+typedef struct gf_ctx gf_ctx;
+
+/* Create a context on the current HIP device.  Copies the material table,
+ * material_cell and drive table, builds interface masks and (when the FFT
+ * demag is on) the Newell kernel spectra.  Replaces initialize_host +
+ * copy_to_device + initialize_device of the magnetic system
+ * (magnetic_initialize.cpp:4-116, 1072-1129, 1559-1788). */
+gf_status gf_create(const gf_desc* desc, gf_ctx** out);
+gf_status gf_destroy(gf_ctx* ctx);
+
+/* Last error message of a context (or of the last failed gf_create when
+ * ctx is NULL).  Never NULL. */
+const char* gf_last_error(const gf_ctx* ctx);
+
+/* Host <-> device copies of one field.  count is the number of doubles
+ * (3*n for vector fields).  Uploading GF_FIELD_M also sets the stage store
+ * to m, as initialize_device does (magnetic_initialize.cpp:1081-1097).
+ * Both calls synchronise the context stream before returning. */
+gf_status gf_upload(gf_ctx* ctx, int field_id, const double* host, size_t count);
+gf_status gf_download(gf_ctx* ctx, int field_id, double* host, size_t count);
+
+/* global_parameters::time_marching_device_serial (global.cpp:513-549):
+ * t += dt and step the prescribed drive table. */
+gf_status gf_time_advance(gf_ctx* ctx);
+
+/* One fused FM RK4 stage s in 1..4: get_dm_RKs followed by update_m_RKs
+ * (magnetic_solver.cpp:3-229, 285-1076 with 1449-1457 and 1515-1636).
+ * Reads the stage-(s-1) store, writes the stage-s store (double-buffered,
+ * so other physics of the same stage still see the old store). */
+gf_status gf_mag_stage(gf_ctx* ctx, int stage);
+
+/* One fused AFM RK4 stage (magnetic_AFM_solver.cpp:3-1834). */
+gf_status gf_afm_stage(gf_ctx* ctx, int stage);
+
+/* magnetic_system::get_H_static (magnetostatic_solver.cpp:114-219) or the
+ * 1D model get_H_static_1D (:222-249) per the descriptor flags. */
+gf_status gf_demag(gf_ctx* ctx);
+
+/* Whole RK4 steps in the protocol of main.cpp:243-483 (time march, four
+ * stages with statics between them, statics after the step). */
+gf_status gf_step(gf_ctx* ctx, int64_t nsteps);
+
+/* get_averagem / get_averagem_AFM (magnetic_solver.cpp:1863-1888):
+ * sums of the field over all cells divided by the magnetic cell count. */
+gf_status gf_average(gf_ctx* ctx, int field_id, double out[3]);
+
+/* Current simulated time (global_parameters::time_device). */
+gf_status gf_get_time(const gf_ctx* ctx, double* t);
+
+/* Wait for all work queued on the context stream. */
+gf_status gf_sync(gf_ctx* ctx);
+
+/* The context's HIP stream (a hipStream_t), for callers that order their
+ * own work (events, copies) with the context's kernels. */
+gf_status gf_get_stream(gf_ctx* ctx, void** stream);
+
+/* Per-kernel timing.  When enabled, every launch of the listed kernel
+ * classes is bracketed by HIP events on the context stream; gf_kernel_times
+ * returns accumulated milliseconds and launch counts per class. */
+enum gf_kernel_class {
+    GF_K_MAG_STAGE = 0,
+    GF_K_AFM_STAGE = 1,
+    GF_K_FFT_Z_FWD = 2,
+    GF_K_FFT_Y_FWD = 3,
+    GF_K_FFT_X_CONV = 4,
+    GF_K_FFT_Y_INV = 5,
+    GF_K_FFT_Z_INV = 6,
+    GF_K_OTHER = 7,
+    GF_K_COUNT = 8
+};
+gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
+gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GOFERRO_H */

@@ -1,0 +1,841 @@
+/*
+ * gf_oracle.c — CPU restatement of the reference's FP64 RK4 hot path
+ * (ferromagnet LLG, two-sublattice antiferromagnet LLG, FFT magnetostatics).
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker.  Only tests/,
+ * __graft_entry__.smoke() and the CPU-baseline leg of bench.py may load it;
+ * the shipped library never links it.
+ *
+ * Each function names the reference file:line it restates.  Arithmetic is
+ * written in the reference's own operation order (the same parenthesisation,
+ * the same divisions) so that with -ffp-contract=off on x86-64 the values
+ * are those the reference's serial build would produce.  The FFT is this
+ * file's own radix-2 transform (cuFFT, which the reference calls, is absent
+ * here; its unnormalised D2Z/Z2D semantics are restated).
+ *
+ * Pinning status of each piece is recorded in the test files and in the
+ * oracle section of the design notes:
+ *   - FFT: pinned against a naive DFT and numpy.fft (tests/test_oracle_fft.py)
+ *   - Newell kernel: self-term sum rule, thin-film and cube limits
+ *   - FM LLG stage: macrospin closed form, exchange spin-wave dispersion
+ *   - AFM LLG stage: uniaxial AFMR frequency, exchange-free macrospin limit
+ *   - RK4 protocol: fourth-order convergence
+ */
+#include "gf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double MU0 = 1.256637061e-6;          /* magnetic_system.h:14 */
+static const double PI = 3.14159265358979323846;   /* magnetic_system.h:16 */
+
+struct gfo_ctx {
+    int64_t nx, ny, nz, n;
+    double dx, dy, dz;
+    int per[3];
+    uint32_t flags;
+    double dt, time;
+    int nmat;
+    gf_material* mat;
+    uint32_t* cell;
+    double Tg2c[3][3], Tc2g[3][3];
+    double Hext_stat[3], Hext_altr[3], H_altr_freq[3], Hext[3];
+    double demag_fac[3];
+    int npre; double* pre; int pre_idx;
+    int if_FM_all, if_AFM_all;
+    int64_t NFM, NAFM;
+    /* FM (magnetic_system.h:109-133) */
+    double *m[3], *ms[3], *k[4][3], *dm[3];
+    unsigned char *surfYZ, *surfXZ, *surfXY;
+    double *Aij[6];   /* xf, xb, yf, yb, zf, zb */
+    /* AFM (magnetic_system.h:116-147) */
+    double *m1[3], *m2[3], *ms1[3], *ms2[3], *k1[4][3], *k2[4][3], *dm1[3], *dm2[3];
+    unsigned char *asurfYZ, *asurfXZ, *asurfXY;
+    double *Aij1[6], *Aij2[6];
+    /* statics */
+    double *H[3], *Hnu[3];
+    double *kspec[9];   /* complex spectra, interleaved, nx*ny*(nz/2+1) each */
+};
+
+/* ------------------------------------------------------------------ FFT */
+/* In-place iterative radix-2 complex FFT, unnormalised, sign -1 forward. */
+static void fft1(double* re, double* im, int64_t n, int inverse) {
+    if (n <= 1) return;
+    for (int64_t i = 1, j = 0; i < n; i++) {
+        int64_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    for (int64_t len = 2; len <= n; len <<= 1) {
+        int64_t h = len >> 1;
+        for (int64_t j = 0; j < h; j++) {
+            /* twiddle exp(-+2 pi i j / len) from the table angle directly */
+            double ang = 2.0 * PI * (double)j / (double)len;
+            double wr = cos(ang), wi = inverse ? sin(ang) : -sin(ang);
+            for (int64_t s = 0; s < n; s += len) {
+                int64_t a = s + j, b = s + j + h;
+                double vr = re[b] * wr - im[b] * wi;
+                double vi = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - vr; im[b] = im[a] - vi;
+                re[a] = re[a] + vr; im[a] = im[a] + vi;
+            }
+        }
+    }
+}
+
+static int is_pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+/* cuFFT D2Z semantics (magnetostatic_solver.cpp:149-151): unnormalised
+ * forward transform over (nx,ny,nz), row-major, half spectrum along z. */
+void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out) {
+    int64_t nzh = nz / 2 + 1;
+    int64_t mx = nx > ny ? nx : ny; if (nz > mx) mx = nz;
+    double* re = (double*)malloc(sizeof(double) * mx);
+    double* im = (double*)malloc(sizeof(double) * mx);
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++) {
+            for (int64_t k = 0; k < nz; k++) { re[k] = in[(i * ny + j) * nz + k]; im[k] = 0.0; }
+            fft1(re, im, nz, 0);
+            for (int64_t k = 0; k < nzh; k++) {
+                out[2 * ((i * ny + j) * nzh + k)] = re[k];
+                out[2 * ((i * ny + j) * nzh + k) + 1] = im[k];
+            }
+        }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t k = 0; k < nzh; k++) {
+            for (int64_t j = 0; j < ny; j++) { re[j] = out[2 * ((i * ny + j) * nzh + k)]; im[j] = out[2 * ((i * ny + j) * nzh + k) + 1]; }
+            fft1(re, im, ny, 0);
+            for (int64_t j = 0; j < ny; j++) { out[2 * ((i * ny + j) * nzh + k)] = re[j]; out[2 * ((i * ny + j) * nzh + k) + 1] = im[j]; }
+        }
+    for (int64_t j = 0; j < ny; j++)
+        for (int64_t k = 0; k < nzh; k++) {
+            for (int64_t i = 0; i < nx; i++) { re[i] = out[2 * ((i * ny + j) * nzh + k)]; im[i] = out[2 * ((i * ny + j) * nzh + k) + 1]; }
+            fft1(re, im, nx, 0);
+            for (int64_t i = 0; i < nx; i++) { out[2 * ((i * ny + j) * nzh + k)] = re[i]; out[2 * ((i * ny + j) * nzh + k) + 1] = im[i]; }
+        }
+    free(re); free(im);
+}
+
+/* cuFFT Z2D semantics (magnetostatic_solver.cpp:183-185): unnormalised
+ * inverse of the half spectrum; x and y inverted first, then z by
+ * Hermitian extension, imaginary parts of the self-conjugate bins unused. */
+void gfo_irfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out) {
+    int64_t nzh = nz / 2 + 1;
+    int64_t mx = nx > ny ? nx : ny; if (nz > mx) mx = nz;
+    double* re = (double*)malloc(sizeof(double) * mx);
+    double* im = (double*)malloc(sizeof(double) * mx);
+    double* tmp = (double*)malloc(sizeof(double) * 2 * nx * ny * nzh);
+    memcpy(tmp, in, sizeof(double) * 2 * nx * ny * nzh);
+    for (int64_t j = 0; j < ny; j++)
+        for (int64_t k = 0; k < nzh; k++) {
+            for (int64_t i = 0; i < nx; i++) { re[i] = tmp[2 * ((i * ny + j) * nzh + k)]; im[i] = tmp[2 * ((i * ny + j) * nzh + k) + 1]; }
+            fft1(re, im, nx, 1);
+            for (int64_t i = 0; i < nx; i++) { tmp[2 * ((i * ny + j) * nzh + k)] = re[i]; tmp[2 * ((i * ny + j) * nzh + k) + 1] = im[i]; }
+        }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t k = 0; k < nzh; k++) {
+            for (int64_t j = 0; j < ny; j++) { re[j] = tmp[2 * ((i * ny + j) * nzh + k)]; im[j] = tmp[2 * ((i * ny + j) * nzh + k) + 1]; }
+            fft1(re, im, ny, 1);
+            for (int64_t j = 0; j < ny; j++) { tmp[2 * ((i * ny + j) * nzh + k)] = re[j]; tmp[2 * ((i * ny + j) * nzh + k) + 1] = im[j]; }
+        }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++) {
+            const double* row = tmp + 2 * ((i * ny + j) * nzh);
+            for (int64_t k = 0; k < nz; k++) {
+                if (k < nzh) { re[k] = row[2 * k]; im[k] = row[2 * k + 1]; }
+                else { re[k] = row[2 * (nz - k)]; im[k] = -row[2 * (nz - k) + 1]; }
+            }
+            fft1(re, im, nz, 1);
+            for (int64_t k = 0; k < nz; k++) out[(i * ny + j) * nz + k] = re[k];
+        }
+    free(tmp); free(re); free(im);
+}
+
+void gfo_rdft3_naive(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out) {
+    int64_t nzh = nz / 2 + 1;
+    for (int64_t a = 0; a < nx; a++)
+        for (int64_t b = 0; b < ny; b++)
+            for (int64_t c = 0; c < nzh; c++) {
+                long double sr = 0.0L, si = 0.0L;
+                for (int64_t i = 0; i < nx; i++)
+                    for (int64_t j = 0; j < ny; j++)
+                        for (int64_t k = 0; k < nz; k++) {
+                            long double ph = -2.0L * 3.14159265358979323846264338327950288L *
+                                ((long double)(a * i % nx) / nx + (long double)(b * j % ny) / ny +
+                                 (long double)(c * k % nz) / nz);
+                            long double v = in[(i * ny + j) * nz + k];
+                            sr += v * cosl(ph); si += v * sinl(ph);
+                        }
+                out[2 * ((a * ny + b) * nzh + c)] = (double)sr;
+                out[2 * ((a * ny + b) * nzh + c) + 1] = (double)si;
+            }
+}
+
+/* --------------------------------------------------------- Newell kernel */
+/* magnetic_initialize.cpp:1427-1476 */
+double gfo_newell_F2(double x, double y, double z) {
+    double f2 = 0.;
+    double sigma = 1.0e-33;
+    double u = x * x, v = y * y, w = z * z;
+    double r = sqrt(u + v + w);
+    if (fabs(x) > sigma && fabs(y) > sigma && fabs(z) > sigma) {
+        f2 = z * (u - v) * log(r - z) / 2.
+           + y * (u - w) * log(r - y) / 2.
+           - x * y * z * atan(y * z / (x * r))
+           + (2. * u - v - w) * r / 6.;
+    } else if (fabs(x) < sigma && fabs(y) > sigma && fabs(z) > sigma) {
+        f2 = -(z * v / 2.) * log(r - z) - (y * w / 2.) * log(r - y) - r * r * r / 6.;
+    } else if (fabs(x) > sigma && fabs(y) < sigma && fabs(z) > sigma) {
+        f2 = (z * u / 2.) * log(r - z) + (2. * u - w) * r / 6.;
+    } else if (fabs(x) > sigma && fabs(y) > sigma && fabs(z) < sigma) {
+        f2 = (y * u / 2.) * log(r - y) + (2. * u - v) * r / 6.;
+    } else if (fabs(x) < sigma && fabs(y) < sigma && fabs(z) > sigma) {
+        f2 = -fabs(z) * fabs(z) * fabs(z) / 6.;
+    } else if (fabs(x) < sigma && fabs(y) > sigma && fabs(z) < sigma) {
+        f2 = -fabs(y) * fabs(y) * fabs(y) / 6.;
+    } else if (fabs(x) > sigma && fabs(y) < sigma && fabs(z) < sigma) {
+        f2 = fabs(x) * fabs(x) * fabs(x) / 3.;
+    } else if (fabs(x) < sigma && fabs(y) < sigma && fabs(z) < sigma) {
+        f2 = 0.;
+    }
+    return f2;
+}
+
+/* magnetic_initialize.cpp:1419-1425 */
+static double newell_F1(double x, double y, double z, double dx, double dy, double dz) {
+    (void)dx; (void)dz;
+    double ym = y - dy, yp = y + dy;
+    return 2. * gfo_newell_F2(x, y, z) - gfo_newell_F2(x, ym, z) - gfo_newell_F2(x, yp, z);
+}
+
+/* magnetic_initialize.cpp:1404-1417 */
+double gfo_newell_A(double x, double y, double z, double dx, double dy, double dz) {
+    double xp = x + dx, xm = x - dx, zp = z + dz, zm = z - dz;
+    return 4. * newell_F1(x, y, z, dx, dy, dz) - 2. * newell_F1(x, y, zp, dx, dy, dz)
+         - 2. * newell_F1(x, y, zm, dx, dy, dz) - 2. * newell_F1(xp, y, z, dx, dy, dz)
+         + newell_F1(xp, y, zp, dx, dy, dz)
+         + newell_F1(xp, y, zm, dx, dy, dz)
+         - 2. * newell_F1(xm, y, z, dx, dy, dz) + newell_F1(xm, y, zp, dx, dy, dz) + newell_F1(xm, y, zm, dx, dy, dz);
+}
+
+/* magnetic_initialize.cpp:1507-1557 */
+double gfo_newell_G2(double x, double y, double z) {
+    double g2 = 0.;
+    double sigma = 1.0e-33;
+    double u = x * x, v = y * y, w = z * z;
+    double r = sqrt(u + v + w);
+    if (fabs(x * y * z) > sigma) {
+        g2 = (y * y * y / 6. - y * w / 2.) * log(r + x)
+           + (x / 6.) * (3. * w - u) * log(r - y)
+           + x * y * r / 3. + x * y * z * log(r - z)
+           + (v * z / 2.) * atan(x * z / (r * y))
+           + (z * z * z / 6.) * atan(x * y / (r * z))
+           + (u * z / 2.) * atan(y * z / (r * x));
+    } else if (fabs(x) < sigma && fabs(y * z) > sigma) {
+        g2 = (y * y * y / 6. - y * w / 2.) * log(r);
+    } else if (fabs(x * z) > sigma && fabs(y) < sigma) {
+        g2 = (x * w / 2. - x * x * x / 6.) * log(r);
+    } else if (fabs(x * y) > sigma && fabs(z) < sigma) {
+        g2 = ((y * y * y) / 6.) * log(r + x) - (x * x * x / 6.) * log(r - y) + x * y * r / 3.;
+    } else if ((fabs(x) + fabs(y)) < sigma && fabs(z) > sigma) {
+        g2 = 0.;
+    } else if ((fabs(x) + fabs(z)) < sigma && fabs(y) > sigma) {
+        g2 = (y * y * y) / 12. * log(v);
+    } else if (fabs(x) > sigma && (fabs(y) + fabs(z)) < sigma) {
+        g2 = -((x * x * x) / 12.) * log(u);
+    } else if (fabs(x) < sigma && fabs(y) < sigma && fabs(z) < sigma) {
+        g2 = 0.;
+    }
+    return g2;
+}
+
+/* magnetic_initialize.cpp:1486-1505 */
+static double newell_G1(double x, double y, double z, double dx, double dy, double dz) {
+    double xp = x + dx, xm = x - dx, yp = y + dy / 2., ym = y - dy / 2., zp = z + dz, zm = z - dz;
+    return 4. * gfo_newell_G2(x, yp, z) - 4. * gfo_newell_G2(x, ym, z)
+         - 2. * gfo_newell_G2(x, yp, zp) + 2. * gfo_newell_G2(x, ym, zp)
+         - 2. * gfo_newell_G2(x, yp, zm) + 2. * gfo_newell_G2(x, ym, zm)
+         - 2. * gfo_newell_G2(xm, yp, z) + 2. * gfo_newell_G2(xm, ym, z)
+         + gfo_newell_G2(xm, yp, zp) - gfo_newell_G2(xm, ym, zp)
+         + gfo_newell_G2(xm, yp, zm) - gfo_newell_G2(xm, ym, zm)
+         - 2. * gfo_newell_G2(xp, yp, z) + 2. * gfo_newell_G2(xp, ym, z)
+         + gfo_newell_G2(xp, yp, zp) - gfo_newell_G2(xp, ym, zp)
+         + gfo_newell_G2(xp, yp, zm) - gfo_newell_G2(xp, ym, zm);
+}
+
+/* magnetic_initialize.cpp:1478-1484 */
+double gfo_newell_B(double x, double y, double z, double dx, double dy, double dz) {
+    double yp = y + dy / 2., ym = y - dy / 2.;
+    return newell_G1(x, yp, z, dx, dy, dz) - newell_G1(x, ym, z, dx, dy, dz);
+}
+
+/* magnetic_initialize.cpp:1176-1373: the nine tensor kernels evaluated on
+ * offsets |i| <= n/2-1 per axis, placed circularly (offset d at d mod n,
+ * index n/2 left zero) and divided by n.  Order of out9:
+ * Axyz, Ayzx, Azxy, Bxyz, Byzx, Bzxy, Byxz, Bzyx, Bxzy. */
+void gfo_newell_real(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz, double* out9) {
+    int64_t n = nx * ny * nz;
+    double twopi = 2. * PI;
+    memset(out9, 0, sizeof(double) * 9 * n);
+    for (int64_t i = 1 - nx / 2; i < nx / 2; i++)
+        for (int64_t j = 1 - ny / 2; j < ny / 2; j++)
+            for (int64_t k = 1 - nz / 2; k < nz / 2; k++) {
+                double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz;
+                int64_t pi_ = (i + nx) % nx, pj = (j + ny) % ny, pk = (k + nz) % nz;
+                int64_t id = (pi_ * ny + pj) * nz + pk;
+                out9[0 * n + id] = gfo_newell_A(x, y, z, dx, dy, dz) / (2. * twopi);
+                out9[1 * n + id] = gfo_newell_A(y, z, x, dy, dz, dx) / (2. * twopi);
+                out9[2 * n + id] = gfo_newell_A(z, x, y, dz, dx, dy) / (2. * twopi);
+                out9[3 * n + id] = gfo_newell_B(x, y, z, dx, dy, dz) / (2. * twopi);
+                out9[4 * n + id] = gfo_newell_B(y, z, x, dy, dz, dx) / (2. * twopi);
+                out9[5 * n + id] = gfo_newell_B(z, x, y, dz, dx, dy) / (2. * twopi);
+                out9[6 * n + id] = gfo_newell_B(y, x, z, dy, dx, dz) / (2. * twopi);
+                out9[7 * n + id] = gfo_newell_B(z, y, x, dz, dy, dx) / (2. * twopi);
+                out9[8 * n + id] = gfo_newell_B(x, z, y, dx, dz, dy) / (2. * twopi);
+            }
+    for (int64_t s = 0; s < 9; s++)
+        for (int64_t id = 0; id < n; id++) out9[s * n + id] = out9[s * n + id] / (double)n;
+}
+
+/* ------------------------------------------------------------ helpers */
+static double* dalloc(int64_t n) { return (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+static int64_t ID(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * c->ny + j) * c->nz + k; }
+static const gf_material* matof(const gfo_ctx* c, int64_t id) {
+    uint32_t t = c->cell[id];
+    return t ? &c->mat[t - 1] : NULL;
+}
+static int isFM(const gfo_ctx* c, int64_t id) { const gf_material* m = matof(c, id); return m && m->if_FM; }
+static int isAFM(const gfo_ctx* c, int64_t id) { const gf_material* m = matof(c, id); return m && m->if_AFM; }
+
+/* mathlib.cpp:46-75 */
+static void glb2crt(const gfo_ctx* c, double ix, double iy, double iz, double* ox, double* oy, double* oz) {
+    if (c->flags & GF_FLAG_ORIENT_ROTATE) {
+        *ox = c->Tg2c[0][0] * ix + c->Tg2c[0][1] * iy + c->Tg2c[0][2] * iz;
+        *oy = c->Tg2c[1][0] * ix + c->Tg2c[1][1] * iy + c->Tg2c[1][2] * iz;
+        *oz = c->Tg2c[2][0] * ix + c->Tg2c[2][1] * iy + c->Tg2c[2][2] * iz;
+    } else { *ox = ix; *oy = iy; *oz = iz; }
+}
+static void crt2glb(const gfo_ctx* c, double ix, double iy, double iz, double* ox, double* oy, double* oz) {
+    if (c->flags & GF_FLAG_ORIENT_ROTATE) {
+        *ox = c->Tc2g[0][0] * ix + c->Tc2g[0][1] * iy + c->Tc2g[0][2] * iz;
+        *oy = c->Tc2g[1][0] * ix + c->Tc2g[1][1] * iy + c->Tc2g[1][2] * iz;
+        *oz = c->Tc2g[2][0] * ix + c->Tc2g[2][1] * iy + c->Tc2g[2][2] * iz;
+    } else { *ox = ix; *oy = iy; *oz = iz; }
+}
+
+/* Interface masks, magnetic_initialize.cpp:136-379 (FM) and :635-~870
+ * (AFM): a face is a surface when exactly one side is magnetic of the
+ * kind, or at a non-periodic wall next to a magnetic cell. */
+static void build_masks(gfo_ctx* c, int (*pred)(const gfo_ctx*, int64_t),
+                        unsigned char* sYZ, unsigned char* sXZ, unsigned char* sXY) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    for (int64_t j = 0; j < ny; j++)
+        for (int64_t k = 0; k < nz; k++) {
+            int f1 = pred(c, ID(c, 0, j, k)), f2 = pred(c, ID(c, nx - 1, j, k));
+            if (c->per[0]) { sYZ[(0 * ny + j) * nz + k] = sYZ[(nx * ny + j) * nz + k] = (unsigned char)(f1 ^ f2); }
+            else { sYZ[(0 * ny + j) * nz + k] = (unsigned char)f1; sYZ[(nx * ny + j) * nz + k] = (unsigned char)f2; }
+        }
+    for (int64_t i = 1; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k < nz; k++)
+                sYZ[(i * ny + j) * nz + k] = (unsigned char)(pred(c, ID(c, i - 1, j, k)) ^ pred(c, ID(c, i, j, k)));
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t k = 0; k < nz; k++) {
+            int f1 = pred(c, ID(c, i, 0, k)), f2 = pred(c, ID(c, i, ny - 1, k));
+            if (c->per[1]) { sXZ[(i * (ny + 1) + 0) * nz + k] = sXZ[(i * (ny + 1) + ny) * nz + k] = (unsigned char)(f1 ^ f2); }
+            else { sXZ[(i * (ny + 1) + 0) * nz + k] = (unsigned char)f1; sXZ[(i * (ny + 1) + ny) * nz + k] = (unsigned char)f2; }
+        }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 1; j < ny; j++)
+            for (int64_t k = 0; k < nz; k++)
+                sXZ[(i * (ny + 1) + j) * nz + k] = (unsigned char)(pred(c, ID(c, i, j - 1, k)) ^ pred(c, ID(c, i, j, k)));
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++) {
+            int f1 = pred(c, ID(c, i, j, 0)), f2 = pred(c, ID(c, i, j, nz - 1));
+            if (c->per[2]) { sXY[(i * ny + j) * (nz + 1) + 0] = sXY[(i * ny + j) * (nz + 1) + nz] = (unsigned char)(f1 ^ f2); }
+            else { sXY[(i * ny + j) * (nz + 1) + 0] = (unsigned char)f1; sXY[(i * ny + j) * (nz + 1) + nz] = (unsigned char)f2; }
+        }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 1; k < nz; k++)
+                sXY[(i * ny + j) * (nz + 1) + k] = (unsigned char)(pred(c, ID(c, i, j, k - 1)) ^ pred(c, ID(c, i, j, k)));
+}
+
+static int sYZ(const unsigned char* s, const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return s[(i * c->ny + j) * c->nz + k]; }
+static int sXZ(const unsigned char* s, const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return s[(i * (c->ny + 1) + j) * c->nz + k]; }
+static int sXY(const unsigned char* s, const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return s[(i * c->ny + j) * (c->nz + 1) + k]; }
+
+static double hmean(double a1, double a2) { return 2. * a1 * a2 / (a1 + a2); }
+
+/* Face exchange constants, magnetic_initialize.cpp:381-524 (FM) and
+ * :870-~1040 (AFM); sel: 0 = FM Aex, 1 = Aex_AFM1, 2 = Aex_AFM2. */
+static double aex_of(const gf_material* m, int sel) { return sel == 0 ? m->Aex : (sel == 1 ? m->Aex_AFM1 : m->Aex_AFM2); }
+static void build_aij(gfo_ctx* c, int (*pred)(const gfo_ctx*, int64_t), const unsigned char* mYZ,
+                      const unsigned char* mXZ, const unsigned char* mXY, int sel, double** A) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    for (int64_t id = 0; id < c->n; id++) {
+        if (!pred(c, id)) continue;
+        const gf_material* m = matof(c, id);
+        int64_t x = id / (ny * nz), y = (id - x * (ny * nz)) / nz, z = id - x * (ny * nz) - y * nz;
+        double a = aex_of(m, sel);
+        A[0][id] = sYZ(mYZ, c, x + 1, y, z) ? a : hmean(a, aex_of(matof(c, ID(c, x == nx - 1 ? 0 : x + 1, y, z)), sel));
+        A[1][id] = sYZ(mYZ, c, x, y, z) ? a : hmean(a, aex_of(matof(c, ID(c, x == 0 ? nx - 1 : x - 1, y, z)), sel));
+        A[2][id] = sXZ(mXZ, c, x, y + 1, z) ? a : hmean(a, aex_of(matof(c, ID(c, x, y == ny - 1 ? 0 : y + 1, z)), sel));
+        A[3][id] = sXZ(mXZ, c, x, y, z) ? a : hmean(a, aex_of(matof(c, ID(c, x, y == 0 ? ny - 1 : y - 1, z)), sel));
+        A[4][id] = sXY(mXY, c, x, y, z + 1) ? a : hmean(a, aex_of(matof(c, ID(c, x, y, z == nz - 1 ? 0 : z + 1)), sel));
+        A[5][id] = sXY(mXY, c, x, y, z) ? a : hmean(a, aex_of(matof(c, ID(c, x, y, z == 0 ? nz - 1 : z - 1)), sel));
+    }
+}
+
+/* ----------------------------------------------------------- create */
+gfo_ctx* gfo_create(const gf_desc* d) {
+    if (!d || d->nx < 1 || d->ny < 1 || d->nz < 1) return NULL;
+    gfo_ctx* c = (gfo_ctx*)calloc(1, sizeof(gfo_ctx));
+    c->nx = d->nx; c->ny = d->ny; c->nz = d->nz; c->n = d->nx * d->ny * d->nz;
+    c->dx = d->dx; c->dy = d->dy; c->dz = d->dz;
+    for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
+    c->flags = d->flags; c->dt = d->dt; c->time = 0.;
+    c->nmat = d->n_materials;
+    c->mat = (gf_material*)malloc(sizeof(gf_material) * (size_t)(c->nmat > 0 ? c->nmat : 1));
+    memcpy(c->mat, d->materials, sizeof(gf_material) * (size_t)c->nmat);
+    c->cell = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)c->n);
+    memcpy(c->cell, d->material_cell, sizeof(uint32_t) * (size_t)c->n);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) { c->Tg2c[a][b] = d->Tg2c[a][b]; c->Tc2g[b][a] = d->Tg2c[a][b]; }  /* read.cpp:105-107 */
+    for (int a = 0; a < 3; a++) {
+        c->Hext_stat[a] = d->Hext_stat[a]; c->Hext_altr[a] = d->Hext_altr[a];
+        c->H_altr_freq[a] = d->H_altr_freq[a]; c->demag_fac[a] = d->demag_fac[a]; c->Hext[a] = 0.;
+    }
+    c->npre = d->n_prescribe_Hext;
+    if (c->npre > 0) {
+        c->pre = (double*)malloc(sizeof(double) * 4 * (size_t)c->npre);
+        memcpy(c->pre, d->prescribe_Hext, sizeof(double) * 4 * (size_t)c->npre);
+    }
+    c->pre_idx = 0;
+    /* global.cpp:79-116 */
+    for (int i = 0; i < c->nmat; i++) { if (c->mat[i].if_FM) c->if_FM_all = 1; if (c->mat[i].if_AFM) c->if_AFM_all = 1; }
+    for (int64_t id = 0; id < c->n; id++) { if (isFM(c, id)) c->NFM++; if (isAFM(c, id)) c->NAFM++; }
+    int64_t n = c->n;
+    for (int a = 0; a < 3; a++) {
+        c->m[a] = dalloc(n); c->ms[a] = dalloc(n); c->dm[a] = dalloc(n);
+        for (int s = 0; s < 4; s++) { c->k[s][a] = dalloc(n); c->k1[s][a] = dalloc(n); c->k2[s][a] = dalloc(n); }
+        c->m1[a] = dalloc(n); c->m2[a] = dalloc(n); c->ms1[a] = dalloc(n); c->ms2[a] = dalloc(n);
+        c->dm1[a] = dalloc(n); c->dm2[a] = dalloc(n);
+        c->H[a] = dalloc(n); c->Hnu[a] = dalloc(n);
+    }
+    int64_t nyz = c->ny * c->nz;
+    c->surfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
+    c->surfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
+    c->surfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
+    c->asurfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
+    c->asurfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
+    c->asurfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
+    for (int f = 0; f < 6; f++) { c->Aij[f] = dalloc(n); c->Aij1[f] = dalloc(n); c->Aij2[f] = dalloc(n); }
+    if (c->if_FM_all) {
+        build_masks(c, isFM, c->surfYZ, c->surfXZ, c->surfXY);
+        build_aij(c, isFM, c->surfYZ, c->surfXZ, c->surfXY, 0, c->Aij);
+    }
+    if (c->if_AFM_all) {
+        build_masks(c, isAFM, c->asurfYZ, c->asurfXZ, c->asurfXY);
+        build_aij(c, isAFM, c->asurfYZ, c->asurfXZ, c->asurfXY, 1, c->Aij1);
+        build_aij(c, isAFM, c->asurfYZ, c->asurfXZ, c->asurfXY, 2, c->Aij2);
+    }
+    /* demag spectra: magnetic_initialize.cpp:1099-1112 */
+    if ((c->flags & GF_FLAG_MAGNETOSTATICS) && !(c->flags & GF_FLAG_MAG_1D) && !(c->flags & GF_FLAG_DEMAG_FACTOR)) {
+        if (!is_pow2(c->nx) || !is_pow2(c->ny) || !is_pow2(c->nz)) { gfo_destroy(c); return NULL; }
+        int64_t nh = c->nx * c->ny * (c->nz / 2 + 1);
+        double* real9 = dalloc(9 * n);
+        gfo_newell_real(c->nx, c->ny, c->nz, c->dx, c->dy, c->dz, real9);
+        for (int s = 0; s < 9; s++) {
+            c->kspec[s] = dalloc(2 * nh);
+            gfo_rfft3(c->nx, c->ny, c->nz, real9 + s * n, c->kspec[s]);
+        }
+        free(real9);
+    }
+    return c;
+}
+
+void gfo_destroy(gfo_ctx* c) {
+    if (!c) return;
+    free(c->mat); free(c->cell); free(c->pre);
+    for (int a = 0; a < 3; a++) {
+        free(c->m[a]); free(c->ms[a]); free(c->dm[a]);
+        for (int s = 0; s < 4; s++) { free(c->k[s][a]); free(c->k1[s][a]); free(c->k2[s][a]); }
+        free(c->m1[a]); free(c->m2[a]); free(c->ms1[a]); free(c->ms2[a]); free(c->dm1[a]); free(c->dm2[a]);
+        free(c->H[a]); free(c->Hnu[a]);
+    }
+    free(c->surfYZ); free(c->surfXZ); free(c->surfXY); free(c->asurfYZ); free(c->asurfXZ); free(c->asurfXY);
+    for (int f = 0; f < 6; f++) { free(c->Aij[f]); free(c->Aij1[f]); free(c->Aij2[f]); }
+    for (int s = 0; s < 9; s++) free(c->kspec[s]);
+    free(c);
+}
+
+static double** field_ptrs(gfo_ctx* c, int id) {
+    switch (id) {
+    case GF_FIELD_M: return c->m;
+    case GF_FIELD_M_STORE: return c->ms;
+    case GF_FIELD_HSTAT: return c->H;
+    case GF_FIELD_HEXT_NONUNIF: return c->Hnu;
+    case GF_FIELD_M_AFM1: return c->m1;
+    case GF_FIELD_M_AFM2: return c->m2;
+    default: return NULL;
+    }
+}
+
+/* Uploading m also resets the store (magnetic_initialize.cpp:1081-1097). */
+int gfo_set_field(gfo_ctx* c, int id, const double* src, size_t count) {
+    double** f = field_ptrs(c, id);
+    if (!f || count != (size_t)(3 * c->n)) return -1;
+    for (int a = 0; a < 3; a++) memcpy(f[a], src + a * c->n, sizeof(double) * (size_t)c->n);
+    if (id == GF_FIELD_M) for (int a = 0; a < 3; a++) memcpy(c->ms[a], c->m[a], sizeof(double) * (size_t)c->n);
+    if (id == GF_FIELD_M_AFM1) for (int a = 0; a < 3; a++) memcpy(c->ms1[a], c->m1[a], sizeof(double) * (size_t)c->n);
+    if (id == GF_FIELD_M_AFM2) for (int a = 0; a < 3; a++) memcpy(c->ms2[a], c->m2[a], sizeof(double) * (size_t)c->n);
+    return 0;
+}
+
+int gfo_get_field(const gfo_ctx* c, int id, double* dst, size_t count) {
+    double** f = field_ptrs((gfo_ctx*)c, id);
+    if (id == GFO_FIELD_DM) f = ((gfo_ctx*)c)->dm;
+    if (!f || count != (size_t)(3 * c->n)) return -1;
+    for (int a = 0; a < 3; a++) memcpy(dst + a * c->n, f[a], sizeof(double) * (size_t)c->n);
+    return 0;
+}
+
+/* ----------------------------------------------------------- drives */
+/* global.cpp:513-549 (the H part; the E table belongs to the FE path) */
+void gfo_time_advance(gfo_ctx* c) {
+    c->time = c->time + c->dt;
+    if (c->flags & GF_FLAG_PRESCRIBE_HEXT) {
+        if (c->pre_idx < c->npre && (c->time - c->dt) > c->pre[4 * c->pre_idx]) c->pre_idx = c->pre_idx + 1;
+        if (c->pre_idx < c->npre) {
+            c->Hext[0] = c->pre[4 * c->pre_idx + 1];
+            c->Hext[1] = c->pre[4 * c->pre_idx + 2];
+            c->Hext[2] = c->pre[4 * c->pre_idx + 3];
+        } else { c->Hext[0] = 0.; c->Hext[1] = 0.; c->Hext[2] = 0.; }
+    }
+}
+
+/* magnetic_solver.cpp:1834-1861: drive at t-dt (stage 1), t-dt/2
+ * (stages 2, 3) and t (stage 4). */
+static void update_external_H(gfo_ctx* c, int stage) {
+    double local_time = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    for (int a = 0; a < 3; a++)
+        c->Hext[a] = c->Hext_stat[a] + c->Hext_altr[a] * sin(2. * PI * c->H_altr_freq[a] * local_time);
+}
+
+/* ------------------------------------------------------------ stencil */
+/* magnetic_solver.cpp:1639-1831 (FM, with DMI surface slope) and
+ * magnetic_AFM_solver.cpp:1837-2219 (AFM, slope zero): d[0..3] =
+ * dmxdx, dmydy, dmzdx, dmzdy; d2[axis][side][comp] with side 0 = f. */
+static void stencil(const gfo_ctx* c, double* const* s, const unsigned char* mYZ, const unsigned char* mXZ,
+                    const unsigned char* mXY, int dmi, double A, double D, int64_t i, int64_t j, int64_t k,
+                    double d[4], double d2[3][2][3]) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    double dx = c->dx, dy = c->dy, dz = c->dz;
+    int64_t id = ID(c, i, j, k);
+    double mx_c = s[0][id], my_c = s[1][id], mz_c = s[2][id];
+    double bw[3], fw[3];
+    int64_t nb;
+    /* x */
+    if (sYZ(mYZ, c, i, j, k)) {
+        if (dmi) { bw[0] = -D / 2. / A * mz_c; bw[1] = 0.; bw[2] = D / 2. / A * mx_c; }
+        else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+    } else {
+        nb = ID(c, i == 0 ? nx - 1 : i - 1, j, k);
+        bw[0] = (mx_c - s[0][nb]) / dx; bw[1] = (my_c - s[1][nb]) / dx; bw[2] = (mz_c - s[2][nb]) / dx;
+    }
+    if (sYZ(mYZ, c, i + 1, j, k)) {
+        if (dmi) { fw[0] = -D / 2. / A * mz_c; fw[1] = 0.; fw[2] = D / 2. / A * mx_c; }
+        else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+    } else {
+        nb = ID(c, i == nx - 1 ? 0 : i + 1, j, k);
+        fw[0] = (s[0][nb] - mx_c) / dx; fw[1] = (s[1][nb] - my_c) / dx; fw[2] = (s[2][nb] - mz_c) / dx;
+    }
+    d[0] = (bw[0] + fw[0]) * 0.5; d[2] = (bw[2] + fw[2]) * 0.5;
+    for (int a = 0; a < 3; a++) { d2[0][0][a] = fw[a] / dx; d2[0][1][a] = -bw[a] / dx; }
+    /* y */
+    if (sXZ(mXZ, c, i, j, k)) {
+        if (dmi) { bw[0] = 0.; bw[1] = -D / 2. / A * mz_c; bw[2] = D / 2. / A * my_c; }
+        else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+    } else {
+        nb = ID(c, i, j == 0 ? ny - 1 : j - 1, k);
+        bw[0] = (mx_c - s[0][nb]) / dy; bw[1] = (my_c - s[1][nb]) / dy; bw[2] = (mz_c - s[2][nb]) / dy;
+    }
+    if (sXZ(mXZ, c, i, j + 1, k)) {
+        if (dmi) { fw[0] = 0.; fw[1] = -D / 2. / A * mz_c; fw[2] = D / 2. / A * my_c; }
+        else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+    } else {
+        nb = ID(c, i, j == ny - 1 ? 0 : j + 1, k);
+        fw[0] = (s[0][nb] - mx_c) / dy; fw[1] = (s[1][nb] - my_c) / dy; fw[2] = (s[2][nb] - mz_c) / dy;
+    }
+    d[1] = (bw[1] + fw[1]) * 0.5; d[3] = (bw[2] + fw[2]) * 0.5;
+    for (int a = 0; a < 3; a++) { d2[1][0][a] = fw[a] / dy; d2[1][1][a] = -bw[a] / dy; }
+    /* z: slope zero at a surface for both kinds */
+    if (sXY(mXY, c, i, j, k)) { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+    else {
+        nb = ID(c, i, j, k == 0 ? nz - 1 : k - 1);
+        bw[0] = (mx_c - s[0][nb]) / dz; bw[1] = (my_c - s[1][nb]) / dz; bw[2] = (mz_c - s[2][nb]) / dz;
+    }
+    if (sXY(mXY, c, i, j, k + 1)) { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+    else {
+        nb = ID(c, i, j, k == nz - 1 ? 0 : k + 1);
+        fw[0] = (s[0][nb] - mx_c) / dz; fw[1] = (s[1][nb] - my_c) / dz; fw[2] = (s[2][nb] - mz_c) / dz;
+    }
+    for (int a = 0; a < 3; a++) { d2[2][0][a] = fw[a] / dz; d2[2][1][a] = -bw[a] / dz; }
+}
+
+/* ------------------------------------------------------- FM stage */
+/* magnetic_solver.cpp:3-229 (RK1; RK2-4 at :285, :568, :849 differ only in
+ * the drive time and the output array), plus get_dm :1449-1457 at s=4. */
+void gfo_mag_get_dm(gfo_ctx* c, int stage) {
+    if (!(c->flags & GF_FLAG_PRESCRIBE_HEXT)) update_external_H(c, stage);
+    int64_t ny = c->ny, nz = c->nz;
+    double** out = c->k[stage - 1];
+    for (int64_t i = 0; i < c->n; i++) {
+        const gf_material* mat = matof(c, i);
+        if (!mat || !mat->if_FM) continue;
+        int64_t x = i / (ny * nz), y = (i - x * (ny * nz)) / nz, z = i - x * (ny * nz) - y * nz;
+        double mgx = c->ms[0][i], mgy = c->ms[1][i], mgz = c->ms[2][i];
+        double mx, my, mz;
+        glb2crt(c, mgx, mgy, mgz, &mx, &my, &mz);
+        double Ms = mat->Ms;
+        double K1 = mat->K1 / MU0 / Ms, K2 = mat->K2 / MU0 / Ms;
+        double B1 = mat->B1 / MU0 / Ms, B2 = mat->B2 / MU0 / Ms;
+        double Aexi = mat->Aex, iDMIi = mat->iDMI;
+        double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
+        double d[4], d2[3][2][3];
+        stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, Aexi, iDMIi, x, y, z, d, d2);
+        double hac[3] = {0., 0., 0.}, ha[3], hec[3], he[3], hx[3], hd[3], H[3];
+        if (mat->anisotropy_type == 1) {
+            hac[0] = -2. * (K1 * (my * my + mz * mz) + K2 * my * my * mz * mz) * mx;
+            hac[1] = -2. * (K1 * (mx * mx + mz * mz) + K2 * mx * mx * mz * mz) * my;
+            hac[2] = -2. * (K1 * (mx * mx + my * my) + K2 * mx * mx * my * my) * mz;
+        } else if (mat->anisotropy_type == 2) {
+            hac[0] = 0.; hac[1] = 0.; hac[2] = 2. * K1 * mz;
+        }
+        crt2glb(c, hac[0], hac[1], hac[2], &ha[0], &ha[1], &ha[2]);
+        hec[0] = -2. * (B1 * (exx)*mx + B2 * ((exy)*my + (exz)*mz));
+        hec[1] = -2. * (B1 * (eyy)*my + B2 * ((exy)*mx + (eyz)*mz));
+        hec[2] = -2. * (B1 * (ezz)*mz + B2 * ((exz)*mx + (eyz)*my));
+        crt2glb(c, hec[0], hec[1], hec[2], &he[0], &he[1], &he[2]);
+        for (int a = 0; a < 3; a++)
+            hx[a] = 2. / MU0 / Ms * (c->Aij[0][i] * d2[0][0][a] + c->Aij[1][i] * d2[0][1][a] + c->Aij[2][i] * d2[1][0][a]
+                                    + c->Aij[3][i] * d2[1][1][a] + c->Aij[4][i] * d2[2][0][a] + c->Aij[5][i] * d2[2][1][a]);
+        hd[0] = 2. * iDMIi / MU0 / Ms * d[2];
+        hd[1] = 2. * iDMIi / MU0 / Ms * d[3];
+        hd[2] = -2. * iDMIi / MU0 / Ms * (d[0] + d[1]);
+        for (int a = 0; a < 3; a++)
+            H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + hd[a] + c->Hext[a] + c->Hnu[a][i];
+        double alpha = mat->FM_damping, gyro = mat->gyro;
+        double precess = -gyro / (1 + alpha * alpha) * c->dt;
+        double px = mgy * H[2] - mgz * H[1], py = mgz * H[0] - mgx * H[2], pz = mgx * H[1] - mgy * H[0];
+        double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
+        out[0][i] = precess * px + alpha * precess * qx;
+        out[1][i] = precess * py + alpha * precess * qy;
+        out[2][i] = precess * pz + alpha * precess * qz;
+    }
+    if (stage == 4)
+        for (int a = 0; a < 3; a++)
+            for (int64_t i = 0; i < c->n; i++)
+                c->dm[a][i] = c->k[0][a][i] / 6. + c->k[1][a][i] / 3. + c->k[2][a][i] / 3. + c->k[3][a][i] / 6.;
+}
+
+/* magnetic_solver.cpp:1515-1636 */
+void gfo_mag_update(gfo_ctx* c, int stage) {
+    for (int a = 0; a < 3; a++) {
+        double *m = c->m[a], *s = c->ms[a];
+        if (stage == 1 || stage == 2) { const double* k = c->k[stage - 1][a]; for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i] * 0.5; }
+        else if (stage == 3) { const double* k = c->k[2][a]; for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i]; }
+        else { const double* dm = c->dm[a]; for (int64_t i = 0; i < c->n; i++) { m[i] = m[i] + dm[i]; s[i] = m[i]; } }
+    }
+}
+
+/* ------------------------------------------------------ AFM stage */
+/* magnetic_AFM_solver.cpp:3-430 (RK1; RK2-4 :431, :859, :1287), get_dm_AFM
+ * :1715-1729.  Sublattice loops read only stores, so their order is free. */
+void gfo_afm_get_dm(gfo_ctx* c, int stage) {
+    update_external_H(c, stage);   /* unconditional in the AFM path (:61-64) */
+    int64_t ny = c->ny, nz = c->nz;
+    for (int sub = 0; sub < 2; sub++) {
+        double** st = sub == 0 ? c->ms1 : c->ms2;
+        double** other = sub == 0 ? c->ms2 : c->ms1;
+        double** A = sub == 0 ? c->Aij1 : c->Aij2;
+        double** out = sub == 0 ? c->k1[stage - 1] : c->k2[stage - 1];
+        for (int64_t i = 0; i < c->n; i++) {
+            const gf_material* mat = matof(c, i);
+            if (!mat || !mat->if_AFM) continue;
+            int64_t x = i / (ny * nz), y = (i - x * (ny * nz)) / nz, z = i - x * (ny * nz) - y * nz;
+            double ux = mat->uniaxis_AFM[0], uy = mat->uniaxis_AFM[1], uz = mat->uniaxis_AFM[2];
+            double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
+            double mgx = st[0][i], mgy = st[1][i], mgz = st[2][i];
+            double mx, my, mz;
+            glb2crt(c, mgx, mgy, mgz, &mx, &my, &mz);
+            double Ms = sub == 0 ? mat->Ms_AFM1 : mat->Ms_AFM2;
+            double K1 = (sub == 0 ? mat->K1_AFM1 : mat->K1_AFM2) / MU0 / Ms;
+            double udotm = mx * ux + my * uy + mz * uz;
+            double B1 = (sub == 0 ? mat->B1_AFM1 : mat->B1_AFM2) / MU0 / Ms;
+            double B2 = (sub == 0 ? mat->B2_AFM1 : mat->B2_AFM2) / MU0 / Ms;
+            double JAFM = -1. * mat->J_AFM / MU0 / Ms;
+            double d[4], d2[3][2][3];
+            stencil(c, st, c->asurfYZ, c->asurfXZ, c->asurfXY, 0, 0., 0., x, y, z, d, d2);
+            double hac[3], ha[3], hec[3], he[3], hx[3], hi[3], H[3];
+            hac[0] = 2. * K1 * udotm * ux; hac[1] = 2. * K1 * udotm * uy; hac[2] = 2. * K1 * udotm * uz;
+            crt2glb(c, hac[0], hac[1], hac[2], &ha[0], &ha[1], &ha[2]);
+            hec[0] = -1. * (B1 * (exx)*mx + B2 * ((exy)*my + (exz)*mz));
+            hec[1] = -1. * (B1 * (eyy)*my + B2 * ((exy)*mx + (eyz)*mz));
+            hec[2] = -1. * (B1 * (ezz)*mz + B2 * ((exz)*mx + (eyz)*my));
+            crt2glb(c, hec[0], hec[1], hec[2], &he[0], &he[1], &he[2]);
+            for (int a = 0; a < 3; a++)
+                hx[a] = 2. / MU0 / Ms * (A[0][i] * d2[0][0][a] + A[1][i] * d2[0][1][a] + A[2][i] * d2[1][0][a]
+                                        + A[3][i] * d2[1][1][a] + A[4][i] * d2[2][0][a] + A[5][i] * d2[2][1][a]);
+            for (int a = 0; a < 3; a++) hi[a] = JAFM * other[a][i];
+            for (int a = 0; a < 3; a++) H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + hi[a] + c->Hext[a];
+            double alpha = sub == 0 ? mat->damping_AFM1 : mat->damping_AFM2;
+            double gyro = sub == 0 ? mat->gyro_AFM1 : mat->gyro_AFM2;
+            double precess = -gyro / (1. + alpha * alpha) * c->dt;
+            double px = mgy * H[2] - mgz * H[1], py = mgz * H[0] - mgx * H[2], pz = mgx * H[1] - mgy * H[0];
+            double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
+            out[0][i] = precess * px + alpha * precess * qx;
+            out[1][i] = precess * py + alpha * precess * qy;
+            out[2][i] = precess * pz + alpha * precess * qz;
+        }
+    }
+    if (stage == 4)
+        for (int a = 0; a < 3; a++)
+            for (int64_t i = 0; i < c->n; i++) {
+                c->dm1[a][i] = c->k1[0][a][i] / 6. + c->k1[1][a][i] / 3. + c->k1[2][a][i] / 3. + c->k1[3][a][i] / 6.;
+                c->dm2[a][i] = c->k2[0][a][i] / 6. + c->k2[1][a][i] / 3. + c->k2[2][a][i] / 3. + c->k2[3][a][i] / 6.;
+            }
+}
+
+/* magnetic_AFM_solver.cpp:1731-1834 */
+void gfo_afm_update(gfo_ctx* c, int stage) {
+    for (int sub = 0; sub < 2; sub++)
+        for (int a = 0; a < 3; a++) {
+            double* m = sub == 0 ? c->m1[a] : c->m2[a];
+            double* s = sub == 0 ? c->ms1[a] : c->ms2[a];
+            if (stage == 1 || stage == 2) {
+                const double* k = (sub == 0 ? c->k1 : c->k2)[stage - 1][a];
+                for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i] * 0.5;
+            } else if (stage == 3) {
+                const double* k = (sub == 0 ? c->k1 : c->k2)[2][a];
+                for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i];
+            } else {
+                const double* dm = sub == 0 ? c->dm1[a] : c->dm2[a];
+                for (int64_t i = 0; i < c->n; i++) { m[i] = m[i] + dm[i]; s[i] = m[i]; }
+            }
+        }
+}
+
+/* ------------------------------------------------------ statics */
+/* magnetostatic_solver.cpp:222-249 */
+static void demag_1d(gfo_ctx* c) {
+    for (int64_t i = 0; i < c->n; i++) {
+        const gf_material* mat = matof(c, i);
+        if (!mat) continue;
+        if (mat->if_FM) { c->H[0][i] = 0.; c->H[1][i] = 0.; c->H[2][i] = -mat->Ms * c->ms[2][i]; }
+        else if (mat->if_AFM) {
+            c->H[0][i] = 0.; c->H[1][i] = 0.;
+            c->H[2][i] = -mat->Ms_AFM1 * c->ms1[2][i] - mat->Ms_AFM2 * c->ms2[2][i];
+        }
+    }
+}
+
+/* magnetostatic_solver.cpp:114-219 */
+static void demag_3d(gfo_ctx* c) {
+    int64_t n = c->n;
+    if (c->flags & GF_FLAG_DEMAG_FACTOR) {
+        for (int64_t i = 0; i < n; i++) {
+            const gf_material* mat = matof(c, i);
+            if (!mat) continue;
+            if (mat->if_FM) {
+                for (int a = 0; a < 3; a++) c->H[a][i] = mat->Ms * c->ms[a][i] * c->demag_fac[a];
+            } else if (mat->if_AFM) {
+                for (int a = 0; a < 3; a++)
+                    c->H[a][i] = (mat->Ms_AFM1 * c->ms1[a][i] + mat->Ms_AFM2 * c->ms2[a][i]) * c->demag_fac[a];
+            }
+        }
+        return;
+    }
+    int64_t nh = c->nx * c->ny * (c->nz / 2 + 1);
+    double* M = dalloc(3 * n);
+    double* Mk = dalloc(3 * 2 * nh);
+    double* Hk = dalloc(2 * nh);
+    for (int64_t i = 0; i < n; i++) {
+        const gf_material* mat = matof(c, i);
+        if (!mat) continue;
+        if (mat->if_FM) {
+            for (int a = 0; a < 3; a++) M[a * n + i] = mat->Ms * c->ms[a][i];
+        } else if (mat->if_AFM) {
+            for (int a = 0; a < 3; a++) M[a * n + i] = mat->Ms_AFM1 * c->ms1[a][i] + mat->Ms_AFM2 * c->ms2[a][i];
+        }
+    }
+    for (int a = 0; a < 3; a++) gfo_rfft3(c->nx, c->ny, c->nz, M + a * n, Mk + a * 2 * nh);
+    /* H_k = (M_a K_a + M_b K_b + M_c K_c) / (-dx dy dz), complex products
+     * in the order of :163-177; spectra order Axyz,Ayzx,Azxy,Bxyz,Byzx,Bzxy,Byxz,Bzyx,Bxzy */
+    static const int comp[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+    static const int spec[3][3] = {{0, 3, 8}, {1, 4, 6}, {2, 5, 7}};
+    double den = -c->dx * c->dy * c->dz;
+    for (int a = 0; a < 3; a++) {
+        for (int64_t q = 0; q < nh; q++) {
+            double sr = 0., si = 0.;
+            for (int t = 0; t < 3; t++) {
+                const double* mk = Mk + comp[a][t] * 2 * nh;
+                const double* kk = c->kspec[spec[a][t]];
+                double ar = mk[2 * q], ai = mk[2 * q + 1], br = kk[2 * q], bi = kk[2 * q + 1];
+                double pr = ar * br - ai * bi, pi_ = ar * bi + ai * br;
+                if (t == 0) { sr = pr; si = pi_; } else { sr = sr + pr; si = si + pi_; }
+            }
+            Hk[2 * q] = sr / den; Hk[2 * q + 1] = si / den;
+        }
+        gfo_irfft3(c->nx, c->ny, c->nz, Hk, c->H[a]);
+    }
+    free(M); free(Mk); free(Hk);
+}
+
+void gfo_demag(gfo_ctx* c) {
+    if (c->flags & GF_FLAG_MAG_1D) demag_1d(c);
+    else demag_3d(c);
+}
+
+/* --------------------------------------------------------- protocol */
+/* main.cpp:243-483 restricted to the magnetic physics: time march; for
+ * each stage all get_d*, then all updates; the 1D stray field inside the
+ * updates (magnetic_solver.cpp:1533-1538, magnetic_AFM_solver.cpp:1746-1751);
+ * the 3D/demag-factor static between stages and after the step unless the
+ * EM solver is on. */
+void gfo_step(gfo_ctx* c, int64_t nsteps) {
+    int mag = c->if_FM_all || c->if_AFM_all;
+    int ms = (c->flags & GF_FLAG_MAGNETOSTATICS) != 0;
+    int em = (c->flags & GF_FLAG_EMDYNAMIC) != 0;
+    int one_d = (c->flags & GF_FLAG_MAG_1D) != 0;
+    for (int64_t st = 0; st < nsteps; st++) {
+        gfo_time_advance(c);
+        for (int s = 1; s <= 4; s++) {
+            if (c->if_FM_all) gfo_mag_get_dm(c, s);
+            if (c->if_AFM_all) gfo_afm_get_dm(c, s);
+            if (c->if_FM_all) gfo_mag_update(c, s);
+            if (c->if_AFM_all) gfo_afm_update(c, s);
+            if (!em && one_d && ((c->if_FM_all && ms) || c->if_AFM_all)) demag_1d(c);
+            if (!em && mag && !one_d && ms) demag_3d(c);
+        }
+    }
+}
+
+/* magnetic_solver.cpp:1863-1888, magnetic_AFM_solver.cpp:2221-2266 */
+void gfo_average(const gfo_ctx* c, int id, double out[3]) {
+    double** f = field_ptrs((gfo_ctx*)c, id);
+    int64_t cnt = id == GF_FIELD_M ? c->NFM : c->NAFM;
+    for (int a = 0; a < 3; a++) {
+        double s = 0.;
+        for (int64_t i = 0; i < c->n; i++) s = s + f[a][i];
+        out[a] = s / (double)cnt;
+    }
+}
+
+double gfo_time(const gfo_ctx* c) { return c->time; }

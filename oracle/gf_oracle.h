@@ -1,0 +1,62 @@
+/*
+ * gf_oracle.h — CPU restatement of the reference's FP64 RK4 hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product is
+ * compared against; only tests/, __graft_entry__.smoke() and the CPU-baseline
+ * leg of bench.py may load it.  The product never links or calls it.
+ *
+ * It follows the reference's own data layout and protocol (separate k-arrays
+ * per stage, one store buffer, the get -> wait -> update order of
+ * main.cpp:243-483), not the fused design of the GPU path, so that a fusion
+ * mistake on the GPU cannot be mirrored here.
+ */
+#ifndef GF_ORACLE_H
+#define GF_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "../include/goferro.h"   /* gf_desc / gf_material / field ids (types only) */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gfo_ctx gfo_ctx;
+
+/* oracle-only field: the combined RK4 increment dm of the last step */
+#define GFO_FIELD_DM 100
+
+gfo_ctx* gfo_create(const gf_desc* desc);
+void gfo_destroy(gfo_ctx* c);
+int gfo_set_field(gfo_ctx* c, int field_id, const double* src, size_t count);
+int gfo_get_field(const gfo_ctx* c, int field_id, double* dst, size_t count);
+
+/* protocol pieces (reference names in the .c file) */
+void gfo_time_advance(gfo_ctx* c);
+void gfo_mag_get_dm(gfo_ctx* c, int stage);   /* get_dm_RKs (+ get_dm at s=4) */
+void gfo_mag_update(gfo_ctx* c, int stage);   /* update_m_RKs / update_m      */
+void gfo_afm_get_dm(gfo_ctx* c, int stage);
+void gfo_afm_update(gfo_ctx* c, int stage);
+void gfo_demag(gfo_ctx* c);                   /* get_H_static / _1D           */
+void gfo_step(gfo_ctx* c, int64_t nsteps);    /* main.cpp:243-483             */
+void gfo_average(const gfo_ctx* c, int field_id, double out[3]);
+double gfo_time(const gfo_ctx* c);
+
+/* building blocks, exposed so tests can pin them one by one */
+void gfo_newell_real(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                     double* out9);   /* 9 placed, /n real kernels, each n doubles */
+double gfo_newell_F2(double x, double y, double z);
+double gfo_newell_G2(double x, double y, double z);
+double gfo_newell_A(double x, double y, double z, double dx, double dy, double dz);
+double gfo_newell_B(double x, double y, double z, double dx, double dy, double dz);
+/* unnormalised real 3D transforms with the half spectrum along z:
+ * in: nx*ny*nz reals; out: nx*ny*(nz/2+1) complex as interleaved doubles */
+void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out);
+void gfo_irfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out);
+/* naive O(N^2) DFT of the same transform, for pinning gfo_rfft3 */
+void gfo_rdft3_naive(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the product.  It consumes the same
+``goferro.Problem`` description as the GPU binding (types only).
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+FIELD_DM = 100   # oracle-only: combined RK4 increment of the last step
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile liboracle.so with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    L = ctypes.CDLL(str(LIB))
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    i64 = ctypes.c_int64
+    sig = {
+        "gfo_create": ([P], P), "gfo_destroy": ([P], None),
+        "gfo_set_field": ([P, ctypes.c_int, dp, ctypes.c_size_t], ctypes.c_int),
+        "gfo_get_field": ([P, ctypes.c_int, dp, ctypes.c_size_t], ctypes.c_int),
+        "gfo_time_advance": ([P], None),
+        "gfo_mag_get_dm": ([P, ctypes.c_int], None), "gfo_mag_update": ([P, ctypes.c_int], None),
+        "gfo_afm_get_dm": ([P, ctypes.c_int], None), "gfo_afm_update": ([P, ctypes.c_int], None),
+        "gfo_demag": ([P], None), "gfo_step": ([P, i64], None),
+        "gfo_average": ([P, ctypes.c_int, dp], None), "gfo_time": ([P], ctypes.c_double),
+        "gfo_newell_real": ([i64, i64, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp], None),
+        "gfo_newell_F2": ([ctypes.c_double] * 3, ctypes.c_double),
+        "gfo_newell_G2": ([ctypes.c_double] * 3, ctypes.c_double),
+        "gfo_newell_A": ([ctypes.c_double] * 6, ctypes.c_double),
+        "gfo_newell_B": ([ctypes.c_double] * 6, ctypes.c_double),
+        "gfo_rfft3": ([i64, i64, i64, dp, dp], None),
+        "gfo_irfft3": ([i64, i64, i64, dp, dp], None),
+        "gfo_rdft3_naive": ([i64, i64, i64, dp, dp], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """The reference's protocol on the CPU for one Problem."""
+
+    def __init__(self, problem):
+        self.L = lib()
+        self.problem = problem
+        self.shape = tuple(int(v) for v in problem.shape)
+        self.n = int(np.prod(self.shape))
+        d, self._keep = problem.desc()
+        self._h = self.L.gfo_create(ctypes.byref(d))
+        if not self._h:
+            raise RuntimeError("gfo_create rejected the problem")
+
+    def close(self):
+        if self._h:
+            self.L.gfo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, field, value):
+        a = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
+        if self.L.gfo_set_field(self._h, field, _dp(a), a.size) != 0:
+            raise ValueError("bad field or size")
+
+    def get(self, field):
+        a = np.empty(3 * self.n)
+        if self.L.gfo_get_field(self._h, field, _dp(a), a.size) != 0:
+            raise ValueError("bad field or size")
+        return a.reshape((3,) + self.shape)
+
+    def time_advance(self):
+        self.L.gfo_time_advance(self._h)
+
+    def mag_stage(self, s):
+        """get_dm_RKs then update_m_RKs (one reference stage, FM only)."""
+        self.L.gfo_mag_get_dm(self._h, s)
+        self.L.gfo_mag_update(self._h, s)
+
+    def afm_stage(self, s):
+        self.L.gfo_afm_get_dm(self._h, s)
+        self.L.gfo_afm_update(self._h, s)
+
+    def demag(self):
+        self.L.gfo_demag(self._h)
+
+    def step(self, n=1):
+        self.L.gfo_step(self._h, int(n))
+
+    def average(self, field=0):
+        out = np.zeros(3)
+        self.L.gfo_average(self._h, field, _dp(out))
+        return out
+
+    @property
+    def time(self):
+        return self.L.gfo_time(self._h)
+
+
+def newell_real(shape, spacing):
+    nx, ny, nz = shape
+    out = np.empty(9 * nx * ny * nz)
+    lib().gfo_newell_real(nx, ny, nz, *spacing, _dp(out))
+    return out.reshape((9, nx, ny, nz))
+
+
+def rfft3(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nx, ny, nz = x.shape
+    out = np.empty(2 * nx * ny * (nz // 2 + 1))
+    lib().gfo_rfft3(nx, ny, nz, _dp(x), _dp(out))
+    o = out.reshape(nx, ny, nz // 2 + 1, 2)
+    return o[..., 0] + 1j * o[..., 1]
+
+
+def irfft3(X, nz):
+    nx, ny, nzh = X.shape
+    a = np.empty((nx, ny, nzh, 2))
+    a[..., 0] = X.real
+    a[..., 1] = X.imag
+    a = np.ascontiguousarray(a)
+    out = np.empty(nx * ny * nz)
+    lib().gfo_irfft3(nx, ny, nz, _dp(a), _dp(out))
+    return out.reshape(nx, ny, nz)
+
+
+def rdft3_naive(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nx, ny, nz = x.shape
+    out = np.empty(2 * nx * ny * (nz // 2 + 1))
+    lib().gfo_rdft3_naive(nx, ny, nz, _dp(x), _dp(out))
+    o = out.reshape(nx, ny, nz // 2 + 1, 2)
+    return o[..., 0] + 1j * o[..., 1]

@@ -1,0 +1,78 @@
+"""The C-ABI library loads on a CPU-only host and exports exactly what
+include/goferro.h declares (no compute calls: no GPU here)."""
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "goferro.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:gf_status|const char\*)\s+(gf_[a-z_]+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_binding_list():
+    import goferro
+    assert declared_functions() == sorted(goferro.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    import goferro
+    if not goferro.LIB_PATH.exists():
+        pytest.skip("libgoferro.so not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-D", "--defined-only", str(goferro.LIB_PATH)], stdout=subprocess.PIPE, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [s for s in declared_functions() if s not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_binds_without_gpu():
+    import goferro
+    if not goferro.LIB_PATH.exists():
+        pytest.skip("libgoferro.so not built (run __graft_entry__.build())")
+    lib = goferro.load_library()
+    for name in goferro.EXPORTED_SYMBOLS:
+        assert getattr(lib, name) is not None
+    # the create-error string is readable without a context
+    assert isinstance(lib.gf_last_error(None), bytes)
+
+
+def test_desc_layout_matches_header():
+    """ctypes mirror of gf_desc / gf_material has the C layout (compiled probe)."""
+    import ctypes
+
+    import goferro
+    probe = ROOT / "_build_probe_sizes.c"
+    exe = ROOT / "_build_probe_sizes"
+    probe.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "goferro.h"\n'
+                     'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(gf_desc), sizeof(gf_material),'
+                     ' offsetof(gf_desc, prescribe_Hext), offsetof(gf_material, B2_AFM2));return 0;}\n')
+    try:
+        subprocess.run(["gcc", "-I", str(ROOT / "include"), str(probe), "-o", str(exe)], check=True)
+        got = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split()
+    finally:
+        probe.unlink(missing_ok=True)
+        exe.unlink(missing_ok=True)
+    want = [ctypes.sizeof(goferro._Desc), ctypes.sizeof(goferro.Material),
+            goferro._Desc.prescribe_Hext.offset, goferro.Material.B2_AFM2.offset]
+    assert [int(v) for v in got] == want
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    import goferro
+    with pytest.raises(goferro.GoFerroError):
+        _load_fresh(goferro, tmp_path / "nope.so")
+
+
+def _load_fresh(goferro, path):
+    saved = goferro._lib
+    goferro._lib = None
+    try:
+        return goferro.load_library(path)
+    finally:
+        goferro._lib = saved

@@ -1,0 +1,212 @@
+"""GPU parity: the gfx950 library against the CPU oracle, through the C ABI.
+
+Tolerances are the north-star ones: 1e-12 per stage/step on m (|m| ~ 1, so
+absolute), 1e-9 on m after 1000 steps; the magnetostatic field is compared
+relative to its peak (1e-12).  Every case prints its name and shape first so
+a failing GPU step names itself.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from goferro import (FIELD_HEXT_NONUNIF, FIELD_HSTAT, FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_M_STORE,
+                     FLAG_DEMAG_FACTOR, FLAG_MAG_1D, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE, FLAG_PRESCRIBE_HEXT,
+                     MU0, Context, Material, Problem, workloads)
+from goferro.workloads import GYRO
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rot():
+    c, s = np.cos(0.3), np.sin(0.3)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    c2, s2 = np.cos(0.7), np.sin(0.7)
+    Ry = np.array([[c2, 0, s2], [0, 1, 0], [-s2, 0, c2]])
+    return Ry @ Rz
+
+
+def mixed_fm_problem(shape=(16, 16, 8), flags=FLAG_MAGNETOSTATICS, periodic=(False, True, False), rotate=True):
+    """Two ferromagnets with different Ms/Aex/anisotropy/DMI, a non-magnetic
+    material and vacuum, a non-periodic axis, a rotation and an AC drive."""
+    rng = np.random.default_rng(7)
+    m1 = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.02, K1=-2e4, K2=5e3,
+                       Aex=3.7e-12, iDMI=1.5e-3)
+    m2 = Material.make(if_FM=1, anisotropy_type=2, Ms=8e5, gyro=GYRO, FM_damping=0.1, K1=4e5, Aex=1.3e-11,
+                       iDMI=0.0)
+    m3 = Material.make(if_FM=0)
+    cell = rng.choice(np.array([0, 1, 2, 3], np.uint32), size=shape, p=[0.1, 0.45, 0.35, 0.1])
+    prob = Problem(shape=shape, spacing=(2e-9, 2.5e-9, 1.5e-9), materials=[m1, m2, m3], material_cell=cell,
+                   dt=2e-14, periodic=periodic, flags=flags | (FLAG_ORIENT_ROTATE if rotate else 0),
+                   Hext_stat=(1e4, -2e4, 5e4), Hext_altr=(3e4, 0, 1e4), H_altr_freq=(5e9, 0, 2e10),
+                   demag_fac=(0.1, 0.2, 0.7), Tg2c=rot() if rotate else None)
+    m = workloads.tilted_m(shape, 9, theta_max=1.2)
+    return prob, m
+
+
+def compare_stages(prob, fields, nsteps=2, hnu=None, check_h=True, afm=False, fm=True):
+    from oracle.oracle import Oracle
+    say("case", prob.shape, hex(prob.flags))
+    g = Context(prob)
+    o = Oracle(prob)
+    for fid, val in fields.items():
+        g.upload(fid, val)
+        o.set(fid, val)
+    if hnu is not None:
+        g.upload(FIELD_HEXT_NONUNIF, hnu)
+        o.set(FIELD_HEXT_NONUNIF, hnu)
+    if prob.flags & FLAG_MAGNETOSTATICS:
+        g.demag()
+        o.demag()
+    worst = 0.0
+    for step in range(nsteps):
+        g.time_advance()
+        o.time_advance()
+        for s in range(1, 5):
+            if fm:
+                g.mag_stage(s)
+                o.L.gfo_mag_get_dm(o._h, s)
+            if afm:
+                g.afm_stage(s)
+                o.L.gfo_afm_get_dm(o._h, s)
+            if fm:
+                o.L.gfo_mag_update(o._h, s)
+            if afm:
+                o.L.gfo_afm_update(o._h, s)
+            if fm:
+                d = np.abs(g.download(FIELD_M_STORE) - o.get(FIELD_M_STORE)).max()
+                worst = max(worst, d)
+                assert d <= 1e-12, (step, s, d)
+            if prob.flags & FLAG_MAGNETOSTATICS:
+                g.demag()
+                o.demag()
+                if check_h:
+                    hg, ho = g.download(FIELD_HSTAT), o.get(FIELD_HSTAT)
+                    dh = np.abs(hg - ho).max() / max(np.abs(ho).max(), 1e-300)
+                    assert dh <= 1e-12, (step, s, dh)
+    if fm:
+        np.testing.assert_allclose(g.download(FIELD_M), o.get(FIELD_M), rtol=0, atol=1e-12)
+        np.testing.assert_allclose(g.average(FIELD_M), o.average(FIELD_M), rtol=0, atol=1e-13)
+    if afm:
+        for fid in (FIELD_M_AFM1, FIELD_M_AFM2):
+            np.testing.assert_allclose(g.download(fid), o.get(fid), rtol=0, atol=1e-12)
+            np.testing.assert_allclose(g.average(fid), o.average(fid), rtol=0, atol=1e-13)
+    assert abs(g.time - o.time) <= 1e-30
+    g.close()
+    o.close()
+    return worst
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 4), (16, 16, 8), (64, 64, 16), (4, 8, 32)])
+def test_fm_film_stages(shape):
+    prob, m = workloads.fm_film(shape)
+    compare_stages(prob, {FIELD_M: m}, nsteps=2)
+
+
+def test_fm_mixed_materials_rotation_dmi_drive():
+    prob, m = mixed_fm_problem()
+    hnu = np.random.default_rng(3).standard_normal((3,) + prob.shape) * 3e4
+    compare_stages(prob, {FIELD_M: m}, nsteps=2, hnu=hnu)
+
+
+@pytest.mark.parametrize("mode", [FLAG_DEMAG_FACTOR | FLAG_MAGNETOSTATICS, FLAG_MAG_1D | FLAG_MAGNETOSTATICS, 0])
+def test_fm_local_statics(mode):
+    prob, m = mixed_fm_problem(shape=(8, 16, 4), flags=mode, periodic=(True, False, True))
+    compare_stages(prob, {FIELD_M: m}, nsteps=2)
+
+
+def test_fm_prescribed_drive_table():
+    prob, m = workloads.fm_film((8, 8, 4))
+    prob.flags |= FLAG_PRESCRIBE_HEXT
+    prob.prescribe_Hext = np.array([[0.0, 1e4, 0, 2e5], [2.5e-14, -3e4, 1e4, 1e5], [5.5e-14, 0, 0, 3e5]])
+    compare_stages(prob, {FIELD_M: m}, nsteps=6)
+
+
+def test_afm_stages_with_demag():
+    prob, m1, m2 = workloads.afm_block((8, 8, 4))
+    prob.flags |= FLAG_MAGNETOSTATICS
+    compare_stages(prob, {FIELD_M_AFM1: m1, FIELD_M_AFM2: m2}, nsteps=2, afm=True, fm=False)
+
+
+def test_fm_and_afm_together():
+    shape = (8, 8, 8)
+    prob_fm, m = workloads.fm_film(shape)
+    prob_afm, m1, m2 = workloads.afm_block(shape)
+    cell = np.ones(shape, np.uint32)
+    cell[:, :, 4:] = 2
+    cell[0, 0, :] = 0
+    prob = Problem(shape=shape, spacing=(2e-9,) * 3, materials=[prob_fm.materials[0], prob_afm.materials[0]],
+                   material_cell=cell, dt=1e-15, periodic=(True, True, False), flags=FLAG_MAGNETOSTATICS,
+                   Hext_stat=(0, 0, 0.1 / MU0), Hext_altr=(0.05 / MU0, 0, 0), H_altr_freq=(1e12, 0, 0))
+    compare_stages(prob, {FIELD_M: m, FIELD_M_AFM1: m1, FIELD_M_AFM2: m2}, nsteps=2, afm=True, fm=True)
+
+
+def test_fm_film_1000_steps():
+    """The north-star bound after 1000 steps, at a size the oracle runs in seconds."""
+    from oracle.oracle import Oracle
+    shape = (16, 16, 8)
+    say("case 1000 steps", shape)
+    prob, m = workloads.fm_film(shape)
+    g, o = Context(prob), Oracle(prob)
+    g.upload(FIELD_M, m)
+    o.set(FIELD_M, m)
+    g.demag()
+    o.demag()
+    g.step(1000)
+    o.step(1000)
+    a, b = g.download(FIELD_M), o.get(FIELD_M)
+    assert np.abs(a - b).max() <= 1e-9
+    # |m| is not renormalised by the reference: the builds must drift alike
+    na, nb = np.sqrt((a * a).sum(0)), np.sqrt((b * b).sum(0))
+    assert np.abs(na - nb).max() <= 1e-12
+
+
+def test_uniform_film_full_size_equals_column():
+    """Full-size stage kernel check: a laterally uniform film without demag
+    evolves exactly like a 1x1 periodic column (translation symmetry)."""
+    from oracle.oracle import Oracle
+    nz = 32
+    shape = (256, 256, nz)
+    say("case uniform film", shape)
+    prob, _ = workloads.fm_film(shape, demag=False)
+    prob_c, _ = workloads.fm_film((1, 1, nz), demag=False)
+    col = workloads.tilted_m((1, 1, nz), 4, theta_max=0.5)
+    m = np.broadcast_to(col, (3,) + shape).copy()
+    g, o = Context(prob), Oracle(prob_c)
+    g.upload(FIELD_M, m)
+    o.set(FIELD_M, col)
+    g.step(50)
+    o.step(50)
+    a = g.download(FIELD_M)
+    ref = o.get(FIELD_M)
+    assert np.abs(a - ref).max() <= 1e-12
+
+
+def test_demag_full_size_shift_and_linearity():
+    """Circular convolution commutes with circular shifts and is linear;
+    checked at the benchmark grid where the oracle is too slow."""
+    shape = (256, 256, 128)
+    say("case demag properties", shape)
+    prob, m = workloads.fm_film(shape)
+    g = Context(prob)
+    g.upload(FIELD_M, m)
+    g.demag()
+    h0 = g.download(FIELD_HSTAT)
+    sh = (37, -101, 5)
+    g.upload(FIELD_M, np.roll(m, sh, axis=(1, 2, 3)))
+    g.demag()
+    h1 = g.download(FIELD_HSTAT)
+    scale = np.abs(h0).max()
+    assert np.abs(np.roll(h0, sh, axis=(1, 2, 3)) - h1).max() <= 1e-12 * scale
+    m2 = workloads.tilted_m(shape, 77, theta_max=2.0)
+    g.upload(FIELD_M, m2)
+    g.demag()
+    h2 = g.download(FIELD_HSTAT)
+    g.upload(FIELD_M, 0.25 * m - 1.5 * m2)
+    g.demag()
+    h3 = g.download(FIELD_HSTAT)
+    assert np.abs(h3 - (0.25 * h0 - 1.5 * h2)).max() <= 1e-12 * np.abs(h3).max()
