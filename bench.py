@@ -46,6 +46,22 @@ def kernel_bytes(cls: str, n: int, nh: int) -> float:
     return 0.0
 
 
+def measured_traffic(kernel_cls: str):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (already scaled by the
+    1 GiB copy calibration), or None when the summary has no such row."""
+    names = {"mag_stage": "mag_stage_kernel", "fft_z_fwd": "zfwd_kernel<false>", "fft_y_fwd": "yfwd_kernel",
+             "fft_x_conv": "xconv_kernel<true>", "fft_y_inv": "yinv_kernel", "fft_z_inv": "zinv_kernel"}
+    path = ROOT / "profiles" / "fm256_demag_kernel_stats.md"
+    if kernel_cls not in names or not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{names[kernel_cls]}`":
+            return float(cells[6]) * 1e6
+    return None
+
+
 def b_alg_fm(shape) -> float:
     """Algorithmic FP64 bytes per cell-step of FM LLG + FFT magnetostatics:
     fused-RK4 state traffic 136*3, magnetostatic field reads 4*24, material
@@ -84,6 +100,24 @@ def cpu_baseline(seconds_hint: float = 15.0) -> dict:
                       f"{el:.1f} s, oracle/gf_oracle.c single thread"}
 
 
+def copy_probe(nbytes: int) -> int:
+    """One HIP device-to-device copy of nbytes (runtime blit kernel): the
+    reference point that scales FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed for the copy probe")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +126,8 @@ def main() -> int:
     ap.add_argument("--grid", default="256x256x128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -150,11 +186,14 @@ def main() -> int:
                             "achieved_GBs": (b / (ms / cnt * 1e-3)) / 1e9 if b else None}
         dom = max(per, key=lambda k: per[k]["avg_us"] * per[k]["launches"])
         d = per[dom]
+        traffic = measured_traffic(dom) if shape == (256, 256, 128) else None
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                           "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                            "bytes_per_launch": kernel_bytes(dom, n, nh), "avg_launch_us": d["avg_us"]}
         out["kernels"] = per
     ctx.close()
+    if args.copy_probe_mib:
+        out["copy_probe_bytes"] = copy_probe(args.copy_probe_mib << 20)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
