@@ -81,6 +81,10 @@ bool is_fm(const gf_ctx* c, int64_t id) {
     const uint32_t t = c->host_code[id] & kMatMask;
     return t && c->host_mats[t - 1].if_FM;
 }
+bool is_fe(const gf_ctx* c, int64_t id) {
+    const uint32_t t = c->host_code[id] & kMatMask;
+    return t && c->host_mats[t - 1].if_FE;
+}
 bool is_afm(const gf_ctx* c, int64_t id) {
     const uint32_t t = c->host_code[id] & kMatMask;
     return t && c->host_mats[t - 1].if_AFM;
@@ -165,6 +169,14 @@ DevMaterial make_dev_material(const gf_material& m, double dt) {
         d.aAlphaPrecess[s] = al[s] * d.aPrecess[s];
     }
     for (int a = 0; a < 3; a++) d.u[a] = m.uniaxis_AFM[a];
+    d.is_fe = m.if_FE ? 1 : 0;
+    if (m.if_FE) {
+        d.fe_mass = -1. * dt / m.FE_mass;
+        d.fe_damping = d.fe_mass * m.FE_damping;
+        d.a1 = m.a1; d.a11 = m.a11; d.a12 = m.a12; d.a111 = m.a111; d.a112 = m.a112; d.a123 = m.a123;
+        d.a1111 = m.a1111; d.a1112 = m.a1112; d.a1122 = m.a1122; d.a1123 = m.a1123;
+        d.G11 = m.G11; d.tv1 = m.tv1; d.tv2 = m.tv2; d.tv3 = m.tv3;
+    }
     return d;
 }
 
@@ -198,6 +210,8 @@ void stage_drive(gf_ctx* c, int stage, double out[3]) {
 double** fm_store(gf_ctx* c, int sel) { return sel == 0 ? c->m : (sel == 1 ? c->st_a : c->st_b); }
 double** afm1_store(gf_ctx* c, int sel) { return sel == 0 ? c->m1 : (sel == 1 ? c->st1_a : c->st1_b); }
 double** afm2_store(gf_ctx* c, int sel) { return sel == 0 ? c->m2 : (sel == 1 ? c->st2_a : c->st2_b); }
+double** fe_pstore(gf_ctx* c, int sel) { return sel == 0 ? c->p : (sel == 1 ? c->p_a : c->p_b); }
+double** fe_qstore(gf_ctx* c, int sel) { return sel == 0 ? c->q : (sel == 1 ? c->q_a : c->q_b); }
 
 // statics after an update phase (main.cpp:299-306 and the 1D field inside
 // update_m*, magnetic_solver.cpp:1533-1538, magnetic_AFM_solver.cpp:1746-1751)
@@ -269,6 +283,12 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         }
         if ((d->flags & GF_FLAG_PRESCRIBE_HEXT) && d->n_prescribe_Hext > 0)
             c->prescribe.assign(d->prescribe_Hext, d->prescribe_Hext + 4 * d->n_prescribe_Hext);
+        for (int a = 0; a < 3; a++) {
+            c->eext_stat[a] = d->Eext_stat[a]; c->eext_altr[a] = d->Eext_altr[a];
+            c->e_altr_freq[a] = d->E_altr_freq[a]; c->eext[a] = 0.;
+        }
+        if ((d->flags & GF_FLAG_PRESCRIBE_EEXT) && d->n_prescribe_Eext > 0 && d->prescribe_Eext)
+            c->prescribe_e.assign(d->prescribe_Eext, d->prescribe_Eext + 4 * d->n_prescribe_Eext);
         c->host_mats.assign(d->materials, d->materials + d->n_materials);
         c->host_code.resize(static_cast<size_t>(c->n));
         for (int64_t i = 0; i < c->n; i++) {
@@ -282,12 +302,15 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
             uint32_t code = t;
             if (t && d->materials[t - 1].if_FM) { code |= kIsFM; c->nfm++; }
             if (t && d->materials[t - 1].if_AFM) { code |= kIsAFM; c->nafm++; }
+            if (t && d->materials[t - 1].if_FE) { code |= kIsFE; c->nfe++; }
             c->host_code[static_cast<size_t>(i)] = code;
         }
         for (const auto& m : c->host_mats) {
             if (m.if_FM) c->any_fm = true;
             if (m.if_AFM) c->any_afm = true;
+            if (m.if_FE) c->any_fe = true;
         }
+        if (c->any_fe) build_surface_bits(c, is_fe, kFeSurfShift);
         if (c->any_fm) build_surface_bits(c, is_fm, kFmSurfShift);
         if (c->any_afm) build_surface_bits(c, is_afm, kAfmSurfShift);
     } catch (const std::bad_alloc&) {
@@ -328,6 +351,11 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         if (!st) st = alloc3(c, c->st2_a, "hipMalloc store2 a");
         if (!st) st = alloc3(c, c->st2_b, "hipMalloc store2 b");
     }
+    if (!st && c->any_fe) {
+        double** fe_bufs[] = {c->p, c->q, c->pacc, c->qacc, c->p_a, c->p_b, c->q_a, c->q_b, c->estat};
+        for (double** b : fe_bufs)
+            if (!st) st = alloc3(c, b, "hipMalloc ferroelectric state");
+    }
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
         st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
@@ -354,6 +382,8 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->m1); free3(c->m2); free3(c->acc1); free3(c->acc2);
     free3(c->st1_a); free3(c->st1_b); free3(c->st2_a); free3(c->st2_b);
     free3(c->hstat); free3(c->hnu);
+    free3(c->p); free3(c->q); free3(c->pacc); free3(c->qacc);
+    free3(c->p_a); free3(c->p_b); free3(c->q_a); free3(c->q_b); free3(c->estat);
     if (c->red_partial) hipFree(c->red_partial);
     if (c->d_code) hipFree(c->d_code);
     if (c->d_mats) hipFree(c->d_mats);
@@ -390,6 +420,19 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     case GF_FIELD_M_AFM2:
         if (!c->any_afm) break;
         return c->m2;
+    case GF_FIELD_P:
+        if (!c->any_fe) break;
+        return c->p;
+    case GF_FIELD_Q:
+        if (!c->any_fe) break;
+        return c->q;
+    case GF_FIELD_P_STORE:
+        if (!c->any_fe) break;
+        if (for_write && c->fe_store_sel == 0) c->fe_store_sel = 2;
+        return fe_pstore(c, c->fe_store_sel);
+    case GF_FIELD_ESTAT:
+        if (!c->any_fe) break;
+        return c->estat;
     default:
         break;
     }
@@ -421,6 +464,18 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
     if (id == GF_FIELD_M) { st = reset(c->m, c->st_a, c->st_b); c->mag_store_sel = 0; c->mag_next_stage = 1; }
     if (id == GF_FIELD_M_AFM1) { st = reset(c->m1, c->st1_a, c->st1_b); c->afm_store_sel = 0; c->afm_next_stage = 1; }
     if (id == GF_FIELD_M_AFM2) { st = reset(c->m2, c->st2_a, c->st2_b); c->afm_store_sel = 0; c->afm_next_stage = 1; }
+    // ferroelectric_initialize.cpp:355-366: the stores start equal to p and q
+    if (id == GF_FIELD_P || id == GF_FIELD_Q) {
+        const bool isp = id == GF_FIELD_P;
+        st = reset(isp ? c->p : c->q, isp ? c->p_a : c->q_a, isp ? c->p_b : c->q_b);
+        if (c->fe_store_sel != 0) {
+            // a stage store that differs from the state was uploaded; the
+            // other one of the pair restarts from its state as well
+            st = st ? st : reset(isp ? c->q : c->p, isp ? c->q_a : c->p_a, isp ? c->q_b : c->p_b);
+        }
+        c->fe_store_sel = 0;
+        c->fe_next_stage = 1;
+    }
     if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
     if (st) return st;
     return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
@@ -444,6 +499,15 @@ gf_status gf_download(gf_ctx* c, int id, double* host, size_t count) {
 gf_status gf_time_advance(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
     c->time = c->time + c->dt;
+    if (c->flags & GF_FLAG_PRESCRIBE_EEXT) {
+        const int np = static_cast<int>(c->prescribe_e.size() / 4);
+        if (c->prescribe_e_idx < np && (c->time - c->dt) > c->prescribe_e[4 * c->prescribe_e_idx]) c->prescribe_e_idx++;
+        if (c->prescribe_e_idx < np) {
+            for (int a = 0; a < 3; a++) c->eext[a] = c->prescribe_e[4 * c->prescribe_e_idx + 1 + a];
+        } else {
+            for (int a = 0; a < 3; a++) c->eext[a] = 0.;
+        }
+    }
     if (c->flags & GF_FLAG_PRESCRIBE_HEXT) {
         const int np = static_cast<int>(c->prescribe.size() / 4);
         if (c->prescribe_idx < np && (c->time - c->dt) > c->prescribe[4 * c->prescribe_idx]) c->prescribe_idx++;
@@ -530,6 +594,45 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
     return GF_OK;
 }
 
+gf_status gf_fe_stage(gf_ctx* c, int stage) {
+    if (!c) return GF_ERR_ARG;
+    if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
+    if (!c->any_fe) return set_error(c, GF_ERR_STATE, "no ferroelectric material in this context");
+    if (stage != c->fe_next_stage) return set_error(c, GF_ERR_STATE, "FE stages must run in the order 1,2,3,4");
+    FeStageArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** ps = fe_pstore(c, c->fe_store_sel);
+    double** qs = fe_qstore(c, c->fe_store_sel);
+    const int out_sel = stage == 4 ? 0 : (c->fe_store_sel == 1 ? 2 : 1);
+    double** po = fe_pstore(c, out_sel);
+    double** qo = fe_qstore(c, out_sel);
+    for (int i = 0; i < 3; i++) {
+        a.pstore[i] = ps[i]; a.qstore[i] = qs[i]; a.pbase[i] = c->p[i]; a.qbase[i] = c->q[i];
+        a.pacc[i] = c->pacc[i]; a.qacc[i] = c->qacc[i]; a.pout[i] = po[i]; a.qout[i] = qo[i];
+        a.estat[i] = c->estat[i];
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
+    // ferroelectric_solver.cpp:2002-2029 (drive time per stage)
+    if (!(c->flags & GF_FLAG_PRESCRIBE_EEXT)) {
+        const double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+        for (int i = 0; i < 3; i++)
+            c->eext[i] = c->eext_stat[i] + c->eext_altr[i] * std::sin(2. * kPi * c->e_altr_freq[i] * t);
+    }
+    for (int i = 0; i < 3; i++) a.eext[i] = c->eext[i];
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    a.stage = stage;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_FE_STAGE, &ev);
+    gf_status st = hip_check(c, launch_fe_stage(a, c->n, c->stream), "fe stage");
+    timer_end(c, GF_K_FE_STAGE, ev);
+    if (st) return st;
+    c->fe_store_sel = out_sel;
+    c->fe_next_stage = stage == 4 ? 1 : stage + 1;
+    return GF_OK;
+}
+
 gf_status gf_demag(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
     if (!(c->any_fm || c->any_afm)) return GF_OK;
@@ -547,13 +650,15 @@ gf_status gf_demag(gf_ctx* c) {
 gf_status gf_step(gf_ctx* c, int64_t nsteps) {
     if (!c) return GF_ERR_ARG;
     if (nsteps < 0) return set_error(c, GF_ERR_ARG, "nsteps must be >= 0");
-    if ((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1))
+    if ((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1) ||
+        (c->any_fe && c->fe_next_stage != 1))
         return set_error(c, GF_ERR_STATE, "gf_step called in the middle of a step");
     for (int64_t s = 0; s < nsteps; s++) {
         gf_status st = gf_time_advance(c);
         for (int stage = 1; stage <= 4 && !st; stage++) {
             if (c->any_fm) st = gf_mag_stage(c, stage);
             if (!st && c->any_afm) st = gf_afm_stage(c, stage);
+            if (!st && c->any_fe) st = gf_fe_stage(c, stage);
             if (!st) st = statics_after_update(c);
         }
         if (st) return st;
@@ -564,8 +669,8 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
 gf_status gf_average(gf_ctx* c, int id, double out[3]) {
     if (!c || !out) return GF_ERR_ARG;
     gf_status st;
-    if (id != GF_FIELD_M && id != GF_FIELD_M_AFM1 && id != GF_FIELD_M_AFM2)
-        return set_error(c, GF_ERR_ARG, "average is defined for m, m_AFM1, m_AFM2");
+    if (id != GF_FIELD_M && id != GF_FIELD_M_AFM1 && id != GF_FIELD_M_AFM2 && id != GF_FIELD_P)
+        return set_error(c, GF_ERR_ARG, "average is defined for m, m_AFM1, m_AFM2, p");
     double** f = field_for(c, id, false, &st);
     if (!f) return st;
     const double* ff[3] = {f[0], f[1], f[2]};
@@ -576,7 +681,7 @@ gf_status gf_average(gf_ctx* c, int id, double out[3]) {
                                      c->stream), "sum copy");
     if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "sum sync");
     if (st) return st;
-    const double cnt = static_cast<double>(id == GF_FIELD_M ? c->nfm : c->nafm);
+    const double cnt = static_cast<double>(id == GF_FIELD_M ? c->nfm : (id == GF_FIELD_P ? c->nfe : c->nafm));
     for (int a = 0; a < 3; a++) {
         double s = 0.;
         for (int64_t b = 0; b < c->red_blocks; b++) s += part[static_cast<size_t>(3 * b + a)];

@@ -1,0 +1,111 @@
+// gf_fe.hip — fused ferroelectric RK4 stage for gfx950 (FP64).
+//
+// One launch per stage does get_dq_RKs, get_dp_RKs, update_q_RKs and
+// update_p_RKs (ferroelectric_solver.cpp:534-1832, 1835-1999):
+//   k_q = -dt/m [ dF_Landau/dp - G11 lap(p) - E_stat - E_ext ] - dt*gamma/m q_s   (FE cells)
+//   k_p = q_s dt                                                                 (all cells)
+//   stage 1..3: next = y + c_s k, acc = y + k/6 | acc + k/3
+//   stage 4:    y = acc + k/6
+// which is the reference's left-to-right y + k1/6 + k2/3 + k3/3 + k4/6.
+// Powers are products instead of pow(); the difference is a few ulp.
+#include "gf_internal.h"
+
+namespace gf {
+
+namespace {
+
+__device__ __forceinline__ void rot(const double T[3][3], double ix, double iy, double iz, double& ox, double& oy,
+                                    double& oz) {
+    ox = T[0][0] * ix + T[0][1] * iy + T[0][2] * iz;
+    oy = T[1][0] * ix + T[1][1] * iy + T[1][2] * iz;
+    oz = T[2][0] * ix + T[2][1] * iy + T[2][2] * iz;
+}
+
+// Derivative of the 8th-order Landau energy along component a, with b and c
+// the other two (ferroelectric_solver.cpp:740-786, strain terms zero).
+__device__ __forceinline__ double landau(const DevMaterial& M, double pa, double pb, double pc) {
+    const double a2 = pa * pa, b2 = pb * pb, c2 = pc * pc;
+    const double a3 = a2 * pa, a4 = a2 * a2, a5 = a4 * pa, a7 = a5 * a2;
+    const double b4 = b2 * b2, c4 = c2 * c2, b6 = b4 * b2, c6 = c4 * c2;
+    const double bc2 = b2 + c2;
+    return 2. * M.a1 * pa + 4. * M.a11 * a3 + 2. * M.a12 * bc2 * pa + 6. * M.a111 * a5 +
+           4. * M.a112 * a3 * bc2 + 2. * M.a112 * (b4 + c4) * pa + 2. * M.a123 * pa * b2 * c2 +
+           8. * M.a1111 * a7 + 2. * M.a1112 * pa * (b6 + c6 + 3. * a4 * bc2) + 4. * M.a1122 * a3 * (b4 + c4) +
+           2. * M.a1123 * pa * b2 * c2 * (2. * a2 + b2 + c2);
+}
+
+__device__ __forceinline__ void rk(int stage, int64_t id, double k, const double* base, double* acc, double* out) {
+    const double y = base[id];
+    if (stage == 1) {
+        out[id] = y + k * 0.5;
+        acc[id] = y + k / 6.;
+    } else if (stage == 2) {
+        out[id] = y + k * 0.5;
+        acc[id] = acc[id] + k / 3.;
+    } else if (stage == 3) {
+        out[id] = y + k;
+        acc[id] = acc[id] + k / 3.;
+    } else {
+        out[id] = acc[id] + k / 6.;
+    }
+}
+
+__global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
+    const int64_t n = a.nx * a.ny * a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const uint32_t own = a.code[id];
+    double kq[3] = {0., 0., 0.};
+    const double q0 = a.qstore[0][id], q1 = a.qstore[1][id], q2 = a.qstore[2][id];
+    if (own & kIsFE) {
+        const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+        const int64_t nyz = a.ny * a.nz;
+        const int64_t i = id / nyz, j = (id - i * nyz) / a.nz, k = id - i * nyz - j * a.nz;
+        const double c[3] = {a.pstore[0][id], a.pstore[1][id], a.pstore[2][id]};
+        // Laplacian, mirror value across a surface face (ferroelectric_solver.cpp:4-131)
+        double lap[3] = {0., 0., 0.};
+        const int64_t nb[6] = {((i == 0 ? a.nx - 1 : i - 1) * a.ny + j) * a.nz + k,
+                               ((i == a.nx - 1 ? 0 : i + 1) * a.ny + j) * a.nz + k,
+                               (i * a.ny + (j == 0 ? a.ny - 1 : j - 1)) * a.nz + k,
+                               (i * a.ny + (j == a.ny - 1 ? 0 : j + 1)) * a.nz + k,
+                               (i * a.ny + j) * a.nz + (k == 0 ? a.nz - 1 : k - 1),
+                               (i * a.ny + j) * a.nz + (k == a.nz - 1 ? 0 : k + 1)};
+        const double h[3] = {a.dx, a.dy, a.dz};
+        for (int ax = 0; ax < 3; ax++) {
+            const bool sb = (own >> (kFeSurfShift + 2 * ax)) & 1u;
+            const bool sf = (own >> (kFeSurfShift + 2 * ax + 1)) & 1u;
+            for (int cc = 0; cc < 3; cc++) {
+                const double* s = cc == 0 ? a.pstore[0] : (cc == 1 ? a.pstore[1] : a.pstore[2]);
+                const double bw = sb ? c[cc] : s[nb[2 * ax]];
+                const double fw = sf ? c[cc] : s[nb[2 * ax + 1]];
+                lap[cc] = lap[cc] + (fw - 2. * c[cc] + bw) / h[ax] / h[ax];
+            }
+        }
+        double px = c[0], py = c[1], pz = c[2];
+        if (a.rotate) rot(a.Tg2c, c[0], c[1], c[2], px, py, pz);
+        const double fx = M.fe_mass * landau(M, px, py, pz);
+        const double fy = M.fe_mass * landau(M, py, pz, px);
+        const double fz = M.fe_mass * landau(M, pz, px, py);
+        double gx = fx, gy = fy, gz = fz;
+        if (a.rotate) rot(a.Tc2g, fx, fy, fz, gx, gy, gz);
+        kq[0] = gx + M.fe_mass * (-1. * M.G11 * lap[0] - a.estat[0][id] - a.eext[0]) + M.fe_damping * q0;
+        kq[1] = gy + M.fe_mass * (-1. * M.G11 * lap[1] - a.estat[1][id] - a.eext[1]) + M.fe_damping * q1;
+        kq[2] = gz + M.fe_mass * (-1. * M.G11 * lap[2] - a.estat[2][id] - a.eext[2]) + M.fe_damping * q2;
+    }
+    rk(a.stage, id, q0 * a.dt, a.pbase[0], a.pacc[0], a.pout[0]);
+    rk(a.stage, id, q1 * a.dt, a.pbase[1], a.pacc[1], a.pout[1]);
+    rk(a.stage, id, q2 * a.dt, a.pbase[2], a.pacc[2], a.pout[2]);
+    rk(a.stage, id, kq[0], a.qbase[0], a.qacc[0], a.qout[0]);
+    rk(a.stage, id, kq[1], a.qbase[1], a.qacc[1], a.qout[1]);
+    rk(a.stage, id, kq[2], a.qbase[2], a.qacc[2], a.qout[2]);
+}
+
+}  // namespace
+
+hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s) {
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(fe_stage_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gf

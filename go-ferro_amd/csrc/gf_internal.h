@@ -25,11 +25,13 @@ constexpr double kMu0 = 1.256637061e-6;               // magnetic_system.h:14
 constexpr double kPi = 3.14159265358979323846;        // magnetic_system.h:16
 
 // cell code bits
-constexpr uint32_t kMatMask = 0xFFFFu;
-constexpr uint32_t kIsFM = 1u << 16;
-constexpr uint32_t kIsAFM = 1u << 17;
-constexpr int kFmSurfShift = 18;    // 6 bits: xb, xf, yb, yf, zb, zf
-constexpr int kAfmSurfShift = 24;   // 6 bits, same order
+constexpr uint32_t kMatMask = 0xFFu;   // material id, up to 255 materials
+constexpr uint32_t kIsFM = 1u << 8;
+constexpr uint32_t kIsAFM = 1u << 9;
+constexpr uint32_t kIsFE = 1u << 10;
+constexpr int kFmSurfShift = 11;    // 6 bits: xb, xf, yb, yf, zb, zf
+constexpr int kAfmSurfShift = 17;   // 6 bits, same order
+constexpr int kFeSurfShift = 23;    // 6 bits, same order (FE_surf*, ferroelectric_initialize.cpp:74-308)
 enum Face { kXB = 0, kXF = 1, kYB = 2, kYF = 3, kZB = 4, kZF = 5 };
 
 // Per-material constants as the stage kernels need them.  Every entry is
@@ -62,7 +64,34 @@ struct DevMaterial {
     double aAlphaPrecess[2];
     double u[3];
     int32_t is_afm;
-    int32_t pad;
+    int32_t is_fe;
+    // FE (ferroelectric_solver.cpp:740-792)
+    double fe_mass;         // -1. * dt / FE_mass
+    double fe_damping;      // fe_mass * FE_damping
+    double a1, a11, a12, a111, a112, a123, a1111, a1112, a1122, a1123;
+    double G11;
+    double tv1, tv2, tv3;
+};
+
+struct FeStageArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* pstore[3];
+    const double* qstore[3];
+    const double* pbase[3];
+    const double* qbase[3];
+    double* pacc[3];
+    double* qacc[3];
+    double* pout[3];
+    double* qout[3];
+    const double* estat[3];
+    int64_t nx, ny, nz;
+    double dx, dy, dz, dt;
+    double eext[3];
+    double Tg2c[3][3];
+    double Tc2g[3][3];
+    int32_t rotate;
+    int32_t stage;
 };
 
 // Everything a stage kernel reads, passed by value as kernel arguments.
@@ -143,11 +172,14 @@ struct gf_ctx {
     double demag_fac[3] = {};
     std::vector<double> prescribe;   // rows of (t, Hx, Hy, Hz)
     int prescribe_idx = 0;
+    double eext_stat[3] = {}, eext_altr[3] = {}, e_altr_freq[3] = {}, eext[3] = {};
+    std::vector<double> prescribe_e; // rows of (t, Ex, Ey, Ez)
+    int prescribe_e_idx = 0;
 
     std::vector<gf_material> host_mats;
     std::vector<uint32_t> host_code;
-    bool any_fm = false, any_afm = false;
-    int64_t nfm = 0, nafm = 0;
+    bool any_fm = false, any_afm = false, any_fe = false;
+    int64_t nfm = 0, nafm = 0, nfe = 0;
     bool has_hnu = false;
 
     uint32_t* d_code = nullptr;
@@ -168,6 +200,16 @@ struct gf_ctx {
     double* st1_b[3] = {};
     double* st2_a[3] = {};
     double* st2_b[3] = {};
+    // FE: p, q, their accumulators and two stage stores each
+    double* p[3] = {};
+    double* q[3] = {};
+    double* pacc[3] = {};
+    double* qacc[3] = {};
+    double* p_a[3] = {};
+    double* p_b[3] = {};
+    double* q_a[3] = {};
+    double* q_b[3] = {};
+    double* estat[3] = {};
     // statics
     double* hstat[3] = {};
     double* hnu[3] = {};
@@ -180,6 +222,8 @@ struct gf_ctx {
     int afm_store_sel = 0;
     int mag_next_stage = 1;
     int afm_next_stage = 1;
+    int fe_store_sel = 0;
+    int fe_next_stage = 1;
 
     gf::DemagPlan demag;
 
@@ -201,6 +245,9 @@ void timer_end(gf_ctx* c, int cls, hipEvent_t start);
 // launchers (gf_llg.hip)
 hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s);
 hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s);
+
+// ferroelectric (gf_fe.hip)
+hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s);
 
 // demag (gf_fft.hip)
 gf_status demag_setup(gf_ctx* c);

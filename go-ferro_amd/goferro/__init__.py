@@ -20,8 +20,9 @@ import numpy as np
 __all__ = [
     "Material", "Problem", "Context", "GoFerroError", "load_library", "LIB_PATH",
     "MU0", "FLAG_MAGNETOSTATICS", "FLAG_MAG_1D", "FLAG_DEMAG_FACTOR", "FLAG_PRESCRIBE_HEXT",
-    "FLAG_ORIENT_ROTATE", "FLAG_HEXT_NONUNIF", "FLAG_EMDYNAMIC",
+    "FLAG_ORIENT_ROTATE", "FLAG_HEXT_NONUNIF", "FLAG_EMDYNAMIC", "FLAG_PRESCRIBE_EEXT",
     "FIELD_M", "FIELD_M_STORE", "FIELD_HSTAT", "FIELD_HEXT_NONUNIF", "FIELD_M_AFM1", "FIELD_M_AFM2",
+    "FIELD_P", "FIELD_Q", "FIELD_P_STORE", "FIELD_ESTAT",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS",
 ]
 
@@ -36,16 +37,18 @@ FLAG_PRESCRIBE_HEXT = 1 << 3
 FLAG_ORIENT_ROTATE = 1 << 4
 FLAG_HEXT_NONUNIF = 1 << 5
 FLAG_EMDYNAMIC = 1 << 6
+FLAG_PRESCRIBE_EEXT = 1 << 7
 
-FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2 = range(6)
+(FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
+ FIELD_ESTAT) = range(10)
 
 KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_conv", "fft_y_inv",
-                  "fft_z_inv", "other"]
+                  "fft_z_inv", "other", "fe_stage"]
 
 # every entry point declared in include/goferro.h
 EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
-    "gf_mag_stage", "gf_afm_stage", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
+    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
     "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times",
 ]
 
@@ -72,6 +75,20 @@ class Material(ctypes.Structure):
         ("J_AFM", ctypes.c_double),
         ("B1_AFM1", ctypes.c_double), ("B1_AFM2", ctypes.c_double),
         ("B2_AFM1", ctypes.c_double), ("B2_AFM2", ctypes.c_double),
+        ("if_FE", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("FE_mass", ctypes.c_double), ("FE_damping", ctypes.c_double),
+        ("a1", ctypes.c_double), ("a11", ctypes.c_double), ("a12", ctypes.c_double), ("a111", ctypes.c_double),
+        ("a112", ctypes.c_double), ("a123", ctypes.c_double), ("a1111", ctypes.c_double), ("a1112", ctypes.c_double),
+        ("a1122", ctypes.c_double), ("a1123", ctypes.c_double),
+        ("G11", ctypes.c_double),
+        ("Q11", ctypes.c_double), ("Q12", ctypes.c_double), ("Q44", ctypes.c_double),
+        ("tv1", ctypes.c_double), ("tv2", ctypes.c_double), ("tv3", ctypes.c_double),
+        ("c11", ctypes.c_double), ("c12", ctypes.c_double), ("c44", ctypes.c_double), ("density", ctypes.c_double),
+        ("elast_mass_damping", ctypes.c_double), ("elast_stiff_damping", ctypes.c_double),
+        ("cijkl_glb", (ctypes.c_double * 6) * 6),
+        ("lamda100", ctypes.c_double), ("lamda111", ctypes.c_double),
+        ("lamda100_AFM1", ctypes.c_double), ("lamda100_AFM2", ctypes.c_double),
+        ("lamda111_AFM1", ctypes.c_double), ("lamda111_AFM2", ctypes.c_double),
     ]
 
     @classmethod
@@ -102,6 +119,10 @@ class _Desc(ctypes.Structure):
         ("H_altr_freq", ctypes.c_double * 3), ("demag_fac", ctypes.c_double * 3),
         ("n_prescribe_Hext", ctypes.c_int32), ("reserved1", ctypes.c_int32),
         ("prescribe_Hext", ctypes.POINTER(ctypes.c_double)),
+        ("Eext_stat", ctypes.c_double * 3), ("Eext_altr", ctypes.c_double * 3),
+        ("E_altr_freq", ctypes.c_double * 3),
+        ("n_prescribe_Eext", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("prescribe_Eext", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -122,6 +143,10 @@ class Problem:
     demag_fac: tuple[float, float, float] = (0.0, 0.0, 0.0)
     Tg2c: Optional[np.ndarray] = None
     prescribe_Hext: Optional[np.ndarray] = None  # rows (t, Hx, Hy, Hz)
+    Eext_stat: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    Eext_altr: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    E_altr_freq: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    prescribe_Eext: Optional[np.ndarray] = None  # rows (t, Ex, Ey, Ez)
 
     @property
     def n(self) -> int:
@@ -152,11 +177,19 @@ class Problem:
         d.Hext_altr[:] = [float(v) for v in self.Hext_altr]
         d.H_altr_freq[:] = [float(v) for v in self.H_altr_freq]
         d.demag_fac[:] = [float(v) for v in self.demag_fac]
+        d.Eext_stat[:] = [float(v) for v in self.Eext_stat]
+        d.Eext_altr[:] = [float(v) for v in self.Eext_altr]
+        d.E_altr_freq[:] = [float(v) for v in self.E_altr_freq]
         keep = [cell, mats]
         if self.prescribe_Hext is not None:
             pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)
             d.n_prescribe_Hext = pre.shape[0]
             d.prescribe_Hext = pre.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            keep.append(pre)
+        if self.prescribe_Eext is not None:
+            pre = np.ascontiguousarray(self.prescribe_Eext, dtype=np.float64).reshape(-1, 4)
+            d.n_prescribe_Eext = pre.shape[0]
+            d.prescribe_Eext = pre.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             keep.append(pre)
         return d, keep
 
@@ -183,6 +216,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_time_advance": ([P], ctypes.c_int),
         "gf_mag_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_afm_stage": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_fe_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_demag": ([P], ctypes.c_int),
         "gf_step": ([P, ctypes.c_int64], ctypes.c_int),
         "gf_average": ([P, ctypes.c_int, dp], ctypes.c_int),
@@ -262,6 +296,9 @@ class Context:
 
     def afm_stage(self, stage: int) -> None:
         self._check(self.lib.gf_afm_stage(self._h, stage), "gf_afm_stage")
+
+    def fe_stage(self, stage: int) -> None:
+        self._check(self.lib.gf_fe_stage(self._h, stage), "gf_fe_stage")
 
     def demag(self) -> None:
         self._check(self.lib.gf_demag(self._h), "gf_demag")

@@ -42,6 +42,16 @@ def fm_film(shape=(64, 64, 16), dx=2e-9, dt=1e-14, seed=1, demag=True, bias_T=0.
     return prob, tilted_m(shape, seed)
 
 
+def pto_material(T=300.0, FE_mass=1e-16, FE_damping=0.0, G11=1e-10, eps_r=10.0) -> Material:
+    """PbTiO3-like ferroelectric with a 6th-order Landau potential:
+    a1 = 3.74e5 (T - 752), a11 = -7.3e7, a12 = 7.5e8, a111 = 2.6e8,
+    a112 = 6.1e8, a123 = -3.7e9 (SI).  The reference's dynamics is
+    second order, m q' + gamma q = -dF/dp (ferroelectric_solver.cpp:740-792),
+    so a first-order mobility L maps to gamma = 1/L with a small mass."""
+    return Material.make(if_FE=1, FE_mass=FE_mass, FE_damping=FE_damping, a1=3.74e5 * (T - 752.0), a11=-7.3e7,
+                         a12=7.5e8, a111=2.6e8, a112=6.1e8, a123=-3.7e9, G11=G11)
+
+
 def afm_block(shape=(32, 32, 16), dt=1e-15, seed=5000, alpha=1e-3, drive_T=0.05, freq=1e12, Aex=1e-12):
     """Two-sublattice NiO-like antiferromagnet: Ms 3.5e5 A/m per sublattice,
     exchange field 900 T (J_AFM = 900*Ms), easy-axis field 0.6 T

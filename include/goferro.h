@@ -47,6 +47,7 @@ typedef int gf_status;
 #define GF_FLAG_ORIENT_ROTATE    (1u << 4)  /* if_OrientRotate */
 #define GF_FLAG_HEXT_NONUNIF     (1u << 5)  /* Hext_nonunif field uploaded */
 #define GF_FLAG_EMDYNAMIC        (1u << 6)  /* if_EMdynamic (statics frozen) */
+#define GF_FLAG_PRESCRIBE_EEXT   (1u << 7)  /* if_prescribe_Eext (table drive) */
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -72,6 +73,20 @@ typedef struct gf_material {
     double Aex_AFM1, Aex_AFM2;
     double J_AFM;
     double B1_AFM1, B1_AFM2, B2_AFM1, B2_AFM2;
+    /* ferroelectric (material.h:41-49), tv1..tv3 derived as read.cpp:188-190 */
+    int32_t if_FE;
+    int32_t reserved1;
+    double FE_mass, FE_damping;
+    double a1, a11, a12, a111, a112, a123, a1111, a1112, a1122, a1123;
+    double G11;
+    double Q11, Q12, Q44;
+    double tv1, tv2, tv3;
+    /* elastic (material.h:6-9) and magnetostriction constants (material.h:17, 32-33) */
+    double c11, c12, c44, density;
+    double elast_mass_damping, elast_stiff_damping;
+    double cijkl_glb[6][6];    /* global-frame stiffness, global.cpp:365-441 */
+    double lamda100, lamda111;
+    double lamda100_AFM1, lamda100_AFM2, lamda111_AFM1, lamda111_AFM2;
 } gf_material;
 
 /*
@@ -98,6 +113,13 @@ typedef struct gf_desc {
     int32_t n_prescribe_Hext;  /* rows of the prescribed drive table */
     int32_t reserved1;
     const double* prescribe_Hext;   /* rows of (t, Hx, Hy, Hz), copied at create */
+    /* electric drive of the ferroelectric (read.cpp:40-43) */
+    double Eext_stat[3];       /* V/m */
+    double Eext_altr[3];
+    double E_altr_freq[3];     /* Hz */
+    int32_t n_prescribe_Eext;
+    int32_t reserved2;
+    const double* prescribe_Eext;   /* rows of (t, Ex, Ey, Ez), copied at create */
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */
@@ -108,6 +130,10 @@ enum gf_field_id {
     GF_FIELD_HEXT_NONUNIF = 3, /* non-uniform external field, 3n          */
     GF_FIELD_M_AFM1 = 4,    /* AFM sublattice 1, 3n                        */
     GF_FIELD_M_AFM2 = 5,    /* AFM sublattice 2, 3n                        */
+    GF_FIELD_P = 6,         /* polarisation p, 3n (C/m^2)                  */
+    GF_FIELD_Q = 7,         /* dp/dt, 3n                                   */
+    GF_FIELD_P_STORE = 8,   /* p at the current RK stage, 3n               */
+    GF_FIELD_ESTAT = 9,     /* static electric field, 3n (input)           */
     GF_FIELD_COUNT
 };
 
@@ -145,6 +171,11 @@ gf_status gf_mag_stage(gf_ctx* ctx, int stage);
 /* One fused AFM RK4 stage (magnetic_AFM_solver.cpp:3-1834). */
 gf_status gf_afm_stage(gf_ctx* ctx, int stage);
 
+/* One fused ferroelectric RK4 stage: get_dq_RKs + get_dp_RKs followed by
+ * update_q_RKs + update_p_RKs (ferroelectric_solver.cpp:4-131, 534-1784,
+ * 1786-2029), with E_stat taken from the uploaded GF_FIELD_ESTAT. */
+gf_status gf_fe_stage(gf_ctx* ctx, int stage);
+
 /* magnetic_system::get_H_static (magnetostatic_solver.cpp:114-219) or the
  * 1D model get_H_static_1D (:222-249) per the descriptor flags. */
 gf_status gf_demag(gf_ctx* ctx);
@@ -153,8 +184,9 @@ gf_status gf_demag(gf_ctx* ctx);
  * stages with statics between them, statics after the step). */
 gf_status gf_step(gf_ctx* ctx, int64_t nsteps);
 
-/* get_averagem / get_averagem_AFM (magnetic_solver.cpp:1863-1888):
- * sums of the field over all cells divided by the magnetic cell count. */
+/* get_averagem / get_averagem_AFM (magnetic_solver.cpp:1863-1888) and
+ * get_averagep (ferroelectric_solver.cpp:2031-2055): sums of the field over
+ * all cells divided by the FM, AFM or FE cell count. */
 gf_status gf_average(gf_ctx* ctx, int field_id, double out[3]);
 
 /* Current simulated time (global_parameters::time_device). */
@@ -179,7 +211,8 @@ enum gf_kernel_class {
     GF_K_FFT_Y_INV = 5,
     GF_K_FFT_Z_INV = 6,
     GF_K_OTHER = 7,
-    GF_K_COUNT = 8
+    GF_K_FE_STAGE = 8,
+    GF_K_COUNT = 9
 };
 gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
 gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]);

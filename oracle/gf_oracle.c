@@ -53,6 +53,12 @@ struct gfo_ctx {
     double *m1[3], *m2[3], *ms1[3], *ms2[3], *k1[4][3], *k2[4][3], *dm1[3], *dm2[3];
     unsigned char *asurfYZ, *asurfXZ, *asurfXY;
     double *Aij1[6], *Aij2[6];
+    /* FE (ferroelectric_system.h) */
+    double *p[3], *q[3], *ps[3], *qs[3], *kp[4][3], *kq[4][3], *Es[3];
+    unsigned char *fsurfYZ, *fsurfXZ, *fsurfXY;
+    int if_FE_all; int64_t NFE;
+    double Eext_stat[3], Eext_altr[3], E_altr_freq[3], Eext[3];
+    int npreE; double* preE; int preE_idx;
     /* statics */
     double *H[3], *Hnu[3];
     double *kspec[9];   /* complex spectra, interleaved, nx*ny*(nz/2+1) each */
@@ -311,6 +317,7 @@ static const gf_material* matof(const gfo_ctx* c, int64_t id) {
 }
 static int isFM(const gfo_ctx* c, int64_t id) { const gf_material* m = matof(c, id); return m && m->if_FM; }
 static int isAFM(const gfo_ctx* c, int64_t id) { const gf_material* m = matof(c, id); return m && m->if_AFM; }
+static int isFE(const gfo_ctx* c, int64_t id) { const gf_material* m = matof(c, id); return m && m->if_FE; }
 
 /* mathlib.cpp:46-75 */
 static void glb2crt(const gfo_ctx* c, double ix, double iy, double iz, double* ox, double* oy, double* oz) {
@@ -419,7 +426,22 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     c->pre_idx = 0;
     /* global.cpp:79-116 */
     for (int i = 0; i < c->nmat; i++) { if (c->mat[i].if_FM) c->if_FM_all = 1; if (c->mat[i].if_AFM) c->if_AFM_all = 1; }
-    for (int64_t id = 0; id < c->n; id++) { if (isFM(c, id)) c->NFM++; if (isAFM(c, id)) c->NAFM++; }
+    for (int i = 0; i < c->nmat; i++) if (c->mat[i].if_FE) c->if_FE_all = 1;
+    for (int64_t id = 0; id < c->n; id++) {
+        if (isFM(c, id)) c->NFM++;
+        if (isAFM(c, id)) c->NAFM++;
+        if (isFE(c, id)) c->NFE++;
+    }
+    for (int a = 0; a < 3; a++) {
+        c->Eext_stat[a] = d->Eext_stat[a]; c->Eext_altr[a] = d->Eext_altr[a];
+        c->E_altr_freq[a] = d->E_altr_freq[a]; c->Eext[a] = 0.;
+    }
+    c->npreE = d->n_prescribe_Eext;
+    if (c->npreE > 0) {
+        c->preE = (double*)malloc(sizeof(double) * 4 * (size_t)c->npreE);
+        memcpy(c->preE, d->prescribe_Eext, sizeof(double) * 4 * (size_t)c->npreE);
+    }
+    c->preE_idx = 0;
     int64_t n = c->n;
     for (int a = 0; a < 3; a++) {
         c->m[a] = dalloc(n); c->ms[a] = dalloc(n); c->dm[a] = dalloc(n);
@@ -427,6 +449,8 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         c->m1[a] = dalloc(n); c->m2[a] = dalloc(n); c->ms1[a] = dalloc(n); c->ms2[a] = dalloc(n);
         c->dm1[a] = dalloc(n); c->dm2[a] = dalloc(n);
         c->H[a] = dalloc(n); c->Hnu[a] = dalloc(n);
+        c->p[a] = dalloc(n); c->q[a] = dalloc(n); c->ps[a] = dalloc(n); c->qs[a] = dalloc(n); c->Es[a] = dalloc(n);
+        for (int s = 0; s < 4; s++) { c->kp[s][a] = dalloc(n); c->kq[s][a] = dalloc(n); }
     }
     int64_t nyz = c->ny * c->nz;
     c->surfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
@@ -435,6 +459,10 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     c->asurfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
     c->asurfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
     c->asurfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
+    c->fsurfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
+    c->fsurfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
+    c->fsurfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
+    if (c->if_FE_all) build_masks(c, isFE, c->fsurfYZ, c->fsurfXZ, c->fsurfXY);   /* ferroelectric_initialize.cpp:74-308 */
     for (int f = 0; f < 6; f++) { c->Aij[f] = dalloc(n); c->Aij1[f] = dalloc(n); c->Aij2[f] = dalloc(n); }
     if (c->if_FM_all) {
         build_masks(c, isFM, c->surfYZ, c->surfXZ, c->surfXY);
@@ -468,7 +496,10 @@ void gfo_destroy(gfo_ctx* c) {
         for (int s = 0; s < 4; s++) { free(c->k[s][a]); free(c->k1[s][a]); free(c->k2[s][a]); }
         free(c->m1[a]); free(c->m2[a]); free(c->ms1[a]); free(c->ms2[a]); free(c->dm1[a]); free(c->dm2[a]);
         free(c->H[a]); free(c->Hnu[a]);
+        free(c->p[a]); free(c->q[a]); free(c->ps[a]); free(c->qs[a]); free(c->Es[a]);
+        for (int s = 0; s < 4; s++) { free(c->kp[s][a]); free(c->kq[s][a]); }
     }
+    free(c->fsurfYZ); free(c->fsurfXZ); free(c->fsurfXY); free(c->preE);
     free(c->surfYZ); free(c->surfXZ); free(c->surfXY); free(c->asurfYZ); free(c->asurfXZ); free(c->asurfXY);
     for (int f = 0; f < 6; f++) { free(c->Aij[f]); free(c->Aij1[f]); free(c->Aij2[f]); }
     for (int s = 0; s < 9; s++) free(c->kspec[s]);
@@ -483,6 +514,11 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_HEXT_NONUNIF: return c->Hnu;
     case GF_FIELD_M_AFM1: return c->m1;
     case GF_FIELD_M_AFM2: return c->m2;
+    case GF_FIELD_P: return c->p;
+    case GF_FIELD_Q: return c->q;
+    case GF_FIELD_P_STORE: return c->ps;
+    case GF_FIELD_ESTAT: return c->Es;
+    case GFO_FIELD_Q_STORE: return c->qs;
     default: return NULL;
     }
 }
@@ -495,6 +531,9 @@ int gfo_set_field(gfo_ctx* c, int id, const double* src, size_t count) {
     if (id == GF_FIELD_M) for (int a = 0; a < 3; a++) memcpy(c->ms[a], c->m[a], sizeof(double) * (size_t)c->n);
     if (id == GF_FIELD_M_AFM1) for (int a = 0; a < 3; a++) memcpy(c->ms1[a], c->m1[a], sizeof(double) * (size_t)c->n);
     if (id == GF_FIELD_M_AFM2) for (int a = 0; a < 3; a++) memcpy(c->ms2[a], c->m2[a], sizeof(double) * (size_t)c->n);
+    /* ferroelectric_initialize.cpp:355-366 */
+    if (id == GF_FIELD_P) for (int a = 0; a < 3; a++) memcpy(c->ps[a], c->p[a], sizeof(double) * (size_t)c->n);
+    if (id == GF_FIELD_Q) for (int a = 0; a < 3; a++) memcpy(c->qs[a], c->q[a], sizeof(double) * (size_t)c->n);
     return 0;
 }
 
@@ -510,6 +549,14 @@ int gfo_get_field(const gfo_ctx* c, int id, double* dst, size_t count) {
 /* global.cpp:513-549 (the H part; the E table belongs to the FE path) */
 void gfo_time_advance(gfo_ctx* c) {
     c->time = c->time + c->dt;
+    if (c->flags & GF_FLAG_PRESCRIBE_EEXT) {
+        if (c->preE_idx < c->npreE && (c->time - c->dt) > c->preE[4 * c->preE_idx]) c->preE_idx = c->preE_idx + 1;
+        if (c->preE_idx < c->npreE) {
+            c->Eext[0] = c->preE[4 * c->preE_idx + 1];
+            c->Eext[1] = c->preE[4 * c->preE_idx + 2];
+            c->Eext[2] = c->preE[4 * c->preE_idx + 3];
+        } else { c->Eext[0] = 0.; c->Eext[1] = 0.; c->Eext[2] = 0.; }
+    }
     if (c->flags & GF_FLAG_PRESCRIBE_HEXT) {
         if (c->pre_idx < c->npre && (c->time - c->dt) > c->pre[4 * c->pre_idx]) c->pre_idx = c->pre_idx + 1;
         if (c->pre_idx < c->npre) {
@@ -732,6 +779,137 @@ void gfo_afm_update(gfo_ctx* c, int stage) {
         }
 }
 
+/* ------------------------------------------------------- FE stage */
+/* ferroelectric_solver.cpp:2002-2029 */
+static void update_external_E(gfo_ctx* c, int stage) {
+    double local_time = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    for (int a = 0; a < 3; a++)
+        c->Eext[a] = c->Eext_stat[a] + c->Eext_altr[a] * sin(2. * PI * c->E_altr_freq[a] * local_time);
+}
+
+/* ferroelectric_solver.cpp:4-131: 7-point Laplacian of the p store, a
+ * surface face mirrors the centre value. d2[comp][axis]. */
+static void fe_laplacian(const gfo_ctx* c, int64_t i, int64_t j, int64_t k, double d2[3][3]) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    double cen[3], bw[3], fw[3];
+    int64_t id = ID(c, i, j, k);
+    for (int a = 0; a < 3; a++) cen[a] = c->ps[a][id];
+    double dd[3] = {c->dx, c->dy, c->dz};
+    for (int ax = 0; ax < 3; ax++) {
+        int sb, sf;
+        int64_t nb, nf;
+        if (ax == 0) {
+            sb = sYZ(c->fsurfYZ, c, i, j, k); sf = sYZ(c->fsurfYZ, c, i + 1, j, k);
+            nb = ID(c, i == 0 ? nx - 1 : i - 1, j, k); nf = ID(c, i == nx - 1 ? 0 : i + 1, j, k);
+        } else if (ax == 1) {
+            sb = sXZ(c->fsurfXZ, c, i, j, k); sf = sXZ(c->fsurfXZ, c, i, j + 1, k);
+            nb = ID(c, i, j == 0 ? ny - 1 : j - 1, k); nf = ID(c, i, j == ny - 1 ? 0 : j + 1, k);
+        } else {
+            sb = sXY(c->fsurfXY, c, i, j, k); sf = sXY(c->fsurfXY, c, i, j, k + 1);
+            nb = ID(c, i, j, k == 0 ? nz - 1 : k - 1); nf = ID(c, i, j, k == nz - 1 ? 0 : k + 1);
+        }
+        for (int a = 0; a < 3; a++) {
+            bw[a] = sb ? cen[a] : c->ps[a][nb];
+            fw[a] = sf ? cen[a] : c->ps[a][nf];
+            d2[a][ax] = (fw[a] - 2. * cen[a] + bw[a]) / dd[ax] / dd[ax];
+        }
+    }
+}
+
+/* get_dq_RKs (ferroelectric_solver.cpp:534-842, RK2-4 at :844, :1154,
+ * :1464 differ only in the drive time) and get_dp_RKs (:1786-1832).
+ * Strain terms are zero without an elastic solver; no flexo, no EM. */
+void gfo_fe_get(gfo_ctx* c, int stage) {
+    if (!(c->flags & GF_FLAG_PRESCRIBE_EEXT)) update_external_E(c, stage);
+    int64_t ny = c->ny, nz = c->nz;
+    double** dq = c->kq[stage - 1];
+    double** dp = c->kp[stage - 1];
+    for (int64_t id = 0; id < c->n; id++) {
+        int64_t i = id / (ny * nz), j = (id - i * (ny * nz)) / nz, k = id - i * (ny * nz) - j * nz;
+        const gf_material* mat = matof(c, id);
+        if (!mat || !mat->if_FE) continue;
+        double pgx = c->ps[0][id], pgy = c->ps[1][id], pgz = c->ps[2][id];
+        double d2[3][3];
+        fe_laplacian(c, i, j, k, d2);
+        double px, py, pz;
+        glb2crt(c, pgx, pgy, pgz, &px, &py, &pz);
+        double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
+        double exx0 = 0, eyy0 = 0, ezz0 = 0, eyz0 = 0, exz0 = 0, exy0 = 0;
+        double a1 = mat->a1, a11 = mat->a11, a12 = mat->a12, a111 = mat->a111, a112 = mat->a112, a123 = mat->a123;
+        double a1111 = mat->a1111, a1112 = mat->a1112, a1122 = mat->a1122, a1123 = mat->a1123;
+        double tv1 = mat->tv1, tv2 = mat->tv2, tv3 = mat->tv3, G11 = mat->G11;
+        double FE_mass = -1. * c->dt / mat->FE_mass;
+        double FE_damping = FE_mass * mat->FE_damping;
+        double qx_c = FE_mass * (
+            2. * a1 * px + 4. * a11 * pow(px, 3.)
+            + 2. * a12 * (pow(py, 2.) + pow(pz, 2.)) * px
+            + 6. * a111 * pow(px, 5.)
+            + 4. * a112 * pow(px, 3.) * (pow(py, 2.) + pow(pz, 2.))
+            + 2. * a112 * (pow(py, 4.) + pow(pz, 4.)) * px
+            + 2. * a123 * px * pow(py, 2.) * pow(pz, 2.)
+            + 8. * a1111 * pow(px, 7.)
+            + 2. * a1112 * px * (pow(py, 6.) + pow(pz, 6.) + 3. * pow(px, 4.) * (pow(py, 2.) + pow(pz, 2.)))
+            + 4. * a1122 * pow(px, 3.) * (pow(py, 4.) + pow(pz, 4.))
+            + 2. * a1123 * px * pow(py, 2.) * pow(pz, 2.) * (2. * pow(px, 2.) + pow(py, 2.) + pow(pz, 2.))
+            + (tv1 * (exx0 - exx) + tv2 * (eyy0 - eyy + ezz0 - ezz)) * px
+            + tv3 * ((exy0 - exy) * py + (exz0 - exz) * pz));
+        double qy_c = FE_mass * (
+            2. * a1 * py + 4. * a11 * pow(py, 3.)
+            + 2. * a12 * (pow(pz, 2.) + pow(px, 2.)) * py
+            + 6. * a111 * pow(py, 5.)
+            + 4. * a112 * pow(py, 3.) * (pow(px, 2.) + pow(pz, 2.))
+            + 2. * a112 * (pow(px, 4.) + pow(pz, 4.)) * py
+            + 2. * a123 * py * pow(px, 2.) * pow(pz, 2.)
+            + 8. * a1111 * pow(py, 7)
+            + 2. * a1112 * py * (pow(px, 6.) + pow(pz, 6.) + 3. * pow(py, 4.) * (pow(px, 2.) + pow(pz, 2.)))
+            + 4. * a1122 * pow(py, 3.) * (pow(px, 4.) + pow(pz, 4.))
+            + 2. * a1123 * py * pow(px, 2.) * pow(pz, 2.) * (pow(px, 2.) + 2 * pow(py, 2.) + pow(pz, 2.))
+            + (tv1 * (eyy0 - eyy) + tv2 * (exx0 - exx + ezz0 - ezz)) * py
+            + tv3 * ((exy0 - exy) * px + (eyz0 - eyz) * pz));
+        double qz_c = FE_mass * (
+            2. * a1 * pz + 4. * a11 * pow(pz, 3.)
+            + 2. * a12 * (pow(py, 2.) + pow(px, 2.)) * pz
+            + 6. * a111 * pow(pz, 5.)
+            + 4. * a112 * pow(pz, 3.) * (pow(py, 2.) + pow(px, 2.))
+            + 2. * a112 * (pow(py, 4.) + pow(px, 4.)) * pz
+            + 2. * a123 * pz * pow(py, 2.) * pow(px, 2.)
+            + 8. * a1111 * pow(pz, 7.)
+            + 2. * a1112 * pz * (pow(py, 6.) + pow(px, 6.) + 3. * pow(pz, 4.) * (pow(py, 2.) + pow(px, 2.)))
+            + 4. * a1122 * pow(pz, 3.) * (pow(py, 4.) + pow(px, 4.))
+            + 2. * a1123 * pz * pow(px, 2.) * pow(py, 2.) * (pow(px, 2.) + pow(py, 2.) + 2. * pow(pz, 2.))
+            + (tv1 * (ezz0 - ezz) + tv2 * (exx0 - exx + eyy0 - eyy)) * pz
+            + tv3 * ((exz0 - exz) * px + (eyz0 - eyz) * py));
+        double qg[3];
+        crt2glb(c, qx_c, qy_c, qz_c, &qg[0], &qg[1], &qg[2]);
+        for (int a = 0; a < 3; a++)
+            dq[a][id] = qg[a] + FE_mass * (-1. * G11 * (d2[a][0] + d2[a][1] + d2[a][2]) - c->Es[a][id] - c->Eext[a])
+                      + FE_damping * c->qs[a][id];
+    }
+    for (int a = 0; a < 3; a++)
+        for (int64_t id = 0; id < c->n; id++) dp[a][id] = c->qs[a][id] * c->dt;
+}
+
+/* update_q_RKs / update_p_RKs / update_q / update_p
+ * (ferroelectric_solver.cpp:1835-1999): the state accumulates term by term,
+ * q = q + k1/6 + k2/3 + k3/3 + k4/6, evaluated left to right. */
+void gfo_fe_update(gfo_ctx* c, int stage) {
+    for (int w = 0; w < 2; w++)
+        for (int a = 0; a < 3; a++) {
+            double* y = w == 0 ? c->q[a] : c->p[a];
+            double* s = w == 0 ? c->qs[a] : c->ps[a];
+            const double* k1 = (w == 0 ? c->kq[0] : c->kp[0])[a];
+            const double* k2 = (w == 0 ? c->kq[1] : c->kp[1])[a];
+            const double* k3 = (w == 0 ? c->kq[2] : c->kp[2])[a];
+            const double* kl = (w == 0 ? c->kq[3] : c->kp[3])[a];
+            for (int64_t i = 0; i < c->n; i++) {
+                if (stage == 1) s[i] = y[i] + k1[i] * 0.5;
+                else if (stage == 2) s[i] = y[i] + k2[i] * 0.5;
+                else if (stage == 3) s[i] = y[i] + k3[i];
+                else { y[i] = y[i] + k1[i] / 6. + k2[i] / 3. + k3[i] / 3. + kl[i] / 6.; s[i] = y[i]; }
+            }
+        }
+}
+
 /* ------------------------------------------------------ statics */
 /* magnetostatic_solver.cpp:222-249 */
 static void demag_1d(gfo_ctx* c) {
@@ -819,18 +997,21 @@ void gfo_step(gfo_ctx* c, int64_t nsteps) {
         for (int s = 1; s <= 4; s++) {
             if (c->if_FM_all) gfo_mag_get_dm(c, s);
             if (c->if_AFM_all) gfo_afm_get_dm(c, s);
+            if (c->if_FE_all) gfo_fe_get(c, s);
             if (c->if_FM_all) gfo_mag_update(c, s);
             if (c->if_AFM_all) gfo_afm_update(c, s);
+            if (c->if_FE_all) gfo_fe_update(c, s);
             if (!em && one_d && ((c->if_FM_all && ms) || c->if_AFM_all)) demag_1d(c);
             if (!em && mag && !one_d && ms) demag_3d(c);
         }
     }
 }
 
-/* magnetic_solver.cpp:1863-1888, magnetic_AFM_solver.cpp:2221-2266 */
+/* magnetic_solver.cpp:1863-1888, magnetic_AFM_solver.cpp:2221-2266,
+ * ferroelectric_solver.cpp:2031-2055 */
 void gfo_average(const gfo_ctx* c, int id, double out[3]) {
     double** f = field_ptrs((gfo_ctx*)c, id);
-    int64_t cnt = id == GF_FIELD_M ? c->NFM : c->NAFM;
+    int64_t cnt = id == GF_FIELD_M ? c->NFM : (id == GF_FIELD_P ? c->NFE : c->NAFM);
     for (int a = 0; a < 3; a++) {
         double s = 0.;
         for (int64_t i = 0; i < c->n; i++) s = s + f[a][i];

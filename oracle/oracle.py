@@ -15,6 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 FIELD_DM = 100   # oracle-only: combined RK4 increment of the last step
+FIELD_Q_STORE = 101   # oracle-only: ferroelectric q store
 
 _lib = None
 
@@ -42,6 +43,7 @@ def lib() -> ctypes.CDLL:
         "gfo_time_advance": ([P], None),
         "gfo_mag_get_dm": ([P, ctypes.c_int], None), "gfo_mag_update": ([P, ctypes.c_int], None),
         "gfo_afm_get_dm": ([P, ctypes.c_int], None), "gfo_afm_update": ([P, ctypes.c_int], None),
+        "gfo_fe_get": ([P, ctypes.c_int], None), "gfo_fe_update": ([P, ctypes.c_int], None),
         "gfo_demag": ([P], None), "gfo_step": ([P, i64], None),
         "gfo_average": ([P, ctypes.c_int, dp], None), "gfo_time": ([P], ctypes.c_double),
         "gfo_newell_real": ([i64, i64, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp], None),
@@ -111,6 +113,10 @@ class Oracle:
     def afm_stage(self, s):
         self.L.gfo_afm_get_dm(self._h, s)
         self.L.gfo_afm_update(self._h, s)
+
+    def fe_stage(self, s):
+        self.L.gfo_fe_get(self._h, s)
+        self.L.gfo_fe_update(self._h, s)
 
     def demag(self):
         self.L.gfo_demag(self._h)

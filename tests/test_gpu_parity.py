@@ -145,6 +145,83 @@ def test_fm_and_afm_together():
     compare_stages(prob, {FIELD_M: m, FIELD_M_AFM1: m1, FIELD_M_AFM2: m2}, nsteps=2, afm=True, fm=True)
 
 
+def fe_problem(shape=(8, 8, 32), rotate=True, periodic=(True, True, False), prescribe=False):
+    """FE film between vacuum layers, a second non-FE material, random
+    p, q and static field, AC drive, rotation."""
+    from goferro.workloads import pto_material
+    rng = np.random.default_rng(12)
+    fe = pto_material(FE_mass=1e-16, FE_damping=4e-4)
+    fe2 = Material.make(if_FE=1, FE_mass=3e-16, FE_damping=1e-3, a1=-1.1e8, a11=-2.0e7, a12=3.5e8, a111=1.6e8,
+                        a112=2.1e8, a123=-1.7e9, a1111=4e7, a1112=-1e7, a1122=3e7, a1123=2e7, G11=3e-10)
+    other = Material.make(if_FE=0)
+    cell = np.zeros(shape, np.uint32)
+    nz = shape[2]
+    cell[:, :, nz // 4: 3 * nz // 4] = 1
+    cell[:, :, nz // 2: nz // 2 + 2] = 2
+    cell[0, 0, :] = 3
+    prob = Problem(shape=shape, spacing=(1e-9, 1.5e-9, 1e-9), materials=[fe, fe2, other], material_cell=cell,
+                   dt=2e-15, periodic=periodic, flags=FLAG_ORIENT_ROTATE if rotate else 0,
+                   Tg2c=rot() if rotate else None, Eext_stat=(1e6, 0, -2e6), Eext_altr=(0, 3e6, 1e7),
+                   E_altr_freq=(0, 2e11, 5e11))
+    if prescribe:
+        prob.flags |= 1 << 7
+        prob.prescribe_Eext = np.array([[0.0, 1e6, 0, 2e7], [3e-15, -3e6, 1e6, 1e7], [7e-15, 0, 0, 3e7]])
+    p = 0.1 * rng.standard_normal((3,) + shape)
+    p[2] += 0.7
+    q = 1e9 * rng.standard_normal((3,) + shape)
+    E = 1e7 * rng.standard_normal((3,) + shape)
+    return prob, p, q, E
+
+
+@pytest.mark.parametrize("rotate,prescribe", [(False, False), (True, False), (True, True)])
+def test_fe_stages(rotate, prescribe):
+    from goferro import FIELD_ESTAT, FIELD_P, FIELD_P_STORE, FIELD_Q
+    from oracle.oracle import Oracle
+    prob, p, q, E = fe_problem(rotate=rotate, prescribe=prescribe)
+    say("case fe", prob.shape, rotate, prescribe)
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in ((FIELD_P, p), (FIELD_Q, q), (FIELD_ESTAT, E)):
+        g.upload(fid, v)
+        o.set(fid, v)
+    pscale = np.abs(p).max()
+    for step in range(3):
+        g.time_advance()
+        o.time_advance()
+        for s in range(1, 5):
+            g.fe_stage(s)
+            o.fe_stage(s)
+            d = np.abs(g.download(FIELD_P_STORE) - o.get(FIELD_P_STORE)).max()
+            assert d <= 1e-12 * pscale, (step, s, d)
+    np.testing.assert_allclose(g.download(FIELD_P), o.get(FIELD_P), rtol=0, atol=1e-12 * pscale)
+    qo = o.get(FIELD_Q)
+    np.testing.assert_allclose(g.download(FIELD_Q), qo, rtol=0, atol=1e-12 * np.abs(qo).max())
+    np.testing.assert_allclose(g.average(FIELD_P), o.average(FIELD_P), rtol=0, atol=1e-13 * pscale)
+
+
+def test_fe_and_fm_step_together():
+    from goferro import FIELD_P, FIELD_Q
+    from oracle.oracle import Oracle
+    shape = (8, 8, 16)
+    prob_fe, p, q, _ = fe_problem(shape=shape, rotate=False)
+    prob_fm, m = workloads.fm_film(shape)
+    cell = prob_fe.material_cell.copy()
+    cell[cell == 0] = 4
+    prob = Problem(shape=shape, spacing=prob_fe.spacing, materials=list(prob_fe.materials) + [prob_fm.materials[0]],
+                   material_cell=cell, dt=1e-15, periodic=(True, True, False), flags=FLAG_MAGNETOSTATICS,
+                   Hext_stat=(0, 0, 8e4), Eext_altr=(0, 0, 1e7), E_altr_freq=(0, 0, 1e12))
+    say("case fe+fm", shape)
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in ((FIELD_M, m), (FIELD_P, p), (FIELD_Q, q)):
+        g.upload(fid, v)
+        o.set(fid, v)
+    g.demag()
+    o.demag()
+    g.step(20)
+    o.step(20)
+    np.testing.assert_allclose(g.download(FIELD_M), o.get(FIELD_M), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g.download(FIELD_P), o.get(FIELD_P), rtol=0, atol=1e-12)
+
+
 def test_fm_film_1000_steps():
     """The north-star bound after 1000 steps, at a size the oracle runs in seconds."""
     from oracle.oracle import Oracle
