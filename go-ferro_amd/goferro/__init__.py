@@ -23,6 +23,9 @@ __all__ = [
     "FLAG_ORIENT_ROTATE", "FLAG_HEXT_NONUNIF", "FLAG_EMDYNAMIC", "FLAG_PRESCRIBE_EEXT",
     "FIELD_M", "FIELD_M_STORE", "FIELD_HSTAT", "FIELD_HEXT_NONUNIF", "FIELD_M_AFM1", "FIELD_M_AFM2",
     "FIELD_P", "FIELD_Q", "FIELD_P_STORE", "FIELD_ESTAT",
+    "FIELD_U", "FIELD_V", "FIELD_U_STORE", "FIELD_STRAIN_DIAG", "FIELD_STRAIN_OFF",
+    "FLAG_ELASTODYNAMICS", "FLAG_ELASTO_BACKACTION", "FLAG_ELASTO_ON_PM", "FLAG_GAUSSIAN_PULSE", "FLAG_1D_ABC",
+    "FLAG_1D_ABC_ELAST_ONLYTOP",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS",
 ]
 
@@ -38,9 +41,15 @@ FLAG_ORIENT_ROTATE = 1 << 4
 FLAG_HEXT_NONUNIF = 1 << 5
 FLAG_EMDYNAMIC = 1 << 6
 FLAG_PRESCRIBE_EEXT = 1 << 7
+FLAG_ELASTODYNAMICS = 1 << 8
+FLAG_ELASTO_BACKACTION = 1 << 9
+FLAG_ELASTO_ON_PM = 1 << 10
+FLAG_GAUSSIAN_PULSE = 1 << 11
+FLAG_1D_ABC = 1 << 12
+FLAG_1D_ABC_ELAST_ONLYTOP = 1 << 13
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
- FIELD_ESTAT) = range(10)
+ FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF) = range(15)
 
 KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_conv", "fft_y_inv",
                   "fft_z_inv", "other", "fe_stage"]
@@ -123,6 +132,10 @@ class _Desc(ctypes.Structure):
         ("E_altr_freq", ctypes.c_double * 3),
         ("n_prescribe_Eext", ctypes.c_int32), ("reserved2", ctypes.c_int32),
         ("prescribe_Eext", ctypes.POINTER(ctypes.c_double)),
+        ("input_strain_type", ctypes.c_int32), ("input_strain_sourcez", ctypes.c_int32),
+        ("input_strain_component", ctypes.c_int32), ("num_strain_cycles", ctypes.c_int32),
+        ("sigma_gauss", ctypes.c_double), ("amplitude_gauss", ctypes.c_double),
+        ("weighting_factor_fourth_Liao", ctypes.c_double),
     ]
 
 
@@ -147,6 +160,13 @@ class Problem:
     Eext_altr: tuple[float, float, float] = (0.0, 0.0, 0.0)
     E_altr_freq: tuple[float, float, float] = (0.0, 0.0, 0.0)
     prescribe_Eext: Optional[np.ndarray] = None  # rows (t, Ex, Ey, Ez)
+    input_strain_type: int = 0
+    input_strain_sourcez: int = 0
+    input_strain_component: str = "x"
+    num_strain_cycles: int = 0
+    sigma_gauss: float = 0.0
+    amplitude_gauss: float = 0.0
+    weighting_factor_fourth_Liao: float = 0.5
 
     @property
     def n(self) -> int:
@@ -180,6 +200,13 @@ class Problem:
         d.Eext_stat[:] = [float(v) for v in self.Eext_stat]
         d.Eext_altr[:] = [float(v) for v in self.Eext_altr]
         d.E_altr_freq[:] = [float(v) for v in self.E_altr_freq]
+        d.input_strain_type = int(self.input_strain_type)
+        d.input_strain_sourcez = int(self.input_strain_sourcez)
+        d.input_strain_component = ord(self.input_strain_component[0]) if self.input_strain_component else 0
+        d.num_strain_cycles = int(self.num_strain_cycles)
+        d.sigma_gauss = float(self.sigma_gauss)
+        d.amplitude_gauss = float(self.amplitude_gauss)
+        d.weighting_factor_fourth_Liao = float(self.weighting_factor_fourth_Liao)
         keep = [cell, mats]
         if self.prescribe_Hext is not None:
             pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)

@@ -42,6 +42,79 @@ def fm_film(shape=(64, 64, 16), dx=2e-9, dt=1e-14, seed=1, demag=True, bias_T=0.
     return prob, tilted_m(shape, seed)
 
 
+def cijkl_glb(c11: float, c12: float, c44: float, Tg2c=None) -> np.ndarray:
+    """6x6 global-frame stiffness of a cubic crystal, Voigt order
+    xx yy zz yz xz xy, rotated as global_parameters::get_cijkl does
+    (global.cpp:365-441): C_wxyz = T_wi T_xj T_yk T_zl C_ijkl with T = Tc2g."""
+    Cc = np.zeros((6, 6))
+    Cc[:3, :3] = c12
+    for a in range(3):
+        Cc[a, a] = c11
+        Cc[3 + a, 3 + a] = c44
+    voigt = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    C4 = np.zeros((3, 3, 3, 3))
+    for I, (w, x) in enumerate(voigt):
+        for J, (y, z) in enumerate(voigt):
+            for (a, b) in ((w, x), (x, w)):
+                for (c, d) in ((y, z), (z, y)):
+                    C4[a, b, c, d] = Cc[I, J]
+    T = np.eye(3) if Tg2c is None else np.asarray(Tg2c, dtype=float).T
+    Cg = np.einsum("wi,xj,yk,zl,ijkl->wxyz", T, T, T, T, C4)
+    out = np.zeros((6, 6))
+    for I, (w, x) in enumerate(voigt):
+        for J, (y, z) in enumerate(voigt):
+            out[I, J] = Cg[w, x, y, z]
+    return out
+
+
+def set_elastic(mat: Material, c11: float, c12: float, c44: float, density: float, Tg2c=None,
+                mass_damping: float = 0.0, stiff_damping: float = 0.0) -> Material:
+    """Fill the elastic record of a material (read.cpp:128-130, global.cpp:365-441)."""
+    mat.c11, mat.c12, mat.c44, mat.density = c11, c12, c44, density
+    mat.elast_mass_damping, mat.elast_stiff_damping = mass_damping, stiff_damping
+    C = cijkl_glb(c11, c12, c44, Tg2c)
+    for a in range(6):
+        for b in range(6):
+            mat.cijkl_glb[a][b] = float(C[a, b])
+    return mat
+
+
+def set_magnetostriction(mat: Material, B1: float, B2: float) -> Material:
+    """Magnetostriction constants from the magnetoelastic coefficients,
+    inverting read.cpp:141-142 (B1 = -3/2 lambda100 (c11-c12), B2 = -3 lambda111 c44)."""
+    mat.lamda100 = -2.0 * B1 / (3.0 * (mat.c11 - mat.c12))
+    mat.lamda111 = -B2 / (3.0 * mat.c44)
+    mat.B1 = -3.0 / 2.0 * mat.lamda100 * (mat.c11 - mat.c12)
+    mat.B2 = -3.0 * mat.lamda111 * mat.c44
+    return mat
+
+
+def magnetoelastic_film(shape=(128, 128, 64), magnet_layers=16, dx=2e-9, dt=5e-15, seed=2):
+    """Magnon-phonon film: a YIG-like magnet (alpha 1e-3, B1 = B2 = 3.5e5
+    J/m^3) on a non-magnetic substrate (c11 2.69e11, c12 1.08e11, c44
+    0.76e11 Pa, rho 5170 kg/m^3) with the same stiffness; elastodynamics
+    coupled both ways; fourth-order Liao absorbing bottom; a Gaussian shear
+    displacement pulse (amplitude 1e-4, width 1 ps) launched at the top
+    layer.  Returns (Problem, m0)."""
+    from . import FLAG_ELASTO_BACKACTION, FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS, FLAG_GAUSSIAN_PULSE
+    c11, c12, c44, rho = 2.69e11, 1.08e11, 0.76e11, 5170.0
+    mag = yig_material()
+    mag.FM_damping = 1e-3
+    set_elastic(mag, c11, c12, c44, rho)
+    set_magnetostriction(mag, 3.5e5, 3.5e5)
+    sub = set_elastic(Material.make(if_FM=0), c11, c12, c44, rho)
+    cell = np.full(shape, 2, dtype=np.uint32)
+    cell[:, :, shape[2] - magnet_layers:] = 1
+    prob = Problem(shape=tuple(shape), spacing=(dx, dx, dx), materials=[mag, sub], material_cell=cell, dt=dt,
+                   periodic=(True, True, False),
+                   flags=FLAG_ELASTODYNAMICS | FLAG_ELASTO_BACKACTION | FLAG_ELASTO_ON_PM | FLAG_GAUSSIAN_PULSE,
+                   input_strain_type=1, input_strain_sourcez=shape[2] - 1, input_strain_component="x",
+                   sigma_gauss=1e-12, amplitude_gauss=1e-4, weighting_factor_fourth_Liao=0.5)
+    m = tilted_m(shape, seed)
+    m[:, :, :, : shape[2] - magnet_layers] = 0.0
+    return prob, m
+
+
 def pto_material(T=300.0, FE_mass=1e-16, FE_damping=0.0, G11=1e-10, eps_r=10.0) -> Material:
     """PbTiO3-like ferroelectric with a 6th-order Landau potential:
     a1 = 3.74e5 (T - 752), a11 = -7.3e7, a12 = 7.5e8, a111 = 2.6e8,

@@ -48,6 +48,12 @@ typedef int gf_status;
 #define GF_FLAG_HEXT_NONUNIF     (1u << 5)  /* Hext_nonunif field uploaded */
 #define GF_FLAG_EMDYNAMIC        (1u << 6)  /* if_EMdynamic (statics frozen) */
 #define GF_FLAG_PRESCRIBE_EEXT   (1u << 7)  /* if_prescribe_Eext (table drive) */
+#define GF_FLAG_ELASTODYNAMICS   (1u << 8)  /* if_elastodynamics */
+#define GF_FLAG_ELASTO_BACKACTION (1u << 9) /* if_elasto_backaction_from_pORm: eigenstrain from m, p */
+#define GF_FLAG_ELASTO_ON_PM     (1u << 10) /* if_elasto_on_pORm: strain acts on m and p */
+#define GF_FLAG_GAUSSIAN_PULSE   (1u << 11) /* if_gaussian_strain_pulse */
+#define GF_FLAG_1D_ABC           (1u << 12) /* if_1D_ABC: Mur instead of Liao */
+#define GF_FLAG_1D_ABC_ELAST_ONLYTOP (1u << 13) /* if_1D_ABC_ELAST_onlytop */
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -120,6 +126,14 @@ typedef struct gf_desc {
     int32_t n_prescribe_Eext;
     int32_t reserved2;
     const double* prescribe_Eext;   /* rows of (t, Ex, Ey, Ez), copied at create */
+    /* elastodynamic sources and boundaries (read.cpp:58-66) */
+    int32_t input_strain_type;      /* 1 Gaussian displacement, 2 cosine, 3 Gaussian traction */
+    int32_t input_strain_sourcez;   /* source plane k */
+    int32_t input_strain_component; /* 'x', 'y' or 'z' */
+    int32_t num_strain_cycles;
+    double sigma_gauss;             /* s (or Hz for type 2) */
+    double amplitude_gauss;
+    double weighting_factor_fourth_Liao;
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */
@@ -134,6 +148,11 @@ enum gf_field_id {
     GF_FIELD_Q = 7,         /* dp/dt, 3n                                   */
     GF_FIELD_P_STORE = 8,   /* p at the current RK stage, 3n               */
     GF_FIELD_ESTAT = 9,     /* static electric field, 3n (input)           */
+    GF_FIELD_U = 10,        /* displacement u, 3n (m)                      */
+    GF_FIELD_V = 11,        /* velocity v, 3n (m/s)                        */
+    GF_FIELD_U_STORE = 12,  /* u at the current RK stage, 3n               */
+    GF_FIELD_STRAIN_DIAG = 13, /* dynamic strain exx, eyy, ezz (global), 3n, read only */
+    GF_FIELD_STRAIN_OFF = 14,  /* dynamic strain eyz, exz, exy (global), 3n, read only */
     GF_FIELD_COUNT
 };
 

@@ -59,10 +59,25 @@ struct gfo_ctx {
     int if_FE_all; int64_t NFE;
     double Eext_stat[3], Eext_altr[3], E_altr_freq[3], Eext[3];
     int npreE; double* preE; int preE_idx;
+    /* elastodynamics (elastic_system.h) */
+    int if_elasto;
+    double *u[3], *v[3], *us[3], *vs[3], *ku[4][3], *kv[4][3];
+    double *force[3], *force_store[3];
+    double *gYZ[3], *gXZ[3], *gXY[3];          /* du{x,y,z}/d{x,y,z} on faces */
+    double *e_glb[6], *e_crt[6];               /* dynamic strain xx yy zz yz xz xy */
+    double *e0_glb[6], *e0_crt[6];             /* dynamic eigenstrain */
+    double *t1[3], *t2[3], *t3[3], *t4[3];     /* displacement history (Liao / Mur) */
+    unsigned char *sfYZ, *sfXZ, *sfXY;         /* stress-free faces */
+    double vs_long_1st, vs_tran_1st, vs_long_last, vs_tran_last;
+    int source_type, source_z, source_comp, num_cycles;
+    double sigma_gauss, amplitude_gauss, w_liao;
     /* statics */
     double *H[3], *Hnu[3];
     double *kspec[9];   /* complex spectra, interleaved, nx*ny*(nz/2+1) each */
 };
+
+static void elastic_gradients(gfo_ctx* c);
+static void elastic_strain(gfo_ctx* c);
 
 /* ------------------------------------------------------------------ FFT */
 /* In-place iterative radix-2 complex FFT, unnormalised, sign -1 forward. */
@@ -442,6 +457,33 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         memcpy(c->preE, d->prescribe_Eext, sizeof(double) * 4 * (size_t)c->npreE);
     }
     c->preE_idx = 0;
+    /* elastic_initialize.cpp:4-83, 245-336 */
+    c->if_elasto = (c->flags & GF_FLAG_ELASTODYNAMICS) != 0;
+    /* periodic z with ny != nz makes the reference read past its arrays
+     * (the k == ny-1 test of elastic_dynamic_solver.cpp:1247) */
+    if (c->if_elasto && c->per[2] && c->ny != c->nz) { free(c->mat); free(c->cell); free(c->pre); free(c); return NULL; }
+    if (c->if_elasto && !(c->flags & GF_FLAG_1D_ABC) && c->nz < 5) { free(c->mat); free(c->cell); free(c->pre); free(c); return NULL; }
+    c->source_type = d->input_strain_type; c->source_z = d->input_strain_sourcez;
+    c->source_comp = d->input_strain_component; c->num_cycles = d->num_strain_cycles;
+    c->sigma_gauss = d->sigma_gauss; c->amplitude_gauss = d->amplitude_gauss;
+    c->w_liao = d->weighting_factor_fourth_Liao;
+    {
+        double c11_1st = 0., c44_1st = 0., rho_1st = 1.e-8, c11_last = 0., c44_last = 0., rho_last = 1.e-8;
+        int found = 0;
+        for (int64_t i = 0; i < c->nx && !found; i++)
+            for (int64_t j = 0; j < c->ny && !found; j++) {
+                const gf_material* m = matof(c, ID(c, i, j, 0));
+                if (m) { c11_1st = m->c11; c44_1st = m->c44; rho_1st = m->density; found = 1; }
+            }
+        found = 0;
+        for (int64_t i = 0; i < c->nx && !found; i++)
+            for (int64_t j = 0; j < c->ny && !found; j++) {
+                const gf_material* m = matof(c, ID(c, i, j, c->nz - 1));
+                if (m) { c11_last = m->c11; c44_last = m->c44; rho_last = m->density; found = 1; }
+            }
+        c->vs_long_1st = sqrt((c11_1st / rho_1st)); c->vs_tran_1st = sqrt((c44_1st / rho_1st));
+        c->vs_long_last = sqrt((c11_last / rho_last)); c->vs_tran_last = sqrt((c44_last / rho_last));
+    }
     int64_t n = c->n;
     for (int a = 0; a < 3; a++) {
         c->m[a] = dalloc(n); c->ms[a] = dalloc(n); c->dm[a] = dalloc(n);
@@ -463,6 +505,43 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     c->fsurfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
     c->fsurfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
     if (c->if_FE_all) build_masks(c, isFE, c->fsurfYZ, c->fsurfXZ, c->fsurfXY);   /* ferroelectric_initialize.cpp:74-308 */
+    for (int a = 0; a < 3; a++) {
+        c->u[a] = dalloc(n); c->v[a] = dalloc(n); c->us[a] = dalloc(n); c->vs[a] = dalloc(n);
+        for (int s = 0; s < 4; s++) { c->ku[s][a] = dalloc(n); c->kv[s][a] = dalloc(n); }
+        c->force[a] = dalloc(n); c->force_store[a] = dalloc(n);
+        c->gYZ[a] = dalloc((c->nx + 1) * nyz); c->gXZ[a] = dalloc(c->nx * (c->ny + 1) * c->nz);
+        c->gXY[a] = dalloc(c->nx * c->ny * (c->nz + 1));
+        c->t1[a] = dalloc(n); c->t2[a] = dalloc(n); c->t3[a] = dalloc(n); c->t4[a] = dalloc(n);
+    }
+    for (int a = 0; a < 6; a++) { c->e_glb[a] = dalloc(n); c->e_crt[a] = dalloc(n); c->e0_glb[a] = dalloc(n); c->e0_crt[a] = dalloc(n); }
+    c->sfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
+    c->sfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
+    c->sfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
+    /* stress-free faces: a vacuum side or a non-periodic wall (elastic_initialize.cpp:245-336) */
+    for (int64_t i = 0; i <= c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++)
+            for (int64_t k = 0; k < c->nz; k++) {
+                int f;
+                if (i == 0 || i == c->nx) f = c->per[0] ? !(c->cell[ID(c, 0, j, k)] && c->cell[ID(c, c->nx - 1, j, k)]) : 1;
+                else f = !(c->cell[ID(c, i - 1, j, k)] && c->cell[ID(c, i, j, k)]);
+                c->sfYZ[(i * c->ny + j) * c->nz + k] = (unsigned char)f;
+            }
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j <= c->ny; j++)
+            for (int64_t k = 0; k < c->nz; k++) {
+                int f;
+                if (j == 0 || j == c->ny) f = c->per[1] ? !(c->cell[ID(c, i, 0, k)] && c->cell[ID(c, i, c->ny - 1, k)]) : 1;
+                else f = !(c->cell[ID(c, i, j - 1, k)] && c->cell[ID(c, i, j, k)]);
+                c->sfXZ[(i * (c->ny + 1) + j) * c->nz + k] = (unsigned char)f;
+            }
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++)
+            for (int64_t k = 0; k <= c->nz; k++) {
+                int f;
+                if (k == 0 || k == c->nz) f = c->per[2] ? !(c->cell[ID(c, i, j, 0)] && c->cell[ID(c, i, j, c->nz - 1)]) : 1;
+                else f = !(c->cell[ID(c, i, j, k - 1)] && c->cell[ID(c, i, j, k)]);
+                c->sfXY[(i * c->ny + j) * (c->nz + 1) + k] = (unsigned char)f;
+            }
     for (int f = 0; f < 6; f++) { c->Aij[f] = dalloc(n); c->Aij1[f] = dalloc(n); c->Aij2[f] = dalloc(n); }
     if (c->if_FM_all) {
         build_masks(c, isFM, c->surfYZ, c->surfXZ, c->surfXY);
@@ -500,6 +579,14 @@ void gfo_destroy(gfo_ctx* c) {
         for (int s = 0; s < 4; s++) { free(c->kp[s][a]); free(c->kq[s][a]); }
     }
     free(c->fsurfYZ); free(c->fsurfXZ); free(c->fsurfXY); free(c->preE);
+    for (int a = 0; a < 3; a++) {
+        free(c->u[a]); free(c->v[a]); free(c->us[a]); free(c->vs[a]);
+        for (int s = 0; s < 4; s++) { free(c->ku[s][a]); free(c->kv[s][a]); }
+        free(c->force[a]); free(c->force_store[a]); free(c->gYZ[a]); free(c->gXZ[a]); free(c->gXY[a]);
+        free(c->t1[a]); free(c->t2[a]); free(c->t3[a]); free(c->t4[a]);
+    }
+    for (int a = 0; a < 6; a++) { free(c->e_glb[a]); free(c->e_crt[a]); free(c->e0_glb[a]); free(c->e0_crt[a]); }
+    free(c->sfYZ); free(c->sfXZ); free(c->sfXY);
     free(c->surfYZ); free(c->surfXZ); free(c->surfXY); free(c->asurfYZ); free(c->asurfXZ); free(c->asurfXY);
     for (int f = 0; f < 6; f++) { free(c->Aij[f]); free(c->Aij1[f]); free(c->Aij2[f]); }
     for (int s = 0; s < 9; s++) free(c->kspec[s]);
@@ -519,6 +606,12 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_P_STORE: return c->ps;
     case GF_FIELD_ESTAT: return c->Es;
     case GFO_FIELD_Q_STORE: return c->qs;
+    case GFO_FIELD_DV1: return c->kv[0];
+    case GF_FIELD_U: return c->u;
+    case GF_FIELD_V: return c->v;
+    case GF_FIELD_U_STORE: return c->us;
+    case GF_FIELD_STRAIN_DIAG: return c->e_glb;
+    case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
     default: return NULL;
     }
 }
@@ -534,6 +627,13 @@ int gfo_set_field(gfo_ctx* c, int id, const double* src, size_t count) {
     /* ferroelectric_initialize.cpp:355-366 */
     if (id == GF_FIELD_P) for (int a = 0; a < 3; a++) memcpy(c->ps[a], c->p[a], sizeof(double) * (size_t)c->n);
     if (id == GF_FIELD_Q) for (int a = 0; a < 3; a++) memcpy(c->qs[a], c->q[a], sizeof(double) * (size_t)c->n);
+    /* elastic_initialize.cpp:386-410: stores equal the state, strain from u */
+    if (id == GF_FIELD_V) for (int a = 0; a < 3; a++) memcpy(c->vs[a], c->v[a], sizeof(double) * (size_t)c->n);
+    if (id == GF_FIELD_U) {
+        for (int a = 0; a < 3; a++) memcpy(c->us[a], c->u[a], sizeof(double) * (size_t)c->n);
+        elastic_gradients(c);
+        elastic_strain(c);
+    }
     return 0;
 }
 
@@ -655,6 +755,10 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
         double B1 = mat->B1 / MU0 / Ms, B2 = mat->B2 / MU0 / Ms;
         double Aexi = mat->Aex, iDMIi = mat->iDMI;
         double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
+        if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* :89-96, static strain zero */
+            exx = 0. + c->e_crt[0][i]; eyy = 0. + c->e_crt[1][i]; ezz = 0. + c->e_crt[2][i];
+            eyz = 0. + c->e_crt[3][i]; exz = 0. + c->e_crt[4][i]; exy = 0. + c->e_crt[5][i];
+        }
         double d[4], d2[3][2][3];
         stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, Aexi, iDMIi, x, y, z, d, d2);
         double hac[3] = {0., 0., 0.}, ha[3], hec[3], he[3], hx[3], hd[3], H[3];
@@ -719,6 +823,10 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
             int64_t x = i / (ny * nz), y = (i - x * (ny * nz)) / nz, z = i - x * (ny * nz) - y * nz;
             double ux = mat->uniaxis_AFM[0], uy = mat->uniaxis_AFM[1], uz = mat->uniaxis_AFM[2];
             double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
+            if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* magnetic_AFM_solver.cpp:89-96 */
+                exx = 0. + c->e_crt[0][i]; eyy = 0. + c->e_crt[1][i]; ezz = 0. + c->e_crt[2][i];
+                eyz = 0. + c->e_crt[3][i]; exz = 0. + c->e_crt[4][i]; exy = 0. + c->e_crt[5][i];
+            }
             double mgx = st[0][i], mgy = st[1][i], mgz = st[2][i];
             double mx, my, mz;
             glb2crt(c, mgx, mgy, mgz, &mx, &my, &mz);
@@ -835,6 +943,12 @@ void gfo_fe_get(gfo_ctx* c, int stage) {
         glb2crt(c, pgx, pgy, pgz, &px, &py, &pz);
         double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
         double exx0 = 0, eyy0 = 0, ezz0 = 0, eyz0 = 0, exz0 = 0, exy0 = 0;
+        if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* ferroelectric_solver.cpp:624-653 (no elastostatics) */
+            exx = c->e_crt[0][id]; eyy = c->e_crt[1][id]; ezz = c->e_crt[2][id];
+            eyz = c->e_crt[3][id]; exz = c->e_crt[4][id]; exy = c->e_crt[5][id];
+            exx0 = c->e0_crt[0][id]; eyy0 = c->e0_crt[1][id]; ezz0 = c->e0_crt[2][id];
+            eyz0 = c->e0_crt[3][id]; exz0 = c->e0_crt[4][id]; exy0 = c->e0_crt[5][id];
+        }
         double a1 = mat->a1, a11 = mat->a11, a12 = mat->a12, a111 = mat->a111, a112 = mat->a112, a123 = mat->a123;
         double a1111 = mat->a1111, a1112 = mat->a1112, a1122 = mat->a1122, a1123 = mat->a1123;
         double tv1 = mat->tv1, tv2 = mat->tv2, tv3 = mat->tv3, G11 = mat->G11;
@@ -908,6 +1022,366 @@ void gfo_fe_update(gfo_ctx* c, int stage) {
                 else { y[i] = y[i] + k1[i] / 6. + k2[i] / 3. + k3[i] / 3. + kl[i] / 6.; s[i] = y[i]; }
             }
         }
+}
+
+/* --------------------------------------------------- elastodynamics */
+static int sfx(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return c->sfYZ[(i * c->ny + j) * c->nz + k]; }
+static int sfy(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return c->sfXZ[(i * (c->ny + 1) + j) * c->nz + k]; }
+static int sfz(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return c->sfXY[(i * c->ny + j) * (c->nz + 1) + k]; }
+#define GYZ(a, i, j, k) c->gYZ[a][((i) * c->ny + (j)) * c->nz + (k)]
+#define GXZ(a, i, j, k) c->gXZ[a][((i) * (c->ny + 1) + (j)) * c->nz + (k)]
+#define GXY(a, i, j, k) c->gXY[a][((i) * c->ny + (j)) * (c->nz + 1) + (k)]
+
+/* mathlib.cpp:78-125 */
+static void mat_rot(const double T[3][3], const double in[3][3], double out[3][3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            out[i][j] = 0.;
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) out[i][j] = out[i][j] + T[i][k] * T[j][l] * in[k][l];
+        }
+}
+
+/* elastic_dynamic_solver.cpp:4-104 (static eigenstrain and flexo terms zero) */
+static void elastic_eigenstrain(gfo_ctx* c) {
+    for (int64_t id = 0; id < c->n; id++) {
+        const gf_material* mat = matof(c, id);
+        if (!mat) {
+            for (int a = 0; a < 6; a++) { c->e0_crt[a][id] = 0.; c->e0_glb[a][id] = 0.; }
+            continue;
+        }
+        double Q11 = mat->Q11, Q12 = mat->Q12, Q44 = mat->Q44;
+        double l100 = mat->lamda100, l111 = mat->lamda111;
+        double l100a = mat->lamda100_AFM1, l111a = mat->lamda111_AFM1, l100b = mat->lamda100_AFM2, l111b = mat->lamda111_AFM2;
+        double px, py, pz, mx, my, mz, ax, ay, az, bx, by, bz;
+        glb2crt(c, c->ps[0][id], c->ps[1][id], c->ps[2][id], &px, &py, &pz);
+        glb2crt(c, c->ms[0][id], c->ms[1][id], c->ms[2][id], &mx, &my, &mz);
+        glb2crt(c, c->ms1[0][id], c->ms1[1][id], c->ms1[2][id], &ax, &ay, &az);
+        glb2crt(c, c->ms2[0][id], c->ms2[1][id], c->ms2[2][id], &bx, &by, &bz);
+        double s[3][3], o[3][3];
+        s[0][0] = 1.5 * l100 * (pow(mx, 2.) - 1. / 3.) + Q11 * pow(px, 2.) + Q12 * (pow(py, 2.) + pow(pz, 2.))
+                + 0.5 * 1.5 * l100a * (pow(ax, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(bx, 2.) - 1. / 3.) - 0.;
+        s[1][1] = 1.5 * l100 * (pow(my, 2.) - 1. / 3.) + Q11 * pow(py, 2.) + Q12 * (pow(px, 2.) + pow(pz, 2.))
+                + 0.5 * 1.5 * l100a * (pow(ay, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(by, 2.) - 1. / 3.) - 0.;
+        s[2][2] = 1.5 * l100 * (pow(mz, 2.) - 1. / 3.) + Q11 * pow(pz, 2.) + Q12 * (pow(py, 2.) + pow(px, 2.))
+                + 0.5 * 1.5 * l100a * (pow(az, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(bz, 2.) - 1. / 3.) - 0.;
+        s[1][2] = 1.5 * l111 * my * mz + Q44 * py * pz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz - 0.;
+        s[0][2] = 1.5 * l111 * mx * mz + Q44 * px * pz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz - 0.;
+        s[0][1] = 1.5 * l111 * my * mx + Q44 * py * px + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx - 0.;
+        c->e0_crt[0][id] = s[0][0]; c->e0_crt[1][id] = s[1][1]; c->e0_crt[2][id] = s[2][2];
+        c->e0_crt[3][id] = s[1][2]; s[2][1] = s[1][2];
+        c->e0_crt[4][id] = s[0][2]; s[2][0] = s[0][2];
+        c->e0_crt[5][id] = s[0][1]; s[1][0] = s[0][1];
+        if (c->flags & GF_FLAG_ORIENT_ROTATE) mat_rot(c->Tc2g, s, o);
+        else memcpy(o, s, sizeof o);
+        c->e0_glb[0][id] = o[0][0]; c->e0_glb[1][id] = o[1][1]; c->e0_glb[2][id] = o[2][2];
+        c->e0_glb[3][id] = o[1][2]; c->e0_glb[4][id] = o[0][2]; c->e0_glb[5][id] = o[0][1];
+    }
+}
+
+/* elastic_dynamic_solver.cpp:107-187 */
+static void elastic_gradients(gfo_ctx* c) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    double** s = c->us;
+    for (int64_t i = 0; i <= nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k < nz; k++)
+                for (int a = 0; a < 3; a++) {
+                    double g;
+                    if (sfx(c, i, j, k)) g = 0.;
+                    else if (i == 0 || i == nx) g = (s[a][ID(c, 0, j, k)] - s[a][ID(c, nx - 1, j, k)]) / c->dx;
+                    else g = (s[a][ID(c, i, j, k)] - s[a][ID(c, i - 1, j, k)]) / c->dx;
+                    GYZ(a, i, j, k) = g;
+                }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j <= ny; j++)
+            for (int64_t k = 0; k < nz; k++)
+                for (int a = 0; a < 3; a++) {
+                    double g;
+                    if (sfy(c, i, j, k)) g = 0.;
+                    else if (j == 0 || j == ny) g = (s[a][ID(c, i, 0, k)] - s[a][ID(c, i, ny - 1, k)]) / c->dy;
+                    else g = (s[a][ID(c, i, j, k)] - s[a][ID(c, i, j - 1, k)]) / c->dy;
+                    GXZ(a, i, j, k) = g;
+                }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k <= nz; k++)
+                for (int a = 0; a < 3; a++) {
+                    double g;
+                    if (sfz(c, i, j, k)) g = 0.;
+                    else if (k == 0 || k == nz) g = (s[a][ID(c, i, j, 0)] - s[a][ID(c, i, j, nz - 1)]) / c->dz;
+                    else g = (s[a][ID(c, i, j, k)] - s[a][ID(c, i, j, k - 1)]) / c->dz;
+                    GXY(a, i, j, k) = g;
+                }
+}
+
+/* elastic_dynamic_solver.cpp:190-227 */
+static void elastic_strain(gfo_ctx* c) {
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++)
+            for (int64_t k = 0; k < c->nz; k++) {
+                int64_t id = ID(c, i, j, k);
+                double s[3][3], o[3][3];
+                s[0][0] = (GYZ(0, i, j, k) + GYZ(0, i + 1, j, k)) / 2.;
+                s[1][1] = (GXZ(1, i, j, k) + GXZ(1, i, j + 1, k)) / 2.;
+                s[2][2] = (GXY(2, i, j, k) + GXY(2, i, j, k + 1)) / 2.;
+                s[1][2] = (GXY(1, i, j, k) + GXY(1, i, j, k + 1) + GXZ(2, i, j, k) + GXZ(2, i, j + 1, k)) / 4.;
+                s[2][1] = s[1][2];
+                s[0][2] = (GXY(0, i, j, k) + GXY(0, i, j, k + 1) + GYZ(2, i, j, k) + GYZ(2, i + 1, j, k)) / 4.;
+                s[2][0] = s[0][2];
+                s[0][1] = (GXZ(0, i, j, k) + GXZ(0, i, j + 1, k) + GYZ(1, i, j, k) + GYZ(1, i + 1, j, k)) / 4.;
+                s[1][0] = s[0][1];
+                if (c->flags & GF_FLAG_ORIENT_ROTATE) mat_rot(c->Tg2c, s, o);
+                else memcpy(o, s, sizeof o);
+                c->e_glb[0][id] = s[0][0]; c->e_crt[0][id] = o[0][0];
+                c->e_glb[1][id] = s[1][1]; c->e_crt[1][id] = o[1][1];
+                c->e_glb[2][id] = s[2][2]; c->e_crt[2][id] = o[2][2];
+                c->e_glb[3][id] = s[1][2]; c->e_crt[3][id] = o[1][2];
+                c->e_glb[4][id] = s[0][2]; c->e_crt[4][id] = o[0][2];
+                c->e_glb[5][id] = s[0][1]; c->e_crt[5][id] = o[0][1];
+            }
+}
+
+/* mathlib.cpp:128-135 */
+static void c_dot(const double C[6][6], const double e[6], double s[6]) {
+    for (int i = 0; i < 6; i++) {
+        s[i] = 0.;
+        for (int j = 0; j < 6; j++) s[i] = s[i] + C[i][j] * e[j];
+    }
+}
+
+#define EG(a, id) c->e_glb[a][id]
+#define E0(a, id) c->e0_glb[a][id]
+
+/* stress on a YZ face fi seen from cell (qi,j,k) (elastic_dynamic_solver.cpp:876-933) */
+static void stress_yz(const gfo_ctx* c, int64_t fi, int64_t qi, int64_t j, int64_t k, double st[6]) {
+    int64_t q = ID(c, qi, j, k);
+    double e[6];
+    e[0] = GYZ(0, fi, j, k) - E0(0, q);
+    e[1] = EG(1, q) - E0(1, q);
+    e[2] = EG(2, q) - E0(2, q);
+    e[3] = (EG(3, q) - E0(3, q)) * 2.;
+    e[4] = ((GXY(0, qi, j, k) + GXY(0, qi, j, k + 1) + GYZ(2, fi, j, k) * 2.) / 4. - E0(4, q)) * 2.;
+    e[5] = ((GXZ(0, qi, j, k) + GXZ(0, qi, j + 1, k) + GYZ(1, fi, j, k) * 2.) / 4. - E0(5, q)) * 2.;
+    c_dot(matof(c, q)->cijkl_glb, e, st);
+}
+
+/* stress on an XZ face fj seen from cell (i,qj,k) (:994-1050) */
+static void stress_xz(const gfo_ctx* c, int64_t i, int64_t fj, int64_t qj, int64_t k, double st[6]) {
+    int64_t q = ID(c, i, qj, k);
+    double e[6];
+    e[0] = EG(0, q) - E0(0, q);
+    e[1] = GXZ(1, i, fj, k) - E0(1, q);
+    e[2] = EG(2, q) - E0(2, q);
+    e[3] = ((GXY(1, i, qj, k) + GXY(1, i, qj, k + 1) + GXZ(2, i, fj, k) * 2.) / 4. - E0(3, q)) * 2.;
+    e[4] = (EG(4, q) - E0(4, q)) * 2.;
+    e[5] = ((GXZ(0, i, fj, k) * 2. + GYZ(1, i, qj, k) + GYZ(1, i + 1, qj, k)) / 4. - E0(5, q)) * 2.;
+    c_dot(matof(c, q)->cijkl_glb, e, st);
+}
+
+/* stress on an XY face fk seen from cell with flat index q and z index qk
+ * (:1112-1166 and :1229-1290; q may alias past the column, see below) */
+static void stress_xy(const gfo_ctx* c, int64_t i, int64_t j, int64_t fk, int64_t qk, double st[6]) {
+    int64_t q = (i * c->ny + j) * c->nz + qk;
+    double e[6];
+    e[0] = EG(0, q) - E0(0, q);
+    e[1] = EG(1, q) - E0(1, q);
+    e[2] = GXY(2, i, j, fk) - E0(2, q);
+    e[3] = ((GXY(1, i, j, fk) * 2. + GXZ(2, i, j, qk) + GXZ(2, i, j + 1, qk)) / 4. - E0(3, q)) * 2.;
+    e[4] = ((GXY(0, i, j, fk) * 2. + GYZ(2, i, j, qk) + GYZ(2, i + 1, j, qk)) / 4. - E0(4, q)) * 2.;
+    e[5] = (EG(5, q) - E0(5, q)) * 2.;
+    c_dot(matof(c, q)->cijkl_glb, e, st);
+}
+
+/* get_dv_RKs (elastic_dynamic_solver.cpp:841-1329; RK2-4 :1331, :1817,
+ * :2303): stiffness damping (F - F_store) in RK1 (which then stores F),
+ * 2(F - F_store) in RK2 and RK3, (F - F_store) in RK4; mass damping on
+ * the base velocity in every stage. */
+void gfo_elastic_get(gfo_ctx* c, int stage) {
+    if (c->flags & GF_FLAG_ELASTO_BACKACTION) elastic_eigenstrain(c);
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    double** dv = c->kv[stage - 1];
+    double** du = c->ku[stage - 1];
+    double ts = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k < nz; k++) {
+                int64_t id = ID(c, i, j, k);
+                const gf_material* mat = matof(c, id);
+                if (!mat) { dv[0][id] = 0.; dv[1][id] = 0.; dv[2][id] = 0.; continue; }
+                double s1x[6] = {0}, s2x[6] = {0}, s1y[6] = {0}, s2y[6] = {0}, s1z[6] = {0}, s2z[6] = {0}, t[6];
+                if (!sfx(c, i, j, k)) {
+                    stress_yz(c, i, i, j, k, s1x);
+                    stress_yz(c, i, i == 0 ? nx - 1 : i - 1, j, k, t);
+                    for (int a = 0; a < 6; a++) s1x[a] = (s1x[a] + t[a]) / 2.;
+                }
+                if (!sfx(c, i + 1, j, k)) {
+                    stress_yz(c, i + 1, i, j, k, s2x);
+                    stress_yz(c, i + 1, i == nx - 1 ? 0 : i + 1, j, k, t);
+                    for (int a = 0; a < 6; a++) s2x[a] = (s2x[a] + t[a]) / 2.;
+                }
+                if (!sfy(c, i, j, k)) {
+                    stress_xz(c, i, j, j, k, s1y);
+                    stress_xz(c, i, j, j == 0 ? ny - 1 : j - 1, k, t);
+                    for (int a = 0; a < 6; a++) s1y[a] = (s1y[a] + t[a]) / 2.;
+                }
+                if (!sfy(c, i, j + 1, k)) {
+                    stress_xz(c, i, j + 1, j, k, s2y);
+                    stress_xz(c, i, j + 1, j == ny - 1 ? 0 : j + 1, k, t);
+                    for (int a = 0; a < 6; a++) s2y[a] = (s2y[a] + t[a]) / 2.;
+                }
+                if (!sfz(c, i, j, k)) {
+                    stress_xy(c, i, j, k, k, s1z);
+                    stress_xy(c, i, j, k, k == 0 ? nz - 1 : k - 1, t);
+                    for (int a = 0; a < 6; a++) s1z[a] = (s1z[a] + t[a]) / 2.;
+                }
+                if (!sfz(c, i, j, k + 1)) {
+                    stress_xy(c, i, j, k + 1, k, s2z);
+                    /* the reference tests k == ny-1 for the periodic wrap (:1247) */
+                    stress_xy(c, i, j, k + 1, k == ny - 1 ? 0 : k + 1, t);
+                    for (int a = 0; a < 6; a++) s2z[a] = (s2z[a] + t[a]) / 2.;
+                } else if ((c->flags & GF_FLAG_GAUSSIAN_PULSE) && c->source_type == 3 && k == c->source_z) {
+                    double tv = ts - 5.0 * c->sigma_gauss;
+                    double amp = c->amplitude_gauss * exp(-1. * pow(tv, 2.) / (2. * pow(c->sigma_gauss, 2.)));
+                    for (int a = 0; a < 6; a++) s2z[a] = 0.;
+                    if (c->source_comp == 'x') s2z[4] = amp;
+                    else if (c->source_comp == 'y') s2z[3] = amp;
+                    else if (c->source_comp == 'z') s2z[2] = amp;
+                }
+                double density = c->dt / mat->density;
+                double mass_damping = -c->dt * mat->elast_mass_damping;
+                double stiff_damping = mat->elast_stiff_damping / mat->density;
+                double f[3];
+                f[0] = (s2x[0] - s1x[0]) / c->dx + (s2y[5] - s1y[5]) / c->dy + (s2z[4] - s1z[4]) / c->dz;
+                f[1] = (s2x[5] - s1x[5]) / c->dx + (s2y[1] - s1y[1]) / c->dy + (s2z[3] - s1z[3]) / c->dz;
+                f[2] = (s2x[4] - s1x[4]) / c->dx + (s2y[3] - s1y[3]) / c->dy + (s2z[2] - s1z[2]) / c->dz;
+                for (int a = 0; a < 3; a++) {
+                    c->force[a][id] = f[a];
+                    if (stage == 2 || stage == 3)
+                        dv[a][id] = f[a] * density + mass_damping * c->v[a][id] + stiff_damping * (f[a] - c->force_store[a][id]) * 2.;
+                    else
+                        dv[a][id] = f[a] * density + mass_damping * c->v[a][id] + stiff_damping * (f[a] - c->force_store[a][id]);
+                    if (stage == 1) c->force_store[a][id] = f[a];
+                }
+            }
+    /* get_du_RKs (:2801-2847) */
+    for (int a = 0; a < 3; a++)
+        for (int64_t id = 0; id < c->n; id++) du[a][id] = c->vs[a][id] * c->dt;
+}
+
+/* transfer_pointer (elastodynamics_AbsorbingBoundary.cpp:3-89) */
+static void elastic_shift_history(gfo_ctx* c) {
+    for (int a = 0; a < 3; a++) {
+        if (!(c->flags & GF_FLAG_1D_ABC)) {
+            memcpy(c->t4[a], c->t3[a], sizeof(double) * (size_t)c->n);
+            memcpy(c->t3[a], c->t2[a], sizeof(double) * (size_t)c->n);
+            memcpy(c->t2[a], c->t1[a], sizeof(double) * (size_t)c->n);
+        }
+        memcpy(c->t1[a], c->u[a], sizeof(double) * (size_t)c->n);
+    }
+}
+
+/* update_Du_Boundary{_half,_full,} (elastodynamics_AbsorbingBoundary.cpp:92-230):
+ * kind 0 = half (stages 1, 2), 1 = full (stage 3), 2 = after the step. */
+static void elastic_boundary(gfo_ctx* c, int kind) {
+    int64_t nz = c->nz;
+    double dz = c->dz, dt = c->dt;
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++) {
+            if (!(c->flags & GF_FLAG_1D_ABC)) {
+                double w = c->w_liao;
+                for (int a = 0; a < 3; a++)
+                    c->u[a][ID(c, i, j, 0)] = (4. * w + 2. * (1. - w)) * c->t1[a][ID(c, i, j, 1)]
+                        - (6. * w + (1. - w)) * c->t2[a][ID(c, i, j, 2)]
+                        + (4. * w) * c->t3[a][ID(c, i, j, 3)]
+                        - w * c->t4[a][ID(c, i, j, 4)];
+                continue;
+            }
+            double num = kind == 0 ? 2. * dz : dz;
+            if (!(c->flags & GF_FLAG_1D_ABC_ELAST_ONLYTOP)) {
+                double cl = (num - c->vs_long_1st * dt) / (num + c->vs_long_1st * dt);
+                double ct = (num - c->vs_tran_1st * dt) / (num + c->vs_tran_1st * dt);
+                for (int a = 0; a < 3; a++) {
+                    double cc = a == 2 ? cl : ct;
+                    if (kind < 2)
+                        c->us[a][ID(c, i, j, 0)] = c->u[a][ID(c, i, j, 1)] + cc * (c->u[a][ID(c, i, j, 0)] - c->us[a][ID(c, i, j, 1)]);
+                    else
+                        c->u[a][ID(c, i, j, 0)] = c->t1[a][ID(c, i, j, 1)] + cc * (c->t1[a][ID(c, i, j, 0)] - c->u[a][ID(c, i, j, 1)]);
+                }
+            }
+            double cl = (num - c->vs_long_last * dt) / (num + c->vs_long_last * dt);
+            double ct = (num - c->vs_tran_last * dt) / (num + c->vs_tran_last * dt);
+            for (int a = 0; a < 3; a++) {
+                double cc = a == 2 ? cl : ct;
+                if (kind < 2)
+                    c->us[a][ID(c, i, j, nz - 1)] = c->u[a][ID(c, i, j, nz - 2)] + cc * (c->u[a][ID(c, i, j, nz - 1)] - c->us[a][ID(c, i, j, nz - 2)]);
+                else
+                    c->u[a][ID(c, i, j, nz - 1)] = c->t1[a][ID(c, i, j, nz - 2)] + cc * (c->t1[a][ID(c, i, j, nz - 1)] - c->u[a][ID(c, i, j, nz - 2)]);
+            }
+        }
+}
+
+/* update_Du_Gaussian{_half,_full,} (elastic_dynamic_solver.cpp:3060-3284):
+ * displacement sources on the plane k = source_z; kind as above; the
+ * store is written during the stages, the state after the step. */
+static void elastic_source(gfo_ctx* c, int kind) {
+    double tsrc = kind == 0 ? c->time - 0.5 * c->dt : c->time;
+    double sg = c->sigma_gauss, A = c->amplitude_gauss;
+    int a = c->source_comp == 'x' ? 0 : (c->source_comp == 'y' ? 1 : (c->source_comp == 'z' ? 2 : -1));
+    if (a < 0 || c->source_z < 0 || c->source_z >= c->nz) return;
+    double vsp = a == 2 ? c->vs_long_1st : c->vs_tran_1st;
+    double** dst = kind < 2 ? c->us : c->u;
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++) {
+            int64_t id = ID(c, i, j, c->source_z);
+            if (!c->cell[id]) continue;
+            if (c->source_type == 1 && c->time < 12.5 * sg) {
+                dst[a][id] = A * 1.6487210 * sg * vsp * exp(-1. * pow((tsrc - 5.0 * sg), 2.) / (2. * sg * sg));
+            } else if (c->source_type == 2) {
+                if (c->num_cycles == 0 || tsrc <= (double)c->num_cycles / sg)
+                    dst[a][id] = A * vsp / 2. / PI / sg * (cos(2. * PI * sg * tsrc) - 1);
+                else
+                    dst[a][id] = 0.;
+            }
+        }
+}
+
+/* update_v_RKs / update_v and update_u_RKs / update_u
+ * (elastic_dynamic_solver.cpp:2849-3057) */
+void gfo_elastic_update(gfo_ctx* c, int stage) {
+    int64_t n = c->n;
+    for (int a = 0; a < 3; a++) {
+        double *v = c->v[a], *vs = c->vs[a];
+        if (stage == 1 || stage == 2) { const double* k = c->kv[stage - 1][a]; for (int64_t i = 0; i < n; i++) vs[i] = v[i] + k[i] * 0.5; }
+        else if (stage == 3) { const double* k = c->kv[2][a]; for (int64_t i = 0; i < n; i++) vs[i] = v[i] + k[i]; }
+        else {
+            for (int64_t i = 0; i < n; i++) {
+                v[i] = v[i] + c->kv[0][a][i] / 6. + c->kv[1][a][i] / 3. + c->kv[2][a][i] / 3. + c->kv[3][a][i] / 6.;
+                vs[i] = v[i];
+            }
+        }
+    }
+    int gauss = (c->flags & GF_FLAG_GAUSSIAN_PULSE) != 0;
+    if (stage < 4) {
+        for (int a = 0; a < 3; a++) {
+            double *u = c->u[a], *us = c->us[a];
+            const double* k = c->ku[stage - 1][a];
+            if (stage < 3) for (int64_t i = 0; i < n; i++) us[i] = u[i] + k[i] * 0.5;
+            else for (int64_t i = 0; i < n; i++) us[i] = u[i] + k[i];
+        }
+        elastic_boundary(c, stage < 3 ? 0 : 1);
+        if (gauss) elastic_source(c, stage < 3 ? 0 : 1);
+    } else {
+        elastic_shift_history(c);
+        for (int a = 0; a < 3; a++)
+            for (int64_t i = 0; i < n; i++)
+                c->u[a][i] = c->u[a][i] + c->ku[0][a][i] / 6. + c->ku[1][a][i] / 3. + c->ku[2][a][i] / 3. + c->ku[3][a][i] / 6.;
+        elastic_boundary(c, 2);
+        if (gauss) elastic_source(c, 2);
+        for (int a = 0; a < 3; a++) memcpy(c->us[a], c->u[a], sizeof(double) * (size_t)n);
+    }
+    elastic_gradients(c);
+    elastic_strain(c);
 }
 
 /* ------------------------------------------------------ statics */
@@ -995,9 +1469,11 @@ void gfo_step(gfo_ctx* c, int64_t nsteps) {
     for (int64_t st = 0; st < nsteps; st++) {
         gfo_time_advance(c);
         for (int s = 1; s <= 4; s++) {
+            if (c->if_elasto) gfo_elastic_get(c, s);
             if (c->if_FM_all) gfo_mag_get_dm(c, s);
             if (c->if_AFM_all) gfo_afm_get_dm(c, s);
             if (c->if_FE_all) gfo_fe_get(c, s);
+            if (c->if_elasto) gfo_elastic_update(c, s);
             if (c->if_FM_all) gfo_mag_update(c, s);
             if (c->if_AFM_all) gfo_afm_update(c, s);
             if (c->if_FE_all) gfo_fe_update(c, s);

@@ -27,6 +27,8 @@ typedef struct gfo_ctx gfo_ctx;
 #define GFO_FIELD_DM 100
 /* oracle-only field: the FE q store (the ABI exposes only the p store) */
 #define GFO_FIELD_Q_STORE 101
+/* oracle-only field: the elastic velocity increment of stage 1 */
+#define GFO_FIELD_DV1 102
 
 gfo_ctx* gfo_create(const gf_desc* desc);
 void gfo_destroy(gfo_ctx* c);
@@ -41,6 +43,8 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage);
 void gfo_afm_update(gfo_ctx* c, int stage);
 void gfo_fe_get(gfo_ctx* c, int stage);       /* get_dq_RKs + get_dp_RKs      */
 void gfo_fe_update(gfo_ctx* c, int stage);    /* update_q_RKs + update_p_RKs  */
+void gfo_elastic_get(gfo_ctx* c, int stage);  /* get_dv_RKs + get_du_RKs      */
+void gfo_elastic_update(gfo_ctx* c, int stage); /* update_v_RKs + update_u_RKs */
 void gfo_demag(gfo_ctx* c);                   /* get_H_static / _1D           */
 void gfo_step(gfo_ctx* c, int64_t nsteps);    /* main.cpp:243-483             */
 void gfo_average(const gfo_ctx* c, int field_id, double out[3]);

@@ -16,6 +16,7 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 FIELD_DM = 100   # oracle-only: combined RK4 increment of the last step
 FIELD_Q_STORE = 101   # oracle-only: ferroelectric q store
+FIELD_DV1 = 102       # oracle-only: elastic velocity increment of stage 1
 
 _lib = None
 
@@ -44,6 +45,7 @@ def lib() -> ctypes.CDLL:
         "gfo_mag_get_dm": ([P, ctypes.c_int], None), "gfo_mag_update": ([P, ctypes.c_int], None),
         "gfo_afm_get_dm": ([P, ctypes.c_int], None), "gfo_afm_update": ([P, ctypes.c_int], None),
         "gfo_fe_get": ([P, ctypes.c_int], None), "gfo_fe_update": ([P, ctypes.c_int], None),
+        "gfo_elastic_get": ([P, ctypes.c_int], None), "gfo_elastic_update": ([P, ctypes.c_int], None),
         "gfo_demag": ([P], None), "gfo_step": ([P, i64], None),
         "gfo_average": ([P, ctypes.c_int, dp], None), "gfo_time": ([P], ctypes.c_double),
         "gfo_newell_real": ([i64, i64, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp], None),
@@ -117,6 +119,10 @@ class Oracle:
     def fe_stage(self, s):
         self.L.gfo_fe_get(self._h, s)
         self.L.gfo_fe_update(self._h, s)
+
+    def elastic_stage(self, s):
+        self.L.gfo_elastic_get(self._h, s)
+        self.L.gfo_elastic_update(self._h, s)
 
     def demag(self):
         self.L.gfo_demag(self._h)

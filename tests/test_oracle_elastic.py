@@ -1,0 +1,131 @@
+"""Pin the oracle's elastodynamics (elastic_dynamic_solver.cpp:4-227,
+841-3284; elastodynamics_AbsorbingBoundary.cpp:3-230) by closed forms:
+the stress divergence of a plane wave along each axis and component equals
+-(C/rho) (2 - 2 cos kd)/d^2 u (C = c11 longitudinal, c44 transverse), a
+pulse advances by the exact RK4 propagator of that discrete operator, the
+reference's mass damping acts on the base velocity (v_N = v_0 (1 - eta
+dt)^N), and the cell strain is the average of face gradients.
+
+Not pinned here: the Liao boundary (elastodynamics_AbsorbingBoundary.cpp:
+92-230) extrapolates from grid planes as if c dt = dz; at the Courant
+numbers of the workloads it reflects rather than absorbs, so no closed
+form applies and its parity rests on the GPU-vs-oracle tests.
+
+The reference applies the z = 0 absorbing boundary whatever the
+periodicity (update_Du_Boundary* run in every update_u*), so waves that
+are time-integrated are kept away from that plane."""
+import numpy as np
+import pytest
+
+from goferro import (FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_U, FIELD_V, FLAG_ELASTODYNAMICS, Material,
+                     Problem)
+from goferro.workloads import set_elastic
+
+C11, C12, C44, RHO = 2.69e11, 1.08e11, 0.76e11, 5170.0
+
+
+def chain(axis, n, d=2e-9, dt=2e-14, periodic=(True, True, True), c=(C11, C12, C44), **kw):
+    shape = [1, 1, 5]          # the z = 0 Liao plane reads k = 1..4
+    shape[axis] = n
+    mat = set_elastic(Material.make(), c[0], c[1], c[2], RHO, **kw)
+    return Problem(shape=tuple(shape), spacing=(d, d, d), materials=[mat], material_cell=np.ones(shape, np.uint32),
+                   dt=dt, periodic=periodic, flags=FLAG_ELASTODYNAMICS)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("comp", [0, 1, 2])
+@pytest.mark.parametrize("q", [1, 3, 8])
+def test_stress_divergence_of_plane_wave(oracle_lib, axis, comp, q):
+    # z is never periodic here: the reference's periodic-z stencil reads
+    # outside its arrays unless ny == nz (elastic_dynamic_solver.cpp:1247);
+    # along z the free-ended chain's eigenmodes cos(k (z + d/2)) are used
+    n, d, dt, eps = 16, 2e-9, 2e-14, 1e-10
+    prob = chain(axis, n, d, dt, periodic=(True, True, False))
+    x = np.arange(n) * d
+    if axis == 2:
+        k = np.pi * q / (n * d)
+        mode = np.cos(k * (x + d / 2))
+    else:
+        k = 2 * np.pi * q / (n * d)
+        mode = np.cos(k * x)
+    u0 = np.zeros((3,) + prob.shape)
+    line = [1, 1, 1]
+    line[axis] = n
+    u0[comp] = np.broadcast_to((eps * mode).reshape(line), prob.shape)
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_U, u0)
+    o.set(FIELD_V, np.zeros_like(u0))
+    o.time_advance()
+    o.L.gfo_elastic_get(o._h, 1)
+    dv = o.get(oracle_lib.FIELD_DV1)
+    C = C11 if comp == axis else C44
+    lam = (2 - 2 * np.cos(k * d)) / d ** 2
+    want = -dt / RHO * C * lam * u0[comp].reshape(-1)
+    np.testing.assert_allclose(dv[comp].reshape(-1), want, rtol=0, atol=1e-12 * np.abs(want).max())
+    # other components vanish away from the free z walls (a longitudinal wave
+    # along x or y pushes the walls through c12: the Poisson effect)
+    for a in range(3):
+        if a != comp:
+            inner = dv[a] if axis == 2 else dv[a][:, :, 1:-1]
+            assert np.abs(inner).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_pulse_follows_rk4_propagator(oracle_lib):
+    n, d, N = 160, 2e-9, 60
+    vt = np.sqrt(C44 / RHO)
+    dt = 0.4 * d / vt
+    prob = chain(2, n, d, dt, periodic=(True, True, False))
+    z = np.arange(n) * d
+    u0 = np.zeros((3, 1, 1, n))
+    u0[0, 0, 0] = 1e-10 * np.exp(-((z - 100 * d) / (6 * d)) ** 2)
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_U, u0)
+    o.set(FIELD_V, np.zeros_like(u0))
+    o.step(N)
+    got = o.get(FIELD_U)[0, 0, 0]
+    # discrete operator of a free-ended chain (stress-free walls)
+    L = np.zeros((n, n))
+    for i in range(n):
+        if i > 0:
+            L[i, i - 1] += 1
+            L[i, i] -= 1
+        if i < n - 1:
+            L[i, i + 1] += 1
+            L[i, i] -= 1
+    A = np.zeros((2 * n, 2 * n))
+    A[:n, n:] = np.eye(n)
+    A[n:, :n] = C44 / RHO / d ** 2 * L
+    A *= dt
+    R = np.eye(2 * n) + A + A @ A / 2 + A @ A @ A / 6 + A @ A @ A @ A / 24
+    y = np.linalg.matrix_power(R, N) @ np.concatenate([u0[0, 0, 0], np.zeros(n)])
+    assert np.abs(y[:20]).max() < 1e-25          # never reached the Liao plane
+    np.testing.assert_allclose(got, y[:n], rtol=0, atol=1e-12 * 1e-10)
+
+
+def test_strain_of_a_linear_shear(oracle_lib):
+    # cell strain averages face gradients: u_x = g z gives e_xz = g/2 inside
+    n, d, g = 12, 2e-9, 1e-3
+    prob = chain(2, n, d, periodic=(True, True, False))
+    z = (np.arange(n) + 0.5) * d
+    u0 = np.zeros((3, 1, 1, n))
+    u0[0, 0, 0] = g * z
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_U, u0)
+    off = o.get(FIELD_STRAIN_OFF)[1].reshape(-1)        # e_xz
+    np.testing.assert_allclose(off[1:-1], g / 2, rtol=1e-12)
+    np.testing.assert_allclose(off[[0, -1]], g / 4, rtol=1e-12)   # stress-free wall faces
+    assert np.abs(o.get(FIELD_STRAIN_DIAG)).max() == 0.0
+
+
+def test_mass_damping_acts_on_base_velocity(oracle_lib):
+    eta, dt, N = 3e10, 1e-13, 50
+    prob = chain(2, 8, dt=dt, periodic=(True, True, False), mass_damping=eta, c=(0.0, 0.0, 0.0))
+    v0 = np.zeros((3,) + prob.shape)
+    v0[1] = 2.5
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_U, np.zeros_like(v0))
+    o.set(FIELD_V, v0)
+    o.step(N)
+    v = o.get(FIELD_V)[1].reshape(-1)
+    # the same base-state derivative in all four stages makes RK4 forward Euler
+    np.testing.assert_allclose(v, 2.5 * (1 - eta * dt) ** N, rtol=1e-12)
