@@ -177,7 +177,72 @@ DevMaterial make_dev_material(const gf_material& m, double dt) {
         d.a1111 = m.a1111; d.a1112 = m.a1112; d.a1122 = m.a1122; d.a1123 = m.a1123;
         d.G11 = m.G11; d.tv1 = m.tv1; d.tv2 = m.tv2; d.tv3 = m.tv3;
     }
+    // elastic_dynamic_solver.cpp:1306-1309 and the eigenstrain constants (:4-104)
+    std::memcpy(d.cij, m.cijkl_glb, sizeof(d.cij));
+    d.el_rho = dt / m.density;
+    d.el_mass = -dt * m.elast_mass_damping;
+    d.el_stiff = m.elast_stiff_damping / m.density;
+    d.Q11 = m.Q11; d.Q12 = m.Q12; d.Q44 = m.Q44;
+    d.l100 = m.lamda100; d.l111 = m.lamda111;
+    d.l100a = m.lamda100_AFM1; d.l111a = m.lamda111_AFM1;
+    d.l100b = m.lamda100_AFM2; d.l111b = m.lamda111_AFM2;
     return d;
+}
+
+gf_status alloc_n(gf_ctx* c, double** p, int64_t count, const char* what) {
+    gf_status st = hip_check(c, hipMalloc(p, sizeof(double) * static_cast<size_t>(count)), what);
+    if (st) return st;
+    return hip_check(c, hipMemsetAsync(*p, 0, sizeof(double) * static_cast<size_t>(count), c->stream), what);
+}
+
+// Elastic set-up checks: the grids on which the reference's indexing
+// differs from its intent in a way that cannot be reproduced, and the
+// planes the absorbing boundaries read (elastic_dynamic_solver.cpp:1247,
+// elastodynamics_AbsorbingBoundary.cpp:92-230).
+std::string elastic_unsupported(const gf_ctx* c) {
+    if (c->per[2] && c->ny != c->nz)
+        return "elastodynamics with periodic z needs ny == nz (the reference wraps the forward z neighbour at "
+               "k == ny-1)";
+    const bool mur = c->flags & GF_FLAG_1D_ABC;
+    if (!mur && c->nz < 5) return "the Liao boundary needs nz >= 5 (it reads planes k = 1..4)";
+    if (mur && c->nz < 2) return "the Mur boundary needs nz >= 2";
+    if (c->ny < c->nz) {
+        // at k = ny-1 the forward z stress reads cell (i, j, 0) instead of
+        // (i, j, ny); that cell must hold a material
+        const int64_t k = c->ny - 1;
+        for (int64_t i = 0; i < c->nx; i++)
+            for (int64_t j = 0; j < c->ny; j++) {
+                const int64_t col = (i * c->ny + j) * c->nz;
+                const bool joined = (c->host_code[col + k] & kMatMask) && (c->host_code[col + k + 1] & kMatMask);
+                if (joined && !(c->host_code[col] & kMatMask))
+                    return "elastodynamics: the reference reads the material of cell (i, j, 0) at k = ny-1, which "
+                           "is vacuum at i=" + std::to_string(i) + " j=" + std::to_string(j);
+            }
+    }
+    return "";
+}
+
+// Sound speeds of the first material met on the planes k = 0 and k = nz-1
+// (elastic_initialize.cpp:4-83), scanning i-major.
+void elastic_speeds(gf_ctx* c) {
+    auto scan = [&](int64_t k, double& vl, double& vt) {
+        double c11 = 0., c44 = 0., rho = 1.e-8;
+        for (int64_t i = 0; i < c->nx; i++)
+            for (int64_t j = 0; j < c->ny; j++) {
+                const uint32_t t = c->host_code[(i * c->ny + j) * c->nz + k] & kMatMask;
+                if (t) {
+                    const gf_material& m = c->host_mats[t - 1];
+                    c11 = m.c11; c44 = m.c44; rho = m.density;
+                    vl = std::sqrt((c11 / rho));
+                    vt = std::sqrt((c44 / rho));
+                    return;
+                }
+            }
+        vl = std::sqrt((c11 / rho));
+        vt = std::sqrt((c44 / rho));
+    };
+    scan(0, c->vl_first, c->vt_first);
+    scan(c->nz - 1, c->vl_last, c->vt_last);
 }
 
 gf_status alloc3(gf_ctx* c, double* p[3], const char* what) {
@@ -225,6 +290,40 @@ gf_status statics_after_update(gf_ctx* c) {
         return GF_OK;
     }
     if ((c->any_fm || c->any_afm) && ms) return gf_demag(c);
+    return GF_OK;
+}
+
+// Face gradients and strain of the current u store.
+gf_status elastic_strain(gf_ctx* c) {
+    ElStrainArgs a{};
+    a.code = c->d_code;
+    for (int i = 0; i < 3; i++) {
+        a.us[i] = c->us[i]; a.gyz[i] = c->gyz[i]; a.gxz[i] = c->gxz[i]; a.gxy[i] = c->gxy[i];
+    }
+    for (int i = 0; i < 6; i++) { a.eg[i] = c->eg[i]; a.ec[i] = c->ec[i]; }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_ELASTIC_STRAIN, &ev);
+    gf_status st = hip_check(c, launch_el_strain(a, c->stream), "elastic strain");
+    timer_end(c, GF_K_ELASTIC_STRAIN, ev);
+    return st;
+}
+
+// Strain arrays handed to the magnetic and ferroelectric stages: the
+// crystal-frame dynamic strain when the elastic solver runs and acts on
+// m and p, else none.
+bool strain_coupled(const gf_ctx* c) { return c->el_on && (c->flags & GF_FLAG_ELASTO_ON_PM); }
+
+// With the elastic solver on, each stage starts with it: its get phase
+// reads every physics' store of the stage (main.cpp:265-297).
+gf_status check_after_elastic(gf_ctx* c, int stage) {
+    if (!c->el_on) return GF_OK;
+    if (c->el_next_stage != (stage == 4 ? 1 : stage + 1))
+        return set_error(c, GF_ERR_STATE, "call gf_elastic_stage(" + std::to_string(stage) +
+                                              ") before the other stage calls of that stage");
     return GF_OK;
 }
 
@@ -313,6 +412,14 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         if (c->any_fe) build_surface_bits(c, is_fe, kFeSurfShift);
         if (c->any_fm) build_surface_bits(c, is_fm, kFmSurfShift);
         if (c->any_afm) build_surface_bits(c, is_afm, kAfmSurfShift);
+        if (c->flags & GF_FLAG_ELASTODYNAMICS) {
+            const std::string why = elastic_unsupported(c);
+            if (!why.empty()) {
+                g_create_error = why;
+                delete c;
+                return GF_ERR_UNSUPPORTED;
+            }
+        }
     } catch (const std::bad_alloc&) {
         g_create_error = "host allocation failed";
         delete c;
@@ -356,6 +463,40 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         for (double** b : fe_bufs)
             if (!st) st = alloc3(c, b, "hipMalloc ferroelectric state");
     }
+    if (!st && (c->flags & GF_FLAG_ELASTODYNAMICS)) {
+        c->el_on = true;
+        double** el_bufs[] = {c->u, c->v, c->us, c->vs, c->accu, c->accv, c->fstore};
+        for (double** b : el_bufs)
+            if (!st) st = alloc3(c, b, "hipMalloc elastic state");
+        const int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+        for (int a = 0; a < 3 && !st; a++) {
+            st = alloc_n(c, &c->gyz[a], (nx + 1) * ny * nz, "hipMalloc gradients");
+            if (!st) st = alloc_n(c, &c->gxz[a], nx * (ny + 1) * nz, "hipMalloc gradients");
+            if (!st) st = alloc_n(c, &c->gxy[a], nx * ny * (nz + 1), "hipMalloc gradients");
+        }
+        const bool rotate = c->flags & GF_FLAG_ORIENT_ROTATE;
+        for (int a = 0; a < 6 && !st; a++) {
+            st = alloc_n(c, &c->eg[a], c->n, "hipMalloc strain");
+            if (!st) st = alloc_n(c, &c->e0g[a], c->n, "hipMalloc eigenstrain");
+            if (!st && rotate) st = alloc_n(c, &c->ec[a], c->n, "hipMalloc strain");
+            if (!st && rotate) st = alloc_n(c, &c->e0c[a], c->n, "hipMalloc eigenstrain");
+            if (!rotate) { c->ec[a] = c->eg[a]; c->e0c[a] = c->e0g[a]; }
+        }
+        c->hist_slots = (c->flags & GF_FLAG_1D_ABC) ? 1 : 4;
+        c->hist_head = 0;
+        if (!st)
+            st = alloc_n(c, &c->hist, static_cast<int64_t>(c->hist_slots) * kHistPlanes * 3 * nx * ny,
+                         "hipMalloc boundary history");
+        c->src_type = d->input_strain_type;
+        c->src_k = d->input_strain_sourcez;
+        const int ch = d->input_strain_component;
+        c->src_comp = ch == 'x' ? 0 : (ch == 'y' ? 1 : (ch == 'z' ? 2 : -1));
+        c->src_cycles = d->num_strain_cycles;
+        c->src_sigma = d->sigma_gauss;
+        c->src_amp = d->amplitude_gauss;
+        c->w_liao = d->weighting_factor_fourth_Liao;
+        elastic_speeds(c);
+    }
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
         st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
@@ -384,6 +525,16 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->hstat); free3(c->hnu);
     free3(c->p); free3(c->q); free3(c->pacc); free3(c->qacc);
     free3(c->p_a); free3(c->p_b); free3(c->q_a); free3(c->q_b); free3(c->estat);
+    free3(c->u); free3(c->v); free3(c->us); free3(c->vs); free3(c->accu); free3(c->accv); free3(c->fstore);
+    free3(c->gyz); free3(c->gxz); free3(c->gxy);
+    for (int a = 0; a < 6; a++) {
+        if (c->ec[a] && c->ec[a] != c->eg[a]) hipFree(c->ec[a]);
+        if (c->e0c[a] && c->e0c[a] != c->e0g[a]) hipFree(c->e0c[a]);
+        if (c->eg[a]) hipFree(c->eg[a]);
+        if (c->e0g[a]) hipFree(c->e0g[a]);
+        c->ec[a] = c->eg[a] = c->e0c[a] = c->e0g[a] = nullptr;
+    }
+    if (c->hist) hipFree(c->hist);
     if (c->red_partial) hipFree(c->red_partial);
     if (c->d_code) hipFree(c->d_code);
     if (c->d_mats) hipFree(c->d_mats);
@@ -433,6 +584,21 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     case GF_FIELD_ESTAT:
         if (!c->any_fe) break;
         return c->estat;
+    case GF_FIELD_U:
+        if (!c->el_on) break;
+        return c->u;
+    case GF_FIELD_V:
+        if (!c->el_on) break;
+        return c->v;
+    case GF_FIELD_U_STORE:
+        if (!c->el_on) break;
+        return c->us;
+    case GF_FIELD_STRAIN_DIAG:
+        if (!c->el_on || for_write) break;
+        return c->eg;
+    case GF_FIELD_STRAIN_OFF:
+        if (!c->el_on || for_write) break;
+        return c->eg + 3;
     default:
         break;
     }
@@ -477,6 +643,14 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
         c->fe_next_stage = 1;
     }
     if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
+    // elastic_initialize.cpp:386-410: the stores start equal to u and v
+    if (id == GF_FIELD_U || id == GF_FIELD_V) {
+        const bool isu = id == GF_FIELD_U;
+        for (int a = 0; a < 3 && !st; a++)
+            st = hip_check(c, hipMemcpyAsync(isu ? c->us[a] : c->vs[a], isu ? c->u[a] : c->v[a], bytes,
+                                             hipMemcpyDeviceToDevice, c->stream), "store");
+        c->el_next_stage = 1;
+    }
     if (st) return st;
     return hip_check(c, hipStreamSynchronize(c->stream), "upload sync");
 }
@@ -487,6 +661,12 @@ gf_status gf_download(gf_ctx* c, int id, double* host, size_t count) {
     gf_status st;
     double** f = field_for(c, id, false, &st);
     if (!f) return st;
+    // the strain arrays hold the strain of the store the last elastic stage
+    // started from; refresh them from the current store
+    if (id == GF_FIELD_STRAIN_DIAG || id == GF_FIELD_STRAIN_OFF) {
+        st = elastic_strain(c);
+        if (st) return st;
+    }
     for (int a = 0; a < 3; a++) {
         st = hip_check(c, hipMemcpyAsync(host + a * c->n, f[a], sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream),
                        "download");
@@ -527,7 +707,9 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
     if (stage != c->mag_next_stage)
         return set_error(c, GF_ERR_STATE, "FM stages must run in the order 1,2,3,4 (expected " +
                                               std::to_string(c->mag_next_stage) + ")");
+    if (gf_status e = check_after_elastic(c, stage)) return e;
     StageArgs a{};
+    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
     a.code = c->d_code; a.mats = c->d_mats;
     double** s = fm_store(c, c->mag_store_sel);
     const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
@@ -558,7 +740,9 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
     if (!c->any_afm) return set_error(c, GF_ERR_STATE, "no antiferromagnetic material in this context");
     if (stage != c->afm_next_stage)
         return set_error(c, GF_ERR_STATE, "AFM stages must run in the order 1,2,3,4");
+    if (gf_status e = check_after_elastic(c, stage)) return e;
     AfmStageArgs a{};
+    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
     a.code = c->d_code; a.mats = c->d_mats;
     double** s1 = afm1_store(c, c->afm_store_sel);
     double** s2 = afm2_store(c, c->afm_store_sel);
@@ -599,7 +783,12 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
     if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
     if (!c->any_fe) return set_error(c, GF_ERR_STATE, "no ferroelectric material in this context");
     if (stage != c->fe_next_stage) return set_error(c, GF_ERR_STATE, "FE stages must run in the order 1,2,3,4");
+    if (gf_status e = check_after_elastic(c, stage)) return e;
     FeStageArgs a{};
+    for (int i = 0; i < 6; i++) {
+        a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+        a.eigen[i] = strain_coupled(c) ? c->e0c[i] : nullptr;
+    }
     a.code = c->d_code; a.mats = c->d_mats;
     double** ps = fe_pstore(c, c->fe_store_sel);
     double** qs = fe_qstore(c, c->fe_store_sel);
@@ -633,6 +822,139 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
     return GF_OK;
 }
 
+gf_status gf_elastic_stage(gf_ctx* c, int stage) {
+    if (!c) return GF_ERR_ARG;
+    if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
+    if (!c->el_on) return set_error(c, GF_ERR_STATE, "elastodynamics is off in this context");
+    if (stage != c->el_next_stage) return set_error(c, GF_ERR_STATE, "elastic stages must run in the order 1,2,3,4");
+    if ((c->any_fm && c->mag_next_stage != stage) || (c->any_afm && c->afm_next_stage != stage) ||
+        (c->any_fe && c->fe_next_stage != stage))
+        return set_error(c, GF_ERR_STATE, "gf_elastic_stage must come first within its stage");
+    const int rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    const bool mur = c->flags & GF_FLAG_1D_ABC;
+    const bool gauss = c->flags & GF_FLAG_GAUSSIAN_PULSE;
+    const int64_t nxy = c->nx * c->ny;
+    const int64_t slot_size = static_cast<int64_t>(kHistPlanes) * 3 * nxy;
+
+    // strain of the stage store (elastic_dynamic_solver.cpp:107-227)
+    gf_status st = elastic_strain(c);
+    // eigenstrain from the stores of this stage (:4-104)
+    if (!st && (c->flags & GF_FLAG_ELASTO_BACKACTION)) {
+        ElEigenArgs e{};
+        e.code = c->d_code; e.mats = c->d_mats;
+        double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : nullptr;
+        double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : nullptr;
+        double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : nullptr;
+        double** ps = c->any_fe ? fe_pstore(c, c->fe_store_sel) : nullptr;
+        for (int i = 0; i < 3; i++) {
+            e.m[i] = fm ? fm[i] : nullptr;
+            e.m1[i] = s1 ? s1[i] : nullptr;
+            e.m2[i] = s2 ? s2[i] : nullptr;
+            e.p[i] = ps ? ps[i] : nullptr;
+        }
+        for (int i = 0; i < 6; i++) { e.e0g[i] = c->e0g[i]; e.e0c[i] = c->e0c[i]; }
+        e.n = c->n;
+        e.rotate = rotate;
+        std::memcpy(e.Tg2c, c->Tg2c, sizeof(e.Tg2c));
+        std::memcpy(e.Tc2g, c->Tc2g, sizeof(e.Tc2g));
+        hipEvent_t ev = nullptr;
+        timer_begin(c, GF_K_ELASTIC_STRAIN, &ev);
+        st = hip_check(c, launch_el_eigen(e, c->stream), "eigenstrain");
+        timer_end(c, GF_K_ELASTIC_STRAIN, ev);
+    }
+    // stage 4: u enters the boundary history before the final combine
+    // (transfer_pointer, elastodynamics_AbsorbingBoundary.cpp:3-89)
+    if (!st && stage == 4) {
+        c->hist_head = (c->hist_head + 1) % c->hist_slots;
+        ElHistArgs h{};
+        for (int i = 0; i < 3; i++) h.u[i] = c->u[i];
+        h.slot = c->hist + c->hist_head * slot_size;
+        h.nx = c->nx; h.ny = c->ny; h.nz = c->nz;
+        const int32_t lp[kHistPlanes] = {1, 2, 3, 4};
+        const int32_t mp[kHistPlanes] = {0, 1, static_cast<int32_t>(c->nz - 2), static_cast<int32_t>(c->nz - 1)};
+        for (int p = 0; p < kHistPlanes; p++) h.planes[p] = mur ? mp[p] : lp[p];
+        hipEvent_t ev = nullptr;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_el_history(h, c->stream), "boundary history");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    const double t_stage = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
+    // stress divergence and the v, u updates (:841-3057)
+    if (!st) {
+        ElForceArgs f{};
+        f.code = c->d_code; f.mats = c->d_mats;
+        for (int i = 0; i < 3; i++) {
+            f.u[i] = c->u[i]; f.v[i] = c->v[i]; f.us[i] = c->us[i]; f.vs[i] = c->vs[i];
+            f.accu[i] = c->accu[i]; f.accv[i] = c->accv[i]; f.fstore[i] = c->fstore[i];
+            f.gyz[i] = c->gyz[i]; f.gxz[i] = c->gxz[i]; f.gxy[i] = c->gxy[i];
+        }
+        for (int i = 0; i < 6; i++) { f.eg[i] = c->eg[i]; f.e0g[i] = c->e0g[i]; }
+        f.nx = c->nx; f.ny = c->ny; f.nz = c->nz; f.dx = c->dx; f.dy = c->dy; f.dz = c->dz; f.dt = c->dt;
+        for (int i = 0; i < 3; i++) f.per[i] = c->per[i];
+        f.stage = stage;
+        f.trac_k = -1;
+        if (gauss && c->src_type == 3 && c->src_comp >= 0) {
+            // Gaussian traction at the stage time (:1190-1224)
+            const double sg = c->src_sigma;
+            const double tv = t_stage - 5.0 * sg;
+            f.trac_k = c->src_k;
+            f.trac_row = c->src_comp;
+            f.trac_amp = c->src_amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(sg, 2.)));
+        }
+        hipEvent_t ev = nullptr;
+        timer_begin(c, GF_K_ELASTIC_STAGE, &ev);
+        st = hip_check(c, launch_el_force(f, c->stream), "elastic stage");
+        timer_end(c, GF_K_ELASTIC_STAGE, ev);
+    }
+    // absorbing boundary and displacement source
+    // (elastodynamics_AbsorbingBoundary.cpp:92-230, elastic_dynamic_solver.cpp:3060-3284)
+    if (!st) {
+        const int kind = stage < 3 ? 0 : (stage == 3 ? 1 : 2);
+        ElBoundaryArgs b{};
+        b.code = c->d_code;
+        for (int i = 0; i < 3; i++) { b.u[i] = c->u[i]; b.us[i] = c->us[i]; }
+        for (int j = 0; j < 4; j++) {
+            const int slot = ((c->hist_head - j) % c->hist_slots + c->hist_slots) % c->hist_slots;
+            b.t[j] = c->hist + slot * slot_size;
+        }
+        b.nx = c->nx; b.ny = c->ny; b.nz = c->nz;
+        b.kind = kind;
+        b.liao = mur ? 0 : 1;
+        b.mur_bottom = (c->flags & GF_FLAG_1D_ABC_ELAST_ONLYTOP) ? 0 : 1;
+        b.w = c->w_liao;
+        const double num = kind == 0 ? 2. * c->dz : c->dz;
+        const double dt = c->dt;
+        b.cb_l = (num - c->vl_first * dt) / (num + c->vl_first * dt);
+        b.cb_t = (num - c->vt_first * dt) / (num + c->vt_first * dt);
+        b.ct_l = (num - c->vl_last * dt) / (num + c->vl_last * dt);
+        b.ct_t = (num - c->vt_last * dt) / (num + c->vt_last * dt);
+        b.src_k = -1;
+        if (gauss && c->src_comp >= 0 && c->src_k >= 0 && c->src_k < c->nz) {
+            const double tsrc = kind == 0 ? c->time - 0.5 * c->dt : c->time;
+            const double sg = c->src_sigma, A = c->src_amp;
+            const double vsp = c->src_comp == 2 ? c->vl_first : c->vt_first;
+            if (c->src_type == 1 && c->time < 12.5 * sg) {
+                b.src_k = c->src_k;
+                b.src_val = A * 1.6487210 * sg * vsp * std::exp(-1. * std::pow((tsrc - 5.0 * sg), 2.) / (2. * sg * sg));
+            } else if (c->src_type == 2) {
+                b.src_k = c->src_k;
+                if (c->src_cycles == 0 || tsrc <= static_cast<double>(c->src_cycles) / sg)
+                    b.src_val = A * vsp / 2. / kPi / sg * (std::cos(2. * kPi * sg * tsrc) - 1);
+                else
+                    b.src_val = 0.;
+            }
+            b.src_comp = c->src_comp;
+        }
+        hipEvent_t ev = nullptr;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_el_boundary(b, c->stream), "elastic boundary");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    if (st) return st;
+    c->el_next_stage = stage == 4 ? 1 : stage + 1;
+    return GF_OK;
+}
+
 gf_status gf_demag(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
     if (!(c->any_fm || c->any_afm)) return GF_OK;
@@ -651,12 +973,13 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
     if (!c) return GF_ERR_ARG;
     if (nsteps < 0) return set_error(c, GF_ERR_ARG, "nsteps must be >= 0");
     if ((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1) ||
-        (c->any_fe && c->fe_next_stage != 1))
+        (c->any_fe && c->fe_next_stage != 1) || (c->el_on && c->el_next_stage != 1))
         return set_error(c, GF_ERR_STATE, "gf_step called in the middle of a step");
     for (int64_t s = 0; s < nsteps; s++) {
         gf_status st = gf_time_advance(c);
         for (int stage = 1; stage <= 4 && !st; stage++) {
-            if (c->any_fm) st = gf_mag_stage(c, stage);
+            if (c->el_on) st = gf_elastic_stage(c, stage);
+            if (!st && c->any_fm) st = gf_mag_stage(c, stage);
             if (!st && c->any_afm) st = gf_afm_stage(c, stage);
             if (!st && c->any_fe) st = gf_fe_stage(c, stage);
             if (!st) st = statics_after_update(c);

@@ -83,9 +83,26 @@ __global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
         }
         double px = c[0], py = c[1], pz = c[2];
         if (a.rotate) rot(a.Tg2c, c[0], c[1], c[2], px, py, pz);
-        const double fx = M.fe_mass * landau(M, px, py, pz);
-        const double fy = M.fe_mass * landau(M, py, pz, px);
-        const double fz = M.fe_mass * landau(M, pz, px, py);
+        double lx = landau(M, px, py, pz);
+        double ly = landau(M, py, pz, px);
+        double lz = landau(M, pz, px, py);
+        if (a.strain[0]) {
+            // electrostrictive coupling to strain minus eigenstrain
+            // (ferroelectric_solver.cpp:624-653, 776-786), same sum order
+            const double exx = a.strain[0][id], eyy = a.strain[1][id], ezz = a.strain[2][id];
+            const double eyz = a.strain[3][id], exz = a.strain[4][id], exy = a.strain[5][id];
+            const double exx0 = a.eigen[0][id], eyy0 = a.eigen[1][id], ezz0 = a.eigen[2][id];
+            const double eyz0 = a.eigen[3][id], exz0 = a.eigen[4][id], exy0 = a.eigen[5][id];
+            lx = lx + (M.tv1 * (exx0 - exx) + M.tv2 * (eyy0 - eyy + ezz0 - ezz)) * px;
+            lx = lx + M.tv3 * ((exy0 - exy) * py + (exz0 - exz) * pz);
+            ly = ly + (M.tv1 * (eyy0 - eyy) + M.tv2 * (exx0 - exx + ezz0 - ezz)) * py;
+            ly = ly + M.tv3 * ((exy0 - exy) * px + (eyz0 - eyz) * pz);
+            lz = lz + (M.tv1 * (ezz0 - ezz) + M.tv2 * (exx0 - exx + eyy0 - eyy)) * pz;
+            lz = lz + M.tv3 * ((exz0 - exz) * px + (eyz0 - eyz) * py);
+        }
+        const double fx = M.fe_mass * lx;
+        const double fy = M.fe_mass * ly;
+        const double fz = M.fe_mass * lz;
         double gx = fx, gy = fy, gz = fz;
         if (a.rotate) rot(a.Tc2g, fx, fy, fz, gx, gy, gz);
         kq[0] = gx + M.fe_mass * (-1. * M.G11 * lap[0] - a.estat[0][id] - a.eext[0]) + M.fe_damping * q0;

@@ -71,6 +71,98 @@ struct DevMaterial {
     double a1, a11, a12, a111, a112, a123, a1111, a1112, a1122, a1123;
     double G11;
     double tv1, tv2, tv3;
+    // elastodynamics (elastic_dynamic_solver.cpp:1306-1329) and the
+    // eigenstrain sources (:4-104)
+    double cij[6][6];       // cijkl_glb, Voigt xx yy zz yz xz xy
+    double el_rho;          // dt / density
+    double el_mass;         // -dt * elast_mass_damping
+    double el_stiff;        // elast_stiff_damping / density
+    double Q11, Q12, Q44;
+    double l100, l111, l100a, l111a, l100b, l111b;
+};
+
+// Elastodynamics.  Face gradients live on the staggered face grids
+// (gyz: (nx+1) x ny x nz, gxz: nx x (ny+1) x nz, gxy: nx x ny x (nz+1));
+// strain and eigenstrain are six cell arrays in Voigt order.
+struct ElStrainArgs {
+    const uint32_t* code;
+    const double* us[3];
+    double* gyz[3];
+    double* gxz[3];
+    double* gxy[3];
+    double* eg[6];
+    double* ec[6];          // may equal eg (no rotation)
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
+    int32_t per[3];
+    int32_t rotate;
+    double Tg2c[3][3];
+};
+
+struct ElEigenArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* m[3];     // FM store, null when there is no FM
+    const double* m1[3];    // AFM stores, null when there is no AFM
+    const double* m2[3];
+    const double* p[3];     // FE p store, null when there is no FE
+    double* e0g[6];
+    double* e0c[6];         // may equal e0g (no rotation)
+    int64_t n;
+    int32_t rotate;
+    double Tg2c[3][3];
+    double Tc2g[3][3];
+};
+
+struct ElForceArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    double* u[3];
+    double* v[3];
+    double* us[3];
+    double* vs[3];
+    double* accu[3];
+    double* accv[3];
+    double* fstore[3];
+    const double* gyz[3];
+    const double* gxz[3];
+    const double* gxy[3];
+    const double* eg[6];
+    const double* e0g[6];
+    int64_t nx, ny, nz;
+    double dx, dy, dz, dt;
+    int32_t per[3];
+    int32_t stage;
+    int32_t trac_k;         // plane of a traction pulse, -1 when off
+    int32_t trac_row;       // 0, 1, 2 for a traction along x, y, z (stress xz, yz, zz)
+    double trac_amp;
+};
+
+// Displacement history of the absorbing boundary: slot = [plane][comp][nx*ny]
+// with the four planes k = 1..4 (Liao) or k = 0, 1, nz-2, nz-1 (Mur).
+constexpr int kHistPlanes = 4;
+
+struct ElHistArgs {
+    const double* u[3];
+    double* slot;
+    int64_t nx, ny, nz;
+    int32_t planes[kHistPlanes];
+};
+
+struct ElBoundaryArgs {
+    const uint32_t* code;
+    double* u[3];
+    double* us[3];
+    const double* t[4];     // slots of t1..t4 (Mur reads t1 only)
+    int64_t nx, ny, nz;
+    int32_t kind;           // 0 = stages 1-2, 1 = stage 3, 2 = after the step
+    int32_t liao;           // 1 = Liao at k = 0, 0 = Mur
+    int32_t mur_bottom;     // Mur also at k = 0
+    double w;               // Liao weight of the fourth-order term
+    double cb_l, cb_t, ct_l, ct_t;   // Mur coefficients, bottom/top, long/trans
+    int32_t src_k;          // source plane written this call, -1 when none
+    int32_t src_comp;
+    double src_val;
 };
 
 struct FeStageArgs {
@@ -85,6 +177,8 @@ struct FeStageArgs {
     double* pout[3];
     double* qout[3];
     const double* estat[3];
+    const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
+    const double* eigen[6];     // eigenstrain (crystal frame), with strain
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
     double eext[3];
@@ -104,6 +198,7 @@ struct StageArgs {
     double* out[3];             // next store, or m at stage 4
     const double* hstat[3];
     const double* hnu[3];       // may be null
+    const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
     double* dm_out[3];          // k of this stage for inspection, may be null
     int64_t nx, ny, nz;
     double dx, dy, dz;
@@ -126,6 +221,7 @@ struct AfmStageArgs {
     double* out1[3];
     double* out2[3];
     const double* hstat[3];
+    const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
     int64_t nx, ny, nz;
     double dx, dy, dz;
     double hext[3];
@@ -210,6 +306,32 @@ struct gf_ctx {
     double* q_a[3] = {};
     double* q_b[3] = {};
     double* estat[3] = {};
+    // elastodynamics: state, single stage stores (each cell's store is read
+    // only by its own thread within a stage), RK accumulators, the force of
+    // stage 1, face gradients, strain and eigenstrain (crystal-frame arrays
+    // alias the global ones without a rotation), boundary history ring
+    bool el_on = false;
+    double* u[3] = {};
+    double* v[3] = {};
+    double* us[3] = {};
+    double* vs[3] = {};
+    double* accu[3] = {};
+    double* accv[3] = {};
+    double* fstore[3] = {};
+    double* gyz[3] = {};
+    double* gxz[3] = {};
+    double* gxy[3] = {};
+    double* eg[6] = {};
+    double* ec[6] = {};
+    double* e0g[6] = {};
+    double* e0c[6] = {};
+    double* hist = nullptr;
+    int hist_slots = 0;
+    int hist_head = 0;
+    int el_next_stage = 1;
+    double vl_first = 0, vt_first = 0, vl_last = 0, vt_last = 0;   // sound speeds at k = 0 and nz-1
+    int src_type = 0, src_k = -1, src_comp = -1, src_cycles = 0;
+    double src_sigma = 0, src_amp = 0, w_liao = 0;
     // statics
     double* hstat[3] = {};
     double* hnu[3] = {};
@@ -248,6 +370,13 @@ hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s);
 
 // ferroelectric (gf_fe.hip)
 hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s);
+
+// elastodynamics (gf_elastic.hip)
+hipError_t launch_el_strain(const ElStrainArgs& a, hipStream_t s);
+hipError_t launch_el_eigen(const ElEigenArgs& a, hipStream_t s);
+hipError_t launch_el_force(const ElForceArgs& a, hipStream_t s);
+hipError_t launch_el_history(const ElHistArgs& a, hipStream_t s);
+hipError_t launch_el_boundary(const ElBoundaryArgs& a, hipStream_t s);
 
 // demag (gf_fft.hip)
 gf_status demag_setup(gf_ctx* c);

@@ -154,6 +154,22 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
     return r;
 }
 
+// Magnetoelastic field of a crystal-frame strain, in the global frame:
+// f (B1 e_aa m_a + B2 (e_ab m_b + e_ac m_c)) with f = -2 (FM) or -1 (per
+// AFM sublattice), B already divided by mu0 Ms (magnetic_solver.cpp:143-150,
+// magnetic_AFM_solver.cpp:156-162).  Strain is Voigt xx yy zz yz xz xy.
+__device__ __forceinline__ void magnetoelastic(const double* const* e, int64_t id, double f, double B1, double B2,
+                                               double mx, double my, double mz, int rotate, const double T[3][3],
+                                               double& h0, double& h1, double& h2) {
+    const double exx = 0. + e[0][id], eyy = 0. + e[1][id], ezz = 0. + e[2][id];
+    const double eyz = 0. + e[3][id], exz = 0. + e[4][id], exy = 0. + e[5][id];
+    const double c0 = f * (B1 * (exx) * mx + B2 * ((exy) * my + (exz) * mz));
+    const double c1 = f * (B1 * (eyy) * my + B2 * ((exy) * mx + (eyz) * mz));
+    const double c2 = f * (B1 * (ezz) * mz + B2 * ((exz) * mx + (eyz) * my));
+    if (rotate) rot(T, c0, c1, c2, h0, h1, h2);
+    else { h0 = c0; h1 = c1; h2 = c2; }
+}
+
 // RK update of one component, see the file comment.
 __device__ __forceinline__ void rk_update(int stage, int64_t id, int a, double k, const double* const* base,
                                           double* const* acc, double* const* out) {
@@ -202,11 +218,22 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
     double ha0 = hac0, ha1 = hac1, ha2 = hac2;
     if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
 
-    // H_eff (magnetic_solver.cpp:164-198); magnetoelastic term is zero
-    // without a strain input and is left out of the sum.
-    double H0 = a.hstat[0][id] + ha0 + M.exch * st.ex[0] + M.dmi_pos * st.d[2] + a.hext[0];
-    double H1 = a.hstat[1][id] + ha1 + M.exch * st.ex[1] + M.dmi_pos * st.d[3] + a.hext[1];
-    double H2 = a.hstat[2][id] + ha2 + M.exch * st.ex[2] + M.dmi_neg * (st.d[0] + st.d[1]) + a.hext[2];
+    // H_eff (magnetic_solver.cpp:164-198); the magnetoelastic term
+    // (:89-96, 143-150) enters only when a strain is coupled, since it is
+    // zero otherwise.
+    double H0 = a.hstat[0][id] + ha0;
+    double H1 = a.hstat[1][id] + ha1;
+    double H2 = a.hstat[2][id] + ha2;
+    if (a.strain[0]) {
+        double he0, he1, he2;
+        magnetoelastic(a.strain, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
+        H0 = H0 + he0;
+        H1 = H1 + he1;
+        H2 = H2 + he2;
+    }
+    H0 = H0 + M.exch * st.ex[0] + M.dmi_pos * st.d[2] + a.hext[0];
+    H1 = H1 + M.exch * st.ex[1] + M.dmi_pos * st.d[3] + a.hext[1];
+    H2 = H2 + M.exch * st.ex[2] + M.dmi_neg * (st.d[0] + st.d[1]) + a.hext[2];
     if (a.hnu[0]) {
         H0 = H0 + a.hnu[0][id];
         H1 = H1 + a.hnu[1][id];
@@ -252,10 +279,21 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
         const double hac2 = 2. * M.aK1n[sub] * udotm * M.u[2];
         double ha0 = hac0, ha1 = hac1, ha2 = hac2;
         if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
-        // exchange + inter-sublattice + drive (magnetic_AFM_solver.cpp:170-200)
-        const double H0 = a.hstat[0][id] + ha0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * other.p[0][id] + a.hext[0];
-        const double H1 = a.hstat[1][id] + ha1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * other.p[1][id] + a.hext[1];
-        const double H2 = a.hstat[2][id] + ha2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * other.p[2][id] + a.hext[2];
+        // magnetoelastic (magnetic_AFM_solver.cpp:89-96, 156-162), exchange,
+        // inter-sublattice coupling and drive (:170-200)
+        double H0 = a.hstat[0][id] + ha0;
+        double H1 = a.hstat[1][id] + ha1;
+        double H2 = a.hstat[2][id] + ha2;
+        if (a.strain[0]) {
+            double he0, he1, he2;
+            magnetoelastic(a.strain, id, -1., M.aB1n[sub], M.aB2n[sub], mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
+            H0 = H0 + he0;
+            H1 = H1 + he1;
+            H2 = H2 + he2;
+        }
+        H0 = H0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * other.p[0][id] + a.hext[0];
+        H1 = H1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * other.p[1][id] + a.hext[1];
+        H2 = H2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * other.p[2][id] + a.hext[2];
         const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
         const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
         const double k0 = M.aPrecess[sub] * px + M.aAlphaPrecess[sub] * qx;

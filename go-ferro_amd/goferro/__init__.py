@@ -52,12 +52,12 @@ FLAG_1D_ABC_ELAST_ONLYTOP = 1 << 13
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF) = range(15)
 
 KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_conv", "fft_y_inv",
-                  "fft_z_inv", "other", "fe_stage"]
+                  "fft_z_inv", "other", "fe_stage", "elastic_stage", "elastic_strain"]
 
 # every entry point declared in include/goferro.h
 EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
-    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
+    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
     "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times",
 ]
 
@@ -244,6 +244,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_mag_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_afm_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_fe_stage": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_elastic_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_demag": ([P], ctypes.c_int),
         "gf_step": ([P, ctypes.c_int64], ctypes.c_int),
         "gf_average": ([P, ctypes.c_int, dp], ctypes.c_int),
@@ -326,6 +327,9 @@ class Context:
 
     def fe_stage(self, stage: int) -> None:
         self._check(self.lib.gf_fe_stage(self._h, stage), "gf_fe_stage")
+
+    def elastic_stage(self, stage: int) -> None:
+        self._check(self.lib.gf_elastic_stage(self._h, stage), "gf_elastic_stage")
 
     def demag(self) -> None:
         self._check(self.lib.gf_demag(self._h), "gf_demag")

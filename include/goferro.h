@@ -195,6 +195,17 @@ gf_status gf_afm_stage(gf_ctx* ctx, int stage);
  * 1786-2029), with E_stat taken from the uploaded GF_FIELD_ESTAT. */
 gf_status gf_fe_stage(gf_ctx* ctx, int stage);
 
+/* One elastodynamics RK4 stage: get_dv_RKs + get_du_RKs followed by
+ * update_v_RKs + update_u_RKs with the absorbing boundary and the strain
+ * source (elastic_dynamic_solver.cpp:4-227, 841-3284;
+ * elastodynamics_AbsorbingBoundary.cpp:3-230).  Its get phase reads the
+ * stage-s stores of m, m_AFM and p (eigenstrain back-action), so within a
+ * stage it must run before gf_mag_stage / gf_afm_stage / gf_fe_stage, which
+ * then read the strain of the stage-s u store.  Grids where the reference's
+ * forward-z wrap (k == ny-1, elastic_dynamic_solver.cpp:1247) reads past a
+ * column or differs from a periodic wrap are rejected by gf_create. */
+gf_status gf_elastic_stage(gf_ctx* ctx, int stage);
+
 /* magnetic_system::get_H_static (magnetostatic_solver.cpp:114-219) or the
  * 1D model get_H_static_1D (:222-249) per the descriptor flags. */
 gf_status gf_demag(gf_ctx* ctx);
@@ -231,7 +242,9 @@ enum gf_kernel_class {
     GF_K_FFT_Z_INV = 6,
     GF_K_OTHER = 7,
     GF_K_FE_STAGE = 8,
-    GF_K_COUNT = 9
+    GF_K_ELASTIC_STAGE = 9,   /* stress divergence + v, u updates */
+    GF_K_ELASTIC_STRAIN = 10, /* face gradients, strain, eigenstrain */
+    GF_K_COUNT = 11
 };
 gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
 gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]);

@@ -463,6 +463,17 @@ gfo_ctx* gfo_create(const gf_desc* d) {
      * (the k == ny-1 test of elastic_dynamic_solver.cpp:1247) */
     if (c->if_elasto && c->per[2] && c->ny != c->nz) { free(c->mat); free(c->cell); free(c->pre); free(c); return NULL; }
     if (c->if_elasto && !(c->flags & GF_FLAG_1D_ABC) && c->nz < 5) { free(c->mat); free(c->cell); free(c->pre); free(c); return NULL; }
+    if (c->if_elasto && (c->flags & GF_FLAG_1D_ABC) && c->nz < 2) { free(c->mat); free(c->cell); free(c->pre); free(c); return NULL; }
+    /* at k = ny-1 the forward z stress takes the material of cell (i,j,0),
+     * undefined in the reference when that cell is vacuum */
+    if (c->if_elasto && c->ny < c->nz)
+        for (int64_t i = 0; i < c->nx; i++)
+            for (int64_t j = 0; j < c->ny; j++) {
+                int64_t k = c->ny - 1;
+                if (c->cell[ID(c, i, j, k)] && c->cell[ID(c, i, j, k + 1)] && !c->cell[ID(c, i, j, 0)]) {
+                    free(c->mat); free(c->cell); free(c->pre); free(c); return NULL;
+                }
+            }
     c->source_type = d->input_strain_type; c->source_z = d->input_strain_sourcez;
     c->source_comp = d->input_strain_component; c->num_cycles = d->num_strain_cycles;
     c->sigma_gauss = d->sigma_gauss; c->amplitude_gauss = d->amplitude_gauss;

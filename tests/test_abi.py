@@ -76,3 +76,37 @@ def _load_fresh(goferro, path):
         return goferro.load_library(path)
     finally:
         goferro._lib = saved
+
+
+@pytest.mark.parametrize("case", ["periodic_z", "liao_thin", "mur_single", "vacuum_wrap"])
+def test_create_rejects_grids_the_reference_cannot_run(case, oracle_lib):
+    """Elastic grids on which the reference indexes outside its arrays or
+    reads an undefined material (elastic_dynamic_solver.cpp:1247,
+    elastodynamics_AbsorbingBoundary.cpp:92-230) are refused by gf_create
+    before any device call, and by the oracle alike."""
+    import numpy as np
+
+    import goferro
+    from goferro import FLAG_1D_ABC, FLAG_ELASTODYNAMICS, Material, Problem
+    from goferro.workloads import set_elastic
+    if not goferro.LIB_PATH.exists():
+        pytest.skip("libgoferro.so not built (run __graft_entry__.build())")
+    shape, periodic, flags = (4, 6, 8), (True, True, False), FLAG_ELASTODYNAMICS
+    cell = np.ones(shape, np.uint32)
+    if case == "periodic_z":
+        periodic = (True, True, True)
+    elif case == "liao_thin":
+        shape = (4, 6, 4)
+        cell = np.ones(shape, np.uint32)
+    elif case == "mur_single":
+        shape, flags = (4, 4, 1), flags | FLAG_1D_ABC
+        cell = np.ones(shape, np.uint32)
+    else:
+        cell[1, 2, 0] = 0
+    mat = set_elastic(Material.make(), 2.69e11, 1.08e11, 0.76e11, 5170.0)
+    prob = Problem(shape=shape, spacing=(2e-9, 2e-9, 2e-9), materials=[mat], material_cell=cell, dt=1e-14,
+                   periodic=periodic, flags=flags)
+    with pytest.raises(goferro.GoFerroError, match="gf_create failed \\(-2\\)"):
+        goferro.Context(prob)
+    with pytest.raises(RuntimeError):
+        oracle_lib.Oracle(prob)
