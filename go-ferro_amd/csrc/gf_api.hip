@@ -186,6 +186,12 @@ DevMaterial make_dev_material(const gf_material& m, double dt) {
     d.l100 = m.lamda100; d.l111 = m.lamda111;
     d.l100a = m.lamda100_AFM1; d.l111a = m.lamda111_AFM1;
     d.l100b = m.lamda100_AFM2; d.l111b = m.lamda111_AFM2;
+    // EMdynamic_solver.cpp:392-499 and get_dJp_RK* (:4318-4451)
+    d.em_er = m.r_permittivity;
+    d.em_cond = m.conductivity;
+    d.cn[0] = m.comp_n1; d.cn[1] = m.comp_n2; d.cn[2] = m.comp_n3;
+    d.wp[0] = m.omega_plasma_n1; d.wp[1] = m.omega_plasma_n2; d.wp[2] = m.omega_plasma_n3;
+    d.re[0] = 1. / m.tao_e_n1; d.re[1] = 1. / m.tao_e_n2; d.re[2] = 1. / m.tao_e_n3;
     return d;
 }
 
@@ -312,6 +318,187 @@ gf_status elastic_strain(gf_ctx* c) {
     return st;
 }
 
+// Electrodynamics set-up checks that need no device: the source plane and
+// the current box must lie inside the grid.
+std::string em_unsupported(const gf_ctx* c, const gf_desc* d) {
+    if ((d->flags & GF_FLAG_PLANE_EM) && (d->planeEM_source_z < 0 || d->planeEM_source_z >= c->nz))
+        return "planeEM_source_z outside the grid";
+    if (d->flags & GF_FLAG_JF_INPUT) {
+        const int64_t dims[3] = {c->nx, c->ny, c->nz};
+        for (int a = 0; a < 3; a++)
+            if (d->Jfin_lo[a] < 0 || d->Jfin_hi[a] < d->Jfin_lo[a] || d->Jfin_hi[a] >= dims[a])
+                return "the Jf input box lies outside the grid";
+    }
+    return "";
+}
+
+// EMdynamic_initialize.cpp:3-156: Mur coefficients from the permittivity of
+// the first material on the bottom and top planes, the PEC edges
+// (initialize_PEC_edges, :158-366), the source tables, and whether the Jp
+// currents can leave zero (a plasma frequency, a non-finite 1/tau, or an
+// uploaded Jp).
+gf_status em_setup(gf_ctx* c, const gf_desc* d) {
+    const int64_t nx = c->nx, ny = c->ny, nz = c->nz, n = c->n;
+    c->em_on = true;
+    c->em_periodic_all = c->per[0] && c->per[1] && c->per[2];
+    c->nE[0] = nx * (ny + 1) * (nz + 1); c->nE[1] = (nx + 1) * ny * (nz + 1); c->nE[2] = (nx + 1) * (ny + 1) * nz;
+    c->nH[0] = (nx + 1) * ny * nz; c->nH[1] = nx * (ny + 1) * nz; c->nH[2] = nx * ny * (nz + 1);
+    gf_status st = GF_OK;
+    for (int a = 0; a < 3 && !st; a++) {
+        double** eb[] = {&c->Eb[a], &c->Ea[a], &c->Ebb[a], &c->Eacc[a], &c->Et1[a]};
+        for (double** p : eb)
+            if (!st) st = alloc_n(c, p, c->nE[a], "hipMalloc E");
+        double** hb[] = {&c->Hb[a], &c->Ha[a], &c->Hbb[a], &c->Hacc[a]};
+        for (double** p : hb)
+            if (!st) st = alloc_n(c, p, c->nH[a], "hipMalloc H");
+    }
+    if (!st) st = alloc3(c, c->Ecell, "hipMalloc E cell");
+    if (!st) st = alloc3(c, c->Hcell, "hipMalloc H cell");
+    if (!st && (c->flags & GF_FLAG_JF_INPUT)) st = alloc3(c, c->Jf, "hipMalloc Jf");
+    for (int q = 0; q < 9 && !st; q++) {
+        st = alloc_n(c, &c->Jp[q], n, "hipMalloc Jp");
+        if (!st) st = alloc_n(c, &c->Jps[q], n, "hipMalloc Jp");
+        if (!st) st = alloc_n(c, &c->Jacc[q], n, "hipMalloc Jp");
+    }
+    if (!st && (c->flags & GF_FLAG_EM_FROM_M)) {
+        if (c->any_fm) st = alloc3(c, c->em_dm, "hipMalloc dm");
+        if (!st && c->any_afm) st = alloc3(c, c->em_dm1, "hipMalloc dm");
+        if (!st && c->any_afm) st = alloc3(c, c->em_dm2, "hipMalloc dm");
+    }
+    if (st) return st;
+    double er_bot = 1., er_top = 1.;
+    auto first = [&](int64_t k, double& er) {
+        for (int64_t i = 0; i < nx; i++)
+            for (int64_t j = 0; j < ny; j++) {
+                const uint32_t t = c->host_code[(i * ny + j) * nz + k] & kMatMask;
+                if (t) { er = std::sqrt(c->host_mats[t - 1].r_permittivity); return; }
+            }
+    };
+    first(0, er_bot);
+    first(nz - 1, er_top);
+    const double dt = c->dt;
+    c->cf_top_half = (2. * c->dz - kCLight / er_top * dt) / (2. * c->dz + kCLight / er_top * dt);
+    c->cf_bot_half = (2. * c->dz - kCLight / er_bot * dt) / (2. * c->dz + kCLight / er_bot * dt);
+    c->cf_x_half = (2. * c->dx - kCLight * dt) / (2. * c->dx + kCLight * dt);
+    c->cf_y_half = (2. * c->dy - kCLight * dt) / (2. * c->dy + kCLight * dt);
+    c->cf_top = (c->dz - kCLight / er_top * dt) / (c->dz + kCLight / er_top * dt);
+    c->cf_bot = (c->dz - kCLight / er_bot * dt) / (c->dz + kCLight / er_bot * dt);
+    c->cf_x = (c->dx - kCLight * dt) / (c->dx + kCLight * dt);
+    c->cf_y = (c->dy - kCLight * dt) / (c->dy + kCLight * dt);
+    for (int a = 0; a < 3; a++) { c->plane_E[a] = d->planeEM_E[a]; c->plane_f[a] = d->planeEM_E_freq[a]; }
+    c->plane_z = d->planeEM_source_z;
+    c->jf_type = d->Jf_input_type;
+    const int ch = d->Jf_input_component;
+    c->jf_comp = ch == 'x' ? 0 : (ch == 'y' ? 1 : (ch == 'z' ? 2 : -1));
+    c->jf_cycles = d->num_Jf_cycles;
+    for (int a = 0; a < 3; a++) { c->jf_lo[a] = d->Jfin_lo[a]; c->jf_hi[a] = d->Jfin_hi[a]; }
+    c->jf_amp = d->Jf_input_amp; c->jf_freq = d->Jf_input_freq; c->jf_sigma = d->Jf_input_sigma;
+    if ((c->flags & GF_FLAG_JF_INPUT) && (c->jf_type == 1 || c->jf_type == 2)) {
+        // sin / cos of the spatial phase per plane of the box (:3902-3903)
+        const int64_t jnz = static_cast<int64_t>(c->jf_hi[2]) - c->jf_lo[2] + 1;
+        std::vector<double> sp(static_cast<size_t>(jnz));
+        for (int64_t k = 0; k < jnz; k++) {
+            const double sv = kPi / (static_cast<double>(jnz) * c->dz) * (static_cast<double>(k) + 0.5) * c->dz;
+            sp[static_cast<size_t>(k)] = c->jf_type == 1 ? std::sin(sv) : std::cos(sv);
+        }
+        st = hip_check(c, hipMalloc(&c->jf_spatial, sizeof(double) * sp.size()), "hipMalloc Jf table");
+        if (!st)
+            st = hip_check(c, hipMemcpy(c->jf_spatial, sp.data(), sizeof(double) * sp.size(), hipMemcpyHostToDevice),
+                           "copy Jf table");
+        if (st) return st;
+    }
+    for (const auto& m : c->host_mats) if (m.if_PEC) c->em_pec_all = true;
+    if (c->em_pec_all) {
+        auto pec = [&](int64_t i, int64_t j, int64_t k) {
+            const uint32_t t = c->host_code[(i * ny + j) * nz + k] & kMatMask;
+            return t && c->host_mats[t - 1].if_PEC;
+        };
+        std::vector<uint8_t> f[3];
+        for (int a = 0; a < 3; a++) f[a].assign(static_cast<size_t>(c->nE[a]), 0);
+        for (int64_t i = 0; i < nx; i++)
+            for (int64_t j = 0; j <= ny; j++)
+                for (int64_t k = 0; k <= nz; k++) {
+                    const bool wj = j == 0 || j == ny, wk = k == 0 || k == nz;
+                    if ((wj && !c->per[1]) || (wk && !c->per[2])) continue;
+                    const int64_t y1 = wj ? ny - 1 : j - 1, y3 = wj ? 0 : j, z1 = wk ? nz - 1 : k - 1, z2 = wk ? 0 : k;
+                    f[0][static_cast<size_t>((i * (ny + 1) + j) * (nz + 1) + k)] =
+                        pec(i, y1, z1) || pec(i, y1, z2) || pec(i, y3, z1) || pec(i, y3, z2);
+                }
+        for (int64_t i = 0; i <= nx; i++)
+            for (int64_t j = 0; j < ny; j++)
+                for (int64_t k = 0; k <= nz; k++) {
+                    const bool wi = i == 0 || i == nx, wk = k == 0 || k == nz;
+                    if ((wi && !c->per[0]) || (wk && !c->per[2])) continue;
+                    const int64_t x1 = wi ? nx - 1 : i - 1, x3 = wi ? 0 : i, z1 = wk ? nz - 1 : k - 1, z2 = wk ? 0 : k;
+                    f[1][static_cast<size_t>((i * ny + j) * (nz + 1) + k)] =
+                        pec(x1, j, z1) || pec(x1, j, z2) || pec(x3, j, z1) || pec(x3, j, z2);
+                }
+        for (int64_t i = 0; i <= nx; i++)
+            for (int64_t j = 0; j <= ny; j++)
+                for (int64_t k = 0; k < nz; k++) {
+                    const bool wi = i == 0 || i == nx, wj = j == 0 || j == ny;
+                    if ((wi && !c->per[0]) || (wj && !c->per[1])) continue;
+                    const int64_t x1 = wi ? nx - 1 : i - 1, x3 = wi ? 0 : i, y1 = wj ? ny - 1 : j - 1, y2 = wj ? 0 : j;
+                    f[2][static_cast<size_t>((i * (ny + 1) + j) * nz + k)] =
+                        pec(x1, y1, k) || pec(x1, y2, k) || pec(x3, y1, k) || pec(x3, y2, k);
+                }
+        for (int a = 0; a < 3 && !st; a++) {
+            st = hip_check(c, hipMalloc(&c->pec[a], f[a].size()), "hipMalloc PEC");
+            if (!st)
+                st = hip_check(c, hipMemcpy(c->pec[a], f[a].data(), f[a].size(), hipMemcpyHostToDevice), "copy PEC");
+        }
+        if (st) return st;
+    }
+    for (int64_t id = 0; id < n && !c->jp_on; id++) {
+        const uint32_t t = c->host_code[static_cast<size_t>(id)] & kMatMask;
+        if (!t) continue;
+        const gf_material& m = c->host_mats[t - 1];
+        const double wp[3] = {m.omega_plasma_n1, m.omega_plasma_n2, m.omega_plasma_n3};
+        const double tau[3] = {m.tao_e_n1, m.tao_e_n2, m.tao_e_n3};
+        const double cn[3] = {m.comp_n1, m.comp_n2, m.comp_n3};
+        for (int p = 0; p < 3; p++)
+            if (wp[p] != 0. || !std::isfinite(1. / tau[p]) || !std::isfinite(cn[p])) c->jp_on = true;
+    }
+    return GF_OK;
+}
+
+double** e_store(gf_ctx* c, int sel) { return sel == 0 ? c->Eb : (sel == 1 ? c->Ea : c->Ebb); }
+double** h_store(gf_ctx* c, int sel) { return sel == 0 ? c->Hb : (sel == 1 ? c->Ha : c->Hbb); }
+
+// Cell-centred E and/or H of the current stores (update_DE_cell /
+// update_DH_cell, EMdynamic_solver.cpp:3780-3826).
+gf_status em_cells(gf_ctx* c, bool do_e, bool do_h) {
+    EmCellArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** es = e_store(c, c->e_sel);
+    double** hs = h_store(c, c->h_sel);
+    for (int i = 0; i < 3; i++) { a.es[i] = es[i]; a.hs[i] = hs[i]; a.ecell[i] = c->Ecell[i]; a.hcell[i] = c->Hcell[i]; }
+    a.one_d = (c->flags & GF_FLAG_MAG_1D) && (c->flags & GF_FLAG_EM_FROM_M);
+    if (a.one_d) {
+        a.mz = c->any_fm ? fm_store(c, c->mag_store_sel)[2] : nullptr;
+        a.mz1 = c->any_afm ? afm1_store(c, c->afm_store_sel)[2] : nullptr;
+        a.mz2 = c->any_afm ? afm2_store(c, c->afm_store_sel)[2] : nullptr;
+        a.hzstat = c->hstat[2];
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+    a.do_e = do_e; a.do_h = do_h;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_OTHER, &ev);
+    gf_status st = hip_check(c, launch_em_cells(a, c->stream), "EM cells");
+    timer_end(c, GF_K_OTHER, ev);
+    return st;
+}
+
+// With the electrodynamics on, each stage ends with it: its get phase reads
+// dp and dm of the stage, and the other stages read the cell fields of the
+// stage-s stores (main.cpp:265-297).
+gf_status check_before_em(gf_ctx* c, int stage) {
+    if (c->em_on && c->em_next_stage != stage)
+        return set_error(c, GF_ERR_STATE, "the electrodynamics expects stage " + std::to_string(c->em_next_stage) +
+                                              "; within a stage gf_em_stage comes after the other stage calls");
+    return GF_OK;
+}
+
 // Strain arrays handed to the magnetic and ferroelectric stages: the
 // crystal-frame dynamic strain when the elastic solver runs and acts on
 // m and p, else none.
@@ -412,6 +599,14 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         if (c->any_fe) build_surface_bits(c, is_fe, kFeSurfShift);
         if (c->any_fm) build_surface_bits(c, is_fm, kFmSurfShift);
         if (c->any_afm) build_surface_bits(c, is_afm, kAfmSurfShift);
+        if (c->flags & GF_FLAG_EMDYNAMIC) {
+            const std::string why = em_unsupported(c, d);
+            if (!why.empty()) {
+                g_create_error = why;
+                delete c;
+                return GF_ERR_UNSUPPORTED;
+            }
+        }
         if (c->flags & GF_FLAG_ELASTODYNAMICS) {
             const std::string why = elastic_unsupported(c);
             if (!why.empty()) {
@@ -497,6 +692,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         c->w_liao = d->weighting_factor_fourth_Liao;
         elastic_speeds(c);
     }
+    if (!st && (c->flags & GF_FLAG_EMDYNAMIC)) st = em_setup(c, d);
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
         st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
@@ -535,6 +731,18 @@ gf_status gf_destroy(gf_ctx* c) {
         c->ec[a] = c->eg[a] = c->e0c[a] = c->e0g[a] = nullptr;
     }
     if (c->hist) hipFree(c->hist);
+    free3(c->Eb); free3(c->Ea); free3(c->Ebb); free3(c->Eacc); free3(c->Et1);
+    free3(c->Hb); free3(c->Ha); free3(c->Hbb); free3(c->Hacc);
+    free3(c->Ecell); free3(c->Hcell); free3(c->Jf);
+    free3(c->em_dm); free3(c->em_dm1); free3(c->em_dm2);
+    for (int q = 0; q < 9; q++) {
+        if (c->Jp[q]) hipFree(c->Jp[q]);
+        if (c->Jps[q]) hipFree(c->Jps[q]);
+        if (c->Jacc[q]) hipFree(c->Jacc[q]);
+    }
+    for (int a = 0; a < 3; a++)
+        if (c->pec[a]) hipFree(c->pec[a]);
+    if (c->jf_spatial) hipFree(c->jf_spatial);
     if (c->red_partial) hipFree(c->red_partial);
     if (c->d_code) hipFree(c->d_code);
     if (c->d_mats) hipFree(c->d_mats);
@@ -599,6 +807,12 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     case GF_FIELD_STRAIN_OFF:
         if (!c->el_on || for_write) break;
         return c->eg + 3;
+    case GF_FIELD_E_CELL:
+        if (!c->em_on || for_write) break;
+        return c->Ecell;
+    case GF_FIELD_H_CELL:
+        if (!c->em_on || for_write) break;
+        return c->Hcell;
     default:
         break;
     }
@@ -606,8 +820,88 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     return nullptr;
 }
 
+// The Yee fields (three staggered arrays, concatenated x|y|z) and Jp (nine
+// cell arrays): the device arrays and their sizes, or 0 when id is none of
+// them or the electrodynamics is off.
+static int em_parts(gf_ctx* c, int id, bool for_write, double* parts[9], int64_t sizes[9]) {
+    if (!c->em_on) return 0;
+    double** arr = nullptr;
+    const int64_t* sz = nullptr;
+    if (id == GF_FIELD_E_EM) { arr = c->Eb; sz = c->nE; }
+    else if (id == GF_FIELD_H_EM) { arr = c->Hb; sz = c->nH; }
+    else if (id == GF_FIELD_E_EM_STORE) {
+        if (for_write && c->e_sel == 0) c->e_sel = 2;   // a store that differs from E lives in a free buffer
+        arr = e_store(c, c->e_sel); sz = c->nE;
+    } else if (id == GF_FIELD_H_EM_STORE) {
+        if (for_write && c->h_sel == 0) c->h_sel = 2;
+        arr = h_store(c, c->h_sel); sz = c->nH;
+    }
+    if (arr) {
+        for (int a = 0; a < 3; a++) { parts[a] = arr[a]; sizes[a] = sz[a]; }
+        return 3;
+    }
+    if (id == GF_FIELD_JP) {
+        for (int q = 0; q < 9; q++) { parts[q] = c->Jp[q]; sizes[q] = c->n; }
+        return 9;
+    }
+    return 0;
+}
+
+static bool is_em_field(int id) {
+    return id == GF_FIELD_E_EM || id == GF_FIELD_H_EM || id == GF_FIELD_E_EM_STORE || id == GF_FIELD_H_EM_STORE ||
+           id == GF_FIELD_JP;
+}
+
+static gf_status em_transfer(gf_ctx* c, int id, double* host, size_t count, bool up) {
+    double* parts[9];
+    int64_t sizes[9];
+    const int np = em_parts(c, id, up, parts, sizes);
+    if (!np) return set_error(c, GF_ERR_ARG, "field " + std::to_string(id) + " is not available in this context");
+    size_t tot = 0;
+    for (int q = 0; q < np; q++) tot += static_cast<size_t>(sizes[q]);
+    if (count != tot) return set_error(c, GF_ERR_ARG, "count must be " + std::to_string(tot) + " for this field");
+    size_t o = 0;
+    gf_status st = GF_OK;
+    for (int q = 0; q < np && !st; q++) {
+        const size_t b = sizeof(double) * static_cast<size_t>(sizes[q]);
+        st = up ? hip_check(c, hipMemcpyAsync(parts[q], host + o, b, hipMemcpyHostToDevice, c->stream), "upload")
+                : hip_check(c, hipMemcpyAsync(host + o, parts[q], b, hipMemcpyDeviceToHost, c->stream), "download");
+        o += static_cast<size_t>(sizes[q]);
+    }
+    if (!st && up) {
+        // EMdynamic_initialize.cpp:376-404: the stores start equal to the
+        // fields and the cell fields follow them
+        if (id == GF_FIELD_E_EM) { c->e_sel = 0; c->em_next_stage = 1; st = em_cells(c, true, false); }
+        if (id == GF_FIELD_H_EM) { c->h_sel = 0; c->em_next_stage = 1; st = em_cells(c, false, true); }
+        if (id == GF_FIELD_JP) {
+            for (int q = 0; q < 9 && !st; q++)
+                st = hip_check(c, hipMemcpyAsync(c->Jps[q], c->Jp[q], sizeof(double) * c->n, hipMemcpyDeviceToDevice,
+                                                 c->stream), "Jp store");
+            c->jp_on = true;
+            c->jp_uploaded = true;
+            c->em_next_stage = 1;
+        }
+    }
+    if (st) return st;
+    return hip_check(c, hipStreamSynchronize(c->stream), up ? "upload sync" : "download sync");
+}
+
+gf_status gf_field_size(const gf_ctx* c, int id, size_t* count) {
+    if (!c || !count) return GF_ERR_ARG;
+    if (is_em_field(id)) {
+        if (!c->em_on) return GF_ERR_ARG;
+        const int64_t* sz = (id == GF_FIELD_H_EM || id == GF_FIELD_H_EM_STORE) ? c->nH : c->nE;
+        *count = id == GF_FIELD_JP ? static_cast<size_t>(9 * c->n) : static_cast<size_t>(sz[0] + sz[1] + sz[2]);
+        return GF_OK;
+    }
+    if (id < 0 || id >= GF_FIELD_COUNT) return GF_ERR_ARG;
+    *count = static_cast<size_t>(3 * c->n);
+    return GF_OK;
+}
+
 gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
     if (!c || !host) return set_error(c, GF_ERR_ARG, "null argument");
+    if (is_em_field(id)) return em_transfer(c, id, const_cast<double*>(host), count, true);
     if (count != static_cast<size_t>(3 * c->n)) return set_error(c, GF_ERR_ARG, "count must be 3*nx*ny*nz");
     gf_status st;
     double** f = field_for(c, id, true, &st);
@@ -657,6 +951,7 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
 
 gf_status gf_download(gf_ctx* c, int id, double* host, size_t count) {
     if (!c || !host) return set_error(c, GF_ERR_ARG, "null argument");
+    if (is_em_field(id)) return em_transfer(c, id, host, count, false);
     if (count != static_cast<size_t>(3 * c->n)) return set_error(c, GF_ERR_ARG, "count must be 3*nx*ny*nz");
     gf_status st;
     double** f = field_for(c, id, false, &st);
@@ -708,15 +1003,18 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
         return set_error(c, GF_ERR_STATE, "FM stages must run in the order 1,2,3,4 (expected " +
                                               std::to_string(c->mag_next_stage) + ")");
     if (gf_status e = check_after_elastic(c, stage)) return e;
+    if (gf_status e = check_before_em(c, stage)) return e;
     StageArgs a{};
     for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+    const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
     a.code = c->d_code; a.mats = c->d_mats;
     double** s = fm_store(c, c->mag_store_sel);
     const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
     double** o = fm_store(c, out_sel);
     for (int i = 0; i < 3; i++) {
         a.store[i] = s[i]; a.base[i] = c->m[i]; a.acc[i] = c->acc[i]; a.out[i] = o[i];
-        a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = nullptr;
+        a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = c->em_dm[i];
+        a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
     }
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
     stage_drive(c, stage, a.hext);
@@ -741,8 +1039,15 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
     if (stage != c->afm_next_stage)
         return set_error(c, GF_ERR_STATE, "AFM stages must run in the order 1,2,3,4");
     if (gf_status e = check_after_elastic(c, stage)) return e;
+    if (gf_status e = check_before_em(c, stage)) return e;
     AfmStageArgs a{};
     for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+    const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
+    for (int i = 0; i < 3; i++) {
+        a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
+        a.dm_out1[i] = c->em_dm1[i];
+        a.dm_out2[i] = c->em_dm2[i];
+    }
     a.code = c->d_code; a.mats = c->d_mats;
     double** s1 = afm1_store(c, c->afm_store_sel);
     double** s2 = afm2_store(c, c->afm_store_sel);
@@ -784,11 +1089,14 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
     if (!c->any_fe) return set_error(c, GF_ERR_STATE, "no ferroelectric material in this context");
     if (stage != c->fe_next_stage) return set_error(c, GF_ERR_STATE, "FE stages must run in the order 1,2,3,4");
     if (gf_status e = check_after_elastic(c, stage)) return e;
+    if (gf_status e = check_before_em(c, stage)) return e;
     FeStageArgs a{};
     for (int i = 0; i < 6; i++) {
         a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
         a.eigen[i] = strain_coupled(c) ? c->e0c[i] : nullptr;
     }
+    const bool em_p = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_P);
+    for (int i = 0; i < 3; i++) a.ecell[i] = em_p ? c->Ecell[i] : nullptr;
     a.code = c->d_code; a.mats = c->d_mats;
     double** ps = fe_pstore(c, c->fe_store_sel);
     double** qs = fe_qstore(c, c->fe_store_sel);
@@ -817,6 +1125,8 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
     gf_status st = hip_check(c, launch_fe_stage(a, c->n, c->stream), "fe stage");
     timer_end(c, GF_K_FE_STAGE, ev);
     if (st) return st;
+    // dp = q_s dt of this stage feeds Ampere's law (get_dp_RK*, :1786-1832)
+    for (int i = 0; i < 3; i++) c->fe_q_used[i] = qs[i];
     c->fe_store_sel = out_sel;
     c->fe_next_stage = stage == 4 ? 1 : stage + 1;
     return GF_OK;
@@ -955,6 +1265,154 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
     return GF_OK;
 }
 
+gf_status gf_em_stage(gf_ctx* c, int stage) {
+    if (!c) return GF_ERR_ARG;
+    if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
+    if (!c->em_on) return set_error(c, GF_ERR_STATE, "the electrodynamics is off in this context");
+    if (stage != c->em_next_stage) return set_error(c, GF_ERR_STATE, "EM stages must run in the order 1,2,3,4");
+    const int done = stage == 4 ? 1 : stage + 1;
+    if ((c->any_fm && c->mag_next_stage != done) || (c->any_afm && c->afm_next_stage != done) ||
+        (c->any_fe && c->fe_next_stage != done) || (c->el_on && c->el_next_stage != done))
+        return set_error(c, GF_ERR_STATE, "gf_em_stage comes after the other stage calls of its stage");
+    const int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    const double t_stage = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
+    const int out_e = stage == 4 ? 0 : (c->e_sel == 1 ? 2 : 1);
+    const int out_h = stage == 4 ? 0 : (c->h_sel == 1 ? 2 : 1);
+    double** es = e_store(c, c->e_sel);
+    double** eo = e_store(c, out_e);
+    double** hs = h_store(c, c->h_sel);
+    double** ho = h_store(c, out_h);
+    gf_status st = GF_OK;
+    hipEvent_t ev = nullptr;
+
+    // free-current box at the stage time (update_Jf_input*, :3885-4315)
+    if (c->flags & GF_FLAG_JF_INPUT) {
+        const double t = t_stage, f = c->jf_freq, amp = c->jf_amp, sg = c->jf_sigma;
+        EmJfArgs j{};
+        j.ny = ny; j.nz = nz;
+        for (int a = 0; a < 3; a++) { j.lo[a] = c->jf_lo[a]; j.hi[a] = c->jf_hi[a]; }
+        bool write = true;
+        if (c->jf_type < 3) {
+            const double tv = std::sin(2. * kPi * f * t);
+            j.value = amp * tv;
+            j.spatial = c->jf_type == 0 ? nullptr : c->jf_spatial;
+        } else if (c->jf_type == 3) {
+            const double tv = t <= static_cast<double>(c->jf_cycles) / f ? std::sin(2. * kPi * f * t) : 0.;
+            j.value = amp * tv;
+        } else if (c->jf_type == 4) {
+            const double tv = t - 5. * f;
+            j.value = amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(f, 2.)));
+        } else if (c->jf_type == 5) {
+            const double tv = t - 5. * f;
+            j.value = amp * std::sqrt(std::exp(1.)) * tv / f * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(f, 2.)));
+        } else if (c->jf_type == 6) {
+            const double tv = t - 5. * sg;
+            j.value = amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(sg, 2.))) * std::sin(2. * kPi * f * t);
+        } else {
+            write = false;
+        }
+        if (write && c->jf_comp >= 0) {
+            j.jf = c->Jf[c->jf_comp];
+            timer_begin(c, GF_K_OTHER, &ev);
+            st = hip_check(c, launch_em_jf(j, c->stream), "Jf input");
+            timer_end(c, GF_K_OTHER, ev);
+        }
+    }
+    // stage 4: the planes the Mur walls read, before the combine
+    // (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
+    if (!st && stage == 4 && !c->em_periodic_all) {
+        EmMiscArgs h{};
+        for (int i = 0; i < 3; i++) { h.src[i] = c->Eb[i]; h.t1[i] = c->Et1[i]; h.per[i] = c->per[i]; }
+        h.nx = nx; h.ny = ny; h.nz = nz;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_em_history(h, c->stream), "EM history");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    EmArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    for (int i = 0; i < 3; i++) {
+        a.es[i] = es[i]; a.hs[i] = hs[i];
+        a.eb[i] = c->Eb[i]; a.eo[i] = eo[i]; a.eacc[i] = c->Eacc[i];
+        a.hb[i] = c->Hb[i]; a.ho[i] = ho[i]; a.hacc[i] = c->Hacc[i];
+        a.jf[i] = (c->flags & GF_FLAG_JF_INPUT) ? c->Jf[i] : nullptr;
+        a.q[i] = ((c->flags & GF_FLAG_EM_FROM_P) && c->any_fe) ? c->fe_q_used[i] : nullptr;
+        a.dm[i] = c->em_dm[i]; a.dma1[i] = c->em_dm1[i]; a.dma2[i] = c->em_dm2[i];
+    }
+    for (int q = 0; q < 9; q++) a.jps[q] = c->jp_on ? c->Jps[q] : nullptr;
+    a.nx = nx; a.ny = ny; a.nz = nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
+    for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
+    a.stage = stage;
+    // Ampere and Faraday with the RK updates (both read the stage-s stores)
+    if (!st) {
+        timer_begin(c, GF_K_EM_E, &ev);
+        st = hip_check(c, launch_em_e(a, c->stream), "EM E");
+        timer_end(c, GF_K_EM_E, ev);
+    }
+    if (!st) {
+        timer_begin(c, GF_K_EM_H, &ev);
+        st = hip_check(c, launch_em_h(a, c->stream), "EM H");
+        timer_end(c, GF_K_EM_H, ev);
+    }
+    // Jp from the cell E of the stage store (get_dJp_RK*, update_Jp*)
+    if (!st && c->jp_on) {
+        EmJpArgs j{};
+        j.code = c->d_code; j.mats = c->d_mats;
+        for (int i = 0; i < 3; i++) j.ecell[i] = c->Ecell[i];
+        for (int q = 0; q < 9; q++) { j.jp[q] = c->Jp[q]; j.jps[q] = c->Jps[q]; j.jacc[q] = c->Jacc[q]; }
+        j.n = c->n; j.dt = c->dt; j.stage = stage;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_em_jp(j, c->stream), "EM Jp");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    const int kind = stage < 3 ? 0 : (stage == 3 ? 1 : 2);
+    // hard plane source (update_planeEM*, :3829-3883)
+    if (!st && (c->flags & GF_FLAG_PLANE_EM)) {
+        EmMiscArgs p{};
+        const double t = kind == 0 ? c->time - 0.5 * c->dt : c->time;
+        for (int i = 0; i < 3; i++) {
+            p.b[i] = c->Eb[i]; p.s[i] = eo[i];
+            p.plane[i] = c->plane_E[i] * std::sin(2. * kPi * c->plane_f[i] * t);
+        }
+        p.nx = nx; p.ny = ny; p.nz = nz; p.plane_z = c->plane_z; p.kind = kind;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_em_plane(p, c->stream), "EM plane source");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    // Mur or PEC walls, x then y then z (EMdynamic_AbsorbingBoundary.cpp:58-874)
+    if (!c->em_periodic_all) {
+        for (int axis = 0; axis < 3 && !st; axis++) {
+            if (c->per[axis]) continue;
+            EmWallArgs w{};
+            for (int i = 0; i < 3; i++) { w.b[i] = c->Eb[i]; w.s[i] = eo[i]; w.t1[i] = c->Et1[i]; }
+            w.nx = nx; w.ny = ny; w.nz = nz; w.axis = axis; w.kind = kind;
+            w.pec = axis < 2 ? ((c->flags & GF_FLAG_PEC_XY) ? 1 : 0) : ((c->flags & GF_FLAG_PEC_Z) ? 1 : 0);
+            w.onlytop = ((c->flags & GF_FLAG_1D_ABC) && (c->flags & GF_FLAG_1D_ABC_EM_ONLYTOP)) ? 1 : 0;
+            if (axis == 0) w.cf = kind == 0 ? c->cf_x_half : c->cf_x;
+            else if (axis == 1) w.cf = kind == 0 ? c->cf_y_half : c->cf_y;
+            else { w.cf = kind == 0 ? c->cf_top_half : c->cf_top; w.cb = kind == 0 ? c->cf_bot_half : c->cf_bot; }
+            timer_begin(c, GF_K_OTHER, &ev);
+            st = hip_check(c, launch_em_wall(w, c->stream), "EM walls");
+            timer_end(c, GF_K_OTHER, ev);
+        }
+    }
+    // PEC edges (update_DE_PEC, :4526-4556)
+    if (!st && c->em_pec_all) {
+        EmMiscArgs p{};
+        for (int i = 0; i < 3; i++) { p.b[i] = c->Eb[i]; p.s[i] = eo[i]; p.pec[i] = c->pec[i]; }
+        p.nx = nx; p.ny = ny; p.nz = nz;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_em_pec(p, c->stream), "EM PEC");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    if (st) return st;
+    c->e_sel = out_e;
+    c->h_sel = out_h;
+    st = em_cells(c, true, true);
+    if (st) return st;
+    c->em_next_stage = stage == 4 ? 1 : stage + 1;
+    return GF_OK;
+}
+
 gf_status gf_demag(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
     if (!(c->any_fm || c->any_afm)) return GF_OK;
@@ -973,7 +1431,8 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
     if (!c) return GF_ERR_ARG;
     if (nsteps < 0) return set_error(c, GF_ERR_ARG, "nsteps must be >= 0");
     if ((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1) ||
-        (c->any_fe && c->fe_next_stage != 1) || (c->el_on && c->el_next_stage != 1))
+        (c->any_fe && c->fe_next_stage != 1) || (c->el_on && c->el_next_stage != 1) ||
+        (c->em_on && c->em_next_stage != 1))
         return set_error(c, GF_ERR_STATE, "gf_step called in the middle of a step");
     for (int64_t s = 0; s < nsteps; s++) {
         gf_status st = gf_time_advance(c);
@@ -982,6 +1441,7 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
             if (!st && c->any_fm) st = gf_mag_stage(c, stage);
             if (!st && c->any_afm) st = gf_afm_stage(c, stage);
             if (!st && c->any_fe) st = gf_fe_stage(c, stage);
+            if (!st && c->em_on) st = gf_em_stage(c, stage);
             if (!st) st = statics_after_update(c);
         }
         if (st) return st;

@@ -1,0 +1,423 @@
+// gf_em.hip — electrodynamics (Yee FDTD with RK4 in time) for gfx950, FP64.
+//
+// One stage of EMdynamic_system (EMdynamic_solver.cpp,
+// EMdynamic_AbsorbingBoundary.cpp) as a handful of launches:
+//   jf        free-current box at the stage time (update_Jf_input*, :3885-4315)
+//   history   stage 4 only: E on the boundary planes before the combine
+//             (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
+//   e         Ampere per edge with the four-cell means of permittivity,
+//             conductivity, Jf, Jp and dp (get_dE_RK*, :337-2237), fused
+//             with the RK update (update_DE_RK*, :3588-3777)
+//   h         Faraday per face with dM of the two cells (get_dH_RK*,
+//             :2239-3475), fused with update_DH_RK* (:3477-3586)
+//   jp        Drude/Lorentz currents (get_dJp_RK*, update_Jp*, :4318-4524)
+//   plane     hard plane source (update_planeEM*, :3829-3883)
+//   wall x3   first-order Mur or PEC walls, x then y then z (:58-874)
+//   pec       PEC edges (update_DE_PEC, :4526-4556)
+//   cells     cell-centred E and H of the new stores (:3780-3826)
+// E and H keep two stage stores each: the e and h launches read the
+// stage-s stores of both fields and write stage s+1 into the other buffer.
+// RK4 keeps a state accumulator, acc = y + k1/6, acc += k2/3, acc += k3/3,
+// y = acc + k4/6, the reference's left-to-right sum.
+#include <algorithm>
+
+#include "gf_internal.h"
+
+namespace gf {
+
+namespace {
+
+__device__ __forceinline__ void rk_state(int stage, int64_t i, double k, double* base, double* acc, double* out) {
+    const double y = base[i];
+    if (stage == 1) {
+        out[i] = y + k * 0.5;
+        acc[i] = y + k / 6.;
+    } else if (stage == 2) {
+        out[i] = y + k * 0.5;
+        acc[i] = acc[i] + k / 3.;
+    } else if (stage == 3) {
+        out[i] = y + k;
+        acc[i] = acc[i] + k / 3.;
+    } else {
+        base[i] = acc[i] + k / 6.;
+    }
+}
+
+// Four-cell means around an edge and the Ampere increment
+// (EMdynamic_solver.cpp:392-499); the spin-pumping current is zero.
+__device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, const int64_t cell[4], double curl,
+                                                 double e_store) {
+    double Jf = 0., Jp = 0., dP = 0., er = 0., cond = 0., pc = 0.;
+    for (int q = 0; q < 4; q++) {
+        const int64_t id = cell[q];
+        const uint32_t t = a.code[id] & kMatMask;
+        if (!t) {
+            er = er + 1.;
+        } else {
+            const DevMaterial& M = a.mats[t - 1];
+            if (a.q[0]) dP = dP + a.q[comp][id] * a.dt;
+            if (M.is_fe) pc = pc + 1.;
+            er = er + M.em_er;
+            cond = cond + M.em_cond;
+        }
+        if (a.jf[0]) Jf = Jf + a.jf[comp][id];
+        if (a.jps[0]) Jp = Jp + a.jps[comp][id] + a.jps[3 + comp][id] + a.jps[6 + comp][id];
+    }
+    if (pc < 0.5) pc = 1.;
+    Jf = Jf / 4.;
+    Jp = Jp / 4.;
+    dP = dP / pc;
+    er = er / 4.;
+    cond = cond / 4.;
+    return a.dt / kEps0 / er * (curl - Jf - Jp - cond * e_store - dP / a.dt);
+}
+
+__global__ void __launch_bounds__(256) em_e_kernel(EmArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1), nEz = (nx + 1) * (ny + 1) * nz;
+    int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nEx + nEy + nEz) return;
+    auto C = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
+    auto HX = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
+    auto HY = [&](int64_t i, int64_t j, int64_t k) { return (i * (ny + 1) + j) * nz + k; };
+    auto HZ = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * (nz + 1) + k; };
+    int comp;
+    double k_ = 0.;
+    if (e < nEx) {
+        comp = 0;
+        const int64_t i = e / ((ny + 1) * (nz + 1)), j = (e / (nz + 1)) % (ny + 1), k = e % (nz + 1);
+        const bool wj = j == 0 || j == ny, wk = k == 0 || k == nz;
+        if (!(wj && !a.per[1]) && !(wk && !a.per[2])) {
+            const int64_t y1 = wj ? ny - 1 : j - 1, y3 = wj ? 0 : j, z1 = wk ? nz - 1 : k - 1, z2 = wk ? 0 : k;
+            const double dHzdy = (a.hs[2][HZ(i, y3, z2)] - a.hs[2][HZ(i, y1, z2)]) / a.dy;
+            const double dHydz = (a.hs[1][HY(i, y3, z2)] - a.hs[1][HY(i, y3, z1)]) / a.dz;
+            const int64_t cell[4] = {C(i, y1, z1), C(i, y1, z2), C(i, y3, z1), C(i, y3, z2)};
+            k_ = edge_increment(a, 0, cell, dHzdy - dHydz, a.es[0][e]);
+        }
+    } else if (e < nEx + nEy) {
+        comp = 1;
+        e -= nEx;
+        const int64_t i = e / (ny * (nz + 1)), j = (e / (nz + 1)) % ny, k = e % (nz + 1);
+        const bool wi = i == 0 || i == nx, wk = k == 0 || k == nz;
+        if (!(wi && !a.per[0]) && !(wk && !a.per[2])) {
+            const int64_t x1 = wi ? nx - 1 : i - 1, x3 = wi ? 0 : i, z1 = wk ? nz - 1 : k - 1, z2 = wk ? 0 : k;
+            const double dHxdz = (a.hs[0][HX(x3, j, z2)] - a.hs[0][HX(x3, j, z1)]) / a.dz;
+            const double dHzdx = (a.hs[2][HZ(x3, j, z2)] - a.hs[2][HZ(x1, j, z2)]) / a.dx;
+            const int64_t cell[4] = {C(x1, j, z1), C(x1, j, z2), C(x3, j, z1), C(x3, j, z2)};
+            k_ = edge_increment(a, 1, cell, dHxdz - dHzdx, a.es[1][e]);
+        }
+    } else {
+        comp = 2;
+        e -= nEx + nEy;
+        const int64_t i = e / ((ny + 1) * nz), j = (e / nz) % (ny + 1), k = e % nz;
+        const bool wi = i == 0 || i == nx, wj = j == 0 || j == ny;
+        if (!(wi && !a.per[0]) && !(wj && !a.per[1])) {
+            const int64_t x1 = wi ? nx - 1 : i - 1, x3 = wi ? 0 : i, y1 = wj ? ny - 1 : j - 1, y2 = wj ? 0 : j;
+            const double dHydx = (a.hs[1][HY(x3, y2, k)] - a.hs[1][HY(x1, y2, k)]) / a.dx;
+            const double dHxdy = (a.hs[0][HX(x3, y2, k)] - a.hs[0][HX(x3, y1, k)]) / a.dy;
+            const int64_t cell[4] = {C(x1, y1, k), C(x1, y2, k), C(x3, y1, k), C(x3, y2, k)};
+            k_ = edge_increment(a, 2, cell, dHydx - dHxdy, a.es[2][e]);
+        }
+    }
+    rk_state(a.stage, e, k_, a.eb[comp], a.eacc[comp], a.eo[comp]);
+}
+
+// dM of one cell next to a face (EMdynamic_solver.cpp:2290-2340)
+__device__ __forceinline__ double cell_dm(const EmArgs& a, int comp, int64_t id, double& count) {
+    const uint32_t t = a.code[id] & kMatMask;
+    if (!t) return 0.;
+    const DevMaterial& M = a.mats[t - 1];
+    double d = 0.;
+    if (a.dm[0] || a.dma1[0]) {
+        const double f = a.dm[0] ? a.dm[comp][id] : 0.;
+        const double f1 = a.dma1[0] ? a.dma1[comp][id] : 0.;
+        const double f2 = a.dma2[0] ? a.dma2[comp][id] : 0.;
+        d = f * M.Ms + f1 * M.aMs[0] + f2 * M.aMs[1];
+    }
+    if (M.is_fm || M.is_afm) count = count + 1.;
+    return d;
+}
+
+__global__ void __launch_bounds__(256) em_h_kernel(EmArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t nHx = (nx + 1) * ny * nz, nHy = nx * (ny + 1) * nz, nHz = nx * ny * (nz + 1);
+    int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (f >= nHx + nHy + nHz) return;
+    auto C = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
+    auto EX = [&](int64_t i, int64_t j, int64_t k) { return (i * (ny + 1) + j) * (nz + 1) + k; };
+    auto EY = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * (nz + 1) + k; };
+    auto EZ = [&](int64_t i, int64_t j, int64_t k) { return (i * (ny + 1) + j) * nz + k; };
+    const double pre = -a.dt / kMu0;
+    int comp;
+    double k_, cnt = 0., d1, d2;
+    if (f < nHx) {
+        comp = 0;
+        const int64_t i = f / (ny * nz), j = (f / nz) % ny, k = f % nz;
+        const int64_t x1 = i == 0 ? (a.per[0] ? nx - 1 : 0) : i - 1, x2 = i == nx ? (a.per[0] ? 0 : nx - 1) : i;
+        const double dEzdy = (a.es[2][EZ(i, j + 1, k)] - a.es[2][EZ(i, j, k)]) / a.dy;
+        const double dEydz = (a.es[1][EY(i, j, k + 1)] - a.es[1][EY(i, j, k)]) / a.dz;
+        d1 = cell_dm(a, 0, C(x1, j, k), cnt);
+        d2 = cell_dm(a, 0, C(x2, j, k), cnt);
+        if (cnt < 0.5) cnt = 1.;
+        k_ = pre * (dEzdy - dEydz) - (d1 + d2) / cnt;
+    } else if (f < nHx + nHy) {
+        comp = 1;
+        f -= nHx;
+        const int64_t i = f / ((ny + 1) * nz), j = (f / nz) % (ny + 1), k = f % nz;
+        const int64_t y1 = j == 0 ? (a.per[1] ? ny - 1 : 0) : j - 1, y2 = j == ny ? (a.per[1] ? 0 : ny - 1) : j;
+        const double dExdz = (a.es[0][EX(i, j, k + 1)] - a.es[0][EX(i, j, k)]) / a.dz;
+        const double dEzdx = (a.es[2][EZ(i + 1, j, k)] - a.es[2][EZ(i, j, k)]) / a.dx;
+        d1 = cell_dm(a, 1, C(i, y1, k), cnt);
+        d2 = cell_dm(a, 1, C(i, y2, k), cnt);
+        if (cnt < 0.5) cnt = 1.;
+        k_ = pre * (dExdz - dEzdx) - (d1 + d2) / cnt;
+    } else {
+        comp = 2;
+        f -= nHx + nHy;
+        const int64_t i = f / (ny * (nz + 1)), j = (f / (nz + 1)) % ny, k = f % (nz + 1);
+        const int64_t z1 = k == 0 ? (a.per[2] ? nz - 1 : 0) : k - 1, z2 = k == nz ? (a.per[2] ? 0 : nz - 1) : k;
+        const double dEydx = (a.es[1][EY(i + 1, j, k)] - a.es[1][EY(i, j, k)]) / a.dx;
+        const double dExdy = (a.es[0][EX(i, j + 1, k)] - a.es[0][EX(i, j, k)]) / a.dy;
+        d1 = cell_dm(a, 2, C(i, j, z1), cnt);
+        d2 = cell_dm(a, 2, C(i, j, z2), cnt);
+        if (cnt < 0.5) cnt = 1.;
+        k_ = pre * (dEydx - dExdy) - (d1 + d2) / cnt;
+    }
+    rk_state(a.stage, f, k_, a.hb[comp], a.hacc[comp], a.ho[comp]);
+}
+
+__global__ void __launch_bounds__(256) em_jp_kernel(EmJpArgs a) {
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= a.n) return;
+    const uint32_t t = a.code[id] & kMatMask;
+    for (int p = 0; p < 3; p++)
+        for (int c = 0; c < 3; c++) {
+            double k_ = 0.;
+            if (t) {
+                const DevMaterial& M = a.mats[t - 1];
+                k_ = (kEps0 * M.wp[p] * M.wp[p] * a.ecell[c][id] * M.cn[p] - M.re[p] * a.jps[3 * p + c][id]) * a.dt;
+            }
+            // the store is updated in place: only this thread reads it now
+            rk_state(a.stage, id, k_, a.jp[3 * p + c], a.jacc[3 * p + c], a.jps[3 * p + c]);
+            if (a.stage == 4) a.jps[3 * p + c][id] = a.jp[3 * p + c][id];
+        }
+}
+
+__global__ void __launch_bounds__(256) em_cells_kernel(EmCellArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= nx * ny * nz) return;
+    const int64_t i = id / (ny * nz), j = (id / nz) % ny, k = id % nz;
+    if (a.do_e) {
+        const double* ex = a.es[0];
+        const double* ey = a.es[1];
+        const double* ez = a.es[2];
+        const int64_t x00 = (i * (ny + 1) + j) * (nz + 1) + k, x10 = (i * (ny + 1) + j + 1) * (nz + 1) + k;
+        a.ecell[0][id] = (ex[x00] + ex[x10] + ex[x00 + 1] + ex[x10 + 1]) / 4.;
+        const int64_t y00 = (i * ny + j) * (nz + 1) + k, y10 = ((i + 1) * ny + j) * (nz + 1) + k;
+        a.ecell[1][id] = (ey[y00] + ey[y10] + ey[y00 + 1] + ey[y10 + 1]) / 4.;
+        const int64_t z00 = (i * (ny + 1) + j) * nz + k, z10 = ((i + 1) * (ny + 1) + j) * nz + k;
+        const int64_t z01 = (i * (ny + 1) + j + 1) * nz + k, z11 = ((i + 1) * (ny + 1) + j + 1) * nz + k;
+        a.ecell[2][id] = (ez[z00] + ez[z10] + ez[z01] + ez[z11]) / 4.;
+    }
+    if (a.do_h) {
+        a.hcell[0][id] = (a.hs[0][(i * ny + j) * nz + k] + a.hs[0][((i + 1) * ny + j) * nz + k]) / 2.;
+        a.hcell[1][id] = (a.hs[1][(i * (ny + 1) + j) * nz + k] + a.hs[1][(i * (ny + 1) + j + 1) * nz + k]) / 2.;
+        if (a.one_d) {
+            // 1D magnetostatics through the cell field (EMdynamic_solver.cpp:3793-3804)
+            const uint32_t t = a.code[id] & kMatMask;
+            double h = 0.;
+            if (t) {
+                const DevMaterial& M = a.mats[t - 1];
+                const double mz = a.mz ? a.mz[id] : 0., mz1 = a.mz1 ? a.mz1[id] : 0., mz2 = a.mz2 ? a.mz2[id] : 0.;
+                h = -1. * M.Ms * mz - 1. * M.aMs[0] * mz1 - 1. * M.aMs[1] * mz2 - a.hzstat[id];
+            }
+            a.hcell[2][id] = h;
+        } else {
+            a.hcell[2][id] = (a.hs[2][(i * ny + j) * (nz + 1) + k] + a.hs[2][(i * ny + j) * (nz + 1) + k + 1]) / 2.;
+        }
+    }
+}
+
+// One wall pair of the first-order Mur condition for one line of edges:
+// kind < 2 writes the out store from the base and the new store, kind 2
+// writes the base from the history (EMdynamic_AbsorbingBoundary.cpp:58-874).
+__device__ __forceinline__ void mur(const EmWallArgs& a, int comp, int64_t wall, int64_t inner, double cf) {
+    if (a.kind < 2) a.s[comp][wall] = a.b[comp][inner] + cf * (a.b[comp][wall] - a.s[comp][inner]);
+    else a.b[comp][wall] = a.t1[comp][inner] + cf * (a.t1[comp][wall] - a.b[comp][inner]);
+}
+
+__device__ __forceinline__ void zero(const EmWallArgs& a, int comp, int64_t at) {
+    if (a.kind < 2) a.s[comp][at] = 0.;
+    else a.b[comp][at] = 0.;
+}
+
+__global__ void __launch_bounds__(256) em_wall_kernel(EmWallArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (a.axis == 0) {
+        // Ey lines (j < ny, k <= nz) and Ez lines (j <= ny, k < nz)
+        if (t < ny * (nz + 1)) {
+            const int64_t j = t / (nz + 1), k = t % (nz + 1);
+            auto EY = [&](int64_t i) { return (i * ny + j) * (nz + 1) + k; };
+            if (!a.pec) { mur(a, 1, EY(0), EY(1), a.cf); mur(a, 1, EY(nx), EY(nx - 1), a.cf); }
+            else { zero(a, 1, EY(0)); zero(a, 1, EY(nx)); }
+        }
+        if (t < (ny + 1) * nz) {
+            const int64_t j = t / nz, k = t % nz;
+            auto EZ = [&](int64_t i) { return (i * (ny + 1) + j) * nz + k; };
+            if (!a.pec) { mur(a, 2, EZ(0), EZ(1), a.cf); mur(a, 2, EZ(nx), EZ(nx - 1), a.cf); }
+            else { zero(a, 2, EZ(0)); zero(a, 2, EZ(nx)); }
+        }
+    } else if (a.axis == 1) {
+        // Ex lines (i < nx, k <= nz) and Ez lines (i <= nx, k < nz)
+        if (t < nx * (nz + 1)) {
+            const int64_t i = t / (nz + 1), k = t % (nz + 1);
+            auto EX = [&](int64_t j) { return (i * (ny + 1) + j) * (nz + 1) + k; };
+            if (!a.pec) { mur(a, 0, EX(0), EX(1), a.cf); mur(a, 0, EX(ny), EX(ny - 1), a.cf); }
+            else { zero(a, 0, EX(0)); zero(a, 0, EX(ny)); }
+        }
+        if (t < (nx + 1) * nz) {
+            const int64_t i = t / nz, k = t % nz;
+            auto EZ = [&](int64_t j) { return (i * (ny + 1) + j) * nz + k; };
+            if (!a.pec) { mur(a, 2, EZ(0), EZ(1), a.cf); mur(a, 2, EZ(ny), EZ(ny - 1), a.cf); }
+            else { zero(a, 2, EZ(0)); zero(a, 2, EZ(ny)); }
+        }
+    } else {
+        // Ex lines (i < nx, j <= ny) and Ey lines (i <= nx, j < ny); top first
+        for (int comp = 0; comp < 2; comp++) {
+            const int64_t lines = comp == 0 ? nx * (ny + 1) : (nx + 1) * ny;
+            if (t >= lines) continue;
+            const int64_t col = t * (nz + 1);   // both layouts end in (nz + 1) along z
+            if (!a.pec) {
+                mur(a, comp, col + nz, col + nz - 1, a.cf);
+                if (a.onlytop) zero(a, comp, col);
+                else mur(a, comp, col, col + 1, a.cb);
+            } else {
+                zero(a, comp, col);
+                zero(a, comp, col + nz);
+            }
+        }
+    }
+}
+
+// Stage 4: E on the planes the Mur walls read, before the combine.
+__global__ void __launch_bounds__(256) em_history_kernel(EmMiscArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1), nEz = (nx + 1) * (ny + 1) * nz;
+    int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nEx + nEy + nEz) return;
+    auto near = [](int64_t v, int64_t n) { return v <= 1 || v >= n - 1; };
+    int comp;
+    bool need;
+    if (e < nEx) {
+        comp = 0;
+        const int64_t j = (e / (nz + 1)) % (ny + 1), k = e % (nz + 1);
+        need = (!a.per[1] && near(j, ny)) || (!a.per[2] && near(k, nz));
+    } else if (e < nEx + nEy) {
+        comp = 1;
+        e -= nEx;
+        const int64_t i = e / (ny * (nz + 1)), k = e % (nz + 1);
+        need = (!a.per[0] && near(i, nx)) || (!a.per[2] && near(k, nz));
+    } else {
+        comp = 2;
+        e -= nEx + nEy;
+        const int64_t i = e / ((ny + 1) * nz), j = (e / nz) % (ny + 1);
+        need = (!a.per[0] && near(i, nx)) || (!a.per[1] && near(j, ny));
+    }
+    if (need) a.t1[comp][e] = a.src[comp][e];
+}
+
+__global__ void __launch_bounds__(256) em_plane_kernel(EmMiscArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nx * ny) return;
+    const int64_t i = t / ny, j = t % ny, z = a.plane_z;
+    double* const* d = a.kind < 2 ? a.s : a.b;
+    d[0][(i * (ny + 1) + j) * (nz + 1) + z] = a.plane[0];
+    d[1][(i * ny + j) * (nz + 1) + z] = a.plane[1];
+    d[2][(i * (ny + 1) + j) * nz + z] = a.plane[2];
+}
+
+__global__ void __launch_bounds__(256) em_pec_kernel(EmMiscArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1), nEz = (nx + 1) * (ny + 1) * nz;
+    int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nEx + nEy + nEz) return;
+    int comp = 0;
+    if (e >= nEx + nEy) { comp = 2; e -= nEx + nEy; }
+    else if (e >= nEx) { comp = 1; e -= nEx; }
+    if (a.pec[comp][e]) {
+        a.b[comp][e] = 0.;
+        a.s[comp][e] = 0.;
+    }
+}
+
+__global__ void __launch_bounds__(256) em_jf_kernel(EmJfArgs a) {
+    const int64_t sx = a.hi[0] - a.lo[0] + 1, sy = a.hi[1] - a.lo[1] + 1, sz = a.hi[2] - a.lo[2] + 1;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= sx * sy * sz) return;
+    const int64_t kk = t % sz, jj = (t / sz) % sy, ii = t / (sy * sz);
+    const int64_t id = ((a.lo[0] + ii) * a.ny + (a.lo[1] + jj)) * a.nz + (a.lo[2] + kk);
+    a.jf[id] = a.spatial ? a.value * a.spatial[kk] : a.value;
+}
+
+unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_em_e(const EmArgs& a, hipStream_t s) {
+    const int64_t n = a.nx * (a.ny + 1) * (a.nz + 1) + (a.nx + 1) * a.ny * (a.nz + 1) + (a.nx + 1) * (a.ny + 1) * a.nz;
+    hipLaunchKernelGGL(em_e_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_h(const EmArgs& a, hipStream_t s) {
+    const int64_t n = (a.nx + 1) * a.ny * a.nz + a.nx * (a.ny + 1) * a.nz + a.nx * a.ny * (a.nz + 1);
+    hipLaunchKernelGGL(em_h_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(em_jp_kernel, dim3(blocks_for(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(em_cells_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s) {
+    int64_t n;
+    if (a.axis == 0) n = std::max(a.ny * (a.nz + 1), (a.ny + 1) * a.nz);
+    else if (a.axis == 1) n = std::max(a.nx * (a.nz + 1), (a.nx + 1) * a.nz);
+    else n = std::max(a.nx * (a.ny + 1), (a.nx + 1) * a.ny);
+    hipLaunchKernelGGL(em_wall_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s) {
+    const int64_t n = a.nx * (a.ny + 1) * (a.nz + 1) + (a.nx + 1) * a.ny * (a.nz + 1) + (a.nx + 1) * (a.ny + 1) * a.nz;
+    hipLaunchKernelGGL(em_history_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_plane(const EmMiscArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(em_plane_kernel, dim3(blocks_for(a.nx * a.ny)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_pec(const EmMiscArgs& a, hipStream_t s) {
+    const int64_t n = a.nx * (a.ny + 1) * (a.nz + 1) + (a.nx + 1) * a.ny * (a.nz + 1) + (a.nx + 1) * (a.ny + 1) * a.nz;
+    hipLaunchKernelGGL(em_pec_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_em_jf(const EmJfArgs& a, hipStream_t s) {
+    const int64_t n = static_cast<int64_t>(a.hi[0] - a.lo[0] + 1) * (a.hi[1] - a.lo[1] + 1) * (a.hi[2] - a.lo[2] + 1);
+    hipLaunchKernelGGL(em_jf_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gf

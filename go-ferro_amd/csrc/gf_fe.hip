@@ -108,6 +108,12 @@ __global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
         kq[0] = gx + M.fe_mass * (-1. * M.G11 * lap[0] - a.estat[0][id] - a.eext[0]) + M.fe_damping * q0;
         kq[1] = gy + M.fe_mass * (-1. * M.G11 * lap[1] - a.estat[1][id] - a.eext[1]) + M.fe_damping * q1;
         kq[2] = gz + M.fe_mass * (-1. * M.G11 * lap[2] - a.estat[2][id] - a.eext[2]) + M.fe_damping * q2;
+        if (a.ecell[0]) {
+            // electrodynamic back-action (ferroelectric_solver.cpp:794-798)
+            kq[0] = kq[0] - M.fe_mass * a.ecell[0][id];
+            kq[1] = kq[1] - M.fe_mass * a.ecell[1][id];
+            kq[2] = kq[2] - M.fe_mass * a.ecell[2][id];
+        }
     }
     rk(a.stage, id, q0 * a.dt, a.pbase[0], a.pacc[0], a.pout[0]);
     rk(a.stage, id, q1 * a.dt, a.pbase[1], a.pacc[1], a.pout[1]);

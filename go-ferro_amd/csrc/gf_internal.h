@@ -79,6 +79,102 @@ struct DevMaterial {
     double el_stiff;        // elast_stiff_damping / density
     double Q11, Q12, Q44;
     double l100, l111, l100a, l111a, l100b, l111b;
+    // electrodynamics (EMdynamic_solver.cpp:392-499, 4318-4451)
+    double em_er;           // r_permittivity
+    double em_cond;         // conductivity
+    double cn[3];           // comp_n1..3
+    double wp[3];           // omega_plasma_n1..3
+    double re[3];           // 1 / tao_e_n1..3
+};
+
+constexpr double kEps0 = 8.854187817e-12;     // EMdynamic_system.h:13
+constexpr double kCLight = 2.99792458e8;      // EMdynamic_system.h:14
+
+// Yee grids: E on edges, Ex nx(ny+1)(nz+1), Ey (nx+1)ny(nz+1), Ez
+// (nx+1)(ny+1)nz; H on faces, Hx (nx+1)ny nz, Hy nx(ny+1)nz, Hz nx ny(nz+1);
+// each z fastest in its own dimensions.
+struct EmArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* es[3];        // E store of the stage
+    const double* hs[3];        // H store of the stage
+    double* eb[3];              // E base, E out store, E accumulator
+    double* eo[3];
+    double* eacc[3];
+    double* hb[3];
+    double* ho[3];
+    double* hacc[3];
+    const double* jf[3];        // free current, null when off
+    const double* jps[9];       // Jp stores (pole-major), null when identically zero
+    const double* q[3];         // ferroelectric q store of the stage (dp = q dt), null when uncoupled
+    const double* dm[3];        // magnetic increments of the stage, null when absent
+    const double* dma1[3];
+    const double* dma2[3];
+    int64_t nx, ny, nz;
+    double dx, dy, dz, dt;
+    int32_t per[3];
+    int32_t stage;
+};
+
+struct EmJpArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* ecell[3];
+    double* jp[9];
+    double* jps[9];
+    double* jacc[9];
+    int64_t n;
+    double dt;
+    int32_t stage;
+};
+
+struct EmCellArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* es[3];
+    const double* hs[3];
+    double* ecell[3];
+    double* hcell[3];
+    const double* mz;           // 1D model: FM and AFM mz stores and Hz_stat, else null
+    const double* mz1;
+    const double* mz2;
+    const double* hzstat;
+    int64_t nx, ny, nz;
+    int32_t do_e, do_h, one_d;
+};
+
+struct EmWallArgs {
+    double* b[3];               // E base
+    double* s[3];               // E out store
+    const double* t1[3];        // E base before the final combine
+    int64_t nx, ny, nz;
+    int32_t axis;               // walls normal to x, y or z
+    int32_t kind;               // 0 = stages 1-2, 1 = stage 3, 2 = after the step
+    int32_t pec;                // the walls are PEC
+    int32_t onlytop;            // bottom z wall held at zero
+    double cf;                  // x / y coefficient, or the top one for z
+    double cb;                  // bottom coefficient for z
+};
+
+struct EmMiscArgs {
+    double* b[3];
+    double* s[3];
+    const double* src[3];       // history copy source (E base)
+    double* t1[3];
+    const uint8_t* pec[3];
+    double plane[3];
+    int64_t nx, ny, nz;
+    int32_t per[3];
+    int32_t plane_z;
+    int32_t kind;
+};
+
+struct EmJfArgs {
+    double* jf;                 // the driven component
+    const double* spatial;      // per-plane factor (types 1, 2), else null
+    double value;               // uniform value, or amp * temporal factor
+    int64_t ny, nz;
+    int32_t lo[3], hi[3];
 };
 
 // Elastodynamics.  Face gradients live on the staggered face grids
@@ -179,6 +275,7 @@ struct FeStageArgs {
     const double* estat[3];
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
     const double* eigen[6];     // eigenstrain (crystal frame), with strain
+    const double* ecell[3];     // electrodynamic E at the cells, null when uncoupled
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
     double eext[3];
@@ -199,7 +296,8 @@ struct StageArgs {
     const double* hstat[3];
     const double* hnu[3];       // may be null
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
-    double* dm_out[3];          // k of this stage for inspection, may be null
+    const double* hcell[3];     // electrodynamic H at the cells, null when uncoupled
+    double* dm_out[3];          // k of this stage, null unless Faraday's law needs it
     int64_t nx, ny, nz;
     double dx, dy, dz;
     double hext[3];
@@ -222,6 +320,9 @@ struct AfmStageArgs {
     double* out2[3];
     const double* hstat[3];
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
+    const double* hcell[3];     // electrodynamic H at the cells, null when uncoupled
+    double* dm_out1[3];         // k of this stage per sublattice, null unless needed
+    double* dm_out2[3];
     int64_t nx, ny, nz;
     double dx, dy, dz;
     double hext[3];
@@ -332,6 +433,39 @@ struct gf_ctx {
     double vl_first = 0, vt_first = 0, vl_last = 0, vt_last = 0;   // sound speeds at k = 0 and nz-1
     int src_type = 0, src_k = -1, src_comp = -1, src_cycles = 0;
     double src_sigma = 0, src_amp = 0, w_liao = 0;
+    // electrodynamics: Yee E/H with double-buffered stage stores (0 = base,
+    // 1 = a, 2 = b), accumulators, boundary history, cell fields, sources
+    bool em_on = false, em_periodic_all = false, em_pec_all = false, jp_on = false, jp_uploaded = false;
+    int64_t nE[3] = {}, nH[3] = {};
+    double* Eb[3] = {};
+    double* Ea[3] = {};
+    double* Ebb[3] = {};
+    double* Eacc[3] = {};
+    double* Hb[3] = {};
+    double* Ha[3] = {};
+    double* Hbb[3] = {};
+    double* Hacc[3] = {};
+    double* Et1[3] = {};
+    double* Ecell[3] = {};
+    double* Hcell[3] = {};
+    double* Jf[3] = {};
+    double* Jp[9] = {};
+    double* Jps[9] = {};
+    double* Jacc[9] = {};
+    uint8_t* pec[3] = {};
+    double* jf_spatial = nullptr;
+    double* em_dm[3] = {};          // FM increments of the stage (Faraday's law)
+    double* em_dm1[3] = {};
+    double* em_dm2[3] = {};
+    const double* fe_q_used[3] = {}; // q store read by the last ferroelectric stage
+    int e_sel = 0, h_sel = 0, em_next_stage = 1;
+    double cf_top_half = 0, cf_bot_half = 0, cf_x_half = 0, cf_y_half = 0;
+    double cf_top = 0, cf_bot = 0, cf_x = 0, cf_y = 0;
+    double plane_E[3] = {}, plane_f[3] = {};
+    int plane_z = 0;
+    int jf_type = 0, jf_comp = -1, jf_cycles = 0;
+    int jf_lo[3] = {}, jf_hi[3] = {};
+    double jf_amp = 0, jf_freq = 0, jf_sigma = 0;
     // statics
     double* hstat[3] = {};
     double* hnu[3] = {};
@@ -377,6 +511,17 @@ hipError_t launch_el_eigen(const ElEigenArgs& a, hipStream_t s);
 hipError_t launch_el_force(const ElForceArgs& a, hipStream_t s);
 hipError_t launch_el_history(const ElHistArgs& a, hipStream_t s);
 hipError_t launch_el_boundary(const ElBoundaryArgs& a, hipStream_t s);
+
+// electrodynamics (gf_em.hip)
+hipError_t launch_em_e(const EmArgs& a, hipStream_t s);
+hipError_t launch_em_h(const EmArgs& a, hipStream_t s);
+hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s);
+hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s);
+hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s);
+hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s);
+hipError_t launch_em_plane(const EmMiscArgs& a, hipStream_t s);
+hipError_t launch_em_pec(const EmMiscArgs& a, hipStream_t s);
+hipError_t launch_em_jf(const EmJfArgs& a, hipStream_t s);
 
 // demag (gf_fft.hip)
 gf_status demag_setup(gf_ctx* c);

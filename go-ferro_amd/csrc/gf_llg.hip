@@ -239,6 +239,12 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
         H1 = H1 + a.hnu[1][id];
         H2 = H2 + a.hnu[2][id];
     }
+    if (a.hcell[0]) {
+        // electrodynamic back-action (magnetic_solver.cpp:200-204)
+        H0 = H0 + a.hcell[0][id];
+        H1 = H1 + a.hcell[1][id];
+        H2 = H2 + a.hcell[2][id];
+    }
 
     // LLG right-hand side times dt (magnetic_solver.cpp:206-220)
     const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
@@ -294,6 +300,12 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
         H0 = H0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * other.p[0][id] + a.hext[0];
         H1 = H1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * other.p[1][id] + a.hext[1];
         H2 = H2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * other.p[2][id] + a.hext[2];
+        if (a.hcell[0]) {
+            // electrodynamic back-action (magnetic_AFM_solver.cpp:222-226)
+            H0 = H0 + a.hcell[0][id];
+            H1 = H1 + a.hcell[1][id];
+            H2 = H2 + a.hcell[2][id];
+        }
         const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
         const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
         const double k0 = M.aPrecess[sub] * px + M.aAlphaPrecess[sub] * qx;
@@ -302,6 +314,10 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
         const double* const* base = sub == 0 ? a.base1 : a.base2;
         double* const* acc = sub == 0 ? a.acc1 : a.acc2;
         double* const* out = sub == 0 ? a.out1 : a.out2;
+        double* const* dmo = sub == 0 ? a.dm_out1 : a.dm_out2;
+        if (dmo[0]) {
+            dmo[0][id] = k0; dmo[1][id] = k1; dmo[2][id] = k2;
+        }
         rk_update(a.stage, id, 0, k0, base, acc, out);
         rk_update(a.stage, id, 1, k1, base, acc, out);
         rk_update(a.stage, id, 2, k2, base, acc, out);

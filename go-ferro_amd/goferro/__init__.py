@@ -26,6 +26,10 @@ __all__ = [
     "FIELD_U", "FIELD_V", "FIELD_U_STORE", "FIELD_STRAIN_DIAG", "FIELD_STRAIN_OFF",
     "FLAG_ELASTODYNAMICS", "FLAG_ELASTO_BACKACTION", "FLAG_ELASTO_ON_PM", "FLAG_GAUSSIAN_PULSE", "FLAG_1D_ABC",
     "FLAG_1D_ABC_ELAST_ONLYTOP",
+    "FLAG_EM_FROM_P", "FLAG_EM_FROM_M", "FLAG_EM_BACKACTION_P", "FLAG_EM_BACKACTION_M", "FLAG_PLANE_EM",
+    "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
+    "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
+    "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS",
 ]
 
@@ -47,17 +51,62 @@ FLAG_ELASTO_ON_PM = 1 << 10
 FLAG_GAUSSIAN_PULSE = 1 << 11
 FLAG_1D_ABC = 1 << 12
 FLAG_1D_ABC_ELAST_ONLYTOP = 1 << 13
+FLAG_EM_FROM_P = 1 << 14
+FLAG_EM_FROM_M = 1 << 15
+FLAG_EM_BACKACTION_P = 1 << 16
+FLAG_EM_BACKACTION_M = 1 << 17
+FLAG_PLANE_EM = 1 << 18
+FLAG_JF_INPUT = 1 << 19
+FLAG_PEC_XY = 1 << 20
+FLAG_PEC_Z = 1 << 21
+FLAG_1D_ABC_EM_ONLYTOP = 1 << 22
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
- FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF) = range(15)
+ FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
+ FIELD_E_EM_STORE, FIELD_H_EM_STORE, FIELD_E_CELL, FIELD_H_CELL, FIELD_JP) = range(22)
+
+
+def yee_shapes(shape, field):
+    """Shapes of the three component arrays of a Yee field
+    (EMdynamic_initialize.cpp:68-95): E on edges, H on faces."""
+    nx, ny, nz = (int(v) for v in shape)
+    if field in (FIELD_E_EM, FIELD_E_EM_STORE):
+        return [(nx, ny + 1, nz + 1), (nx + 1, ny, nz + 1), (nx + 1, ny + 1, nz)]
+    if field in (FIELD_H_EM, FIELD_H_EM_STORE):
+        return [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+    raise ValueError("not a Yee field")
+
+
+def field_size(shape, field) -> int:
+    """Number of doubles a field occupies in gf_upload / gf_download."""
+    n = int(shape[0] * shape[1] * shape[2])
+    if field in (FIELD_E_EM, FIELD_E_EM_STORE, FIELD_H_EM, FIELD_H_EM_STORE):
+        return sum(a * b * c for a, b, c in yee_shapes(shape, field))
+    if field == FIELD_JP:
+        return 9 * n
+    return 3 * n
+
+
+def split_yee(flat, shape, field):
+    """Concatenated x|y|z Yee array -> list of three component arrays."""
+    out, o = [], 0
+    for s in yee_shapes(shape, field):
+        k = s[0] * s[1] * s[2]
+        out.append(np.asarray(flat[o:o + k]).reshape(s))
+        o += k
+    return out
+
+
+def join_yee(parts):
+    return np.concatenate([np.ascontiguousarray(p, dtype=np.float64).reshape(-1) for p in parts])
 
 KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_conv", "fft_y_inv",
-                  "fft_z_inv", "other", "fe_stage", "elastic_stage", "elastic_strain"]
+                  "fft_z_inv", "other", "fe_stage", "elastic_stage", "elastic_strain", "em_e", "em_h"]
 
 # every entry point declared in include/goferro.h
 EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
-    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
+    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_em_stage", "gf_field_size", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
     "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times",
 ]
 
@@ -98,6 +147,11 @@ class Material(ctypes.Structure):
         ("lamda100", ctypes.c_double), ("lamda111", ctypes.c_double),
         ("lamda100_AFM1", ctypes.c_double), ("lamda100_AFM2", ctypes.c_double),
         ("lamda111_AFM1", ctypes.c_double), ("lamda111_AFM2", ctypes.c_double),
+        ("if_PEC", ctypes.c_int32), ("reserved2", ctypes.c_int32),
+        ("r_permittivity", ctypes.c_double), ("conductivity", ctypes.c_double),
+        ("comp_n1", ctypes.c_double), ("omega_plasma_n1", ctypes.c_double), ("tao_e_n1", ctypes.c_double),
+        ("comp_n2", ctypes.c_double), ("omega_plasma_n2", ctypes.c_double), ("tao_e_n2", ctypes.c_double),
+        ("comp_n3", ctypes.c_double), ("omega_plasma_n3", ctypes.c_double), ("tao_e_n3", ctypes.c_double),
     ]
 
     @classmethod
@@ -136,6 +190,11 @@ class _Desc(ctypes.Structure):
         ("input_strain_component", ctypes.c_int32), ("num_strain_cycles", ctypes.c_int32),
         ("sigma_gauss", ctypes.c_double), ("amplitude_gauss", ctypes.c_double),
         ("weighting_factor_fourth_Liao", ctypes.c_double),
+        ("planeEM_E", ctypes.c_double * 3), ("planeEM_E_freq", ctypes.c_double * 3),
+        ("planeEM_source_z", ctypes.c_int32), ("Jf_input_type", ctypes.c_int32),
+        ("Jf_input_component", ctypes.c_int32), ("num_Jf_cycles", ctypes.c_int32),
+        ("Jfin_lo", ctypes.c_int32 * 3), ("Jfin_hi", ctypes.c_int32 * 3),
+        ("Jf_input_amp", ctypes.c_double), ("Jf_input_freq", ctypes.c_double), ("Jf_input_sigma", ctypes.c_double),
     ]
 
 
@@ -167,6 +226,17 @@ class Problem:
     sigma_gauss: float = 0.0
     amplitude_gauss: float = 0.0
     weighting_factor_fourth_Liao: float = 0.5
+    planeEM_E: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    planeEM_E_freq: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    planeEM_source_z: int = 0
+    Jf_input_type: int = 0
+    Jf_input_component: str = "x"
+    num_Jf_cycles: int = 0
+    Jfin_lo: tuple[int, int, int] = (0, 0, 0)
+    Jfin_hi: tuple[int, int, int] = (0, 0, 0)
+    Jf_input_amp: float = 0.0
+    Jf_input_freq: float = 0.0
+    Jf_input_sigma: float = 0.0
 
     @property
     def n(self) -> int:
@@ -207,6 +277,17 @@ class Problem:
         d.sigma_gauss = float(self.sigma_gauss)
         d.amplitude_gauss = float(self.amplitude_gauss)
         d.weighting_factor_fourth_Liao = float(self.weighting_factor_fourth_Liao)
+        d.planeEM_E[:] = [float(v) for v in self.planeEM_E]
+        d.planeEM_E_freq[:] = [float(v) for v in self.planeEM_E_freq]
+        d.planeEM_source_z = int(self.planeEM_source_z)
+        d.Jf_input_type = int(self.Jf_input_type)
+        d.Jf_input_component = ord(self.Jf_input_component[0]) if self.Jf_input_component else 0
+        d.num_Jf_cycles = int(self.num_Jf_cycles)
+        d.Jfin_lo[:] = [int(v) for v in self.Jfin_lo]
+        d.Jfin_hi[:] = [int(v) for v in self.Jfin_hi]
+        d.Jf_input_amp = float(self.Jf_input_amp)
+        d.Jf_input_freq = float(self.Jf_input_freq)
+        d.Jf_input_sigma = float(self.Jf_input_sigma)
         keep = [cell, mats]
         if self.prescribe_Hext is not None:
             pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)
@@ -245,6 +326,8 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_afm_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_fe_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_elastic_stage": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_em_stage": ([P, ctypes.c_int], ctypes.c_int),
+        "gf_field_size": ([P, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
         "gf_demag": ([P], ctypes.c_int),
         "gf_step": ([P, ctypes.c_int64], ctypes.c_int),
         "gf_average": ([P, ctypes.c_int, dp], ctypes.c_int),
@@ -304,16 +387,32 @@ class Context:
         self.close()
 
     # -- fields
-    def upload(self, field: int, value: np.ndarray) -> None:
+    def upload(self, field: int, value) -> None:
+        """Upload a field: (3, nx, ny, nz) for cell vectors, (3, 3, nx, ny, nz)
+        for Jp, the concatenated x|y|z array (or a list of the three
+        component arrays) for the Yee E and H fields."""
+        if isinstance(value, (list, tuple)):
+            value = join_yee(value)
         a = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
-        if a.size != 3 * self.n:
-            raise ValueError("field must hold 3*nx*ny*nz values")
+        if a.size != field_size(self.shape, field):
+            raise ValueError(f"field {field} must hold {field_size(self.shape, field)} values")
         self._check(self.lib.gf_upload(self._h, field, _dptr(a), a.size), "gf_upload")
 
     def download(self, field: int) -> np.ndarray:
-        a = np.empty(3 * self.n, dtype=np.float64)
+        """Download a field; Yee fields come back concatenated x|y|z
+        (see split_yee)."""
+        a = np.empty(field_size(self.shape, field), dtype=np.float64)
         self._check(self.lib.gf_download(self._h, field, _dptr(a), a.size), "gf_download")
+        if field in (FIELD_E_EM, FIELD_E_EM_STORE, FIELD_H_EM, FIELD_H_EM_STORE):
+            return a
+        if field == FIELD_JP:
+            return a.reshape((3, 3) + self.shape)
         return a.reshape((3,) + self.shape)
+
+    def field_size(self, field: int) -> int:
+        n = ctypes.c_size_t()
+        self._check(self.lib.gf_field_size(self._h, field, ctypes.byref(n)), "gf_field_size")
+        return int(n.value)
 
     # -- protocol
     def time_advance(self) -> None:
@@ -330,6 +429,9 @@ class Context:
 
     def elastic_stage(self, stage: int) -> None:
         self._check(self.lib.gf_elastic_stage(self._h, stage), "gf_elastic_stage")
+
+    def em_stage(self, stage: int) -> None:
+        self._check(self.lib.gf_em_stage(self._h, stage), "gf_em_stage")
 
     def demag(self) -> None:
         self._check(self.lib.gf_demag(self._h), "gf_demag")

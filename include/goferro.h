@@ -54,6 +54,15 @@ typedef int gf_status;
 #define GF_FLAG_GAUSSIAN_PULSE   (1u << 11) /* if_gaussian_strain_pulse */
 #define GF_FLAG_1D_ABC           (1u << 12) /* if_1D_ABC: Mur instead of Liao */
 #define GF_FLAG_1D_ABC_ELAST_ONLYTOP (1u << 13) /* if_1D_ABC_ELAST_onlytop */
+#define GF_FLAG_EM_FROM_P        (1u << 14) /* if_EM_fromP: dP/dt is a source current */
+#define GF_FLAG_EM_FROM_M        (1u << 15) /* if_EM_fromM: dM/dt enters Faraday's law */
+#define GF_FLAG_EM_BACKACTION_P  (1u << 16) /* if_EM_backaction_on_P: E_em drives p */
+#define GF_FLAG_EM_BACKACTION_M  (1u << 17) /* if_EM_backaction_on_M: H_em drives m */
+#define GF_FLAG_PLANE_EM         (1u << 18) /* if_input_planeEM_E: hard plane source */
+#define GF_FLAG_JF_INPUT         (1u << 19) /* if_Jf_input: free-current source box */
+#define GF_FLAG_PEC_XY           (1u << 20) /* if_PEC_XY: x and y walls are PEC */
+#define GF_FLAG_PEC_Z            (1u << 21) /* if_PEC_Z: z walls are PEC */
+#define GF_FLAG_1D_ABC_EM_ONLYTOP (1u << 22) /* if_1D_ABC_EM_onlytop (with GF_FLAG_1D_ABC) */
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -93,6 +102,14 @@ typedef struct gf_material {
     double cijkl_glb[6][6];    /* global-frame stiffness, global.cpp:365-441 */
     double lamda100, lamda111;
     double lamda100_AFM1, lamda100_AFM2, lamda111_AFM1, lamda111_AFM2;
+    /* electromagnetic (material.h:52-57), comp_n* already normalised as
+     * global.cpp:360-362 does */
+    int32_t if_PEC;
+    int32_t reserved2;
+    double r_permittivity, conductivity;
+    double comp_n1, omega_plasma_n1, tao_e_n1;
+    double comp_n2, omega_plasma_n2, tao_e_n2;
+    double comp_n3, omega_plasma_n3, tao_e_n3;
 } gf_material;
 
 /*
@@ -134,6 +151,18 @@ typedef struct gf_desc {
     double sigma_gauss;             /* s (or Hz for type 2) */
     double amplitude_gauss;
     double weighting_factor_fourth_Liao;
+    /* electrodynamic sources (read.cpp:48-53, 97-100) */
+    double planeEM_E[3];            /* V/m */
+    double planeEM_E_freq[3];       /* Hz */
+    int32_t planeEM_source_z;       /* plane k of the hard source (EMdynamic_initialize.cpp:18) */
+    int32_t Jf_input_type;          /* 0..6 (EMdynamic_solver.cpp:3885-4025) */
+    int32_t Jf_input_component;     /* 'x', 'y' or 'z' */
+    int32_t num_Jf_cycles;
+    int32_t Jfin_lo[3];             /* Jfin_xi, Jfin_yi, Jfin_zi */
+    int32_t Jfin_hi[3];             /* Jfin_xf, Jfin_yf, Jfin_zf (inclusive) */
+    double Jf_input_amp;            /* A/m^2 */
+    double Jf_input_freq;           /* Hz (sigma in s for types 4, 5) */
+    double Jf_input_sigma;          /* s (type 6) */
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */
@@ -153,6 +182,17 @@ enum gf_field_id {
     GF_FIELD_U_STORE = 12,  /* u at the current RK stage, 3n               */
     GF_FIELD_STRAIN_DIAG = 13, /* dynamic strain exx, eyy, ezz (global), 3n, read only */
     GF_FIELD_STRAIN_OFF = 14,  /* dynamic strain eyz, exz, exy (global), 3n, read only */
+    /* Yee fields on their staggered grids, concatenated x|y|z:
+     *   E edges: Ex nx(ny+1)(nz+1), Ey (nx+1)ny(nz+1), Ez (nx+1)(ny+1)nz
+     *   H faces: Hx (nx+1)ny nz,    Hy nx(ny+1)nz,     Hz nx ny(nz+1)
+     * (EMdynamic_initialize.cpp:68-95); gf_field_size gives the count. */
+    GF_FIELD_E_EM = 15,
+    GF_FIELD_H_EM = 16,
+    GF_FIELD_E_EM_STORE = 17,  /* E at the current RK stage              */
+    GF_FIELD_H_EM_STORE = 18,
+    GF_FIELD_E_CELL = 19,      /* cell-centred E of the stage store, 3n, read only */
+    GF_FIELD_H_CELL = 20,      /* cell-centred H of the stage store, 3n, read only */
+    GF_FIELD_JP = 21,          /* polarisation currents, 9n: poles 1..3, each x,y,z */
     GF_FIELD_COUNT
 };
 
@@ -171,7 +211,7 @@ gf_status gf_destroy(gf_ctx* ctx);
 const char* gf_last_error(const gf_ctx* ctx);
 
 /* Host <-> device copies of one field.  count is the number of doubles
- * (3*n for vector fields).  Uploading GF_FIELD_M also sets the stage store
+ * (3*n for vector fields, see gf_field_size).  Uploading GF_FIELD_M also sets the stage store
  * to m, as initialize_device does (magnetic_initialize.cpp:1081-1097).
  * Both calls synchronise the context stream before returning. */
 gf_status gf_upload(gf_ctx* ctx, int field_id, const double* host, size_t count);
@@ -205,6 +245,19 @@ gf_status gf_fe_stage(gf_ctx* ctx, int stage);
  * forward-z wrap (k == ny-1, elastic_dynamic_solver.cpp:1247) reads past a
  * column or differs from a periodic wrap are rejected by gf_create. */
 gf_status gf_elastic_stage(gf_ctx* ctx, int stage);
+
+/* One electrodynamics RK4 stage: get_dE_RK + get_dH_RK (with the Jf source
+ * and the Jp currents) followed by update_DE_RK + update_DH_RK with the
+ * plane source, Mur walls, PEC edges and the cell-centred fields
+ * (EMdynamic_solver.cpp:337-3476, 3477-4556; EMdynamic_AbsorbingBoundary.cpp:
+ * 3-874).  Its get phase reads dp and dm of the same stage (ferroelectric
+ * and magnetic get phases), so within a stage it runs after gf_fe_stage,
+ * gf_mag_stage and gf_afm_stage, which read the cell fields of the
+ * stage-s E and H stores. */
+gf_status gf_em_stage(gf_ctx* ctx, int stage);
+
+/* Number of doubles gf_upload / gf_download move for a field. */
+gf_status gf_field_size(const gf_ctx* ctx, int field_id, size_t* count);
 
 /* magnetic_system::get_H_static (magnetostatic_solver.cpp:114-219) or the
  * 1D model get_H_static_1D (:222-249) per the descriptor flags. */
@@ -244,7 +297,9 @@ enum gf_kernel_class {
     GF_K_FE_STAGE = 8,
     GF_K_ELASTIC_STAGE = 9,   /* stress divergence + v, u updates */
     GF_K_ELASTIC_STRAIN = 10, /* face gradients, strain, eigenstrain */
-    GF_K_COUNT = 11
+    GF_K_EM_E = 11,           /* Ampere update of the E edges */
+    GF_K_EM_H = 12,           /* Faraday update of the H faces */
+    GF_K_COUNT = 13
 };
 gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
 gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]);

@@ -73,11 +73,23 @@ struct gfo_ctx {
     double sigma_gauss, amplitude_gauss, w_liao;
     /* statics */
     double *H[3], *Hnu[3];
+    /* electrodynamics (EMdynamic_system.h): E on edges, H on faces */
+    int if_em, periodic_all, pec_all, jp_on;
+    int64_t nE[3], nH[3];
+    double *Em[3], *Ems[3], *kE[4][3], *Et1[3];
+    double *Hm[3], *Hms[3], *kH[4][3];
+    double *Ecell[3], *Hcell[3], *Jf[3];
+    double *Jp[3][3], *Jps[3][3], *kJ[4][3][3];   /* [pole][comp] */
+    unsigned char *pec[3];
+    gf_desc desc;                              /* scalars of the descriptor */
+    double cf_top_half, cf_bot_half, cf_x_half, cf_y_half, cf_top, cf_bot, cf_x, cf_y;
     double *kspec[9];   /* complex spectra, interleaved, nx*ny*(nz/2+1) each */
 };
 
 static void elastic_gradients(gfo_ctx* c);
 static void elastic_strain(gfo_ctx* c);
+static int em_setup(gfo_ctx* c, const gf_desc* d);
+static void em_cells(gfo_ctx* c, int which);
 
 /* ------------------------------------------------------------------ FFT */
 /* In-place iterative radix-2 complex FFT, unnormalised, sign -1 forward. */
@@ -575,6 +587,7 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         }
         free(real9);
     }
+    if (em_setup(c, d)) { gfo_destroy(c); return NULL; }
     return c;
 }
 
@@ -601,6 +614,15 @@ void gfo_destroy(gfo_ctx* c) {
     free(c->surfYZ); free(c->surfXZ); free(c->surfXY); free(c->asurfYZ); free(c->asurfXZ); free(c->asurfXY);
     for (int f = 0; f < 6; f++) { free(c->Aij[f]); free(c->Aij1[f]); free(c->Aij2[f]); }
     for (int s = 0; s < 9; s++) free(c->kspec[s]);
+    for (int a = 0; a < 3; a++) {
+        free(c->Em[a]); free(c->Ems[a]); free(c->Et1[a]); free(c->Hm[a]); free(c->Hms[a]);
+        for (int s = 0; s < 4; s++) { free(c->kE[s][a]); free(c->kH[s][a]); }
+        free(c->Ecell[a]); free(c->Hcell[a]); free(c->Jf[a]); free(c->pec[a]);
+        for (int p = 0; p < 3; p++) {
+            free(c->Jp[p][a]); free(c->Jps[p][a]);
+            for (int s = 0; s < 4; s++) free(c->kJ[s][p][a]);
+        }
+    }
     free(c);
 }
 
@@ -623,12 +645,71 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_U_STORE: return c->us;
     case GF_FIELD_STRAIN_DIAG: return c->e_glb;
     case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
+    case GF_FIELD_E_CELL: return c->if_em ? c->Ecell : NULL;
+    case GF_FIELD_H_CELL: return c->if_em ? c->Hcell : NULL;
     default: return NULL;
     }
 }
 
 /* Uploading m also resets the store (magnetic_initialize.cpp:1081-1097). */
+/* The Yee fields (three staggered arrays, concatenated) and Jp (nine cell
+ * arrays); returns the number of arrays, 0 for other ids. */
+static int em_parts(gfo_ctx* c, int id, double** parts[9], int64_t sizes[9]) {
+    if (!c->if_em) return 0;
+    double** src = NULL;
+    const int64_t* sz = NULL;
+    if (id == GF_FIELD_E_EM) { src = c->Em; sz = c->nE; }
+    else if (id == GF_FIELD_E_EM_STORE) { src = c->Ems; sz = c->nE; }
+    else if (id == GF_FIELD_H_EM) { src = c->Hm; sz = c->nH; }
+    else if (id == GF_FIELD_H_EM_STORE) { src = c->Hms; sz = c->nH; }
+    if (src) {
+        for (int a = 0; a < 3; a++) { parts[a] = &src[a]; sizes[a] = sz[a]; }
+        return 3;
+    }
+    if (id == GF_FIELD_JP) {
+        for (int p = 0; p < 3; p++)
+            for (int a = 0; a < 3; a++) { parts[3 * p + a] = &c->Jp[p][a]; sizes[3 * p + a] = c->n; }
+        return 9;
+    }
+    return 0;
+}
+
+static int em_copy(gfo_ctx* c, int id, double* host, size_t count, int to_ctx) {
+    double** parts[9];
+    int64_t sizes[9];
+    int np = em_parts(c, id, parts, sizes);
+    if (!np) return -1;
+    size_t tot = 0;
+    for (int q = 0; q < np; q++) tot += (size_t)sizes[q];
+    if (count != tot) return -1;
+    size_t o = 0;
+    for (int q = 0; q < np; q++) {
+        if (to_ctx) memcpy(*parts[q], host + o, sizeof(double) * (size_t)sizes[q]);
+        else memcpy(host + o, *parts[q], sizeof(double) * (size_t)sizes[q]);
+        o += (size_t)sizes[q];
+    }
+    if (!to_ctx) return 0;
+    /* EMdynamic_initialize.cpp:376-404: the stores start equal to the
+     * fields and the cell fields follow them */
+    if (id == GF_FIELD_E_EM) {
+        for (int a = 0; a < 3; a++) memcpy(c->Ems[a], c->Em[a], sizeof(double) * (size_t)c->nE[a]);
+        em_cells(c, 1);
+    }
+    if (id == GF_FIELD_H_EM) {
+        for (int a = 0; a < 3; a++) memcpy(c->Hms[a], c->Hm[a], sizeof(double) * (size_t)c->nH[a]);
+        em_cells(c, 2);
+    }
+    if (id == GF_FIELD_JP)
+        for (int p = 0; p < 3; p++)
+            for (int a = 0; a < 3; a++) memcpy(c->Jps[p][a], c->Jp[p][a], sizeof(double) * (size_t)c->n);
+    return 0;
+}
+
 int gfo_set_field(gfo_ctx* c, int id, const double* src, size_t count) {
+    if (id == GF_FIELD_E_EM || id == GF_FIELD_E_EM_STORE || id == GF_FIELD_H_EM || id == GF_FIELD_H_EM_STORE ||
+        id == GF_FIELD_JP)
+        return em_copy(c, id, (double*)src, count, 1);
+    if (id == GF_FIELD_E_CELL || id == GF_FIELD_H_CELL) return -1;
     double** f = field_ptrs(c, id);
     if (!f || count != (size_t)(3 * c->n)) return -1;
     for (int a = 0; a < 3; a++) memcpy(f[a], src + a * c->n, sizeof(double) * (size_t)c->n);
@@ -649,6 +730,9 @@ int gfo_set_field(gfo_ctx* c, int id, const double* src, size_t count) {
 }
 
 int gfo_get_field(const gfo_ctx* c, int id, double* dst, size_t count) {
+    if (id == GF_FIELD_E_EM || id == GF_FIELD_E_EM_STORE || id == GF_FIELD_H_EM || id == GF_FIELD_H_EM_STORE ||
+        id == GF_FIELD_JP)
+        return em_copy((gfo_ctx*)c, id, dst, count, 0);
     double** f = field_ptrs((gfo_ctx*)c, id);
     if (id == GFO_FIELD_DM) f = ((gfo_ctx*)c)->dm;
     if (!f || count != (size_t)(3 * c->n)) return -1;
@@ -793,6 +877,8 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
         hd[2] = -2. * iDMIi / MU0 / Ms * (d[0] + d[1]);
         for (int a = 0; a < 3; a++)
             H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + hd[a] + c->Hext[a] + c->Hnu[a][i];
+        if (c->if_em && (c->flags & GF_FLAG_EM_BACKACTION_M))   /* magnetic_solver.cpp:200-204 */
+            for (int a = 0; a < 3; a++) H[a] = H[a] + c->Hcell[a][i];
         double alpha = mat->FM_damping, gyro = mat->gyro;
         double precess = -gyro / (1 + alpha * alpha) * c->dt;
         double px = mgy * H[2] - mgz * H[1], py = mgz * H[0] - mgx * H[2], pz = mgx * H[1] - mgy * H[0];
@@ -861,6 +947,8 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
                                         + A[3][i] * d2[1][1][a] + A[4][i] * d2[2][0][a] + A[5][i] * d2[2][1][a]);
             for (int a = 0; a < 3; a++) hi[a] = JAFM * other[a][i];
             for (int a = 0; a < 3; a++) H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + hi[a] + c->Hext[a];
+            if (c->if_em && (c->flags & GF_FLAG_EM_BACKACTION_M))   /* magnetic_AFM_solver.cpp:222-226 */
+                for (int a = 0; a < 3; a++) H[a] = H[a] + c->Hcell[a][i];
             double alpha = sub == 0 ? mat->damping_AFM1 : mat->damping_AFM2;
             double gyro = sub == 0 ? mat->gyro_AFM1 : mat->gyro_AFM2;
             double precess = -gyro / (1. + alpha * alpha) * c->dt;
@@ -1009,6 +1097,8 @@ void gfo_fe_get(gfo_ctx* c, int stage) {
         for (int a = 0; a < 3; a++)
             dq[a][id] = qg[a] + FE_mass * (-1. * G11 * (d2[a][0] + d2[a][1] + d2[a][2]) - c->Es[a][id] - c->Eext[a])
                       + FE_damping * c->qs[a][id];
+        if (c->if_em && (c->flags & GF_FLAG_EM_BACKACTION_P))   /* ferroelectric_solver.cpp:794-798 */
+            for (int a = 0; a < 3; a++) dq[a][id] = dq[a][id] - FE_mass * c->Ecell[a][id];
     }
     for (int a = 0; a < 3; a++)
         for (int64_t id = 0; id < c->n; id++) dp[a][id] = c->qs[a][id] * c->dt;
@@ -1395,6 +1485,454 @@ void gfo_elastic_update(gfo_ctx* c, int stage) {
     elastic_strain(c);
 }
 
+/* -------------------------------------------------- electrodynamics */
+static const double EPS0 = 8.854187817e-12;    /* EMdynamic_system.h:13 */
+static const double C_LIGHT = 2.99792458e8;    /* EMdynamic_system.h:14 */
+
+/* staggered indices: E on edges, H on faces (EMdynamic_initialize.cpp:68-95) */
+static int64_t EXI(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * (c->ny + 1) + j) * (c->nz + 1) + k; }
+static int64_t EYI(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * c->ny + j) * (c->nz + 1) + k; }
+static int64_t EZI(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * (c->ny + 1) + j) * c->nz + k; }
+static int64_t HXI(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * c->ny + j) * c->nz + k; }
+static int64_t HYI(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * (c->ny + 1) + j) * c->nz + k; }
+static int64_t HZI(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return (i * c->ny + j) * (c->nz + 1) + k; }
+
+static int pec_cell(const gfo_ctx* c, int64_t id) { const gf_material* m = matof(c, id); return m && m->if_PEC; }
+
+/* EMdynamic_initialize.cpp:3-156 and initialize_PEC_edges (:158-366) */
+static int em_setup(gfo_ctx* c, const gf_desc* d) {
+    c->desc = *d;
+    c->desc.materials = NULL; c->desc.material_cell = NULL;
+    c->desc.prescribe_Hext = NULL; c->desc.prescribe_Eext = NULL;
+    c->if_em = (c->flags & GF_FLAG_EMDYNAMIC) != 0;
+    if (!c->if_em) return 0;
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz, n = c->n;
+    if ((c->flags & GF_FLAG_PLANE_EM) && (d->planeEM_source_z < 0 || d->planeEM_source_z >= nz)) return 1;
+    if (c->flags & GF_FLAG_JF_INPUT) {
+        int64_t dims[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; a++)
+            if (d->Jfin_lo[a] < 0 || d->Jfin_hi[a] < d->Jfin_lo[a] || d->Jfin_hi[a] >= dims[a]) return 1;
+    }
+    c->periodic_all = c->per[0] && c->per[1] && c->per[2];   /* global.cpp:136-139 */
+    c->nE[0] = nx * (ny + 1) * (nz + 1); c->nE[1] = (nx + 1) * ny * (nz + 1); c->nE[2] = (nx + 1) * (ny + 1) * nz;
+    c->nH[0] = (nx + 1) * ny * nz; c->nH[1] = nx * (ny + 1) * nz; c->nH[2] = nx * ny * (nz + 1);
+    for (int a = 0; a < 3; a++) {
+        c->Em[a] = dalloc(c->nE[a]); c->Ems[a] = dalloc(c->nE[a]); c->Et1[a] = dalloc(c->nE[a]);
+        c->Hm[a] = dalloc(c->nH[a]); c->Hms[a] = dalloc(c->nH[a]);
+        for (int s = 0; s < 4; s++) { c->kE[s][a] = dalloc(c->nE[a]); c->kH[s][a] = dalloc(c->nH[a]); }
+        c->Ecell[a] = dalloc(n); c->Hcell[a] = dalloc(n); c->Jf[a] = dalloc(n);
+        for (int p = 0; p < 3; p++) {
+            c->Jp[p][a] = dalloc(n); c->Jps[p][a] = dalloc(n);
+            for (int s = 0; s < 4; s++) c->kJ[s][p][a] = dalloc(n);
+        }
+        c->pec[a] = (unsigned char*)calloc((size_t)c->nE[a], 1);
+    }
+    /* Mur coefficients from the permittivity of the first material met on
+     * the bottom and top planes */
+    double er_bot = 1., er_top = 1.;
+    int found = 0;
+    for (int64_t i = 0; i < nx && !found; i++)
+        for (int64_t j = 0; j < ny && !found; j++) {
+            const gf_material* m = matof(c, ID(c, i, j, 0));
+            if (m) { er_bot = sqrt(m->r_permittivity); found = 1; }
+        }
+    found = 0;
+    for (int64_t i = 0; i < nx && !found; i++)
+        for (int64_t j = 0; j < ny && !found; j++) {
+            const gf_material* m = matof(c, ID(c, i, j, nz - 1));
+            if (m) { er_top = sqrt(m->r_permittivity); found = 1; }
+        }
+    double dt = c->dt;
+    c->cf_top_half = (2. * c->dz - C_LIGHT / er_top * dt) / (2. * c->dz + C_LIGHT / er_top * dt);
+    c->cf_bot_half = (2. * c->dz - C_LIGHT / er_bot * dt) / (2. * c->dz + C_LIGHT / er_bot * dt);
+    c->cf_x_half = (2. * c->dx - C_LIGHT * dt) / (2. * c->dx + C_LIGHT * dt);
+    c->cf_y_half = (2. * c->dy - C_LIGHT * dt) / (2. * c->dy + C_LIGHT * dt);
+    c->cf_top = (c->dz - C_LIGHT / er_top * dt) / (c->dz + C_LIGHT / er_top * dt);
+    c->cf_bot = (c->dz - C_LIGHT / er_bot * dt) / (c->dz + C_LIGHT / er_bot * dt);
+    c->cf_x = (c->dx - C_LIGHT * dt) / (c->dx + C_LIGHT * dt);
+    c->cf_y = (c->dy - C_LIGHT * dt) / (c->dy + C_LIGHT * dt);
+    for (int i = 0; i < c->nmat; i++) if (c->mat[i].if_PEC) c->pec_all = 1;
+    if (c->pec_all) {
+        /* an edge is PEC when one of its four cells is, walls that are not
+         * periodic excluded */
+        for (int64_t i = 0; i < nx; i++)
+            for (int64_t j = 0; j <= ny; j++)
+                for (int64_t k = 0; k <= nz; k++) {
+                    if ((j == 0 || j == ny) && !c->per[1]) continue;
+                    if ((k == 0 || k == nz) && !c->per[2]) continue;
+                    int64_t y1 = (j == 0 || j == ny) ? ny - 1 : j - 1, y3 = (j == 0 || j == ny) ? 0 : j;
+                    int64_t z1 = (k == 0 || k == nz) ? nz - 1 : k - 1, z2 = (k == 0 || k == nz) ? 0 : k;
+                    c->pec[0][EXI(c, i, j, k)] = (unsigned char)(pec_cell(c, ID(c, i, y1, z1)) || pec_cell(c, ID(c, i, y1, z2)) ||
+                                                                 pec_cell(c, ID(c, i, y3, z1)) || pec_cell(c, ID(c, i, y3, z2)));
+                }
+        for (int64_t i = 0; i <= nx; i++)
+            for (int64_t j = 0; j < ny; j++)
+                for (int64_t k = 0; k <= nz; k++) {
+                    if ((i == 0 || i == nx) && !c->per[0]) continue;
+                    if ((k == 0 || k == nz) && !c->per[2]) continue;
+                    int64_t x1 = (i == 0 || i == nx) ? nx - 1 : i - 1, x3 = (i == 0 || i == nx) ? 0 : i;
+                    int64_t z1 = (k == 0 || k == nz) ? nz - 1 : k - 1, z2 = (k == 0 || k == nz) ? 0 : k;
+                    c->pec[1][EYI(c, i, j, k)] = (unsigned char)(pec_cell(c, ID(c, x1, j, z1)) || pec_cell(c, ID(c, x1, j, z2)) ||
+                                                                 pec_cell(c, ID(c, x3, j, z1)) || pec_cell(c, ID(c, x3, j, z2)));
+                }
+        for (int64_t i = 0; i <= nx; i++)
+            for (int64_t j = 0; j <= ny; j++)
+                for (int64_t k = 0; k < nz; k++) {
+                    if ((i == 0 || i == nx) && !c->per[0]) continue;
+                    if ((j == 0 || j == ny) && !c->per[1]) continue;
+                    int64_t x1 = (i == 0 || i == nx) ? nx - 1 : i - 1, x3 = (i == 0 || i == nx) ? 0 : i;
+                    int64_t y1 = (j == 0 || j == ny) ? ny - 1 : j - 1, y2 = (j == 0 || j == ny) ? 0 : j;
+                    c->pec[2][EZI(c, i, j, k)] = (unsigned char)(pec_cell(c, ID(c, x1, y1, k)) || pec_cell(c, ID(c, x1, y2, k)) ||
+                                                                 pec_cell(c, ID(c, x3, y1, k)) || pec_cell(c, ID(c, x3, y2, k)));
+                }
+    }
+    return 0;
+}
+
+/* update_DE_cell / update_DH_cell (EMdynamic_solver.cpp:3780-3826); which:
+ * 1 = E, 2 = H, 3 = both */
+static void em_cells(gfo_ctx* c, int which) {
+    int one_d = (c->flags & GF_FLAG_MAG_1D) && (c->flags & GF_FLAG_EM_FROM_M);
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++)
+            for (int64_t k = 0; k < c->nz; k++) {
+                int64_t id = ID(c, i, j, k);
+                if (which & 1) {
+                    c->Ecell[0][id] = (c->Ems[0][EXI(c, i, j, k)] + c->Ems[0][EXI(c, i, j + 1, k)] + c->Ems[0][EXI(c, i, j, k + 1)]
+                                       + c->Ems[0][EXI(c, i, j + 1, k + 1)]) / 4.;
+                    c->Ecell[1][id] = (c->Ems[1][EYI(c, i, j, k)] + c->Ems[1][EYI(c, i + 1, j, k)] + c->Ems[1][EYI(c, i, j, k + 1)]
+                                       + c->Ems[1][EYI(c, i + 1, j, k + 1)]) / 4.;
+                    c->Ecell[2][id] = (c->Ems[2][EZI(c, i, j, k)] + c->Ems[2][EZI(c, i + 1, j, k)] + c->Ems[2][EZI(c, i, j + 1, k)]
+                                       + c->Ems[2][EZI(c, i + 1, j + 1, k)]) / 4.;
+                }
+                if (which & 2) {
+                    c->Hcell[0][id] = (c->Hms[0][HXI(c, i, j, k)] + c->Hms[0][HXI(c, i + 1, j, k)]) / 2.;
+                    c->Hcell[1][id] = (c->Hms[1][HYI(c, i, j, k)] + c->Hms[1][HYI(c, i, j + 1, k)]) / 2.;
+                    if (one_d) {
+                        const gf_material* m = matof(c, id);
+                        c->Hcell[2][id] = !m ? 0. : -1. * m->Ms * c->ms[2][id] - 1. * m->Ms_AFM1 * c->ms1[2][id]
+                                                    - 1. * m->Ms_AFM2 * c->ms2[2][id] - c->H[2][id];
+                    } else {
+                        c->Hcell[2][id] = (c->Hms[2][HZI(c, i, j, k)] + c->Hms[2][HZI(c, i, j, k + 1)]) / 2.;
+                    }
+                }
+            }
+}
+
+/* update_Jf_input{,_half,_full} (EMdynamic_solver.cpp:3885-4315) */
+static void em_jf(gfo_ctx* c, int stage) {
+    const gf_desc* d = &c->desc;
+    double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
+    int a = d->Jf_input_component == 'x' ? 0 : (d->Jf_input_component == 'y' ? 1 : (d->Jf_input_component == 'z' ? 2 : -1));
+    int64_t Jnz = (int64_t)d->Jfin_hi[2] - d->Jfin_lo[2] + 1;
+    double amp = d->Jf_input_amp, f = d->Jf_input_freq, sg = d->Jf_input_sigma;
+    for (int64_t i = d->Jfin_lo[0]; i <= d->Jfin_hi[0]; i++)
+        for (int64_t j = d->Jfin_lo[1]; j <= d->Jfin_hi[1]; j++)
+            for (int64_t k = d->Jfin_lo[2]; k <= d->Jfin_hi[2]; k++) {
+                double val, tv, sv;
+                int type = d->Jf_input_type;
+                if (type < 3) {
+                    tv = sin(2. * PI * f * t);
+                    sv = PI / ((double)Jnz * c->dz) * ((double)(k - d->Jfin_lo[2]) + 0.5) * c->dz;
+                    if (type == 0) val = amp * tv;
+                    else if (type == 1) val = amp * tv * sin(sv);
+                    else val = amp * tv * cos(sv);
+                } else if (type == 3) {
+                    tv = t <= (double)d->num_Jf_cycles / f ? sin(2. * PI * f * t) : 0.;
+                    val = amp * tv;
+                } else if (type == 4) {
+                    tv = t - 5. * f;
+                    val = amp * exp(-1. * pow(tv, 2.) / (2. * pow(f, 2.)));
+                } else if (type == 5) {
+                    tv = t - 5. * f;
+                    val = amp * sqrt(exp(1.)) * tv / f * exp(-1. * pow(tv, 2.) / (2. * pow(f, 2.)));
+                } else if (type == 6) {
+                    tv = t - 5. * sg;
+                    val = amp * exp(-1. * pow(tv, 2.) / (2. * pow(sg, 2.))) * sin(2. * PI * f * t);
+                } else {
+                    continue;
+                }
+                if (a >= 0) c->Jf[a][ID(c, i, j, k)] = val;
+            }
+}
+
+/* The four-cell means of get_dE_RK* (EMdynamic_solver.cpp:392-499) and the
+ * Ampere increment; the spin-pumping current is not modelled (zero). */
+static double edge_dE(const gfo_ctx* c, int comp, const int64_t cell[4], double curl, double e_store, int stage) {
+    double Jf = 0., Jp = 0., dP = 0., er = 0., cond = 0., Pc = 0.;
+    for (int q = 0; q < 4; q++) {
+        int64_t id = cell[q];
+        const gf_material* mat = matof(c, id);
+        if (!mat) {
+            er = er + 1.;
+        } else {
+            if (c->flags & GF_FLAG_EM_FROM_P) dP = dP + c->kp[stage - 1][comp][id];
+            if (mat->if_FE) Pc = Pc + 1.;
+            er = er + mat->r_permittivity;
+            cond = cond + mat->conductivity;
+        }
+        Jf = Jf + c->Jf[comp][id];
+        Jp = Jp + c->Jps[0][comp][id] + c->Jps[1][comp][id] + c->Jps[2][comp][id];
+    }
+    if (Pc < 0.5) Pc = 1.;
+    Jf = Jf / 4.;
+    Jp = Jp / 4.;
+    dP = dP / Pc; er = er / 4.; cond = cond / 4.;
+    return c->dt / EPS0 / er * (curl - Jf - Jp - cond * e_store - dP / c->dt);
+}
+
+/* dM of one cell next to an H face (EMdynamic_solver.cpp:2290-2340) */
+static double face_dM(const gfo_ctx* c, int comp, int64_t id, int stage, double* count) {
+    const gf_material* mat = matof(c, id);
+    if (!mat) return 0.;
+    double d = 0.;
+    if (c->flags & GF_FLAG_EM_FROM_M)
+        d = c->k[stage - 1][comp][id] * mat->Ms + c->k1[stage - 1][comp][id] * mat->Ms_AFM1
+          + c->k2[stage - 1][comp][id] * mat->Ms_AFM2;
+    if (mat->if_FM || mat->if_AFM) *count = *count + 1.;
+    return d;
+}
+
+/* get_dE_RK*, get_dH_RK*, get_dJp_RK* with the Jf source
+ * (EMdynamic_solver.cpp:337-3476, 4318-4451) */
+void gfo_em_get(gfo_ctx* c, int stage) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    int px = c->per[0], py = c->per[1], pz = c->per[2];
+    double dt = c->dt;
+    if (c->flags & GF_FLAG_JF_INPUT) em_jf(c, stage);
+    for (int64_t id = 0; id < c->n; id++) {
+        const gf_material* mat = matof(c, id);
+        if (!mat) continue;
+        double cn[3] = {mat->comp_n1, mat->comp_n2, mat->comp_n3};
+        double wp[3] = {mat->omega_plasma_n1, mat->omega_plasma_n2, mat->omega_plasma_n3};
+        double re[3] = {1. / mat->tao_e_n1, 1. / mat->tao_e_n2, 1. / mat->tao_e_n3};
+        for (int p = 0; p < 3; p++)
+            for (int a = 0; a < 3; a++)
+                c->kJ[stage - 1][p][a][id] = (EPS0 * wp[p] * wp[p] * c->Ecell[a][id] * cn[p] - re[p] * c->Jps[p][a][id]) * dt;
+    }
+    double** Hs = c->Hms;
+    double** Es = c->Ems;
+    /* Ex edges */
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j <= ny; j++)
+            for (int64_t k = 0; k <= nz; k++) {
+                if ((j == 0 || j == ny) && !py) continue;
+                if ((k == 0 || k == nz) && !pz) continue;
+                int64_t y1 = (j == 0 || j == ny) ? ny - 1 : j - 1, y3 = (j == 0 || j == ny) ? 0 : j;
+                int64_t z1 = (k == 0 || k == nz) ? nz - 1 : k - 1, z2 = (k == 0 || k == nz) ? 0 : k;
+                double dHzdy = (Hs[2][HZI(c, i, y3, z2)] - Hs[2][HZI(c, i, y1, z2)]) / c->dy;
+                double dHydz = (Hs[1][HYI(c, i, y3, z2)] - Hs[1][HYI(c, i, y3, z1)]) / c->dz;
+                int64_t cell[4] = {ID(c, i, y1, z1), ID(c, i, y1, z2), ID(c, i, y3, z1), ID(c, i, y3, z2)};
+                int64_t e = EXI(c, i, j, k);
+                c->kE[stage - 1][0][e] = edge_dE(c, 0, cell, dHzdy - dHydz, Es[0][e], stage);
+            }
+    /* Ey edges */
+    for (int64_t i = 0; i <= nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k <= nz; k++) {
+                if ((i == 0 || i == nx) && !px) continue;
+                if ((k == 0 || k == nz) && !pz) continue;
+                int64_t x1 = (i == 0 || i == nx) ? nx - 1 : i - 1, x3 = (i == 0 || i == nx) ? 0 : i;
+                int64_t z1 = (k == 0 || k == nz) ? nz - 1 : k - 1, z2 = (k == 0 || k == nz) ? 0 : k;
+                double dHxdz = (Hs[0][HXI(c, x3, j, z2)] - Hs[0][HXI(c, x3, j, z1)]) / c->dz;
+                double dHzdx = (Hs[2][HZI(c, x3, j, z2)] - Hs[2][HZI(c, x1, j, z2)]) / c->dx;
+                int64_t cell[4] = {ID(c, x1, j, z1), ID(c, x1, j, z2), ID(c, x3, j, z1), ID(c, x3, j, z2)};
+                int64_t e = EYI(c, i, j, k);
+                c->kE[stage - 1][1][e] = edge_dE(c, 1, cell, dHxdz - dHzdx, Es[1][e], stage);
+            }
+    /* Ez edges */
+    for (int64_t i = 0; i <= nx; i++)
+        for (int64_t j = 0; j <= ny; j++)
+            for (int64_t k = 0; k < nz; k++) {
+                if ((i == 0 || i == nx) && !px) continue;
+                if ((j == 0 || j == ny) && !py) continue;
+                int64_t x1 = (i == 0 || i == nx) ? nx - 1 : i - 1, x3 = (i == 0 || i == nx) ? 0 : i;
+                int64_t y1 = (j == 0 || j == ny) ? ny - 1 : j - 1, y2 = (j == 0 || j == ny) ? 0 : j;
+                double dHydx = (Hs[1][HYI(c, x3, y2, k)] - Hs[1][HYI(c, x1, y2, k)]) / c->dx;
+                double dHxdy = (Hs[0][HXI(c, x3, y2, k)] - Hs[0][HXI(c, x3, y1, k)]) / c->dy;
+                int64_t cell[4] = {ID(c, x1, y1, k), ID(c, x1, y2, k), ID(c, x3, y1, k), ID(c, x3, y2, k)};
+                int64_t e = EZI(c, i, j, k);
+                c->kE[stage - 1][2][e] = edge_dE(c, 2, cell, dHydx - dHxdy, Es[2][e], stage);
+            }
+    /* H faces (EMdynamic_solver.cpp:2239-2547) */
+    for (int64_t i = 0; i <= nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k < nz; k++) {
+                int64_t x1 = i == 0 ? (px ? nx - 1 : 0) : i - 1, x2 = i == nx ? (px ? 0 : nx - 1) : i;
+                double dEzdy = (Es[2][EZI(c, i, j + 1, k)] - Es[2][EZI(c, i, j, k)]) / c->dy;
+                double dEydz = (Es[1][EYI(c, i, j, k + 1)] - Es[1][EYI(c, i, j, k)]) / c->dz;
+                double cnt = 0.;
+                double d1 = face_dM(c, 0, ID(c, x1, j, k), stage, &cnt), d2 = face_dM(c, 0, ID(c, x2, j, k), stage, &cnt);
+                if (cnt < 0.5) cnt = 1.;
+                c->kH[stage - 1][0][HXI(c, i, j, k)] = -dt / MU0 * (dEzdy - dEydz) - (d1 + d2) / cnt;
+            }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j <= ny; j++)
+            for (int64_t k = 0; k < nz; k++) {
+                int64_t y1 = j == 0 ? (py ? ny - 1 : 0) : j - 1, y2 = j == ny ? (py ? 0 : ny - 1) : j;
+                double dExdz = (Es[0][EXI(c, i, j, k + 1)] - Es[0][EXI(c, i, j, k)]) / c->dz;
+                double dEzdx = (Es[2][EZI(c, i + 1, j, k)] - Es[2][EZI(c, i, j, k)]) / c->dx;
+                double cnt = 0.;
+                double d1 = face_dM(c, 1, ID(c, i, y1, k), stage, &cnt), d2 = face_dM(c, 1, ID(c, i, y2, k), stage, &cnt);
+                if (cnt < 0.5) cnt = 1.;
+                c->kH[stage - 1][1][HYI(c, i, j, k)] = -dt / MU0 * (dExdz - dEzdx) - (d1 + d2) / cnt;
+            }
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k <= nz; k++) {
+                int64_t z1 = k == 0 ? (pz ? nz - 1 : 0) : k - 1, z2 = k == nz ? (pz ? 0 : nz - 1) : k;
+                double dEydx = (Es[1][EYI(c, i + 1, j, k)] - Es[1][EYI(c, i, j, k)]) / c->dx;
+                double dExdy = (Es[0][EXI(c, i, j + 1, k)] - Es[0][EXI(c, i, j, k)]) / c->dy;
+                double cnt = 0.;
+                double d1 = face_dM(c, 2, ID(c, i, j, z1), stage, &cnt), d2 = face_dM(c, 2, ID(c, i, j, z2), stage, &cnt);
+                if (cnt < 0.5) cnt = 1.;
+                c->kH[stage - 1][2][HZI(c, i, j, k)] = -dt / MU0 * (dEydx - dExdy) - (d1 + d2) / cnt;
+            }
+}
+
+/* update_planeEM{_half,_full,} (EMdynamic_solver.cpp:3829-3883); kind as
+ * for the boundary */
+static void em_plane(gfo_ctx* c, int kind) {
+    const gf_desc* d = &c->desc;
+    double t = kind == 0 ? c->time - 0.5 * c->dt : c->time;
+    double** dst = kind < 2 ? c->Ems : c->Em;
+    int64_t z = d->planeEM_source_z;
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++) {
+            dst[0][EXI(c, i, j, z)] = d->planeEM_E[0] * sin(2. * PI * d->planeEM_E_freq[0] * t);
+            dst[1][EYI(c, i, j, z)] = d->planeEM_E[1] * sin(2. * PI * d->planeEM_E_freq[1] * t);
+            dst[2][EZI(c, i, j, z)] = d->planeEM_E[2] * sin(2. * PI * d->planeEM_E_freq[2] * t);
+        }
+}
+
+/* first-order Mur walls (EMdynamic_AbsorbingBoundary.cpp:58-874): kind 0 =
+ * stages 1-2 (coefficients with 2d), 1 = stage 3, 2 = after the step
+ * (from the history t1); x walls, then y walls, then z walls */
+static void em_mur(double* dst, const double* b, const double* s, const double* t1, int kind, double cf,
+                   int64_t wall, int64_t inner) {
+    if (kind < 2) dst[wall] = b[inner] + cf * (b[wall] - s[inner]);
+    else dst[wall] = t1[inner] + cf * (t1[wall] - b[inner]);
+}
+
+static void em_boundary(gfo_ctx* c, int kind) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    double cx = kind == 0 ? c->cf_x_half : c->cf_x, cy = kind == 0 ? c->cf_y_half : c->cf_y;
+    double ct = kind == 0 ? c->cf_top_half : c->cf_top, cb = kind == 0 ? c->cf_bot_half : c->cf_bot;
+    int pec_xy = (c->flags & GF_FLAG_PEC_XY) != 0, pec_z = (c->flags & GF_FLAG_PEC_Z) != 0;
+    int onlytop = (c->flags & GF_FLAG_1D_ABC) && (c->flags & GF_FLAG_1D_ABC_EM_ONLYTOP);
+    double** D = kind < 2 ? c->Ems : c->Em;
+    if (!c->per[0]) {
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k <= nz; k++) {
+                int64_t w0 = EYI(c, 0, j, k), w1 = EYI(c, 1, j, k), wn = EYI(c, nx, j, k), wm = EYI(c, nx - 1, j, k);
+                if (!pec_xy) {
+                    em_mur(D[1], c->Em[1], c->Ems[1], c->Et1[1], kind, cx, w0, w1);
+                    em_mur(D[1], c->Em[1], c->Ems[1], c->Et1[1], kind, cx, wn, wm);
+                } else { D[1][w0] = 0.; D[1][wn] = 0.; }
+            }
+        for (int64_t j = 0; j <= ny; j++)
+            for (int64_t k = 0; k < nz; k++) {
+                int64_t w0 = EZI(c, 0, j, k), w1 = EZI(c, 1, j, k), wn = EZI(c, nx, j, k), wm = EZI(c, nx - 1, j, k);
+                if (!pec_xy) {
+                    em_mur(D[2], c->Em[2], c->Ems[2], c->Et1[2], kind, cx, w0, w1);
+                    em_mur(D[2], c->Em[2], c->Ems[2], c->Et1[2], kind, cx, wn, wm);
+                } else { D[2][w0] = 0.; D[2][wn] = 0.; }
+            }
+    }
+    if (!c->per[1]) {
+        for (int64_t i = 0; i < nx; i++)
+            for (int64_t k = 0; k <= nz; k++) {
+                int64_t w0 = EXI(c, i, 0, k), w1 = EXI(c, i, 1, k), wn = EXI(c, i, ny, k), wm = EXI(c, i, ny - 1, k);
+                if (!pec_xy) {
+                    em_mur(D[0], c->Em[0], c->Ems[0], c->Et1[0], kind, cy, w0, w1);
+                    em_mur(D[0], c->Em[0], c->Ems[0], c->Et1[0], kind, cy, wn, wm);
+                } else { D[0][w0] = 0.; D[0][wn] = 0.; }
+            }
+        for (int64_t i = 0; i <= nx; i++)
+            for (int64_t k = 0; k < nz; k++) {
+                int64_t w0 = EZI(c, i, 0, k), w1 = EZI(c, i, 1, k), wn = EZI(c, i, ny, k), wm = EZI(c, i, ny - 1, k);
+                if (!pec_xy) {
+                    em_mur(D[2], c->Em[2], c->Ems[2], c->Et1[2], kind, cy, w0, w1);
+                    em_mur(D[2], c->Em[2], c->Ems[2], c->Et1[2], kind, cy, wn, wm);
+                } else { D[2][w0] = 0.; D[2][wn] = 0.; }
+            }
+    }
+    if (!c->per[2]) {
+        for (int comp = 0; comp < 2; comp++) {
+            int64_t ni = comp == 0 ? nx : nx + 1, nj = comp == 0 ? ny + 1 : ny;
+            for (int64_t i = 0; i < ni; i++)
+                for (int64_t j = 0; j < nj; j++) {
+                    int64_t w0, w1, wn, wm;
+                    if (comp == 0) { w0 = EXI(c, i, j, 0); w1 = EXI(c, i, j, 1); wn = EXI(c, i, j, nz); wm = EXI(c, i, j, nz - 1); }
+                    else { w0 = EYI(c, i, j, 0); w1 = EYI(c, i, j, 1); wn = EYI(c, i, j, nz); wm = EYI(c, i, j, nz - 1); }
+                    if (!pec_z) {
+                        em_mur(D[comp], c->Em[comp], c->Ems[comp], c->Et1[comp], kind, ct, wn, wm);
+                        if (onlytop) D[comp][w0] = 0.;
+                        else em_mur(D[comp], c->Em[comp], c->Ems[comp], c->Et1[comp], kind, cb, w0, w1);
+                    } else { D[comp][w0] = 0.; D[comp][wn] = 0.; }
+                }
+        }
+    }
+}
+
+/* update_DE_PEC (EMdynamic_solver.cpp:4526-4556) */
+static void em_pec(gfo_ctx* c) {
+    for (int a = 0; a < 3; a++)
+        for (int64_t e = 0; e < c->nE[a]; e++)
+            if (c->pec[a][e]) { c->Em[a][e] = 0.; c->Ems[a][e] = 0.; }
+}
+
+/* update_DE_RK* / update_DE, update_DH_RK* / update_DH, update_Jp_RK* /
+ * update_Jp (EMdynamic_solver.cpp:3477-3777, 4453-4524) */
+void gfo_em_update(gfo_ctx* c, int stage) {
+    int plane = (c->flags & GF_FLAG_PLANE_EM) != 0;
+    if (stage < 4) {
+        double w = stage == 3 ? 1. : 0.5;
+        int kind = stage < 3 ? 0 : 1;
+        for (int a = 0; a < 3; a++)
+            for (int64_t e = 0; e < c->nE[a]; e++)
+                c->Ems[a][e] = stage == 3 ? c->Em[a][e] + c->kE[2][a][e] : c->Em[a][e] + c->kE[stage - 1][a][e] * 0.5;
+        if (plane) em_plane(c, kind);
+        if (!c->periodic_all) em_boundary(c, kind);
+        if (c->pec_all) em_pec(c);
+        em_cells(c, 1);
+        for (int p = 0; p < 3; p++)
+            for (int a = 0; a < 3; a++)
+                for (int64_t id = 0; id < c->n; id++)
+                    c->Jps[p][a][id] = stage == 3 ? c->Jp[p][a][id] + c->kJ[2][p][a][id]
+                                                  : c->Jp[p][a][id] + c->kJ[stage - 1][p][a][id] * 0.5;
+        for (int a = 0; a < 3; a++)
+            for (int64_t f = 0; f < c->nH[a]; f++)
+                c->Hms[a][f] = stage == 3 ? c->Hm[a][f] + c->kH[2][a][f] : c->Hm[a][f] + c->kH[stage - 1][a][f] * 0.5;
+        (void)w;
+        em_cells(c, 2);
+        return;
+    }
+    if (!c->periodic_all)
+        for (int a = 0; a < 3; a++) memcpy(c->Et1[a], c->Em[a], sizeof(double) * (size_t)c->nE[a]);
+    for (int a = 0; a < 3; a++)
+        for (int64_t e = 0; e < c->nE[a]; e++)
+            c->Em[a][e] = c->Em[a][e] + c->kE[0][a][e] / 6. + c->kE[1][a][e] / 3. + c->kE[2][a][e] / 3. + c->kE[3][a][e] / 6.;
+    if (plane) em_plane(c, 2);
+    if (!c->periodic_all) em_boundary(c, 2);
+    if (c->pec_all) em_pec(c);
+    for (int a = 0; a < 3; a++) memcpy(c->Ems[a], c->Em[a], sizeof(double) * (size_t)c->nE[a]);
+    em_cells(c, 1);
+    for (int p = 0; p < 3; p++)
+        for (int a = 0; a < 3; a++)
+            for (int64_t id = 0; id < c->n; id++) {
+                c->Jp[p][a][id] = c->Jp[p][a][id] + c->kJ[0][p][a][id] / 6. + c->kJ[1][p][a][id] / 3.
+                                + c->kJ[2][p][a][id] / 3. + c->kJ[3][p][a][id] / 6.;
+                c->Jps[p][a][id] = c->Jp[p][a][id];
+            }
+    for (int a = 0; a < 3; a++)
+        for (int64_t f = 0; f < c->nH[a]; f++) {
+            c->Hm[a][f] = c->Hm[a][f] + c->kH[0][a][f] / 6. + c->kH[1][a][f] / 3. + c->kH[2][a][f] / 3. + c->kH[3][a][f] / 6.;
+            c->Hms[a][f] = c->Hm[a][f];
+        }
+    em_cells(c, 2);
+}
+
 /* ------------------------------------------------------ statics */
 /* magnetostatic_solver.cpp:222-249 */
 static void demag_1d(gfo_ctx* c) {
@@ -1484,10 +2022,12 @@ void gfo_step(gfo_ctx* c, int64_t nsteps) {
             if (c->if_FM_all) gfo_mag_get_dm(c, s);
             if (c->if_AFM_all) gfo_afm_get_dm(c, s);
             if (c->if_FE_all) gfo_fe_get(c, s);
+            if (c->if_em) gfo_em_get(c, s);
             if (c->if_elasto) gfo_elastic_update(c, s);
             if (c->if_FM_all) gfo_mag_update(c, s);
             if (c->if_AFM_all) gfo_afm_update(c, s);
             if (c->if_FE_all) gfo_fe_update(c, s);
+            if (c->if_em) gfo_em_update(c, s);
             if (!em && one_d && ((c->if_FM_all && ms) || c->if_AFM_all)) demag_1d(c);
             if (!em && mag && !one_d && ms) demag_3d(c);
         }

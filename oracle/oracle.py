@@ -46,6 +46,7 @@ def lib() -> ctypes.CDLL:
         "gfo_afm_get_dm": ([P, ctypes.c_int], None), "gfo_afm_update": ([P, ctypes.c_int], None),
         "gfo_fe_get": ([P, ctypes.c_int], None), "gfo_fe_update": ([P, ctypes.c_int], None),
         "gfo_elastic_get": ([P, ctypes.c_int], None), "gfo_elastic_update": ([P, ctypes.c_int], None),
+        "gfo_em_get": ([P, ctypes.c_int], None), "gfo_em_update": ([P, ctypes.c_int], None),
         "gfo_demag": ([P], None), "gfo_step": ([P, i64], None),
         "gfo_average": ([P, ctypes.c_int, dp], None), "gfo_time": ([P], ctypes.c_double),
         "gfo_newell_real": ([i64, i64, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp], None),
@@ -94,14 +95,23 @@ class Oracle:
             pass
 
     def set(self, field, value):
+        from goferro import join_yee
+        if isinstance(value, (list, tuple)):
+            value = join_yee(value)
         a = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
         if self.L.gfo_set_field(self._h, field, _dp(a), a.size) != 0:
             raise ValueError("bad field or size")
 
     def get(self, field):
-        a = np.empty(3 * self.n)
+        from goferro import FIELD_E_EM, FIELD_E_EM_STORE, FIELD_H_EM, FIELD_H_EM_STORE, FIELD_JP, field_size
+        size = field_size(self.shape, field) if field < 100 else 3 * self.n
+        a = np.empty(size)
         if self.L.gfo_get_field(self._h, field, _dp(a), a.size) != 0:
             raise ValueError("bad field or size")
+        if field in (FIELD_E_EM, FIELD_E_EM_STORE, FIELD_H_EM, FIELD_H_EM_STORE):
+            return a
+        if field == FIELD_JP:
+            return a.reshape((3, 3) + self.shape)
         return a.reshape((3,) + self.shape)
 
     def time_advance(self):
@@ -123,6 +133,10 @@ class Oracle:
     def elastic_stage(self, s):
         self.L.gfo_elastic_get(self._h, s)
         self.L.gfo_elastic_update(self._h, s)
+
+    def em_stage(self, s):
+        self.L.gfo_em_get(self._h, s)
+        self.L.gfo_em_update(self._h, s)
 
     def demag(self):
         self.L.gfo_demag(self._h)
