@@ -28,19 +28,24 @@ METRIC = "coupled RK4 cell-steps/s + % HBM roofline; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(cls: str, n: int, nh: int) -> float:
+def kernel_bytes(cls: str, n: int, nh: int, fused: bool = False) -> float:
     """Algorithmic bytes of one launch of a kernel class (each input read
     once, each output written once).  n cells, nh = nx*ny*(nz/2+1) complex
-    points per component."""
+    points per component.  With the z transforms fused into the magnetic
+    stage, that stage reads the H z-spectrum instead of the real field and
+    writes the M z-spectrum."""
     c16 = 16.0
     if cls == "mag_stage":
+        if fused:
+            return 106.0 * n + 6 * nh * c16   # 130 - 24 (real H) per cell, + H and M z-spectra
         return 130.0 * n          # average over the four stages, see bytes table
     if cls == "fft_z_fwd":
         return 28.0 * n + 3 * nh * c16
     if cls in ("fft_y_fwd", "fft_y_inv"):
         return 6 * nh * c16
     if cls == "fft_x_conv":
-        return 15 * nh * c16
+        # 3 M spectra in, 3 H spectra out, 9 tensor spectra at fp64 real + fp32 imaginary
+        return (6 * c16 + 9 * 12.0) * nh
     if cls == "fft_z_inv":
         return 3 * nh * c16 + 24.0 * n
     return 0.0
@@ -50,14 +55,14 @@ def measured_traffic(kernel_cls: str):
     """HBM bytes per launch of a kernel class from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary of this workload (already scaled by the
     1 GiB copy calibration), or None when the summary has no such row."""
-    names = {"mag_stage": "mag_stage_kernel", "fft_z_fwd": "zfwd_kernel<false>", "fft_y_fwd": "yfwd_kernel",
+    names = {"mag_stage": "mag_stage_fused_kernel<true", "fft_z_fwd": "zfwd_kernel<false>", "fft_y_fwd": "yfwd_kernel",
              "fft_x_conv": "xconv_kernel<true>", "fft_y_inv": "yinv_kernel", "fft_z_inv": "zinv_kernel"}
     path = ROOT / "profiles" / "fm256_demag_kernel_stats.md"
     if kernel_cls not in names or not path.exists():
         return None
     for line in path.read_text().splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 7 and cells[0] == f"`{names[kernel_cls]}`":
+        if len(cells) == 7 and cells[0].strip("`").startswith(names[kernel_cls]):
             return float(cells[6]) * 1e6
     return None
 
@@ -178,9 +183,10 @@ def main() -> int:
     }
     if kt:
         per = {}
+        fused = kt.get("fft_z_fwd", (0.0, 0))[1] == 0
         for cls, (ms, cnt) in kt.items():
             if cnt:
-                b = kernel_bytes(cls, n, nh)
+                b = kernel_bytes(cls, n, nh, fused)
                 per[cls] = {"launches": cnt, "avg_us": 1e3 * ms / cnt,
                             "share": ms / max(1e-12, sum(v[0] for v in kt.values())),
                             "achieved_GBs": (b / (ms / cnt * 1e-3)) / 1e9 if b else None}
@@ -189,7 +195,7 @@ def main() -> int:
         traffic = measured_traffic(dom) if shape == (256, 256, 128) else None
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                           "bytes_per_launch": kernel_bytes(dom, n, nh), "avg_launch_us": d["avg_us"]}
+                           "bytes_per_launch": kernel_bytes(dom, n, nh, fused), "avg_launch_us": d["avg_us"]}
         out["kernels"] = per
     ctx.close()
     if args.copy_probe_mib:

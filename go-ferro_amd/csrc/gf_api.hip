@@ -937,6 +937,8 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
         c->fe_next_stage = 1;
     }
     if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
+    if (id == GF_FIELD_M || id == GF_FIELD_M_STORE) c->demag.m_in_spec = false;
+    if (id == GF_FIELD_HSTAT) { c->demag.h_in_spec = false; c->demag.h_real_valid = true; }
     // elastic_initialize.cpp:386-410: the stores start equal to u and v
     if (id == GF_FIELD_U || id == GF_FIELD_V) {
         const bool isu = id == GF_FIELD_U;
@@ -962,6 +964,7 @@ gf_status gf_download(gf_ctx* c, int id, double* host, size_t count) {
         st = elastic_strain(c);
         if (st) return st;
     }
+    if (id == GF_FIELD_HSTAT && (st = demag_materialize_h(c))) return st;
     for (int a = 0; a < 3; a++) {
         st = hip_check(c, hipMemcpyAsync(host + a * c->n, f[a], sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream),
                        "download");
@@ -995,6 +998,66 @@ gf_status gf_time_advance(gf_ctx* c) {
     return GF_OK;
 }
 
+}  // extern "C"
+
+namespace gf {
+
+// Arguments of one FM stage reading store in_sel and writing store out_sel
+// (0 = m, 1 = st_a, 2 = st_b); the drive is left at zero.
+StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
+    StageArgs a{};
+    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+    const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** s = fm_store(c, in_sel);
+    double** o = fm_store(c, out_sel);
+    for (int i = 0; i < 3; i++) {
+        a.store[i] = s[i]; a.base[i] = c->m[i]; a.acc[i] = c->acc[i]; a.out[i] = o[i];
+        a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = c->em_dm[i];
+        a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    a.stage = stage;
+    return a;
+}
+
+// The same for the two AFM sublattices.
+AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
+    AfmStageArgs a{};
+    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+    const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
+    for (int i = 0; i < 3; i++) {
+        a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
+        a.dm_out1[i] = c->em_dm1[i];
+        a.dm_out2[i] = c->em_dm2[i];
+    }
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** s1 = afm1_store(c, in_sel);
+    double** s2 = afm2_store(c, in_sel);
+    double** o1 = afm1_store(c, out_sel);
+    double** o2 = afm2_store(c, out_sel);
+    for (int i = 0; i < 3; i++) {
+        a.store1[i] = s1[i]; a.store2[i] = s2[i];
+        a.base1[i] = c->m1[i]; a.base2[i] = c->m2[i];
+        a.acc1[i] = c->acc1[i]; a.acc2[i] = c->acc2[i];
+        a.out1[i] = o1[i]; a.out2[i] = o2[i];
+        a.hstat[i] = c->hstat[i];
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    a.stage = stage;
+    return a;
+}
+
+}  // namespace gf
+
+extern "C" {
+
 gf_status gf_mag_stage(gf_ctx* c, int stage) {
     if (!c) return GF_ERR_ARG;
     if (stage < 1 || stage > 4) return set_error(c, GF_ERR_ARG, "stage must be 1..4");
@@ -1004,27 +1067,26 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
                                               std::to_string(c->mag_next_stage) + ")");
     if (gf_status e = check_after_elastic(c, stage)) return e;
     if (gf_status e = check_before_em(c, stage)) return e;
-    StageArgs a{};
-    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
-    const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
-    a.code = c->d_code; a.mats = c->d_mats;
-    double** s = fm_store(c, c->mag_store_sel);
     const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
-    double** o = fm_store(c, out_sel);
-    for (int i = 0; i < 3; i++) {
-        a.store[i] = s[i]; a.base[i] = c->m[i]; a.acc[i] = c->acc[i]; a.out[i] = o[i];
-        a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = c->em_dm[i];
-        a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
-    }
-    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    StageArgs a = fm_stage_args(c, stage, c->mag_store_sel, out_sel);
     stage_drive(c, stage, a.hext);
-    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
-    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
-    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
-    a.stage = stage;
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_MAG_STAGE, &ev);
-    gf_status st = hip_check(c, launch_mag_stage(a, c->n, c->stream), "mag stage");
+    gf_status st;
+    if (c->demag.fused) {
+        FusedZArgs f{};
+        f.hspec = c->demag.spec_a; f.mspec = c->demag.spec_m; f.tw = c->demag.twz.dev;
+        f.nh = c->nx * c->ny * c->demag.nzh;
+        f.nzh = static_cast<int>(c->demag.nzh);
+        int lz = 0;
+        while ((int64_t{1} << lz) < c->nz) lz++;
+        f.lognz = lz;
+        f.P = c->demag.P;
+        st = hip_check(c, launch_mag_stage_fused(a, f, c->demag.h_in_spec, c->stream), "mag stage");
+        if (!st) c->demag.m_in_spec = true;
+    } else {
+        st = hip_check(c, launch_mag_stage(a, c->n, c->stream), "mag stage");
+    }
     timer_end(c, GF_K_MAG_STAGE, ev);
     if (st) return st;
     c->mag_store_sel = out_sel;
@@ -1040,28 +1102,8 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
         return set_error(c, GF_ERR_STATE, "AFM stages must run in the order 1,2,3,4");
     if (gf_status e = check_after_elastic(c, stage)) return e;
     if (gf_status e = check_before_em(c, stage)) return e;
-    AfmStageArgs a{};
-    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
-    const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
-    for (int i = 0; i < 3; i++) {
-        a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
-        a.dm_out1[i] = c->em_dm1[i];
-        a.dm_out2[i] = c->em_dm2[i];
-    }
-    a.code = c->d_code; a.mats = c->d_mats;
-    double** s1 = afm1_store(c, c->afm_store_sel);
-    double** s2 = afm2_store(c, c->afm_store_sel);
     const int out_sel = stage == 4 ? 0 : (c->afm_store_sel == 1 ? 2 : 1);
-    double** o1 = afm1_store(c, out_sel);
-    double** o2 = afm2_store(c, out_sel);
-    for (int i = 0; i < 3; i++) {
-        a.store1[i] = s1[i]; a.store2[i] = s2[i];
-        a.base1[i] = c->m1[i]; a.base2[i] = c->m2[i];
-        a.acc1[i] = c->acc1[i]; a.acc2[i] = c->acc2[i];
-        a.out1[i] = o1[i]; a.out2[i] = o2[i];
-        a.hstat[i] = c->hstat[i];
-    }
-    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    AfmStageArgs a = afm_stage_args(c, stage, c->afm_store_sel, out_sel);
     // the AFM path refreshes the sinusoidal drive unconditionally
     // (magnetic_AFM_solver.cpp:61-64)
     const double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
@@ -1069,10 +1111,6 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
         c->hext[i] = c->hext_stat[i] + c->hext_altr[i] * std::sin(2. * kPi * c->h_altr_freq[i] * t);
         a.hext[i] = c->hext[i];
     }
-    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
-    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
-    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
-    a.stage = stage;
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_AFM_STAGE, &ev);
     gf_status st = hip_check(c, launch_afm_stage(a, c->n, c->stream), "afm stage");
@@ -1423,6 +1461,7 @@ gf_status gf_demag(gf_ctx* c) {
     if (c->flags & GF_FLAG_MAG_1D) return hip_check(c, launch_demag_local(c, fm, s1, s2, 1), "demag 1d");
     if (c->flags & GF_FLAG_DEMAG_FACTOR) return hip_check(c, launch_demag_local(c, fm, s1, s2, 0), "demag factor");
     if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT magnetostatics was not set up (flags)");
+    if (c->demag.fused) return demag_run_fused(c, fm);
     return demag_run(c, fm, s1, s2);
 }
 

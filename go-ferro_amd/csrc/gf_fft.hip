@@ -19,10 +19,12 @@
 // natural result), inverse decimation in frequency (natural load,
 // bit-reversed result read back at the store), twiddles staged in LDS.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
-#include "gf_internal.h"
+#include "gf_fft_dev.h"
 
 namespace gf {
 
@@ -30,67 +32,9 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ int bitrev(int j, int logn) {
-    return logn ? static_cast<int>(__brev(static_cast<unsigned>(j)) >> (32 - logn)) : 0;
-}
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// P pencils of length N = 2^logn at buf[p*N], bit-reversed in, natural out.
-__device__ void fft_dit(double2* buf, int pencils, int logn, const double2* tw) {
-    const int N = 1 << logn;
-    const int half = N >> 1;
-    const int total = pencils * half;
-    for (int s = 0; s < logn; s++) {
-        const int h = 1 << s;
-        const int tstep = N >> (s + 1);
-        for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            const int p = b >> (logn - 1);
-            const int q = b & (half - 1);
-            const int j = q & (h - 1);
-            const int i0 = p * N + ((q >> s) << (s + 1)) + j;
-            const int i1 = i0 + h;
-            const double2 w = tw[j * tstep];
-            const double2 u = buf[i0];
-            const double2 v = buf[i1];
-            const double2 t = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
-            buf[i0] = make_double2(u.x + t.x, u.y + t.y);
-            buf[i1] = make_double2(u.x - t.x, u.y - t.y);
-        }
-        __syncthreads();
-    }
-}
-
-// Inverse transform (conjugate twiddles), natural in, bit-reversed out.
-__device__ void ifft_dif(double2* buf, int pencils, int logn, const double2* tw) {
-    const int N = 1 << logn;
-    const int half = N >> 1;
-    const int total = pencils * half;
-    for (int s = logn - 1; s >= 0; s--) {
-        const int h = 1 << s;
-        const int tstep = N >> (s + 1);
-        for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            const int p = b >> (logn - 1);
-            const int q = b & (half - 1);
-            const int j = q & (h - 1);
-            const int i0 = p * N + ((q >> s) << (s + 1)) + j;
-            const int i1 = i0 + h;
-            const double2 w = make_double2(tw[j * tstep].x, -tw[j * tstep].y);
-            const double2 u = buf[i0];
-            const double2 v = buf[i1];
-            const double2 d = make_double2(u.x - v.x, u.y - v.y);
-            buf[i0] = make_double2(u.x + v.x, u.y + v.y);
-            buf[i1] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-        }
-        __syncthreads();
-    }
-}
-
-__device__ __forceinline__ void load_twiddles(double2* dst, const double2* src, int n) {
-    for (int j = threadIdx.x; j < (n >> 1); j += blockDim.x) dst[j] = src[j];
-}
+__device__ __forceinline__ int bitrev(int j, int logn) { return fft_bitrev(j, logn); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) { return fft_cmul(a, b); }
+__device__ __forceinline__ void load_twiddles(double2* dst, const double2* src, int n) { fft_load_twiddles(dst, src, n); }
 
 struct ZArgs {
     const uint32_t* code;
@@ -226,7 +170,8 @@ __global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
 
 struct XArgs {
     double2* data;              // component c at data + c*nh, layout [kz][y][x]
-    const double2* kspec;       // spectrum s at kspec + s*nh
+    const double* kre;          // real part of tensor spectrum s at kre + s*nh
+    const float* kim;           // its imaginary part (round-off of an even/odd tensor) at kim + s*nh
     const double2* tw;
     int64_t nh;
     double den;                 // -dx*dy*dz
@@ -272,13 +217,15 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
             const int iy = (1 * a.pencils + r) * a.nx + x;
             const int iz = (2 * a.pencils + r) * a.nx + x;
             const double2 mx = buf[ix], my = buf[iy], mz = buf[iz];
-            const double2* K = a.kspec + g;
             const int64_t s = a.nh;
-            double2 t0 = cmul(mx, K[0 * s]), t1 = cmul(my, K[3 * s]), t2 = cmul(mz, K[8 * s]);
+            auto K = [&](int q) {
+                return make_double2(a.kre[q * s + g], static_cast<double>(a.kim[q * s + g]));
+            };
+            double2 t0 = cmul(mx, K(0)), t1 = cmul(my, K(3)), t2 = cmul(mz, K(8));
             const double2 hx = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
-            t0 = cmul(my, K[1 * s]); t1 = cmul(mz, K[4 * s]); t2 = cmul(mx, K[6 * s]);
+            t0 = cmul(my, K(1)); t1 = cmul(mz, K(4)); t2 = cmul(mx, K(6));
             const double2 hy = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
-            t0 = cmul(mz, K[2 * s]); t1 = cmul(mx, K[5 * s]); t2 = cmul(my, K[7 * s]);
+            t0 = cmul(mz, K(2)); t1 = cmul(mx, K(5)); t2 = cmul(my, K(7));
             const double2 hz = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
             buf[ix] = hx; buf[iy] = hy; buf[iz] = hz;
         }
@@ -377,6 +324,18 @@ __global__ void __launch_bounds__(kThreads) demag_local_kernel(LocalArgs a) {
     }
 }
 
+// Tensor spectra as an FP64 real part and an FP32 imaginary part: the
+// tensor components are even or odd along every axis, so their spectra are
+// real up to the round-off of the reference's kernel evaluation (~1e-9 of
+// the peak); FP32 keeps that residue to ~1e-16 of the peak.
+__global__ void __launch_bounds__(kThreads) split_spectrum_kernel(const double2* in, double* re, float* im, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = in[i];
+    re[i] = v.x;
+    im[i] = static_cast<float>(v.y);
+}
+
 int ilog2(int64_t n) {
     int l = 0;
     while ((int64_t{1} << l) < n) l++;
@@ -413,11 +372,25 @@ struct Shapes {
     size_t lds_z, lds_y, lds_x;
 };
 
+// Pencils per workgroup.  z and y passes write their output transposed,
+// so their pencil count sets the length of the store runs (16 B each);
+// the x pass reads and writes whole rows, so it keeps one pencil per
+// component and more workgroups per CU (at 256x256x128 one pencil beat two
+// by 7% and eight y pencils matched four).  GOFERRO_FFT_PENCILS="z,y,x"
+// overrides the defaults (powers of two) for tuning sweeps.
 Shapes shapes(const gf_ctx* c) {
     Shapes s;
-    s.pz = pick_pencils(c->nz, 2048, 16, c->ny);
-    s.py = pick_pencils(c->ny, 2048, 16, c->nx);
-    s.px = pick_pencils(c->nx, 1024, 8, c->ny);
+    int64_t bz = 2048, by = 2048, bx = 512, cz = 16, cy = 16, cx = 1;
+    if (const char* env = std::getenv("GOFERRO_FFT_PENCILS")) {
+        int z = 0, y = 0, x = 0;
+        if (std::sscanf(env, "%d,%d,%d", &z, &y, &x) == 3 && z > 0 && y > 0 && x > 0) {
+            bz = int64_t{z} * c->nz; by = int64_t{y} * c->ny; bx = int64_t{x} * c->nx;
+            cz = z; cy = y; cx = x;
+        }
+    }
+    s.pz = pick_pencils(c->nz, bz, cz, c->ny);
+    s.py = pick_pencils(c->ny, by, cy, c->nx);
+    s.px = pick_pencils(c->nx, bx, cx, c->ny);
     s.lds_z = sizeof(double2) * (c->nz / 2 + s.pz * c->nz);
     s.lds_y = sizeof(double2) * (c->ny / 2 + s.py * c->ny);
     s.lds_x = sizeof(double2) * (c->nx / 2 + 3 * s.px * c->nx);
@@ -439,7 +412,7 @@ static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* rawp, const double
         a.a1[i] = s1 ? s1[i] : nullptr;
         a.a2[i] = s2 ? s2[i] : nullptr;
     }
-    a.raw = rawp; a.out = c->demag.spec_a; a.tw = c->demag.twz.dev;
+    a.raw = rawp; a.out = c->demag.spec_m; a.tw = c->demag.twz.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
     a.nzh = static_cast<int>(c->demag.nzh);
@@ -456,7 +429,7 @@ static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* rawp, const double
 static hipError_t run_y(gf_ctx* c, bool inverse, int comps) {
     const Shapes sh = shapes(c);
     YArgs a{};
-    a.in = inverse ? c->demag.spec_b : c->demag.spec_a;
+    a.in = inverse ? c->demag.spec_b : c->demag.spec_m;
     a.out = inverse ? c->demag.spec_a : c->demag.spec_b;
     a.tw = c->demag.twy.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
@@ -475,7 +448,7 @@ static hipError_t run_y(gf_ctx* c, bool inverse, int comps) {
 static hipError_t run_x(gf_ctx* c, bool conv) {
     const Shapes sh = shapes(c);
     XArgs a{};
-    a.data = c->demag.spec_b; a.kspec = c->demag.kspec; a.tw = c->demag.twx.dev;
+    a.data = c->demag.spec_b; a.kre = c->demag.kre; a.kim = c->demag.kim; a.tw = c->demag.twx.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
     a.den = -c->dx * c->dy * c->dz;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
@@ -518,9 +491,11 @@ gf_status demag_setup(gf_ctx* c) {
     d.nzh = c->nz / 2 + 1;
     const int64_t nh = c->nx * c->ny * d.nzh;
     gf_status st;
+    if ((st = hip_check(c, hipMalloc(&d.spec_m, sizeof(double2) * 3 * nh), "hipMalloc spec_m"))) return st;
     if ((st = hip_check(c, hipMalloc(&d.spec_a, sizeof(double2) * 3 * nh), "hipMalloc spec_a"))) return st;
     if ((st = hip_check(c, hipMalloc(&d.spec_b, sizeof(double2) * 3 * nh), "hipMalloc spec_b"))) return st;
-    if ((st = hip_check(c, hipMalloc(&d.kspec, sizeof(double2) * 9 * nh), "hipMalloc kspec"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.kre, sizeof(double) * 9 * nh), "hipMalloc kre"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.kim, sizeof(float) * 9 * nh), "hipMalloc kim"))) return st;
     if ((st = make_twiddles(c, d.twx, c->nx))) return st;
     if ((st = make_twiddles(c, d.twy, c->ny))) return st;
     if ((st = make_twiddles(c, d.twz, c->nz))) return st;
@@ -540,15 +515,55 @@ gf_status demag_setup(gf_ctx* c) {
         if (!st) st = hip_check(c, run_zfwd(c, true, draw, nullptr, nullptr, nullptr, 1), "zfwd kernel");
         if (!st) st = hip_check(c, run_y(c, false, 1), "yfwd kernel");
         if (!st) st = hip_check(c, run_x(c, false), "xfwd kernel");
-        if (!st)
-            st = hip_check(c, hipMemcpyAsync(d.kspec + s * nh, d.spec_b, sizeof(double2) * nh,
-                                             hipMemcpyDeviceToDevice, c->stream), "copy spectrum");
+        if (!st) {
+            hipLaunchKernelGGL(split_spectrum_kernel, dim3(blocks_for(nh, kThreads)), dim3(kThreads), 0, c->stream,
+                               d.spec_b, d.kre + s * nh, d.kim + s * nh, nh);
+            st = hip_check(c, hipGetLastError(), "split spectrum");
+        }
         if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "kernel spectra");
         if (st) { hipFree(draw); return st; }
     }
     hipFree(draw);
     d.ready = true;
+    // Fused z transforms when the magnetisation is FM only and P whole z
+    // columns fit the LDS budget of the fused stage (gf_llg.hip).
+    d.fused = false;
+    if (c->any_fm && !c->any_afm && !(c->flags & GF_FLAG_EMDYNAMIC) && c->nz >= 2) {
+        int P = 1;
+        auto lds = [&](int p) {
+            const int pairs = p > 1 ? p / 2 : 1;
+            return sizeof(double2) * (c->nz / 2 + pairs * c->nz) + sizeof(double) * 3 * p * c->nz;
+        };
+        while (P * 2 <= 8 && P * 2 <= c->ny && lds(P * 2) <= 48 * 1024) P *= 2;
+        if (lds(P) <= 48 * 1024) { d.fused = true; d.P = P; }
+    }
+    d.h_in_spec = false;
+    d.h_real_valid = true;
+    d.m_in_spec = false;
     return GF_OK;
+}
+
+// Fused mode: M z-spectrum (from the stage, or a z pass when the store
+// changed otherwise) -> y, x with the tensor product, y inverse -> H
+// z-spectrum in spec_a; the real field is built only when asked for.
+gf_status demag_run_fused(gf_ctx* c, const double* const* fm) {
+    gf_status st;
+    if (!c->demag.m_in_spec && (st = hip_check(c, run_zfwd(c, false, nullptr, fm, nullptr, nullptr, 3), "zfwd")))
+        return st;
+    if ((st = hip_check(c, run_y(c, false, 3), "yfwd"))) return st;
+    if ((st = hip_check(c, run_x(c, true), "xconv"))) return st;
+    if ((st = hip_check(c, run_y(c, true, 3), "yinv"))) return st;
+    c->demag.h_in_spec = true;
+    c->demag.h_real_valid = false;
+    c->demag.m_in_spec = false;
+    return GF_OK;
+}
+
+gf_status demag_materialize_h(gf_ctx* c) {
+    if (!c->demag.ready || !c->demag.fused || c->demag.h_real_valid) return GF_OK;
+    gf_status st = hip_check(c, run_zinv(c), "zinv");
+    if (!st) c->demag.h_real_valid = true;
+    return st;
 }
 
 gf_status demag_run(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2) {
@@ -577,7 +592,7 @@ hipError_t launch_demag_local(gf_ctx* c, const double* const* fm, const double* 
 
 void demag_free(gf_ctx* c) {
     DemagPlan& d = c->demag;
-    hipFree(d.spec_a); hipFree(d.spec_b); hipFree(d.kspec);
+    hipFree(d.spec_m); hipFree(d.spec_a); hipFree(d.spec_b); hipFree(d.kre); hipFree(d.kim);
     hipFree(d.twx.dev); hipFree(d.twy.dev); hipFree(d.twz.dev);
     d = DemagPlan{};
 }

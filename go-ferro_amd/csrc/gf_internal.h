@@ -6,7 +6,7 @@
 //   hstat[3]                                magnetostatic field
 //   code[n] (uint32)                        material id + interface bits (below)
 //   spec_a/spec_b                           complex FFT work buffers, 3 components
-//   kspec[9]                                demag tensor spectra
+//   kre[9], kim[9]                          demag tensor spectra (FP64 real, FP32 imaginary)
 // A cell code packs what the reference keeps in material_cell plus the six
 // FM_surf* / AFM_surf* face masks (magnetic_initialize.cpp:136-379, 635-870).
 #pragma once
@@ -332,6 +332,12 @@ struct AfmStageArgs {
     int32_t stage;
 };
 
+// Clock and sinusoidal drive of one instance of a batch.
+struct BatchClock {
+    double t, dt;
+    double stat[3], altr[3], freq[3];
+};
+
 // Twiddle table of one transform length: w[j] = exp(-2 pi i j / N), j < N/2.
 struct Twiddles {
     int64_t n = 0;
@@ -341,10 +347,27 @@ struct Twiddles {
 struct DemagPlan {
     bool ready = false;
     int64_t nzh = 0;                 // nz/2 + 1
-    double2* spec_a = nullptr;       // 3 components, layout [x][kz][y]
+    double2* spec_m = nullptr;       // M z-spectrum, 3 components, layout [x][kz][y]
+    double2* spec_a = nullptr;       // H z-spectrum, 3 components, layout [x][kz][y]
     double2* spec_b = nullptr;       // 3 components, layout [kz][y][x]
-    double2* kspec = nullptr;        // 9 spectra, layout [kz][y][x]
+    double* kre = nullptr;           // 9 tensor spectra, real parts, layout [kz][y][x]
+    float* kim = nullptr;            // their imaginary parts (round-off residue)
     Twiddles twx, twy, twz;
+    // Fused mode (FM-only magnetisation): the FM stage consumes the H
+    // z-spectrum and produces the M z-spectrum itself.
+    bool fused = false;
+    int P = 0;                       // z columns per fused workgroup
+    bool h_in_spec = false;          // spec_a holds the current H
+    bool h_real_valid = true;        // hstat holds the current H
+    bool m_in_spec = false;          // spec_m holds Ms m of the current FM store
+};
+
+struct FusedZArgs {
+    const double2* hspec;
+    double2* mspec;
+    const double2* tw;
+    int64_t nh;
+    int nzh, lognz, P;
 };
 
 struct TimerSlot {
@@ -498,9 +521,23 @@ gf_status hip_check(gf_ctx* c, hipError_t e, const char* what);
 void timer_begin(gf_ctx* c, int cls, hipEvent_t* ev);
 void timer_end(gf_ctx* c, int cls, hipEvent_t start);
 
+// stage arguments of a context (gf_api.hip): stage store in_sel is read and
+// out_sel written (0 = m, 1 = first, 2 = second double-buffered store);
+// the drive is left at zero
+StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
+AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
+// true when cell indices of an n-cell grid fit 32-bit byte offsets (gf_llg.hip)
+bool narrow(int64_t n);
+
 // launchers (gf_llg.hip)
 hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s);
 hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s);
+hipError_t launch_mag_batch(const StageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
+                            uint32_t total, unsigned grid, unsigned threads, bool narrow_ids, hipStream_t s);
+hipError_t launch_afm_batch(const AfmStageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
+                            uint32_t total, unsigned grid, unsigned threads, hipStream_t s);
+hipError_t launch_batch_clock(BatchClock* clk, StageArgs* fm, AfmStageArgs* afm, int count, hipStream_t s);
+hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool hspec, hipStream_t s);
 
 // ferroelectric (gf_fe.hip)
 hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s);
@@ -527,6 +564,8 @@ hipError_t launch_em_jf(const EmJfArgs& a, hipStream_t s);
 gf_status demag_setup(gf_ctx* c);
 gf_status demag_run(gf_ctx* c, const double* const* fm_store, const double* const* s1,
                     const double* const* s2);
+gf_status demag_materialize_h(gf_ctx* c);
+gf_status demag_run_fused(gf_ctx* c, const double* const* fm_store);
 void demag_free(gf_ctx* c);
 hipError_t launch_demag_local(gf_ctx* c, const double* const* fm_store, const double* const* s1,
                               const double* const* s2, int one_d);

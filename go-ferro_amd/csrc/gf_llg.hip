@@ -13,7 +13,7 @@
 // k-arrays and the separate combine/update/copy launches disappear.  The
 // stencil store is read-only within a stage and the next store goes to the
 // other buffer, so physics that share a stage still see the old store.
-#include "gf_internal.h"
+#include "gf_fft_dev.h"
 
 namespace gf {
 
@@ -24,6 +24,24 @@ __device__ __forceinline__ void rot(const double T[3][3], double ix, double iy, 
     ox = T[0][0] * ix + T[0][1] * iy + T[0][2] * iz;
     oy = T[1][0] * ix + T[1][1] * iy + T[1][2] * iz;
     oz = T[2][0] * ix + T[2][1] * iy + T[2][2] * iz;
+}
+
+// Element access by a 64-bit index, or by a 32-bit index turned into a
+// 32-bit byte offset from the (uniform) array base: global loads and stores
+// then take the base from scalar registers and one 32-bit offset from
+// vector registers, which saves registers and address arithmetic.  The
+// 32-bit form is used when every array holds fewer than 2^28 elements.
+__device__ __forceinline__ double ldd(const double* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ double ldd(const double* p, uint32_t i) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + (i << 3));
+}
+__device__ __forceinline__ void std_(double* p, int64_t i, double v) { p[i] = v; }
+__device__ __forceinline__ void std_(double* p, uint32_t i, double v) {
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + (i << 3)) = v;
+}
+__device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ uint32_t ldu(const uint32_t* p, uint32_t i) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(p) + (i << 2));
 }
 
 // Exchange-weighted neighbour differences of one store for one cell
@@ -39,28 +57,28 @@ struct P3 {
     const double* p[3];
 };
 
-template <bool kDmi>
+template <bool kDmi, class I>
 __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, const DevMaterial* mats,
-                                           uint32_t own, int surf_shift, int sub, int64_t i, int64_t j, int64_t k,
-                                           int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,
+                                           uint32_t own, int surf_shift, int sub, I i, I j, I k,
+                                           I nx, I ny, I nz, double dx, double dy, double dz,
                                            double A, double A_self, double slope) {
-    const int64_t id = (i * ny + j) * nz + k;
+    const I id = (i * ny + j) * nz + k;
     const double* s[3] = {s3.p[0], s3.p[1], s3.p[2]};
-    const double c0 = s[0][id], c1 = s[1][id], c2 = s[2][id];
+    const double c0 = ldd(s[0], id), c1 = ldd(s[1], id), c2 = ldd(s[2], id);
     const uint32_t mat = own & kMatMask;
     Stencil r;
     double bw[3], fw[3], ab, af;
     // x faces
     {
-        int64_t nb;
+        I nb;
         if ((own >> (surf_shift + kXB)) & 1u) {
             if (kDmi) { bw[0] = -slope * c2; bw[1] = 0.; bw[2] = slope * c0; }
             else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
             ab = A;
         } else {
             nb = ((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k;
-            bw[0] = (c0 - s[0][nb]) / dx; bw[1] = (c1 - s[1][nb]) / dx; bw[2] = (c2 - s[2][nb]) / dx;
-            const uint32_t nm = code[nb] & kMatMask;
+            bw[0] = (c0 - ldd(s[0], nb)) / dx; bw[1] = (c1 - ldd(s[1], nb)) / dx; bw[2] = (c2 - ldd(s[2], nb)) / dx;
+            const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) ab = A_self;
             else {
                 const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -73,8 +91,8 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             af = A;
         } else {
             nb = ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k;
-            fw[0] = (s[0][nb] - c0) / dx; fw[1] = (s[1][nb] - c1) / dx; fw[2] = (s[2][nb] - c2) / dx;
-            const uint32_t nm = code[nb] & kMatMask;
+            fw[0] = (ldd(s[0], nb) - c0) / dx; fw[1] = (ldd(s[1], nb) - c1) / dx; fw[2] = (ldd(s[2], nb) - c2) / dx;
+            const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) af = A_self;
             else {
                 const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -87,15 +105,15 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
     }
     // y faces
     {
-        int64_t nb;
+        I nb;
         if ((own >> (surf_shift + kYB)) & 1u) {
             if (kDmi) { bw[0] = 0.; bw[1] = -slope * c2; bw[2] = slope * c1; }
             else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
             ab = A;
         } else {
             nb = (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k;
-            bw[0] = (c0 - s[0][nb]) / dy; bw[1] = (c1 - s[1][nb]) / dy; bw[2] = (c2 - s[2][nb]) / dy;
-            const uint32_t nm = code[nb] & kMatMask;
+            bw[0] = (c0 - ldd(s[0], nb)) / dy; bw[1] = (c1 - ldd(s[1], nb)) / dy; bw[2] = (c2 - ldd(s[2], nb)) / dy;
+            const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) ab = A_self;
             else {
                 const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -108,8 +126,8 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             af = A;
         } else {
             nb = (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k;
-            fw[0] = (s[0][nb] - c0) / dy; fw[1] = (s[1][nb] - c1) / dy; fw[2] = (s[2][nb] - c2) / dy;
-            const uint32_t nm = code[nb] & kMatMask;
+            fw[0] = (ldd(s[0], nb) - c0) / dy; fw[1] = (ldd(s[1], nb) - c1) / dy; fw[2] = (ldd(s[2], nb) - c2) / dy;
+            const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) af = A_self;
             else {
                 const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -122,14 +140,14 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
     }
     // z faces: zero slope at surfaces for both kinds
     {
-        int64_t nb;
+        I nb;
         if ((own >> (surf_shift + kZB)) & 1u) {
             bw[0] = 0.; bw[1] = 0.; bw[2] = 0.;
             ab = A;
         } else {
             nb = (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1);
-            bw[0] = (c0 - s[0][nb]) / dz; bw[1] = (c1 - s[1][nb]) / dz; bw[2] = (c2 - s[2][nb]) / dz;
-            const uint32_t nm = code[nb] & kMatMask;
+            bw[0] = (c0 - ldd(s[0], nb)) / dz; bw[1] = (c1 - ldd(s[1], nb)) / dz; bw[2] = (c2 - ldd(s[2], nb)) / dz;
+            const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) ab = A_self;
             else {
                 const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -141,8 +159,8 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             af = A;
         } else {
             nb = (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1);
-            fw[0] = (s[0][nb] - c0) / dz; fw[1] = (s[1][nb] - c1) / dz; fw[2] = (s[2][nb] - c2) / dz;
-            const uint32_t nm = code[nb] & kMatMask;
+            fw[0] = (ldd(s[0], nb) - c0) / dz; fw[1] = (ldd(s[1], nb) - c1) / dz; fw[2] = (ldd(s[2], nb) - c2) / dz;
+            const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) af = A_self;
             else {
                 const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -158,11 +176,12 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
 // f (B1 e_aa m_a + B2 (e_ab m_b + e_ac m_c)) with f = -2 (FM) or -1 (per
 // AFM sublattice), B already divided by mu0 Ms (magnetic_solver.cpp:143-150,
 // magnetic_AFM_solver.cpp:156-162).  Strain is Voigt xx yy zz yz xz xy.
-__device__ __forceinline__ void magnetoelastic(const double* const* e, int64_t id, double f, double B1, double B2,
+template <class I>
+__device__ __forceinline__ void magnetoelastic(const double* const* e, I id, double f, double B1, double B2,
                                                double mx, double my, double mz, int rotate, const double T[3][3],
                                                double& h0, double& h1, double& h2) {
-    const double exx = 0. + e[0][id], eyy = 0. + e[1][id], ezz = 0. + e[2][id];
-    const double eyz = 0. + e[3][id], exz = 0. + e[4][id], exy = 0. + e[5][id];
+    const double exx = 0. + ldd(e[0], id), eyy = 0. + ldd(e[1], id), ezz = 0. + ldd(e[2], id);
+    const double eyz = 0. + ldd(e[3], id), exz = 0. + ldd(e[4], id), exy = 0. + ldd(e[5], id);
     const double c0 = f * (B1 * (exx) * mx + B2 * ((exy) * my + (exz) * mz));
     const double c1 = f * (B1 * (eyy) * my + B2 * ((exy) * mx + (eyz) * mz));
     const double c2 = f * (B1 * (ezz) * mz + B2 * ((exz) * mx + (eyz) * my));
@@ -170,41 +189,44 @@ __device__ __forceinline__ void magnetoelastic(const double* const* e, int64_t i
     else { h0 = c0; h1 = c1; h2 = c2; }
 }
 
-// RK update of one component, see the file comment.
-__device__ __forceinline__ void rk_update(int stage, int64_t id, int a, double k, const double* const* base,
-                                          double* const* acc, double* const* out) {
-    const double m = base[a][id];
+// RK update of one component, see the file comment; returns the value
+// written to the next store (m itself at stage 4).
+template <class I>
+__device__ __forceinline__ double rk_update(int stage, I id, int a, double k, const double* const* base,
+                                            double* const* acc, double* const* out) {
+    const double m = ldd(base[a], id);
+    double v;
     if (stage == 1) {
-        out[a][id] = m + k * 0.5;
-        acc[a][id] = k / 6.;
+        v = m + k * 0.5;
+        std_(acc[a], id, k / 6.);
     } else if (stage == 2) {
-        out[a][id] = m + k * 0.5;
-        acc[a][id] = acc[a][id] + k / 3.;
+        v = m + k * 0.5;
+        std_(acc[a], id, ldd(acc[a], id) + k / 3.);
     } else if (stage == 3) {
-        out[a][id] = m + k;
-        acc[a][id] = acc[a][id] + k / 3.;
+        v = m + k;
+        std_(acc[a], id, ldd(acc[a], id) + k / 3.);
     } else {
-        out[a][id] = m + (acc[a][id] + k / 6.);
+        v = m + (ldd(acc[a], id) + k / 6.);
     }
+    std_(out[a], id, v);
+    return v;
 }
 
-__global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
-    const int64_t n = a.nx * a.ny * a.nz;
-    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const uint32_t own = a.code[id];
-    if (!(own & kIsFM)) return;
+// One FM cell of a stage: H_eff with the static field hs, the LLG
+// increment and the RK update; n receives the next store value.
+template <class I>
+__device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, I x, I y, I z,
+                                        double hs0, double hs1, double hs2, double& n0, double& n1, double& n2) {
     const DevMaterial& M = a.mats[(own & kMatMask) - 1];
-    const int64_t nyz = a.ny * a.nz;
-    const int64_t x = id / nyz, y = (id - x * nyz) / a.nz, z = id - x * nyz - y * a.nz;
 
-    const double mgx = a.store[0][id], mgy = a.store[1][id], mgz = a.store[2][id];
+    const double mgx = ldd(a.store[0], id), mgy = ldd(a.store[1], id), mgz = ldd(a.store[2], id);
     double mx = mgx, my = mgy, mz = mgz;
     if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
 
     const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
-    const Stencil st = stencil<true>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, a.nx, a.ny, a.nz, a.dx,
-                                     a.dy, a.dz, M.Aex, M.Aex_self, M.dmi_slope);
+    const Stencil st = stencil<true, I>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, static_cast<I>(a.nx),
+                                        static_cast<I>(a.ny), static_cast<I>(a.nz), a.dx, a.dy, a.dz, M.Aex,
+                                        M.Aex_self, M.dmi_slope);
 
     // magnetocrystalline anisotropy (magnetic_solver.cpp:124-141)
     double hac0 = 0., hac1 = 0., hac2 = 0.;
@@ -221,9 +243,9 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
     // H_eff (magnetic_solver.cpp:164-198); the magnetoelastic term
     // (:89-96, 143-150) enters only when a strain is coupled, since it is
     // zero otherwise.
-    double H0 = a.hstat[0][id] + ha0;
-    double H1 = a.hstat[1][id] + ha1;
-    double H2 = a.hstat[2][id] + ha2;
+    double H0 = hs0 + ha0;
+    double H1 = hs1 + ha1;
+    double H2 = hs2 + ha2;
     if (a.strain[0]) {
         double he0, he1, he2;
         magnetoelastic(a.strain, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
@@ -235,15 +257,15 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
     H1 = H1 + M.exch * st.ex[1] + M.dmi_pos * st.d[3] + a.hext[1];
     H2 = H2 + M.exch * st.ex[2] + M.dmi_neg * (st.d[0] + st.d[1]) + a.hext[2];
     if (a.hnu[0]) {
-        H0 = H0 + a.hnu[0][id];
-        H1 = H1 + a.hnu[1][id];
-        H2 = H2 + a.hnu[2][id];
+        H0 = H0 + ldd(a.hnu[0], id);
+        H1 = H1 + ldd(a.hnu[1], id);
+        H2 = H2 + ldd(a.hnu[2], id);
     }
     if (a.hcell[0]) {
         // electrodynamic back-action (magnetic_solver.cpp:200-204)
-        H0 = H0 + a.hcell[0][id];
-        H1 = H1 + a.hcell[1][id];
-        H2 = H2 + a.hcell[2][id];
+        H0 = H0 + ldd(a.hcell[0], id);
+        H1 = H1 + ldd(a.hcell[1], id);
+        H2 = H2 + ldd(a.hcell[2], id);
     }
 
     // LLG right-hand side times dt (magnetic_solver.cpp:206-220)
@@ -253,17 +275,135 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
     const double k1 = M.precess * py + M.alpha_precess * qy;
     const double k2 = M.precess * pz + M.alpha_precess * qz;
     if (a.dm_out[0]) {
-        a.dm_out[0][id] = k0; a.dm_out[1][id] = k1; a.dm_out[2][id] = k2;
+        std_(a.dm_out[0], id, k0); std_(a.dm_out[1], id, k1); std_(a.dm_out[2], id, k2);
     }
-    rk_update(a.stage, id, 0, k0, a.base, a.acc, a.out);
-    rk_update(a.stage, id, 1, k1, a.base, a.acc, a.out);
-    rk_update(a.stage, id, 2, k2, a.base, a.acc, a.out);
+    n0 = rk_update(a.stage, id, 0, k0, a.base, a.acc, a.out);
+    n1 = rk_update(a.stage, id, 1, k1, a.base, a.acc, a.out);
+    n2 = rk_update(a.stage, id, 2, k2, a.base, a.acc, a.out);
 }
 
-__global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
-    const int64_t n = a.nx * a.ny * a.nz;
-    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// One FM cell of the unfused stage: coordinates from the linear index,
+// magnetostatic field from Hstat.
+template <class I>
+__device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
+    const uint32_t own = ldu(a.code, id);
+    if (!(own & kIsFM)) return;
+    const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
+    const I col = id / nz, z = id - col * nz;
+    const I x = col / ny, y = col - x * ny;
+    double n0, n1, n2;
+    fm_cell<I>(a, own, id, x, y, z, ldd(a.hstat[0], id), ldd(a.hstat[1], id), ldd(a.hstat[2], id), n0, n1, n2);
+}
+
+template <class I>
+__global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
+    const I n = static_cast<I>(a.nx * a.ny * a.nz);
+    const I id = static_cast<I>(blockIdx.x) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
     if (id >= n) return;
+    fm_point<I>(a, id);
+}
+
+// The FM stage fused with the z transforms of the FFT magnetostatics, for
+// contexts whose magnetisation is FM only.  A workgroup owns P whole z
+// columns (fixed x, consecutive y): it rebuilds their static field from
+// the H z-spectrum (inverse real transform in LDS, two real columns per
+// complex transform, the imaginary parts of the self-conjugate bins
+// ignored as a complex-to-real transform does), runs the stage on every
+// cell, and forward-transforms Ms m of the new store into the M z-spectrum.
+// Both spectra use the layout [x][kz][y] of the separate z passes
+// (gf_fft.hip), so the y and x passes are unchanged.  Workgroups are
+// ordered so that each XCD takes a contiguous range of x planes, keeping
+// stencil neighbours in its L2.
+template <bool kHspec, class I>
+__global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, FusedZArgs f) {
+    extern __shared__ double2 fsm[];
+    const int nz = static_cast<int>(a.nz), ny = static_cast<int>(a.ny);
+    const int P = f.P, lognz = f.lognz, nzh = f.nzh;
+    const int pairs = P > 1 ? P >> 1 : 1;
+    double2* tw = fsm;
+    double2* cbuf = tw + (nz >> 1);
+    double* hbuf = reinterpret_cast<double*>(cbuf + pairs * nz);
+    const unsigned b = blockIdx.x, G = gridDim.x;
+    const unsigned q = G / 8u, r = G % 8u, xcd = b % 8u, slot = b / 8u;
+    const unsigned L = xcd * q + (xcd < r ? xcd : r) + slot;
+    const int gpx = ny / P;
+    const int x = static_cast<int>(L) / gpx, y0 = (static_cast<int>(L) % gpx) * P;
+    const int cells = P * nz;
+    fft_load_twiddles(tw, f.tw, nz);
+    if (kHspec) {
+        __syncthreads();
+        for (int c = 0; c < 3; c++) {
+            const double2* in = f.hspec + c * f.nh;
+            for (int t = threadIdx.x; t < pairs * nzh; t += blockDim.x) {
+                const int p = t % pairs, kz = t / pairs;
+                const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
+                double2 A = in[row + p];
+                double2 B = P > 1 ? in[row + p + pairs] : make_double2(0., 0.);
+                if (kz == 0 || 2 * kz == nz) { A.y = 0.; B.y = 0.; }
+                cbuf[p * nz + kz] = make_double2(A.x - B.y, A.y + B.x);
+                if (kz > 0 && 2 * kz < nz) cbuf[p * nz + nz - kz] = make_double2(A.x + B.y, B.x - A.y);
+            }
+            __syncthreads();
+            ifft_dif(cbuf, pairs, lognz, tw);
+            for (int t = threadIdx.x; t < pairs * nz; t += blockDim.x) {
+                const int p = t >> lognz, k = t & (nz - 1);
+                const double2 z = cbuf[p * nz + fft_bitrev(k, lognz)];
+                hbuf[(c * P + p) * nz + k] = z.x;
+                if (P > 1) hbuf[(c * P + p + pairs) * nz + k] = z.y;
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int t = threadIdx.x; t < 3 * cells; t += blockDim.x) {
+            const int c = t / cells, w = t - c * cells;
+            const int col = w >> lognz, k = w & (nz - 1);
+            hbuf[t] = a.hstat[c][(static_cast<int64_t>(x) * ny + y0 + col) * nz + k];
+        }
+        __syncthreads();
+    }
+    for (int w = threadIdx.x; w < cells; w += blockDim.x) {
+        const int col = w >> lognz, k = w & (nz - 1);
+        const int y = y0 + col;
+        const I id = (static_cast<I>(x) * static_cast<I>(ny) + static_cast<I>(y)) * static_cast<I>(nz) + static_cast<I>(k);
+        const uint32_t own = ldu(a.code, id);
+        double mx = 0., my = 0., mz = 0.;
+        if (own & kIsFM) {
+            double n0, n1, n2;
+            fm_cell<I>(a, own, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), hbuf[w], hbuf[cells + w],
+                       hbuf[2 * cells + w], n0, n1, n2);
+            const double Ms = a.mats[(own & kMatMask) - 1].Ms;
+            mx = Ms * n0; my = Ms * n1; mz = Ms * n2;
+        }
+        hbuf[w] = mx; hbuf[cells + w] = my; hbuf[2 * cells + w] = mz;
+    }
+    __syncthreads();
+    for (int c = 0; c < 3; c++) {
+        for (int t = threadIdx.x; t < pairs * nz; t += blockDim.x) {
+            const int p = t >> lognz, k = t & (nz - 1);
+            const double re = hbuf[(c * P + p) * nz + k];
+            const double im = P > 1 ? hbuf[(c * P + p + pairs) * nz + k] : 0.;
+            cbuf[p * nz + fft_bitrev(k, lognz)] = make_double2(re, im);
+        }
+        __syncthreads();
+        fft_dit(cbuf, pairs, lognz, tw);
+        double2* out = f.mspec + c * f.nh;
+        for (int t = threadIdx.x; t < pairs * nzh; t += blockDim.x) {
+            const int p = t % pairs, kz = t / pairs;
+            const double2 Z = cbuf[p * nz + kz];
+            const double2 Zm = cbuf[p * nz + ((nz - kz) & (nz - 1))];
+            const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
+            if (P > 1) {
+                out[row + p] = make_double2((Z.x + Zm.x) * 0.5, (Z.y - Zm.y) * 0.5);
+                out[row + p + pairs] = make_double2((Z.y + Zm.y) * 0.5, (Zm.x - Z.x) * 0.5);
+            } else {
+                out[row + p] = Z;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void afm_point(const AfmStageArgs& a, int64_t id) {
     const uint32_t own = a.code[id];
     if (!(own & kIsAFM)) return;
     const DevMaterial& M = a.mats[(own & kMatMask) - 1];
@@ -276,8 +416,8 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
         const double mgx = st.p[0][id], mgy = st.p[1][id], mgz = st.p[2][id];
         double mx = mgx, my = mgy, mz = mgz;
         if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
-        const Stencil s = stencil<false>(st, a.code, a.mats, own, kAfmSurfShift, sub, x, y, z, a.nx, a.ny, a.nz,
-                                         a.dx, a.dy, a.dz, M.aAex[sub], M.aAex_self[sub], 0.);
+        const Stencil s = stencil<false, int64_t>(st, a.code, a.mats, own, kAfmSurfShift, sub, x, y, z, a.nx, a.ny,
+                                                  a.nz, a.dx, a.dy, a.dz, M.aAex[sub], M.aAex_self[sub], 0.);
         // uniaxial anisotropy 2K1 (u.m) u (magnetic_AFM_solver.cpp:148-154)
         const double udotm = mx * M.u[0] + my * M.u[1] + mz * M.u[2];
         const double hac0 = 2. * M.aK1n[sub] * udotm * M.u[0];
@@ -324,6 +464,66 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
     }
 }
 
+__global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
+    const int64_t n = a.nx * a.ny * a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    afm_point(a, id);
+}
+
+// Batched stages: one launch covers every instance of a batch.  The stage
+// arguments of instance i and stage s live in device memory at
+// items[4*i + s - 1]; workgroup slot b (of `total` slots) serves instance
+// block_item[b], cells (b - block0[instance]) * blockDim.x + threadIdx.x.
+// A grid smaller than `total` walks the slots with a stride of gridDim.x.
+template <class I>
+__global__ void __launch_bounds__(256) mag_batch_kernel(const StageArgs* __restrict__ items,
+                                                         const uint32_t* __restrict__ block_item,
+                                                         const uint32_t* __restrict__ block0, int stage,
+                                                         uint32_t total) {
+    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const uint32_t inst = block_item[b];
+        const StageArgs& a = items[4 * inst + stage - 1];
+        const I n = static_cast<I>(a.nx * a.ny * a.nz);
+        const I id = static_cast<I>(b - block0[inst]) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
+        if (id < n) fm_point<I>(a, id);
+    }
+}
+
+__global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __restrict__ items,
+                                                         const uint32_t* __restrict__ block_item,
+                                                         const uint32_t* __restrict__ block0, int stage,
+                                                         uint32_t total) {
+    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+        const uint32_t inst = block_item[b];
+        const AfmStageArgs& a = items[4 * inst + stage - 1];
+        const int64_t n = a.nx * a.ny * a.nz;
+        const int64_t id = static_cast<int64_t>(b - block0[inst]) * blockDim.x + threadIdx.x;
+        if (id < n) afm_point(a, id);
+    }
+}
+
+// Per-instance clock of a batch: t += dt (global.cpp:513-549), then the
+// sinusoidal drive at each stage time t - dt, t - dt/2, t - dt/2, t
+// (magnetic_solver.cpp:1834-1861, magnetic_AFM_solver.cpp:61-64) written
+// into the stage arguments of both magnet kinds.
+__global__ void __launch_bounds__(256) batch_clock_kernel(BatchClock* clk, StageArgs* fm, AfmStageArgs* afm,
+                                                           int count) {
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= count) return;
+    const BatchClock c = clk[i];
+    const double t = c.t + c.dt;
+    clk[i].t = t;
+    for (int s = 1; s <= 4; s++) {
+        const double ts = s == 1 ? t - c.dt : (s == 4 ? t : t - c.dt * 0.5);
+        for (int k = 0; k < 3; k++) {
+            const double h = c.stat[k] + c.altr[k] * sin(2. * kPi * c.freq[k] * ts);
+            fm[4 * i + s - 1].hext[k] = h;
+            afm[4 * i + s - 1].hext[k] = h;
+        }
+    }
+}
+
 // Per-block partial sums of three fields; block b writes partial[3*b + c].
 __global__ void __launch_bounds__(256) sum3_kernel(const double* f0, const double* f1, const double* f2, int64_t n,
                                                     double* partial) {
@@ -352,15 +552,55 @@ __global__ void __launch_bounds__(256) sum3_kernel(const double* f0, const doubl
 
 }  // namespace
 
+// 32-bit element offsets when every cell array holds fewer than 2^28 doubles.
+bool narrow(int64_t n) { return n < (int64_t{1} << 28); }
+
 hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s) {
     const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(mag_stage_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    if (narrow(n)) hipLaunchKernelGGL(mag_stage_kernel<uint32_t>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(mag_stage_kernel<int64_t>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool hspec, hipStream_t s) {
+    const int nz = static_cast<int>(a.nz);
+    const int pairs = f.P > 1 ? f.P / 2 : 1;
+    const size_t lds = sizeof(double2) * (nz / 2 + pairs * nz) + sizeof(double) * 3 * f.P * nz;
+    const unsigned groups = static_cast<unsigned>(a.nx * a.ny / f.P);
+    const bool n32 = narrow(a.nx * a.ny * a.nz);
+    if (hspec && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, uint32_t>), dim3(groups), dim3(256), lds, s, a, f);
+    else if (hspec) hipLaunchKernelGGL((mag_stage_fused_kernel<true, int64_t>), dim3(groups), dim3(256), lds, s, a, f);
+    else if (n32) hipLaunchKernelGGL((mag_stage_fused_kernel<false, uint32_t>), dim3(groups), dim3(256), lds, s, a, f);
+    else hipLaunchKernelGGL((mag_stage_fused_kernel<false, int64_t>), dim3(groups), dim3(256), lds, s, a, f);
     return hipGetLastError();
 }
 
 hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s) {
     const int64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(afm_stage_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mag_batch(const StageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
+                            uint32_t total, unsigned grid, unsigned threads, bool narrow_ids, hipStream_t s) {
+    if (narrow_ids)
+        hipLaunchKernelGGL(mag_batch_kernel<uint32_t>, dim3(grid), dim3(threads), 0, s, items, block_item, block0, stage,
+                           total);
+    else
+        hipLaunchKernelGGL(mag_batch_kernel<int64_t>, dim3(grid), dim3(threads), 0, s, items, block_item, block0, stage,
+                           total);
+    return hipGetLastError();
+}
+
+hipError_t launch_afm_batch(const AfmStageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
+                            uint32_t total, unsigned grid, unsigned threads, hipStream_t s) {
+    hipLaunchKernelGGL(afm_batch_kernel, dim3(grid), dim3(threads), 0, s, items, block_item, block0, stage, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_clock(BatchClock* clk, StageArgs* fm, AfmStageArgs* afm, int count, hipStream_t s) {
+    hipLaunchKernelGGL(batch_clock_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s, clk, fm,
+                       afm, count);
     return hipGetLastError();
 }
 

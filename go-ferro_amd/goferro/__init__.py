@@ -30,7 +30,7 @@ __all__ = [
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
     "field_size", "yee_shapes", "split_yee", "join_yee",
-    "KERNEL_CLASSES", "EXPORTED_SYMBOLS",
+    "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "libgoferro.so"
@@ -108,6 +108,8 @@ EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
     "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_em_stage", "gf_field_size", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
     "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times",
+    "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_sync",
+    "gf_batch_get_stream", "gf_batch_config", "gf_batch_set_config", "gf_batch_tune",
 ]
 
 
@@ -336,6 +338,15 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_get_stream": ([P, ctypes.POINTER(P)], ctypes.c_int),
         "gf_set_kernel_timing": ([P, ctypes.c_int], ctypes.c_int),
         "gf_kernel_times": ([P, dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gf_batch_create": ([ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)], ctypes.c_int),
+        "gf_batch_destroy": ([P], ctypes.c_int),
+        "gf_batch_last_error": ([P], ctypes.c_char_p),
+        "gf_batch_step": ([P, ctypes.c_int64], ctypes.c_int),
+        "gf_batch_sync": ([P], ctypes.c_int),
+        "gf_batch_get_stream": ([P, ctypes.POINTER(P)], ctypes.c_int),
+        "gf_batch_config": ([P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "gf_batch_set_config": ([P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "gf_batch_tune": ([P, ctypes.c_int, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -461,3 +472,61 @@ class Context:
         cnt = (ctypes.c_int64 * len(KERNEL_CLASSES))()
         self._check(self.lib.gf_kernel_times(self._h, ms, cnt), "gf_kernel_times")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+class Batch:
+    """Independent contexts stepped together (gf_batch): one launch per RK4
+    stage covers every instance and each step replays a HIP graph.  The
+    contexts are borrowed and stay usable between batch steps."""
+
+    def __init__(self, contexts):
+        self.lib = load_library()
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c._h.value for c in self.contexts])
+        h = ctypes.c_void_p()
+        st = self.lib.gf_batch_create(arr, len(self.contexts), ctypes.byref(h))
+        if st != 0:
+            raise GoFerroError(f"gf_batch_create failed ({st}): {self.lib.gf_batch_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, st: int, what: str) -> None:
+        if st != 0:
+            raise GoFerroError(f"{what} failed ({st}): {self.lib.gf_batch_last_error(self._h).decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.gf_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, nsteps: int = 1) -> None:
+        self._check(self.lib.gf_batch_step(self._h, int(nsteps)), "gf_batch_step")
+
+    def sync(self) -> None:
+        self._check(self.lib.gf_batch_sync(self._h), "gf_batch_sync")
+
+    @property
+    def config(self) -> tuple:
+        t, g = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.gf_batch_config(self._h, ctypes.byref(t), ctypes.byref(g)), "gf_batch_config")
+        return t.value, g.value
+
+    def set_config(self, threads: int, grid_cap: int = 0) -> None:
+        self._check(self.lib.gf_batch_set_config(self._h, int(threads), int(grid_cap)), "gf_batch_set_config")
+
+    def tune(self, reps: int = 3) -> float:
+        """Pick the fastest launch shape; returns its ms per stage."""
+        ms = ctypes.c_double()
+        self._check(self.lib.gf_batch_tune(self._h, int(reps), ctypes.byref(ms)), "gf_batch_tune")
+        return ms.value

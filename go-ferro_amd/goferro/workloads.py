@@ -142,3 +142,23 @@ def afm_block(shape=(32, 32, 16), dt=1e-15, seed=5000, alpha=1e-3, drive_T=0.05,
     m1 = tilted_m(shape, seed)
     m2 = -tilted_m(shape, seed + 100000)
     return prob, m1, m2
+
+
+SWEEP_SIZES = ((16, 16, 16), (32, 32, 16), (32, 32, 32), (64, 64, 16), (64, 64, 32))
+
+
+def afm_sweep(count=256, seed=5, sizes=SWEEP_SIZES, dt=1e-15):
+    """Batched-sweep instances: NiO-like two-sublattice antiferromagnets
+    (afm_block) with a grid drawn per instance from `sizes`, damping
+    log-uniform in [1e-4, 1e-2], AC drive along x with amplitude U[0.01, 0.1] T
+    and frequency U[0.5, 2] THz, all drawn from one seed; instance i starts
+    from seeds 5000 + i.  Returns a list of (Problem, m1, m2)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        shape = sizes[int(rng.integers(len(sizes)))]
+        alpha = float(10.0 ** rng.uniform(-4.0, -2.0))
+        amp = float(rng.uniform(0.01, 0.1))
+        freq = float(rng.uniform(0.5e12, 2e12))
+        out.append(afm_block(shape, dt=dt, seed=5000 + i, alpha=alpha, drive_T=amp, freq=freq))
+    return out

@@ -304,6 +304,52 @@ enum gf_kernel_class {
 gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
 gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]);
 
+/* ---- Batched sweeps ---------------------------------------------------
+ * No counterpart in the reference, which runs one simulation per process.
+ * A batch steps many independent contexts of uneven grid sizes together:
+ * one launch per RK4 stage and magnet kind covers every instance, and a
+ * whole RK4 step (per-instance clock and drive, four stages) is replayed
+ * from a HIP graph.  Contexts keep their own state and stay usable through
+ * the single-context calls between batch steps (gf_download, gf_average,
+ * gf_step).  Batchable contexts hold FM and/or AFM material only, with
+ * sinusoidal or no drive, optional Hext_nonunif and rotation, and no
+ * magnetostatics, 1D model, drive tables or coupled physics; their Hstat
+ * stays as uploaded. */
+typedef struct gf_batch gf_batch;
+
+/* Build a batch over count contexts created on the current device.  The
+ * contexts are borrowed: destroy the batch before them. */
+gf_status gf_batch_create(gf_ctx* const* ctxs, int count, gf_batch** out);
+gf_status gf_batch_destroy(gf_batch* batch);
+
+/* Last error of a batch (or of the last failed gf_batch_create when batch
+ * is NULL).  Never NULL. */
+const char* gf_batch_last_error(const gf_batch* batch);
+
+/* nsteps RK4 steps of every instance (gf_time_advance and the four magnet
+ * stages, per instance).  Ordered after work queued on every context
+ * stream; every context stream is ordered after it.  Returns
+ * asynchronously. */
+gf_status gf_batch_step(gf_batch* batch, int64_t nsteps);
+
+/* Wait for the batch's queued work. */
+gf_status gf_batch_sync(gf_batch* batch);
+
+/* The batch's HIP stream (a hipStream_t). */
+gf_status gf_batch_get_stream(gf_batch* batch, void** stream);
+
+/* Launch shape of the batched stage kernels: workgroup size (64, 128 or
+ * 256) and grid cap (0 = one workgroup per 'threads' cells of each
+ * instance; otherwise at most grid_cap workgroups, each looping over the
+ * workgroup slots).  Changing it re-captures the step graph. */
+gf_status gf_batch_config(const gf_batch* batch, int* threads, int* grid_cap);
+gf_status gf_batch_set_config(gf_batch* batch, int threads, int grid_cap);
+
+/* Autotuner: times stage 1 of every (workgroup size, grid cap) candidate
+ * reps times on the live instances (leaving the state as it was) and keeps
+ * the fastest.  best_ms (may be NULL) receives its time per stage. */
+gf_status gf_batch_tune(gf_batch* batch, int reps, double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif
