@@ -37,7 +37,7 @@ struct gf_batch {
     BatchClock* d_clk = nullptr;
     std::vector<BatchClock> clk;
 
-    bool any_fm = false, any_afm = false, fm_narrow = true;
+    bool any_fm = false, any_afm = false, fm_narrow = true, afm_narrow = true;
     int64_t fm_cells = 0, afm_cells = 0;
     uint32_t* d_fm_item = nullptr;   // workgroup slot -> instance
     uint32_t* d_fm_block0 = nullptr; // instance -> first slot
@@ -46,6 +46,7 @@ struct gf_batch {
     uint32_t fm_total = 0, afm_total = 0;
 
     int threads = 256;               // workgroup size of the batched stage launches
+    int cpt = 1;                     // cells per thread of a workgroup slot
     unsigned grid_cap = 0;           // 0: one workgroup per slot; else at most this many, striding
     hipGraphExec_t graph1 = nullptr; // one RK4 step
     hipGraphExec_t graphN = nullptr; // kGraphSteps RK4 steps
@@ -91,14 +92,14 @@ gf_status dev_copy(gf_batch* b, T** dst, const std::vector<T>& src, const char* 
     return bhip(b, hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice), what);
 }
 
-// Workgroup slot tables of one magnet kind for a workgroup size.
-gf_status build_tables(gf_batch* b, bool fm, int threads) {
+// Workgroup slot tables of one magnet kind for a slot of `chunk` cells.
+gf_status build_tables(gf_batch* b, bool fm, int64_t chunk) {
     std::vector<uint32_t> item, block0(b->ctxs.size(), 0);
     for (size_t i = 0; i < b->ctxs.size(); i++) {
         const gf_ctx* c = b->ctxs[i];
         if (fm ? !c->any_fm : !c->any_afm) continue;
         block0[i] = static_cast<uint32_t>(item.size());
-        const int64_t blocks = (c->n + threads - 1) / threads;
+        const int64_t blocks = (c->n + chunk - 1) / chunk;
         if (static_cast<int64_t>(item.size()) + blocks > int64_t{0xffffffff})
             return berr(b, GF_ERR_UNSUPPORTED, "batch has more than 2^32 workgroups");
         item.insert(item.end(), static_cast<size_t>(blocks), static_cast<uint32_t>(i));
@@ -123,13 +124,16 @@ unsigned grid_for(uint32_t total, unsigned cap) {
 
 hipError_t launch_stage(gf_batch* b, int stage) {
     hipError_t e = hipSuccess;
-    if (b->any_fm && b->fm_total)
-        e = launch_mag_batch(b->d_fm, b->d_fm_item, b->d_fm_block0, stage, b->fm_total,
-                             grid_for(b->fm_total, b->grid_cap), static_cast<unsigned>(b->threads), b->fm_narrow,
-                             b->stream);
-    if (e == hipSuccess && b->any_afm && b->afm_total)
-        e = launch_afm_batch(b->d_afm, b->d_afm_item, b->d_afm_block0, stage, b->afm_total,
-                             grid_for(b->afm_total, b->grid_cap), static_cast<unsigned>(b->threads), b->stream);
+    if (b->any_fm && b->fm_total) {
+        const BatchLaunch L{b->d_fm_item, b->d_fm_block0, b->fm_total, grid_for(b->fm_total, b->grid_cap),
+                            static_cast<unsigned>(b->threads), b->cpt, b->fm_narrow};
+        e = launch_mag_batch(b->d_fm, L, stage, b->stream);
+    }
+    if (e == hipSuccess && b->any_afm && b->afm_total) {
+        const BatchLaunch L{b->d_afm_item, b->d_afm_block0, b->afm_total, grid_for(b->afm_total, b->grid_cap),
+                            static_cast<unsigned>(b->threads), b->cpt, b->afm_narrow};
+        e = launch_afm_batch(b->d_afm, L, stage, b->stream);
+    }
     return e;
 }
 
@@ -165,8 +169,9 @@ gf_status capture(gf_batch* b, int steps, hipGraphExec_t* out) {
 gf_status rebuild(gf_batch* b) {
     drop_graphs(b);
     gf_status st;
-    if (b->any_fm && (st = build_tables(b, true, b->threads))) return st;
-    if (b->any_afm && (st = build_tables(b, false, b->threads))) return st;
+    const int64_t chunk = int64_t{b->threads} * b->cpt;
+    if (b->any_fm && (st = build_tables(b, true, chunk))) return st;
+    if (b->any_afm && (st = build_tables(b, false, chunk))) return st;
     if ((st = capture(b, 1, &b->graph1))) return st;
     return capture(b, kGraphSteps, &b->graphN);
 }
@@ -247,6 +252,7 @@ gf_status gf_batch_create(gf_ctx* const* ctxs, int count, gf_batch** out) {
         if (c->any_afm) {
             b->any_afm = true;
             b->afm_cells += c->n;
+            if (!narrow(c->n)) b->afm_narrow = false;
         }
     }
     b->clk.resize(static_cast<size_t>(count));
@@ -321,21 +327,25 @@ gf_status gf_batch_get_stream(gf_batch* b, void** stream) {
     return GF_OK;
 }
 
-gf_status gf_batch_config(const gf_batch* b, int* threads, int* grid_cap) {
-    if (!b || !threads || !grid_cap) return GF_ERR_ARG;
+gf_status gf_batch_config(const gf_batch* b, int* threads, int* cells_per_thread, int* grid_cap) {
+    if (!b || !threads || !cells_per_thread || !grid_cap) return GF_ERR_ARG;
     *threads = b->threads;
+    *cells_per_thread = b->cpt;
     *grid_cap = static_cast<int>(b->grid_cap);
     return GF_OK;
 }
 
-gf_status gf_batch_set_config(gf_batch* b, int threads, int grid_cap) {
+gf_status gf_batch_set_config(gf_batch* b, int threads, int cells_per_thread, int grid_cap) {
     if (!b) return GF_ERR_ARG;
     if (threads != 64 && threads != 128 && threads != 256)
         return berr(b, GF_ERR_ARG, "threads must be 64, 128 or 256");
+    if (cells_per_thread < 1 || cells_per_thread > 16)
+        return berr(b, GF_ERR_ARG, "cells_per_thread must be in 1..16");
     if (grid_cap < 0) return berr(b, GF_ERR_ARG, "grid_cap must be >= 0");
     gf_status st = bhip(b, hipStreamSynchronize(b->stream), "batch sync");
     if (st) return st;
     b->threads = threads;
+    b->cpt = cells_per_thread;
     b->grid_cap = static_cast<unsigned>(grid_cap);
     return rebuild(b);
 }
@@ -354,18 +364,21 @@ gf_status gf_batch_tune(gf_batch* b, int reps, double* best_ms) {
     if ((st = bhip(b, hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, b->device), "CU count")))
         return st;
     const int threads[3] = {64, 128, 256};
-    const unsigned caps[3] = {0u, static_cast<unsigned>(8 * cu), static_cast<unsigned>(32 * cu)};
+    const int cpts[4] = {1, 2, 4, 8};
+    const unsigned caps[2] = {0u, static_cast<unsigned>(16 * cu)};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if ((st = bhip(b, hipEventCreate(&e0), "tune event"))) return st;
     if ((st = bhip(b, hipEventCreate(&e1), "tune event"))) { hipEventDestroy(e0); return st; }
     double best = -1.;
-    int best_t = b->threads;
+    int best_t = b->threads, best_p = b->cpt;
     unsigned best_c = b->grid_cap;
-    for (int ti = 0; ti < 3 && !st; ti++) {
-        b->threads = threads[ti];
-        if (b->any_fm && (st = build_tables(b, true, b->threads))) break;
-        if (b->any_afm && (st = build_tables(b, false, b->threads))) break;
-        for (int ci = 0; ci < 3 && !st; ci++) {
+    for (int k = 0; k < 12 && !st; k++) {
+        b->threads = threads[k / 4];
+        b->cpt = cpts[k % 4];
+        const int64_t chunk = int64_t{b->threads} * b->cpt;
+        if (b->any_fm && (st = build_tables(b, true, chunk))) break;
+        if (b->any_afm && (st = build_tables(b, false, chunk))) break;
+        for (int ci = 0; ci < 2 && !st; ci++) {
             b->grid_cap = caps[ci];
             st = bhip(b, launch_stage(b, 1), "tune launch");
             if (!st) st = bhip(b, hipEventRecord(e0, b->stream), "tune event");
@@ -374,12 +387,15 @@ gf_status gf_batch_tune(gf_batch* b, int reps, double* best_ms) {
             if (!st) st = bhip(b, hipEventSynchronize(e1), "tune sync");
             float ms = 0.f;
             if (!st) st = bhip(b, hipEventElapsedTime(&ms, e0, e1), "tune time");
-            if (!st && (best < 0. || ms < best)) { best = ms; best_t = b->threads; best_c = b->grid_cap; }
+            if (!st && (best < 0. || ms < best)) {
+                best = ms; best_t = b->threads; best_p = b->cpt; best_c = b->grid_cap;
+            }
         }
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     b->threads = best_t;
+    b->cpt = best_p;
     b->grid_cap = best_c;
     gf_status st2 = rebuild(b);
     if (st) return st;

@@ -529,12 +529,16 @@ gf_status demag_setup(gf_ctx* c) {
     // columns fit the LDS budget of the fused stage (gf_llg.hip).
     d.fused = false;
     if (c->any_fm && !c->any_afm && !(c->flags & GF_FLAG_EMDYNAMIC) && c->nz >= 2) {
-        int P = 1;
+        int P = 1, pmax = 8;
+        if (const char* env = std::getenv("GOFERRO_FUSED_P")) {
+            const int v = std::atoi(env);
+            if (v >= 1 && v <= 64) pmax = v;
+        }
         auto lds = [&](int p) {
             const int pairs = p > 1 ? p / 2 : 1;
-            return sizeof(double2) * (c->nz / 2 + pairs * c->nz) + sizeof(double) * 3 * p * c->nz;
+            return sizeof(double2) * (c->nz / 2 + 3 * pairs * c->nz);
         };
-        while (P * 2 <= 8 && P * 2 <= c->ny && lds(P * 2) <= 48 * 1024) P *= 2;
+        while (P * 2 <= pmax && P * 2 <= c->ny && lds(P * 2) <= 64 * 1024) P *= 2;
         if (lds(P) <= 48 * 1024) { d.fused = true; d.P = P; }
     }
     d.h_in_spec = false;

@@ -332,6 +332,17 @@ struct AfmStageArgs {
     int32_t stage;
 };
 
+// Launch shape of a batched stage: slot -> instance table, first slot of
+// each instance, slot count, grid, workgroup size, cells per thread.
+struct BatchLaunch {
+    const uint32_t* item;
+    const uint32_t* block0;
+    uint32_t total;
+    unsigned grid, threads;
+    int cpt;
+    bool narrow;
+};
+
 // Clock and sinusoidal drive of one instance of a batch.
 struct BatchClock {
     double t, dt;
@@ -532,10 +543,8 @@ bool narrow(int64_t n);
 // launchers (gf_llg.hip)
 hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s);
 hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s);
-hipError_t launch_mag_batch(const StageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
-                            uint32_t total, unsigned grid, unsigned threads, bool narrow_ids, hipStream_t s);
-hipError_t launch_afm_batch(const AfmStageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
-                            uint32_t total, unsigned grid, unsigned threads, hipStream_t s);
+hipError_t launch_mag_batch(const StageArgs* items, const BatchLaunch& L, int stage, hipStream_t s);
+hipError_t launch_afm_batch(const AfmStageArgs* items, const BatchLaunch& L, int stage, hipStream_t s);
 hipError_t launch_batch_clock(BatchClock* clk, StageArgs* fm, AfmStageArgs* afm, int count, hipStream_t s);
 hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool hspec, hipStream_t s);
 

@@ -66,6 +66,9 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
     const double* s[3] = {s3.p[0], s3.p[1], s3.p[2]};
     const double c0 = ldd(s[0], id), c1 = ldd(s[1], id), c2 = ldd(s[2], id);
     const uint32_t mat = own & kMatMask;
+    // one reciprocal per spacing instead of a division per difference: each
+    // quotient moves by at most an ulp against the reference's c/dx
+    const double ix = 1. / dx, iy = 1. / dy, iz = 1. / dz;
     Stencil r;
     double bw[3], fw[3], ab, af;
     // x faces
@@ -77,7 +80,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             ab = A;
         } else {
             nb = ((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k;
-            bw[0] = (c0 - ldd(s[0], nb)) / dx; bw[1] = (c1 - ldd(s[1], nb)) / dx; bw[2] = (c2 - ldd(s[2], nb)) / dx;
+            bw[0] = (c0 - ldd(s[0], nb)) * ix; bw[1] = (c1 - ldd(s[1], nb)) * ix; bw[2] = (c2 - ldd(s[2], nb)) * ix;
             const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) ab = A_self;
             else {
@@ -91,7 +94,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             af = A;
         } else {
             nb = ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k;
-            fw[0] = (ldd(s[0], nb) - c0) / dx; fw[1] = (ldd(s[1], nb) - c1) / dx; fw[2] = (ldd(s[2], nb) - c2) / dx;
+            fw[0] = (ldd(s[0], nb) - c0) * ix; fw[1] = (ldd(s[1], nb) - c1) * ix; fw[2] = (ldd(s[2], nb) - c2) * ix;
             const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) af = A_self;
             else {
@@ -101,7 +104,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
         }
         r.d[0] = (bw[0] + fw[0]) * 0.5;
         r.d[2] = (bw[2] + fw[2]) * 0.5;
-        for (int a = 0; a < 3; a++) r.ex[a] = af * (fw[a] / dx) + ab * (-bw[a] / dx);
+        for (int a = 0; a < 3; a++) r.ex[a] = af * (fw[a] * ix) + ab * (-bw[a] * ix);
     }
     // y faces
     {
@@ -112,7 +115,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             ab = A;
         } else {
             nb = (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k;
-            bw[0] = (c0 - ldd(s[0], nb)) / dy; bw[1] = (c1 - ldd(s[1], nb)) / dy; bw[2] = (c2 - ldd(s[2], nb)) / dy;
+            bw[0] = (c0 - ldd(s[0], nb)) * iy; bw[1] = (c1 - ldd(s[1], nb)) * iy; bw[2] = (c2 - ldd(s[2], nb)) * iy;
             const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) ab = A_self;
             else {
@@ -126,7 +129,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             af = A;
         } else {
             nb = (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k;
-            fw[0] = (ldd(s[0], nb) - c0) / dy; fw[1] = (ldd(s[1], nb) - c1) / dy; fw[2] = (ldd(s[2], nb) - c2) / dy;
+            fw[0] = (ldd(s[0], nb) - c0) * iy; fw[1] = (ldd(s[1], nb) - c1) * iy; fw[2] = (ldd(s[2], nb) - c2) * iy;
             const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) af = A_self;
             else {
@@ -136,7 +139,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
         }
         r.d[1] = (bw[1] + fw[1]) * 0.5;
         r.d[3] = (bw[2] + fw[2]) * 0.5;
-        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] / dy) + ab * (-bw[a] / dy);
+        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] * iy) + ab * (-bw[a] * iy);
     }
     // z faces: zero slope at surfaces for both kinds
     {
@@ -146,7 +149,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             ab = A;
         } else {
             nb = (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1);
-            bw[0] = (c0 - ldd(s[0], nb)) / dz; bw[1] = (c1 - ldd(s[1], nb)) / dz; bw[2] = (c2 - ldd(s[2], nb)) / dz;
+            bw[0] = (c0 - ldd(s[0], nb)) * iz; bw[1] = (c1 - ldd(s[1], nb)) * iz; bw[2] = (c2 - ldd(s[2], nb)) * iz;
             const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) ab = A_self;
             else {
@@ -159,7 +162,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
             af = A;
         } else {
             nb = (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1);
-            fw[0] = (ldd(s[0], nb) - c0) / dz; fw[1] = (ldd(s[1], nb) - c1) / dz; fw[2] = (ldd(s[2], nb) - c2) / dz;
+            fw[0] = (ldd(s[0], nb) - c0) * iz; fw[1] = (ldd(s[1], nb) - c1) * iz; fw[2] = (ldd(s[2], nb) - c2) * iz;
             const uint32_t nm = ldu(code, nb) & kMatMask;
             if (nm == mat) af = A_self;
             else {
@@ -167,7 +170,7 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
                 af = 2. * A * a2 / (A + a2);
             }
         }
-        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] / dz) + ab * (-bw[a] / dz);
+        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] * iz) + ab * (-bw[a] * iz);
     }
     return r;
 }
@@ -310,10 +313,14 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
 // complex transform, the imaginary parts of the self-conjugate bins
 // ignored as a complex-to-real transform does), runs the stage on every
 // cell, and forward-transforms Ms m of the new store into the M z-spectrum.
-// Both spectra use the layout [x][kz][y] of the separate z passes
-// (gf_fft.hip), so the y and x passes are unchanged.  Workgroups are
-// ordered so that each XCD takes a contiguous range of x planes, keeping
-// stencil neighbours in its L2.
+// The three components are transformed together.  Column p < P/2 lives in
+// the real parts and column p + P/2 in the imaginary parts of transform p;
+// the inverse leaves sample k at bit-reversed position, which is where the
+// forward transform wants its input, so each cell's H is read and its M
+// written in the same LDS slot.  Both spectra use the layout [x][kz][y] of
+// the separate z passes (gf_fft.hip), so the y and x passes are unchanged.
+// Workgroups are ordered so that each XCD takes a contiguous range of x
+// planes, keeping stencil neighbours in its L2.
 template <bool kHspec, class I>
 __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, FusedZArgs f) {
     extern __shared__ double2 fsm[];
@@ -321,8 +328,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
     const int P = f.P, lognz = f.lognz, nzh = f.nzh;
     const int pairs = P > 1 ? P >> 1 : 1;
     double2* tw = fsm;
-    double2* cbuf = tw + (nz >> 1);
-    double* hbuf = reinterpret_cast<double*>(cbuf + pairs * nz);
+    double2* cbuf = tw + (nz >> 1);          // [3][pairs][nz]
     const unsigned b = blockIdx.x, G = gridDim.x;
     const unsigned q = G / 8u, r = G % 8u, xcd = b % 8u, slot = b / 8u;
     const unsigned L = xcd * q + (xcd < r ? xcd : r) + slot;
@@ -331,93 +337,96 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
     const int cells = P * nz;
     fft_load_twiddles(tw, f.tw, nz);
     if (kHspec) {
-        __syncthreads();
-        for (int c = 0; c < 3; c++) {
+        const int per = pairs * nzh;
+        for (int t = threadIdx.x; t < 3 * per; t += blockDim.x) {
+            const int c = t / per, u = t - c * per;
+            const int p = u % pairs, kz = u / pairs;
+            const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
             const double2* in = f.hspec + c * f.nh;
-            for (int t = threadIdx.x; t < pairs * nzh; t += blockDim.x) {
-                const int p = t % pairs, kz = t / pairs;
-                const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
-                double2 A = in[row + p];
-                double2 B = P > 1 ? in[row + p + pairs] : make_double2(0., 0.);
-                if (kz == 0 || 2 * kz == nz) { A.y = 0.; B.y = 0.; }
-                cbuf[p * nz + kz] = make_double2(A.x - B.y, A.y + B.x);
-                if (kz > 0 && 2 * kz < nz) cbuf[p * nz + nz - kz] = make_double2(A.x + B.y, B.x - A.y);
-            }
-            __syncthreads();
-            ifft_dif(cbuf, pairs, lognz, tw);
-            for (int t = threadIdx.x; t < pairs * nz; t += blockDim.x) {
-                const int p = t >> lognz, k = t & (nz - 1);
-                const double2 z = cbuf[p * nz + fft_bitrev(k, lognz)];
-                hbuf[(c * P + p) * nz + k] = z.x;
-                if (P > 1) hbuf[(c * P + p + pairs) * nz + k] = z.y;
-            }
-            __syncthreads();
+            double2 A = in[row + p];
+            double2 B = P > 1 ? in[row + p + pairs] : make_double2(0., 0.);
+            if (kz == 0 || 2 * kz == nz) { A.y = 0.; B.y = 0.; }
+            double2* cb = cbuf + (c * pairs + p) * nz;
+            cb[kz] = make_double2(A.x - B.y, A.y + B.x);
+            if (kz > 0 && 2 * kz < nz) cb[nz - kz] = make_double2(A.x + B.y, B.x - A.y);
         }
+        __syncthreads();
+        ifft_dif(cbuf, 3 * pairs, lognz, tw);
     } else {
         for (int t = threadIdx.x; t < 3 * cells; t += blockDim.x) {
             const int c = t / cells, w = t - c * cells;
             const int col = w >> lognz, k = w & (nz - 1);
-            hbuf[t] = a.hstat[c][(static_cast<int64_t>(x) * ny + y0 + col) * nz + k];
+            const double h = a.hstat[c][(static_cast<int64_t>(x) * ny + y0 + col) * nz + k];
+            const int p = col < pairs ? col : col - pairs;
+            double* slot2 = reinterpret_cast<double*>(cbuf + (c * pairs + p) * nz + fft_bitrev(k, lognz));
+            slot2[col < pairs ? 0 : 1] = h;
         }
         __syncthreads();
     }
     for (int w = threadIdx.x; w < cells; w += blockDim.x) {
         const int col = w >> lognz, k = w & (nz - 1);
         const int y = y0 + col;
+        const int p = col < pairs ? col : col - pairs;
+        const int part = col < pairs ? 0 : 1;
+        const int pos = p * nz + fft_bitrev(k, lognz);
+        double* h0 = reinterpret_cast<double*>(cbuf + pos) + part;
+        double* h1 = reinterpret_cast<double*>(cbuf + pairs * nz + pos) + part;
+        double* h2 = reinterpret_cast<double*>(cbuf + 2 * pairs * nz + pos) + part;
         const I id = (static_cast<I>(x) * static_cast<I>(ny) + static_cast<I>(y)) * static_cast<I>(nz) + static_cast<I>(k);
         const uint32_t own = ldu(a.code, id);
         double mx = 0., my = 0., mz = 0.;
         if (own & kIsFM) {
             double n0, n1, n2;
-            fm_cell<I>(a, own, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), hbuf[w], hbuf[cells + w],
-                       hbuf[2 * cells + w], n0, n1, n2);
+            fm_cell<I>(a, own, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), *h0, *h1, *h2, n0, n1,
+                       n2);
             const double Ms = a.mats[(own & kMatMask) - 1].Ms;
             mx = Ms * n0; my = Ms * n1; mz = Ms * n2;
         }
-        hbuf[w] = mx; hbuf[cells + w] = my; hbuf[2 * cells + w] = mz;
+        *h0 = mx; *h1 = my; *h2 = mz;
+        if (P == 1) {
+            // a lone column: the imaginary parts of the forward input are zero
+            h0[1] = 0.; h1[1] = 0.; h2[1] = 0.;
+        }
     }
     __syncthreads();
-    for (int c = 0; c < 3; c++) {
-        for (int t = threadIdx.x; t < pairs * nz; t += blockDim.x) {
-            const int p = t >> lognz, k = t & (nz - 1);
-            const double re = hbuf[(c * P + p) * nz + k];
-            const double im = P > 1 ? hbuf[(c * P + p + pairs) * nz + k] : 0.;
-            cbuf[p * nz + fft_bitrev(k, lognz)] = make_double2(re, im);
-        }
-        __syncthreads();
-        fft_dit(cbuf, pairs, lognz, tw);
+    fft_dit(cbuf, 3 * pairs, lognz, tw);
+    const int per = pairs * nzh;
+    for (int t = threadIdx.x; t < 3 * per; t += blockDim.x) {
+        const int c = t / per, u = t - c * per;
+        const int p = u % pairs, kz = u / pairs;
+        const double2* cb = cbuf + (c * pairs + p) * nz;
+        const double2 Z = cb[kz];
+        const double2 Zm = cb[(nz - kz) & (nz - 1)];
+        const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
         double2* out = f.mspec + c * f.nh;
-        for (int t = threadIdx.x; t < pairs * nzh; t += blockDim.x) {
-            const int p = t % pairs, kz = t / pairs;
-            const double2 Z = cbuf[p * nz + kz];
-            const double2 Zm = cbuf[p * nz + ((nz - kz) & (nz - 1))];
-            const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
-            if (P > 1) {
-                out[row + p] = make_double2((Z.x + Zm.x) * 0.5, (Z.y - Zm.y) * 0.5);
-                out[row + p + pairs] = make_double2((Z.y + Zm.y) * 0.5, (Zm.x - Z.x) * 0.5);
-            } else {
-                out[row + p] = Z;
-            }
+        if (P > 1) {
+            out[row + p] = make_double2((Z.x + Zm.x) * 0.5, (Z.y - Zm.y) * 0.5);
+            out[row + p + pairs] = make_double2((Z.y + Zm.y) * 0.5, (Zm.x - Z.x) * 0.5);
+        } else {
+            out[row + p] = Z;
         }
-        __syncthreads();
     }
 }
 
-__device__ __forceinline__ void afm_point(const AfmStageArgs& a, int64_t id) {
-    const uint32_t own = a.code[id];
+// One sublattice of one AFM cell: the two sublattices of a stage read only
+// stage-(s-1) stores, so they run as independent threads.
+template <class I>
+__device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) {
+    const uint32_t own = ldu(a.code, id);
     if (!(own & kIsAFM)) return;
     const DevMaterial& M = a.mats[(own & kMatMask) - 1];
-    const int64_t nyz = a.ny * a.nz;
-    const int64_t x = id / nyz, y = (id - x * nyz) / a.nz, z = id - x * nyz - y * a.nz;
-
-    for (int sub = 0; sub < 2; sub++) {
+    const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
+    const I col = id / nz, z = id - col * nz;
+    const I x = col / ny, y = col - x * ny;
+    {
         const P3 st = sub == 0 ? P3{{a.store1[0], a.store1[1], a.store1[2]}} : P3{{a.store2[0], a.store2[1], a.store2[2]}};
         const P3 other = sub == 0 ? P3{{a.store2[0], a.store2[1], a.store2[2]}} : P3{{a.store1[0], a.store1[1], a.store1[2]}};
-        const double mgx = st.p[0][id], mgy = st.p[1][id], mgz = st.p[2][id];
+        const double mgx = ldd(st.p[0], id), mgy = ldd(st.p[1], id), mgz = ldd(st.p[2], id);
         double mx = mgx, my = mgy, mz = mgz;
         if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
-        const Stencil s = stencil<false, int64_t>(st, a.code, a.mats, own, kAfmSurfShift, sub, x, y, z, a.nx, a.ny,
-                                                  a.nz, a.dx, a.dy, a.dz, M.aAex[sub], M.aAex_self[sub], 0.);
+        const Stencil s = stencil<false, I>(st, a.code, a.mats, own, kAfmSurfShift, sub, x, y, z,
+                                            static_cast<I>(a.nx), ny, nz, a.dx, a.dy, a.dz, M.aAex[sub],
+                                            M.aAex_self[sub], 0.);
         // uniaxial anisotropy 2K1 (u.m) u (magnetic_AFM_solver.cpp:148-154)
         const double udotm = mx * M.u[0] + my * M.u[1] + mz * M.u[2];
         const double hac0 = 2. * M.aK1n[sub] * udotm * M.u[0];
@@ -427,9 +436,9 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, int64_t id) {
         if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
         // magnetoelastic (magnetic_AFM_solver.cpp:89-96, 156-162), exchange,
         // inter-sublattice coupling and drive (:170-200)
-        double H0 = a.hstat[0][id] + ha0;
-        double H1 = a.hstat[1][id] + ha1;
-        double H2 = a.hstat[2][id] + ha2;
+        double H0 = ldd(a.hstat[0], id) + ha0;
+        double H1 = ldd(a.hstat[1], id) + ha1;
+        double H2 = ldd(a.hstat[2], id) + ha2;
         if (a.strain[0]) {
             double he0, he1, he2;
             magnetoelastic(a.strain, id, -1., M.aB1n[sub], M.aB2n[sub], mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
@@ -437,14 +446,14 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, int64_t id) {
             H1 = H1 + he1;
             H2 = H2 + he2;
         }
-        H0 = H0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * other.p[0][id] + a.hext[0];
-        H1 = H1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * other.p[1][id] + a.hext[1];
-        H2 = H2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * other.p[2][id] + a.hext[2];
+        H0 = H0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * ldd(other.p[0], id) + a.hext[0];
+        H1 = H1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * ldd(other.p[1], id) + a.hext[1];
+        H2 = H2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * ldd(other.p[2], id) + a.hext[2];
         if (a.hcell[0]) {
             // electrodynamic back-action (magnetic_AFM_solver.cpp:222-226)
-            H0 = H0 + a.hcell[0][id];
-            H1 = H1 + a.hcell[1][id];
-            H2 = H2 + a.hcell[2][id];
+            H0 = H0 + ldd(a.hcell[0], id);
+            H1 = H1 + ldd(a.hcell[1], id);
+            H2 = H2 + ldd(a.hcell[2], id);
         }
         const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
         const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
@@ -456,7 +465,7 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, int64_t id) {
         double* const* out = sub == 0 ? a.out1 : a.out2;
         double* const* dmo = sub == 0 ? a.dm_out1 : a.dm_out2;
         if (dmo[0]) {
-            dmo[0][id] = k0; dmo[1][id] = k1; dmo[2][id] = k2;
+            std_(dmo[0], id, k0); std_(dmo[1], id, k1); std_(dmo[2], id, k2);
         }
         rk_update(a.stage, id, 0, k0, base, acc, out);
         rk_update(a.stage, id, 1, k1, base, acc, out);
@@ -464,42 +473,57 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, int64_t id) {
     }
 }
 
+// Workgroup b serves cell block (b / 16) * 8 + b % 8 and sublattice
+// (b / 8) % 2: the two sublattices of a cell block land on the same XCD
+// (workgroups are dealt to the 8 XCDs round-robin) one dispatch round
+// apart, so the second finds the shared reads (code, Hstat, the other
+// sublattice's store) in that XCD's L2.
+template <class I>
 __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
-    const int64_t n = a.nx * a.ny * a.nz;
-    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const unsigned b = blockIdx.x;
+    const unsigned blk = (b >> 4) * 8u + (b & 7u);
+    const int sub = static_cast<int>((b >> 3) & 1u);
+    const I n = static_cast<I>(a.nx * a.ny * a.nz);
+    const I id = static_cast<I>(blk) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
     if (id >= n) return;
-    afm_point(a, id);
+    afm_point<I>(a, id, sub);
 }
 
 // Batched stages: one launch covers every instance of a batch.  The stage
 // arguments of instance i and stage s live in device memory at
-// items[4*i + s - 1]; workgroup slot b (of `total` slots) serves instance
-// block_item[b], cells (b - block0[instance]) * blockDim.x + threadIdx.x.
-// A grid smaller than `total` walks the slots with a stride of gridDim.x.
+// items[4*i + s - 1]; workgroup slot b (of L.total slots) serves instance
+// L.item[b], the chunk of L.cpt * blockDim.x cells starting at
+// (b - L.block0[instance]) * L.cpt * blockDim.x, one cell per thread per
+// pass.  A grid smaller than L.total walks the slots with a stride of
+// gridDim.x; for the AFM kernel blockIdx.y selects the sublattice.
 template <class I>
-__global__ void __launch_bounds__(256) mag_batch_kernel(const StageArgs* __restrict__ items,
-                                                         const uint32_t* __restrict__ block_item,
-                                                         const uint32_t* __restrict__ block0, int stage,
-                                                         uint32_t total) {
-    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
-        const uint32_t inst = block_item[b];
+__global__ void __launch_bounds__(256) mag_batch_kernel(const StageArgs* __restrict__ items, BatchLaunch L, int stage) {
+    for (uint32_t b = blockIdx.x; b < L.total; b += gridDim.x) {
+        const uint32_t inst = L.item[b];
         const StageArgs& a = items[4 * inst + stage - 1];
         const I n = static_cast<I>(a.nx * a.ny * a.nz);
-        const I id = static_cast<I>(b - block0[inst]) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
-        if (id < n) fm_point<I>(a, id);
+        const I chunk = static_cast<I>(L.cpt) * static_cast<I>(blockDim.x);
+        const I first = static_cast<I>(b - L.block0[inst]) * chunk + static_cast<I>(threadIdx.x);
+        for (int j = 0; j < L.cpt; j++) {
+            const I id = first + static_cast<I>(j) * static_cast<I>(blockDim.x);
+            if (id < n) fm_point<I>(a, id);
+        }
     }
 }
 
-__global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __restrict__ items,
-                                                         const uint32_t* __restrict__ block_item,
-                                                         const uint32_t* __restrict__ block0, int stage,
-                                                         uint32_t total) {
-    for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
-        const uint32_t inst = block_item[b];
+template <class I>
+__global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __restrict__ items, BatchLaunch L,
+                                                         int stage) {
+    for (uint32_t b = blockIdx.x; b < L.total; b += gridDim.x) {
+        const uint32_t inst = L.item[b];
         const AfmStageArgs& a = items[4 * inst + stage - 1];
-        const int64_t n = a.nx * a.ny * a.nz;
-        const int64_t id = static_cast<int64_t>(b - block0[inst]) * blockDim.x + threadIdx.x;
-        if (id < n) afm_point(a, id);
+        const I n = static_cast<I>(a.nx * a.ny * a.nz);
+        const I chunk = static_cast<I>(L.cpt) * static_cast<I>(blockDim.x);
+        const I first = static_cast<I>(b - L.block0[inst]) * chunk + static_cast<I>(threadIdx.x);
+        for (int j = 0; j < L.cpt; j++) {
+            const I id = first + static_cast<I>(j) * static_cast<I>(blockDim.x);
+            if (id < n) afm_point<I>(a, id, static_cast<int>(blockIdx.y));
+        }
     }
 }
 
@@ -565,7 +589,7 @@ hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s) {
 hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool hspec, hipStream_t s) {
     const int nz = static_cast<int>(a.nz);
     const int pairs = f.P > 1 ? f.P / 2 : 1;
-    const size_t lds = sizeof(double2) * (nz / 2 + pairs * nz) + sizeof(double) * 3 * f.P * nz;
+    const size_t lds = sizeof(double2) * (nz / 2 + 3 * pairs * nz);
     const unsigned groups = static_cast<unsigned>(a.nx * a.ny / f.P);
     const bool n32 = narrow(a.nx * a.ny * a.nz);
     if (hspec && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, uint32_t>), dim3(groups), dim3(256), lds, s, a, f);
@@ -577,24 +601,21 @@ hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool 
 
 hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s) {
     const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(afm_stage_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    const dim3 grid(static_cast<unsigned>((blocks + 7) / 8 * 16));
+    if (narrow(n)) hipLaunchKernelGGL(afm_stage_kernel<uint32_t>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(afm_stage_kernel<int64_t>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_mag_batch(const StageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
-                            uint32_t total, unsigned grid, unsigned threads, bool narrow_ids, hipStream_t s) {
-    if (narrow_ids)
-        hipLaunchKernelGGL(mag_batch_kernel<uint32_t>, dim3(grid), dim3(threads), 0, s, items, block_item, block0, stage,
-                           total);
-    else
-        hipLaunchKernelGGL(mag_batch_kernel<int64_t>, dim3(grid), dim3(threads), 0, s, items, block_item, block0, stage,
-                           total);
+hipError_t launch_mag_batch(const StageArgs* items, const BatchLaunch& L, int stage, hipStream_t s) {
+    if (L.narrow) hipLaunchKernelGGL(mag_batch_kernel<uint32_t>, dim3(L.grid), dim3(L.threads), 0, s, items, L, stage);
+    else hipLaunchKernelGGL(mag_batch_kernel<int64_t>, dim3(L.grid), dim3(L.threads), 0, s, items, L, stage);
     return hipGetLastError();
 }
 
-hipError_t launch_afm_batch(const AfmStageArgs* items, const uint32_t* block_item, const uint32_t* block0, int stage,
-                            uint32_t total, unsigned grid, unsigned threads, hipStream_t s) {
-    hipLaunchKernelGGL(afm_batch_kernel, dim3(grid), dim3(threads), 0, s, items, block_item, block0, stage, total);
+hipError_t launch_afm_batch(const AfmStageArgs* items, const BatchLaunch& L, int stage, hipStream_t s) {
+    if (L.narrow) hipLaunchKernelGGL(afm_batch_kernel<uint32_t>, dim3(L.grid, 2), dim3(L.threads), 0, s, items, L, stage);
+    else hipLaunchKernelGGL(afm_batch_kernel<int64_t>, dim3(L.grid, 2), dim3(L.threads), 0, s, items, L, stage);
     return hipGetLastError();
 }
 

@@ -344,8 +344,9 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_batch_step": ([P, ctypes.c_int64], ctypes.c_int),
         "gf_batch_sync": ([P], ctypes.c_int),
         "gf_batch_get_stream": ([P, ctypes.POINTER(P)], ctypes.c_int),
-        "gf_batch_config": ([P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
-        "gf_batch_set_config": ([P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "gf_batch_config": ([P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "gf_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gf_batch_tune": ([P, ctypes.c_int, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -518,12 +519,15 @@ class Batch:
 
     @property
     def config(self) -> tuple:
-        t, g = ctypes.c_int(), ctypes.c_int()
-        self._check(self.lib.gf_batch_config(self._h, ctypes.byref(t), ctypes.byref(g)), "gf_batch_config")
-        return t.value, g.value
+        """(workgroup size, cells per thread, grid cap)."""
+        t, p, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.gf_batch_config(self._h, ctypes.byref(t), ctypes.byref(p), ctypes.byref(g)),
+                    "gf_batch_config")
+        return t.value, p.value, g.value
 
-    def set_config(self, threads: int, grid_cap: int = 0) -> None:
-        self._check(self.lib.gf_batch_set_config(self._h, int(threads), int(grid_cap)), "gf_batch_set_config")
+    def set_config(self, threads: int, cells_per_thread: int = 1, grid_cap: int = 0) -> None:
+        self._check(self.lib.gf_batch_set_config(self._h, int(threads), int(cells_per_thread), int(grid_cap)),
+                    "gf_batch_set_config")
 
     def tune(self, reps: int = 3) -> float:
         """Pick the fastest launch shape; returns its ms per stage."""

@@ -339,15 +339,17 @@ gf_status gf_batch_sync(gf_batch* batch);
 gf_status gf_batch_get_stream(gf_batch* batch, void** stream);
 
 /* Launch shape of the batched stage kernels: workgroup size (64, 128 or
- * 256) and grid cap (0 = one workgroup per 'threads' cells of each
- * instance; otherwise at most grid_cap workgroups, each looping over the
- * workgroup slots).  Changing it re-captures the step graph. */
-gf_status gf_batch_config(const gf_batch* batch, int* threads, int* grid_cap);
-gf_status gf_batch_set_config(gf_batch* batch, int threads, int grid_cap);
+ * 256), cells per thread (1..16: a workgroup slot covers threads *
+ * cells_per_thread cells of one instance) and grid cap (0 = one workgroup
+ * per slot; otherwise at most grid_cap workgroups, each looping over the
+ * slots).  Changing it re-captures the step graph. */
+gf_status gf_batch_config(const gf_batch* batch, int* threads, int* cells_per_thread, int* grid_cap);
+gf_status gf_batch_set_config(gf_batch* batch, int threads, int cells_per_thread, int grid_cap);
 
-/* Autotuner: times stage 1 of every (workgroup size, grid cap) candidate
- * reps times on the live instances (leaving the state as it was) and keeps
- * the fastest.  best_ms (may be NULL) receives its time per stage. */
+/* Autotuner: times stage 1 of every (workgroup size, cells per thread,
+ * grid cap) candidate reps times on the live instances (leaving the state
+ * as it was) and keeps the fastest.  best_ms (may be NULL) receives its
+ * time per stage. */
 gf_status gf_batch_tune(gf_batch* batch, int reps, double* best_ms);
 
 #ifdef __cplusplus

@@ -120,14 +120,14 @@ def test_batch_equals_single_context_steps():
     batch.close()
 
 
-@pytest.mark.parametrize("threads,cap", [(64, 0), (128, 7), (256, 3)])
-def test_batch_launch_shapes(threads, cap):
-    say("case batch launch shape", threads, cap)
+@pytest.mark.parametrize("threads,cpt,cap", [(64, 1, 0), (128, 2, 7), (256, 4, 3), (64, 8, 0)])
+def test_batch_launch_shapes(threads, cpt, cap):
+    say("case batch launch shape", threads, cpt, cap)
     inst = mixed_instances()
     A, B = make(inst), make(inst)
     batch = Batch(A)
-    batch.set_config(threads, cap)
-    assert batch.config == (threads, cap)
+    batch.set_config(threads, cpt, cap)
+    assert batch.config == (threads, cpt, cap)
     batch.step(5)
     for c in B:
         c.step(5)
@@ -145,8 +145,8 @@ def test_batch_tune_leaves_state():
         c.step(2)
     ms = batch.tune(reps=2)
     assert ms > 0
-    t, cap = batch.config
-    assert t in (64, 128, 256) and cap >= 0
+    t, cpt, cap = batch.config
+    assert t in (64, 128, 256) and cpt in (1, 2, 4, 8) and cap >= 0
     batch.step(4)
     for c in B:
         c.step(4)
@@ -207,7 +207,9 @@ def test_batch_rejects_unbatchable_and_mid_step():
     with pytest.raises(GoFerroError, match="middle of a step"):
         batch.tune(1)
     with pytest.raises(GoFerroError, match="threads"):
-        batch.set_config(96, 0)
+        batch.set_config(96, 1, 0)
+    with pytest.raises(GoFerroError, match="cells_per_thread"):
+        batch.set_config(64, 0, 0)
     for s in (2, 3, 4):
         A[0].afm_stage(s)
     batch.step(1)
