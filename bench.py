@@ -161,6 +161,53 @@ def bench_batch(count: int, steps_batch: int, steps_solo: int) -> dict:
             "speedup": rate_b / rate_s}
 
 
+def bench_adjoint(shape, nsteps: int) -> dict:
+    """Adjoint sweep of the probe spectral loss (30 GHz, 16x16x4 probe) on
+    the FM film with a shaped top-layer drive and static strain: wall time
+    of the reverse sweep (segment recomputation included) against the plain
+    forward (gf_step) of the same steps, plus the checkpointed forward."""
+    import goferro
+    from goferro import workloads
+
+    prob, fields, _ = workloads.adjoint_film(shape)
+    lo, hi = workloads.adjoint_probe(shape)
+    ctx = goferro.Context(prob)
+
+    def reset():
+        for fid, v in fields.items():
+            ctx.upload(fid, v)
+        ctx.time = 0.0
+        ctx.demag()
+        ctx.sync()
+
+    reset()
+    ctx.step(2)                      # warm-up (kernels, allocations)
+    ctx.sync()
+    reset()
+    t0 = time.perf_counter()
+    ctx.step(nsteps)
+    ctx.sync()
+    t_plain = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    L = ctx.adjoint_forward(1, lo, hi, 3e10, nsteps)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad, ghd = ctx.adjoint_backward()
+    t_bwd = time.perf_counter() - t0
+    ctx.close()
+    n = shape[0] * shape[1] * shape[2]
+    print(f"bench: adjoint {shape}, {nsteps} steps: plain forward {t_plain:.3f} s, checkpointed forward "
+          f"{t_fwd:.3f} s, reverse sweep {t_bwd:.3f} s", file=sys.stderr, flush=True)
+    return {"workload": f"FM film {shape[0]}x{shape[1]}x{shape[2]} + shaped top-layer drive (30 GHz) + static strain, "
+                        f"probe 16x16x4 spectral power at 30 GHz, {nsteps} steps, sqrt(N) checkpoints",
+            "dtype": "f64", "nsteps": nsteps, "forward_s": t_plain, "checkpointed_forward_s": t_fwd,
+            "reverse_s": t_bwd, "adjoint_over_forward": t_bwd / t_plain,
+            "forward_cell_steps_per_s": n * nsteps / t_plain, "loss": L,
+            "grad": {"alpha": grad[0], "Aex": grad[1], "B1": grad[2]},
+            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5)}
+
+
 def copy_probe(nbytes: int) -> int:
     """One HIP device-to-device copy of nbytes (runtime blit kernel): the
     reference point that scales FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -191,6 +238,8 @@ def main() -> int:
                     help="instances of the batched AFM sweep measured after the main workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--solo-steps", type=int, default=10)
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="steps of the adjoint measurement on the 128x128x32 film (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -262,6 +311,8 @@ def main() -> int:
         out["copy_probe_bytes"] = copy_probe(args.copy_probe_mib << 20)
     if args.batch_instances > 0:
         out["batched_sweep"] = bench_batch(args.batch_instances, args.batch_steps, args.solo_steps)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = bench_adjoint((128, 128, 32), args.adjoint_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

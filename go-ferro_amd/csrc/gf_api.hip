@@ -504,6 +504,25 @@ gf_status check_before_em(gf_ctx* c, int stage) {
 // m and p, else none.
 bool strain_coupled(const gf_ctx* c) { return c->el_on && (c->flags & GF_FLAG_ELASTO_ON_PM); }
 
+// The magnets read a static strain (the reference's ex*t0_crt) when strain
+// acts on them and no elastodynamics supplies one.
+bool static_strain(const gf_ctx* c) {
+    return (c->flags & GF_FLAG_ELASTO_ON_PM) && !(c->flags & GF_FLAG_ELASTODYNAMICS);
+}
+
+// Strain arrays the magnet stages read (crystal frame), or nulls.
+void magnet_strain(gf_ctx* c, const double* out[6]) {
+    for (int i = 0; i < 6; i++)
+        out[i] = strain_coupled(c) ? c->ec[i] : (static_strain(c) ? c->e_t0[i] : nullptr);
+}
+
+// sin(2 pi f t) of the shaped drive at the stage time of the uniform drive
+// (magnetic_solver.cpp:1834-1861).
+double shaped_drive(const gf_ctx* c, int stage) {
+    const double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    return std::sin(2. * kPi * c->hdrive_freq * t);
+}
+
 // With the elastic solver on, each stage starts with it: its get phase
 // reads every physics' store of the stage (main.cpp:265-297).
 gf_status check_after_elastic(gf_ctx* c, int stage) {
@@ -561,6 +580,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         c->dx = d->dx; c->dy = d->dy; c->dz = d->dz;
         for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
         c->flags = d->flags; c->dt = d->dt; c->time = 0.;
+        c->hdrive_freq = d->Hdrive_freq;
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) { c->Tg2c[a][b] = d->Tg2c[a][b]; c->Tc2g[b][a] = d->Tg2c[a][b]; }
         for (int a = 0; a < 3; a++) {
@@ -692,6 +712,11 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         c->w_liao = d->weighting_factor_fourth_Liao;
         elastic_speeds(c);
     }
+    if (!st && (c->flags & GF_FLAG_HDRIVE)) st = alloc3(c, c->hdrive, "hipMalloc drive profile");
+    if (!st && static_strain(c)) {
+        st = alloc3(c, c->e_t0, "hipMalloc static strain");
+        if (!st) st = alloc3(c, c->e_t0 + 3, "hipMalloc static strain");
+    }
     if (!st && (c->flags & GF_FLAG_EMDYNAMIC)) st = em_setup(c, d);
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
@@ -718,7 +743,7 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->m); free3(c->acc); free3(c->st_a); free3(c->st_b); free3(c->dm_last);
     free3(c->m1); free3(c->m2); free3(c->acc1); free3(c->acc2);
     free3(c->st1_a); free3(c->st1_b); free3(c->st2_a); free3(c->st2_b);
-    free3(c->hstat); free3(c->hnu);
+    free3(c->hstat); free3(c->hnu); free3(c->hdrive); free3(c->e_t0); free3(c->e_t0 + 3);
     free3(c->p); free3(c->q); free3(c->pacc); free3(c->qacc);
     free3(c->p_a); free3(c->p_b); free3(c->q_a); free3(c->q_b); free3(c->estat);
     free3(c->u); free3(c->v); free3(c->us); free3(c->vs); free3(c->accu); free3(c->accv); free3(c->fstore);
@@ -746,6 +771,7 @@ gf_status gf_destroy(gf_ctx* c) {
     if (c->red_partial) hipFree(c->red_partial);
     if (c->d_code) hipFree(c->d_code);
     if (c->d_mats) hipFree(c->d_mats);
+    adjoint_free(c);
     demag_free(c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -807,6 +833,15 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     case GF_FIELD_STRAIN_OFF:
         if (!c->el_on || for_write) break;
         return c->eg + 3;
+    case GF_FIELD_HDRIVE:
+        if (!c->hdrive[0]) break;
+        return c->hdrive;
+    case GF_FIELD_STRAIN_T0_DIAG:
+        if (!c->e_t0[0]) break;
+        return c->e_t0;
+    case GF_FIELD_STRAIN_T0_OFF:
+        if (!c->e_t0[0]) break;
+        return c->e_t0 + 3;
     case GF_FIELD_E_CELL:
         if (!c->em_on || for_write) break;
         return c->Ecell;
@@ -884,6 +919,25 @@ static gf_status em_transfer(gf_ctx* c, int id, double* host, size_t count, bool
     }
     if (st) return st;
     return hip_check(c, hipStreamSynchronize(c->stream), up ? "upload sync" : "download sync");
+}
+
+gf_status gf_set_material(gf_ctx* c, int index, const gf_material* m) {
+    if (!c || !m) return set_error(c, GF_ERR_ARG, "null argument");
+    if (index < 1 || index > static_cast<int>(c->host_mats.size()))
+        return set_error(c, GF_ERR_ARG, "material index out of range");
+    if (c->el_on || c->em_on)
+        return set_error(c, GF_ERR_UNSUPPORTED,
+                         "material updates need a context without elastodynamics or electrodynamics "
+                         "(their set-up derives per-cell constants from the materials)");
+    const gf_material& o = c->host_mats[static_cast<size_t>(index - 1)];
+    if (o.if_FM != m->if_FM || o.if_AFM != m->if_AFM || o.if_FE != m->if_FE || o.if_PEC != m->if_PEC)
+        return set_error(c, GF_ERR_ARG, "the kind flags of a material cannot change");
+    gf_status st = hip_check(c, hipStreamSynchronize(c->stream), "material sync");
+    if (st) return st;
+    c->host_mats[static_cast<size_t>(index - 1)] = *m;
+    const DevMaterial d = make_dev_material(*m, c->dt);
+    return hip_check(c, hipMemcpy(c->d_mats + (index - 1), &d, sizeof(DevMaterial), hipMemcpyHostToDevice),
+                     "copy material");
 }
 
 gf_status gf_field_size(const gf_ctx* c, int id, size_t* count) {
@@ -1006,7 +1060,7 @@ namespace gf {
 // (0 = m, 1 = st_a, 2 = st_b); the drive is left at zero.
 StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     StageArgs a{};
-    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+    magnet_strain(c, a.strain);
     const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
     a.code = c->d_code; a.mats = c->d_mats;
     double** s = fm_store(c, in_sel);
@@ -1014,6 +1068,7 @@ StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     for (int i = 0; i < 3; i++) {
         a.store[i] = s[i]; a.base[i] = c->m[i]; a.acc[i] = c->acc[i]; a.out[i] = o[i];
         a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = c->em_dm[i];
+        a.hdrive[i] = c->hdrive[i];
         a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
     }
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
@@ -1024,10 +1079,16 @@ StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     return a;
 }
 
+// Uniform and shaped drives of FM stage s at the context's current time.
+void fm_stage_drives(gf_ctx* c, int stage, StageArgs& a) {
+    stage_drive(c, stage, a.hext);
+    a.drive_s = a.hdrive[0] ? shaped_drive(c, stage) : 0.;
+}
+
 // The same for the two AFM sublattices.
 AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     AfmStageArgs a{};
-    for (int i = 0; i < 6; i++) a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+    magnet_strain(c, a.strain);
     const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
     for (int i = 0; i < 3; i++) {
         a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
@@ -1069,7 +1130,7 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
     if (gf_status e = check_before_em(c, stage)) return e;
     const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
     StageArgs a = fm_stage_args(c, stage, c->mag_store_sel, out_sel);
-    stage_drive(c, stage, a.hext);
+    fm_stage_drives(c, stage, a);
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_MAG_STAGE, &ev);
     gf_status st;
@@ -1515,6 +1576,18 @@ gf_status gf_average(gf_ctx* c, int id, double out[3]) {
 gf_status gf_get_time(const gf_ctx* c, double* t) {
     if (!c || !t) return GF_ERR_ARG;
     *t = c->time;
+    return GF_OK;
+}
+
+gf_status gf_set_time(gf_ctx* c, double t) {
+    if (!c) return GF_ERR_ARG;
+    if ((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1) ||
+        (c->any_fe && c->fe_next_stage != 1) || (c->el_on && c->el_next_stage != 1) ||
+        (c->em_on && c->em_next_stage != 1))
+        return set_error(c, GF_ERR_STATE, "gf_set_time needs a step boundary");
+    c->time = t;
+    c->prescribe_idx = 0;
+    c->prescribe_e_idx = 0;
     return GF_OK;
 }
 

@@ -42,7 +42,7 @@ struct ZArgs {
     const double* fm[3];
     const double* a1[3];
     const double* a2[3];
-    const double* raw;          // mode 1 input
+    const double* raw[3];       // kRaw input, component c at raw[c]
     double2* out;               // component c at out + c*nh, layout [x][kz][y]
     const double2* tw;
     int64_t nh;
@@ -51,8 +51,8 @@ struct ZArgs {
 };
 
 // Magnetisation assembly (magnetostatic_solver.cpp:121-145) fused with the
-// z real-to-complex transform.  kRaw = true transforms a plain real array
-// (used once, for the tensor kernels).
+// z real-to-complex transform.  kRaw = true transforms plain real arrays
+// (the tensor kernels at set-up, the adjoint field of the adjoint sweep).
 template <bool kRaw>
 __global__ void __launch_bounds__(kThreads) zfwd_kernel(ZArgs a) {
     extern __shared__ double2 smem[];
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kThreads) zfwd_kernel(ZArgs a) {
         if (y < a.ny) {
             const int64_t cell = (static_cast<int64_t>(x) * a.ny + y) * a.nz + k;
             if (kRaw) {
-                v = a.raw[cell];
+                v = (c == 0 ? a.raw[0] : (c == 1 ? a.raw[1] : a.raw[2]))[cell];
             } else {
                 const uint32_t code = a.code[cell];
                 if (code & kIsFM) {
@@ -177,6 +177,7 @@ struct XArgs {
     double den;                 // -dx*dy*dz
     int nx, ny, nzh;
     int lognx, pencils, logp;
+    int adjoint;                // 1: conjugate transpose of the tensor (adjoint convolution)
 };
 
 // x forward transform, tensor product and x inverse transform, in place.
@@ -218,8 +219,12 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
             const int iz = (2 * a.pencils + r) * a.nx + x;
             const double2 mx = buf[ix], my = buf[iy], mz = buf[iz];
             const int64_t s = a.nh;
+            // the adjoint convolution uses N^T with conjugated spectra: xy<->yx,
+            // yz<->zy, zx<->xz (the real-space correlation with the kernel)
             auto K = [&](int q) {
-                return make_double2(a.kre[q * s + g], static_cast<double>(a.kim[q * s + g]));
+                const int t = a.adjoint ? (q < 3 ? q : (q < 6 ? q + 3 : q - 3)) : q;
+                const double im = static_cast<double>(a.kim[t * s + g]);
+                return make_double2(a.kre[t * s + g], a.adjoint ? -im : im);
             };
             double2 t0 = cmul(mx, K(0)), t1 = cmul(my, K(3)), t2 = cmul(mz, K(8));
             const double2 hx = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
@@ -402,8 +407,8 @@ unsigned blocks_for(int64_t n, int p) { return static_cast<unsigned>((n + p - 1)
 }  // namespace
 
 // Forward pipeline helpers shared by the tensor set-up and the convolution.
-static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* rawp, const double* const* fm, const double* const* s1,
-                           const double* const* s2, int comps) {
+static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* const* rawp, const double* const* fm,
+                           const double* const* s1, const double* const* s2, int comps) {
     const Shapes sh = shapes(c);
     ZArgs a{};
     a.code = c->d_code; a.mats = c->d_mats;
@@ -412,7 +417,8 @@ static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* rawp, const double
         a.a1[i] = s1 ? s1[i] : nullptr;
         a.a2[i] = s2 ? s2[i] : nullptr;
     }
-    a.raw = rawp; a.out = c->demag.spec_m; a.tw = c->demag.twz.dev;
+    for (int i = 0; i < 3; i++) a.raw[i] = rawp ? rawp[i] : nullptr;
+    a.out = c->demag.spec_m; a.tw = c->demag.twz.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
     a.nzh = static_cast<int>(c->demag.nzh);
@@ -445,9 +451,10 @@ static hipError_t run_y(gf_ctx* c, bool inverse, int comps) {
     return hipGetLastError();
 }
 
-static hipError_t run_x(gf_ctx* c, bool conv) {
+static hipError_t run_x(gf_ctx* c, bool conv, bool adjoint = false) {
     const Shapes sh = shapes(c);
     XArgs a{};
+    a.adjoint = adjoint ? 1 : 0;
     a.data = c->demag.spec_b; a.kre = c->demag.kre; a.kim = c->demag.kim; a.tw = c->demag.twx.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
     a.den = -c->dx * c->dy * c->dz;
@@ -462,11 +469,11 @@ static hipError_t run_x(gf_ctx* c, bool conv) {
     return hipGetLastError();
 }
 
-static hipError_t run_zinv(gf_ctx* c) {
+static hipError_t run_zinv(gf_ctx* c, double* const* out = nullptr) {
     const Shapes sh = shapes(c);
     ZInvArgs a{};
     a.in = c->demag.spec_a;
-    for (int i = 0; i < 3; i++) a.out[i] = c->hstat[i];
+    for (int i = 0; i < 3; i++) a.out[i] = out ? out[i] : c->hstat[i];
     a.tw = c->demag.twz.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
@@ -512,7 +519,8 @@ gf_status demag_setup(gf_ctx* c) {
     for (int s = 0; s < 9; s++) {
         st = hip_check(c, hipMemcpyAsync(draw, real9.data() + s * c->n, sizeof(double) * c->n, hipMemcpyHostToDevice,
                                          c->stream), "copy kernel");
-        if (!st) st = hip_check(c, run_zfwd(c, true, draw, nullptr, nullptr, nullptr, 1), "zfwd kernel");
+        const double* raw3[3] = {draw, nullptr, nullptr};
+        if (!st) st = hip_check(c, run_zfwd(c, true, raw3, nullptr, nullptr, nullptr, 1), "zfwd kernel");
         if (!st) st = hip_check(c, run_y(c, false, 1), "yfwd kernel");
         if (!st) st = hip_check(c, run_x(c, false), "xfwd kernel");
         if (!st) {
@@ -568,6 +576,27 @@ gf_status demag_materialize_h(gf_ctx* c) {
     gf_status st = hip_check(c, run_zinv(c), "zinv");
     if (!st) c->demag.h_real_valid = true;
     return st;
+}
+
+// FM-only convolution H = N * (Ms m) of a given store into out (3 arrays).
+gf_status demag_run_to(gf_ctx* c, const double* const* fm, double* const* out) {
+    gf_status st;
+    if ((st = hip_check(c, run_zfwd(c, false, nullptr, fm, nullptr, nullptr, 3), "zfwd"))) return st;
+    if ((st = hip_check(c, run_y(c, false, 3), "yfwd"))) return st;
+    if ((st = hip_check(c, run_x(c, true), "xconv"))) return st;
+    if ((st = hip_check(c, run_y(c, true, 3), "yinv"))) return st;
+    return hip_check(c, run_zinv(c, out), "zinv");
+}
+
+// Adjoint convolution out = N^T (*) in of plain real arrays (no Ms): the
+// transpose of demag_run_to up to the Ms factor the caller applies.
+gf_status demag_adjoint(gf_ctx* c, const double* const* in, double* const* out) {
+    gf_status st;
+    if ((st = hip_check(c, run_zfwd(c, true, in, nullptr, nullptr, nullptr, 3), "zfwd adjoint"))) return st;
+    if ((st = hip_check(c, run_y(c, false, 3), "yfwd adjoint"))) return st;
+    if ((st = hip_check(c, run_x(c, true, true), "xconv adjoint"))) return st;
+    if ((st = hip_check(c, run_y(c, true, 3), "yinv adjoint"))) return st;
+    return hip_check(c, run_zinv(c, out), "zinv adjoint");
 }
 
 gf_status demag_run(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2) {

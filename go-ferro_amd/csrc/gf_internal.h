@@ -295,12 +295,14 @@ struct StageArgs {
     double* out[3];             // next store, or m at stage 4
     const double* hstat[3];
     const double* hnu[3];       // may be null
+    const double* hdrive[3];    // shaped drive profile, null when off
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
     const double* hcell[3];     // electrodynamic H at the cells, null when uncoupled
     double* dm_out[3];          // k of this stage, null unless Faraday's law needs it
     int64_t nx, ny, nz;
     double dx, dy, dz;
     double hext[3];
+    double drive_s;             // sin(2 pi f t) of the shaped drive at the stage time
     double Tg2c[3][3];
     double Tc2g[3][3];
     int32_t rotate;
@@ -342,6 +344,8 @@ struct BatchLaunch {
     int cpt;
     bool narrow;
 };
+
+struct AdjState;
 
 // Clock and sinusoidal drive of one instance of a batch.
 struct BatchClock {
@@ -503,6 +507,9 @@ struct gf_ctx {
     // statics
     double* hstat[3] = {};
     double* hnu[3] = {};
+    double* hdrive[3] = {};         // shaped drive profile (GF_FLAG_HDRIVE)
+    double hdrive_freq = 0;
+    double* e_t0[6] = {};           // static crystal-frame strain (magnets, no elastodynamics)
     // reduction scratch
     double* red_partial = nullptr;
     int64_t red_blocks = 0;
@@ -516,6 +523,7 @@ struct gf_ctx {
     int fe_next_stage = 1;
 
     gf::DemagPlan demag;
+    gf::AdjState* adj = nullptr;    // checkpoints and buffers of the adjoint sweep
 
     bool timing = false;
     std::vector<gf::TimerSlot> timers;
@@ -536,6 +544,9 @@ void timer_end(gf_ctx* c, int cls, hipEvent_t start);
 // out_sel written (0 = m, 1 = first, 2 = second double-buffered store);
 // the drive is left at zero
 StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
+void fm_stage_drives(gf_ctx* c, int stage, StageArgs& a);
+// adjoint sweep state (gf_adjoint.hip)
+void adjoint_free(gf_ctx* c);
 AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
 // true when cell indices of an n-cell grid fit 32-bit byte offsets (gf_llg.hip)
 bool narrow(int64_t n);
@@ -575,6 +586,8 @@ gf_status demag_run(gf_ctx* c, const double* const* fm_store, const double* cons
                     const double* const* s2);
 gf_status demag_materialize_h(gf_ctx* c);
 gf_status demag_run_fused(gf_ctx* c, const double* const* fm_store);
+gf_status demag_run_to(gf_ctx* c, const double* const* fm_store, double* const* out);
+gf_status demag_adjoint(gf_ctx* c, const double* const* in, double* const* out);
 void demag_free(gf_ctx* c);
 hipError_t launch_demag_local(gf_ctx* c, const double* const* fm_store, const double* const* s1,
                               const double* const* s2, int one_d);

@@ -29,6 +29,7 @@ __all__ = [
     "FLAG_EM_FROM_P", "FLAG_EM_FROM_M", "FLAG_EM_BACKACTION_P", "FLAG_EM_BACKACTION_M", "FLAG_PLANE_EM",
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
+    "FLAG_HDRIVE", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG", "FIELD_STRAIN_T0_OFF",
     "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
 ]
@@ -60,10 +61,12 @@ FLAG_JF_INPUT = 1 << 19
 FLAG_PEC_XY = 1 << 20
 FLAG_PEC_Z = 1 << 21
 FLAG_1D_ABC_EM_ONLYTOP = 1 << 22
+FLAG_HDRIVE = 1 << 23
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
- FIELD_E_EM_STORE, FIELD_H_EM_STORE, FIELD_E_CELL, FIELD_H_CELL, FIELD_JP) = range(22)
+ FIELD_E_EM_STORE, FIELD_H_EM_STORE, FIELD_E_CELL, FIELD_H_CELL, FIELD_JP, FIELD_HDRIVE, FIELD_STRAIN_T0_DIAG,
+ FIELD_STRAIN_T0_OFF) = range(25)
 
 
 def yee_shapes(shape, field):
@@ -107,7 +110,8 @@ KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_con
 EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
     "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_em_stage", "gf_field_size", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
-    "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times",
+    "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times", "gf_set_material", "gf_set_time",
+    "gf_adjoint_forward", "gf_adjoint_backward", "gf_adjoint_stage",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_sync",
     "gf_batch_get_stream", "gf_batch_config", "gf_batch_set_config", "gf_batch_tune",
 ]
@@ -197,6 +201,15 @@ class _Desc(ctypes.Structure):
         ("Jf_input_component", ctypes.c_int32), ("num_Jf_cycles", ctypes.c_int32),
         ("Jfin_lo", ctypes.c_int32 * 3), ("Jfin_hi", ctypes.c_int32 * 3),
         ("Jf_input_amp", ctypes.c_double), ("Jf_input_freq", ctypes.c_double), ("Jf_input_sigma", ctypes.c_double),
+        ("Hdrive_freq", ctypes.c_double),
+    ]
+
+
+class _AdjDesc(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("material", ctypes.c_int32),
+        ("probe_lo", ctypes.c_int32 * 3), ("probe_hi", ctypes.c_int32 * 3),
+        ("probe_freq", ctypes.c_double), ("nsteps", ctypes.c_int64), ("checkpoint_every", ctypes.c_int64),
     ]
 
 
@@ -239,6 +252,7 @@ class Problem:
     Jf_input_amp: float = 0.0
     Jf_input_freq: float = 0.0
     Jf_input_sigma: float = 0.0
+    Hdrive_freq: float = 0.0
 
     @property
     def n(self) -> int:
@@ -290,6 +304,7 @@ class Problem:
         d.Jf_input_amp = float(self.Jf_input_amp)
         d.Jf_input_freq = float(self.Jf_input_freq)
         d.Jf_input_sigma = float(self.Jf_input_sigma)
+        d.Hdrive_freq = float(self.Hdrive_freq)
         keep = [cell, mats]
         if self.prescribe_Hext is not None:
             pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)
@@ -338,6 +353,11 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_get_stream": ([P, ctypes.POINTER(P)], ctypes.c_int),
         "gf_set_kernel_timing": ([P, ctypes.c_int], ctypes.c_int),
         "gf_kernel_times": ([P, dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gf_set_material": ([P, ctypes.c_int, ctypes.POINTER(Material)], ctypes.c_int),
+        "gf_set_time": ([P, ctypes.c_double], ctypes.c_int),
+        "gf_adjoint_forward": ([P, ctypes.POINTER(_AdjDesc), dp], ctypes.c_int),
+        "gf_adjoint_backward": ([P, dp, dp, ctypes.c_size_t], ctypes.c_int),
+        "gf_adjoint_stage": ([P, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_size_t], ctypes.c_int),
         "gf_batch_create": ([ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)], ctypes.c_int),
         "gf_batch_destroy": ([P], ctypes.c_int),
         "gf_batch_last_error": ([P], ctypes.c_char_p),
@@ -421,6 +441,13 @@ class Context:
             return a.reshape((3, 3) + self.shape)
         return a.reshape((3,) + self.shape)
 
+    def set_material(self, index: int, material: "Material") -> None:
+        """Replace the numbers of material `index` (1-based); kinds stay."""
+        self._check(self.lib.gf_set_material(self._h, int(index), ctypes.byref(material)), "gf_set_material")
+        mats = list(self.problem.materials)
+        mats[index - 1] = material
+        self.problem.materials = mats
+
     def field_size(self, field: int) -> int:
         n = ctypes.c_size_t()
         self._check(self.lib.gf_field_size(self._h, field, ctypes.byref(n)), "gf_field_size")
@@ -462,8 +489,50 @@ class Context:
         self._check(self.lib.gf_get_time(self._h, ctypes.byref(t)), "gf_get_time")
         return t.value
 
+    @time.setter
+    def time(self, t: float) -> None:
+        self._check(self.lib.gf_set_time(self._h, float(t)), "gf_set_time")
+
     def sync(self) -> None:
         self._check(self.lib.gf_sync(self._h), "gf_sync")
+
+    # -- adjoint sweep
+    def adjoint_forward(self, material: int, probe_lo, probe_hi, freq: float, nsteps: int,
+                        checkpoint_every: int = 0) -> float:
+        """N steps keeping checkpoints; returns the probe spectral loss."""
+        d = _AdjDesc()
+        d.struct_size = ctypes.sizeof(_AdjDesc)
+        d.material = int(material)
+        d.probe_lo[:] = [int(v) for v in probe_lo]
+        d.probe_hi[:] = [int(v) for v in probe_hi]
+        d.probe_freq = float(freq)
+        d.nsteps = int(nsteps)
+        d.checkpoint_every = int(checkpoint_every)
+        L = ctypes.c_double()
+        self._check(self.lib.gf_adjoint_forward(self._h, ctypes.byref(d), ctypes.byref(L)), "gf_adjoint_forward")
+        return L.value
+
+    def adjoint_backward(self, drive: bool = True):
+        """Reverse sweep: (dL/d(alpha, Aex, B1), dL/dHdrive as (3,)+shape or None)."""
+        g = np.zeros(3)
+        gh = np.zeros(3 * self.n) if drive else None
+        null = ctypes.POINTER(ctypes.c_double)()
+        self._check(self.lib.gf_adjoint_backward(self._h, _dptr(g), null if gh is None else _dptr(gh),
+                                                 0 if gh is None else gh.size), "gf_adjoint_backward")
+        return g, (None if gh is None else gh.reshape((3,) + self.shape))
+
+    def adjoint_stage(self, material: int, stage: int, store, kbar, drive: bool = True):
+        """Vector-Jacobian product of one stage increment: (ybar, grad3, grad_hdrive)."""
+        y = np.ascontiguousarray(store, dtype=np.float64).reshape(-1)
+        kb = np.ascontiguousarray(kbar, dtype=np.float64).reshape(-1)
+        yb = np.zeros(3 * self.n)
+        g = np.zeros(3)
+        gh = np.zeros(3 * self.n) if drive else None
+        null = ctypes.POINTER(ctypes.c_double)()
+        self._check(self.lib.gf_adjoint_stage(self._h, int(material), int(stage), _dptr(y), _dptr(kb), _dptr(yb),
+                                              _dptr(g), null if gh is None else _dptr(gh), yb.size),
+                    "gf_adjoint_stage")
+        return yb.reshape((3,) + self.shape), g, (None if gh is None else gh.reshape((3,) + self.shape))
 
     def set_kernel_timing(self, enable: bool) -> None:
         self._check(self.lib.gf_set_kernel_timing(self._h, int(bool(enable))), "gf_set_kernel_timing")

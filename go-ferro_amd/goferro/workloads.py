@@ -162,3 +162,36 @@ def afm_sweep(count=256, seed=5, sizes=SWEEP_SIZES, dt=1e-15):
         freq = float(rng.uniform(0.5e12, 2e12))
         out.append(afm_block(shape, dt=dt, seed=5000 + i, alpha=alpha, drive_T=amp, freq=freq))
     return out
+
+
+def adjoint_film(shape=(128, 128, 32), seed=4, drive_T=1e-3, drive_freq=3e10, strain=1e-4, dt=1e-14):
+    """The adjoint workload: the plain ferromagnet film physics (fm_film) with
+    a shaped AC drive along x on the top layer k = nz-1, amplitude
+    a(x, y) * drive_T with a ~ U[0.5, 1] (seeded), at drive_freq, and a
+    static biaxial strain exx = eyy = `strain` so that B1 acts.  Returns
+    (Problem, fields, amp) with fields the uploads (m, drive profile,
+    static strain) and amp the (nx, ny) amplitude map."""
+    from . import FIELD_HDRIVE, FIELD_M, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FLAG_ELASTO_ON_PM, FLAG_HDRIVE
+    prob, m = fm_film(shape, seed=seed, dt=dt)
+    mat = prob.materials[0]
+    mat.B1, mat.B2 = 3.5e5, 3.5e5
+    prob.flags |= FLAG_HDRIVE | FLAG_ELASTO_ON_PM
+    prob.Hdrive_freq = drive_freq
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(0.5, 1.0, shape[:2])
+    hd = np.zeros((3,) + tuple(shape))
+    hd[0, :, :, -1] = amp * drive_T / MU0
+    diag = np.zeros((3,) + tuple(shape))
+    diag[0] = strain
+    diag[1] = strain
+    fields = {FIELD_M: m, FIELD_HDRIVE: hd, FIELD_STRAIN_T0_DIAG: diag,
+              FIELD_STRAIN_T0_OFF: np.zeros((3,) + tuple(shape))}
+    return prob, fields, amp
+
+
+def adjoint_probe(shape=(128, 128, 32), size=(16, 16, 4)):
+    """Probe box of the adjoint workload: centred in x and y, the bottom
+    `size[2]` layers (away from the driven top layer)."""
+    lo = ((shape[0] - size[0]) // 2, (shape[1] - size[1]) // 2, 0)
+    hi = (lo[0] + size[0] - 1, lo[1] + size[1] - 1, size[2] - 1)
+    return lo, hi

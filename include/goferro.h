@@ -63,6 +63,12 @@ typedef int gf_status;
 #define GF_FLAG_PEC_XY           (1u << 20) /* if_PEC_XY: x and y walls are PEC */
 #define GF_FLAG_PEC_Z            (1u << 21) /* if_PEC_Z: z walls are PEC */
 #define GF_FLAG_1D_ABC_EM_ONLYTOP (1u << 22) /* if_1D_ABC_EM_onlytop (with GF_FLAG_1D_ABC) */
+/* Spatially shaped AC drive of the ferromagnet: H += Hdrive(r) sin(2 pi
+ * Hdrive_freq t) at the stage times of the uniform drive, Hdrive uploaded as
+ * GF_FIELD_HDRIVE.  No counterpart in the reference (its non-uniform field
+ * Hext_nonunif is static); it carries the per-cell drive amplitudes whose
+ * gradients the adjoint returns. */
+#define GF_FLAG_HDRIVE           (1u << 23)
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -163,6 +169,8 @@ typedef struct gf_desc {
     double Jf_input_amp;            /* A/m^2 */
     double Jf_input_freq;           /* Hz (sigma in s for types 4, 5) */
     double Jf_input_sigma;          /* s (type 6) */
+    /* shaped drive (GF_FLAG_HDRIVE) */
+    double Hdrive_freq;             /* Hz */
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */
@@ -193,6 +201,14 @@ enum gf_field_id {
     GF_FIELD_E_CELL = 19,      /* cell-centred E of the stage store, 3n, read only */
     GF_FIELD_H_CELL = 20,      /* cell-centred H of the stage store, 3n, read only */
     GF_FIELD_JP = 21,          /* polarisation currents, 9n: poles 1..3, each x,y,z */
+    GF_FIELD_HDRIVE = 22,      /* profile of the shaped drive, 3n (GF_FLAG_HDRIVE) */
+    /* static strain in the crystal frame, exx eyy ezz | eyz exz exy, 3n each:
+     * the reference's ex*t0_crt, which the magnets read under
+     * if_elasto_on_pORm (magnetic_solver.cpp:89-96, magnetic_AFM_solver.cpp:
+     * 89-96).  Used with GF_FLAG_ELASTO_ON_PM when elastodynamics is off;
+     * zero until uploaded. */
+    GF_FIELD_STRAIN_T0_DIAG = 23,
+    GF_FIELD_STRAIN_T0_OFF = 24,
     GF_FIELD_COUNT
 };
 
@@ -256,6 +272,13 @@ gf_status gf_elastic_stage(gf_ctx* ctx, int stage);
  * stage-s E and H stores. */
 gf_status gf_em_stage(gf_ctx* ctx, int stage);
 
+/* Replace the numeric parameters of material `index` (1-based, as in
+ * material_cell) and rebuild its derived constants; the kind flags (if_FM,
+ * if_AFM, if_FE, elastic, ...) must stay as they were.  Takes effect from
+ * the next stage.  No counterpart in the reference (its materials are fixed
+ * at read time); parameter sweeps and the adjoint's parameter updates use it. */
+gf_status gf_set_material(gf_ctx* ctx, int index, const gf_material* material);
+
 /* Number of doubles gf_upload / gf_download move for a field. */
 gf_status gf_field_size(const gf_ctx* ctx, int field_id, size_t* count);
 
@@ -274,6 +297,10 @@ gf_status gf_average(gf_ctx* ctx, int field_id, double out[3]);
 
 /* Current simulated time (global_parameters::time_device). */
 gf_status gf_get_time(const gf_ctx* ctx, double* t);
+
+/* Set the simulated time, e.g. to restart from a saved state at its time
+ * (the drive tables restart from their first row).  At a step boundary. */
+gf_status gf_set_time(gf_ctx* ctx, double t);
 
 /* Wait for all work queued on the context stream. */
 gf_status gf_sync(gf_ctx* ctx);
@@ -303,6 +330,50 @@ enum gf_kernel_class {
 };
 gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
 gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[GF_K_COUNT]);
+
+/* ---- Adjoint sweep ----------------------------------------------------
+ * No counterpart in the reference.  Gradients of a probe-region spectral
+ * loss through the discrete RK4 trajectory of an FM context:
+ *   X_n = mean of m_x over the probe box after step n (n = 1..N),
+ *   S   = sum_n X_n exp(-i 2 pi f t_n) dt,   L = |S|^2,
+ * with respect to the damping alpha, the exchange constant Aex and B1 of
+ * one FM material and to the shaped-drive profile (GF_FIELD_HDRIVE).  The
+ * forward pass keeps a checkpoint of m every checkpoint_every steps; the
+ * backward pass recomputes one segment at a time (keeping its stage stores
+ * and fields) and runs the adjoint of each stage: transposed exchange
+ * stencil, adjoint FFT convolution (conjugate-transposed tensor spectra),
+ * adjoint LLG cross products, anisotropy and magnetoelastic Jacobians.
+ * Supported contexts: FM only (one FM material), no DMI, FFT or no
+ * magnetostatics, uniform/shaped sinusoidal drives, optional static strain
+ * and rotation; no coupled physics or drive tables. */
+typedef struct gf_adjoint_desc {
+    uint32_t struct_size;      /* sizeof(gf_adjoint_desc) */
+    int32_t material;          /* 1-based FM material whose alpha, Aex, B1 are differentiated */
+    int32_t probe_lo[3];       /* probe box, inclusive cell indices */
+    int32_t probe_hi[3];
+    double probe_freq;         /* Hz */
+    int64_t nsteps;            /* N */
+    int64_t checkpoint_every;  /* steps per checkpoint; 0 = ceil(sqrt(N)) */
+} gf_adjoint_desc;
+
+/* Run N steps from the current state (the magnetostatic field is rebuilt
+ * from it first), keep the checkpoints and return L.  Leaves the context at
+ * the final state, at a step boundary. */
+gf_status gf_adjoint_forward(gf_ctx* ctx, const gf_adjoint_desc* desc, double* loss);
+
+/* Reverse sweep of the last gf_adjoint_forward: grad = dL/d(alpha, Aex, B1)
+ * of the chosen material; grad_hdrive (3n doubles, may be NULL) = dL/dHdrive.
+ * Leaves the context at the forward's final state. */
+gf_status gf_adjoint_backward(gf_ctx* ctx, double grad[3], double* grad_hdrive, size_t count);
+
+/* The adjoint of one stage increment k = f(store) (magnetic_solver.cpp:
+ * 3-229 differentiated) at the stage time of `stage` for the context's
+ * current time: ybar = (dk/dstore)^T kbar including the magnetostatic field
+ * of the store, grad += (dk/d(alpha, Aex, B1))^T kbar, grad_hdrive (may be
+ * NULL) += (dk/dHdrive)^T kbar.  Host arrays of 3n doubles; the context's
+ * state is not changed. */
+gf_status gf_adjoint_stage(gf_ctx* ctx, int material, int stage, const double* store, const double* kbar,
+                           double* ybar, double grad[3], double* grad_hdrive, size_t count);
 
 /* ---- Batched sweeps ---------------------------------------------------
  * No counterpart in the reference, which runs one simulation per process.

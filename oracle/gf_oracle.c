@@ -73,6 +73,8 @@ struct gfo_ctx {
     double sigma_gauss, amplitude_gauss, w_liao;
     /* statics */
     double *H[3], *Hnu[3];
+    double *Hdrive[3], hdrive_freq;            /* shaped drive (GF_FLAG_HDRIVE), no reference counterpart */
+    double *e_t0[6];                           /* static crystal-frame strain, ex*t0_crt */
     /* electrodynamics (EMdynamic_system.h): E on edges, H on faces */
     int if_em, periodic_all, pec_all, jp_on;
     int64_t nE[3], nH[3];
@@ -434,6 +436,7 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     c->dx = d->dx; c->dy = d->dy; c->dz = d->dz;
     for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
     c->flags = d->flags; c->dt = d->dt; c->time = 0.;
+    c->hdrive_freq = d->Hdrive_freq;
     c->nmat = d->n_materials;
     c->mat = (gf_material*)malloc(sizeof(gf_material) * (size_t)(c->nmat > 0 ? c->nmat : 1));
     memcpy(c->mat, d->materials, sizeof(gf_material) * (size_t)c->nmat);
@@ -513,7 +516,7 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         for (int s = 0; s < 4; s++) { c->k[s][a] = dalloc(n); c->k1[s][a] = dalloc(n); c->k2[s][a] = dalloc(n); }
         c->m1[a] = dalloc(n); c->m2[a] = dalloc(n); c->ms1[a] = dalloc(n); c->ms2[a] = dalloc(n);
         c->dm1[a] = dalloc(n); c->dm2[a] = dalloc(n);
-        c->H[a] = dalloc(n); c->Hnu[a] = dalloc(n);
+        c->H[a] = dalloc(n); c->Hnu[a] = dalloc(n); c->Hdrive[a] = dalloc(n);
         c->p[a] = dalloc(n); c->q[a] = dalloc(n); c->ps[a] = dalloc(n); c->qs[a] = dalloc(n); c->Es[a] = dalloc(n);
         for (int s = 0; s < 4; s++) { c->kp[s][a] = dalloc(n); c->kq[s][a] = dalloc(n); }
     }
@@ -537,6 +540,7 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         c->t1[a] = dalloc(n); c->t2[a] = dalloc(n); c->t3[a] = dalloc(n); c->t4[a] = dalloc(n);
     }
     for (int a = 0; a < 6; a++) { c->e_glb[a] = dalloc(n); c->e_crt[a] = dalloc(n); c->e0_glb[a] = dalloc(n); c->e0_crt[a] = dalloc(n); }
+    for (int a = 0; a < 6; a++) c->e_t0[a] = dalloc(n);
     c->sfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
     c->sfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
     c->sfXY = (unsigned char*)calloc((size_t)(c->nx * c->ny * (c->nz + 1)), 1);
@@ -598,7 +602,7 @@ void gfo_destroy(gfo_ctx* c) {
         free(c->m[a]); free(c->ms[a]); free(c->dm[a]);
         for (int s = 0; s < 4; s++) { free(c->k[s][a]); free(c->k1[s][a]); free(c->k2[s][a]); }
         free(c->m1[a]); free(c->m2[a]); free(c->ms1[a]); free(c->ms2[a]); free(c->dm1[a]); free(c->dm2[a]);
-        free(c->H[a]); free(c->Hnu[a]);
+        free(c->H[a]); free(c->Hnu[a]); free(c->Hdrive[a]);
         free(c->p[a]); free(c->q[a]); free(c->ps[a]); free(c->qs[a]); free(c->Es[a]);
         for (int s = 0; s < 4; s++) { free(c->kp[s][a]); free(c->kq[s][a]); }
     }
@@ -610,6 +614,7 @@ void gfo_destroy(gfo_ctx* c) {
         free(c->t1[a]); free(c->t2[a]); free(c->t3[a]); free(c->t4[a]);
     }
     for (int a = 0; a < 6; a++) { free(c->e_glb[a]); free(c->e_crt[a]); free(c->e0_glb[a]); free(c->e0_crt[a]); }
+    for (int a = 0; a < 6; a++) free(c->e_t0[a]);
     free(c->sfYZ); free(c->sfXZ); free(c->sfXY);
     free(c->surfYZ); free(c->surfXZ); free(c->surfXY); free(c->asurfYZ); free(c->asurfXZ); free(c->asurfXY);
     for (int f = 0; f < 6; f++) { free(c->Aij[f]); free(c->Aij1[f]); free(c->Aij2[f]); }
@@ -626,6 +631,13 @@ void gfo_destroy(gfo_ctx* c) {
     free(c);
 }
 
+/* The magnets read the static strain ex*t0_crt (magnetic_solver.cpp:89-96:
+ * exx = exxt0_crt + Dexx_crt) when strain acts on them without
+ * elastodynamics, whose dynamic part is then zero. */
+static int static_strain(const gfo_ctx* c) {
+    return (c->flags & GF_FLAG_ELASTO_ON_PM) && !(c->flags & GF_FLAG_ELASTODYNAMICS);
+}
+
 static double** field_ptrs(gfo_ctx* c, int id) {
     switch (id) {
     case GF_FIELD_M: return c->m;
@@ -640,11 +652,15 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_ESTAT: return c->Es;
     case GFO_FIELD_Q_STORE: return c->qs;
     case GFO_FIELD_DV1: return c->kv[0];
+    case GFO_FIELD_K1: return c->k[0];
     case GF_FIELD_U: return c->u;
     case GF_FIELD_V: return c->v;
     case GF_FIELD_U_STORE: return c->us;
     case GF_FIELD_STRAIN_DIAG: return c->e_glb;
     case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
+    case GF_FIELD_HDRIVE: return (c->flags & GF_FLAG_HDRIVE) ? c->Hdrive : NULL;
+    case GF_FIELD_STRAIN_T0_DIAG: return static_strain(c) ? c->e_t0 : NULL;
+    case GF_FIELD_STRAIN_T0_OFF: return static_strain(c) ? c->e_t0 + 3 : NULL;
     case GF_FIELD_E_CELL: return c->if_em ? c->Ecell : NULL;
     case GF_FIELD_H_CELL: return c->if_em ? c->Hcell : NULL;
     default: return NULL;
@@ -836,6 +852,8 @@ static void stencil(const gfo_ctx* c, double* const* s, const unsigned char* mYZ
  * the drive time and the output array), plus get_dm :1449-1457 at s=4. */
 void gfo_mag_get_dm(gfo_ctx* c, int stage) {
     if (!(c->flags & GF_FLAG_PRESCRIBE_HEXT)) update_external_H(c, stage);
+    const double drive_t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    const double drive_s = sin(2. * PI * c->hdrive_freq * drive_t);
     int64_t ny = c->ny, nz = c->nz;
     double** out = c->k[stage - 1];
     for (int64_t i = 0; i < c->n; i++) {
@@ -850,7 +868,10 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
         double B1 = mat->B1 / MU0 / Ms, B2 = mat->B2 / MU0 / Ms;
         double Aexi = mat->Aex, iDMIi = mat->iDMI;
         double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
-        if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* :89-96, static strain zero */
+        if (static_strain(c)) {                  /* :89-96, dynamic strain zero */
+            exx = c->e_t0[0][i] + 0.; eyy = c->e_t0[1][i] + 0.; ezz = c->e_t0[2][i] + 0.;
+            eyz = c->e_t0[3][i] + 0.; exz = c->e_t0[4][i] + 0.; exy = c->e_t0[5][i] + 0.;
+        } else if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* :89-96, static strain zero */
             exx = 0. + c->e_crt[0][i]; eyy = 0. + c->e_crt[1][i]; ezz = 0. + c->e_crt[2][i];
             eyz = 0. + c->e_crt[3][i]; exz = 0. + c->e_crt[4][i]; exy = 0. + c->e_crt[5][i];
         }
@@ -877,6 +898,8 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
         hd[2] = -2. * iDMIi / MU0 / Ms * (d[0] + d[1]);
         for (int a = 0; a < 3; a++)
             H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + hd[a] + c->Hext[a] + c->Hnu[a][i];
+        if (c->flags & GF_FLAG_HDRIVE)   /* shaped drive at the stage time of the uniform one */
+            for (int a = 0; a < 3; a++) H[a] = H[a] + c->Hdrive[a][i] * drive_s;
         if (c->if_em && (c->flags & GF_FLAG_EM_BACKACTION_M))   /* magnetic_solver.cpp:200-204 */
             for (int a = 0; a < 3; a++) H[a] = H[a] + c->Hcell[a][i];
         double alpha = mat->FM_damping, gyro = mat->gyro;
@@ -920,7 +943,10 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
             int64_t x = i / (ny * nz), y = (i - x * (ny * nz)) / nz, z = i - x * (ny * nz) - y * nz;
             double ux = mat->uniaxis_AFM[0], uy = mat->uniaxis_AFM[1], uz = mat->uniaxis_AFM[2];
             double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
-            if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* magnetic_AFM_solver.cpp:89-96 */
+            if (static_strain(c)) {                  /* magnetic_AFM_solver.cpp:89-96 */
+                exx = c->e_t0[0][i] + 0.; eyy = c->e_t0[1][i] + 0.; ezz = c->e_t0[2][i] + 0.;
+                eyz = c->e_t0[3][i] + 0.; exz = c->e_t0[4][i] + 0.; exy = c->e_t0[5][i] + 0.;
+            } else if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* magnetic_AFM_solver.cpp:89-96 */
                 exx = 0. + c->e_crt[0][i]; eyy = 0. + c->e_crt[1][i]; ezz = 0. + c->e_crt[2][i];
                 eyz = 0. + c->e_crt[3][i]; exz = 0. + c->e_crt[4][i]; exy = 0. + c->e_crt[5][i];
             }
@@ -1948,6 +1974,8 @@ static void demag_1d(gfo_ctx* c) {
 }
 
 /* magnetostatic_solver.cpp:114-219 */
+static void conv_3d(gfo_ctx* c, const double* M, double* const* out);
+
 static void demag_3d(gfo_ctx* c) {
     int64_t n = c->n;
     if (c->flags & GF_FLAG_DEMAG_FACTOR) {
@@ -1963,10 +1991,7 @@ static void demag_3d(gfo_ctx* c) {
         }
         return;
     }
-    int64_t nh = c->nx * c->ny * (c->nz / 2 + 1);
     double* M = dalloc(3 * n);
-    double* Mk = dalloc(3 * 2 * nh);
-    double* Hk = dalloc(2 * nh);
     for (int64_t i = 0; i < n; i++) {
         const gf_material* mat = matof(c, i);
         if (!mat) continue;
@@ -1976,6 +2001,17 @@ static void demag_3d(gfo_ctx* c) {
             for (int a = 0; a < 3; a++) M[a * n + i] = mat->Ms_AFM1 * c->ms1[a][i] + mat->Ms_AFM2 * c->ms2[a][i];
         }
     }
+    conv_3d(c, M, c->H);
+    free(M);
+}
+
+/* magnetostatic_solver.cpp:114-219: H = N (*) M for a magnetisation M of
+ * 3n doubles (component a at M + a n) into out. */
+static void conv_3d(gfo_ctx* c, const double* M, double* const* out) {
+    int64_t n = c->n;
+    int64_t nh = c->nx * c->ny * (c->nz / 2 + 1);
+    double* Mk = dalloc(3 * 2 * nh);
+    double* Hk = dalloc(2 * nh);
     for (int a = 0; a < 3; a++) gfo_rfft3(c->nx, c->ny, c->nz, M + a * n, Mk + a * 2 * nh);
     /* H_k = (M_a K_a + M_b K_b + M_c K_c) / (-dx dy dz), complex products
      * in the order of :163-177; spectra order Axyz,Ayzx,Azxy,Bxyz,Byzx,Bzxy,Byxz,Bzyx,Bxzy */
@@ -1994,9 +2030,9 @@ static void demag_3d(gfo_ctx* c) {
             }
             Hk[2 * q] = sr / den; Hk[2 * q + 1] = si / den;
         }
-        gfo_irfft3(c->nx, c->ny, c->nz, Hk, c->H[a]);
+        gfo_irfft3(c->nx, c->ny, c->nz, Hk, out[a]);
     }
-    free(M); free(Mk); free(Hk);
+    free(Mk); free(Hk);
 }
 
 void gfo_demag(gfo_ctx* c) {
@@ -2047,3 +2083,176 @@ void gfo_average(const gfo_ctx* c, int id, double out[3]) {
 }
 
 double gfo_time(const gfo_ctx* c) { return c->time; }
+
+/* ------------------------------------------- tangent-linear reference */
+/* Forward-mode derivative of the probe spectral loss of the adjoint sweep
+ * (no counterpart in the reference; test infrastructure): the primal runs
+ * through the oracle's own stages (gfo_mag_get_dm / gfo_mag_update /
+ * demag_3d), the tangent through the linearisation of
+ * magnetic_solver.cpp:3-229 written out term by term:
+ *   dH = N (*) (Ms dm) + 2/(mu0 Ms) sum Aij d2(dm) + Ja(m) dm + Me dm
+ *        + dHdrive s(t) + (dAex/Aex) Hex + (dB1/(mu0 Ms)) f (e_aa m_a)
+ *   dk = P (dm x H + m x dH) + Q (dm x p + m x dp) + dP p + dQ q
+ * and the RK4 combination of the tangents.  FM only, one FM material, no DMI. */
+static void tl_stage(gfo_ctx* c, int stage, const gf_material* mat, double* const* dms, double* const* dk,
+                     const double dpar[3], double* const* dhdrive, double* const* dHd, double* const* dhscratch) {
+    int64_t n = c->n, ny = c->ny, nz = c->nz;
+    const double drive_t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
+    const double drive_s = sin(2. * PI * c->hdrive_freq * drive_t);
+    double Hext[3];
+    for (int a = 0; a < 3; a++)
+        Hext[a] = c->Hext_stat[a] + c->Hext_altr[a] * sin(2. * PI * c->H_altr_freq[a] * drive_t);
+    /* tangent of the magnetostatic field */
+    int ms_on = (c->flags & GF_FLAG_MAGNETOSTATICS) != 0;
+    if (ms_on) {
+        double* M = dalloc(3 * n);
+        for (int64_t i = 0; i < n; i++) {
+            const gf_material* m = matof(c, i);
+            if (m && m->if_FM) for (int a = 0; a < 3; a++) M[a * n + i] = m->Ms * dms[a][i];
+        }
+        conv_3d(c, M, dHd);
+        free(M);
+    }
+    (void)dhscratch;
+    for (int64_t i = 0; i < n; i++) {
+        const gf_material* m = matof(c, i);
+        if (!m || !m->if_FM) { for (int a = 0; a < 3; a++) dk[a][i] = 0.; continue; }
+        int64_t x = i / (ny * nz), y = (i - x * (ny * nz)) / nz, z = i - x * (ny * nz) - y * nz;
+        double Ms = m->Ms;
+        double K1 = m->K1 / MU0 / Ms, K2 = m->K2 / MU0 / Ms, B1 = m->B1 / MU0 / Ms, B2 = m->B2 / MU0 / Ms;
+        double mg[3] = {c->ms[0][i], c->ms[1][i], c->ms[2][i]}, dg[3] = {dms[0][i], dms[1][i], dms[2][i]};
+        double mc[3], dc[3];
+        glb2crt(c, mg[0], mg[1], mg[2], &mc[0], &mc[1], &mc[2]);
+        glb2crt(c, dg[0], dg[1], dg[2], &dc[0], &dc[1], &dc[2]);
+        /* primal field (same terms as gfo_mag_get_dm) */
+        double d[4], d2[3][2][3], dd[4], dd2[3][2][3];
+        stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, 0., x, y, z, d, d2);
+        stencil(c, dms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, 0., x, y, z, dd, dd2);
+        double hx[3], dhx[3];
+        for (int a = 0; a < 3; a++) {
+            hx[a] = 2. / MU0 / Ms * (c->Aij[0][i] * d2[0][0][a] + c->Aij[1][i] * d2[0][1][a] + c->Aij[2][i] * d2[1][0][a]
+                                    + c->Aij[3][i] * d2[1][1][a] + c->Aij[4][i] * d2[2][0][a] + c->Aij[5][i] * d2[2][1][a]);
+            dhx[a] = 2. / MU0 / Ms * (c->Aij[0][i] * dd2[0][0][a] + c->Aij[1][i] * dd2[0][1][a] + c->Aij[2][i] * dd2[1][0][a]
+                                     + c->Aij[3][i] * dd2[1][1][a] + c->Aij[4][i] * dd2[2][0][a] + c->Aij[5][i] * dd2[2][1][a]);
+            dhx[a] += hx[a] / m->Aex * dpar[1];
+        }
+        double hac[3] = {0., 0., 0.}, dhac[3] = {0., 0., 0.};
+        double X = mc[0], Y = mc[1], Z = mc[2], dX = dc[0], dY = dc[1], dZ = dc[2];
+        if (mat->anisotropy_type == 1) {
+            hac[0] = -2. * (K1 * (Y * Y + Z * Z) + K2 * Y * Y * Z * Z) * X;
+            hac[1] = -2. * (K1 * (X * X + Z * Z) + K2 * X * X * Z * Z) * Y;
+            hac[2] = -2. * (K1 * (X * X + Y * Y) + K2 * X * X * Y * Y) * Z;
+            /* product rule on each component */
+            dhac[0] = -2. * (K1 * (2. * Y * dY + 2. * Z * dZ) + K2 * (2. * Y * dY * Z * Z + Y * Y * 2. * Z * dZ)) * X
+                      - 2. * (K1 * (Y * Y + Z * Z) + K2 * Y * Y * Z * Z) * dX;
+            dhac[1] = -2. * (K1 * (2. * X * dX + 2. * Z * dZ) + K2 * (2. * X * dX * Z * Z + X * X * 2. * Z * dZ)) * Y
+                      - 2. * (K1 * (X * X + Z * Z) + K2 * X * X * Z * Z) * dY;
+            dhac[2] = -2. * (K1 * (2. * X * dX + 2. * Y * dY) + K2 * (2. * X * dX * Y * Y + X * X * 2. * Y * dY)) * Z
+                      - 2. * (K1 * (X * X + Y * Y) + K2 * X * X * Y * Y) * dZ;
+        } else if (mat->anisotropy_type == 2) {
+            hac[2] = 2. * K1 * Z;
+            dhac[2] = 2. * K1 * dZ;
+        }
+        double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
+        if (static_strain(c)) {
+            exx = c->e_t0[0][i]; eyy = c->e_t0[1][i]; ezz = c->e_t0[2][i];
+            eyz = c->e_t0[3][i]; exz = c->e_t0[4][i]; exy = c->e_t0[5][i];
+        }
+        double hec[3], dhec[3];
+        hec[0] = -2. * (B1 * exx * X + B2 * (exy * Y + exz * Z));
+        hec[1] = -2. * (B1 * eyy * Y + B2 * (exy * X + eyz * Z));
+        hec[2] = -2. * (B1 * ezz * Z + B2 * (exz * X + eyz * Y));
+        dhec[0] = -2. * (B1 * exx * dX + B2 * (exy * dY + exz * dZ)) - 2. * dpar[2] / MU0 / Ms * exx * X;
+        dhec[1] = -2. * (B1 * eyy * dY + B2 * (exy * dX + eyz * dZ)) - 2. * dpar[2] / MU0 / Ms * eyy * Y;
+        dhec[2] = -2. * (B1 * ezz * dZ + B2 * (exz * dX + eyz * dY)) - 2. * dpar[2] / MU0 / Ms * ezz * Z;
+        double ha[3], dha[3], he[3], dhe[3];
+        crt2glb(c, hac[0], hac[1], hac[2], &ha[0], &ha[1], &ha[2]);
+        crt2glb(c, dhac[0], dhac[1], dhac[2], &dha[0], &dha[1], &dha[2]);
+        crt2glb(c, hec[0], hec[1], hec[2], &he[0], &he[1], &he[2]);
+        crt2glb(c, dhec[0], dhec[1], dhec[2], &dhe[0], &dhe[1], &dhe[2]);
+        double H[3], dH[3];
+        for (int a = 0; a < 3; a++) {
+            H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + Hext[a] + c->Hnu[a][i];
+            dH[a] = dha[a] + dhe[a] + dhx[a];
+            if (ms_on) dH[a] += dHd[a][i];
+            if (c->flags & GF_FLAG_HDRIVE) {
+                H[a] += c->Hdrive[a][i] * drive_s;
+                if (dhdrive) dH[a] += dhdrive[a][i] * drive_s;
+            }
+        }
+        double alpha = mat->FM_damping, gyro = mat->gyro;
+        double P = -gyro / (1 + alpha * alpha) * c->dt, Q = alpha * P;
+        double dP = -2. * alpha * P / (1. + alpha * alpha) * dpar[0];
+        double dQ = P * dpar[0] + alpha * dP;
+        double p[3] = {mg[1] * H[2] - mg[2] * H[1], mg[2] * H[0] - mg[0] * H[2], mg[0] * H[1] - mg[1] * H[0]};
+        double q[3] = {mg[1] * p[2] - mg[2] * p[1], mg[2] * p[0] - mg[0] * p[2], mg[0] * p[1] - mg[1] * p[0]};
+        double dp[3] = {dg[1] * H[2] - dg[2] * H[1] + mg[1] * dH[2] - mg[2] * dH[1],
+                        dg[2] * H[0] - dg[0] * H[2] + mg[2] * dH[0] - mg[0] * dH[2],
+                        dg[0] * H[1] - dg[1] * H[0] + mg[0] * dH[1] - mg[1] * dH[0]};
+        double dq[3] = {dg[1] * p[2] - dg[2] * p[1] + mg[1] * dp[2] - mg[2] * dp[1],
+                        dg[2] * p[0] - dg[0] * p[2] + mg[2] * dp[0] - mg[0] * dp[2],
+                        dg[0] * p[1] - dg[1] * p[0] + mg[0] * dp[1] - mg[1] * dp[0]};
+        for (int a = 0; a < 3; a++) dk[a][i] = P * dp[a] + Q * dq[a] + dP * p[a] + dQ * q[a];
+    }
+}
+
+int gfo_tl_probe_loss(gfo_ctx* c, int material, const int lo[3], const int hi[3], double freq, int64_t nsteps,
+                      const double dpar[3], const double* dhdrive_flat, const double* dm0_flat, double* loss,
+                      double* dloss) {
+    if (material < 1 || material > c->nmat || !c->mat[material - 1].if_FM) return -1;
+    if (c->if_AFM_all || c->if_FE_all || c->if_elasto || c->if_em) return -1;
+    const gf_material* mat = &c->mat[material - 1];
+    if (mat->iDMI != 0.) return -1;
+    int64_t n = c->n;
+    double *dm[3], *dms[3], *dk[4][3], *dHd[3], *dhd[3];
+    for (int a = 0; a < 3; a++) {
+        dm[a] = dalloc(n); dms[a] = dalloc(n); dHd[a] = dalloc(n);
+        for (int s = 0; s < 4; s++) dk[s][a] = dalloc(n);
+        if (dm0_flat) memcpy(dm[a], dm0_flat + a * n, sizeof(double) * (size_t)n);
+        memcpy(dms[a], dm[a], sizeof(double) * (size_t)n);
+        dhd[a] = (double*)(dhdrive_flat ? dhdrive_flat + a * n : NULL);
+    }
+    int ms_on = (c->flags & GF_FLAG_MAGNETOSTATICS) != 0;
+    if (ms_on) demag_3d(c);
+    double Sr = 0., Si = 0., dSr = 0., dSi = 0.;
+    double w = 2. * PI * freq;
+    int64_t np = (int64_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+    for (int64_t st = 0; st < nsteps; st++) {
+        gfo_time_advance(c);
+        for (int s = 1; s <= 4; s++) {
+            tl_stage(c, s, mat, dms, dk[s - 1], dpar, dhdrive_flat ? dhd : NULL, dHd, NULL);
+            gfo_mag_get_dm(c, s);
+            gfo_mag_update(c, s);
+            for (int a = 0; a < 3; a++)
+                for (int64_t i = 0; i < n; i++) {
+                    if (s == 1 || s == 2) dms[a][i] = dm[a][i] + dk[s - 1][a][i] * 0.5;
+                    else if (s == 3) dms[a][i] = dm[a][i] + dk[2][a][i];
+                    else {
+                        dm[a][i] = dm[a][i] + (dk[0][a][i] / 6. + dk[1][a][i] / 3. + dk[2][a][i] / 3. + dk[3][a][i] / 6.);
+                        dms[a][i] = dm[a][i];
+                    }
+                }
+            if (ms_on) demag_3d(c);
+        }
+        double X = 0., dX = 0.;
+        for (int i = lo[0]; i <= hi[0]; i++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int k = lo[2]; k <= hi[2]; k++) {
+                    int64_t id = ((int64_t)i * c->ny + j) * c->nz + k;
+                    X += c->m[0][id];
+                    dX += dm[0][id];
+                }
+        X /= (double)np;
+        dX /= (double)np;
+        double wr = c->dt * cos(-w * c->time), wi = c->dt * sin(-w * c->time);
+        Sr += X * wr; Si += X * wi;
+        dSr += dX * wr; dSi += dX * wi;
+    }
+    *loss = Sr * Sr + Si * Si;
+    *dloss = 2. * (Sr * dSr + Si * dSi);
+    for (int a = 0; a < 3; a++) {
+        free(dm[a]); free(dms[a]); free(dHd[a]);
+        for (int s = 0; s < 4; s++) free(dk[s][a]);
+    }
+    return 0;
+}

@@ -29,6 +29,8 @@ typedef struct gfo_ctx gfo_ctx;
 #define GFO_FIELD_Q_STORE 101
 /* oracle-only field: the elastic velocity increment of stage 1 */
 #define GFO_FIELD_DV1 102
+/* oracle-only field: the FM increment k of the last stage-1 get_dm */
+#define GFO_FIELD_K1 103
 
 gfo_ctx* gfo_create(const gf_desc* desc);
 void gfo_destroy(gfo_ctx* c);
@@ -51,6 +53,14 @@ void gfo_demag(gfo_ctx* c);                   /* get_H_static / _1D           */
 void gfo_step(gfo_ctx* c, int64_t nsteps);    /* main.cpp:243-483             */
 void gfo_average(const gfo_ctx* c, int field_id, double out[3]);
 double gfo_time(const gfo_ctx* c);
+
+/* Tangent-linear reference of the adjoint sweep's probe spectral loss:
+ * nsteps FM steps from the current state; loss = |sum_n X_n e^{-i w t_n} dt|^2
+ * with X_n the probe mean of m_x; dloss = its derivative along
+ * (dpar = dalpha, dAex, dB1 of `material`, dhdrive 3n or NULL, dm0 3n or
+ * NULL).  Returns 0, or -1 for an unsupported context. */
+int gfo_tl_probe_loss(gfo_ctx* c, int material, const int lo[3], const int hi[3], double freq, int64_t nsteps,
+                      const double dpar[3], const double* dhdrive, const double* dm0, double* loss, double* dloss);
 
 /* building blocks, exposed so tests can pin them one by one */
 void gfo_newell_real(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz,

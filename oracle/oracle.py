@@ -17,6 +17,7 @@ LIB = HERE / "liboracle.so"
 FIELD_DM = 100   # oracle-only: combined RK4 increment of the last step
 FIELD_Q_STORE = 101   # oracle-only: ferroelectric q store
 FIELD_DV1 = 102       # oracle-only: elastic velocity increment of stage 1
+FIELD_K1 = 103        # oracle-only: FM increment of the last stage-1 get_dm
 
 _lib = None
 
@@ -57,6 +58,8 @@ def lib() -> ctypes.CDLL:
         "gfo_rfft3": ([i64, i64, i64, dp, dp], None),
         "gfo_irfft3": ([i64, i64, i64, dp, dp], None),
         "gfo_rdft3_naive": ([i64, i64, i64, dp, dp], None),
+        "gfo_tl_probe_loss": ([P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                               ctypes.c_double, i64, dp, dp, dp, dp, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -148,6 +151,22 @@ class Oracle:
         out = np.zeros(3)
         self.L.gfo_average(self._h, field, _dp(out))
         return out
+
+    def tl_probe_loss(self, material, lo, hi, freq, nsteps, dpar=(0.0, 0.0, 0.0), dhdrive=None, dm0=None):
+        """Tangent-linear reference: (L, dL) of the probe spectral loss over
+        nsteps steps from the current state along (dpar, dhdrive, dm0)."""
+        lo_a = (ctypes.c_int * 3)(*[int(v) for v in lo])
+        hi_a = (ctypes.c_int * 3)(*[int(v) for v in hi])
+        dp3 = np.ascontiguousarray(dpar, dtype=np.float64)
+        dh = None if dhdrive is None else np.ascontiguousarray(dhdrive, dtype=np.float64).reshape(-1)
+        d0 = None if dm0 is None else np.ascontiguousarray(dm0, dtype=np.float64).reshape(-1)
+        L, dL = np.zeros(1), np.zeros(1)
+        null = ctypes.POINTER(ctypes.c_double)()
+        st = self.L.gfo_tl_probe_loss(self._h, int(material), lo_a, hi_a, float(freq), int(nsteps), _dp(dp3),
+                                      null if dh is None else _dp(dh), null if d0 is None else _dp(d0), _dp(L), _dp(dL))
+        if st != 0:
+            raise ValueError("unsupported context for the tangent-linear reference")
+        return float(L[0]), float(dL[0])
 
     @property
     def time(self):

@@ -174,3 +174,86 @@ def test_afm_resonance_frequency(oracle_lib):
         if best is None or r < best[0]:
             best = (r, w)
     assert abs(best[1] / omega - 1) < 5e-6, best[1] / omega - 1
+
+
+def test_shaped_drive_reduces_to_uniform_drive(oracle_lib):
+    """A shaped drive whose profile is the same vector in every cell is the
+    uniform AC drive (magnetic_solver.cpp:1834-1861): same stage times, same
+    sums, so the trajectories agree bit for bit."""
+    from goferro import FIELD_HDRIVE, FLAG_HDRIVE
+    shape = (4, 3, 2)
+    h, f = np.array([3e4, -1e4, 2e4]), 9e9
+    base = dict(shape=shape, spacing=(2e-9,) * 3, material_cell=np.ones(shape, np.uint32), dt=2e-14,
+                periodic=(True, True, False), Hext_stat=(0, 0, 5e4))
+    mat = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.02, K1=-2e4, Aex=3.7e-12)
+    m0 = np.random.default_rng(4).standard_normal((3,) + shape)
+    m0 /= np.linalg.norm(m0, axis=0)
+    a = oracle_lib.Oracle(Problem(materials=[mat], Hext_altr=tuple(h), H_altr_freq=(f, f, f), **base))
+    b = oracle_lib.Oracle(Problem(materials=[mat], flags=FLAG_HDRIVE, Hdrive_freq=f, **base))
+    b.set(FIELD_HDRIVE, np.broadcast_to(h.reshape(3, 1, 1, 1), (3,) + shape).copy())
+    for o in (a, b):
+        o.set(FIELD_M, m0)
+        o.step(7)
+    assert np.array_equal(a.get(FIELD_M), b.get(FIELD_M))
+    assert np.abs(a.get(FIELD_M) - m0).max() > 1e-4   # the drive did act
+
+
+def test_shaped_drive_is_local_and_linear(oracle_lib):
+    """A profile in one cell only: the stage increment of that cell is the
+    drive's precession term at the stage time, other
+    cells are untouched after one stage (uniform m: no exchange field)."""
+    from goferro import FIELD_HDRIVE, FLAG_HDRIVE
+    shape = (3, 3, 1)
+    mat = Material.make(if_FM=1, anisotropy_type=0, Ms=1.4e5, gyro=GYRO, FM_damping=0.0, Aex=1e-11)
+    prob = Problem(shape=shape, spacing=(2e-9,) * 3, materials=[mat], material_cell=np.ones(shape, np.uint32),
+                   dt=1e-13, periodic=(False, False, False), flags=FLAG_HDRIVE, Hdrive_freq=2e10)
+    m0 = np.zeros((3,) + shape)
+    m0[0] = 1.0
+    prof = np.zeros((3,) + shape)
+    prof[2, 1, 1, 0] = 1e5
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_M, m0)
+    o.set(FIELD_HDRIVE, prof)
+    o.time_advance()
+    o.L.gfo_mag_get_dm(o._h, 1)
+    o.L.gfo_mag_update(o._h, 1)
+    ms = o.get(FIELD_M_STORE)
+    # stage 1 samples t - dt = 0: sin(0) = 0, nothing moves yet
+    np.testing.assert_array_equal(ms, m0)
+    o.L.gfo_mag_get_dm(o._h, 2)
+    o.L.gfo_mag_update(o._h, 2)
+    ms = o.get(FIELD_M_STORE)
+    # stage 2 samples t - dt/2: k2 = -gyro dt (m x H), m = x, H = h z  ->  k_y = +gyro dt h
+    s = np.sin(2 * np.pi * 2e10 * 0.5e-13)
+    ky = GYRO * 1e-13 * 1e5 * s
+    np.testing.assert_allclose(ms[1, 1, 1, 0], 0.5 * ky, rtol=1e-14)
+    ms[1, 1, 1, 0] = 0.0
+    np.testing.assert_array_equal(ms, m0)
+
+
+def test_static_strain_equals_equivalent_anisotropy(oracle_lib):
+    """Static strain ezz with B1 acts as a uniaxial anisotropy along z with
+    K1 = -B1 ezz (magnetic_solver.cpp:124-150): H_z = -2 B1 ezz mz / (mu0 Ms)
+    against 2 K1 mz / (mu0 Ms).  Equal up to rounding of the two forms."""
+    from goferro import FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FLAG_ELASTO_ON_PM
+    shape = (4, 4, 2)
+    B1, ezz = 3.5e5, 2e-3
+    base = dict(shape=shape, spacing=(2e-9,) * 3, material_cell=np.ones(shape, np.uint32), dt=1e-14,
+                periodic=(True, True, False), Hext_stat=(0, 0, 1e4))
+    m_me = Material.make(if_FM=1, anisotropy_type=0, Ms=1.4e5, gyro=GYRO, FM_damping=0.01, Aex=3.7e-12, B1=B1, B2=7e5)
+    m_an = Material.make(if_FM=1, anisotropy_type=2, Ms=1.4e5, gyro=GYRO, FM_damping=0.01, Aex=3.7e-12,
+                         K1=-B1 * ezz)
+    m0 = np.random.default_rng(5).standard_normal((3,) + shape)
+    m0 /= np.linalg.norm(m0, axis=0)
+    a = oracle_lib.Oracle(Problem(materials=[m_me], flags=FLAG_ELASTO_ON_PM, **base))
+    diag = np.zeros((3,) + shape)
+    diag[2] = ezz
+    a.set(FIELD_STRAIN_T0_DIAG, diag)
+    a.set(FIELD_STRAIN_T0_OFF, np.zeros((3,) + shape))
+    b = oracle_lib.Oracle(Problem(materials=[m_an], **base))
+    for o in (a, b):
+        o.set(FIELD_M, m0)
+        o.step(20)
+    d = np.abs(a.get(FIELD_M) - b.get(FIELD_M)).max()
+    assert d <= 1e-13, d
+    assert np.abs(a.get(FIELD_M) - m0).max() > 1e-3
