@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kThreads) zfwd_kernel(ZArgs a) {
                 }
             }
         }
-        buf[r * a.nz + bitrev(k, a.lognz)] = make_double2(v, 0.);
+        buf[fpad(r * a.nz + bitrev(k, a.lognz))] = make_double2(v, 0.);
     }
     __syncthreads();
     fft_dit(buf, a.pencils, a.lognz, tw);
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(kThreads) zfwd_kernel(ZArgs a) {
         const int r = idx & (a.pencils - 1);
         const int kz = idx >> a.logp;
         const int y = y0 + r;
-        if (y < a.ny) out[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y] = buf[r * a.nz + kz];
+        if (y < a.ny) out[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y] = buf[fpad(r * a.nz + kz)];
     }
 }
 
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
         const int x = x0 + r;
         double2 v = make_double2(0., 0.);
         if (x < a.nx) v = in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y];
-        buf[r * a.ny + bitrev(y, a.logny)] = v;
+        buf[fpad(r * a.ny + bitrev(y, a.logny))] = v;
     }
     __syncthreads();
     fft_dit(buf, a.pencils, a.logny, tw);
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
         const int r = idx & (a.pencils - 1);
         const int y = idx >> a.logp;
         const int x = x0 + r;
-        if (x < a.nx) out[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[r * a.ny + y];
+        if (x < a.nx) out[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[fpad(r * a.ny + y)];
     }
 }
 
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
         const int x = x0 + r;
         double2 v = make_double2(0., 0.);
         if (x < a.nx) v = in[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x];
-        buf[r * a.ny + y] = v;
+        buf[fpad(r * a.ny + y)] = v;
     }
     __syncthreads();
     ifft_dif(buf, a.pencils, a.logny, tw);
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
         const int r = idx >> a.logny;
         const int y = idx & (a.ny - 1);
         const int x = x0 + r;
-        if (x < a.nx) out[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y] = buf[r * a.ny + bitrev(y, a.logny)];
+        if (x < a.nx) out[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y] = buf[fpad(r * a.ny + bitrev(y, a.logny))];
     }
 }
 
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
         const int y = y0 + r;
         double2 v = make_double2(0., 0.);
         if (y < a.ny) v = a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x];
-        buf[t * a.nx + bitrev(x, a.lognx)] = v;
+        buf[fpad(t * a.nx + bitrev(x, a.lognx))] = v;
     }
     __syncthreads();
     fft_dit(buf, comps * a.pencils, a.lognx, tw);
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
             const int ix = (0 * a.pencils + r) * a.nx + x;
             const int iy = (1 * a.pencils + r) * a.nx + x;
             const int iz = (2 * a.pencils + r) * a.nx + x;
-            const double2 mx = buf[ix], my = buf[iy], mz = buf[iz];
+            const double2 mx = buf[fpad(ix)], my = buf[fpad(iy)], mz = buf[fpad(iz)];
             const int64_t s = a.nh;
             // the adjoint convolution uses N^T with conjugated spectra: xy<->yx,
             // yz<->zy, zx<->xz (the real-space correlation with the kernel)
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
             const double2 hy = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
             t0 = cmul(mz, K(2)); t1 = cmul(mx, K(5)); t2 = cmul(my, K(7));
             const double2 hz = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
-            buf[ix] = hx; buf[iy] = hy; buf[iz] = hz;
+            buf[fpad(ix)] = hx; buf[fpad(iy)] = hy; buf[fpad(iz)] = hz;
         }
         __syncthreads();
         ifft_dif(buf, comps * a.pencils, a.lognx, tw);
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
         const int y = y0 + r;
         if (y < a.ny) {
             const int src = kConv ? bitrev(x, a.lognx) : x;
-            a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[t * a.nx + src];
+            a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[fpad(t * a.nx + src)];
         }
     }
 }
@@ -278,9 +278,9 @@ __global__ void __launch_bounds__(kThreads) zinv_kernel(ZInvArgs a) {
         const int y = y0 + r;
         double2 v = make_double2(0., 0.);
         if (y < a.ny) v = in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y];
-        buf[r * a.nz + kz] = v;
+        buf[fpad(r * a.nz + kz)] = v;
         const int mirror = a.nz - kz;
-        if (kz >= 1 && mirror > kz && mirror < a.nz) buf[r * a.nz + mirror] = make_double2(v.x, -v.y);
+        if (kz >= 1 && mirror > kz && mirror < a.nz) buf[fpad(r * a.nz + mirror)] = make_double2(v.x, -v.y);
     }
     __syncthreads();
     ifft_dif(buf, a.pencils, a.lognz, tw);
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kThreads) zinv_kernel(ZInvArgs a) {
         const int r = idx >> a.lognz;
         const int k = idx & (a.nz - 1);
         const int y = y0 + r;
-        if (y < a.ny) out[(static_cast<int64_t>(x) * a.ny + y) * a.nz + k] = buf[r * a.nz + bitrev(k, a.lognz)].x;
+        if (y < a.ny) out[(static_cast<int64_t>(x) * a.ny + y) * a.nz + k] = buf[fpad(r * a.nz + bitrev(k, a.lognz))].x;
     }
 }
 
@@ -396,9 +396,9 @@ Shapes shapes(const gf_ctx* c) {
     s.pz = pick_pencils(c->nz, bz, cz, c->ny);
     s.py = pick_pencils(c->ny, by, cy, c->nx);
     s.px = pick_pencils(c->nx, bx, cx, c->ny);
-    s.lds_z = sizeof(double2) * (c->nz / 2 + s.pz * c->nz);
-    s.lds_y = sizeof(double2) * (c->ny / 2 + s.py * c->ny);
-    s.lds_x = sizeof(double2) * (c->nx / 2 + 3 * s.px * c->nx);
+    s.lds_z = sizeof(double2) * (c->nz / 2 + fft_lds_elems(s.pz * c->nz));
+    s.lds_y = sizeof(double2) * (c->ny / 2 + fft_lds_elems(s.py * c->ny));
+    s.lds_x = sizeof(double2) * (c->nx / 2 + fft_lds_elems(3 * s.px * c->nx));
     return s;
 }
 
@@ -544,7 +544,7 @@ gf_status demag_setup(gf_ctx* c) {
         }
         auto lds = [&](int p) {
             const int pairs = p > 1 ? p / 2 : 1;
-            return sizeof(double2) * (c->nz / 2 + 3 * pairs * c->nz);
+            return sizeof(double2) * (c->nz / 2 + fft_lds_elems(3 * pairs * c->nz));
         };
         while (P * 2 <= pmax && P * 2 <= c->ny && lds(P * 2) <= 64 * 1024) P *= 2;
         if (lds(P) <= 48 * 1024) { d.fused = true; d.P = P; }

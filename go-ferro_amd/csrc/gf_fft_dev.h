@@ -12,6 +12,13 @@ __device__ __forceinline__ int fft_bitrev(int j, int logn) {
     return logn ? static_cast<int>(__brev(static_cast<unsigned>(j)) >> (32 - logn)) : 0;
 }
 
+// LDS slot of element i of a transform buffer: one spare slot after every
+// 16 elements, so that the power-of-two strides of the butterflies and of
+// the bit-reversed loads do not pile onto the same LDS banks.  A buffer of
+// m elements takes fft_lds_elems(m) slots.
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+__host__ __device__ __forceinline__ int64_t fft_lds_elems(int64_t m) { return m + (m >> 4) + 1; }
+
 __device__ __forceinline__ double2 fft_cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -29,11 +36,11 @@ __device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, con
         for (int b = threadIdx.x; b < total; b += blockDim.x) {
             const int i0 = 2 * b, i1 = i0 + 1;
             const double2 w = tw[0];
-            const double2 u = buf[i0];
-            const double2 v = buf[i1];
+            const double2 u = buf[fpad(i0)];
+            const double2 v = buf[fpad(i1)];
             const double2 t = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
-            buf[i0] = make_double2(u.x + t.x, u.y + t.y);
-            buf[i1] = make_double2(u.x - t.x, u.y - t.y);
+            buf[fpad(i0)] = make_double2(u.x + t.x, u.y + t.y);
+            buf[fpad(i1)] = make_double2(u.x - t.x, u.y - t.y);
         }
         __syncthreads();
         s = 1;
@@ -47,7 +54,7 @@ __device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, con
             const int q = b & ((N >> 2) - 1);
             const int j = q & (h - 1);
             const int g = p * N + ((q >> s) << (s + 2)) + j;
-            double2 a0 = buf[g], a1 = buf[g + h], a2 = buf[g + 2 * h], a3 = buf[g + 3 * h];
+            double2 a0 = buf[fpad(g)], a1 = buf[fpad(g + h)], a2 = buf[fpad(g + 2 * h)], a3 = buf[fpad(g + 3 * h)];
             // stage s: (a0, a1) and (a2, a3), twiddle j at span 2h
             double2 w = tw[j * ts0];
             double2 t = make_double2(a1.x * w.x - a1.y * w.y, a1.x * w.y + a1.y * w.x);
@@ -57,12 +64,12 @@ __device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, con
             // stage s+1: (b0, b2) twiddle j, (b1, b3) twiddle j + h, at span 4h
             w = tw[j * ts1];
             t = make_double2(b2.x * w.x - b2.y * w.y, b2.x * w.y + b2.y * w.x);
-            buf[g] = make_double2(b0.x + t.x, b0.y + t.y);
-            buf[g + 2 * h] = make_double2(b0.x - t.x, b0.y - t.y);
+            buf[fpad(g)] = make_double2(b0.x + t.x, b0.y + t.y);
+            buf[fpad(g + 2 * h)] = make_double2(b0.x - t.x, b0.y - t.y);
             w = tw[(j + h) * ts1];
             t = make_double2(b3.x * w.x - b3.y * w.y, b3.x * w.y + b3.y * w.x);
-            buf[g + h] = make_double2(b1.x + t.x, b1.y + t.y);
-            buf[g + 3 * h] = make_double2(b1.x - t.x, b1.y - t.y);
+            buf[fpad(g + h)] = make_double2(b1.x + t.x, b1.y + t.y);
+            buf[fpad(g + 3 * h)] = make_double2(b1.x - t.x, b1.y - t.y);
         }
         __syncthreads();
     }
@@ -82,7 +89,7 @@ __device__ __forceinline__ void ifft_dif(double2* buf, int pencils, int logn, co
             const int q = b & ((N >> 2) - 1);
             const int j = q & (h2 - 1);
             const int g = p * N + ((q >> (s - 1)) << (s + 1)) + j;
-            const double2 a0 = buf[g], a1 = buf[g + h2], a2 = buf[g + h], a3 = buf[g + h + h2];
+            const double2 a0 = buf[fpad(g)], a1 = buf[fpad(g + h2)], a2 = buf[fpad(g + h)], a3 = buf[fpad(g + h + h2)];
             // stage s: (a0, a2) twiddle j, (a1, a3) twiddle j + h/2, at span 2h
             double2 w = make_double2(tw[j * ts0].x, -tw[j * ts0].y);
             double2 d = make_double2(a0.x - a2.x, a0.y - a2.y);
@@ -95,11 +102,11 @@ __device__ __forceinline__ void ifft_dif(double2* buf, int pencils, int logn, co
             // stage s-1: (b0, b1) and (b2, b3), twiddle j at span h
             w = make_double2(tw[j * ts1].x, -tw[j * ts1].y);
             d = make_double2(b0.x - b1.x, b0.y - b1.y);
-            buf[g] = make_double2(b0.x + b1.x, b0.y + b1.y);
-            buf[g + h2] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+            buf[fpad(g)] = make_double2(b0.x + b1.x, b0.y + b1.y);
+            buf[fpad(g + h2)] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
             d = make_double2(b2.x - b3.x, b2.y - b3.y);
-            buf[g + h] = make_double2(b2.x + b3.x, b2.y + b3.y);
-            buf[g + h + h2] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+            buf[fpad(g + h)] = make_double2(b2.x + b3.x, b2.y + b3.y);
+            buf[fpad(g + h + h2)] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
         }
         __syncthreads();
     }
@@ -109,11 +116,11 @@ __device__ __forceinline__ void ifft_dif(double2* buf, int pencils, int logn, co
         for (int b = threadIdx.x; b < total; b += blockDim.x) {
             const int i0 = 2 * b, i1 = i0 + 1;
             const double2 w = make_double2(tw[0].x, -tw[0].y);
-            const double2 u = buf[i0];
-            const double2 v = buf[i1];
+            const double2 u = buf[fpad(i0)];
+            const double2 v = buf[fpad(i1)];
             const double2 d = make_double2(u.x - v.x, u.y - v.y);
-            buf[i0] = make_double2(u.x + v.x, u.y + v.y);
-            buf[i1] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+            buf[fpad(i0)] = make_double2(u.x + v.x, u.y + v.y);
+            buf[fpad(i1)] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
         }
         __syncthreads();
     }

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
     const int P = f.P, lognz = f.lognz, nzh = f.nzh;
     const int pairs = P > 1 ? P >> 1 : 1;
     double2* tw = fsm;
-    double2* cbuf = tw + (nz >> 1);          // [3][pairs][nz]
+    double2* cbuf = tw + (nz >> 1);          // [3][pairs][nz], element i at slot fpad(i)
     const unsigned b = blockIdx.x, G = gridDim.x;
     const unsigned q = G / 8u, r = G % 8u, xcd = b % 8u, slot = b / 8u;
     const unsigned L = xcd * q + (xcd < r ? xcd : r) + slot;
@@ -178,9 +178,9 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
             double2 A = in[row + p];
             double2 B = P > 1 ? in[row + p + pairs] : make_double2(0., 0.);
             if (kz == 0 || 2 * kz == nz) { A.y = 0.; B.y = 0.; }
-            double2* cb = cbuf + (c * pairs + p) * nz;
-            cb[kz] = make_double2(A.x - B.y, A.y + B.x);
-            if (kz > 0 && 2 * kz < nz) cb[nz - kz] = make_double2(A.x + B.y, B.x - A.y);
+            const int cb = (c * pairs + p) * nz;
+            cbuf[fpad(cb + kz)] = make_double2(A.x - B.y, A.y + B.x);
+            if (kz > 0 && 2 * kz < nz) cbuf[fpad(cb + nz - kz)] = make_double2(A.x + B.y, B.x - A.y);
         }
         __syncthreads();
         ifft_dif(cbuf, 3 * pairs, lognz, tw);
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
             const int col = w >> lognz, k = w & (nz - 1);
             const double h = a.hstat[c][(static_cast<int64_t>(x) * ny + y0 + col) * nz + k];
             const int p = col < pairs ? col : col - pairs;
-            double* slot2 = reinterpret_cast<double*>(cbuf + (c * pairs + p) * nz + fft_bitrev(k, lognz));
+            double* slot2 = reinterpret_cast<double*>(cbuf + fpad((c * pairs + p) * nz + fft_bitrev(k, lognz)));
             slot2[col < pairs ? 0 : 1] = h;
         }
         __syncthreads();
@@ -201,9 +201,9 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
         const int p = col < pairs ? col : col - pairs;
         const int part = col < pairs ? 0 : 1;
         const int pos = p * nz + fft_bitrev(k, lognz);
-        double* h0 = reinterpret_cast<double*>(cbuf + pos) + part;
-        double* h1 = reinterpret_cast<double*>(cbuf + pairs * nz + pos) + part;
-        double* h2 = reinterpret_cast<double*>(cbuf + 2 * pairs * nz + pos) + part;
+        double* h0 = reinterpret_cast<double*>(cbuf + fpad(pos)) + part;
+        double* h1 = reinterpret_cast<double*>(cbuf + fpad(pairs * nz + pos)) + part;
+        double* h2 = reinterpret_cast<double*>(cbuf + fpad(2 * pairs * nz + pos)) + part;
         const I id = (static_cast<I>(x) * static_cast<I>(ny) + static_cast<I>(y)) * static_cast<I>(nz) + static_cast<I>(k);
         const uint32_t own = ldu(a.code, id);
         double mx = 0., my = 0., mz = 0.;
@@ -226,9 +226,9 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
     for (int t = threadIdx.x; t < 3 * per; t += blockDim.x) {
         const int c = t / per, u = t - c * per;
         const int p = u % pairs, kz = u / pairs;
-        const double2* cb = cbuf + (c * pairs + p) * nz;
-        const double2 Z = cb[kz];
-        const double2 Zm = cb[(nz - kz) & (nz - 1)];
+        const int cb = (c * pairs + p) * nz;
+        const double2 Z = cbuf[fpad(cb + kz)];
+        const double2 Zm = cbuf[fpad(cb + ((nz - kz) & (nz - 1)))];
         const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
         double2* out = f.mspec + c * f.nh;
         if (P > 1) {
@@ -421,7 +421,7 @@ hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s) {
 hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool hspec, hipStream_t s) {
     const int nz = static_cast<int>(a.nz);
     const int pairs = f.P > 1 ? f.P / 2 : 1;
-    const size_t lds = sizeof(double2) * (nz / 2 + 3 * pairs * nz);
+    const size_t lds = sizeof(double2) * (nz / 2 + fft_lds_elems(3 * pairs * nz));
     const unsigned groups = static_cast<unsigned>(a.nx * a.ny / f.P);
     const bool n32 = narrow(a.nx * a.ny * a.nz);
     if (hspec && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, uint32_t>), dim3(groups), dim3(256), lds, s, a, f);
