@@ -48,14 +48,47 @@ struct P3 {
     const double* p[3];
 };
 
-template <bool kDmi, class I>
-__device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, const DevMaterial* mats,
-                                           uint32_t own, int surf_shift, int sub, I i, I j, I k,
-                                           I nx, I ny, I nz, double dx, double dy, double dz,
-                                           double A, double A_self, double slope) {
-    const I id = (i * ny + j) * nz + k;
-    const double* s[3] = {s3.p[0], s3.p[1], s3.p[2]};
-    const double c0 = ldd(s[0], id), c1 = ldd(s[1], id), c2 = ldd(s[2], id);
+// The six face neighbours of cell (i, j, k) in the order x-, x+, y-, y+,
+// z-, z+, with the reference's wrap (used or not, every index is in
+// bounds), and their cell codes.  Loaded whether or not a face is a
+// surface, so that the loads do not wait for the cell's own code.
+template <class I>
+__device__ __forceinline__ void gather6(const P3 s3, const uint32_t* code, I i, I j, I k, I nx, I ny, I nz,
+                                        double (&v)[6][3], uint32_t (&cd)[6]) {
+    I nb[6];
+    nb[0] = ((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k;
+    nb[1] = ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k;
+    nb[2] = (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k;
+    nb[3] = (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k;
+    nb[4] = (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1);
+    nb[5] = (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1);
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        v[f][0] = ldd(s3.p[0], nb[f]);
+        v[f][1] = ldd(s3.p[1], nb[f]);
+        v[f][2] = ldd(s3.p[2], nb[f]);
+        cd[f] = ldu(code, nb[f]);
+    }
+}
+
+// Face coefficient between a cell of material `mat` (constant A, self mean
+// A_self) and a neighbour of code nc: the harmonic mean 2 A1 A2 / (A1 + A2)
+// of magnetic_initialize.cpp:381-524, A_self for the same material.
+__device__ __forceinline__ double face_coef(const DevMaterial* mats, uint32_t mat, uint32_t nc, int sub, double A,
+                                            double A_self) {
+    const uint32_t nm = nc & kMatMask;
+    if (nm == mat) return A_self;
+    const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
+    return 2. * A * a2 / (A + a2);
+}
+
+// The stencil of get_m_spatial_derivative_local from the cell's store
+// (c0, c1, c2) and its gathered neighbours; see gather6.
+template <bool kDmi>
+__device__ __forceinline__ Stencil stencil_core(double c0, double c1, double c2, const double (&v)[6][3],
+                                                const uint32_t (&cd)[6], const DevMaterial* mats, uint32_t own,
+                                                int surf_shift, int sub, double dx, double dy, double dz, double A,
+                                                double A_self, double slope) {
     const uint32_t mat = own & kMatMask;
     // one reciprocal per spacing instead of a division per difference: each
     // quotient moves by at most an ulp against the reference's c/dx
@@ -63,107 +96,75 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
     Stencil r;
     double bw[3], fw[3], ab, af;
     // x faces
-    {
-        I nb;
-        if ((own >> (surf_shift + kXB)) & 1u) {
-            if (kDmi) { bw[0] = -slope * c2; bw[1] = 0.; bw[2] = slope * c0; }
-            else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
-            ab = A;
-        } else {
-            nb = ((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k;
-            bw[0] = (c0 - ldd(s[0], nb)) * ix; bw[1] = (c1 - ldd(s[1], nb)) * ix; bw[2] = (c2 - ldd(s[2], nb)) * ix;
-            const uint32_t nm = ldu(code, nb) & kMatMask;
-            if (nm == mat) ab = A_self;
-            else {
-                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
-                ab = 2. * A * a2 / (A + a2);
-            }
-        }
-        if ((own >> (surf_shift + kXF)) & 1u) {
-            if (kDmi) { fw[0] = -slope * c2; fw[1] = 0.; fw[2] = slope * c0; }
-            else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
-            af = A;
-        } else {
-            nb = ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k;
-            fw[0] = (ldd(s[0], nb) - c0) * ix; fw[1] = (ldd(s[1], nb) - c1) * ix; fw[2] = (ldd(s[2], nb) - c2) * ix;
-            const uint32_t nm = ldu(code, nb) & kMatMask;
-            if (nm == mat) af = A_self;
-            else {
-                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
-                af = 2. * A * a2 / (A + a2);
-            }
-        }
-        r.d[0] = (bw[0] + fw[0]) * 0.5;
-        r.d[2] = (bw[2] + fw[2]) * 0.5;
-        for (int a = 0; a < 3; a++) r.ex[a] = af * (fw[a] * ix) + ab * (-bw[a] * ix);
+    if ((own >> (surf_shift + kXB)) & 1u) {
+        if (kDmi) { bw[0] = -slope * c2; bw[1] = 0.; bw[2] = slope * c0; }
+        else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+        ab = A;
+    } else {
+        bw[0] = (c0 - v[0][0]) * ix; bw[1] = (c1 - v[0][1]) * ix; bw[2] = (c2 - v[0][2]) * ix;
+        ab = face_coef(mats, mat, cd[0], sub, A, A_self);
     }
+    if ((own >> (surf_shift + kXF)) & 1u) {
+        if (kDmi) { fw[0] = -slope * c2; fw[1] = 0.; fw[2] = slope * c0; }
+        else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+        af = A;
+    } else {
+        fw[0] = (v[1][0] - c0) * ix; fw[1] = (v[1][1] - c1) * ix; fw[2] = (v[1][2] - c2) * ix;
+        af = face_coef(mats, mat, cd[1], sub, A, A_self);
+    }
+    r.d[0] = (bw[0] + fw[0]) * 0.5;
+    r.d[2] = (bw[2] + fw[2]) * 0.5;
+    for (int a = 0; a < 3; a++) r.ex[a] = af * (fw[a] * ix) + ab * (-bw[a] * ix);
     // y faces
-    {
-        I nb;
-        if ((own >> (surf_shift + kYB)) & 1u) {
-            if (kDmi) { bw[0] = 0.; bw[1] = -slope * c2; bw[2] = slope * c1; }
-            else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
-            ab = A;
-        } else {
-            nb = (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k;
-            bw[0] = (c0 - ldd(s[0], nb)) * iy; bw[1] = (c1 - ldd(s[1], nb)) * iy; bw[2] = (c2 - ldd(s[2], nb)) * iy;
-            const uint32_t nm = ldu(code, nb) & kMatMask;
-            if (nm == mat) ab = A_self;
-            else {
-                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
-                ab = 2. * A * a2 / (A + a2);
-            }
-        }
-        if ((own >> (surf_shift + kYF)) & 1u) {
-            if (kDmi) { fw[0] = 0.; fw[1] = -slope * c2; fw[2] = slope * c1; }
-            else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
-            af = A;
-        } else {
-            nb = (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k;
-            fw[0] = (ldd(s[0], nb) - c0) * iy; fw[1] = (ldd(s[1], nb) - c1) * iy; fw[2] = (ldd(s[2], nb) - c2) * iy;
-            const uint32_t nm = ldu(code, nb) & kMatMask;
-            if (nm == mat) af = A_self;
-            else {
-                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
-                af = 2. * A * a2 / (A + a2);
-            }
-        }
-        r.d[1] = (bw[1] + fw[1]) * 0.5;
-        r.d[3] = (bw[2] + fw[2]) * 0.5;
-        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] * iy) + ab * (-bw[a] * iy);
+    if ((own >> (surf_shift + kYB)) & 1u) {
+        if (kDmi) { bw[0] = 0.; bw[1] = -slope * c2; bw[2] = slope * c1; }
+        else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
+        ab = A;
+    } else {
+        bw[0] = (c0 - v[2][0]) * iy; bw[1] = (c1 - v[2][1]) * iy; bw[2] = (c2 - v[2][2]) * iy;
+        ab = face_coef(mats, mat, cd[2], sub, A, A_self);
     }
+    if ((own >> (surf_shift + kYF)) & 1u) {
+        if (kDmi) { fw[0] = 0.; fw[1] = -slope * c2; fw[2] = slope * c1; }
+        else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
+        af = A;
+    } else {
+        fw[0] = (v[3][0] - c0) * iy; fw[1] = (v[3][1] - c1) * iy; fw[2] = (v[3][2] - c2) * iy;
+        af = face_coef(mats, mat, cd[3], sub, A, A_self);
+    }
+    r.d[1] = (bw[1] + fw[1]) * 0.5;
+    r.d[3] = (bw[2] + fw[2]) * 0.5;
+    for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] * iy) + ab * (-bw[a] * iy);
     // z faces: zero slope at surfaces for both kinds
-    {
-        I nb;
-        if ((own >> (surf_shift + kZB)) & 1u) {
-            bw[0] = 0.; bw[1] = 0.; bw[2] = 0.;
-            ab = A;
-        } else {
-            nb = (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1);
-            bw[0] = (c0 - ldd(s[0], nb)) * iz; bw[1] = (c1 - ldd(s[1], nb)) * iz; bw[2] = (c2 - ldd(s[2], nb)) * iz;
-            const uint32_t nm = ldu(code, nb) & kMatMask;
-            if (nm == mat) ab = A_self;
-            else {
-                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
-                ab = 2. * A * a2 / (A + a2);
-            }
-        }
-        if ((own >> (surf_shift + kZF)) & 1u) {
-            fw[0] = 0.; fw[1] = 0.; fw[2] = 0.;
-            af = A;
-        } else {
-            nb = (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1);
-            fw[0] = (ldd(s[0], nb) - c0) * iz; fw[1] = (ldd(s[1], nb) - c1) * iz; fw[2] = (ldd(s[2], nb) - c2) * iz;
-            const uint32_t nm = ldu(code, nb) & kMatMask;
-            if (nm == mat) af = A_self;
-            else {
-                const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
-                af = 2. * A * a2 / (A + a2);
-            }
-        }
-        for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] * iz) + ab * (-bw[a] * iz);
+    if ((own >> (surf_shift + kZB)) & 1u) {
+        bw[0] = 0.; bw[1] = 0.; bw[2] = 0.;
+        ab = A;
+    } else {
+        bw[0] = (c0 - v[4][0]) * iz; bw[1] = (c1 - v[4][1]) * iz; bw[2] = (c2 - v[4][2]) * iz;
+        ab = face_coef(mats, mat, cd[4], sub, A, A_self);
     }
+    if ((own >> (surf_shift + kZF)) & 1u) {
+        fw[0] = 0.; fw[1] = 0.; fw[2] = 0.;
+        af = A;
+    } else {
+        fw[0] = (v[5][0] - c0) * iz; fw[1] = (v[5][1] - c1) * iz; fw[2] = (v[5][2] - c2) * iz;
+        af = face_coef(mats, mat, cd[5], sub, A, A_self);
+    }
+    for (int a = 0; a < 3; a++) r.ex[a] = r.ex[a] + af * (fw[a] * iz) + ab * (-bw[a] * iz);
     return r;
+}
+
+template <bool kDmi, class I>
+__device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, const DevMaterial* mats,
+                                           uint32_t own, int surf_shift, int sub, I i, I j, I k,
+                                           I nx, I ny, I nz, double dx, double dy, double dz,
+                                           double A, double A_self, double slope) {
+    const I id = (i * ny + j) * nz + k;
+    const double c0 = ldd(s3.p[0], id), c1 = ldd(s3.p[1], id), c2 = ldd(s3.p[2], id);
+    double v[6][3];
+    uint32_t cd[6];
+    gather6<I>(s3, code, i, j, k, nx, ny, nz, v, cd);
+    return stencil_core<kDmi>(c0, c1, c2, v, cd, mats, own, surf_shift, sub, dx, dy, dz, A, A_self, slope);
 }
 
 // Magnetoelastic field of a crystal-frame strain, in the global frame:
