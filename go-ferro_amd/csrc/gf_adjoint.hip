@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
         for (int c = 0; c < 3; c++) { std_(a.mbar[c], id, 0.); std_(a.hbar[c], id, 0.); }
         return;
     }
-    const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+    GMat& M = mat_of(a.mats, own);
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(256) adj_combine_kernel(AdjCombineArgs a) {
         if (a.mode == 0) for (int c = 0; c < 3; c++) std_(a.ybar[c], id, 0.);
         return;
     }
-    const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+    GMat& M = mat_of(a.mats, own);
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;

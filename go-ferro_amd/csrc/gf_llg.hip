@@ -47,7 +47,7 @@ __device__ __forceinline__ double rk_update(int stage, I id, int a, double k, co
 template <class I>
 __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, I x, I y, I z,
                                         double hs0, double hs1, double hs2, double& n0, double& n1, double& n2) {
-    const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+    GMat& M = mat_of(a.mats, own);
 
     const double mgx = ldd(a.store[0], id), mgy = ldd(a.store[1], id), mgz = ldd(a.store[2], id);
     double mx = mgx, my = mgy, mz = mgz;
@@ -246,7 +246,7 @@ template <class I>
 __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) {
     const uint32_t own = ldu(a.code, id);
     if (!(own & kIsAFM)) return;
-    const DevMaterial& M = a.mats[(own & kMatMask) - 1];
+    GMat& M = mat_of(a.mats, own);
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
 // L.item[b], the chunk of L.cpt * blockDim.x cells starting at
 // (b - L.block0[instance]) * L.cpt * blockDim.x, one cell per thread per
 // pass.  A grid smaller than L.total walks the slots with a stride of
-// gridDim.x; for the AFM kernel blockIdx.y selects the sublattice.
+// gridDim.x.
 template <class I>
 __global__ void __launch_bounds__(256) mag_batch_kernel(const StageArgs* __restrict__ items, BatchLaunch L, int stage) {
     for (uint32_t b = blockIdx.x; b < L.total; b += gridDim.x) {
@@ -343,10 +343,29 @@ __global__ void __launch_bounds__(256) mag_batch_kernel(const StageArgs* __restr
     }
 }
 
-template <class I>
+// Work item w of 2 * ceil(total / 8) * 8: slot (w / 16) * 8 + w % 8 and
+// sublattice (w / 8) % 2, so that the two sublattices of a slot run on one
+// XCD one dispatch round apart (see afm_stage_kernel).
+// kOne: one work item per workgroup and one cell per thread (no loops).
+template <class I, bool kOne>
 __global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __restrict__ items, BatchLaunch L,
                                                          int stage) {
-    for (uint32_t b = blockIdx.x; b < L.total; b += gridDim.x) {
+    const uint32_t work = 2u * ((L.total + 7u) / 8u * 8u);
+    if (kOne) {
+        const uint32_t w = blockIdx.x;
+        const uint32_t b = (w >> 4) * 8u + (w & 7u);
+        if (w >= work || b >= L.total) return;
+        const uint32_t inst = L.item[b];
+        const AfmStageArgs& a = items[4 * inst + stage - 1];
+        const I n = static_cast<I>(a.nx * a.ny * a.nz);
+        const I id = static_cast<I>(b - L.block0[inst]) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
+        if (id < n) afm_point<I>(a, id, static_cast<int>((w >> 3) & 1u));
+        return;
+    }
+    for (uint32_t w = blockIdx.x; w < work; w += gridDim.x) {
+        const uint32_t b = (w >> 4) * 8u + (w & 7u);
+        const int sub = static_cast<int>((w >> 3) & 1u);
+        if (b >= L.total) continue;
         const uint32_t inst = L.item[b];
         const AfmStageArgs& a = items[4 * inst + stage - 1];
         const I n = static_cast<I>(a.nx * a.ny * a.nz);
@@ -354,7 +373,7 @@ __global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __re
         const I first = static_cast<I>(b - L.block0[inst]) * chunk + static_cast<I>(threadIdx.x);
         for (int j = 0; j < L.cpt; j++) {
             const I id = first + static_cast<I>(j) * static_cast<I>(blockDim.x);
-            if (id < n) afm_point<I>(a, id, static_cast<int>(blockIdx.y));
+            if (id < n) afm_point<I>(a, id, sub);
         }
     }
 }
@@ -446,8 +465,17 @@ hipError_t launch_mag_batch(const StageArgs* items, const BatchLaunch& L, int st
 }
 
 hipError_t launch_afm_batch(const AfmStageArgs* items, const BatchLaunch& L, int stage, hipStream_t s) {
-    if (L.narrow) hipLaunchKernelGGL(afm_batch_kernel<uint32_t>, dim3(L.grid, 2), dim3(L.threads), 0, s, items, L, stage);
-    else hipLaunchKernelGGL(afm_batch_kernel<int64_t>, dim3(L.grid, 2), dim3(L.threads), 0, s, items, L, stage);
+    // L.grid counts slots; the grid covers both sublattices of each
+    const unsigned grid = 2u * ((L.grid + 7u) / 8u * 8u);
+    const bool one = L.cpt == 1 && L.grid == L.total;
+    if (L.narrow && one)
+        hipLaunchKernelGGL((afm_batch_kernel<uint32_t, true>), dim3(grid), dim3(L.threads), 0, s, items, L, stage);
+    else if (L.narrow)
+        hipLaunchKernelGGL((afm_batch_kernel<uint32_t, false>), dim3(grid), dim3(L.threads), 0, s, items, L, stage);
+    else if (one)
+        hipLaunchKernelGGL((afm_batch_kernel<int64_t, true>), dim3(grid), dim3(L.threads), 0, s, items, L, stage);
+    else
+        hipLaunchKernelGGL((afm_batch_kernel<int64_t, false>), dim3(grid), dim3(L.threads), 0, s, items, L, stage);
     return hipGetLastError();
 }
 

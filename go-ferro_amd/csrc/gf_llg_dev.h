@@ -22,17 +22,30 @@ __device__ __forceinline__ void rot(const double T[3][3], double ix, double iy, 
 // then take the base from scalar registers and one 32-bit offset from
 // vector registers, which saves registers and address arithmetic.  The
 // 32-bit form is used when every array holds fewer than 2^28 elements.
-__device__ __forceinline__ double ldd(const double* p, int64_t i) { return p[i]; }
+// Every access goes through the global address space: pointers read from
+// memory (the per-instance arguments of a batch) would otherwise be generic
+// and compile to slower flat accesses with 64-bit addresses.
+typedef const __attribute__((address_space(1))) double GConstD;
+typedef __attribute__((address_space(1))) double GD;
+typedef const __attribute__((address_space(1))) uint32_t GConstU;
+typedef const __attribute__((address_space(1))) char GConstC;
+typedef __attribute__((address_space(1))) char GC;
+
+__device__ __forceinline__ double ldd(const double* p, int64_t i) { return ((GConstD*)p)[i]; }
 __device__ __forceinline__ double ldd(const double* p, uint32_t i) {
-    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + (i << 3));
+    return *(GConstD*)((GConstC*)p + (i << 3));
 }
-__device__ __forceinline__ void std_(double* p, int64_t i, double v) { p[i] = v; }
-__device__ __forceinline__ void std_(double* p, uint32_t i, double v) {
-    *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + (i << 3)) = v;
+__device__ __forceinline__ void std_(double* p, int64_t i, double v) { ((GD*)p)[i] = v; }
+__device__ __forceinline__ void std_(double* p, uint32_t i, double v) { *(GD*)((GC*)p + (i << 3)) = v; }
+// The material record of a cell, read through the global address space.
+typedef const __attribute__((address_space(1))) DevMaterial GMat;
+__device__ __forceinline__ GMat& mat_of(const DevMaterial* mats, uint32_t code) {
+    return ((GMat*)mats)[(code & kMatMask) - 1];
 }
-__device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t i) { return p[i]; }
+
+__device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t i) { return ((GConstU*)p)[i]; }
 __device__ __forceinline__ uint32_t ldu(const uint32_t* p, uint32_t i) {
-    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(p) + (i << 2));
+    return *(GConstU*)((GConstC*)p + (i << 2));
 }
 
 // Exchange-weighted neighbour differences of one store for one cell
