@@ -219,6 +219,67 @@ def bench_adjoint(shape, nsteps: int) -> dict:
             "grad_drive_norm": float((ghd ** 2).sum() ** 0.5)}
 
 
+def bench_coupled(which: str, steps: int) -> dict:
+    """The coupled workloads of the other physics rows, timed on the GPU with
+    per-kernel-class HIP-event times: 'magnetoelastic' (128x128x64 magnet on
+    substrate, elastodynamics both ways, Liao bottom, shear pulse) and
+    'fe_thz' (8x8x1024 ferroelectric film + electrodynamics, Mur walls,
+    THz current sheet).  Per-class achieved bandwidth uses the compulsory
+    state bytes of each class per launch (each state array read and written
+    once per stage as in the fused RK4 accounting)."""
+    import goferro
+    from goferro import FIELD_M, FIELD_P, FIELD_Q, workloads
+
+    if which == "magnetoelastic":
+        prob, m = workloads.magnetoelastic_film()
+        ctx = goferro.Context(prob)
+        ctx.upload(FIELD_M, m)
+        name = "magnetoelastic film 128x128x64 (16 magnet / 48 substrate), dt 5e-15 s"
+    else:
+        prob, p, q = workloads.fe_thz_column()
+        ctx = goferro.Context(prob)
+        ctx.upload(FIELD_P, p)
+        ctx.upload(FIELD_Q, q)
+        name = "FE + FDTD column 8x8x1024 (vacuum / PbTiO3-like 200 cells / vacuum), dt 1e-18 s"
+    n = prob.n
+    ctx.step(3)
+    ctx.sync()
+    ctx.set_kernel_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    t1 = time.perf_counter()
+    ctx.step(steps)             # untimed per kernel: the event bracketing adds launch overhead
+    ctx.sync()
+    el_plain = time.perf_counter() - t1
+    ctx.close()
+    # compulsory bytes per launch of the state each class owns (fused RK4: 17 double
+    # accesses per state component per step = 4.25 per stage) plus its cross reads
+    if which == "magnetoelastic":
+        nmag = n * 16 // 64
+        per = {"elastic_stage": n * (6 * 4.25 * 8 + 4), "elastic_strain": n * (3 * 8 + 12 * 8),
+               "mag_stage": nmag * (3 * 4.25 * 8 + 6 * 8 + 4 + 24)}
+    else:
+        nfe = n * 200 // 1024
+        ne = 6.77 * n
+        per = {"em_e": ne / 2 * 4.25 * 8 + 3 * 8 * n, "em_h": ne / 2 * 4.25 * 8,
+               "fe_stage": nfe * (6 * 4.25 * 8 + 3 * 8 + 4)}
+    kernels = {}
+    for cls, (ms, cnt) in kt.items():
+        if cnt:
+            b = per.get(cls)
+            kernels[cls] = {"launches": cnt, "avg_us": 1e3 * ms / cnt,
+                            "achieved_GBs": (b / (ms / cnt * 1e-3) / 1e9) if b else None}
+    print(f"bench: {which}: {1e3 * el_plain / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"workload": name, "cells": n, "steps": steps, "ms_per_step": 1e3 * el_plain / steps,
+            "value": n * steps / el_plain, "unit": "cell-steps/s", "kernels": kernels,
+            "ms_per_step_with_kernel_events": 1e3 * el / steps}
+
+
 def copy_probe(nbytes: int) -> int:
     """One HIP device-to-device copy of nbytes (runtime blit kernel): the
     reference point that scales FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -249,6 +310,8 @@ def main() -> int:
                     help="instances of the batched AFM sweep measured after the main workload (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--solo-steps", type=int, default=10)
+    ap.add_argument("--coupled-steps", type=int, default=200,
+                    help="steps of the magnetoelastic and FE+FDTD measurements (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint measurement on the 128x128x32 film (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -324,6 +387,9 @@ def main() -> int:
         out["batched_sweep"] = bench_batch(args.batch_instances, args.batch_steps, args.solo_steps)
     if args.adjoint_steps > 0:
         out["adjoint"] = bench_adjoint((128, 128, 32), args.adjoint_steps)
+    if args.coupled_steps > 0:
+        out["magnetoelastic"] = bench_coupled("magnetoelastic", args.coupled_steps)
+        out["fe_thz"] = bench_coupled("fe_thz", 10 * args.coupled_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

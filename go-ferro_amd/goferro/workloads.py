@@ -195,3 +195,32 @@ def adjoint_probe(shape=(128, 128, 32), size=(16, 16, 4)):
     lo = ((shape[0] - size[0]) // 2, (shape[1] - size[1]) // 2, 0)
     hi = (lo[0] + size[0] - 1, lo[1] + size[1] - 1, size[2] - 1)
     return lo, hi
+
+
+def fe_thz_column(shape=(8, 8, 1024), fe_layers=200, dz=1e-9, dt=1e-18, seed=3, sigma=2e-17):
+    """Ferroelectric + electrodynamics THz column: vacuum / PbTiO3-like film
+    (fe_layers cells, relative permittivity 10) / vacuum, Mur walls in z,
+    periodic x and y, a Gaussian-modulated 1 THz free-current sheet (type 6,
+    amplitude 2 E0 / (eta0 dz) for E0 = 1e7 V/m) in the lower vacuum; dP/dt
+    drives the field and the field acts back on P.  p starts at
+    (0, 0, 0.754)(1 + 0.01 N(0, 1)) in the film, q = 0.
+    Returns (Problem, p0, q0)."""
+    from . import FLAG_EM_BACKACTION_P, FLAG_EM_FROM_P, FLAG_EMDYNAMIC, FLAG_JF_INPUT
+    fe = pto_material(FE_mass=1e-16, FE_damping=4e-4)
+    fe.r_permittivity = 10.0
+    fe.comp_n1, fe.tao_e_n1, fe.tao_e_n2, fe.tao_e_n3 = 1.0, 1.0, 1.0, 1.0
+    nz = shape[2]
+    lo = (nz - fe_layers) // 2
+    cell = np.zeros(shape, np.uint32)
+    cell[:, :, lo:lo + fe_layers] = 1
+    src = lo // 2
+    prob = Problem(shape=tuple(shape), spacing=(dz, dz, dz), materials=[fe], material_cell=cell, dt=dt,
+                   periodic=(True, True, False), flags=FLAG_EMDYNAMIC | FLAG_EM_FROM_P | FLAG_EM_BACKACTION_P |
+                   FLAG_JF_INPUT, Jf_input_type=6, Jf_input_component="x", Jf_input_amp=2e7 / (376.73 * dz),
+                   Jf_input_freq=1e12, Jf_input_sigma=sigma, Jfin_lo=(0, 0, src),
+                   Jfin_hi=(shape[0] - 1, shape[1] - 1, src))
+    rng = np.random.default_rng(seed)
+    p = np.zeros((3,) + tuple(shape))
+    p[2] = 0.754 * (1 + 0.01 * rng.standard_normal(shape))
+    p[:, cell == 0] = 0.0
+    return prob, p, np.zeros_like(p)
