@@ -1292,6 +1292,7 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
     if (!st) {
         ElForceArgs f{};
         f.code = c->d_code; f.mats = c->d_mats;
+        f.nmat = static_cast<int32_t>(c->host_mats.size());
         for (int i = 0; i < 3; i++) {
             f.u[i] = c->u[i]; f.v[i] = c->v[i]; f.us[i] = c->us[i]; f.vs[i] = c->vs[i];
             f.accu[i] = c->accu[i]; f.accv[i] = c->accv[i]; f.fstore[i] = c->fstore[i];

@@ -143,14 +143,17 @@ __device__ __forceinline__ double crow(const double (*C)[6], int r, const double
     return s;
 }
 
-__device__ __forceinline__ const double (*cij_of(const ElForceArgs& a, int64_t q))[6] {
-    return a.mats[(a.code[q] & kMatMask) - 1].cij;
+// The stiffness matrix of cell q from the workgroup's LDS copy of every
+// material's matrix (a zero matrix for vacuum, whose faces are free).
+__device__ __forceinline__ const double (*cij_of(const ElForceArgs& a, const double* ctab, int64_t q))[6] {
+    const uint32_t m = a.code[q] & kMatMask;
+    return reinterpret_cast<const double (*)[6]>(ctab + (m == 0u ? 0 : (m - 1) * 36));
 }
 
 // Stress on the YZ face fi evaluated with the strain of cell (qi, j, k):
 // rows xx, xy, xz of C e (elastic_dynamic_solver.cpp:876-933).
-__device__ __forceinline__ void stress_yz(const ElForceArgs& a, int64_t fi, int64_t qi, int64_t j, int64_t k,
-                                          double out[3]) {
+__device__ __forceinline__ void stress_yz(const ElForceArgs& a, const double* ctab,
+                                          int64_t fi, int64_t qi, int64_t j, int64_t k, double out[3]) {
     const int64_t ny = a.ny, nz = a.nz;
     const int64_t q = (qi * ny + j) * nz + k;
     const int64_t f = (fi * ny + j) * nz + k;
@@ -163,15 +166,15 @@ __device__ __forceinline__ void stress_yz(const ElForceArgs& a, int64_t fi, int6
             a.e0g[4][q]) * 2.;
     e[5] = ((a.gxz[0][(qi * (ny + 1) + j) * nz + k] + a.gxz[0][(qi * (ny + 1) + j + 1) * nz + k] + a.gyz[1][f] * 2.) / 4. -
             a.e0g[5][q]) * 2.;
-    const double(*C)[6] = cij_of(a, q);
+    const double(*C)[6] = cij_of(a, ctab, q);
     out[0] = crow(C, 0, e);
     out[1] = crow(C, 5, e);
     out[2] = crow(C, 4, e);
 }
 
 // XZ face fj with the strain of cell (i, qj, k): rows xy, yy, yz (:994-1050).
-__device__ __forceinline__ void stress_xz(const ElForceArgs& a, int64_t i, int64_t fj, int64_t qj, int64_t k,
-                                          double out[3]) {
+__device__ __forceinline__ void stress_xz(const ElForceArgs& a, const double* ctab,
+                                          int64_t i, int64_t fj, int64_t qj, int64_t k, double out[3]) {
     const int64_t ny = a.ny, nz = a.nz;
     const int64_t q = (i * ny + qj) * nz + k;
     const int64_t f = (i * (ny + 1) + fj) * nz + k;
@@ -184,7 +187,7 @@ __device__ __forceinline__ void stress_xz(const ElForceArgs& a, int64_t i, int64
     e[4] = (a.eg[4][q] - a.e0g[4][q]) * 2.;
     e[5] = ((a.gxz[0][f] * 2. + a.gyz[1][(i * ny + qj) * nz + k] + a.gyz[1][((i + 1) * ny + qj) * nz + k]) / 4. -
             a.e0g[5][q]) * 2.;
-    const double(*C)[6] = cij_of(a, q);
+    const double(*C)[6] = cij_of(a, ctab, q);
     out[0] = crow(C, 5, e);
     out[1] = crow(C, 1, e);
     out[2] = crow(C, 3, e);
@@ -192,8 +195,8 @@ __device__ __forceinline__ void stress_xz(const ElForceArgs& a, int64_t i, int64
 
 // XY face fk with the strain of cell (i, j, qk): rows xz, yz, zz
 // (:1112-1166, :1229-1290).
-__device__ __forceinline__ void stress_xy(const ElForceArgs& a, int64_t i, int64_t j, int64_t fk, int64_t qk,
-                                          double out[3]) {
+__device__ __forceinline__ void stress_xy(const ElForceArgs& a, const double* ctab,
+                                          int64_t i, int64_t j, int64_t fk, int64_t qk, double out[3]) {
     const int64_t ny = a.ny, nz = a.nz;
     const int64_t q = (i * ny + j) * nz + qk;
     const int64_t f = (i * ny + j) * (nz + 1) + fk;
@@ -206,7 +209,7 @@ __device__ __forceinline__ void stress_xy(const ElForceArgs& a, int64_t i, int64
     e[4] = ((a.gxy[0][f] * 2. + a.gyz[2][(i * ny + j) * nz + qk] + a.gyz[2][((i + 1) * ny + j) * nz + qk]) / 4. -
             a.e0g[4][q]) * 2.;
     e[5] = (a.eg[5][q] - a.e0g[5][q]) * 2.;
-    const double(*C)[6] = cij_of(a, q);
+    const double(*C)[6] = cij_of(a, ctab, q);
     out[0] = crow(C, 4, e);
     out[1] = crow(C, 3, e);
     out[2] = crow(C, 2, e);
@@ -217,6 +220,10 @@ __device__ __forceinline__ void avg3(double s[3], const double t[3]) {
 }
 
 __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
+    extern __shared__ double ctab[];   // nmat stiffness matrices, 36 doubles each
+    for (int t = threadIdx.x; t < 36 * a.nmat; t += blockDim.x)
+        ctab[t] = a.mats[t / 36].cij[(t % 36) / 6][t % 6];
+    __syncthreads();
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= nx * ny * nz) return;
@@ -241,14 +248,22 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
         const bool free_zf = (k == nz - 1 && !a.per[2]) || !solid(a.code, (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1));
         double s1x[3] = {0., 0., 0.}, s2x[3] = {0., 0., 0.}, s1y[3] = {0., 0., 0.};
         double s2y[3] = {0., 0., 0.}, s1z[3] = {0., 0., 0.}, s2z[3] = {0., 0., 0.}, t[3];
-        if (!free_xb) { stress_yz(a, i, i, j, k, s1x); stress_yz(a, i, ib, j, k, t); avg3(s1x, t); }
-        if (!free_xf) { stress_yz(a, i + 1, i, j, k, s2x); stress_yz(a, i + 1, iff, j, k, t); avg3(s2x, t); }
-        if (!free_yb) { stress_xz(a, i, j, j, k, s1y); stress_xz(a, i, j, jb, k, t); avg3(s1y, t); }
-        if (!free_yf) { stress_xz(a, i, j + 1, j, k, s2y); stress_xz(a, i, j + 1, jf, k, t); avg3(s2y, t); }
-        if (!free_zb) { stress_xy(a, i, j, k, k, s1z); stress_xy(a, i, j, k, kb, t); avg3(s1z, t); }
+        if (!free_xb) { stress_yz(a, ctab, i, i, j, k, s1x); stress_yz(a, ctab, i, ib, j, k, t); avg3(s1x, t); }
+        if (!free_xf) {
+            stress_yz(a, ctab, i + 1, i, j, k, s2x);
+            stress_yz(a, ctab, i + 1, iff, j, k, t);
+            avg3(s2x, t);
+        }
+        if (!free_yb) { stress_xz(a, ctab, i, j, j, k, s1y); stress_xz(a, ctab, i, j, jb, k, t); avg3(s1y, t); }
+        if (!free_yf) {
+            stress_xz(a, ctab, i, j + 1, j, k, s2y);
+            stress_xz(a, ctab, i, j + 1, jf, k, t);
+            avg3(s2y, t);
+        }
+        if (!free_zb) { stress_xy(a, ctab, i, j, k, k, s1z); stress_xy(a, ctab, i, j, k, kb, t); avg3(s1z, t); }
         if (!free_zf) {
-            stress_xy(a, i, j, k + 1, k, s2z);
-            stress_xy(a, i, j, k + 1, kf, t);
+            stress_xy(a, ctab, i, j, k + 1, k, s2z);
+            stress_xy(a, ctab, i, j, k + 1, kf, t);
             avg3(s2z, t);
         } else if (k == a.trac_k) {
             // Gaussian traction on a stress-free top face (input_strain_type 3)
@@ -372,7 +387,13 @@ hipError_t launch_el_eigen(const ElEigenArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_el_force(const ElForceArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(el_force_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), 0, s, a);
+    const size_t lds = sizeof(double) * 36 * static_cast<size_t>(a.nmat > 0 ? a.nmat : 1);
+    if (lds > 65536) {   // up to 255 materials: 72 KiB of the CU's 160 KiB
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(el_force_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(el_force_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

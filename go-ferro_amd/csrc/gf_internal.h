@@ -232,6 +232,7 @@ struct ElForceArgs {
     int32_t trac_k;         // plane of a traction pulse, -1 when off
     int32_t trac_row;       // 0, 1, 2 for a traction along x, y, z (stress xz, yz, zz)
     double trac_amp;
+    int32_t nmat;           // material records (their stiffness matrices are staged in LDS)
 };
 
 // Displacement history of the absorbing boundary: slot = [plane][comp][nx*ny]
