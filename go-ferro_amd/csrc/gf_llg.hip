@@ -43,15 +43,19 @@ __device__ __forceinline__ double rk_update(int stage, I id, int a, double k, co
 }
 
 // One FM cell of a stage: H_eff with the static field hs, the LLG
-// increment and the RK update; n receives the next store value.
-template <class I>
+// increment and the RK update; n receives the next store value.  kLean
+// compiles out the frame rotation and the optional fields (strain, non-
+// uniform and shaped drives, electrodynamic back-action, k output) for
+// contexts that have none of them: their pointers and the rotation
+// matrices then occupy no scalar registers across the cell loop.
+template <bool kLean, class I>
 __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, I x, I y, I z,
                                         double hs0, double hs1, double hs2, double& n0, double& n1, double& n2) {
     GMat& M = mat_of(a.mats, own);
 
     const double mgx = ldd(a.store[0], id), mgy = ldd(a.store[1], id), mgz = ldd(a.store[2], id);
     double mx = mgx, my = mgy, mz = mgz;
-    if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
+    if (!kLean && a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
 
     const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
     const Stencil st = stencil<true, I>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, static_cast<I>(a.nx),
@@ -68,7 +72,7 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, 
         hac2 = 2. * M.K1n * mz;
     }
     double ha0 = hac0, ha1 = hac1, ha2 = hac2;
-    if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
+    if (!kLean && a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
 
     // H_eff (magnetic_solver.cpp:164-198); the magnetoelastic term
     // (:89-96, 143-150) enters only when a strain is coupled, since it is
@@ -76,7 +80,7 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, 
     double H0 = hs0 + ha0;
     double H1 = hs1 + ha1;
     double H2 = hs2 + ha2;
-    if (a.strain[0]) {
+    if (!kLean && a.strain[0]) {
         double he0, he1, he2;
         magnetoelastic(a.strain, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
         H0 = H0 + he0;
@@ -86,17 +90,17 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, 
     H0 = H0 + M.exch * st.ex[0] + M.dmi_pos * st.d[2] + a.hext[0];
     H1 = H1 + M.exch * st.ex[1] + M.dmi_pos * st.d[3] + a.hext[1];
     H2 = H2 + M.exch * st.ex[2] + M.dmi_neg * (st.d[0] + st.d[1]) + a.hext[2];
-    if (a.hnu[0]) {
+    if (!kLean && a.hnu[0]) {
         H0 = H0 + ldd(a.hnu[0], id);
         H1 = H1 + ldd(a.hnu[1], id);
         H2 = H2 + ldd(a.hnu[2], id);
     }
-    if (a.hdrive[0]) {
+    if (!kLean && a.hdrive[0]) {
         H0 = H0 + ldd(a.hdrive[0], id) * a.drive_s;
         H1 = H1 + ldd(a.hdrive[1], id) * a.drive_s;
         H2 = H2 + ldd(a.hdrive[2], id) * a.drive_s;
     }
-    if (a.hcell[0]) {
+    if (!kLean && a.hcell[0]) {
         // electrodynamic back-action (magnetic_solver.cpp:200-204)
         H0 = H0 + ldd(a.hcell[0], id);
         H1 = H1 + ldd(a.hcell[1], id);
@@ -109,7 +113,7 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, 
     const double k0 = M.precess * px + M.alpha_precess * qx;
     const double k1 = M.precess * py + M.alpha_precess * qy;
     const double k2 = M.precess * pz + M.alpha_precess * qz;
-    if (a.dm_out[0]) {
+    if (!kLean && a.dm_out[0]) {
         std_(a.dm_out[0], id, k0); std_(a.dm_out[1], id, k1); std_(a.dm_out[2], id, k2);
     }
     n0 = rk_update(a.stage, id, 0, k0, a.base, a.acc, a.out);
@@ -127,7 +131,7 @@ __device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
     double n0, n1, n2;
-    fm_cell<I>(a, own, id, x, y, z, ldd(a.hstat[0], id), ldd(a.hstat[1], id), ldd(a.hstat[2], id), n0, n1, n2);
+    fm_cell<false, I>(a, own, id, x, y, z, ldd(a.hstat[0], id), ldd(a.hstat[1], id), ldd(a.hstat[2], id), n0, n1, n2);
 }
 
 template <class I>
@@ -153,7 +157,7 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
 // the separate z passes (gf_fft.hip), so the y and x passes are unchanged.
 // Workgroups are ordered so that each XCD takes a contiguous range of x
 // planes, keeping stencil neighbours in its L2.
-template <bool kHspec, class I>
+template <bool kHspec, bool kLean, class I>
 __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, FusedZArgs f) {
     extern __shared__ double2 fsm[];
     const int nz = static_cast<int>(a.nz), ny = static_cast<int>(a.ny);
@@ -209,7 +213,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
         double mx = 0., my = 0., mz = 0.;
         if (own & kIsFM) {
             double n0, n1, n2;
-            fm_cell<I>(a, own, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), *h0, *h1, *h2, n0, n1,
+            fm_cell<kLean, I>(a, own, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), *h0, *h1, *h2, n0, n1,
                        n2);
             const double Ms = a.mats[(own & kMatMask) - 1].Ms;
             mx = Ms * n0; my = Ms * n1; mz = Ms * n2;
@@ -443,10 +447,14 @@ hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool 
     const size_t lds = sizeof(double2) * (nz / 2 + fft_lds_elems(3 * pairs * nz));
     const unsigned groups = static_cast<unsigned>(a.nx * a.ny / f.P);
     const bool n32 = narrow(a.nx * a.ny * a.nz);
-    if (hspec && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, uint32_t>), dim3(groups), dim3(256), lds, s, a, f);
-    else if (hspec) hipLaunchKernelGGL((mag_stage_fused_kernel<true, int64_t>), dim3(groups), dim3(256), lds, s, a, f);
-    else if (n32) hipLaunchKernelGGL((mag_stage_fused_kernel<false, uint32_t>), dim3(groups), dim3(256), lds, s, a, f);
-    else hipLaunchKernelGGL((mag_stage_fused_kernel<false, int64_t>), dim3(groups), dim3(256), lds, s, a, f);
+    const bool lean = !a.rotate && !a.strain[0] && !a.hnu[0] && !a.hdrive[0] && !a.hcell[0] && !a.dm_out[0];
+    const dim3 g(groups), t(256);
+    if (hspec && lean && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, true, uint32_t>), g, t, lds, s, a, f);
+    else if (hspec && lean) hipLaunchKernelGGL((mag_stage_fused_kernel<true, true, int64_t>), g, t, lds, s, a, f);
+    else if (hspec && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, false, uint32_t>), g, t, lds, s, a, f);
+    else if (hspec) hipLaunchKernelGGL((mag_stage_fused_kernel<true, false, int64_t>), g, t, lds, s, a, f);
+    else if (n32) hipLaunchKernelGGL((mag_stage_fused_kernel<false, false, uint32_t>), g, t, lds, s, a, f);
+    else hipLaunchKernelGGL((mag_stage_fused_kernel<false, false, int64_t>), g, t, lds, s, a, f);
     return hipGetLastError();
 }
 
