@@ -42,6 +42,51 @@ __device__ __forceinline__ double rk_update(int stage, I id, int a, double k, co
     return v;
 }
 
+// Every global input of one FM cell of a stage except the static field:
+// loaded together, before the cell's own code is known, so that one
+// memory round trip feeds the cell (all indices are in bounds whatever the
+// code says).
+struct FmIn {
+    uint32_t own;
+    double c[3];        // stage store at the cell
+    double v[6][3];     // stage store at the six face neighbours (gather6)
+    uint32_t cd[6];     // their codes
+    double m[3];        // m at the start of the step
+    double acc[3];      // RK accumulator (read at stages 2-4)
+};
+
+template <class I>
+__device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z, FmIn& in) {
+    in.own = ldu(a.code, id);
+    const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
+    for (int c = 0; c < 3; c++) in.c[c] = ldd(a.store[c], id);
+    gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), static_cast<I>(a.ny), static_cast<I>(a.nz), in.v, in.cd);
+    for (int c = 0; c < 3; c++) in.m[c] = ldd(a.base[c], id);
+    for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd(a.acc[c], id) : 0.;
+}
+
+// RK update of one component from its preloaded m and accumulator; the
+// same arithmetic as rk_update.
+template <class I>
+__device__ __forceinline__ double rk_store(int stage, I id, int c, double k, double m, double acc,
+                                           double* const* accp, double* const* out) {
+    double v;
+    if (stage == 1) {
+        v = m + k * 0.5;
+        std_(accp[c], id, k / 6.);
+    } else if (stage == 2) {
+        v = m + k * 0.5;
+        std_(accp[c], id, acc + k / 3.);
+    } else if (stage == 3) {
+        v = m + k;
+        std_(accp[c], id, acc + k / 3.);
+    } else {
+        v = m + (acc + k / 6.);
+    }
+    std_(out[c], id, v);
+    return v;
+}
+
 // One FM cell of a stage: H_eff with the static field hs, the LLG
 // increment and the RK update; n receives the next store value.  kLean
 // compiles out the frame rotation and the optional fields (strain, non-
@@ -49,18 +94,16 @@ __device__ __forceinline__ double rk_update(int stage, I id, int a, double k, co
 // contexts that have none of them: their pointers and the rotation
 // matrices then occupy no scalar registers across the cell loop.
 template <bool kLean, class I>
-__device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, I x, I y, I z,
-                                        double hs0, double hs1, double hs2, double& n0, double& n1, double& n2) {
-    GMat& M = mat_of(a.mats, own);
+__device__ __forceinline__ void fm_cell(const StageArgs& a, const FmIn& in, I id, double hs0, double hs1, double hs2,
+                                        double& n0, double& n1, double& n2) {
+    GMat& M = mat_of(a.mats, in.own);
 
-    const double mgx = ldd(a.store[0], id), mgy = ldd(a.store[1], id), mgz = ldd(a.store[2], id);
+    const double mgx = in.c[0], mgy = in.c[1], mgz = in.c[2];
     double mx = mgx, my = mgy, mz = mgz;
     if (!kLean && a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
 
-    const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
-    const Stencil st = stencil<true, I>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, static_cast<I>(a.nx),
-                                        static_cast<I>(a.ny), static_cast<I>(a.nz), a.dx, a.dy, a.dz, M.Aex,
-                                        M.Aex_self, M.dmi_slope);
+    const Stencil st = stencil_core<true>(mgx, mgy, mgz, in.v, in.cd, a.mats, in.own, kFmSurfShift, -1, a.dx, a.dy,
+                                          a.dz, M.Aex, M.Aex_self, M.dmi_slope);
 
     // magnetocrystalline anisotropy (magnetic_solver.cpp:124-141)
     double hac0 = 0., hac1 = 0., hac2 = 0.;
@@ -116,22 +159,24 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, uint32_t own, I id, 
     if (!kLean && a.dm_out[0]) {
         std_(a.dm_out[0], id, k0); std_(a.dm_out[1], id, k1); std_(a.dm_out[2], id, k2);
     }
-    n0 = rk_update(a.stage, id, 0, k0, a.base, a.acc, a.out);
-    n1 = rk_update(a.stage, id, 1, k1, a.base, a.acc, a.out);
-    n2 = rk_update(a.stage, id, 2, k2, a.base, a.acc, a.out);
+    n0 = rk_store(a.stage, id, 0, k0, in.m[0], in.acc[0], a.acc, a.out);
+    n1 = rk_store(a.stage, id, 1, k1, in.m[1], in.acc[1], a.acc, a.out);
+    n2 = rk_store(a.stage, id, 2, k2, in.m[2], in.acc[2], a.acc, a.out);
 }
 
 // One FM cell of the unfused stage: coordinates from the linear index,
 // magnetostatic field from Hstat.
 template <class I>
 __device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
-    const uint32_t own = ldu(a.code, id);
-    if (!(own & kIsFM)) return;
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
+    FmIn in;
+    fm_load<I>(a, id, x, y, z, in);
+    const double hs0 = ldd(a.hstat[0], id), hs1 = ldd(a.hstat[1], id), hs2 = ldd(a.hstat[2], id);
+    if (!(in.own & kIsFM)) return;
     double n0, n1, n2;
-    fm_cell<false, I>(a, own, id, x, y, z, ldd(a.hstat[0], id), ldd(a.hstat[1], id), ldd(a.hstat[2], id), n0, n1, n2);
+    fm_cell<false, I>(a, in, id, hs0, hs1, hs2, n0, n1, n2);
 }
 
 template <class I>
@@ -209,13 +254,13 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
         double* h1 = reinterpret_cast<double*>(cbuf + fpad(pairs * nz + pos)) + part;
         double* h2 = reinterpret_cast<double*>(cbuf + fpad(2 * pairs * nz + pos)) + part;
         const I id = (static_cast<I>(x) * static_cast<I>(ny) + static_cast<I>(y)) * static_cast<I>(nz) + static_cast<I>(k);
-        const uint32_t own = ldu(a.code, id);
+        FmIn in;
+        fm_load<I>(a, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), in);
         double mx = 0., my = 0., mz = 0.;
-        if (own & kIsFM) {
+        if (in.own & kIsFM) {
             double n0, n1, n2;
-            fm_cell<kLean, I>(a, own, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), *h0, *h1, *h2, n0, n1,
-                       n2);
-            const double Ms = a.mats[(own & kMatMask) - 1].Ms;
+            fm_cell<kLean, I>(a, in, id, *h0, *h1, *h2, n0, n1, n2);
+            const double Ms = mat_of(a.mats, in.own).Ms;
             mx = Ms * n0; my = Ms * n1; mz = Ms * n2;
         }
         *h0 = mx; *h1 = my; *h2 = mz;
