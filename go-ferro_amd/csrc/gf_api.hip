@@ -1076,6 +1076,7 @@ StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
     a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
     a.stage = stage;
+    a.nmat = static_cast<int32_t>(c->host_mats.size());
     return a;
 }
 

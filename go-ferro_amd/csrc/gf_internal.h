@@ -308,6 +308,7 @@ struct StageArgs {
     double Tc2g[3][3];
     int32_t rotate;
     int32_t stage;
+    int32_t nmat;               // material records at mats
 };
 
 struct AfmStageArgs {

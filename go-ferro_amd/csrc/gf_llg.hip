@@ -19,6 +19,16 @@ namespace gf {
 
 namespace {
 
+// Material records the lean fused stage copies into LDS (about 0.9 KB each),
+// and the set-up loads per thread of a 256-thread workgroup: up to
+// kMatWordsPerThread words of those records, and the field spectrum in
+// chunks of kSetupChunk complex values.
+constexpr int kLeanMats = 8;
+constexpr int kMatWordsPerThread = 4;
+constexpr int kSetupChunk = 4;
+static_assert(kLeanMats * sizeof(DevMaterial) / sizeof(double) <= kMatWordsPerThread * 256,
+              "the lean stage's material records must fit the set-up loads of 256 threads");
+
 // RK update of one component, see the file comment; returns the value
 // written to the next store (m itself at stage 4).
 template <class I>
@@ -92,17 +102,19 @@ __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, dou
 // compiles out the frame rotation and the optional fields (strain, non-
 // uniform and shaped drives, electrodynamic back-action, k output) for
 // contexts that have none of them: their pointers and the rotation
-// matrices then occupy no scalar registers across the cell loop.
-template <bool kLean, class I>
-__device__ __forceinline__ void fm_cell(const StageArgs& a, const FmIn& in, I id, double hs0, double hs1, double hs2,
-                                        double& n0, double& n1, double& n2) {
-    GMat& M = mat_of(a.mats, in.own);
+// matrices then occupy no scalar registers across the cell loop.  mats is
+// the material table, in global memory (GMat) or a workgroup's LDS copy
+// (LMat).
+template <bool kLean, class I, class MT>
+__device__ __forceinline__ void fm_cell(const StageArgs& a, MT* mats, const FmIn& in, I id, double hs0, double hs1,
+                                        double hs2, double& n0, double& n1, double& n2) {
+    MT& M = mats[(in.own & kMatMask) - 1];
 
     const double mgx = in.c[0], mgy = in.c[1], mgz = in.c[2];
     double mx = mgx, my = mgy, mz = mgz;
     if (!kLean && a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
 
-    const Stencil st = stencil_core<true>(mgx, mgy, mgz, in.v, in.cd, a.mats, in.own, kFmSurfShift, -1, a.dx, a.dy,
+    const Stencil st = stencil_core<true>(mgx, mgy, mgz, in.v, in.cd, mats, in.own, kFmSurfShift, -1, a.dx, a.dy,
                                           a.dz, M.Aex, M.Aex_self, M.dmi_slope);
 
     // magnetocrystalline anisotropy (magnetic_solver.cpp:124-141)
@@ -176,7 +188,7 @@ __device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
     const double hs0 = ldd(a.hstat[0], id), hs1 = ldd(a.hstat[1], id), hs2 = ldd(a.hstat[2], id);
     if (!(in.own & kIsFM)) return;
     double n0, n1, n2;
-    fm_cell<false, I>(a, in, id, hs0, hs1, hs2, n0, n1, n2);
+    fm_cell<false, I>(a, (GMat*)a.mats, in, id, hs0, hs1, hs2, n0, n1, n2);
 }
 
 template <class I>
@@ -201,7 +213,8 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
 // written in the same LDS slot.  Both spectra use the layout [x][kz][y] of
 // the separate z passes (gf_fft.hip), so the y and x passes are unchanged.
 // Workgroups are ordered so that each XCD takes a contiguous range of x
-// planes, keeping stencil neighbours in its L2.
+// planes, keeping stencil neighbours in its L2.  The lean form also copies
+// the material records into LDS (at most kLeanMats of them).
 template <bool kHspec, bool kLean, class I>
 __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, FusedZArgs f) {
     extern __shared__ double2 fsm[];
@@ -210,30 +223,75 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
     const int pairs = P > 1 ? P >> 1 : 1;
     double2* tw = fsm;
     double2* cbuf = tw + (nz >> 1);          // [3][pairs][nz], element i at slot fpad(i)
+    DevMaterial* ltab = reinterpret_cast<DevMaterial*>(cbuf + fft_lds_elems(3 * pairs * nz));
+    const int tid = threadIdx.x, bd = blockDim.x;
     const unsigned b = blockIdx.x, G = gridDim.x;
     const unsigned q = G / 8u, r = G % 8u, xcd = b % 8u, slot = b / 8u;
     const unsigned L = xcd * q + (xcd < r ? xcd : r) + slot;
     const int gpx = ny / P;
     const int x = static_cast<int>(L) / gpx, y0 = (static_cast<int>(L) % gpx) * P;
     const int cells = P * nz;
-    fft_load_twiddles(tw, f.tw, nz);
+    // The set-up loads (twiddles, material records, the field spectrum in
+    // chunks of kSetupChunk elements per thread) are issued before any of
+    // them is written to LDS, so that together they cost one memory round
+    // trip per chunk rather than one each.
+    const int ntw = nz >> 1;
+    const int words = kLean ? a.nmat * static_cast<int>(sizeof(DevMaterial) / sizeof(double)) : 0;
+    const double2 twv = tid < ntw ? f.tw[tid] : make_double2(0., 0.);
+    double mw[kMatWordsPerThread];
+#pragma unroll
+    for (int u = 0; u < kMatWordsPerThread; u++) {
+        const int t = tid + u * bd;
+        mw[u] = t < words ? ldd(reinterpret_cast<const double*>(a.mats), static_cast<uint32_t>(t)) : 0.;
+    }
+    bool setup_stored = false;
+    auto store_setup = [&]() {
+        if (tid < ntw) tw[tid] = twv;
+        for (int j = tid + bd; j < ntw; j += bd) tw[j] = f.tw[j];
+#pragma unroll
+        for (int u = 0; u < kMatWordsPerThread; u++) {
+            const int t = tid + u * bd;
+            if (t < words) reinterpret_cast<double*>(ltab)[t] = mw[u];
+        }
+        setup_stored = true;
+    };
     if (kHspec) {
-        const int per = pairs * nzh;
-        for (int t = threadIdx.x; t < 3 * per; t += blockDim.x) {
-            const int c = t / per, u = t - c * per;
-            const int p = u % pairs, kz = u / pairs;
-            const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
-            const double2* in = f.hspec + c * f.nh;
-            double2 A = in[row + p];
-            double2 B = P > 1 ? in[row + p + pairs] : make_double2(0., 0.);
-            if (kz == 0 || 2 * kz == nz) { A.y = 0.; B.y = 0.; }
-            const int cb = (c * pairs + p) * nz;
-            cbuf[fpad(cb + kz)] = make_double2(A.x - B.y, A.y + B.x);
-            if (kz > 0 && 2 * kz < nz) cbuf[fpad(cb + nz - kz)] = make_double2(A.x + B.y, B.x - A.y);
+        const int per = pairs * nzh, tot = 3 * per;
+        for (int t0 = 0; t0 < tot; t0 += kSetupChunk * bd) {
+            double2 A[kSetupChunk], B[kSetupChunk];
+#pragma unroll
+            for (int u = 0; u < kSetupChunk; u++) {
+                const int t = t0 + u * bd + tid;
+                A[u] = make_double2(0., 0.);
+                B[u] = make_double2(0., 0.);
+                if (t < tot) {
+                    const int c = t / per, w = t - c * per;
+                    const int p = w % pairs, kz = w / pairs;
+                    const int64_t row = (static_cast<int64_t>(x) * nzh + kz) * ny + y0;
+                    const double2* in = f.hspec + c * f.nh;
+                    A[u] = in[row + p];
+                    if (P > 1) B[u] = in[row + p + pairs];
+                }
+            }
+            if (!setup_stored) store_setup();
+#pragma unroll
+            for (int u = 0; u < kSetupChunk; u++) {
+                const int t = t0 + u * bd + tid;
+                if (t < tot) {
+                    const int c = t / per, w = t - c * per;
+                    const int p = w % pairs, kz = w / pairs;
+                    double2 Av = A[u], Bv = B[u];
+                    if (kz == 0 || 2 * kz == nz) { Av.y = 0.; Bv.y = 0.; }
+                    const int cb = (c * pairs + p) * nz;
+                    cbuf[fpad(cb + kz)] = make_double2(Av.x - Bv.y, Av.y + Bv.x);
+                    if (kz > 0 && 2 * kz < nz) cbuf[fpad(cb + nz - kz)] = make_double2(Av.x + Bv.y, Bv.x - Av.y);
+                }
+            }
         }
         __syncthreads();
         ifft_dif(cbuf, 3 * pairs, lognz, tw);
     } else {
+        store_setup();
         for (int t = threadIdx.x; t < 3 * cells; t += blockDim.x) {
             const int c = t / cells, w = t - c * cells;
             const int col = w >> lognz, k = w & (nz - 1);
@@ -259,8 +317,15 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
         double mx = 0., my = 0., mz = 0.;
         if (in.own & kIsFM) {
             double n0, n1, n2;
-            fm_cell<kLean, I>(a, in, id, *h0, *h1, *h2, n0, n1, n2);
-            const double Ms = mat_of(a.mats, in.own).Ms;
+            double Ms;
+            if (kLean) {
+                LMat* tab = (LMat*)ltab;
+                fm_cell<kLean, I>(a, tab, in, id, *h0, *h1, *h2, n0, n1, n2);
+                Ms = tab[(in.own & kMatMask) - 1].Ms;
+            } else {
+                fm_cell<kLean, I>(a, (GMat*)a.mats, in, id, *h0, *h1, *h2, n0, n1, n2);
+                Ms = mat_of(a.mats, in.own).Ms;
+            }
             mx = Ms * n0; my = Ms * n1; mz = Ms * n2;
         }
         *h0 = mx; *h1 = my; *h2 = mz;
@@ -489,10 +554,12 @@ hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s) {
 hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool hspec, hipStream_t s) {
     const int nz = static_cast<int>(a.nz);
     const int pairs = f.P > 1 ? f.P / 2 : 1;
-    const size_t lds = sizeof(double2) * (nz / 2 + fft_lds_elems(3 * pairs * nz));
+    const bool lean = !a.rotate && !a.strain[0] && !a.hnu[0] && !a.hdrive[0] && !a.hcell[0] && !a.dm_out[0] &&
+                      a.nmat >= 1 && a.nmat <= kLeanMats;
+    const size_t lds = sizeof(double2) * (nz / 2 + fft_lds_elems(3 * pairs * nz)) +
+                       (lean ? sizeof(DevMaterial) * static_cast<size_t>(a.nmat) : 0);
     const unsigned groups = static_cast<unsigned>(a.nx * a.ny / f.P);
     const bool n32 = narrow(a.nx * a.ny * a.nz);
-    const bool lean = !a.rotate && !a.strain[0] && !a.hnu[0] && !a.hdrive[0] && !a.hcell[0] && !a.dm_out[0];
     const dim3 g(groups), t(256);
     if (hspec && lean && n32) hipLaunchKernelGGL((mag_stage_fused_kernel<true, true, uint32_t>), g, t, lds, s, a, f);
     else if (hspec && lean) hipLaunchKernelGGL((mag_stage_fused_kernel<true, true, int64_t>), g, t, lds, s, a, f);

@@ -37,8 +37,10 @@ __device__ __forceinline__ double ldd(const double* p, uint32_t i) {
 }
 __device__ __forceinline__ void std_(double* p, int64_t i, double v) { ((GD*)p)[i] = v; }
 __device__ __forceinline__ void std_(double* p, uint32_t i, double v) { *(GD*)((GC*)p + (i << 3)) = v; }
-// The material record of a cell, read through the global address space.
+// The material record of a cell, read through the global address space,
+// or from a workgroup's LDS copy of the records (LMat).
 typedef const __attribute__((address_space(1))) DevMaterial GMat;
+typedef const __attribute__((address_space(3))) DevMaterial LMat;
 __device__ __forceinline__ GMat& mat_of(const DevMaterial* mats, uint32_t code) {
     return ((GMat*)mats)[(code & kMatMask) - 1];
 }
@@ -87,8 +89,8 @@ __device__ __forceinline__ void gather6(const P3 s3, const uint32_t* code, I i, 
 // Face coefficient between a cell of material `mat` (constant A, self mean
 // A_self) and a neighbour of code nc: the harmonic mean 2 A1 A2 / (A1 + A2)
 // of magnetic_initialize.cpp:381-524, A_self for the same material.
-__device__ __forceinline__ double face_coef(const DevMaterial* mats, uint32_t mat, uint32_t nc, int sub, double A,
-                                            double A_self) {
+template <class MP>
+__device__ __forceinline__ double face_coef(MP mats, uint32_t mat, uint32_t nc, int sub, double A, double A_self) {
     const uint32_t nm = nc & kMatMask;
     if (nm == mat) return A_self;
     const double a2 = sub < 0 ? mats[nm - 1].Aex : mats[nm - 1].aAex[sub];
@@ -97,9 +99,9 @@ __device__ __forceinline__ double face_coef(const DevMaterial* mats, uint32_t ma
 
 // The stencil of get_m_spatial_derivative_local from the cell's store
 // (c0, c1, c2) and its gathered neighbours; see gather6.
-template <bool kDmi>
+template <bool kDmi, class MP>
 __device__ __forceinline__ Stencil stencil_core(double c0, double c1, double c2, const double (&v)[6][3],
-                                                const uint32_t (&cd)[6], const DevMaterial* mats, uint32_t own,
+                                                const uint32_t (&cd)[6], MP mats, uint32_t own,
                                                 int surf_shift, int sub, double dx, double dy, double dz, double A,
                                                 double A_self, double slope) {
     const uint32_t mat = own & kMatMask;
