@@ -31,6 +31,8 @@ namespace gf {
 namespace {
 
 constexpr int kThreads = 256;
+// Global loads per thread issued together when a pass fills its LDS buffer.
+constexpr int kBatch = 8;
 
 __device__ __forceinline__ int bitrev(int j, int logn) { return fft_bitrev(j, logn); }
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) { return fft_cmul(a, b); }
@@ -116,17 +118,18 @@ __global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
     const int x0 = blockIdx.x * a.pencils;
     double2* tw = smem;
     double2* buf = smem + (a.ny >> 1);
-    load_twiddles(tw, a.tw, a.ny);
+    const double2 tw0 = fft_twiddle_prefetch(a.tw, a.ny);
     const double2* in = a.in + c * a.nh;
     const int total = a.pencils * a.ny;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int r = idx >> a.logny;
-        const int y = idx & (a.ny - 1);
-        const int x = x0 + r;
-        double2 v = make_double2(0., 0.);
-        if (x < a.nx) v = in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y];
-        buf[fpad(r * a.ny + bitrev(y, a.logny))] = v;
-    }
+    fft_staged_load<kBatch, double2>(
+        total,
+        [&](int idx) {
+            const int x = x0 + (idx >> a.logny);
+            return x < a.nx ? in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + (idx & (a.ny - 1))]
+                            : make_double2(0., 0.);
+        },
+        [&](int idx, double2 v) { buf[fpad((idx >> a.logny) * a.ny + bitrev(idx & (a.ny - 1), a.logny))] = v; });
+    fft_twiddle_store(tw, a.tw, a.ny, tw0);
     __syncthreads();
     fft_dit(buf, a.pencils, a.logny, tw);
     double2* out = a.out + c * a.nh;
@@ -146,17 +149,17 @@ __global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
     const int x0 = blockIdx.x * a.pencils;
     double2* tw = smem;
     double2* buf = smem + (a.ny >> 1);
-    load_twiddles(tw, a.tw, a.ny);
+    const double2 tw0 = fft_twiddle_prefetch(a.tw, a.ny);
     const double2* in = a.in + c * a.nh;
     const int total = a.pencils * a.ny;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int r = idx & (a.pencils - 1);
-        const int y = idx >> a.logp;
-        const int x = x0 + r;
-        double2 v = make_double2(0., 0.);
-        if (x < a.nx) v = in[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x];
-        buf[fpad(r * a.ny + y)] = v;
-    }
+    fft_staged_load<kBatch, double2>(
+        total,
+        [&](int idx) {
+            const int x = x0 + (idx & (a.pencils - 1));
+            return x < a.nx ? in[(static_cast<int64_t>(kz) * a.ny + (idx >> a.logp)) * a.nx + x] : make_double2(0., 0.);
+        },
+        [&](int idx, double2 v) { buf[fpad((idx & (a.pencils - 1)) * a.ny + (idx >> a.logp))] = v; });
+    fft_twiddle_store(tw, a.tw, a.ny, tw0);
     __syncthreads();
     ifft_dif(buf, a.pencils, a.logny, tw);
     double2* out = a.out + c * a.nh;
@@ -190,42 +193,53 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
     const int comps = kConv ? 3 : 1;
     double2* tw = smem;
     double2* buf = smem + (a.nx >> 1);
-    load_twiddles(tw, a.tw, a.nx);
+    const double2 tw0 = fft_twiddle_prefetch(a.tw, a.nx);
     const int total = comps * a.pencils * a.nx;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int x = idx & (a.nx - 1);
-        const int t = idx >> a.lognx;
-        const int r = t & (a.pencils - 1);
-        const int c = t >> a.logp;
-        const int y = y0 + r;
-        double2 v = make_double2(0., 0.);
-        if (y < a.ny) v = a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x];
-        buf[fpad(t * a.nx + bitrev(x, a.lognx))] = v;
-    }
+    const int pts = a.pencils * a.nx;
+    const int64_t s = a.nh;
+    // Tensor spectra of one k-point in the order xx, yy, zz, xy, yz, zx, yx,
+    // zy, xz; the adjoint convolution uses N^T with conjugated spectra:
+    // xy<->yx, yz<->zy, zx<->xz (the real-space correlation with the kernel).
+    auto load_tensor = [&](int64_t g, double (&kr)[9], double (&ki)[9]) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const int t = a.adjoint ? (q < 3 ? q : (q < 6 ? q + 3 : q - 3)) : q;
+            kr[q] = a.kre[t * s + g];
+            const double im = static_cast<double>(a.kim[t * s + g]);
+            ki[q] = a.adjoint ? -im : im;
+        }
+    };
+    auto point_of = [&](int idx) {
+        return (static_cast<int64_t>(kz) * a.ny + y0 + (idx >> a.lognx)) * a.nx + (idx & (a.nx - 1));
+    };
+    // this thread's first k-point's tensor is loaded with the magnetisation
+    // spectra, so the product does not wait for a second round trip
+    double kr0[9], ki0[9];
+    const int first = threadIdx.x;
+    const bool first_ok = kConv && first < pts && y0 + (first >> a.lognx) < a.ny;
+    if (first_ok) load_tensor(point_of(first), kr0, ki0);
+    fft_staged_load<kBatch, double2>(
+        total,
+        [&](int idx) {
+            const int t = idx >> a.lognx;
+            const int y = y0 + (t & (a.pencils - 1));
+            const int64_t g = (static_cast<int64_t>(kz) * a.ny + y) * a.nx + (idx & (a.nx - 1));
+            return y < a.ny ? a.data[(t >> a.logp) * a.nh + g] : make_double2(0., 0.);
+        },
+        [&](int idx, double2 v) { buf[fpad((idx >> a.lognx) * a.nx + bitrev(idx & (a.nx - 1), a.lognx))] = v; });
+    fft_twiddle_store(tw, a.tw, a.nx, tw0);
     __syncthreads();
     fft_dit(buf, comps * a.pencils, a.lognx, tw);
     if (kConv) {
-        // magnetostatic_solver.cpp:163-177; spectra order
-        // xx, yy, zz, xy, yz, zx, yx, zy, xz
-        const int pts = a.pencils * a.nx;
-        for (int idx = threadIdx.x; idx < pts; idx += blockDim.x) {
+        // magnetostatic_solver.cpp:163-177
+        auto product = [&](int idx, const double (&kr)[9], const double (&ki)[9]) {
             const int x = idx & (a.nx - 1);
             const int r = idx >> a.lognx;
-            const int y = y0 + r;
-            if (y >= a.ny) continue;
-            const int64_t g = (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x;
             const int ix = (0 * a.pencils + r) * a.nx + x;
             const int iy = (1 * a.pencils + r) * a.nx + x;
             const int iz = (2 * a.pencils + r) * a.nx + x;
             const double2 mx = buf[fpad(ix)], my = buf[fpad(iy)], mz = buf[fpad(iz)];
-            const int64_t s = a.nh;
-            // the adjoint convolution uses N^T with conjugated spectra: xy<->yx,
-            // yz<->zy, zx<->xz (the real-space correlation with the kernel)
-            auto K = [&](int q) {
-                const int t = a.adjoint ? (q < 3 ? q : (q < 6 ? q + 3 : q - 3)) : q;
-                const double im = static_cast<double>(a.kim[t * s + g]);
-                return make_double2(a.kre[t * s + g], a.adjoint ? -im : im);
-            };
+            auto K = [&](int q) { return make_double2(kr[q], ki[q]); };
             double2 t0 = cmul(mx, K(0)), t1 = cmul(my, K(3)), t2 = cmul(mz, K(8));
             const double2 hx = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
             t0 = cmul(my, K(1)); t1 = cmul(mz, K(4)); t2 = cmul(mx, K(6));
@@ -233,6 +247,13 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
             t0 = cmul(mz, K(2)); t1 = cmul(mx, K(5)); t2 = cmul(my, K(7));
             const double2 hz = make_double2((t0.x + t1.x + t2.x) / a.den, (t0.y + t1.y + t2.y) / a.den);
             buf[fpad(ix)] = hx; buf[fpad(iy)] = hy; buf[fpad(iz)] = hz;
+        };
+        if (first_ok) product(first, kr0, ki0);
+        for (int idx = first + static_cast<int>(blockDim.x); idx < pts; idx += blockDim.x) {
+            if (y0 + (idx >> a.lognx) >= a.ny) continue;
+            double kr[9], ki[9];
+            load_tensor(point_of(idx), kr, ki);
+            product(idx, kr, ki);
         }
         __syncthreads();
         ifft_dif(buf, comps * a.pencils, a.lognx, tw);
@@ -269,19 +290,23 @@ __global__ void __launch_bounds__(kThreads) zinv_kernel(ZInvArgs a) {
     const int y0 = blockIdx.x * a.pencils;
     double2* tw = smem;
     double2* buf = smem + (a.nz >> 1);
-    load_twiddles(tw, a.tw, a.nz);
+    const double2 tw0 = fft_twiddle_prefetch(a.tw, a.nz);
     const double2* in = a.in + c * a.nh;
     const int ztotal = a.pencils * a.nzh;
-    for (int idx = threadIdx.x; idx < ztotal; idx += blockDim.x) {
-        const int r = idx & (a.pencils - 1);
-        const int kz = idx >> a.logp;
-        const int y = y0 + r;
-        double2 v = make_double2(0., 0.);
-        if (y < a.ny) v = in[(static_cast<int64_t>(x) * a.nzh + kz) * a.ny + y];
-        buf[fpad(r * a.nz + kz)] = v;
-        const int mirror = a.nz - kz;
-        if (kz >= 1 && mirror > kz && mirror < a.nz) buf[fpad(r * a.nz + mirror)] = make_double2(v.x, -v.y);
-    }
+    fft_staged_load<kBatch, double2>(
+        ztotal,
+        [&](int idx) {
+            const int y = y0 + (idx & (a.pencils - 1));
+            return y < a.ny ? in[(static_cast<int64_t>(x) * a.nzh + (idx >> a.logp)) * a.ny + y] : make_double2(0., 0.);
+        },
+        [&](int idx, double2 v) {
+            const int r = idx & (a.pencils - 1);
+            const int kz = idx >> a.logp;
+            buf[fpad(r * a.nz + kz)] = v;
+            const int mirror = a.nz - kz;
+            if (kz >= 1 && mirror > kz && mirror < a.nz) buf[fpad(r * a.nz + mirror)] = make_double2(v.x, -v.y);
+        });
+    fft_twiddle_store(tw, a.tw, a.nz, tw0);
     __syncthreads();
     ifft_dif(buf, a.pencils, a.lognz, tw);
     double* out = c == 0 ? a.out[0] : (c == 1 ? a.out[1] : a.out[2]);

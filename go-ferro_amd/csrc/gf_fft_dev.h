@@ -130,4 +130,37 @@ __device__ __forceinline__ void fft_load_twiddles(double2* dst, const double2* s
     for (int j = threadIdx.x; j < (n >> 1); j += blockDim.x) dst[j] = src[j];
 }
 
+// The same in two halves, so that the first load can be issued together
+// with a kernel's other set-up loads: fft_twiddle_prefetch returns this
+// thread's first twiddle, fft_twiddle_store writes it and loads the rest.
+__device__ __forceinline__ double2 fft_twiddle_prefetch(const double2* src, int n) {
+    return static_cast<int>(threadIdx.x) < (n >> 1) ? src[threadIdx.x] : make_double2(0., 0.);
+}
+__device__ __forceinline__ void fft_twiddle_store(double2* dst, const double2* src, int n, double2 first) {
+    if (static_cast<int>(threadIdx.x) < (n >> 1)) dst[threadIdx.x] = first;
+    for (int j = threadIdx.x + blockDim.x; j < (n >> 1); j += blockDim.x) dst[j] = src[j];
+}
+
+// Copies elements 0..total-1 from global memory into LDS through
+// registers, kBatch elements per thread at a time: ld(idx) loads element
+// idx and st(idx, v) writes it.  Every load of a batch is issued before the
+// first LDS write waits for one, so a workgroup pays one memory round trip
+// per batch rather than one per element.
+template <int kBatch, class T, class Ld, class St>
+__device__ __forceinline__ void fft_staged_load(int total, Ld ld, St st) {
+    for (int b0 = threadIdx.x; b0 < total; b0 += kBatch * blockDim.x) {
+        T v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int idx = b0 + u * static_cast<int>(blockDim.x);
+            if (idx < total) v[u] = ld(idx);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int idx = b0 + u * static_cast<int>(blockDim.x);
+            if (idx < total) st(idx, v[u]);
+        }
+    }
+}
+
 }  // namespace gf
