@@ -29,29 +29,6 @@ constexpr int kSetupChunk = 4;
 static_assert(kLeanMats * sizeof(DevMaterial) / sizeof(double) <= kMatWordsPerThread * 256,
               "the lean stage's material records must fit the set-up loads of 256 threads");
 
-// RK update of one component, see the file comment; returns the value
-// written to the next store (m itself at stage 4).
-template <class I>
-__device__ __forceinline__ double rk_update(int stage, I id, int a, double k, const double* const* base,
-                                            double* const* acc, double* const* out) {
-    const double m = ldd(base[a], id);
-    double v;
-    if (stage == 1) {
-        v = m + k * 0.5;
-        std_(acc[a], id, k / 6.);
-    } else if (stage == 2) {
-        v = m + k * 0.5;
-        std_(acc[a], id, ldd(acc[a], id) + k / 3.);
-    } else if (stage == 3) {
-        v = m + k;
-        std_(acc[a], id, ldd(acc[a], id) + k / 3.);
-    } else {
-        v = m + (ldd(acc[a], id) + k / 6.);
-    }
-    std_(out[a], id, v);
-    return v;
-}
-
 // Every global input of one FM cell of a stage except the static field:
 // loaded together, before the cell's own code is known, so that one
 // memory round trip feeds the cell (all indices are in bounds whatever the
@@ -75,8 +52,9 @@ __device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z,
     for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd(a.acc[c], id) : 0.;
 }
 
-// RK update of one component from its preloaded m and accumulator; the
-// same arithmetic as rk_update.
+// RK update of one component (see the file comment) from its preloaded m
+// and accumulator; returns the value written to the next store (m itself
+// at stage 4).
 template <class I>
 __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, double m, double acc,
                                            double* const* accp, double* const* out) {
@@ -355,68 +333,79 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
 }
 
 // One sublattice of one AFM cell: the two sublattices of a stage read only
-// stage-(s-1) stores, so they run as independent threads.
+// stage-(s-1) stores, so they run as independent threads.  Every global
+// input of the sublattice is loaded before the cell's code is known (all
+// indices are in bounds), as for the FM cell, so one memory round trip
+// feeds the cell.
 template <class I>
 __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) {
-    const uint32_t own = ldu(a.code, id);
-    if (!(own & kIsAFM)) return;
-    GMat& M = mat_of(a.mats, own);
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
-    {
-        const P3 st = sub == 0 ? P3{{a.store1[0], a.store1[1], a.store1[2]}} : P3{{a.store2[0], a.store2[1], a.store2[2]}};
-        const P3 other = sub == 0 ? P3{{a.store2[0], a.store2[1], a.store2[2]}} : P3{{a.store1[0], a.store1[1], a.store1[2]}};
-        const double mgx = ldd(st.p[0], id), mgy = ldd(st.p[1], id), mgz = ldd(st.p[2], id);
-        double mx = mgx, my = mgy, mz = mgz;
-        if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
-        const Stencil s = stencil<false, I>(st, a.code, a.mats, own, kAfmSurfShift, sub, x, y, z,
-                                            static_cast<I>(a.nx), ny, nz, a.dx, a.dy, a.dz, M.aAex[sub],
-                                            M.aAex_self[sub], 0.);
-        // uniaxial anisotropy 2K1 (u.m) u (magnetic_AFM_solver.cpp:148-154)
-        const double udotm = mx * M.u[0] + my * M.u[1] + mz * M.u[2];
-        const double hac0 = 2. * M.aK1n[sub] * udotm * M.u[0];
-        const double hac1 = 2. * M.aK1n[sub] * udotm * M.u[1];
-        const double hac2 = 2. * M.aK1n[sub] * udotm * M.u[2];
-        double ha0 = hac0, ha1 = hac1, ha2 = hac2;
-        if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
-        // magnetoelastic (magnetic_AFM_solver.cpp:89-96, 156-162), exchange,
-        // inter-sublattice coupling and drive (:170-200)
-        double H0 = ldd(a.hstat[0], id) + ha0;
-        double H1 = ldd(a.hstat[1], id) + ha1;
-        double H2 = ldd(a.hstat[2], id) + ha2;
-        if (a.strain[0]) {
-            double he0, he1, he2;
-            magnetoelastic(a.strain, id, -1., M.aB1n[sub], M.aB2n[sub], mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
-            H0 = H0 + he0;
-            H1 = H1 + he1;
-            H2 = H2 + he2;
-        }
-        H0 = H0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * ldd(other.p[0], id) + a.hext[0];
-        H1 = H1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * ldd(other.p[1], id) + a.hext[1];
-        H2 = H2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * ldd(other.p[2], id) + a.hext[2];
-        if (a.hcell[0]) {
-            // electrodynamic back-action (magnetic_AFM_solver.cpp:222-226)
-            H0 = H0 + ldd(a.hcell[0], id);
-            H1 = H1 + ldd(a.hcell[1], id);
-            H2 = H2 + ldd(a.hcell[2], id);
-        }
-        const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
-        const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
-        const double k0 = M.aPrecess[sub] * px + M.aAlphaPrecess[sub] * qx;
-        const double k1 = M.aPrecess[sub] * py + M.aAlphaPrecess[sub] * qy;
-        const double k2 = M.aPrecess[sub] * pz + M.aAlphaPrecess[sub] * qz;
-        const double* const* base = sub == 0 ? a.base1 : a.base2;
-        double* const* acc = sub == 0 ? a.acc1 : a.acc2;
-        double* const* out = sub == 0 ? a.out1 : a.out2;
-        double* const* dmo = sub == 0 ? a.dm_out1 : a.dm_out2;
-        if (dmo[0]) {
-            std_(dmo[0], id, k0); std_(dmo[1], id, k1); std_(dmo[2], id, k2);
-        }
-        rk_update(a.stage, id, 0, k0, base, acc, out);
-        rk_update(a.stage, id, 1, k1, base, acc, out);
-        rk_update(a.stage, id, 2, k2, base, acc, out);
+    const P3 s1 = {{a.store1[0], a.store1[1], a.store1[2]}}, s2 = {{a.store2[0], a.store2[1], a.store2[2]}};
+    const P3 st = sub == 0 ? s1 : s2;
+    const P3 other = sub == 0 ? s2 : s1;
+    const double* const* base = sub == 0 ? a.base1 : a.base2;
+    double* const* acc = sub == 0 ? a.acc1 : a.acc2;
+    double* const* out = sub == 0 ? a.out1 : a.out2;
+    const uint32_t own = ldu(a.code, id);
+    const double mgx = ldd(st.p[0], id), mgy = ldd(st.p[1], id), mgz = ldd(st.p[2], id);
+    double v[6][3];
+    uint32_t cd[6];
+    gather6<I>(st, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, v, cd);
+    const double o0 = ldd(other.p[0], id), o1 = ldd(other.p[1], id), o2 = ldd(other.p[2], id);
+    const double hs0 = ldd(a.hstat[0], id), hs1 = ldd(a.hstat[1], id), hs2 = ldd(a.hstat[2], id);
+    double mb[3], ac[3];
+    for (int c = 0; c < 3; c++) {
+        mb[c] = ldd(base[c], id);
+        ac[c] = a.stage > 1 ? ldd(acc[c], id) : 0.;
     }
+    if (!(own & kIsAFM)) return;
+    GMat& M = mat_of(a.mats, own);
+    double mx = mgx, my = mgy, mz = mgz;
+    if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
+    const Stencil s = stencil_core<false>(mgx, mgy, mgz, v, cd, (GMat*)a.mats, own, kAfmSurfShift, sub, a.dx, a.dy,
+                                          a.dz, M.aAex[sub], M.aAex_self[sub], 0.);
+    // uniaxial anisotropy 2K1 (u.m) u (magnetic_AFM_solver.cpp:148-154)
+    const double udotm = mx * M.u[0] + my * M.u[1] + mz * M.u[2];
+    const double hac0 = 2. * M.aK1n[sub] * udotm * M.u[0];
+    const double hac1 = 2. * M.aK1n[sub] * udotm * M.u[1];
+    const double hac2 = 2. * M.aK1n[sub] * udotm * M.u[2];
+    double ha0 = hac0, ha1 = hac1, ha2 = hac2;
+    if (a.rotate) rot(a.Tc2g, hac0, hac1, hac2, ha0, ha1, ha2);
+    // magnetoelastic (magnetic_AFM_solver.cpp:89-96, 156-162), exchange,
+    // inter-sublattice coupling and drive (:170-200)
+    double H0 = hs0 + ha0;
+    double H1 = hs1 + ha1;
+    double H2 = hs2 + ha2;
+    if (a.strain[0]) {
+        double he0, he1, he2;
+        magnetoelastic(a.strain, id, -1., M.aB1n[sub], M.aB2n[sub], mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
+        H0 = H0 + he0;
+        H1 = H1 + he1;
+        H2 = H2 + he2;
+    }
+    H0 = H0 + M.aExch[sub] * s.ex[0] + M.aJ[sub] * o0 + a.hext[0];
+    H1 = H1 + M.aExch[sub] * s.ex[1] + M.aJ[sub] * o1 + a.hext[1];
+    H2 = H2 + M.aExch[sub] * s.ex[2] + M.aJ[sub] * o2 + a.hext[2];
+    if (a.hcell[0]) {
+        // electrodynamic back-action (magnetic_AFM_solver.cpp:222-226)
+        H0 = H0 + ldd(a.hcell[0], id);
+        H1 = H1 + ldd(a.hcell[1], id);
+        H2 = H2 + ldd(a.hcell[2], id);
+    }
+    const double px = mgy * H2 - mgz * H1, py = mgz * H0 - mgx * H2, pz = mgx * H1 - mgy * H0;
+    const double qx = mgy * pz - mgz * py, qy = mgz * px - mgx * pz, qz = mgx * py - mgy * px;
+    const double k0 = M.aPrecess[sub] * px + M.aAlphaPrecess[sub] * qx;
+    const double k1 = M.aPrecess[sub] * py + M.aAlphaPrecess[sub] * qy;
+    const double k2 = M.aPrecess[sub] * pz + M.aAlphaPrecess[sub] * qz;
+    double* const* dmo = sub == 0 ? a.dm_out1 : a.dm_out2;
+    if (dmo[0]) {
+        std_(dmo[0], id, k0); std_(dmo[1], id, k1); std_(dmo[2], id, k2);
+    }
+    rk_store(a.stage, id, 0, k0, mb[0], ac[0], acc, out);
+    rk_store(a.stage, id, 1, k1, mb[1], ac[1], acc, out);
+    rk_store(a.stage, id, 2, k2, mb[2], ac[2], acc, out);
 }
 
 // Workgroup b serves cell block (b / 16) * 8 + b % 8 and sublattice
