@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
         [&](int idx, double2 v) { buf[fpad((idx >> a.lognx) * a.nx + bitrev(idx & (a.nx - 1), a.lognx))] = v; });
     fft_twiddle_store(tw, a.tw, a.nx, tw0);
     __syncthreads();
-    fft_dit(buf, comps * a.pencils, a.lognx, tw);
+    fft_dit<4>(buf, comps * a.pencils, a.lognx, tw);
     if (kConv) {
         // magnetostatic_solver.cpp:163-177
         auto product = [&](int idx, const double (&kr)[9], const double (&ki)[9]) {
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
             product(idx, kr, ki);
         }
         __syncthreads();
-        ifft_dif(buf, comps * a.pencils, a.lognx, tw);
+        ifft_dif<4>(buf, comps * a.pencils, a.lognx, tw);
     }
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int x = idx & (a.nx - 1);

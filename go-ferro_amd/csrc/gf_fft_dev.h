@@ -23,107 +23,82 @@ __device__ __forceinline__ double2 fft_cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// P pencils of length N = 2^logn at buf[p*N], bit-reversed in, natural out.
-// Radix-2 decimation in time; two consecutive stages run in one pass per
-// thread (four elements, the same butterflies in the same order), which
-// halves the LDS round trips and barriers without changing a rounding.
-__device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, const double2* tw) {
-    const int N = 1 << logn;
-    int s = 0;
-    if (logn & 1) {
-        // lone first stage (span 2, twiddle 1)
-        const int total = pencils * (N >> 1);
-        for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            const int i0 = 2 * b, i1 = i0 + 1;
-            const double2 w = tw[0];
-            const double2 u = buf[fpad(i0)];
-            const double2 v = buf[fpad(i1)];
-            const double2 t = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
-            buf[fpad(i0)] = make_double2(u.x + t.x, u.y + t.y);
-            buf[fpad(i1)] = make_double2(u.x - t.x, u.y - t.y);
+// One pass of R = 2, 4 or 8 elements per thread over log2(R) consecutive
+// radix-2 stages s, s+1, ... of P pencils of length N = 2^logn at buf[p*N]:
+// the elements g + m*2^s (m < R) of each group are read once, go through
+// the stages' butterflies in registers, and are written back once.  The
+// butterflies, their twiddles tw[(i mod 2^t) * N/2^(t+1)] (stage t, pair
+// (i, i + 2^t)) and their arithmetic are those of the plain radix-2
+// transform, so the grouping changes no rounding.  Decimation in time
+// (forward, kInverse = false) runs the stages upwards with
+// (u, v) -> (u + v w, u - v w); decimation in frequency (inverse,
+// conjugate twiddles) runs them downwards with (u, v) -> (u + v, (u - v) w*).
+template <int R, bool kInverse>
+__device__ __forceinline__ void fft_pass(double2* buf, int pencils, int logn, int s, const double2* tw) {
+    constexpr int LR = R == 2 ? 1 : (R == 4 ? 2 : 3);
+    const int N = 1 << logn, h = 1 << s;
+    const int groups = N >> LR;
+    const int total = pencils * groups;
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+        const int p = b >> (logn - LR);
+        const int q = b & (groups - 1);
+        const int j = q & (h - 1);
+        const int g = p * N + ((q >> s) << (s + LR)) + j;
+        double2 a[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) a[m] = buf[fpad(g + m * h)];
+#pragma unroll
+        for (int rr = 0; rr < LR; rr++) {
+            const int r = kInverse ? LR - 1 - rr : rr;
+            const int span = 1 << r;
+            const int ts = N >> (s + r + 1);
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                if (m & span) continue;
+                const double2 tv = tw[(j + (m & (span - 1)) * h) * ts];
+                const double2 u = a[m], v = a[m + span];
+                if (!kInverse) {
+                    const double2 w = tv;
+                    const double2 t = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+                    a[m] = make_double2(u.x + t.x, u.y + t.y);
+                    a[m + span] = make_double2(u.x - t.x, u.y - t.y);
+                } else {
+                    const double2 w = make_double2(tv.x, -tv.y);
+                    const double2 d = make_double2(u.x - v.x, u.y - v.y);
+                    a[m] = make_double2(u.x + v.x, u.y + v.y);
+                    a[m + span] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+                }
+            }
         }
-        __syncthreads();
-        s = 1;
+#pragma unroll
+        for (int m = 0; m < R; m++) buf[fpad(g + m * h)] = a[m];
     }
-    const int total4 = pencils * (N >> 2);
-    for (; s < logn; s += 2) {
-        const int h = 1 << s;
-        const int ts0 = N >> (s + 1), ts1 = N >> (s + 2);
-        for (int b = threadIdx.x; b < total4; b += blockDim.x) {
-            const int p = b >> (logn - 2);
-            const int q = b & ((N >> 2) - 1);
-            const int j = q & (h - 1);
-            const int g = p * N + ((q >> s) << (s + 2)) + j;
-            double2 a0 = buf[fpad(g)], a1 = buf[fpad(g + h)], a2 = buf[fpad(g + 2 * h)], a3 = buf[fpad(g + 3 * h)];
-            // stage s: (a0, a1) and (a2, a3), twiddle j at span 2h
-            double2 w = tw[j * ts0];
-            double2 t = make_double2(a1.x * w.x - a1.y * w.y, a1.x * w.y + a1.y * w.x);
-            double2 b0 = make_double2(a0.x + t.x, a0.y + t.y), b1 = make_double2(a0.x - t.x, a0.y - t.y);
-            t = make_double2(a3.x * w.x - a3.y * w.y, a3.x * w.y + a3.y * w.x);
-            double2 b2 = make_double2(a2.x + t.x, a2.y + t.y), b3 = make_double2(a2.x - t.x, a2.y - t.y);
-            // stage s+1: (b0, b2) twiddle j, (b1, b3) twiddle j + h, at span 4h
-            w = tw[j * ts1];
-            t = make_double2(b2.x * w.x - b2.y * w.y, b2.x * w.y + b2.y * w.x);
-            buf[fpad(g)] = make_double2(b0.x + t.x, b0.y + t.y);
-            buf[fpad(g + 2 * h)] = make_double2(b0.x - t.x, b0.y - t.y);
-            w = tw[(j + h) * ts1];
-            t = make_double2(b3.x * w.x - b3.y * w.y, b3.x * w.y + b3.y * w.x);
-            buf[fpad(g + h)] = make_double2(b1.x + t.x, b1.y + t.y);
-            buf[fpad(g + 3 * h)] = make_double2(b1.x - t.x, b1.y - t.y);
-        }
-        __syncthreads();
-    }
+    __syncthreads();
 }
 
-// Inverse transform (conjugate twiddles), natural in, bit-reversed out;
-// radix-2 decimation in frequency with two stages per pass as above.
+// P pencils of length N = 2^logn at buf[p*N], bit-reversed in, natural out:
+// radix-2 decimation in time, log2(kRadix) stages per LDS pass (the
+// leftover stages first).  Radix 8 takes fewer passes and barriers; radix 4
+// needs fewer registers and keeps more threads busy on short transforms.
+template <int kRadix = 8>
+__device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, const double2* tw) {
+    constexpr int LR = kRadix == 8 ? 3 : 2;
+    int s = logn % LR;
+    if (s == 1) fft_pass<2, false>(buf, pencils, logn, 0, tw);
+    else if (s == 2) fft_pass<4, false>(buf, pencils, logn, 0, tw);
+    for (; s < logn; s += LR) fft_pass<kRadix, false>(buf, pencils, logn, s, tw);
+}
+
+// Inverse transform (conjugate twiddles), natural in, bit-reversed out:
+// radix-2 decimation in frequency, log2(kRadix) stages per pass from the
+// top, the leftover stages last.
+template <int kRadix = 8>
 __device__ __forceinline__ void ifft_dif(double2* buf, int pencils, int logn, const double2* tw) {
-    const int N = 1 << logn;
-    const int total4 = pencils * (N >> 2);
-    int s = logn - 1;
-    for (; s >= 1; s -= 2) {
-        const int h = 1 << s, h2 = h >> 1;
-        const int ts0 = N >> (s + 1), ts1 = N >> s;
-        for (int b = threadIdx.x; b < total4; b += blockDim.x) {
-            const int p = b >> (logn - 2);
-            const int q = b & ((N >> 2) - 1);
-            const int j = q & (h2 - 1);
-            const int g = p * N + ((q >> (s - 1)) << (s + 1)) + j;
-            const double2 a0 = buf[fpad(g)], a1 = buf[fpad(g + h2)], a2 = buf[fpad(g + h)], a3 = buf[fpad(g + h + h2)];
-            // stage s: (a0, a2) twiddle j, (a1, a3) twiddle j + h/2, at span 2h
-            double2 w = make_double2(tw[j * ts0].x, -tw[j * ts0].y);
-            double2 d = make_double2(a0.x - a2.x, a0.y - a2.y);
-            const double2 b0 = make_double2(a0.x + a2.x, a0.y + a2.y);
-            const double2 b2 = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-            w = make_double2(tw[(j + h2) * ts0].x, -tw[(j + h2) * ts0].y);
-            d = make_double2(a1.x - a3.x, a1.y - a3.y);
-            const double2 b1 = make_double2(a1.x + a3.x, a1.y + a3.y);
-            const double2 b3 = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-            // stage s-1: (b0, b1) and (b2, b3), twiddle j at span h
-            w = make_double2(tw[j * ts1].x, -tw[j * ts1].y);
-            d = make_double2(b0.x - b1.x, b0.y - b1.y);
-            buf[fpad(g)] = make_double2(b0.x + b1.x, b0.y + b1.y);
-            buf[fpad(g + h2)] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-            d = make_double2(b2.x - b3.x, b2.y - b3.y);
-            buf[fpad(g + h)] = make_double2(b2.x + b3.x, b2.y + b3.y);
-            buf[fpad(g + h + h2)] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-        }
-        __syncthreads();
-    }
-    if (s == 0) {
-        // lone last stage (span 2, twiddle 1)
-        const int total = pencils * (N >> 1);
-        for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            const int i0 = 2 * b, i1 = i0 + 1;
-            const double2 w = make_double2(tw[0].x, -tw[0].y);
-            const double2 u = buf[fpad(i0)];
-            const double2 v = buf[fpad(i1)];
-            const double2 d = make_double2(u.x - v.x, u.y - v.y);
-            buf[fpad(i0)] = make_double2(u.x + v.x, u.y + v.y);
-            buf[fpad(i1)] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
-        }
-        __syncthreads();
-    }
+    constexpr int LR = kRadix == 8 ? 3 : 2;
+    int s = logn - LR;
+    for (; s >= 0; s -= LR) fft_pass<kRadix, true>(buf, pencils, logn, s, tw);
+    if (s + LR == 2) fft_pass<4, true>(buf, pencils, logn, 0, tw);
+    else if (s + LR == 1) fft_pass<2, true>(buf, pencils, logn, 0, tw);
 }
 
 __device__ __forceinline__ void fft_load_twiddles(double2* dst, const double2* src, int n) {
