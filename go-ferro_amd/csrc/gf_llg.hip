@@ -48,7 +48,9 @@ __device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z,
     const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
     for (int c = 0; c < 3; c++) in.c[c] = ldd(a.store[c], id);
     gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), static_cast<I>(a.ny), static_cast<I>(a.nz), in.v, in.cd);
-    for (int c = 0; c < 3; c++) in.m[c] = ldd(a.base[c], id);
+    // at stage 1 the stage store is m itself: read it once
+    const bool same = a.base[0] == a.store[0] && a.base[1] == a.store[1] && a.base[2] == a.store[2];
+    for (int c = 0; c < 3; c++) in.m[c] = same ? in.c[c] : ldd(a.base[c], id);
     for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd(a.acc[c], id) : 0.;
 }
 
@@ -355,11 +357,13 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) 
     gather6<I>(st, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, v, cd);
     const double o0 = ldd(other.p[0], id), o1 = ldd(other.p[1], id), o2 = ldd(other.p[2], id);
     const double hs0 = ldd(a.hstat[0], id), hs1 = ldd(a.hstat[1], id), hs2 = ldd(a.hstat[2], id);
+    // at stage 1 the stage store is m itself: read it once
+    const bool same = base[0] == st.p[0] && base[1] == st.p[1] && base[2] == st.p[2];
     double mb[3], ac[3];
-    for (int c = 0; c < 3; c++) {
-        mb[c] = ldd(base[c], id);
-        ac[c] = a.stage > 1 ? ldd(acc[c], id) : 0.;
-    }
+    mb[0] = same ? mgx : ldd(base[0], id);
+    mb[1] = same ? mgy : ldd(base[1], id);
+    mb[2] = same ? mgz : ldd(base[2], id);
+    for (int c = 0; c < 3; c++) ac[c] = a.stage > 1 ? ldd(acc[c], id) : 0.;
     if (!(own & kIsAFM)) return;
     GMat& M = mat_of(a.mats, own);
     double mx = mgx, my = mgy, mz = mgz;
