@@ -50,8 +50,8 @@ __device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z,
     gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), static_cast<I>(a.ny), static_cast<I>(a.nz), in.v, in.cd);
     // at stage 1 the stage store is m itself: read it once
     const bool same = a.base[0] == a.store[0] && a.base[1] == a.store[1] && a.base[2] == a.store[2];
-    for (int c = 0; c < 3; c++) in.m[c] = same ? in.c[c] : ldd(a.base[c], id);
-    for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd(a.acc[c], id) : 0.;
+    for (int c = 0; c < 3; c++) in.m[c] = same ? in.c[c] : ldd_nt(a.base[c], id);
+    for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd_nt(a.acc[c], id) : 0.;
 }
 
 // RK update of one component (see the file comment) from its preloaded m
@@ -63,17 +63,17 @@ __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, dou
     double v;
     if (stage == 1) {
         v = m + k * 0.5;
-        std_(accp[c], id, k / 6.);
+        std_nt(accp[c], id, k / 6.);
     } else if (stage == 2) {
         v = m + k * 0.5;
-        std_(accp[c], id, acc + k / 3.);
+        std_nt(accp[c], id, acc + k / 3.);
     } else if (stage == 3) {
         v = m + k;
-        std_(accp[c], id, acc + k / 3.);
+        std_nt(accp[c], id, acc + k / 3.);
     } else {
         v = m + (acc + k / 6.);
     }
-    std_(out[c], id, v);
+    std_nt(out[c], id, v);
     return v;
 }
 

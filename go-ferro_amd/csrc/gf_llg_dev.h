@@ -45,6 +45,20 @@ __device__ __forceinline__ GMat& mat_of(const DevMaterial* mats, uint32_t code) 
     return ((GMat*)mats)[(code & kMatMask) - 1];
 }
 
+// Nontemporal forms for arrays a stage streams through once (m, the RK
+// accumulator, the next store): they then displace less of the stencil
+// stores and cell codes that neighbouring workgroups re-read from L2.
+__device__ __forceinline__ double ldd_nt(const double* p, int64_t i) {
+    return __builtin_nontemporal_load(&((GConstD*)p)[i]);
+}
+__device__ __forceinline__ double ldd_nt(const double* p, uint32_t i) {
+    return __builtin_nontemporal_load((GConstD*)((GConstC*)p + (i << 3)));
+}
+__device__ __forceinline__ void std_nt(double* p, int64_t i, double v) { __builtin_nontemporal_store(v, &((GD*)p)[i]); }
+__device__ __forceinline__ void std_nt(double* p, uint32_t i, double v) {
+    __builtin_nontemporal_store(v, (GD*)((GC*)p + (i << 3)));
+}
+
 __device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t i) { return ((GConstU*)p)[i]; }
 __device__ __forceinline__ uint32_t ldu(const uint32_t* p, uint32_t i) {
     return *(GConstU*)((GConstC*)p + (i << 2));
