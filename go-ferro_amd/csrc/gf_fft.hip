@@ -204,8 +204,9 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
 #pragma unroll
         for (int q = 0; q < 9; q++) {
             const int t = a.adjoint ? (q < 3 ? q : (q < 6 ? q + 3 : q - 3)) : q;
-            kr[q] = a.kre[t * s + g];
-            const double im = static_cast<double>(a.kim[t * s + g]);
+            // the tensor is streamed once per convolution
+            kr[q] = __builtin_nontemporal_load(&a.kre[t * s + g]);
+            const double im = static_cast<double>(__builtin_nontemporal_load(&a.kim[t * s + g]));
             ki[q] = a.adjoint ? -im : im;
         }
     };

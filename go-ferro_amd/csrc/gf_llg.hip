@@ -360,10 +360,10 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) 
     // at stage 1 the stage store is m itself: read it once
     const bool same = base[0] == st.p[0] && base[1] == st.p[1] && base[2] == st.p[2];
     double mb[3], ac[3];
-    mb[0] = same ? mgx : ldd(base[0], id);
-    mb[1] = same ? mgy : ldd(base[1], id);
-    mb[2] = same ? mgz : ldd(base[2], id);
-    for (int c = 0; c < 3; c++) ac[c] = a.stage > 1 ? ldd(acc[c], id) : 0.;
+    mb[0] = same ? mgx : ldd_nt(base[0], id);
+    mb[1] = same ? mgy : ldd_nt(base[1], id);
+    mb[2] = same ? mgz : ldd_nt(base[2], id);
+    for (int c = 0; c < 3; c++) ac[c] = a.stage > 1 ? ldd_nt(acc[c], id) : 0.;
     if (!(own & kIsAFM)) return;
     GMat& M = mat_of(a.mats, own);
     double mx = mgx, my = mgy, mz = mgz;
