@@ -792,6 +792,7 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
         }
         return fm_store(c, c->mag_store_sel);
     case GF_FIELD_HSTAT:
+        if (for_write) c->hstat_written = true;
         return c->hstat;
     case GF_FIELD_HEXT_NONUNIF:
         if (for_write && !c->hnu[0]) {
@@ -1058,6 +1059,12 @@ namespace gf {
 
 // Arguments of one FM stage reading store in_sel and writing store out_sel
 // (0 = m, 1 = st_a, 2 = st_b); the drive is left at zero.
+// Hstat can be non-zero only when statics run or it was uploaded; the
+// stage kernels skip reading it otherwise (it is zero from allocation).
+static bool hstat_live(const gf_ctx* c) {
+    return c->hstat_written || (c->flags & (GF_FLAG_MAGNETOSTATICS | GF_FLAG_MAG_1D | GF_FLAG_DEMAG_FACTOR));
+}
+
 StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     StageArgs a{};
     magnet_strain(c, a.strain);
@@ -1067,7 +1074,8 @@ StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     double** o = fm_store(c, out_sel);
     for (int i = 0; i < 3; i++) {
         a.store[i] = s[i]; a.base[i] = c->m[i]; a.acc[i] = c->acc[i]; a.out[i] = o[i];
-        a.hstat[i] = c->hstat[i]; a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = c->em_dm[i];
+        a.hstat[i] = hstat_live(c) ? c->hstat[i] : nullptr;
+        a.hnu[i] = c->has_hnu ? c->hnu[i] : nullptr; a.dm_out[i] = c->em_dm[i];
         a.hdrive[i] = c->hdrive[i];
         a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
     }
@@ -1106,7 +1114,7 @@ AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
         a.base1[i] = c->m1[i]; a.base2[i] = c->m2[i];
         a.acc1[i] = c->acc1[i]; a.acc2[i] = c->acc2[i];
         a.out1[i] = o1[i]; a.out2[i] = o2[i];
-        a.hstat[i] = c->hstat[i];
+        a.hstat[i] = hstat_live(c) ? c->hstat[i] : nullptr;
     }
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
     std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));

@@ -508,6 +508,7 @@ struct gf_ctx {
     double jf_amp = 0, jf_freq = 0, jf_sigma = 0;
     // statics
     double* hstat[3] = {};
+    bool hstat_written = false;     // Hstat uploaded (it is zero otherwise unless statics run)
     double* hnu[3] = {};
     double* hdrive[3] = {};         // shaped drive profile (GF_FLAG_HDRIVE)
     double hdrive_freq = 0;

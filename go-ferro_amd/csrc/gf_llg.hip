@@ -165,7 +165,10 @@ __device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
     const I x = col / ny, y = col - x * ny;
     FmIn in;
     fm_load<I>(a, id, x, y, z, in);
-    const double hs0 = ldd(a.hstat[0], id), hs1 = ldd(a.hstat[1], id), hs2 = ldd(a.hstat[2], id);
+    // a null Hstat is zero (no statics, nothing uploaded)
+    const bool hs = a.hstat[0] != nullptr;
+    const double hs0 = hs ? ldd(a.hstat[0], id) : 0., hs1 = hs ? ldd(a.hstat[1], id) : 0.;
+    const double hs2 = hs ? ldd(a.hstat[2], id) : 0.;
     if (!(in.own & kIsFM)) return;
     double n0, n1, n2;
     fm_cell<false, I>(a, (GMat*)a.mats, in, id, hs0, hs1, hs2, n0, n1, n2);
@@ -356,7 +359,10 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) 
     uint32_t cd[6];
     gather6<I>(st, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, v, cd);
     const double o0 = ldd(other.p[0], id), o1 = ldd(other.p[1], id), o2 = ldd(other.p[2], id);
-    const double hs0 = ldd(a.hstat[0], id), hs1 = ldd(a.hstat[1], id), hs2 = ldd(a.hstat[2], id);
+    // a null Hstat is zero (no statics, nothing uploaded)
+    const bool hs = a.hstat[0] != nullptr;
+    const double hs0 = hs ? ldd(a.hstat[0], id) : 0., hs1 = hs ? ldd(a.hstat[1], id) : 0.;
+    const double hs2 = hs ? ldd(a.hstat[2], id) : 0.;
     // at stage 1 the stage store is m itself: read it once
     const bool same = base[0] == st.p[0] && base[1] == st.p[1] && base[2] == st.p[2];
     double mb[3], ac[3];
