@@ -113,6 +113,15 @@ def test_fm_mixed_materials_rotation_dmi_drive():
     compare_stages(prob, {FIELD_M: m}, nsteps=2, hnu=hnu)
 
 
+def test_fm_mixed_materials_lean_fused_stage():
+    """Two ferromagnets (one with DMI), a non-magnet and vacuum, FFT
+    magnetostatics, no rotation and no optional fields: the fused stage's
+    lean form, which reads the material records from its LDS copy, with
+    interfaces between every pair of materials."""
+    prob, m = mixed_fm_problem(rotate=False)
+    compare_stages(prob, {FIELD_M: m}, nsteps=3)
+
+
 @pytest.mark.parametrize("mode", [FLAG_DEMAG_FACTOR | FLAG_MAGNETOSTATICS, FLAG_MAG_1D | FLAG_MAGNETOSTATICS, 0])
 def test_fm_local_statics(mode):
     prob, m = mixed_fm_problem(shape=(8, 16, 4), flags=mode, periodic=(True, False, True))
