@@ -206,17 +206,39 @@ def bench_adjoint(shape, nsteps: int) -> dict:
     t0 = time.perf_counter()
     grad, ghd = ctx.adjoint_backward()
     t_bwd = time.perf_counter() - t0
+    # the FP32 variant of the same sweep (its state is a copy; the context is
+    # reset anyway so that it starts from the same m and time)
+    reset()
+    ctx.adjoint_forward(1, lo, hi, 3e10, 4, precision=32)   # warm-up
+    ctx.adjoint_backward()
+    reset()
+    t0 = time.perf_counter()
+    L32 = ctx.adjoint_forward(1, lo, hi, 3e10, nsteps, precision=32)
+    t_fwd32 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad32, ghd32 = ctx.adjoint_backward()
+    t_bwd32 = time.perf_counter() - t0
     ctx.close()
     n = shape[0] * shape[1] * shape[2]
     print(f"bench: adjoint {shape}, {nsteps} steps: plain forward {t_plain:.3f} s, checkpointed forward "
-          f"{t_fwd:.3f} s, reverse sweep {t_bwd:.3f} s", file=sys.stderr, flush=True)
+          f"{t_fwd:.3f} s, reverse sweep {t_bwd:.3f} s; FP32 forward {t_fwd32:.3f} s, reverse {t_bwd32:.3f} s",
+          file=sys.stderr, flush=True)
+    fp32 = {"dtype": "f32", "checkpointed_forward_s": t_fwd32, "reverse_s": t_bwd32,
+            "reverse_over_forward": t_bwd32 / t_fwd32, "forward_cell_steps_per_s": n * nsteps / t_fwd32,
+            "speedup_vs_f64": {"forward": t_fwd / t_fwd32, "reverse": t_bwd / t_bwd32},
+            "loss": L32, "grad": {"alpha": grad32[0], "Aex": grad32[1], "B1": grad32[2]},
+            "rel_to_f64": {"loss": abs(L32 - L) / abs(L),
+                           "alpha": abs(grad32[0] - grad[0]) / abs(grad[0]),
+                           "Aex": abs(grad32[1] - grad[1]) / abs(grad[1]),
+                           "B1": abs(grad32[2] - grad[2]) / abs(grad[2]),
+                           "drive": float(((ghd32 - ghd) ** 2).sum() ** 0.5 / (ghd ** 2).sum() ** 0.5)}}
     return {"workload": f"FM film {shape[0]}x{shape[1]}x{shape[2]} + shaped top-layer drive (30 GHz) + static strain, "
                         f"probe 16x16x4 spectral power at 30 GHz, {nsteps} steps, sqrt(N) checkpoints",
             "dtype": "f64", "nsteps": nsteps, "forward_s": t_plain, "checkpointed_forward_s": t_fwd,
             "reverse_s": t_bwd, "adjoint_over_forward": t_bwd / t_plain,
             "forward_cell_steps_per_s": n * nsteps / t_plain, "loss": L,
             "grad": {"alpha": grad[0], "Aex": grad[1], "B1": grad[2]},
-            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5)}
+            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5), "fp32": fp32}
 
 
 def bench_coupled(which: str, steps: int) -> dict:

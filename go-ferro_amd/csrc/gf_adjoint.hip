@@ -487,7 +487,11 @@ gf_status gf_adjoint_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss) 
     for (int a = 0; a < 3; a++)
         if (d->probe_lo[a] < 0 || d->probe_hi[a] < d->probe_lo[a] || d->probe_hi[a] >= dims[a])
             return set_error(c, GF_ERR_ARG, "probe box outside the grid");
+    if (d->precision != 0 && d->precision != 64 && d->precision != 32)
+        return set_error(c, GF_ERR_ARG, "precision must be 64 or 32");
     adjoint_free(c);
+    adjoint32_free(c);
+    if (d->precision == 32) return adjoint32_forward(c, d, loss);
     AdjState* s = new AdjState();
     c->adj = s;
     s->d = *d;
@@ -560,6 +564,10 @@ gf_status gf_adjoint_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss) 
 
 gf_status gf_adjoint_backward(gf_ctx* c, double grad[3], double* grad_hdrive, size_t count) {
     if (!c || !grad) return set_error(c, GF_ERR_ARG, "null argument");
+    if (c->adj32) {
+        if (gf_status e = check_context(c, adjoint32_material(c))) return e;
+        return adjoint32_backward(c, grad, grad_hdrive, count);
+    }
     AdjState* s = c->adj;
     if (!s || !s->ready) return set_error(c, GF_ERR_STATE, "gf_adjoint_backward needs a gf_adjoint_forward first");
     if (grad_hdrive && !(c->flags & GF_FLAG_HDRIVE))

@@ -772,6 +772,7 @@ gf_status gf_destroy(gf_ctx* c) {
     if (c->d_code) hipFree(c->d_code);
     if (c->d_mats) hipFree(c->d_mats);
     adjoint_free(c);
+    adjoint32_free(c);
     demag_free(c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

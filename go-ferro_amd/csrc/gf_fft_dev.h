@@ -19,6 +19,16 @@ __device__ __forceinline__ int fft_bitrev(int j, int logn) {
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
 __host__ __device__ __forceinline__ int64_t fft_lds_elems(int64_t m) { return m + (m >> 4) + 1; }
 
+// Complex values of the LDS transforms: double2 for the FP64 path, float2
+// for the FP32 variant of the adjoint sweep.
+template <class C> struct FftComplex;
+template <> struct FftComplex<double2> {
+    static __device__ __forceinline__ double2 make(double re, double im) { return make_double2(re, im); }
+};
+template <> struct FftComplex<float2> {
+    static __device__ __forceinline__ float2 make(float re, float im) { return make_float2(re, im); }
+};
+
 __device__ __forceinline__ double2 fft_cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -33,8 +43,9 @@ __device__ __forceinline__ double2 fft_cmul(double2 a, double2 b) {
 // (forward, kInverse = false) runs the stages upwards with
 // (u, v) -> (u + v w, u - v w); decimation in frequency (inverse,
 // conjugate twiddles) runs them downwards with (u, v) -> (u + v, (u - v) w*).
-template <int R, bool kInverse>
-__device__ __forceinline__ void fft_pass(double2* buf, int pencils, int logn, int s, const double2* tw) {
+template <int R, bool kInverse, class C>
+__device__ __forceinline__ void fft_pass(C* buf, int pencils, int logn, int s, const C* tw) {
+    using F = FftComplex<C>;
     constexpr int LR = R == 2 ? 1 : (R == 4 ? 2 : 3);
     const int N = 1 << logn, h = 1 << s;
     const int groups = N >> LR;
@@ -44,7 +55,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, int pencils, int logn, in
         const int q = b & (groups - 1);
         const int j = q & (h - 1);
         const int g = p * N + ((q >> s) << (s + LR)) + j;
-        double2 a[R];
+        C a[R];
 #pragma unroll
         for (int m = 0; m < R; m++) a[m] = buf[fpad(g + m * h)];
 #pragma unroll
@@ -55,18 +66,18 @@ __device__ __forceinline__ void fft_pass(double2* buf, int pencils, int logn, in
 #pragma unroll
             for (int m = 0; m < R; m++) {
                 if (m & span) continue;
-                const double2 tv = tw[(j + (m & (span - 1)) * h) * ts];
-                const double2 u = a[m], v = a[m + span];
+                const C tv = tw[(j + (m & (span - 1)) * h) * ts];
+                const C u = a[m], v = a[m + span];
                 if (!kInverse) {
-                    const double2 w = tv;
-                    const double2 t = make_double2(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
-                    a[m] = make_double2(u.x + t.x, u.y + t.y);
-                    a[m + span] = make_double2(u.x - t.x, u.y - t.y);
+                    const C w = tv;
+                    const C t = F::make(v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x);
+                    a[m] = F::make(u.x + t.x, u.y + t.y);
+                    a[m + span] = F::make(u.x - t.x, u.y - t.y);
                 } else {
-                    const double2 w = make_double2(tv.x, -tv.y);
-                    const double2 d = make_double2(u.x - v.x, u.y - v.y);
-                    a[m] = make_double2(u.x + v.x, u.y + v.y);
-                    a[m + span] = make_double2(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
+                    const C w = F::make(tv.x, -tv.y);
+                    const C d = F::make(u.x - v.x, u.y - v.y);
+                    a[m] = F::make(u.x + v.x, u.y + v.y);
+                    a[m + span] = F::make(d.x * w.x - d.y * w.y, d.x * w.y + d.y * w.x);
                 }
             }
         }
@@ -80,8 +91,8 @@ __device__ __forceinline__ void fft_pass(double2* buf, int pencils, int logn, in
 // radix-2 decimation in time, log2(kRadix) stages per LDS pass (the
 // leftover stages first).  Radix 8 takes fewer passes and barriers; radix 4
 // needs fewer registers and keeps more threads busy on short transforms.
-template <int kRadix = 8>
-__device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, const double2* tw) {
+template <int kRadix = 8, class C>
+__device__ __forceinline__ void fft_dit(C* buf, int pencils, int logn, const C* tw) {
     constexpr int LR = kRadix == 8 ? 3 : 2;
     int s = logn % LR;
     if (s == 1) fft_pass<2, false>(buf, pencils, logn, 0, tw);
@@ -92,8 +103,8 @@ __device__ __forceinline__ void fft_dit(double2* buf, int pencils, int logn, con
 // Inverse transform (conjugate twiddles), natural in, bit-reversed out:
 // radix-2 decimation in frequency, log2(kRadix) stages per pass from the
 // top, the leftover stages last.
-template <int kRadix = 8>
-__device__ __forceinline__ void ifft_dif(double2* buf, int pencils, int logn, const double2* tw) {
+template <int kRadix = 8, class C>
+__device__ __forceinline__ void ifft_dif(C* buf, int pencils, int logn, const C* tw) {
     constexpr int LR = kRadix == 8 ? 3 : 2;
     int s = logn - LR;
     for (; s >= 0; s -= LR) fft_pass<kRadix, true>(buf, pencils, logn, s, tw);
@@ -108,10 +119,12 @@ __device__ __forceinline__ void fft_load_twiddles(double2* dst, const double2* s
 // The same in two halves, so that the first load can be issued together
 // with a kernel's other set-up loads: fft_twiddle_prefetch returns this
 // thread's first twiddle, fft_twiddle_store writes it and loads the rest.
-__device__ __forceinline__ double2 fft_twiddle_prefetch(const double2* src, int n) {
-    return static_cast<int>(threadIdx.x) < (n >> 1) ? src[threadIdx.x] : make_double2(0., 0.);
+template <class C>
+__device__ __forceinline__ C fft_twiddle_prefetch(const C* src, int n) {
+    return static_cast<int>(threadIdx.x) < (n >> 1) ? src[threadIdx.x] : FftComplex<C>::make(0, 0);
 }
-__device__ __forceinline__ void fft_twiddle_store(double2* dst, const double2* src, int n, double2 first) {
+template <class C>
+__device__ __forceinline__ void fft_twiddle_store(C* dst, const C* src, int n, C first) {
     if (static_cast<int>(threadIdx.x) < (n >> 1)) dst[threadIdx.x] = first;
     for (int j = threadIdx.x + blockDim.x; j < (n >> 1); j += blockDim.x) dst[j] = src[j];
 }

@@ -348,6 +348,7 @@ struct BatchLaunch {
 };
 
 struct AdjState;
+struct Adj32;
 
 // Clock and sinusoidal drive of one instance of a batch.
 struct BatchClock {
@@ -527,6 +528,7 @@ struct gf_ctx {
 
     gf::DemagPlan demag;
     gf::AdjState* adj = nullptr;    // checkpoints and buffers of the adjoint sweep
+    gf::Adj32* adj32 = nullptr;     // the same for its FP32 variant
 
     bool timing = false;
     std::vector<gf::TimerSlot> timers;
@@ -548,8 +550,12 @@ void timer_end(gf_ctx* c, int cls, hipEvent_t start);
 // the drive is left at zero
 StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
 void fm_stage_drives(gf_ctx* c, int stage, StageArgs& a);
-// adjoint sweep state (gf_adjoint.hip)
+// adjoint sweep state (gf_adjoint.hip) and its FP32 variant (gf_fp32.hip)
 void adjoint_free(gf_ctx* c);
+void adjoint32_free(gf_ctx* c);
+int adjoint32_material(const gf_ctx* c);
+gf_status adjoint32_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss);
+gf_status adjoint32_backward(gf_ctx* c, double grad[3], double* grad_hdrive, size_t count);
 AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
 // true when cell indices of an n-cell grid fit 32-bit byte offsets (gf_llg.hip)
 bool narrow(int64_t n);

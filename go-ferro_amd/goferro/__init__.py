@@ -210,6 +210,7 @@ class _AdjDesc(ctypes.Structure):
         ("struct_size", ctypes.c_uint32), ("material", ctypes.c_int32),
         ("probe_lo", ctypes.c_int32 * 3), ("probe_hi", ctypes.c_int32 * 3),
         ("probe_freq", ctypes.c_double), ("nsteps", ctypes.c_int64), ("checkpoint_every", ctypes.c_int64),
+        ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -498,9 +499,12 @@ class Context:
 
     # -- adjoint sweep
     def adjoint_forward(self, material: int, probe_lo, probe_hi, freq: float, nsteps: int,
-                        checkpoint_every: int = 0) -> float:
-        """N steps keeping checkpoints; returns the probe spectral loss."""
+                        checkpoint_every: int = 0, precision: int = 64) -> float:
+        """N steps keeping checkpoints; returns the probe spectral loss.
+        precision=32 runs the FP32 variant on copies of the state (the
+        context's state and time are left unchanged)."""
         d = _AdjDesc()
+        d.precision = int(precision)
         d.struct_size = ctypes.sizeof(_AdjDesc)
         d.material = int(material)
         d.probe_lo[:] = [int(v) for v in probe_lo]

@@ -354,11 +354,16 @@ typedef struct gf_adjoint_desc {
     double probe_freq;         /* Hz */
     int64_t nsteps;            /* N */
     int64_t checkpoint_every;  /* steps per checkpoint; 0 = ceil(sqrt(N)) */
+    int32_t precision;         /* 64 (or 0): FP64 sweep; 32: FP32 variant (see gf_adjoint_forward) */
+    int32_t reserved;          /* 0 */
 } gf_adjoint_desc;
 
 /* Run N steps from the current state (the magnetostatic field is rebuilt
  * from it first), keep the checkpoints and return L.  Leaves the context at
- * the final state, at a step boundary. */
+ * the final state, at a step boundary.  With precision 32 the trajectory,
+ * checkpoints and the reverse sweep run in FP32 on copies of the state
+ * (frame rotation and Hext_nonunif unsupported), and the context's own
+ * state and time are left unchanged. */
 gf_status gf_adjoint_forward(gf_ctx* ctx, const gf_adjoint_desc* desc, double* loss);
 
 /* Reverse sweep of the last gf_adjoint_forward: grad = dL/d(alpha, Aex, B1)

@@ -318,3 +318,74 @@ def test_torch_autograd_function():
     np.testing.assert_allclose(amp.grad.numpy(), 4e4 * ghd[0, :, :, -1], rtol=1e-12, atol=0)
     g.close()
     sim.close()
+
+
+# --- FP32 variant (gf_adjoint_desc.precision = 32) ------------------------
+# The FP32 sweep runs the same forward and discrete adjoint in FP32.  Its
+# bar is agreement with the FP64 sweep to FP32 accuracy over the trajectory:
+# loss 1e-5, scalar gradients 1e-4, drive gradient 1e-5 relative (observed
+# 4e-7, 1e-5 for alpha and below 1e-6 for the rest, 3e-7).
+
+def run_adjoint32(prob, fields, nsteps, K=0):
+    lo, hi = probe(prob.shape)
+    g = gpu(prob, fields)
+    L = g.adjoint_forward(1, lo, hi, FREQ, nsteps, checkpoint_every=K, precision=32)
+    grad, ghd = g.adjoint_backward()
+    return g, L, grad, ghd
+
+
+@pytest.mark.parametrize("demag,anis", [(True, 1), (False, 2)])
+def test_fp32_sweep_tracks_fp64(demag, anis):
+    say("case FP32 adjoint vs FP64, 16x16x8, 40 steps, demag", demag, "anisotropy", anis)
+    prob = problem(rotate=False, demag=demag, anis=anis)
+    f = inputs(prob.shape)
+    N = 40
+    g64, L64, grad64, ghd64 = run_adjoint(prob, f, N)
+    g32, L32, grad32, ghd32 = run_adjoint32(prob, f, N)
+    say("  loss", L64, L32, "rel", abs(L32 - L64) / L64)
+    assert abs(L32 - L64) <= 1e-5 * L64
+    for i in range(3):
+        rel = abs(grad32[i] - grad64[i]) / abs(grad64[i])
+        say("  param", i, grad64[i], grad32[i], "rel", rel)
+        assert rel <= 1e-4
+    rel = np.linalg.norm(ghd32 - ghd64) / np.linalg.norm(ghd64)
+    say("  drive gradient rel", rel)
+    assert rel <= 1e-5
+    g64.close()
+    g32.close()
+
+
+def test_fp32_checkpoints_and_state():
+    """The FP32 gradients do not depend on the checkpoint spacing (the
+    recomputed FP32 trajectory is bit-identical), and the sweep leaves the
+    context's FP64 state and time unchanged."""
+    say("case FP32 checkpoints and state")
+    prob = problem(shape=(8, 8, 4), rotate=False)
+    f = inputs(prob.shape)
+    out = []
+    for K in (1, 4, 0):
+        g = gpu(prob, f)
+        m0, t0 = g.download(FIELD_M), g.time
+        lo, hi = probe(prob.shape)
+        L = g.adjoint_forward(1, lo, hi, FREQ, 10, checkpoint_every=K, precision=32)
+        assert np.array_equal(g.download(FIELD_M), m0) and g.time == t0
+        grad, ghd = g.adjoint_backward()
+        assert np.array_equal(g.download(FIELD_M), m0) and g.time == t0
+        out.append((L, grad, ghd))
+        g.close()
+    for L, grad, ghd in out[1:]:
+        assert L == out[0][0]
+        assert np.array_equal(grad, out[0][1])
+        assert np.array_equal(ghd, out[0][2])
+
+
+def test_fp32_rejections():
+    say("case FP32 rejections")
+    prob = problem(shape=(4, 4, 4), rotate=True)
+    f = inputs(prob.shape)
+    g = gpu(prob, f)
+    with pytest.raises(GoFerroError, match="rotation"):
+        g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=32)
+    with pytest.raises(GoFerroError, match="precision"):
+        g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=16)
+    g.close()
