@@ -5,7 +5,8 @@ an FM simulation as a differentiable torch scalar of the damping, exchange
 constant and B1 of one material and of a per-cell drive-amplitude map on one
 layer.  The forward pass runs the native forward with checkpoints
 (gf_adjoint_forward), the backward pass the native reverse sweep
-(gf_adjoint_backward); nothing is differentiated by torch itself.
+(gf_adjoint_backward); nothing is differentiated by torch itself.  With
+``precision=32`` the simulation runs the FP32 variant of both sweeps.
 """
 from __future__ import annotations
 
@@ -21,7 +22,8 @@ class ProbeLossSim:
     drive_vector * amp(x, y) on layer k = drive_layer and zero elsewhere."""
 
     def __init__(self, problem: Problem, m0, material: int, probe_lo, probe_hi, freq: float, nsteps: int,
-                 drive_vector, drive_layer: int, static_fields=None, checkpoint_every: int = 0):
+                 drive_vector, drive_layer: int, static_fields=None, checkpoint_every: int = 0,
+                 precision: int = 64):
         self.problem = problem
         self.ctx = Context(problem)
         self.m0 = np.ascontiguousarray(m0, dtype=np.float64)
@@ -29,6 +31,7 @@ class ProbeLossSim:
         self.base = type(problem.materials[material - 1]).from_buffer_copy(problem.materials[material - 1])
         self.probe = (tuple(probe_lo), tuple(probe_hi))
         self.freq, self.nsteps, self.every = float(freq), int(nsteps), int(checkpoint_every)
+        self.precision = int(precision)
         self.vec = np.asarray(drive_vector, dtype=np.float64)
         self.layer = int(drive_layer)
         for fid, v in (static_fields or {}).items():
@@ -45,7 +48,7 @@ class ProbeLossSim:
         self.ctx.upload(FIELD_M, self.m0)
         self.ctx.time = 0.0
         return self.ctx.adjoint_forward(self.material, self.probe[0], self.probe[1], self.freq, self.nsteps,
-                                        self.every)
+                                        self.every, precision=self.precision)
 
     def grads(self):
         """(dL/dalpha, dL/dAex, dL/dB1, dL/damp) of the last loss."""

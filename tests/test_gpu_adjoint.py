@@ -389,3 +389,29 @@ def test_fp32_rejections():
     with pytest.raises(GoFerroError, match="precision"):
         g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=16)
     g.close()
+
+
+def test_fp32_torch_autograd_function():
+    import torch
+
+    from goferro.autograd import ProbeLossSim, probe_loss
+    say("case torch autograd FP32")
+    prob = problem(shape=(8, 8, 4), rotate=False)
+    f = inputs(prob.shape)
+    lo, hi = probe(prob.shape)
+    sim = ProbeLossSim(prob, f[FIELD_M], material=1, probe_lo=lo, probe_hi=hi, freq=FREQ, nsteps=12,
+                       drive_vector=(4e4, 0.0, 0.0), drive_layer=prob.shape[2] - 1,
+                       static_fields={FIELD_STRAIN_T0_DIAG: f[FIELD_STRAIN_T0_DIAG],
+                                      FIELD_STRAIN_T0_OFF: f[FIELD_STRAIN_T0_OFF]}, precision=32)
+    amp = torch.tensor(f[FIELD_HDRIVE][0, :, :, -1] / 4e4, dtype=torch.float64, requires_grad=True)
+    alpha = torch.tensor(0.02, dtype=torch.float64, requires_grad=True)
+    aex = torch.tensor(3.7e-12, dtype=torch.float64, requires_grad=True)
+    b1 = torch.tensor(3.5e5, dtype=torch.float64, requires_grad=True)
+    L = probe_loss(sim, alpha, aex, b1, amp)
+    L.backward()
+    g, L2, grad, ghd = run_adjoint32(prob, f, 12)
+    np.testing.assert_allclose(float(L), L2, rtol=1e-12)
+    np.testing.assert_allclose([alpha.grad.item(), aex.grad.item(), b1.grad.item()], grad, rtol=1e-12)
+    np.testing.assert_allclose(amp.grad.numpy(), 4e4 * ghd[0, :, :, -1], rtol=1e-12, atol=0)
+    g.close()
+    sim.close()
