@@ -291,6 +291,10 @@ gf_status statics_after_update(gf_ctx* c) {
     const bool ms = c->flags & GF_FLAG_MAGNETOSTATICS;
     const bool one_d = c->flags & GF_FLAG_MAG_1D;
     if (em) return GF_OK;
+    if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->any_fe) {
+        // get_E_static after the update phase (main.cpp:299-302)
+        if (gf_status st = elec_run(c, fe_pstore(c, c->fe_store_sel))) return st;
+    }
     if (one_d) {
         if ((c->any_fm && ms) || c->any_afm) return gf_demag(c);
         return GF_OK;
@@ -725,6 +729,19 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     if (!st && (c->flags & GF_FLAG_MAGNETOSTATICS) && !(c->flags & GF_FLAG_MAG_1D) &&
         !(c->flags & GF_FLAG_DEMAG_FACTOR) && (c->any_fm || c->any_afm))
         st = demag_setup(c);
+    if (!st && (c->flags & GF_FLAG_ELECTROSTATICS)) {
+        if (!c->any_fe) {
+            st = set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics needs a ferroelectric material");
+        } else if (c->flags & GF_FLAG_MAGNETOSTATICS) {
+            st = set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics cannot be combined with FFT magnetostatics");
+        } else {
+            // er_homo: the largest relative permittivity times scale_elec
+            // (ferroelectric_initialize.cpp:311-318)
+            double er = 0.;
+            for (const auto& m : c->host_mats) er = std::max(er, m.r_permittivity);
+            st = elec_setup(c, er * (d->scale_elec > 0. ? d->scale_elec : 1.));
+        }
+    }
     if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "create sync");
     if (st) {
         g_create_error = c->err;
@@ -1525,6 +1542,11 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
 
 gf_status gf_demag(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
+    if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->any_fe) {
+        // initial get_E_static (main.cpp:120-131)
+        if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT electrostatics was not set up");
+        if (gf_status st = elec_run(c, fe_pstore(c, c->fe_store_sel))) return st;
+    }
     if (!(c->any_fm || c->any_afm)) return GF_OK;
     double* zero[3] = {nullptr, nullptr, nullptr};
     double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : zero;
@@ -1532,6 +1554,7 @@ gf_status gf_demag(gf_ctx* c) {
     double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : zero;
     if (c->flags & GF_FLAG_MAG_1D) return hip_check(c, launch_demag_local(c, fm, s1, s2, 1), "demag 1d");
     if (c->flags & GF_FLAG_DEMAG_FACTOR) return hip_check(c, launch_demag_local(c, fm, s1, s2, 0), "demag factor");
+    if (!(c->flags & GF_FLAG_MAGNETOSTATICS) && (c->flags & GF_FLAG_ELECTROSTATICS)) return GF_OK;
     if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT magnetostatics was not set up (flags)");
     if (c->demag.fused) return demag_run_fused(c, fm);
     return demag_run(c, fm, s1, s2);

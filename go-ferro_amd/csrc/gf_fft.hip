@@ -581,6 +581,82 @@ gf_status demag_setup(gf_ctx* c) {
     return GF_OK;
 }
 
+// FFT electrostatics of the polarisation on the same five passes (the
+// reference declares get_E_static, ferroelectric_system.h:103, without a
+// definition; its set-up, ferroelectric_initialize.cpp:310-338 and
+// global.cpp:270-318, fixes a homogeneous permittivity er and continuous
+// wave vectors k_a = t 2 pi / (n_a d_a), t = i for i <= n/2 else i - n).  The
+// nine "tensor" spectra are E(k) = G(k) P(k) with G_ab = -k_a k_b /
+// (eps0 er |k|^2), G(0) = 0, stored real (imaginary parts zero) and scaled
+// by den/n for the passes' unnormalised transforms and their division by
+// den = -dx dy dz.  An off-diagonal product with a component on its Nyquist
+// plane (t = n/2, its own mirror) is set to zero, so that the spectrum is
+// Hermitian and the field real.
+gf_status elec_setup(gf_ctx* c, double er) {
+    if (!pow2(c->nx) || !pow2(c->ny) || !pow2(c->nz))
+        return set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics needs power-of-two nx, ny, nz");
+    if (c->nx > 1024 || c->ny > 2048 || c->nz > 2048)
+        return set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics supports nx <= 1024, ny, nz <= 2048");
+    if (!(er > 0.)) return set_error(c, GF_ERR_ARG, "FFT electrostatics needs a positive r_permittivity");
+    DemagPlan& d = c->demag;
+    d.nzh = c->nz / 2 + 1;
+    const int64_t nx = c->nx, ny = c->ny, nzh = d.nzh, nh = nx * ny * nzh;
+    gf_status st;
+    if ((st = hip_check(c, hipMalloc(&d.spec_m, sizeof(double2) * 3 * nh), "hipMalloc spec_m"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.spec_a, sizeof(double2) * 3 * nh), "hipMalloc spec_a"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.spec_b, sizeof(double2) * 3 * nh), "hipMalloc spec_b"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.kre, sizeof(double) * 9 * nh), "hipMalloc kre"))) return st;
+    if ((st = hip_check(c, hipMalloc(&d.kim, sizeof(float) * 9 * nh), "hipMalloc kim"))) return st;
+    if ((st = make_twiddles(c, d.twx, c->nx))) return st;
+    if ((st = make_twiddles(c, d.twy, c->ny))) return st;
+    if ((st = make_twiddles(c, d.twz, c->nz))) return st;
+    auto wave = [](int64_t i, int64_t n, double h, bool& nyq) {
+        const int64_t t = i <= n / 2 ? i : i - n;
+        nyq = n >= 2 && 2 * i == n;
+        return static_cast<double>(t) * (2. * kPi / (static_cast<double>(n) * h));
+    };
+    const double scale = (-c->dx * c->dy * c->dz) / static_cast<double>(c->n);
+    std::vector<double> g(static_cast<size_t>(9 * nh));
+    // spectra order xx, yy, zz, xy, yz, zx, yx, zy, xz; layout [kz][y][x]
+    const int pa[9] = {0, 1, 2, 0, 1, 2, 1, 2, 0}, pb[9] = {0, 1, 2, 1, 2, 0, 0, 1, 2};
+    for (int64_t k = 0; k < nzh; k++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t i = 0; i < nx; i++) {
+                bool ny3[3];
+                const double kv[3] = {wave(i, nx, c->dx, ny3[0]), wave(j, ny, c->dy, ny3[1]),
+                                      wave(k, c->nz, c->dz, ny3[2])};
+                const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                const int64_t q = (k * ny + j) * nx + i;
+                for (int s = 0; s < 9; s++) {
+                    const int a = pa[s], b = pb[s];
+                    double v = 0.;
+                    if (k2 > 0. && (a == b || (!ny3[a] && !ny3[b])))
+                        v = -kv[a] * kv[b] / (kEps0 * er * k2) * scale;
+                    g[static_cast<size_t>(s * nh + q)] = v;
+                }
+            }
+    if ((st = hip_check(c, hipMemcpy(d.kre, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice),
+                        "copy electrostatic spectra")))
+        return st;
+    if ((st = hip_check(c, hipMemset(d.kim, 0, sizeof(float) * 9 * nh), "clear electrostatic spectra"))) return st;
+    d.ready = true;
+    d.fused = false;
+    d.h_in_spec = false;
+    d.h_real_valid = true;
+    d.m_in_spec = false;
+    return GF_OK;
+}
+
+// The static field of the current polarisation store into Estat.
+gf_status elec_run(gf_ctx* c, const double* const* p) {
+    gf_status st;
+    if ((st = hip_check(c, run_zfwd(c, true, p, nullptr, nullptr, nullptr, 3), "zfwd electrostatics"))) return st;
+    if ((st = hip_check(c, run_y(c, false, 3), "yfwd electrostatics"))) return st;
+    if ((st = hip_check(c, run_x(c, true), "xconv electrostatics"))) return st;
+    if ((st = hip_check(c, run_y(c, true, 3), "yinv electrostatics"))) return st;
+    return hip_check(c, run_zinv(c, c->estat), "zinv electrostatics");
+}
+
 // Fused mode: M z-spectrum (from the stage, or a z pass when the store
 // changed otherwise) -> y, x with the tensor product, y inverse -> H
 // z-spectrum in spec_a; the real field is built only when asked for.

@@ -597,6 +597,9 @@ gf_status demag_materialize_h(gf_ctx* c);
 gf_status demag_run_fused(gf_ctx* c, const double* const* fm_store);
 gf_status demag_run_to(gf_ctx* c, const double* const* fm_store, double* const* out);
 gf_status demag_adjoint(gf_ctx* c, const double* const* in, double* const* out);
+// FFT electrostatics on the same plan (gf_fft.hip)
+gf_status elec_setup(gf_ctx* c, double er);
+gf_status elec_run(gf_ctx* c, const double* const* p);
 void demag_free(gf_ctx* c);
 hipError_t launch_demag_local(gf_ctx* c, const double* const* fm_store, const double* const* s1,
                               const double* const* s2, int one_d);

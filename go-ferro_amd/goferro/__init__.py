@@ -29,7 +29,7 @@ __all__ = [
     "FLAG_EM_FROM_P", "FLAG_EM_FROM_M", "FLAG_EM_BACKACTION_P", "FLAG_EM_BACKACTION_M", "FLAG_PLANE_EM",
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
-    "FLAG_HDRIVE", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG", "FIELD_STRAIN_T0_OFF",
+    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG", "FIELD_STRAIN_T0_OFF",
     "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
 ]
@@ -62,6 +62,7 @@ FLAG_PEC_XY = 1 << 20
 FLAG_PEC_Z = 1 << 21
 FLAG_1D_ABC_EM_ONLYTOP = 1 << 22
 FLAG_HDRIVE = 1 << 23
+FLAG_ELECTROSTATICS = 1 << 24
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
@@ -202,6 +203,7 @@ class _Desc(ctypes.Structure):
         ("Jfin_lo", ctypes.c_int32 * 3), ("Jfin_hi", ctypes.c_int32 * 3),
         ("Jf_input_amp", ctypes.c_double), ("Jf_input_freq", ctypes.c_double), ("Jf_input_sigma", ctypes.c_double),
         ("Hdrive_freq", ctypes.c_double),
+        ("scale_elec", ctypes.c_double),
     ]
 
 
@@ -254,6 +256,7 @@ class Problem:
     Jf_input_freq: float = 0.0
     Jf_input_sigma: float = 0.0
     Hdrive_freq: float = 0.0
+    scale_elec: float = 1.0
 
     @property
     def n(self) -> int:
@@ -306,6 +309,7 @@ class Problem:
         d.Jf_input_freq = float(self.Jf_input_freq)
         d.Jf_input_sigma = float(self.Jf_input_sigma)
         d.Hdrive_freq = float(self.Hdrive_freq)
+        d.scale_elec = float(self.scale_elec)
         keep = [cell, mats]
         if self.prescribe_Hext is not None:
             pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)

@@ -69,6 +69,15 @@ typedef int gf_status;
  * Hext_nonunif is static); it carries the per-cell drive amplitudes whose
  * gradients the adjoint returns. */
 #define GF_FLAG_HDRIVE           (1u << 23)
+/* FFT electrostatics of the ferroelectric (get_E_static, declared at
+ * ferroelectric_system.h:103 but not defined in the reference): the static
+ * field of the polarisation, E(k) = -k (k.P(k)) / (eps0 er |k|^2), E(0) = 0,
+ * with er = max r_permittivity * scale_elec and the continuous wave vectors
+ * of global.cpp:270-318, recomputed into GF_FIELD_ESTAT by the initial
+ * statics (gf_demag) and after every update phase unless the
+ * electrodynamics is on (main.cpp:120-167, 299-306).  Not together with
+ * GF_FLAG_MAGNETOSTATICS (the two would share one FFT plan). */
+#define GF_FLAG_ELECTROSTATICS   (1u << 24)
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -171,6 +180,8 @@ typedef struct gf_desc {
     double Jf_input_sigma;          /* s (type 6) */
     /* shaped drive (GF_FLAG_HDRIVE) */
     double Hdrive_freq;             /* Hz */
+    /* permittivity scale of the electrostatics (read.cpp:56); 0 = 1 */
+    double scale_elec;
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */

@@ -74,6 +74,7 @@ struct gfo_ctx {
     /* statics */
     double *H[3], *Hnu[3];
     double *Hdrive[3], hdrive_freq;            /* shaped drive (GF_FLAG_HDRIVE), no reference counterpart */
+    double er_homo;                            /* electrostatics (GF_FLAG_ELECTROSTATICS) */
     double *e_t0[6];                           /* static crystal-frame strain, ex*t0_crt */
     /* electrodynamics (EMdynamic_system.h): E on edges, H on faces */
     int if_em, periodic_all, pec_all, jp_on;
@@ -437,6 +438,11 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
     c->flags = d->flags; c->dt = d->dt; c->time = 0.;
     c->hdrive_freq = d->Hdrive_freq;
+    /* er_homo = max r_permittivity * scale_elec (ferroelectric_initialize.cpp:311-318) */
+    c->er_homo = 0.;
+    for (int t = 0; t < d->n_materials; t++)
+        if (d->materials[t].r_permittivity > c->er_homo) c->er_homo = d->materials[t].r_permittivity;
+    c->er_homo = c->er_homo * (d->scale_elec > 0. ? d->scale_elec : 1.);
     c->nmat = d->n_materials;
     c->mat = (gf_material*)malloc(sizeof(gf_material) * (size_t)(c->nmat > 0 ? c->nmat : 1));
     memcpy(c->mat, d->materials, sizeof(gf_material) * (size_t)c->nmat);
@@ -2035,7 +2041,57 @@ static void conv_3d(gfo_ctx* c, const double* M, double* const* out) {
     free(Mk); free(Hk);
 }
 
+/* FFT electrostatics (GF_FLAG_ELECTROSTATICS; no counterpart in the
+ * reference: get_E_static is declared at ferroelectric_system.h:103 but not
+ * defined).  From the reference's set-up (ferroelectric_initialize.cpp:
+ * 310-338, global.cpp:270-318): homogeneous permittivity er_homo and
+ * continuous wave vectors k_a = t 2 pi / (n_a d_a), t = i for i <= n/2, else
+ * i - n.  E(k) = -k (k.P(k)) / (eps0 er |k|^2) from the current P store,
+ * E(0) = 0; an off-diagonal product with a component on its Nyquist plane
+ * (t = n/2) is zero so that the field is real. */
+static double kwave(int64_t i, int64_t n, double h, int* nyq) {
+    int64_t t = i <= n / 2 ? i : i - n;
+    *nyq = n >= 2 && 2 * i == n;
+    return (double)t * (2. * PI / ((double)n * h));
+}
+
+void gfo_estatic(gfo_ctx* c) {
+    const double EPS0_ = 8.854187817e-12;   /* EMdynamic_system.h:13 */
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz, nzh = nz / 2 + 1, nh = nx * ny * nzh;
+    double* Pk = (double*)malloc(sizeof(double) * 2 * nh * 3);
+    double* Ek = (double*)malloc(sizeof(double) * 2 * nh);
+    for (int a = 0; a < 3; a++) gfo_rfft3(nx, ny, nz, c->ps[a], Pk + a * 2 * nh);
+    for (int a = 0; a < 3; a++) {
+        for (int64_t i = 0; i < nx; i++)
+            for (int64_t j = 0; j < ny; j++)
+                for (int64_t k = 0; k < nzh; k++) {
+                    int nyq[3];
+                    double kv[3];
+                    kv[0] = kwave(i, nx, c->dx, &nyq[0]);
+                    kv[1] = kwave(j, ny, c->dy, &nyq[1]);
+                    kv[2] = kwave(k, nz, c->dz, &nyq[2]);
+                    double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                    int64_t q = (i * ny + j) * nzh + k;
+                    double er = 0., ei = 0.;
+                    if (k2 > 0.)
+                        for (int b = 0; b < 3; b++) {
+                            if (a != b && (nyq[a] || nyq[b])) continue;
+                            double g = -kv[a] * kv[b] / (EPS0_ * c->er_homo * k2);
+                            er += g * Pk[b * 2 * nh + 2 * q];
+                            ei += g * Pk[b * 2 * nh + 2 * q + 1];
+                        }
+                    Ek[2 * q] = er;
+                    Ek[2 * q + 1] = ei;
+                }
+        gfo_irfft3(nx, ny, nz, Ek, c->Es[a]);
+        for (int64_t q = 0; q < c->n; q++) c->Es[a][q] = c->Es[a][q] / (double)c->n;
+    }
+    free(Pk); free(Ek);
+}
+
 void gfo_demag(gfo_ctx* c) {
+    if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->if_FE_all) gfo_estatic(c);
+    if (!(c->if_FM_all || c->if_AFM_all)) return;
     if (c->flags & GF_FLAG_MAG_1D) demag_1d(c);
     else demag_3d(c);
 }
@@ -2064,6 +2120,7 @@ void gfo_step(gfo_ctx* c, int64_t nsteps) {
             if (c->if_AFM_all) gfo_afm_update(c, s);
             if (c->if_FE_all) gfo_fe_update(c, s);
             if (c->if_em) gfo_em_update(c, s);
+            if (!em && c->if_FE_all && (c->flags & GF_FLAG_ELECTROSTATICS)) gfo_estatic(c);
             if (!em && one_d && ((c->if_FM_all && ms) || c->if_AFM_all)) demag_1d(c);
             if (!em && mag && !one_d && ms) demag_3d(c);
         }

@@ -1,0 +1,97 @@
+"""GPU tests of the FFT electrostatics (GF_FLAG_ELECTROSTATICS) through the
+C ABI against the oracle (itself pinned by closed forms in
+test_oracle_estatic.py): the static field of a random polarisation on a
+block of ferroelectric, non-polar material and vacuum, and coupled
+ferroelectric steps whose field is recomputed after every update phase.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from goferro import (FIELD_ESTAT, FIELD_P, FIELD_Q, FLAG_ELECTROSTATICS, FLAG_MAGNETOSTATICS, Context,
+                     GoFerroError, Material, Problem)
+from goferro.workloads import pto_material
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def block(shape=(8, 8, 16), periodic=(True, True, False), flags=FLAG_ELECTROSTATICS, scale=1.0):
+    fe = pto_material(FE_mass=1e-16, FE_damping=4e-4)
+    fe.r_permittivity = 12.0
+    sub = Material.make(r_permittivity=25.0)
+    cell = np.zeros(shape, np.uint32)
+    cell[:, :, : shape[2] // 4] = 2
+    cell[:, :, shape[2] // 4: 3 * shape[2] // 4] = 1
+    prob = Problem(shape=shape, spacing=(1e-9, 1.2e-9, 0.8e-9), materials=[fe, sub], material_cell=cell,
+                   dt=2e-15, periodic=periodic, flags=flags, Eext_stat=(0.0, 0.0, 2e6), scale_elec=scale)
+    rng = np.random.default_rng(17)
+    p = 0.05 * rng.standard_normal((3,) + shape)
+    p[2] += 0.7
+    p[:, cell != 1] = 0.0
+    q = 1e8 * rng.standard_normal((3,) + shape)
+    q[:, cell != 1] = 0.0
+    return prob, p, q
+
+
+def pair(prob, p, q):
+    from oracle.oracle import Oracle
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in ((FIELD_P, p), (FIELD_Q, q)):
+        g.upload(fid, v)
+        o.set(fid, v)
+    return g, o
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("shape,scale", [((8, 8, 16), 1.0), ((16, 4, 32), 1.7), ((1, 1, 64), 1.0)])
+def test_static_field_matches_oracle(shape, scale):
+    say("case electrostatic field", shape, scale)
+    prob, p, q = block(shape, scale=scale)
+    g, o = pair(prob, p, q)
+    g.demag()
+    o.demag()
+    d = rel(g.download(FIELD_ESTAT), o.get(FIELD_ESTAT))
+    say("  rel", d)
+    assert d <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_ferroelectric_steps_with_electrostatics():
+    say("case FE steps with electrostatics")
+    prob, p, q = block()
+    g, o = pair(prob, p, q)
+    g.demag()
+    o.demag()
+    g.step(6)
+    o.step(6)
+    for fid in (FIELD_P, FIELD_Q, FIELD_ESTAT):
+        d = rel(g.download(fid), o.get(fid))
+        say("  field", fid, "rel", d)
+        assert d <= 1e-11
+    assert abs(g.time - o.time) <= 1e-30
+    g.close()
+    o.close()
+
+
+def test_electrostatics_rejections():
+    say("case electrostatics rejections")
+    prob, _, _ = block(flags=FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
+    with pytest.raises(GoFerroError, match="magnetostatics"):
+        Context(prob)
+    prob, _, _ = block(shape=(8, 8, 12))
+    with pytest.raises(GoFerroError, match="power-of-two"):
+        Context(prob)
+    shape = (4, 4, 4)
+    prob = Problem(shape=shape, spacing=(1e-9,) * 3, materials=[Material.make(r_permittivity=3.0)],
+                   material_cell=np.ones(shape, np.uint32), dt=1e-15, flags=FLAG_ELECTROSTATICS)
+    with pytest.raises(GoFerroError, match="ferroelectric"):
+        Context(prob)
