@@ -8,6 +8,7 @@
 // mirrors main.cpp:243-483 for the magnetic physics.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -284,6 +285,21 @@ double** afm2_store(gf_ctx* c, int sel) { return sel == 0 ? c->m2 : (sel == 1 ? 
 double** fe_pstore(gf_ctx* c, int sel) { return sel == 0 ? c->p : (sel == 1 ? c->p_a : c->p_b); }
 double** fe_qstore(gf_ctx* c, int sel) { return sel == 0 ? c->q : (sel == 1 ? c->q_a : c->q_b); }
 
+// The magnetostatic part of gf_demag: H of the current magnet stores.
+gf_status mag_statics(gf_ctx* c) {
+    if (!(c->any_fm || c->any_afm)) return GF_OK;
+    double* zero[3] = {nullptr, nullptr, nullptr};
+    double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : zero;
+    double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : zero;
+    double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : zero;
+    if (c->flags & GF_FLAG_MAG_1D) return hip_check(c, launch_demag_local(c, fm, s1, s2, 1), "demag 1d");
+    if (c->flags & GF_FLAG_DEMAG_FACTOR) return hip_check(c, launch_demag_local(c, fm, s1, s2, 0), "demag factor");
+    if (!(c->flags & GF_FLAG_MAGNETOSTATICS) && (c->flags & GF_FLAG_ELECTROSTATICS)) return GF_OK;
+    if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT magnetostatics was not set up (flags)");
+    if (c->demag.fused) return demag_run_fused(c, fm);
+    return demag_run(c, fm, s1, s2);
+}
+
 // statics after an update phase (main.cpp:299-306 and the 1D field inside
 // update_m*, magnetic_solver.cpp:1533-1538, magnetic_AFM_solver.cpp:1746-1751)
 gf_status statics_after_update(gf_ctx* c) {
@@ -296,10 +312,10 @@ gf_status statics_after_update(gf_ctx* c) {
         if (gf_status st = elec_run(c, fe_pstore(c, c->fe_store_sel))) return st;
     }
     if (one_d) {
-        if ((c->any_fm && ms) || c->any_afm) return gf_demag(c);
+        if ((c->any_fm && ms) || c->any_afm) return mag_statics(c);
         return GF_OK;
     }
-    if ((c->any_fm || c->any_afm) && ms) return gf_demag(c);
+    if ((c->any_fm || c->any_afm) && ms) return mag_statics(c);
     return GF_OK;
 }
 
@@ -539,6 +555,13 @@ gf_status check_after_elastic(gf_ctx* c, int stage) {
 
 }  // namespace
 
+namespace gf {
+gf_status set_thread_error(gf_status code, const std::string& msg) {
+    g_create_error = msg;
+    return code;
+}
+}  // namespace gf
+
 extern "C" {
 
 const char* gf_last_error(const gf_ctx* ctx) {
@@ -732,7 +755,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     if (!st && (c->flags & GF_FLAG_ELECTROSTATICS)) {
         if (!c->any_fe) {
             st = set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics needs a ferroelectric material");
-        } else if (c->flags & GF_FLAG_MAGNETOSTATICS) {
+        } else if (c->demag.ready) {
             st = set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics cannot be combined with FFT magnetostatics");
         } else {
             // er_homo: the largest relative permittivity times scale_elec
@@ -1547,18 +1570,9 @@ gf_status gf_demag(gf_ctx* c) {
         if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT electrostatics was not set up");
         if (gf_status st = elec_run(c, fe_pstore(c, c->fe_store_sel))) return st;
     }
-    if (!(c->any_fm || c->any_afm)) return GF_OK;
-    double* zero[3] = {nullptr, nullptr, nullptr};
-    double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : zero;
-    double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : zero;
-    double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : zero;
-    if (c->flags & GF_FLAG_MAG_1D) return hip_check(c, launch_demag_local(c, fm, s1, s2, 1), "demag 1d");
-    if (c->flags & GF_FLAG_DEMAG_FACTOR) return hip_check(c, launch_demag_local(c, fm, s1, s2, 0), "demag factor");
-    if (!(c->flags & GF_FLAG_MAGNETOSTATICS) && (c->flags & GF_FLAG_ELECTROSTATICS)) return GF_OK;
-    if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT magnetostatics was not set up (flags)");
-    if (c->demag.fused) return demag_run_fused(c, fm);
-    return demag_run(c, fm, s1, s2);
+    return mag_statics(c);
 }
+
 
 // main.cpp:243-483 for the magnetic physics
 gf_status gf_step(gf_ctx* c, int64_t nsteps) {
@@ -1649,6 +1663,370 @@ gf_status gf_kernel_times(gf_ctx* c, double ms[GF_K_COUNT], int64_t launches[GF_
     drain_timers(c);
     for (int k = 0; k < GF_K_COUNT; k++) { ms[k] = c->kernel_ms[k]; launches[k] = c->kernel_launches[k]; }
     return GF_OK;
+}
+
+}  // extern "C"
+
+// ---- checkpoint / restart --------------------------------------------------
+//
+// One host image of everything a step boundary carries into the next step:
+// a header (grid, flags, dt, a hash of the material ids, t, the drive-table
+// cursors, the store selectors and the boundary-history head), the material
+// table, every persistent device array under a tag, and a checksum.  The
+// reference restarts from its text inputs (inoutput.cpp:11-327: m, m_AFM,
+// Hstat, p, q, Estat, E and H on the Yee grid, u, v, the elastic force
+// store, the strains, Jp) at seven digits and t = 0; the image holds them at
+// full precision together with t, the Mur and Liao histories and the H
+// z-spectrum of the fused magnetostatics, so a restarted run continues bit
+// for bit.
+
+namespace {
+
+constexpr char kStateMagic[8] = {'G', 'F', 'S', 'T', 'A', 'T', 'E', '1'};
+constexpr uint32_t kStateVersion = 1;
+constexpr uint64_t kHashSeed = 14695981039346656037ull;
+
+struct StateHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t header_bytes;
+    int64_t nx, ny, nz;
+    double dx, dy, dz, dt;
+    int32_t per[3];
+    uint32_t flags;
+    uint64_t cell_hash;
+    double time;
+    int64_t prescribe_idx, prescribe_e_idx;
+    int32_t hist_head, hist_slots;
+    int32_t mag_sel, afm_sel, fe_sel, e_sel, h_sel;
+    uint32_t bits;
+    int32_t n_materials;
+    int32_t n_arrays;
+    uint64_t material_bytes;
+};
+
+struct StateArrayHeader {
+    uint32_t tag;
+    uint32_t reserved;
+    int64_t count;   // doubles
+};
+
+enum : uint32_t {
+    kBitHstatWritten = 1u, kBitJpOn = 2u, kBitJpUploaded = 4u, kBitHasHnu = 8u,
+    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u
+};
+
+// FNV-1a over 64-bit words, then over the tail bytes.
+uint64_t hash_bytes(uint64_t h, const void* p, size_t n) {
+    constexpr uint64_t kPrime = 1099511628211ull;
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        h = (h ^ w) * kPrime;
+    }
+    for (; i < n; i++) h = (h ^ b[i]) * kPrime;
+    return h;
+}
+
+struct StateArray {
+    uint32_t tag;
+    double* p;
+    int64_t count;
+};
+
+// The persistent device arrays for the given selectors and bits, tagged
+// group << 8 | component.  Stage stores appear only when a selector points
+// away from the state (an uploaded store); spectra only while they hold the
+// current field.
+struct StateSel {
+    int mag, afm, fe, e, h;
+};
+
+StateSel current_sel(const gf_ctx* c) {
+    return {c->mag_store_sel, c->afm_store_sel, c->fe_store_sel, c->e_sel, c->h_sel};
+}
+
+std::vector<StateArray> state_arrays(gf_ctx* c, const StateSel& sel, uint32_t bits) {
+    std::vector<StateArray> v;
+    const int64_t n = c->n;
+    auto add = [&](uint32_t group, double* const* p, int k, int64_t count) {
+        for (int i = 0; i < k; i++)
+            if (p[i]) v.push_back({group << 8 | static_cast<uint32_t>(i), p[i], count});
+    };
+    if (c->any_fm) {
+        add(1, c->m, 3, n);
+        if (sel.mag != 0) add(2, fm_store(c, sel.mag), 3, n);
+    }
+    if (c->any_afm) {
+        add(3, c->m1, 3, n);
+        add(4, c->m2, 3, n);
+        if (sel.afm != 0) { add(5, afm1_store(c, sel.afm), 3, n); add(6, afm2_store(c, sel.afm), 3, n); }
+    }
+    if (c->any_fe) {
+        add(7, c->p, 3, n);
+        add(8, c->q, 3, n);
+        if (sel.fe != 0) { add(9, fe_pstore(c, sel.fe), 3, n); add(10, fe_qstore(c, sel.fe), 3, n); }
+        add(11, c->estat, 3, n);
+    }
+    if (c->el_on) {
+        add(12, c->u, 3, n);
+        add(13, c->v, 3, n);
+        add(14, c->us, 3, n);
+        add(15, c->vs, 3, n);
+        add(16, c->fstore, 3, n);
+        add(17, c->eg, 6, n);
+        add(18, c->e0g, 6, n);
+        if (c->flags & GF_FLAG_ORIENT_ROTATE) { add(19, c->ec, 6, n); add(20, c->e0c, 6, n); }
+        if (c->hist)
+            v.push_back({21u << 8, c->hist, static_cast<int64_t>(c->hist_slots) * kHistPlanes * 3 * c->nx * c->ny});
+    }
+    if (c->em_on) {
+        for (int a = 0; a < 3; a++) {
+            v.push_back({22u << 8 | static_cast<uint32_t>(a), c->Eb[a], c->nE[a]});
+            v.push_back({23u << 8 | static_cast<uint32_t>(a), c->Hb[a], c->nH[a]});
+            v.push_back({24u << 8 | static_cast<uint32_t>(a), c->Et1[a], c->nE[a]});
+        }
+        for (int a = 0; a < 3 && sel.e != 0; a++)
+            v.push_back({33u << 8 | static_cast<uint32_t>(a), e_store(c, sel.e)[a], c->nE[a]});
+        for (int a = 0; a < 3 && sel.h != 0; a++)
+            v.push_back({34u << 8 | static_cast<uint32_t>(a), h_store(c, sel.h)[a], c->nH[a]});
+        add(25, c->Ecell, 3, n);
+        add(26, c->Hcell, 3, n);
+        add(27, c->Jf, 3, n);
+        add(28, c->Jp, 9, n);
+        add(35, c->Jps, 9, n);
+    }
+    add(29, c->hstat, 3, n);
+    if (bits & kBitHasHnu) add(30, c->hnu, 3, n);
+    add(31, c->hdrive, 3, n);
+    add(32, c->e_t0, 6, n);
+    const DemagPlan& d = c->demag;
+    if (d.ready) {
+        const int64_t spec = 2 * 3 * c->nx * c->ny * d.nzh;
+        if (bits & kBitHInSpec) v.push_back({40u << 8, reinterpret_cast<double*>(d.spec_a), spec});
+        if (bits & kBitMInSpec) v.push_back({41u << 8, reinterpret_cast<double*>(d.spec_m), spec});
+    }
+    return v;
+}
+
+bool at_step_boundary(const gf_ctx* c) {
+    return !((c->any_fm && c->mag_next_stage != 1) || (c->any_afm && c->afm_next_stage != 1) ||
+             (c->any_fe && c->fe_next_stage != 1) || (c->el_on && c->el_next_stage != 1) ||
+             (c->em_on && c->em_next_stage != 1));
+}
+
+uint32_t state_bits(const gf_ctx* c) {
+    uint32_t b = 0;
+    if (c->hstat_written) b |= kBitHstatWritten;
+    if (c->jp_on) b |= kBitJpOn;
+    if (c->jp_uploaded) b |= kBitJpUploaded;
+    if (c->has_hnu) b |= kBitHasHnu;
+    if (c->demag.h_in_spec) b |= kBitHInSpec;
+    if (c->demag.h_real_valid) b |= kBitHRealValid;
+    if (c->demag.m_in_spec) b |= kBitMInSpec;
+    return b;
+}
+
+size_t state_bytes(const gf_ctx* c, const std::vector<StateArray>& arr) {
+    size_t b = sizeof(StateHeader) + sizeof(gf_material) * c->host_mats.size() + sizeof(uint64_t);
+    for (const auto& a : arr) b += sizeof(StateArrayHeader) + sizeof(double) * static_cast<size_t>(a.count);
+    return b;
+}
+
+uint64_t cell_hash(const gf_ctx* c) {
+    return hash_bytes(kHashSeed, c->host_code.data(), sizeof(uint32_t) * c->host_code.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+gf_status gf_state_size(gf_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return set_error(c, GF_ERR_ARG, "null argument");
+    const auto arr = state_arrays(c, current_sel(c), state_bits(c));
+    *bytes = state_bytes(c, arr);
+    return GF_OK;
+}
+
+gf_status gf_state_save(gf_ctx* c, void* host, size_t bytes) {
+    if (!c || !host) return set_error(c, GF_ERR_ARG, "null argument");
+    if (!at_step_boundary(c)) return set_error(c, GF_ERR_STATE, "the state can be saved only at a step boundary");
+    const uint32_t bits = state_bits(c);
+    const auto arr = state_arrays(c, current_sel(c), bits);
+    const size_t need = state_bytes(c, arr);
+    if (bytes != need) return set_error(c, GF_ERR_ARG, "the state image takes " + std::to_string(need) + " bytes");
+    unsigned char* out = static_cast<unsigned char*>(host);
+    StateHeader h{};
+    std::memcpy(h.magic, kStateMagic, sizeof(kStateMagic));
+    h.version = kStateVersion;
+    h.header_bytes = sizeof(StateHeader);
+    h.nx = c->nx; h.ny = c->ny; h.nz = c->nz;
+    h.dx = c->dx; h.dy = c->dy; h.dz = c->dz; h.dt = c->dt;
+    for (int a = 0; a < 3; a++) h.per[a] = c->per[a];
+    h.flags = c->flags;
+    h.cell_hash = cell_hash(c);
+    h.time = c->time;
+    h.prescribe_idx = c->prescribe_idx;
+    h.prescribe_e_idx = c->prescribe_e_idx;
+    h.hist_head = c->hist_head;
+    h.hist_slots = c->hist_slots;
+    h.mag_sel = c->mag_store_sel; h.afm_sel = c->afm_store_sel; h.fe_sel = c->fe_store_sel;
+    h.e_sel = c->e_sel; h.h_sel = c->h_sel;
+    h.bits = bits;
+    h.n_materials = static_cast<int32_t>(c->host_mats.size());
+    h.n_arrays = static_cast<int32_t>(arr.size());
+    h.material_bytes = sizeof(gf_material) * c->host_mats.size();
+    size_t o = 0;
+    std::memcpy(out + o, &h, sizeof(h));
+    o += sizeof(h);
+    std::memcpy(out + o, c->host_mats.data(), h.material_bytes);
+    o += h.material_bytes;
+    gf_status st = GF_OK;
+    for (const auto& a : arr) {
+        StateArrayHeader ah{a.tag, 0u, a.count};
+        std::memcpy(out + o, &ah, sizeof(ah));
+        o += sizeof(ah);
+        const size_t b = sizeof(double) * static_cast<size_t>(a.count);
+        if (!st) st = hip_check(c, hipMemcpyAsync(out + o, a.p, b, hipMemcpyDeviceToHost, c->stream), "state save");
+        o += b;
+    }
+    if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "state save sync");
+    if (st) return st;
+    const uint64_t sum = hash_bytes(kHashSeed, out, o);
+    std::memcpy(out + o, &sum, sizeof(sum));
+    return GF_OK;
+}
+
+gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
+    if (!c || !host) return set_error(c, GF_ERR_ARG, "null argument");
+    if (!at_step_boundary(c)) return set_error(c, GF_ERR_STATE, "the state can be loaded only at a step boundary");
+    const unsigned char* in = static_cast<const unsigned char*>(host);
+    StateHeader h{};
+    if (bytes < sizeof(h) + sizeof(uint64_t)) return set_error(c, GF_ERR_ARG, "the state image is truncated");
+    std::memcpy(&h, in, sizeof(h));
+    if (std::memcmp(h.magic, kStateMagic, sizeof(kStateMagic)) != 0)
+        return set_error(c, GF_ERR_ARG, "not a goferro state image");
+    if (h.version != kStateVersion || h.header_bytes != sizeof(StateHeader))
+        return set_error(c, GF_ERR_ARG, "state image version " + std::to_string(h.version) + " is not supported");
+    uint64_t sum = 0;
+    std::memcpy(&sum, in + bytes - sizeof(sum), sizeof(sum));
+    if (hash_bytes(kHashSeed, in, bytes - sizeof(sum)) != sum)
+        return set_error(c, GF_ERR_ARG, "the state image checksum does not match (corrupt or truncated)");
+    if (h.nx != c->nx || h.ny != c->ny || h.nz != c->nz)
+        return set_error(c, GF_ERR_ARG, "the state image is of another grid");
+    if (std::memcmp(&h.dx, &c->dx, sizeof(double)) || std::memcmp(&h.dy, &c->dy, sizeof(double)) ||
+        std::memcmp(&h.dz, &c->dz, sizeof(double)) || std::memcmp(&h.dt, &c->dt, sizeof(double)))
+        return set_error(c, GF_ERR_ARG, "the state image has other cell sizes or another dt");
+    if (h.per[0] != c->per[0] || h.per[1] != c->per[1] || h.per[2] != c->per[2] || h.flags != c->flags)
+        return set_error(c, GF_ERR_ARG, "the state image has other boundary conditions or flags");
+    if (h.cell_hash != cell_hash(c)) return set_error(c, GF_ERR_ARG, "the state image has other material ids");
+    if (h.n_materials != static_cast<int32_t>(c->host_mats.size()) ||
+        h.material_bytes != sizeof(gf_material) * c->host_mats.size())
+        return set_error(c, GF_ERR_ARG, "the state image has another material table");
+    if (h.hist_slots != c->hist_slots || h.hist_head < 0 || (c->hist_slots > 0 && h.hist_head >= c->hist_slots))
+        return set_error(c, GF_ERR_ARG, "the state image has another boundary history");
+    auto sel_ok = [](int32_t s) { return s >= 0 && s <= 2; };
+    if (!sel_ok(h.mag_sel) || !sel_ok(h.afm_sel) || !sel_ok(h.fe_sel) || !sel_ok(h.e_sel) || !sel_ok(h.h_sel))
+        return set_error(c, GF_ERR_ARG, "the state image has invalid store selectors");
+    if ((h.bits & (kBitHInSpec | kBitMInSpec)) && !c->demag.ready)
+        return set_error(c, GF_ERR_ARG, "the state image holds spectra this context has no plan for");
+    // the material table: equal, or applied record by record where allowed
+    size_t o = sizeof(h);
+    std::vector<gf_material> mats(c->host_mats.size());
+    std::memcpy(mats.data(), in + o, h.material_bytes);
+    o += h.material_bytes;
+    // the array list this context expects for the image's selectors
+    gf_status st = GF_OK;
+    if ((h.bits & kBitHasHnu) && !c->hnu[0]) {
+        st = alloc3(c, c->hnu, "hipMalloc hext_nonunif");
+        if (st) return st;
+    }
+    const auto arr = state_arrays(c, StateSel{h.mag_sel, h.afm_sel, h.fe_sel, h.e_sel, h.h_sel}, h.bits);
+    if (h.n_arrays != static_cast<int32_t>(arr.size()) || bytes != state_bytes(c, arr))
+        return set_error(c, GF_ERR_ARG, "the state image does not hold the arrays of this context");
+    size_t scan = o;
+    for (const auto& a : arr) {
+        StateArrayHeader ah{};
+        std::memcpy(&ah, in + scan, sizeof(ah));
+        if (ah.tag != a.tag || ah.count != a.count)
+            return set_error(c, GF_ERR_ARG, "the state image does not hold the arrays of this context");
+        scan += sizeof(ah) + sizeof(double) * static_cast<size_t>(a.count);
+    }
+    for (size_t i = 0; i < mats.size() && !st; i++) {
+        if (std::memcmp(&mats[i], &c->host_mats[i], sizeof(gf_material)) == 0) continue;
+        st = gf_set_material(c, static_cast<int>(i + 1), &mats[i]);
+    }
+    if (st) return st;
+    for (const auto& a : arr) {
+        o += sizeof(StateArrayHeader);
+        const size_t b = sizeof(double) * static_cast<size_t>(a.count);
+        if (!st) st = hip_check(c, hipMemcpyAsync(a.p, in + o, b, hipMemcpyHostToDevice, c->stream), "state load");
+        o += b;
+    }
+    if (!st) st = hip_check(c, hipStreamSynchronize(c->stream), "state load sync");
+    if (st) return st;
+    c->time = h.time;
+    c->prescribe_idx = static_cast<int>(h.prescribe_idx);
+    c->prescribe_e_idx = static_cast<int>(h.prescribe_e_idx);
+    c->hist_head = h.hist_head;
+    c->mag_store_sel = h.mag_sel; c->afm_store_sel = h.afm_sel; c->fe_store_sel = h.fe_sel;
+    c->e_sel = h.e_sel; c->h_sel = h.h_sel;
+    c->hstat_written = h.bits & kBitHstatWritten;
+    c->jp_on = h.bits & kBitJpOn;
+    c->jp_uploaded = h.bits & kBitJpUploaded;
+    c->has_hnu = h.bits & kBitHasHnu;
+    c->demag.h_in_spec = h.bits & kBitHInSpec;
+    c->demag.h_real_valid = h.bits & kBitHRealValid;
+    c->demag.m_in_spec = h.bits & kBitMInSpec;
+    return GF_OK;
+}
+
+gf_status gf_state_write(gf_ctx* c, const char* path) {
+    if (!c || !path) return set_error(c, GF_ERR_ARG, "null argument");
+    size_t bytes = 0;
+    gf_status st = gf_state_size(c, &bytes);
+    if (st) return st;
+    std::vector<unsigned char> buf;
+    try {
+        buf.resize(bytes);
+    } catch (const std::bad_alloc&) {
+        return set_error(c, GF_ERR_ALLOC, "host allocation of the state image failed");
+    }
+    if ((st = gf_state_save(c, buf.data(), bytes))) return st;
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return set_error(c, GF_ERR_ARG, std::string("cannot open ") + tmp + " for writing");
+    const bool ok = std::fwrite(buf.data(), 1, bytes, f) == bytes;
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return set_error(c, GF_ERR_ARG, std::string("writing ") + path + " failed");
+    }
+    return GF_OK;
+}
+
+gf_status gf_state_read(gf_ctx* c, const char* path) {
+    if (!c || !path) return set_error(c, GF_ERR_ARG, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(c, GF_ERR_ARG, std::string("cannot open ") + path);
+    std::vector<unsigned char> buf;
+    bool ok = std::fseek(f, 0, SEEK_END) == 0;
+    const long len = ok ? std::ftell(f) : -1;
+    ok = ok && len >= 0 && std::fseek(f, 0, SEEK_SET) == 0;
+    if (ok) {
+        try {
+            buf.resize(static_cast<size_t>(len));
+        } catch (const std::bad_alloc&) {
+            std::fclose(f);
+            return set_error(c, GF_ERR_ALLOC, "host allocation of the state image failed");
+        }
+        ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    std::fclose(f);
+    if (!ok) return set_error(c, GF_ERR_ARG, std::string("reading ") + path + " failed");
+    return gf_state_load(c, buf.data(), buf.size());
 }
 
 }  // extern "C"

@@ -542,6 +542,8 @@ namespace gf {
 // error helpers (gf_api.hip)
 gf_status set_error(gf_ctx* c, gf_status code, const std::string& msg);
 gf_status hip_check(gf_ctx* c, hipError_t e, const char* what);
+// the thread's error of a call without a context (gf_last_error(NULL))
+gf_status set_thread_error(gf_status code, const std::string& msg);
 void timer_begin(gf_ctx* c, int cls, hipEvent_t* ev);
 void timer_end(gf_ctx* c, int cls, hipEvent_t start);
 

@@ -115,6 +115,8 @@ EXPORTED_SYMBOLS = [
     "gf_adjoint_forward", "gf_adjoint_backward", "gf_adjoint_stage",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_sync",
     "gf_batch_get_stream", "gf_batch_config", "gf_batch_set_config", "gf_batch_tune",
+    "gf_state_size", "gf_state_save", "gf_state_load", "gf_state_write", "gf_state_read",
+    "gf_write_cells", "gf_read_cells",
 ]
 
 
@@ -373,6 +375,11 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
                              ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "gf_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "gf_batch_tune": ([P, ctypes.c_int, dp], ctypes.c_int),
+        "gf_state_size": ([P, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+        "gf_state_save": ([P, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
+        "gf_state_load": ([P, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
+        "gf_state_write": ([P, ctypes.c_char_p], ctypes.c_int),
+        "gf_state_read": ([P, ctypes.c_char_p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -500,6 +507,38 @@ class Context:
 
     def sync(self) -> None:
         self._check(self.lib.gf_sync(self._h), "gf_sync")
+
+    # -- restart image (gf_state_*)
+    def state_image(self) -> bytes:
+        """Everything the next step reads, at full precision, as bytes."""
+        n = ctypes.c_size_t()
+        self._check(self.lib.gf_state_size(self._h, ctypes.byref(n)), "gf_state_size")
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self.lib.gf_state_save(self._h, buf, n.value), "gf_state_save")
+        return buf.raw
+
+    def restore_state(self, image: bytes) -> None:
+        """Continue from a state_image() of a context of the same problem."""
+        buf = ctypes.create_string_buffer(bytes(image), len(image))
+        self._check(self.lib.gf_state_load(self._h, buf, len(image)), "gf_state_load")
+        self._sync_materials(bytes(image))
+
+    def save_state(self, path) -> None:
+        self._check(self.lib.gf_state_write(self._h, str(path).encode()), "gf_state_write")
+
+    def load_state(self, path) -> None:
+        self._check(self.lib.gf_state_read(self._h, str(path).encode()), "gf_state_read")
+        self._sync_materials(Path(path).read_bytes())
+
+    def _sync_materials(self, image: bytes) -> None:
+        # the image's material table replaces the problem's (gf_state_load
+        # applied any record that differs)
+        from .io import _StateHeader
+        hs = ctypes.sizeof(_StateHeader)
+        h = _StateHeader.from_buffer_copy(image[:hs])
+        sz = ctypes.sizeof(Material)
+        self.problem.materials = [Material.from_buffer_copy(image[hs + i * sz:hs + (i + 1) * sz])
+                                  for i in range(h.n_materials)]
 
     # -- adjoint sweep
     def adjoint_forward(self, material: int, probe_lo, probe_hi, freq: float, nsteps: int,

@@ -76,7 +76,8 @@ typedef int gf_status;
  * of global.cpp:270-318, recomputed into GF_FIELD_ESTAT by the initial
  * statics (gf_demag) and after every update phase unless the
  * electrodynamics is on (main.cpp:120-167, 299-306).  Not together with
- * GF_FLAG_MAGNETOSTATICS (the two would share one FFT plan). */
+ * the FFT magnetostatics (the two would share one FFT plan); the 1D and
+ * demag-factor magnetostatics combine with it. */
 #define GF_FLAG_ELECTROSTATICS   (1u << 24)
 
 /*
@@ -233,8 +234,9 @@ typedef struct gf_ctx gf_ctx;
 gf_status gf_create(const gf_desc* desc, gf_ctx** out);
 gf_status gf_destroy(gf_ctx* ctx);
 
-/* Last error message of a context (or of the last failed gf_create when
- * ctx is NULL).  Never NULL. */
+/* Last error message of a context (or, when ctx is NULL, of the calling
+ * thread's last failed gf_create or context-free call such as
+ * gf_read_cells).  Never NULL. */
 const char* gf_last_error(const gf_ctx* ctx);
 
 /* Host <-> device copies of one field.  count is the number of doubles
@@ -438,6 +440,40 @@ gf_status gf_batch_set_config(gf_batch* batch, int threads, int cells_per_thread
  * as it was) and keeps the fastest.  best_ms (may be NULL) receives its
  * time per stage. */
 gf_status gf_batch_tune(gf_batch* batch, int reps, double* best_ms);
+
+/* ---- restart and the reference's text files ------------------------------
+ *
+ * State image: everything a step boundary carries into the next step, at
+ * full precision -- t, the drive-table cursors, m / m_AFM / p / q / u / v /
+ * E / H / Jp and any uploaded stage stores, the elastic force store and
+ * strains, the Liao or Mur boundary history, the Mur planes of the Yee
+ * field, Hstat (or the H z-spectrum of the fused magnetostatics), the
+ * external-field and strain inputs, and the material table.  The reference
+ * restarts from its text inputs instead (input_m ... input_Jp,
+ * inoutput.cpp:11-327, called at main.cpp:44-89): seven digits, t = 0, no
+ * force store or boundary history.  An image loads only into a context
+ * created from the same grid, cell sizes, dt, flags and material ids, at a
+ * step boundary; a material record that differs is applied as
+ * gf_set_material would.  Layout: a header, the material records, then per
+ * array a (tag, count) pair and the doubles, then a 64-bit FNV-1a checksum
+ * of everything before it (little-endian, this library's build only). */
+gf_status gf_state_size(gf_ctx* ctx, size_t* bytes);
+gf_status gf_state_save(gf_ctx* ctx, void* host, size_t bytes);      /* bytes must equal gf_state_size */
+gf_status gf_state_load(gf_ctx* ctx, const void* host, size_t bytes);
+gf_status gf_state_write(gf_ctx* ctx, const char* path);             /* the image as a file */
+gf_status gf_state_read(gf_ctx* ctx, const char* path);
+
+/* Per-cell text files of the reference (output_m and its kind,
+ * inoutput.cpp:330-864; input_m and its kind, :11-327): one line per cell,
+ * "i  j  k" 1-based with k fastest, then ncomp values "%.7e" two spaces
+ * apart.  data holds ncomp arrays of dims[0]*dims[1]*dims[2] doubles, cell
+ * (i, j, k) at (i*dims[1] + j)*dims[2] + k; a Yee component passes its own
+ * staggered dims.  Reading accepts the records in any order but, unlike the
+ * reference (which reads n records blindly and keeps the old value of a
+ * cell it never saw), requires every cell exactly once and nothing after
+ * them.  No context: gf_last_error(NULL) holds the reason of a failure. */
+gf_status gf_write_cells(const char* path, const int64_t dims[3], int32_t ncomp, const double* data);
+gf_status gf_read_cells(const char* path, const int64_t dims[3], int32_t ncomp, double* data);
 
 #ifdef __cplusplus
 }

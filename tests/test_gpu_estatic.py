@@ -9,9 +9,9 @@ import sys
 import numpy as np
 import pytest
 
-from goferro import (FIELD_ESTAT, FIELD_P, FIELD_Q, FLAG_ELECTROSTATICS, FLAG_MAGNETOSTATICS, Context,
-                     GoFerroError, Material, Problem)
-from goferro.workloads import pto_material
+from goferro import (FIELD_ESTAT, FIELD_HSTAT, FIELD_M, FIELD_P, FIELD_Q, FLAG_DEMAG_FACTOR, FLAG_ELECTROSTATICS,
+                     FLAG_MAG_1D, FLAG_MAGNETOSTATICS, Context, GoFerroError, Material, Problem, workloads)
+from goferro.workloads import GYRO, pto_material
 
 pytestmark = pytest.mark.gpu
 
@@ -82,11 +82,48 @@ def test_ferroelectric_steps_with_electrostatics():
     o.close()
 
 
+def with_magnet(flags):
+    """The block with its non-polar layer made a ferromagnet."""
+    prob, p, q = block(flags=flags)
+    prob.materials[1] = Material.make(if_FM=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.05, Aex=3.7e-12,
+                                      r_permittivity=25.0)
+    prob.demag_fac = (-0.2, -0.3, -0.5)
+    m = workloads.tilted_m(prob.shape, 4, theta_max=0.4)
+    m[:, np.asarray(prob.material_cell) != 2] = 0.0
+    return prob, p, q, m
+
+
+@pytest.mark.parametrize("mode", [FLAG_DEMAG_FACTOR, FLAG_MAG_1D])
+def test_electrostatics_with_local_magnetostatics(mode):
+    """The demag-factor and 1D magnetostatics need no FFT plan and run
+    beside the FFT electrostatics."""
+    from oracle.oracle import Oracle
+    say("case electrostatics with local magnetostatics", hex(mode))
+    prob, p, q, m = with_magnet(FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS | mode)
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in ((FIELD_P, p), (FIELD_Q, q), (FIELD_M, m)):
+        g.upload(fid, v)
+        o.set(fid, v)
+    g.demag()
+    o.demag()
+    g.step(4)
+    o.step(4)
+    for fid in (FIELD_P, FIELD_Q, FIELD_ESTAT, FIELD_M, FIELD_HSTAT):
+        d = rel(g.download(fid), o.get(fid))
+        say("  field", fid, "rel", d)
+        assert d <= 1e-11
+    g.close()
+    o.close()
+
+
 def test_electrostatics_rejections():
     say("case electrostatics rejections")
-    prob, _, _ = block(flags=FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
+    prob, _, _, _ = with_magnet(FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
     with pytest.raises(GoFerroError, match="magnetostatics"):
         Context(prob)
+    # without magnets the magnetostatics flag sets nothing up
+    prob, _, _ = block(flags=FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
+    Context(prob).close()
     prob, _, _ = block(shape=(8, 8, 12))
     with pytest.raises(GoFerroError, match="power-of-two"):
         Context(prob)

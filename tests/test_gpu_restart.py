@@ -1,0 +1,225 @@
+"""Restart through the state image, and the reference's deck driven end to
+end, on the GPU through the C ABI.
+
+A run of n1 + n2 steps and a run of n1 steps, saved, loaded into a fresh
+context and continued for n2 steps must agree bit for bit (np.array_equal
+on every field, and t): the image has to carry every array the next step
+reads (stage stores, the elastic force store, the Liao and Mur histories,
+the fused-magnetostatics H spectrum, the drive-table cursor).  The cases
+cover each of those.  The deck run is checked against the oracle on the
+same deck (8 written digits) and its restart against the uninterrupted run
+(identical files).  Every case prints its name first.
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from goferro import (FIELD_E_CELL, FIELD_E_EM, FIELD_ESTAT, FIELD_H_CELL, FIELD_H_EM, FIELD_HSTAT, FIELD_JP,
+                     FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_M_STORE, FIELD_P, FIELD_Q, FIELD_STRAIN_DIAG,
+                     FIELD_STRAIN_OFF, FIELD_U, FIELD_V, FLAG_1D_ABC, FLAG_MAGNETOSTATICS, FLAG_PRESCRIBE_HEXT,
+                     Context, GoFerroError, workloads)
+from goferro import io as gio
+
+pytestmark = pytest.mark.gpu
+
+DECK = Path(__file__).resolve().parent / "golden" / "deck_fm"
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fields_of(ctx, ids):
+    return {f: ctx.download(f) for f in ids}
+
+
+def split_run(name, prob, fields, ids, n1, n2, statics=True):
+    """Continuous n1+n2 steps against n1, image, fresh context, n2."""
+    say("case", name, prob.shape, hex(prob.flags))
+    a = Context(prob)
+    for f, v in fields.items():
+        a.upload(f, v)
+    if statics:
+        a.demag()
+    a.step(n1 + n2)
+    want = fields_of(a, ids)
+    t_want = a.time
+    a.close()
+
+    b = Context(prob)
+    for f, v in fields.items():
+        b.upload(f, v)
+    if statics:
+        b.demag()
+    b.step(n1)
+    image = b.state_image()
+    b.close()
+    c = Context(prob)
+    c.restore_state(image)
+    c.step(n2)
+    got = fields_of(c, ids)
+    assert c.time == t_want
+    c.close()
+    for f in ids:
+        assert np.array_equal(got[f], want[f]), (name, f, np.abs(got[f] - want[f]).max())
+
+
+def test_restart_fm_fused_magnetostatics():
+    prob, m = workloads.fm_film(shape=(32, 32, 16))
+    prob.Hext_altr = (4e3, 0.0, 0.0)
+    prob.H_altr_freq = (7e9, 0.0, 0.0)
+    split_run("fm fused", prob, {FIELD_M: m}, (FIELD_M, FIELD_HSTAT), 7, 6)
+
+
+def test_restart_mixed_fm_with_drive_table():
+    from test_gpu_parity import mixed_fm_problem
+    prob, m = mixed_fm_problem()
+    prob.flags |= FLAG_PRESCRIBE_HEXT
+    prob.prescribe_Hext = np.array([[5e-14, 1e4, 0, 0], [1.5e-13, 0, 2e4, 0], [2.5e-13, 0, 0, 3e4]])
+    split_run("mixed fm + table", prob, {FIELD_M: m}, (FIELD_M, FIELD_HSTAT), 6, 9)
+
+
+def test_restart_uploaded_stage_store():
+    # a stage store that differs from m (uploaded) travels in the image
+    prob, m = workloads.fm_film(shape=(16, 16, 8))
+    ms = m.copy()
+    ms[0] += 1e-3
+    split_run("fm store", prob, {FIELD_M: m, FIELD_M_STORE: ms}, (FIELD_M,), 0, 3, statics=False)
+
+
+def test_restart_afm():
+    prob, m1, m2 = workloads.afm_block(shape=(16, 16, 8))
+    split_run("afm", prob, {FIELD_M_AFM1: m1, FIELD_M_AFM2: m2}, (FIELD_M_AFM1, FIELD_M_AFM2), 5, 5, statics=False)
+
+
+@pytest.mark.parametrize("mur", [False, True])
+def test_restart_magnetoelastic(mur):
+    from test_gpu_elastic import magnetoelastic, random_uv
+    prob, m = magnetoelastic(shape=(16, 16, 12))
+    if mur:
+        prob.flags |= FLAG_1D_ABC
+    u, v = random_uv(prob.shape, 6)
+    split_run("magnetoelastic" + (" mur" if mur else " liao"), prob, {FIELD_M: m, FIELD_U: u, FIELD_V: v},
+              (FIELD_M, FIELD_U, FIELD_V, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF), 6, 7, statics=False)
+
+
+def test_restart_ferroelectric_electrostatics():
+    from test_gpu_estatic import block
+    prob, p, q = block()
+    split_run("fe estatic", prob, {FIELD_P: p, FIELD_Q: q}, (FIELD_P, FIELD_Q, FIELD_ESTAT), 4, 5)
+
+
+def test_restart_em_walls_sources_drude():
+    from test_gpu_em import boxed_media, random_yee
+    prob = boxed_media(jf_type=6)
+    E = random_yee(prob.shape, FIELD_E_EM, 1e6, 4)
+    H = random_yee(prob.shape, FIELD_H_EM, 3e3, 5)
+    split_run("em boxed", prob, {FIELD_E_EM: E, FIELD_H_EM: H},
+              (FIELD_E_EM, FIELD_H_EM, FIELD_JP, FIELD_E_CELL, FIELD_H_CELL), 5, 6, statics=False)
+
+
+def test_restart_fe_em_column():
+    from test_gpu_em import fe_film
+    prob, p, q = fe_film(flags=FLAG_1D_ABC)
+    split_run("fe em", prob, {FIELD_P: p, FIELD_Q: q}, (FIELD_P, FIELD_Q, FIELD_E_EM, FIELD_H_EM), 4, 4,
+              statics=False)
+
+
+def test_state_file_and_rejections(tmp_path):
+    say("case state file and rejections")
+    prob, m = workloads.fm_film(shape=(16, 16, 8))
+    a = Context(prob)
+    a.upload(FIELD_M, m)
+    a.demag()
+    a.step(3)
+    path = tmp_path / "state.gfs"
+    a.save_state(path)
+    info = gio.state_info(path)          # pure-Python checksum agrees with the library's
+    assert info["time"] == a.time and (info["nx"], info["ny"], info["nz"]) == (16, 16, 8)
+    assert (1, 0, 16 * 16 * 8) in info["arrays"]
+    b = Context(prob)
+    b.load_state(path)
+    assert np.array_equal(b.download(FIELD_M), a.download(FIELD_M)) and b.time == a.time
+    # another grid
+    other, m2 = workloads.fm_film(shape=(16, 16, 4))
+    c = Context(other)
+    with pytest.raises(GoFerroError, match="another grid"):
+        c.load_state(path)
+    c.close()
+    # a flipped byte
+    raw = bytearray(path.read_bytes())
+    raw[len(raw) // 2] ^= 1
+    with pytest.raises(GoFerroError, match="checksum"):
+        b.restore_state(bytes(raw))
+    with pytest.raises(GoFerroError, match="truncated|checksum"):
+        b.restore_state(bytes(raw[:100]))
+    # only at a step boundary
+    b.time_advance()
+    b.mag_stage(1)
+    with pytest.raises(GoFerroError, match="step boundary"):
+        b.state_image()
+    a.close()
+    b.close()
+
+
+def test_material_update_travels_in_the_image():
+    say("case material in image")
+    prob, m = workloads.fm_film(shape=(16, 16, 8))
+    a = Context(prob)
+    a.upload(FIELD_M, m)
+    mat = prob.materials[0]
+    changed = type(mat).from_buffer_copy(mat)
+    changed.FM_damping = 0.2
+    a.set_material(1, changed)
+    image = a.state_image()
+    a.step(4)
+    fresh = Context(workloads.fm_film(shape=(16, 16, 8))[0])
+    fresh.restore_state(image)
+    assert fresh.problem.materials[0].FM_damping == 0.2
+    fresh.step(4)
+    assert np.array_equal(fresh.download(FIELD_M), a.download(FIELD_M))
+    a.close()
+    fresh.close()
+
+
+def test_deck_run_matches_oracle(tmp_path):
+    """The golden deck through io.run against the oracle on the same deck."""
+    from oracle.oracle import Oracle
+    say("case deck run")
+    ctx = gio.run(DECK, out_dir=tmp_path)
+    deck = gio.read_deck(DECK)
+    o = Oracle(deck.problem)
+    o.set(FIELD_M, gio.read_cells(DECK / "m.in", deck.problem.shape, 3))
+    o.demag()
+    got10 = gio.read_cells(tmp_path / "m.10.dat", deck.problem.shape, 3)
+    o.step(10)
+    want10 = o.get(FIELD_M)
+    o.step(10)
+    want20 = o.get(FIELD_M)
+    got20 = gio.read_cells(tmp_path / "m.20.dat", deck.problem.shape, 3)
+    # 8 significant digits written; the GPU state agrees with the oracle to 1e-12
+    for got, want in ((got10, want10), (got20, want20)):
+        assert np.abs(got - want).max() <= 5e-8 * np.abs(want).max() + 1e-12
+    np.testing.assert_allclose(ctx.download(FIELD_M), want20, rtol=0, atol=1e-12)
+    ave = np.loadtxt(tmp_path / "average_m.dat")
+    assert ave.shape == (5, 4) and list(ave[:, 0]) == [0, 5, 10, 15, 20]
+    np.testing.assert_allclose(ave[-1, 1:], want20.sum(axis=(1, 2, 3)) / (8 * 8 * 3), rtol=0, atol=1e-7)
+    assert (tmp_path / "Hstat.0.dat").exists() and (tmp_path / "Hstat.20.dat").exists()
+    mag = np.loadtxt(tmp_path / "m_magcell.20.dat")
+    assert mag.shape == (8 * 8 * 3, 6) and mag[:, 2].max() == 3
+    ctx.close()
+    o.close()
+
+
+def test_deck_restart_reproduces_the_run(tmp_path):
+    say("case deck restart")
+    full, part = tmp_path / "full", tmp_path / "part"
+    gio.run(DECK, out_dir=full).close()
+    ck = tmp_path / "ck.gfs"
+    gio.run(DECK, steps=10, out_dir=part, checkpoint=str(ck)).close()
+    gio.run(DECK, out_dir=part, restart=str(ck)).close()
+    for name in ("m.20.dat", "m_magcell.20.dat", "Hstat.20.dat"):
+        assert (part / name).read_bytes() == (full / name).read_bytes(), name
+    assert (part / "average_m.dat").read_text() == (full / "average_m.dat").read_text()
