@@ -300,6 +300,113 @@ gf_status mag_statics(gf_ctx* c) {
     return demag_run(c, fm, s1, s2);
 }
 
+// Spin-pumping planes (global.cpp:169-235) and their coefficients
+// (magnetic_solver.cpp:245-259).  The layers are walked bottom to top from
+// k = 0 (gf_desc.nz_layer, else the runs of equal ids along z of column
+// (0, 0)); a layer whose material at column (0, 0) pumps, with a magnet on
+// exactly one side, carries J_ISHE on each plane k, with the source plane
+// next to it and the layer thickness t: coefficient theta_SH e / (2 pi)
+// g_mix times +-sinh((t - d) / l_sd) / sinh(t / l_sd), d = (|k - sz| -
+// 1/2) dz, + when the source lies below.  The magnet stages then also write
+// their increments, which the current reads.
+gf_status spin_setup(gf_ctx* c, const gf_desc* d) {
+    const int64_t nz = c->nz;
+    std::vector<int64_t> lay;
+    if (d->n_layers > 0) {
+        if (!d->nz_layer) return set_error(c, GF_ERR_ARG, "nz_layer is null");
+        int64_t tot = 0;
+        for (int i = 0; i < d->n_layers; i++) {
+            if (d->nz_layer[i] < 1) return set_error(c, GF_ERR_ARG, "nz_layer entries must be positive");
+            tot += d->nz_layer[i];
+            lay.push_back(d->nz_layer[i]);
+        }
+        if (tot > nz) return set_error(c, GF_ERR_ARG, "the layers hold more planes than nz");
+    } else {
+        for (int64_t k = 0; k < nz;) {
+            int64_t e = k + 1;
+            while (e < nz && c->host_code[e] == c->host_code[k]) e++;
+            lay.push_back(e - k);
+            k = e;
+        }
+    }
+    auto id_of = [&](int64_t k) { return c->host_code[k] & kMatMask; };   // column (0, 0)
+    auto magnet = [&](int64_t k) {
+        const uint32_t t = id_of(k);
+        return t && (c->host_mats[t - 1].if_FM || c->host_mats[t - 1].if_AFM);
+    };
+    std::vector<int32_t> src(static_cast<size_t>(nz), -1);
+    std::vector<double> thick(static_cast<size_t>(nz), 0.), coef(static_cast<size_t>(nz), 0.);
+    int64_t zf = -1;
+    for (int64_t len : lay) {
+        const int64_t zi = zf + 1;
+        zf = zi + len - 1;
+        const uint32_t t = id_of(zi);
+        if (!t || !c->host_mats[t - 1].if_spin_pump) continue;
+        const bool si = zi != 0 && magnet(zi - 1);
+        const bool sf = zf != nz - 1 && magnet(zf + 1);
+        int64_t sz = -1;
+        if (si && !sf) sz = zi - 1;
+        else if (!si && sf) sz = zf + 1;
+        if (sz < 0) continue;
+        for (int64_t k = zi; k <= zf; k++) {
+            src[static_cast<size_t>(k)] = static_cast<int32_t>(sz);
+            thick[static_cast<size_t>(k)] = static_cast<double>(len) * c->dz;
+        }
+    }
+    for (int64_t k = 0; k < nz; k++) {
+        const size_t kk = static_cast<size_t>(k);
+        if (src[kk] < 0 || !id_of(k)) { src[kk] = -1; continue; }
+        const gf_material& m = c->host_mats[id_of(k) - 1];
+        const double cjs = m.spin_hall_angle * kUnitE / 2. / kPi * m.spin_mix_cond;
+        const double dist = (std::fabs(static_cast<double>(k - src[kk])) - 0.5) * c->dz;
+        double cl = std::sinh((thick[kk] - dist) / m.spin_diffus_length) / std::sinh(thick[kk] / m.spin_diffus_length);
+        if (!(k > src[kk])) cl = -1. * cl;
+        coef[kk] = cjs * cl;
+    }
+    c->spin_on = true;
+    gf_status st = alloc3(c, c->jis, "hipMalloc J_ISHE");
+    for (int a = 0; a < 3 && !st; a++)
+        st = hip_check(c, hipMemset(c->jis[a], 0, sizeof(double) * c->n), "clear J_ISHE");
+    if (!st) st = hip_check(c, hipMalloc(&c->spin_src, sizeof(int32_t) * src.size()), "hipMalloc spin planes");
+    if (!st) st = hip_check(c, hipMalloc(&c->spin_coef, sizeof(double) * coef.size()), "hipMalloc spin planes");
+    if (!st)
+        st = hip_check(c, hipMemcpy(c->spin_src, src.data(), sizeof(int32_t) * src.size(), hipMemcpyHostToDevice),
+                       "copy spin planes");
+    if (!st)
+        st = hip_check(c, hipMemcpy(c->spin_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice),
+                       "copy spin planes");
+    if (!st && c->any_fm && !c->em_dm[0]) st = alloc3(c, c->em_dm, "hipMalloc dm");
+    if (!st && c->any_afm && !c->em_dm1[0]) st = alloc3(c, c->em_dm1, "hipMalloc dm");
+    if (!st && c->any_afm && !c->em_dm2[0]) st = alloc3(c, c->em_dm2, "hipMalloc dm");
+    return st;
+}
+
+// J_ISHE of the stage that just ran its magnet stages (get_J_ISHE_RK*).
+gf_status spin_pump(gf_ctx* c) {
+    SpinArgs a{};
+    double** fs = c->any_fm ? fm_store(c, c->mag_in_sel) : nullptr;
+    double** s1 = c->any_afm ? afm1_store(c, c->afm_in_sel) : nullptr;
+    double** s2 = c->any_afm ? afm2_store(c, c->afm_in_sel) : nullptr;
+    for (int i = 0; i < 3; i++) {
+        a.fs[i] = fs ? fs[i] : nullptr;
+        a.fd[i] = fs ? c->em_dm[i] : nullptr;
+        a.s1[i] = s1 ? s1[i] : nullptr;
+        a.d1[i] = s1 ? c->em_dm1[i] : nullptr;
+        a.s2[i] = s2 ? s2[i] : nullptr;
+        a.d2[i] = s2 ? c->em_dm2[i] : nullptr;
+    }
+    a.src = c->spin_src;
+    a.coef = c->spin_coef;
+    a.j[0] = c->jis[0];
+    a.j[1] = c->jis[1];
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dt = c->dt;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_OTHER, &ev);
+    gf_status st = hip_check(c, launch_spin_pump(a, c->stream), "spin pumping");
+    timer_end(c, GF_K_OTHER, ev);
+    return st;
+}
+
 // statics after an update phase (main.cpp:299-306 and the 1D field inside
 // update_m*, magnetic_solver.cpp:1533-1538, magnetic_AFM_solver.cpp:1746-1751)
 gf_status statics_after_update(gf_ctx* c) {
@@ -745,6 +852,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         if (!st) st = alloc3(c, c->e_t0 + 3, "hipMalloc static strain");
     }
     if (!st && (c->flags & GF_FLAG_EMDYNAMIC)) st = em_setup(c, d);
+    if (!st && (c->flags & GF_FLAG_SPIN_PUMPING)) st = spin_setup(c, d);
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
         st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
@@ -800,6 +908,9 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->Hb); free3(c->Ha); free3(c->Hbb); free3(c->Hacc);
     free3(c->Ecell); free3(c->Hcell); free3(c->Jf);
     free3(c->em_dm); free3(c->em_dm1); free3(c->em_dm2);
+    free3(c->jis);
+    if (c->spin_src) hipFree(c->spin_src);
+    if (c->spin_coef) hipFree(c->spin_coef);
     for (int q = 0; q < 9; q++) {
         if (c->Jp[q]) hipFree(c->Jp[q]);
         if (c->Jps[q]) hipFree(c->Jps[q]);
@@ -884,6 +995,9 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     case GF_FIELD_STRAIN_T0_OFF:
         if (!c->e_t0[0]) break;
         return c->e_t0 + 3;
+    case GF_FIELD_J_ISHE:
+        if (!c->spin_on || for_write) break;
+        return c->jis;
     case GF_FIELD_E_CELL:
         if (!c->em_on || for_write) break;
         return c->Ecell;
@@ -1200,8 +1314,12 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
     }
     timer_end(c, GF_K_MAG_STAGE, ev);
     if (st) return st;
+    c->mag_in_sel = c->mag_store_sel;
     c->mag_store_sel = out_sel;
     c->mag_next_stage = stage == 4 ? 1 : stage + 1;
+    // get_J_ISHE_RK* at the end of get_dm_RK* (magnetic_solver.cpp:226-228);
+    // with an AFM it is recomputed after the AFM stage, whose value stands
+    if (c->spin_on && !c->any_afm) return spin_pump(c);
     return GF_OK;
 }
 
@@ -1227,8 +1345,11 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
     gf_status st = hip_check(c, launch_afm_stage(a, c->n, c->stream), "afm stage");
     timer_end(c, GF_K_AFM_STAGE, ev);
     if (st) return st;
+    c->afm_in_sel = c->afm_store_sel;
     c->afm_store_sel = out_sel;
     c->afm_next_stage = stage == 4 ? 1 : stage + 1;
+    // magnetic_AFM_solver.cpp:426-428: after the FM stage of the same stage
+    if (c->spin_on) return spin_pump(c);
     return GF_OK;
 }
 
@@ -1486,8 +1607,13 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
         a.hb[i] = c->Hb[i]; a.ho[i] = ho[i]; a.hacc[i] = c->Hacc[i];
         a.jf[i] = (c->flags & GF_FLAG_JF_INPUT) ? c->Jf[i] : nullptr;
         a.q[i] = ((c->flags & GF_FLAG_EM_FROM_P) && c->any_fe) ? c->fe_q_used[i] : nullptr;
-        a.dm[i] = c->em_dm[i]; a.dma1[i] = c->em_dm1[i]; a.dma2[i] = c->em_dm2[i];
+        const bool from_m = c->flags & GF_FLAG_EM_FROM_M;
+        a.dm[i] = from_m ? c->em_dm[i] : nullptr;
+        a.dma1[i] = from_m ? c->em_dm1[i] : nullptr;
+        a.dma2[i] = from_m ? c->em_dm2[i] : nullptr;
     }
+    a.jishe[0] = c->spin_on ? c->jis[0] : nullptr;
+    a.jishe[1] = c->spin_on ? c->jis[1] : nullptr;
     for (int q = 0; q < 9; q++) a.jps[q] = c->jp_on ? c->Jps[q] : nullptr;
     a.nx = nx; a.ny = ny; a.nz = nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
     for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
@@ -1802,6 +1928,7 @@ std::vector<StateArray> state_arrays(gf_ctx* c, const StateSel& sel, uint32_t bi
     if (bits & kBitHasHnu) add(30, c->hnu, 3, n);
     add(31, c->hdrive, 3, n);
     add(32, c->e_t0, 6, n);
+    add(36, c->jis, 2, n);
     const DemagPlan& d = c->demag;
     if (d.ready) {
         const int64_t spec = 2 * 3 * c->nx * c->ny * d.nzh;

@@ -44,10 +44,10 @@ __device__ __forceinline__ void rk_state(int stage, int64_t i, double k, double*
 }
 
 // Four-cell means around an edge and the Ampere increment
-// (EMdynamic_solver.cpp:392-499); the spin-pumping current is zero.
+// (EMdynamic_solver.cpp:392-499), with the spin-pumping current on x and y.
 __device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, const int64_t cell[4], double curl,
                                                  double e_store) {
-    double Jf = 0., Jp = 0., dP = 0., er = 0., cond = 0., pc = 0.;
+    double Jf = 0., Jp = 0., Ji = 0., dP = 0., er = 0., cond = 0., pc = 0.;
     for (int q = 0; q < 4; q++) {
         const int64_t id = cell[q];
         const uint32_t t = a.code[id] & kMatMask;
@@ -62,14 +62,16 @@ __device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, cons
         }
         if (a.jf[0]) Jf = Jf + a.jf[comp][id];
         if (a.jps[0]) Jp = Jp + a.jps[comp][id] + a.jps[3 + comp][id] + a.jps[6 + comp][id];
+        if (a.jishe[0] && comp < 2) Ji = Ji + a.jishe[comp][id];
     }
     if (pc < 0.5) pc = 1.;
     Jf = Jf / 4.;
     Jp = Jp / 4.;
+    Ji = Ji / 4.;
     dP = dP / pc;
     er = er / 4.;
     cond = cond / 4.;
-    return a.dt / kEps0 / er * (curl - Jf - Jp - cond * e_store - dP / a.dt);
+    return a.dt / kEps0 / er * (curl - Jf - Jp - Ji - cond * e_store - dP / a.dt);
 }
 
 __global__ void __launch_bounds__(256) em_e_kernel(EmArgs a) {
@@ -380,6 +382,36 @@ hipError_t launch_em_h(const EmArgs& a, hipStream_t s) {
 
 hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(em_jp_kernel, dim3(blocks_for(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// One thread per cell; cells of planes without a source keep their zero.
+// The sums run FM, AFM1, AFM2 as the reference writes them.
+__global__ void __launch_bounds__(256) spin_pump_kernel(SpinArgs a) {
+    const int64_t n = a.nx * a.ny * a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const int64_t k = id % a.nz;
+    const int32_t sz = a.src[k];
+    if (sz < 0) return;
+    const int64_t is = id - k + sz;
+    double sx = 0., sy = 0.;
+    if (a.fs[0]) {
+        sx = a.fs[0][is] * a.fd[2][is] - a.fs[2][is] * a.fd[0][is];
+        sy = a.fs[1][is] * a.fd[2][is] - a.fs[2][is] * a.fd[1][is];
+    }
+    if (a.s1[0]) {
+        sx = sx + a.s1[0][is] * a.d1[2][is] - a.s1[2][is] * a.d1[0][is] + a.s2[0][is] * a.d2[2][is] -
+             a.s2[2][is] * a.d2[0][is];
+        sy = sy + a.s1[1][is] * a.d1[2][is] - a.s1[2][is] * a.d1[1][is] + a.s2[1][is] * a.d2[2][is] -
+             a.s2[2][is] * a.d2[1][is];
+    }
+    a.j[0][id] = a.coef[k] * sx / a.dt;
+    a.j[1][id] = a.coef[k] * sy / a.dt;
+}
+
+hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(spin_pump_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

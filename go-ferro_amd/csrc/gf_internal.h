@@ -23,6 +23,7 @@ namespace gf {
 
 constexpr double kMu0 = 1.256637061e-6;               // magnetic_system.h:14
 constexpr double kPi = 3.14159265358979323846;        // magnetic_system.h:16
+constexpr double kUnitE = 1.602176634e-19;          // magnetic_system.h:15
 
 // cell code bits
 constexpr uint32_t kMatMask = 0xFFu;   // material id, up to 255 materials
@@ -110,10 +111,27 @@ struct EmArgs {
     const double* dm[3];        // magnetic increments of the stage, null when absent
     const double* dma1[3];
     const double* dma2[3];
+    const double* jishe[2];     // inverse spin Hall current Jx, Jy, null when off
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
     int32_t per[3];
     int32_t stage;
+};
+
+// get_J_ISHE_RK* (magnetic_solver.cpp:231-283): per pumping plane k, the
+// current from the stage store and increment of its source plane src[k].
+struct SpinArgs {
+    const double* fs[3];        // FM store and increment of the stage, null when no FM
+    const double* fd[3];
+    const double* s1[3];        // AFM sublattices, null when no AFM
+    const double* d1[3];
+    const double* s2[3];
+    const double* d2[3];
+    const int32_t* src;         // source plane per plane, -1 = none
+    const double* coef;         // theta_SH e / (2 pi) g_mix * profile, per plane
+    double* j[2];
+    int64_t nx, ny, nz;
+    double dt;
 };
 
 struct EmJpArgs {
@@ -507,6 +525,13 @@ struct gf_ctx {
     int jf_type = 0, jf_comp = -1, jf_cycles = 0;
     int jf_lo[3] = {}, jf_hi[3] = {};
     double jf_amp = 0, jf_freq = 0, jf_sigma = 0;
+    // spin pumping (GF_FLAG_SPIN_PUMPING): J_ISHE (x, y, zero z), the
+    // per-plane source and coefficient, the stores the last magnet stages read
+    bool spin_on = false;
+    double* jis[3] = {};
+    int32_t* spin_src = nullptr;
+    double* spin_coef = nullptr;
+    int mag_in_sel = 0, afm_in_sel = 0;
     // statics
     double* hstat[3] = {};
     bool hstat_written = false;     // Hstat uploaded (it is zero otherwise unless statics run)
@@ -585,6 +610,7 @@ hipError_t launch_em_e(const EmArgs& a, hipStream_t s);
 hipError_t launch_em_h(const EmArgs& a, hipStream_t s);
 hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s);
 hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s);
+hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s);
 hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s);
 hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s);
 hipError_t launch_em_plane(const EmMiscArgs& a, hipStream_t s);

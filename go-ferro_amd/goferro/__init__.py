@@ -29,7 +29,8 @@ __all__ = [
     "FLAG_EM_FROM_P", "FLAG_EM_FROM_M", "FLAG_EM_BACKACTION_P", "FLAG_EM_BACKACTION_M", "FLAG_PLANE_EM",
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
-    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG", "FIELD_STRAIN_T0_OFF",
+    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FLAG_SPIN_PUMPING", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG",
+    "FIELD_STRAIN_T0_OFF", "FIELD_J_ISHE",
     "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
 ]
@@ -37,7 +38,7 @@ __all__ = [
 LIB_PATH = Path(__file__).resolve().parent / "libgoferro.so"
 MU0 = 1.256637061e-6  # magnetic_system.h:14
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 FLAG_MAGNETOSTATICS = 1 << 0
 FLAG_MAG_1D = 1 << 1
 FLAG_DEMAG_FACTOR = 1 << 2
@@ -63,11 +64,12 @@ FLAG_PEC_Z = 1 << 21
 FLAG_1D_ABC_EM_ONLYTOP = 1 << 22
 FLAG_HDRIVE = 1 << 23
 FLAG_ELECTROSTATICS = 1 << 24
+FLAG_SPIN_PUMPING = 1 << 25
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
  FIELD_E_EM_STORE, FIELD_H_EM_STORE, FIELD_E_CELL, FIELD_H_CELL, FIELD_JP, FIELD_HDRIVE, FIELD_STRAIN_T0_DIAG,
- FIELD_STRAIN_T0_OFF) = range(25)
+ FIELD_STRAIN_T0_OFF, FIELD_J_ISHE) = range(26)
 
 
 def yee_shapes(shape, field):
@@ -161,6 +163,9 @@ class Material(ctypes.Structure):
         ("comp_n1", ctypes.c_double), ("omega_plasma_n1", ctypes.c_double), ("tao_e_n1", ctypes.c_double),
         ("comp_n2", ctypes.c_double), ("omega_plasma_n2", ctypes.c_double), ("tao_e_n2", ctypes.c_double),
         ("comp_n3", ctypes.c_double), ("omega_plasma_n3", ctypes.c_double), ("tao_e_n3", ctypes.c_double),
+        ("if_spin_pump", ctypes.c_int32), ("reserved3", ctypes.c_int32),
+        ("spin_hall_angle", ctypes.c_double), ("spin_mix_cond", ctypes.c_double),
+        ("spin_diffus_length", ctypes.c_double),
     ]
 
     @classmethod
@@ -206,6 +211,8 @@ class _Desc(ctypes.Structure):
         ("Jf_input_amp", ctypes.c_double), ("Jf_input_freq", ctypes.c_double), ("Jf_input_sigma", ctypes.c_double),
         ("Hdrive_freq", ctypes.c_double),
         ("scale_elec", ctypes.c_double),
+        ("n_layers", ctypes.c_int32), ("reserved3", ctypes.c_int32),
+        ("nz_layer", ctypes.POINTER(ctypes.c_int32)),
     ]
 
 
@@ -259,6 +266,7 @@ class Problem:
     Jf_input_sigma: float = 0.0
     Hdrive_freq: float = 0.0
     scale_elec: float = 1.0
+    nz_layer: Optional[Sequence[int]] = None       # layer stack bottom to top; None: runs of column (0, 0)
 
     @property
     def n(self) -> int:
@@ -313,6 +321,11 @@ class Problem:
         d.Hdrive_freq = float(self.Hdrive_freq)
         d.scale_elec = float(self.scale_elec)
         keep = [cell, mats]
+        if self.nz_layer is not None:
+            lay = np.ascontiguousarray(self.nz_layer, dtype=np.int32)
+            d.n_layers = lay.size
+            d.nz_layer = lay.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            keep.append(lay)
         if self.prescribe_Hext is not None:
             pre = np.ascontiguousarray(self.prescribe_Hext, dtype=np.float64).reshape(-1, 4)
             d.n_prescribe_Hext = pre.shape[0]

@@ -39,7 +39,7 @@ from . import (
     FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS, FLAG_ELECTROSTATICS, FLAG_EM_BACKACTION_M, FLAG_EM_BACKACTION_P,
     FLAG_EM_FROM_M, FLAG_EM_FROM_P, FLAG_EMDYNAMIC, FLAG_GAUSSIAN_PULSE, FLAG_HEXT_NONUNIF, FLAG_JF_INPUT,
     FLAG_MAG_1D, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE, FLAG_PEC_XY, FLAG_PEC_Z, FLAG_PLANE_EM,
-    FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT,
+    FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FIELD_J_ISHE,
     Context, GoFerroError, Material, Problem, load_library, split_yee, yee_shapes,
 )
 
@@ -465,8 +465,6 @@ def _unsupported(g: dict, mats: list) -> list[str]:
     out = []
     if g["if_flexo"]:
         out.append("if_flexo (flexoelectric coupling)")
-    if g["if_spin_pumping"]:
-        out.append("if_spin_pumping (spin pumping and the inverse spin Hall current)")
     if g["if_prescribed_m"]:
         out.append("if_prescribed_m (prescribed precession)")
     if g["if_elastostatic"] or g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
@@ -516,6 +514,7 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
         ("if_EM_backaction_on_P", FLAG_EM_BACKACTION_P), ("if_EM_backaction_on_M", FLAG_EM_BACKACTION_M),
         ("if_input_planeEM_E", FLAG_PLANE_EM), ("if_Jf_input", FLAG_JF_INPUT), ("if_PEC_XY", FLAG_PEC_XY),
         ("if_PEC_Z", FLAG_PEC_Z), ("if_1D_ABC_EM_onlytop", FLAG_1D_ABC_EM_ONLYTOP),
+        ("if_spin_pumping", FLAG_SPIN_PUMPING),
     ]
     flags = 0
     for key, bit in flag_of:
@@ -547,6 +546,7 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
         Jfin_lo=(g["Jfin_xi"], g["Jfin_yi"], g["Jfin_zi"]), Jfin_hi=(g["Jfin_xf"], g["Jfin_yf"], g["Jfin_zf"]),
         Jf_input_amp=g["Jf_input_amp"], Jf_input_freq=g["Jf_input_freq"], Jf_input_sigma=g["Jf_input_sigma"],
         scale_elec=g["scale_elec"],
+        nz_layer=None if g["if_read_struct"] else list(nzl),
     )
     if g["if_prescribe_Hext"]:
         prob.prescribe_Hext = _table(d / "Hext.in", g["num_prescribe_Hext"])
@@ -555,8 +555,9 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
     deck = Deck(problem=prob, settings=g, materials=raw, directory=d, steps=g["step"])
     if g["if_read_Hext_nonunif"]:
         deck.fields[FIELD_HEXT_NONUNIF] = read_cells(d / "Hext_nonunif.in", (nx, ny, nz), 3)
-    for key, what in (("if_output_Jishe", "J_ISHE (spin pumping is not implemented)"),
-                      ("if_output_elastoforce", "elastoforce (the force store is in the state image)"),
+    if g["if_output_Jishe"] and not g["if_spin_pumping"]:
+        deck.skipped_outputs.append("J_ISHE without spin pumping")
+    for key, what in (("if_output_elastoforce", "elastoforce (the force store is in the state image)"),
                       ("if_output_eigenstraint0_crt", "eigenstraint0_crt (no eigenstrain output)")):
         if g[key]:
             deck.skipped_outputs.append(what)
@@ -733,7 +734,8 @@ class Outputs:
                 ("if_output_em", "output_step_em"), ("if_output_em_onecell", "output_step_em_onecell"),
                 ("if_output_emYee", "output_step_emYee"), ("if_output_Estat", "output_step_Estat"),
                 ("if_output_Hstat", "output_step_Hstat"), ("if_output_strain", "output_step_strain"),
-                ("if_output_uandv", "output_step_uandv"), ("if_output_Jp", "output_step_Jp")]
+                ("if_output_uandv", "output_step_uandv"), ("if_output_Jp", "output_step_Jp"),
+                ("if_output_Jishe", "output_step_Jishe")]
 
     def _cells(self, name: str, arrays) -> None:
         write_cells(self.dir / name, arrays)
@@ -811,6 +813,8 @@ class Outputs:
             jp = ctx.download(FIELD_JP)
             for p in range(3):
                 self._cells(f"Jp{p + 1}.{nstep}.dat", jp[p])
+        if g["if_spin_pumping"] and self._due("if_output_Jishe", "output_step_Jishe", nstep):
+            self._cells(f"J_ISHE.{nstep}.dat", ctx.download(FIELD_J_ISHE))   # Jx, Jy, 0 (inoutput.cpp:846-864)
 
     def _magcell(self, ctx: Context, nstep: int) -> None:
         """output_magcell (inoutput.cpp:349-366): FM cells only, in index order."""

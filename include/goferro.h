@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GF_ABI_VERSION 1
+#define GF_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum gf_status_code {
@@ -79,6 +79,14 @@ typedef int gf_status;
  * the FFT magnetostatics (the two would share one FFT plan); the 1D and
  * demag-factor magnetostatics combine with it. */
 #define GF_FLAG_ELECTROSTATICS   (1u << 24)
+/* if_spin_pumping: the inverse spin Hall current J_ISHE of the normal-metal
+ * layers next to a magnet (get_J_ISHE_RK*, magnetic_solver.cpp:231-283 and
+ * its per-stage copies; set-up global.cpp:169-235), recomputed after the
+ * magnet stages of every RK stage and added to Ampere's law of the x and y
+ * edges as a four-cell mean (EMdynamic_solver.cpp:422-482, 554-614 and the
+ * later stages).  The layers come from gf_desc.nz_layer; GF_FIELD_J_ISHE
+ * downloads it. */
+#define GF_FLAG_SPIN_PUMPING     (1u << 25)
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -126,6 +134,10 @@ typedef struct gf_material {
     double comp_n1, omega_plasma_n1, tao_e_n1;
     double comp_n2, omega_plasma_n2, tao_e_n2;
     double comp_n3, omega_plasma_n3, tao_e_n3;
+    /* spin pumping into a normal metal (material.h, read.cpp:170-171) */
+    int32_t if_spin_pump;
+    int32_t reserved3;
+    double spin_hall_angle, spin_mix_cond, spin_diffus_length;
 } gf_material;
 
 /*
@@ -183,6 +195,13 @@ typedef struct gf_desc {
     double Hdrive_freq;             /* Hz */
     /* permittivity scale of the electrostatics (read.cpp:56); 0 = 1 */
     double scale_elec;
+    /* the layer stack, bottom to top from k = 0 (pt_nz_layer after
+     * set_geometry, geometry.cpp:3-26): the spin-pumping set-up walks it
+     * (global.cpp:180-235).  n_layers = 0 takes the runs of equal material
+     * ids along z of column (0, 0) as the layers. */
+    int32_t n_layers;
+    int32_t reserved3;
+    const int32_t* nz_layer;
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */
@@ -221,6 +240,9 @@ enum gf_field_id {
      * zero until uploaded. */
     GF_FIELD_STRAIN_T0_DIAG = 23,
     GF_FIELD_STRAIN_T0_OFF = 24,
+    /* inverse spin Hall current Jx, Jy (and a zero z), 3n, read only
+     * (GF_FLAG_SPIN_PUMPING; output_Jishe, inoutput.cpp:846-864) */
+    GF_FIELD_J_ISHE = 25,
     GF_FIELD_COUNT
 };
 

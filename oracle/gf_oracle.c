@@ -29,6 +29,7 @@
 
 static const double MU0 = 1.256637061e-6;          /* magnetic_system.h:14 */
 static const double PI = 3.14159265358979323846;   /* magnetic_system.h:16 */
+static const double UNIT_E = 1.602176634e-19;      /* magnetic_system.h:15 */
 
 struct gfo_ctx {
     int64_t nx, ny, nz, n;
@@ -87,6 +88,11 @@ struct gfo_ctx {
     gf_desc desc;                              /* scalars of the descriptor */
     double cf_top_half, cf_bot_half, cf_x_half, cf_y_half, cf_top, cf_bot, cf_x, cf_y;
     double *kspec[9];   /* complex spectra, interleaved, nx*ny*(nz/2+1) each */
+    /* spin pumping (GF_FLAG_SPIN_PUMPING, global.cpp:169-235) */
+    int spin_on;
+    int64_t* spin_src;  /* source_z per plane, -1 = no J_ISHE */
+    double* spin_coef;  /* coeff_Js * coeff_L per plane */
+    double* Jis[3];     /* Jx_ISHE, Jy_ISHE, zero */
 };
 
 static void elastic_gradients(gfo_ctx* c);
@@ -430,6 +436,112 @@ static void build_aij(gfo_ctx* c, int (*pred)(const gfo_ctx*, int64_t), const un
 }
 
 /* ----------------------------------------------------------- create */
+/* The spin-pumping planes (global.cpp:169-235): the layers bottom to top
+ * (d->nz_layer, else the runs of equal ids along z of column (0, 0)); a
+ * layer whose material (at column (0, 0)) pumps, with a magnet (FM or AFM)
+ * on exactly one side, carries J_ISHE on each of its planes k, with the
+ * source plane sz next to it and the layer thickness t.  Per plane, the
+ * coefficient of get_J_ISHE_RK1 (magnetic_solver.cpp:245-259):
+ * theta_SH e / (2 pi) g_mix times +-sinh((t - d) / l_sd) / sinh(t / l_sd),
+ * d = (|k - sz| - 1/2) dz, + when the source lies below. */
+static int spin_setup(gfo_ctx* c, const gf_desc* d) {
+    int64_t nz = c->nz;
+    c->spin_on = 1;
+    c->spin_src = (int64_t*)malloc(sizeof(int64_t) * (size_t)nz);
+    c->spin_coef = (double*)calloc((size_t)nz, sizeof(double));
+    double* thick = (double*)calloc((size_t)nz, sizeof(double));
+    for (int64_t k = 0; k < nz; k++) c->spin_src[k] = -1;
+    for (int a = 0; a < 3; a++) c->Jis[a] = dalloc(c->n);
+    int64_t nl = 0;
+    int64_t* lay = (int64_t*)malloc(sizeof(int64_t) * (size_t)nz);
+    if (d->n_layers > 0) {
+        int64_t tot = 0;
+        for (int i = 0; i < d->n_layers; i++) {
+            if (d->nz_layer[i] < 1 || nl >= nz) { free(lay); free(thick); return 1; }
+            lay[nl++] = d->nz_layer[i];
+            tot += d->nz_layer[i];
+        }
+        if (tot > nz) { free(lay); free(thick); return 1; }
+    } else {
+        for (int64_t k = 0; k < nz;) {
+            int64_t e = k + 1;
+            while (e < nz && c->cell[e] == c->cell[k]) e++;
+            lay[nl++] = e - k;
+            k = e;
+        }
+    }
+    int64_t zf = -1;
+    for (int64_t l = 0; l < nl; l++) {
+        int64_t zi = zf + 1;
+        zf = zi + lay[l] - 1;
+        uint32_t t = c->cell[zi];          /* column (0, 0): index k */
+        if (!t || !c->mat[t - 1].if_spin_pump) continue;
+        int src_i = 0, src_f = 0;
+        if (zi != 0) {
+            uint32_t ti = c->cell[zi - 1];
+            src_i = ti && (c->mat[ti - 1].if_FM || c->mat[ti - 1].if_AFM);
+        }
+        if (zf != nz - 1) {
+            uint32_t tf = c->cell[zf + 1];
+            src_f = tf && (c->mat[tf - 1].if_FM || c->mat[tf - 1].if_AFM);
+        }
+        int64_t sz = -1;
+        if (src_i && !src_f) sz = zi - 1;
+        else if (!src_i && src_f) sz = zf + 1;
+        if (sz < 0) continue;
+        for (int64_t k = zi; k <= zf; k++) {
+            c->spin_src[k] = sz;
+            thick[k] = (double)lay[l] * c->dz;
+        }
+    }
+    for (int64_t k = 0; k < nz; k++) {
+        if (c->spin_src[k] < 0 || !c->cell[k]) continue;
+        const gf_material* mat = &c->mat[c->cell[k] - 1];
+        double cjs = mat->spin_hall_angle * UNIT_E / 2. / PI * mat->spin_mix_cond;
+        double dist = (fabs((double)(k - c->spin_src[k])) - 0.5) * c->dz;
+        double cl = sinh((thick[k] - dist) / mat->spin_diffus_length) / sinh(thick[k] / mat->spin_diffus_length);
+        if (!(k > c->spin_src[k])) cl = -1. * cl;
+        c->spin_coef[k] = cjs * cl;
+    }
+    free(lay);
+    free(thick);
+    return 0;
+}
+
+/* get_J_ISHE_RK* (magnetic_solver.cpp:231-283 and its stage copies): the
+ * current of each pumping plane from the stage store and the stage
+ * increment of its source plane, FM then both AFM sublattices. */
+static void spin_pump(gfo_ctx* c, int stage) {
+    int64_t ny = c->ny, nz = c->nz;
+    for (int64_t k = 0; k < nz; k++) {
+        int64_t sz = c->spin_src[k];
+        if (sz < 0) continue;
+        for (int64_t i = 0; i < c->nx; i++)
+            for (int64_t j = 0; j < ny; j++) {
+                int64_t id = (i * ny + j) * nz + k, is = (i * ny + j) * nz + sz;
+                double sx = 0., sy = 0.;
+                if (c->if_FM_all) {
+                    const double* const* st = (const double* const*)c->ms;
+                    const double* const* dm = (const double* const*)c->k[stage - 1];
+                    sx = st[0][is] * dm[2][is] - st[2][is] * dm[0][is];
+                    sy = st[1][is] * dm[2][is] - st[2][is] * dm[1][is];
+                }
+                if (c->if_AFM_all) {
+                    const double* const* s1 = (const double* const*)c->ms1;
+                    const double* const* d1 = (const double* const*)c->k1[stage - 1];
+                    const double* const* s2 = (const double* const*)c->ms2;
+                    const double* const* d2 = (const double* const*)c->k2[stage - 1];
+                    sx = sx + s1[0][is] * d1[2][is] - s1[2][is] * d1[0][is] + s2[0][is] * d2[2][is]
+                         - s2[2][is] * d2[0][is];
+                    sy = sy + s1[1][is] * d1[2][is] - s1[2][is] * d1[1][is] + s2[1][is] * d2[2][is]
+                         - s2[2][is] * d2[1][is];
+                }
+                c->Jis[0][id] = c->spin_coef[k] * sx / c->dt;
+                c->Jis[1][id] = c->spin_coef[k] * sy / c->dt;
+            }
+    }
+}
+
 gfo_ctx* gfo_create(const gf_desc* d) {
     if (!d || d->nx < 1 || d->ny < 1 || d->nz < 1) return NULL;
     gfo_ctx* c = (gfo_ctx*)calloc(1, sizeof(gfo_ctx));
@@ -598,12 +710,15 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         free(real9);
     }
     if (em_setup(c, d)) { gfo_destroy(c); return NULL; }
+    if ((c->flags & GF_FLAG_SPIN_PUMPING) && spin_setup(c, d)) { gfo_destroy(c); return NULL; }
     return c;
 }
 
 void gfo_destroy(gfo_ctx* c) {
     if (!c) return;
     free(c->mat); free(c->cell); free(c->pre);
+    free(c->spin_src); free(c->spin_coef);
+    for (int a = 0; a < 3; a++) free(c->Jis[a]);
     for (int a = 0; a < 3; a++) {
         free(c->m[a]); free(c->ms[a]); free(c->dm[a]);
         for (int s = 0; s < 4; s++) { free(c->k[s][a]); free(c->k1[s][a]); free(c->k2[s][a]); }
@@ -659,6 +774,8 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GFO_FIELD_Q_STORE: return c->qs;
     case GFO_FIELD_DV1: return c->kv[0];
     case GFO_FIELD_K1: return c->k[0];
+    case GFO_FIELD_KA1: return c->k1[0];
+    case GFO_FIELD_KA2: return c->k2[0];
     case GF_FIELD_U: return c->u;
     case GF_FIELD_V: return c->v;
     case GF_FIELD_U_STORE: return c->us;
@@ -669,6 +786,7 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_STRAIN_T0_OFF: return static_strain(c) ? c->e_t0 + 3 : NULL;
     case GF_FIELD_E_CELL: return c->if_em ? c->Ecell : NULL;
     case GF_FIELD_H_CELL: return c->if_em ? c->Hcell : NULL;
+    case GF_FIELD_J_ISHE: return c->spin_on ? c->Jis : NULL;
     default: return NULL;
     }
 }
@@ -920,6 +1038,7 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
         for (int a = 0; a < 3; a++)
             for (int64_t i = 0; i < c->n; i++)
                 c->dm[a][i] = c->k[0][a][i] / 6. + c->k[1][a][i] / 3. + c->k[2][a][i] / 3. + c->k[3][a][i] / 6.;
+    if (c->spin_on) spin_pump(c, stage);     /* magnetic_solver.cpp:226-228 */
 }
 
 /* magnetic_solver.cpp:1515-1636 */
@@ -997,6 +1116,7 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
                 c->dm1[a][i] = c->k1[0][a][i] / 6. + c->k1[1][a][i] / 3. + c->k1[2][a][i] / 3. + c->k1[3][a][i] / 6.;
                 c->dm2[a][i] = c->k2[0][a][i] / 6. + c->k2[1][a][i] / 3. + c->k2[2][a][i] / 3. + c->k2[3][a][i] / 6.;
             }
+    if (c->spin_on) spin_pump(c, stage);     /* magnetic_AFM_solver.cpp:426-428 */
 }
 
 /* magnetic_AFM_solver.cpp:1731-1834 */
@@ -1689,9 +1809,9 @@ static void em_jf(gfo_ctx* c, int stage) {
 }
 
 /* The four-cell means of get_dE_RK* (EMdynamic_solver.cpp:392-499) and the
- * Ampere increment; the spin-pumping current is not modelled (zero). */
+ * Ampere increment, with the spin-pumping current on x and y edges. */
 static double edge_dE(const gfo_ctx* c, int comp, const int64_t cell[4], double curl, double e_store, int stage) {
-    double Jf = 0., Jp = 0., dP = 0., er = 0., cond = 0., Pc = 0.;
+    double Jf = 0., Jp = 0., Ji = 0., dP = 0., er = 0., cond = 0., Pc = 0.;
     for (int q = 0; q < 4; q++) {
         int64_t id = cell[q];
         const gf_material* mat = matof(c, id);
@@ -1705,12 +1825,14 @@ static double edge_dE(const gfo_ctx* c, int comp, const int64_t cell[4], double 
         }
         Jf = Jf + c->Jf[comp][id];
         Jp = Jp + c->Jps[0][comp][id] + c->Jps[1][comp][id] + c->Jps[2][comp][id];
+        if (c->spin_on && comp < 2) Ji = Ji + c->Jis[comp][id];
     }
     if (Pc < 0.5) Pc = 1.;
     Jf = Jf / 4.;
     Jp = Jp / 4.;
+    Ji = Ji / 4.;
     dP = dP / Pc; er = er / 4.; cond = cond / 4.;
-    return c->dt / EPS0 / er * (curl - Jf - Jp - cond * e_store - dP / c->dt);
+    return c->dt / EPS0 / er * (curl - Jf - Jp - Ji - cond * e_store - dP / c->dt);
 }
 
 /* dM of one cell next to an H face (EMdynamic_solver.cpp:2290-2340) */

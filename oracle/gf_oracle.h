@@ -31,6 +31,8 @@ typedef struct gfo_ctx gfo_ctx;
 #define GFO_FIELD_DV1 102
 /* oracle-only field: the FM increment k of the last stage-1 get_dm */
 #define GFO_FIELD_K1 103
+#define GFO_FIELD_KA1 104   /* stage-1 increments of the AFM sublattices */
+#define GFO_FIELD_KA2 105
 
 gfo_ctx* gfo_create(const gf_desc* desc);
 void gfo_destroy(gfo_ctx* c);

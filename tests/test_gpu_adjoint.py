@@ -410,7 +410,7 @@ def test_fp32_torch_autograd_function():
     L = probe_loss(sim, alpha, aex, b1, amp)
     L.backward()
     g, L2, grad, ghd = run_adjoint32(prob, f, 12)
-    np.testing.assert_allclose(float(L), L2, rtol=1e-12)
+    np.testing.assert_allclose(float(L.detach()), L2, rtol=1e-12)
     np.testing.assert_allclose([alpha.grad.item(), aex.grad.item(), b1.grad.item()], grad, rtol=1e-12)
     np.testing.assert_allclose(amp.grad.numpy(), 4e4 * ghd[0, :, :, -1], rtol=1e-12, atol=0)
     g.close()

@@ -18,7 +18,7 @@ import pytest
 
 from goferro import (FIELD_HEXT_NONUNIF, FLAG_DEMAG_FACTOR, FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS,
                      FLAG_ELECTROSTATICS, FLAG_EMDYNAMIC, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE,
-                     FLAG_PRESCRIBE_HEXT, GoFerroError, Material, workloads)
+                     FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, GoFerroError, Material, workloads)
 from goferro import io as gio
 
 DECK = Path(__file__).resolve().parent / "golden" / "deck_fm"
@@ -161,8 +161,20 @@ def test_ferroelectric_sets_electrostatics_and_elastic_flags(tmp_path):
     assert (p[2, :, :, 3] == 0.5).all() and (p[:, :, :, :3] == 0).all()
 
 
+def test_spin_pumping_deck(tmp_path):
+    d = deck_copy(tmp_path, edits=[("if_spin_pumping", "true")],
+                  mat_edits=[(2, "if_spin_pump", "true"), (2, "spin_hall_angle", "0.08 5e18 3e-9")])
+    deck = gio.read_deck(d)
+    p = deck.problem
+    assert p.flags & FLAG_SPIN_PUMPING
+    sub = p.materials[1]
+    assert sub.if_spin_pump == 1 and (sub.spin_hall_angle, sub.spin_mix_cond, sub.spin_diffus_length) == (0.08, 5e18,
+                                                                                                         3e-9)
+    assert list(p.nz_layer) == [3, 1]
+
+
 @pytest.mark.parametrize("key,text,what", [
-    ("if_flexo", "true", "if_flexo"), ("if_spin_pumping", "true", "if_spin_pumping"),
+    ("if_flexo", "true", "if_flexo"),
     ("if_prescribed_m", "true 0 0", "if_prescribed_m"), ("if_elastostatic ", "true false", "elastostatic"),
     ("if_input_elastoforce", "true", "if_input_elastoforce")])
 def test_unsupported_options_are_named(tmp_path, key, text, what):
