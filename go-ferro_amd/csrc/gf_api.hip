@@ -300,6 +300,43 @@ gf_status mag_statics(gf_ctx* c) {
     return demag_run(c, fm, s1, s2);
 }
 
+// Flexoelectric constants of one material (read.cpp:181-199), zero for a
+// non-ferroelectric as check_material leaves them (global.cpp:339-358).
+FlexoMat make_flexo(const gf_material& m) {
+    FlexoMat f{};
+    if (!m.if_FE) return f;
+    const double c11 = m.c11, c12 = m.c12, c44 = m.c44, Q11 = m.Q11, Q12 = m.Q12, Q44 = m.Q44;
+    f.F11 = ((c11 + c12) * m.f11 - 2. * c12 * m.f12) / (c11 + 2. * c12) / (c11 - c12);
+    f.F12 = (c11 * m.f12 - c12 * m.f11) / (c11 + 2. * c12) / (c11 - c12);
+    f.F44 = m.f44 / c44;
+    f.tv4 = 2. * (c11 * (Q11 * f.F11 + 2. * Q12 * f.F12) + 2. * c12 * (Q11 * f.F12 + Q12 * f.F11 + Q12 * f.F12));
+    f.tv5 = 2. * (c11 * (Q11 * f.F12 + Q12 * f.F11 + Q12 * f.F12) +
+                  c12 * (Q11 * f.F11 + Q11 * f.F12 + Q12 * f.F11 + 3. * Q12 * f.F12));
+    f.tv6 = 2. * c44 * Q44 * f.F44;
+    f.tv7 = c11 * (f.F11 * f.F11 + 2. * f.F12 * f.F12) + 2. * c12 * (f.F12 * f.F12 + 2. * f.F11 * f.F12);
+    f.tv8 = c44 * f.F44 * f.F44;
+    f.f11 = m.f11; f.f12 = m.f12; f.f44 = m.f44;
+    return f;
+}
+
+gf_status flexo_setup(gf_ctx* c) {
+    if (!c->any_fe) return set_error(c, GF_ERR_UNSUPPORTED, "flexoelectricity needs a ferroelectric material");
+    std::vector<FlexoMat> tab;
+    for (const auto& m : c->host_mats) {
+        if (m.if_FE && (m.c11 == m.c12 || m.c11 + 2. * m.c12 == 0. || m.c44 == 0.))
+            return set_error(c, GF_ERR_ARG, "flexoelectricity needs c11 != c12, c11 + 2 c12 != 0 and c44 != 0");
+        tab.push_back(make_flexo(m));
+    }
+    gf_status st = hip_check(c, hipMalloc(&c->d_flexo, sizeof(FlexoMat) * tab.size()), "hipMalloc flexo table");
+    if (!st)
+        st = hip_check(c, hipMemcpy(c->d_flexo, tab.data(), sizeof(FlexoMat) * tab.size(), hipMemcpyHostToDevice),
+                       "copy flexo table");
+    if (!st) st = alloc3(c, c->flexo_force, "hipMalloc flexo force");
+    for (int a = 0; a < 3 && !st; a++)
+        st = hip_check(c, hipMemset(c->flexo_force[a], 0, sizeof(double) * c->n), "clear flexo force");
+    return st;
+}
+
 // Spin-pumping planes (global.cpp:169-235) and their coefficients
 // (magnetic_solver.cpp:245-259).  The layers are walked bottom to top from
 // k = 0 (gf_desc.nz_layer, else the runs of equal ids along z of column
@@ -853,6 +890,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     }
     if (!st && (c->flags & GF_FLAG_EMDYNAMIC)) st = em_setup(c, d);
     if (!st && (c->flags & GF_FLAG_SPIN_PUMPING)) st = spin_setup(c, d);
+    if (!st && (c->flags & GF_FLAG_FLEXO)) st = flexo_setup(c);
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
         st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
@@ -909,6 +947,8 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->Ecell); free3(c->Hcell); free3(c->Jf);
     free3(c->em_dm); free3(c->em_dm1); free3(c->em_dm2);
     free3(c->jis);
+    free3(c->flexo_force);
+    if (c->d_flexo) hipFree(c->d_flexo);
     if (c->spin_src) hipFree(c->spin_src);
     if (c->spin_coef) hipFree(c->spin_coef);
     for (int q = 0; q < 9; q++) {
@@ -1092,8 +1132,14 @@ gf_status gf_set_material(gf_ctx* c, int index, const gf_material* m) {
     if (st) return st;
     c->host_mats[static_cast<size_t>(index - 1)] = *m;
     const DevMaterial d = make_dev_material(*m, c->dt);
-    return hip_check(c, hipMemcpy(c->d_mats + (index - 1), &d, sizeof(DevMaterial), hipMemcpyHostToDevice),
-                     "copy material");
+    st = hip_check(c, hipMemcpy(c->d_mats + (index - 1), &d, sizeof(DevMaterial), hipMemcpyHostToDevice),
+                   "copy material");
+    if (!st && c->d_flexo) {
+        const FlexoMat f = make_flexo(*m);
+        st = hip_check(c, hipMemcpy(c->d_flexo + (index - 1), &f, sizeof(FlexoMat), hipMemcpyHostToDevice),
+                       "copy flexo constants");
+    }
+    return st;
 }
 
 gf_status gf_field_size(const gf_ctx* c, int id, size_t* count) {
@@ -1391,8 +1437,28 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
     a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
     a.stage = stage;
     hipEvent_t ev = nullptr;
+    gf_status st = GF_OK;
+    if (c->d_flexo) {
+        // the flexoelectric force of the stage store and of the strain the
+        // elastic stage found for it (get_p_gradient, get_Dstrain_gradient)
+        FeFlexoArgs f{};
+        f.code = c->d_code; f.fm = c->d_flexo;
+        for (int i = 0; i < 3; i++) {
+            f.p[i] = ps[i]; f.out[i] = c->flexo_force[i];
+            f.gyz[i] = c->el_on ? c->gyz[i] : nullptr;
+            f.gxz[i] = c->el_on ? c->gxz[i] : nullptr;
+            f.gxy[i] = c->el_on ? c->gxy[i] : nullptr;
+            f.per[i] = c->per[i];
+            a.flexo[i] = c->flexo_force[i];
+        }
+        for (int i = 0; i < 6; i++) f.eg[i] = c->el_on ? c->eg[i] : nullptr;
+        f.nx = c->nx; f.ny = c->ny; f.nz = c->nz; f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_fe_flexo(f, c->stream), "flexo force");
+        timer_end(c, GF_K_OTHER, ev);
+    }
     timer_begin(c, GF_K_FE_STAGE, &ev);
-    gf_status st = hip_check(c, launch_fe_stage(a, c->n, c->stream), "fe stage");
+    if (!st) st = hip_check(c, launch_fe_stage(a, c->n, c->stream), "fe stage");
     timer_end(c, GF_K_FE_STAGE, ev);
     if (st) return st;
     // dp = q_s dt of this stage feeds Ampere's law (get_dp_RK*, :1786-1832)
@@ -1433,7 +1499,9 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
             e.p[i] = ps ? ps[i] : nullptr;
         }
         for (int i = 0; i < 6; i++) { e.e0g[i] = c->e0g[i]; e.e0c[i] = c->e0c[i]; }
+        e.flexo = c->any_fe ? c->d_flexo : nullptr;
         e.n = c->n;
+        e.nx = c->nx; e.ny = c->ny; e.nz = c->nz; e.dx = c->dx; e.dy = c->dy; e.dz = c->dz;
         e.rotate = rotate;
         std::memcpy(e.Tg2c, c->Tg2c, sizeof(e.Tg2c));
         std::memcpy(e.Tc2g, c->Tc2g, sizeof(e.Tc2g));

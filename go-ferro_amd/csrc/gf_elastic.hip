@@ -128,6 +128,25 @@ __global__ void __launch_bounds__(256) el_eigen_kernel(ElEigenArgs a) {
         s[3] = 1.5 * M.l111 * my * mz + M.Q44 * py * pz + 0.5 * 1.5 * M.l111a * ay * az + 0.5 * 1.5 * M.l111b * by * bz - 0.;
         s[4] = 1.5 * M.l111 * mx * mz + M.Q44 * px * pz + 0.5 * 1.5 * M.l111a * ax * az + 0.5 * 1.5 * M.l111b * bx * bz - 0.;
         s[5] = 1.5 * M.l111 * my * mx + M.Q44 * py * px + 0.5 * 1.5 * M.l111a * ay * ax + 0.5 * 1.5 * M.l111b * by * bx - 0.;
+        if (a.flexo) {
+            // -F grad p (:82-89): the global-frame gradient of FE cells,
+            // zero elsewhere, enters the crystal-frame tensor
+            const FlexoMat& F = a.flexo[t - 1];
+            double g[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+            const uint32_t own = a.code[id];
+            if (own & kIsFE) {
+                const int64_t nyz = a.ny * a.nz;
+                const int64_t i = id / nyz, j = (id - i * nyz) / a.nz, k = id - i * nyz - j * a.nz;
+                const double h[3] = {a.dx, a.dy, a.dz};
+                fe_p_gradient(own, a.p, id, i, j, k, a.nx, a.ny, a.nz, h, g);
+            }
+            s[0] = s[0] - F.F11 * g[0][0] - F.F12 * (g[2][2] + g[1][1]);
+            s[1] = s[1] - F.F11 * g[1][1] - F.F12 * (g[2][2] + g[0][0]);
+            s[2] = s[2] - F.F11 * g[2][2] - F.F12 * (g[0][0] + g[1][1]);
+            s[3] = s[3] - F.F44 / 2. * (g[1][2] + g[2][1]);
+            s[4] = s[4] - F.F44 / 2. * (g[0][2] + g[2][0]);
+            s[5] = s[5] - F.F44 / 2. * (g[1][0] + g[0][1]);
+        }
         if (a.rotate) rot_voigt(a.Tc2g, s, o);
         else for (int c = 0; c < 6; c++) o[c] = s[c];
     }

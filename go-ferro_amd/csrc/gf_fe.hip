@@ -114,6 +114,12 @@ __global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
             kq[1] = kq[1] - M.fe_mass * a.ecell[1][id];
             kq[2] = kq[2] - M.fe_mass * a.ecell[2][id];
         }
+        if (a.flexo[0]) {
+            // flexoelectric force, computed by fe_flexo_kernel (:800-832)
+            kq[0] = kq[0] + M.fe_mass * a.flexo[0][id];
+            kq[1] = kq[1] + M.fe_mass * a.flexo[1][id];
+            kq[2] = kq[2] + M.fe_mass * a.flexo[2][id];
+        }
     }
     rk(a.stage, id, q0 * a.dt, a.pbase[0], a.pacc[0], a.pout[0]);
     rk(a.stage, id, q1 * a.dt, a.pbase[1], a.pacc[1], a.pout[1]);
@@ -123,7 +129,84 @@ __global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
     rk(a.stage, id, kq[2], a.qbase[2], a.qacc[2], a.qout[2]);
 }
 
+// The bracket of the flexoelectric term of get_dq_RK* at each FE cell
+// (ferroelectric_solver.cpp:800-832), the same sum order:
+//   + tv4 p_a dp_a/da + tv5 (...) + tv6 (...) - tv7 d2p_a/da2 - tv8 (...)
+//   - f11 de_aa/da - f12 (...) - 2 f44 (...)
+// with the p gradients of get_p_gradient, the second derivatives of the
+// Laplacian (FE surfaces mirror the centre) and the gradients of the
+// dynamic strain of get_Dstrain_gradient (elastic_dynamic_solver.cpp:
+// 230-399: the normal strain from the two face gradients, the others
+// central between faces that are not stress free, one-sided against a free
+// face, zero between two free faces).
+__global__ void __launch_bounds__(256) fe_flexo_kernel(FeFlexoArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= nx * ny * nz) return;
+    const uint32_t own = a.code[id];
+    if (!(own & kIsFE)) return;
+    const FlexoMat& F = a.fm[(own & kMatMask) - 1];
+    const int64_t nyz = ny * nz;
+    const int64_t i = id / nyz, j = (id - i * nyz) / nz, k = id - i * nyz - j * nz;
+    const double h[3] = {a.dx, a.dy, a.dz};
+    double g[3][3];
+    fe_p_gradient(own, a.p, id, i, j, k, nx, ny, nz, h, g);
+    const int64_t nb[6] = {((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k, ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k,
+                           (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k, (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k,
+                           (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1), (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1)};
+    const double c[3] = {a.p[0][id], a.p[1][id], a.p[2][id]};
+    double d2[3][3];
+    for (int ax = 0; ax < 3; ax++) {
+        const bool sb = (own >> (kFeSurfShift + 2 * ax)) & 1u;
+        const bool sf = (own >> (kFeSurfShift + 2 * ax + 1)) & 1u;
+        for (int cc = 0; cc < 3; cc++) {
+            const double bw = sb ? c[cc] : a.p[cc][nb[2 * ax]];
+            const double fw = sf ? c[cc] : a.p[cc][nb[2 * ax + 1]];
+            d2[cc][ax] = (fw - 2. * c[cc] + bw) / h[ax] / h[ax];
+        }
+    }
+    double e[6][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    if (a.eg[0]) {
+        // stress-free faces as the strain kernel finds them: a wall that is
+        // not periodic, or a vacuum neighbour
+        const bool wall[6] = {i == 0 && !a.per[0], i == nx - 1 && !a.per[0], j == 0 && !a.per[1],
+                              j == ny - 1 && !a.per[1], k == 0 && !a.per[2], k == nz - 1 && !a.per[2]};
+        for (int ax = 0; ax < 3; ax++) {
+            const bool sb = wall[2 * ax] || (a.code[nb[2 * ax]] & kMatMask) == 0u;
+            const bool sf = wall[2 * ax + 1] || (a.code[nb[2 * ax + 1]] & kMatMask) == 0u;
+            int64_t bwd = nb[2 * ax], fwd = nb[2 * ax + 1];
+            double scale = 2.;
+            if (sb && sf) { bwd = id; fwd = id; scale = 1.; }
+            else if (sb) { bwd = id; scale = 1.; }
+            else if (sf) { fwd = id; scale = 1.; }
+            for (int q = 0; q < 6; q++) e[q][ax] = (a.eg[q][fwd] - a.eg[q][bwd]) / h[ax] / scale;
+        }
+        e[0][0] = (a.gyz[0][((i + 1) * ny + j) * nz + k] - a.gyz[0][(i * ny + j) * nz + k]) / a.dx;
+        e[1][1] = (a.gxz[1][(i * (ny + 1) + j + 1) * nz + k] - a.gxz[1][(i * (ny + 1) + j) * nz + k]) / a.dy;
+        e[2][2] = (a.gxy[2][(i * ny + j) * (nz + 1) + k + 1] - a.gxy[2][(i * ny + j) * (nz + 1) + k]) / a.dz;
+    }
+    const double px = c[0], py = c[1], pz = c[2];
+    a.out[0][id] = +F.tv4 * px * g[0][0] + F.tv5 * (py * g[1][0] + pz * g[2][0]) +
+                   F.tv6 * (px * g[1][1] + px * g[2][2] + py * g[0][1] + pz * g[0][2]) - F.tv7 * d2[0][0] -
+                   F.tv8 * (d2[0][2] + d2[0][1]) - F.f11 * e[0][0] - F.f12 * (e[1][0] + e[2][0]) -
+                   2. * F.f44 * (e[5][1] + e[4][2]);
+    a.out[1][id] = +F.tv4 * py * g[1][1] + F.tv5 * (px * g[0][1] + pz * g[2][1]) +
+                   F.tv6 * (py * g[0][0] + py * g[2][2] + px * g[1][0] + pz * g[1][2]) - F.tv7 * d2[1][1] -
+                   F.tv8 * (d2[1][0] + d2[1][2]) - F.f11 * e[1][1] - F.f12 * (e[0][1] + e[2][1]) -
+                   2. * F.f44 * (e[5][0] + e[3][2]);
+    a.out[2][id] = +F.tv4 * pz * g[2][2] + F.tv5 * (px * g[0][2] + py * g[1][2]) +
+                   F.tv6 * (pz * g[0][0] + pz * g[1][1] + px * g[2][0] + py * g[2][1]) - F.tv7 * d2[2][2] -
+                   F.tv8 * (d2[2][0] + d2[2][1]) - F.f11 * e[2][2] - F.f12 * (e[0][2] + e[1][2]) -
+                   2. * F.f44 * (e[3][1] + e[4][0]);
+}
+
 }  // namespace
+
+hipError_t launch_fe_flexo(const FeFlexoArgs& a, hipStream_t s) {
+    const int64_t blocks = (a.nx * a.ny * a.nz + 255) / 256;
+    hipLaunchKernelGGL(fe_flexo_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s) {
     const int64_t blocks = (n + 255) / 256;

@@ -213,6 +213,31 @@ struct ElStrainArgs {
     double Tg2c[3][3];
 };
 
+// Flexoelectric constants of one material (read.cpp:181-199; zero unless FE).
+struct FlexoMat {
+    double F11, F12, F44, tv4, tv5, tv6, tv7, tv8, f11, f12, f44;
+};
+
+// get_p_gradient (ferroelectric_solver.cpp:134-281) at an FE cell of code
+// `own`: (forward - backward) / h / 2 of the p store, the cell's own value
+// standing in across an FE surface face (also halved there).  g[comp][axis].
+__device__ __forceinline__ void fe_p_gradient(uint32_t own, const double* const* p, int64_t id, int64_t i, int64_t j,
+                                              int64_t k, int64_t nx, int64_t ny, int64_t nz, const double h[3],
+                                              double g[3][3]) {
+    const int64_t nb[6] = {((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k, ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k,
+                           (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k, (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k,
+                           (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1), (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1)};
+    for (int ax = 0; ax < 3; ax++) {
+        const bool sb = (own >> (kFeSurfShift + 2 * ax)) & 1u;
+        const bool sf = (own >> (kFeSurfShift + 2 * ax + 1)) & 1u;
+        for (int c = 0; c < 3; c++) {
+            const double bw = sb ? p[c][id] : p[c][nb[2 * ax]];
+            const double fw = sf ? p[c][id] : p[c][nb[2 * ax + 1]];
+            g[c][ax] = (fw - bw) / h[ax] / 2.;
+        }
+    }
+}
+
 struct ElEigenArgs {
     const uint32_t* code;
     const DevMaterial* mats;
@@ -222,7 +247,10 @@ struct ElEigenArgs {
     const double* p[3];     // FE p store, null when there is no FE
     double* e0g[6];
     double* e0c[6];         // may equal e0g (no rotation)
+    const FlexoMat* flexo;  // flexoelectric -F grad p of the FE cells, null when off
     int64_t n;
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
     int32_t rotate;
     double Tg2c[3][3];
     double Tc2g[3][3];
@@ -280,6 +308,25 @@ struct ElBoundaryArgs {
     double src_val;
 };
 
+// The flexoelectric force of one stage at the FE cells
+// (ferroelectric_solver.cpp:800-832): p store, its gradients and second
+// derivatives, and the gradients of the dynamic strain
+// (elastic_dynamic_solver.cpp:230-399: strain eg and face gradients gyz,
+// gxz, gxy of the stage, null without elastodynamics).
+struct FeFlexoArgs {
+    const uint32_t* code;
+    const FlexoMat* fm;
+    const double* p[3];
+    const double* eg[6];
+    const double* gyz[3];
+    const double* gxz[3];
+    const double* gxy[3];
+    double* out[3];
+    int64_t nx, ny, nz;
+    double dx, dy, dz;
+    int32_t per[3];
+};
+
 struct FeStageArgs {
     const uint32_t* code;
     const DevMaterial* mats;
@@ -295,6 +342,7 @@ struct FeStageArgs {
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
     const double* eigen[6];     // eigenstrain (crystal frame), with strain
     const double* ecell[3];     // electrodynamic E at the cells, null when uncoupled
+    const double* flexo[3];     // flexoelectric force of the stage, null when off
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
     double eext[3];
@@ -532,6 +580,10 @@ struct gf_ctx {
     int32_t* spin_src = nullptr;
     double* spin_coef = nullptr;
     int mag_in_sel = 0, afm_in_sel = 0;
+    // flexoelectricity (GF_FLAG_FLEXO): per-material constants, the force
+    // of the current stage
+    gf::FlexoMat* d_flexo = nullptr;
+    double* flexo_force[3] = {};
     // statics
     double* hstat[3] = {};
     bool hstat_written = false;     // Hstat uploaded (it is zero otherwise unless statics run)
@@ -611,6 +663,7 @@ hipError_t launch_em_h(const EmArgs& a, hipStream_t s);
 hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s);
 hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s);
 hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s);
+hipError_t launch_fe_flexo(const FeFlexoArgs& a, hipStream_t s);
 hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s);
 hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s);
 hipError_t launch_em_plane(const EmMiscArgs& a, hipStream_t s);

@@ -29,7 +29,7 @@ __all__ = [
     "FLAG_EM_FROM_P", "FLAG_EM_FROM_M", "FLAG_EM_BACKACTION_P", "FLAG_EM_BACKACTION_M", "FLAG_PLANE_EM",
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
-    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FLAG_SPIN_PUMPING", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG",
+    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FLAG_SPIN_PUMPING", "FLAG_FLEXO", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG",
     "FIELD_STRAIN_T0_OFF", "FIELD_J_ISHE",
     "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
@@ -38,7 +38,7 @@ __all__ = [
 LIB_PATH = Path(__file__).resolve().parent / "libgoferro.so"
 MU0 = 1.256637061e-6  # magnetic_system.h:14
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 FLAG_MAGNETOSTATICS = 1 << 0
 FLAG_MAG_1D = 1 << 1
 FLAG_DEMAG_FACTOR = 1 << 2
@@ -65,6 +65,7 @@ FLAG_1D_ABC_EM_ONLYTOP = 1 << 22
 FLAG_HDRIVE = 1 << 23
 FLAG_ELECTROSTATICS = 1 << 24
 FLAG_SPIN_PUMPING = 1 << 25
+FLAG_FLEXO = 1 << 26
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
@@ -166,6 +167,7 @@ class Material(ctypes.Structure):
         ("if_spin_pump", ctypes.c_int32), ("reserved3", ctypes.c_int32),
         ("spin_hall_angle", ctypes.c_double), ("spin_mix_cond", ctypes.c_double),
         ("spin_diffus_length", ctypes.c_double),
+        ("f11", ctypes.c_double), ("f12", ctypes.c_double), ("f44", ctypes.c_double),
     ]
 
     @classmethod

@@ -39,7 +39,7 @@ from . import (
     FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS, FLAG_ELECTROSTATICS, FLAG_EM_BACKACTION_M, FLAG_EM_BACKACTION_P,
     FLAG_EM_FROM_M, FLAG_EM_FROM_P, FLAG_EMDYNAMIC, FLAG_GAUSSIAN_PULSE, FLAG_HEXT_NONUNIF, FLAG_JF_INPUT,
     FLAG_MAG_1D, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE, FLAG_PEC_XY, FLAG_PEC_Z, FLAG_PLANE_EM,
-    FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FIELD_J_ISHE,
+    FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FLAG_FLEXO, FIELD_J_ISHE,
     Context, GoFerroError, Material, Problem, load_library, split_yee, yee_shapes,
 )
 
@@ -351,7 +351,7 @@ def _material(rec: dict, Tc2g) -> Material:
             r[k] = 0.0
     if not r["if_FE"]:
         for k in ("FE_mass", "FE_damping", "a1", "a11", "a12", "a111", "a112", "a123", "a1111", "a1112", "a1122",
-                  "a1123", "G11", "Q11", "Q12", "Q44", "tv1", "tv2", "tv3"):
+                  "a1123", "G11", "Q11", "Q12", "Q44", "tv1", "tv2", "tv3", "f11", "f12", "f44"):
             r[k] = 0.0
     # global.cpp:360-362 divides each share by a sum that already holds the
     # shares normalised before it
@@ -463,8 +463,6 @@ def _unsupported(g: dict, mats: list) -> list[str]:
     any_fe = any(m["if_FE"] for m in mats)
     any_mag = any(m["if_FM"] or m["if_AFM"] for m in mats)
     out = []
-    if g["if_flexo"]:
-        out.append("if_flexo (flexoelectric coupling)")
     if g["if_prescribed_m"]:
         out.append("if_prescribed_m (prescribed precession)")
     if g["if_elastostatic"] or g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
@@ -520,6 +518,10 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
     for key, bit in flag_of:
         if g[key]:
             flags |= bit
+    # the flexoelectric terms act only with a ferroelectric (if_FE_all &&
+    # if_flexo, elastic_dynamic_solver.cpp:63, main.cpp:161-166)
+    if g["if_flexo"] and any_fe:
+        flags |= FLAG_FLEXO
     # get_E_static runs whenever there is a ferroelectric (main.cpp:120-131,
     # 214-226, 299-302)
     if any_fe and not g["if_elec_1Dmodel"]:

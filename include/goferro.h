@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GF_ABI_VERSION 2
+#define GF_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum gf_status_code {
@@ -87,6 +87,15 @@ typedef int gf_status;
  * later stages).  The layers come from gf_desc.nz_layer; GF_FIELD_J_ISHE
  * downloads it. */
 #define GF_FLAG_SPIN_PUMPING     (1u << 25)
+/* if_flexo: flexoelectric coupling of the ferroelectric.  The driving
+ * force gains tv4..tv8 times p and its gradients and f11, f12, f44 times
+ * the dynamic strain gradients (ferroelectric_solver.cpp:800-832 and the
+ * later stages; the gradients of get_p_gradient, :134-281, and
+ * get_Dstrain_gradient, elastic_dynamic_solver.cpp:230-399); the dynamic
+ * eigenstrain gains -F grad p (elastic_dynamic_solver.cpp:82-89).  F11,
+ * F12, F44 and tv4..tv8 are derived from c11, c12, c44, Q11, Q12, Q44 and
+ * f11, f12, f44 as read.cpp:181-199 does (zero for a non-ferroelectric). */
+#define GF_FLAG_FLEXO            (1u << 26)
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -138,6 +147,8 @@ typedef struct gf_material {
     int32_t if_spin_pump;
     int32_t reserved3;
     double spin_hall_angle, spin_mix_cond, spin_diffus_length;
+    /* flexoelectric coefficients (material.h, read.cpp:179) */
+    double f11, f12, f44;
 } gf_material;
 
 /*

@@ -93,6 +93,10 @@ struct gfo_ctx {
     int64_t* spin_src;  /* source_z per plane, -1 = no J_ISHE */
     double* spin_coef;  /* coeff_Js * coeff_L per plane */
     double* Jis[3];     /* Jx_ISHE, Jy_ISHE, zero */
+    /* flexoelectricity (GF_FLAG_FLEXO): per material F11, F12, F44,
+     * tv4..tv8, f11, f12, f44 (read.cpp:181-199, zero unless FE) */
+    int flexo_on;
+    double (*flx)[11];
 };
 
 static void elastic_gradients(gfo_ctx* c);
@@ -711,13 +715,34 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     }
     if (em_setup(c, d)) { gfo_destroy(c); return NULL; }
     if ((c->flags & GF_FLAG_SPIN_PUMPING) && spin_setup(c, d)) { gfo_destroy(c); return NULL; }
+    if (c->flags & GF_FLAG_FLEXO) {
+        /* read.cpp:181-199 and check_material's zeroing (global.cpp:339-358) */
+        c->flexo_on = 1;
+        c->flx = (double (*)[11])calloc((size_t)(c->nmat > 0 ? c->nmat : 1), sizeof(double[11]));
+        for (int t = 0; t < c->nmat; t++) {
+            const gf_material* m = &c->mat[t];
+            if (!m->if_FE) continue;
+            double c11 = m->c11, c12 = m->c12, c44 = m->c44, Q11 = m->Q11, Q12 = m->Q12, Q44 = m->Q44;
+            double F11 = ((c11 + c12) * m->f11 - 2. * c12 * m->f12) / (c11 + 2. * c12) / (c11 - c12);
+            double F12 = (c11 * m->f12 - c12 * m->f11) / (c11 + 2. * c12) / (c11 - c12);
+            double F44 = m->f44 / c44;
+            double* f = c->flx[t];
+            f[0] = F11; f[1] = F12; f[2] = F44;
+            f[3] = 2. * (c11 * (Q11 * F11 + 2. * Q12 * F12) + 2. * c12 * (Q11 * F12 + Q12 * F11 + Q12 * F12));
+            f[4] = 2. * (c11 * (Q11 * F12 + Q12 * F11 + Q12 * F12) + c12 * (Q11 * F11 + Q11 * F12 + Q12 * F11 + 3. * Q12 * F12));
+            f[5] = 2. * c44 * Q44 * F44;
+            f[6] = c11 * (F11 * F11 + 2. * F12 * F12) + 2. * c12 * (F12 * F12 + 2. * F11 * F12);
+            f[7] = c44 * F44 * F44;
+            f[8] = m->f11; f[9] = m->f12; f[10] = m->f44;
+        }
+    }
     return c;
 }
 
 void gfo_destroy(gfo_ctx* c) {
     if (!c) return;
     free(c->mat); free(c->cell); free(c->pre);
-    free(c->spin_src); free(c->spin_coef);
+    free(c->spin_src); free(c->spin_coef); free(c->flx);
     for (int a = 0; a < 3; a++) free(c->Jis[a]);
     for (int a = 0; a < 3; a++) {
         free(c->m[a]); free(c->ms[a]); free(c->dm[a]);
@@ -776,6 +801,9 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GFO_FIELD_K1: return c->k[0];
     case GFO_FIELD_KA1: return c->k1[0];
     case GFO_FIELD_KA2: return c->k2[0];
+    case GFO_FIELD_KQ1: return c->kq[0];
+    case GFO_FIELD_E0_DIAG: return c->if_elasto ? c->e0_glb : NULL;
+    case GFO_FIELD_E0_OFF: return c->if_elasto ? c->e0_glb + 3 : NULL;
     case GF_FIELD_U: return c->u;
     case GF_FIELD_V: return c->v;
     case GF_FIELD_U_STORE: return c->us;
@@ -1175,6 +1203,36 @@ static void fe_laplacian(const gfo_ctx* c, int64_t i, int64_t j, int64_t k, doub
     }
 }
 
+/* get_p_gradient (ferroelectric_solver.cpp:134-281) of the p store at an FE
+ * cell: (forward - backward) / h / 2 with the cell's own value across an
+ * FE surface face, halved at the surface too.  g[comp][axis]. */
+static void fe_p_gradient(const gfo_ctx* c, int64_t i, int64_t j, int64_t k, double g[3][3]) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    int64_t id = ID(c, i, j, k);
+    double dd[3] = {c->dx, c->dy, c->dz};
+    for (int ax = 0; ax < 3; ax++) {
+        int sb, sf;
+        int64_t nb, nf;
+        if (ax == 0) {
+            sb = sYZ(c->fsurfYZ, c, i, j, k); sf = sYZ(c->fsurfYZ, c, i + 1, j, k);
+            nb = ID(c, i == 0 ? nx - 1 : i - 1, j, k); nf = ID(c, i == nx - 1 ? 0 : i + 1, j, k);
+        } else if (ax == 1) {
+            sb = sXZ(c->fsurfXZ, c, i, j, k); sf = sXZ(c->fsurfXZ, c, i, j + 1, k);
+            nb = ID(c, i, j == 0 ? ny - 1 : j - 1, k); nf = ID(c, i, j == ny - 1 ? 0 : j + 1, k);
+        } else {
+            sb = sXY(c->fsurfXY, c, i, j, k); sf = sXY(c->fsurfXY, c, i, j, k + 1);
+            nb = ID(c, i, j, k == 0 ? nz - 1 : k - 1); nf = ID(c, i, j, k == nz - 1 ? 0 : k + 1);
+        }
+        for (int a = 0; a < 3; a++) {
+            double bw = sb ? c->ps[a][id] : c->ps[a][nb];
+            double fw = sf ? c->ps[a][id] : c->ps[a][nf];
+            g[a][ax] = (fw - bw) / dd[ax] / 2.;
+        }
+    }
+}
+
+static void strain_gradients(const gfo_ctx* c, int64_t i, int64_t j, int64_t k, double d[6][3]);
+
 /* get_dq_RKs (ferroelectric_solver.cpp:534-842, RK2-4 at :844, :1154,
  * :1464 differ only in the drive time) and get_dp_RKs (:1786-1832).
  * Strain terms are zero without an elastic solver; no flexo, no EM. */
@@ -1251,6 +1309,40 @@ void gfo_fe_get(gfo_ctx* c, int stage) {
                       + FE_damping * c->qs[a][id];
         if (c->if_em && (c->flags & GF_FLAG_EM_BACKACTION_P))   /* ferroelectric_solver.cpp:794-798 */
             for (int a = 0; a < 3; a++) dq[a][id] = dq[a][id] - FE_mass * c->Ecell[a][id];
+        if (c->flexo_on) {
+            /* ferroelectric_solver.cpp:800-832; the strain gradients are
+             * those of the dynamic strain (zero without elastodynamics) */
+            const double* f = c->flx[c->cell[id] - 1];
+            double tv4 = f[3], tv5 = f[4], tv6 = f[5], tv7 = f[6], tv8 = f[7], f11 = f[8], f12 = f[9], f44 = f[10];
+            double g[3][3], e[6][3] = {{0.}};
+            fe_p_gradient(c, i, j, k, g);
+            if (c->if_elasto) strain_gradients(c, i, j, k, e);
+            /* e[comp][axis], comp xx yy zz yz xz xy */
+            dq[0][id] = dq[0][id] + FE_mass * (
+                + tv4 * pgx * g[0][0]
+                + tv5 * (pgy * g[1][0] + pgz * g[2][0])
+                + tv6 * (pgx * g[1][1] + pgx * g[2][2] + pgy * g[0][1] + pgz * g[0][2])
+                - tv7 * d2[0][0]
+                - tv8 * (d2[0][2] + d2[0][1])
+                - f11 * e[0][0] - f12 * (e[1][0] + e[2][0])
+                - 2. * f44 * (e[5][1] + e[4][2]));
+            dq[1][id] = dq[1][id] + FE_mass * (
+                + tv4 * pgy * g[1][1]
+                + tv5 * (pgx * g[0][1] + pgz * g[2][1])
+                + tv6 * (pgy * g[0][0] + pgy * g[2][2] + pgx * g[1][0] + pgz * g[1][2])
+                - tv7 * d2[1][1]
+                - tv8 * (d2[1][0] + d2[1][2])
+                - f11 * e[1][1] - f12 * (e[0][1] + e[2][1])
+                - 2. * f44 * (e[5][0] + e[3][2]));
+            dq[2][id] = dq[2][id] + FE_mass * (
+                + tv4 * pgz * g[2][2]
+                + tv5 * (pgx * g[0][2] + pgy * g[1][2])
+                + tv6 * (pgz * g[0][0] + pgz * g[1][1] + pgx * g[2][0] + pgy * g[2][1])
+                - tv7 * d2[2][2]
+                - tv8 * (d2[2][0] + d2[2][1])
+                - f11 * e[2][2] - f12 * (e[0][2] + e[1][2])
+                - 2. * f44 * (e[3][1] + e[4][0]));
+        }
     }
     for (int a = 0; a < 3; a++)
         for (int64_t id = 0; id < c->n; id++) dp[a][id] = c->qs[a][id] * c->dt;
@@ -1284,6 +1376,40 @@ static int sfz(const gfo_ctx* c, int64_t i, int64_t j, int64_t k) { return c->sf
 #define GYZ(a, i, j, k) c->gYZ[a][((i) * c->ny + (j)) * c->nz + (k)]
 #define GXZ(a, i, j, k) c->gXZ[a][((i) * (c->ny + 1) + (j)) * c->nz + (k)]
 #define GXY(a, i, j, k) c->gXY[a][((i) * c->ny + (j)) * (c->nz + 1) + (k)]
+
+/* get_Dstrain_gradient (elastic_dynamic_solver.cpp:230-399) at a solid cell:
+ * along each axis the normal strain from its two face gradients, the other
+ * components as (forward - backward) / h / scale, central (scale 2) between
+ * two faces that are not stress free, one-sided (scale 1) against a free
+ * face, zero between two free faces.  d[comp][axis]. */
+static void strain_gradients(const gfo_ctx* c, int64_t i, int64_t j, int64_t k, double d[6][3]) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+    double dd[3] = {c->dx, c->dy, c->dz};
+    for (int ax = 0; ax < 3; ax++) {
+        int sb, sf;
+        int64_t pos, n;
+        if (ax == 0) { sb = sfx(c, i, j, k); sf = sfx(c, i + 1, j, k); pos = i; n = nx; }
+        else if (ax == 1) { sb = sfy(c, i, j, k); sf = sfy(c, i, j + 1, k); pos = j; n = ny; }
+        else { sb = sfz(c, i, j, k); sf = sfz(c, i, j, k + 1); pos = k; n = nz; }
+        int64_t bwd, fwd;
+        double scale;
+        int64_t prev = pos == 0 ? n - 1 : pos - 1, next = pos == n - 1 ? 0 : pos + 1;
+        if (sb == sf) {
+            if (sb) { bwd = pos; fwd = pos; scale = 1.; }
+            else { scale = 2.; bwd = prev; fwd = next; }
+        } else if (sb) {
+            bwd = pos; scale = 1.; fwd = next;
+        } else {
+            fwd = pos; scale = 1.; bwd = prev;
+        }
+        int64_t ib = ax == 0 ? ID(c, bwd, j, k) : (ax == 1 ? ID(c, i, bwd, k) : ID(c, i, j, bwd));
+        int64_t ifw = ax == 0 ? ID(c, fwd, j, k) : (ax == 1 ? ID(c, i, fwd, k) : ID(c, i, j, fwd));
+        for (int e = 0; e < 6; e++) d[e][ax] = (c->e_glb[e][ifw] - c->e_glb[e][ib]) / dd[ax] / scale;
+        if (ax == 0) d[0][0] = (GYZ(0, i + 1, j, k) - GYZ(0, i, j, k)) / c->dx;
+        else if (ax == 1) d[1][1] = (GXZ(1, i, j + 1, k) - GXZ(1, i, j, k)) / c->dy;
+        else d[2][2] = (GXY(2, i, j, k + 1) - GXY(2, i, j, k)) / c->dz;
+    }
+}
 
 /* mathlib.cpp:78-125 */
 static void mat_rot(const double T[3][3], const double in[3][3], double out[3][3]) {
@@ -1321,6 +1447,22 @@ static void elastic_eigenstrain(gfo_ctx* c) {
         s[1][2] = 1.5 * l111 * my * mz + Q44 * py * pz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz - 0.;
         s[0][2] = 1.5 * l111 * mx * mz + Q44 * px * pz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz - 0.;
         s[0][1] = 1.5 * l111 * my * mx + Q44 * py * px + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx - 0.;
+        if (c->flexo_on && c->if_FE_all) {
+            /* elastic_dynamic_solver.cpp:82-89: the global-frame p gradient
+             * of FE cells (zero elsewhere) enters the crystal-frame tensor */
+            const double* f = c->flx[c->cell[id] - 1];
+            double F11 = f[0], F12 = f[1], F44 = f[2], g[3][3] = {{0.}};
+            if (mat->if_FE) {
+                int64_t i = id / (c->ny * c->nz), j = (id / c->nz) % c->ny, k = id % c->nz;
+                fe_p_gradient(c, i, j, k, g);
+            }
+            s[0][0] = s[0][0] - F11 * g[0][0] - F12 * (g[2][2] + g[1][1]);
+            s[1][1] = s[1][1] - F11 * g[1][1] - F12 * (g[2][2] + g[0][0]);
+            s[2][2] = s[2][2] - F11 * g[2][2] - F12 * (g[0][0] + g[1][1]);
+            s[1][2] = s[1][2] - F44 / 2. * (g[1][2] + g[2][1]);
+            s[0][2] = s[0][2] - F44 / 2. * (g[0][2] + g[2][0]);
+            s[0][1] = s[0][1] - F44 / 2. * (g[1][0] + g[0][1]);
+        }
         c->e0_crt[0][id] = s[0][0]; c->e0_crt[1][id] = s[1][1]; c->e0_crt[2][id] = s[2][2];
         c->e0_crt[3][id] = s[1][2]; s[2][1] = s[1][2];
         c->e0_crt[4][id] = s[0][2]; s[2][0] = s[0][2];

@@ -33,6 +33,9 @@ typedef struct gfo_ctx gfo_ctx;
 #define GFO_FIELD_K1 103
 #define GFO_FIELD_KA1 104   /* stage-1 increments of the AFM sublattices */
 #define GFO_FIELD_KA2 105
+#define GFO_FIELD_KQ1 106   /* stage-1 increment of q */
+#define GFO_FIELD_E0_DIAG 107   /* dynamic eigenstrain (global frame) */
+#define GFO_FIELD_E0_OFF 108
 
 gfo_ctx* gfo_create(const gf_desc* desc);
 void gfo_destroy(gfo_ctx* c);

@@ -18,7 +18,7 @@ import pytest
 
 from goferro import (FIELD_HEXT_NONUNIF, FLAG_DEMAG_FACTOR, FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS,
                      FLAG_ELECTROSTATICS, FLAG_EMDYNAMIC, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE,
-                     FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, GoFerroError, Material, workloads)
+                     FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FLAG_FLEXO, GoFerroError, Material, workloads)
 from goferro import io as gio
 
 DECK = Path(__file__).resolve().parent / "golden" / "deck_fm"
@@ -173,8 +173,21 @@ def test_spin_pumping_deck(tmp_path):
     assert list(p.nz_layer) == [3, 1]
 
 
+def test_flexo_deck(tmp_path):
+    # flexo acts only with a ferroelectric (if_FE_all && if_flexo)
+    d = deck_copy(tmp_path, edits=[("if_flexo", "true"), ("if_magnetostatics", "false false")],
+                  mat_edits=[(2, "if_FE", "true"), (2, "f11 f12 f44", "1e-10 -2e-11 3e-11")])
+    deck = gio.read_deck(d)
+    assert deck.problem.flags & FLAG_FLEXO
+    sub = deck.problem.materials[1]
+    assert (sub.f11, sub.f12, sub.f44) == (1e-10, -2e-11, 3e-11)
+    assert deck.problem.materials[0].f11 == 0.0     # not a ferroelectric: zeroed
+    (tmp_path / "b").mkdir()
+    d2 = deck_copy(tmp_path / "b", edits=[("if_flexo", "true")])
+    assert not gio.read_deck(d2).problem.flags & FLAG_FLEXO
+
+
 @pytest.mark.parametrize("key,text,what", [
-    ("if_flexo", "true", "if_flexo"),
     ("if_prescribed_m", "true 0 0", "if_prescribed_m"), ("if_elastostatic ", "true false", "elastostatic"),
     ("if_input_elastoforce", "true", "if_input_elastoforce")])
 def test_unsupported_options_are_named(tmp_path, key, text, what):
