@@ -285,6 +285,14 @@ double** afm2_store(gf_ctx* c, int sel) { return sel == 0 ? c->m2 : (sel == 1 ? 
 double** fe_pstore(gf_ctx* c, int sel) { return sel == 0 ? c->p : (sel == 1 ? c->p_a : c->p_b); }
 double** fe_qstore(gf_ctx* c, int sel) { return sel == 0 ? c->q : (sel == 1 ? c->q_a : c->q_b); }
 
+// The static strain of the current magnet stores.
+gf_status elastostatic_static(gf_ctx* c) {
+    double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : nullptr;
+    double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : nullptr;
+    double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : nullptr;
+    return elastostatic_run(c, fm, s1, s2);
+}
+
 // The magnetostatic part of gf_demag: H of the current magnet stores.
 gf_status mag_statics(gf_ctx* c) {
     if (!(c->any_fm || c->any_afm)) return GF_OK;
@@ -294,7 +302,8 @@ gf_status mag_statics(gf_ctx* c) {
     double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : zero;
     if (c->flags & GF_FLAG_MAG_1D) return hip_check(c, launch_demag_local(c, fm, s1, s2, 1), "demag 1d");
     if (c->flags & GF_FLAG_DEMAG_FACTOR) return hip_check(c, launch_demag_local(c, fm, s1, s2, 0), "demag factor");
-    if (!(c->flags & GF_FLAG_MAGNETOSTATICS) && (c->flags & GF_FLAG_ELECTROSTATICS)) return GF_OK;
+    // no magnetostatics asked for (the static fields of other physics may be)
+    if (!(c->flags & GF_FLAG_MAGNETOSTATICS)) return GF_OK;
     if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT magnetostatics was not set up (flags)");
     if (c->demag.fused) return demag_run_fused(c, fm);
     return demag_run(c, fm, s1, s2);
@@ -891,6 +900,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     if (!st && (c->flags & GF_FLAG_EMDYNAMIC)) st = em_setup(c, d);
     if (!st && (c->flags & GF_FLAG_SPIN_PUMPING)) st = spin_setup(c, d);
     if (!st && (c->flags & GF_FLAG_FLEXO)) st = flexo_setup(c);
+    if (!st && (c->flags & GF_FLAG_ELASTOSTATIC)) st = elastostatic_setup(c, d);
     if (!st) {
         c->red_blocks = std::min<int64_t>(1024, (c->n + 255) / 256);
         st = hip_check(c, hipMalloc(&c->red_partial, sizeof(double) * 3 * c->red_blocks), "hipMalloc reduction");
@@ -949,6 +959,7 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->jis);
     free3(c->flexo_force);
     if (c->d_flexo) hipFree(c->d_flexo);
+    if (c->es_buf) hipFree(c->es_buf);
     if (c->spin_src) hipFree(c->spin_src);
     if (c->spin_coef) hipFree(c->spin_coef);
     for (int q = 0; q < 9; q++) {
@@ -1759,6 +1770,10 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
 
 gf_status gf_demag(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
+    if (c->flags & GF_FLAG_ELASTOSTATIC) {
+        // the static strain of the current magnet stores (main.cpp:148-158, 214)
+        if (gf_status st = elastostatic_static(c)) return st;
+    }
     if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->any_fe) {
         // initial get_E_static (main.cpp:120-131)
         if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT electrostatics was not set up");
@@ -1786,6 +1801,8 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
             if (!st && c->em_on) st = gf_em_stage(c, stage);
             if (!st) st = statics_after_update(c);
         }
+        // the static strain after the step without elastodynamics (main.cpp:472-474)
+        if (!st && (c->flags & GF_FLAG_ELASTOSTATIC)) st = elastostatic_static(c);
         if (st) return st;
     }
     return GF_OK;

@@ -584,6 +584,10 @@ struct gf_ctx {
     // of the current stage
     gf::FlexoMat* d_flexo = nullptr;
     double* flexo_force[3] = {};
+    // elastostatics (GF_FLAG_ELASTOSTATIC): six complex work fields and the
+    // homogeneous c11, c12, c44
+    double2* es_buf = nullptr;
+    double es_c[3] = {};
     // statics
     double* hstat[3] = {};
     bool hstat_written = false;     // Hstat uploaded (it is zero otherwise unless statics run)
@@ -619,6 +623,10 @@ namespace gf {
 // error helpers (gf_api.hip)
 gf_status set_error(gf_ctx* c, gf_status code, const std::string& msg);
 gf_status hip_check(gf_ctx* c, hipError_t e, const char* what);
+// elastostatics (gf_elastostatic.hip): set-up, and the static strain of the
+// given magnet stores into e_t0
+gf_status elastostatic_setup(gf_ctx* c, const gf_desc* d);
+gf_status elastostatic_run(gf_ctx* c, double* const* fm, double* const* a1, double* const* a2);
 // the thread's error of a call without a context (gf_last_error(NULL))
 gf_status set_thread_error(gf_status code, const std::string& msg);
 void timer_begin(gf_ctx* c, int cls, hipEvent_t* ev);

@@ -29,7 +29,7 @@ __all__ = [
     "FLAG_EM_FROM_P", "FLAG_EM_FROM_M", "FLAG_EM_BACKACTION_P", "FLAG_EM_BACKACTION_M", "FLAG_PLANE_EM",
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
-    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FLAG_SPIN_PUMPING", "FLAG_FLEXO", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG",
+    "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FLAG_SPIN_PUMPING", "FLAG_FLEXO", "FLAG_ELASTOSTATIC", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG",
     "FIELD_STRAIN_T0_OFF", "FIELD_J_ISHE",
     "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
@@ -66,6 +66,7 @@ FLAG_HDRIVE = 1 << 23
 FLAG_ELECTROSTATICS = 1 << 24
 FLAG_SPIN_PUMPING = 1 << 25
 FLAG_FLEXO = 1 << 26
+FLAG_ELASTOSTATIC = 1 << 27
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
@@ -215,6 +216,7 @@ class _Desc(ctypes.Structure):
         ("scale_elec", ctypes.c_double),
         ("n_layers", ctypes.c_int32), ("reserved3", ctypes.c_int32),
         ("nz_layer", ctypes.POINTER(ctypes.c_int32)),
+        ("scale_elasto", ctypes.c_double),
     ]
 
 
@@ -269,6 +271,7 @@ class Problem:
     Hdrive_freq: float = 0.0
     scale_elec: float = 1.0
     nz_layer: Optional[Sequence[int]] = None       # layer stack bottom to top; None: runs of column (0, 0)
+    scale_elasto: float = 1.0
 
     @property
     def n(self) -> int:
@@ -322,6 +325,7 @@ class Problem:
         d.Jf_input_sigma = float(self.Jf_input_sigma)
         d.Hdrive_freq = float(self.Hdrive_freq)
         d.scale_elec = float(self.scale_elec)
+        d.scale_elasto = float(self.scale_elasto)
         keep = [cell, mats]
         if self.nz_layer is not None:
             lay = np.ascontiguousarray(self.nz_layer, dtype=np.int32)

@@ -39,7 +39,7 @@ from . import (
     FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS, FLAG_ELECTROSTATICS, FLAG_EM_BACKACTION_M, FLAG_EM_BACKACTION_P,
     FLAG_EM_FROM_M, FLAG_EM_FROM_P, FLAG_EMDYNAMIC, FLAG_GAUSSIAN_PULSE, FLAG_HEXT_NONUNIF, FLAG_JF_INPUT,
     FLAG_MAG_1D, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE, FLAG_PEC_XY, FLAG_PEC_Z, FLAG_PLANE_EM,
-    FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FLAG_FLEXO, FIELD_J_ISHE,
+    FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FLAG_FLEXO, FLAG_ELASTOSTATIC, FIELD_J_ISHE,
     Context, GoFerroError, Material, Problem, load_library, split_yee, yee_shapes,
 )
 
@@ -465,9 +465,11 @@ def _unsupported(g: dict, mats: list) -> list[str]:
     out = []
     if g["if_prescribed_m"]:
         out.append("if_prescribed_m (prescribed precession)")
-    if g["if_elastostatic"] or g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
-        out.append("the elastostatic solver (if_elastostatic*, if_read_strain_ext; get_strain_static is declared "
-                   "but not defined in the reference)")
+    if g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
+        out.append("the 1D and film elastostatics and external strain (if_elastostatic_1D, if_elastostatic_film, "
+                   "if_read_strain_ext)")
+    if g["if_elastostatic"] and (g["if_elastodynamics"] or any_fe):
+        out.append("if_elastostatic with elastodynamics or a ferroelectric (the static solve covers magnets alone)")
     if any_fe and g["if_elec_1Dmodel"]:
         out.append("if_elec_1Dmodel (get_E_static_1D is declared but not defined in the reference)")
     fft_h = any_mag and g["if_magnetostatics"] and not g["if_mag_1Dmodel"] and not g["if_demag_factor"]
@@ -522,6 +524,9 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
     # if_flexo, elastic_dynamic_solver.cpp:63, main.cpp:161-166)
     if g["if_flexo"] and any_fe:
         flags |= FLAG_FLEXO
+    # get_strain_static after every step without elastodynamics (main.cpp:214, 472-474)
+    if g["if_elastostatic"] and not g["if_elastodynamics"]:
+        flags |= FLAG_ELASTOSTATIC
     # get_E_static runs whenever there is a ferroelectric (main.cpp:120-131,
     # 214-226, 299-302)
     if any_fe and not g["if_elec_1Dmodel"]:
@@ -549,6 +554,7 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
         Jf_input_amp=g["Jf_input_amp"], Jf_input_freq=g["Jf_input_freq"], Jf_input_sigma=g["Jf_input_sigma"],
         scale_elec=g["scale_elec"],
         nz_layer=None if g["if_read_struct"] else list(nzl),
+        scale_elasto=g["scale_elasto"],
     )
     if g["if_prescribe_Hext"]:
         prob.prescribe_Hext = _table(d / "Hext.in", g["num_prescribe_Hext"])

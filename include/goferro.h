@@ -96,6 +96,19 @@ typedef int gf_status;
  * F12, F44 and tv4..tv8 are derived from c11, c12, c44, Q11, Q12, Q44 and
  * f11, f12, f44 as read.cpp:181-199 does (zero for a non-ferroelectric). */
 #define GF_FLAG_FLEXO            (1u << 26)
+/* if_elastostatic: the static strain of the magnets' eigenstrain
+ * (get_strain_static, declared at elastic_system.h:190 but not defined in
+ * the reference), in a periodic body of the homogeneous cubic stiffness of
+ * elastic_initialize.cpp:22-40 (component-wise largest c11, c12, c44 times
+ * scale_elasto) through its Green tensor G_ik (:340-389):
+ * e(k) = sym(k (x) G(k) sigma0(k) k) for k != 0, with sigma0 = C e0 and
+ * e0 the eigenstrain of elastic_dynamic_solver.cpp:4-104, and the mean
+ * strain equal to the mean eigenstrain (zero mean stress).  A Nyquist
+ * component enters symmetrised over its sign, so the strain is real.
+ * Written into GF_FIELD_STRAIN_T0_* (crystal frame) by gf_demag and after
+ * every gf_step step (main.cpp:214, 473).  Magnets only, without
+ * elastodynamics; power-of-two nx, ny, nz up to 2048. */
+#define GF_FLAG_ELASTOSTATIC     (1u << 27)
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -213,6 +226,8 @@ typedef struct gf_desc {
     int32_t n_layers;
     int32_t reserved3;
     const int32_t* nz_layer;
+    /* stiffness scale of the elastostatics (read.cpp:65); 0 = 1 */
+    double scale_elasto;
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */

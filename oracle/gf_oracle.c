@@ -97,6 +97,8 @@ struct gfo_ctx {
      * tv4..tv8, f11, f12, f44 (read.cpp:181-199, zero unless FE) */
     int flexo_on;
     double (*flx)[11];
+    /* elastostatics (GF_FLAG_ELASTOSTATIC): homogeneous c11, c12, c44 */
+    double ch11, ch12, ch44;
 };
 
 static void elastic_gradients(gfo_ctx* c);
@@ -715,6 +717,18 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     }
     if (em_setup(c, d)) { gfo_destroy(c); return NULL; }
     if ((c->flags & GF_FLAG_SPIN_PUMPING) && spin_setup(c, d)) { gfo_destroy(c); return NULL; }
+    if (c->flags & GF_FLAG_ELASTOSTATIC) {
+        /* elastic_initialize.cpp:22-40: the component-wise largest
+         * constants, scaled */
+        double sc = d->scale_elasto > 0. ? d->scale_elasto : 1.;
+        c->ch11 = c->mat[0].c11; c->ch12 = c->mat[0].c12; c->ch44 = c->mat[0].c44;
+        for (int t = 1; t < c->nmat; t++) {
+            if (c->mat[t].c11 > c->ch11) c->ch11 = c->mat[t].c11;
+            if (c->mat[t].c12 > c->ch12) c->ch12 = c->mat[t].c12;
+            if (c->mat[t].c44 > c->ch44) c->ch44 = c->mat[t].c44;
+        }
+        c->ch11 = c->ch11 * sc; c->ch12 = c->ch12 * sc; c->ch44 = c->ch44 * sc;
+    }
     if (c->flags & GF_FLAG_FLEXO) {
         /* read.cpp:181-199 and check_material's zeroing (global.cpp:339-358) */
         c->flexo_on = 1;
@@ -810,8 +824,8 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_STRAIN_DIAG: return c->e_glb;
     case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
     case GF_FIELD_HDRIVE: return (c->flags & GF_FLAG_HDRIVE) ? c->Hdrive : NULL;
-    case GF_FIELD_STRAIN_T0_DIAG: return static_strain(c) ? c->e_t0 : NULL;
-    case GF_FIELD_STRAIN_T0_OFF: return static_strain(c) ? c->e_t0 + 3 : NULL;
+    case GF_FIELD_STRAIN_T0_DIAG: return (static_strain(c) || (c->flags & GF_FLAG_ELASTOSTATIC)) ? c->e_t0 : NULL;
+    case GF_FIELD_STRAIN_T0_OFF: return (static_strain(c) || (c->flags & GF_FLAG_ELASTOSTATIC)) ? c->e_t0 + 3 : NULL;
     case GF_FIELD_E_CELL: return c->if_em ? c->Ecell : NULL;
     case GF_FIELD_H_CELL: return c->if_em ? c->Hcell : NULL;
     case GF_FIELD_J_ISHE: return c->spin_on ? c->Jis : NULL;
@@ -2353,9 +2367,133 @@ void gfo_estatic(gfo_ctx* c) {
     free(Pk); free(Ek);
 }
 
+/* Static strain of the magnets' eigenstrain (GF_FLAG_ELASTOSTATIC;
+ * get_strain_static is declared at elastic_system.h:190 without a
+ * definition).  Periodic body of the homogeneous cubic stiffness
+ * (elastic_initialize.cpp:22-40), crystal frame.  With sigma0 = C e0 and
+ * the Green tensor G = (C_ijkl k_j k_l)^-1 of elastic_initialize.cpp:
+ * 340-389 (k in the crystal frame, |k|^2 of the global grid),
+ * u = -i G (sigma0 k) and e_ij = (k_j w_i + k_i w_j) / 2, w = G (sigma0 k);
+ * e(0) = e0(0) (zero mean stress).  A Nyquist component is averaged over
+ * its two signs. */
+static void es_operator(const gfo_ctx* c, const double kg[3], const double sig[6][2], double eps[6][2]) {
+    double c11 = c->ch11, c12 = c->ch12, c44 = c->ch44;
+    double kq[3];
+    glb2crt(c, kg[0], kg[1], kg[2], &kq[0], &kq[1], &kq[2]);
+    double kn = kg[0] * kg[0] + kg[1] * kg[1] + kg[2] * kg[2];
+    if (kn == 0.) {
+        double den = (c11 - c12) * (c11 + 2. * c12);
+        for (int r = 0; r < 2; r++) {
+            eps[0][r] = ((c11 + c12) * sig[0][r] - c12 * (sig[1][r] + sig[2][r])) / den;
+            eps[1][r] = ((c11 + c12) * sig[1][r] - c12 * (sig[0][r] + sig[2][r])) / den;
+            eps[2][r] = ((c11 + c12) * sig[2][r] - c12 * (sig[0][r] + sig[1][r])) / den;
+            eps[3][r] = sig[3][r] / (2. * c44);
+            eps[4][r] = sig[4][r] / (2. * c44);
+            eps[5][r] = sig[5][r] / (2. * c44);
+        }
+        return;
+    }
+    double chi = (c11 - c12 - 2. * c44) / c44;
+    double x2 = kq[0] * kq[0], y2 = kq[1] * kq[1], z2 = kq[2] * kq[2];
+    double d0 = c11 * kn * kn * kn + chi * (c11 + c12) * kn * (x2 * y2 + y2 * z2 + z2 * x2)
+              + chi * chi * (c11 + 2. * c12 + c44) * x2 * y2 * z2;
+    double G[3][3];
+    G[0][0] = (c44 * kn * kn + (c11 + c12) * chi * y2 * z2 + (c11 - c44) * kn * (y2 + z2)) / (c44 * d0);
+    G[1][1] = (c44 * kn * kn + (c11 + c12) * chi * z2 * x2 + (c11 - c44) * kn * (z2 + x2)) / (c44 * d0);
+    G[2][2] = (c44 * kn * kn + (c11 + c12) * chi * x2 * y2 + (c11 - c44) * kn * (x2 + y2)) / (c44 * d0);
+    G[0][1] = G[1][0] = -(c12 + c44) * (kn + chi * z2) * kq[0] * kq[1] / (c44 * d0);
+    G[1][2] = G[2][1] = -(c12 + c44) * (kn + chi * x2) * kq[1] * kq[2] / (c44 * d0);
+    G[0][2] = G[2][0] = -(c12 + c44) * (kn + chi * y2) * kq[0] * kq[2] / (c44 * d0);
+    for (int r = 0; r < 2; r++) {
+        double S[3][3] = {{sig[0][r], sig[5][r], sig[4][r]}, {sig[5][r], sig[1][r], sig[3][r]},
+                          {sig[4][r], sig[3][r], sig[2][r]}};
+        double t[3], w[3];
+        for (int m = 0; m < 3; m++) t[m] = S[m][0] * kq[0] + S[m][1] * kq[1] + S[m][2] * kq[2];
+        for (int i = 0; i < 3; i++) w[i] = G[i][0] * t[0] + G[i][1] * t[1] + G[i][2] * t[2];
+        eps[0][r] = kq[0] * w[0];
+        eps[1][r] = kq[1] * w[1];
+        eps[2][r] = kq[2] * w[2];
+        eps[3][r] = (kq[2] * w[1] + kq[1] * w[2]) / 2.;
+        eps[4][r] = (kq[2] * w[0] + kq[0] * w[2]) / 2.;
+        eps[5][r] = (kq[1] * w[0] + kq[0] * w[1]) / 2.;
+    }
+}
+
+/* The magnets' eigenstrain in the crystal frame (elastic_dynamic_solver.cpp:
+ * 21-73 without the ferroelectric terms), Voigt order xx yy zz yz xz xy. */
+static void es_eigen(const gfo_ctx* c, int64_t id, double e[6]) {
+    const gf_material* mat = matof(c, id);
+    for (int a = 0; a < 6; a++) e[a] = 0.;
+    if (!mat) return;
+    double mx = 0, my = 0, mz = 0, ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    if (c->if_FM_all) glb2crt(c, c->ms[0][id], c->ms[1][id], c->ms[2][id], &mx, &my, &mz);
+    if (c->if_AFM_all) {
+        glb2crt(c, c->ms1[0][id], c->ms1[1][id], c->ms1[2][id], &ax, &ay, &az);
+        glb2crt(c, c->ms2[0][id], c->ms2[1][id], c->ms2[2][id], &bx, &by, &bz);
+    }
+    double l100 = mat->lamda100, l111 = mat->lamda111, l100a = mat->lamda100_AFM1, l111a = mat->lamda111_AFM1;
+    double l100b = mat->lamda100_AFM2, l111b = mat->lamda111_AFM2;
+    e[0] = 1.5 * l100 * (mx * mx - 1. / 3.) + 0.5 * 1.5 * l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * l100b * (bx * bx - 1. / 3.);
+    e[1] = 1.5 * l100 * (my * my - 1. / 3.) + 0.5 * 1.5 * l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * l100b * (by * by - 1. / 3.);
+    e[2] = 1.5 * l100 * (mz * mz - 1. / 3.) + 0.5 * 1.5 * l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * l100b * (bz * bz - 1. / 3.);
+    e[3] = 1.5 * l111 * my * mz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz;
+    e[4] = 1.5 * l111 * mx * mz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz;
+    e[5] = 1.5 * l111 * my * mx + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx;
+}
+
+void gfo_elastostatic(gfo_ctx* c) {
+    int64_t nx = c->nx, ny = c->ny, nz = c->nz, nzh = nz / 2 + 1, nh = nx * ny * nzh, n = c->n;
+    double c11 = c->ch11, c12 = c->ch12, c44 = c->ch44;
+    double* sg = (double*)malloc(sizeof(double) * 6 * n);
+    double* S = (double*)malloc(sizeof(double) * 2 * nh * 6);
+    double* E = (double*)malloc(sizeof(double) * 2 * nh * 6);
+    for (int64_t id = 0; id < n; id++) {
+        double e[6];
+        es_eigen(c, id, e);
+        sg[0 * n + id] = c11 * e[0] + c12 * (e[1] + e[2]);
+        sg[1 * n + id] = c11 * e[1] + c12 * (e[0] + e[2]);
+        sg[2 * n + id] = c11 * e[2] + c12 * (e[0] + e[1]);
+        sg[3 * n + id] = 2. * c44 * e[3];
+        sg[4 * n + id] = 2. * c44 * e[4];
+        sg[5 * n + id] = 2. * c44 * e[5];
+    }
+    for (int a = 0; a < 6; a++) gfo_rfft3(nx, ny, nz, sg + a * n, S + a * 2 * nh);
+    for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t k = 0; k < nzh; k++) {
+                int nyq[3];
+                double kg[3];
+                kg[0] = kwave(i, nx, c->dx, &nyq[0]);
+                kg[1] = kwave(j, ny, c->dy, &nyq[1]);
+                kg[2] = kwave(k, nz, c->dz, &nyq[2]);
+                int64_t q = (i * ny + j) * nzh + k;
+                double sig[6][2], acc[6][2] = {{0.}}, eps[6][2];
+                for (int a = 0; a < 6; a++) { sig[a][0] = S[a * 2 * nh + 2 * q]; sig[a][1] = S[a * 2 * nh + 2 * q + 1]; }
+                int nv = 0;
+                for (int f = 0; f < 8; f++) {
+                    if (((f & 1) && !nyq[0]) || ((f & 2) && !nyq[1]) || ((f & 4) && !nyq[2])) continue;
+                    double kf[3] = {(f & 1) ? -kg[0] : kg[0], (f & 2) ? -kg[1] : kg[1], (f & 4) ? -kg[2] : kg[2]};
+                    es_operator(c, kf, sig, eps);
+                    for (int a = 0; a < 6; a++) { acc[a][0] += eps[a][0]; acc[a][1] += eps[a][1]; }
+                    nv++;
+                }
+                for (int a = 0; a < 6; a++) {
+                    E[a * 2 * nh + 2 * q] = acc[a][0] / (double)nv;
+                    E[a * 2 * nh + 2 * q + 1] = acc[a][1] / (double)nv;
+                }
+            }
+    for (int a = 0; a < 6; a++) {
+        gfo_irfft3(nx, ny, nz, E + a * 2 * nh, c->e_t0[a]);
+        for (int64_t q = 0; q < n; q++) c->e_t0[a][q] = c->e_t0[a][q] / (double)n;
+    }
+    free(sg); free(S); free(E);
+}
+
 void gfo_demag(gfo_ctx* c) {
+    if (c->flags & GF_FLAG_ELASTOSTATIC) gfo_elastostatic(c);
     if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->if_FE_all) gfo_estatic(c);
     if (!(c->if_FM_all || c->if_AFM_all)) return;
+    if (!(c->flags & (GF_FLAG_MAGNETOSTATICS | GF_FLAG_MAG_1D | GF_FLAG_DEMAG_FACTOR))) return;
     if (c->flags & GF_FLAG_MAG_1D) demag_1d(c);
     else demag_3d(c);
 }
@@ -2388,6 +2526,8 @@ void gfo_step(gfo_ctx* c, int64_t nsteps) {
             if (!em && one_d && ((c->if_FM_all && ms) || c->if_AFM_all)) demag_1d(c);
             if (!em && mag && !one_d && ms) demag_3d(c);
         }
+        /* main.cpp:472-474: the static strain after the step without elastodynamics */
+        if (c->flags & GF_FLAG_ELASTOSTATIC) gfo_elastostatic(c);
     }
 }
 

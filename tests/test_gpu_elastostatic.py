@@ -1,0 +1,119 @@
+"""GPU parity of the elastostatics (GF_FLAG_ELASTOSTATIC) through the C ABI
+against the oracle (pinned by closed forms in test_oracle_elastostatic.py):
+the static strain of random magnetisations on mixed blocks (rotated
+crystal, two magnets and a non-magnetic material, an antiferromagnet), and
+magnets stepping with the static strain recomputed after every step and
+acting back through the magnetoelastic field.  1e-12 of the peak.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from goferro import (FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF,
+                     FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS, FLAG_ELASTOSTATIC, FLAG_ORIENT_ROTATE, Context,
+                     GoFerroError, Material, Problem, workloads)
+from goferro.workloads import GYRO, set_elastic, set_magnetostriction
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def rot():
+    c, s = np.cos(0.3), np.sin(0.3)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    c2, s2 = np.cos(0.7), np.sin(0.7)
+    Ry = np.array([[c2, 0, s2], [0, 1, 0], [-s2, 0, c2]])
+    return Ry @ Rz
+
+
+def mixed(shape=(16, 8, 32), rotate=True, flags=0):
+    T = rot() if rotate else None
+    a = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.05, K1=-610.0, Aex=3.7e-12)
+    set_elastic(a, 2.69e11, 1.08e11, 0.76e11, 5170.0, Tg2c=T)
+    set_magnetostriction(a, 3.5e5, 2.0e5)
+    b = Material.make(if_FM=1, anisotropy_type=2, Ms=8e5, gyro=GYRO, FM_damping=0.1, K1=2e4, Aex=1.3e-11)
+    set_elastic(b, 2.2e11, 1.2e11, 0.9e11, 8900.0, Tg2c=T)
+    set_magnetostriction(b, -6e5, 4e5)
+    sub = set_elastic(Material.make(), 3.0e11, 1.0e11, 1.1e11, 5100.0, Tg2c=T)
+    rng = np.random.default_rng(3)
+    cell = rng.choice(np.array([0, 1, 2, 3], np.uint32), size=shape, p=[0.1, 0.45, 0.3, 0.15])
+    prob = Problem(shape=shape, spacing=(2e-9, 2.5e-9, 1.5e-9), materials=[a, b, sub], material_cell=cell,
+                   dt=5e-15, periodic=(True, True, False), Tg2c=T, Hext_stat=(1e4, 0, 6e4),
+                   flags=FLAG_ELASTOSTATIC | flags | (FLAG_ORIENT_ROTATE if rotate else 0), scale_elasto=1.3)
+    m = workloads.tilted_m(shape, 5, theta_max=0.6)
+    return prob, {FIELD_M: m}
+
+
+def strain(obj):
+    return np.concatenate([obj.get(FIELD_STRAIN_T0_DIAG) if hasattr(obj, "get") else obj.download(FIELD_STRAIN_T0_DIAG),
+                           obj.get(FIELD_STRAIN_T0_OFF) if hasattr(obj, "get") else obj.download(FIELD_STRAIN_T0_OFF)])
+
+
+def pair(prob, fields):
+    from oracle.oracle import Oracle
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in fields.items():
+        g.upload(fid, v)
+        o.set(fid, v)
+    return g, o
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_static_strain_matches_oracle(rotate):
+    say("case elastostatic strain, rotate", rotate)
+    prob, f = mixed(rotate=rotate)
+    g, o = pair(prob, f)
+    g.demag()
+    o.demag()
+    d = rel(strain(g), strain(o))
+    say("  rel", d)
+    assert d <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_afm_static_strain():
+    say("case elastostatic strain, AFM")
+    prob, m1, m2 = workloads.afm_block(shape=(8, 16, 16))
+    mat = prob.materials[0]
+    set_elastic(mat, 2.5e11, 1.3e11, 0.8e11, 6800.0)
+    mat.lamda100_AFM1, mat.lamda100_AFM2, mat.lamda111_AFM1, mat.lamda111_AFM2 = -2e-5, -1e-5, 1e-5, 3e-5
+    prob.flags |= FLAG_ELASTOSTATIC
+    g, o = pair(prob, {FIELD_M_AFM1: m1, FIELD_M_AFM2: m2})
+    g.demag()
+    o.demag()
+    assert rel(strain(g), strain(o)) <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_magnets_step_with_the_static_strain():
+    say("case elastostatic coupled steps")
+    prob, f = mixed(shape=(8, 8, 16), flags=FLAG_ELASTO_ON_PM)
+    g, o = pair(prob, f)
+    g.demag()
+    o.demag()
+    g.step(3)
+    o.step(3)
+    assert np.abs(g.download(FIELD_M) - o.get(FIELD_M)).max() <= 1e-12
+    assert rel(strain(g), strain(o)) <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_elastostatic_rejections():
+    say("case elastostatic rejections")
+    prob, _ = mixed(flags=FLAG_ELASTODYNAMICS)
+    with pytest.raises(GoFerroError, match="elastodynamics"):
+        Context(prob)
+    prob, _ = mixed(shape=(8, 8, 12))
+    with pytest.raises(GoFerroError, match="power-of-two"):
+        Context(prob)

@@ -188,7 +188,7 @@ def test_flexo_deck(tmp_path):
 
 
 @pytest.mark.parametrize("key,text,what", [
-    ("if_prescribed_m", "true 0 0", "if_prescribed_m"), ("if_elastostatic ", "true false", "elastostatic"),
+    ("if_prescribed_m", "true 0 0", "if_prescribed_m"), ("if_elastostatic ", "true true", "elastostatics"),
     ("if_input_elastoforce", "true", "if_input_elastoforce")])
 def test_unsupported_options_are_named(tmp_path, key, text, what):
     d = deck_copy(tmp_path, edits=[(key, text)])

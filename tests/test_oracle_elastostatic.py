@@ -1,0 +1,153 @@
+"""Pin the oracle's elastostatics (GF_FLAG_ELASTOSTATIC; get_strain_static
+is declared at elastic_system.h:190 but not defined in the reference, so
+its set-up, elastic_initialize.cpp:22-40 and :340-389, and closed forms fix
+it).  The eigenstrain is the magnetostrictive one of a cubic crystal,
+e0_ii = 3/2 lambda100 (m_i^2 - 1/3), e0_ij = 3/2 lambda111 m_i m_j
+(textbook form, restated here in numpy):
+
+* uniform m: the strain is the eigenstrain, everywhere;
+* m varying along z only: the laterally periodic solution
+  e_xx, e_yy, e_xy = their means, e_xz = e0_xz, e_yz = e0_yz,
+  e_zz = e0_zz + (c12 / c11) (e0_xx + e0_yy - mean), from sigma_iz = 0;
+* random m: at every non-Nyquist k != 0 the stress C (e - e0) is
+  divergence-free (k . sigma = 0) and the strain is compatible
+  (Saint-Venant: k x (k x e)^T = 0), and the mean strain is the mean
+  eigenstrain; together these fix the solution;
+* the homogeneous stiffness is the component-wise largest c11, c12, c44
+  times scale_elasto.
+"""
+import numpy as np
+import pytest
+
+from goferro import (FIELD_M, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FLAG_ELASTOSTATIC, Material, Problem,
+                     workloads)
+from goferro.workloads import GYRO, set_elastic
+
+C11, C12, C44 = 2.69e11, 1.08e11, 0.76e11
+L100, L111 = -1.4e-5, 2.2e-5
+
+
+def magnet(c=(C11, C12, C44)):
+    m = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.1, Aex=3.7e-12,
+                      lamda100=L100, lamda111=L111)
+    return set_elastic(m, c[0], c[1], c[2], 5170.0)
+
+
+def problem(shape, mats=None, cell=None, scale=1.0, spacing=(1e-9, 1.5e-9, 2e-9)):
+    mats = mats or [magnet()]
+    cell = np.ones(shape, np.uint32) if cell is None else cell
+    return Problem(shape=shape, spacing=spacing, materials=mats, material_cell=cell, dt=1e-14,
+                   periodic=(True, True, True), flags=FLAG_ELASTOSTATIC, scale_elasto=scale)
+
+
+def static_strain(oracle_lib, prob, m):
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_M, m)
+    o.demag()
+    e = np.concatenate([o.get(FIELD_STRAIN_T0_DIAG), o.get(FIELD_STRAIN_T0_OFF)])
+    o.close()
+    return e
+
+
+def eigen(m, l100=L100, l111=L111):
+    e = np.empty((6,) + m.shape[1:])
+    for a in range(3):
+        e[a] = 1.5 * l100 * (m[a] ** 2 - 1. / 3.)
+    e[3] = 1.5 * l111 * m[1] * m[2]
+    e[4] = 1.5 * l111 * m[0] * m[2]
+    e[5] = 1.5 * l111 * m[0] * m[1]
+    return e
+
+
+def test_uniform_magnetisation_gives_its_eigenstrain(oracle_lib):
+    shape = (4, 4, 8)
+    m = np.zeros((3,) + shape)
+    m[:] = (np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8]))[:, None, None, None]
+    e = static_strain(oracle_lib, problem(shape), m)
+    want = eigen(m)
+    np.testing.assert_allclose(e, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_profile_along_z(oracle_lib):
+    shape = (2, 2, 16)
+    z = np.arange(16)
+    th = 0.9 * np.sin(2 * np.pi * z / 16) + 0.4 * np.cos(6 * np.pi * z / 16)
+    ph = 0.3 + 0.5 * np.sin(4 * np.pi * z / 16)
+    m = np.zeros((3,) + shape)
+    m[0] = (np.sin(th) * np.cos(ph))[None, None, :]
+    m[1] = (np.sin(th) * np.sin(ph))[None, None, :]
+    m[2] = np.cos(th)[None, None, :]
+    e = static_strain(oracle_lib, problem(shape), m)
+    e0 = eigen(m)
+    want = np.empty_like(e0)
+    for a in (0, 1, 5):
+        want[a] = e0[a].mean()
+    want[3], want[4] = e0[3], e0[4]
+    s = e0[0] + e0[1]
+    want[2] = e0[2] + C12 / C11 * (s - s.mean())
+    np.testing.assert_allclose(e, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_random_field_is_in_equilibrium_and_compatible(oracle_lib, rotate):
+    shape = (8, 8, 8)
+    spacing = (1e-9, 1.5e-9, 2e-9)
+    prob = problem(shape, spacing=spacing)
+    T = np.eye(3)
+    if rotate:
+        c, s = np.cos(0.4), np.sin(0.4)
+        T = np.array([[c, s, 0], [-s, c, 0], [0, 0, 1]]) @ np.array([[1, 0, 0], [0, np.cos(0.3), np.sin(0.3)],
+                                                                       [0, -np.sin(0.3), np.cos(0.3)]])
+        from goferro import FLAG_ORIENT_ROTATE
+        prob.flags |= FLAG_ORIENT_ROTATE
+        prob.Tg2c = T
+    m = workloads.tilted_m(shape, 17, theta_max=1.4)
+    e = static_strain(oracle_lib, prob, m)
+    mc = np.einsum("ab,b...->a...", T, m)              # crystal frame
+    e0 = eigen(mc)
+    assert np.abs(e.reshape(6, -1).mean(axis=1) - e0.reshape(6, -1).mean(axis=1)).max() <= 1e-12 * np.abs(e0).max()
+    d = e - e0
+    sig = np.empty_like(d)
+    sig[0] = C11 * d[0] + C12 * (d[1] + d[2])
+    sig[1] = C11 * d[1] + C12 * (d[0] + d[2])
+    sig[2] = C11 * d[2] + C12 * (d[0] + d[1])
+    sig[3:] = 2 * C44 * d[3:]
+    Sk = np.array([np.fft.fftn(x) for x in sig])
+    Ek = np.array([np.fft.fftn(x) for x in e])
+    kv = [2 * np.pi * np.fft.fftfreq(n, h) for n, h in zip(shape, spacing)]
+    nyq = [np.isclose(np.abs(np.fft.fftfreq(n)), 0.5) for n in shape]
+    K = np.array(np.meshgrid(*kv, indexing="ij"))
+    Kc = np.einsum("ab,b...->a...", T, K)              # crystal-frame wave vectors
+    N = np.meshgrid(*nyq, indexing="ij")
+    ok = ~(N[0] | N[1] | N[2]) & ((K ** 2).sum(0) > 0)
+    full = lambda v: np.array([[v[0], v[5], v[4]], [v[5], v[1], v[3]], [v[4], v[3], v[2]]])
+    S, E = full(Sk), full(Ek)
+    div = np.einsum("ij...,j...->i...", S, Kc)
+    smax = np.abs(np.einsum("ij...,j...->i...", np.abs(S), np.abs(Kc))).max()
+    assert np.abs(div[:, ok]).max() <= 1e-10 * smax
+    # Saint-Venant: eps_ijk eps_lmn k_j k_m E_kn = 0
+    lc = np.zeros((3, 3, 3))
+    lc[0, 1, 2] = lc[1, 2, 0] = lc[2, 0, 1] = 1
+    lc[0, 2, 1] = lc[2, 1, 0] = lc[1, 0, 2] = -1
+    inc = np.einsum("ijk,lmn,j...,m...,kn...->il...", lc, lc, Kc, Kc, E)
+    emax = np.abs(np.einsum("j...,m...,kn...->...", np.abs(Kc), np.abs(Kc), np.abs(E))).max()
+    assert np.abs(inc[:, :, ok]).max() <= 1e-10 * emax
+
+
+def test_homogeneous_stiffness_is_the_largest_scaled(oracle_lib):
+    # two magnets with different constants: the z-profile closed form
+    # holds with the component-wise largest c11 and c12 (times the scale)
+    shape = (2, 2, 8)
+    mats = [magnet((2.0e11, 1.3e11, 0.5e11)), magnet((2.5e11, 0.9e11, 0.7e11))]
+    cell = np.ones(shape, np.uint32)
+    cell[:, :, 4:] = 2
+    z = np.arange(8)
+    th = 0.7 * np.sin(2 * np.pi * z / 8)
+    m = np.zeros((3,) + shape)
+    m[0] = np.sin(th)[None, None, :]
+    m[2] = np.cos(th)[None, None, :]
+    e = static_strain(oracle_lib, problem(shape, mats=mats, cell=cell, scale=1.7), m)
+    e0 = eigen(m)
+    s = e0[0] + e0[1]
+    want = e0[2] + 1.3e11 / 2.5e11 * (s - s.mean())
+    np.testing.assert_allclose(e[2], want, rtol=0, atol=1e-12 * np.abs(want).max())
