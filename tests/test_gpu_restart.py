@@ -223,3 +223,19 @@ def test_deck_restart_reproduces_the_run(tmp_path):
     for name in ("m.20.dat", "m_magcell.20.dat", "Hstat.20.dat"):
         assert (part / name).read_bytes() == (full / name).read_bytes(), name
     assert (part / "average_m.dat").read_text() == (full / "average_m.dat").read_text()
+
+
+def test_restart_spin_pumping_flexo_elastostatics():
+    """The couplings added to the stages carry through the image too."""
+    from test_gpu_elastostatic import mixed
+    from test_gpu_flexo import flexo_film
+    from test_gpu_spin import film
+    from goferro import FIELD_J_ISHE, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FLAG_EMDYNAMIC, FLAG_ELASTO_ON_PM
+    prob, f = film(flags=FLAG_EMDYNAMIC | FLAG_1D_ABC)
+    split_run("spin + em", prob, f, (FIELD_M, FIELD_E_EM, FIELD_H_EM, FIELD_J_ISHE), 3, 4, statics=False)
+    prob, p = flexo_film()
+    z = np.zeros((3,) + prob.shape)
+    split_run("flexo", prob, {FIELD_P: p, FIELD_Q: z, FIELD_U: z, FIELD_V: z}, (FIELD_P, FIELD_Q, FIELD_U, FIELD_V),
+              3, 3, statics=False)
+    prob, f = mixed(shape=(8, 8, 16), flags=FLAG_ELASTO_ON_PM)
+    split_run("elastostatics", prob, f, (FIELD_M, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF), 2, 3)
