@@ -239,3 +239,43 @@ def test_restart_spin_pumping_flexo_elastostatics():
               3, 3, statics=False)
     prob, f = mixed(shape=(8, 8, 16), flags=FLAG_ELASTO_ON_PM)
     split_run("elastostatics", prob, f, (FIELD_M, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF), 2, 3)
+
+
+def test_fe_em_deck_run_matches_oracle(tmp_path):
+    """A ferroelectric/electrodynamics deck (the golden deck edited) through
+    io.run: p, the Yee files, the one-cell EM record and the averages."""
+    from oracle.oracle import Oracle
+    from test_io import deck_copy
+    say("case fe + em deck run")
+    fe_rows = [(1, "if_FM", "false"), (1, "if_FE", "true"), (1, "FE_mass", "1e-16 4e-4"), (1, "a1", "-1.69e8"),
+               (1, "a11 a12", "-7.3e7 7.5e8"), (1, "a111", "2.6e8 6.1e8 -3.7e9"), (1, "G11", "1e-10"),
+               (1, "r_permittivity", "10"), (2, "r_permittivity", "4")]
+    d = deck_copy(tmp_path, mat_edits=fe_rows, edits=[
+        ("dt (s)", "1e-18"), ("px_init", "0 0 0.5"), ("if_magnetostatics", "false false"),
+        ("if_EMdynamic", "true false"), ("if_EM_fromP", "true false"), ("if_EM_backaction_on_P", "true false"),
+        ("if_input_m", "false false false"), ("if_output_m", "false 1 false 1"), ("if_output_Hstat", "false 1"),
+        ("if_output_p", "true 10"), ("if_output_em ", "true 10 true 5 3"), ("if_output_emYee", "true 20"),
+        ("if_output_Estat", "true 20"), ("if_Jf_input", "true"), ("Jfin_xi", "0 0 0"), ("Jfin_xf", "7 7 0"),
+        ("Jf_input_type", "6 x 5e15 1e12 2e-17 1")])
+    out = tmp_path / "out"
+    ctx = gio.run(d, out_dir=out)
+    deck = gio.read_deck(d)
+    o = Oracle(deck.problem)
+    for fid, v in gio.initial_fields(deck).items():
+        o.set(fid, v)
+    o.demag()
+    o.step(20)
+    want = o.get(FIELD_P)
+    got = gio.read_cells(out / "p.20.dat", deck.problem.shape, 3)
+    assert np.abs(got - want).max() <= 5e-8 * np.abs(want).max()
+    np.testing.assert_allclose(ctx.download(FIELD_P), want, rtol=0, atol=1e-12 * np.abs(want).max())
+    ex = gio.read_cells(out / "Ex_Yee.20.dat", (8, 9, 5), 1)[0]
+    ex_want = np.asarray(o.get(FIELD_E_EM)[:8 * 9 * 5]).reshape(8, 9, 5)
+    assert np.abs(ex - ex_want).max() <= 5e-8 * np.abs(ex_want).max() + 1e-300
+    one = np.loadtxt(out / "EM_onecell.dat")
+    assert one.shape == (4, 7) and list(one[:, 0]) == [5, 10, 15, 20]
+    ave = np.loadtxt(out / "average_p.dat")
+    assert list(ave[:, 0]) == [0, 5, 10, 15, 20]
+    assert (out / "Estat.0.dat").exists() and (out / "Eem.20.dat").exists() and (out / "Hz_Yee.20.dat").exists()
+    ctx.close()
+    o.close()
