@@ -1,0 +1,128 @@
+"""Further pins of the oracle against textbook forms.
+
+* The dynamic eigenstrain (elastic_dynamic_solver.cpp:4-104) of random
+  magnetisations and polarisations in a rotated crystal equals the
+  magnetostrictive (3/2 lambda100 (a_i^2 - 1/3), 3/2 lambda111 a_i a_j;
+  half weight per AFM sublattice) plus electrostrictive (Q11 p_i^2 +
+  Q12 (p_j^2 + p_k^2), Q44 p_i p_j) tensor, built in the crystal frame
+  and rotated back with the tensor law (numpy here).
+* Ampere's law with the four-cell means of permittivity and conductivity
+  at material interfaces (EMdynamic_solver.cpp:392-499): from a uniform
+  E_x and H = 0, one tiny step of a layered conductor changes every x
+  edge by -dt <sigma> E / (eps0 <er>), the means over the four cells
+  around the edge.
+"""
+import numpy as np
+import pytest
+
+from goferro import (FIELD_E_EM, FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FLAG_ELASTO_BACKACTION,
+                     FLAG_ELASTODYNAMICS, FLAG_EMDYNAMIC, FLAG_ORIENT_ROTATE, Material, Problem, split_yee,
+                     workloads)
+from goferro.workloads import GYRO, set_elastic
+
+EPS0 = 8.854187817e-12
+E0D, E0O = 107, 108
+
+
+def rot():
+    c, s = np.cos(0.5), np.sin(0.5)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    c2, s2 = np.cos(0.8), np.sin(0.8)
+    Rx = np.array([[1, 0, 0], [0, c2, -s2], [0, s2, c2]])
+    return Rx @ Rz
+
+
+def tensor(v):
+    return np.array([[v[0], v[5], v[4]], [v[5], v[1], v[3]], [v[4], v[3], v[2]]])
+
+
+@pytest.mark.parametrize("kind", ["fm", "afm", "fe"])
+def test_eigenstrain_is_the_textbook_tensor(oracle_lib, kind):
+    shape = (6, 6, 6)
+    T = rot()
+    if kind == "fm":
+        mat = Material.make(if_FM=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.1, Aex=3.7e-12, lamda100=-2e-5, lamda111=3e-5)
+    elif kind == "afm":
+        mat = Material.make(if_AFM=1, Ms_AFM1=3e5, Ms_AFM2=3e5, gyro_AFM1=GYRO, gyro_AFM2=GYRO, damping_AFM1=0.01,
+                            damping_AFM2=0.01, uniaxis_AFM=(0, 0, 1), lamda100_AFM1=-2e-5, lamda100_AFM2=1e-5,
+                            lamda111_AFM1=3e-5, lamda111_AFM2=-4e-5)
+    else:
+        mat = Material.make(if_FE=1, FE_mass=1e-16, Q11=0.089, Q12=-0.026, Q44=0.0675)
+    set_elastic(mat, 2.6e11, 1.1e11, 0.8e11, 6000.0, Tg2c=T)
+    prob = Problem(shape=shape, spacing=(1e-9,) * 3, materials=[mat], material_cell=np.ones(shape, np.uint32),
+                   dt=1e-15, periodic=(True, True, True), Tg2c=T,
+                   flags=FLAG_ELASTODYNAMICS | FLAG_ELASTO_BACKACTION | FLAG_ORIENT_ROTATE)
+    o = oracle_lib.Oracle(prob)
+    rng = np.random.default_rng(9)
+    if kind == "fm":
+        a = workloads.tilted_m(shape, 3, theta_max=2.5)
+        o.set(FIELD_M, a)
+        parts = [(a, 1.0, mat.lamda100, mat.lamda111)]
+    elif kind == "afm":
+        a1 = workloads.tilted_m(shape, 4, theta_max=2.5)
+        a2 = workloads.tilted_m(shape, 5, theta_max=2.5)
+        o.set(FIELD_M_AFM1, a1)
+        o.set(FIELD_M_AFM2, a2)
+        parts = [(a1, 0.5, mat.lamda100_AFM1, mat.lamda111_AFM1), (a2, 0.5, mat.lamda100_AFM2, mat.lamda111_AFM2)]
+    else:
+        p = 0.4 * rng.standard_normal((3,) + shape)
+        o.set(FIELD_P, p)
+    o.L.gfo_time_advance(o._h)
+    o.L.gfo_elastic_get(o._h, 1)
+    got = np.concatenate([o.get(E0D), o.get(E0O)])
+    o.close()
+    want = np.zeros((6,) + shape)
+    for idx in np.ndindex(*shape):
+        ec = np.zeros((3, 3))
+        if kind == "fe":
+            pc = T @ p[(slice(None),) + idx]
+            for i in range(3):
+                j, k = (i + 1) % 3, (i + 2) % 3
+                ec[i, i] = mat.Q11 * pc[i] ** 2 + mat.Q12 * (pc[j] ** 2 + pc[k] ** 2)
+            for i, j in ((1, 2), (0, 2), (0, 1)):
+                ec[i, j] = ec[j, i] = mat.Q44 * pc[i] * pc[j]
+        else:
+            for vec, w, l100, l111 in parts:
+                ac = T @ vec[(slice(None),) + idx]
+                for i in range(3):
+                    ec[i, i] += w * 1.5 * l100 * (ac[i] ** 2 - 1. / 3.)
+                for i, j in ((1, 2), (0, 2), (0, 1)):
+                    ec[i, j] += w * 1.5 * l111 * ac[i] * ac[j]
+                    ec[j, i] = ec[i, j]
+        eg = T.T @ ec @ T                                 # crystal -> global
+        want[(slice(None),) + idx] = [eg[0, 0], eg[1, 1], eg[2, 2], eg[1, 2], eg[0, 2], eg[0, 1]]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_ampere_interface_means(oracle_lib):
+    shape = (2, 4, 8)
+    cond = Material.make(r_permittivity=3.0, conductivity=2e5, comp_n1=1.0, tao_e_n1=1.0, tao_e_n2=1.0, tao_e_n3=1.0)
+    diel = Material.make(r_permittivity=7.0, conductivity=5e4, comp_n1=1.0, tao_e_n1=1.0, tao_e_n2=1.0, tao_e_n3=1.0)
+    cell = np.zeros(shape, np.uint32)
+    cell[:, :, 0:3] = 1
+    cell[:, :, 3:5] = 2
+    cell[:, 1, :] = np.where(cell[:, 1, :] == 1, 2, cell[:, 1, :])     # an interface along y too
+    prob = Problem(shape=shape, spacing=(1e-9, 1.2e-9, 0.8e-9), materials=[cond, diel], material_cell=cell,
+                   dt=1e-21, periodic=(True, True, True), flags=FLAG_EMDYNAMIC)
+    E0 = 3e5
+    nx, ny, nz = shape
+    Ex = np.full((nx, ny + 1, nz + 1), E0)
+    Ey = np.zeros((nx + 1, ny, nz + 1))
+    Ez = np.zeros((nx + 1, ny + 1, nz))
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_E_EM, np.concatenate([Ex.ravel(), Ey.ravel(), Ez.ravel()]))
+    o.step(1)
+    got = split_yee(o.get(FIELD_E_EM), shape, FIELD_E_EM)[0]
+    o.close()
+    er = {0: 1.0, 1: 3.0, 2: 7.0}
+    sg = {0: 0.0, 1: 2e5, 2: 5e4}
+    want = np.empty((ny + 1, nz + 1))
+    rates = np.empty((ny + 1, nz + 1))
+    for j in range(ny + 1):
+        for k in range(nz + 1):
+            y1, y3, z1, z2 = (j - 1) % ny, j % ny, (k - 1) % nz, k % nz
+            cells = [cell[0, y1, z1], cell[0, y1, z2], cell[0, y3, z1], cell[0, y3, z2]]
+            rates[j, k] = np.mean([sg[int(c)] for c in cells]) / (EPS0 * np.mean([er[int(c)] for c in cells]))
+            want[j, k] = E0 * np.exp(-rates[j, k] * prob.dt)
+    assert len(np.unique(np.round(rates / rates.max(), 9))) >= 5       # many distinct means checked
+    assert np.abs(got[0] - want).max() <= 1e-6 * E0 * rates.max() * prob.dt
