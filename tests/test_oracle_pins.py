@@ -126,3 +126,43 @@ def test_ampere_interface_means(oracle_lib):
             want[j, k] = E0 * np.exp(-rates[j, k] * prob.dt)
     assert len(np.unique(np.round(rates / rates.max(), 9))) >= 5       # many distinct means checked
     assert np.abs(got[0] - want).max() <= 1e-6 * E0 * rates.max() * prob.dt
+
+
+def test_interface_exchange_is_the_harmonic_mean(oracle_lib):
+    """magnetic_initialize.cpp:381-524: across a face between two
+    ferromagnets the exchange stiffness is 2 A1 A2 / (A1 + A2); the
+    exchange field of cell i is 2 / (mu0 Ms_i) sum_faces A_f (m_nb - m_i) /
+    h^2, and the stage-1 increment is the LLG step
+    -gyro dt / (1 + a^2) [m x H + a m x (m x H)] (textbook forms)."""
+    MU0 = 1.256637061e-6
+    shape = (8, 1, 1)
+    A = {1: 3e-12, 2: 1.2e-11}
+    MS = {1: 1.4e5, 2: 8e5}
+    AL = {1: 0.02, 2: 0.3}
+    mats = [Material.make(if_FM=1, anisotropy_type=1, K1=0.0, Ms=MS[t], gyro=GYRO, FM_damping=AL[t], Aex=A[t])
+            for t in (1, 2)]
+    cell = np.array([1, 1, 1, 1, 2, 2, 2, 2], np.uint32).reshape(shape)
+    h = 2e-9
+    prob = Problem(shape=shape, spacing=(h, 2e-9, 2e-9), materials=mats, material_cell=cell, dt=1e-14,
+                   periodic=(True, True, True))
+    m = workloads.tilted_m(shape, 6, theta_max=1.5)
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_M, m)
+    o.L.gfo_time_advance(o._h)
+    o.L.gfo_mag_get_dm(o._h, 1)
+    k1 = o.get(103)
+    o.close()
+    want = np.zeros_like(m)
+    for i in range(8):
+        ti = int(cell[i, 0, 0])
+        H = np.zeros(3)
+        for nb in ((i - 1) % 8, (i + 1) % 8):
+            tn = int(cell[nb, 0, 0])
+            Af = 2 * A[ti] * A[tn] / (A[ti] + A[tn])
+            H += Af * (m[:, nb, 0, 0] - m[:, i, 0, 0]) / h ** 2
+        H *= 2 / (MU0 * MS[ti])
+        mi = m[:, i, 0, 0]
+        pre = -GYRO / (1 + AL[ti] ** 2) * prob.dt
+        mxh = np.cross(mi, H)
+        want[:, i, 0, 0] = pre * mxh + AL[ti] * pre * np.cross(mi, mxh)
+    np.testing.assert_allclose(k1, want, rtol=0, atol=1e-12 * np.abs(want).max())
