@@ -166,3 +166,38 @@ def test_interface_exchange_is_the_harmonic_mean(oracle_lib):
         mxh = np.cross(mi, H)
         want[:, i, 0, 0] = pre * mxh + AL[ti] * pre * np.cross(mi, mxh)
     np.testing.assert_allclose(k1, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_gaussian_displacement_source_emits_its_amplitude(oracle_lib):
+    """The type-1 source (elastic_dynamic_solver.cpp:3211-3232) holds the
+    source plane at u_x = A sqrt(e) sigma v_t exp(-(t - 5 sigma)^2 / (2
+    sigma^2)), v_t = sqrt(c44 / rho): the plane follows it after every
+    step, and since its slowness-scaled rate peaks at A, the shear wave it
+    launches into the column carries du_x/dz of peak A (2 e_xz = A),
+    the textbook transmitted-strain amplitude."""
+    from goferro import FIELD_STRAIN_OFF, FIELD_U, FLAG_GAUSSIAN_PULSE
+    c11, c12, c44, rho = 2.69e11, 1.08e11, 0.76e11, 5170.0
+    mat = set_elastic(Material.make(), c11, c12, c44, rho)
+    nz = 256
+    shape = (1, 1, nz)
+    src = nz - 1
+    sigma, amp = 5e-13, 1e-4
+    prob = Problem(shape=shape, spacing=(1e-9, 1e-9, 0.25e-9), materials=[mat], material_cell=np.ones(shape, np.uint32),
+                   dt=1e-14, periodic=(True, True, False), flags=FLAG_ELASTODYNAMICS | FLAG_GAUSSIAN_PULSE,
+                   input_strain_type=1, input_strain_sourcez=src, input_strain_component="x", sigma_gauss=sigma,
+                   amplitude_gauss=amp, weighting_factor_fourth_Liao=0.5)
+    vt = np.sqrt(c44 / rho)
+    o = oracle_lib.Oracle(prob)
+    peak = 0.0
+    for step in range(1, 1301):
+        o.step(1)
+        t = step * prob.dt
+        if step % 100 == 0 and t < 12.5 * sigma:
+            ux = o.get(FIELD_U)[0, 0, 0, src]
+            want = amp * 1.6487210 * sigma * vt * np.exp(-(t - 5 * sigma) ** 2 / (2 * sigma ** 2))
+            assert ux == pytest.approx(want, rel=1e-12, abs=1e-30), (step, ux, want)
+        if step in (1000, 1100, 1200, 1300):
+            exz = o.get(FIELD_STRAIN_OFF)[1, 0, 0, : src - 40]          # the pulse away from the source
+            peak = max(peak, 2 * np.abs(exz).max())
+    o.close()
+    assert peak == pytest.approx(amp, rel=0.03)
