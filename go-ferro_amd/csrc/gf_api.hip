@@ -2137,6 +2137,8 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
     if (h.n_materials != static_cast<int32_t>(c->host_mats.size()) ||
         h.material_bytes != sizeof(gf_material) * c->host_mats.size())
         return set_error(c, GF_ERR_ARG, "the state image has another material table");
+    if (bytes < sizeof(h) + h.material_bytes + sizeof(uint64_t))
+        return set_error(c, GF_ERR_ARG, "the state image is truncated");
     if (h.hist_slots != c->hist_slots || h.hist_head < 0 || (c->hist_slots > 0 && h.hist_head >= c->hist_slots))
         return set_error(c, GF_ERR_ARG, "the state image has another boundary history");
     auto sel_ok = [](int32_t s) { return s >= 0 && s <= 2; };
