@@ -201,3 +201,69 @@ def test_gaussian_displacement_source_emits_its_amplitude(oracle_lib):
             peak = max(peak, 2 * np.abs(exz).max())
     o.close()
     assert peak == pytest.approx(amp, rel=0.03)
+
+
+def shear_column(src_type, sigma, amp, nz=256, cycles=0):
+    from goferro import FLAG_GAUSSIAN_PULSE
+    c11, c12, c44, rho = 2.69e11, 1.08e11, 0.76e11, 5170.0
+    mat = set_elastic(Material.make(), c11, c12, c44, rho)
+    shape = (1, 1, nz)
+    prob = Problem(shape=shape, spacing=(1e-9, 1e-9, 0.25e-9), materials=[mat], material_cell=np.ones(shape, np.uint32),
+                   dt=1e-14, periodic=(True, True, False), flags=FLAG_ELASTODYNAMICS | FLAG_GAUSSIAN_PULSE,
+                   input_strain_type=src_type, input_strain_sourcez=nz - 1, input_strain_component="x",
+                   sigma_gauss=sigma, amplitude_gauss=amp, num_strain_cycles=cycles, weighting_factor_fourth_Liao=0.5)
+    return prob, c44, np.sqrt(c44 / rho)
+
+
+def test_sinusoidal_displacement_source_emits_its_amplitude(oracle_lib):
+    """The type-2 source (elastic_dynamic_solver.cpp:3211-3232; sigma_gauss
+    is its frequency f) holds the source plane at u_x = A v_t / (2 pi f)
+    (cos(2 pi f t) - 1) for num_cycles periods and at 0 after them: the
+    plane follows that after every step, and since its slowness-scaled rate
+    -A sin(2 pi f t) peaks at A, the one-period shear wave it launches
+    carries du_x/dz of peak A, the textbook transmitted-strain amplitude."""
+    from goferro import FIELD_STRAIN_OFF, FIELD_U
+    f, amp = 1e11, 1e-4
+    prob, _, vt = shear_column(2, f, amp, cycles=1)
+    src = prob.shape[2] - 1
+    o = oracle_lib.Oracle(prob)
+    peak = 0.0
+    for step in range(1, 1301):
+        o.step(1)
+        t = step * prob.dt
+        if step % 50 == 0 and step != 1000:
+            ux = o.get(FIELD_U)[0, 0, 0, src]
+            want = amp * vt / (2 * np.pi * f) * (np.cos(2 * np.pi * f * t) - 1) if t < 1 / f else 0.0
+            assert ux == pytest.approx(want, rel=1e-12, abs=1e-12 * amp * vt / f), (step, ux, want)
+        if step in (1100, 1200, 1300):
+            exz = o.get(FIELD_STRAIN_OFF)[1, 0, 0, : src - 40]
+            peak = max(peak, 2 * np.abs(exz).max())
+    o.close()
+    assert peak == pytest.approx(amp, rel=0.01)
+
+
+def test_gaussian_traction_source_emits_its_stress(oracle_lib):
+    """The type-3 source (elastic_dynamic_solver.cpp:1249-1262) replaces the
+    stress on the stress-free top face of the source plane by a traction
+    sigma_xz = A exp(-(t - 5 sigma)^2 / (2 sigma^2)).  A traction T(t) on a
+    free face launches the shear wave whose stress is T, so the pulse in
+    the column carries du_x/dz = sigma_xz / c44 of peak A / c44 (textbook
+    1D wave equation); with no traction the column stays at rest."""
+    from goferro import FIELD_STRAIN_OFF, FIELD_U
+    sigma = 5e-13
+    prob, c44, _ = shear_column(3, sigma, 1e-4 * 0.76e11)
+    src = prob.shape[2] - 1
+    o = oracle_lib.Oracle(prob)
+    peak = 0.0
+    for step in range(1, 1301):
+        o.step(1)
+        if step in (1000, 1100, 1200, 1300):
+            exz = o.get(FIELD_STRAIN_OFF)[1, 0, 0, : src - 40]
+            peak = max(peak, 2 * np.abs(exz).max())
+    o.close()
+    assert peak == pytest.approx(1e-4, rel=0.01)
+    quiet, _, _ = shear_column(3, sigma, 0.0)
+    o = oracle_lib.Oracle(quiet)
+    o.step(200)
+    assert np.abs(o.get(FIELD_U)).max() == 0.0
+    o.close()
