@@ -267,3 +267,51 @@ def test_gaussian_traction_source_emits_its_stress(oracle_lib):
     o.step(200)
     assert np.abs(o.get(FIELD_U)).max() == 0.0
     o.close()
+
+
+@pytest.mark.parametrize("eta,beta", [(2.5e-13, 0.0), (0.0, 3e10)])
+def test_staged_damping_is_kelvin_voigt(oracle_lib, eta, beta):
+    """The stiffness damping (elastic_dynamic_solver.cpp:1270-1290) adds
+    (eta / rho)(f - F_store) to the velocity increment of RK stages 1 and 4
+    and twice that in stages 2 and 3, F_store being the stage-1 force (of
+    the previous step in stage 1); each stage then approximates
+    (eta / rho) dt df/dt, the Kelvin-Voigt stress C (e + eta de/dt).  With
+    the mass damping -beta v, a standing shear mode u_y = a(t) cos(k x)
+    obeys a'' + (beta + eta w^2) a' + w^2 a = 0, w^2 = c44 k^2 / rho, whose
+    velocity extrema shrink by exp(-g T_d / 2) per half period with
+    g = (beta + eta w^2) / 2 (textbook damped oscillator).  Starting from
+    u = 0 keeps the zero F_store of the first step exact.  The mode is
+    uniform in z inside a block with stress-free top and bottom faces
+    (which it satisfies), under a vacuum plane at z = 0: the Liao plane is
+    written there on every step whatever the periodicity
+    (elastic_dynamic_solver.cpp:2916-3029), from a displacement history
+    that starts at zero."""
+    from goferro import FIELD_V
+    c44, rho = 0.76e11, 5170.0
+    mat = set_elastic(Material.make(), 2.69e11, 1.08e11, c44, rho, mass_damping=beta, stiff_damping=eta)
+    shape = (64, 6, 6)
+    h = 1e-9
+    cell = np.ones(shape, np.uint32)
+    cell[:, :, 0] = 0
+    prob = Problem(shape=shape, spacing=(h, h, h), materials=[mat], material_cell=cell,
+                   dt=2e-14, periodic=(True, True, False), flags=FLAG_ELASTODYNAMICS)
+    k = 2 * np.pi / (shape[0] * h)
+    w2 = c44 * k * k / rho
+    mode = np.cos(k * h * np.arange(shape[0]))
+    v0 = np.zeros((3,) + shape)
+    v0[1] = mode[:, None, None] * cell
+    o = oracle_lib.Oracle(prob)
+    o.set(FIELD_V, v0)
+    b = [1.0]
+    for _ in range(2100):
+        o.step(1)
+        vy = o.get(FIELD_V)[1]
+        assert np.abs(vy[:, :, 1:] - vy[:, :1, 3:4]).max() <= 1e-12
+        b.append(float(vy[:, 0, 3] @ mode / (mode @ mode)))
+    o.close()
+    b = np.abs(np.array(b))
+    peaks = [i for i in range(1, len(b) - 1) if b[i] >= b[i - 1] and b[i] > b[i + 1]]
+    peaks = [0] + peaks
+    g = -np.log(b[peaks[-1]] / b[peaks[0]]) / ((peaks[-1] - peaks[0]) * prob.dt)
+    assert len(peaks) >= 4
+    assert g == pytest.approx((beta + eta * w2) / 2, rel=0.02)
