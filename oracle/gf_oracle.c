@@ -27,6 +27,15 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Loops over cells, edges, faces and FFT lines run on all host cores
+ * (OpenMP) once a grid holds GFO_OMP_MIN cells.  Every parallel loop writes
+ * each element from one iteration only and keeps the per-element operation
+ * order, so the results are bit-identical for any thread count; sums over
+ * cells (averages, loss) stay serial. */
+#ifndef GFO_OMP_MIN
+#define GFO_OMP_MIN 32768
+#endif
+
 static const double MU0 = 1.256637061e-6;          /* magnetic_system.h:14 */
 static const double PI = 3.14159265358979323846;   /* magnetic_system.h:16 */
 static const double UNIT_E = 1.602176634e-19;      /* magnetic_system.h:15 */
@@ -143,8 +152,13 @@ static int is_pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
 void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out) {
     int64_t nzh = nz / 2 + 1;
     int64_t mx = nx > ny ? nx : ny; if (nz > mx) mx = nz;
+    /* lines are independent: each thread transforms whole lines with its
+     * own scratch, so the result does not depend on the thread count */
+    #pragma omp parallel if (nx * ny * nz >= GFO_OMP_MIN)
+    {
     double* re = (double*)malloc(sizeof(double) * mx);
     double* im = (double*)malloc(sizeof(double) * mx);
+    #pragma omp for collapse(2) schedule(static)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j < ny; j++) {
             for (int64_t k = 0; k < nz; k++) { re[k] = in[(i * ny + j) * nz + k]; im[k] = 0.0; }
@@ -154,12 +168,14 @@ void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out
                 out[2 * ((i * ny + j) * nzh + k) + 1] = im[k];
             }
         }
+    #pragma omp for collapse(2) schedule(static)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t k = 0; k < nzh; k++) {
             for (int64_t j = 0; j < ny; j++) { re[j] = out[2 * ((i * ny + j) * nzh + k)]; im[j] = out[2 * ((i * ny + j) * nzh + k) + 1]; }
             fft1(re, im, ny, 0);
             for (int64_t j = 0; j < ny; j++) { out[2 * ((i * ny + j) * nzh + k)] = re[j]; out[2 * ((i * ny + j) * nzh + k) + 1] = im[j]; }
         }
+    #pragma omp for collapse(2) schedule(static)
     for (int64_t j = 0; j < ny; j++)
         for (int64_t k = 0; k < nzh; k++) {
             for (int64_t i = 0; i < nx; i++) { re[i] = out[2 * ((i * ny + j) * nzh + k)]; im[i] = out[2 * ((i * ny + j) * nzh + k) + 1]; }
@@ -167,6 +183,7 @@ void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out
             for (int64_t i = 0; i < nx; i++) { out[2 * ((i * ny + j) * nzh + k)] = re[i]; out[2 * ((i * ny + j) * nzh + k) + 1] = im[i]; }
         }
     free(re); free(im);
+    }
 }
 
 /* cuFFT Z2D semantics (magnetostatic_solver.cpp:183-185): unnormalised
@@ -175,22 +192,27 @@ void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out
 void gfo_irfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out) {
     int64_t nzh = nz / 2 + 1;
     int64_t mx = nx > ny ? nx : ny; if (nz > mx) mx = nz;
-    double* re = (double*)malloc(sizeof(double) * mx);
-    double* im = (double*)malloc(sizeof(double) * mx);
     double* tmp = (double*)malloc(sizeof(double) * 2 * nx * ny * nzh);
     memcpy(tmp, in, sizeof(double) * 2 * nx * ny * nzh);
+    #pragma omp parallel if (nx * ny * nz >= GFO_OMP_MIN)
+    {
+    double* re = (double*)malloc(sizeof(double) * mx);
+    double* im = (double*)malloc(sizeof(double) * mx);
+    #pragma omp for collapse(2) schedule(static)
     for (int64_t j = 0; j < ny; j++)
         for (int64_t k = 0; k < nzh; k++) {
             for (int64_t i = 0; i < nx; i++) { re[i] = tmp[2 * ((i * ny + j) * nzh + k)]; im[i] = tmp[2 * ((i * ny + j) * nzh + k) + 1]; }
             fft1(re, im, nx, 1);
             for (int64_t i = 0; i < nx; i++) { tmp[2 * ((i * ny + j) * nzh + k)] = re[i]; tmp[2 * ((i * ny + j) * nzh + k) + 1] = im[i]; }
         }
+    #pragma omp for collapse(2) schedule(static)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t k = 0; k < nzh; k++) {
             for (int64_t j = 0; j < ny; j++) { re[j] = tmp[2 * ((i * ny + j) * nzh + k)]; im[j] = tmp[2 * ((i * ny + j) * nzh + k) + 1]; }
             fft1(re, im, ny, 1);
             for (int64_t j = 0; j < ny; j++) { tmp[2 * ((i * ny + j) * nzh + k)] = re[j]; tmp[2 * ((i * ny + j) * nzh + k) + 1] = im[j]; }
         }
+    #pragma omp for collapse(2) schedule(static)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j < ny; j++) {
             const double* row = tmp + 2 * ((i * ny + j) * nzh);
@@ -201,7 +223,9 @@ void gfo_irfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* ou
             fft1(re, im, nz, 1);
             for (int64_t k = 0; k < nz; k++) out[(i * ny + j) * nz + k] = re[k];
         }
-    free(tmp); free(re); free(im);
+    free(re); free(im);
+    }
+    free(tmp);
 }
 
 void gfo_rdft3_naive(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out) {
@@ -1022,6 +1046,7 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
     const double drive_s = sin(2. * PI * c->hdrive_freq * drive_t);
     int64_t ny = c->ny, nz = c->nz;
     double** out = c->k[stage - 1];
+    #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < c->n; i++) {
         const gf_material* mat = matof(c, i);
         if (!mat || !mat->if_FM) continue;
@@ -1078,6 +1103,7 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
     }
     if (stage == 4)
         for (int a = 0; a < 3; a++)
+            #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
             for (int64_t i = 0; i < c->n; i++)
                 c->dm[a][i] = c->k[0][a][i] / 6. + c->k[1][a][i] / 3. + c->k[2][a][i] / 3. + c->k[3][a][i] / 6.;
     if (c->spin_on) spin_pump(c, stage);     /* magnetic_solver.cpp:226-228 */
@@ -1104,6 +1130,7 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
         double** other = sub == 0 ? c->ms2 : c->ms1;
         double** A = sub == 0 ? c->Aij1 : c->Aij2;
         double** out = sub == 0 ? c->k1[stage - 1] : c->k2[stage - 1];
+        #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
         for (int64_t i = 0; i < c->n; i++) {
             const gf_material* mat = matof(c, i);
             if (!mat || !mat->if_AFM) continue;
@@ -1154,6 +1181,7 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
     }
     if (stage == 4)
         for (int a = 0; a < 3; a++)
+            #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
             for (int64_t i = 0; i < c->n; i++) {
                 c->dm1[a][i] = c->k1[0][a][i] / 6. + c->k1[1][a][i] / 3. + c->k1[2][a][i] / 3. + c->k1[3][a][i] / 6.;
                 c->dm2[a][i] = c->k2[0][a][i] / 6. + c->k2[1][a][i] / 3. + c->k2[2][a][i] / 3. + c->k2[3][a][i] / 6.;
@@ -1255,6 +1283,7 @@ void gfo_fe_get(gfo_ctx* c, int stage) {
     int64_t ny = c->ny, nz = c->nz;
     double** dq = c->kq[stage - 1];
     double** dp = c->kp[stage - 1];
+    #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t id = 0; id < c->n; id++) {
         int64_t i = id / (ny * nz), j = (id - i * (ny * nz)) / nz, k = id - i * (ny * nz) - j * nz;
         const gf_material* mat = matof(c, id);
@@ -1374,6 +1403,7 @@ void gfo_fe_update(gfo_ctx* c, int stage) {
             const double* k2 = (w == 0 ? c->kq[1] : c->kp[1])[a];
             const double* k3 = (w == 0 ? c->kq[2] : c->kp[2])[a];
             const double* kl = (w == 0 ? c->kq[3] : c->kp[3])[a];
+            #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
             for (int64_t i = 0; i < c->n; i++) {
                 if (stage == 1) s[i] = y[i] + k1[i] * 0.5;
                 else if (stage == 2) s[i] = y[i] + k2[i] * 0.5;
@@ -1437,6 +1467,7 @@ static void mat_rot(const double T[3][3], const double in[3][3], double out[3][3
 
 /* elastic_dynamic_solver.cpp:4-104 (static eigenstrain and flexo terms zero) */
 static void elastic_eigenstrain(gfo_ctx* c) {
+    #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t id = 0; id < c->n; id++) {
         const gf_material* mat = matof(c, id);
         if (!mat) {
@@ -1492,6 +1523,7 @@ static void elastic_eigenstrain(gfo_ctx* c) {
 static void elastic_gradients(gfo_ctx* c) {
     int64_t nx = c->nx, ny = c->ny, nz = c->nz;
     double** s = c->us;
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i <= nx; i++)
         for (int64_t j = 0; j < ny; j++)
             for (int64_t k = 0; k < nz; k++)
@@ -1502,6 +1534,7 @@ static void elastic_gradients(gfo_ctx* c) {
                     else g = (s[a][ID(c, i, j, k)] - s[a][ID(c, i - 1, j, k)]) / c->dx;
                     GYZ(a, i, j, k) = g;
                 }
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j <= ny; j++)
             for (int64_t k = 0; k < nz; k++)
@@ -1512,6 +1545,7 @@ static void elastic_gradients(gfo_ctx* c) {
                     else g = (s[a][ID(c, i, j, k)] - s[a][ID(c, i, j - 1, k)]) / c->dy;
                     GXZ(a, i, j, k) = g;
                 }
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j < ny; j++)
             for (int64_t k = 0; k <= nz; k++)
@@ -1526,6 +1560,7 @@ static void elastic_gradients(gfo_ctx* c) {
 
 /* elastic_dynamic_solver.cpp:190-227 */
 static void elastic_strain(gfo_ctx* c) {
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < c->nx; i++)
         for (int64_t j = 0; j < c->ny; j++)
             for (int64_t k = 0; k < c->nz; k++) {
@@ -1612,6 +1647,7 @@ void gfo_elastic_get(gfo_ctx* c, int stage) {
     double** dv = c->kv[stage - 1];
     double** du = c->ku[stage - 1];
     double ts = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j < ny; j++)
             for (int64_t k = 0; k < nz; k++) {
@@ -1901,6 +1937,7 @@ static int em_setup(gfo_ctx* c, const gf_desc* d) {
  * 1 = E, 2 = H, 3 = both */
 static void em_cells(gfo_ctx* c, int which) {
     int one_d = (c->flags & GF_FLAG_MAG_1D) && (c->flags & GF_FLAG_EM_FROM_M);
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < c->nx; i++)
         for (int64_t j = 0; j < c->ny; j++)
             for (int64_t k = 0; k < c->nz; k++) {
@@ -2010,6 +2047,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
     int px = c->per[0], py = c->per[1], pz = c->per[2];
     double dt = c->dt;
     if (c->flags & GF_FLAG_JF_INPUT) em_jf(c, stage);
+    #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t id = 0; id < c->n; id++) {
         const gf_material* mat = matof(c, id);
         if (!mat) continue;
@@ -2023,6 +2061,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
     double** Hs = c->Hms;
     double** Es = c->Ems;
     /* Ex edges */
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j <= ny; j++)
             for (int64_t k = 0; k <= nz; k++) {
@@ -2037,6 +2076,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
                 c->kE[stage - 1][0][e] = edge_dE(c, 0, cell, dHzdy - dHydz, Es[0][e], stage);
             }
     /* Ey edges */
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i <= nx; i++)
         for (int64_t j = 0; j < ny; j++)
             for (int64_t k = 0; k <= nz; k++) {
@@ -2051,6 +2091,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
                 c->kE[stage - 1][1][e] = edge_dE(c, 1, cell, dHxdz - dHzdx, Es[1][e], stage);
             }
     /* Ez edges */
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i <= nx; i++)
         for (int64_t j = 0; j <= ny; j++)
             for (int64_t k = 0; k < nz; k++) {
@@ -2065,6 +2106,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
                 c->kE[stage - 1][2][e] = edge_dE(c, 2, cell, dHydx - dHxdy, Es[2][e], stage);
             }
     /* H faces (EMdynamic_solver.cpp:2239-2547) */
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i <= nx; i++)
         for (int64_t j = 0; j < ny; j++)
             for (int64_t k = 0; k < nz; k++) {
@@ -2076,6 +2118,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
                 if (cnt < 0.5) cnt = 1.;
                 c->kH[stage - 1][0][HXI(c, i, j, k)] = -dt / MU0 * (dEzdy - dEydz) - (d1 + d2) / cnt;
             }
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j <= ny; j++)
             for (int64_t k = 0; k < nz; k++) {
@@ -2087,6 +2130,7 @@ void gfo_em_get(gfo_ctx* c, int stage) {
                 if (cnt < 0.5) cnt = 1.;
                 c->kH[stage - 1][1][HYI(c, i, j, k)] = -dt / MU0 * (dExdz - dEzdx) - (d1 + d2) / cnt;
             }
+    #pragma omp parallel for collapse(3) schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j < ny; j++)
             for (int64_t k = 0; k <= nz; k++) {
@@ -2200,6 +2244,7 @@ void gfo_em_update(gfo_ctx* c, int stage) {
         double w = stage == 3 ? 1. : 0.5;
         int kind = stage < 3 ? 0 : 1;
         for (int a = 0; a < 3; a++)
+            #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
             for (int64_t e = 0; e < c->nE[a]; e++)
                 c->Ems[a][e] = stage == 3 ? c->Em[a][e] + c->kE[2][a][e] : c->Em[a][e] + c->kE[stage - 1][a][e] * 0.5;
         if (plane) em_plane(c, kind);
@@ -2208,10 +2253,12 @@ void gfo_em_update(gfo_ctx* c, int stage) {
         em_cells(c, 1);
         for (int p = 0; p < 3; p++)
             for (int a = 0; a < 3; a++)
+                #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
                 for (int64_t id = 0; id < c->n; id++)
                     c->Jps[p][a][id] = stage == 3 ? c->Jp[p][a][id] + c->kJ[2][p][a][id]
                                                   : c->Jp[p][a][id] + c->kJ[stage - 1][p][a][id] * 0.5;
         for (int a = 0; a < 3; a++)
+            #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
             for (int64_t f = 0; f < c->nH[a]; f++)
                 c->Hms[a][f] = stage == 3 ? c->Hm[a][f] + c->kH[2][a][f] : c->Hm[a][f] + c->kH[stage - 1][a][f] * 0.5;
         (void)w;
@@ -2221,6 +2268,7 @@ void gfo_em_update(gfo_ctx* c, int stage) {
     if (!c->periodic_all)
         for (int a = 0; a < 3; a++) memcpy(c->Et1[a], c->Em[a], sizeof(double) * (size_t)c->nE[a]);
     for (int a = 0; a < 3; a++)
+        #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
         for (int64_t e = 0; e < c->nE[a]; e++)
             c->Em[a][e] = c->Em[a][e] + c->kE[0][a][e] / 6. + c->kE[1][a][e] / 3. + c->kE[2][a][e] / 3. + c->kE[3][a][e] / 6.;
     if (plane) em_plane(c, 2);
@@ -2230,12 +2278,14 @@ void gfo_em_update(gfo_ctx* c, int stage) {
     em_cells(c, 1);
     for (int p = 0; p < 3; p++)
         for (int a = 0; a < 3; a++)
+            #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
             for (int64_t id = 0; id < c->n; id++) {
                 c->Jp[p][a][id] = c->Jp[p][a][id] + c->kJ[0][p][a][id] / 6. + c->kJ[1][p][a][id] / 3.
                                 + c->kJ[2][p][a][id] / 3. + c->kJ[3][p][a][id] / 6.;
                 c->Jps[p][a][id] = c->Jp[p][a][id];
             }
     for (int a = 0; a < 3; a++)
+        #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
         for (int64_t f = 0; f < c->nH[a]; f++) {
             c->Hm[a][f] = c->Hm[a][f] + c->kH[0][a][f] / 6. + c->kH[1][a][f] / 3. + c->kH[2][a][f] / 3. + c->kH[3][a][f] / 6.;
             c->Hms[a][f] = c->Hm[a][f];
@@ -2276,6 +2326,7 @@ static void demag_3d(gfo_ctx* c) {
         return;
     }
     double* M = dalloc(3 * n);
+    #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
     for (int64_t i = 0; i < n; i++) {
         const gf_material* mat = matof(c, i);
         if (!mat) continue;
@@ -2303,6 +2354,7 @@ static void conv_3d(gfo_ctx* c, const double* M, double* const* out) {
     static const int spec[3][3] = {{0, 3, 8}, {1, 4, 6}, {2, 5, 7}};
     double den = -c->dx * c->dy * c->dz;
     for (int a = 0; a < 3; a++) {
+        #pragma omp parallel for schedule(static) if (c->n >= GFO_OMP_MIN)
         for (int64_t q = 0; q < nh; q++) {
             double sr = 0., si = 0.;
             for (int t = 0; t < 3; t++) {
