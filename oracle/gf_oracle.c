@@ -36,6 +36,25 @@
 #define GFO_OMP_MIN 32768
 #endif
 
+/* GFO_PLANT selects one deliberate mistake for the pin tests, which build
+ * a variant of this file with it to show that their check fails on it:
+ *   1 DMI surface slope with the wrong sign,  2 DMI slope D/A for D/(2A),
+ *   3 Liao reading the history one step too old, 4 Liao coefficients of
+ *   t1 and t2 swapped, 5 Mur half- and full-stage coefficients swapped,
+ *   6 Mur half stages reading the base state for the stage store,
+ *   7 elastic Mur half- and full-stage coefficients swapped.
+ * The default build (0) plants nothing. */
+#ifndef GFO_PLANT
+#define GFO_PLANT 0
+#endif
+#if GFO_PLANT == 1
+#define DMI_SLOPE(D, A) (-(D) / 2. / (A))
+#elif GFO_PLANT == 2
+#define DMI_SLOPE(D, A) ((D) / (A))
+#else
+#define DMI_SLOPE(D, A) ((D) / 2. / (A))
+#endif
+
 static const double MU0 = 1.256637061e-6;          /* magnetic_system.h:14 */
 static const double PI = 3.14159265358979323846;   /* magnetic_system.h:16 */
 static const double UNIT_E = 1.602176634e-19;      /* magnetic_system.h:15 */
@@ -991,14 +1010,14 @@ static void stencil(const gfo_ctx* c, double* const* s, const unsigned char* mYZ
     int64_t nb;
     /* x */
     if (sYZ(mYZ, c, i, j, k)) {
-        if (dmi) { bw[0] = -D / 2. / A * mz_c; bw[1] = 0.; bw[2] = D / 2. / A * mx_c; }
+        if (dmi) { bw[0] = -DMI_SLOPE(D, A) * mz_c; bw[1] = 0.; bw[2] = DMI_SLOPE(D, A) * mx_c; }
         else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
     } else {
         nb = ID(c, i == 0 ? nx - 1 : i - 1, j, k);
         bw[0] = (mx_c - s[0][nb]) / dx; bw[1] = (my_c - s[1][nb]) / dx; bw[2] = (mz_c - s[2][nb]) / dx;
     }
     if (sYZ(mYZ, c, i + 1, j, k)) {
-        if (dmi) { fw[0] = -D / 2. / A * mz_c; fw[1] = 0.; fw[2] = D / 2. / A * mx_c; }
+        if (dmi) { fw[0] = -DMI_SLOPE(D, A) * mz_c; fw[1] = 0.; fw[2] = DMI_SLOPE(D, A) * mx_c; }
         else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
     } else {
         nb = ID(c, i == nx - 1 ? 0 : i + 1, j, k);
@@ -1008,14 +1027,14 @@ static void stencil(const gfo_ctx* c, double* const* s, const unsigned char* mYZ
     for (int a = 0; a < 3; a++) { d2[0][0][a] = fw[a] / dx; d2[0][1][a] = -bw[a] / dx; }
     /* y */
     if (sXZ(mXZ, c, i, j, k)) {
-        if (dmi) { bw[0] = 0.; bw[1] = -D / 2. / A * mz_c; bw[2] = D / 2. / A * my_c; }
+        if (dmi) { bw[0] = 0.; bw[1] = -DMI_SLOPE(D, A) * mz_c; bw[2] = DMI_SLOPE(D, A) * my_c; }
         else { bw[0] = 0.; bw[1] = 0.; bw[2] = 0.; }
     } else {
         nb = ID(c, i, j == 0 ? ny - 1 : j - 1, k);
         bw[0] = (mx_c - s[0][nb]) / dy; bw[1] = (my_c - s[1][nb]) / dy; bw[2] = (mz_c - s[2][nb]) / dy;
     }
     if (sXZ(mXZ, c, i, j + 1, k)) {
-        if (dmi) { fw[0] = 0.; fw[1] = -D / 2. / A * mz_c; fw[2] = D / 2. / A * my_c; }
+        if (dmi) { fw[0] = 0.; fw[1] = -DMI_SLOPE(D, A) * mz_c; fw[2] = DMI_SLOPE(D, A) * my_c; }
         else { fw[0] = 0.; fw[1] = 0.; fw[2] = 0.; }
     } else {
         nb = ID(c, i, j == ny - 1 ? 0 : j + 1, k);
@@ -1735,14 +1754,28 @@ static void elastic_boundary(gfo_ctx* c, int kind) {
         for (int64_t j = 0; j < c->ny; j++) {
             if (!(c->flags & GF_FLAG_1D_ABC)) {
                 double w = c->w_liao;
+#if GFO_PLANT == 3
+                double **h1 = c->t2, **h2 = c->t3, **h3 = c->t4, **h4 = c->t4;
+#else
+                double **h1 = c->t1, **h2 = c->t2, **h3 = c->t3, **h4 = c->t4;
+#endif
+#if GFO_PLANT == 4
+                double c1 = 6. * w + (1. - w), c2 = 4. * w + 2. * (1. - w);
+#else
+                double c1 = 4. * w + 2. * (1. - w), c2 = 6. * w + (1. - w);
+#endif
                 for (int a = 0; a < 3; a++)
-                    c->u[a][ID(c, i, j, 0)] = (4. * w + 2. * (1. - w)) * c->t1[a][ID(c, i, j, 1)]
-                        - (6. * w + (1. - w)) * c->t2[a][ID(c, i, j, 2)]
-                        + (4. * w) * c->t3[a][ID(c, i, j, 3)]
-                        - w * c->t4[a][ID(c, i, j, 4)];
+                    c->u[a][ID(c, i, j, 0)] = c1 * h1[a][ID(c, i, j, 1)]
+                        - c2 * h2[a][ID(c, i, j, 2)]
+                        + (4. * w) * h3[a][ID(c, i, j, 3)]
+                        - w * h4[a][ID(c, i, j, 4)];
                 continue;
             }
+#if GFO_PLANT == 7
+            double num = kind == 0 ? dz : 2. * dz;
+#else
             double num = kind == 0 ? 2. * dz : dz;
+#endif
             if (!(c->flags & GF_FLAG_1D_ABC_ELAST_ONLYTOP)) {
                 double cl = (num - c->vs_long_1st * dt) / (num + c->vs_long_1st * dt);
                 double ct = (num - c->vs_tran_1st * dt) / (num + c->vs_tran_1st * dt);
@@ -2164,12 +2197,18 @@ static void em_plane(gfo_ctx* c, int kind) {
  * (from the history t1); x walls, then y walls, then z walls */
 static void em_mur(double* dst, const double* b, const double* s, const double* t1, int kind, double cf,
                    int64_t wall, int64_t inner) {
+#if GFO_PLANT == 6
+    s = b;
+#endif
     if (kind < 2) dst[wall] = b[inner] + cf * (b[wall] - s[inner]);
     else dst[wall] = t1[inner] + cf * (t1[wall] - b[inner]);
 }
 
 static void em_boundary(gfo_ctx* c, int kind) {
     int64_t nx = c->nx, ny = c->ny, nz = c->nz;
+#if GFO_PLANT == 5
+    kind = kind == 0 ? 1 : (kind == 1 ? 0 : kind);
+#endif
     double cx = kind == 0 ? c->cf_x_half : c->cf_x, cy = kind == 0 ? c->cf_y_half : c->cf_y;
     double ct = kind == 0 ? c->cf_top_half : c->cf_top, cb = kind == 0 ? c->cf_bot_half : c->cf_bot;
     int pec_xy = (c->flags & GF_FLAG_PEC_XY) != 0, pec_z = (c->flags & GF_FLAG_PEC_Z) != 0;

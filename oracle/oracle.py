@@ -19,7 +19,7 @@ FIELD_Q_STORE = 101   # oracle-only: ferroelectric q store
 FIELD_DV1 = 102       # oracle-only: elastic velocity increment of stage 1
 FIELD_K1 = 103        # oracle-only: FM increment of the last stage-1 get_dm
 
-_lib = None
+_libs = {}
 
 
 def build() -> Path:
@@ -28,13 +28,24 @@ def build() -> Path:
     return LIB
 
 
-def lib() -> ctypes.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not LIB.exists():
+def build_planted(plant: int, outdir) -> Path:
+    """Compile a variant of the oracle with the deliberate mistake number
+    `plant` (GFO_PLANT in gf_oracle.c) into outdir, for the tests that show
+    a pin fails on it.  Same flags as the Makefile."""
+    out = Path(outdir) / f"liboracle_plant{int(plant)}.so"
+    subprocess.run(["gcc", "-O2", "-std=c99", "-fPIC", "-fopenmp", "-ffp-contract=off", f"-DGFO_PLANT={int(plant)}",
+                    "-shared", "-o", str(out), str(HERE / "gf_oracle.c"), f"-I{HERE.parent / 'include'}", "-lm"],
+                   check=True)
+    return out
+
+
+def lib(path=None) -> ctypes.CDLL:
+    path = str(LIB if path is None else path)
+    if path in _libs:
+        return _libs[path]
+    if path == str(LIB) and not LIB.exists():
         build()
-    L = ctypes.CDLL(str(LIB))
+    L = ctypes.CDLL(path)
     P = ctypes.c_void_p
     dp = ctypes.POINTER(ctypes.c_double)
     i64 = ctypes.c_int64
@@ -65,7 +76,7 @@ def lib() -> ctypes.CDLL:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
-    _lib = L
+    _libs[path] = L
     return L
 
 
@@ -76,8 +87,8 @@ def _dp(a):
 class Oracle:
     """The reference's protocol on the CPU for one Problem."""
 
-    def __init__(self, problem):
-        self.L = lib()
+    def __init__(self, problem, lib_path=None):
+        self.L = lib(lib_path)
         self.problem = problem
         self.shape = tuple(int(v) for v in problem.shape)
         self.n = int(np.prod(self.shape))

@@ -6,10 +6,9 @@ pulse advances by the exact RK4 propagator of that discrete operator, the
 reference's mass damping acts on the base velocity (v_N = v_0 (1 - eta
 dt)^N), and the cell strain is the average of face gradients.
 
-Not pinned here: the Liao boundary (elastodynamics_AbsorbingBoundary.cpp:
-92-230) extrapolates from grid planes as if c dt = dz; at the Courant
-numbers of the workloads it reflects rather than absorbs, so no closed
-form applies and its parity rests on the GPU-vs-oracle tests.
+The Liao boundary (elastodynamics_AbsorbingBoundary.cpp:92-230) is pinned
+in test_oracle_boundary_pins.py: it reproduces displacement histories that
+are cubic (weight 1) or linear (any weight) along its characteristic.
 
 The reference applies the z = 0 absorbing boundary whatever the
 periodicity (update_Du_Boundary* run in every update_u*), so waves that

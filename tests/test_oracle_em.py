@@ -14,9 +14,9 @@ EMdynamic_AbsorbingBoundary.cpp:3-874) by closed forms:
   below 3 %), a PEC wall reflects it with the tangential E held at zero;
 - a current sheet K = Jf dz radiates plane waves of amplitude eta0 K / 2.
 
-Not pinned by a closed form: the four-cell averaging at material
-interfaces and the half/full-stage Mur coefficients beyond the absorption
-bound; their parity rests on the GPU-vs-oracle tests.
+The four-cell averaging at material interfaces is pinned in
+test_oracle_pins.py; the half/full-stage Mur coefficients are pinned in
+test_oracle_boundary_pins.py (exact for outgoing linear waves).
 """
 import numpy as np
 import pytest
