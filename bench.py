@@ -28,17 +28,21 @@ METRIC = "coupled RK4 cell-steps/s + % HBM roofline; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(cls: str, n: int, nh: int, fused: bool = False) -> float:
+def kernel_bytes(cls: str, n: int, nh: int) -> float:
     """Algorithmic bytes of one launch of a kernel class (each input read
     once, each output written once).  n cells, nh = nx*ny*(nz/2+1) complex
-    points per component.  With the z transforms fused into the magnetic
-    stage, that stage reads the H z-spectrum instead of the real field and
-    writes the M z-spectrum."""
+    points per component.
+
+    The magnetic stage counts the stage share of the fused-RK4 definition:
+    per cell 102 B of state (136 B per state component and step over four
+    stages, three components), 24 B of magnetostatic field and 4 B of
+    material id, 130 B in all.  With the z transforms fused into it, the
+    stage reads the H z-spectrum instead of the real field and writes the M
+    z-spectrum; those are FFT intermediates, outside the definition, and
+    stage_bytes_with_spectra() states them separately."""
     c16 = 16.0
     if cls == "mag_stage":
-        if fused:
-            return 106.0 * n + 6 * nh * c16   # 130 - 24 (real H) per cell, + H and M z-spectra
-        return 130.0 * n          # average over the four stages, see bytes table
+        return 130.0 * n
     if cls == "fft_z_fwd":
         return 28.0 * n + 3 * nh * c16
     if cls in ("fft_y_fwd", "fft_y_inv"):
@@ -49,6 +53,13 @@ def kernel_bytes(cls: str, n: int, nh: int, fused: bool = False) -> float:
     if cls == "fft_z_inv":
         return 3 * nh * c16 + 24.0 * n
     return 0.0
+
+
+def stage_bytes_with_spectra(n: int, nh: int) -> float:
+    """Bytes the fused magnetic stage moves if its z-spectrum input and
+    output are counted once each: 106 B per cell of state and id plus the
+    H z-spectrum read and the M z-spectrum written (3 components each)."""
+    return 106.0 * n + 6 * nh * 16.0
 
 
 def measured_traffic(kernel_cls: str):
@@ -77,32 +88,49 @@ def b_alg_fm(shape) -> float:
     return 136 * 3 + 4 * 24 + 4 * 4 + 4 * conv
 
 
-def cpu_baseline(seconds_hint: float = 15.0) -> dict:
-    """The oracle (single-threaded C port of the reference algorithm) on a
-    bounded sample of the same physics."""
-    import numpy as np  # noqa: F401
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(shape, seconds_hint: float = 12.0, max_steps: int = 8) -> dict:
+    """The oracle (the C port of the reference algorithm, its cell and
+    FFT-line loops on all host cores with OpenMP) on the bench workload
+    itself: the same grid and physics, a few RK4 steps after one untimed
+    step.  Returns the rate with the thread count, nproc and CPU model."""
     from goferro import FIELD_M, workloads
-    from oracle.oracle import Oracle
+    from oracle.oracle import Oracle, threads
 
-    shape = (128, 128, 32)
     prob, m = workloads.fm_film(shape)
+    t_set = time.perf_counter()
     o = Oracle(prob)
     o.set(FIELD_M, m)
     o.demag()
     o.step(1)
+    t_set = time.perf_counter() - t_set
     steps = 0
     t0 = time.perf_counter()
     while True:
         o.step(1)
         steps += 1
         el = time.perf_counter() - t0
-        if el > seconds_hint or steps >= 50:
+        if el > seconds_hint or steps >= max_steps:
             break
+    o.close()
     n = shape[0] * shape[1] * shape[2]
-    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FM LLG + FFT magnetostatics {shape[0]}x{shape[1]}x{shape[2]}, {steps} RK4 steps, "
-                      f"{el:.1f} s, oracle/gf_oracle.c single thread"}
+    nthr = threads()
+    model = cpu_model()
+    print(f"bench: cpu baseline {nthr} threads (nproc {os.cpu_count()}, {model}): {steps} steps in {el:.1f} s "
+          f"(+{t_set:.1f} s set-up and one untimed step)", file=sys.stderr, flush=True)
+    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": nthr, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": model,
+            "sample": f"FM LLG + FFT magnetostatics {shape[0]}x{shape[1]}x{shape[2]} (the bench workload), "
+                      f"{steps} RK4 steps in {el:.1f} s, oracle/gf_oracle.c with OpenMP on {nthr} threads"}
 
 
 def bench_batch(count: int, steps_batch: int, steps_solo: int) -> dict:
@@ -236,6 +264,9 @@ def bench_adjoint(shape, nsteps: int) -> dict:
                         f"probe 16x16x4 spectral power at 30 GHz, {nsteps} steps, sqrt(N) checkpoints",
             "dtype": "f64", "nsteps": nsteps, "forward_s": t_plain, "checkpointed_forward_s": t_fwd,
             "reverse_s": t_bwd, "adjoint_over_forward": t_bwd / t_plain,
+            "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
+            "ratio_note": "adjoint_over_forward = reverse sweep / plain forward; gradient_over_forward = "
+                          "(checkpointed forward + reverse sweep) / plain forward, the whole gradient's cost",
             "forward_cell_steps_per_s": n * nsteps / t_plain, "loss": L,
             "grad": {"alpha": grad[0], "Aex": grad[1], "B1": grad[2]},
             "grad_drive_norm": float((ghd ** 2).sum() ** 0.5), "fp32": fp32}
@@ -391,7 +422,7 @@ def main() -> int:
         fused = kt.get("fft_z_fwd", (0.0, 0))[1] == 0
         for cls, (ms, cnt) in kt.items():
             if cnt:
-                b = kernel_bytes(cls, n, nh, fused)
+                b = kernel_bytes(cls, n, nh)
                 per[cls] = {"launches": cnt, "avg_us": 1e3 * ms / cnt,
                             "share": ms / max(1e-12, sum(v[0] for v in kt.values())),
                             "achieved_GBs": (b / (ms / cnt * 1e-3)) / 1e9 if b else None}
@@ -400,7 +431,14 @@ def main() -> int:
         traffic = measured_traffic(dom) if shape == (256, 256, 128) else None
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                           "bytes_per_launch": kernel_bytes(dom, n, nh, fused), "avg_launch_us": d["avg_us"]}
+                           "bytes_per_launch": kernel_bytes(dom, n, nh), "avg_launch_us": d["avg_us"]}
+        if dom == "mag_stage" and fused:
+            bs = stage_bytes_with_spectra(n, nh)
+            out["roofline"]["with_z_spectra"] = {
+                "bytes_per_launch": bs, "achieved": bs / (d["avg_us"] * 1e-6) / 1e9,
+                "frac": bs / (d["avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                "note": "the fused stage also reads the H z-spectrum and writes the M z-spectrum "
+                        "(FFT intermediates, not in the algorithmic bytes)"}
         out["kernels"] = per
     ctx.close()
     if args.copy_probe_mib:
@@ -413,7 +451,7 @@ def main() -> int:
         out["magnetoelastic"] = bench_coupled("magnetoelastic", args.coupled_steps)
         out["fe_thz"] = bench_coupled("fe_thz", 10 * args.coupled_steps)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(shape)
     print(json.dumps(out), flush=True)
     return 0
 

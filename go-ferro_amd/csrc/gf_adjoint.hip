@@ -312,6 +312,12 @@ gf_status check_context(gf_ctx* c, int material) {
     if (c->flags & bad)
         return set_error(c, GF_ERR_UNSUPPORTED,
                          "the adjoint sweep supports FFT or no magnetostatics and sinusoidal drives only");
+    // gf_step recomputes the static strain from m after every step; the
+    // sweep's forward would keep it fixed and its gradient would miss
+    // d(strain)/dm, so a context that solves the elastostatics is refused
+    if (c->flags & GF_FLAG_ELASTOSTATIC)
+        return set_error(c, GF_ERR_UNSUPPORTED,
+                         "the adjoint sweep does not support the elastostatic solver (static strain from m)");
     if (material < 1 || material > static_cast<int>(c->host_mats.size()) || !c->host_mats[material - 1].if_FM)
         return set_error(c, GF_ERR_ARG, "material must name an FM material");
     int fm_mats = 0;

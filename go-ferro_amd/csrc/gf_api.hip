@@ -424,6 +424,42 @@ gf_status spin_setup(gf_ctx* c, const gf_desc* d) {
     if (!st && c->any_fm && !c->em_dm[0]) st = alloc3(c, c->em_dm, "hipMalloc dm");
     if (!st && c->any_afm && !c->em_dm1[0]) st = alloc3(c, c->em_dm1, "hipMalloc dm");
     if (!st && c->any_afm && !c->em_dm2[0]) st = alloc3(c, c->em_dm2, "hipMalloc dm");
+    // the step-boundary current reads the source planes of the state before
+    // the stage-4 update: the FM one only without an AFM (with both magnets
+    // the AFM update's call sees the FM state already updated)
+    std::vector<int32_t> is_src(static_cast<size_t>(nz), 0);
+    for (int64_t k = 0; k < nz; k++)
+        if (src[static_cast<size_t>(k)] >= 0) is_src[static_cast<size_t>(src[static_cast<size_t>(k)])] = 1;
+    if (!st) st = hip_check(c, hipMalloc(&c->spin_is_src, sizeof(int32_t) * is_src.size()), "hipMalloc spin planes");
+    if (!st)
+        st = hip_check(c, hipMemcpy(c->spin_is_src, is_src.data(), sizeof(int32_t) * is_src.size(),
+                                    hipMemcpyHostToDevice), "copy spin planes");
+    if (!st && c->any_fm && !c->any_afm) st = alloc3(c, c->spin_m0, "hipMalloc spin state");
+    if (!st && c->any_afm) st = alloc3(c, c->spin_m10, "hipMalloc spin state");
+    if (!st && c->any_afm) st = alloc3(c, c->spin_m20, "hipMalloc spin state");
+    return st;
+}
+
+// The source planes of a magnet's state before its stage-4 kernel updates
+// it in place (FM: which = 0, AFM: which = 1).
+gf_status spin_save(gf_ctx* c, int which) {
+    SpinSaveArgs a{};
+    if (which == 0) {
+        for (int i = 0; i < 3; i++) { a.src[i] = c->m[i]; a.dst[i] = c->spin_m0[i]; }
+        a.ncomp = 3;
+    } else {
+        for (int i = 0; i < 3; i++) {
+            a.src[i] = c->m1[i]; a.dst[i] = c->spin_m10[i];
+            a.src[3 + i] = c->m2[i]; a.dst[3 + i] = c->spin_m20[i];
+        }
+        a.ncomp = 6;
+    }
+    a.is_src = c->spin_is_src;
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_OTHER, &ev);
+    gf_status st = hip_check(c, launch_spin_save(a, c->stream), "spin pumping");
+    timer_end(c, GF_K_OTHER, ev);
     return st;
 }
 
@@ -440,6 +476,37 @@ gf_status spin_pump(gf_ctx* c) {
         a.d1[i] = s1 ? c->em_dm1[i] : nullptr;
         a.s2[i] = s2 ? s2[i] : nullptr;
         a.d2[i] = s2 ? c->em_dm2[i] : nullptr;
+    }
+    a.src = c->spin_src;
+    a.coef = c->spin_coef;
+    a.j[0] = c->jis[0];
+    a.j[1] = c->jis[1];
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dt = c->dt;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_OTHER, &ev);
+    gf_status st = hip_check(c, launch_spin_pump(a, c->stream), "spin pumping");
+    timer_end(c, GF_K_OTHER, ev);
+    return st;
+}
+
+// get_J_ISHE at the step boundary (magnetic_solver.cpp:1460-1511, called by
+// update_m and update_m_AFM before m += dm, :1593-1597 and
+// magnetic_AFM_solver.cpp:1801-1803), after every stage-4 call that reads
+// the stage-4 current: the states before the update (the saved source
+// planes; the updated FM state when an AFM is present, whose call comes
+// last) and the combined increments acc + k4/6.
+gf_status spin_pump_step(gf_ctx* c) {
+    SpinArgs a{};
+    for (int i = 0; i < 3; i++) {
+        if (c->any_fm) {
+            a.fs[i] = c->any_afm ? c->m[i] : c->spin_m0[i];
+            a.fd[i] = c->em_dm[i];
+            a.fa[i] = c->acc[i];
+        }
+        if (c->any_afm) {
+            a.s1[i] = c->spin_m10[i]; a.d1[i] = c->em_dm1[i]; a.a1[i] = c->acc1[i];
+            a.s2[i] = c->spin_m20[i]; a.d2[i] = c->em_dm2[i]; a.a2[i] = c->acc2[i];
+        }
     }
     a.src = c->spin_src;
     a.coef = c->spin_coef;
@@ -962,6 +1029,8 @@ gf_status gf_destroy(gf_ctx* c) {
     if (c->es_buf) hipFree(c->es_buf);
     if (c->spin_src) hipFree(c->spin_src);
     if (c->spin_coef) hipFree(c->spin_coef);
+    if (c->spin_is_src) hipFree(c->spin_is_src);
+    free3(c->spin_m0); free3(c->spin_m10); free3(c->spin_m20);
     for (int q = 0; q < 9; q++) {
         if (c->Jp[q]) hipFree(c->Jp[q]);
         if (c->Jps[q]) hipFree(c->Jps[q]);
@@ -1352,6 +1421,8 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
     const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
     StageArgs a = fm_stage_args(c, stage, c->mag_store_sel, out_sel);
     fm_stage_drives(c, stage, a);
+    if (stage == 4 && c->spin_on && !c->any_afm)
+        if (gf_status e = spin_save(c, 0)) return e;
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_MAG_STAGE, &ev);
     gf_status st;
@@ -1376,7 +1447,9 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
     c->mag_next_stage = stage == 4 ? 1 : stage + 1;
     // get_J_ISHE_RK* at the end of get_dm_RK* (magnetic_solver.cpp:226-228);
     // with an AFM it is recomputed after the AFM stage, whose value stands
-    if (c->spin_on && !c->any_afm) return spin_pump(c);
+    // at stage 4 without electrodynamics nothing reads the stage current:
+    // the step-boundary one replaces it at once
+    if (c->spin_on && !c->any_afm) return stage == 4 && !c->em_on ? spin_pump_step(c) : spin_pump(c);
     return GF_OK;
 }
 
@@ -1397,6 +1470,8 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
         c->hext[i] = c->hext_stat[i] + c->hext_altr[i] * std::sin(2. * kPi * c->h_altr_freq[i] * t);
         a.hext[i] = c->hext[i];
     }
+    if (stage == 4 && c->spin_on)
+        if (gf_status e = spin_save(c, 1)) return e;
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_AFM_STAGE, &ev);
     gf_status st = hip_check(c, launch_afm_stage(a, c->n, c->stream), "afm stage");
@@ -1406,7 +1481,7 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
     c->afm_store_sel = out_sel;
     c->afm_next_stage = stage == 4 ? 1 : stage + 1;
     // magnetic_AFM_solver.cpp:426-428: after the FM stage of the same stage
-    if (c->spin_on) return spin_pump(c);
+    if (c->spin_on) return stage == 4 && !c->em_on ? spin_pump_step(c) : spin_pump(c);
     return GF_OK;
 }
 
@@ -1763,6 +1838,8 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
     c->e_sel = out_e;
     c->h_sel = out_h;
     st = em_cells(c, true, true);
+    // Ampere's law of stage 4 has read the stage current: now the step one
+    if (!st && stage == 4 && c->spin_on) st = spin_pump_step(c);
     if (st) return st;
     c->em_next_stage = stage == 4 ? 1 : stage + 1;
     return GF_OK;

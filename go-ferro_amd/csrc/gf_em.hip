@@ -395,16 +395,22 @@ __global__ void __launch_bounds__(256) spin_pump_kernel(SpinArgs a) {
     const int32_t sz = a.src[k];
     if (sz < 0) return;
     const int64_t is = id - k + sz;
+    // the increment: the stage's k, or at the step boundary acc + k4/6 (the
+    // RK combination in the order the stage-4 update forms it)
+    auto inc = [&](const double* const* d, const double* const* acc, int c) {
+        return acc[0] ? acc[c][is] + d[c][is] / 6. : d[c][is];
+    };
     double sx = 0., sy = 0.;
     if (a.fs[0]) {
-        sx = a.fs[0][is] * a.fd[2][is] - a.fs[2][is] * a.fd[0][is];
-        sy = a.fs[1][is] * a.fd[2][is] - a.fs[2][is] * a.fd[1][is];
+        const double d0 = inc(a.fd, a.fa, 0), d1 = inc(a.fd, a.fa, 1), d2 = inc(a.fd, a.fa, 2);
+        sx = a.fs[0][is] * d2 - a.fs[2][is] * d0;
+        sy = a.fs[1][is] * d2 - a.fs[2][is] * d1;
     }
     if (a.s1[0]) {
-        sx = sx + a.s1[0][is] * a.d1[2][is] - a.s1[2][is] * a.d1[0][is] + a.s2[0][is] * a.d2[2][is] -
-             a.s2[2][is] * a.d2[0][is];
-        sy = sy + a.s1[1][is] * a.d1[2][is] - a.s1[2][is] * a.d1[1][is] + a.s2[1][is] * a.d2[2][is] -
-             a.s2[2][is] * a.d2[1][is];
+        const double p0 = inc(a.d1, a.a1, 0), p1 = inc(a.d1, a.a1, 1), p2 = inc(a.d1, a.a1, 2);
+        const double q0 = inc(a.d2, a.a2, 0), q1 = inc(a.d2, a.a2, 1), q2 = inc(a.d2, a.a2, 2);
+        sx = sx + a.s1[0][is] * p2 - a.s1[2][is] * p0 + a.s2[0][is] * q2 - a.s2[2][is] * q0;
+        sy = sy + a.s1[1][is] * p2 - a.s1[2][is] * p1 + a.s2[1][is] * q2 - a.s2[2][is] * q1;
     }
     a.j[0][id] = a.coef[k] * sx / a.dt;
     a.j[1][id] = a.coef[k] * sy / a.dt;
@@ -412,6 +418,18 @@ __global__ void __launch_bounds__(256) spin_pump_kernel(SpinArgs a) {
 
 hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(spin_pump_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) spin_save_kernel(SpinSaveArgs a) {
+    const int64_t n = a.nx * a.ny * a.nz;
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= n || !a.is_src[id % a.nz]) return;
+    for (int c = 0; c < a.ncomp; c++) a.dst[c][id] = a.src[c][id];
+}
+
+hipError_t launch_spin_save(const SpinSaveArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(spin_save_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -120,6 +120,9 @@ struct EmArgs {
 
 // get_J_ISHE_RK* (magnetic_solver.cpp:231-283): per pumping plane k, the
 // current from the stage store and increment of its source plane src[k].
+// With the accumulators set it is get_J_ISHE (:1460-1511) at the step
+// boundary instead: the increment is the RK combination acc + k4/6 and the
+// states are those before the update.
 struct SpinArgs {
     const double* fs[3];        // FM store and increment of the stage, null when no FM
     const double* fd[3];
@@ -127,6 +130,9 @@ struct SpinArgs {
     const double* d1[3];
     const double* s2[3];
     const double* d2[3];
+    const double* fa[3];        // RK accumulators after stage 3, null for a stage current
+    const double* a1[3];
+    const double* a2[3];
     const int32_t* src;         // source plane per plane, -1 = none
     const double* coef;         // theta_SH e / (2 pi) g_mix * profile, per plane
     double* j[2];
@@ -579,6 +585,10 @@ struct gf_ctx {
     double* jis[3] = {};
     int32_t* spin_src = nullptr;
     double* spin_coef = nullptr;
+    int32_t* spin_is_src = nullptr;  // per plane: a source plane of some pumping plane
+    double* spin_m0[3] = {};         // source planes of the state before the stage-4 update
+    double* spin_m10[3] = {};
+    double* spin_m20[3] = {};
     int mag_in_sel = 0, afm_in_sel = 0;
     // flexoelectricity (GF_FLAG_FLEXO): per-material constants, the force
     // of the current stage
@@ -671,6 +681,15 @@ hipError_t launch_em_h(const EmArgs& a, hipStream_t s);
 hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s);
 hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s);
 hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s);
+// copies the source planes (is_src[k] != 0) of ncomp arrays
+struct SpinSaveArgs {
+    const double* src[6];
+    double* dst[6];
+    int32_t ncomp;
+    const int32_t* is_src;
+    int64_t nx, ny, nz;
+};
+hipError_t launch_spin_save(const SpinSaveArgs& a, hipStream_t s);
 hipError_t launch_fe_flexo(const FeFlexoArgs& a, hipStream_t s);
 hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s);
 hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s);

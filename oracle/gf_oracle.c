@@ -26,6 +26,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* Loops over cells, edges, faces and FFT lines run on all host cores
  * (OpenMP) once a grid holds GFO_OMP_MIN cells.  Every parallel loop writes
@@ -373,6 +376,7 @@ void gfo_newell_real(int64_t nx, int64_t ny, int64_t nz, double dx, double dy, d
     int64_t n = nx * ny * nz;
     double twopi = 2. * PI;
     memset(out9, 0, sizeof(double) * 9 * n);
+    #pragma omp parallel for collapse(3) schedule(dynamic, 256) if (n >= GFO_OMP_MIN)
     for (int64_t i = 1 - nx / 2; i < nx / 2; i++)
         for (int64_t j = 1 - ny / 2; j < ny / 2; j++)
             for (int64_t k = 1 - nz / 2; k < nz / 2; k++) {
@@ -560,7 +564,11 @@ static int spin_setup(gfo_ctx* c, const gf_desc* d) {
 /* get_J_ISHE_RK* (magnetic_solver.cpp:231-283 and its stage copies): the
  * current of each pumping plane from the stage store and the stage
  * increment of its source plane, FM then both AFM sublattices. */
-static void spin_pump(gfo_ctx* c, int stage) {
+/* J_ISHE at the source planes from the given magnetisations and
+ * increments (FM, AFM1, AFM2; NULL when absent), the bracket of
+ * magnetic_solver.cpp:245-259 evaluated left to right. */
+static void spin_current(gfo_ctx* c, double* const* fm, double* const* fd, double* const* s1, double* const* d1,
+                         double* const* s2, double* const* d2) {
     int64_t ny = c->ny, nz = c->nz;
     for (int64_t k = 0; k < nz; k++) {
         int64_t sz = c->spin_src[k];
@@ -569,17 +577,11 @@ static void spin_pump(gfo_ctx* c, int stage) {
             for (int64_t j = 0; j < ny; j++) {
                 int64_t id = (i * ny + j) * nz + k, is = (i * ny + j) * nz + sz;
                 double sx = 0., sy = 0.;
-                if (c->if_FM_all) {
-                    const double* const* st = (const double* const*)c->ms;
-                    const double* const* dm = (const double* const*)c->k[stage - 1];
-                    sx = st[0][is] * dm[2][is] - st[2][is] * dm[0][is];
-                    sy = st[1][is] * dm[2][is] - st[2][is] * dm[1][is];
+                if (fm) {
+                    sx = fm[0][is] * fd[2][is] - fm[2][is] * fd[0][is];
+                    sy = fm[1][is] * fd[2][is] - fm[2][is] * fd[1][is];
                 }
-                if (c->if_AFM_all) {
-                    const double* const* s1 = (const double* const*)c->ms1;
-                    const double* const* d1 = (const double* const*)c->k1[stage - 1];
-                    const double* const* s2 = (const double* const*)c->ms2;
-                    const double* const* d2 = (const double* const*)c->k2[stage - 1];
+                if (s1) {
                     sx = sx + s1[0][is] * d1[2][is] - s1[2][is] * d1[0][is] + s2[0][is] * d2[2][is]
                          - s2[2][is] * d2[0][is];
                     sy = sy + s1[1][is] * d1[2][is] - s1[2][is] * d1[1][is] + s2[1][is] * d2[2][is]
@@ -589,6 +591,22 @@ static void spin_pump(gfo_ctx* c, int stage) {
                 c->Jis[1][id] = c->spin_coef[k] * sy / c->dt;
             }
     }
+}
+
+/* get_J_ISHE_RKs (magnetic_solver.cpp:230-283 and its RK2-4 copies): the
+ * stage stores and the stage increments */
+static void spin_pump(gfo_ctx* c, int stage) {
+    spin_current(c, c->if_FM_all ? c->ms : NULL, c->k[stage - 1], c->if_AFM_all ? c->ms1 : NULL,
+                 c->k1[stage - 1], c->ms2, c->k2[stage - 1]);
+}
+
+/* get_J_ISHE (magnetic_solver.cpp:1460-1511), called by update_m and
+ * update_m_AFM before their m += dm (:1593-1597, magnetic_AFM_solver.cpp:
+ * 1801-1803): the states before the update and the RK-combined increments.
+ * With both magnets the AFM call comes second, after the FM state has been
+ * updated, and its value stands. */
+static void spin_pump_step(gfo_ctx* c) {
+    spin_current(c, c->if_FM_all ? c->m : NULL, c->dm, c->if_AFM_all ? c->m1 : NULL, c->dm1, c->m2, c->dm2);
 }
 
 gfo_ctx* gfo_create(const gf_desc* d) {
@@ -960,6 +978,8 @@ int gfo_get_field(const gfo_ctx* c, int id, double* dst, size_t count) {
         return em_copy((gfo_ctx*)c, id, dst, count, 0);
     double** f = field_ptrs((gfo_ctx*)c, id);
     if (id == GFO_FIELD_DM) f = ((gfo_ctx*)c)->dm;
+    if (id == GFO_FIELD_DM1) f = ((gfo_ctx*)c)->dm1;
+    if (id == GFO_FIELD_DM2) f = ((gfo_ctx*)c)->dm2;
     if (!f || count != (size_t)(3 * c->n)) return -1;
     for (int a = 0; a < 3; a++) memcpy(dst + a * c->n, f[a], sizeof(double) * (size_t)c->n);
     return 0;
@@ -1130,6 +1150,7 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
 
 /* magnetic_solver.cpp:1515-1636 */
 void gfo_mag_update(gfo_ctx* c, int stage) {
+    if (stage == 4 && c->spin_on) spin_pump_step(c);
     for (int a = 0; a < 3; a++) {
         double *m = c->m[a], *s = c->ms[a];
         if (stage == 1 || stage == 2) { const double* k = c->k[stage - 1][a]; for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i] * 0.5; }
@@ -1210,6 +1231,7 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
 
 /* magnetic_AFM_solver.cpp:1731-1834 */
 void gfo_afm_update(gfo_ctx* c, int stage) {
+    if (stage == 4 && c->spin_on) spin_pump_step(c);
     for (int sub = 0; sub < 2; sub++)
         for (int a = 0; a < 3; a++) {
             double* m = sub == 0 ? c->m1[a] : c->m2[a];
@@ -2635,6 +2657,15 @@ void gfo_average(const gfo_ctx* c, int id, double out[3]) {
 }
 
 double gfo_time(const gfo_ctx* c) { return c->time; }
+
+/* Host threads the parallel loops use (1 in a build without OpenMP). */
+int gfo_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 /* ------------------------------------------- tangent-linear reference */
 /* Forward-mode derivative of the probe spectral loss of the adjoint sweep

@@ -36,6 +36,8 @@ typedef struct gfo_ctx gfo_ctx;
 #define GFO_FIELD_KQ1 106   /* stage-1 increment of q */
 #define GFO_FIELD_E0_DIAG 107   /* dynamic eigenstrain (global frame) */
 #define GFO_FIELD_E0_OFF 108
+#define GFO_FIELD_DM1 109   /* combined RK4 increments of the AFM sublattices, last step */
+#define GFO_FIELD_DM2 110
 
 gfo_ctx* gfo_create(const gf_desc* desc);
 void gfo_destroy(gfo_ctx* c);
@@ -82,6 +84,8 @@ void gfo_rfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out
 void gfo_irfft3(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out);
 /* naive O(N^2) DFT of the same transform, for pinning gfo_rfft3 */
 void gfo_rdft3_naive(int64_t nx, int64_t ny, int64_t nz, const double* in, double* out);
+/* host threads the oracle's parallel loops use */
+int gfo_threads(void);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,7 @@ def lib(path=None) -> ctypes.CDLL:
         "gfo_rfft3": ([i64, i64, i64, dp, dp], None),
         "gfo_irfft3": ([i64, i64, i64, dp, dp], None),
         "gfo_rdft3_naive": ([i64, i64, i64, dp, dp], None),
+        "gfo_threads": ([], ctypes.c_int),
         "gfo_tl_probe_loss": ([P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                ctypes.c_double, i64, dp, dp, dp, dp, dp], ctypes.c_int),
     }
@@ -182,6 +183,11 @@ class Oracle:
     @property
     def time(self):
         return self.L.gfo_time(self._h)
+
+
+def threads() -> int:
+    """Host threads the oracle's parallel loops use."""
+    return int(lib().gfo_threads())
 
 
 def newell_real(shape, spacing):

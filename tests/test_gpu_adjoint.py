@@ -292,6 +292,19 @@ def test_adjoint_rejections():
     with pytest.raises(GoFerroError, match="DMI"):
         d.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2)
     d.close()
+    # the elastostatic solver recomputes the static strain from m after
+    # every step, which the sweep would not differentiate: refused, in FP64
+    # and FP32
+    from goferro import FLAG_ELASTOSTATIC
+    pe = problem(shape=(4, 4, 4), rotate=False)
+    pe.flags |= FLAG_ELASTOSTATIC
+    from goferro.workloads import set_elastic
+    set_elastic(pe.materials[0], 2.69e11, 1.08e11, 0.76e11, 5170.0)
+    e = gpu(pe, {FIELD_M: f[FIELD_M], FIELD_HDRIVE: f[FIELD_HDRIVE]})
+    for prec in (64, 32):
+        with pytest.raises(GoFerroError, match="elastostatic"):
+            e.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=prec)
+    e.close()
 
 
 def test_torch_autograd_function():

@@ -141,3 +141,52 @@ def test_layer_stack_given():
     assert np.abs(g.download(FIELD_M) - o.get(FIELD_M)).max() <= 1e-12
     g.close()
     o.close()
+
+
+def test_fm_and_afm_pump_into_two_metals():
+    """vacuum | metal 2 | FM 3 | vacuum | AFM 3 | metal 2 | vacuum: the lower
+    metal pumps from the FM, the upper from the AFM; at the step boundary
+    the AFM update's current stands (the FM state already updated)."""
+    shape = (6, 5, 14)
+    prob0, a1, a2 = workloads.afm_block(shape=shape, dt=1e-18)
+    afm = prob0.materials[0]
+    fm = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.05, K1=-610.0, Aex=3.7e-12)
+    col = np.array([0, 2, 2, 1, 1, 1, 0, 3, 3, 3, 2, 2, 0, 0], np.uint32)
+    cell = np.broadcast_to(col, shape).copy()
+    prob = Problem(shape=shape, spacing=(2e-9, 2e-9, 1e-9), materials=[fm, metal(), afm], material_cell=cell,
+                   dt=1e-18, periodic=(True, True, False), flags=FLAG_SPIN_PUMPING, Hext_stat=(2e4, 0, 8e4),
+                   Hext_altr=(3e4, 1e4, 0), H_altr_freq=(5e11, 7e11, 0))
+    m = workloads.tilted_m(shape, 23, theta_max=0.8)
+    m[:, cell != 1] = 0.0
+    m1, m2 = a1.copy(), a2.copy()
+    m1[:, cell != 3] = 0.0
+    m2[:, cell != 3] = 0.0
+    fields = {FIELD_M: m, FIELD_M_AFM1: m1, FIELD_M_AFM2: m2}
+    from oracle.oracle import Oracle
+    say("case spin pumping fm + afm", shape)
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in fields.items():
+        g.upload(fid, v)
+        o.set(fid, v)
+    L, h = o.L, o._h
+    for step in range(2):
+        g.time_advance()
+        o.time_advance()
+        for s in range(1, 5):
+            g.mag_stage(s)
+            g.afm_stage(s)
+            L.gfo_mag_get_dm(h, s)
+            L.gfo_afm_get_dm(h, s)
+            L.gfo_mag_update(h, s)
+            L.gfo_afm_update(h, s)
+            d = rel(g.download(FIELD_J_ISHE), o.get(FIELD_J_ISHE))
+            assert d <= 1e-12, (step, s, d)
+    J = o.get(FIELD_J_ISHE)
+    assert (np.abs(J[:2][..., [1, 2, 10, 11]]).max(axis=(0, 1, 2)) > 0).all()
+    g.step(3)
+    o.step(3)
+    assert rel(g.download(FIELD_J_ISHE), o.get(FIELD_J_ISHE)) <= 1e-12
+    for fid in fields:
+        assert np.abs(g.download(fid) - o.get(fid)).max() <= 1e-12, fid
+    g.close()
+    o.close()

@@ -13,6 +13,10 @@ magnetic_solver.cpp:231-283, set-up global.cpp:169-235):
 * in Ampere's law it is a current density: starting from E = H = 0 with a
   tiny dt, one step changes an edge's E by -dt J / (eps0 er) (the mean of
   the four cells around it).
+* at a step boundary the current is get_J_ISHE's (magnetic_solver.cpp:
+  1460-1511, called from update_m before m += dm): the state before the
+  update crossed with the RK-combined increment; with an FM and an AFM the
+  AFM update's call stands, which sees the FM state already updated.
 """
 import numpy as np
 import pytest
@@ -180,3 +184,68 @@ def test_ampere_source_sign_and_mean(oracle_lib):
             worst = max(worst, abs(Ex[0, j, k] - want))
     scale = prob.dt * np.abs(J[0]).max() / EPS0
     assert scale > 0 and worst <= 1e-6 * scale
+
+
+
+def step_and_stage4_current(oracle_lib, prob, fields):
+    """One step stage by stage: the current after stage 4's get phase (the
+    stage current) and after its update phase (the step current)."""
+    o = oracle_lib.Oracle(prob)
+    for fid, v in fields.items():
+        o.set(fid, v)
+    fm, afm = FIELD_M in fields, FIELD_M_AFM1 in fields
+    o.time_advance()
+    for s in range(1, 5):
+        if fm:
+            o.L.gfo_mag_get_dm(o._h, s)
+        if afm:
+            o.L.gfo_afm_get_dm(o._h, s)
+        if s == 4:
+            J4 = o.get(FIELD_J_ISHE)
+        if fm:
+            o.L.gfo_mag_update(o._h, s)
+        if afm:
+            o.L.gfo_afm_update(o._h, s)
+    out = {"J4": J4, "J": o.get(FIELD_J_ISHE), "dm": o.get(100), "dm1": o.get(109), "dm2": o.get(110)}
+    o.close()
+    return out
+
+
+def test_step_boundary_current_uses_base_state_and_combined_increment(oracle_lib):
+    col = [0, 1, 1, 1, 2, 2, 2, 0]
+    prob = stack(col)
+    m0 = workloads.tilted_m(prob.shape, 11, theta_max=0.7)
+    m0[:, np.asarray(prob.material_cell) != 1] = 0.0
+    r = step_and_stage4_current(oracle_lib, prob, {FIELD_M: m0})
+    dz = prob.spacing[2]
+    planes = {k: (3, coef(k, 3, 3 * dz, dz)) for k in (4, 5, 6)}
+    want = expected(m0, r["dm"], planes, prob.dt)
+    scale = np.abs(want).max()
+    assert scale > 0
+    np.testing.assert_allclose(r["J"], want, rtol=0, atol=1e-13 * scale)
+    # the stage-4 current (stage-3 store and k4) is a different quantity
+    assert np.abs(r["J4"] - want).max() > 1e-3 * scale
+
+
+def test_step_boundary_current_of_afm_sublattices(oracle_lib):
+    """Metal under an AFM, an FM elsewhere in the stack: the current is the
+    AFM states before the update crossed with their combined increments
+    (the FM update's call is overwritten by the AFM update's)."""
+    prob0, a1, a2 = workloads.afm_block(shape=(4, 3, 8))
+    mats = [magnet(), metal(), prob0.materials[0]]
+    col = [0, 2, 2, 3, 3, 0, 1, 1]
+    prob = stack(col, mats=mats)
+    cell = np.asarray(prob.material_cell)
+    m = workloads.tilted_m(prob.shape, 13, theta_max=0.7)
+    m[:, cell != 1] = 0.0
+    s1, s2 = a1.copy(), a2.copy()
+    s1[:, cell != 3] = 0.0
+    s2[:, cell != 3] = 0.0
+    r = step_and_stage4_current(oracle_lib, prob, {FIELD_M: m, FIELD_M_AFM1: s1, FIELD_M_AFM2: s2})
+    dz = prob.spacing[2]
+    planes = {k: (3, coef(k, 3, 2 * dz, dz)) for k in (1, 2)}
+    want = expected(s1, r["dm1"], planes, prob.dt) + expected(s2, r["dm2"], planes, prob.dt)
+    scale = np.abs(want).max()
+    assert scale > 0
+    np.testing.assert_allclose(r["J"], want, rtol=0, atol=1e-13 * scale)
+    assert np.abs(r["J4"] - want).max() > 1e-3 * scale
