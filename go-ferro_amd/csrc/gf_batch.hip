@@ -34,6 +34,8 @@ struct gf_batch {
 
     StageArgs* d_fm = nullptr;       // [4*count], stage s of instance i at 4*i + s - 1
     AfmStageArgs* d_afm = nullptr;
+    std::vector<StageArgs> h_fm;     // host tables of the last upload
+    std::vector<AfmStageArgs> h_afm;
     BatchClock* d_clk = nullptr;
     std::vector<BatchClock> clk;
 
@@ -90,6 +92,43 @@ gf_status dev_copy(gf_batch* b, T** dst, const std::vector<T>& src, const char* 
     gf_status st = bhip(b, hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * src.size()), what);
     if (st) return st;
     return bhip(b, hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice), what);
+}
+
+// The stage arguments of every instance, from the contexts as they are now
+// (fields uploaded after the batch was made, the static field's state).
+// Stage s reads store {0,1,2,1}[s-1] and writes {1,2,1,0}[s-1], the
+// sequence single-context stages follow from a step boundary; with
+// `from_store`, stage 1 reads the store a context's stage 1 would read
+// instead (a stage store uploaded or loaded at the boundary, selector 2).
+void build_args(gf_batch* b, bool from_store) {
+    const int in_sel[4] = {0, 1, 2, 1}, out_sel[4] = {1, 2, 1, 0};
+    const size_t count = b->ctxs.size();
+    b->h_fm.assign(4 * count, StageArgs{});
+    b->h_afm.assign(4 * count, AfmStageArgs{});
+    for (size_t i = 0; i < count; i++) {
+        gf_ctx* c = b->ctxs[i];
+        for (int s = 0; s < 4; s++) {
+            const int fin = s == 0 && from_store ? c->mag_store_sel : in_sel[s];
+            const int ain = s == 0 && from_store ? c->afm_store_sel : in_sel[s];
+            if (c->any_fm) b->h_fm[4 * i + static_cast<size_t>(s)] = fm_stage_args(c, s + 1, fin, out_sel[s]);
+            if (c->any_afm) b->h_afm[4 * i + static_cast<size_t>(s)] = afm_stage_args(c, s + 1, ain, out_sel[s]);
+        }
+    }
+}
+
+// Upload the host tables into the existing device tables (the graphs keep
+// their addresses); the copies are ordered on the batch stream.
+gf_status upload_args(gf_batch* b) {
+    gf_status st = GF_OK;
+    if (b->any_fm)
+        st = bhip(b, hipMemcpyAsync(b->d_fm, b->h_fm.data(), sizeof(StageArgs) * b->h_fm.size(),
+                                    hipMemcpyHostToDevice, b->stream), "batch stage arguments");
+    if (!st && b->any_afm)
+        st = bhip(b, hipMemcpyAsync(b->d_afm, b->h_afm.data(), sizeof(AfmStageArgs) * b->h_afm.size(),
+                                    hipMemcpyHostToDevice, b->stream), "batch stage arguments");
+    // the host tables are rewritten by the next build: wait for the copies
+    if (!st) st = bhip(b, hipStreamSynchronize(b->stream), "batch stage arguments");
+    return st;
 }
 
 // Workgroup slot tables of one magnet kind for a slot of `chunk` cells.
@@ -233,17 +272,9 @@ gf_status gf_batch_create(gf_ctx* const* ctxs, int count, gf_batch** out) {
     b->device = ctxs[0]->device;
     gf_status st = bhip(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "batch stream");
     if (!st) st = bhip(b, hipEventCreateWithFlags(&b->order, hipEventDisableTiming), "batch event");
-    // stage s reads store {0,1,2,1}[s-1] and writes {1,2,1,0}[s-1], the
-    // sequence single-context stages follow from a step boundary
-    const int in_sel[4] = {0, 1, 2, 1}, out_sel[4] = {1, 2, 1, 0};
-    std::vector<StageArgs> fm(static_cast<size_t>(4 * count));
-    std::vector<AfmStageArgs> afm(static_cast<size_t>(4 * count));
+    build_args(b, false);
     for (int i = 0; i < count && !st; i++) {
         gf_ctx* c = ctxs[i];
-        for (int s = 0; s < 4; s++) {
-            if (c->any_fm) fm[static_cast<size_t>(4 * i + s)] = fm_stage_args(c, s + 1, in_sel[s], out_sel[s]);
-            if (c->any_afm) afm[static_cast<size_t>(4 * i + s)] = afm_stage_args(c, s + 1, in_sel[s], out_sel[s]);
-        }
         if (c->any_fm) {
             b->any_fm = true;
             b->fm_cells += c->n;
@@ -256,8 +287,8 @@ gf_status gf_batch_create(gf_ctx* const* ctxs, int count, gf_batch** out) {
         }
     }
     b->clk.resize(static_cast<size_t>(count));
-    if (!st) st = dev_copy(b, &b->d_fm, fm, "batch stage arguments");
-    if (!st) st = dev_copy(b, &b->d_afm, afm, "batch stage arguments");
+    if (!st && b->any_fm) st = dev_copy(b, &b->d_fm, b->h_fm, "batch stage arguments");
+    if (!st && b->any_afm) st = dev_copy(b, &b->d_afm, b->h_afm, "batch stage arguments");
     if (!st) st = dev_copy(b, &b->d_clk, b->clk, "batch clocks");
     if (!st) st = rebuild(b);
     if (st) {
@@ -301,12 +332,35 @@ gf_status gf_batch_step(gf_batch* b, int64_t nsteps) {
     if ((st = join(b, true))) return st;
     st = bhip(b, hipMemcpyAsync(b->d_clk, b->clk.data(), sizeof(BatchClock) * b->clk.size(), hipMemcpyHostToDevice,
                                 b->stream), "batch clock upload");
-    for (int64_t k = 0; k + kGraphSteps <= nsteps && !st; k += kGraphSteps)
+    // the stage arguments follow the contexts as they are now; a context
+    // whose stage-1 store is not m (an uploaded store) runs its first step
+    // from that store, as gf_step would
+    int64_t left = nsteps;
+    bool odd = false;
+    for (const gf_ctx* c : b->ctxs)
+        odd = odd || (c->any_fm && c->mag_store_sel != 0) || (c->any_afm && c->afm_store_sel != 0);
+    if (!st && odd) {
+        build_args(b, true);
+        st = upload_args(b);
+        if (!st) st = bhip(b, hipGraphLaunch(b->graph1, b->stream), "batch graph launch");
+        left--;
+    }
+    if (!st) {
+        build_args(b, false);
+        st = upload_args(b);
+    }
+    for (int64_t k = 0; k + kGraphSteps <= left && !st; k += kGraphSteps)
         st = bhip(b, hipGraphLaunch(b->graphN, b->stream), "batch graph launch");
-    for (int64_t k = 0; k < nsteps % kGraphSteps && !st; k++)
+    for (int64_t k = 0; k < left % kGraphSteps && !st; k++)
         st = bhip(b, hipGraphLaunch(b->graph1, b->stream), "batch graph launch");
     if (st) return st;
     if ((st = join(b, false))) return st;
+    // every context is now where a stage 4 leaves it: the state in m,
+    // stage 4 having read store 1
+    for (gf_ctx* c : b->ctxs) {
+        if (c->any_fm) { c->mag_store_sel = 0; c->mag_in_sel = 1; }
+        if (c->any_afm) { c->afm_store_sel = 0; c->afm_in_sel = 1; }
+    }
     // host copies of the clocks follow the same additions as the device
     for (gf_ctx* c : b->ctxs) {
         for (int64_t k = 0; k < nsteps; k++) c->time = c->time + c->dt;

@@ -524,8 +524,11 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
     # if_flexo, elastic_dynamic_solver.cpp:63, main.cpp:161-166)
     if g["if_flexo"] and any_fe:
         flags |= FLAG_FLEXO
-    # get_strain_static after every step without elastodynamics (main.cpp:214, 472-474)
-    if g["if_elastostatic"] and not g["if_elastodynamics"]:
+    # get_strain_static after every step without elastodynamics (main.cpp:214,
+    # 472-474); without a magnet (a ferroelectric is refused above) its
+    # eigenstrain, and so the static strain, is zero, and nothing is solved
+    any_mag = any(r["if_FM"] or r["if_AFM"] for r in raw)
+    if g["if_elastostatic"] and not g["if_elastodynamics"] and any_mag:
         flags |= FLAG_ELASTOSTATIC
     # get_E_static runs whenever there is a ferroelectric (main.cpp:120-131,
     # 214-226, 299-302)
@@ -569,8 +572,9 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
                       ("if_output_eigenstraint0_crt", "eigenstraint0_crt (no eigenstrain output)")):
         if g[key]:
             deck.skipped_outputs.append(what)
-    if g["if_output_strain"] and not g["if_elastodynamics"]:
-        deck.skipped_outputs.append("strain without elastodynamics (the elastostatic solver is not implemented)")
+    if g["if_output_strain"] and not g["if_elastodynamics"] and not g["if_elastostatic"]:
+        # main.cpp:582-594 writes nothing then (neither branch copies a strain)
+        deck.skipped_outputs.append("strain without elastodynamics or elastostatics (the reference copies none)")
     return deck
 
 
@@ -814,6 +818,10 @@ class Outputs:
         if self._due("if_output_strain", "output_step_strain", nstep) and g["if_elastodynamics"]:
             self._cells(f"strain.{nstep}.dat",
                         np.concatenate([ctx.download(FIELD_STRAIN_DIAG), ctx.download(FIELD_STRAIN_OFF)]))
+        elif self._due("if_output_strain", "output_step_strain", nstep) and g["if_elastostatic"]:
+            # main.cpp:587-593 and output_strain (inoutput.cpp:648-672): the
+            # global-frame static strain plus a dynamic strain that stays zero
+            self._cells(f"strain.{nstep}.dat", self._static_strain_glb(ctx))
         if self._due("if_output_uandv", "output_step_uandv", nstep) and g["if_elastodynamics"]:
             self._cells(f"u.{nstep}.dat", ctx.download(FIELD_U))
             self._cells(f"v.{nstep}.dat", ctx.download(FIELD_V))
@@ -823,6 +831,27 @@ class Outputs:
                 self._cells(f"Jp{p + 1}.{nstep}.dat", jp[p])
         if g["if_spin_pumping"] and self._due("if_output_Jishe", "output_step_Jishe", nstep):
             self._cells(f"J_ISHE.{nstep}.dat", ctx.download(FIELD_J_ISHE))   # Jx, Jy, 0 (inoutput.cpp:846-864)
+
+    def _static_strain_glb(self, ctx: Context) -> np.ndarray:
+        """get_strain_static_glb (elastic_system.h:196): the crystal-frame
+        static strain rotated to the global frame, e_g = Tc2g e_c Tc2g^T, in
+        the order xx yy zz yz xz xy; zero when nothing is solved."""
+        prob = self.deck.problem
+        shape = tuple(prob.shape)
+        if not prob.flags & FLAG_ELASTOSTATIC:
+            return np.zeros((6,) + shape)
+        d, o = ctx.download(FIELD_STRAIN_T0_DIAG), ctx.download(FIELD_STRAIN_T0_OFF)
+        if not prob.flags & FLAG_ORIENT_ROTATE:
+            return np.concatenate([d, o])
+        e = np.empty((3, 3) + shape)
+        for a in range(3):
+            e[a, a] = d[a]
+        e[1, 2] = e[2, 1] = o[0]
+        e[0, 2] = e[2, 0] = o[1]
+        e[0, 1] = e[1, 0] = o[2]
+        T = np.asarray(prob.Tg2c, dtype=np.float64).T          # Tc2g
+        g = np.einsum("ik,jl,kl...->ij...", T, T, e)
+        return np.stack([g[0, 0], g[1, 1], g[2, 2], g[1, 2], g[0, 2], g[0, 1]])
 
     def _magcell(self, ctx: Context, nstep: int) -> None:
         """output_magcell (inoutput.cpp:349-366): FM cells only, in index order."""

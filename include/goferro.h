@@ -465,8 +465,11 @@ const char* gf_batch_last_error(const gf_batch* batch);
 
 /* nsteps RK4 steps of every instance (gf_time_advance and the four magnet
  * stages, per instance).  Ordered after work queued on every context
- * stream; every context stream is ordered after it.  Returns
- * asynchronously. */
+ * stream; every context stream is ordered after it.  The stage arguments
+ * are rebuilt from the contexts at each call, so uploads made between
+ * calls (Hstat, Hext_nonunif, a stage store, a loaded state image) take
+ * effect as they would for gf_step.  Waits for the previous call's queued
+ * work before it starts; returns after queueing the steps. */
 gf_status gf_batch_step(gf_batch* batch, int64_t nsteps);
 
 /* Wait for the batch's queued work. */

@@ -190,6 +190,47 @@ def test_batch_sweep_instances_step():
     assert np.isfinite(m1).all()
 
 
+def test_batch_follows_uploads_made_after_create():
+    """Hext_nonunif and Hstat uploaded after the batch exists, and a stage-1
+    store that is not m (an uploaded M_STORE), act on the next batch step as
+    on gf_step; afterwards batch and single-context steps alternate."""
+    from goferro import FIELD_HEXT_NONUNIF, FIELD_M_STORE, FLAG_HEXT_NONUNIF
+    say("case batch after uploads")
+    inst = []
+    for i, sh in enumerate([(8, 6, 4), (5, 9, 3)]):
+        prob, f = fm_instance(sh, 40 + i)
+        prob.flags |= FLAG_HEXT_NONUNIF
+        inst.append((prob, {FIELD_M: f[FIELD_M]}))
+    inst.append(afm_instance((8, 8, 4), 41))
+    A, B = make(inst), make(inst)
+    batch = Batch(A)
+    rng = np.random.default_rng(5)
+    for i in range(2):
+        shape = inst[i][0].shape
+        hnu = rng.standard_normal((3,) + shape) * 3e4
+        hst = rng.standard_normal((3,) + shape) * 2e4
+        mst = workloads.tilted_m(shape, 90 + i, theta_max=0.9)
+        for c in (A[i], B[i]):
+            c.upload(FIELD_HEXT_NONUNIF, hnu)
+            c.upload(FIELD_HSTAT, hst)
+            c.upload(FIELD_M_STORE, mst)
+    batch.step(3)
+    for c in B:
+        c.step(3)
+    batch.sync()
+    compare(A, B, inst, "after uploads")
+    for c in A:
+        c.step(2)
+    for c in B:
+        c.step(2)
+    batch.step(2)
+    for c in B:
+        c.step(2)
+    batch.sync()
+    compare(A, B, inst, "alternating")
+    batch.close()
+
+
 def test_batch_rejects_unbatchable_and_mid_step():
     say("case batch rejections")
     prob, m = workloads.fm_film((8, 8, 4))          # FFT magnetostatics on

@@ -279,3 +279,38 @@ def test_fe_em_deck_run_matches_oracle(tmp_path):
     assert (out / "Estat.0.dat").exists() and (out / "Eem.20.dat").exists() and (out / "Hz_Yee.20.dat").exists()
     ctx.close()
     o.close()
+
+
+def test_elastostatic_deck_writes_the_global_static_strain(tmp_path):
+    """if_elastostatic without elastodynamics: strain.<n>.dat holds the
+    static strain in the global frame (main.cpp:587-593, output_strain), the
+    crystal-frame solution rotated by Tc2g = Tg2c^T."""
+    from test_io import deck_copy
+    from goferro import FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FLAG_ELASTOSTATIC
+    say("case elastostatic deck, rotated crystal")
+    c, s = np.cos(0.5), np.sin(0.5)
+    d = deck_copy(tmp_path, edits=[
+        ("if_OrientRotate", "true"), ("Tg2c row 1", f"{c:.17e} {-s:.17e} 0"), ("Tg2c row 2", f"{s:.17e} {c:.17e} 0"),
+        ("if_elastostatic if_elastostatic_1D", "true false"),
+        ("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "false false true"),
+        ("if_output_strain", "true 10")])
+    deck = gio.read_deck(d)
+    assert deck.problem.flags & FLAG_ELASTOSTATIC
+    out = tmp_path / "out"
+    ctx = gio.run(d, out_dir=out)
+    got = gio.read_cells(out / "strain.20.dat", deck.problem.shape, 6)
+    dg, og = ctx.download(FIELD_STRAIN_T0_DIAG), ctx.download(FIELD_STRAIN_T0_OFF)
+    ctx.close()
+    ec = np.zeros(deck.problem.shape + (3, 3))
+    for a in range(3):
+        ec[..., a, a] = dg[a]
+    ec[..., 1, 2] = ec[..., 2, 1] = og[0]
+    ec[..., 0, 2] = ec[..., 2, 0] = og[1]
+    ec[..., 0, 1] = ec[..., 1, 0] = og[2]
+    R = np.asarray(deck.problem.Tg2c).T
+    eg = R @ ec @ R.T
+    want = np.stack([eg[..., 0, 0], eg[..., 1, 1], eg[..., 2, 2], eg[..., 1, 2], eg[..., 0, 2], eg[..., 0, 1]])
+    assert np.abs(want).max() > 0
+    assert np.abs(got - want).max() <= 1e-7 * np.abs(want).max()
+    # a rotation about z mixes xx and yy: the file is not the crystal frame
+    assert np.abs(got[0] - dg[0]).max() > 1e-3 * np.abs(want).max()
