@@ -1405,6 +1405,157 @@ AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     return a;
 }
 
+// ---- argument aliasing ----------------------------------------------------
+// A stage launch reads some arrays at neighbouring cells (the stencil's
+// store, the cell codes) and the rest at its own cell, most of them before
+// the cell's code is known (all of a cell's loads are issued together);
+// it writes its outputs at its own cell.  The launch is well defined only
+// if every array it writes holds one quantity for the launch (one role) and
+// every array it reads is either not written by it, or read only at the
+// cell and as the same quantity (the in-place RK update of the state and
+// the accumulator).  stage_io_conflict names the first pair that breaks it.
+struct IoRef {
+    const void* p;
+    const char* role;
+    int comp;
+    bool gather;      // read at neighbouring cells
+};
+
+struct IoList {
+    IoRef r[80];
+    int n = 0;
+    void add(const void* p, const char* role, int comp, bool gather = false) {
+        if (p && n < 80) r[n++] = IoRef{p, role, comp, gather};
+    }
+};
+
+static bool same_role(const IoRef& a, const IoRef& b) {
+    return a.comp == b.comp && std::strcmp(a.role, b.role) == 0;
+}
+
+static std::string io_conflict(const IoList& rd, const IoList& wr) {
+    auto name = [](const IoRef& x) { return std::string(x.role) + "[" + std::to_string(x.comp) + "]"; };
+    for (int i = 0; i < wr.n; i++) {
+        for (int j = 0; j < i; j++)
+            if (wr.r[j].p == wr.r[i].p) return "it writes " + name(wr.r[j]) + " and " + name(wr.r[i]) + " to one array";
+        for (int j = 0; j < rd.n; j++) {
+            const IoRef& r = rd.r[j];
+            if (r.p != wr.r[i].p) continue;
+            if (r.gather) return "it reads " + name(r) + " at neighbouring cells from the array it writes as " +
+                                 name(wr.r[i]);
+            if (!same_role(r, wr.r[i])) return "it reads " + name(r) + " from the array it writes as " + name(wr.r[i]);
+        }
+    }
+    return "";
+}
+
+std::string stage_io_conflict(const StageArgs& a, const FusedZArgs* f) {
+    IoList rd, wr;
+    rd.add(a.code, "code", 0, true);
+    for (int c = 0; c < 3; c++) {
+        rd.add(a.store[c], "m store", c, true);
+        rd.add(a.base[c], "m", c);
+        if (a.stage > 1) rd.add(a.acc[c], "acc", c);
+        rd.add(a.hstat[c], "Hstat", c);
+        rd.add(a.hnu[c], "Hext_nonunif", c);
+        rd.add(a.hdrive[c], "Hdrive", c);
+        rd.add(a.hcell[c], "H cell", c);
+        wr.add(a.out[c], a.stage == 4 ? "m" : "next m store", c);
+        if (a.stage < 4) wr.add(a.acc[c], "acc", c);
+        wr.add(a.dm_out[c], "k", c);
+    }
+    for (int c = 0; c < 6; c++) rd.add(a.strain[c], "strain", c);
+    if (f) {
+        rd.add(f->hspec, "H z-spectrum", 0, true);
+        rd.add(f->tw, "twiddles", 0, true);
+        wr.add(f->mspec, "M z-spectrum", 0);
+    }
+    return io_conflict(rd, wr);
+}
+
+std::string stage_io_conflict(const AfmStageArgs& a) {
+    IoList rd, wr;
+    rd.add(a.code, "code", 0, true);
+    for (int c = 0; c < 3; c++) {
+        rd.add(a.store1[c], "m1 store", c, true);
+        rd.add(a.store2[c], "m2 store", c, true);
+        rd.add(a.base1[c], "m1", c);
+        rd.add(a.base2[c], "m2", c);
+        if (a.stage > 1) {
+            rd.add(a.acc1[c], "acc1", c);
+            rd.add(a.acc2[c], "acc2", c);
+        }
+        rd.add(a.hstat[c], "Hstat", c);
+        rd.add(a.hcell[c], "H cell", c);
+        wr.add(a.out1[c], a.stage == 4 ? "m1" : "next m1 store", c);
+        wr.add(a.out2[c], a.stage == 4 ? "m2" : "next m2 store", c);
+        if (a.stage < 4) {
+            wr.add(a.acc1[c], "acc1", c);
+            wr.add(a.acc2[c], "acc2", c);
+        }
+        wr.add(a.dm_out1[c], "k1", c);
+        wr.add(a.dm_out2[c], "k2", c);
+    }
+    for (int c = 0; c < 6; c++) rd.add(a.strain[c], "strain", c);
+    return io_conflict(rd, wr);
+}
+
+std::string stage_io_conflict(const FeStageArgs& a) {
+    IoList rd, wr;
+    rd.add(a.code, "code", 0, true);
+    for (int c = 0; c < 3; c++) {
+        rd.add(a.pstore[c], "p store", c, true);
+        rd.add(a.qstore[c], "q store", c);
+        rd.add(a.pbase[c], "p", c);
+        rd.add(a.qbase[c], "q", c);
+        if (a.stage > 1) {
+            rd.add(a.pacc[c], "p acc", c);
+            rd.add(a.qacc[c], "q acc", c);
+        }
+        rd.add(a.estat[c], "Estat", c);
+        rd.add(a.ecell[c], "E cell", c);
+        rd.add(a.flexo[c], "flexo force", c);
+        wr.add(a.pout[c], a.stage == 4 ? "p" : "next p store", c);
+        wr.add(a.qout[c], a.stage == 4 ? "q" : "next q store", c);
+        wr.add(a.pacc[c], "p acc", c);
+        wr.add(a.qacc[c], "q acc", c);
+    }
+    for (int c = 0; c < 6; c++) {
+        rd.add(a.strain[c], "strain", c);
+        rd.add(a.eigen[c], "eigenstrain", c);
+    }
+    return io_conflict(rd, wr);
+}
+
+// Arguments of one FE stage reading store in_sel and writing out_sel
+// (0 = p/q, 1 = the a stores, 2 = the b stores); the drive is left at zero.
+FeStageArgs fe_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
+    FeStageArgs a{};
+    for (int i = 0; i < 6; i++) {
+        a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
+        a.eigen[i] = strain_coupled(c) ? c->e0c[i] : nullptr;
+    }
+    const bool em_p = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_P);
+    for (int i = 0; i < 3; i++) a.ecell[i] = em_p ? c->Ecell[i] : nullptr;
+    a.code = c->d_code; a.mats = c->d_mats;
+    double** ps = fe_pstore(c, in_sel);
+    double** qs = fe_qstore(c, in_sel);
+    double** po = fe_pstore(c, out_sel);
+    double** qo = fe_qstore(c, out_sel);
+    for (int i = 0; i < 3; i++) {
+        a.pstore[i] = ps[i]; a.qstore[i] = qs[i]; a.pbase[i] = c->p[i]; a.qbase[i] = c->q[i];
+        a.pacc[i] = c->pacc[i]; a.qacc[i] = c->qacc[i]; a.pout[i] = po[i]; a.qout[i] = qo[i];
+        a.estat[i] = c->estat[i];
+        a.flexo[i] = c->d_flexo ? c->flexo_force[i] : nullptr;
+    }
+    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
+    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
+    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
+    a.stage = stage;
+    return a;
+}
+
 }  // namespace gf
 
 extern "C" {
@@ -1421,13 +1572,8 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
     const int out_sel = stage == 4 ? 0 : (c->mag_store_sel == 1 ? 2 : 1);
     StageArgs a = fm_stage_args(c, stage, c->mag_store_sel, out_sel);
     fm_stage_drives(c, stage, a);
-    if (stage == 4 && c->spin_on && !c->any_afm)
-        if (gf_status e = spin_save(c, 0)) return e;
-    hipEvent_t ev = nullptr;
-    timer_begin(c, GF_K_MAG_STAGE, &ev);
-    gf_status st;
+    FusedZArgs f{};
     if (c->demag.fused) {
-        FusedZArgs f{};
         f.hspec = c->demag.spec_a; f.mspec = c->demag.spec_m; f.tw = c->demag.twz.dev;
         f.nh = c->nx * c->ny * c->demag.nzh;
         f.nzh = static_cast<int>(c->demag.nzh);
@@ -1435,6 +1581,17 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
         while ((int64_t{1} << lz) < c->nz) lz++;
         f.lognz = lz;
         f.P = c->demag.P;
+    }
+    {
+        const std::string why = stage_io_conflict(a, c->demag.fused ? &f : nullptr);
+        if (!why.empty()) return set_error(c, GF_ERR_STATE, "FM stage arguments alias: " + why);
+    }
+    if (stage == 4 && c->spin_on && !c->any_afm)
+        if (gf_status e = spin_save(c, 0)) return e;
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_MAG_STAGE, &ev);
+    gf_status st;
+    if (c->demag.fused) {
         st = hip_check(c, launch_mag_stage_fused(a, f, c->demag.h_in_spec, c->stream), "mag stage");
         if (!st) c->demag.m_in_spec = true;
     } else {
@@ -1470,6 +1627,10 @@ gf_status gf_afm_stage(gf_ctx* c, int stage) {
         c->hext[i] = c->hext_stat[i] + c->hext_altr[i] * std::sin(2. * kPi * c->h_altr_freq[i] * t);
         a.hext[i] = c->hext[i];
     }
+    {
+        const std::string why = stage_io_conflict(a);
+        if (!why.empty()) return set_error(c, GF_ERR_STATE, "AFM stage arguments alias: " + why);
+    }
     if (stage == 4 && c->spin_on)
         if (gf_status e = spin_save(c, 1)) return e;
     hipEvent_t ev = nullptr;
@@ -1492,25 +1653,14 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
     if (stage != c->fe_next_stage) return set_error(c, GF_ERR_STATE, "FE stages must run in the order 1,2,3,4");
     if (gf_status e = check_after_elastic(c, stage)) return e;
     if (gf_status e = check_before_em(c, stage)) return e;
-    FeStageArgs a{};
-    for (int i = 0; i < 6; i++) {
-        a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
-        a.eigen[i] = strain_coupled(c) ? c->e0c[i] : nullptr;
-    }
-    const bool em_p = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_P);
-    for (int i = 0; i < 3; i++) a.ecell[i] = em_p ? c->Ecell[i] : nullptr;
-    a.code = c->d_code; a.mats = c->d_mats;
+    const int out_sel = stage == 4 ? 0 : (c->fe_store_sel == 1 ? 2 : 1);
+    FeStageArgs a = fe_stage_args(c, stage, c->fe_store_sel, out_sel);
     double** ps = fe_pstore(c, c->fe_store_sel);
     double** qs = fe_qstore(c, c->fe_store_sel);
-    const int out_sel = stage == 4 ? 0 : (c->fe_store_sel == 1 ? 2 : 1);
-    double** po = fe_pstore(c, out_sel);
-    double** qo = fe_qstore(c, out_sel);
-    for (int i = 0; i < 3; i++) {
-        a.pstore[i] = ps[i]; a.qstore[i] = qs[i]; a.pbase[i] = c->p[i]; a.qbase[i] = c->q[i];
-        a.pacc[i] = c->pacc[i]; a.qacc[i] = c->qacc[i]; a.pout[i] = po[i]; a.qout[i] = qo[i];
-        a.estat[i] = c->estat[i];
+    {
+        const std::string why = stage_io_conflict(a);
+        if (!why.empty()) return set_error(c, GF_ERR_STATE, "FE stage arguments alias: " + why);
     }
-    a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
     // ferroelectric_solver.cpp:2002-2029 (drive time per stage)
     if (!(c->flags & GF_FLAG_PRESCRIBE_EEXT)) {
         const double t = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - c->dt * 0.5);
@@ -1518,10 +1668,6 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
             c->eext[i] = c->eext_stat[i] + c->eext_altr[i] * std::sin(2. * kPi * c->e_altr_freq[i] * t);
     }
     for (int i = 0; i < 3; i++) a.eext[i] = c->eext[i];
-    std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
-    std::memcpy(a.Tc2g, c->Tc2g, sizeof(a.Tc2g));
-    a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
-    a.stage = stage;
     hipEvent_t ev = nullptr;
     gf_status st = GF_OK;
     if (c->d_flexo) {
@@ -1535,7 +1681,6 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
             f.gxz[i] = c->el_on ? c->gxz[i] : nullptr;
             f.gxy[i] = c->el_on ? c->gxy[i] : nullptr;
             f.per[i] = c->per[i];
-            a.flexo[i] = c->flexo_force[i];
         }
         for (int i = 0; i < 6; i++) f.eg[i] = c->el_on ? c->eg[i] : nullptr;
         f.nx = c->nx; f.ny = c->ny; f.nz = c->nz; f.dx = c->dx; f.dy = c->dy; f.dz = c->dz;
@@ -2318,6 +2463,96 @@ gf_status gf_state_read(gf_ctx* c, const char* path) {
     std::fclose(f);
     if (!ok) return set_error(c, GF_ERR_ARG, std::string("reading ") + path + " failed");
     return gf_state_load(c, buf.data(), buf.size());
+}
+
+
+// The aliasing check over the stage arguments the builders produce for
+// every combination of physics, couplings and optional fields, every stage
+// and both stage-1 stores (m, or an uploaded store), on host contexts whose
+// arrays are distinct placeholder addresses (nothing touches the device).
+gf_status gf_selftest_stage_io(int plant, char* msg, size_t cap) {
+    auto put = [&](const std::string& s) {
+        if (msg && cap) {
+            std::strncpy(msg, s.c_str(), cap - 1);
+            msg[cap - 1] = 0;
+        }
+    };
+    if (plant < 0 || plant > 4) { put("plant must be 0..4"); return GF_ERR_ARG; }
+    int64_t checked = 0, found = 0;
+    std::string first;
+    for (uint32_t combo = 0; combo < (1u << 12); combo++) {
+        gf_ctx* c = new gf_ctx();
+        uintptr_t next = 0x100000;
+        auto fake = [&]() { next += 0x1000; return reinterpret_cast<double*>(next); };
+        auto fake3 = [&](double** a) { for (int i = 0; i < 3; i++) a[i] = fake(); };
+        auto bit = [&](int k) { return ((combo >> k) & 1u) != 0; };
+        c->any_fm = bit(0);
+        c->any_afm = bit(1);
+        c->any_fe = bit(2);
+        c->el_on = bit(3);
+        c->em_on = bit(4);
+        c->flags = (bit(5) ? GF_FLAG_ELASTO_ON_PM : 0u) | (c->el_on ? GF_FLAG_ELASTODYNAMICS : 0u) |
+                   (bit(6) ? GF_FLAG_EM_BACKACTION_M | GF_FLAG_EM_BACKACTION_P : 0u) |
+                   (bit(7) ? GF_FLAG_MAGNETOSTATICS : 0u) | (bit(8) ? GF_FLAG_ORIENT_ROTATE : 0u);
+        c->has_hnu = bit(9);
+        c->hstat_written = bit(10);
+        c->d_code = reinterpret_cast<uint32_t*>(fake());
+        c->d_mats = reinterpret_cast<DevMaterial*>(fake());
+        fake3(c->m); fake3(c->st_a); fake3(c->st_b); fake3(c->acc);
+        fake3(c->m1); fake3(c->m2); fake3(c->st1_a); fake3(c->st1_b); fake3(c->st2_a); fake3(c->st2_b);
+        fake3(c->acc1); fake3(c->acc2);
+        fake3(c->p); fake3(c->q); fake3(c->p_a); fake3(c->p_b); fake3(c->q_a); fake3(c->q_b);
+        fake3(c->pacc); fake3(c->qacc);
+        fake3(c->hstat); fake3(c->hnu); fake3(c->Hcell); fake3(c->Ecell); fake3(c->estat);
+        if (bit(11)) {   // optional fields: k outputs (Faraday, spin pumping), shaped drive, flexo force
+            fake3(c->em_dm); fake3(c->em_dm1); fake3(c->em_dm2); fake3(c->hdrive); fake3(c->flexo_force);
+            c->d_flexo = reinterpret_cast<FlexoMat*>(fake());
+        }
+        for (int i = 0; i < 6; i++) { c->ec[i] = fake(); c->e0c[i] = fake(); c->e_t0[i] = fake(); }
+        FusedZArgs f{};
+        f.hspec = reinterpret_cast<double2*>(fake());
+        f.mspec = reinterpret_cast<double2*>(fake());
+        f.tw = reinterpret_cast<double2*>(fake());
+        for (int s0 : {0, 2}) {
+            int sel = s0;
+            for (int stage = 1; stage <= 4; stage++) {
+                const int out = stage == 4 ? 0 : (sel == 1 ? 2 : 1);
+                std::string why;
+                if (c->any_fm) {
+                    StageArgs a = fm_stage_args(c, stage, sel, out);
+                    if (plant == 2) a.out[1] = const_cast<double*>(a.store[1]);
+                    why = stage_io_conflict(a, bit(7) ? &f : nullptr);
+                    checked++;
+                    if (!why.empty()) { found++; if (first.empty()) first = "FM stage: " + why; }
+                }
+                if (c->any_afm) {
+                    AfmStageArgs a = afm_stage_args(c, stage, sel, out);
+                    if (plant == 3) a.acc1[0] = const_cast<double*>(a.base2[0]);
+                    why = stage_io_conflict(a);
+                    checked++;
+                    if (!why.empty()) { found++; if (first.empty()) first = "AFM stage: " + why; }
+                }
+                if (c->any_fe) {
+                    FeStageArgs a = fe_stage_args(c, stage, sel, out);
+                    if (plant == 1) a.pout[0] = const_cast<double*>(a.qstore[1]);
+                    if (plant == 4) a.qacc[2] = a.pacc[2];
+                    why = stage_io_conflict(a);
+                    checked++;
+                    if (!why.empty()) { found++; if (first.empty()) first = "FE stage: " + why; }
+                }
+                sel = out;
+            }
+        }
+        delete c;
+    }
+    if (plant == 0) {
+        if (found) { put(first); return GF_ERR_STATE; }
+        put("no aliasing in " + std::to_string(checked) + " stage argument sets");
+        return GF_OK;
+    }
+    // a planted alias must be reported
+    put(first.empty() ? std::string("planted alias not reported") : first);
+    return found ? GF_OK : GF_ERR_STATE;
 }
 
 }  // extern "C"

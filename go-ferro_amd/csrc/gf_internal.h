@@ -654,6 +654,12 @@ int adjoint32_material(const gf_ctx* c);
 gf_status adjoint32_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss);
 gf_status adjoint32_backward(gf_ctx* c, double grad[3], double* grad_hdrive, size_t count);
 AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
+FeStageArgs fe_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel);
+// "" when no array a stage launch writes is read by it at neighbouring
+// cells or as another quantity, else what aliases (gf_api.hip)
+std::string stage_io_conflict(const StageArgs& a, const FusedZArgs* f);
+std::string stage_io_conflict(const AfmStageArgs& a);
+std::string stage_io_conflict(const FeStageArgs& a);
 // true when cell indices of an n-cell grid fit 32-bit byte offsets (gf_llg.hip)
 bool narrow(int64_t n);
 

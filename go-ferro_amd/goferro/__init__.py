@@ -120,7 +120,7 @@ EXPORTED_SYMBOLS = [
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_sync",
     "gf_batch_get_stream", "gf_batch_config", "gf_batch_set_config", "gf_batch_tune",
     "gf_state_size", "gf_state_save", "gf_state_load", "gf_state_write", "gf_state_read",
-    "gf_write_cells", "gf_read_cells",
+    "gf_write_cells", "gf_read_cells", "gf_selftest_stage_io",
 ]
 
 

@@ -526,6 +526,20 @@ gf_status gf_state_read(gf_ctx* ctx, const char* path);
 gf_status gf_write_cells(const char* path, const int64_t dims[3], int32_t ncomp, const double* data);
 gf_status gf_read_cells(const char* path, const int64_t dims[3], int32_t ncomp, double* data);
 
+/* Diagnostic, with no counterpart in the reference.  Every stage kernel
+ * issues all of a cell's loads before the cell's code is known, so no array
+ * a launch writes may be read by it at neighbouring cells or as another
+ * quantity; gf_mag_stage, gf_afm_stage and gf_fe_stage refuse arguments
+ * that break this (GF_ERR_STATE).  This call runs that check over the
+ * stage arguments of every combination of physics, couplings, optional
+ * fields, stage and stage-1 store, on host placeholder contexts (no device
+ * work).  plant = 0 returns GF_OK when nothing aliases; plant = 1..4
+ * plants one alias (an FE output on a q store, an FM output on its stencil
+ * store, an AFM accumulator on the other sublattice's state, two FE
+ * accumulators on one array) and returns GF_OK when it is reported.  msg
+ * (cap bytes) receives a summary or the first conflict. */
+gf_status gf_selftest_stage_io(int plant, char* msg, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,3 +110,31 @@ def test_create_rejects_grids_the_reference_cannot_run(case, oracle_lib):
         goferro.Context(prob)
     with pytest.raises(RuntimeError):
         oracle_lib.Oracle(prob)
+
+
+@pytest.mark.parametrize("plant,where", [(0, None), (1, "FE stage"), (2, "FM stage"), (3, "AFM stage"),
+                                         (4, "FE stage")])
+def test_stage_arguments_never_alias(plant, where):
+    """The stage kernels load every input of a cell before its code is
+    known, so no array a launch writes may be read by it at neighbouring
+    cells or as another quantity.  The library checks this before every
+    stage launch; gf_selftest_stage_io runs the check over the arguments
+    its builders produce for every physics/coupling/optional-field
+    combination, stage and stage-1 store on host placeholder contexts (no
+    device work), and shows that a planted alias is reported."""
+    import ctypes
+
+    import goferro
+    if not goferro.LIB_PATH.exists():
+        pytest.skip("libgoferro.so not built (run __graft_entry__.build())")
+    lib = goferro.load_library()
+    fn = lib.gf_selftest_stage_io
+    fn.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    fn.restype = ctypes.c_int
+    buf = ctypes.create_string_buffer(512)
+    assert fn(plant, buf, len(buf)) == 0, buf.value.decode()
+    text = buf.value.decode()
+    if where is None:
+        assert text.startswith("no aliasing in") and int(text.split()[3]) > 10000
+    else:
+        assert text.startswith(where), text
