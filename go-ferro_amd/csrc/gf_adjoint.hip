@@ -308,10 +308,11 @@ gf_status check_context(gf_ctx* c, int material) {
     if (!c->any_fm || c->any_afm || c->any_fe || c->el_on || c->em_on)
         return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep needs a ferromagnet-only context");
     const uint32_t bad = GF_FLAG_MAG_1D | GF_FLAG_DEMAG_FACTOR | GF_FLAG_PRESCRIBE_HEXT | GF_FLAG_EMDYNAMIC |
-                         GF_FLAG_PRESCRIBE_EEXT | GF_FLAG_ELASTODYNAMICS;
+                         GF_FLAG_PRESCRIBE_EEXT | GF_FLAG_ELASTODYNAMICS | GF_FLAG_PRESCRIBED_M;
     if (c->flags & bad)
         return set_error(c, GF_ERR_UNSUPPORTED,
-                         "the adjoint sweep supports FFT or no magnetostatics and sinusoidal drives only");
+                         "the adjoint sweep supports FFT or no magnetostatics and sinusoidal drives only "
+                         "(no prescribed layer)");
     // gf_step recomputes the static strain from m after every step; the
     // sweep's forward would keep it fixed and its gradient would miss
     // d(strain)/dm, so a context that solves the elastostatics is refused

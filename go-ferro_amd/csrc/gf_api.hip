@@ -828,6 +828,8 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
         for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
         c->flags = d->flags; c->dt = d->dt; c->time = 0.;
         c->hdrive_freq = d->Hdrive_freq;
+        c->precess_angle = d->precess_angle;
+        c->precess_freq = d->precess_frequency;
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) { c->Tg2c[a][b] = d->Tg2c[a][b]; c->Tc2g[b][a] = d->Tg2c[a][b]; }
         for (int a = 0; a < 3; a++) {
@@ -1370,9 +1372,23 @@ StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
 }
 
 // Uniform and shaped drives of FM stage s at the context's current time.
+// The prescribed cone of the bottom layer at time t
+// (magnetic_solver.cpp:1891-1957, the same expressions).
+void prescribed_cone(const gf_ctx* c, double t, double out[3]) {
+    const double m0 = std::sin(c->precess_angle / 180. * kPi);
+    const double omega = 2. * kPi * c->precess_freq;
+    out[0] = std::sqrt((1. - m0 * m0));
+    out[1] = m0 * std::sin(omega * t);
+    out[2] = m0 * std::cos(omega * t);
+}
+
 void fm_stage_drives(gf_ctx* c, int stage, StageArgs& a) {
     stage_drive(c, stage, a.hext);
     a.drive_s = a.hdrive[0] ? shaped_drive(c, stage) : 0.;
+    // update_m_RK1/RK2 prescribe the half-step cone, update_m_RK3 and
+    // update_m the full-step one (magnetic_solver.cpp:1526-1609)
+    a.prescribe = (c->flags & GF_FLAG_PRESCRIBED_M) ? 1 : 0;
+    if (a.prescribe) prescribed_cone(c, stage <= 2 ? c->time - c->dt * 0.5 : c->time, a.presc_m);
 }
 
 // The same for the two AFM sublattices.
@@ -2004,6 +2020,21 @@ gf_status gf_demag(gf_ctx* c) {
     return mag_statics(c);
 }
 
+gf_status gf_prescribe_m(gf_ctx* c) {
+    if (!c) return GF_ERR_ARG;
+    if (!(c->flags & GF_FLAG_PRESCRIBED_M)) return set_error(c, GF_ERR_STATE, "GF_FLAG_PRESCRIBED_M is not set");
+    if (!c->any_fm) return set_error(c, GF_ERR_STATE, "no ferromagnetic material in this context");
+    if (c->mag_next_stage != 1) return set_error(c, GF_ERR_STATE, "gf_prescribe_m needs a step boundary");
+    double v[3];
+    prescribed_cone(c, c->time, v);
+    gf_status st = hip_check(c, launch_prescribe_plane(c->m, c->d_code, c->nx, c->ny, c->nz, v, c->stream),
+                             "prescribe m");
+    if (st) return st;
+    // initialize_device then copies m into the store (magnetic_initialize.cpp:1081-1086)
+    c->mag_store_sel = 0;
+    c->demag.m_in_spec = false;
+    return hip_check(c, hipStreamSynchronize(c->stream), "prescribe m sync");
+}
 
 // main.cpp:243-483 for the magnetic physics
 gf_status gf_step(gf_ctx* c, int64_t nsteps) {

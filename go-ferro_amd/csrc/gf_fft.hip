@@ -101,20 +101,25 @@ __global__ void __launch_bounds__(kThreads) zfwd_kernel(ZArgs a) {
     }
 }
 
+// The y and x passes of a convolution may run over a range of kz planes
+// at a time (kz = kz0 + blockIdx.y); the [kz][y][x] buffer between them
+// then holds plane kz at kz - kzb0, component c at c*nhb, so that a chunk
+// of planes can reuse the start of the buffer while it is still in cache.
 struct YArgs {
     const double2* in;
     double2* out;
     const double2* tw;
-    int64_t nh;
+    int64_t nh, nhb;
     int nx, ny, nzh;
     int logny, pencils, logp;
+    int kz0, kzb0;
 };
 
 // y forward transform: [x][kz][y] -> [kz][y][x]
 __global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     const int c = blockIdx.z;
-    const int kz = blockIdx.y;
+    const int kz = a.kz0 + static_cast<int>(blockIdx.y);
     const int x0 = blockIdx.x * a.pencils;
     double2* tw = smem;
     double2* buf = smem + (a.ny >> 1);
@@ -132,12 +137,13 @@ __global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
     fft_twiddle_store(tw, a.tw, a.ny, tw0);
     __syncthreads();
     fft_dit(buf, a.pencils, a.logny, tw);
-    double2* out = a.out + c * a.nh;
+    double2* out = a.out + c * a.nhb;
+    const int kb = kz - a.kzb0;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int r = idx & (a.pencils - 1);
         const int y = idx >> a.logp;
         const int x = x0 + r;
-        if (x < a.nx) out[(static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[fpad(r * a.ny + y)];
+        if (x < a.nx) out[(static_cast<int64_t>(kb) * a.ny + y) * a.nx + x] = buf[fpad(r * a.ny + y)];
     }
 }
 
@@ -145,18 +151,19 @@ __global__ void __launch_bounds__(kThreads) yfwd_kernel(YArgs a) {
 __global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
     extern __shared__ double2 smem[];
     const int c = blockIdx.z;
-    const int kz = blockIdx.y;
+    const int kz = a.kz0 + static_cast<int>(blockIdx.y);
+    const int kb = kz - a.kzb0;
     const int x0 = blockIdx.x * a.pencils;
     double2* tw = smem;
     double2* buf = smem + (a.ny >> 1);
     const double2 tw0 = fft_twiddle_prefetch(a.tw, a.ny);
-    const double2* in = a.in + c * a.nh;
+    const double2* in = a.in + c * a.nhb;
     const int total = a.pencils * a.ny;
     fft_staged_load<kBatch, double2>(
         total,
         [&](int idx) {
             const int x = x0 + (idx & (a.pencils - 1));
-            return x < a.nx ? in[(static_cast<int64_t>(kz) * a.ny + (idx >> a.logp)) * a.nx + x] : make_double2(0., 0.);
+            return x < a.nx ? in[(static_cast<int64_t>(kb) * a.ny + (idx >> a.logp)) * a.nx + x] : make_double2(0., 0.);
         },
         [&](int idx, double2 v) { buf[fpad((idx & (a.pencils - 1)) * a.ny + (idx >> a.logp))] = v; });
     fft_twiddle_store(tw, a.tw, a.ny, tw0);
@@ -172,11 +179,12 @@ __global__ void __launch_bounds__(kThreads) yinv_kernel(YArgs a) {
 }
 
 struct XArgs {
-    double2* data;              // component c at data + c*nh, layout [kz][y][x]
+    double2* data;              // component c at data + c*nhb, plane kz at kz - kzb0, layout [kz][y][x]
     const double* kre;          // real part of tensor spectrum s at kre + s*nh
     const float* kim;           // its imaginary part (round-off of an even/odd tensor) at kim + s*nh
     const double2* tw;
-    int64_t nh;
+    int64_t nh, nhb;
+    int kz0, kzb0;
     double den;                 // -dx*dy*dz
     int nx, ny, nzh;
     int lognx, pencils, logp;
@@ -188,7 +196,8 @@ struct XArgs {
 template <bool kConv>
 __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
     extern __shared__ double2 smem[];
-    const int kz = blockIdx.y;
+    const int kz = a.kz0 + static_cast<int>(blockIdx.y);
+    const int kb = kz - a.kzb0;
     const int y0 = blockIdx.x * a.pencils;
     const int comps = kConv ? 3 : 1;
     double2* tw = smem;
@@ -224,8 +233,8 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
         [&](int idx) {
             const int t = idx >> a.lognx;
             const int y = y0 + (t & (a.pencils - 1));
-            const int64_t g = (static_cast<int64_t>(kz) * a.ny + y) * a.nx + (idx & (a.nx - 1));
-            return y < a.ny ? a.data[(t >> a.logp) * a.nh + g] : make_double2(0., 0.);
+            const int64_t g = (static_cast<int64_t>(kb) * a.ny + y) * a.nx + (idx & (a.nx - 1));
+            return y < a.ny ? a.data[(t >> a.logp) * a.nhb + g] : make_double2(0., 0.);
         },
         [&](int idx, double2 v) { buf[fpad((idx >> a.lognx) * a.nx + bitrev(idx & (a.nx - 1), a.lognx))] = v; });
     fft_twiddle_store(tw, a.tw, a.nx, tw0);
@@ -267,7 +276,7 @@ __global__ void __launch_bounds__(kThreads) xconv_kernel(XArgs a) {
         const int y = y0 + r;
         if (y < a.ny) {
             const int src = kConv ? bitrev(x, a.lognx) : x;
-            a.data[c * a.nh + (static_cast<int64_t>(kz) * a.ny + y) * a.nx + x] = buf[fpad(t * a.nx + src)];
+            a.data[c * a.nhb + (static_cast<int64_t>(kb) * a.ny + y) * a.nx + x] = buf[fpad(t * a.nx + src)];
         }
     }
 }
@@ -458,16 +467,29 @@ static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* const* rawp, const
     return hipGetLastError();
 }
 
-static hipError_t run_y(gf_ctx* c, bool inverse, int comps) {
+// A range of kz planes of the y and x passes: planes k0 .. k0+cnt-1, and
+// where the [kz][y][x] buffer holds them (plane kz at kz - kb0, component
+// stride nhb).  The whole spectrum is {0, nzh, 0, nh}.
+struct KzRange {
+    int k0, cnt, kb0;
+    int64_t nhb;
+};
+
+static KzRange whole_range(const gf_ctx* c) {
+    return KzRange{0, static_cast<int>(c->demag.nzh), 0, c->nx * c->ny * c->demag.nzh};
+}
+
+static hipError_t run_y(gf_ctx* c, bool inverse, int comps, KzRange kr) {
     const Shapes sh = shapes(c);
     YArgs a{};
     a.in = inverse ? c->demag.spec_b : c->demag.spec_m;
     a.out = inverse ? c->demag.spec_a : c->demag.spec_b;
     a.tw = c->demag.twy.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
+    a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
     a.logny = ilog2(c->ny); a.pencils = sh.py; a.logp = ilog2(sh.py);
-    dim3 grid(blocks_for(c->nx, sh.py), static_cast<unsigned>(c->demag.nzh), static_cast<unsigned>(comps));
+    dim3 grid(blocks_for(c->nx, sh.py), static_cast<unsigned>(kr.cnt), static_cast<unsigned>(comps));
     hipEvent_t ev = nullptr;
     const int cls = inverse ? GF_K_FFT_Y_INV : GF_K_FFT_Y_FWD;
     timer_begin(c, cls, &ev);
@@ -477,16 +499,17 @@ static hipError_t run_y(gf_ctx* c, bool inverse, int comps) {
     return hipGetLastError();
 }
 
-static hipError_t run_x(gf_ctx* c, bool conv, bool adjoint = false) {
+static hipError_t run_x(gf_ctx* c, bool conv, KzRange kr, bool adjoint = false) {
     const Shapes sh = shapes(c);
     XArgs a{};
     a.adjoint = adjoint ? 1 : 0;
     a.data = c->demag.spec_b; a.kre = c->demag.kre; a.kim = c->demag.kim; a.tw = c->demag.twx.dev;
     a.nh = c->nx * c->ny * c->demag.nzh;
+    a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
     a.den = -c->dx * c->dy * c->dz;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
     a.lognx = ilog2(c->nx); a.pencils = sh.px; a.logp = ilog2(sh.px);
-    dim3 grid(blocks_for(c->ny, sh.px), static_cast<unsigned>(c->demag.nzh));
+    dim3 grid(blocks_for(c->ny, sh.px), static_cast<unsigned>(kr.cnt));
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_FFT_X_CONV, &ev);
     if (conv) hipLaunchKernelGGL(xconv_kernel<true>, grid, dim3(kThreads), sh.lds_x, c->stream, a);
@@ -547,8 +570,8 @@ gf_status demag_setup(gf_ctx* c) {
                                          c->stream), "copy kernel");
         const double* raw3[3] = {draw, nullptr, nullptr};
         if (!st) st = hip_check(c, run_zfwd(c, true, raw3, nullptr, nullptr, nullptr, 1), "zfwd kernel");
-        if (!st) st = hip_check(c, run_y(c, false, 1), "yfwd kernel");
-        if (!st) st = hip_check(c, run_x(c, false), "xfwd kernel");
+        if (!st) st = hip_check(c, run_y(c, false, 1, whole_range(c)), "yfwd kernel");
+        if (!st) st = hip_check(c, run_x(c, false, whole_range(c)), "xfwd kernel");
         if (!st) {
             hipLaunchKernelGGL(split_spectrum_kernel, dim3(blocks_for(nh, kThreads)), dim3(kThreads), 0, c->stream,
                                d.spec_b, d.kre + s * nh, d.kim + s * nh, nh);
@@ -647,13 +670,43 @@ gf_status elec_setup(gf_ctx* c, double er) {
     return GF_OK;
 }
 
+// The y forward, x convolution and y inverse passes of one convolution
+// (spec_m -> spec_a through spec_b).  With GOFERRO_FFT_KZCHUNKS = C > 1 the
+// three passes run over C ranges of kz planes in turn, each range's
+// intermediate at the start of spec_b, so that it can be read back from
+// the last-level cache instead of HBM.
+static gf_status conv_passes(gf_ctx* c, bool adjoint, const char* what) {
+    int chunks = 1;
+    if (const char* env = std::getenv("GOFERRO_FFT_KZCHUNKS")) {
+        const int v = std::atoi(env);
+        if (v >= 1 && v <= 1024) chunks = v;
+    }
+    const int nzh = static_cast<int>(c->demag.nzh);
+    if (chunks > nzh) chunks = nzh;
+    if (chunks <= 1) {
+        const KzRange w = whole_range(c);
+        gf_status st;
+        if ((st = hip_check(c, run_y(c, false, 3, w), what))) return st;
+        if ((st = hip_check(c, run_x(c, true, w, adjoint), what))) return st;
+        return hip_check(c, run_y(c, true, 3, w), what);
+    }
+    const int per = (nzh + chunks - 1) / chunks;
+    const int64_t nhb = static_cast<int64_t>(per) * c->nx * c->ny;
+    for (int k0 = 0; k0 < nzh; k0 += per) {
+        const KzRange r{k0, k0 + per <= nzh ? per : nzh - k0, k0, nhb};
+        gf_status st;
+        if ((st = hip_check(c, run_y(c, false, 3, r), what))) return st;
+        if ((st = hip_check(c, run_x(c, true, r, adjoint), what))) return st;
+        if ((st = hip_check(c, run_y(c, true, 3, r), what))) return st;
+    }
+    return GF_OK;
+}
+
 // The static field of the current polarisation store into Estat.
 gf_status elec_run(gf_ctx* c, const double* const* p) {
     gf_status st;
     if ((st = hip_check(c, run_zfwd(c, true, p, nullptr, nullptr, nullptr, 3), "zfwd electrostatics"))) return st;
-    if ((st = hip_check(c, run_y(c, false, 3), "yfwd electrostatics"))) return st;
-    if ((st = hip_check(c, run_x(c, true), "xconv electrostatics"))) return st;
-    if ((st = hip_check(c, run_y(c, true, 3), "yinv electrostatics"))) return st;
+    if ((st = conv_passes(c, false, "y/x/y passes electrostatics"))) return st;
     return hip_check(c, run_zinv(c, c->estat), "zinv electrostatics");
 }
 
@@ -664,9 +717,7 @@ gf_status demag_run_fused(gf_ctx* c, const double* const* fm) {
     gf_status st;
     if (!c->demag.m_in_spec && (st = hip_check(c, run_zfwd(c, false, nullptr, fm, nullptr, nullptr, 3), "zfwd")))
         return st;
-    if ((st = hip_check(c, run_y(c, false, 3), "yfwd"))) return st;
-    if ((st = hip_check(c, run_x(c, true), "xconv"))) return st;
-    if ((st = hip_check(c, run_y(c, true, 3), "yinv"))) return st;
+    if ((st = conv_passes(c, false, "y/x/y passes"))) return st;
     c->demag.h_in_spec = true;
     c->demag.h_real_valid = false;
     c->demag.m_in_spec = false;
@@ -684,9 +735,7 @@ gf_status demag_materialize_h(gf_ctx* c) {
 gf_status demag_run_to(gf_ctx* c, const double* const* fm, double* const* out) {
     gf_status st;
     if ((st = hip_check(c, run_zfwd(c, false, nullptr, fm, nullptr, nullptr, 3), "zfwd"))) return st;
-    if ((st = hip_check(c, run_y(c, false, 3), "yfwd"))) return st;
-    if ((st = hip_check(c, run_x(c, true), "xconv"))) return st;
-    if ((st = hip_check(c, run_y(c, true, 3), "yinv"))) return st;
+    if ((st = conv_passes(c, false, "y/x/y passes"))) return st;
     return hip_check(c, run_zinv(c, out), "zinv");
 }
 
@@ -695,18 +744,14 @@ gf_status demag_run_to(gf_ctx* c, const double* const* fm, double* const* out) {
 gf_status demag_adjoint(gf_ctx* c, const double* const* in, double* const* out) {
     gf_status st;
     if ((st = hip_check(c, run_zfwd(c, true, in, nullptr, nullptr, nullptr, 3), "zfwd adjoint"))) return st;
-    if ((st = hip_check(c, run_y(c, false, 3), "yfwd adjoint"))) return st;
-    if ((st = hip_check(c, run_x(c, true, true), "xconv adjoint"))) return st;
-    if ((st = hip_check(c, run_y(c, true, 3), "yinv adjoint"))) return st;
+    if ((st = conv_passes(c, true, "y/x/y passes adjoint"))) return st;
     return hip_check(c, run_zinv(c, out), "zinv adjoint");
 }
 
 gf_status demag_run(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2) {
     gf_status st;
     if ((st = hip_check(c, run_zfwd(c, false, nullptr, fm, s1, s2, 3), "zfwd"))) return st;
-    if ((st = hip_check(c, run_y(c, false, 3), "yfwd"))) return st;
-    if ((st = hip_check(c, run_x(c, true), "xconv"))) return st;
-    if ((st = hip_check(c, run_y(c, true, 3), "yinv"))) return st;
+    if ((st = conv_passes(c, false, "y/x/y passes"))) return st;
     return hip_check(c, run_zinv(c), "zinv");
 }
 

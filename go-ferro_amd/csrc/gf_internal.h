@@ -381,6 +381,8 @@ struct StageArgs {
     int32_t rotate;
     int32_t stage;
     int32_t nmat;               // material records at mats
+    int32_t prescribe;          // the FM cells at k = 0 take presc_m as their next store
+    double presc_m[3];
 };
 
 struct AfmStageArgs {
@@ -604,6 +606,8 @@ struct gf_ctx {
     double* hnu[3] = {};
     double* hdrive[3] = {};         // shaped drive profile (GF_FLAG_HDRIVE)
     double hdrive_freq = 0;
+    double precess_angle = 0;       // prescribed bottom layer (GF_FLAG_PRESCRIBED_M), degrees
+    double precess_freq = 0;        // its frequency, Hz
     double* e_t0[6] = {};           // static crystal-frame strain (magnets, no elastodynamics)
     // reduction scratch
     double* red_partial = nullptr;
@@ -719,6 +723,8 @@ hipError_t launch_demag_local(gf_ctx* c, const double* const* fm_store, const do
                               const double* const* s2, int one_d);
 
 // reductions (gf_llg.hip)
+hipError_t launch_prescribe_plane(double* const* m, const uint32_t* code, int64_t nx, int64_t ny, int64_t nz,
+                                  const double v[3], hipStream_t s);
 hipError_t launch_sum3(const double* const* f, int64_t n, double* partial, int64_t blocks, hipStream_t s);
 
 // host Newell kernel (gf_newell.cpp)

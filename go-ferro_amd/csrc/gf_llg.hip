@@ -56,10 +56,11 @@ __device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z,
 
 // RK update of one component (see the file comment) from its preloaded m
 // and accumulator; returns the value written to the next store (m itself
-// at stage 4).
+// at stage 4): the RK value, or pv where the cell's value is prescribed.
 template <class I>
 __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, double m, double acc,
-                                           double* const* accp, double* const* out) {
+                                           double* const* accp, double* const* out, bool prescribed = false,
+                                           double pv = 0.) {
     double v;
     if (stage == 1) {
         v = m + k * 0.5;
@@ -73,6 +74,7 @@ __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, dou
     } else {
         v = m + (acc + k / 6.);
     }
+    if (prescribed) v = pv;
     std_nt(out[c], id, v);
     return v;
 }
@@ -80,14 +82,14 @@ __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, dou
 // One FM cell of a stage: H_eff with the static field hs, the LLG
 // increment and the RK update; n receives the next store value.  kLean
 // compiles out the frame rotation and the optional fields (strain, non-
-// uniform and shaped drives, electrodynamic back-action, k output) for
-// contexts that have none of them: their pointers and the rotation
-// matrices then occupy no scalar registers across the cell loop.  mats is
-// the material table, in global memory (GMat) or a workgroup's LDS copy
-// (LMat).
+// uniform and shaped drives, electrodynamic back-action, k output, the
+// prescribed bottom layer) for contexts that have none of them: their
+// pointers and the rotation matrices then occupy no scalar registers
+// across the cell loop.  mats is the material table, in global memory
+// (GMat) or a workgroup's LDS copy (LMat).  bottom: the cell is at k = 0.
 template <bool kLean, class I, class MT>
 __device__ __forceinline__ void fm_cell(const StageArgs& a, MT* mats, const FmIn& in, I id, double hs0, double hs1,
-                                        double hs2, double& n0, double& n1, double& n2) {
+                                        double hs2, double& n0, double& n1, double& n2, bool bottom) {
     MT& M = mats[(in.own & kMatMask) - 1];
 
     const double mgx = in.c[0], mgy = in.c[1], mgz = in.c[2];
@@ -151,9 +153,12 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, MT* mats, const FmIn
     if (!kLean && a.dm_out[0]) {
         std_(a.dm_out[0], id, k0); std_(a.dm_out[1], id, k1); std_(a.dm_out[2], id, k2);
     }
-    n0 = rk_store(a.stage, id, 0, k0, in.m[0], in.acc[0], a.acc, a.out);
-    n1 = rk_store(a.stage, id, 1, k1, in.m[1], in.acc[1], a.acc, a.out);
-    n2 = rk_store(a.stage, id, 2, k2, in.m[2], in.acc[2], a.acc, a.out);
+    // update_prescribed_m{_half,_full,} over the update (magnetic_solver.cpp:
+    // 1526-1609, 1891-1957); the accumulator keeps the LLG increment
+    const bool pres = !kLean && a.prescribe && bottom;
+    n0 = rk_store(a.stage, id, 0, k0, in.m[0], in.acc[0], a.acc, a.out, pres, a.presc_m[0]);
+    n1 = rk_store(a.stage, id, 1, k1, in.m[1], in.acc[1], a.acc, a.out, pres, a.presc_m[1]);
+    n2 = rk_store(a.stage, id, 2, k2, in.m[2], in.acc[2], a.acc, a.out, pres, a.presc_m[2]);
 }
 
 // One FM cell of the unfused stage: coordinates from the linear index,
@@ -171,7 +176,7 @@ __device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
     const double hs2 = hs ? ldd(a.hstat[2], id) : 0.;
     if (!(in.own & kIsFM)) return;
     double n0, n1, n2;
-    fm_cell<false, I>(a, (GMat*)a.mats, in, id, hs0, hs1, hs2, n0, n1, n2);
+    fm_cell<false, I>(a, (GMat*)a.mats, in, id, hs0, hs1, hs2, n0, n1, n2, z == 0);
 }
 
 template <class I>
@@ -303,10 +308,10 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
             double Ms;
             if (kLean) {
                 LMat* tab = (LMat*)ltab;
-                fm_cell<kLean, I>(a, tab, in, id, *h0, *h1, *h2, n0, n1, n2);
+                fm_cell<kLean, I>(a, tab, in, id, *h0, *h1, *h2, n0, n1, n2, k == 0);
                 Ms = tab[(in.own & kMatMask) - 1].Ms;
             } else {
-                fm_cell<kLean, I>(a, (GMat*)a.mats, in, id, *h0, *h1, *h2, n0, n1, n2);
+                fm_cell<kLean, I>(a, (GMat*)a.mats, in, id, *h0, *h1, *h2, n0, n1, n2, k == 0);
                 Ms = mat_of(a.mats, in.own).Ms;
             }
             mx = Ms * n0; my = Ms * n1; mz = Ms * n2;
@@ -512,6 +517,18 @@ __global__ void __launch_bounds__(256) batch_clock_kernel(BatchClock* clk, Stage
     }
 }
 
+// update_prescribed_m (magnetic_solver.cpp:1937-1957): the FM cells of the
+// bottom layer k = 0 take the cone value v.
+__global__ void __launch_bounds__(256) prescribe_plane_kernel(double* m0, double* m1, double* m2,
+                                                              const uint32_t* code, int64_t nxy, int64_t nz,
+                                                              double v0, double v1, double v2) {
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= nxy) return;
+    const int64_t id = col * nz;
+    if (!(code[id] & kIsFM)) return;
+    m0[id] = v0; m1[id] = v1; m2[id] = v2;
+}
+
 // Per-block partial sums of three fields; block b writes partial[3*b + c].
 __global__ void __launch_bounds__(256) sum3_kernel(const double* f0, const double* f1, const double* f2, int64_t n,
                                                     double* partial) {
@@ -554,7 +571,7 @@ hipError_t launch_mag_stage_fused(const StageArgs& a, const FusedZArgs& f, bool 
     const int nz = static_cast<int>(a.nz);
     const int pairs = f.P > 1 ? f.P / 2 : 1;
     const bool lean = !a.rotate && !a.strain[0] && !a.hnu[0] && !a.hdrive[0] && !a.hcell[0] && !a.dm_out[0] &&
-                      a.nmat >= 1 && a.nmat <= kLeanMats;
+                      !a.prescribe && a.nmat >= 1 && a.nmat <= kLeanMats;
     const size_t lds = sizeof(double2) * (nz / 2 + fft_lds_elems(3 * pairs * nz)) +
                        (lean ? sizeof(DevMaterial) * static_cast<size_t>(a.nmat) : 0);
     const unsigned groups = static_cast<unsigned>(a.nx * a.ny / f.P);
@@ -601,6 +618,14 @@ hipError_t launch_afm_batch(const AfmStageArgs* items, const BatchLaunch& L, int
 hipError_t launch_batch_clock(BatchClock* clk, StageArgs* fm, AfmStageArgs* afm, int count, hipStream_t s) {
     hipLaunchKernelGGL(batch_clock_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s, clk, fm,
                        afm, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_prescribe_plane(double* const* m, const uint32_t* code, int64_t nx, int64_t ny, int64_t nz,
+                                  const double v[3], hipStream_t s) {
+    const int64_t nxy = nx * ny;
+    hipLaunchKernelGGL(prescribe_plane_kernel, dim3(static_cast<unsigned>((nxy + 255) / 256)), dim3(256), 0, s,
+                       m[0], m[1], m[2], code, nxy, nz, v[0], v[1], v[2]);
     return hipGetLastError();
 }
 

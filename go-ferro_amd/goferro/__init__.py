@@ -11,6 +11,7 @@ loudly when it is missing: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import os
 import dataclasses
 from pathlib import Path
 from typing import Optional, Sequence
@@ -38,7 +39,7 @@ __all__ = [
 LIB_PATH = Path(__file__).resolve().parent / "libgoferro.so"
 MU0 = 1.256637061e-6  # magnetic_system.h:14
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 FLAG_MAGNETOSTATICS = 1 << 0
 FLAG_MAG_1D = 1 << 1
 FLAG_DEMAG_FACTOR = 1 << 2
@@ -67,6 +68,7 @@ FLAG_ELECTROSTATICS = 1 << 24
 FLAG_SPIN_PUMPING = 1 << 25
 FLAG_FLEXO = 1 << 26
 FLAG_ELASTOSTATIC = 1 << 27
+FLAG_PRESCRIBED_M = 1 << 28
 
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
@@ -114,7 +116,8 @@ KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_con
 # every entry point declared in include/goferro.h
 EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
-    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_em_stage", "gf_field_size", "gf_demag", "gf_step", "gf_average", "gf_get_time", "gf_sync",
+    "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_em_stage", "gf_field_size", "gf_demag",
+    "gf_prescribe_m", "gf_step", "gf_average", "gf_get_time", "gf_sync",
     "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times", "gf_set_material", "gf_set_time",
     "gf_adjoint_forward", "gf_adjoint_backward", "gf_adjoint_stage",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_sync",
@@ -217,6 +220,7 @@ class _Desc(ctypes.Structure):
         ("n_layers", ctypes.c_int32), ("reserved3", ctypes.c_int32),
         ("nz_layer", ctypes.POINTER(ctypes.c_int32)),
         ("scale_elasto", ctypes.c_double),
+        ("precess_angle", ctypes.c_double), ("precess_frequency", ctypes.c_double),
     ]
 
 
@@ -272,6 +276,8 @@ class Problem:
     scale_elec: float = 1.0
     nz_layer: Optional[Sequence[int]] = None       # layer stack bottom to top; None: runs of column (0, 0)
     scale_elasto: float = 1.0
+    precess_angle: float = 0.0                     # degrees (FLAG_PRESCRIBED_M)
+    precess_frequency: float = 0.0                 # Hz
 
     @property
     def n(self) -> int:
@@ -326,6 +332,8 @@ class Problem:
         d.Hdrive_freq = float(self.Hdrive_freq)
         d.scale_elec = float(self.scale_elec)
         d.scale_elasto = float(self.scale_elasto)
+        d.precess_angle = float(self.precess_angle)
+        d.precess_frequency = float(self.precess_frequency)
         keep = [cell, mats]
         if self.nz_layer is not None:
             lay = np.ascontiguousarray(self.nz_layer, dtype=np.int32)
@@ -349,10 +357,14 @@ _lib = None
 
 
 def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
-    """Load libgoferro.so (built in-tree); raise if it is missing."""
+    """Load libgoferro.so (built in-tree); raise if it is missing.  The
+    environment variable GOFERRO_LIB names another build of the same
+    library (a tuning variant) to load instead."""
     global _lib
     if _lib is not None:
         return _lib
+    if path == LIB_PATH and os.environ.get("GOFERRO_LIB"):
+        path = Path(os.environ["GOFERRO_LIB"])
     if not Path(path).exists():
         raise GoFerroError(f"{path} is missing: build it with goferro._build.build() (no CPU fallback exists)")
     lib = ctypes.CDLL(str(path))
@@ -372,6 +384,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_em_stage": ([P, ctypes.c_int], ctypes.c_int),
         "gf_field_size": ([P, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
         "gf_demag": ([P], ctypes.c_int),
+        "gf_prescribe_m": ([P], ctypes.c_int),
         "gf_step": ([P, ctypes.c_int64], ctypes.c_int),
         "gf_average": ([P, ctypes.c_int, dp], ctypes.c_int),
         "gf_get_time": ([P, dp], ctypes.c_int),
@@ -505,6 +518,10 @@ class Context:
 
     def demag(self) -> None:
         self._check(self.lib.gf_demag(self._h), "gf_demag")
+
+    def prescribe_m(self) -> None:
+        """Set the prescribed bottom layer at the current time (start-up)."""
+        self._check(self.lib.gf_prescribe_m(self._h), "gf_prescribe_m")
 
     def step(self, nsteps: int = 1) -> None:
         self._check(self.lib.gf_step(self._h, int(nsteps)), "gf_step")

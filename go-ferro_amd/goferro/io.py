@@ -40,6 +40,7 @@ from . import (
     FLAG_EM_FROM_M, FLAG_EM_FROM_P, FLAG_EMDYNAMIC, FLAG_GAUSSIAN_PULSE, FLAG_HEXT_NONUNIF, FLAG_JF_INPUT,
     FLAG_MAG_1D, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE, FLAG_PEC_XY, FLAG_PEC_Z, FLAG_PLANE_EM,
     FLAG_PRESCRIBE_EEXT, FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FLAG_FLEXO, FLAG_ELASTOSTATIC, FIELD_J_ISHE,
+    FLAG_PRESCRIBED_M,
     Context, GoFerroError, Material, Problem, load_library, split_yee, yee_shapes,
 )
 
@@ -463,8 +464,6 @@ def _unsupported(g: dict, mats: list) -> list[str]:
     any_fe = any(m["if_FE"] for m in mats)
     any_mag = any(m["if_FM"] or m["if_AFM"] for m in mats)
     out = []
-    if g["if_prescribed_m"]:
-        out.append("if_prescribed_m (prescribed precession)")
     if g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
         out.append("the 1D and film elastostatics and external strain (if_elastostatic_1D, if_elastostatic_film, "
                    "if_read_strain_ext)")
@@ -520,6 +519,10 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
     for key, bit in flag_of:
         if g[key]:
             flags |= bit
+    # update_prescribed_m* act inside the FM updates (magnetic_solver.cpp:
+    # 1526-1609), which run only with a ferromagnet
+    if g["if_prescribed_m"] and any(r["if_FM"] for r in raw):
+        flags |= FLAG_PRESCRIBED_M
     # the flexoelectric terms act only with a ferroelectric (if_FE_all &&
     # if_flexo, elastic_dynamic_solver.cpp:63, main.cpp:161-166)
     if g["if_flexo"] and any_fe:
@@ -558,6 +561,7 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
         scale_elec=g["scale_elec"],
         nz_layer=None if g["if_read_struct"] else list(nzl),
         scale_elasto=g["scale_elasto"],
+        precess_angle=g["precess_angle"], precess_frequency=g["precess_frequency"],
     )
     if g["if_prescribe_Hext"]:
         prob.prescribe_Hext = _table(d / "Hext.in", g["num_prescribe_Hext"])
@@ -897,6 +901,11 @@ def run(directory, steps: Optional[int] = None, out_dir=None, restart: Optional[
     else:
         load_inputs(ctx, deck)
         nstep = 0
+        # initialize_device sets the prescribed layer before the initial
+        # statics (main.cpp:113, magnetic_initialize.cpp:1074-1077); a
+        # restart image already holds it at its time
+        if deck.problem.flags & FLAG_PRESCRIBED_M:
+            ctx.prescribe_m()
         # initial statics (main.cpp:120-137): each static field is computed
         # unless it was read; gf_demag computes both, so a field that was
         # read is uploaded again afterwards

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define GF_ABI_VERSION 3
+#define GF_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------ */
 enum gf_status_code {
@@ -109,6 +109,15 @@ typedef int gf_status;
  * every gf_step step (main.cpp:214, 473).  Magnets only, without
  * elastodynamics; power-of-two nx, ny, nz up to 2048. */
 #define GF_FLAG_ELASTOSTATIC     (1u << 27)
+/* if_prescribed_m: the ferromagnetic cells of the bottom layer (k = 0)
+ * precess on a prescribed cone, m = (sqrt(1 - m0^2), m0 sin wt, m0 cos wt)
+ * with m0 = sin(precess_angle / 180 pi) and w = 2 pi precess_frequency
+ * (magnetic_solver.cpp:1891-1957), written over the RK update of every
+ * stage: the stage-2 and stage-3 stores at t - dt/2, the stage-4 store at
+ * t and m itself after the step at t (:1526-1609).  gf_prescribe_m sets
+ * the layer at the current time, as initialize_device does at start-up
+ * (magnetic_initialize.cpp:1074-1077). */
+#define GF_FLAG_PRESCRIBED_M     (1u << 28)
 
 /*
  * One material record.  Field meaning and units follow the reference's
@@ -228,6 +237,10 @@ typedef struct gf_desc {
     const int32_t* nz_layer;
     /* stiffness scale of the elastostatics (read.cpp:65); 0 = 1 */
     double scale_elasto;
+    /* prescribed precession of the bottom layer (GF_FLAG_PRESCRIBED_M,
+     * read.cpp:101) */
+    double precess_angle;           /* degrees */
+    double precess_frequency;       /* Hz */
 } gf_desc;
 
 /* ---- field identifiers for upload/download ------------------------------ */
@@ -346,6 +359,14 @@ gf_status gf_field_size(const gf_ctx* ctx, int field_id, size_t* count);
 /* magnetic_system::get_H_static (magnetostatic_solver.cpp:114-219) or the
  * 1D model get_H_static_1D (:222-249) per the descriptor flags. */
 gf_status gf_demag(gf_ctx* ctx);
+
+/* update_prescribed_m at start-up (magnetic_initialize.cpp:1074-1077, the
+ * body at magnetic_solver.cpp:1937-1957): the ferromagnetic cells of the
+ * bottom layer take the prescribed cone at the context's time, in m and
+ * in its stage store.  Needs GF_FLAG_PRESCRIBED_M and a step boundary;
+ * call it after uploading m and before the initial gf_demag, as the
+ * reference's start-up order does (main.cpp:113-143). */
+gf_status gf_prescribe_m(gf_ctx* ctx);
 
 /* Whole RK4 steps in the protocol of main.cpp:243-483 (time march, four
  * stages with statics between them, statics after the step). */

@@ -45,7 +45,8 @@
  *   3 Liao reading the history one step too old, 4 Liao coefficients of
  *   t1 and t2 swapped, 5 Mur half- and full-stage coefficients swapped,
  *   6 Mur half stages reading the base state for the stage store,
- *   7 elastic Mur half- and full-stage coefficients swapped.
+ *   7 elastic Mur half- and full-stage coefficients swapped, 8 the
+ *   prescribed layer of the stage-2 update at t instead of t - dt/2.
  * The default build (0) plants nothing. */
 #ifndef GFO_PLANT
 #define GFO_PLANT 0
@@ -106,6 +107,7 @@ struct gfo_ctx {
     /* statics */
     double *H[3], *Hnu[3];
     double *Hdrive[3], hdrive_freq;            /* shaped drive (GF_FLAG_HDRIVE), no reference counterpart */
+    double precess_angle, precess_freq;        /* prescribed bottom layer (GF_FLAG_PRESCRIBED_M, read.cpp:101) */
     double er_homo;                            /* electrostatics (GF_FLAG_ELECTROSTATICS) */
     double *e_t0[6];                           /* static crystal-frame strain, ex*t0_crt */
     /* electrodynamics (EMdynamic_system.h): E on edges, H on faces */
@@ -617,6 +619,8 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     for (int a = 0; a < 3; a++) c->per[a] = d->periodic[a] != 0;
     c->flags = d->flags; c->dt = d->dt; c->time = 0.;
     c->hdrive_freq = d->Hdrive_freq;
+    c->precess_angle = d->precess_angle;
+    c->precess_freq = d->precess_frequency;
     /* er_homo = max r_permittivity * scale_elec (ferroelectric_initialize.cpp:311-318) */
     c->er_homo = 0.;
     for (int t = 0; t < d->n_materials; t++)
@@ -1148,15 +1152,58 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
     if (c->spin_on) spin_pump(c, stage);     /* magnetic_solver.cpp:226-228 */
 }
 
+/* update_prescribed_m_half / _full / update_prescribed_m
+ * (magnetic_solver.cpp:1891-1957): every FM cell of the bottom layer k = 0
+ * takes the cone (sqrt(1 - m0^2), m0 sin wt, m0 cos wt) at time t. */
+static void prescribe_m(gfo_ctx* c, double* const* dst, double t) {
+    double m0 = sin(c->precess_angle / 180. * PI);
+    double omega = 2. * PI * c->precess_freq;
+    for (int64_t i = 0; i < c->nx; i++)
+        for (int64_t j = 0; j < c->ny; j++) {
+            int64_t id = ID(c, i, j, 0);
+            if (!isFM(c, id)) continue;
+            dst[0][id] = sqrt((1. - m0 * m0));
+            dst[1][id] = m0 * sin(omega * t);
+            dst[2][id] = m0 * cos(omega * t);
+        }
+}
+
+/* The time of the prescribed cone written by the update of a stage:
+ * update_m_RK1 and _RK2 call the half-step form (t - dt/2), update_m_RK3
+ * and update_m the full-step one (t) (magnetic_solver.cpp:1526-1609). */
+static double prescribe_time(const gfo_ctx* c, int stage) {
+#if GFO_PLANT == 8
+    return stage == 1 ? c->time - c->dt * 0.5 : c->time;
+#else
+    return stage <= 2 ? c->time - c->dt * 0.5 : c->time;
+#endif
+}
+
 /* magnetic_solver.cpp:1515-1636 */
 void gfo_mag_update(gfo_ctx* c, int stage) {
+    int presc = (c->flags & GF_FLAG_PRESCRIBED_M) != 0;
     if (stage == 4 && c->spin_on) spin_pump_step(c);
     for (int a = 0; a < 3; a++) {
         double *m = c->m[a], *s = c->ms[a];
         if (stage == 1 || stage == 2) { const double* k = c->k[stage - 1][a]; for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i] * 0.5; }
         else if (stage == 3) { const double* k = c->k[2][a]; for (int64_t i = 0; i < c->n; i++) s[i] = m[i] + k[i]; }
-        else { const double* dm = c->dm[a]; for (int64_t i = 0; i < c->n; i++) { m[i] = m[i] + dm[i]; s[i] = m[i]; } }
+        else { const double* dm = c->dm[a]; for (int64_t i = 0; i < c->n; i++) m[i] = m[i] + dm[i]; }
     }
+    if (stage < 4) {
+        if (presc) prescribe_m(c, c->ms, prescribe_time(c, stage));
+        return;
+    }
+    /* update_m: m += dm, the prescribed layer, then the store copies m */
+    if (presc) prescribe_m(c, c->m, prescribe_time(c, 4));
+    for (int a = 0; a < 3; a++) memcpy(c->ms[a], c->m[a], sizeof(double) * (size_t)c->n);
+}
+
+/* initialize_device (magnetic_initialize.cpp:1074-1086): the prescribed
+ * layer at the current time, then the store copies m */
+void gfo_prescribe_m(gfo_ctx* c) {
+    if (!(c->flags & GF_FLAG_PRESCRIBED_M) || !c->if_FM_all) return;
+    prescribe_m(c, c->m, c->time);
+    for (int a = 0; a < 3; a++) memcpy(c->ms[a], c->m[a], sizeof(double) * (size_t)c->n);
 }
 
 /* ------------------------------------------------------ AFM stage */

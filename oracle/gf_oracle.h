@@ -57,6 +57,7 @@ void gfo_elastic_update(gfo_ctx* c, int stage); /* update_v_RKs + update_u_RKs *
 void gfo_em_get(gfo_ctx* c, int stage);       /* get_dE_RK + get_dH_RK + Jp/Jf */
 void gfo_em_update(gfo_ctx* c, int stage);    /* update_DE_RK + update_DH_RK  */
 void gfo_demag(gfo_ctx* c);
+void gfo_prescribe_m(gfo_ctx* c);              /* update_prescribed_m at start-up */
 void gfo_estatic(gfo_ctx* c);
 void gfo_elastostatic(gfo_ctx* c);             /* static strain of the magnets' eigenstrain */                   /* get_H_static / _1D           */
 void gfo_step(gfo_ctx* c, int64_t nsteps);    /* main.cpp:243-483             */

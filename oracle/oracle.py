@@ -59,7 +59,7 @@ def lib(path=None) -> ctypes.CDLL:
         "gfo_fe_get": ([P, ctypes.c_int], None), "gfo_fe_update": ([P, ctypes.c_int], None),
         "gfo_elastic_get": ([P, ctypes.c_int], None), "gfo_elastic_update": ([P, ctypes.c_int], None),
         "gfo_em_get": ([P, ctypes.c_int], None), "gfo_em_update": ([P, ctypes.c_int], None),
-        "gfo_demag": ([P], None), "gfo_step": ([P, i64], None),
+        "gfo_demag": ([P], None), "gfo_step": ([P, i64], None), "gfo_prescribe_m": ([P], None),
         "gfo_average": ([P, ctypes.c_int, dp], None), "gfo_time": ([P], ctypes.c_double),
         "gfo_newell_real": ([i64, i64, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp], None),
         "gfo_newell_F2": ([ctypes.c_double] * 3, ctypes.c_double),
@@ -155,6 +155,9 @@ class Oracle:
 
     def demag(self):
         self.L.gfo_demag(self._h)
+
+    def prescribe_m(self):
+        self.L.gfo_prescribe_m(self._h)
 
     def step(self, n=1):
         self.L.gfo_step(self._h, int(n))

@@ -18,7 +18,8 @@ import pytest
 
 from goferro import (FIELD_HEXT_NONUNIF, FLAG_DEMAG_FACTOR, FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS,
                      FLAG_ELECTROSTATICS, FLAG_EMDYNAMIC, FLAG_MAGNETOSTATICS, FLAG_ORIENT_ROTATE,
-                     FLAG_PRESCRIBE_HEXT, FLAG_SPIN_PUMPING, FLAG_FLEXO, GoFerroError, Material, workloads)
+                     FLAG_PRESCRIBE_HEXT, FLAG_PRESCRIBED_M, FLAG_SPIN_PUMPING, FLAG_FLEXO, GoFerroError, Material,
+                     workloads)
 from goferro import io as gio
 
 DECK = Path(__file__).resolve().parent / "golden" / "deck_fm"
@@ -187,8 +188,15 @@ def test_flexo_deck(tmp_path):
     assert not gio.read_deck(d2).problem.flags & FLAG_FLEXO
 
 
+def test_prescribed_m_deck(tmp_path):
+    d = deck_copy(tmp_path, edits=[("if_prescribed_m", "true 30 2.5e10")])
+    p = gio.read_deck(d).problem
+    assert p.flags & FLAG_PRESCRIBED_M
+    assert (p.precess_angle, p.precess_frequency) == (30.0, 2.5e10)
+
+
 @pytest.mark.parametrize("key,text,what", [
-    ("if_prescribed_m", "true 0 0", "if_prescribed_m"), ("if_elastostatic ", "true true", "elastostatics"),
+    ("if_elastostatic ", "true true", "elastostatics"),
     ("if_input_elastoforce", "true", "if_input_elastoforce")])
 def test_unsupported_options_are_named(tmp_path, key, text, what):
     d = deck_copy(tmp_path, edits=[(key, text)])
