@@ -500,6 +500,8 @@ __global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __re
 // sinusoidal drive at each stage time t - dt, t - dt/2, t - dt/2, t
 // (magnetic_solver.cpp:1834-1861, magnetic_AFM_solver.cpp:61-64) written
 // into the stage arguments of both magnet kinds.
+// A batch without FM (or AFM) instances has no table of that kind: its
+// pointer is null and is not written.
 __global__ void __launch_bounds__(256) batch_clock_kernel(BatchClock* clk, StageArgs* fm, AfmStageArgs* afm,
                                                            int count) {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -511,8 +513,8 @@ __global__ void __launch_bounds__(256) batch_clock_kernel(BatchClock* clk, Stage
         const double ts = s == 1 ? t - c.dt : (s == 4 ? t : t - c.dt * 0.5);
         for (int k = 0; k < 3; k++) {
             const double h = c.stat[k] + c.altr[k] * sin(2. * kPi * c.freq[k] * ts);
-            fm[4 * i + s - 1].hext[k] = h;
-            afm[4 * i + s - 1].hext[k] = h;
+            if (fm) fm[4 * i + s - 1].hext[k] = h;
+            if (afm) afm[4 * i + s - 1].hext[k] = h;
         }
     }
 }

@@ -190,6 +190,22 @@ def test_batch_sweep_instances_step():
     assert np.isfinite(m1).all()
 
 
+def test_batch_of_one_magnet_kind():
+    """Batches with only FM or only AFM instances have no stage table of the
+    other kind; the clock kernel must leave the missing table alone."""
+    say("case fm-only and afm-only batches")
+    for inst in ([fm_instance((8, 6, 4), 51), fm_instance((5, 9, 3), 52)],
+                 [afm_instance((8, 8, 4), 53), afm_instance((3, 5, 7), 54)]):
+        A, B = make(inst), make(inst)
+        batch = Batch(A)
+        batch.step(3)
+        for c in B:
+            c.step(3)
+        batch.sync()
+        compare(A, B, inst, "one kind")
+        batch.close()
+
+
 def test_batch_follows_uploads_made_after_create():
     """Hext_nonunif and Hstat uploaded after the batch exists, and a stage-1
     store that is not m (an uploaded M_STORE), act on the next batch step as
