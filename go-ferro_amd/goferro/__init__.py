@@ -117,7 +117,7 @@ KERNEL_CLASSES = ["mag_stage", "afm_stage", "fft_z_fwd", "fft_y_fwd", "fft_x_con
 EXPORTED_SYMBOLS = [
     "gf_create", "gf_destroy", "gf_last_error", "gf_upload", "gf_download", "gf_time_advance",
     "gf_mag_stage", "gf_afm_stage", "gf_fe_stage", "gf_elastic_stage", "gf_em_stage", "gf_field_size", "gf_demag",
-    "gf_prescribe_m", "gf_step", "gf_average", "gf_get_time", "gf_sync",
+    "gf_prescribe_m", "gf_step", "gf_average", "gf_average_noalloc", "gf_get_time", "gf_sync",
     "gf_get_stream", "gf_set_kernel_timing", "gf_kernel_times", "gf_set_material", "gf_set_time",
     "gf_adjoint_forward", "gf_adjoint_backward", "gf_adjoint_stage",
     "gf_batch_create", "gf_batch_destroy", "gf_batch_last_error", "gf_batch_step", "gf_batch_sync",
@@ -387,6 +387,7 @@ def load_library(path: Path = LIB_PATH) -> ctypes.CDLL:
         "gf_prescribe_m": ([P], ctypes.c_int),
         "gf_step": ([P, ctypes.c_int64], ctypes.c_int),
         "gf_average": ([P, ctypes.c_int, dp], ctypes.c_int),
+        "gf_average_noalloc": ([P, ctypes.c_int, P], ctypes.c_int),
         "gf_get_time": ([P, dp], ctypes.c_int),
         "gf_sync": ([P], ctypes.c_int),
         "gf_get_stream": ([P, ctypes.POINTER(P)], ctypes.c_int),

@@ -377,6 +377,18 @@ gf_status gf_step(gf_ctx* ctx, int64_t nsteps);
  * all cells divided by the FM, AFM or FE cell count. */
 gf_status gf_average(gf_ctx* ctx, int field_id, double out[3]);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory or wait on the host (code under HIP graph capture, serving loops
+ * that forbid hipMalloc).  The average of gf_average, bit for bit, written
+ * to out_dev (device memory, 3 doubles) on the context stream; it returns
+ * without waiting (gf_sync or later work on that stream orders the read).
+ * The partial sums live in a fixed 24 KiB device array per GPU, resolved
+ * on the first call on that GPU and held until the process ends, so
+ * nothing is allocated; at most 1024 partial sums per component (every
+ * context fits).  Calls that overlap on one GPU (different streams or
+ * threads) race on that array; calls ordered on one stream do not. */
+gf_status gf_average_noalloc(gf_ctx* ctx, int field_id, double* out_dev);
+
 /* Current simulated time (global_parameters::time_device). */
 gf_status gf_get_time(const gf_ctx* ctx, double* t);
 
