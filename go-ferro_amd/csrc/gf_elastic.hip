@@ -16,7 +16,8 @@
 //
 // RK4 bookkeeping keeps a state accumulator per field, acc = y + k1/6,
 // acc = acc + k2/3, acc = acc + k3/3, y = acc + k4/6, which is the
-// reference's left-to-right y + k1/6 + k2/3 + k3/3 + k4/6.
+// reference's left-to-right y + k1/6 + k2/3 + k3/3 + k4/6 (the weights applied
+// as products with the rounded reciprocals, within about an ulp of the quotients).
 #include "gf_internal.h"
 
 namespace gf {
@@ -304,22 +305,22 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
         const double v = a.v[c][id], u = a.u[c][id];
         if (stage == 1) {
             a.vs[c][id] = v + dv[c] * 0.5;
-            a.accv[c][id] = v + dv[c] / 6.;
+            a.accv[c][id] = v + dv[c] * kSixth;
             a.us[c][id] = u + du * 0.5;
-            a.accu[c][id] = u + du / 6.;
+            a.accu[c][id] = u + du * kSixth;
         } else if (stage == 2) {
             a.vs[c][id] = v + dv[c] * 0.5;
-            a.accv[c][id] = a.accv[c][id] + dv[c] / 3.;
+            a.accv[c][id] = a.accv[c][id] + dv[c] * kThird;
             a.us[c][id] = u + du * 0.5;
-            a.accu[c][id] = a.accu[c][id] + du / 3.;
+            a.accu[c][id] = a.accu[c][id] + du * kThird;
         } else if (stage == 3) {
             a.vs[c][id] = v + dv[c];
-            a.accv[c][id] = a.accv[c][id] + dv[c] / 3.;
+            a.accv[c][id] = a.accv[c][id] + dv[c] * kThird;
             a.us[c][id] = u + du;
-            a.accu[c][id] = a.accu[c][id] + du / 3.;
+            a.accu[c][id] = a.accu[c][id] + du * kThird;
         } else {
-            const double vn = a.accv[c][id] + dv[c] / 6.;
-            const double un = a.accu[c][id] + du / 6.;
+            const double vn = a.accv[c][id] + dv[c] * kSixth;
+            const double un = a.accu[c][id] + du * kSixth;
             a.v[c][id] = vn;
             a.vs[c][id] = vn;
             a.u[c][id] = un;

@@ -31,15 +31,15 @@ __device__ __forceinline__ void rk_state(int stage, int64_t i, double k, double*
     const double y = base[i];
     if (stage == 1) {
         out[i] = y + k * 0.5;
-        acc[i] = y + k / 6.;
+        acc[i] = y + k * kSixth;
     } else if (stage == 2) {
         out[i] = y + k * 0.5;
-        acc[i] = acc[i] + k / 3.;
+        acc[i] = acc[i] + k * kThird;
     } else if (stage == 3) {
         out[i] = y + k;
-        acc[i] = acc[i] + k / 3.;
+        acc[i] = acc[i] + k * kThird;
     } else {
-        base[i] = acc[i] + k / 6.;
+        base[i] = acc[i] + k * kSixth;
     }
 }
 
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(256) spin_pump_kernel(SpinArgs a) {
     // the increment: the stage's k, or at the step boundary acc + k4/6 (the
     // RK combination in the order the stage-4 update forms it)
     auto inc = [&](const double* const* d, const double* const* acc, int c) {
-        return acc[0] ? acc[c][is] + d[c][is] / 6. : d[c][is];
+        return acc[0] ? acc[c][is] + d[c][is] * kSixth : d[c][is];
     };
     double sx = 0., sy = 0.;
     if (a.fs[0]) {

@@ -7,7 +7,8 @@
 //   stage 1..3: next = y + c_s k, acc = y + k/6 | acc + k/3
 //   stage 4:    y = acc + k/6
 // which is the reference's left-to-right y + k1/6 + k2/3 + k3/3 + k4/6.
-// Powers are products instead of pow(); the difference is a few ulp.
+// Powers are products instead of pow(), and the RK weights products with the
+// rounded reciprocals instead of quotients; the difference is a few ulp.
 #include "gf_internal.h"
 
 namespace gf {
@@ -38,15 +39,15 @@ __device__ __forceinline__ void rk(int stage, int64_t id, double k, const double
     const double y = base[id];
     if (stage == 1) {
         out[id] = y + k * 0.5;
-        acc[id] = y + k / 6.;
+        acc[id] = y + k * kSixth;
     } else if (stage == 2) {
         out[id] = y + k * 0.5;
-        acc[id] = acc[id] + k / 3.;
+        acc[id] = acc[id] + k * kThird;
     } else if (stage == 3) {
         out[id] = y + k;
-        acc[id] = acc[id] + k / 3.;
+        acc[id] = acc[id] + k * kThird;
     } else {
-        out[id] = acc[id] + k / 6.;
+        out[id] = acc[id] + k * kSixth;
     }
 }
 

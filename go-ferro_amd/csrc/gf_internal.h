@@ -23,6 +23,11 @@ namespace gf {
 
 constexpr double kMu0 = 1.256637061e-6;               // magnetic_system.h:14
 constexpr double kPi = 3.14159265358979323846;        // magnetic_system.h:16
+// RK4 weights 1/6 and 1/3 as rounded reciprocals: the stage updates multiply
+// by them instead of dividing (an FP64 division is about ten instructions);
+// each product is within about an ulp of the quotient.
+constexpr double kSixth = 1. / 6.;
+constexpr double kThird = 1. / 3.;
 constexpr double kUnitE = 1.602176634e-19;          // magnetic_system.h:15
 
 // cell code bits

@@ -9,8 +9,9 @@
 //   stage 3:  k = f(s3);     next = m + k;    acc = acc + k/3
 //   stage 4:  k = f(s4);     m = m + (acc + k/6)
 // The accumulator is summed in the reference's order
-// (((k1/6 + k2/3) + k3/3) + k4/6), so m(t+dt) keeps its rounding; the four
-// k-arrays and the separate combine/update/copy launches disappear.  The
+// (((k1/6 + k2/3) + k3/3) + k4/6), the weights applied as products with the
+// rounded reciprocals (within about an ulp of the reference's quotients);
+// the four k-arrays and the separate combine/update/copy launches disappear.  The
 // stencil store is read-only within a stage and the next store goes to the
 // other buffer, so physics that share a stage still see the old store.
 #include "gf_llg_dev.h"
@@ -64,15 +65,15 @@ __device__ __forceinline__ double rk_store(int stage, I id, int c, double k, dou
     double v;
     if (stage == 1) {
         v = m + k * 0.5;
-        std_nt(accp[c], id, k / 6.);
+        std_nt(accp[c], id, k * kSixth);
     } else if (stage == 2) {
         v = m + k * 0.5;
-        std_nt(accp[c], id, acc + k / 3.);
+        std_nt(accp[c], id, acc + k * kThird);
     } else if (stage == 3) {
         v = m + k;
-        std_nt(accp[c], id, acc + k / 3.);
+        std_nt(accp[c], id, acc + k * kThird);
     } else {
-        v = m + (acc + k / 6.);
+        v = m + (acc + k * kSixth);
     }
     if (prescribed) v = pv;
     std_nt(out[c], id, v);
