@@ -980,8 +980,6 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     if (!st && (c->flags & GF_FLAG_ELECTROSTATICS)) {
         if (!c->any_fe) {
             st = set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics needs a ferroelectric material");
-        } else if (c->demag.ready) {
-            st = set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics cannot be combined with FFT magnetostatics");
         } else {
             // er_homo: the largest relative permittivity times scale_elec
             // (ferroelectric_initialize.cpp:311-318)
@@ -2014,7 +2012,7 @@ gf_status gf_demag(gf_ctx* c) {
     }
     if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->any_fe) {
         // initial get_E_static (main.cpp:120-131)
-        if (!c->demag.ready) return set_error(c, GF_ERR_STATE, "FFT electrostatics was not set up");
+        if (!c->eplan.ready) return set_error(c, GF_ERR_STATE, "FFT electrostatics was not set up");
         if (gf_status st = elec_run(c, fe_pstore(c, c->fe_store_sel))) return st;
     }
     return mag_statics(c);

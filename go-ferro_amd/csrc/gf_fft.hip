@@ -441,9 +441,81 @@ unsigned blocks_for(int64_t n, int p) { return static_cast<unsigned>((n + p - 1)
 
 }  // namespace
 
+// Twiddle table of n entries w[t] = exp(-2 pi i t / n) for the mixed-radix
+// passes.
+static gf_status make_table(gf_ctx* c, Twiddles& t, int64_t n) {
+    t.n = n;
+    std::vector<double2> h(static_cast<size_t>(n));
+    for (int64_t j = 0; j < n; j++) {
+        const double ang = 2.0 * kPi * static_cast<double>(j) / static_cast<double>(n);
+        h[static_cast<size_t>(j)] = make_double2(std::cos(ang), -std::sin(ang));
+    }
+    gf_status st = hip_check(c, hipMalloc(&t.dev, sizeof(double2) * n), "hipMalloc twiddles");
+    if (st) return st;
+    return hip_check(c, hipMemcpy(t.dev, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice), "copy twiddles");
+}
+
+// Radices of n from {4, 2, 3, 5, 7}, or false when n has another prime
+// factor.
+static bool factor(int64_t n, FftFactors& f) {
+    f = FftFactors{};
+    f.n = static_cast<int>(n);
+    int64_t m = n;
+    auto push = [&](int r) {
+        if (f.count >= kMaxRadices) return false;
+        f.r[f.count++] = r;
+        m /= r;
+        return true;
+    };
+    while (m % 4 == 0) if (!push(4)) return false;
+    while (m % 2 == 0) if (!push(2)) return false;
+    for (int r : {3, 5, 7})
+        while (m % r == 0) if (!push(r)) return false;
+    return m == 1;
+}
+
+// Pencils per workgroup of the mixed-radix passes: as many lines as keep
+// the two LDS buffers within 2048 elements (z and y) or 1024 (x, three
+// components), a power of two, at most 16.
+static int any_pencils(int64_t len, int64_t budget, int64_t other) {
+    int64_t p = 1;
+    while (p * 2 <= 16 && p * 2 * len <= budget && p * 2 <= other) p *= 2;
+    return static_cast<int>(p);
+}
+
+static AnyPassArgs any_args(const gf_ctx* c, const DemagPlan& d) {
+    AnyPassArgs a{};
+    a.code = c->d_code; a.mats = c->d_mats;
+    a.spec = d.spec_m; a.spec2 = d.spec_b;
+    a.kre = d.kre; a.kim = d.kim;
+    a.nh = c->nx * c->ny * d.nzh;
+    a.nhb = a.nh;
+    a.den = -c->dx * c->dy * c->dz;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(d.nzh);
+    return a;
+}
+
 // Forward pipeline helpers shared by the tensor set-up and the convolution.
-static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* const* rawp, const double* const* fm,
+static hipError_t run_zfwd(gf_ctx* c, DemagPlan& d, bool raw, const double* const* rawp, const double* const* fm,
                            const double* const* s1, const double* const* s2, int comps) {
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_FFT_Z_FWD, &ev);
+    if (d.general) {
+        AnyPassArgs a = any_args(c, d);
+        for (int i = 0; i < 3; i++) {
+            a.fm[i] = fm ? fm[i] : nullptr;
+            a.a1[i] = s1 ? s1[i] : nullptr;
+            a.a2[i] = s2 ? s2[i] : nullptr;
+            a.raw[i] = rawp ? rawp[i] : nullptr;
+        }
+        a.raw_in = raw ? 1 : 0;
+        a.f = d.fz; a.tw = d.gtz.dev;
+        a.pencils = any_pencils(c->nz, 2048, c->ny);
+        dim3 grid(blocks_for(c->ny, a.pencils), static_cast<unsigned>(c->nx), static_cast<unsigned>(comps));
+        const hipError_t e = launch_any_pass(AnyPass::kZFwd, a, grid, c->stream);
+        timer_end(c, GF_K_FFT_Z_FWD, ev);
+        return e;
+    }
     const Shapes sh = shapes(c);
     ZArgs a{};
     a.code = c->d_code; a.mats = c->d_mats;
@@ -453,14 +525,12 @@ static hipError_t run_zfwd(gf_ctx* c, bool raw, const double* const* rawp, const
         a.a2[i] = s2 ? s2[i] : nullptr;
     }
     for (int i = 0; i < 3; i++) a.raw[i] = rawp ? rawp[i] : nullptr;
-    a.out = c->demag.spec_m; a.tw = c->demag.twz.dev;
-    a.nh = c->nx * c->ny * c->demag.nzh;
+    a.out = d.spec_m; a.tw = d.twz.dev;
+    a.nh = c->nx * c->ny * d.nzh;
     a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
-    a.nzh = static_cast<int>(c->demag.nzh);
+    a.nzh = static_cast<int>(d.nzh);
     a.logny = ilog2(c->ny); a.lognz = ilog2(c->nz); a.pencils = sh.pz; a.logp = ilog2(sh.pz);
     dim3 grid(blocks_for(c->ny, sh.pz), static_cast<unsigned>(c->nx), static_cast<unsigned>(comps));
-    hipEvent_t ev = nullptr;
-    timer_begin(c, GF_K_FFT_Z_FWD, &ev);
     if (raw) hipLaunchKernelGGL(zfwd_kernel<true>, grid, dim3(kThreads), sh.lds_z, c->stream, a);
     else hipLaunchKernelGGL(zfwd_kernel<false>, grid, dim3(kThreads), sh.lds_z, c->stream, a);
     timer_end(c, GF_K_FFT_Z_FWD, ev);
@@ -475,62 +545,96 @@ struct KzRange {
     int64_t nhb;
 };
 
-static KzRange whole_range(const gf_ctx* c) {
-    return KzRange{0, static_cast<int>(c->demag.nzh), 0, c->nx * c->ny * c->demag.nzh};
+static KzRange whole_range(const gf_ctx* c, const DemagPlan& d) {
+    return KzRange{0, static_cast<int>(d.nzh), 0, c->nx * c->ny * d.nzh};
 }
 
-static hipError_t run_y(gf_ctx* c, bool inverse, int comps, KzRange kr) {
-    const Shapes sh = shapes(c);
-    YArgs a{};
-    a.in = inverse ? c->demag.spec_b : c->demag.spec_m;
-    a.out = inverse ? c->demag.spec_a : c->demag.spec_b;
-    a.tw = c->demag.twy.dev;
-    a.nh = c->nx * c->ny * c->demag.nzh;
-    a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
-    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
-    a.logny = ilog2(c->ny); a.pencils = sh.py; a.logp = ilog2(sh.py);
-    dim3 grid(blocks_for(c->nx, sh.py), static_cast<unsigned>(kr.cnt), static_cast<unsigned>(comps));
+static hipError_t run_y(gf_ctx* c, DemagPlan& d, bool inverse, int comps, KzRange kr) {
     hipEvent_t ev = nullptr;
     const int cls = inverse ? GF_K_FFT_Y_INV : GF_K_FFT_Y_FWD;
     timer_begin(c, cls, &ev);
+    if (d.general) {
+        AnyPassArgs a = any_args(c, d);
+        a.spec = inverse ? d.spec_a : d.spec_m;
+        a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
+        a.f = d.fy; a.tw = d.gty.dev;
+        a.pencils = any_pencils(c->ny, 2048, c->nx);
+        dim3 grid(blocks_for(c->nx, a.pencils), static_cast<unsigned>(kr.cnt), static_cast<unsigned>(comps));
+        const hipError_t e = launch_any_pass(inverse ? AnyPass::kYInv : AnyPass::kYFwd, a, grid, c->stream);
+        timer_end(c, cls, ev);
+        return e;
+    }
+    const Shapes sh = shapes(c);
+    YArgs a{};
+    a.in = inverse ? d.spec_b : d.spec_m;
+    a.out = inverse ? d.spec_a : d.spec_b;
+    a.tw = d.twy.dev;
+    a.nh = c->nx * c->ny * d.nzh;
+    a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(d.nzh);
+    a.logny = ilog2(c->ny); a.pencils = sh.py; a.logp = ilog2(sh.py);
+    dim3 grid(blocks_for(c->nx, sh.py), static_cast<unsigned>(kr.cnt), static_cast<unsigned>(comps));
     if (inverse) hipLaunchKernelGGL(yinv_kernel, grid, dim3(kThreads), sh.lds_y, c->stream, a);
     else hipLaunchKernelGGL(yfwd_kernel, grid, dim3(kThreads), sh.lds_y, c->stream, a);
     timer_end(c, cls, ev);
     return hipGetLastError();
 }
 
-static hipError_t run_x(gf_ctx* c, bool conv, KzRange kr, bool adjoint = false) {
+static hipError_t run_x(gf_ctx* c, DemagPlan& d, bool conv, KzRange kr, bool adjoint = false) {
+    hipEvent_t ev = nullptr;
+    timer_begin(c, GF_K_FFT_X_CONV, &ev);
+    if (d.general) {
+        AnyPassArgs a = any_args(c, d);
+        a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
+        a.conv = conv ? 1 : 0;
+        a.adjoint = adjoint ? 1 : 0;
+        a.f = d.fx; a.tw = d.gtx.dev;
+        a.pencils = any_pencils(c->nx, conv ? 1024 / 3 : 1024, c->ny);
+        dim3 grid(blocks_for(c->ny, a.pencils), static_cast<unsigned>(kr.cnt));
+        const hipError_t e = launch_any_pass(AnyPass::kXConv, a, grid, c->stream);
+        timer_end(c, GF_K_FFT_X_CONV, ev);
+        return e;
+    }
     const Shapes sh = shapes(c);
     XArgs a{};
     a.adjoint = adjoint ? 1 : 0;
-    a.data = c->demag.spec_b; a.kre = c->demag.kre; a.kim = c->demag.kim; a.tw = c->demag.twx.dev;
-    a.nh = c->nx * c->ny * c->demag.nzh;
+    a.data = d.spec_b; a.kre = d.kre; a.kim = d.kim; a.tw = d.twx.dev;
+    a.nh = c->nx * c->ny * d.nzh;
     a.nhb = kr.nhb; a.kz0 = kr.k0; a.kzb0 = kr.kb0;
     a.den = -c->dx * c->dy * c->dz;
-    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(c->demag.nzh);
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nzh = static_cast<int>(d.nzh);
     a.lognx = ilog2(c->nx); a.pencils = sh.px; a.logp = ilog2(sh.px);
     dim3 grid(blocks_for(c->ny, sh.px), static_cast<unsigned>(kr.cnt));
-    hipEvent_t ev = nullptr;
-    timer_begin(c, GF_K_FFT_X_CONV, &ev);
     if (conv) hipLaunchKernelGGL(xconv_kernel<true>, grid, dim3(kThreads), sh.lds_x, c->stream, a);
     else hipLaunchKernelGGL(xconv_kernel<false>, grid, dim3(kThreads), sh.lds_x, c->stream, a);
     timer_end(c, GF_K_FFT_X_CONV, ev);
     return hipGetLastError();
 }
 
-static hipError_t run_zinv(gf_ctx* c, double* const* out = nullptr) {
-    const Shapes sh = shapes(c);
-    ZInvArgs a{};
-    a.in = c->demag.spec_a;
-    for (int i = 0; i < 3; i++) a.out[i] = out ? out[i] : c->hstat[i];
-    a.tw = c->demag.twz.dev;
-    a.nh = c->nx * c->ny * c->demag.nzh;
-    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
-    a.nzh = static_cast<int>(c->demag.nzh);
-    a.lognz = ilog2(c->nz); a.pencils = sh.pz; a.logp = ilog2(sh.pz);
-    dim3 grid(blocks_for(c->ny, sh.pz), static_cast<unsigned>(c->nx), 3);
+static hipError_t run_zinv(gf_ctx* c, DemagPlan& d, double* const* out = nullptr) {
     hipEvent_t ev = nullptr;
     timer_begin(c, GF_K_FFT_Z_INV, &ev);
+    if (d.general) {
+        AnyPassArgs a = any_args(c, d);
+        a.spec = d.spec_a;
+        for (int i = 0; i < 3; i++) a.out[i] = out ? out[i] : c->hstat[i];
+        a.f = d.fz; a.tw = d.gtz.dev;
+        a.pencils = any_pencils(c->nz, 2048, c->ny);
+        dim3 grid(blocks_for(c->ny, a.pencils), static_cast<unsigned>(c->nx), 3);
+        const hipError_t e = launch_any_pass(AnyPass::kZInv, a, grid, c->stream);
+        timer_end(c, GF_K_FFT_Z_INV, ev);
+        return e;
+    }
+    const Shapes sh = shapes(c);
+    ZInvArgs a{};
+    a.in = d.spec_a;
+    for (int i = 0; i < 3; i++) a.out[i] = out ? out[i] : c->hstat[i];
+    a.tw = d.twz.dev;
+    a.nh = c->nx * c->ny * d.nzh;
+    a.nx = static_cast<int>(c->nx); a.ny = static_cast<int>(c->ny); a.nz = static_cast<int>(c->nz);
+    a.nzh = static_cast<int>(d.nzh);
+    a.lognz = ilog2(c->nz); a.pencils = sh.pz; a.logp = ilog2(sh.pz);
+    dim3 grid(blocks_for(c->ny, sh.pz), static_cast<unsigned>(c->nx), 3);
     hipLaunchKernelGGL(zinv_kernel, grid, dim3(kThreads), sh.lds_z, c->stream, a);
     timer_end(c, GF_K_FFT_Z_INV, ev);
     return hipGetLastError();
@@ -538,12 +642,23 @@ static hipError_t run_zinv(gf_ctx* c, double* const* out = nullptr) {
 
 static bool pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
 
-gf_status demag_setup(gf_ctx* c) {
-    if (!pow2(c->nx) || !pow2(c->ny) || !pow2(c->nz))
-        return set_error(c, GF_ERR_UNSUPPORTED, "FFT magnetostatics needs power-of-two nx, ny, nz");
-    if (c->nx > 1024 || c->ny > 2048 || c->nz > 2048)
-        return set_error(c, GF_ERR_UNSUPPORTED, "FFT magnetostatics supports nx <= 1024, ny, nz <= 2048");
-    DemagPlan& d = c->demag;
+// The work buffers, the tensor spectra storage and the twiddles of a plan:
+// radix-2 tables for power-of-two grids, or the mixed-radix factors and
+// full tables when a size is not a power of two (prime factors 2, 3, 5, 7;
+// the LDS of the x pass, three components of one line through two
+// buffers, bounds nx).
+static gf_status plan_alloc(gf_ctx* c, DemagPlan& d, const char* what) {
+    const std::string name(what);
+    d.general = !pow2(c->nx) || !pow2(c->ny) || !pow2(c->nz);
+    if (d.general) {
+        if (!factor(c->nx, d.fx) || !factor(c->ny, d.fy) || !factor(c->nz, d.fz))
+            return set_error(c, GF_ERR_UNSUPPORTED,
+                             name + " needs nx, ny, nz whose prime factors are 2, 3, 5 and 7");
+        if (c->nx > 1024 || c->ny > 2048 || c->nz > 2048)
+            return set_error(c, GF_ERR_UNSUPPORTED, name + " supports nx <= 1024, ny, nz <= 2048");
+    } else if (c->nx > 1024 || c->ny > 2048 || c->nz > 2048) {
+        return set_error(c, GF_ERR_UNSUPPORTED, name + " supports nx <= 1024, ny, nz <= 2048");
+    }
     d.nzh = c->nz / 2 + 1;
     const int64_t nh = c->nx * c->ny * d.nzh;
     gf_status st;
@@ -552,9 +667,23 @@ gf_status demag_setup(gf_ctx* c) {
     if ((st = hip_check(c, hipMalloc(&d.spec_b, sizeof(double2) * 3 * nh), "hipMalloc spec_b"))) return st;
     if ((st = hip_check(c, hipMalloc(&d.kre, sizeof(double) * 9 * nh), "hipMalloc kre"))) return st;
     if ((st = hip_check(c, hipMalloc(&d.kim, sizeof(float) * 9 * nh), "hipMalloc kim"))) return st;
+    if (d.general) {
+        if ((st = make_table(c, d.gtx, c->nx))) return st;
+        if ((st = make_table(c, d.gty, c->ny))) return st;
+        if ((st = make_table(c, d.gtz, c->nz))) return st;
+        // the fused stage's radix-2 z transforms (power-of-two nz)
+        return pow2(c->nz) ? make_twiddles(c, d.twz, c->nz) : GF_OK;
+    }
     if ((st = make_twiddles(c, d.twx, c->nx))) return st;
     if ((st = make_twiddles(c, d.twy, c->ny))) return st;
-    if ((st = make_twiddles(c, d.twz, c->nz))) return st;
+    return make_twiddles(c, d.twz, c->nz);
+}
+
+gf_status demag_setup(gf_ctx* c) {
+    DemagPlan& d = c->demag;
+    gf_status st;
+    if ((st = plan_alloc(c, d, "FFT magnetostatics"))) return st;
+    const int64_t nh = c->nx * c->ny * d.nzh;
 
     // Real-space tensor kernels on the host (bit-exact with the reference's
     // evaluation), then forward transforms on the device.
@@ -569,9 +698,9 @@ gf_status demag_setup(gf_ctx* c) {
         st = hip_check(c, hipMemcpyAsync(draw, real9.data() + s * c->n, sizeof(double) * c->n, hipMemcpyHostToDevice,
                                          c->stream), "copy kernel");
         const double* raw3[3] = {draw, nullptr, nullptr};
-        if (!st) st = hip_check(c, run_zfwd(c, true, raw3, nullptr, nullptr, nullptr, 1), "zfwd kernel");
-        if (!st) st = hip_check(c, run_y(c, false, 1, whole_range(c)), "yfwd kernel");
-        if (!st) st = hip_check(c, run_x(c, false, whole_range(c)), "xfwd kernel");
+        if (!st) st = hip_check(c, run_zfwd(c, d, true, raw3, nullptr, nullptr, nullptr, 1), "zfwd kernel");
+        if (!st) st = hip_check(c, run_y(c, d, false, 1, whole_range(c, d)), "yfwd kernel");
+        if (!st) st = hip_check(c, run_x(c, d, false, whole_range(c, d)), "xfwd kernel");
         if (!st) {
             hipLaunchKernelGGL(split_spectrum_kernel, dim3(blocks_for(nh, kThreads)), dim3(kThreads), 0, c->stream,
                                d.spec_b, d.kre + s * nh, d.kim + s * nh, nh);
@@ -582,10 +711,11 @@ gf_status demag_setup(gf_ctx* c) {
     }
     hipFree(draw);
     d.ready = true;
-    // Fused z transforms when the magnetisation is FM only and P whole z
-    // columns fit the LDS budget of the fused stage (gf_llg.hip).
+    // Fused z transforms when the magnetisation is FM only, nz is a power of
+    // two and P whole z columns (P dividing ny) fit the LDS budget of the
+    // fused stage (gf_llg.hip); the y and x passes may be mixed-radix.
     d.fused = false;
-    if (c->any_fm && !c->any_afm && !(c->flags & GF_FLAG_EMDYNAMIC) && c->nz >= 2) {
+    if (c->any_fm && !c->any_afm && !(c->flags & GF_FLAG_EMDYNAMIC) && c->nz >= 2 && pow2(c->nz) && d.twz.dev) {
         int P = 1, pmax = 8;
         if (const char* env = std::getenv("GOFERRO_FUSED_P")) {
             const int v = std::atoi(env);
@@ -595,7 +725,7 @@ gf_status demag_setup(gf_ctx* c) {
             const int pairs = p > 1 ? p / 2 : 1;
             return sizeof(double2) * (c->nz / 2 + fft_lds_elems(3 * pairs * c->nz));
         };
-        while (P * 2 <= pmax && P * 2 <= c->ny && lds(P * 2) <= 64 * 1024) P *= 2;
+        while (P * 2 <= pmax && P * 2 <= c->ny && c->ny % (P * 2) == 0 && lds(P * 2) <= 64 * 1024) P *= 2;
         if (lds(P) <= 48 * 1024) { d.fused = true; d.P = P; }
     }
     d.h_in_spec = false;
@@ -614,25 +744,14 @@ gf_status demag_setup(gf_ctx* c) {
 // by den/n for the passes' unnormalised transforms and their division by
 // den = -dx dy dz.  An off-diagonal product with a component on its Nyquist
 // plane (t = n/2, its own mirror) is set to zero, so that the spectrum is
-// Hermitian and the field real.
+// Hermitian and the field real.  The plan is the context's second one, so
+// the electrostatics runs beside the FFT magnetostatics of a multiferroic.
 gf_status elec_setup(gf_ctx* c, double er) {
-    if (!pow2(c->nx) || !pow2(c->ny) || !pow2(c->nz))
-        return set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics needs power-of-two nx, ny, nz");
-    if (c->nx > 1024 || c->ny > 2048 || c->nz > 2048)
-        return set_error(c, GF_ERR_UNSUPPORTED, "FFT electrostatics supports nx <= 1024, ny, nz <= 2048");
     if (!(er > 0.)) return set_error(c, GF_ERR_ARG, "FFT electrostatics needs a positive r_permittivity");
-    DemagPlan& d = c->demag;
-    d.nzh = c->nz / 2 + 1;
-    const int64_t nx = c->nx, ny = c->ny, nzh = d.nzh, nh = nx * ny * nzh;
+    DemagPlan& d = c->eplan;
     gf_status st;
-    if ((st = hip_check(c, hipMalloc(&d.spec_m, sizeof(double2) * 3 * nh), "hipMalloc spec_m"))) return st;
-    if ((st = hip_check(c, hipMalloc(&d.spec_a, sizeof(double2) * 3 * nh), "hipMalloc spec_a"))) return st;
-    if ((st = hip_check(c, hipMalloc(&d.spec_b, sizeof(double2) * 3 * nh), "hipMalloc spec_b"))) return st;
-    if ((st = hip_check(c, hipMalloc(&d.kre, sizeof(double) * 9 * nh), "hipMalloc kre"))) return st;
-    if ((st = hip_check(c, hipMalloc(&d.kim, sizeof(float) * 9 * nh), "hipMalloc kim"))) return st;
-    if ((st = make_twiddles(c, d.twx, c->nx))) return st;
-    if ((st = make_twiddles(c, d.twy, c->ny))) return st;
-    if ((st = make_twiddles(c, d.twz, c->nz))) return st;
+    if ((st = plan_alloc(c, d, "FFT electrostatics"))) return st;
+    const int64_t nx = c->nx, ny = c->ny, nzh = d.nzh, nh = nx * ny * nzh;
     auto wave = [](int64_t i, int64_t n, double h, bool& nyq) {
         const int64_t t = i <= n / 2 ? i : i - n;
         nyq = n >= 2 && 2 * i == n;
@@ -673,86 +792,92 @@ gf_status elec_setup(gf_ctx* c, double er) {
 // The y forward, x convolution and y inverse passes of one convolution
 // (spec_m -> spec_a through spec_b).  With GOFERRO_FFT_KZCHUNKS = C > 1 the
 // three passes run over C ranges of kz planes in turn, each range's
-// intermediate at the start of spec_b, so that it can be read back from
-// the last-level cache instead of HBM.
-static gf_status conv_passes(gf_ctx* c, bool adjoint, const char* what) {
+// intermediate at the start of spec_b (measured slower on the MI355X at
+// 256x256x128 for every C > 1; kept for tuning other grids).
+static gf_status conv_passes(gf_ctx* c, DemagPlan& d, bool adjoint, const char* what) {
     int chunks = 1;
     if (const char* env = std::getenv("GOFERRO_FFT_KZCHUNKS")) {
         const int v = std::atoi(env);
         if (v >= 1 && v <= 1024) chunks = v;
     }
-    const int nzh = static_cast<int>(c->demag.nzh);
+    const int nzh = static_cast<int>(d.nzh);
     if (chunks > nzh) chunks = nzh;
     if (chunks <= 1) {
-        const KzRange w = whole_range(c);
+        const KzRange w = whole_range(c, d);
         gf_status st;
-        if ((st = hip_check(c, run_y(c, false, 3, w), what))) return st;
-        if ((st = hip_check(c, run_x(c, true, w, adjoint), what))) return st;
-        return hip_check(c, run_y(c, true, 3, w), what);
+        if ((st = hip_check(c, run_y(c, d, false, 3, w), what))) return st;
+        if ((st = hip_check(c, run_x(c, d, true, w, adjoint), what))) return st;
+        return hip_check(c, run_y(c, d, true, 3, w), what);
     }
     const int per = (nzh + chunks - 1) / chunks;
     const int64_t nhb = static_cast<int64_t>(per) * c->nx * c->ny;
     for (int k0 = 0; k0 < nzh; k0 += per) {
         const KzRange r{k0, k0 + per <= nzh ? per : nzh - k0, k0, nhb};
         gf_status st;
-        if ((st = hip_check(c, run_y(c, false, 3, r), what))) return st;
-        if ((st = hip_check(c, run_x(c, true, r, adjoint), what))) return st;
-        if ((st = hip_check(c, run_y(c, true, 3, r), what))) return st;
+        if ((st = hip_check(c, run_y(c, d, false, 3, r), what))) return st;
+        if ((st = hip_check(c, run_x(c, d, true, r, adjoint), what))) return st;
+        if ((st = hip_check(c, run_y(c, d, true, 3, r), what))) return st;
     }
     return GF_OK;
 }
 
 // The static field of the current polarisation store into Estat.
 gf_status elec_run(gf_ctx* c, const double* const* p) {
+    DemagPlan& d = c->eplan;
     gf_status st;
-    if ((st = hip_check(c, run_zfwd(c, true, p, nullptr, nullptr, nullptr, 3), "zfwd electrostatics"))) return st;
-    if ((st = conv_passes(c, false, "y/x/y passes electrostatics"))) return st;
-    return hip_check(c, run_zinv(c, c->estat), "zinv electrostatics");
+    if ((st = hip_check(c, run_zfwd(c, d, true, p, nullptr, nullptr, nullptr, 3), "zfwd electrostatics"))) return st;
+    if ((st = conv_passes(c, d, false, "y/x/y passes electrostatics"))) return st;
+    return hip_check(c, run_zinv(c, d, c->estat), "zinv electrostatics");
 }
 
 // Fused mode: M z-spectrum (from the stage, or a z pass when the store
 // changed otherwise) -> y, x with the tensor product, y inverse -> H
 // z-spectrum in spec_a; the real field is built only when asked for.
 gf_status demag_run_fused(gf_ctx* c, const double* const* fm) {
+    DemagPlan& d = c->demag;
     gf_status st;
-    if (!c->demag.m_in_spec && (st = hip_check(c, run_zfwd(c, false, nullptr, fm, nullptr, nullptr, 3), "zfwd")))
+    if (!d.m_in_spec && (st = hip_check(c, run_zfwd(c, d, false, nullptr, fm, nullptr, nullptr, 3), "zfwd")))
         return st;
-    if ((st = conv_passes(c, false, "y/x/y passes"))) return st;
-    c->demag.h_in_spec = true;
-    c->demag.h_real_valid = false;
-    c->demag.m_in_spec = false;
+    if ((st = conv_passes(c, d, false, "y/x/y passes"))) return st;
+    d.h_in_spec = true;
+    d.h_real_valid = false;
+    d.m_in_spec = false;
     return GF_OK;
 }
 
 gf_status demag_materialize_h(gf_ctx* c) {
-    if (!c->demag.ready || !c->demag.fused || c->demag.h_real_valid) return GF_OK;
-    gf_status st = hip_check(c, run_zinv(c), "zinv");
-    if (!st) c->demag.h_real_valid = true;
+    DemagPlan& d = c->demag;
+    if (!d.ready || !d.fused || d.h_real_valid) return GF_OK;
+    gf_status st = hip_check(c, run_zinv(c, d), "zinv");
+    if (!st) d.h_real_valid = true;
     return st;
 }
 
 // FM-only convolution H = N * (Ms m) of a given store into out (3 arrays).
 gf_status demag_run_to(gf_ctx* c, const double* const* fm, double* const* out) {
+    DemagPlan& d = c->demag;
     gf_status st;
-    if ((st = hip_check(c, run_zfwd(c, false, nullptr, fm, nullptr, nullptr, 3), "zfwd"))) return st;
-    if ((st = conv_passes(c, false, "y/x/y passes"))) return st;
-    return hip_check(c, run_zinv(c, out), "zinv");
+    if ((st = hip_check(c, run_zfwd(c, d, false, nullptr, fm, nullptr, nullptr, 3), "zfwd"))) return st;
+    if ((st = conv_passes(c, d, false, "y/x/y passes"))) return st;
+    return hip_check(c, run_zinv(c, d, out), "zinv");
 }
 
 // Adjoint convolution out = N^T (*) in of plain real arrays (no Ms): the
 // transpose of demag_run_to up to the Ms factor the caller applies.
 gf_status demag_adjoint(gf_ctx* c, const double* const* in, double* const* out) {
+    DemagPlan& d = c->demag;
     gf_status st;
-    if ((st = hip_check(c, run_zfwd(c, true, in, nullptr, nullptr, nullptr, 3), "zfwd adjoint"))) return st;
-    if ((st = conv_passes(c, true, "y/x/y passes adjoint"))) return st;
-    return hip_check(c, run_zinv(c, out), "zinv adjoint");
+    if ((st = hip_check(c, run_zfwd(c, d, true, in, nullptr, nullptr, nullptr, 3), "zfwd adjoint"))) return st;
+    if ((st = conv_passes(c, d, true, "y/x/y passes adjoint"))) return st;
+    return hip_check(c, run_zinv(c, d, out), "zinv adjoint");
 }
 
 gf_status demag_run(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2) {
+    DemagPlan& d = c->demag;
     gf_status st;
-    if ((st = hip_check(c, run_zfwd(c, false, nullptr, fm, s1, s2, 3), "zfwd"))) return st;
-    if ((st = conv_passes(c, false, "y/x/y passes"))) return st;
-    return hip_check(c, run_zinv(c), "zinv");
+    if ((st = hip_check(c, run_zfwd(c, d, false, nullptr, fm, s1, s2, 3), "zfwd"))) return st;
+    if ((st = conv_passes(c, d, false, "y/x/y passes"))) return st;
+    return hip_check(c, run_zinv(c, d), "zinv");
 }
 
 hipError_t launch_demag_local(gf_ctx* c, const double* const* fm, const double* const* s1, const double* const* s2,
@@ -770,11 +895,16 @@ hipError_t launch_demag_local(gf_ctx* c, const double* const* fm, const double* 
     return hipGetLastError();
 }
 
-void demag_free(gf_ctx* c) {
-    DemagPlan& d = c->demag;
+static void plan_free(DemagPlan& d) {
     hipFree(d.spec_m); hipFree(d.spec_a); hipFree(d.spec_b); hipFree(d.kre); hipFree(d.kim);
     hipFree(d.twx.dev); hipFree(d.twy.dev); hipFree(d.twz.dev);
+    hipFree(d.gtx.dev); hipFree(d.gty.dev); hipFree(d.gtz.dev);
     d = DemagPlan{};
+}
+
+void demag_free(gf_ctx* c) {
+    plan_free(c->demag);
+    plan_free(c->eplan);
 }
 
 }  // namespace gf

@@ -780,6 +780,9 @@ gf_status adjoint32_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss) {
         return set_error(c, GF_ERR_UNSUPPORTED, "the FP32 adjoint sweep does not support frame rotation");
     if (c->has_hnu)
         return set_error(c, GF_ERR_UNSUPPORTED, "the FP32 adjoint sweep does not support Hext_nonunif");
+    if ((c->flags & GF_FLAG_MAGNETOSTATICS) && c->demag.ready && c->demag.general)
+        return set_error(c, GF_ERR_UNSUPPORTED,
+                         "the FP32 adjoint sweep needs power-of-two nx, ny, nz with FFT magnetostatics");
     adjoint32_free(c);
     Adj32* s = new Adj32();
     c->adj32 = s;

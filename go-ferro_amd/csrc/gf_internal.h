@@ -441,8 +441,23 @@ struct Twiddles {
     double2* dev = nullptr;
 };
 
+// Radices of a mixed-radix line transform of length n (their product), for
+// the passes of grids that are not powers of two.
+constexpr int kMaxRadices = 16;
+struct FftFactors {
+    int n = 0;
+    int count = 0;
+    int r[kMaxRadices] = {};
+};
+
 struct DemagPlan {
     bool ready = false;
+    // Any of nx, ny, nz not a power of two: the passes use the mixed-radix
+    // kernels (gf_fft_any.hip) with natural-order twiddle tables of n
+    // entries, w[t] = exp(-2 pi i t / n).
+    bool general = false;
+    FftFactors fx, fy, fz;
+    Twiddles gtx, gty, gtz;
     int64_t nzh = 0;                 // nz/2 + 1
     double2* spec_m = nullptr;       // M z-spectrum, 3 components, layout [x][kz][y]
     double2* spec_a = nullptr;       // H z-spectrum, 3 components, layout [x][kz][y]
@@ -466,6 +481,33 @@ struct FusedZArgs {
     int64_t nh;
     int nzh, lognz, P;
 };
+
+// Arguments of the mixed-radix passes (gf_fft_any.hip); each pass reads
+// the fields it needs.
+enum class AnyPass { kZFwd, kYFwd, kYInv, kXConv, kZInv };
+struct AnyPassArgs {
+    const uint32_t* code;
+    const DevMaterial* mats;
+    const double* fm[3];
+    const double* a1[3];
+    const double* a2[3];
+    const double* raw[3];       // raw_in: plain real input arrays
+    double* out[3];             // z inverse output
+    double2* spec;              // [x][kz][y], component c at spec + c*nh
+    double2* spec2;             // [kz][y][x], plane kz at kz - kzb0, component c at spec2 + c*nhb
+    const double* kre;          // tensor spectra (real, FP32 imaginary), [kz][y][x]
+    const float* kim;
+    const double2* tw;          // twiddle table of the pass's length
+    FftFactors f;               // the pass's transform length and radices
+    int64_t nh, nhb;
+    double den;                 // -dx dy dz
+    int nx, ny, nzh;
+    int pencils;
+    int kz0, kzb0;
+    int raw_in, conv, adjoint;
+};
+size_t any_pass_lds(int lines, int n);
+hipError_t launch_any_pass(AnyPass kind, const AnyPassArgs& a, dim3 grid, hipStream_t s);
 
 struct TimerSlot {
     hipEvent_t start, stop;
@@ -627,6 +669,7 @@ struct gf_ctx {
     int fe_next_stage = 1;
 
     gf::DemagPlan demag;
+    gf::DemagPlan eplan;            // FFT electrostatics (GF_FLAG_ELECTROSTATICS)
     gf::AdjState* adj = nullptr;    // checkpoints and buffers of the adjoint sweep
     gf::Adj32* adj32 = nullptr;     // the same for its FP32 variant
 

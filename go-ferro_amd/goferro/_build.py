@@ -19,7 +19,7 @@ INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "libgoferro.so"
 OBJDIR = PKG.parent / "_build"
 
-HIP_SOURCES = ["gf_api.hip", "gf_llg.hip", "gf_fft.hip", "gf_fe.hip", "gf_elastic.hip", "gf_em.hip", "gf_batch.hip", "gf_adjoint.hip", "gf_fp32.hip", "gf_textio.hip", "gf_elastostatic.hip", "gf_noalloc.hip"]
+HIP_SOURCES = ["gf_api.hip", "gf_llg.hip", "gf_fft.hip", "gf_fft_any.hip", "gf_fe.hip", "gf_elastic.hip", "gf_em.hip", "gf_batch.hip", "gf_adjoint.hip", "gf_fp32.hip", "gf_textio.hip", "gf_elastostatic.hip", "gf_noalloc.hip"]
 HOST_SOURCES = ["gf_newell.cpp"]
 
 

@@ -462,7 +462,6 @@ class Deck:
 
 def _unsupported(g: dict, mats: list) -> list[str]:
     any_fe = any(m["if_FE"] for m in mats)
-    any_mag = any(m["if_FM"] or m["if_AFM"] for m in mats)
     out = []
     if g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
         out.append("the 1D and film elastostatics and external strain (if_elastostatic_1D, if_elastostatic_film, "
@@ -471,9 +470,6 @@ def _unsupported(g: dict, mats: list) -> list[str]:
         out.append("if_elastostatic with elastodynamics or a ferroelectric (the static solve covers magnets alone)")
     if any_fe and g["if_elec_1Dmodel"]:
         out.append("if_elec_1Dmodel (get_E_static_1D is declared but not defined in the reference)")
-    fft_h = any_mag and g["if_magnetostatics"] and not g["if_mag_1Dmodel"] and not g["if_demag_factor"]
-    if any_fe and fft_h and not g["if_elec_1Dmodel"]:
-        out.append("FFT electrostatics together with FFT magnetostatics")
     if g["if_input_eigenstraint0_crt"]:
         out.append("if_input_eigenstraint0_crt (eigenstrain input)")
     if g["if_input_elastoforce"]:

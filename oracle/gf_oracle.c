@@ -140,9 +140,54 @@ static int em_setup(gfo_ctx* c, const gf_desc* d);
 static void em_cells(gfo_ctx* c, int which);
 
 /* ------------------------------------------------------------------ FFT */
-/* In-place iterative radix-2 complex FFT, unnormalised, sign -1 forward. */
+/* Mixed-radix decimation in time for lengths that are not powers of two
+ * (cuFFT plans any size, magnetic_initialize.cpp:114-115): the smallest
+ * prime factor p splits x (stride `stride`) into p interleaved sequences
+ * of length m = n/p, each transformed recursively into y[q*m ..], then
+ * X[k + s m] = sum_q exp(-+2 pi i q (k + s m) / n) Y_q[k], the angle taken
+ * from the index reduced mod n. */
+static void dft_rec(const double* xr, const double* xi, int64_t stride, int64_t n, double* yr, double* yi,
+                    int inverse) {
+    if (n == 1) { yr[0] = xr[0]; yi[0] = xi[0]; return; }
+    int64_t p = 2;
+    while (n % p) p++;
+    const int64_t m = n / p;
+    double* tr = (double*)malloc(sizeof(double) * (size_t)n);
+    double* ti = (double*)malloc(sizeof(double) * (size_t)n);
+    for (int64_t q = 0; q < p; q++) dft_rec(xr + q * stride, xi + q * stride, stride * p, m, tr + q * m, ti + q * m, inverse);
+    for (int64_t k = 0; k < m; k++)
+        for (int64_t s = 0; s < p; s++) {
+            double ar = 0.0, ai = 0.0;
+            for (int64_t q = 0; q < p; q++) {
+                const int64_t t = (q * (k + s * m)) % n;
+                const double ang = 2.0 * PI * (double)t / (double)n;
+                const double wr = cos(ang), wi = inverse ? sin(ang) : -sin(ang);
+                ar += tr[q * m + k] * wr - ti[q * m + k] * wi;
+                ai += tr[q * m + k] * wi + ti[q * m + k] * wr;
+            }
+            yr[k + s * m] = ar;
+            yi[k + s * m] = ai;
+        }
+    free(tr);
+    free(ti);
+}
+
+static int is_pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+/* In-place complex FFT, unnormalised, sign -1 forward: iterative radix 2
+ * for powers of two, dft_rec otherwise. */
 static void fft1(double* re, double* im, int64_t n, int inverse) {
     if (n <= 1) return;
+    if (!is_pow2(n)) {
+        double* yr = (double*)malloc(sizeof(double) * (size_t)n);
+        double* yi = (double*)malloc(sizeof(double) * (size_t)n);
+        dft_rec(re, im, 1, n, yr, yi, inverse);
+        memcpy(re, yr, sizeof(double) * (size_t)n);
+        memcpy(im, yi, sizeof(double) * (size_t)n);
+        free(yr);
+        free(yi);
+        return;
+    }
     for (int64_t i = 1, j = 0; i < n; i++) {
         int64_t bit = n >> 1;
         for (; j & bit; bit >>= 1) j ^= bit;
@@ -168,8 +213,6 @@ static void fft1(double* re, double* im, int64_t n, int inverse) {
         }
     }
 }
-
-static int is_pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
 
 /* cuFFT D2Z semantics (magnetostatic_solver.cpp:149-151): unnormalised
  * forward transform over (nx,ny,nz), row-major, half spectrum along z. */
@@ -770,7 +813,6 @@ gfo_ctx* gfo_create(const gf_desc* d) {
     }
     /* demag spectra: magnetic_initialize.cpp:1099-1112 */
     if ((c->flags & GF_FLAG_MAGNETOSTATICS) && !(c->flags & GF_FLAG_MAG_1D) && !(c->flags & GF_FLAG_DEMAG_FACTOR)) {
-        if (!is_pow2(c->nx) || !is_pow2(c->ny) || !is_pow2(c->nz)) { gfo_destroy(c); return NULL; }
         int64_t nh = c->nx * c->ny * (c->nz / 2 + 1);
         double* real9 = dalloc(9 * n);
         gfo_newell_real(c->nx, c->ny, c->nz, c->dx, c->dy, c->dz, real9);

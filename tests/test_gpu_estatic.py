@@ -51,7 +51,8 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("shape,scale", [((8, 8, 16), 1.0), ((16, 4, 32), 1.7), ((1, 1, 64), 1.0)])
+@pytest.mark.parametrize("shape,scale", [((8, 8, 16), 1.0), ((16, 4, 32), 1.7), ((1, 1, 64), 1.0), ((6, 10, 15), 1.0),
+                                         ((1, 1, 60), 1.3), ((7, 3, 20), 1.0)])
 def test_static_field_matches_oracle(shape, scale):
     say("case electrostatic field", shape, scale)
     prob, p, q = block(shape, scale=scale)
@@ -116,16 +117,45 @@ def test_electrostatics_with_local_magnetostatics(mode):
     o.close()
 
 
+@pytest.mark.parametrize("shape", [(8, 8, 16), (6, 5, 12)])
+def test_electrostatics_with_fft_magnetostatics(shape):
+    """A multiferroic block: FFT electrostatics of the polar layer and FFT
+    magnetostatics of the magnetic one in one context, each on its own
+    plan, after every update phase (main.cpp:299-306)."""
+    from oracle.oracle import Oracle
+    say("case electrostatics with fft magnetostatics", shape)
+    prob, p, q = block(shape=shape, flags=FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
+    prob.materials[1] = Material.make(if_FM=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.05, Aex=3.7e-12,
+                                      r_permittivity=25.0)
+    m = workloads.tilted_m(prob.shape, 4, theta_max=0.4)
+    m[:, np.asarray(prob.material_cell) != 2] = 0.0
+    g, o = Context(prob), Oracle(prob)
+    for fid, v in ((FIELD_P, p), (FIELD_Q, q), (FIELD_M, m)):
+        g.upload(fid, v)
+        o.set(fid, v)
+    g.demag()
+    o.demag()
+    for fid in (FIELD_ESTAT, FIELD_HSTAT):
+        d = rel(g.download(fid), o.get(fid))
+        say("  initial field", fid, "rel", d)
+        assert d <= 1e-12
+    g.step(4)
+    o.step(4)
+    for fid in (FIELD_P, FIELD_Q, FIELD_ESTAT, FIELD_M, FIELD_HSTAT):
+        d = rel(g.download(fid), o.get(fid))
+        say("  field", fid, "rel", d)
+        assert d <= 1e-11
+    g.close()
+    o.close()
+
+
 def test_electrostatics_rejections():
     say("case electrostatics rejections")
-    prob, _, _, _ = with_magnet(FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
-    with pytest.raises(GoFerroError, match="magnetostatics"):
-        Context(prob)
     # without magnets the magnetostatics flag sets nothing up
     prob, _, _ = block(flags=FLAG_ELECTROSTATICS | FLAG_MAGNETOSTATICS)
     Context(prob).close()
-    prob, _, _ = block(shape=(8, 8, 12))
-    with pytest.raises(GoFerroError, match="power-of-two"):
+    prob, _, _ = block(shape=(8, 8, 22))
+    with pytest.raises(GoFerroError, match="prime factors"):
         Context(prob)
     shape = (4, 4, 4)
     prob = Problem(shape=shape, spacing=(1e-9,) * 3, materials=[Material.make(r_permittivity=3.0)],

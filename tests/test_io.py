@@ -205,10 +205,13 @@ def test_unsupported_options_are_named(tmp_path, key, text, what):
     gio.read_deck(d, allow_unsupported=True)
 
 
-def test_fft_electrostatics_with_fft_magnetostatics_is_refused(tmp_path):
+def test_fft_electrostatics_with_fft_magnetostatics_is_read(tmp_path):
+    """A multiferroic deck with both FFT statics (main.cpp:299-306) maps to
+    both flags."""
     d = deck_copy(tmp_path, mat_edits=[(2, "if_FE", "true")])
-    with pytest.raises(gio.UnsupportedInput, match="FFT electrostatics together with FFT magnetostatics"):
-        gio.read_deck(d)
+    deck = gio.read_deck(d)
+    assert deck.problem.flags & FLAG_ELECTROSTATICS
+    assert deck.problem.flags & FLAG_MAGNETOSTATICS
 
 
 def test_malformed_token_names_its_line(tmp_path):

@@ -88,8 +88,8 @@ def test_isolated_cube_self_and_dipole_field(oracle_lib):
         assert np.linalg.norm(got - Hd) < 3e-3 * np.linalg.norm(Hd), off
 
 
-def test_fft_convolution_equals_real_space_sum(oracle_lib):
-    shape = (8, 8, 4)
+@pytest.mark.parametrize("shape", [(8, 8, 4), (6, 5, 3), (5, 6, 7)])
+def test_fft_convolution_equals_real_space_sum(oracle_lib, shape):
     d = (2e-9, 2.5e-9, 3e-9)
     rng = np.random.default_rng(21)
     mat = Material.make(if_FM=1, anisotropy_type=0, Ms=1.4e5, gyro=GYRO, FM_damping=0.0, Aex=1e-11)
@@ -141,3 +141,31 @@ def test_thin_film_limit(oracle_lib):
     hz = H[2, :, :, 0]
     assert np.all(np.abs(hz / -1.4e5 - 1) < 0.04)
     assert np.abs(H[0, :, :, 0]).max() < 1e-6 * 1.4e5
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 7), (9, 10, 3), (4, 12, 15)])
+def test_kernel_placement_on_any_grid(oracle_lib, shape):
+    """magnetic_initialize.cpp:1204-1357 places the kernel of offset d at
+    index d (d >= 0) or n + d (d < 0) for 1 - n/2 <= d <= n/2 - 1 (integer
+    halves) and leaves the other indices zero, for odd and even n alike:
+    check the zero pattern and that the placed values are those of the
+    offsets, through the evenness of the diagonal kernels."""
+    d = (2e-9, 2.5e-9, 3e-9)
+    K = oracle_lib.newell_real(shape, d)
+    for ax, n in enumerate(shape):
+        half = n // 2
+        kept = set(range(0, half)) | set(range(n - half + 1, n))
+        for i in range(n):
+            sl = [slice(None)] * 3
+            sl[ax] = i
+            plane = K[(slice(None),) + tuple(sl)]
+            if i in kept:
+                assert np.abs(plane[0]).max() > 0, (ax, i)
+            else:
+                assert np.abs(plane).max() == 0, (ax, i)
+        # the xx kernel is even in each offset: K(d) at index d equals K(-d)
+        # at index n - d, up to the evaluation's round-off
+        for dd in range(1, half):
+            a = np.take(K[0], dd, axis=ax)
+            b = np.take(K[0], n - dd, axis=ax)
+            np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-9 * np.abs(K[0]).max())

@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 
-@pytest.mark.parametrize("shape", [(8, 4, 8), (4, 8, 2), (2, 2, 16), (1, 4, 4), (4, 1, 1)])
+@pytest.mark.parametrize("shape", [(8, 4, 8), (4, 8, 2), (2, 2, 16), (1, 4, 4), (4, 1, 1),
+                                   (6, 5, 10), (3, 7, 9), (12, 1, 15), (5, 4, 7), (14, 3, 6)])
 def test_rfft3_matches_naive_dft(oracle_lib, shape):
     rng = np.random.default_rng(11)
     x = rng.standard_normal(shape)
@@ -15,7 +16,8 @@ def test_rfft3_matches_naive_dft(oracle_lib, shape):
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("shape", [(16, 8, 32), (32, 32, 4), (8, 16, 1)])
+@pytest.mark.parametrize("shape", [(16, 8, 32), (32, 32, 4), (8, 16, 1), (20, 12, 10), (48, 9, 25), (7, 49, 6),
+                                   (100, 6, 20)])
 def test_rfft3_matches_numpy(oracle_lib, shape):
     rng = np.random.default_rng(3)
     x = rng.standard_normal(shape)
@@ -24,7 +26,7 @@ def test_rfft3_matches_numpy(oracle_lib, shape):
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("shape", [(16, 8, 32), (8, 8, 2), (4, 4, 1)])
+@pytest.mark.parametrize("shape", [(16, 8, 32), (8, 8, 2), (4, 4, 1), (6, 10, 15), (12, 7, 5), (3, 5, 14)])
 def test_irfft3_is_unnormalised_inverse(oracle_lib, shape):
     rng = np.random.default_rng(5)
     x = rng.standard_normal(shape)
