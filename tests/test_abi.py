@@ -138,3 +138,28 @@ def test_stage_arguments_never_alias(plant, where):
         assert text.startswith("no aliasing in") and int(text.split()[3]) > 10000
     else:
         assert text.startswith(where), text
+
+
+def build_binding_driver(out_dir) -> Path:
+    """tests/cpp/binding_driver.cpp (a C++ caller in the pattern of the
+    reference-side binding) compiled with warnings as errors against
+    include/goferro.h and linked to the built library."""
+    import goferro
+    exe = Path(out_dir) / "binding_driver"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "cpp" / "binding_driver.cpp"), f"-L{goferro.LIB_PATH.parent}", "-lgoferro",
+                    f"-Wl,-rpath,{goferro.LIB_PATH.parent}", "-o", str(exe)], check=True)
+    return exe
+
+
+def test_cpp_binding_compiles_and_reports_create_errors(tmp_path):
+    """The C++ caller builds against the header and the library; creating a
+    context either succeeds (a GPU is present) or is refused with the
+    message gf_last_error(NULL) returns (no GPU here)."""
+    import goferro
+    if not goferro.LIB_PATH.exists():
+        pytest.skip("libgoferro.so not built (run __graft_entry__.build())")
+    exe = build_binding_driver(tmp_path)
+    r = subprocess.run([str(exe), "probe"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("probe: context created") or "create refused: " in r.stdout
