@@ -548,6 +548,7 @@ gf_status elastic_strain(gf_ctx* c) {
     }
     for (int i = 0; i < 6; i++) { a.eg[i] = c->eg[i]; a.ec[i] = c->ec[i]; }
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+    a.idx = 1. / c->dx; a.idy = 1. / c->dy; a.idz = 1. / c->dz;
     for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
     a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;
     std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
@@ -929,7 +930,7 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     }
     if (!st && (c->flags & GF_FLAG_ELASTODYNAMICS)) {
         c->el_on = true;
-        double** el_bufs[] = {c->u, c->v, c->us, c->vs, c->accu, c->accv, c->fstore};
+        double** el_bufs[] = {c->u, c->v, c->us, c->vs, c->accu, c->accv, c->fstore, c->epx, c->epy, c->epz};
         for (double** b : el_bufs)
             if (!st) st = alloc3(c, b, "hipMalloc elastic state");
         const int64_t nx = c->nx, ny = c->ny, nz = c->nz;
@@ -1010,7 +1011,7 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->p); free3(c->q); free3(c->pacc); free3(c->qacc);
     free3(c->p_a); free3(c->p_b); free3(c->q_a); free3(c->q_b); free3(c->estat);
     free3(c->u); free3(c->v); free3(c->us); free3(c->vs); free3(c->accu); free3(c->accv); free3(c->fstore);
-    free3(c->gyz); free3(c->gxz); free3(c->gxy);
+    free3(c->gyz); free3(c->gxz); free3(c->gxy); free3(c->epx); free3(c->epy); free3(c->epz);
     for (int a = 0; a < 6; a++) {
         if (c->ec[a] && c->ec[a] != c->eg[a]) hipFree(c->ec[a]);
         if (c->e0c[a] && c->e0c[a] != c->e0g[a]) hipFree(c->e0c[a]);
@@ -1727,11 +1728,24 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
     const int64_t nxy = c->nx * c->ny;
     const int64_t slot_size = static_cast<int64_t>(kHistPlanes) * 3 * nxy;
 
-    // strain of the stage store (elastic_dynamic_solver.cpp:107-227)
-    gf_status st = elastic_strain(c);
-    // eigenstrain from the stores of this stage (:4-104)
-    if (!st && (c->flags & GF_FLAG_ELASTO_BACKACTION)) {
-        ElEigenArgs e{};
+    // strain of the stage store (elastic_dynamic_solver.cpp:107-227), the
+    // eigenstrain from the stores of this stage (:4-104) and the cells' parts
+    // of their face stresses, in one launch
+    gf_status st = GF_OK;
+    {
+        ElPrepArgs pa{};
+        ElStrainArgs& a = pa.s;
+        a.code = c->d_code;
+        for (int i = 0; i < 3; i++) {
+            a.us[i] = c->us[i]; a.gyz[i] = c->gyz[i]; a.gxz[i] = c->gxz[i]; a.gxy[i] = c->gxy[i];
+        }
+        for (int i = 0; i < 6; i++) { a.eg[i] = c->eg[i]; a.ec[i] = c->ec[i]; }
+        a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
+        a.idx = 1. / c->dx; a.idy = 1. / c->dy; a.idz = 1. / c->dz;
+        for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
+        a.rotate = rotate;
+        std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
+        ElEigenArgs& e = pa.e;
         e.code = c->d_code; e.mats = c->d_mats;
         double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : nullptr;
         double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : nullptr;
@@ -1750,9 +1764,15 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
         e.rotate = rotate;
         std::memcpy(e.Tg2c, c->Tg2c, sizeof(e.Tg2c));
         std::memcpy(e.Tc2g, c->Tc2g, sizeof(e.Tc2g));
+        pa.eig = (c->flags & GF_FLAG_ELASTO_BACKACTION) ? 1 : 0;
+        // only a ferroelectric stage reads the eigenstrain back (the next
+        // elastic stage recomputes it before any other use)
+        pa.e0_out = (c->any_fe && strain_coupled(c)) ? 1 : 0;
+        pa.nmat = static_cast<int32_t>(c->host_mats.size());
+        for (int i = 0; i < 3; i++) { pa.px[i] = c->epx[i]; pa.py[i] = c->epy[i]; pa.pz[i] = c->epz[i]; }
         hipEvent_t ev = nullptr;
         timer_begin(c, GF_K_ELASTIC_STRAIN, &ev);
-        st = hip_check(c, launch_el_eigen(e, c->stream), "eigenstrain");
+        st = hip_check(c, launch_el_prep(pa, c->stream), "elastic strain and stress parts");
         timer_end(c, GF_K_ELASTIC_STRAIN, ev);
     }
     // stage 4: u enters the boundary history before the final combine
@@ -1782,8 +1802,9 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
             f.accu[i] = c->accu[i]; f.accv[i] = c->accv[i]; f.fstore[i] = c->fstore[i];
             f.gyz[i] = c->gyz[i]; f.gxz[i] = c->gxz[i]; f.gxy[i] = c->gxy[i];
         }
-        for (int i = 0; i < 6; i++) { f.eg[i] = c->eg[i]; f.e0g[i] = c->e0g[i]; }
+        for (int i = 0; i < 3; i++) { f.px[i] = c->epx[i]; f.py[i] = c->epy[i]; f.pz[i] = c->epz[i]; }
         f.nx = c->nx; f.ny = c->ny; f.nz = c->nz; f.dx = c->dx; f.dy = c->dy; f.dz = c->dz; f.dt = c->dt;
+        f.idx = 1. / c->dx; f.idy = 1. / c->dy; f.idz = 1. / c->dz;
         for (int i = 0; i < 3; i++) f.per[i] = c->per[i];
         f.stage = stage;
         f.trac_k = -1;

@@ -1,9 +1,10 @@
 // gf_elastic.hip — elastodynamics RK4 stage for gfx950 (FP64).
 //
 // One stage of the reference's elastic solver (elastic_dynamic_solver.cpp,
-// elastodynamics_AbsorbingBoundary.cpp) is up to five launches:
-//   strain    face gradients of the u store and the cell strain (:107-227)
-//   eigen     eigenstrain from the m, m_AFM and p stores (:4-104), with back-action
+// elastodynamics_AbsorbingBoundary.cpp) is up to four launches:
+//   prep      face gradients of the u store and the cell strain (:107-227),
+//             the eigenstrain from the m, m_AFM and p stores (:4-104) with
+//             back-action, and each cell's part of its face stresses
 //   history   stage 4 only: the boundary planes of u into the history ring
 //             (transfer_pointer, elastodynamics_AbsorbingBoundary.cpp:3-89)
 //   force     stress divergence, dv and du = v_s dt, the v and u RK updates
@@ -12,7 +13,7 @@
 //             (elastodynamics_AbsorbingBoundary.cpp:92-230, :3060-3284)
 // The u and v stores are updated in place: within a stage a cell's store is
 // read only by its own thread, because the stress reads face gradients and
-// strain that the strain launch computed from the store beforehand.
+// stress parts that the prep launch computed from the store beforehand.
 //
 // RK4 bookkeeping keeps a state accumulator per field, acc = y + k1/6,
 // acc = acc + k2/3, acc = acc + k3/3, y = acc + k4/6, which is the
@@ -49,29 +50,43 @@ __device__ __forceinline__ void rot_voigt(const double T[3][3], const double v[6
 }
 
 // ---------------------------------------------------------------- strain
-__global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= nx * ny * nz) return;
+// (i, j, k) of cell id = (i ny + j) nz + k, by 32-bit divisions when the
+// index and ny nz fit 32 bits (64-bit divisions are emulated and slow).
+__device__ __forceinline__ void cell_ijk(int64_t id, int64_t ny, int64_t nz, int64_t& i, int64_t& j, int64_t& k) {
     const int64_t nyz = ny * nz;
-    const int64_t i = id / nyz, j = (id - i * nyz) / nz, k = id - i * nyz - j * nz;
+    if (((id | nyz) >> 32) == 0) {
+        const uint32_t u = static_cast<uint32_t>(id), q = static_cast<uint32_t>(nyz), z = static_cast<uint32_t>(nz);
+        const uint32_t ii = u / q, r = u - ii * q, jj = r / z;
+        i = ii; j = jj; k = r - jj * z;
+    } else {
+        i = id / nyz; j = (id - i * nyz) / nz; k = id - i * nyz - j * nz;
+    }
+}
+
+// Face gradients g[face][comp] and cell strain e of cell id = (i, j, k) of
+// the u store, both written out (the strain also in the crystal frame).
+// The differences are multiplied by the reciprocal spacings (within an ulp
+// of the reference's quotients).
+__device__ __forceinline__ void cell_strain(const ElStrainArgs& a, int64_t id, double (&g)[6][3], double (&e)[6]) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    int64_t i, j, k;
+    cell_ijk(id, ny, nz, i, j, k);
     const bool me = solid(a.code, id);
     const int64_t nb[6] = {((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k, ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k,
                            (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k, (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k,
                            (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1), (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1)};
     const bool wall[6] = {i == 0 && !a.per[0], i == nx - 1 && !a.per[0], j == 0 && !a.per[1],
                           j == ny - 1 && !a.per[1], k == 0 && !a.per[2], k == nz - 1 && !a.per[2]};
-    const double h[3] = {a.dx, a.dy, a.dz};
+    const double ih[3] = {a.idx, a.idy, a.idz};
     // g[face][comp]: a stress-free face (vacuum on a side, or a
     // non-periodic wall) has zero gradient; across a periodic wall the
     // difference wraps (elastic_dynamic_solver.cpp:107-187)
-    double g[6][3];
     for (int f = 0; f < 6; f++) {
         const bool fr = wall[f] || !(me && solid(a.code, nb[f]));
         for (int c = 0; c < 3; c++) {
             if (fr) g[f][c] = 0.;
-            else if (f & 1) g[f][c] = (a.us[c][nb[f]] - a.us[c][id]) / h[f >> 1];
-            else g[f][c] = (a.us[c][id] - a.us[c][nb[f]]) / h[f >> 1];
+            else if (f & 1) g[f][c] = (a.us[c][nb[f]] - a.us[c][id]) * ih[f >> 1];
+            else g[f][c] = (a.us[c][id] - a.us[c][nb[f]]) * ih[f >> 1];
         }
     }
     for (int c = 0; c < 3; c++) {
@@ -83,7 +98,6 @@ __global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
         if (k == nz - 1) a.gxy[c][(i * ny + j) * (nz + 1) + nz] = g[5][c];
     }
     // cell strain as the average of its face gradients (:190-227)
-    double e[6];
     e[0] = (g[0][0] + g[1][0]) / 2.;
     e[1] = (g[2][1] + g[3][1]) / 2.;
     e[2] = (g[4][2] + g[5][2]) / 2.;
@@ -99,13 +113,19 @@ __global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
     }
 }
 
-// ----------------------------------------------------------- eigenstrain
-__global__ void __launch_bounds__(256) el_eigen_kernel(ElEigenArgs a) {
+__global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= a.n) return;
+    if (id >= a.nx * a.ny * a.nz) return;
+    double g[6][3], e[6];
+    cell_strain(a, id, g, e);
+}
+
+// ----------------------------------------------------------- eigenstrain
+// Eigenstrain of cell id from the stage stores (elastic_dynamic_solver.cpp:
+// 4-104): s in the crystal frame, o in the global frame, zero for vacuum.
+__device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, double s[6], double o[6]) {
     const uint32_t t = a.code[id] & kMatMask;
-    double s[6] = {0., 0., 0., 0., 0., 0.};
-    double o[6] = {0., 0., 0., 0., 0., 0.};
+    for (int c = 0; c < 6; c++) { s[c] = 0.; o[c] = 0.; }
     if (t) {
         const DevMaterial& M = a.mats[t - 1];
         double v[4][3];
@@ -151,9 +171,6 @@ __global__ void __launch_bounds__(256) el_eigen_kernel(ElEigenArgs a) {
         if (a.rotate) rot_voigt(a.Tc2g, s, o);
         else for (int c = 0; c < 6; c++) o[c] = s[c];
     }
-    for (int c = 0; c < 6; c++) a.e0g[c][id] = o[c];
-    if (a.e0c[0] != a.e0g[0])
-        for (int c = 0; c < 6; c++) a.e0c[c][id] = s[c];
 }
 
 // ----------------------------------------------------------------- force
@@ -163,80 +180,92 @@ __device__ __forceinline__ double crow(const double (*C)[6], int r, const double
     return s;
 }
 
+// The stress the reference evaluates on a face from the strain of one cell
+// beside it is C e with (elastic_dynamic_solver.cpp:876-933, 994-1050,
+// 1112-1166, 1229-1290)
+//   YZ face (rows xx, xy, xz): e = [gx_f - e0xx, eyy - e0yy, ezz - e0zz,
+//       2(eyz - e0yz), (gx_zb + gx_zt + 2 gz_f)/2 - 2 e0xz, (gx_yb + gx_yt + 2 gy_f)/2 - 2 e0xy]
+//   XZ face (rows xy, yy, yz) and XY face (rows xz, yz, zz) likewise,
+// where g?_f is the face's own gradient and the others belong to the cell
+// (its strain e, eigenstrain e0, the gradients on its faces across the
+// axis).  C e is linear in e, so it splits into a part of the cell, P =
+// C (e with the face gradients zero), computed once per cell and axis by
+// the prep launch, and C fp with fp holding the face gradients: the face
+// gradient of component c of the face's axis a enters strain entry (a, c).
+// The face stress is the mean of its two cells' evaluations,
+// (P_q + C_q fp + P_n + C_n fp) / 2.
+
+// Rows of C per face axis, and the strain entries the face gradient
+// components (x, y, z) enter.
+__constant__ int kFaceRows[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+__constant__ int kFaceCols[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+
+// ---------------------------------------------------------------- prep
+__global__ void __launch_bounds__(256) el_prep_kernel(ElPrepArgs p) {
+    extern __shared__ double ctab[];   // nmat stiffness matrices, 36 doubles each
+    for (int t = threadIdx.x; t < 36 * p.nmat; t += blockDim.x)
+        ctab[t] = p.e.mats[t / 36].cij[(t % 36) / 6][t % 6];
+    __syncthreads();
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= p.s.nx * p.s.ny * p.s.nz) return;
+    double g[6][3], e[6];
+    cell_strain(p.s, id, g, e);
+    double e0[6] = {0., 0., 0., 0., 0., 0.};
+    if (p.eig) {
+        double sc[6];
+        eigenstrain(p.e, id, sc, e0);
+        if (p.e0_out) {
+            for (int c = 0; c < 6; c++) p.e.e0g[c][id] = e0[c];
+            if (p.e.e0c[0] != p.e.e0g[0])
+                for (int c = 0; c < 6; c++) p.e.e0c[c][id] = sc[c];
+        }
+    }
+    double P[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    const uint32_t mat = p.s.code[id] & kMatMask;
+    if (mat) {
+        const double(*C)[6] = reinterpret_cast<const double(*)[6]>(ctab + (mat - 1) * 36);
+        // the strain of each axis's faces with their own gradients zero
+        const double b[3][6] = {
+            {-e0[0], e[1] - e0[1], e[2] - e0[2], (e[3] - e0[3]) * 2., ((g[4][0] + g[5][0]) / 4. - e0[4]) * 2.,
+             ((g[2][0] + g[3][0]) / 4. - e0[5]) * 2.},
+            {e[0] - e0[0], -e0[1], e[2] - e0[2], ((g[4][1] + g[5][1]) / 4. - e0[3]) * 2., (e[4] - e0[4]) * 2.,
+             ((g[0][1] + g[1][1]) / 4. - e0[5]) * 2.},
+            {e[0] - e0[0], e[1] - e0[1], -e0[2], ((g[2][2] + g[3][2]) / 4. - e0[3]) * 2.,
+             ((g[0][2] + g[1][2]) / 4. - e0[4]) * 2., (e[5] - e0[5]) * 2.}};
+        for (int ax = 0; ax < 3; ax++)
+            for (int o = 0; o < 3; o++) P[ax][o] = crow(C, kFaceRows[ax][o], b[ax]);
+    }
+    for (int o = 0; o < 3; o++) {
+        p.px[o][id] = P[0][o];
+        p.py[o][id] = P[1][o];
+        p.pz[o][id] = P[2][o];
+    }
+}
+
 // The stiffness matrix of cell q from the workgroup's LDS copy of every
-// material's matrix (a zero matrix for vacuum, whose faces are free).
+// material's matrix (q is not vacuum: a face with vacuum beside it is free).
 __device__ __forceinline__ const double (*cij_of(const ElForceArgs& a, const double* ctab, int64_t q))[6] {
     const uint32_t m = a.code[q] & kMatMask;
     return reinterpret_cast<const double (*)[6]>(ctab + (m == 0u ? 0 : (m - 1) * 36));
 }
 
-// Stress on the YZ face fi evaluated with the strain of cell (qi, j, k):
-// rows xx, xy, xz of C e (elastic_dynamic_solver.cpp:876-933).
-__device__ __forceinline__ void stress_yz(const ElForceArgs& a, const double* ctab,
-                                          int64_t fi, int64_t qi, int64_t j, int64_t k, double out[3]) {
-    const int64_t ny = a.ny, nz = a.nz;
-    const int64_t q = (qi * ny + j) * nz + k;
-    const int64_t f = (fi * ny + j) * nz + k;
-    double e[6];
-    e[0] = a.gyz[0][f] - a.e0g[0][q];
-    e[1] = a.eg[1][q] - a.e0g[1][q];
-    e[2] = a.eg[2][q] - a.e0g[2][q];
-    e[3] = (a.eg[3][q] - a.e0g[3][q]) * 2.;
-    e[4] = ((a.gxy[0][(qi * ny + j) * (nz + 1) + k] + a.gxy[0][(qi * ny + j) * (nz + 1) + k + 1] + a.gyz[2][f] * 2.) / 4. -
-            a.e0g[4][q]) * 2.;
-    e[5] = ((a.gxz[0][(qi * (ny + 1) + j) * nz + k] + a.gxz[0][(qi * (ny + 1) + j + 1) * nz + k] + a.gyz[1][f] * 2.) / 4. -
-            a.e0g[5][q]) * 2.;
-    const double(*C)[6] = cij_of(a, ctab, q);
-    out[0] = crow(C, 0, e);
-    out[1] = crow(C, 5, e);
-    out[2] = crow(C, 4, e);
-}
-
-// XZ face fj with the strain of cell (i, qj, k): rows xy, yy, yz (:994-1050).
-__device__ __forceinline__ void stress_xz(const ElForceArgs& a, const double* ctab,
-                                          int64_t i, int64_t fj, int64_t qj, int64_t k, double out[3]) {
-    const int64_t ny = a.ny, nz = a.nz;
-    const int64_t q = (i * ny + qj) * nz + k;
-    const int64_t f = (i * (ny + 1) + fj) * nz + k;
-    double e[6];
-    e[0] = a.eg[0][q] - a.e0g[0][q];
-    e[1] = a.gxz[1][f] - a.e0g[1][q];
-    e[2] = a.eg[2][q] - a.e0g[2][q];
-    e[3] = ((a.gxy[1][(i * ny + qj) * (nz + 1) + k] + a.gxy[1][(i * ny + qj) * (nz + 1) + k + 1] + a.gxz[2][f] * 2.) / 4. -
-            a.e0g[3][q]) * 2.;
-    e[4] = (a.eg[4][q] - a.e0g[4][q]) * 2.;
-    e[5] = ((a.gxz[0][f] * 2. + a.gyz[1][(i * ny + qj) * nz + k] + a.gyz[1][((i + 1) * ny + qj) * nz + k]) / 4. -
-            a.e0g[5][q]) * 2.;
-    const double(*C)[6] = cij_of(a, ctab, q);
-    out[0] = crow(C, 5, e);
-    out[1] = crow(C, 1, e);
-    out[2] = crow(C, 3, e);
-}
-
-// XY face fk with the strain of cell (i, j, qk): rows xz, yz, zz
-// (:1112-1166, :1229-1290).
-__device__ __forceinline__ void stress_xy(const ElForceArgs& a, const double* ctab,
-                                          int64_t i, int64_t j, int64_t fk, int64_t qk, double out[3]) {
-    const int64_t ny = a.ny, nz = a.nz;
-    const int64_t q = (i * ny + j) * nz + qk;
-    const int64_t f = (i * ny + j) * (nz + 1) + fk;
-    double e[6];
-    e[0] = a.eg[0][q] - a.e0g[0][q];
-    e[1] = a.eg[1][q] - a.e0g[1][q];
-    e[2] = a.gxy[2][f] - a.e0g[2][q];
-    e[3] = ((a.gxy[1][f] * 2. + a.gxz[2][(i * (ny + 1) + j) * nz + qk] + a.gxz[2][(i * (ny + 1) + j + 1) * nz + qk]) / 4. -
-            a.e0g[3][q]) * 2.;
-    e[4] = ((a.gxy[0][f] * 2. + a.gyz[2][(i * ny + j) * nz + qk] + a.gyz[2][((i + 1) * ny + j) * nz + qk]) / 4. -
-            a.e0g[4][q]) * 2.;
-    e[5] = (a.eg[5][q] - a.e0g[5][q]) * 2.;
-    const double(*C)[6] = cij_of(a, ctab, q);
-    out[0] = crow(C, 4, e);
-    out[1] = crow(C, 3, e);
-    out[2] = crow(C, 2, e);
-}
-
-__device__ __forceinline__ void avg3(double s[3], const double t[3]) {
-    for (int c = 0; c < 3; c++) s[c] = (s[c] + t[c]) / 2.;
+// The mean stress of a face of axis ax between cells q and n, whose
+// gradients (x, y, z components) are fg.
+__device__ __forceinline__ void face_stress(const ElForceArgs& a, const double* ctab, int ax,
+                                            const double* const* P, int64_t q, int64_t n, const double fg[3],
+                                            double out[3]) {
+    const double(*Cq)[6] = cij_of(a, ctab, q);
+    const double(*Cn)[6] = cij_of(a, ctab, n);
+    for (int o = 0; o < 3; o++) {
+        const int r = kFaceRows[ax][o];
+        double tq = P[o][q], tn = P[o][n];
+        for (int c = 0; c < 3; c++) {
+            const int col = kFaceCols[ax][c];
+            tq = tq + Cq[r][col] * fg[c];
+            tn = tn + Cn[r][col] * fg[c];
+        }
+        out[o] = (tq + tn) / 2.;
+    }
 }
 
 __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
@@ -247,8 +276,8 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= nx * ny * nz) return;
-    const int64_t nyz = ny * nz;
-    const int64_t i = id / nyz, j = (id - i * nyz) / nz, k = id - i * nyz - j * nz;
+    int64_t i, j, k;
+    cell_ijk(id, ny, nz, i, j, k);
     const uint32_t mat = a.code[id] & kMatMask;
     const int stage = a.stage;
     double dv[3] = {0., 0., 0.};
@@ -267,31 +296,35 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
         const bool free_zb = (k == 0 && !a.per[2]) || !solid(a.code, (i * ny + j) * nz + kb);
         const bool free_zf = (k == nz - 1 && !a.per[2]) || !solid(a.code, (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1));
         double s1x[3] = {0., 0., 0.}, s2x[3] = {0., 0., 0.}, s1y[3] = {0., 0., 0.};
-        double s2y[3] = {0., 0., 0.}, s1z[3] = {0., 0., 0.}, s2z[3] = {0., 0., 0.}, t[3];
-        if (!free_xb) { stress_yz(a, ctab, i, i, j, k, s1x); stress_yz(a, ctab, i, ib, j, k, t); avg3(s1x, t); }
-        if (!free_xf) {
-            stress_yz(a, ctab, i + 1, i, j, k, s2x);
-            stress_yz(a, ctab, i + 1, iff, j, k, t);
-            avg3(s2x, t);
-        }
-        if (!free_yb) { stress_xz(a, ctab, i, j, j, k, s1y); stress_xz(a, ctab, i, j, jb, k, t); avg3(s1y, t); }
-        if (!free_yf) {
-            stress_xz(a, ctab, i, j + 1, j, k, s2y);
-            stress_xz(a, ctab, i, j + 1, jf, k, t);
-            avg3(s2y, t);
-        }
-        if (!free_zb) { stress_xy(a, ctab, i, j, k, k, s1z); stress_xy(a, ctab, i, j, k, kb, t); avg3(s1z, t); }
+        double s2y[3] = {0., 0., 0.}, s1z[3] = {0., 0., 0.}, s2z[3] = {0., 0., 0.};
+        auto yz = [&](int64_t fi, double fg[3]) {
+            const int64_t f = (fi * ny + j) * nz + k;
+            for (int c = 0; c < 3; c++) fg[c] = a.gyz[c][f];
+        };
+        auto xz = [&](int64_t fj, double fg[3]) {
+            const int64_t f = (i * (ny + 1) + fj) * nz + k;
+            for (int c = 0; c < 3; c++) fg[c] = a.gxz[c][f];
+        };
+        auto xy = [&](int64_t fk, double fg[3]) {
+            const int64_t f = (i * ny + j) * (nz + 1) + fk;
+            for (int c = 0; c < 3; c++) fg[c] = a.gxy[c][f];
+        };
+        double fg[3];
+        if (!free_xb) { yz(i, fg); face_stress(a, ctab, 0, a.px, id, (ib * ny + j) * nz + k, fg, s1x); }
+        if (!free_xf) { yz(i + 1, fg); face_stress(a, ctab, 0, a.px, id, (iff * ny + j) * nz + k, fg, s2x); }
+        if (!free_yb) { xz(j, fg); face_stress(a, ctab, 1, a.py, id, (i * ny + jb) * nz + k, fg, s1y); }
+        if (!free_yf) { xz(j + 1, fg); face_stress(a, ctab, 1, a.py, id, (i * ny + jf) * nz + k, fg, s2y); }
+        if (!free_zb) { xy(k, fg); face_stress(a, ctab, 2, a.pz, id, (i * ny + j) * nz + kb, fg, s1z); }
         if (!free_zf) {
-            stress_xy(a, ctab, i, j, k + 1, k, s2z);
-            stress_xy(a, ctab, i, j, k + 1, kf, t);
-            avg3(s2z, t);
+            xy(k + 1, fg);
+            face_stress(a, ctab, 2, a.pz, id, (i * ny + j) * nz + kf, fg, s2z);
         } else if (k == a.trac_k) {
             // Gaussian traction on a stress-free top face (input_strain_type 3)
             s2z[a.trac_row] = a.trac_amp;
         }
         const DevMaterial& M = a.mats[mat - 1];
         for (int c = 0; c < 3; c++) {
-            const double f = (s2x[c] - s1x[c]) / a.dx + (s2y[c] - s1y[c]) / a.dy + (s2z[c] - s1z[c]) / a.dz;
+            const double f = (s2x[c] - s1x[c]) * a.idx + (s2y[c] - s1y[c]) * a.idy + (s2z[c] - s1z[c]) * a.idz;
             const double fs = a.fstore[c][id];
             if (stage == 2 || stage == 3)
                 dv[c] = f * M.el_rho + M.el_mass * a.v[c][id] + M.el_stiff * (f - fs) * 2.;
@@ -401,8 +434,14 @@ hipError_t launch_el_strain(const ElStrainArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_el_eigen(const ElEigenArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(el_eigen_kernel, dim3(blocks_for(a.n)), dim3(256), 0, s, a);
+hipError_t launch_el_prep(const ElPrepArgs& a, hipStream_t s) {
+    const size_t lds = sizeof(double) * 36 * static_cast<size_t>(a.nmat > 0 ? a.nmat : 1);
+    if (lds > 65536) {   // up to 255 materials: 72 KiB of the CU's 160 KiB
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(el_prep_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(el_prep_kernel, dim3(blocks_for(a.s.nx * a.s.ny * a.s.nz)), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

@@ -219,6 +219,7 @@ struct ElStrainArgs {
     double* ec[6];          // may equal eg (no rotation)
     int64_t nx, ny, nz;
     double dx, dy, dz;
+    double idx, idy, idz;   // 1/dx, 1/dy, 1/dz
     int32_t per[3];
     int32_t rotate;
     double Tg2c[3][3];
@@ -267,6 +268,23 @@ struct ElEigenArgs {
     double Tc2g[3][3];
 };
 
+// The first launch of an elastic stage: the face gradients and strain of
+// the u store (s), the eigenstrain of the stage stores (e, when eig is set;
+// zero otherwise) and each cell's part of the stress its faces carry,
+// P^x, P^y, P^z: the rows of C (the cell's stiffness) applied to the
+// strain terms that do not depend on which of its two faces along the axis
+// is evaluated (see el_prep_kernel).
+struct ElPrepArgs {
+    ElStrainArgs s;
+    ElEigenArgs e;
+    int32_t eig;
+    int32_t e0_out;         // write the eigenstrain out (a ferroelectric stage reads it)
+    int32_t nmat;
+    double* px[3];
+    double* py[3];
+    double* pz[3];
+};
+
 struct ElForceArgs {
     const uint32_t* code;
     const DevMaterial* mats;
@@ -280,10 +298,12 @@ struct ElForceArgs {
     const double* gyz[3];
     const double* gxz[3];
     const double* gxy[3];
-    const double* eg[6];
-    const double* e0g[6];
+    const double* px[3];    // the cells' face-stress parts (ElPrepArgs)
+    const double* py[3];
+    const double* pz[3];
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
+    double idx, idy, idz;   // 1/dx, 1/dy, 1/dz
     int32_t per[3];
     int32_t stage;
     int32_t trac_k;         // plane of a traction pulse, -1 when off
@@ -588,6 +608,9 @@ struct gf_ctx {
     double* ec[6] = {};
     double* e0g[6] = {};
     double* e0c[6] = {};
+    double* epx[3] = {};            // face-stress parts of the cells (ElPrepArgs)
+    double* epy[3] = {};
+    double* epz[3] = {};
     double* hist = nullptr;
     int hist_slots = 0;
     int hist_head = 0;
@@ -728,7 +751,7 @@ hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s);
 
 // elastodynamics (gf_elastic.hip)
 hipError_t launch_el_strain(const ElStrainArgs& a, hipStream_t s);
-hipError_t launch_el_eigen(const ElEigenArgs& a, hipStream_t s);
+hipError_t launch_el_prep(const ElPrepArgs& a, hipStream_t s);
 hipError_t launch_el_force(const ElForceArgs& a, hipStream_t s);
 hipError_t launch_el_history(const ElHistArgs& a, hipStream_t s);
 hipError_t launch_el_boundary(const ElBoundaryArgs& a, hipStream_t s);
