@@ -1008,6 +1008,7 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->m1); free3(c->m2); free3(c->acc1); free3(c->acc2);
     free3(c->st1_a); free3(c->st1_b); free3(c->st2_a); free3(c->st2_b);
     free3(c->hstat); free3(c->hnu); free3(c->hdrive); free3(c->e_t0); free3(c->e_t0 + 3);
+    free3(c->e0t0); free3(c->e0t0 + 3);
     free3(c->p); free3(c->q); free3(c->pacc); free3(c->qacc);
     free3(c->p_a); free3(c->p_b); free3(c->q_a); free3(c->q_b); free3(c->estat);
     free3(c->u); free3(c->v); free3(c->us); free3(c->vs); free3(c->accu); free3(c->accv); free3(c->fstore);
@@ -1101,6 +1102,18 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
     case GF_FIELD_U_STORE:
         if (!c->el_on) break;
         return c->us;
+    case GF_FIELD_ELASTOFORCE:
+        if (!c->el_on) break;
+        return c->fstore;
+    case GF_FIELD_EIGENSTRAIN_T0_DIAG:
+    case GF_FIELD_EIGENSTRAIN_T0_OFF:
+        if (!c->el_on) break;
+        if (!c->e0t0[0]) {
+            // six zeroed arrays on first use; the stage reads them only once written
+            if ((*st = alloc3(c, c->e0t0, "hipMalloc static eigenstrain"))) return nullptr;
+            if ((*st = alloc3(c, c->e0t0 + 3, "hipMalloc static eigenstrain"))) return nullptr;
+        }
+        return id == GF_FIELD_EIGENSTRAIN_T0_DIAG ? c->e0t0 : c->e0t0 + 3;
     case GF_FIELD_STRAIN_DIAG:
         if (!c->el_on || for_write) break;
         return c->eg;
@@ -1274,6 +1287,7 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
         c->fe_next_stage = 1;
     }
     if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
+    if (id == GF_FIELD_EIGENSTRAIN_T0_DIAG || id == GF_FIELD_EIGENSTRAIN_T0_OFF) c->has_e0t0 = true;
     if (id == GF_FIELD_M || id == GF_FIELD_M_STORE) c->demag.m_in_spec = false;
     if (id == GF_FIELD_HSTAT) { c->demag.h_in_spec = false; c->demag.h_real_valid = true; }
     // elastic_initialize.cpp:386-410: the stores start equal to u and v
@@ -1757,7 +1771,10 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
             e.m2[i] = s2 ? s2[i] : nullptr;
             e.p[i] = ps ? ps[i] : nullptr;
         }
-        for (int i = 0; i < 6; i++) { e.e0g[i] = c->e0g[i]; e.e0c[i] = c->e0c[i]; }
+        for (int i = 0; i < 6; i++) {
+            e.e0g[i] = c->e0g[i]; e.e0c[i] = c->e0c[i];
+            e.e0t0[i] = c->has_e0t0 ? c->e0t0[i] : nullptr;
+        }
         e.flexo = c->any_fe ? c->d_flexo : nullptr;
         e.n = c->n;
         e.nx = c->nx; e.ny = c->ny; e.nz = c->nz; e.dx = c->dx; e.dy = c->dy; e.dz = c->dz;
@@ -2196,7 +2213,7 @@ struct StateArrayHeader {
 
 enum : uint32_t {
     kBitHstatWritten = 1u, kBitJpOn = 2u, kBitJpUploaded = 4u, kBitHasHnu = 8u,
-    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u
+    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u, kBitHasE0t0 = 128u
 };
 
 // FNV-1a over 64-bit words, then over the tail bytes.
@@ -2264,6 +2281,7 @@ std::vector<StateArray> state_arrays(gf_ctx* c, const StateSel& sel, uint32_t bi
         if (c->flags & GF_FLAG_ORIENT_ROTATE) { add(19, c->ec, 6, n); add(20, c->e0c, 6, n); }
         if (c->hist)
             v.push_back({21u << 8, c->hist, static_cast<int64_t>(c->hist_slots) * kHistPlanes * 3 * c->nx * c->ny});
+        if (bits & kBitHasE0t0) add(37, c->e0t0, 6, n);
     }
     if (c->em_on) {
         for (int a = 0; a < 3; a++) {
@@ -2307,6 +2325,7 @@ uint32_t state_bits(const gf_ctx* c) {
     if (c->jp_on) b |= kBitJpOn;
     if (c->jp_uploaded) b |= kBitJpUploaded;
     if (c->has_hnu) b |= kBitHasHnu;
+    if (c->has_e0t0) b |= kBitHasE0t0;
     if (c->demag.h_in_spec) b |= kBitHInSpec;
     if (c->demag.h_real_valid) b |= kBitHRealValid;
     if (c->demag.m_in_spec) b |= kBitMInSpec;
@@ -2429,6 +2448,13 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
         st = alloc3(c, c->hnu, "hipMalloc hext_nonunif");
         if (st) return st;
     }
+    if ((h.bits & kBitHasE0t0) && !c->el_on)
+        return set_error(c, GF_ERR_ARG, "the state image holds a static eigenstrain and this context has no elastodynamics");
+    if ((h.bits & kBitHasE0t0) && !c->e0t0[0]) {
+        st = alloc3(c, c->e0t0, "hipMalloc static eigenstrain");
+        if (!st) st = alloc3(c, c->e0t0 + 3, "hipMalloc static eigenstrain");
+        if (st) return st;
+    }
     const auto arr = state_arrays(c, StateSel{h.mag_sel, h.afm_sel, h.fe_sel, h.e_sel, h.h_sel}, h.bits);
     if (h.n_arrays != static_cast<int32_t>(arr.size()) || bytes != state_bytes(c, arr))
         return set_error(c, GF_ERR_ARG, "the state image does not hold the arrays of this context");
@@ -2463,6 +2489,7 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
     c->jp_on = h.bits & kBitJpOn;
     c->jp_uploaded = h.bits & kBitJpUploaded;
     c->has_hnu = h.bits & kBitHasHnu;
+    c->has_e0t0 = h.bits & kBitHasE0t0;
     c->demag.h_in_spec = h.bits & kBitHInSpec;
     c->demag.h_real_valid = h.bits & kBitHRealValid;
     c->demag.m_in_spec = h.bits & kBitMInSpec;

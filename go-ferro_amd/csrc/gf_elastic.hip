@@ -122,12 +122,16 @@ __global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
 
 // ----------------------------------------------------------- eigenstrain
 // Eigenstrain of cell id from the stage stores (elastic_dynamic_solver.cpp:
-// 4-104): s in the crystal frame, o in the global frame, zero for vacuum.
+// 4-104) less the static eigenstrain at t0 when there is one: s in the
+// crystal frame, o in the global frame, zero for vacuum.
 __device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, double s[6], double o[6]) {
     const uint32_t t = a.code[id] & kMatMask;
     for (int c = 0; c < 6; c++) { s[c] = 0.; o[c] = 0.; }
     if (t) {
         const DevMaterial& M = a.mats[t - 1];
+        double t0[6] = {0., 0., 0., 0., 0., 0.};
+        if (a.e0t0[0])
+            for (int c = 0; c < 6; c++) t0[c] = a.e0t0[c][id];
         double v[4][3];
         const double* const* src[4] = {a.p, a.m, a.m1, a.m2};
         for (int f = 0; f < 4; f++) {
@@ -141,14 +145,14 @@ __device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, do
         const double ax = v[2][0], ay = v[2][1], az = v[2][2];
         const double bx = v[3][0], by = v[3][1], bz = v[3][2];
         s[0] = 1.5 * M.l100 * (mx * mx - 1. / 3.) + M.Q11 * (px * px) + M.Q12 * (py * py + pz * pz) +
-               0.5 * 1.5 * M.l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bx * bx - 1. / 3.) - 0.;
+               0.5 * 1.5 * M.l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bx * bx - 1. / 3.) - t0[0];
         s[1] = 1.5 * M.l100 * (my * my - 1. / 3.) + M.Q11 * (py * py) + M.Q12 * (px * px + pz * pz) +
-               0.5 * 1.5 * M.l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * M.l100b * (by * by - 1. / 3.) - 0.;
+               0.5 * 1.5 * M.l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * M.l100b * (by * by - 1. / 3.) - t0[1];
         s[2] = 1.5 * M.l100 * (mz * mz - 1. / 3.) + M.Q11 * (pz * pz) + M.Q12 * (py * py + px * px) +
-               0.5 * 1.5 * M.l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bz * bz - 1. / 3.) - 0.;
-        s[3] = 1.5 * M.l111 * my * mz + M.Q44 * py * pz + 0.5 * 1.5 * M.l111a * ay * az + 0.5 * 1.5 * M.l111b * by * bz - 0.;
-        s[4] = 1.5 * M.l111 * mx * mz + M.Q44 * px * pz + 0.5 * 1.5 * M.l111a * ax * az + 0.5 * 1.5 * M.l111b * bx * bz - 0.;
-        s[5] = 1.5 * M.l111 * my * mx + M.Q44 * py * px + 0.5 * 1.5 * M.l111a * ay * ax + 0.5 * 1.5 * M.l111b * by * bx - 0.;
+               0.5 * 1.5 * M.l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bz * bz - 1. / 3.) - t0[2];
+        s[3] = 1.5 * M.l111 * my * mz + M.Q44 * py * pz + 0.5 * 1.5 * M.l111a * ay * az + 0.5 * 1.5 * M.l111b * by * bz - t0[3];
+        s[4] = 1.5 * M.l111 * mx * mz + M.Q44 * px * pz + 0.5 * 1.5 * M.l111a * ax * az + 0.5 * 1.5 * M.l111b * bx * bz - t0[4];
+        s[5] = 1.5 * M.l111 * my * mx + M.Q44 * py * px + 0.5 * 1.5 * M.l111a * ay * ax + 0.5 * 1.5 * M.l111b * by * bx - t0[5];
         if (a.flexo) {
             // -F grad p (:82-89): the global-frame gradient of FE cells,
             // zero elsewhere, enters the crystal-frame tensor

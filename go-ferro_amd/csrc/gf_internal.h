@@ -259,6 +259,7 @@ struct ElEigenArgs {
     const double* p[3];     // FE p store, null when there is no FE
     double* e0g[6];
     double* e0c[6];         // may equal e0g (no rotation)
+    const double* e0t0[6];  // static eigenstrain at t0 (crystal frame), null when not in use
     const FlexoMat* flexo;  // flexoelectric -F grad p of the FE cells, null when off
     int64_t n;
     int64_t nx, ny, nz;
@@ -560,6 +561,7 @@ struct gf_ctx {
     bool any_fm = false, any_afm = false, any_fe = false;
     int64_t nfm = 0, nafm = 0, nfe = 0;
     bool has_hnu = false;
+    bool has_e0t0 = false;          // the static eigenstrain was written: the elastic stage subtracts it
 
     uint32_t* d_code = nullptr;
     gf::DevMaterial* d_mats = nullptr;
@@ -679,6 +681,7 @@ struct gf_ctx {
     double precess_angle = 0;       // prescribed bottom layer (GF_FLAG_PRESCRIBED_M), degrees
     double precess_freq = 0;        // its frequency, Hz
     double* e_t0[6] = {};           // static crystal-frame strain (magnets, no elastodynamics)
+    double* e0t0[6] = {};           // static crystal-frame eigenstrain at t0 (elastodynamics), on first use
     // reduction scratch
     double* red_partial = nullptr;
     int64_t red_blocks = 0;

@@ -31,7 +31,8 @@ __all__ = [
     "FLAG_JF_INPUT", "FLAG_PEC_XY", "FLAG_PEC_Z", "FLAG_1D_ABC_EM_ONLYTOP",
     "FIELD_E_EM", "FIELD_H_EM", "FIELD_E_EM_STORE", "FIELD_H_EM_STORE", "FIELD_E_CELL", "FIELD_H_CELL", "FIELD_JP",
     "FLAG_HDRIVE", "FLAG_ELECTROSTATICS", "FLAG_SPIN_PUMPING", "FLAG_FLEXO", "FLAG_ELASTOSTATIC", "FIELD_HDRIVE", "FIELD_STRAIN_T0_DIAG",
-    "FIELD_STRAIN_T0_OFF", "FIELD_J_ISHE",
+    "FIELD_STRAIN_T0_OFF", "FIELD_J_ISHE", "FIELD_ELASTOFORCE", "FIELD_EIGENSTRAIN_T0_DIAG",
+    "FIELD_EIGENSTRAIN_T0_OFF",
     "field_size", "yee_shapes", "split_yee", "join_yee",
     "KERNEL_CLASSES", "EXPORTED_SYMBOLS", "Batch",
 ]
@@ -73,7 +74,7 @@ FLAG_PRESCRIBED_M = 1 << 28
 (FIELD_M, FIELD_M_STORE, FIELD_HSTAT, FIELD_HEXT_NONUNIF, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_P_STORE,
  FIELD_ESTAT, FIELD_U, FIELD_V, FIELD_U_STORE, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_E_EM, FIELD_H_EM,
  FIELD_E_EM_STORE, FIELD_H_EM_STORE, FIELD_E_CELL, FIELD_H_CELL, FIELD_JP, FIELD_HDRIVE, FIELD_STRAIN_T0_DIAG,
- FIELD_STRAIN_T0_OFF, FIELD_J_ISHE) = range(26)
+ FIELD_STRAIN_T0_OFF, FIELD_J_ISHE, FIELD_ELASTOFORCE, FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF) = range(29)
 
 
 def yee_shapes(shape, field):

@@ -34,7 +34,8 @@ import numpy as np
 from . import (
     FIELD_E_CELL, FIELD_E_EM, FIELD_ESTAT, FIELD_H_CELL, FIELD_H_EM, FIELD_HEXT_NONUNIF, FIELD_HSTAT, FIELD_JP,
     FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_P, FIELD_Q, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF,
-    FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FIELD_U, FIELD_V,
+    FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FIELD_U, FIELD_V, FIELD_ELASTOFORCE, FIELD_EIGENSTRAIN_T0_DIAG,
+    FIELD_EIGENSTRAIN_T0_OFF,
     FLAG_1D_ABC, FLAG_1D_ABC_ELAST_ONLYTOP, FLAG_1D_ABC_EM_ONLYTOP, FLAG_DEMAG_FACTOR, FLAG_ELASTO_BACKACTION,
     FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS, FLAG_ELECTROSTATICS, FLAG_EM_BACKACTION_M, FLAG_EM_BACKACTION_P,
     FLAG_EM_FROM_M, FLAG_EM_FROM_P, FLAG_EMDYNAMIC, FLAG_GAUSSIAN_PULSE, FLAG_HEXT_NONUNIF, FLAG_JF_INPUT,
@@ -470,10 +471,6 @@ def _unsupported(g: dict, mats: list) -> list[str]:
         out.append("if_elastostatic with elastodynamics or a ferroelectric (the static solve covers magnets alone)")
     if any_fe and g["if_elec_1Dmodel"]:
         out.append("if_elec_1Dmodel (get_E_static_1D is declared but not defined in the reference)")
-    if g["if_input_eigenstraint0_crt"]:
-        out.append("if_input_eigenstraint0_crt (eigenstrain input)")
-    if g["if_input_elastoforce"]:
-        out.append("if_input_elastoforce (the force store restarts through the state image instead)")
     if g["if_input_straint0"] and (g["if_elastodynamics"] or g["if_OrientRotate"]):
         out.append("if_input_straint0 with elastodynamics or a rotated crystal")
     return out
@@ -568,10 +565,8 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
         deck.fields[FIELD_HEXT_NONUNIF] = read_cells(d / "Hext_nonunif.in", (nx, ny, nz), 3)
     if g["if_output_Jishe"] and not g["if_spin_pumping"]:
         deck.skipped_outputs.append("J_ISHE without spin pumping")
-    for key, what in (("if_output_elastoforce", "elastoforce (the force store is in the state image)"),
-                      ("if_output_eigenstraint0_crt", "eigenstraint0_crt (no eigenstrain output)")):
-        if g[key]:
-            deck.skipped_outputs.append(what)
+    if g["if_output_elastoforce"] and not g["if_elastodynamics"]:
+        deck.skipped_outputs.append("elastoforce without elastodynamics (no force store is kept)")
     if g["if_output_strain"] and not g["if_elastodynamics"] and not g["if_elastostatic"]:
         # main.cpp:582-594 writes nothing then (neither branch copies a strain)
         deck.skipped_outputs.append("strain without elastodynamics or elastostatics (the reference copies none)")
@@ -697,6 +692,16 @@ def load_inputs(ctx: Context, deck: Deck, directory=None) -> list[str]:
     if g["if_input_uandv"] and g["if_elastodynamics"]:
         ctx.upload(FIELD_U, cells("u.in", 3))
         ctx.upload(FIELD_V, cells("v.in", 3))
+    if g["if_input_elastoforce"] and g["if_elastodynamics"]:
+        # input_elastoforce (inoutput.cpp:220-237): the stage-1 force that the
+        # stiffness damping of the first step differences against
+        ctx.upload(FIELD_ELASTOFORCE, cells("elastoforce.in", 3))
+    if g["if_input_eigenstraint0_crt"] and g["if_elastodynamics"]:
+        # input_eigenstraint0 (inoutput.cpp:259-280): the static eigenstrain
+        # the elastic stage subtracts from that of the stores
+        e = cells("eigenstraint0_crt.in", 6)
+        ctx.upload(FIELD_EIGENSTRAIN_T0_DIAG, e[:3])
+        ctx.upload(FIELD_EIGENSTRAIN_T0_OFF, e[3:])
     if g["if_input_straint0"] and g["if_elasto_on_pORm"] and not g["if_elastodynamics"]:
         e = cells("strain_stat.in", 6)
         ctx.upload(FIELD_STRAIN_T0_DIAG, e[:3])
@@ -747,7 +752,8 @@ class Outputs:
                 ("if_output_emYee", "output_step_emYee"), ("if_output_Estat", "output_step_Estat"),
                 ("if_output_Hstat", "output_step_Hstat"), ("if_output_strain", "output_step_strain"),
                 ("if_output_uandv", "output_step_uandv"), ("if_output_Jp", "output_step_Jp"),
-                ("if_output_Jishe", "output_step_Jishe")]
+                ("if_output_Jishe", "output_step_Jishe"), ("if_output_elastoforce", "output_step_elastoforce"),
+                ("if_output_eigenstraint0_crt", "output_step_eigenstraint0_crt")]
 
     def _cells(self, name: str, arrays) -> None:
         write_cells(self.dir / name, arrays)
@@ -780,6 +786,8 @@ class Outputs:
             self._cells("Hstat.0.dat", ctx.download(FIELD_HSTAT))
         if g["if_elastodynamics"] and g["if_output_strain"]:
             self._cells("strain_stat.0.dat", np.zeros((6,) + tuple(self.deck.problem.shape)))
+        if g["if_output_eigenstraint0_crt"]:
+            self._eigenstrain_t0(ctx, 0)
 
     def step(self, ctx: Context, nstep: int) -> None:
         """The files of step nstep, in main.cpp's order (:601-650)."""
@@ -825,12 +833,29 @@ class Outputs:
         if self._due("if_output_uandv", "output_step_uandv", nstep) and g["if_elastodynamics"]:
             self._cells(f"u.{nstep}.dat", ctx.download(FIELD_U))
             self._cells(f"v.{nstep}.dat", ctx.download(FIELD_V))
+        if self._due("if_output_elastoforce", "output_step_elastoforce", nstep) and g["if_elastodynamics"]:
+            self._cells(f"elastoforce.{nstep}.dat", ctx.download(FIELD_ELASTOFORCE))   # inoutput.cpp:742-761
+        if self._due("if_output_eigenstraint0_crt", "output_step_eigenstraint0_crt", nstep):
+            self._eigenstrain_t0(ctx, nstep)
         if em and self._due("if_output_Jp", "output_step_Jp", nstep):
             jp = ctx.download(FIELD_JP)
             for p in range(3):
                 self._cells(f"Jp{p + 1}.{nstep}.dat", jp[p])
         if g["if_spin_pumping"] and self._due("if_output_Jishe", "output_step_Jishe", nstep):
             self._cells(f"J_ISHE.{nstep}.dat", ctx.download(FIELD_J_ISHE))   # Jx, Jy, 0 (inoutput.cpp:846-864)
+
+    def _eigenstrain_t0(self, ctx: Context, nstep: int) -> None:
+        """output_eigenstraint0_crt (inoutput.cpp:762-789): the static
+        eigenstrain at t0, which nothing updates; without elastodynamics the
+        context holds none and the file repeats what was read (or zeros)."""
+        g = self.g
+        if g["if_elastodynamics"]:
+            e = np.concatenate([ctx.download(FIELD_EIGENSTRAIN_T0_DIAG), ctx.download(FIELD_EIGENSTRAIN_T0_OFF)])
+        elif g["if_input_eigenstraint0_crt"]:
+            e = read_cells(self.deck.directory / "eigenstraint0_crt.in", self.deck.problem.shape, 6)
+        else:
+            e = np.zeros((6,) + tuple(self.deck.problem.shape))
+        self._cells(f"eigenstraint0_crt.{nstep}.dat", e)
 
     def _static_strain_glb(self, ctx: Context) -> np.ndarray:
         """get_strain_static_glb (elastic_system.h:196): the crystal-frame

@@ -282,6 +282,20 @@ enum gf_field_id {
     /* inverse spin Hall current Jx, Jy (and a zero z), 3n, read only
      * (GF_FLAG_SPIN_PUMPING; output_Jishe, inoutput.cpp:846-864) */
     GF_FIELD_J_ISHE = 25,
+    /* elastic force per unit volume stored at the first RK stage, 3n (N/m^3):
+     * the reference's force_*_store that the stiffness damping differences
+     * against (elastic_dynamic_solver.cpp:1313-1323); read and written by
+     * input_elastoforce / output_elastoforce (inoutput.cpp:220-237,
+     * 742-761).  With GF_FLAG_ELASTODYNAMICS. */
+    GF_FIELD_ELASTOFORCE = 26,
+    /* static eigenstrain at t0 in the crystal frame, exx eyy ezz | eyz exz
+     * exy, 3n each: the reference's ex*0t0_crt (elastic_system.h:154), read
+     * by input_eigenstraint0 (inoutput.cpp:259-280).  The elastic stage
+     * subtracts it from the eigenstrain of the m, m_AFM and p stores
+     * (elastic_dynamic_solver.cpp:39-65), so the elastodynamics carries
+     * the change from t0.  With GF_FLAG_ELASTODYNAMICS; zero until uploaded. */
+    GF_FIELD_EIGENSTRAIN_T0_DIAG = 27,
+    GF_FIELD_EIGENSTRAIN_T0_OFF = 28,
     GF_FIELD_COUNT
 };
 

@@ -99,6 +99,7 @@ struct gfo_ctx {
     double *gYZ[3], *gXZ[3], *gXY[3];          /* du{x,y,z}/d{x,y,z} on faces */
     double *e_glb[6], *e_crt[6];               /* dynamic strain xx yy zz yz xz xy */
     double *e0_glb[6], *e0_crt[6];             /* dynamic eigenstrain */
+    double *e0t0[6];                           /* static crystal-frame eigenstrain at t0 (input) */
     double *t1[3], *t2[3], *t3[3], *t4[3];     /* displacement history (Liao / Mur) */
     unsigned char *sfYZ, *sfXZ, *sfXY;         /* stress-free faces */
     double vs_long_1st, vs_tran_1st, vs_long_last, vs_tran_last;
@@ -771,7 +772,7 @@ gfo_ctx* gfo_create(const gf_desc* d) {
         c->gXY[a] = dalloc(c->nx * c->ny * (c->nz + 1));
         c->t1[a] = dalloc(n); c->t2[a] = dalloc(n); c->t3[a] = dalloc(n); c->t4[a] = dalloc(n);
     }
-    for (int a = 0; a < 6; a++) { c->e_glb[a] = dalloc(n); c->e_crt[a] = dalloc(n); c->e0_glb[a] = dalloc(n); c->e0_crt[a] = dalloc(n); }
+    for (int a = 0; a < 6; a++) { c->e_glb[a] = dalloc(n); c->e_crt[a] = dalloc(n); c->e0_glb[a] = dalloc(n); c->e0_crt[a] = dalloc(n); c->e0t0[a] = dalloc(n); }
     for (int a = 0; a < 6; a++) c->e_t0[a] = dalloc(n);
     c->sfYZ = (unsigned char*)calloc((size_t)((c->nx + 1) * nyz), 1);
     c->sfXZ = (unsigned char*)calloc((size_t)(c->nx * (c->ny + 1) * c->nz), 1);
@@ -880,7 +881,7 @@ void gfo_destroy(gfo_ctx* c) {
         free(c->force[a]); free(c->force_store[a]); free(c->gYZ[a]); free(c->gXZ[a]); free(c->gXY[a]);
         free(c->t1[a]); free(c->t2[a]); free(c->t3[a]); free(c->t4[a]);
     }
-    for (int a = 0; a < 6; a++) { free(c->e_glb[a]); free(c->e_crt[a]); free(c->e0_glb[a]); free(c->e0_crt[a]); }
+    for (int a = 0; a < 6; a++) { free(c->e_glb[a]); free(c->e_crt[a]); free(c->e0_glb[a]); free(c->e0_crt[a]); free(c->e0t0[a]); }
     for (int a = 0; a < 6; a++) free(c->e_t0[a]);
     free(c->sfYZ); free(c->sfXZ); free(c->sfXY);
     free(c->surfYZ); free(c->surfXZ); free(c->surfXY); free(c->asurfYZ); free(c->asurfXZ); free(c->asurfXY);
@@ -928,6 +929,9 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_U: return c->u;
     case GF_FIELD_V: return c->v;
     case GF_FIELD_U_STORE: return c->us;
+    case GF_FIELD_ELASTOFORCE: return (c->flags & GF_FLAG_ELASTODYNAMICS) ? c->force_store : NULL;
+    case GF_FIELD_EIGENSTRAIN_T0_DIAG: return (c->flags & GF_FLAG_ELASTODYNAMICS) ? c->e0t0 : NULL;
+    case GF_FIELD_EIGENSTRAIN_T0_OFF: return (c->flags & GF_FLAG_ELASTODYNAMICS) ? c->e0t0 + 3 : NULL;
     case GF_FIELD_STRAIN_DIAG: return c->e_glb;
     case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
     case GF_FIELD_HDRIVE: return (c->flags & GF_FLAG_HDRIVE) ? c->Hdrive : NULL;
@@ -1614,14 +1618,14 @@ static void elastic_eigenstrain(gfo_ctx* c) {
         glb2crt(c, c->ms2[0][id], c->ms2[1][id], c->ms2[2][id], &bx, &by, &bz);
         double s[3][3], o[3][3];
         s[0][0] = 1.5 * l100 * (pow(mx, 2.) - 1. / 3.) + Q11 * pow(px, 2.) + Q12 * (pow(py, 2.) + pow(pz, 2.))
-                + 0.5 * 1.5 * l100a * (pow(ax, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(bx, 2.) - 1. / 3.) - 0.;
+                + 0.5 * 1.5 * l100a * (pow(ax, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(bx, 2.) - 1. / 3.) - c->e0t0[0][id];
         s[1][1] = 1.5 * l100 * (pow(my, 2.) - 1. / 3.) + Q11 * pow(py, 2.) + Q12 * (pow(px, 2.) + pow(pz, 2.))
-                + 0.5 * 1.5 * l100a * (pow(ay, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(by, 2.) - 1. / 3.) - 0.;
+                + 0.5 * 1.5 * l100a * (pow(ay, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(by, 2.) - 1. / 3.) - c->e0t0[1][id];
         s[2][2] = 1.5 * l100 * (pow(mz, 2.) - 1. / 3.) + Q11 * pow(pz, 2.) + Q12 * (pow(py, 2.) + pow(px, 2.))
-                + 0.5 * 1.5 * l100a * (pow(az, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(bz, 2.) - 1. / 3.) - 0.;
-        s[1][2] = 1.5 * l111 * my * mz + Q44 * py * pz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz - 0.;
-        s[0][2] = 1.5 * l111 * mx * mz + Q44 * px * pz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz - 0.;
-        s[0][1] = 1.5 * l111 * my * mx + Q44 * py * px + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx - 0.;
+                + 0.5 * 1.5 * l100a * (pow(az, 2.) - 1. / 3.) + 0.5 * 1.5 * l100b * (pow(bz, 2.) - 1. / 3.) - c->e0t0[2][id];
+        s[1][2] = 1.5 * l111 * my * mz + Q44 * py * pz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz - c->e0t0[3][id];
+        s[0][2] = 1.5 * l111 * mx * mz + Q44 * px * pz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz - c->e0t0[4][id];
+        s[0][1] = 1.5 * l111 * my * mx + Q44 * py * px + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx - c->e0t0[5][id];
         if (c->flexo_on && c->if_FE_all) {
             /* elastic_dynamic_solver.cpp:82-89: the global-frame p gradient
              * of FE cells (zero elsewhere) enters the crystal-frame tensor */

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from goferro import (FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_M_STORE, FIELD_P, FIELD_P_STORE, FIELD_Q,
+from goferro import (FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF, FIELD_M, FIELD_M_AFM1, FIELD_M_AFM2, FIELD_M_STORE, FIELD_P, FIELD_P_STORE, FIELD_Q,
                      FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF, FIELD_U, FIELD_U_STORE, FIELD_V, FLAG_1D_ABC,
                      FLAG_1D_ABC_ELAST_ONLYTOP, FLAG_ELASTO_BACKACTION, FLAG_ELASTO_ON_PM, FLAG_ELASTODYNAMICS,
                      FLAG_GAUSSIAN_PULSE, FLAG_ORIENT_ROTATE, Context, Material, Problem, workloads)
@@ -225,6 +225,74 @@ def test_afm_elastic_stages():
         np.testing.assert_allclose(g.download(fid), o.get(fid), rtol=0, atol=1e-12)
     g.close()
     o.close()
+
+
+def random_e0t0(shape, seed, scale=2e-5):
+    rng = np.random.default_rng(seed)
+    e = scale * rng.standard_normal((6,) + tuple(shape))
+    return {FIELD_EIGENSTRAIN_T0_DIAG: e[:3], FIELD_EIGENSTRAIN_T0_OFF: e[3:]}
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_magnetoelastic_stages_with_static_eigenstrain(rotate):
+    """The static eigenstrain at t0 (ex*0t0_crt) subtracted from the
+    store eigenstrain (elastic_dynamic_solver.cpp:39-65), stage by stage."""
+    prob, m = magnetoelastic(rotate=rotate)
+    u, v = random_uv(prob.shape, 16)
+    fields = {FIELD_M: m, FIELD_U: u, FIELD_V: v, **random_e0t0(prob.shape, 17)}
+    g, o = run_stages(prob, fields, nsteps=2, phys=("el", "fm"), check=(FIELD_M_STORE,))
+    np.testing.assert_allclose(g.download(FIELD_M), o.get(FIELD_M), rtol=0, atol=1e-12)
+    for fid in (FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF):
+        np.testing.assert_array_equal(g.download(fid), fields[fid])
+    g.close()
+    o.close()
+
+
+def test_ferroelectric_elastic_stages_with_static_eigenstrain():
+    """The ferroelectric reads the same eigenstrain back (its elastic driving
+    force, ferroelectric_solver.cpp:630-640)."""
+    prob, p, q = fe_elastic()
+    u, v = random_uv(prob.shape, 18)
+    fields = {FIELD_P: p, FIELD_Q: q, FIELD_U: u, FIELD_V: v, **random_e0t0(prob.shape, 19, scale=1e-3)}
+    g, o = run_stages(prob, fields, nsteps=2, phys=("el", "fe"))
+    for fid in (FIELD_P_STORE, FIELD_P, FIELD_Q):
+        assert rel(g.download(fid), o.get(fid)) <= 1e-12, fid
+    g.close()
+    o.close()
+
+
+def test_static_eigenstrain_cancels_a_frozen_magnet():
+    """With the eigenstrain of a frozen uniform m as the static one the film
+    stays at rest through gf_step; without it the top surface moves."""
+    from goferro import FLAG_GAUSSIAN_PULSE
+    say("case static eigenstrain, frozen magnet")
+    prob, _ = workloads.magnetoelastic_film(shape=(8, 8, 12), magnet_layers=4)
+    prob.flags &= ~FLAG_GAUSSIAN_PULSE
+    mag = prob.materials[0]
+    mag.gyro = 0.0
+    mdir = np.array([0.36, -0.48, 0.8])
+    magnet = np.asarray(prob.material_cell) == 1
+    m = np.zeros((3,) + prob.shape)
+    m[:, magnet] = mdir[:, None]
+    mx, my, mz = mdir
+    l100, l111 = mag.lamda100, mag.lamda111
+    e0 = np.zeros((6,) + prob.shape)
+    e0[:, magnet] = np.array([1.5 * l100 * (mx * mx - 1.0 / 3.0), 1.5 * l100 * (my * my - 1.0 / 3.0),
+                              1.5 * l100 * (mz * mz - 1.0 / 3.0), 1.5 * l111 * my * mz, 1.5 * l111 * mx * mz,
+                              1.5 * l111 * my * mx])[:, None]
+    peaks = []
+    for t0 in (None, e0):
+        g = Context(prob)
+        g.upload(FIELD_M, m)
+        if t0 is not None:
+            g.upload(FIELD_EIGENSTRAIN_T0_DIAG, t0[:3])
+            g.upload(FIELD_EIGENSTRAIN_T0_OFF, t0[3:])
+        g.step(20)
+        peaks.append(np.abs(g.download(FIELD_V)).max())
+        g.close()
+    say("  peak |v| without, with", peaks)
+    assert peaks[0] > 1e-6
+    assert peaks[1] <= 1e-12 * peaks[0]
 
 
 def test_stage_order_is_enforced():

@@ -105,6 +105,16 @@ def test_restart_magnetoelastic(mur):
               (FIELD_M, FIELD_U, FIELD_V, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF), 6, 7, statics=False)
 
 
+def test_restart_carries_the_static_eigenstrain():
+    from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF
+    from test_gpu_elastic import magnetoelastic, random_e0t0, random_uv
+    prob, m = magnetoelastic(shape=(8, 8, 12))
+    u, v = random_uv(prob.shape, 26)
+    f = {FIELD_M: m, FIELD_U: u, FIELD_V: v, **random_e0t0(prob.shape, 27)}
+    split_run("magnetoelastic, static eigenstrain", prob, f,
+              (FIELD_M, FIELD_U, FIELD_V, FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF), 3, 4, statics=False)
+
+
 def test_restart_ferroelectric_electrostatics():
     from test_gpu_estatic import block
     prob, p, q = block()
@@ -314,3 +324,59 @@ def test_elastostatic_deck_writes_the_global_static_strain(tmp_path):
     assert np.abs(got - want).max() <= 1e-7 * np.abs(want).max()
     # a rotation about z mixes xx and yy: the file is not the crystal frame
     assert np.abs(got[0] - dg[0]).max() > 1e-3 * np.abs(want).max()
+
+
+def test_elastoforce_deck_run_matches_oracle(tmp_path):
+    """A magnetoelastic deck with stiffness damping whose force store is read
+    from elastoforce.in (main.cpp:74-76): the first stage-1 velocity update
+    differences against it (elastic_dynamic_solver.cpp:1313-1323), so v and
+    the store written to elastoforce.<n>.dat follow the oracle given the
+    same store, and differ from a run without it."""
+    from oracle.oracle import Oracle
+    from test_io import deck_copy
+    from goferro import FIELD_ELASTOFORCE
+    say("case elastoforce deck run")
+    # dv gains (eta/rho)(f - f_store) with no dt (elastic_dynamic_solver.cpp:1310-1313):
+    # eta = 1e-13 Pa s keeps that jump the size of the magnetostrictive response
+    damping = "0 1e-13                     elast_mass_damping elast_stiff_damping"
+    d = deck_copy(tmp_path, edits=[
+        ("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "true true true"),
+        ("if_1D_ABC", "true false false 0.5"), ("if_input_elastoforce", "true"),
+        ("if_output_elastoforce", "true 10"), ("if_output_uandv", "true 20")],
+        mat_edits=[(1, "elast_mass_damping", damping), (2, "elast_mass_damping", damping)])
+    rng = np.random.default_rng(5)
+    gio.write_cells(d / "elastoforce.in", rng.standard_normal((3, 8, 8, 4)) * 1e13)
+    out = tmp_path / "out"
+    ctx = gio.run(d, out_dir=out)
+    deck = gio.read_deck(d)
+    shape = deck.problem.shape
+    store0 = gio.read_cells(d / "elastoforce.in", shape, 3)
+
+    def oracle_run(with_store):
+        o = Oracle(deck.problem)
+        for fid, v in gio.initial_fields(deck).items():
+            o.set(fid, v)
+        o.set(FIELD_M, gio.read_cells(d / "m.in", shape, 3))
+        if with_store:
+            o.set(FIELD_ELASTOFORCE, store0)
+        o.demag()
+        o.step(10)
+        f10 = o.get(FIELD_ELASTOFORCE)
+        o.step(10)
+        res = f10, o.get(FIELD_ELASTOFORCE), o.get(FIELD_V), o.get(FIELD_U)
+        o.close()
+        return res
+
+    f10, f20, v20, u20 = oracle_run(True)
+    scale_v = np.abs(v20).max()
+    assert scale_v > 0
+    assert np.abs(ctx.download(FIELD_V) - v20).max() <= 1e-11 * scale_v
+    assert np.abs(ctx.download(FIELD_ELASTOFORCE) - f20).max() <= 1e-11 * np.abs(f20).max()
+    for n, want in ((10, f10), (20, f20)):
+        got = gio.read_cells(out / f"elastoforce.{n}.dat", shape, 3)
+        assert np.abs(got - want).max() <= 5e-8 * np.abs(want).max(), n
+    got_v = gio.read_cells(out / "v.20.dat", shape, 3)
+    assert np.abs(got_v - v20).max() <= 5e-8 * scale_v
+    ctx.close()
+    _, _, v_free, _ = oracle_run(False)
+    assert np.abs(v_free - v20).max() > 1e-3 * scale_v

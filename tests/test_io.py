@@ -197,12 +197,101 @@ def test_prescribed_m_deck(tmp_path):
 
 @pytest.mark.parametrize("key,text,what", [
     ("if_elastostatic ", "true true", "elastostatics"),
-    ("if_input_elastoforce", "true", "if_input_elastoforce")])
+    ("if_elastostatic ", "false true", "1D")])
 def test_unsupported_options_are_named(tmp_path, key, text, what):
     d = deck_copy(tmp_path, edits=[(key, text)])
     with pytest.raises(gio.UnsupportedInput, match=what):
         gio.read_deck(d)
     gio.read_deck(d, allow_unsupported=True)
+
+
+class _RecordingContext:
+    """Stands in for a Context in the deck runner's host logic: records
+    uploads and serves them (zeros otherwise) to the output writer."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.uploads = {}
+
+    def upload(self, field, value):
+        self.uploads[field] = np.asarray(value, dtype=np.float64)
+
+    def download(self, field):
+        return self.uploads.get(field, np.zeros((3,) + self.shape))
+
+    def average(self, field):
+        return np.zeros(3)
+
+
+def test_elastoforce_input_and_output(tmp_path):
+    """if_input_elastoforce reads elastoforce.in into the force store
+    (main.cpp:74-76, inoutput.cpp:220-237) and if_output_elastoforce writes
+    elastoforce.<n>.dat in the reference's cell format (inoutput.cpp:742-761)
+    at its own interval."""
+    from goferro import FIELD_ELASTOFORCE
+    d = deck_copy(tmp_path, edits=[
+        ("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "true true true"),
+        ("if_input_elastoforce", "true"), ("if_output_elastoforce", "true 4")])
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((3, 8, 8, 4)) * 1e13
+    gio.write_cells(d / "elastoforce.in", f)
+    deck = gio.read_deck(d)
+    assert deck.problem.flags & FLAG_ELASTODYNAMICS
+    assert not any("elastoforce" in s for s in deck.skipped_outputs)
+    ctx = _RecordingContext(deck.problem.shape)
+    read = gio.load_inputs(ctx, deck)
+    assert "elastoforce.in" in read
+    np.testing.assert_array_equal(ctx.uploads[FIELD_ELASTOFORCE], gio.read_cells(d / "elastoforce.in", (8, 8, 4), 3))
+    assert np.abs(ctx.uploads[FIELD_ELASTOFORCE] - f).max() <= 5e-8 * np.abs(f).max()
+    out = gio.Outputs(deck, tmp_path / "out")
+    assert out.next_due(0) == 4
+    for n in (3, 4):
+        out.step(ctx, n)
+    assert not (tmp_path / "out" / "elastoforce.3.dat").exists()
+    assert (tmp_path / "out" / "elastoforce.4.dat").read_text() == reference_lines(ctx.uploads[FIELD_ELASTOFORCE])
+    # without elastodynamics nothing is read and the output is reported skipped
+    (tmp_path / "b").mkdir()
+    d2 = deck_copy(tmp_path / "b", edits=[("if_input_elastoforce", "true"), ("if_output_elastoforce", "true 4")])
+    deck2 = gio.read_deck(d2)
+    assert any("elastoforce" in s for s in deck2.skipped_outputs)
+    assert "elastoforce.in" not in gio.load_inputs(_RecordingContext(deck2.problem.shape), deck2)
+
+
+def test_eigenstrain_t0_input_and_output(tmp_path):
+    """if_input_eigenstraint0_crt reads eigenstraint0_crt.in (six values a
+    cell, inoutput.cpp:259-280) into the static eigenstrain, and
+    if_output_eigenstraint0_crt writes it at step 0 and at its interval
+    (main.cpp:196-198, 696-700; inoutput.cpp:762-789)."""
+    from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF
+    d = deck_copy(tmp_path, edits=[
+        ("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "true true true"),
+        ("if_input_uandv", "false false true"), ("if_output_elastoforce", "false 1"),
+        ("if_output_eigenstraint0_crt", "true 5")])
+    rng = np.random.default_rng(12)
+    e = rng.standard_normal((6, 8, 8, 4)) * 1e-5
+    gio.write_cells(d / "eigenstraint0_crt.in", e)
+    deck = gio.read_deck(d)
+    ctx = _RecordingContext(deck.problem.shape)
+    assert "eigenstraint0_crt.in" in gio.load_inputs(ctx, deck)
+    back = gio.read_cells(d / "eigenstraint0_crt.in", (8, 8, 4), 6)
+    np.testing.assert_array_equal(ctx.uploads[FIELD_EIGENSTRAIN_T0_DIAG], back[:3])
+    np.testing.assert_array_equal(ctx.uploads[FIELD_EIGENSTRAIN_T0_OFF], back[3:])
+    out = gio.Outputs(deck, tmp_path / "out")
+    out.initial(ctx)
+    out.step(ctx, 5)
+    for n in (0, 5):
+        assert (tmp_path / "out" / f"eigenstraint0_crt.{n}.dat").read_text() == reference_lines(back)
+    # without elastodynamics the file repeats what was read
+    (tmp_path / "b").mkdir()
+    d2 = deck_copy(tmp_path / "b", edits=[("if_input_uandv", "false false true"),
+                                          ("if_output_eigenstraint0_crt", "true 5")])
+    gio.write_cells(d2 / "eigenstraint0_crt.in", e)
+    deck2 = gio.read_deck(d2)
+    ctx2 = _RecordingContext(deck2.problem.shape)
+    assert "eigenstraint0_crt.in" not in gio.load_inputs(ctx2, deck2)
+    out2 = gio.Outputs(deck2, tmp_path / "out2")
+    out2.initial(ctx2)
+    assert (tmp_path / "out2" / "eigenstraint0_crt.0.dat").read_text() == reference_lines(back)
 
 
 def test_fft_electrostatics_with_fft_magnetostatics_is_read(tmp_path):

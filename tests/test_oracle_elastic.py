@@ -128,3 +128,61 @@ def test_mass_damping_acts_on_base_velocity(oracle_lib):
     v = o.get(FIELD_V)[1].reshape(-1)
     # the same base-state derivative in all four stages makes RK4 forward Euler
     np.testing.assert_allclose(v, 2.5 * (1 - eta * dt) ** N, rtol=1e-12)
+
+
+def _store_eigenstrain_crt(mat, m_crt):
+    """The magnetostrictive eigenstrain of a uniform crystal-frame m
+    (elastic_dynamic_solver.cpp:39-65 with p = 0 and no AFM), in the
+    reference's evaluation order: xx yy zz | yz xz xy."""
+    mx, my, mz = (float(v) for v in m_crt)
+    l100, l111 = mat.lamda100, mat.lamda111
+    return [1.5 * l100 * (mx * mx - 1.0 / 3.0), 1.5 * l100 * (my * my - 1.0 / 3.0), 1.5 * l100 * (mz * mz - 1.0 / 3.0),
+            1.5 * l111 * my * mz, 1.5 * l111 * mx * mz, 1.5 * l111 * my * mx]
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_static_eigenstrain_cancels_a_frozen_magnet(oracle_lib, rotate):
+    """A static eigenstrain at t0 equal to the eigenstrain of the magnet's
+    m (elastic_dynamic_solver.cpp:39-65 subtracts ex*0t0_crt) leaves no
+    stress: with m frozen (gyro = 0) and u = v = 0 the film stays at rest,
+    while without it, or with its shear entries permuted, the free top
+    surface moves."""
+    from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF, FIELD_M, FLAG_GAUSSIAN_PULSE, \
+        FLAG_ORIENT_ROTATE
+    from goferro.workloads import magnetoelastic_film
+    prob, _ = magnetoelastic_film(shape=(4, 4, 8), magnet_layers=3)
+    prob.flags &= ~FLAG_GAUSSIAN_PULSE
+    mag = prob.materials[0]
+    mag.gyro = 0.0
+    T = np.eye(3)
+    if rotate:
+        c, s = np.cos(0.7), np.sin(0.7)
+        T = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]]) @ np.array([[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]])
+        for mat in prob.materials:
+            set_elastic(mat, mat.c11, mat.c12, mat.c44, mat.density, Tg2c=T)
+        prob.flags |= FLAG_ORIENT_ROTATE
+        prob.Tg2c = T
+    mdir = np.array([0.36, -0.48, 0.8])
+    magnet = np.asarray(prob.material_cell) == 1
+    m = np.zeros((3,) + prob.shape)
+    m[:, magnet] = mdir[:, None]
+    e0 = np.zeros((6,) + prob.shape)
+    e0[:, magnet] = np.asarray(_store_eigenstrain_crt(mag, T @ mdir))[:, None]
+
+    def run(t0):
+        o = oracle_lib.Oracle(prob)
+        o.set(FIELD_M, m)
+        if t0 is not None:
+            o.set(FIELD_EIGENSTRAIN_T0_DIAG, t0[:3])
+            o.set(FIELD_EIGENSTRAIN_T0_OFF, t0[3:])
+        o.step(20)
+        v = o.get(FIELD_V)
+        o.close()
+        return np.abs(v).max()
+
+    free = run(None)
+    assert free > 1e-6
+    assert run(e0) <= 1e-12 * free
+    swapped = e0.copy()
+    swapped[[3, 5]] = e0[[5, 3]]
+    assert run(swapped) > 1e-3 * free
