@@ -930,7 +930,8 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     }
     if (!st && (c->flags & GF_FLAG_ELASTODYNAMICS)) {
         c->el_on = true;
-        double** el_bufs[] = {c->u, c->v, c->us, c->vs, c->accu, c->accv, c->fstore, c->epx, c->epy, c->epz};
+        double** el_bufs[] = {c->u, c->v, c->us, c->us_alt, c->vs, c->accu, c->accv, c->fstore, c->epx, c->epy,
+                              c->epz};
         for (double** b : el_bufs)
             if (!st) st = alloc3(c, b, "hipMalloc elastic state");
         const int64_t nx = c->nx, ny = c->ny, nz = c->nz;
@@ -1012,7 +1013,7 @@ gf_status gf_destroy(gf_ctx* c) {
     free3(c->p); free3(c->q); free3(c->pacc); free3(c->qacc);
     free3(c->p_a); free3(c->p_b); free3(c->q_a); free3(c->q_b); free3(c->estat);
     free3(c->u); free3(c->v); free3(c->us); free3(c->vs); free3(c->accu); free3(c->accv); free3(c->fstore);
-    free3(c->gyz); free3(c->gxz); free3(c->gxy); free3(c->epx); free3(c->epy); free3(c->epz);
+    free3(c->gyz); free3(c->gxz); free3(c->gxy); free3(c->us_alt); free3(c->epx); free3(c->epy); free3(c->epz);
     for (int a = 0; a < 6; a++) {
         if (c->ec[a] && c->ec[a] != c->eg[a]) hipFree(c->ec[a]);
         if (c->e0c[a] && c->e0c[a] != c->e0g[a]) hipFree(c->e0c[a]);
@@ -1742,16 +1743,39 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
     const int64_t nxy = c->nx * c->ny;
     const int64_t slot_size = static_cast<int64_t>(kHistPlanes) * 3 * nxy;
 
-    // strain of the stage store (elastic_dynamic_solver.cpp:107-227), the
-    // eigenstrain from the stores of this stage (:4-104) and the cells' parts
-    // of their face stresses, in one launch
     gf_status st = GF_OK;
-    {
-        ElPrepArgs pa{};
-        ElStrainArgs& a = pa.s;
+    // stage 4: u enters the boundary history before the final combine
+    // (transfer_pointer, elastodynamics_AbsorbingBoundary.cpp:3-89)
+    if (stage == 4) {
+        c->hist_head = (c->hist_head + 1) % c->hist_slots;
+        ElHistArgs h{};
+        for (int i = 0; i < 3; i++) h.u[i] = c->u[i];
+        h.slot = c->hist + c->hist_head * slot_size;
+        h.nx = c->nx; h.ny = c->ny; h.nz = c->nz;
+        const int32_t lp[kHistPlanes] = {1, 2, 3, 4};
+        const int32_t mp[kHistPlanes] = {0, 1, static_cast<int32_t>(c->nz - 2), static_cast<int32_t>(c->nz - 1)};
+        for (int p = 0; p < kHistPlanes; p++) h.planes[p] = mur ? mp[p] : lp[p];
+        hipEvent_t ev = nullptr;
+        timer_begin(c, GF_K_OTHER, &ev);
+        st = hip_check(c, launch_el_history(h, c->stream), "boundary history");
+        timer_end(c, GF_K_OTHER, ev);
+    }
+    const double t_stage = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
+    // strain of the stage store (elastic_dynamic_solver.cpp:107-227), the
+    // eigenstrain from the stores of this stage (:4-104) and the cells'
+    // parts of their face stresses; then the stress divergence and the v, u
+    // updates (:841-3057)
+    if (!st) {
+        ElStageArgs sa{};
+        ElStrainArgs& a = sa.s;
         a.code = c->d_code;
+        // the face gradients are kept only for the flexoelectric force
+        const bool faces = c->d_flexo != nullptr;
         for (int i = 0; i < 3; i++) {
-            a.us[i] = c->us[i]; a.gyz[i] = c->gyz[i]; a.gxz[i] = c->gxz[i]; a.gxy[i] = c->gxy[i];
+            a.us[i] = c->us[i];
+            a.gyz[i] = faces ? c->gyz[i] : nullptr;
+            a.gxz[i] = faces ? c->gxz[i] : nullptr;
+            a.gxy[i] = faces ? c->gxy[i] : nullptr;
         }
         for (int i = 0; i < 6; i++) { a.eg[i] = c->eg[i]; a.ec[i] = c->ec[i]; }
         a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
@@ -1759,7 +1783,7 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
         for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
         a.rotate = rotate;
         std::memcpy(a.Tg2c, c->Tg2c, sizeof(a.Tg2c));
-        ElEigenArgs& e = pa.e;
+        ElEigenArgs& e = sa.e;
         e.code = c->d_code; e.mats = c->d_mats;
         double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : nullptr;
         double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : nullptr;
@@ -1781,45 +1805,18 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
         e.rotate = rotate;
         std::memcpy(e.Tg2c, c->Tg2c, sizeof(e.Tg2c));
         std::memcpy(e.Tc2g, c->Tc2g, sizeof(e.Tc2g));
-        pa.eig = (c->flags & GF_FLAG_ELASTO_BACKACTION) ? 1 : 0;
+        sa.eig = (c->flags & GF_FLAG_ELASTO_BACKACTION) ? 1 : 0;
         // only a ferroelectric stage reads the eigenstrain back (the next
         // elastic stage recomputes it before any other use)
-        pa.e0_out = (c->any_fe && strain_coupled(c)) ? 1 : 0;
-        pa.nmat = static_cast<int32_t>(c->host_mats.size());
-        for (int i = 0; i < 3; i++) { pa.px[i] = c->epx[i]; pa.py[i] = c->epy[i]; pa.pz[i] = c->epz[i]; }
-        hipEvent_t ev = nullptr;
-        timer_begin(c, GF_K_ELASTIC_STRAIN, &ev);
-        st = hip_check(c, launch_el_prep(pa, c->stream), "elastic strain and stress parts");
-        timer_end(c, GF_K_ELASTIC_STRAIN, ev);
-    }
-    // stage 4: u enters the boundary history before the final combine
-    // (transfer_pointer, elastodynamics_AbsorbingBoundary.cpp:3-89)
-    if (!st && stage == 4) {
-        c->hist_head = (c->hist_head + 1) % c->hist_slots;
-        ElHistArgs h{};
-        for (int i = 0; i < 3; i++) h.u[i] = c->u[i];
-        h.slot = c->hist + c->hist_head * slot_size;
-        h.nx = c->nx; h.ny = c->ny; h.nz = c->nz;
-        const int32_t lp[kHistPlanes] = {1, 2, 3, 4};
-        const int32_t mp[kHistPlanes] = {0, 1, static_cast<int32_t>(c->nz - 2), static_cast<int32_t>(c->nz - 1)};
-        for (int p = 0; p < kHistPlanes; p++) h.planes[p] = mur ? mp[p] : lp[p];
-        hipEvent_t ev = nullptr;
-        timer_begin(c, GF_K_OTHER, &ev);
-        st = hip_check(c, launch_el_history(h, c->stream), "boundary history");
-        timer_end(c, GF_K_OTHER, ev);
-    }
-    const double t_stage = stage == 1 ? c->time - c->dt : (stage == 4 ? c->time : c->time - 0.5 * c->dt);
-    // stress divergence and the v, u updates (:841-3057)
-    if (!st) {
-        ElForceArgs f{};
+        sa.e0_out = (c->any_fe && strain_coupled(c)) ? 1 : 0;
+        for (int i = 0; i < 3; i++) { sa.px[i] = c->epx[i]; sa.py[i] = c->epy[i]; sa.pz[i] = c->epz[i]; }
+        ElForceArgs& f = sa.f;
         f.code = c->d_code; f.mats = c->d_mats;
         f.nmat = static_cast<int32_t>(c->host_mats.size());
         for (int i = 0; i < 3; i++) {
-            f.u[i] = c->u[i]; f.v[i] = c->v[i]; f.us[i] = c->us[i]; f.vs[i] = c->vs[i];
+            f.u[i] = c->u[i]; f.v[i] = c->v[i]; f.us[i] = c->us_alt[i]; f.vs[i] = c->vs[i];
             f.accu[i] = c->accu[i]; f.accv[i] = c->accv[i]; f.fstore[i] = c->fstore[i];
-            f.gyz[i] = c->gyz[i]; f.gxz[i] = c->gxz[i]; f.gxy[i] = c->gxy[i];
         }
-        for (int i = 0; i < 3; i++) { f.px[i] = c->epx[i]; f.py[i] = c->epy[i]; f.pz[i] = c->epz[i]; }
         f.nx = c->nx; f.ny = c->ny; f.nz = c->nz; f.dx = c->dx; f.dy = c->dy; f.dz = c->dz; f.dt = c->dt;
         f.idx = 1. / c->dx; f.idy = 1. / c->dy; f.idz = 1. / c->dz;
         for (int i = 0; i < 3; i++) f.per[i] = c->per[i];
@@ -1835,8 +1832,10 @@ gf_status gf_elastic_stage(gf_ctx* c, int stage) {
         }
         hipEvent_t ev = nullptr;
         timer_begin(c, GF_K_ELASTIC_STAGE, &ev);
-        st = hip_check(c, launch_el_force(f, c->stream), "elastic stage");
+        st = hip_check(c, launch_el_stage(sa, c->stream), "elastic stage");
         timer_end(c, GF_K_ELASTIC_STAGE, ev);
+        // the store just written is the stage's u store from here on
+        if (!st) std::swap(c->us, c->us_alt);
     }
     // absorbing boundary and displacement source
     // (elastodynamics_AbsorbingBoundary.cpp:92-230, elastic_dynamic_solver.cpp:3060-3284)

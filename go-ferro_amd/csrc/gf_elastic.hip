@@ -2,18 +2,18 @@
 //
 // One stage of the reference's elastic solver (elastic_dynamic_solver.cpp,
 // elastodynamics_AbsorbingBoundary.cpp) is up to four launches:
+//   history   stage 4 only: the boundary planes of u into the history ring
+//             (transfer_pointer, elastodynamics_AbsorbingBoundary.cpp:3-89)
 //   prep      face gradients of the u store and the cell strain (:107-227),
 //             the eigenstrain from the m, m_AFM and p stores (:4-104) with
 //             back-action, and each cell's part of its face stresses
-//   history   stage 4 only: the boundary planes of u into the history ring
-//             (transfer_pointer, elastodynamics_AbsorbingBoundary.cpp:3-89)
 //   force     stress divergence, dv and du = v_s dt, the v and u RK updates
 //             (get_dv_RKs / get_du_RKs / update_v_RKs / update_u_RKs, :841-3057)
 //   boundary  Liao or Mur planes and the displacement source
 //             (elastodynamics_AbsorbingBoundary.cpp:92-230, :3060-3284)
-// The u and v stores are updated in place: within a stage a cell's store is
-// read only by its own thread, because the stress reads face gradients and
-// stress parts that the prep launch computed from the store beforehand.
+// The force launch reads the u store around each cell (its face gradients)
+// and writes the next store into a second array (the two are swapped after
+// it); v and its store are read and written by their own cell's thread only.
 //
 // RK4 bookkeeping keeps a state accumulator per field, acc = y + k1/6,
 // acc = acc + k2/3, acc = acc + k3/3, y = acc + k4/6, which is the
@@ -63,14 +63,16 @@ __device__ __forceinline__ void cell_ijk(int64_t id, int64_t ny, int64_t nz, int
     }
 }
 
-// Face gradients g[face][comp] and cell strain e of cell id = (i, j, k) of
-// the u store, both written out (the strain also in the crystal frame).
-// The differences are multiplied by the reciprocal spacings (within an ulp
-// of the reference's quotients).
-__device__ __forceinline__ void cell_strain(const ElStrainArgs& a, int64_t id, double (&g)[6][3], double (&e)[6]) {
+// Face gradients g[face][comp] of cell id = (i, j, k) of the u store,
+// faces ordered x-, x+, y-, y+, z-, z+.  A stress-free face (vacuum on a
+// side, or a non-periodic wall) has zero gradient; across a periodic wall
+// the difference wraps (elastic_dynamic_solver.cpp:107-187).  The
+// differences are multiplied by the reciprocal spacings (within an ulp of
+// the reference's quotients); the upper face of a cell and the lower face
+// of the cell above it are the same expression, hence the same value.
+__device__ __forceinline__ void cell_grad(const ElStrainArgs& a, int64_t id, int64_t i, int64_t j, int64_t k,
+                                          double (&g)[6][3]) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    int64_t i, j, k;
-    cell_ijk(id, ny, nz, i, j, k);
     const bool me = solid(a.code, id);
     const int64_t nb[6] = {((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k, ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k,
                            (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k, (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k,
@@ -78,9 +80,6 @@ __device__ __forceinline__ void cell_strain(const ElStrainArgs& a, int64_t id, d
     const bool wall[6] = {i == 0 && !a.per[0], i == nx - 1 && !a.per[0], j == 0 && !a.per[1],
                           j == ny - 1 && !a.per[1], k == 0 && !a.per[2], k == nz - 1 && !a.per[2]};
     const double ih[3] = {a.idx, a.idy, a.idz};
-    // g[face][comp]: a stress-free face (vacuum on a side, or a
-    // non-periodic wall) has zero gradient; across a periodic wall the
-    // difference wraps (elastic_dynamic_solver.cpp:107-187)
     for (int f = 0; f < 6; f++) {
         const bool fr = wall[f] || !(me && solid(a.code, nb[f]));
         for (int c = 0; c < 3; c++) {
@@ -89,6 +88,13 @@ __device__ __forceinline__ void cell_strain(const ElStrainArgs& a, int64_t id, d
             else g[f][c] = (a.us[c][id] - a.us[c][nb[f]]) * ih[f >> 1];
         }
     }
+}
+
+// The face arrays: a cell writes its lower faces, the last cell along an
+// axis its upper face too.
+__device__ __forceinline__ void grad_out(const ElStrainArgs& a, int64_t i, int64_t j, int64_t k,
+                                         const double (&g)[6][3]) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
     for (int c = 0; c < 3; c++) {
         a.gyz[c][(i * ny + j) * nz + k] = g[0][c];
         if (i == nx - 1) a.gyz[c][(nx * ny + j) * nz + k] = g[1][c];
@@ -97,13 +103,20 @@ __device__ __forceinline__ void cell_strain(const ElStrainArgs& a, int64_t id, d
         a.gxy[c][(i * ny + j) * (nz + 1) + k] = g[4][c];
         if (k == nz - 1) a.gxy[c][(i * ny + j) * (nz + 1) + nz] = g[5][c];
     }
-    // cell strain as the average of its face gradients (:190-227)
+}
+
+// Cell strain as the average of its face gradients (:190-227).
+__device__ __forceinline__ void strain_of(const double (&g)[6][3], double (&e)[6]) {
     e[0] = (g[0][0] + g[1][0]) / 2.;
     e[1] = (g[2][1] + g[3][1]) / 2.;
     e[2] = (g[4][2] + g[5][2]) / 2.;
     e[3] = (g[4][1] + g[5][1] + g[2][2] + g[3][2]) / 4.;
     e[4] = (g[4][0] + g[5][0] + g[0][2] + g[1][2]) / 4.;
     e[5] = (g[2][0] + g[3][0] + g[0][1] + g[1][1]) / 4.;
+}
+
+// The strain out, global frame and (rotated) crystal frame.
+__device__ __forceinline__ void strain_out(const ElStrainArgs& a, int64_t id, const double (&e)[6]) {
     for (int c = 0; c < 6; c++) a.eg[c][id] = e[c];
     if (a.ec[0] != a.eg[0]) {
         double o[6];
@@ -116,14 +129,22 @@ __device__ __forceinline__ void cell_strain(const ElStrainArgs& a, int64_t id, d
 __global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= a.nx * a.ny * a.nz) return;
+    int64_t i, j, k;
+    cell_ijk(id, a.ny, a.nz, i, j, k);
     double g[6][3], e[6];
-    cell_strain(a, id, g, e);
+    cell_grad(a, id, i, j, k, g);
+    grad_out(a, i, j, k, g);
+    strain_of(g, e);
+    strain_out(a, id, e);
 }
 
 // ----------------------------------------------------------- eigenstrain
 // Eigenstrain of cell id from the stage stores (elastic_dynamic_solver.cpp:
 // 4-104) less the static eigenstrain at t0 when there is one: s in the
 // crystal frame, o in the global frame, zero for vacuum.
+// kGeneral: with rotation and the flexoelectric terms; the lean form is
+// used only when the context has neither.
+template <bool kGeneral = true>
 __device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, double s[6], double o[6]) {
     const uint32_t t = a.code[id] & kMatMask;
     for (int c = 0; c < 6; c++) { s[c] = 0.; o[c] = 0.; }
@@ -137,7 +158,7 @@ __device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, do
         for (int f = 0; f < 4; f++) {
             double x = 0., y = 0., z = 0.;
             if (src[f][0]) { x = src[f][0][id]; y = src[f][1][id]; z = src[f][2][id]; }
-            if (a.rotate) rot_vec(a.Tg2c, x, y, z, v[f][0], v[f][1], v[f][2]);
+            if (kGeneral && a.rotate) rot_vec(a.Tg2c, x, y, z, v[f][0], v[f][1], v[f][2]);
             else { v[f][0] = x; v[f][1] = y; v[f][2] = z; }
         }
         const double px = v[0][0], py = v[0][1], pz = v[0][2];
@@ -153,7 +174,7 @@ __device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, do
         s[3] = 1.5 * M.l111 * my * mz + M.Q44 * py * pz + 0.5 * 1.5 * M.l111a * ay * az + 0.5 * 1.5 * M.l111b * by * bz - t0[3];
         s[4] = 1.5 * M.l111 * mx * mz + M.Q44 * px * pz + 0.5 * 1.5 * M.l111a * ax * az + 0.5 * 1.5 * M.l111b * bx * bz - t0[4];
         s[5] = 1.5 * M.l111 * my * mx + M.Q44 * py * px + 0.5 * 1.5 * M.l111a * ay * ax + 0.5 * 1.5 * M.l111b * by * bx - t0[5];
-        if (a.flexo) {
+        if (kGeneral && a.flexo) {
             // -F grad p (:82-89): the global-frame gradient of FE cells,
             // zero elsewhere, enters the crystal-frame tensor
             const FlexoMat& F = a.flexo[t - 1];
@@ -172,7 +193,7 @@ __device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, do
             s[4] = s[4] - F.F44 / 2. * (g[0][2] + g[2][0]);
             s[5] = s[5] - F.F44 / 2. * (g[1][0] + g[0][1]);
         }
-        if (a.rotate) rot_voigt(a.Tc2g, s, o);
+        if (kGeneral && a.rotate) rot_voigt(a.Tc2g, s, o);
         else for (int c = 0; c < 6; c++) o[c] = s[c];
     }
 }
@@ -193,99 +214,104 @@ __device__ __forceinline__ double crow(const double (*C)[6], int r, const double
 // where g?_f is the face's own gradient and the others belong to the cell
 // (its strain e, eigenstrain e0, the gradients on its faces across the
 // axis).  C e is linear in e, so it splits into a part of the cell, P =
-// C (e with the face gradients zero), computed once per cell and axis by
-// the prep launch, and C fp with fp holding the face gradients: the face
-// gradient of component c of the face's axis a enters strain entry (a, c).
-// The face stress is the mean of its two cells' evaluations,
-// (P_q + C_q fp + P_n + C_n fp) / 2.
+// C (e with the face gradients zero), and C fp with fp holding the face
+// gradients: the face gradient of component c of the face's axis a enters
+// strain entry (a, c).  The face stress is the mean of its two cells'
+// evaluations, (P_q + C_q fp + P_n + C_n fp) / 2: the prep launch writes
+// P^x, P^y, P^z of every cell, the force launch forms the face stresses.
 
 // Rows of C per face axis, and the strain entries the face gradient
 // components (x, y, z) enter.
 __constant__ int kFaceRows[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
 __constant__ int kFaceCols[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
 
+// P^ax of a cell (its strain e, eigenstrain e0, face gradients g; C its
+// stiffness): the strain of its faces along ax with their own gradients
+// zero, times the rows of C those faces carry.
+__device__ __forceinline__ void p_axis(const double (*C)[6], int ax, const double (&g)[6][3], const double (&e)[6],
+                                       const double (&e0)[6], double (&P)[3]) {
+    double b[6];
+    if (ax == 0) {
+        b[0] = -e0[0]; b[1] = e[1] - e0[1]; b[2] = e[2] - e0[2]; b[3] = (e[3] - e0[3]) * 2.;
+        b[4] = ((g[4][0] + g[5][0]) / 4. - e0[4]) * 2.; b[5] = ((g[2][0] + g[3][0]) / 4. - e0[5]) * 2.;
+    } else if (ax == 1) {
+        b[0] = e[0] - e0[0]; b[1] = -e0[1]; b[2] = e[2] - e0[2]; b[3] = ((g[4][1] + g[5][1]) / 4. - e0[3]) * 2.;
+        b[4] = (e[4] - e0[4]) * 2.; b[5] = ((g[0][1] + g[1][1]) / 4. - e0[5]) * 2.;
+    } else {
+        b[0] = e[0] - e0[0]; b[1] = e[1] - e0[1]; b[2] = -e0[2]; b[3] = ((g[2][2] + g[3][2]) / 4. - e0[3]) * 2.;
+        b[4] = ((g[0][2] + g[1][2]) / 4. - e0[4]) * 2.; b[5] = (e[5] - e0[5]) * 2.;
+    }
+    for (int o = 0; o < 3; o++) P[o] = crow(C, kFaceRows[ax][o], b);
+}
+
+__device__ __forceinline__ const double (*stiffness(const double* ctab, uint32_t code))[6] {
+    return reinterpret_cast<const double (*)[6]>(ctab + ((code & kMatMask) - 1u) * 36u);
+}
+
+__device__ __forceinline__ void load_ctab(double* ctab, const DevMaterial* mats, int nmat) {
+    for (int t = threadIdx.x; t < 36 * nmat; t += blockDim.x) ctab[t] = mats[t / 36].cij[(t % 36) / 6][t % 6];
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------- prep
-__global__ void __launch_bounds__(256) el_prep_kernel(ElPrepArgs p) {
+// The strain of the u store and the eigenstrain of the stage stores, both
+// written out, and the cell's P^x, P^y, P^z.  kGeneral: rotation and the
+// flexoelectric terms (the face gradients are written out for the latter).
+template <bool kGeneral>
+__global__ void __launch_bounds__(256) el_prep_kernel(ElStageArgs a) {
     extern __shared__ double ctab[];   // nmat stiffness matrices, 36 doubles each
-    for (int t = threadIdx.x; t < 36 * p.nmat; t += blockDim.x)
-        ctab[t] = p.e.mats[t / 36].cij[(t % 36) / 6][t % 6];
-    __syncthreads();
+    load_ctab(ctab, a.e.mats, a.f.nmat);
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= p.s.nx * p.s.ny * p.s.nz) return;
-    double g[6][3], e[6];
-    cell_strain(p.s, id, g, e);
-    double e0[6] = {0., 0., 0., 0., 0., 0.};
-    if (p.eig) {
+    if (id >= a.s.nx * a.s.ny * a.s.nz) return;
+    int64_t i, j, k;
+    cell_ijk(id, a.s.ny, a.s.nz, i, j, k);
+    double g[6][3], e[6], e0[6] = {0., 0., 0., 0., 0., 0.};
+    cell_grad(a.s, id, i, j, k, g);
+    if (kGeneral && a.s.gyz[0]) grad_out(a.s, i, j, k, g);
+    strain_of(g, e);
+    strain_out(a.s, id, e);
+    if (a.eig) {
         double sc[6];
-        eigenstrain(p.e, id, sc, e0);
-        if (p.e0_out) {
-            for (int c = 0; c < 6; c++) p.e.e0g[c][id] = e0[c];
-            if (p.e.e0c[0] != p.e.e0g[0])
-                for (int c = 0; c < 6; c++) p.e.e0c[c][id] = sc[c];
+        eigenstrain<kGeneral>(a.e, id, sc, e0);
+        if (a.e0_out) {
+            for (int c = 0; c < 6; c++) a.e.e0g[c][id] = e0[c];
+            if (a.e.e0c[0] != a.e.e0g[0])
+                for (int c = 0; c < 6; c++) a.e.e0c[c][id] = sc[c];
         }
     }
+    const uint32_t code = a.s.code[id];
     double P[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-    const uint32_t mat = p.s.code[id] & kMatMask;
-    if (mat) {
-        const double(*C)[6] = reinterpret_cast<const double(*)[6]>(ctab + (mat - 1) * 36);
-        // the strain of each axis's faces with their own gradients zero
-        const double b[3][6] = {
-            {-e0[0], e[1] - e0[1], e[2] - e0[2], (e[3] - e0[3]) * 2., ((g[4][0] + g[5][0]) / 4. - e0[4]) * 2.,
-             ((g[2][0] + g[3][0]) / 4. - e0[5]) * 2.},
-            {e[0] - e0[0], -e0[1], e[2] - e0[2], ((g[4][1] + g[5][1]) / 4. - e0[3]) * 2., (e[4] - e0[4]) * 2.,
-             ((g[0][1] + g[1][1]) / 4. - e0[5]) * 2.},
-            {e[0] - e0[0], e[1] - e0[1], -e0[2], ((g[2][2] + g[3][2]) / 4. - e0[3]) * 2.,
-             ((g[0][2] + g[1][2]) / 4. - e0[4]) * 2., (e[5] - e0[5]) * 2.}};
-        for (int ax = 0; ax < 3; ax++)
-            for (int o = 0; o < 3; o++) P[ax][o] = crow(C, kFaceRows[ax][o], b[ax]);
+    if (code & kMatMask) {
+        const double(*C)[6] = stiffness(ctab, code);
+        p_axis(C, 0, g, e, e0, P[0]);
+        p_axis(C, 1, g, e, e0, P[1]);
+        p_axis(C, 2, g, e, e0, P[2]);
     }
     for (int o = 0; o < 3; o++) {
-        p.px[o][id] = P[0][o];
-        p.py[o][id] = P[1][o];
-        p.pz[o][id] = P[2][o];
+        a.px[o][id] = P[0][o];
+        a.py[o][id] = P[1][o];
+        a.pz[o][id] = P[2][o];
     }
 }
 
-// The stiffness matrix of cell q from the workgroup's LDS copy of every
-// material's matrix (q is not vacuum: a face with vacuum beside it is free).
-__device__ __forceinline__ const double (*cij_of(const ElForceArgs& a, const double* ctab, int64_t q))[6] {
-    const uint32_t m = a.code[q] & kMatMask;
-    return reinterpret_cast<const double (*)[6]>(ctab + (m == 0u ? 0 : (m - 1) * 36));
-}
-
-// The mean stress of a face of axis ax between cells q and n, whose
-// gradients (x, y, z components) are fg.
-__device__ __forceinline__ void face_stress(const ElForceArgs& a, const double* ctab, int ax,
-                                            const double* const* P, int64_t q, int64_t n, const double fg[3],
-                                            double out[3]) {
-    const double(*Cq)[6] = cij_of(a, ctab, q);
-    const double(*Cn)[6] = cij_of(a, ctab, n);
-    for (int o = 0; o < 3; o++) {
-        const int r = kFaceRows[ax][o];
-        double tq = P[o][q], tn = P[o][n];
-        for (int c = 0; c < 3; c++) {
-            const int col = kFaceCols[ax][c];
-            tq = tq + Cq[r][col] * fg[c];
-            tn = tn + Cn[r][col] * fg[c];
-        }
-        out[o] = (tq + tn) / 2.;
-    }
-}
-
-__global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
+// ----------------------------------------------------------------- force
+// The stress of each face from the P's of its two cells, its gradient (from
+// the u store the prep launch read) and the two stiffnesses; dv; the v and
+// u RK updates, the next u store into f.us.
+__global__ void __launch_bounds__(256) el_force_kernel(ElStageArgs a) {
     extern __shared__ double ctab[];   // nmat stiffness matrices, 36 doubles each
-    for (int t = threadIdx.x; t < 36 * a.nmat; t += blockDim.x)
-        ctab[t] = a.mats[t / 36].cij[(t % 36) / 6][t % 6];
-    __syncthreads();
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const ElForceArgs& f = a.f;
+    load_ctab(ctab, f.mats, f.nmat);
+    const int64_t nx = f.nx, ny = f.ny, nz = f.nz;
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= nx * ny * nz) return;
     int64_t i, j, k;
     cell_ijk(id, ny, nz, i, j, k);
-    const uint32_t mat = a.code[id] & kMatMask;
-    const int stage = a.stage;
+    const uint32_t code = f.code[id];
+    const int stage = f.stage;
     double dv[3] = {0., 0., 0.};
-    if (mat) {
+    if (code & kMatMask) {
+        const double(*Cq)[6] = stiffness(ctab, code);
         const int64_t ib = i == 0 ? nx - 1 : i - 1, iff = i == nx - 1 ? 0 : i + 1;
         const int64_t jb = j == 0 ? ny - 1 : j - 1, jf = j == ny - 1 ? 0 : j + 1;
         const int64_t kb = k == 0 ? nz - 1 : k - 1;
@@ -293,75 +319,85 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElForceArgs a) {
         // (elastic_dynamic_solver.cpp:1247); set-up rejects the grids where
         // that differs from the periodic wrap and reads outside a column
         const int64_t kf = k == ny - 1 ? 0 : k + 1;
-        const bool free_xb = (i == 0 && !a.per[0]) || !solid(a.code, (ib * ny + j) * nz + k);
-        const bool free_xf = (i == nx - 1 && !a.per[0]) || !solid(a.code, (iff * ny + j) * nz + k);
-        const bool free_yb = (j == 0 && !a.per[1]) || !solid(a.code, (i * ny + jb) * nz + k);
-        const bool free_yf = (j == ny - 1 && !a.per[1]) || !solid(a.code, (i * ny + jf) * nz + k);
-        const bool free_zb = (k == 0 && !a.per[2]) || !solid(a.code, (i * ny + j) * nz + kb);
-        const bool free_zf = (k == nz - 1 && !a.per[2]) || !solid(a.code, (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1));
-        double s1x[3] = {0., 0., 0.}, s2x[3] = {0., 0., 0.}, s1y[3] = {0., 0., 0.};
-        double s2y[3] = {0., 0., 0.}, s1z[3] = {0., 0., 0.}, s2z[3] = {0., 0., 0.};
-        auto yz = [&](int64_t fi, double fg[3]) {
-            const int64_t f = (fi * ny + j) * nz + k;
-            for (int c = 0; c < 3; c++) fg[c] = a.gyz[c][f];
-        };
-        auto xz = [&](int64_t fj, double fg[3]) {
-            const int64_t f = (i * (ny + 1) + fj) * nz + k;
-            for (int c = 0; c < 3; c++) fg[c] = a.gxz[c][f];
-        };
-        auto xy = [&](int64_t fk, double fg[3]) {
-            const int64_t f = (i * ny + j) * (nz + 1) + fk;
-            for (int c = 0; c < 3; c++) fg[c] = a.gxy[c][f];
-        };
-        double fg[3];
-        if (!free_xb) { yz(i, fg); face_stress(a, ctab, 0, a.px, id, (ib * ny + j) * nz + k, fg, s1x); }
-        if (!free_xf) { yz(i + 1, fg); face_stress(a, ctab, 0, a.px, id, (iff * ny + j) * nz + k, fg, s2x); }
-        if (!free_yb) { xz(j, fg); face_stress(a, ctab, 1, a.py, id, (i * ny + jb) * nz + k, fg, s1y); }
-        if (!free_yf) { xz(j + 1, fg); face_stress(a, ctab, 1, a.py, id, (i * ny + jf) * nz + k, fg, s2y); }
-        if (!free_zb) { xy(k, fg); face_stress(a, ctab, 2, a.pz, id, (i * ny + j) * nz + kb, fg, s1z); }
-        if (!free_zf) {
-            xy(k + 1, fg);
-            face_stress(a, ctab, 2, a.pz, id, (i * ny + j) * nz + kf, fg, s2z);
-        } else if (k == a.trac_k) {
-            // Gaussian traction on a stress-free top face (input_strain_type 3)
-            s2z[a.trac_row] = a.trac_amp;
+        const int64_t kt = k == nz - 1 ? 0 : k + 1;   // the true forward z neighbour
+        const int64_t nb[6] = {(ib * ny + j) * nz + k, (iff * ny + j) * nz + k, (i * ny + jb) * nz + k,
+                               (i * ny + jf) * nz + k, (i * ny + j) * nz + kb, (i * ny + j) * nz + kf};
+        const int64_t nbt = (i * ny + j) * nz + kt;
+        const bool fr[6] = {(i == 0 && !f.per[0]) || !solid(f.code, nb[0]),
+                            (i == nx - 1 && !f.per[0]) || !solid(f.code, nb[1]),
+                            (j == 0 && !f.per[1]) || !solid(f.code, nb[2]),
+                            (j == ny - 1 && !f.per[1]) || !solid(f.code, nb[3]),
+                            (k == 0 && !f.per[2]) || !solid(f.code, nb[4]),
+                            (k == nz - 1 && !f.per[2]) || !solid(f.code, nbt)};
+        const double ih[3] = {f.idx, f.idy, f.idz};
+        const double* const* Pa[3] = {a.px, a.py, a.pz};
+        double d[3][3];
+        for (int ax = 0; ax < 3; ax++) {
+            double s1[3] = {0., 0., 0.}, s2[3] = {0., 0., 0.};
+            for (int side = 0; side < 2; side++) {
+                const int fc = 2 * ax + side;
+                double* sf = side ? s2 : s1;
+                if (fr[fc]) {
+                    // Gaussian traction on a stress-free top face (input_strain_type 3)
+                    if (fc == 5 && k == f.trac_k) sf[f.trac_row] = f.trac_amp;
+                    continue;
+                }
+                // the face gradient, across to the face's true neighbour (the
+                // value the cell on either side computes for it)
+                const int64_t n = fc == 5 ? nbt : nb[fc];
+                double fg[3];
+                for (int c = 0; c < 3; c++)
+                    fg[c] = side ? (a.s.us[c][n] - a.s.us[c][id]) * ih[ax] : (a.s.us[c][id] - a.s.us[c][n]) * ih[ax];
+                const double(*Cn)[6] = stiffness(ctab, f.code[nb[fc]]);
+                for (int o = 0; o < 3; o++) {
+                    const int r = kFaceRows[ax][o];
+                    double tq = Pa[ax][o][id], tn = Pa[ax][o][nb[fc]];
+                    for (int c = 0; c < 3; c++) {
+                        const int col = kFaceCols[ax][c];
+                        tq = tq + Cq[r][col] * fg[c];
+                        tn = tn + Cn[r][col] * fg[c];
+                    }
+                    sf[o] = (tq + tn) / 2.;
+                }
+            }
+            for (int c = 0; c < 3; c++) d[ax][c] = s2[c] - s1[c];
         }
-        const DevMaterial& M = a.mats[mat - 1];
+        const DevMaterial& M = f.mats[(code & kMatMask) - 1];
         for (int c = 0; c < 3; c++) {
-            const double f = (s2x[c] - s1x[c]) * a.idx + (s2y[c] - s1y[c]) * a.idy + (s2z[c] - s1z[c]) * a.idz;
-            const double fs = a.fstore[c][id];
+            const double fo = d[0][c] * f.idx + d[1][c] * f.idy + d[2][c] * f.idz;
+            const double fs = f.fstore[c][id];
             if (stage == 2 || stage == 3)
-                dv[c] = f * M.el_rho + M.el_mass * a.v[c][id] + M.el_stiff * (f - fs) * 2.;
+                dv[c] = fo * M.el_rho + M.el_mass * f.v[c][id] + M.el_stiff * (fo - fs) * 2.;
             else
-                dv[c] = f * M.el_rho + M.el_mass * a.v[c][id] + M.el_stiff * (f - fs);
-            if (stage == 1) a.fstore[c][id] = f;
+                dv[c] = fo * M.el_rho + M.el_mass * f.v[c][id] + M.el_stiff * (fo - fs);
+            if (stage == 1) f.fstore[c][id] = fo;
         }
     }
     for (int c = 0; c < 3; c++) {
-        const double du = a.vs[c][id] * a.dt;
-        const double v = a.v[c][id], u = a.u[c][id];
+        const double du = f.vs[c][id] * f.dt;
+        const double v = f.v[c][id], u = f.u[c][id];
         if (stage == 1) {
-            a.vs[c][id] = v + dv[c] * 0.5;
-            a.accv[c][id] = v + dv[c] * kSixth;
-            a.us[c][id] = u + du * 0.5;
-            a.accu[c][id] = u + du * kSixth;
+            f.vs[c][id] = v + dv[c] * 0.5;
+            f.accv[c][id] = v + dv[c] * kSixth;
+            f.us[c][id] = u + du * 0.5;
+            f.accu[c][id] = u + du * kSixth;
         } else if (stage == 2) {
-            a.vs[c][id] = v + dv[c] * 0.5;
-            a.accv[c][id] = a.accv[c][id] + dv[c] * kThird;
-            a.us[c][id] = u + du * 0.5;
-            a.accu[c][id] = a.accu[c][id] + du * kThird;
+            f.vs[c][id] = v + dv[c] * 0.5;
+            f.accv[c][id] = f.accv[c][id] + dv[c] * kThird;
+            f.us[c][id] = u + du * 0.5;
+            f.accu[c][id] = f.accu[c][id] + du * kThird;
         } else if (stage == 3) {
-            a.vs[c][id] = v + dv[c];
-            a.accv[c][id] = a.accv[c][id] + dv[c] * kThird;
-            a.us[c][id] = u + du;
-            a.accu[c][id] = a.accu[c][id] + du * kThird;
+            f.vs[c][id] = v + dv[c];
+            f.accv[c][id] = f.accv[c][id] + dv[c] * kThird;
+            f.us[c][id] = u + du;
+            f.accu[c][id] = f.accu[c][id] + du * kThird;
         } else {
-            const double vn = a.accv[c][id] + dv[c] * kSixth;
-            const double un = a.accu[c][id] + du * kSixth;
-            a.v[c][id] = vn;
-            a.vs[c][id] = vn;
-            a.u[c][id] = un;
-            a.us[c][id] = un;
+            const double vn = f.accv[c][id] + dv[c] * kSixth;
+            const double un = f.accu[c][id] + du * kSixth;
+            f.v[c][id] = vn;
+            f.vs[c][id] = vn;
+            f.u[c][id] = un;
+            f.us[c][id] = un;
         }
     }
 }
@@ -438,25 +474,24 @@ hipError_t launch_el_strain(const ElStrainArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_el_prep(const ElPrepArgs& a, hipStream_t s) {
-    const size_t lds = sizeof(double) * 36 * static_cast<size_t>(a.nmat > 0 ? a.nmat : 1);
+hipError_t launch_el_stage(const ElStageArgs& a, hipStream_t s) {
+    const bool general = a.s.rotate || a.e.rotate || a.e.flexo;
+    const void* prep = general ? reinterpret_cast<const void*>(el_prep_kernel<true>)
+                               : reinterpret_cast<const void*>(el_prep_kernel<false>);
+    const size_t lds = sizeof(double) * 36 * static_cast<size_t>(a.f.nmat > 0 ? a.f.nmat : 1);
     if (lds > 65536) {   // up to 255 materials: 72 KiB of the CU's 160 KiB
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(el_prep_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        hipError_t e = hipFuncSetAttribute(prep, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(el_force_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(el_prep_kernel, dim3(blocks_for(a.s.nx * a.s.ny * a.s.nz)), dim3(256), lds, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_el_force(const ElForceArgs& a, hipStream_t s) {
-    const size_t lds = sizeof(double) * 36 * static_cast<size_t>(a.nmat > 0 ? a.nmat : 1);
-    if (lds > 65536) {   // up to 255 materials: 72 KiB of the CU's 160 KiB
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(el_force_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(el_force_kernel, dim3(blocks_for(a.nx * a.ny * a.nz)), dim3(256), lds, s, a);
+    const dim3 grid(blocks_for(a.f.nx * a.f.ny * a.f.nz));
+    if (general) hipLaunchKernelGGL(el_prep_kernel<true>, grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL(el_prep_kernel<false>, grid, dim3(256), lds, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(el_force_kernel, grid, dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

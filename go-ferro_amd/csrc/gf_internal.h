@@ -269,23 +269,9 @@ struct ElEigenArgs {
     double Tc2g[3][3];
 };
 
-// The first launch of an elastic stage: the face gradients and strain of
-// the u store (s), the eigenstrain of the stage stores (e, when eig is set;
-// zero otherwise) and each cell's part of the stress its faces carry,
-// P^x, P^y, P^z: the rows of C (the cell's stiffness) applied to the
-// strain terms that do not depend on which of its two faces along the axis
-// is evaluated (see el_prep_kernel).
-struct ElPrepArgs {
-    ElStrainArgs s;
-    ElEigenArgs e;
-    int32_t eig;
-    int32_t e0_out;         // write the eigenstrain out (a ferroelectric stage reads it)
-    int32_t nmat;
-    double* px[3];
-    double* py[3];
-    double* pz[3];
-};
-
+// The v and u RK updates of an elastic stage.  us is the u store the stage
+// writes: a different array from the store its strain is read from, since
+// a cell's stress reads the store around it up to two cells away.
 struct ElForceArgs {
     const uint32_t* code;
     const DevMaterial* mats;
@@ -296,12 +282,6 @@ struct ElForceArgs {
     double* accu[3];
     double* accv[3];
     double* fstore[3];
-    const double* gyz[3];
-    const double* gxz[3];
-    const double* gxy[3];
-    const double* px[3];    // the cells' face-stress parts (ElPrepArgs)
-    const double* py[3];
-    const double* pz[3];
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
     double idx, idy, idz;   // 1/dx, 1/dy, 1/dz
@@ -311,6 +291,25 @@ struct ElForceArgs {
     int32_t trac_row;       // 0, 1, 2 for a traction along x, y, z (stress xz, yz, zz)
     double trac_amp;
     int32_t nmat;           // material records (their stiffness matrices are staged in LDS)
+};
+
+// One elastic stage (two launches): the face gradients and strain of the
+// u store s.us (written out: the strain for the magnets and ferroelectrics,
+// the face gradients only when s.gyz is set, for the flexoelectric force),
+// the eigenstrain of the stage stores (e, when eig is set; zero otherwise;
+// written out when e0_out is set) and each cell's part of the stress its
+// faces carry, P^x, P^y, P^z: the rows of C (the cell's stiffness) applied
+// to the strain terms that do not depend on which of its two faces along
+// the axis is evaluated; then the face stresses and the updates (f).
+struct ElStageArgs {
+    ElStrainArgs s;
+    ElEigenArgs e;
+    ElForceArgs f;
+    double* px[3];
+    double* py[3];
+    double* pz[3];
+    int32_t eig;
+    int32_t e0_out;
 };
 
 // Displacement history of the absorbing boundary: slot = [plane][comp][nx*ny]
@@ -610,7 +609,8 @@ struct gf_ctx {
     double* ec[6] = {};
     double* e0g[6] = {};
     double* e0c[6] = {};
-    double* epx[3] = {};            // face-stress parts of the cells (ElPrepArgs)
+    double* us_alt[3] = {};         // the u store an elastic stage writes (swapped with us after it)
+    double* epx[3] = {};            // face-stress parts of the cells (ElStageArgs)
     double* epy[3] = {};
     double* epz[3] = {};
     double* hist = nullptr;
@@ -754,8 +754,7 @@ hipError_t launch_fe_stage(const FeStageArgs& a, int64_t n, hipStream_t s);
 
 // elastodynamics (gf_elastic.hip)
 hipError_t launch_el_strain(const ElStrainArgs& a, hipStream_t s);
-hipError_t launch_el_prep(const ElPrepArgs& a, hipStream_t s);
-hipError_t launch_el_force(const ElForceArgs& a, hipStream_t s);
+hipError_t launch_el_stage(const ElStageArgs& a, hipStream_t s);
 hipError_t launch_el_history(const ElHistArgs& a, hipStream_t s);
 hipError_t launch_el_boundary(const ElBoundaryArgs& a, hipStream_t s);
 
