@@ -595,7 +595,6 @@ gf_status em_setup(gf_ctx* c, const gf_desc* d) {
     }
     if (!st) st = alloc3(c, c->Ecell, "hipMalloc E cell");
     if (!st) st = alloc3(c, c->Hcell, "hipMalloc H cell");
-    if (!st && (c->flags & GF_FLAG_JF_INPUT)) st = alloc3(c, c->Jf, "hipMalloc Jf");
     for (int q = 0; q < 9 && !st; q++) {
         st = alloc_n(c, &c->Jp[q], n, "hipMalloc Jp");
         if (!st) st = alloc_n(c, &c->Jps[q], n, "hipMalloc Jp");
@@ -1024,7 +1023,7 @@ gf_status gf_destroy(gf_ctx* c) {
     if (c->hist) hipFree(c->hist);
     free3(c->Eb); free3(c->Ea); free3(c->Ebb); free3(c->Eacc); free3(c->Et1);
     free3(c->Hb); free3(c->Ha); free3(c->Hbb); free3(c->Hacc);
-    free3(c->Ecell); free3(c->Hcell); free3(c->Jf);
+    free3(c->Ecell); free3(c->Hcell);
     free3(c->em_dm); free3(c->em_dm1); free3(c->em_dm2);
     free3(c->jis);
     free3(c->flexo_force);
@@ -1906,38 +1905,34 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
     gf_status st = GF_OK;
     hipEvent_t ev = nullptr;
 
-    // free-current box at the stage time (update_Jf_input*, :3885-4315)
+    // free-current box at the stage time (update_Jf_input*, :3885-4315): its
+    // value, which the Ampere launch evaluates per cell
+    int32_t jf_comp = -1;
+    double jf_value = 0.;
+    const double* jf_spatial = nullptr;
     if (c->flags & GF_FLAG_JF_INPUT) {
         const double t = t_stage, f = c->jf_freq, amp = c->jf_amp, sg = c->jf_sigma;
-        EmJfArgs j{};
-        j.ny = ny; j.nz = nz;
-        for (int a = 0; a < 3; a++) { j.lo[a] = c->jf_lo[a]; j.hi[a] = c->jf_hi[a]; }
         bool write = true;
         if (c->jf_type < 3) {
             const double tv = std::sin(2. * kPi * f * t);
-            j.value = amp * tv;
-            j.spatial = c->jf_type == 0 ? nullptr : c->jf_spatial;
+            jf_value = amp * tv;
+            jf_spatial = c->jf_type == 0 ? nullptr : c->jf_spatial;
         } else if (c->jf_type == 3) {
             const double tv = t <= static_cast<double>(c->jf_cycles) / f ? std::sin(2. * kPi * f * t) : 0.;
-            j.value = amp * tv;
+            jf_value = amp * tv;
         } else if (c->jf_type == 4) {
             const double tv = t - 5. * f;
-            j.value = amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(f, 2.)));
+            jf_value = amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(f, 2.)));
         } else if (c->jf_type == 5) {
             const double tv = t - 5. * f;
-            j.value = amp * std::sqrt(std::exp(1.)) * tv / f * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(f, 2.)));
+            jf_value = amp * std::sqrt(std::exp(1.)) * tv / f * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(f, 2.)));
         } else if (c->jf_type == 6) {
             const double tv = t - 5. * sg;
-            j.value = amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(sg, 2.))) * std::sin(2. * kPi * f * t);
+            jf_value = amp * std::exp(-1. * std::pow(tv, 2.) / (2. * std::pow(sg, 2.))) * std::sin(2. * kPi * f * t);
         } else {
             write = false;
         }
-        if (write && c->jf_comp >= 0) {
-            j.jf = c->Jf[c->jf_comp];
-            timer_begin(c, GF_K_OTHER, &ev);
-            st = hip_check(c, launch_em_jf(j, c->stream), "Jf input");
-            timer_end(c, GF_K_OTHER, ev);
-        }
+        if (write) jf_comp = c->jf_comp;
     }
     // stage 4: the planes the Mur walls read, before the combine
     // (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
@@ -1955,7 +1950,6 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
         a.es[i] = es[i]; a.hs[i] = hs[i];
         a.eb[i] = c->Eb[i]; a.eo[i] = eo[i]; a.eacc[i] = c->Eacc[i];
         a.hb[i] = c->Hb[i]; a.ho[i] = ho[i]; a.hacc[i] = c->Hacc[i];
-        a.jf[i] = (c->flags & GF_FLAG_JF_INPUT) ? c->Jf[i] : nullptr;
         a.q[i] = ((c->flags & GF_FLAG_EM_FROM_P) && c->any_fe) ? c->fe_q_used[i] : nullptr;
         const bool from_m = c->flags & GF_FLAG_EM_FROM_M;
         a.dm[i] = from_m ? c->em_dm[i] : nullptr;
@@ -1965,19 +1959,16 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
     a.jishe[0] = c->spin_on ? c->jis[0] : nullptr;
     a.jishe[1] = c->spin_on ? c->jis[1] : nullptr;
     for (int q = 0; q < 9; q++) a.jps[q] = c->jp_on ? c->Jps[q] : nullptr;
+    a.jf_comp = jf_comp; a.jf_value = jf_value; a.jf_spatial = jf_spatial;
+    for (int i = 0; i < 3; i++) { a.jf_lo[i] = c->jf_lo[i]; a.jf_hi[i] = c->jf_hi[i]; }
     a.nx = nx; a.ny = ny; a.nz = nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
     for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
     a.stage = stage;
     // Ampere and Faraday with the RK updates (both read the stage-s stores)
     if (!st) {
         timer_begin(c, GF_K_EM_E, &ev);
-        st = hip_check(c, launch_em_e(a, c->stream), "EM E");
+        st = hip_check(c, launch_em_eh(a, c->stream), "EM E and H");
         timer_end(c, GF_K_EM_E, ev);
-    }
-    if (!st) {
-        timer_begin(c, GF_K_EM_H, &ev);
-        st = hip_check(c, launch_em_h(a, c->stream), "EM H");
-        timer_end(c, GF_K_EM_H, ev);
     }
     // Jp from the cell E of the stage store (get_dJp_RK*, update_Jp*)
     if (!st && c->jp_on) {
@@ -2294,7 +2285,6 @@ std::vector<StateArray> state_arrays(gf_ctx* c, const StateSel& sel, uint32_t bi
             v.push_back({34u << 8 | static_cast<uint32_t>(a), h_store(c, sel.h)[a], c->nH[a]});
         add(25, c->Ecell, 3, n);
         add(26, c->Hcell, 3, n);
-        add(27, c->Jf, 3, n);
         add(28, c->Jp, 9, n);
         add(35, c->Jps, 9, n);
     }

@@ -2,21 +2,22 @@
 //
 // One stage of EMdynamic_system (EMdynamic_solver.cpp,
 // EMdynamic_AbsorbingBoundary.cpp) as a handful of launches:
-//   jf        free-current box at the stage time (update_Jf_input*, :3885-4315)
 //   history   stage 4 only: E on the boundary planes before the combine
 //             (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
-//   e         Ampere per edge with the four-cell means of permittivity,
+//   eh        Ampere per edge with the four-cell means of permittivity,
 //             conductivity, Jf, Jp and dp (get_dE_RK*, :337-2237), fused
-//             with the RK update (update_DE_RK*, :3588-3777)
-//   h         Faraday per face with dM of the two cells (get_dH_RK*,
-//             :2239-3475), fused with update_DH_RK* (:3477-3586)
+//             with the RK update (update_DE_RK*, :3588-3777); and in the
+//             same launch Faraday per face with dM of the two cells
+//             (get_dH_RK*, :2239-3475), fused with update_DH_RK*
+//             (:3477-3586).  The free current of the box is evaluated where
+//             an edge reads it (update_Jf_input*, :3885-4315).
 //   jp        Drude/Lorentz currents (get_dJp_RK*, update_Jp*, :4318-4524)
 //   plane     hard plane source (update_planeEM*, :3829-3883)
 //   wall x3   first-order Mur or PEC walls, x then y then z (:58-874)
 //   pec       PEC edges (update_DE_PEC, :4526-4556)
 //   cells     cell-centred E and H of the new stores (:3780-3826)
-// E and H keep two stage stores each: the e and h launches read the
-// stage-s stores of both fields and write stage s+1 into the other buffer.
+// E and H keep two stage stores each: the eh launch reads the stage-s
+// stores of both fields and writes stage s+1 into the other buffer.
 // RK4 keeps a state accumulator, acc = y + k1/6, acc += k2/3, acc += k3/3,
 // y = acc + k4/6, the reference's left-to-right sum.
 #include <algorithm>
@@ -43,9 +44,21 @@ __device__ __forceinline__ void rk_state(int stage, int64_t i, double k, double*
     }
 }
 
+// The free current of cell (i, j, k), component comp, at the stage time:
+// the value update_Jf_input* writes in its box (EMdynamic_solver.cpp:
+// 3885-4315), zero elsewhere.
+__device__ __forceinline__ double jf_at(const EmArgs& a, int comp, int64_t i, int64_t j, int64_t k) {
+    if (comp != a.jf_comp || i < a.jf_lo[0] || i > a.jf_hi[0] || j < a.jf_lo[1] || j > a.jf_hi[1] || k < a.jf_lo[2] ||
+        k > a.jf_hi[2])
+        return 0.;
+    return a.jf_spatial ? a.jf_value * a.jf_spatial[k - a.jf_lo[2]] : a.jf_value;
+}
+
 // Four-cell means around an edge and the Ampere increment
 // (EMdynamic_solver.cpp:392-499), with the spin-pumping current on x and y.
-__device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, const int64_t cell[4], double curl,
+// The cells are (ci, cj, ck)[q], index cell[q].
+__device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, const int64_t cell[4], const int64_t ci[4],
+                                                 const int64_t cj[4], const int64_t ck[4], double curl,
                                                  double e_store) {
     double Jf = 0., Jp = 0., Ji = 0., dP = 0., er = 0., cond = 0., pc = 0.;
     for (int q = 0; q < 4; q++) {
@@ -60,7 +73,7 @@ __device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, cons
             er = er + M.em_er;
             cond = cond + M.em_cond;
         }
-        if (a.jf[0]) Jf = Jf + a.jf[comp][id];
+        if (a.jf_comp >= 0) Jf = Jf + jf_at(a, comp, ci[q], cj[q], ck[q]);
         if (a.jps[0]) Jp = Jp + a.jps[comp][id] + a.jps[3 + comp][id] + a.jps[6 + comp][id];
         if (a.jishe[0] && comp < 2) Ji = Ji + a.jishe[comp][id];
     }
@@ -74,11 +87,10 @@ __device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, cons
     return a.dt / kEps0 / er * (curl - Jf - Jp - Ji - cond * e_store - dP / a.dt);
 }
 
-__global__ void __launch_bounds__(256) em_e_kernel(EmArgs a) {
+// Ampere on edge e of the concatenated Ex|Ey|Ez grids, with its RK update.
+__device__ __forceinline__ void em_edge(const EmArgs& a, int64_t e) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1), nEz = (nx + 1) * (ny + 1) * nz;
-    int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= nEx + nEy + nEz) return;
+    const int64_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1);
     auto C = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
     auto HX = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
     auto HY = [&](int64_t i, int64_t j, int64_t k) { return (i * (ny + 1) + j) * nz + k; };
@@ -94,7 +106,8 @@ __global__ void __launch_bounds__(256) em_e_kernel(EmArgs a) {
             const double dHzdy = (a.hs[2][HZ(i, y3, z2)] - a.hs[2][HZ(i, y1, z2)]) / a.dy;
             const double dHydz = (a.hs[1][HY(i, y3, z2)] - a.hs[1][HY(i, y3, z1)]) / a.dz;
             const int64_t cell[4] = {C(i, y1, z1), C(i, y1, z2), C(i, y3, z1), C(i, y3, z2)};
-            k_ = edge_increment(a, 0, cell, dHzdy - dHydz, a.es[0][e]);
+            const int64_t ci[4] = {i, i, i, i}, cj[4] = {y1, y1, y3, y3}, ck[4] = {z1, z2, z1, z2};
+            k_ = edge_increment(a, 0, cell, ci, cj, ck, dHzdy - dHydz, a.es[0][e]);
         }
     } else if (e < nEx + nEy) {
         comp = 1;
@@ -106,7 +119,8 @@ __global__ void __launch_bounds__(256) em_e_kernel(EmArgs a) {
             const double dHxdz = (a.hs[0][HX(x3, j, z2)] - a.hs[0][HX(x3, j, z1)]) / a.dz;
             const double dHzdx = (a.hs[2][HZ(x3, j, z2)] - a.hs[2][HZ(x1, j, z2)]) / a.dx;
             const int64_t cell[4] = {C(x1, j, z1), C(x1, j, z2), C(x3, j, z1), C(x3, j, z2)};
-            k_ = edge_increment(a, 1, cell, dHxdz - dHzdx, a.es[1][e]);
+            const int64_t ci[4] = {x1, x1, x3, x3}, cj[4] = {j, j, j, j}, ck[4] = {z1, z2, z1, z2};
+            k_ = edge_increment(a, 1, cell, ci, cj, ck, dHxdz - dHzdx, a.es[1][e]);
         }
     } else {
         comp = 2;
@@ -118,7 +132,8 @@ __global__ void __launch_bounds__(256) em_e_kernel(EmArgs a) {
             const double dHydx = (a.hs[1][HY(x3, y2, k)] - a.hs[1][HY(x1, y2, k)]) / a.dx;
             const double dHxdy = (a.hs[0][HX(x3, y2, k)] - a.hs[0][HX(x3, y1, k)]) / a.dy;
             const int64_t cell[4] = {C(x1, y1, k), C(x1, y2, k), C(x3, y1, k), C(x3, y2, k)};
-            k_ = edge_increment(a, 2, cell, dHydx - dHxdy, a.es[2][e]);
+            const int64_t ci[4] = {x1, x1, x3, x3}, cj[4] = {y1, y2, y1, y2}, ck[4] = {k, k, k, k};
+            k_ = edge_increment(a, 2, cell, ci, cj, ck, dHydx - dHxdy, a.es[2][e]);
         }
     }
     rk_state(a.stage, e, k_, a.eb[comp], a.eacc[comp], a.eo[comp]);
@@ -140,11 +155,10 @@ __device__ __forceinline__ double cell_dm(const EmArgs& a, int comp, int64_t id,
     return d;
 }
 
-__global__ void __launch_bounds__(256) em_h_kernel(EmArgs a) {
+// Faraday on face f of the concatenated Hx|Hy|Hz grids, with its RK update.
+__device__ __forceinline__ void em_face(const EmArgs& a, int64_t f) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t nHx = (nx + 1) * ny * nz, nHy = nx * (ny + 1) * nz, nHz = nx * ny * (nz + 1);
-    int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (f >= nHx + nHy + nHz) return;
+    const int64_t nHx = (nx + 1) * ny * nz, nHy = nx * (ny + 1) * nz;
     auto C = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
     auto EX = [&](int64_t i, int64_t j, int64_t k) { return (i * (ny + 1) + j) * (nz + 1) + k; };
     auto EY = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * (nz + 1) + k; };
@@ -186,6 +200,18 @@ __global__ void __launch_bounds__(256) em_h_kernel(EmArgs a) {
         k_ = pre * (dEydx - dExdy) - (d1 + d2) / cnt;
     }
     rk_state(a.stage, f, k_, a.hb[comp], a.hacc[comp], a.ho[comp]);
+}
+
+// Ampere and Faraday of a stage in one launch: both read only the stage
+// stores and write other arrays, so the edges (first nE threads) and the
+// faces (the rest) are independent.
+__global__ void __launch_bounds__(256) em_eh_kernel(EmArgs a) {
+    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t nE = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz;
+    const int64_t nH = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < nE) em_edge(a, t);
+    else if (t < nE + nH) em_face(a, t - nE);
 }
 
 __global__ void __launch_bounds__(256) em_jp_kernel(EmJpArgs a) {
@@ -355,28 +381,14 @@ __global__ void __launch_bounds__(256) em_pec_kernel(EmMiscArgs a) {
     }
 }
 
-__global__ void __launch_bounds__(256) em_jf_kernel(EmJfArgs a) {
-    const int64_t sx = a.hi[0] - a.lo[0] + 1, sy = a.hi[1] - a.lo[1] + 1, sz = a.hi[2] - a.lo[2] + 1;
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= sx * sy * sz) return;
-    const int64_t kk = t % sz, jj = (t / sz) % sy, ii = t / (sy * sz);
-    const int64_t id = ((a.lo[0] + ii) * a.ny + (a.lo[1] + jj)) * a.nz + (a.lo[2] + kk);
-    a.jf[id] = a.spatial ? a.value * a.spatial[kk] : a.value;
-}
-
 unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 }  // namespace
 
-hipError_t launch_em_e(const EmArgs& a, hipStream_t s) {
-    const int64_t n = a.nx * (a.ny + 1) * (a.nz + 1) + (a.nx + 1) * a.ny * (a.nz + 1) + (a.nx + 1) * (a.ny + 1) * a.nz;
-    hipLaunchKernelGGL(em_e_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_em_h(const EmArgs& a, hipStream_t s) {
-    const int64_t n = (a.nx + 1) * a.ny * a.nz + a.nx * (a.ny + 1) * a.nz + a.nx * a.ny * (a.nz + 1);
-    hipLaunchKernelGGL(em_h_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
+hipError_t launch_em_eh(const EmArgs& a, hipStream_t s) {
+    const int64_t nE = a.nx * (a.ny + 1) * (a.nz + 1) + (a.nx + 1) * a.ny * (a.nz + 1) + (a.nx + 1) * (a.ny + 1) * a.nz;
+    const int64_t nH = (a.nx + 1) * a.ny * a.nz + a.nx * (a.ny + 1) * a.nz + a.nx * a.ny * (a.nz + 1);
+    hipLaunchKernelGGL(em_eh_kernel, dim3(blocks_for(nE + nH)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -464,10 +476,5 @@ hipError_t launch_em_pec(const EmMiscArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_em_jf(const EmJfArgs& a, hipStream_t s) {
-    const int64_t n = static_cast<int64_t>(a.hi[0] - a.lo[0] + 1) * (a.hi[1] - a.lo[1] + 1) * (a.hi[2] - a.lo[2] + 1);
-    hipLaunchKernelGGL(em_jf_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
 
 }  // namespace gf

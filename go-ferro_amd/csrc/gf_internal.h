@@ -110,7 +110,10 @@ struct EmArgs {
     double* hb[3];
     double* ho[3];
     double* hacc[3];
-    const double* jf[3];        // free current, null when off
+    const double* jf_spatial;   // free current in its box (update_Jf_input*): the per-plane factor
+    double jf_value;            //   (types 1, 2) or null, and the value at the stage time
+    int32_t jf_comp;            //   of the driven component, -1 when there is none
+    int32_t jf_lo[3], jf_hi[3];
     const double* jps[9];       // Jp stores (pole-major), null when identically zero
     const double* q[3];         // ferroelectric q store of the stage (dp = q dt), null when uncoupled
     const double* dm[3];        // magnetic increments of the stage, null when absent
@@ -196,14 +199,6 @@ struct EmMiscArgs {
     int32_t per[3];
     int32_t plane_z;
     int32_t kind;
-};
-
-struct EmJfArgs {
-    double* jf;                 // the driven component
-    const double* spatial;      // per-plane factor (types 1, 2), else null
-    double value;               // uniform value, or amp * temporal factor
-    int64_t ny, nz;
-    int32_t lo[3], hi[3];
 };
 
 // Elastodynamics.  Face gradients live on the staggered face grids
@@ -635,7 +630,6 @@ struct gf_ctx {
     double* Et1[3] = {};
     double* Ecell[3] = {};
     double* Hcell[3] = {};
-    double* Jf[3] = {};
     double* Jp[9] = {};
     double* Jps[9] = {};
     double* Jacc[9] = {};
@@ -759,8 +753,7 @@ hipError_t launch_el_history(const ElHistArgs& a, hipStream_t s);
 hipError_t launch_el_boundary(const ElBoundaryArgs& a, hipStream_t s);
 
 // electrodynamics (gf_em.hip)
-hipError_t launch_em_e(const EmArgs& a, hipStream_t s);
-hipError_t launch_em_h(const EmArgs& a, hipStream_t s);
+hipError_t launch_em_eh(const EmArgs& a, hipStream_t s);
 hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s);
 hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s);
 hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s);
@@ -778,7 +771,6 @@ hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s);
 hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s);
 hipError_t launch_em_plane(const EmMiscArgs& a, hipStream_t s);
 hipError_t launch_em_pec(const EmMiscArgs& a, hipStream_t s);
-hipError_t launch_em_jf(const EmJfArgs& a, hipStream_t s);
 
 // demag (gf_fft.hip)
 gf_status demag_setup(gf_ctx* c);

@@ -432,8 +432,8 @@ enum gf_kernel_class {
     GF_K_FE_STAGE = 8,
     GF_K_ELASTIC_STAGE = 9,   /* stress divergence + v, u updates */
     GF_K_ELASTIC_STRAIN = 10, /* face gradients, strain, eigenstrain */
-    GF_K_EM_E = 11,           /* Ampere update of the E edges */
-    GF_K_EM_H = 12,           /* Faraday update of the H faces */
+    GF_K_EM_E = 11,           /* Ampere update of the E edges and Faraday update of the H faces (one launch) */
+    GF_K_EM_H = 12,           /* no longer timed apart: the Faraday update runs in the GF_K_EM_E launch */
     GF_K_COUNT = 13
 };
 gf_status gf_set_kernel_timing(gf_ctx* ctx, int enable);
