@@ -1717,9 +1717,15 @@ gf_status gf_fe_stage(gf_ctx* c, int stage) {
         st = hip_check(c, launch_fe_flexo(f, c->stream), "flexo force");
         timer_end(c, GF_K_OTHER, ev);
     }
-    timer_begin(c, GF_K_FE_STAGE, &ev);
-    if (!st) st = hip_check(c, launch_fe_stage(a, c->n, c->stream), "fe stage");
-    timer_end(c, GF_K_FE_STAGE, ev);
+    if (!st && c->fe_defer && c->em_on) {
+        // runs in the EM stage's launch, which comes next in this stage
+        c->fe_args = a;
+        c->fe_pending = true;
+    } else {
+        timer_begin(c, GF_K_FE_STAGE, &ev);
+        if (!st) st = hip_check(c, launch_fe_stage(a, c->n, c->stream), "fe stage");
+        timer_end(c, GF_K_FE_STAGE, ev);
+    }
     if (st) return st;
     // dp = q_s dt of this stage feeds Ampere's law (get_dp_RK*, :1786-1832)
     for (int i = 0; i < 3; i++) c->fe_q_used[i] = qs[i];
@@ -1964,11 +1970,14 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
     a.nx = nx; a.ny = ny; a.nz = nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz; a.dt = c->dt;
     for (int i = 0; i < 3; i++) a.per[i] = c->per[i];
     a.stage = stage;
-    // Ampere and Faraday with the RK updates (both read the stage-s stores)
+    // Ampere and Faraday with the RK updates (both read the stage-s stores),
+    // with a deferred ferroelectric stage in the same launch
     if (!st) {
         timer_begin(c, GF_K_EM_E, &ev);
-        st = hip_check(c, launch_em_eh(a, c->stream), "EM E and H");
+        if (c->fe_pending) st = hip_check(c, launch_fe_em(c->fe_args, a, c->stream), "FE stage, EM E and H");
+        else st = hip_check(c, launch_em_eh(a, c->stream), "EM E and H");
         timer_end(c, GF_K_EM_E, ev);
+        c->fe_pending = false;
     }
     // Jp from the cell E of the stage store (get_dJp_RK*, update_Jp*)
     if (!st && c->jp_on) {
@@ -2070,6 +2079,8 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
         (c->any_fe && c->fe_next_stage != 1) || (c->el_on && c->el_next_stage != 1) ||
         (c->em_on && c->em_next_stage != 1))
         return set_error(c, GF_ERR_STATE, "gf_step called in the middle of a step");
+    // the FE stage runs inside the EM stage's launch (they are independent)
+    c->fe_defer = c->any_fe && c->em_on;
     for (int64_t s = 0; s < nsteps; s++) {
         gf_status st = gf_time_advance(c);
         for (int stage = 1; stage <= 4 && !st; stage++) {
@@ -2082,8 +2093,13 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
         }
         // the static strain after the step without elastodynamics (main.cpp:472-474)
         if (!st && (c->flags & GF_FLAG_ELASTOSTATIC)) st = elastostatic_static(c);
-        if (st) return st;
+        if (st) {
+            c->fe_defer = false;
+            c->fe_pending = false;
+            return st;
+        }
     }
+    c->fe_defer = false;
     return GF_OK;
 }
 

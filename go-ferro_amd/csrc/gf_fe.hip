@@ -1,7 +1,9 @@
 // gf_fe.hip — fused ferroelectric RK4 stage for gfx950 (FP64).
 //
 // One launch per stage does get_dq_RKs, get_dp_RKs, update_q_RKs and
-// update_p_RKs (ferroelectric_solver.cpp:534-1832, 1835-1999):
+// update_p_RKs (ferroelectric_solver.cpp:534-1832, 1835-1999), alone or, in
+// a context with electrodynamics stepped by gf_step, together with the EM
+// stage's Ampere and Faraday updates:
 //   k_q = -dt/m [ dF_Landau/dp - G11 lap(p) - E_stat - E_ext ] - dt*gamma/m q_s   (FE cells)
 //   k_p = q_s dt                                                                 (all cells)
 //   stage 1..3: next = y + c_s k, acc = y + k/6 | acc + k/3
@@ -9,6 +11,7 @@
 // which is the reference's left-to-right y + k1/6 + k2/3 + k3/3 + k4/6.
 // Powers are products instead of pow(), and the RK weights products with the
 // rounded reciprocals instead of quotients; the difference is a few ulp.
+#include "gf_em_dev.h"
 #include "gf_internal.h"
 
 namespace gf {
@@ -51,10 +54,8 @@ __device__ __forceinline__ void rk(int stage, int64_t id, double k, const double
     }
 }
 
-__global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
-    const int64_t n = a.nx * a.ny * a.nz;
-    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= n) return;
+// The stage of cell id.
+__device__ __forceinline__ void fe_cell(const FeStageArgs& a, int64_t id) {
     const uint32_t own = a.code[id];
     double kq[3] = {0., 0., 0.};
     const double q0 = a.qstore[0][id], q1 = a.qstore[1][id], q2 = a.qstore[2][id];
@@ -128,6 +129,27 @@ __global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
     rk(a.stage, id, kq[0], a.qbase[0], a.qacc[0], a.qout[0]);
     rk(a.stage, id, kq[1], a.qbase[1], a.qacc[1], a.qout[1]);
     rk(a.stage, id, kq[2], a.qbase[2], a.qacc[2], a.qout[2]);
+}
+
+__global__ void __launch_bounds__(256) fe_stage_kernel(FeStageArgs a) {
+    const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id < a.nx * a.ny * a.nz) fe_cell(a, id);
+}
+
+// The ferroelectric stage and the electrodynamics stage's Ampere and
+// Faraday updates in one launch (cells, then edges, then faces).  They are
+// independent: the EM update reads the q store the FE stage reads (dp =
+// q_s dt) and neither reads what the other writes; the FE stage reads the
+// cell-centred E of the stage store, which the cells launch after the EM
+// update writes.
+__global__ void __launch_bounds__(256) fe_em_kernel(FeStageArgs f, EmArgs e) {
+    const int64_t n = f.nx * f.ny * f.nz, nx = e.nx, ny = e.ny, nz = e.nz;
+    const int64_t nE = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz;
+    const int64_t nH = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) fe_cell(f, t);
+    else if (t < n + nE) em_dev::em_edge(e, t - n);
+    else if (t < n + nE + nH) em_dev::em_face(e, t - n - nE);
 }
 
 // The bracket of the flexoelectric term of get_dq_RK* at each FE cell
@@ -206,6 +228,15 @@ __global__ void __launch_bounds__(256) fe_flexo_kernel(FeFlexoArgs a) {
 hipError_t launch_fe_flexo(const FeFlexoArgs& a, hipStream_t s) {
     const int64_t blocks = (a.nx * a.ny * a.nz + 255) / 256;
     hipLaunchKernelGGL(fe_flexo_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fe_em(const FeStageArgs& f, const EmArgs& e, hipStream_t s) {
+    const int64_t n = f.nx * f.ny * f.nz, nx = e.nx, ny = e.ny, nz = e.nz;
+    const int64_t nE = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz;
+    const int64_t nH = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1);
+    const int64_t blocks = (n + nE + nH + 255) / 256;
+    hipLaunchKernelGGL(fe_em_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, f, e);
     return hipGetLastError();
 }
 

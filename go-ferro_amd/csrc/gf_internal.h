@@ -639,6 +639,11 @@ struct gf_ctx {
     double* em_dm1[3] = {};
     double* em_dm2[3] = {};
     const double* fe_q_used[3] = {}; // q store read by the last ferroelectric stage
+    // gf_step with a ferroelectric and electrodynamics: the FE stage's
+    // launch is deferred and runs in the EM stage's Ampere/Faraday launch
+    bool fe_defer = false;
+    bool fe_pending = false;
+    gf::FeStageArgs fe_args{};
     int e_sel = 0, h_sel = 0, em_next_stage = 1;
     double cf_top_half = 0, cf_bot_half = 0, cf_x_half = 0, cf_y_half = 0;
     double cf_top = 0, cf_bot = 0, cf_x = 0, cf_y = 0;
@@ -754,6 +759,7 @@ hipError_t launch_el_boundary(const ElBoundaryArgs& a, hipStream_t s);
 
 // electrodynamics (gf_em.hip)
 hipError_t launch_em_eh(const EmArgs& a, hipStream_t s);
+hipError_t launch_fe_em(const FeStageArgs& f, const EmArgs& e, hipStream_t s);
 hipError_t launch_em_jp(const EmJpArgs& a, hipStream_t s);
 hipError_t launch_em_cells(const EmCellArgs& a, hipStream_t s);
 hipError_t launch_spin_pump(const SpinArgs& a, hipStream_t s);
