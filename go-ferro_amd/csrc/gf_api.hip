@@ -707,8 +707,12 @@ double** h_store(gf_ctx* c, int sel) { return sel == 0 ? c->Hb : (sel == 1 ? c->
 
 // Cell-centred E and/or H of the current stores (update_DE_cell /
 // update_DH_cell, EMdynamic_solver.cpp:3780-3826).
-gf_status em_cells(gf_ctx* c, bool do_e, bool do_h) {
+gf_status em_cells(gf_ctx* c, bool do_e, bool do_h, const EmWallArgs* zwall = nullptr) {
     EmCellArgs a{};
+    if (zwall) {
+        a.zwall = 1;
+        a.w = *zwall;
+    }
     a.code = c->d_code; a.mats = c->d_mats;
     double** es = e_store(c, c->e_sel);
     double** hs = h_store(c, c->h_sel);
@@ -2004,7 +2008,12 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
         st = hip_check(c, launch_em_plane(p, c->stream), "EM plane source");
         timer_end(c, GF_K_OTHER, ev);
     }
-    // Mur or PEC walls, x then y then z (EMdynamic_AbsorbingBoundary.cpp:58-874)
+    // Mur or PEC walls, x then y then z (EMdynamic_AbsorbingBoundary.cpp:58-874);
+    // with only z walls (and no PEC edges after them) the cells launch
+    // applies them to the edges it reads
+    const bool zwall_in_cells = !c->em_periodic_all && c->per[0] && c->per[1] && !c->per[2] && !c->em_pec_all &&
+                                c->nz >= 2;
+    EmWallArgs zw{};
     if (!c->em_periodic_all) {
         for (int axis = 0; axis < 3 && !st; axis++) {
             if (c->per[axis]) continue;
@@ -2016,6 +2025,10 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
             if (axis == 0) w.cf = kind == 0 ? c->cf_x_half : c->cf_x;
             else if (axis == 1) w.cf = kind == 0 ? c->cf_y_half : c->cf_y;
             else { w.cf = kind == 0 ? c->cf_top_half : c->cf_top; w.cb = kind == 0 ? c->cf_bot_half : c->cf_bot; }
+            if (axis == 2 && zwall_in_cells) {
+                zw = w;
+                continue;
+            }
             timer_begin(c, GF_K_OTHER, &ev);
             st = hip_check(c, launch_em_wall(w, c->stream), "EM walls");
             timer_end(c, GF_K_OTHER, ev);
@@ -2033,7 +2046,7 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
     if (st) return st;
     c->e_sel = out_e;
     c->h_sel = out_h;
-    st = em_cells(c, true, true);
+    st = em_cells(c, true, true, zwall_in_cells ? &zw : nullptr);
     // Ampere's law of stage 4 has read the stage current: now the step one
     if (!st && stage == 4 && c->spin_on) st = spin_pump_step(c);
     if (st) return st;

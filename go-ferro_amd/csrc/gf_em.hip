@@ -62,6 +62,18 @@ __global__ void __launch_bounds__(256) em_jp_kernel(EmJpArgs a) {
         }
 }
 
+// The new value of the z-wall edge of line col (x or y component comp;
+// both layouts end in nz + 1 along z) at the top (k = nz) or the bottom
+// (k = 0), as em_wall_kernel writes it.
+__device__ __forceinline__ double z_wall_value(const EmWallArgs& w, int comp, int64_t col, bool top) {
+    const int64_t nz = w.nz;
+    if (w.pec || (!top && w.onlytop)) return 0.;
+    const int64_t wall = top ? col + nz : col, inner = top ? col + nz - 1 : col + 1;
+    const double cf = top ? w.cf : w.cb;
+    if (w.kind < 2) return w.b[comp][inner] + cf * (w.b[comp][wall] - w.s[comp][inner]);
+    return w.t1[comp][inner] + cf * (w.t1[comp][wall] - w.b[comp][inner]);
+}
+
 __global__ void __launch_bounds__(256) em_cells_kernel(EmCellArgs a) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -72,9 +84,32 @@ __global__ void __launch_bounds__(256) em_cells_kernel(EmCellArgs a) {
         const double* ey = a.es[1];
         const double* ez = a.es[2];
         const int64_t x00 = (i * (ny + 1) + j) * (nz + 1) + k, x10 = (i * (ny + 1) + j + 1) * (nz + 1) + k;
-        a.ecell[0][id] = (ex[x00] + ex[x10] + ex[x00 + 1] + ex[x10 + 1]) / 4.;
         const int64_t y00 = (i * ny + j) * (nz + 1) + k, y10 = ((i + 1) * ny + j) * (nz + 1) + k;
-        a.ecell[1][id] = (ey[y00] + ey[y10] + ey[y00 + 1] + ey[y10 + 1]) / 4.;
+        double vx[4] = {ex[x00], ex[x10], ex[x00 + 1], ex[x10 + 1]};
+        double vy[4] = {ey[y00], ey[y10], ey[y00 + 1], ey[y10 + 1]};
+        if (a.zwall && (k == 0 || k == nz - 1)) {
+            // the wall edges this cell reads, from the wall condition; the
+            // cell of a line's own (i, j) writes it (and the last cell the
+            // line past the grid)
+            double* wx = a.w.kind < 2 ? a.w.s[0] : a.w.b[0];
+            double* wy = a.w.kind < 2 ? a.w.s[1] : a.w.b[1];
+            for (int top = 0; top < 2; top++) {
+                if (top ? k != nz - 1 : k != 0) continue;
+                const int slot = top ? 2 : 0;
+                const int64_t cx0 = x00 - k, cx1 = x10 - k, cy0 = y00 - k, cy1 = y10 - k;   // line starts
+                vx[slot] = z_wall_value(a.w, 0, cx0, top);
+                vx[slot + 1] = z_wall_value(a.w, 0, cx1, top);
+                vy[slot] = z_wall_value(a.w, 1, cy0, top);
+                vy[slot + 1] = z_wall_value(a.w, 1, cy1, top);
+                const int64_t off = top ? nz : 0;
+                wx[cx0 + off] = vx[slot];
+                if (j == ny - 1) wx[cx1 + off] = vx[slot + 1];
+                wy[cy0 + off] = vy[slot];
+                if (i == nx - 1) wy[cy1 + off] = vy[slot + 1];
+            }
+        }
+        a.ecell[0][id] = (vx[0] + vx[1] + vx[2] + vx[3]) / 4.;
+        a.ecell[1][id] = (vy[0] + vy[1] + vy[2] + vy[3]) / 4.;
         const int64_t z00 = (i * (ny + 1) + j) * nz + k, z10 = ((i + 1) * (ny + 1) + j) * nz + k;
         const int64_t z01 = (i * (ny + 1) + j + 1) * nz + k, z11 = ((i + 1) * (ny + 1) + j + 1) * nz + k;
         a.ecell[2][id] = (ez[z00] + ez[z10] + ez[z01] + ez[z11]) / 4.;

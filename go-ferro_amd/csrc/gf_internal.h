@@ -160,6 +160,19 @@ struct EmJpArgs {
     int32_t stage;
 };
 
+struct EmWallArgs {
+    double* b[3];               // E base
+    double* s[3];               // E out store
+    const double* t1[3];        // E base before the final combine
+    int64_t nx, ny, nz;
+    int32_t axis;               // walls normal to x, y or z
+    int32_t kind;               // 0 = stages 1-2, 1 = stage 3, 2 = after the step
+    int32_t pec;                // the walls are PEC
+    int32_t onlytop;            // bottom z wall held at zero
+    double cf;                  // x / y coefficient, or the top one for z
+    double cb;                  // bottom coefficient for z
+};
+
 struct EmCellArgs {
     const uint32_t* code;
     const DevMaterial* mats;
@@ -173,20 +186,12 @@ struct EmCellArgs {
     const double* hzstat;
     int64_t nx, ny, nz;
     int32_t do_e, do_h, one_d;
+    // the z walls (axis 2) applied by the cells launch to the edges it
+    // reads, when the x and y walls are periodic: set zwall
+    int32_t zwall;
+    EmWallArgs w;
 };
 
-struct EmWallArgs {
-    double* b[3];               // E base
-    double* s[3];               // E out store
-    const double* t1[3];        // E base before the final combine
-    int64_t nx, ny, nz;
-    int32_t axis;               // walls normal to x, y or z
-    int32_t kind;               // 0 = stages 1-2, 1 = stage 3, 2 = after the step
-    int32_t pec;                // the walls are PEC
-    int32_t onlytop;            // bottom z wall held at zero
-    double cf;                  // x / y coefficient, or the top one for z
-    double cb;                  // bottom coefficient for z
-};
 
 struct EmMiscArgs {
     double* b[3];
