@@ -310,17 +310,20 @@ def bench_coupled(which: str, steps: int) -> dict:
     ctx.sync()
     el_plain = time.perf_counter() - t1
     ctx.close()
-    # compulsory bytes per launch of the state each class owns (fused RK4: 17 double
-    # accesses per state component per step = 4.25 per stage) plus its cross reads
+    # compulsory bytes per stage of the state each class owns (fused RK4: 17 double
+    # accesses per state component per step = 4.25 per stage) plus its cross reads.
+    # The elastic class is its two launches (strain, eigenstrain and stress parts; then
+    # the stress divergence and updates): u, v state, the strain written for the
+    # magnets, the m store read for the eigenstrain.  The EM class is one launch
+    # for Ampere, Faraday and the ferroelectric stage (gf_step runs them together).
     if which == "magnetoelastic":
         nmag = n * 16 // 64
-        per = {"elastic_stage": n * (6 * 4.25 * 8 + 4), "elastic_strain": n * (3 * 8 + 12 * 8),
+        per = {"elastic_stage": n * (6 * 4.25 * 8 + 4 + 6 * 8) + nmag * 3 * 8,
                "mag_stage": nmag * (3 * 4.25 * 8 + 6 * 8 + 4 + 24)}
     else:
         nfe = n * 200 // 1024
         ne = 6.77 * n
-        per = {"em_e": ne / 2 * 4.25 * 8 + 3 * 8 * n, "em_h": ne / 2 * 4.25 * 8,
-               "fe_stage": nfe * (6 * 4.25 * 8 + 3 * 8 + 4)}
+        per = {"em_e": ne * 4.25 * 8 + 3 * 8 * n + nfe * (6 * 4.25 * 8 + 3 * 8 + 4)}
     kernels = {}
     for cls, (ms, cnt) in kt.items():
         if cnt:
