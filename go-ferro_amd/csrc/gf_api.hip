@@ -285,12 +285,13 @@ double** afm2_store(gf_ctx* c, int sel) { return sel == 0 ? c->m2 : (sel == 1 ? 
 double** fe_pstore(gf_ctx* c, int sel) { return sel == 0 ? c->p : (sel == 1 ? c->p_a : c->p_b); }
 double** fe_qstore(gf_ctx* c, int sel) { return sel == 0 ? c->q : (sel == 1 ? c->q_a : c->q_b); }
 
-// The static strain of the current magnet stores.
+// The static strain of the current magnet and ferroelectric stores.
 gf_status elastostatic_static(gf_ctx* c) {
     double** fm = c->any_fm ? fm_store(c, c->mag_store_sel) : nullptr;
     double** s1 = c->any_afm ? afm1_store(c, c->afm_store_sel) : nullptr;
     double** s2 = c->any_afm ? afm2_store(c, c->afm_store_sel) : nullptr;
-    return elastostatic_run(c, fm, s1, s2);
+    double** ps = c->any_fe ? fe_pstore(c, c->fe_store_sel) : nullptr;
+    return elastostatic_run(c, fm, s1, s2, ps);
 }
 
 // The magnetostatic part of gf_demag: H of the current magnet stores.
@@ -1111,7 +1112,7 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
         return c->fstore;
     case GF_FIELD_EIGENSTRAIN_T0_DIAG:
     case GF_FIELD_EIGENSTRAIN_T0_OFF:
-        if (!c->el_on) break;
+        if (!c->el_on && !(c->flags & GF_FLAG_ELASTOSTATIC)) break;
         if (!c->e0t0[0]) {
             // six zeroed arrays on first use; the stage reads them only once written
             if ((*st = alloc3(c, c->e0t0, "hipMalloc static eigenstrain"))) return nullptr;
@@ -1564,9 +1565,13 @@ std::string stage_io_conflict(const FeStageArgs& a) {
 // (0 = p/q, 1 = the a stores, 2 = the b stores); the drive is left at zero.
 FeStageArgs fe_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     FeStageArgs a{};
+    // the strain and eigenstrain of the elastodynamics, or, with the
+    // elastostatics instead, the static ones (ex*t0_crt, ex*0t0_crt:
+    // ferroelectric_solver.cpp:624-653 with the dynamic parts zero)
+    const bool stat = !c->el_on && (c->flags & GF_FLAG_ELASTOSTATIC) && (c->flags & GF_FLAG_ELASTO_ON_PM);
     for (int i = 0; i < 6; i++) {
-        a.strain[i] = strain_coupled(c) ? c->ec[i] : nullptr;
-        a.eigen[i] = strain_coupled(c) ? c->e0c[i] : nullptr;
+        a.strain[i] = strain_coupled(c) ? c->ec[i] : (stat ? c->e_t0[i] : nullptr);
+        a.eigen[i] = strain_coupled(c) ? c->e0c[i] : (stat ? c->e0t0[i] : nullptr);
     }
     const bool em_p = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_P);
     for (int i = 0; i < 3; i++) a.ecell[i] = em_p ? c->Ecell[i] : nullptr;
@@ -2300,7 +2305,6 @@ std::vector<StateArray> state_arrays(gf_ctx* c, const StateSel& sel, uint32_t bi
         if (c->flags & GF_FLAG_ORIENT_ROTATE) { add(19, c->ec, 6, n); add(20, c->e0c, 6, n); }
         if (c->hist)
             v.push_back({21u << 8, c->hist, static_cast<int64_t>(c->hist_slots) * kHistPlanes * 3 * c->nx * c->ny});
-        if (bits & kBitHasE0t0) add(37, c->e0t0, 6, n);
     }
     if (c->em_on) {
         for (int a = 0; a < 3; a++) {
@@ -2321,6 +2325,7 @@ std::vector<StateArray> state_arrays(gf_ctx* c, const StateSel& sel, uint32_t bi
     if (bits & kBitHasHnu) add(30, c->hnu, 3, n);
     add(31, c->hdrive, 3, n);
     add(32, c->e_t0, 6, n);
+    if (bits & kBitHasE0t0) add(37, c->e0t0, 6, n);
     add(36, c->jis, 2, n);
     const DemagPlan& d = c->demag;
     if (d.ready) {
@@ -2466,8 +2471,9 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
         st = alloc3(c, c->hnu, "hipMalloc hext_nonunif");
         if (st) return st;
     }
-    if ((h.bits & kBitHasE0t0) && !c->el_on)
-        return set_error(c, GF_ERR_ARG, "the state image holds a static eigenstrain and this context has no elastodynamics");
+    if ((h.bits & kBitHasE0t0) && !c->el_on && !(c->flags & GF_FLAG_ELASTOSTATIC))
+        return set_error(c, GF_ERR_ARG,
+                         "the state image holds a static eigenstrain and this context has no elastodynamics or elastostatics");
     if ((h.bits & kBitHasE0t0) && !c->e0t0[0]) {
         st = alloc3(c, c->e0t0, "hipMalloc static eigenstrain");
         if (!st) st = alloc3(c, c->e0t0 + 3, "hipMalloc static eigenstrain");

@@ -1,15 +1,19 @@
-// gf_elastostatic.hip — static strain of the magnets' eigenstrain
-// (GF_FLAG_ELASTOSTATIC) for gfx950, FP64.
+// gf_elastostatic.hip — static strain of the eigenstrain of the magnets and
+// ferroelectrics (GF_FLAG_ELASTOSTATIC) for gfx950, FP64.
 //
 // The reference declares get_strain_static (elastic_system.h:190) without a
 // definition; its set-up fixes a homogeneous cubic stiffness (the
 // component-wise largest c11, c12, c44 times scale_elasto,
 // elastic_initialize.cpp:22-40) and the Green tensor G_ik of that medium
 // over crystal-frame wave vectors (:340-389).  Here, in a periodic body:
-//   sigma0 = C e0                    e0 the eigenstrain (elastic_dynamic_solver.cpp:21-73)
+//   sigma0 = C e0                    e0 the eigenstrain (elastic_dynamic_solver.cpp:21-73:
+//                                    magnetostriction and electrostriction)
 //   w(k)   = G(k) (sigma0(k) k)      u = -i w
 //   e(k)   = (k (x) w + w (x) k) / 2 for k != 0,   e(0) = C^-1 sigma0(0)
 // with a Nyquist component averaged over its two signs so that e is real.
+// e0 is written out as the static eigenstrain ex*0t0_crt, which a
+// ferroelectric reads beside the static strain (ferroelectric_solver.cpp:
+// 636-650) and which main.cpp copies out with it (:148-156).
 // This is not a hot path (once per step at most): six complex fields are
 // transformed by one workgroup per line with the line in LDS.
 #include <algorithm>
@@ -59,8 +63,10 @@ struct EsArgs {
     const double* m[3];     // FM store, null when no FM
     const double* m1[3];    // AFM stores, null when no AFM
     const double* m2[3];
+    const double* p[3];     // FE p store, null when no FE
     double2* f[6];          // sigma0, then the strain spectra, then the strain
     double* out[6];
+    double* e0[6];          // the eigenstrain e0 (crystal frame), null when not kept
     int64_t nx, ny, nz;
     double dx, dy, dz;
     double c11, c12, c44;
@@ -79,8 +85,9 @@ __device__ __forceinline__ void to_crystal(const EsArgs& a, double x, double y, 
     }
 }
 
-// sigma0 = C e0 of every cell, e0 the magnetostrictive eigenstrain in the
-// crystal frame (zero in vacuum).
+// sigma0 = C e0 of every cell, e0 the magnetostrictive and electrostrictive
+// eigenstrain in the crystal frame (zero in vacuum), in the reference's sum
+// order; e0 itself written out when kept.
 __global__ void __launch_bounds__(256) es_sigma_kernel(EsArgs a) {
     const int64_t n = a.nx * a.ny * a.nz;
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -90,21 +97,28 @@ __global__ void __launch_bounds__(256) es_sigma_kernel(EsArgs a) {
     if (t) {
         const DevMaterial& M = a.mats[t - 1];
         double mx = 0., my = 0., mz = 0., ax = 0., ay = 0., az = 0., bx = 0., by = 0., bz = 0.;
+        double px = 0., py = 0., pz = 0.;
         if (a.m[0]) to_crystal(a, a.m[0][id], a.m[1][id], a.m[2][id], mx, my, mz);
         if (a.m1[0]) {
             to_crystal(a, a.m1[0][id], a.m1[1][id], a.m1[2][id], ax, ay, az);
             to_crystal(a, a.m2[0][id], a.m2[1][id], a.m2[2][id], bx, by, bz);
         }
-        e[0] = 1.5 * M.l100 * (mx * mx - 1. / 3.) + 0.5 * 1.5 * M.l100a * (ax * ax - 1. / 3.) +
-               0.5 * 1.5 * M.l100b * (bx * bx - 1. / 3.);
-        e[1] = 1.5 * M.l100 * (my * my - 1. / 3.) + 0.5 * 1.5 * M.l100a * (ay * ay - 1. / 3.) +
-               0.5 * 1.5 * M.l100b * (by * by - 1. / 3.);
-        e[2] = 1.5 * M.l100 * (mz * mz - 1. / 3.) + 0.5 * 1.5 * M.l100a * (az * az - 1. / 3.) +
-               0.5 * 1.5 * M.l100b * (bz * bz - 1. / 3.);
-        e[3] = 1.5 * M.l111 * my * mz + 0.5 * 1.5 * M.l111a * ay * az + 0.5 * 1.5 * M.l111b * by * bz;
-        e[4] = 1.5 * M.l111 * mx * mz + 0.5 * 1.5 * M.l111a * ax * az + 0.5 * 1.5 * M.l111b * bx * bz;
-        e[5] = 1.5 * M.l111 * my * mx + 0.5 * 1.5 * M.l111a * ay * ax + 0.5 * 1.5 * M.l111b * by * bx;
+        if (a.p[0]) to_crystal(a, a.p[0][id], a.p[1][id], a.p[2][id], px, py, pz);
+        e[0] = 1.5 * M.l100 * (mx * mx - 1. / 3.) + M.Q11 * (px * px) + M.Q12 * (py * py + pz * pz) +
+               0.5 * 1.5 * M.l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bx * bx - 1. / 3.);
+        e[1] = 1.5 * M.l100 * (my * my - 1. / 3.) + M.Q11 * (py * py) + M.Q12 * (px * px + pz * pz) +
+               0.5 * 1.5 * M.l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * M.l100b * (by * by - 1. / 3.);
+        e[2] = 1.5 * M.l100 * (mz * mz - 1. / 3.) + M.Q11 * (pz * pz) + M.Q12 * (py * py + px * px) +
+               0.5 * 1.5 * M.l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bz * bz - 1. / 3.);
+        e[3] = 1.5 * M.l111 * my * mz + M.Q44 * py * pz + 0.5 * 1.5 * M.l111a * ay * az +
+               0.5 * 1.5 * M.l111b * by * bz;
+        e[4] = 1.5 * M.l111 * mx * mz + M.Q44 * px * pz + 0.5 * 1.5 * M.l111a * ax * az +
+               0.5 * 1.5 * M.l111b * bx * bz;
+        e[5] = 1.5 * M.l111 * my * mx + M.Q44 * py * px + 0.5 * 1.5 * M.l111a * ay * ax +
+               0.5 * 1.5 * M.l111b * by * bx;
     }
+    if (a.e0[0])
+        for (int c = 0; c < 6; c++) a.e0[c][id] = e[c];
     const double s[6] = {a.c11 * e[0] + a.c12 * (e[1] + e[2]), a.c11 * e[1] + a.c12 * (e[0] + e[2]),
                          a.c11 * e[2] + a.c12 * (e[0] + e[1]), 2. * a.c44 * e[3],
                          2. * a.c44 * e[4], 2. * a.c44 * e[5]};
@@ -224,11 +238,11 @@ gf_status elastostatic_setup(gf_ctx* c, const gf_desc* d) {
     auto pow2 = [](int64_t v) { return v >= 1 && (v & (v - 1)) == 0; };
     if (!pow2(c->nx) || !pow2(c->ny) || !pow2(c->nz) || c->nx > kLineMax || c->ny > kLineMax || c->nz > kLineMax)
         return set_error(c, GF_ERR_UNSUPPORTED, "the elastostatics needs power-of-two nx, ny, nz up to 2048");
-    if (!(c->any_fm || c->any_afm))
-        return set_error(c, GF_ERR_UNSUPPORTED, "the elastostatics needs a magnetic material");
-    if ((c->flags & GF_FLAG_ELASTODYNAMICS) || c->any_fe || (c->flags & GF_FLAG_FLEXO))
+    if (!(c->any_fm || c->any_afm || c->any_fe))
+        return set_error(c, GF_ERR_UNSUPPORTED, "the elastostatics needs a magnetic or ferroelectric material");
+    if ((c->flags & GF_FLAG_ELASTODYNAMICS) || (c->flags & GF_FLAG_FLEXO))
         return set_error(c, GF_ERR_UNSUPPORTED,
-                         "the elastostatics runs for magnets without elastodynamics or a ferroelectric");
+                         "the elastostatics runs without elastodynamics and without the flexoelectric coupling");
     const double sc = d->scale_elasto > 0. ? d->scale_elasto : 1.;
     double c11 = c->host_mats[0].c11, c12 = c->host_mats[0].c12, c44 = c->host_mats[0].c44;
     for (const auto& m : c->host_mats) {
@@ -246,18 +260,27 @@ gf_status elastostatic_setup(gf_ctx* c, const gf_desc* d) {
         st = hip_check(c, hipMalloc(&c->e_t0[a], sizeof(double) * static_cast<size_t>(c->n)), "hipMalloc static strain");
         if (!st) st = hip_check(c, hipMemset(c->e_t0[a], 0, sizeof(double) * static_cast<size_t>(c->n)), "clear static strain");
     }
+    // the static eigenstrain of each solve, kept (and carried by the state
+    // image) for the ferroelectric stage and the eigenstrain output
+    for (int a = 0; a < 6 && !st; a++) {
+        if (c->e0t0[a]) continue;
+        st = hip_check(c, hipMalloc(&c->e0t0[a], sizeof(double) * static_cast<size_t>(c->n)), "hipMalloc static eigenstrain");
+        if (!st) st = hip_check(c, hipMemset(c->e0t0[a], 0, sizeof(double) * static_cast<size_t>(c->n)), "clear static eigenstrain");
+    }
+    if (!st) c->has_e0t0 = true;
     return st;
 }
 
-gf_status elastostatic_run(gf_ctx* c, double* const* fm, double* const* a1, double* const* a2) {
+gf_status elastostatic_run(gf_ctx* c, double* const* fm, double* const* a1, double* const* a2, double* const* p) {
     EsArgs a{};
     a.code = c->d_code; a.mats = c->d_mats;
     for (int i = 0; i < 3; i++) {
         a.m[i] = fm ? fm[i] : nullptr;
         a.m1[i] = a1 ? a1[i] : nullptr;
         a.m2[i] = a2 ? a2[i] : nullptr;
+        a.p[i] = p ? p[i] : nullptr;
     }
-    for (int q = 0; q < 6; q++) { a.f[q] = c->es_buf + q * c->n; a.out[q] = c->e_t0[q]; }
+    for (int q = 0; q < 6; q++) { a.f[q] = c->es_buf + q * c->n; a.out[q] = c->e_t0[q]; a.e0[q] = c->e0t0[q]; }
     a.nx = c->nx; a.ny = c->ny; a.nz = c->nz; a.dx = c->dx; a.dy = c->dy; a.dz = c->dz;
     a.c11 = c->es_c[0]; a.c12 = c->es_c[1]; a.c44 = c->es_c[2];
     a.rotate = (c->flags & GF_FLAG_ORIENT_ROTATE) ? 1 : 0;

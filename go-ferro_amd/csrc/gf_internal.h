@@ -718,7 +718,7 @@ gf_status hip_check(gf_ctx* c, hipError_t e, const char* what);
 // elastostatics (gf_elastostatic.hip): set-up, and the static strain of the
 // given magnet stores into e_t0
 gf_status elastostatic_setup(gf_ctx* c, const gf_desc* d);
-gf_status elastostatic_run(gf_ctx* c, double* const* fm, double* const* a1, double* const* a2);
+gf_status elastostatic_run(gf_ctx* c, double* const* fm, double* const* a1, double* const* a2, double* const* p);
 // the thread's error of a call without a context (gf_last_error(NULL))
 gf_status set_thread_error(gf_status code, const std::string& msg);
 void timer_begin(gf_ctx* c, int cls, hipEvent_t* ev);

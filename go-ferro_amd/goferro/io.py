@@ -467,8 +467,8 @@ def _unsupported(g: dict, mats: list) -> list[str]:
     if g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
         out.append("the 1D and film elastostatics and external strain (if_elastostatic_1D, if_elastostatic_film, "
                    "if_read_strain_ext)")
-    if g["if_elastostatic"] and (g["if_elastodynamics"] or any_fe):
-        out.append("if_elastostatic with elastodynamics or a ferroelectric (the static solve covers magnets alone)")
+    if g["if_elastostatic"] and (g["if_elastodynamics"] or (any_fe and g["if_flexo"])):
+        out.append("if_elastostatic with elastodynamics or the flexoelectric coupling")
     if any_fe and g["if_elec_1Dmodel"]:
         out.append("if_elec_1Dmodel (get_E_static_1D is declared but not defined in the reference)")
     if g["if_input_straint0"] and (g["if_elastodynamics"] or g["if_OrientRotate"]):
@@ -846,10 +846,11 @@ class Outputs:
 
     def _eigenstrain_t0(self, ctx: Context, nstep: int) -> None:
         """output_eigenstraint0_crt (inoutput.cpp:762-789): the static
-        eigenstrain at t0, which nothing updates; without elastodynamics the
-        context holds none and the file repeats what was read (or zeros)."""
+        eigenstrain at t0 (with elastodynamics, as read; with elastostatics,
+        that of the last static solve); otherwise the context holds none and
+        the file repeats what was read (or zeros)."""
         g = self.g
-        if g["if_elastodynamics"]:
+        if g["if_elastodynamics"] or g["if_elastostatic"]:
             e = np.concatenate([ctx.download(FIELD_EIGENSTRAIN_T0_DIAG), ctx.download(FIELD_EIGENSTRAIN_T0_OFF)])
         elif g["if_input_eigenstraint0_crt"]:
             e = read_cells(self.deck.directory / "eigenstraint0_crt.in", self.deck.problem.shape, 6)

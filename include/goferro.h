@@ -96,18 +96,22 @@ typedef int gf_status;
  * F12, F44 and tv4..tv8 are derived from c11, c12, c44, Q11, Q12, Q44 and
  * f11, f12, f44 as read.cpp:181-199 does (zero for a non-ferroelectric). */
 #define GF_FLAG_FLEXO            (1u << 26)
-/* if_elastostatic: the static strain of the magnets' eigenstrain
- * (get_strain_static, declared at elastic_system.h:190 but not defined in
- * the reference), in a periodic body of the homogeneous cubic stiffness of
+/* if_elastostatic: the static strain of the eigenstrain of the magnets
+ * and ferroelectrics (get_strain_static, declared at elastic_system.h:190
+ * but not defined in the reference), in a periodic body of the homogeneous
+ * cubic stiffness of
  * elastic_initialize.cpp:22-40 (component-wise largest c11, c12, c44 times
  * scale_elasto) through its Green tensor G_ik (:340-389):
  * e(k) = sym(k (x) G(k) sigma0(k) k) for k != 0, with sigma0 = C e0 and
  * e0 the eigenstrain of elastic_dynamic_solver.cpp:4-104, and the mean
  * strain equal to the mean eigenstrain (zero mean stress).  A Nyquist
  * component enters symmetrised over its sign, so the strain is real.
- * Written into GF_FIELD_STRAIN_T0_* (crystal frame) by gf_demag and after
- * every gf_step step (main.cpp:214, 473).  Magnets only, without
- * elastodynamics; power-of-two nx, ny, nz up to 2048. */
+ * Written into GF_FIELD_STRAIN_T0_* (crystal frame), with e0 into
+ * GF_FIELD_EIGENSTRAIN_T0_*, by gf_demag and after every gf_step step
+ * (main.cpp:214, 473).  With GF_FLAG_ELASTO_ON_PM the magnets read the
+ * static strain and the ferroelectrics both (ferroelectric_solver.cpp:
+ * 636-650).  Without elastodynamics and the flexoelectric coupling;
+ * power-of-two nx, ny, nz up to 2048. */
 #define GF_FLAG_ELASTOSTATIC     (1u << 27)
 /* if_prescribed_m: the ferromagnetic cells of the bottom layer (k = 0)
  * precess on a prescribed cone, m = (sqrt(1 - m0^2), m0 sin wt, m0 cos wt)
@@ -293,7 +297,9 @@ enum gf_field_id {
      * by input_eigenstraint0 (inoutput.cpp:259-280).  The elastic stage
      * subtracts it from the eigenstrain of the m, m_AFM and p stores
      * (elastic_dynamic_solver.cpp:39-65), so the elastodynamics carries
-     * the change from t0.  With GF_FLAG_ELASTODYNAMICS; zero until uploaded. */
+     * the change from t0.  With GF_FLAG_ELASTODYNAMICS; zero until uploaded.
+     * With GF_FLAG_ELASTOSTATIC each static solve writes it (the e0 of the
+     * solve). */
     GF_FIELD_EIGENSTRAIN_T0_DIAG = 27,
     GF_FIELD_EIGENSTRAIN_T0_OFF = 28,
     GF_FIELD_COUNT

@@ -930,8 +930,8 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_V: return c->v;
     case GF_FIELD_U_STORE: return c->us;
     case GF_FIELD_ELASTOFORCE: return (c->flags & GF_FLAG_ELASTODYNAMICS) ? c->force_store : NULL;
-    case GF_FIELD_EIGENSTRAIN_T0_DIAG: return (c->flags & GF_FLAG_ELASTODYNAMICS) ? c->e0t0 : NULL;
-    case GF_FIELD_EIGENSTRAIN_T0_OFF: return (c->flags & GF_FLAG_ELASTODYNAMICS) ? c->e0t0 + 3 : NULL;
+    case GF_FIELD_EIGENSTRAIN_T0_DIAG: return (c->flags & (GF_FLAG_ELASTODYNAMICS | GF_FLAG_ELASTOSTATIC)) ? c->e0t0 : NULL;
+    case GF_FIELD_EIGENSTRAIN_T0_OFF: return (c->flags & (GF_FLAG_ELASTODYNAMICS | GF_FLAG_ELASTOSTATIC)) ? c->e0t0 + 3 : NULL;
     case GF_FIELD_STRAIN_DIAG: return c->e_glb;
     case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
     case GF_FIELD_HDRIVE: return (c->flags & GF_FLAG_HDRIVE) ? c->Hdrive : NULL;
@@ -1429,11 +1429,19 @@ void gfo_fe_get(gfo_ctx* c, int stage) {
         glb2crt(c, pgx, pgy, pgz, &px, &py, &pz);
         double exx = 0., eyy = 0., ezz = 0., eyz = 0., exz = 0., exy = 0.;
         double exx0 = 0, eyy0 = 0, ezz0 = 0, eyz0 = 0, exz0 = 0, exy0 = 0;
-        if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* ferroelectric_solver.cpp:624-653 (no elastostatics) */
+        if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* ferroelectric_solver.cpp:624-653 */
             exx = c->e_crt[0][id]; eyy = c->e_crt[1][id]; ezz = c->e_crt[2][id];
             eyz = c->e_crt[3][id]; exz = c->e_crt[4][id]; exy = c->e_crt[5][id];
             exx0 = c->e0_crt[0][id]; eyy0 = c->e0_crt[1][id]; ezz0 = c->e0_crt[2][id];
             eyz0 = c->e0_crt[3][id]; exz0 = c->e0_crt[4][id]; exy0 = c->e0_crt[5][id];
+            if (c->flags & GF_FLAG_ELASTOSTATIC) {
+                /* :636-650: the static strain and eigenstrain added (the
+                 * dynamic ones are zero without elastodynamics) */
+                exx = exx + c->e_t0[0][id]; eyy = eyy + c->e_t0[1][id]; ezz = ezz + c->e_t0[2][id];
+                eyz = eyz + c->e_t0[3][id]; exz = exz + c->e_t0[4][id]; exy = exy + c->e_t0[5][id];
+                exx0 = exx0 + c->e0t0[0][id]; eyy0 = eyy0 + c->e0t0[1][id]; ezz0 = ezz0 + c->e0t0[2][id];
+                eyz0 = eyz0 + c->e0t0[3][id]; exz0 = exz0 + c->e0t0[4][id]; exy0 = exy0 + c->e0t0[5][id];
+            }
         }
         double a1 = mat->a1, a11 = mat->a11, a12 = mat->a12, a111 = mat->a111, a112 = mat->a112, a123 = mat->a123;
         double a1111 = mat->a1111, a1112 = mat->a1112, a1122 = mat->a1122, a1123 = mat->a1123;
@@ -2573,7 +2581,7 @@ void gfo_estatic(gfo_ctx* c) {
     free(Pk); free(Ek);
 }
 
-/* Static strain of the magnets' eigenstrain (GF_FLAG_ELASTOSTATIC;
+/* Static strain of the eigenstrain of the magnets and ferroelectrics (GF_FLAG_ELASTOSTATIC;
  * get_strain_static is declared at elastic_system.h:190 without a
  * definition).  Periodic body of the homogeneous cubic stiffness
  * (elastic_initialize.cpp:22-40), crystal frame.  With sigma0 = C e0 and
@@ -2625,26 +2633,32 @@ static void es_operator(const gfo_ctx* c, const double kg[3], const double sig[6
     }
 }
 
-/* The magnets' eigenstrain in the crystal frame (elastic_dynamic_solver.cpp:
- * 21-73 without the ferroelectric terms), Voigt order xx yy zz yz xz xy. */
+/* The eigenstrain of the magnets and ferroelectrics in the crystal frame
+ * (elastic_dynamic_solver.cpp:21-73 in its sum order, without the flexo
+ * terms), Voigt order xx yy zz yz xz xy. */
 static void es_eigen(const gfo_ctx* c, int64_t id, double e[6]) {
     const gf_material* mat = matof(c, id);
     for (int a = 0; a < 6; a++) e[a] = 0.;
     if (!mat) return;
-    double mx = 0, my = 0, mz = 0, ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    double mx = 0, my = 0, mz = 0, ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0, px = 0, py = 0, pz = 0;
     if (c->if_FM_all) glb2crt(c, c->ms[0][id], c->ms[1][id], c->ms[2][id], &mx, &my, &mz);
     if (c->if_AFM_all) {
         glb2crt(c, c->ms1[0][id], c->ms1[1][id], c->ms1[2][id], &ax, &ay, &az);
         glb2crt(c, c->ms2[0][id], c->ms2[1][id], c->ms2[2][id], &bx, &by, &bz);
     }
+    if (c->if_FE_all) glb2crt(c, c->ps[0][id], c->ps[1][id], c->ps[2][id], &px, &py, &pz);
     double l100 = mat->lamda100, l111 = mat->lamda111, l100a = mat->lamda100_AFM1, l111a = mat->lamda111_AFM1;
     double l100b = mat->lamda100_AFM2, l111b = mat->lamda111_AFM2;
-    e[0] = 1.5 * l100 * (mx * mx - 1. / 3.) + 0.5 * 1.5 * l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * l100b * (bx * bx - 1. / 3.);
-    e[1] = 1.5 * l100 * (my * my - 1. / 3.) + 0.5 * 1.5 * l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * l100b * (by * by - 1. / 3.);
-    e[2] = 1.5 * l100 * (mz * mz - 1. / 3.) + 0.5 * 1.5 * l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * l100b * (bz * bz - 1. / 3.);
-    e[3] = 1.5 * l111 * my * mz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz;
-    e[4] = 1.5 * l111 * mx * mz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz;
-    e[5] = 1.5 * l111 * my * mx + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx;
+    double Q11 = mat->Q11, Q12 = mat->Q12, Q44 = mat->Q44;
+    e[0] = 1.5 * l100 * (mx * mx - 1. / 3.) + Q11 * (px * px) + Q12 * (py * py + pz * pz)
+         + 0.5 * 1.5 * l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * l100b * (bx * bx - 1. / 3.);
+    e[1] = 1.5 * l100 * (my * my - 1. / 3.) + Q11 * (py * py) + Q12 * (px * px + pz * pz)
+         + 0.5 * 1.5 * l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * l100b * (by * by - 1. / 3.);
+    e[2] = 1.5 * l100 * (mz * mz - 1. / 3.) + Q11 * (pz * pz) + Q12 * (py * py + px * px)
+         + 0.5 * 1.5 * l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * l100b * (bz * bz - 1. / 3.);
+    e[3] = 1.5 * l111 * my * mz + Q44 * py * pz + 0.5 * 1.5 * l111a * ay * az + 0.5 * 1.5 * l111b * by * bz;
+    e[4] = 1.5 * l111 * mx * mz + Q44 * px * pz + 0.5 * 1.5 * l111a * ax * az + 0.5 * 1.5 * l111b * bx * bz;
+    e[5] = 1.5 * l111 * my * mx + Q44 * py * px + 0.5 * 1.5 * l111a * ay * ax + 0.5 * 1.5 * l111b * by * bx;
 }
 
 void gfo_elastostatic(gfo_ctx* c) {
@@ -2656,6 +2670,8 @@ void gfo_elastostatic(gfo_ctx* c) {
     for (int64_t id = 0; id < n; id++) {
         double e[6];
         es_eigen(c, id, e);
+        /* the static eigenstrain ex*0t0_crt, kept beside the static strain */
+        for (int a = 0; a < 6; a++) c->e0t0[a][id] = e[a];
         sg[0 * n + id] = c11 * e[0] + c12 * (e[1] + e[2]);
         sg[1 * n + id] = c11 * e[1] + c12 * (e[0] + e[2]);
         sg[2 * n + id] = c11 * e[2] + c12 * (e[0] + e[1]);

@@ -3,7 +3,9 @@ against the oracle (pinned by closed forms in test_oracle_elastostatic.py):
 the static strain of random magnetisations on mixed blocks (rotated
 crystal, two magnets and a non-magnetic material, an antiferromagnet), and
 magnets stepping with the static strain recomputed after every step and
-acting back through the magnetoelastic field.  1e-12 of the peak.
+acting back through the magnetoelastic field; a ferroelectric layer whose
+electrostrictive eigenstrain enters the solve and which reads the static
+strain and eigenstrain back, with its restart.  1e-12 of the peak.
 """
 import sys
 
@@ -107,6 +109,89 @@ def test_magnets_step_with_the_static_strain():
     assert rel(strain(g), strain(o)) <= 1e-12
     g.close()
     o.close()
+
+
+def fe_mixed(shape=(8, 8, 16), rotate=True):
+    """A ferroelectric layer (PbTiO3-like, electrostrictive), a magnet and a
+    substrate, random p and m, the strain acting on both."""
+    from goferro.workloads import pto_material
+    T = rot() if rotate else None
+    fe = pto_material(FE_mass=1e-16, FE_damping=4e-4)
+    fe.Q11, fe.Q12, fe.Q44 = 0.089, -0.026, 0.0675
+    fe.tv1, fe.tv2, fe.tv3 = 3.05e10, 6.4e8, 2.97e10
+    set_elastic(fe, 2.3e11, 1.0e11, 1.1e11, 7900.0, Tg2c=T)
+    mag = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.05, K1=-610.0, Aex=3.7e-12)
+    set_elastic(mag, 2.69e11, 1.08e11, 0.76e11, 5170.0, Tg2c=T)
+    set_magnetostriction(mag, 3.5e5, 2.0e5)
+    sub = set_elastic(Material.make(), 3.2e11, 1.0e11, 1.2e11, 5100.0, Tg2c=T)
+    cell = np.full(shape, 3, np.uint32)
+    cell[:, :, 4:10] = 1
+    cell[:, :, 10:] = 2
+    prob = Problem(shape=shape, spacing=(1e-9, 1e-9, 1e-9), materials=[fe, mag, sub], material_cell=cell, dt=2e-15,
+                   periodic=(True, True, False), Tg2c=T, Hext_stat=(0, 0, 5e4), Eext_stat=(0, 0, 1e6),
+                   flags=FLAG_ELASTOSTATIC | FLAG_ELASTO_ON_PM | (FLAG_ORIENT_ROTATE if rotate else 0))
+    rng = np.random.default_rng(41)
+    p = 0.05 * rng.standard_normal((3,) + shape)
+    p[2] += 0.6
+    p[:, cell != 1] = 0.0
+    q = 1e8 * rng.standard_normal((3,) + shape)
+    q[:, cell != 1] = 0.0
+    m = workloads.tilted_m(shape, 9, theta_max=0.5)
+    m[:, cell != 2] = 0.0
+    from goferro import FIELD_P, FIELD_Q
+    return prob, {FIELD_P: p, FIELD_Q: q, FIELD_M: m}
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_ferroelectric_static_strain_and_steps(rotate):
+    """The static strain and eigenstrain of a ferroelectric and a magnet, and
+    three gf_steps with both reading them back."""
+    from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF, FIELD_P, FIELD_Q
+    say("case elastostatic ferroelectric, rotate", rotate)
+    prob, f = fe_mixed(rotate=rotate)
+    g, o = pair(prob, f)
+    g.demag()
+    o.demag()
+    assert rel(strain(g), strain(o)) <= 1e-12
+    for fid in (FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF):
+        assert rel(g.download(fid), o.get(fid)) <= 1e-12, fid
+    g.step(3)
+    o.step(3)
+    for fid in (FIELD_P, FIELD_Q):
+        d = rel(g.download(fid), o.get(fid))
+        say("  rel", fid, d)
+        assert d <= 1e-12, fid
+    assert np.abs(g.download(FIELD_M) - o.get(FIELD_M)).max() <= 1e-12
+    assert rel(strain(g), strain(o)) <= 1e-12
+    g.close()
+    o.close()
+
+
+def test_ferroelectric_elastostatic_restart():
+    """The state image carries the static eigenstrain the ferroelectric
+    reads: n1 + n2 steps equal n1, image, fresh context, n2, bit for bit."""
+    from goferro import FIELD_P, FIELD_Q
+    say("case elastostatic ferroelectric restart")
+    prob, f = fe_mixed(rotate=False)
+    a = Context(prob)
+    for fid, v in f.items():
+        a.upload(fid, v)
+    a.demag()
+    a.step(5)
+    want = a.download(FIELD_P), a.download(FIELD_Q)
+    a.close()
+    b = Context(prob)
+    for fid, v in f.items():
+        b.upload(fid, v)
+    b.demag()
+    b.step(2)
+    image = b.state_image()
+    b.close()
+    c = Context(prob)
+    c.restore_state(image)
+    c.step(3)
+    assert np.array_equal(c.download(FIELD_P), want[0]) and np.array_equal(c.download(FIELD_Q), want[1])
+    c.close()
 
 
 def test_elastostatic_rejections():

@@ -151,3 +151,114 @@ def test_homogeneous_stiffness_is_the_largest_scaled(oracle_lib):
     s = e0[0] + e0[1]
     want = e0[2] + 1.3e11 / 2.5e11 * (s - s.mean())
     np.testing.assert_allclose(e[2], want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+# ---- ferroelectrics: the electrostrictive eigenstrain e0_ii = Q11 p_i^2 +
+# Q12 (p_j^2 + p_k^2), e0_ij = Q44 p_i p_j (elastic_dynamic_solver.cpp:
+# 39-65, textbook form restated in numpy), and the ferroelectric reading the
+# static strain and eigenstrain (ferroelectric_solver.cpp:636-650).
+
+Q11, Q12, Q44 = 0.089, -0.026, 0.0675
+TV1, TV2, TV3 = 3.05e10, 6.4e8, 2.97e10
+
+
+def ferroelectric(c=(2.3e11, 1.0e11, 1.1e11)):
+    from goferro.workloads import pto_material
+    f = pto_material(FE_mass=1e-16, FE_damping=4e-4)
+    f.Q11, f.Q12, f.Q44 = Q11, Q12, Q44
+    f.tv1, f.tv2, f.tv3 = TV1, TV2, TV3
+    return set_elastic(f, c[0], c[1], c[2], 7900.0)
+
+
+def fe_eigen(p):
+    e = np.empty((6,) + p.shape[1:])
+    for a in range(3):
+        b, c = [x for x in range(3) if x != a]
+        e[a] = Q11 * p[a] ** 2 + Q12 * (p[b] ** 2 + p[c] ** 2)
+    e[3] = Q44 * p[1] * p[2]
+    e[4] = Q44 * p[0] * p[2]
+    e[5] = Q44 * p[0] * p[1]
+    return e
+
+
+def fe_profile(shape):
+    z = np.arange(shape[2])
+    p = np.zeros((3,) + shape)
+    p[0] = (0.1 * np.sin(2 * np.pi * z / shape[2]))[None, None, :]
+    p[1] = (-0.05 + 0.08 * np.cos(4 * np.pi * z / shape[2]))[None, None, :]
+    p[2] = (0.5 + 0.2 * np.sin(2 * np.pi * z / shape[2] + 0.3))[None, None, :]
+    return p
+
+
+def fe_problem(shape, flags=0):
+    from goferro import FLAG_ELASTO_ON_PM
+    cell = np.ones(shape, np.uint32)
+    return Problem(shape=shape, spacing=(1e-9, 1e-9, 1e-9), materials=[ferroelectric()], material_cell=cell,
+                   dt=2e-15, periodic=(True, True, True), flags=FLAG_ELASTOSTATIC | flags)
+
+
+def test_ferroelectric_uniform_and_profile(oracle_lib):
+    """Uniform p: the strain and the kept static eigenstrain are the
+    eigenstrain; p along z: the laterally periodic closed form, with the
+    component-wise largest stiffness (here the one material's)."""
+    from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF, FIELD_P
+    c11, c12 = 2.3e11, 1.0e11
+    for uniform in (True, False):
+        shape = (2, 2, 16)
+        p = fe_profile(shape)
+        if uniform:
+            p[:] = p[:, :, :, :1]
+        o = oracle_lib.Oracle(fe_problem(shape))
+        o.set(FIELD_P, p)
+        o.demag()
+        e = np.concatenate([o.get(FIELD_STRAIN_T0_DIAG), o.get(FIELD_STRAIN_T0_OFF)])
+        e0k = np.concatenate([o.get(FIELD_EIGENSTRAIN_T0_DIAG), o.get(FIELD_EIGENSTRAIN_T0_OFF)])
+        o.close()
+        e0 = fe_eigen(p)
+        np.testing.assert_allclose(e0k, e0, rtol=0, atol=1e-15 * np.abs(e0).max())
+        if uniform:
+            want = e0
+        else:
+            want = np.empty_like(e0)
+            for a in (0, 1, 5):
+                want[a] = e0[a].mean()
+            want[3], want[4] = e0[3], e0[4]
+            s = e0[0] + e0[1]
+            want[2] = e0[2] + c12 / c11 * (s - s.mean())
+        np.testing.assert_allclose(e, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_ferroelectric_reads_static_strain_and_eigenstrain(oracle_lib):
+    """The stage-1 q increment with the coupling on, less the one without,
+    is -dt/m [(tv1 (e0xx - exx) + tv2 (e0yy - eyy + e0zz - ezz)) px
+    + tv3 ((e0xy - exy) py + (e0xz - exz) pz)] and its y, z analogues, with
+    the closed-form static strain of a p profile along z: a wrong array, sign
+    or frame in the wiring breaks it."""
+    from goferro import FIELD_P, FLAG_ELASTO_ON_PM
+    shape = (2, 2, 16)
+    p = fe_profile(shape)
+    c11, c12 = 2.3e11, 1.0e11
+    kq = []
+    for flags in (FLAG_ELASTO_ON_PM, 0):
+        o = oracle_lib.Oracle(fe_problem(shape, flags))
+        o.set(FIELD_P, p)
+        o.demag()
+        o.L.gfo_fe_get(o._h, 1)
+        kq.append(o.get(106))   # the stage-1 q increment
+        o.close()
+    e0 = fe_eigen(p)
+    e = np.empty_like(e0)
+    for a in (0, 1, 5):
+        e[a] = e0[a].mean()
+    e[3], e[4] = e0[3], e0[4]
+    s = e0[0] + e0[1]
+    e[2] = e0[2] + c12 / c11 * (s - s.mean())
+    d = e0 - e     # e0 - e, Voigt
+    pre = -2e-15 / 1e-16
+    want = np.empty((3,) + shape)
+    want[0] = pre * ((TV1 * d[0] + TV2 * (d[1] + d[2])) * p[0] + TV3 * (d[5] * p[1] + d[4] * p[2]))
+    want[1] = pre * ((TV1 * d[1] + TV2 * (d[0] + d[2])) * p[1] + TV3 * (d[5] * p[0] + d[3] * p[2]))
+    want[2] = pre * ((TV1 * d[2] + TV2 * (d[0] + d[1])) * p[2] + TV3 * (d[4] * p[0] + d[3] * p[1]))
+    got = kq[0] - kq[1]
+    assert np.abs(want).max() > 1e-3 * np.abs(kq[1]).max()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * np.abs(kq[1]).max())
