@@ -901,6 +901,21 @@ gf_status gf_create(const gf_desc* d, gf_ctx** out) {
     if (!st)
         st = hip_check(c, hipMemcpy(c->d_code, c->host_code.data(), sizeof(uint32_t) * c->n, hipMemcpyHostToDevice),
                        "copy code");
+    if (!st && c->any_fm && c->nfm < c->n && c->n <= INT64_C(0xffffffff)) {
+        // the FM cells in index order: the unfused FM stage runs over them
+        // alone (material kinds cannot change after set-up)
+        std::vector<uint32_t> ids;
+        ids.reserve(static_cast<size_t>(c->nfm));
+        for (int64_t i = 0; i < c->n; i++)
+            if (c->host_code[static_cast<size_t>(i)] & kIsFM) ids.push_back(static_cast<uint32_t>(i));
+        c->n_fm_ids = static_cast<int64_t>(ids.size());
+        if (!ids.empty()) {
+            st = hip_check(c, hipMalloc(&c->d_fm_ids, sizeof(uint32_t) * ids.size()), "hipMalloc FM cell list");
+            if (!st)
+                st = hip_check(c, hipMemcpy(c->d_fm_ids, ids.data(), sizeof(uint32_t) * ids.size(), hipMemcpyHostToDevice),
+                               "copy FM cell list");
+        }
+    }
     if (!st) {
         std::vector<DevMaterial> dm;
         for (const auto& m : c->host_mats) dm.push_back(make_dev_material(m, c->dt));
@@ -1048,6 +1063,7 @@ gf_status gf_destroy(gf_ctx* c) {
     if (c->jf_spatial) hipFree(c->jf_spatial);
     if (c->red_partial) hipFree(c->red_partial);
     if (c->d_code) hipFree(c->d_code);
+    if (c->d_fm_ids) hipFree(c->d_fm_ids);
     if (c->d_mats) hipFree(c->d_mats);
     adjoint_free(c);
     adjoint32_free(c);
@@ -1633,7 +1649,7 @@ gf_status gf_mag_stage(gf_ctx* c, int stage) {
         st = hip_check(c, launch_mag_stage_fused(a, f, c->demag.h_in_spec, c->stream), "mag stage");
         if (!st) c->demag.m_in_spec = true;
     } else {
-        st = hip_check(c, launch_mag_stage(a, c->n, c->stream), "mag stage");
+        st = hip_check(c, launch_mag_stage(a, c->n, c->stream, c->d_fm_ids, c->n_fm_ids), "mag stage");
     }
     timer_end(c, GF_K_MAG_STAGE, ev);
     if (st) return st;

@@ -563,6 +563,8 @@ struct gf_ctx {
     bool has_e0t0 = false;          // the static eigenstrain was written: the elastic stage subtracts it
 
     uint32_t* d_code = nullptr;
+    uint32_t* d_fm_ids = nullptr;   // FM cells in index order, when not every cell is FM
+    int64_t n_fm_ids = 0;
     gf::DevMaterial* d_mats = nullptr;
 
     // FM
@@ -746,7 +748,9 @@ std::string stage_io_conflict(const FeStageArgs& a);
 bool narrow(int64_t n);
 
 // launchers (gf_llg.hip)
-hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s);
+// ids: when set, the count FM cells the stage runs over (else all n cells)
+hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s, const uint32_t* ids = nullptr,
+                            int64_t count = 0);
 hipError_t launch_afm_stage(const AfmStageArgs& a, int64_t n, hipStream_t s);
 hipError_t launch_mag_batch(const StageArgs* items, const BatchLaunch& L, int stage, hipStream_t s);
 hipError_t launch_afm_batch(const AfmStageArgs* items, const BatchLaunch& L, int stage, hipStream_t s);

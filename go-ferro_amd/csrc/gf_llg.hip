@@ -188,6 +188,14 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
     fm_point<I>(a, id);
 }
 
+// The same over a list of the FM cells: a layered magnet fills whole
+// waves instead of loading the inputs of every non-magnetic cell first.
+__global__ void __launch_bounds__(256) mag_stage_list_kernel(StageArgs a, const uint32_t* ids, uint32_t count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    fm_point<uint32_t>(a, ldu(ids, t));
+}
+
 // The FM stage fused with the z transforms of the FFT magnetostatics, for
 // contexts whose magnetisation is FM only.  A workgroup owns P whole z
 // columns (fixed x, consecutive y): it rebuilds their static field from
@@ -563,7 +571,13 @@ __global__ void __launch_bounds__(256) sum3_kernel(const double* f0, const doubl
 // 32-bit element offsets when every cell array holds fewer than 2^28 doubles.
 bool narrow(int64_t n) { return n < (int64_t{1} << 28); }
 
-hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s) {
+hipError_t launch_mag_stage(const StageArgs& a, int64_t n, hipStream_t s, const uint32_t* ids, int64_t count) {
+    if (ids) {
+        if (count > 0)
+            hipLaunchKernelGGL(mag_stage_list_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, s,
+                               a, ids, static_cast<uint32_t>(count));
+        return hipGetLastError();
+    }
     const int64_t blocks = (n + 255) / 256;
     if (narrow(n)) hipLaunchKernelGGL(mag_stage_kernel<uint32_t>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(mag_stage_kernel<int64_t>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
