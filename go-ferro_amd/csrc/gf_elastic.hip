@@ -266,10 +266,14 @@ __global__ void __launch_bounds__(256) el_prep_kernel(ElStageArgs a) {
     int64_t i, j, k;
     cell_ijk(id, a.s.ny, a.s.nz, i, j, k);
     double g[6][3], e[6], e0[6] = {0., 0., 0., 0., 0., 0.};
+    const uint32_t code = a.s.code[id];
     cell_grad(a.s, id, i, j, k, g);
     if (kGeneral && a.s.gyz[0]) grad_out(a.s, i, j, k, g);
     strain_of(g, e);
-    strain_out(a.s, id, e);
+    // the strain array is read by the magnets and ferroelectrics at their own
+    // cells (and, for the flexoelectric force, around them: the general form);
+    // a download recomputes it everywhere
+    if (kGeneral || (code & (kIsFM | kIsAFM | kIsFE))) strain_out(a.s, id, e);
     if (a.eig) {
         double sc[6];
         eigenstrain<kGeneral>(a.e, id, sc, e0);
@@ -279,7 +283,6 @@ __global__ void __launch_bounds__(256) el_prep_kernel(ElStageArgs a) {
                 for (int c = 0; c < 6; c++) a.e.e0c[c][id] = sc[c];
         }
     }
-    const uint32_t code = a.s.code[id];
     double P[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
     if (code & kMatMask) {
         const double(*C)[6] = stiffness(ctab, code);
@@ -365,7 +368,8 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElStageArgs a) {
         const DevMaterial& M = f.mats[(code & kMatMask) - 1];
         for (int c = 0; c < 3; c++) {
             const double fo = d[0][c] * f.idx + d[1][c] * f.idy + d[2][c] * f.idz;
-            const double fs = f.fstore[c][id];
+            // the stored force only matters with stiffness damping
+            const double fs = M.el_stiff != 0. ? f.fstore[c][id] : 0.;
             if (stage == 2 || stage == 3)
                 dv[c] = fo * M.el_rho + M.el_mass * f.v[c][id] + M.el_stiff * (fo - fs) * 2.;
             else
