@@ -8,10 +8,13 @@
 // alpha P.  Its vector-Jacobian product for a cotangent kbar is
 //   qbar = Q kbar,  pbar = P kbar + qbar x y,
 //   ybar = p x qbar + H x pbar + (dHa/dy)^T Hbar + (dHme/dy)^T Hbar
-//          + exch S(Hbar) + Ms N^T (*) Hbar,       Hbar = pbar x y,
-// where S is symmetric (each face carries one harmonic-mean coefficient)
-// and N^T (*) is the convolution with the transposed, conjugated tensor
-// spectra.  Two kernels per stage: a pointwise one (cross products, local
+//          + S(exch Hbar) + Ms N^T (*) Hbar,       Hbar = pbar x y,
+// where S is symmetric (each face carries one harmonic-mean coefficient),
+// exch = 2/(mu0 Ms) of each cell's material, and N^T (*) is the convolution
+// with the transposed, conjugated tensor spectra.  The parameter gradients
+// are those of one material: alpha and B1 at its cells, Aex through the
+// coefficient of each face it takes part in (its own faces and, through the
+// harmonic mean, its interfaces with other ferromagnets).  Two kernels per stage: a pointwise one (cross products, local
 // Jacobians, parameter gradients) writing ybar_local and Hbar, and after the
 // adjoint convolution of Hbar a stencil one that adds exch S(Hbar) and
 // Ms N^T (*) Hbar and does the RK4 bookkeeping
@@ -79,8 +82,21 @@ struct AdjPointArgs {
     double Tg2c[3][3];
     double Tc2g[3][3];
     int32_t rotate;
-    double alpha, aex;             // the material's damping and exchange constant
+    int32_t mat;                   // the material whose alpha, Aex, B1 are differentiated (1-based)
+    double alpha;                  // its damping
 };
+
+// d(face coefficient)/dA_mat of the face between a cell of material mi and
+// a neighbour of material mj: the coefficient is A_self (= A) within a
+// material and the harmonic mean 2 Ai Aj / (Ai + Aj) across an interface
+// (magnetic_initialize.cpp:381-524).
+__device__ __forceinline__ double dcoef_dA(const DevMaterial* mats, uint32_t mi, uint32_t mj, uint32_t mat) {
+    if (mi == mj) return mi == mat ? 1. : 0.;
+    if (mi != mat && mj != mat) return 0.;
+    const double Ai = mats[mi - 1].Aex, Aj = mats[mj - 1].Aex;
+    const double other = mi == mat ? Aj : Ai;
+    return 2. * other * other / ((Ai + Aj) * (Ai + Aj));
+}
 
 __device__ __forceinline__ void cross(double ax, double ay, double az, double bx, double by, double bz, double& cx,
                                       double& cy, double& cz) {
@@ -116,9 +132,26 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
     double mx = mgx, my = mgy, mz = mgz;
     if (a.rotate) rot(a.Tg2c, mgx, mgy, mgz, mx, my, mz);
     const P3 sp = {{a.y[0], a.y[1], a.y[2]}};
-    const Stencil st = stencil<false, I>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, static_cast<I>(a.nx),
-                                         ny, nz, a.dx, a.dy, a.dz, M.Aex, M.Aex_self, 0.);
+    double nv[6][3];
+    uint32_t ncd[6];
+    gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, nv, ncd);
+    const Stencil st = stencil_core<false>(mgx, mgy, mgz, nv, ncd, a.mats, own, kFmSurfShift, -1, a.dx, a.dy, a.dz,
+                                           M.Aex, M.Aex_self, 0.);
     const double hex0 = M.exch * st.ex[0], hex1 = M.exch * st.ex[1], hex2 = M.exch * st.ex[2];
+    // the exchange field's derivative in Aex of material a.mat: each
+    // non-surface face's difference weighted by d(coefficient)/dAex
+    double exd0 = 0., exd1 = 0., exd2 = 0.;
+    {
+        const double ih2[3] = {1. / (a.dx * a.dx), 1. / (a.dy * a.dy), 1. / (a.dz * a.dz)};
+        const uint32_t mi = own & kMatMask;
+        for (int f = 0; f < 6; f++) {
+            if ((own >> (kFmSurfShift + f)) & 1u) continue;
+            const double w = dcoef_dA(a.mats, mi, ncd[f] & kMatMask, static_cast<uint32_t>(a.mat)) * ih2[f >> 1];
+            exd0 += w * (nv[f][0] - mgx);
+            exd1 += w * (nv[f][1] - mgy);
+            exd2 += w * (nv[f][2] - mgz);
+        }
+    }
 
     // anisotropy field and Jacobian in the crystal frame (symmetric)
     double hac0 = 0., hac1 = 0., hac2 = 0.;
@@ -188,12 +221,18 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
     if (a.rotate) rot_t(a.Tg2c, mc0, mc1, mc2, mg0, mg1, mg2);
     mb0 += mg0; mb1 += mg1; mb2 += mg2;
 
-    // parameter gradients: P, Q in alpha; exchange linear in Aex; Hme linear in B1
-    const double dP = -2. * a.alpha * P / (1. + a.alpha * a.alpha);
-    const double dQ = P + a.alpha * dP;
-    std_(a.galpha, id, ldd(a.galpha, id) + dP * (px * kb0 + py * kb1 + pz * kb2) + dQ * (qx * kb0 + qy * kb1 + qz * kb2));
-    std_(a.gaex, id, ldd(a.gaex, id) + (hb0 * hex0 + hb1 * hex1 + hb2 * hex2) / a.aex);
-    if (a.strain[0])
+    // parameter gradients of material a.mat: P, Q in alpha and Hme linear in
+    // B1 at its own cells; the exchange field through the face coefficients
+    // (its own faces and its interfaces)
+    const bool mine = (own & kMatMask) == static_cast<uint32_t>(a.mat);
+    if (mine) {
+        const double dP = -2. * a.alpha * P / (1. + a.alpha * a.alpha);
+        const double dQ = P + a.alpha * dP;
+        std_(a.galpha, id,
+             ldd(a.galpha, id) + dP * (px * kb0 + py * kb1 + pz * kb2) + dQ * (qx * kb0 + qy * kb1 + qz * kb2));
+    }
+    std_(a.gaex, id, ldd(a.gaex, id) + M.exch * (hb0 * exd0 + hb1 * exd1 + hb2 * exd2));
+    if (mine && a.strain[0])
         std_(a.gb1, id, ldd(a.gb1, id) + -2. / kMu0 / M.Ms * (hc0 * exx * mx + hc1 * eyy * my + hc2 * ezz * mz));
     if (a.ghdrive[0]) {
         std_(a.ghdrive[0], id, ldd(a.ghdrive[0], id) + a.drive_s * hb0);
@@ -235,13 +274,25 @@ __global__ void __launch_bounds__(256) adj_combine_kernel(AdjCombineArgs a) {
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
+    // the exchange part of the transposed Jacobian: the field at cell i is
+    // exch_i sum_j A_ij (y_j - y_i)/h^2, so cell j receives
+    // sum_i A_ij (exch_i Hbar_i - exch_j Hbar_j)/h^2, the stencil of
+    // exch Hbar (A_ij symmetric; a surface face carries nothing)
     const P3 sp = {{a.hbar[0], a.hbar[1], a.hbar[2]}};
-    const Stencil st = stencil<false, I>(sp, a.code, a.mats, own, kFmSurfShift, -1, x, y, z, static_cast<I>(a.nx),
-                                         ny, nz, a.dx, a.dy, a.dz, M.Aex, M.Aex_self, 0.);
+    double nv[6][3];
+    uint32_t ncd[6];
+    gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, nv, ncd);
+    for (int f = 0; f < 6; f++) {
+        const double e = (ncd[f] & kIsFM) ? a.mats[(ncd[f] & kMatMask) - 1].exch : 0.;
+        for (int c = 0; c < 3; c++) nv[f][c] = e * nv[f][c];
+    }
+    const Stencil st = stencil_core<false>(M.exch * ldd(a.hbar[0], id), M.exch * ldd(a.hbar[1], id),
+                                           M.exch * ldd(a.hbar[2], id), nv, ncd, a.mats, own, kFmSurfShift, -1, a.dx,
+                                           a.dy, a.dz, M.Aex, M.Aex_self, 0.);
     const bool probe = x >= static_cast<I>(a.lo[0]) && x <= static_cast<I>(a.hi[0]) && y >= static_cast<I>(a.lo[1]) &&
                        y <= static_cast<I>(a.hi[1]) && z >= static_cast<I>(a.lo[2]) && z <= static_cast<I>(a.hi[2]);
     for (int c = 0; c < 3; c++) {
-        double yb = ldd(a.mbar[c], id) + M.exch * st.ex[c];
+        double yb = ldd(a.mbar[c], id) + st.ex[c];
         if (a.dadj[0]) yb += M.Ms * ldd(a.dadj[c], id);
         if (a.mode == 0) {
             std_(a.ybar[c], id, yb);
@@ -321,11 +372,9 @@ gf_status check_context(gf_ctx* c, int material) {
                          "the adjoint sweep does not support the elastostatic solver (static strain from m)");
     if (material < 1 || material > static_cast<int>(c->host_mats.size()) || !c->host_mats[material - 1].if_FM)
         return set_error(c, GF_ERR_ARG, "material must name an FM material");
-    int fm_mats = 0;
-    for (const auto& m : c->host_mats) fm_mats += m.if_FM ? 1 : 0;
-    if (fm_mats != 1) return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep supports one FM material");
+    for (const auto& m : c->host_mats)
+        if (m.if_FM && m.iDMI != 0.) return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep does not support DMI");
     const gf_material& m = c->host_mats[material - 1];
-    if (m.iDMI != 0.) return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep does not support DMI");
     if (m.Aex == 0.) return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep needs Aex != 0");
     if (c->mag_next_stage != 1) return set_error(c, GF_ERR_STATE, "the context is in the middle of a step");
     return GF_OK;
@@ -401,8 +450,8 @@ void stage_point_args(gf_ctx* c, int material, AdjPointArgs& p) {
     std::memcpy(p.Tc2g, c->Tc2g, sizeof(p.Tc2g));
     p.rotate = sa.rotate;
     const gf_material& m = c->host_mats[material - 1];
+    p.mat = material;
     p.alpha = m.FM_damping;
-    p.aex = m.Aex;
 }
 
 // Stage VJP on device buffers: y (store), h (its magnetostatic field or

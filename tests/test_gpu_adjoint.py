@@ -142,6 +142,105 @@ def test_stage_vjp_against_oracle_differences(nadv, rotate, demag):
     g.close()
 
 
+def two_material_problem(shape=(8, 8, 4), rotate=True, demag=True):
+    """Two ferromagnets (different Ms, Aex, damping, B1, anisotropy) in
+    stripes along x, a non-magnetic pocket, the same drives as problem()."""
+    prob = problem(shape=shape, rotate=rotate, demag=demag)
+    a = prob.materials[0]
+    b = Material.make(if_FM=1, anisotropy_type=2, Ms=8e5, gyro=GYRO, FM_damping=0.05, K1=4e4, K2=0.0, Aex=1.1e-11,
+                      B1=-6e5, B2=3e5)
+    vac = Material.make()
+    cell = np.ones(shape, np.uint32)
+    cell[shape[0] // 2:, :, :] = 2
+    cell[0, 0, 0] = 3
+    cell[shape[0] - 1, shape[1] - 1, 1] = 3
+    return dataclasses.replace(prob, materials=[a, b, vac], material_cell=cell)
+
+
+@pytest.mark.parametrize("material,rotate,demag", [(1, True, True), (2, False, True), (2, True, False)])
+def test_stage_vjp_two_materials(material, rotate, demag):
+    """The stage VJP with two ferromagnets: <ybar, v> against the oracle's
+    central differences, and the gradients of each material's alpha, Aex
+    (its own faces and the harmonic-mean interface) and B1."""
+    from oracle.oracle import FIELD_K1
+    say("case stage vjp two materials", material, rotate, demag)
+    prob = two_material_problem(rotate=rotate, demag=demag)
+    f = inputs(prob.shape)
+    magnet = np.asarray(prob.material_cell) < 3
+    f[FIELD_M] = np.where(magnet[None], f[FIELD_M], 0.0)
+    rng = np.random.default_rng(60 + material)
+    y = f[FIELD_M] + 0.05 * rng.standard_normal((3,) + prob.shape) * magnet[None]
+    kbar = rng.standard_normal((3,) + prob.shape)
+    v = rng.standard_normal((3,) + prob.shape) * magnet[None]
+    g = gpu(prob, f)
+    g.time_advance()
+    ybar, grad, _ = g.adjoint_stage(material, 1, y, kbar)
+
+    def k_of(yy, mats=None):
+        p = prob if mats is None else dataclasses.replace(prob, materials=mats)
+        o = oracle(p, {**f, FIELD_M: yy})
+        if demag:
+            o.demag()
+        o.time_advance()
+        o.L.gfo_mag_get_dm(o._h, 1)
+        k = o.get(FIELD_K1)
+        o.close()
+        return k
+
+    eps = 1e-4
+    jv = (k_of(y + eps * v) - k_of(y - eps * v)) / (2 * eps)
+    lhs, rhs = float((ybar * v).sum()), float((kbar * jv).sum())
+    say("  <ybar,v>", lhs, "<kbar,Jv>", rhs)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-7)
+    for idx, name in ((0, "FM_damping"), (1, "Aex"), (2, "B1")):
+        mats_up = [clone(m) for m in prob.materials]
+        mats_dn = [clone(m) for m in prob.materials]
+        base = getattr(prob.materials[material - 1], name)
+        h = 1e-4 * base
+        setattr(mats_up[material - 1], name, base + h)
+        setattr(mats_dn[material - 1], name, base - h)
+        fd = float((kbar * (k_of(y, mats_up) - k_of(y, mats_dn)) / (2 * h)).sum())
+        say("  ", name, "adjoint", grad[idx], "fd", fd)
+        np.testing.assert_allclose(grad[idx], fd, rtol=1e-6, err_msg=name)
+    g.close()
+
+
+def test_two_material_sweep_against_gpu_differences():
+    """The reverse sweep with two ferromagnets: each material's gradients
+    against central differences of the GPU forward loss, 20 steps."""
+    say("case adjoint two materials vs gpu finite differences")
+    prob = two_material_problem(shape=(16, 16, 8))
+    f = inputs(prob.shape)
+    magnet = np.asarray(prob.material_cell) < 3
+    f[FIELD_M] = np.where(magnet[None], f[FIELD_M], 0.0)
+    N = 20
+    lo, hi = probe(prob.shape)
+    g = gpu(prob, f)
+    for material in (1, 2):
+        g.upload(FIELD_M, f[FIELD_M])
+        g.time = 0.0
+        g.adjoint_forward(material, lo, hi, FREQ, N)
+        grad, _ = g.adjoint_backward()
+        base = clone(prob.materials[material - 1])
+
+        def loss(p):
+            g.set_material(material, p)
+            g.upload(FIELD_M, f[FIELD_M])
+            g.time = 0.0
+            return g.adjoint_forward(material, lo, hi, FREQ, N)
+
+        for idx, name in ((0, "FM_damping"), (1, "Aex"), (2, "B1")):
+            h = 1e-5 * getattr(base, name)
+            up, dn = clone(base), clone(base)
+            setattr(up, name, getattr(base, name) + h)
+            setattr(dn, name, getattr(base, name) - h)
+            fd = (loss(up) - loss(dn)) / (2 * h)
+            g.set_material(material, base)
+            say("  material", material, name, "adjoint", grad[idx], "fd", fd)
+            np.testing.assert_allclose(grad[idx], fd, rtol=1e-6)
+    g.close()
+
+
 def run_adjoint(prob, fields, nsteps, K=0):
     lo, hi = probe(prob.shape)
     g = gpu(prob, fields)
@@ -402,6 +501,11 @@ def test_fp32_rejections():
         g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=32)
     with pytest.raises(GoFerroError, match="precision"):
         g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=16)
+    g.close()
+    p2 = two_material_problem(shape=(4, 4, 4), rotate=False)
+    g = gpu(p2, {FIELD_M: f[FIELD_M], FIELD_HDRIVE: f[FIELD_HDRIVE]})
+    with pytest.raises(GoFerroError, match="one FM material"):
+        g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=32)
     g.close()
 
 
