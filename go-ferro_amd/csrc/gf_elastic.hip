@@ -143,37 +143,53 @@ __global__ void __launch_bounds__(256) el_strain_kernel(ElStrainArgs a) {
 // 4-104) less the static eigenstrain at t0 when there is one: s in the
 // crystal frame, o in the global frame, zero for vacuum.
 // kGeneral: with rotation and the flexoelectric terms; the lean form is
-// used only when the context has neither.
+// used only when the context has neither.  code is the cell's code; lc, when
+// set, the workgroup's LDS copy of every material's constants (kEigC each:
+// l100 l111 l100a l111a l100b l111b Q11 Q12 Q44).  The stores are read
+// before the code is known (every index is in bounds), so one memory round
+// trip feeds the cell.
+constexpr int kEigC = 9;
+
 template <bool kGeneral = true>
-__device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, int64_t id, double s[6], double o[6]) {
-    const uint32_t t = a.code[id] & kMatMask;
+__device__ __forceinline__ void eigenstrain(const ElEigenArgs& a, const double* lc, int64_t id, uint32_t code,
+                                            double s[6], double o[6]) {
+    const uint32_t t = code & kMatMask;
+    double v[4][3];
+    const double* const* src[4] = {a.p, a.m, a.m1, a.m2};
+    for (int f = 0; f < 4; f++) {
+        v[f][0] = 0.; v[f][1] = 0.; v[f][2] = 0.;
+        if (src[f][0]) { v[f][0] = src[f][0][id]; v[f][1] = src[f][1][id]; v[f][2] = src[f][2][id]; }
+    }
+    double t0[6] = {0., 0., 0., 0., 0., 0.};
+    if (a.e0t0[0])
+        for (int c = 0; c < 6; c++) t0[c] = a.e0t0[c][id];
     for (int c = 0; c < 6; c++) { s[c] = 0.; o[c] = 0.; }
     if (t) {
-        const DevMaterial& M = a.mats[t - 1];
-        double t0[6] = {0., 0., 0., 0., 0., 0.};
-        if (a.e0t0[0])
-            for (int c = 0; c < 6; c++) t0[c] = a.e0t0[c][id];
-        double v[4][3];
-        const double* const* src[4] = {a.p, a.m, a.m1, a.m2};
+        double C[kEigC];
+        if (lc) {
+            for (int q = 0; q < kEigC; q++) C[q] = lc[(t - 1) * kEigC + q];
+        } else {
+            const DevMaterial& M = a.mats[t - 1];
+            C[0] = M.l100; C[1] = M.l111; C[2] = M.l100a; C[3] = M.l111a; C[4] = M.l100b; C[5] = M.l111b;
+            C[6] = M.Q11; C[7] = M.Q12; C[8] = M.Q44;
+        }
         for (int f = 0; f < 4; f++) {
-            double x = 0., y = 0., z = 0.;
-            if (src[f][0]) { x = src[f][0][id]; y = src[f][1][id]; z = src[f][2][id]; }
+            const double x = v[f][0], y = v[f][1], z = v[f][2];
             if (kGeneral && a.rotate) rot_vec(a.Tg2c, x, y, z, v[f][0], v[f][1], v[f][2]);
-            else { v[f][0] = x; v[f][1] = y; v[f][2] = z; }
         }
         const double px = v[0][0], py = v[0][1], pz = v[0][2];
         const double mx = v[1][0], my = v[1][1], mz = v[1][2];
         const double ax = v[2][0], ay = v[2][1], az = v[2][2];
         const double bx = v[3][0], by = v[3][1], bz = v[3][2];
-        s[0] = 1.5 * M.l100 * (mx * mx - 1. / 3.) + M.Q11 * (px * px) + M.Q12 * (py * py + pz * pz) +
-               0.5 * 1.5 * M.l100a * (ax * ax - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bx * bx - 1. / 3.) - t0[0];
-        s[1] = 1.5 * M.l100 * (my * my - 1. / 3.) + M.Q11 * (py * py) + M.Q12 * (px * px + pz * pz) +
-               0.5 * 1.5 * M.l100a * (ay * ay - 1. / 3.) + 0.5 * 1.5 * M.l100b * (by * by - 1. / 3.) - t0[1];
-        s[2] = 1.5 * M.l100 * (mz * mz - 1. / 3.) + M.Q11 * (pz * pz) + M.Q12 * (py * py + px * px) +
-               0.5 * 1.5 * M.l100a * (az * az - 1. / 3.) + 0.5 * 1.5 * M.l100b * (bz * bz - 1. / 3.) - t0[2];
-        s[3] = 1.5 * M.l111 * my * mz + M.Q44 * py * pz + 0.5 * 1.5 * M.l111a * ay * az + 0.5 * 1.5 * M.l111b * by * bz - t0[3];
-        s[4] = 1.5 * M.l111 * mx * mz + M.Q44 * px * pz + 0.5 * 1.5 * M.l111a * ax * az + 0.5 * 1.5 * M.l111b * bx * bz - t0[4];
-        s[5] = 1.5 * M.l111 * my * mx + M.Q44 * py * px + 0.5 * 1.5 * M.l111a * ay * ax + 0.5 * 1.5 * M.l111b * by * bx - t0[5];
+        s[0] = 1.5 * C[0] * (mx * mx - 1. / 3.) + C[6] * (px * px) + C[7] * (py * py + pz * pz) +
+               0.5 * 1.5 * C[2] * (ax * ax - 1. / 3.) + 0.5 * 1.5 * C[4] * (bx * bx - 1. / 3.) - t0[0];
+        s[1] = 1.5 * C[0] * (my * my - 1. / 3.) + C[6] * (py * py) + C[7] * (px * px + pz * pz) +
+               0.5 * 1.5 * C[2] * (ay * ay - 1. / 3.) + 0.5 * 1.5 * C[4] * (by * by - 1. / 3.) - t0[1];
+        s[2] = 1.5 * C[0] * (mz * mz - 1. / 3.) + C[6] * (pz * pz) + C[7] * (py * py + px * px) +
+               0.5 * 1.5 * C[2] * (az * az - 1. / 3.) + 0.5 * 1.5 * C[4] * (bz * bz - 1. / 3.) - t0[2];
+        s[3] = 1.5 * C[1] * my * mz + C[8] * py * pz + 0.5 * 1.5 * C[3] * ay * az + 0.5 * 1.5 * C[5] * by * bz - t0[3];
+        s[4] = 1.5 * C[1] * mx * mz + C[8] * px * pz + 0.5 * 1.5 * C[3] * ax * az + 0.5 * 1.5 * C[5] * bx * bz - t0[4];
+        s[5] = 1.5 * C[1] * my * mx + C[8] * py * px + 0.5 * 1.5 * C[3] * ay * ax + 0.5 * 1.5 * C[5] * by * bx - t0[5];
         if (kGeneral && a.flexo) {
             // -F grad p (:82-89): the global-frame gradient of FE cells,
             // zero elsewhere, enters the crystal-frame tensor
@@ -248,8 +264,27 @@ __device__ __forceinline__ const double (*stiffness(const double* ctab, uint32_t
     return reinterpret_cast<const double (*)[6]>(ctab + ((code & kMatMask) - 1u) * 36u);
 }
 
+// The workgroup's LDS copy of every material's constants: the stiffness
+// matrices (36 each), then the eigenstrain constants (kEigC each), then
+// el_rho, el_mass, el_stiff (kElC each).
+constexpr int kElC = 3;
+constexpr int kLdsPerMat = 36 + kEigC + kElC;
+
 __device__ __forceinline__ void load_ctab(double* ctab, const DevMaterial* mats, int nmat) {
     for (int t = threadIdx.x; t < 36 * nmat; t += blockDim.x) ctab[t] = mats[t / 36].cij[(t % 36) / 6][t % 6];
+    double* ec = ctab + 36 * nmat;
+    for (int t = threadIdx.x; t < kEigC * nmat; t += blockDim.x) {
+        const DevMaterial& M = mats[t / kEigC];
+        const int q = t % kEigC;
+        ec[t] = q == 0 ? M.l100 : q == 1 ? M.l111 : q == 2 ? M.l100a : q == 3 ? M.l111a : q == 4 ? M.l100b
+              : q == 5 ? M.l111b : q == 6 ? M.Q11 : q == 7 ? M.Q12 : M.Q44;
+    }
+    double* lc = ec + kEigC * nmat;
+    for (int t = threadIdx.x; t < kElC * nmat; t += blockDim.x) {
+        const DevMaterial& M = mats[t / kElC];
+        const int q = t % kElC;
+        lc[t] = q == 0 ? M.el_rho : q == 1 ? M.el_mass : M.el_stiff;
+    }
     __syncthreads();
 }
 
@@ -259,14 +294,17 @@ __device__ __forceinline__ void load_ctab(double* ctab, const DevMaterial* mats,
 // flexoelectric terms (the face gradients are written out for the latter).
 template <bool kGeneral>
 __global__ void __launch_bounds__(256) el_prep_kernel(ElStageArgs a) {
-    extern __shared__ double ctab[];   // nmat stiffness matrices, 36 doubles each
+    extern __shared__ double ctab[];   // the materials' constants (load_ctab)
     load_ctab(ctab, a.e.mats, a.f.nmat);
     const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= a.s.nx * a.s.ny * a.s.nz) return;
     int64_t i, j, k;
     cell_ijk(id, a.s.ny, a.s.nz, i, j, k);
-    double g[6][3], e[6], e0[6] = {0., 0., 0., 0., 0., 0.};
+    double g[6][3], e[6], e0[6] = {0., 0., 0., 0., 0., 0.}, sc[6];
     const uint32_t code = a.s.code[id];
+    // the eigenstrain first: its loads then go out with the store's, before
+    // any of the launch's writes
+    if (a.eig) eigenstrain<kGeneral>(a.e, ctab + 36 * a.f.nmat, id, code, sc, e0);
     cell_grad(a.s, id, i, j, k, g);
     if (kGeneral && a.s.gyz[0]) grad_out(a.s, i, j, k, g);
     strain_of(g, e);
@@ -274,14 +312,10 @@ __global__ void __launch_bounds__(256) el_prep_kernel(ElStageArgs a) {
     // cells (and, for the flexoelectric force, around them: the general form);
     // a download recomputes it everywhere
     if (kGeneral || (code & (kIsFM | kIsAFM | kIsFE))) strain_out(a.s, id, e);
-    if (a.eig) {
-        double sc[6];
-        eigenstrain<kGeneral>(a.e, id, sc, e0);
-        if (a.e0_out) {
-            for (int c = 0; c < 6; c++) a.e.e0g[c][id] = e0[c];
-            if (a.e.e0c[0] != a.e.e0g[0])
-                for (int c = 0; c < 6; c++) a.e.e0c[c][id] = sc[c];
-        }
+    if (a.eig && a.e0_out) {
+        for (int c = 0; c < 6; c++) a.e.e0g[c][id] = e0[c];
+        if (a.e.e0c[0] != a.e.e0g[0])
+            for (int c = 0; c < 6; c++) a.e.e0c[c][id] = sc[c];
     }
     double P[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
     if (code & kMatMask) {
@@ -302,7 +336,7 @@ __global__ void __launch_bounds__(256) el_prep_kernel(ElStageArgs a) {
 // the u store the prep launch read) and the two stiffnesses; dv; the v and
 // u RK updates, the next u store into f.us.
 __global__ void __launch_bounds__(256) el_force_kernel(ElStageArgs a) {
-    extern __shared__ double ctab[];   // nmat stiffness matrices, 36 doubles each
+    extern __shared__ double ctab[];   // the materials' constants (load_ctab)
     const ElForceArgs& f = a.f;
     load_ctab(ctab, f.mats, f.nmat);
     const int64_t nx = f.nx, ny = f.ny, nz = f.nz;
@@ -365,15 +399,16 @@ __global__ void __launch_bounds__(256) el_force_kernel(ElStageArgs a) {
             }
             for (int c = 0; c < 3; c++) d[ax][c] = s2[c] - s1[c];
         }
-        const DevMaterial& M = f.mats[(code & kMatMask) - 1];
+        const double* lc = ctab + (36 + kEigC) * f.nmat + ((code & kMatMask) - 1) * kElC;
+        const double el_rho = lc[0], el_mass = lc[1], el_stiff = lc[2];
         for (int c = 0; c < 3; c++) {
             const double fo = d[0][c] * f.idx + d[1][c] * f.idy + d[2][c] * f.idz;
             // the stored force only matters with stiffness damping
-            const double fs = M.el_stiff != 0. ? f.fstore[c][id] : 0.;
+            const double fs = el_stiff != 0. ? f.fstore[c][id] : 0.;
             if (stage == 2 || stage == 3)
-                dv[c] = fo * M.el_rho + M.el_mass * f.v[c][id] + M.el_stiff * (fo - fs) * 2.;
+                dv[c] = fo * el_rho + el_mass * f.v[c][id] + el_stiff * (fo - fs) * 2.;
             else
-                dv[c] = fo * M.el_rho + M.el_mass * f.v[c][id] + M.el_stiff * (fo - fs);
+                dv[c] = fo * el_rho + el_mass * f.v[c][id] + el_stiff * (fo - fs);
             if (stage == 1) f.fstore[c][id] = fo;
         }
     }
@@ -482,8 +517,8 @@ hipError_t launch_el_stage(const ElStageArgs& a, hipStream_t s) {
     const bool general = a.s.rotate || a.e.rotate || a.e.flexo;
     const void* prep = general ? reinterpret_cast<const void*>(el_prep_kernel<true>)
                                : reinterpret_cast<const void*>(el_prep_kernel<false>);
-    const size_t lds = sizeof(double) * 36 * static_cast<size_t>(a.f.nmat > 0 ? a.f.nmat : 1);
-    if (lds > 65536) {   // up to 255 materials: 72 KiB of the CU's 160 KiB
+    const size_t lds = sizeof(double) * kLdsPerMat * static_cast<size_t>(a.f.nmat > 0 ? a.f.nmat : 1);
+    if (lds > 65536) {   // up to 255 materials: 96 KiB of the CU's 160 KiB
         hipError_t e = hipFuncSetAttribute(prep, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void*>(el_force_kernel),
