@@ -180,7 +180,8 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
         exx = ldd(a.strain[0], id); eyy = ldd(a.strain[1], id); ezz = ldd(a.strain[2], id);
         eyz = ldd(a.strain[3], id); exz = ldd(a.strain[4], id); exy = ldd(a.strain[5], id);
         double he0, he1, he2;
-        magnetoelastic(a.strain, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
+        const double* none[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        magnetoelastic(a.strain, none, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
         H0 += he0; H1 += he1; H2 += he2;
     }
     if (a.hnu[0]) { H0 += ldd(a.hnu[0], id); H1 += ldd(a.hnu[1], id); H2 += ldd(a.hnu[2], id); }

@@ -755,10 +755,14 @@ bool static_strain(const gf_ctx* c) {
     return (c->flags & GF_FLAG_ELASTO_ON_PM) && !(c->flags & GF_FLAG_ELASTODYNAMICS);
 }
 
-// Strain arrays the magnet stages read (crystal frame), or nulls.
-void magnet_strain(gf_ctx* c, const double* out[6]) {
-    for (int i = 0; i < 6; i++)
+// Strain arrays the magnet stages read (crystal frame), or nulls: the
+// dynamic or the static strain, and a static strain uploaded beside the
+// dynamic one (exx = exxt0_crt + Dexx_crt, magnetic_solver.cpp:89-96).
+void magnet_strain(gf_ctx* c, const double* out[6], const double* out0[6]) {
+    for (int i = 0; i < 6; i++) {
         out[i] = strain_coupled(c) ? c->ec[i] : (static_strain(c) ? c->e_t0[i] : nullptr);
+        out0[i] = strain_coupled(c) && c->has_et0_dyn ? c->e_t0[i] : nullptr;
+    }
 }
 
 // sin(2 pi f t) of the shaped drive at the stage time of the uniform drive
@@ -1145,11 +1149,14 @@ static double** field_for(gf_ctx* c, int id, bool for_write, gf_status* st) {
         if (!c->hdrive[0]) break;
         return c->hdrive;
     case GF_FIELD_STRAIN_T0_DIAG:
-        if (!c->e_t0[0]) break;
-        return c->e_t0;
     case GF_FIELD_STRAIN_T0_OFF:
+        if (!c->e_t0[0] && c->el_on && for_write) {
+            // a static strain beside the elastodynamics, on first upload
+            if ((*st = alloc3(c, c->e_t0, "hipMalloc static strain"))) return nullptr;
+            if ((*st = alloc3(c, c->e_t0 + 3, "hipMalloc static strain"))) return nullptr;
+        }
         if (!c->e_t0[0]) break;
-        return c->e_t0 + 3;
+        return id == GF_FIELD_STRAIN_T0_DIAG ? c->e_t0 : c->e_t0 + 3;
     case GF_FIELD_J_ISHE:
         if (!c->spin_on || for_write) break;
         return c->jis;
@@ -1309,6 +1316,7 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
     }
     if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
     if (id == GF_FIELD_EIGENSTRAIN_T0_DIAG || id == GF_FIELD_EIGENSTRAIN_T0_OFF) c->has_e0t0 = true;
+    if ((id == GF_FIELD_STRAIN_T0_DIAG || id == GF_FIELD_STRAIN_T0_OFF) && c->el_on) c->has_et0_dyn = true;
     if (id == GF_FIELD_M || id == GF_FIELD_M_STORE) c->demag.m_in_spec = false;
     if (id == GF_FIELD_HSTAT) { c->demag.h_in_spec = false; c->demag.h_real_valid = true; }
     // elastic_initialize.cpp:386-410: the stores start equal to u and v
@@ -1384,7 +1392,7 @@ static bool hstat_live(const gf_ctx* c) {
 
 StageArgs fm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     StageArgs a{};
-    magnet_strain(c, a.strain);
+    magnet_strain(c, a.strain, a.strain0);
     const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
     a.code = c->d_code; a.mats = c->d_mats;
     double** s = fm_store(c, in_sel);
@@ -1428,7 +1436,7 @@ void fm_stage_drives(gf_ctx* c, int stage, StageArgs& a) {
 // The same for the two AFM sublattices.
 AfmStageArgs afm_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     AfmStageArgs a{};
-    magnet_strain(c, a.strain);
+    magnet_strain(c, a.strain, a.strain0);
     const bool em_m = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_M);
     for (int i = 0; i < 3; i++) {
         a.hcell[i] = em_m ? c->Hcell[i] : nullptr;
@@ -2253,7 +2261,7 @@ struct StateArrayHeader {
 
 enum : uint32_t {
     kBitHstatWritten = 1u, kBitJpOn = 2u, kBitJpUploaded = 4u, kBitHasHnu = 8u,
-    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u, kBitHasE0t0 = 128u
+    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u, kBitHasE0t0 = 128u, kBitEt0Dyn = 256u
 };
 
 // FNV-1a over 64-bit words, then over the tail bytes.
@@ -2365,6 +2373,7 @@ uint32_t state_bits(const gf_ctx* c) {
     if (c->jp_uploaded) b |= kBitJpUploaded;
     if (c->has_hnu) b |= kBitHasHnu;
     if (c->has_e0t0) b |= kBitHasE0t0;
+    if (c->has_et0_dyn) b |= kBitEt0Dyn;
     if (c->demag.h_in_spec) b |= kBitHInSpec;
     if (c->demag.h_real_valid) b |= kBitHRealValid;
     if (c->demag.m_in_spec) b |= kBitMInSpec;
@@ -2487,6 +2496,13 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
         st = alloc3(c, c->hnu, "hipMalloc hext_nonunif");
         if (st) return st;
     }
+    if ((h.bits & kBitEt0Dyn) && !c->el_on)
+        return set_error(c, GF_ERR_ARG, "the state image holds a static strain beside elastodynamics this context lacks");
+    if ((h.bits & kBitEt0Dyn) && !c->e_t0[0]) {
+        st = alloc3(c, c->e_t0, "hipMalloc static strain");
+        if (!st) st = alloc3(c, c->e_t0 + 3, "hipMalloc static strain");
+        if (st) return st;
+    }
     if ((h.bits & kBitHasE0t0) && !c->el_on && !(c->flags & GF_FLAG_ELASTOSTATIC))
         return set_error(c, GF_ERR_ARG,
                          "the state image holds a static eigenstrain and this context has no elastodynamics or elastostatics");
@@ -2530,6 +2546,7 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
     c->jp_uploaded = h.bits & kBitJpUploaded;
     c->has_hnu = h.bits & kBitHasHnu;
     c->has_e0t0 = h.bits & kBitHasE0t0;
+    c->has_et0_dyn = h.bits & kBitEt0Dyn;
     c->demag.h_in_spec = h.bits & kBitHInSpec;
     c->demag.h_real_valid = h.bits & kBitHRealValid;
     c->demag.m_in_spec = h.bits & kBitMInSpec;

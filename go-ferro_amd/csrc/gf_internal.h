@@ -395,6 +395,7 @@ struct StageArgs {
     const double* hnu[3];       // may be null
     const double* hdrive[3];    // shaped drive profile, null when off
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
+    const double* strain0[6];   // static strain added to it (crystal frame), null when none
     const double* hcell[3];     // electrodynamic H at the cells, null when uncoupled
     double* dm_out[3];          // k of this stage, null unless Faraday's law needs it
     int64_t nx, ny, nz;
@@ -423,6 +424,7 @@ struct AfmStageArgs {
     double* out2[3];
     const double* hstat[3];
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
+    const double* strain0[6];   // static strain added to it (crystal frame), null when none
     const double* hcell[3];     // electrodynamic H at the cells, null when uncoupled
     double* dm_out1[3];         // k of this stage per sublattice, null unless needed
     double* dm_out2[3];
@@ -561,6 +563,7 @@ struct gf_ctx {
     int64_t nfm = 0, nafm = 0, nfe = 0;
     bool has_hnu = false;
     bool has_e0t0 = false;          // the static eigenstrain was written: the elastic stage subtracts it
+    bool has_et0_dyn = false;       // a static strain was uploaded beside the elastodynamics: the magnets add it
 
     uint32_t* d_code = nullptr;
     uint32_t* d_fm_ids = nullptr;   // FM cells in index order, when not every cell is FM

@@ -120,7 +120,7 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, MT* mats, const FmIn
     double H2 = hs2 + ha2;
     if (!kLean && a.strain[0]) {
         double he0, he1, he2;
-        magnetoelastic(a.strain, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
+        magnetoelastic(a.strain, a.strain0, id, -2., M.B1n, M.B2n, mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
         H0 = H0 + he0;
         H1 = H1 + he1;
         H2 = H2 + he2;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) 
     double H2 = hs2 + ha2;
     if (a.strain[0]) {
         double he0, he1, he2;
-        magnetoelastic(a.strain, id, -1., M.aB1n[sub], M.aB2n[sub], mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
+        magnetoelastic(a.strain, a.strain0, id, -1., M.aB1n[sub], M.aB2n[sub], mx, my, mz, a.rotate, a.Tc2g, he0, he1, he2);
         H0 = H0 + he0;
         H1 = H1 + he1;
         H2 = H2 + he2;

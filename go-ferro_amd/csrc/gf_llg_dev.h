@@ -200,12 +200,17 @@ __device__ __forceinline__ Stencil stencil(const P3 s3, const uint32_t* code, co
 // f (B1 e_aa m_a + B2 (e_ab m_b + e_ac m_c)) with f = -2 (FM) or -1 (per
 // AFM sublattice), B already divided by mu0 Ms (magnetic_solver.cpp:143-150,
 // magnetic_AFM_solver.cpp:156-162).  Strain is Voigt xx yy zz yz xz xy.
+// e0, when set, is a static strain added to e (exx = exxt0 + Dexx, the
+// uploaded static strain beside the elastodynamics).
 template <class I>
-__device__ __forceinline__ void magnetoelastic(const double* const* e, I id, double f, double B1, double B2,
-                                               double mx, double my, double mz, int rotate, const double T[3][3],
-                                               double& h0, double& h1, double& h2) {
-    const double exx = 0. + ldd(e[0], id), eyy = 0. + ldd(e[1], id), ezz = 0. + ldd(e[2], id);
-    const double eyz = 0. + ldd(e[3], id), exz = 0. + ldd(e[4], id), exy = 0. + ldd(e[5], id);
+__device__ __forceinline__ void magnetoelastic(const double* const* e, const double* const* e0, I id, double f,
+                                               double B1, double B2, double mx, double my, double mz, int rotate,
+                                               const double T[3][3], double& h0, double& h1, double& h2) {
+    double s[6] = {0., 0., 0., 0., 0., 0.};
+    if (e0[0])
+        for (int c = 0; c < 6; c++) s[c] = ldd(e0[c], id);
+    const double exx = s[0] + ldd(e[0], id), eyy = s[1] + ldd(e[1], id), ezz = s[2] + ldd(e[2], id);
+    const double eyz = s[3] + ldd(e[3], id), exz = s[4] + ldd(e[4], id), exy = s[5] + ldd(e[5], id);
     const double c0 = f * (B1 * (exx) * mx + B2 * ((exy) * my + (exz) * mz));
     const double c1 = f * (B1 * (eyy) * my + B2 * ((exy) * mx + (eyz) * mz));
     const double c2 = f * (B1 * (ezz) * mz + B2 * ((exz) * mx + (eyz) * my));

@@ -471,8 +471,6 @@ def _unsupported(g: dict, mats: list) -> list[str]:
         out.append("if_elastostatic with elastodynamics or the flexoelectric coupling")
     if any_fe and g["if_elec_1Dmodel"]:
         out.append("if_elec_1Dmodel (get_E_static_1D is declared but not defined in the reference)")
-    if g["if_input_straint0"] and (g["if_elastodynamics"] or g["if_OrientRotate"]):
-        out.append("if_input_straint0 with elastodynamics or a rotated crystal")
     return out
 
 
@@ -702,11 +700,34 @@ def load_inputs(ctx: Context, deck: Deck, directory=None) -> list[str]:
         e = cells("eigenstraint0_crt.in", 6)
         ctx.upload(FIELD_EIGENSTRAIN_T0_DIAG, e[:3])
         ctx.upload(FIELD_EIGENSTRAIN_T0_OFF, e[3:])
-    if g["if_input_straint0"] and g["if_elasto_on_pORm"] and not g["if_elastodynamics"]:
+    if g["if_input_straint0"] and g["if_elasto_on_pORm"]:
+        # input_straint0 (inoutput.cpp:238-257) reads the global-frame static
+        # strain; set-up turns it into the crystal frame the magnets read
+        # (elastic_initialize.cpp:430-460), beside the dynamic strain when
+        # there is one (magnetic_solver.cpp:89-96)
         e = cells("strain_stat.in", 6)
+        if deck.problem.flags & FLAG_ORIENT_ROTATE:
+            e = glb2crt_voigt(e, deck.problem.Tg2c)
         ctx.upload(FIELD_STRAIN_T0_DIAG, e[:3])
         ctx.upload(FIELD_STRAIN_T0_OFF, e[3:])
     return read
+
+
+def glb2crt_voigt(e, Tg2c) -> np.ndarray:
+    """transform_matrix_glb2crt (mathlib.cpp:78-100) of a Voigt field
+    (xx yy zz yz xz xy, leading axis): out_ij = sum_k sum_l T_ik T_jl in_kl,
+    summed k-major as there."""
+    T = np.asarray(Tg2c, dtype=np.float64)
+    e = np.asarray(e, dtype=np.float64)
+    full = [[e[0], e[5], e[4]], [e[5], e[1], e[3]], [e[4], e[3], e[2]]]
+    out = {}
+    for (i, j) in ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)):
+        acc = np.zeros_like(e[0])
+        for k in range(3):
+            for l in range(3):
+                acc = acc + T[i][k] * T[j][l] * full[k][l]
+        out[(i, j)] = acc
+    return np.stack([out[(0, 0)], out[(1, 1)], out[(2, 2)], out[(1, 2)], out[(0, 2)], out[(0, 1)]])
 
 
 # ---------------------------------------------------------------------------

@@ -279,8 +279,9 @@ enum gf_field_id {
     /* static strain in the crystal frame, exx eyy ezz | eyz exz exy, 3n each:
      * the reference's ex*t0_crt, which the magnets read under
      * if_elasto_on_pORm (magnetic_solver.cpp:89-96, magnetic_AFM_solver.cpp:
-     * 89-96).  Used with GF_FLAG_ELASTO_ON_PM when elastodynamics is off;
-     * zero until uploaded. */
+     * 89-96).  Used with GF_FLAG_ELASTO_ON_PM: alone when elastodynamics
+     * is off, added to the dynamic strain when it is on (exx = exxt0_crt +
+     * Dexx_crt); zero until uploaded. */
     GF_FIELD_STRAIN_T0_DIAG = 23,
     GF_FIELD_STRAIN_T0_OFF = 24,
     /* inverse spin Hall current Jx, Jy (and a zero z), 3n, read only

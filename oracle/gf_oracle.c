@@ -935,8 +935,10 @@ static double** field_ptrs(gfo_ctx* c, int id) {
     case GF_FIELD_STRAIN_DIAG: return c->e_glb;
     case GF_FIELD_STRAIN_OFF: return c->e_glb + 3;
     case GF_FIELD_HDRIVE: return (c->flags & GF_FLAG_HDRIVE) ? c->Hdrive : NULL;
-    case GF_FIELD_STRAIN_T0_DIAG: return (static_strain(c) || (c->flags & GF_FLAG_ELASTOSTATIC)) ? c->e_t0 : NULL;
-    case GF_FIELD_STRAIN_T0_OFF: return (static_strain(c) || (c->flags & GF_FLAG_ELASTOSTATIC)) ? c->e_t0 + 3 : NULL;
+    case GF_FIELD_STRAIN_T0_DIAG:
+        return (static_strain(c) || (c->flags & (GF_FLAG_ELASTOSTATIC | GF_FLAG_ELASTODYNAMICS))) ? c->e_t0 : NULL;
+    case GF_FIELD_STRAIN_T0_OFF:
+        return (static_strain(c) || (c->flags & (GF_FLAG_ELASTOSTATIC | GF_FLAG_ELASTODYNAMICS))) ? c->e_t0 + 3 : NULL;
     case GF_FIELD_E_CELL: return c->if_em ? c->Ecell : NULL;
     case GF_FIELD_H_CELL: return c->if_em ? c->Hcell : NULL;
     case GF_FIELD_J_ISHE: return c->spin_on ? c->Jis : NULL;
@@ -1151,9 +1153,9 @@ void gfo_mag_get_dm(gfo_ctx* c, int stage) {
         if (static_strain(c)) {                  /* :89-96, dynamic strain zero */
             exx = c->e_t0[0][i] + 0.; eyy = c->e_t0[1][i] + 0.; ezz = c->e_t0[2][i] + 0.;
             eyz = c->e_t0[3][i] + 0.; exz = c->e_t0[4][i] + 0.; exy = c->e_t0[5][i] + 0.;
-        } else if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* :89-96, static strain zero */
-            exx = 0. + c->e_crt[0][i]; eyy = 0. + c->e_crt[1][i]; ezz = 0. + c->e_crt[2][i];
-            eyz = 0. + c->e_crt[3][i]; exz = 0. + c->e_crt[4][i]; exy = 0. + c->e_crt[5][i];
+        } else if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* :89-96: the static strain (zero unless uploaded) plus the dynamic one */
+            exx = c->e_t0[0][i] + c->e_crt[0][i]; eyy = c->e_t0[1][i] + c->e_crt[1][i]; ezz = c->e_t0[2][i] + c->e_crt[2][i];
+            eyz = c->e_t0[3][i] + c->e_crt[3][i]; exz = c->e_t0[4][i] + c->e_crt[4][i]; exy = c->e_t0[5][i] + c->e_crt[5][i];
         }
         double d[4], d2[3][2][3];
         stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, Aexi, iDMIi, x, y, z, d, d2);
@@ -1273,9 +1275,9 @@ void gfo_afm_get_dm(gfo_ctx* c, int stage) {
             if (static_strain(c)) {                  /* magnetic_AFM_solver.cpp:89-96 */
                 exx = c->e_t0[0][i] + 0.; eyy = c->e_t0[1][i] + 0.; ezz = c->e_t0[2][i] + 0.;
                 eyz = c->e_t0[3][i] + 0.; exz = c->e_t0[4][i] + 0.; exy = c->e_t0[5][i] + 0.;
-            } else if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* magnetic_AFM_solver.cpp:89-96 */
-                exx = 0. + c->e_crt[0][i]; eyy = 0. + c->e_crt[1][i]; ezz = 0. + c->e_crt[2][i];
-                eyz = 0. + c->e_crt[3][i]; exz = 0. + c->e_crt[4][i]; exy = 0. + c->e_crt[5][i];
+            } else if (c->flags & GF_FLAG_ELASTO_ON_PM) {   /* magnetic_AFM_solver.cpp:89-96: static plus dynamic */
+                exx = c->e_t0[0][i] + c->e_crt[0][i]; eyy = c->e_t0[1][i] + c->e_crt[1][i]; ezz = c->e_t0[2][i] + c->e_crt[2][i];
+                eyz = c->e_t0[3][i] + c->e_crt[3][i]; exz = c->e_t0[4][i] + c->e_crt[4][i]; exy = c->e_t0[5][i] + c->e_crt[5][i];
             }
             double mgx = st[0][i], mgy = st[1][i], mgz = st[2][i];
             double mx, my, mz;

@@ -248,6 +248,43 @@ def test_magnetoelastic_stages_with_static_eigenstrain(rotate):
     o.close()
 
 
+@pytest.mark.parametrize("rotate", [False, True])
+def test_magnetoelastic_stages_with_static_strain(rotate):
+    """A static strain uploaded beside the elastodynamics: the magnets read
+    exxt0 + Dexx (magnetic_solver.cpp:89-96), stage by stage."""
+    from goferro import FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF
+    prob, m = magnetoelastic(rotate=rotate)
+    u, v = random_uv(prob.shape, 36)
+    rng = np.random.default_rng(37)
+    e = 3e-5 * rng.standard_normal((6,) + prob.shape)
+    fields = {FIELD_M: m, FIELD_U: u, FIELD_V: v, FIELD_STRAIN_T0_DIAG: e[:3], FIELD_STRAIN_T0_OFF: e[3:]}
+    g, o = run_stages(prob, fields, nsteps=2, phys=("el", "fm"), check=(FIELD_M_STORE,))
+    np.testing.assert_allclose(g.download(FIELD_M), o.get(FIELD_M), rtol=0, atol=1e-12)
+    g.close()
+    o.close()
+
+
+def test_afm_elastic_stages_with_static_strain():
+    from goferro import FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF
+    prob, m1, m2 = workloads.afm_block(shape=(8, 8, 8))
+    mat = prob.materials[0]
+    set_elastic(mat, C11, C12, C44, RHO)
+    mat.B1_AFM1 = mat.B1_AFM2 = 6e5
+    mat.B2_AFM1 = mat.B2_AFM2 = 4e5
+    mat.lamda100_AFM1 = mat.lamda100_AFM2 = -2e-5
+    mat.lamda111_AFM1 = mat.lamda111_AFM2 = 1e-5
+    prob.flags |= FLAG_ELASTODYNAMICS | FLAG_ELASTO_BACKACTION | FLAG_ELASTO_ON_PM
+    u, v = random_uv(prob.shape, 38)
+    e = 3e-5 * np.random.default_rng(39).standard_normal((6,) + prob.shape)
+    fields = {FIELD_M_AFM1: m1, FIELD_M_AFM2: m2, FIELD_U: u, FIELD_V: v, FIELD_STRAIN_T0_DIAG: e[:3],
+              FIELD_STRAIN_T0_OFF: e[3:]}
+    g, o = run_stages(prob, fields, nsteps=2, phys=("el", "afm"))
+    for fid in (FIELD_M_AFM1, FIELD_M_AFM2):
+        np.testing.assert_allclose(g.download(fid), o.get(fid), rtol=0, atol=1e-12)
+    g.close()
+    o.close()
+
+
 def test_ferroelectric_elastic_stages_with_static_eigenstrain():
     """The ferroelectric reads the same eigenstrain back (its elastic driving
     force, ferroelectric_solver.cpp:630-640)."""

@@ -105,6 +105,16 @@ def test_restart_magnetoelastic(mur):
               (FIELD_M, FIELD_U, FIELD_V, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF), 6, 7, statics=False)
 
 
+def test_restart_carries_the_static_strain_beside_elastodynamics():
+    from goferro import FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF
+    from test_gpu_elastic import magnetoelastic, random_uv
+    prob, m = magnetoelastic(shape=(8, 8, 12))
+    u, v = random_uv(prob.shape, 28)
+    e = 3e-5 * np.random.default_rng(29).standard_normal((6,) + prob.shape)
+    f = {FIELD_M: m, FIELD_U: u, FIELD_V: v, FIELD_STRAIN_T0_DIAG: e[:3], FIELD_STRAIN_T0_OFF: e[3:]}
+    split_run("magnetoelastic, static strain", prob, f, (FIELD_M, FIELD_U, FIELD_V), 3, 4, statics=False)
+
+
 def test_restart_carries_the_static_eigenstrain():
     from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF
     from test_gpu_elastic import magnetoelastic, random_e0t0, random_uv

@@ -294,6 +294,38 @@ def test_eigenstrain_t0_input_and_output(tmp_path):
     assert (tmp_path / "out2" / "eigenstraint0_crt.0.dat").read_text() == reference_lines(back)
 
 
+def test_input_static_strain_rotated_and_with_elastodynamics(tmp_path):
+    """input_straint0 reads the global-frame static strain; with a rotated
+    crystal the upload is Tg2c e Tg2c^T (transform_matrix_glb2crt,
+    mathlib.cpp:78-100), also beside elastodynamics."""
+    from goferro import FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF
+    c, s = np.cos(0.6), np.sin(0.6)
+    d = deck_copy(tmp_path, edits=[
+        ("if_OrientRotate", "true"), ("Tg2c row 1", f"{c:.17e} {-s:.17e} 0"), ("Tg2c row 2", f"{s:.17e} {c:.17e} 0"),
+        ("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "true true true"),
+        ("if_input_uandv", "false true false")])
+    rng = np.random.default_rng(13)
+    e = rng.standard_normal((6, 8, 8, 4)) * 1e-4
+    gio.write_cells(d / "strain_stat.in", e)
+    deck = gio.read_deck(d)
+    ctx = _RecordingContext(deck.problem.shape)
+    assert "strain_stat.in" in gio.load_inputs(ctx, deck)
+    back = gio.read_cells(d / "strain_stat.in", (8, 8, 4), 6)
+    full = np.empty((3, 3, 8, 8, 4))
+    for a in range(3):
+        full[a, a] = back[a]
+    full[1, 2] = full[2, 1] = back[3]
+    full[0, 2] = full[2, 0] = back[4]
+    full[0, 1] = full[1, 0] = back[5]
+    T = np.asarray(deck.problem.Tg2c)
+    want = np.einsum("ik,jl,kl...->ij...", T, T, full)
+    got = np.concatenate([ctx.uploads[FIELD_STRAIN_T0_DIAG], ctx.uploads[FIELD_STRAIN_T0_OFF]])
+    for q, (i, j) in enumerate(((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1))):
+        np.testing.assert_allclose(got[q], want[i, j], rtol=0, atol=1e-19)
+    # a rotation about z mixes xx and yy
+    assert np.abs(got[0] - back[0]).max() > 1e-6
+
+
 def test_fft_electrostatics_with_fft_magnetostatics_is_read(tmp_path):
     """A multiferroic deck with both FFT statics (main.cpp:299-306) maps to
     both flags."""

@@ -186,3 +186,46 @@ def test_static_eigenstrain_cancels_a_frozen_magnet(oracle_lib, rotate):
     swapped = e0.copy()
     swapped[[3, 5]] = e0[[5, 3]]
     assert run(swapped) > 1e-3 * free
+
+
+def test_static_strain_adds_to_the_dynamic_one_for_magnets(oracle_lib):
+    """magnetic_solver.cpp:89-96: the magnet reads exx = exxt0_crt + Dexx_crt.
+    The stage-1 increment is linear in H for a fixed m, and H linear in the
+    strain, so k(S + D) - k(D) equals k(S) - k(0), where S is an uploaded
+    static strain and D the dynamic strain of a displacement field; and
+    with u = 0 the increment equals that of a context without
+    elastodynamics given the same static strain."""
+    from goferro import (FIELD_M, FIELD_STRAIN_T0_DIAG, FIELD_STRAIN_T0_OFF, FLAG_ELASTO_BACKACTION,
+                         FLAG_ELASTODYNAMICS, FLAG_GAUSSIAN_PULSE)
+    from goferro.workloads import magnetoelastic_film
+    from oracle.oracle import FIELD_K1
+    prob, m = magnetoelastic_film(shape=(6, 6, 8), magnet_layers=3)
+    prob.flags &= ~(FLAG_GAUSSIAN_PULSE | FLAG_ELASTO_BACKACTION)
+    rng = np.random.default_rng(77)
+    S = 1e-4 * rng.standard_normal((6,) + prob.shape)
+    u = 1e-11 * rng.standard_normal((3,) + prob.shape)
+
+    def k1(problem, static, disp):
+        o = oracle_lib.Oracle(problem)
+        o.set(FIELD_M, m)
+        if static is not None:
+            o.set(FIELD_STRAIN_T0_DIAG, static[:3])
+            o.set(FIELD_STRAIN_T0_OFF, static[3:])
+        if disp is not None:
+            o.set(FIELD_U, disp)
+        o.time_advance()
+        if problem.flags & FLAG_ELASTODYNAMICS:
+            o.L.gfo_elastic_get(o._h, 1)
+        o.L.gfo_mag_get_dm(o._h, 1)
+        k = o.get(FIELD_K1)
+        o.close()
+        return k
+
+    zero = np.zeros_like(S)
+    a = k1(prob, S, u) - k1(prob, zero, u)
+    b = k1(prob, S, None) - k1(prob, zero, None)
+    assert np.abs(b).max() > 0
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-9 * np.abs(b).max())
+    import dataclasses
+    plain = dataclasses.replace(prob, flags=prob.flags & ~FLAG_ELASTODYNAMICS)
+    np.testing.assert_array_equal(k1(prob, S, None), k1(plain, S, None))
