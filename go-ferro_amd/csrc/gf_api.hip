@@ -1317,6 +1317,11 @@ gf_status gf_upload(gf_ctx* c, int id, const double* host, size_t count) {
     if (id == GF_FIELD_HEXT_NONUNIF) c->has_hnu = true;
     if (id == GF_FIELD_EIGENSTRAIN_T0_DIAG || id == GF_FIELD_EIGENSTRAIN_T0_OFF) c->has_e0t0 = true;
     if ((id == GF_FIELD_STRAIN_T0_DIAG || id == GF_FIELD_STRAIN_T0_OFF) && c->el_on) c->has_et0_dyn = true;
+    // an uploaded static strain or eigenstrain replaces the start-up solve
+    // of the elastodynamics (main.cpp:148)
+    if (id == GF_FIELD_STRAIN_T0_DIAG || id == GF_FIELD_STRAIN_T0_OFF || id == GF_FIELD_EIGENSTRAIN_T0_DIAG ||
+        id == GF_FIELD_EIGENSTRAIN_T0_OFF)
+        c->t0_fixed = true;
     if (id == GF_FIELD_M || id == GF_FIELD_M_STORE) c->demag.m_in_spec = false;
     if (id == GF_FIELD_HSTAT) { c->demag.h_in_spec = false; c->demag.h_real_valid = true; }
     // elastic_initialize.cpp:386-410: the stores start equal to u and v
@@ -1581,6 +1586,8 @@ std::string stage_io_conflict(const FeStageArgs& a) {
     for (int c = 0; c < 6; c++) {
         rd.add(a.strain[c], "strain", c);
         rd.add(a.eigen[c], "eigenstrain", c);
+        rd.add(a.strain0[c], "static strain", c);
+        rd.add(a.eigen0[c], "static eigenstrain", c);
     }
     return io_conflict(rd, wr);
 }
@@ -1591,11 +1598,15 @@ FeStageArgs fe_stage_args(gf_ctx* c, int stage, int in_sel, int out_sel) {
     FeStageArgs a{};
     // the strain and eigenstrain of the elastodynamics, or, with the
     // elastostatics instead, the static ones (ex*t0_crt, ex*0t0_crt:
-    // ferroelectric_solver.cpp:624-653 with the dynamic parts zero)
+    // ferroelectric_solver.cpp:624-653 with the dynamic parts zero); with
+    // both, the static ones added to the dynamic ones (:636-650)
     const bool stat = !c->el_on && (c->flags & GF_FLAG_ELASTOSTATIC) && (c->flags & GF_FLAG_ELASTO_ON_PM);
+    const bool both = strain_coupled(c) && (c->flags & GF_FLAG_ELASTOSTATIC);
     for (int i = 0; i < 6; i++) {
         a.strain[i] = strain_coupled(c) ? c->ec[i] : (stat ? c->e_t0[i] : nullptr);
         a.eigen[i] = strain_coupled(c) ? c->e0c[i] : (stat ? c->e0t0[i] : nullptr);
+        a.strain0[i] = both ? c->e_t0[i] : nullptr;
+        a.eigen0[i] = both ? c->e0t0[i] : nullptr;
     }
     const bool em_p = c->em_on && (c->flags & GF_FLAG_EM_BACKACTION_P);
     for (int i = 0; i < 3; i++) a.ecell[i] = em_p ? c->Ecell[i] : nullptr;
@@ -2085,9 +2096,13 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
 
 gf_status gf_demag(gf_ctx* c) {
     if (!c) return GF_ERR_ARG;
-    if (c->flags & GF_FLAG_ELASTOSTATIC) {
-        // the static strain of the current magnet stores (main.cpp:148-158, 214)
+    if ((c->flags & GF_FLAG_ELASTOSTATIC) && !(c->el_on && c->t0_fixed)) {
+        // the static strain of the current magnet and ferroelectric stores:
+        // before the loop without elastodynamics (main.cpp:214), once at
+        // start-up with it unless a static strain or eigenstrain was read
+        // (main.cpp:148-158)
         if (gf_status st = elastostatic_static(c)) return st;
+        c->t0_fixed = true;
     }
     if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->any_fe) {
         // initial get_E_static (main.cpp:120-131)
@@ -2134,7 +2149,7 @@ gf_status gf_step(gf_ctx* c, int64_t nsteps) {
             if (!st) st = statics_after_update(c);
         }
         // the static strain after the step without elastodynamics (main.cpp:472-474)
-        if (!st && (c->flags & GF_FLAG_ELASTOSTATIC)) st = elastostatic_static(c);
+        if (!st && (c->flags & GF_FLAG_ELASTOSTATIC) && !c->el_on) st = elastostatic_static(c);
         if (st) {
             c->fe_defer = false;
             c->fe_pending = false;
@@ -2261,7 +2276,8 @@ struct StateArrayHeader {
 
 enum : uint32_t {
     kBitHstatWritten = 1u, kBitJpOn = 2u, kBitJpUploaded = 4u, kBitHasHnu = 8u,
-    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u, kBitHasE0t0 = 128u, kBitEt0Dyn = 256u
+    kBitHInSpec = 16u, kBitHRealValid = 32u, kBitMInSpec = 64u, kBitHasE0t0 = 128u, kBitEt0Dyn = 256u,
+    kBitT0Fixed = 512u
 };
 
 // FNV-1a over 64-bit words, then over the tail bytes.
@@ -2374,6 +2390,7 @@ uint32_t state_bits(const gf_ctx* c) {
     if (c->has_hnu) b |= kBitHasHnu;
     if (c->has_e0t0) b |= kBitHasE0t0;
     if (c->has_et0_dyn) b |= kBitEt0Dyn;
+    if (c->t0_fixed) b |= kBitT0Fixed;
     if (c->demag.h_in_spec) b |= kBitHInSpec;
     if (c->demag.h_real_valid) b |= kBitHRealValid;
     if (c->demag.m_in_spec) b |= kBitMInSpec;
@@ -2547,6 +2564,7 @@ gf_status gf_state_load(gf_ctx* c, const void* host, size_t bytes) {
     c->has_hnu = h.bits & kBitHasHnu;
     c->has_e0t0 = h.bits & kBitHasE0t0;
     c->has_et0_dyn = h.bits & kBitEt0Dyn;
+    c->t0_fixed = h.bits & kBitT0Fixed;
     c->demag.h_in_spec = h.bits & kBitHInSpec;
     c->demag.h_real_valid = h.bits & kBitHRealValid;
     c->demag.m_in_spec = h.bits & kBitMInSpec;

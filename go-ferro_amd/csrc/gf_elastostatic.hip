@@ -240,9 +240,8 @@ gf_status elastostatic_setup(gf_ctx* c, const gf_desc* d) {
         return set_error(c, GF_ERR_UNSUPPORTED, "the elastostatics needs power-of-two nx, ny, nz up to 2048");
     if (!(c->any_fm || c->any_afm || c->any_fe))
         return set_error(c, GF_ERR_UNSUPPORTED, "the elastostatics needs a magnetic or ferroelectric material");
-    if ((c->flags & GF_FLAG_ELASTODYNAMICS) || (c->flags & GF_FLAG_FLEXO))
-        return set_error(c, GF_ERR_UNSUPPORTED,
-                         "the elastostatics runs without elastodynamics and without the flexoelectric coupling");
+    if (c->flags & GF_FLAG_FLEXO)
+        return set_error(c, GF_ERR_UNSUPPORTED, "the elastostatics runs without the flexoelectric coupling");
     const double sc = d->scale_elasto > 0. ? d->scale_elasto : 1.;
     double c11 = c->host_mats[0].c11, c12 = c->host_mats[0].c12, c44 = c->host_mats[0].c44;
     for (const auto& m : c->host_mats) {
@@ -268,6 +267,9 @@ gf_status elastostatic_setup(gf_ctx* c, const gf_desc* d) {
         if (!st) st = hip_check(c, hipMemset(c->e0t0[a], 0, sizeof(double) * static_cast<size_t>(c->n)), "clear static eigenstrain");
     }
     if (!st) c->has_e0t0 = true;
+    // with elastodynamics the magnets read the static strain beside the
+    // dynamic one, and the prep launch subtracts the static eigenstrain
+    if (!st && c->el_on) c->has_et0_dyn = true;
     return st;
 }
 

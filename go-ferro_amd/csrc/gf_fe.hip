@@ -91,10 +91,18 @@ __device__ __forceinline__ void fe_cell(const FeStageArgs& a, int64_t id) {
         if (a.strain[0]) {
             // electrostrictive coupling to strain minus eigenstrain
             // (ferroelectric_solver.cpp:624-653, 776-786), same sum order
-            const double exx = a.strain[0][id], eyy = a.strain[1][id], ezz = a.strain[2][id];
-            const double eyz = a.strain[3][id], exz = a.strain[4][id], exy = a.strain[5][id];
-            const double exx0 = a.eigen[0][id], eyy0 = a.eigen[1][id], ezz0 = a.eigen[2][id];
-            const double eyz0 = a.eigen[3][id], exz0 = a.eigen[4][id], exy0 = a.eigen[5][id];
+            double exx = a.strain[0][id], eyy = a.strain[1][id], ezz = a.strain[2][id];
+            double eyz = a.strain[3][id], exz = a.strain[4][id], exy = a.strain[5][id];
+            double exx0 = a.eigen[0][id], eyy0 = a.eigen[1][id], ezz0 = a.eigen[2][id];
+            double eyz0 = a.eigen[3][id], exz0 = a.eigen[4][id], exy0 = a.eigen[5][id];
+            if (a.strain0[0]) {
+                // the static strain and eigenstrain added to the dynamic
+                // ones (ferroelectric_solver.cpp:636-650)
+                exx = exx + a.strain0[0][id]; eyy = eyy + a.strain0[1][id]; ezz = ezz + a.strain0[2][id];
+                eyz = eyz + a.strain0[3][id]; exz = exz + a.strain0[4][id]; exy = exy + a.strain0[5][id];
+                exx0 = exx0 + a.eigen0[0][id]; eyy0 = eyy0 + a.eigen0[1][id]; ezz0 = ezz0 + a.eigen0[2][id];
+                eyz0 = eyz0 + a.eigen0[3][id]; exz0 = exz0 + a.eigen0[4][id]; exy0 = exy0 + a.eigen0[5][id];
+            }
             lx = lx + (M.tv1 * (exx0 - exx) + M.tv2 * (eyy0 - eyy + ezz0 - ezz)) * px;
             lx = lx + M.tv3 * ((exy0 - exy) * py + (exz0 - exz) * pz);
             ly = ly + (M.tv1 * (eyy0 - eyy) + M.tv2 * (exx0 - exx + ezz0 - ezz)) * py;

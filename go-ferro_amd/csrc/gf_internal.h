@@ -372,6 +372,8 @@ struct FeStageArgs {
     const double* estat[3];
     const double* strain[6];    // dynamic strain (crystal frame), null when uncoupled
     const double* eigen[6];     // eigenstrain (crystal frame), with strain
+    const double* strain0[6];   // static strain added to strain, or null
+    const double* eigen0[6];    // static eigenstrain added to eigen, with strain0
     const double* ecell[3];     // electrodynamic E at the cells, null when uncoupled
     const double* flexo[3];     // flexoelectric force of the stage, null when off
     int64_t nx, ny, nz;
@@ -564,6 +566,7 @@ struct gf_ctx {
     bool has_hnu = false;
     bool has_e0t0 = false;          // the static eigenstrain was written: the elastic stage subtracts it
     bool has_et0_dyn = false;       // a static strain was uploaded beside the elastodynamics: the magnets add it
+    bool t0_fixed = false;          // the static strain at t0 was solved for, uploaded or loaded
 
     uint32_t* d_code = nullptr;
     uint32_t* d_fm_ids = nullptr;   // FM cells in index order, when not every cell is FM

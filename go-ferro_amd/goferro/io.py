@@ -467,8 +467,8 @@ def _unsupported(g: dict, mats: list) -> list[str]:
     if g["if_elastostatic_1D"] or g["if_read_strain_ext"] or g["if_elastostatic_film"]:
         out.append("the 1D and film elastostatics and external strain (if_elastostatic_1D, if_elastostatic_film, "
                    "if_read_strain_ext)")
-    if g["if_elastostatic"] and (g["if_elastodynamics"] or (any_fe and g["if_flexo"])):
-        out.append("if_elastostatic with elastodynamics or the flexoelectric coupling")
+    if g["if_elastostatic"] and any_fe and g["if_flexo"]:
+        out.append("if_elastostatic with the flexoelectric coupling")
     if any_fe and g["if_elec_1Dmodel"]:
         out.append("if_elec_1Dmodel (get_E_static_1D is declared but not defined in the reference)")
     return out
@@ -518,11 +518,12 @@ def read_deck(directory, allow_unsupported: bool = False) -> Deck:
     # if_flexo, elastic_dynamic_solver.cpp:63, main.cpp:161-166)
     if g["if_flexo"] and any_fe:
         flags |= FLAG_FLEXO
-    # get_strain_static after every step without elastodynamics (main.cpp:214,
-    # 472-474); without a magnet (a ferroelectric is refused above) its
-    # eigenstrain, and so the static strain, is zero, and nothing is solved
+    # get_strain_static before the loop and after every step without
+    # elastodynamics (main.cpp:214, 472-474), once at start-up with it
+    # (main.cpp:148-158); without a magnet or ferroelectric the eigenstrain,
+    # and so the static strain, is zero, and nothing is solved
     any_mag = any(r["if_FM"] or r["if_AFM"] for r in raw)
-    if g["if_elastostatic"] and not g["if_elastodynamics"] and any_mag:
+    if g["if_elastostatic"] and (any_mag or any_fe):
         flags |= FLAG_ELASTOSTATIC
     # get_E_static runs whenever there is a ferroelectric (main.cpp:120-131,
     # 214-226, 299-302)
@@ -845,8 +846,14 @@ class Outputs:
         if self._due("if_output_Hstat", "output_step_Hstat", nstep):
             self._cells(f"Hstat.{nstep}.dat", ctx.download(FIELD_HSTAT))
         if self._due("if_output_strain", "output_step_strain", nstep) and g["if_elastodynamics"]:
-            self._cells(f"strain.{nstep}.dat",
-                        np.concatenate([ctx.download(FIELD_STRAIN_DIAG), ctx.download(FIELD_STRAIN_OFF)]))
+            # output_strain (inoutput.cpp:648-672): exxt0_glb + Dexx_glb, the
+            # static part as read (input_straint0) or solved at start-up
+            dyn = np.concatenate([ctx.download(FIELD_STRAIN_DIAG), ctx.download(FIELD_STRAIN_OFF)])
+            if g["if_input_straint0"]:
+                t0 = read_cells(self.deck.directory / "strain_stat.in", self.deck.problem.shape, 6)
+            else:
+                t0 = self._static_strain_glb(ctx)
+            self._cells(f"strain.{nstep}.dat", t0 + dyn)
         elif self._due("if_output_strain", "output_step_strain", nstep) and g["if_elastostatic"]:
             # main.cpp:587-593 and output_strain (inoutput.cpp:648-672): the
             # global-frame static strain plus a dynamic strain that stays zero
@@ -937,6 +944,7 @@ def run(directory, steps: Optional[int] = None, out_dir=None, restart: Optional[
     ctx = Context(deck.problem)
     outs = Outputs(deck, out_dir)
     need_e, need_h = _statics_needed(deck)
+    need_s = bool(deck.problem.flags & FLAG_ELASTOSTATIC)
     g = deck.settings
     if restart is not None:
         ctx.load_state(restart)
@@ -952,15 +960,18 @@ def run(directory, steps: Optional[int] = None, out_dir=None, restart: Optional[
         # initial statics (main.cpp:120-137): each static field is computed
         # unless it was read; gf_demag computes both, so a field that was
         # read is uploaded again afterwards
-        if (need_e and not g["if_input_Estat"]) or (need_h and not g["if_input_Hstat"]):
+        # gf_demag also solves the static strain: at start-up with
+        # elastodynamics unless a static strain or eigenstrain was read
+        # (main.cpp:148-158; load_inputs uploads those first)
+        if (need_e and not g["if_input_Estat"]) or (need_h and not g["if_input_Hstat"]) or need_s:
             ctx.demag()
             if need_e and g["if_input_Estat"]:
                 ctx.upload(FIELD_ESTAT, read_cells(deck.directory / "Estat.in", deck.problem.shape, 3))
             if need_h and g["if_input_Hstat"]:
                 ctx.upload(FIELD_HSTAT, read_cells(deck.directory / "Hstat.in", deck.problem.shape, 3))
         outs.initial(ctx)
-        # statics before the loop without electrodynamics (main.cpp:215-238)
-        if not g["if_EMdynamic"] and (need_e or need_h):
+        # statics before the loop without electrodynamics (main.cpp:214-238)
+        if not g["if_EMdynamic"] and (need_e or need_h or need_s):
             ctx.demag()
     while nstep < total:
         nxt = outs.next_due(nstep)

@@ -107,10 +107,13 @@ typedef int gf_status;
  * strain equal to the mean eigenstrain (zero mean stress).  A Nyquist
  * component enters symmetrised over its sign, so the strain is real.
  * Written into GF_FIELD_STRAIN_T0_* (crystal frame), with e0 into
- * GF_FIELD_EIGENSTRAIN_T0_*, by gf_demag and after every gf_step step
- * (main.cpp:214, 473).  With GF_FLAG_ELASTO_ON_PM the magnets read the
- * static strain and the ferroelectrics both (ferroelectric_solver.cpp:
- * 636-650).  Without elastodynamics and the flexoelectric coupling;
+ * GF_FIELD_EIGENSTRAIN_T0_*: without elastodynamics by gf_demag and after
+ * every gf_step step (main.cpp:214, 473); with it once, by the first
+ * gf_demag, unless a static strain or eigenstrain was uploaded first
+ * (main.cpp:148-158), and the elastodynamics then carries the change from
+ * t0.  With GF_FLAG_ELASTO_ON_PM the magnets read the static strain (plus
+ * the dynamic one) and the ferroelectrics both, added to the dynamic ones
+ * (ferroelectric_solver.cpp:636-650).  Without the flexoelectric coupling;
  * power-of-two nx, ny, nz up to 2048. */
 #define GF_FLAG_ELASTOSTATIC     (1u << 27)
 /* if_prescribed_m: the ferromagnetic cells of the bottom layer (k = 0)

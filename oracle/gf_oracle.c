@@ -111,6 +111,7 @@ struct gfo_ctx {
     double precess_angle, precess_freq;        /* prescribed bottom layer (GF_FLAG_PRESCRIBED_M, read.cpp:101) */
     double er_homo;                            /* electrostatics (GF_FLAG_ELECTROSTATICS) */
     double *e_t0[6];                           /* static crystal-frame strain, ex*t0_crt */
+    int t0_fixed;                              /* the static strain at t0 was solved for or uploaded */
     /* electrodynamics (EMdynamic_system.h): E on edges, H on faces */
     int if_em, periodic_all, pec_all, jp_on;
     int64_t nE[3], nH[3];
@@ -1009,6 +1010,10 @@ int gfo_set_field(gfo_ctx* c, int id, const double* src, size_t count) {
     if (!f || count != (size_t)(3 * c->n)) return -1;
     for (int a = 0; a < 3; a++) memcpy(f[a], src + a * c->n, sizeof(double) * (size_t)c->n);
     if (id == GF_FIELD_M) for (int a = 0; a < 3; a++) memcpy(c->ms[a], c->m[a], sizeof(double) * (size_t)c->n);
+    /* main.cpp:148: read static strain or eigenstrain replaces the start-up solve */
+    if (id == GF_FIELD_STRAIN_T0_DIAG || id == GF_FIELD_STRAIN_T0_OFF || id == GF_FIELD_EIGENSTRAIN_T0_DIAG ||
+        id == GF_FIELD_EIGENSTRAIN_T0_OFF)
+        c->t0_fixed = 1;
     if (id == GF_FIELD_M_AFM1) for (int a = 0; a < 3; a++) memcpy(c->ms1[a], c->m1[a], sizeof(double) * (size_t)c->n);
     if (id == GF_FIELD_M_AFM2) for (int a = 0; a < 3; a++) memcpy(c->ms2[a], c->m2[a], sizeof(double) * (size_t)c->n);
     /* ferroelectric_initialize.cpp:355-366 */
@@ -2714,7 +2719,12 @@ void gfo_elastostatic(gfo_ctx* c) {
 }
 
 void gfo_demag(gfo_ctx* c) {
-    if (c->flags & GF_FLAG_ELASTOSTATIC) gfo_elastostatic(c);
+    /* main.cpp:214 without elastodynamics; with it once at start-up unless
+     * a static strain or eigenstrain was read (main.cpp:148-158) */
+    if ((c->flags & GF_FLAG_ELASTOSTATIC) && !(c->if_elasto && c->t0_fixed)) {
+        gfo_elastostatic(c);
+        c->t0_fixed = 1;
+    }
     if ((c->flags & GF_FLAG_ELECTROSTATICS) && c->if_FE_all) gfo_estatic(c);
     if (!(c->if_FM_all || c->if_AFM_all)) return;
     if (!(c->flags & (GF_FLAG_MAGNETOSTATICS | GF_FLAG_MAG_1D | GF_FLAG_DEMAG_FACTOR))) return;
@@ -2751,7 +2761,7 @@ void gfo_step(gfo_ctx* c, int64_t nsteps) {
             if (!em && mag && !one_d && ms) demag_3d(c);
         }
         /* main.cpp:472-474: the static strain after the step without elastodynamics */
-        if (c->flags & GF_FLAG_ELASTOSTATIC) gfo_elastostatic(c);
+        if ((c->flags & GF_FLAG_ELASTOSTATIC) && !c->if_elasto) gfo_elastostatic(c);
     }
 }
 

@@ -5,7 +5,8 @@ crystal, two magnets and a non-magnetic material, an antiferromagnet), and
 magnets stepping with the static strain recomputed after every step and
 acting back through the magnetoelastic field; a ferroelectric layer whose
 electrostrictive eigenstrain enters the solve and which reads the static
-strain and eigenstrain back, with its restart.  1e-12 of the peak.
+strain and eigenstrain back, with its restart; the same stack with
+elastodynamics, the static solve once at start-up.  1e-12 of the peak.
 """
 import sys
 
@@ -194,10 +195,89 @@ def test_ferroelectric_elastostatic_restart():
     c.close()
 
 
+def beside(rotate):
+    """fe_mixed with elastodynamics acting back, random u and v."""
+    from goferro import FIELD_U, FIELD_V, FLAG_ELASTO_BACKACTION
+    prob, f = fe_mixed(rotate=rotate)
+    prob.flags |= FLAG_ELASTODYNAMICS | FLAG_ELASTO_BACKACTION
+    rng = np.random.default_rng(47)
+    f[FIELD_U] = 1e-12 * rng.standard_normal((3,) + prob.shape)
+    f[FIELD_V] = 0.5 * rng.standard_normal((3,) + prob.shape)
+    return prob, f
+
+
+@pytest.mark.parametrize("rotate", [False, True])
+def test_elastostatics_beside_elastodynamics(rotate):
+    """The static strain and eigenstrain solved once at start-up
+    (main.cpp:148-158); three steps of the magnet, the ferroelectric and the
+    elastodynamics reading them beside the dynamic ones; the static strain
+    kept through the steps; an uploaded static strain replaces the solve."""
+    from goferro import FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF, FIELD_P, FIELD_Q, FIELD_U, FIELD_V
+    say("case elastostatics beside elastodynamics, rotate", rotate)
+    prob, f = beside(rotate)
+    g, o = pair(prob, f)
+    g.demag()
+    o.demag()
+    assert rel(strain(g), strain(o)) <= 1e-12
+    for fid in (FIELD_EIGENSTRAIN_T0_DIAG, FIELD_EIGENSTRAIN_T0_OFF):
+        assert rel(g.download(fid), o.get(fid)) <= 1e-12, fid
+    s0 = strain(g)
+    g.step(3)
+    o.step(3)
+    for fid in (FIELD_P, FIELD_Q, FIELD_U, FIELD_V):
+        d = rel(g.download(fid), o.get(fid))
+        say("  rel", fid, d)
+        assert d <= 1e-12, fid
+    assert np.abs(g.download(FIELD_M) - o.get(FIELD_M)).max() <= 1e-12
+    g.demag()
+    assert np.array_equal(strain(g), s0)
+    g.close()
+    o.close()
+    g = Context(prob)
+    for fid, v in f.items():
+        g.upload(fid, v)
+    up = 1e-5 * np.random.default_rng(5).standard_normal((3,) + prob.shape)
+    g.upload(FIELD_STRAIN_T0_DIAG, up)
+    g.demag()
+    assert np.array_equal(g.download(FIELD_STRAIN_T0_DIAG), up)
+    g.close()
+
+
+def test_elastostatics_beside_elastodynamics_restart():
+    """The image carries the start-up static strain and that it was
+    solved: n1 + n2 steps equal n1, image, fresh context, gf_demag, n2."""
+    from goferro import FIELD_P, FIELD_U
+    say("case elastostatics beside elastodynamics, restart")
+    prob, f = beside(False)
+    a = Context(prob)
+    for fid, v in f.items():
+        a.upload(fid, v)
+    a.demag()
+    a.step(5)
+    want = a.download(FIELD_P), a.download(FIELD_U), a.download(FIELD_M)
+    a.close()
+    b = Context(prob)
+    for fid, v in f.items():
+        b.upload(fid, v)
+    b.demag()
+    b.step(2)
+    image = b.state_image()
+    b.close()
+    c = Context(prob)
+    c.restore_state(image)
+    c.demag()   # does not solve again: the image holds the start-up solve
+    c.step(3)
+    got = c.download(FIELD_P), c.download(FIELD_U), c.download(FIELD_M)
+    assert all(np.array_equal(x, y) for x, y in zip(got, want))
+    c.close()
+
+
 def test_elastostatic_rejections():
     say("case elastostatic rejections")
-    prob, _ = mixed(flags=FLAG_ELASTODYNAMICS)
-    with pytest.raises(GoFerroError, match="elastodynamics"):
+    from goferro import FLAG_FLEXO
+    prob, _ = fe_mixed()
+    prob.flags |= FLAG_FLEXO | FLAG_ELASTODYNAMICS
+    with pytest.raises(GoFerroError, match="flexoelectric"):
         Context(prob)
     prob, _ = mixed(shape=(8, 8, 12))
     with pytest.raises(GoFerroError, match="power-of-two"):

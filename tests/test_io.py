@@ -326,6 +326,37 @@ def test_input_static_strain_rotated_and_with_elastodynamics(tmp_path):
     assert np.abs(got[0] - back[0]).max() > 1e-6
 
 
+def test_elastostatics_beside_elastodynamics_deck(tmp_path):
+    """if_elastostatic with elastodynamics is read (the static strain solved
+    at start-up, main.cpp:148-158); a deck whose only polar material is a
+    ferroelectric sets the elastostatics as well; with elastodynamics the
+    strain output is exxt0_glb + Dexx_glb (inoutput.cpp:648-672), the
+    static part as read by input_straint0."""
+    from goferro import FLAG_ELASTOSTATIC, FIELD_STRAIN_DIAG, FIELD_STRAIN_OFF
+    d = deck_copy(tmp_path, edits=[
+        ("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "true true true"),
+        ("if_elastostatic if_elastostatic_1D", "true false"), ("if_input_uandv", "false true false"),
+        ("if_output_strain", "true 5")])
+    rng = np.random.default_rng(17)
+    gio.write_cells(d / "strain_stat.in", rng.standard_normal((6, 8, 8, 4)) * 1e-4)
+    deck = gio.read_deck(d)
+    assert deck.problem.flags & FLAG_ELASTOSTATIC and deck.problem.flags & FLAG_ELASTODYNAMICS
+    ctx = _RecordingContext(deck.problem.shape)
+    assert "strain_stat.in" in gio.load_inputs(ctx, deck)
+    dyn = rng.standard_normal((6, 8, 8, 4)) * 1e-5
+    ctx.uploads[FIELD_STRAIN_DIAG], ctx.uploads[FIELD_STRAIN_OFF] = dyn[:3], dyn[3:]
+    out = gio.Outputs(deck, tmp_path / "out")
+    out.step(ctx, 5)
+    t0 = gio.read_cells(d / "strain_stat.in", (8, 8, 4), 6)
+    assert (tmp_path / "out" / "strain.5.dat").read_text() == reference_lines(t0 + dyn)
+    (tmp_path / "b").mkdir()
+    d2 = deck_copy(tmp_path / "b", edits=[("if_elastostatic if_elastostatic_1D", "true false")],
+                   mat_edits=[(1, "if_FM", "false"), (1, "if_FE", "true")])
+    deck2 = gio.read_deck(d2)
+    assert deck2.any_fe and not deck2.any_fm
+    assert deck2.problem.flags & FLAG_ELASTOSTATIC
+
+
 def test_fft_electrostatics_with_fft_magnetostatics_is_read(tmp_path):
     """A multiferroic deck with both FFT statics (main.cpp:299-306) maps to
     both flags."""

@@ -262,3 +262,87 @@ def test_ferroelectric_reads_static_strain_and_eigenstrain(oracle_lib):
     got = kq[0] - kq[1]
     assert np.abs(want).max() > 1e-3 * np.abs(kq[1]).max()
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * np.abs(kq[1]).max())
+
+
+def beside_problem(flags, shape=(8, 8, 16)):
+    """A ferroelectric layer, a magnet and a substrate (the reference's
+    multiferroic stack) with random p and m, the strain acting on both."""
+    from goferro import FIELD_M, FIELD_P, FLAG_ELASTO_ON_PM
+    from goferro.workloads import GYRO, set_magnetostriction
+    fe = ferroelectric()
+    mag = Material.make(if_FM=1, anisotropy_type=1, Ms=1.4e5, gyro=GYRO, FM_damping=0.05, K1=-610.0, Aex=3.7e-12)
+    set_elastic(mag, 2.69e11, 1.08e11, 0.76e11, 5170.0)
+    set_magnetostriction(mag, 3.5e5, 2.0e5)
+    sub = set_elastic(Material.make(), 3.2e11, 1.0e11, 1.2e11, 5100.0)
+    cell = np.full(shape, 3, np.uint32)
+    cell[:, :, 4:10] = 1
+    cell[:, :, 10:] = 2
+    prob = Problem(shape=shape, spacing=(1e-9, 1e-9, 1e-9), materials=[fe, mag, sub], material_cell=cell,
+                   dt=2e-15, periodic=(True, True, False), Hext_stat=(0, 0, 5e4),
+                   flags=FLAG_ELASTO_ON_PM | flags)
+    rng = np.random.default_rng(43)
+    p = 0.05 * rng.standard_normal((3,) + shape)
+    p[2] += 0.6
+    p[:, cell != 1] = 0.0
+    m = np.zeros((3,) + shape)
+    m[:, cell == 2] = np.array([0.3, 0.2, np.sqrt(1 - 0.13)])[:, None]
+    m[0] += 0.05 * rng.standard_normal(shape) * (cell == 2)
+    m /= np.maximum(np.linalg.norm(m, axis=0), 1e-300)
+    return prob, {FIELD_P: p, FIELD_M: m}
+
+
+def test_start_up_solve_beside_elastodynamics(oracle_lib):
+    """With elastodynamics the static strain is solved once at start-up
+    (main.cpp:148-158) and the dynamics carries the change from t0: the
+    eigenstrain of the elastic stage less the static one is zero at t0, so
+    with u = 0 the stage-1 velocity increment vanishes; the magnet reads
+    exxt0 + Dexx and the ferroelectric Dexx + exxt0, Dexx0 + exx0t0, so their
+    stage-1 increments equal those of the elastostatics alone.  After steps
+    the static strain is still the start-up one (without elastodynamics it
+    follows the stores), and an uploaded static strain replaces the solve."""
+    from goferro import (FIELD_STRAIN_T0_DIAG, FLAG_ELASTO_BACKACTION, FLAG_ELASTODYNAMICS)
+    dyn = FLAG_ELASTODYNAMICS | FLAG_ELASTO_BACKACTION
+    out = {}
+    for name, flags in (("static", FLAG_ELASTOSTATIC), ("both", FLAG_ELASTOSTATIC | dyn), ("dyn", dyn)):
+        prob, f = beside_problem(flags)
+        o = oracle_lib.Oracle(prob)
+        for fid, v in f.items():
+            o.set(fid, v)
+        o.demag()
+        e_start = o.get(FIELD_STRAIN_T0_DIAG)
+        if flags & FLAG_ELASTODYNAMICS:
+            o.L.gfo_elastic_get(o._h, 1)
+        o.L.gfo_mag_get_dm(o._h, 1)
+        o.L.gfo_fe_get(o._h, 1)
+        out[name] = {"k1": o.get(103), "kq": o.get(106), "e": e_start,
+                     "dv": o.get(102) if flags & FLAG_ELASTODYNAMICS else None}
+        o.close()
+    s, b, d = out["static"], out["both"], out["dyn"]
+    assert np.array_equal(b["e"], s["e"]) and np.abs(s["e"]).max() > 1e-6
+    assert np.abs(d["dv"]).max() > 0.
+    assert np.abs(b["dv"]).max() <= 1e-12 * np.abs(d["dv"]).max()
+    for key in ("k1", "kq"):
+        scale = np.abs(s[key]).max()
+        assert np.abs(s[key] - d[key]).max() > 1e-6 * scale, key   # the static strain matters
+        np.testing.assert_allclose(b[key], s[key], rtol=0, atol=1e-13 * scale)
+    # steps: the start-up solve is kept with elastodynamics, redone without
+    for flags, kept in ((FLAG_ELASTOSTATIC | dyn, True), (FLAG_ELASTOSTATIC, False)):
+        prob, f = beside_problem(flags)
+        o = oracle_lib.Oracle(prob)
+        for fid, v in f.items():
+            o.set(fid, v)
+        o.demag()
+        e0 = o.get(FIELD_STRAIN_T0_DIAG)
+        o.step(3)
+        o.demag()
+        assert np.array_equal(o.get(FIELD_STRAIN_T0_DIAG), e0) == kept
+        o.close()
+    prob, f = beside_problem(FLAG_ELASTOSTATIC | dyn)
+    o = oracle_lib.Oracle(prob)
+    for fid, v in f.items():
+        o.set(fid, v)
+    up = 1e-5 * np.random.default_rng(5).standard_normal((3,) + prob.shape)
+    o.set(FIELD_STRAIN_T0_DIAG, up)
+    o.demag()
+    assert np.array_equal(o.get(FIELD_STRAIN_T0_DIAG), up)
+    o.close()
