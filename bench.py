@@ -234,6 +234,37 @@ def bench_adjoint(shape, nsteps: int) -> dict:
     t0 = time.perf_counter()
     grad, ghd = ctx.adjoint_backward()
     t_bwd = time.perf_counter() - t0
+    # the same sweep with the stage stores and fields of as many steps as fit
+    # in 80% of the free HBM kept by the forward pass (all of them when they
+    # fit), so that the reverse sweep recomputes none of those steps
+    import ctypes
+
+    import numpy as np
+    n = shape[0] * shape[1] * shape[2]
+    per_step = 8 * 3 * n * 8
+    hip = ctypes.CDLL("libamdhip64.so")
+    free_b, total_b = ctypes.c_size_t(), ctypes.c_size_t()
+    if hip.hipMemGetInfo(ctypes.byref(free_b), ctypes.byref(total_b)):
+        raise RuntimeError("hipMemGetInfo failed")
+    free = free_b.value
+    mib = int(-(-min(nsteps * per_step, 0.8 * free) // 1048576))
+    resident = {"resident_mib": mib, "steps_resident": min(nsteps, (mib * 1048576) // per_step)}
+    if mib > 0:
+        reset()
+        t0 = time.perf_counter()
+        L_r = ctx.adjoint_forward(1, lo, hi, 3e10, nsteps, resident_mib=mib)
+        t_fwd_r = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        grad_r, ghd_r = ctx.adjoint_backward()
+        t_bwd_r = time.perf_counter() - t0
+        ctx.adjoint_forward(1, lo, hi, 3e10, 1)   # frees the resident buffers
+        resident.update({"forward_s": t_fwd_r, "reverse_s": t_bwd_r,
+                         "adjoint_over_forward": t_bwd_r / t_plain,
+                         "gradient_over_forward": (t_fwd_r + t_bwd_r) / t_plain,
+                         "identical_to_checkpointed": bool(L_r == L and np.array_equal(grad_r, grad)
+                                                           and np.array_equal(ghd_r, ghd))})
+        print(f"bench: adjoint with {resident['steps_resident']} resident steps ({mib} MiB): forward "
+              f"{t_fwd_r:.3f} s, reverse {t_bwd_r:.3f} s", file=sys.stderr, flush=True)
     # the FP32 variant of the same sweep (its state is a copy; the context is
     # reset anyway so that it starts from the same m and time)
     reset()
@@ -247,7 +278,6 @@ def bench_adjoint(shape, nsteps: int) -> dict:
     grad32, ghd32 = ctx.adjoint_backward()
     t_bwd32 = time.perf_counter() - t0
     ctx.close()
-    n = shape[0] * shape[1] * shape[2]
     print(f"bench: adjoint {shape}, {nsteps} steps: plain forward {t_plain:.3f} s, checkpointed forward "
           f"{t_fwd:.3f} s, reverse sweep {t_bwd:.3f} s; FP32 forward {t_fwd32:.3f} s, reverse {t_bwd32:.3f} s",
           file=sys.stderr, flush=True)
@@ -261,7 +291,8 @@ def bench_adjoint(shape, nsteps: int) -> dict:
                            "B1": abs(grad32[2] - grad[2]) / abs(grad[2]),
                            "drive": float(((ghd32 - ghd) ** 2).sum() ** 0.5 / (ghd ** 2).sum() ** 0.5)}}
     return {"workload": f"FM film {shape[0]}x{shape[1]}x{shape[2]} + shaped top-layer drive (30 GHz) + static strain, "
-                        f"probe 16x16x4 spectral power at 30 GHz, {nsteps} steps, sqrt(N) checkpoints",
+                        f"probe 16x16x4 spectral power at 30 GHz, {nsteps} steps, sqrt(N) checkpoints "
+                        f"(and, under 'resident', the stage data of every step that fits kept in HBM)",
             "dtype": "f64", "nsteps": nsteps, "forward_s": t_plain, "checkpointed_forward_s": t_fwd,
             "reverse_s": t_bwd, "adjoint_over_forward": t_bwd / t_plain,
             "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
@@ -269,7 +300,7 @@ def bench_adjoint(shape, nsteps: int) -> dict:
                           "(checkpointed forward + reverse sweep) / plain forward, the whole gradient's cost",
             "forward_cell_steps_per_s": n * nsteps / t_plain, "loss": L,
             "grad": {"alpha": grad[0], "Aex": grad[1], "B1": grad[2]},
-            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5), "fp32": fp32}
+            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5), "resident": resident, "fp32": fp32}
 
 
 def bench_coupled(which: str, steps: int) -> dict:

@@ -26,7 +26,11 @@
 // Checkpointing: the forward pass keeps m every K steps (K = ceil(sqrt N)
 // by default); the reverse pass recomputes one segment with the same
 // kernels (bit-identical trajectory), keeping its stage stores and fields,
-// then walks it backwards.  Per-cell gradient accumulators are summed once
+// then walks it backwards.  Given a memory budget, the forward pass keeps
+// the stage stores and fields of the last M steps that fit it (all N when
+// they do), the reverse sweep walks those without recomputing them, and
+// only the first N - M steps are checkpointed.  The trajectory and the
+// gradients are the same bit for bit.  Per-cell gradient accumulators are summed once
 // at the end in a fixed order, so the gradients are deterministic.
 #include "gf_llg_dev.h"
 
@@ -43,7 +47,10 @@ struct AdjState {
     gf_adjoint_desc d{};
     bool ready = false;            // a forward pass whose reverse sweep can run
     int64_t N = 0, K = 0, nseg = 0;
+    int64_t M = 0;                 // the last M steps are resident; the first N - M are checkpointed
     double* pool = nullptr;        // (nseg + 1) checkpoints of 3n: segment starts, then the final state
+    double* RY = nullptr;          // resident steps: 4M stage stores of 3n (store s of step j at 4j + s - 1)
+    double* RH = nullptr;          // and their 4M magnetostatic fields, with magnetostatics
     std::vector<double> ck_time;   // time at each segment start
     double t_final = 0.;
     std::vector<double> tstep;     // time after each step
@@ -435,6 +442,7 @@ gf_status step_to(gf_ctx* c, double* const* y0, double* const* const st[3], doub
 void free_state(AdjState* s) {
     if (!s) return;
     hipFree(s->pool); hipFree(s->X); hipFree(s->Y); hipFree(s->HD); hipFree(s->work); hipFree(s->gpar);
+    hipFree(s->RY); hipFree(s->RH);
     delete s;
 }
 
@@ -546,6 +554,7 @@ gf_status gf_adjoint_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss) 
             return set_error(c, GF_ERR_ARG, "probe box outside the grid");
     if (d->precision != 0 && d->precision != 64 && d->precision != 32)
         return set_error(c, GF_ERR_ARG, "precision must be 64 or 32");
+    if (d->resident_mib < 0) return set_error(c, GF_ERR_ARG, "resident_mib must be >= 0");
     adjoint_free(c);
     adjoint32_free(c);
     if (d->precision == 32) return adjoint32_forward(c, d, loss);
@@ -553,23 +562,63 @@ gf_status gf_adjoint_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss) 
     c->adj = s;
     s->d = *d;
     s->N = d->nsteps;
-    s->K = d->checkpoint_every > 0 ? d->checkpoint_every
-                                   : static_cast<int64_t>(std::ceil(std::sqrt(static_cast<double>(d->nsteps))));
-    s->K = std::min(s->K, s->N);
-    s->nseg = (s->N + s->K - 1) / s->K;
     const int64_t n = c->n;
+    const bool ms = demag_on(c);
+    // resident steps: the stage stores and fields of each, within the budget
+    const double per_step = static_cast<double>(ms ? 8 : 4) * 3. * static_cast<double>(n) * sizeof(double);
+    const double budget = static_cast<double>(d->resident_mib) * 1048576.;
+    s->M = std::min<int64_t>(s->N, static_cast<int64_t>(std::floor(budget / per_step)));
+    const int64_t P = s->N - s->M;   // checkpointed steps
+    s->K = d->checkpoint_every > 0 ? d->checkpoint_every
+                                   : static_cast<int64_t>(std::ceil(std::sqrt(static_cast<double>(P))));
+    s->K = std::max<int64_t>(1, std::min(s->K, P));
+    s->nseg = (P + s->K - 1) / s->K;
     gf_status st = dalloc(c, &s->pool, (s->nseg + 1) * 3 * n, "hipMalloc adjoint checkpoints");
     if (!st) st = dalloc(c, &s->X, s->N, "hipMalloc probe means");
+    if (!st && s->M > 0) st = dalloc(c, &s->RY, 4 * s->M * 3 * n, "hipMalloc adjoint resident stores");
+    if (!st && s->M > 0 && ms) st = dalloc(c, &s->RH, 4 * s->M * 3 * n, "hipMalloc adjoint resident fields");
     if (st) { adjoint_free(c); return st; }
     s->ck_time.assign(static_cast<size_t>(s->nseg), 0.);
     s->tstep.assign(static_cast<size_t>(s->N), 0.);
-    const bool ms = demag_on(c);
     double* const* h_all[5] = {c->hstat, c->hstat, c->hstat, c->hstat, c->hstat};
     double* const* st3[3] = {c->st_a, c->st_b, c->st_a};
     // stage stores alternate a, b, a as in gf_step; the new state goes to m
     // (base m is read per cell before the cell's own stage-4 write)
     if (ms) st = demag_run_to(c, c->m, c->hstat);
     for (int64_t k = 0; k < s->N && !st; k++) {
+        if (k >= P) {
+            // a resident step: stage stores and fields kept where the
+            // reverse sweep reads them; the new state into m and its field
+            // into Hstat after the last one
+            const int64_t j = k - P;
+            double *ys[4][3], *hs[5][3];
+            for (int q = 0; q < 4; q++) {
+                slot3(s->RY, 4 * j + q, n, ys[q]);
+                if (ms) slot3(s->RH, 4 * j + q, n, hs[q]);
+                else for (int a = 0; a < 3; a++) hs[q][a] = c->hstat[a];
+            }
+            const bool last = k + 1 == s->N;
+            if (ms && !last) slot3(s->RH, 4 * (j + 1), n, hs[4]);
+            else for (int a = 0; a < 3; a++) hs[4][a] = c->hstat[a];
+            double* yn[3];
+            if (last) for (int a = 0; a < 3; a++) yn[a] = c->m[a];
+            else slot3(s->RY, 4 * (j + 1), n, yn);
+            if (j == 0) {
+                st = copy3(c, ys[0], c->m);
+                if (!st && ms) st = copy3(c, hs[0], c->hstat);
+            }
+            double* const* sts[3] = {ys[1], ys[2], ys[3]};
+            double* const* hp[5] = {hs[0], hs[1], hs[2], hs[3], ms ? hs[4] : nullptr};
+            if (!st) st = step_to(c, ys[0], sts, yn, hp);
+            if (!st) {
+                hipLaunchKernelGGL(probe_mean_kernel, dim3(1), dim3(256), 0, c->stream, yn[0], c->ny, c->nz,
+                                   d->probe_lo[0], d->probe_lo[1], d->probe_lo[2], d->probe_hi[0], d->probe_hi[1],
+                                   d->probe_hi[2], s->X + k);
+                st = hip_check(c, hipGetLastError(), "probe mean");
+            }
+            s->tstep[static_cast<size_t>(k)] = c->time;
+            continue;
+        }
         if (k % s->K == 0) {
             double* ck[3];
             slot3(s->pool, k / s->K, n, ck);
@@ -634,7 +683,7 @@ gf_status gf_adjoint_backward(gf_ctx* c, double grad[3], double* grad_hdrive, si
     const int64_t n = c->n, K = s->K;
     const bool ms = demag_on(c);
     gf_status st = ensure_work(c, s);
-    if (!st && s->kbuf != K) {
+    if (!st && s->nseg > 0 && s->kbuf != K) {
         st = dalloc(c, &s->Y, (4 * K + 1) * 3 * n, "hipMalloc adjoint stage stores");
         if (!st && ms) st = dalloc(c, &s->HD, 4 * K * 3 * n, "hipMalloc adjoint stage fields");
         s->kbuf = st ? 0 : K;
@@ -652,9 +701,28 @@ gf_status gf_adjoint_backward(gf_ctx* c, double grad[3], double* grad_hdrive, si
                            c->nx, c->ny, c->nz, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], s->g.back());
         st = hip_check(c, hipGetLastError(), "adjoint init");
     }
+    // one step of the reverse sweep over its four stage stores and fields
+    auto walk_step = [&](int64_t step, double* ybuf, double* hbuf, int64_t j) {
+        gf_status e = GF_OK;
+        for (int stg = 4; stg >= 1 && !e; stg--) {
+            double *ys[3], *hs[3];
+            slot3(ybuf, 4 * j + (stg - 1), n, ys);
+            if (ms) slot3(hbuf, 4 * j + (stg - 1), n, hs);
+            e = stage_vjp(c, s->d.material, stg, ys, ms ? hs : nullptr, kbar, mbar, hbar, ms ? dadj : nullptr,
+                          s->gpar, ghd);
+            const double g = (stg == 1 && step >= 1) ? s->g[static_cast<size_t>(step - 1)] : 0.;
+            if (!e) e = combine(c, stg, mbar, hbar, ms ? dadj : nullptr, lam, acc, kbar, nullptr, lo, hi, g);
+        }
+        return e;
+    };
+    const int64_t P = s->N - s->M;
+    for (int64_t j = s->M - 1; j >= 0 && !st; j--) {
+        c->time = s->tstep[static_cast<size_t>(P + j)];
+        st = walk_step(P + j, s->RY, s->RH, j);
+    }
     std::vector<double> tseg(static_cast<size_t>(K));
     for (int64_t seg = s->nseg - 1; seg >= 0 && !st; seg--) {
-        const int64_t n0 = seg * K, n1 = std::min(s->N, n0 + K), J = n1 - n0;
+        const int64_t n0 = seg * K, n1 = std::min(P, n0 + K), J = n1 - n0;
         // recompute the segment, keeping every stage store and field
         double* y0[3];
         slot3(s->Y, 0, n, y0);
@@ -686,16 +754,7 @@ gf_status gf_adjoint_backward(gf_ctx* c, double grad[3], double* grad_hdrive, si
         // walk the segment backwards
         for (int64_t j = J - 1; j >= 0 && !st; j--) {
             c->time = tseg[static_cast<size_t>(j)];
-            const int64_t step = n0 + j;   // maps y_step -> y_step+1
-            for (int stg = 4; stg >= 1 && !st; stg--) {
-                double *ys[3], *hs[3];
-                slot3(s->Y, 4 * j + (stg - 1), n, ys);
-                if (ms) slot3(s->HD, 4 * j + (stg - 1), n, hs);
-                st = stage_vjp(c, s->d.material, stg, ys, ms ? hs : nullptr, kbar, mbar, hbar, ms ? dadj : nullptr,
-                               s->gpar, ghd);
-                const double g = (stg == 1 && step >= 1) ? s->g[static_cast<size_t>(step - 1)] : 0.;
-                if (!st) st = combine(c, stg, mbar, hbar, ms ? dadj : nullptr, lam, acc, kbar, nullptr, lo, hi, g);
-            }
+            st = walk_step(n0 + j, s->Y, s->HD, j);   // step n0 + j maps y_step -> y_step+1
         }
     }
     // back to the forward's final state with its field

@@ -230,7 +230,7 @@ class _AdjDesc(ctypes.Structure):
         ("struct_size", ctypes.c_uint32), ("material", ctypes.c_int32),
         ("probe_lo", ctypes.c_int32 * 3), ("probe_hi", ctypes.c_int32 * 3),
         ("probe_freq", ctypes.c_double), ("nsteps", ctypes.c_int64), ("checkpoint_every", ctypes.c_int64),
-        ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("resident_mib", ctypes.c_int32),
     ]
 
 
@@ -580,12 +580,15 @@ class Context:
 
     # -- adjoint sweep
     def adjoint_forward(self, material: int, probe_lo, probe_hi, freq: float, nsteps: int,
-                        checkpoint_every: int = 0, precision: int = 64) -> float:
+                        checkpoint_every: int = 0, precision: int = 64, resident_mib: int = 0) -> float:
         """N steps keeping checkpoints; returns the probe spectral loss.
         precision=32 runs the FP32 variant on copies of the state (the
-        context's state and time are left unchanged)."""
+        context's state and time are left unchanged).  resident_mib: device
+        memory for the stage stores and fields of the last steps, which the
+        reverse sweep then walks without recomputing them."""
         d = _AdjDesc()
         d.precision = int(precision)
+        d.resident_mib = int(resident_mib)
         d.struct_size = ctypes.sizeof(_AdjDesc)
         d.material = int(material)
         d.probe_lo[:] = [int(v) for v in probe_lo]

@@ -23,7 +23,7 @@ class ProbeLossSim:
 
     def __init__(self, problem: Problem, m0, material: int, probe_lo, probe_hi, freq: float, nsteps: int,
                  drive_vector, drive_layer: int, static_fields=None, checkpoint_every: int = 0,
-                 precision: int = 64):
+                 precision: int = 64, resident_mib: int = 0):
         self.problem = problem
         self.ctx = Context(problem)
         self.m0 = np.ascontiguousarray(m0, dtype=np.float64)
@@ -32,6 +32,7 @@ class ProbeLossSim:
         self.probe = (tuple(probe_lo), tuple(probe_hi))
         self.freq, self.nsteps, self.every = float(freq), int(nsteps), int(checkpoint_every)
         self.precision = int(precision)
+        self.resident_mib = int(resident_mib)
         self.vec = np.asarray(drive_vector, dtype=np.float64)
         self.layer = int(drive_layer)
         for fid, v in (static_fields or {}).items():
@@ -48,7 +49,7 @@ class ProbeLossSim:
         self.ctx.upload(FIELD_M, self.m0)
         self.ctx.time = 0.0
         return self.ctx.adjoint_forward(self.material, self.probe[0], self.probe[1], self.freq, self.nsteps,
-                                        self.every, precision=self.precision)
+                                        self.every, precision=self.precision, resident_mib=self.resident_mib)
 
     def grads(self):
         """(dL/dalpha, dL/dAex, dL/dB1, dL/damp) of the last loss."""

@@ -471,9 +471,12 @@ typedef struct gf_adjoint_desc {
     int32_t probe_hi[3];
     double probe_freq;         /* Hz */
     int64_t nsteps;            /* N */
-    int64_t checkpoint_every;  /* steps per checkpoint; 0 = ceil(sqrt(N)) */
+    int64_t checkpoint_every;  /* steps per checkpoint of the checkpointed steps; 0 = ceil(sqrt) of their count */
     int32_t precision;         /* 64 (or 0): FP64 sweep; 32: FP32 variant (see gf_adjoint_forward) */
-    int32_t reserved;          /* 0 */
+    int32_t resident_mib;      /* FP64 sweep: device memory (MiB) for the stage stores and fields of the
+                                * last steps, kept by the forward pass so that the reverse sweep does
+                                * not recompute them (8 x 3n doubles a step with magnetostatics, 4
+                                * without); 0 = checkpoint every step */
 } gf_adjoint_desc;
 
 /* Run N steps from the current state (the magnetostatic field is rebuilt

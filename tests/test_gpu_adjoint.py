@@ -293,6 +293,31 @@ def test_checkpoint_spacing_does_not_change_gradients():
         assert np.array_equal(ghd, out[0][2])
 
 
+@pytest.mark.parametrize("demag", [True, False])
+def test_resident_steps_do_not_change_gradients(demag):
+    """Stage stores and fields kept by the forward pass for the last M steps
+    (none, some, all of them) give the same loss, gradients, final state and
+    time, bit for bit, as checkpointing every step."""
+    say("case resident steps, demag", demag)
+    prob = problem(shape=(16, 16, 8), demag=demag)
+    f = inputs(prob.shape)
+    lo, hi = probe(prob.shape)
+    out = []
+    for mib in (0, 1, 3, 100):
+        g = gpu(prob, f)
+        L = g.adjoint_forward(1, lo, hi, FREQ, 12, checkpoint_every=5 if mib == 1 else 0, resident_mib=mib)
+        m_fwd, t_fwd = g.download(FIELD_M), g.time
+        grad, ghd = g.adjoint_backward()
+        assert np.array_equal(g.download(FIELD_M), m_fwd) and g.time == t_fwd
+        g.step(2)
+        out.append((L, grad, ghd, m_fwd, g.download(FIELD_M)))
+        g.close()
+    for L, grad, ghd, m_fwd, m_after in out[1:]:
+        assert L == out[0][0]
+        assert np.array_equal(grad, out[0][1]) and np.array_equal(ghd, out[0][2])
+        assert np.array_equal(m_fwd, out[0][3]) and np.array_equal(m_after, out[0][4])
+
+
 def test_gradients_match_gpu_finite_differences():
     """Central differences of the GPU forward loss at 32x32x8, 60 steps:
     the three scalars and four random drive cells of the top layer."""
@@ -377,6 +402,8 @@ def test_adjoint_rejections():
         g.adjoint_backward()
     with pytest.raises(GoFerroError, match="probe"):
         g.adjoint_forward(1, (0, 0, 0), (9, 0, 0), FREQ, 2)
+    with pytest.raises(GoFerroError, match="resident_mib"):
+        g.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, resident_mib=-1)
     g.close()
     pa, m1, m2 = workloads.afm_block((4, 4, 4))
     a = Context(pa)
