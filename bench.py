@@ -249,11 +249,16 @@ def bench_adjoint(shape, nsteps: int) -> dict:
     free = free_b.value
     mib = int(-(-min(nsteps * per_step, 0.8 * free) // 1048576))
     resident = {"resident_mib": mib, "steps_resident": min(nsteps, (mib * 1048576) // per_step)}
+    L_r = None
     if mib > 0:
         reset()
         t0 = time.perf_counter()
-        L_r = ctx.adjoint_forward(1, lo, hi, 3e10, nsteps, resident_mib=mib)
+        try:
+            L_r = ctx.adjoint_forward(1, lo, hi, 3e10, nsteps, resident_mib=mib)
+        except goferro.GoFerroError as e:   # the budget could not be allocated after all
+            resident["error"] = str(e)
         t_fwd_r = time.perf_counter() - t0
+    if L_r is not None:
         t0 = time.perf_counter()
         grad_r, ghd_r = ctx.adjoint_backward()
         t_bwd_r = time.perf_counter() - t0
