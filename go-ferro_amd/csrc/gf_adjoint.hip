@@ -1,7 +1,7 @@
 // gf_adjoint.hip — discrete adjoint of the FM RK4 trajectory on gfx950.
 //
 // The forward map of one stage is k = f(y) = P (y x H) + Q (y x (y x H))
-// with H = Hd(y) + Ha(y) + Hme(y) + exch S(y) + Hext + Hnu + Hdrive s(t)
+// with H = Hd(y) + Ha(y) + Hme(y) + exch S(y) + Hdmi(y) + Hext + Hnu + Hdrive s(t)
 // (magnetic_solver.cpp:3-229): Hd the FFT convolution with the Newell
 // tensor, Ha the anisotropy, Hme the magnetoelastic field of a static
 // strain, S the exchange stencil and P, Q = -gyro dt / (1 + alpha^2),
@@ -9,7 +9,8 @@
 //   qbar = Q kbar,  pbar = P kbar + qbar x y,
 //   ybar = p x qbar + H x pbar + (dHa/dy)^T Hbar + (dHme/dy)^T Hbar
 //          + S(exch Hbar) + Ms N^T (*) Hbar,       Hbar = pbar x y,
-// where S is symmetric (each face carries one harmonic-mean coefficient),
+// (+ the transposed DMI stencil, which is not symmetric), where S is
+// symmetric (each face carries one harmonic-mean coefficient),
 // exch = 2/(mu0 Ms) of each cell's material, and N^T (*) is the convolution
 // with the transposed, conjugated tensor spectra.  The parameter gradients
 // are those of one material: alpha and B1 at its cells, Aex through the
@@ -142,9 +143,11 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
     double nv[6][3];
     uint32_t ncd[6];
     gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, nv, ncd);
-    const Stencil st = stencil_core<false>(mgx, mgy, mgz, nv, ncd, a.mats, own, kFmSurfShift, -1, a.dx, a.dy, a.dz,
-                                           M.Aex, M.Aex_self, 0.);
-    const double hex0 = M.exch * st.ex[0], hex1 = M.exch * st.ex[1], hex2 = M.exch * st.ex[2];
+    const Stencil st = stencil_core<true>(mgx, mgy, mgz, nv, ncd, a.mats, own, kFmSurfShift, -1, a.dx, a.dy, a.dz,
+                                          M.Aex, M.Aex_self, M.dmi_slope);
+    // exchange and interfacial DMI (zero without it) of the forward stage
+    const double hex0 = M.exch * st.ex[0] + M.dmi_pos * st.d[2], hex1 = M.exch * st.ex[1] + M.dmi_pos * st.d[3];
+    const double hex2 = M.exch * st.ex[2] + M.dmi_neg * (st.d[0] + st.d[1]);
     // the exchange field's derivative in Aex of material a.mat: each
     // non-surface face's difference weighted by d(coefficient)/dAex
     double exd0 = 0., exd1 = 0., exd2 = 0.;
@@ -229,6 +232,45 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
     if (a.rotate) rot_t(a.Tg2c, mc0, mc1, mc2, mg0, mg1, mg2);
     mb0 += mg0; mb1 += mg1; mb2 += mg2;
 
+    // the transposed DMI terms of the cell's own field: the DMI field is
+    // (P d_zx, P d_zy, -P (d_xx + d_yy)) with each d the mean of a backward
+    // and a forward slope of one x or y face: the difference to the
+    // neighbour, or at a surface the slope s = D/(2A) times a component of
+    // the cell itself (magnetic_solver.cpp:1639-1831); the exchange carries
+    // A s times that component at a surface.  The neighbours' parts of the
+    // differences are gathered by the combine kernel.
+    int nxs = 0, nys = 0;
+    {
+        const double s = M.dmi_slope;
+        const double ga = M.dmi_neg * hb2 * 0.5, al = M.dmi_pos * hb0 * 0.5, be = M.dmi_pos * hb1 * 0.5;
+        const double ix = 1. / a.dx, iy = 1. / a.dy;
+        const double kx = M.exch * M.Aex * s * ix, ky = M.exch * M.Aex * s * iy;
+        const bool sf[4] = {((own >> (kFmSurfShift + kXB)) & 1u) != 0, ((own >> (kFmSurfShift + kXF)) & 1u) != 0,
+                            ((own >> (kFmSurfShift + kYB)) & 1u) != 0, ((own >> (kFmSurfShift + kYF)) & 1u) != 0};
+        // x faces: d_xx from the x component, d_zx from the z component
+        for (int side = 0; side < 2; side++) {
+            const double sg = side ? -1. : 1.;   // backward slope c - L, forward R - c
+            if (sf[side]) {
+                mb2 += -ga * s; mb0 += al * s;
+                mb2 += sg * kx * hb0; mb0 += -sg * kx * hb2;
+                nxs++;
+            } else {
+                mb0 += sg * ga * ix; mb2 += sg * al * ix;
+            }
+        }
+        // y faces: d_yy from the y component, d_zy from the z component
+        for (int side = 0; side < 2; side++) {
+            const double sg = side ? -1. : 1.;
+            if (sf[2 + side]) {
+                mb2 += -ga * s; mb1 += be * s;
+                mb2 += sg * ky * hb1; mb1 += -sg * ky * hb2;
+                nys++;
+            } else {
+                mb1 += sg * ga * iy; mb2 += sg * be * iy;
+            }
+        }
+    }
+
     // parameter gradients of material a.mat: P, Q in alpha and Hme linear in
     // B1 at its own cells; the exchange field through the face coefficients
     // (its own faces and its interfaces)
@@ -239,7 +281,14 @@ __global__ void __launch_bounds__(256) adj_point_kernel(AdjPointArgs a) {
         std_(a.galpha, id,
              ldd(a.galpha, id) + dP * (px * kb0 + py * kb1 + pz * kb2) + dQ * (qx * kb0 + qy * kb1 + qz * kb2));
     }
-    std_(a.gaex, id, ldd(a.gaex, id) + M.exch * (hb0 * exd0 + hb1 * exd1 + hb2 * exd2));
+    double gx = M.exch * (hb0 * exd0 + hb1 * exd1 + hb2 * exd2);
+    if (mine && nxs + nys > 0) {
+        // the DMI field's surface slopes, s = D/(2A) with ds/dA = -s/A (the
+        // exchange's A s at a surface does not depend on A)
+        const double r = -M.dmi_slope / M.Aex * 0.5;
+        gx += r * (M.dmi_pos * (hb0 * nxs * mgx + hb1 * nys * mgy) - M.dmi_neg * hb2 * (nxs + nys) * mgz);
+    }
+    std_(a.gaex, id, ldd(a.gaex, id) + gx);
     if (mine && a.strain[0])
         std_(a.gb1, id, ldd(a.gb1, id) + -2. / kMu0 / M.Ms * (hc0 * exx * mx + hc1 * eyy * my + hc2 * ezz * mz));
     if (a.ghdrive[0]) {
@@ -290,6 +339,21 @@ __global__ void __launch_bounds__(256) adj_combine_kernel(AdjCombineArgs a) {
     double nv[6][3];
     uint32_t ncd[6];
     gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, nv, ncd);
+    // the DMI differences of the x and y neighbours that read this cell (a
+    // face that is not a surface): a neighbour below reads it in its forward
+    // slope (+), one above in its backward slope (-)
+    double yd[3] = {0., 0., 0.};
+    {
+        const double ih[2] = {1. / a.dx, 1. / a.dy};
+        for (int f = 0; f < 4; f++) {
+            if (((own >> (kFmSurfShift + f)) & 1u) || !(ncd[f] & kIsFM)) continue;
+            const DevMaterial& Mn = a.mats[(ncd[f] & kMatMask) - 1];
+            const double w = ((f & 1) ? -1. : 1.) * ih[f >> 1];
+            const double ga = Mn.dmi_neg * nv[f][2] * 0.5, td = Mn.dmi_pos * nv[f][f >> 1] * 0.5;
+            yd[f >> 1] += w * ga;   // d_xx (x faces) or d_yy (y faces)
+            yd[2] += w * td;        // d_zx or d_zy
+        }
+    }
     for (int f = 0; f < 6; f++) {
         const double e = (ncd[f] & kIsFM) ? a.mats[(ncd[f] & kMatMask) - 1].exch : 0.;
         for (int c = 0; c < 3; c++) nv[f][c] = e * nv[f][c];
@@ -300,7 +364,7 @@ __global__ void __launch_bounds__(256) adj_combine_kernel(AdjCombineArgs a) {
     const bool probe = x >= static_cast<I>(a.lo[0]) && x <= static_cast<I>(a.hi[0]) && y >= static_cast<I>(a.lo[1]) &&
                        y <= static_cast<I>(a.hi[1]) && z >= static_cast<I>(a.lo[2]) && z <= static_cast<I>(a.hi[2]);
     for (int c = 0; c < 3; c++) {
-        double yb = ldd(a.mbar[c], id) + st.ex[c];
+        double yb = ldd(a.mbar[c], id) + st.ex[c] + yd[c];
         if (a.dadj[0]) yb += M.Ms * ldd(a.dadj[c], id);
         if (a.mode == 0) {
             std_(a.ybar[c], id, yb);
@@ -380,8 +444,6 @@ gf_status check_context(gf_ctx* c, int material) {
                          "the adjoint sweep does not support the elastostatic solver (static strain from m)");
     if (material < 1 || material > static_cast<int>(c->host_mats.size()) || !c->host_mats[material - 1].if_FM)
         return set_error(c, GF_ERR_ARG, "material must name an FM material");
-    for (const auto& m : c->host_mats)
-        if (m.if_FM && m.iDMI != 0.) return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep does not support DMI");
     const gf_material& m = c->host_mats[material - 1];
     if (m.Aex == 0.) return set_error(c, GF_ERR_UNSUPPORTED, "the adjoint sweep needs Aex != 0");
     if (c->mag_next_stage != 1) return set_error(c, GF_ERR_STATE, "the context is in the middle of a step");

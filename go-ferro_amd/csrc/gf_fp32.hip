@@ -784,6 +784,9 @@ gf_status adjoint32_forward(gf_ctx* c, const gf_adjoint_desc* d, double* loss) {
     for (const auto& m : c->host_mats) fm_mats += m.if_FM ? 1 : 0;
     if (fm_mats != 1)
         return set_error(c, GF_ERR_UNSUPPORTED, "the FP32 adjoint sweep supports one FM material (the FP64 sweep several)");
+    for (const auto& m : c->host_mats)
+        if (m.if_FM && m.iDMI != 0.)
+            return set_error(c, GF_ERR_UNSUPPORTED, "the FP32 adjoint sweep does not support DMI (the FP64 sweep does)");
     if ((c->flags & GF_FLAG_MAGNETOSTATICS) && c->demag.ready && c->demag.general)
         return set_error(c, GF_ERR_UNSUPPORTED,
                          "the FP32 adjoint sweep needs power-of-two nx, ny, nz with FFT magnetostatics");

@@ -461,7 +461,8 @@ gf_status gf_kernel_times(gf_ctx* ctx, double ms[GF_K_COUNT], int64_t launches[G
  * and fields) and runs the adjoint of each stage: transposed exchange
  * stencil, adjoint FFT convolution (conjugate-transposed tensor spectra),
  * adjoint LLG cross products, anisotropy and magnetoelastic Jacobians.
- * Supported contexts: FM only (one FM material), no DMI, FFT or no
+ * Supported contexts: FM only (several FM materials; the gradients are
+ * one material's), interfacial DMI with its surface slopes, FFT or no
  * magnetostatics, uniform/shaped sinusoidal drives, optional static strain
  * and rotation; no coupled physics or drive tables. */
 typedef struct gf_adjoint_desc {

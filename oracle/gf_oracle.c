@@ -46,7 +46,8 @@
  *   t1 and t2 swapped, 5 Mur half- and full-stage coefficients swapped,
  *   6 Mur half stages reading the base state for the stage store,
  *   7 elastic Mur half- and full-stage coefficients swapped, 8 the
- *   prescribed layer of the stage-2 update at t instead of t - dt/2.
+ *   prescribed layer of the stage-2 update at t instead of t - dt/2, 9 the
+ *   tangent-linear reference without the DMI slope's derivative in Aex.
  * The default build (0) plants nothing. */
 #ifndef GFO_PLANT
 #define GFO_PLANT 0
@@ -2796,8 +2797,11 @@ int gfo_threads(void) {
  * magnetic_solver.cpp:3-229 written out term by term:
  *   dH = N (*) (Ms dm) + 2/(mu0 Ms) sum Aij d2(dm) + Ja(m) dm + Me dm
  *        + dHdrive s(t) + (dAex/Aex) Hex + (dB1/(mu0 Ms)) f (e_aa m_a)
+ *        + 2D/(mu0 Ms) (d_zx, d_zy, -(d_xx + d_yy))(dm) - (dAex/Aex) (its
+ *          surface-slope part, linear in D/(2A))
  *   dk = P (dm x H + m x dH) + Q (dm x p + m x dp) + dP p + dQ q
- * and the RK4 combination of the tangents.  FM only, one FM material, no DMI. */
+ * and the RK4 combination of the tangents (Hex without the surface slope,
+ * which enters the exchange as A D/(2A)).  FM only, one FM material. */
 static void tl_stage(gfo_ctx* c, int stage, const gf_material* mat, double* const* dms, double* const* dk,
                      const double dpar[3], double* const* dhdrive, double* const* dHd, double* const* dhscratch) {
     int64_t n = c->n, ny = c->ny, nz = c->nz;
@@ -2828,18 +2832,37 @@ static void tl_stage(gfo_ctx* c, int stage, const gf_material* mat, double* cons
         double mc[3], dc[3];
         glb2crt(c, mg[0], mg[1], mg[2], &mc[0], &mc[1], &mc[2]);
         glb2crt(c, dg[0], dg[1], dg[2], &dc[0], &dc[1], &dc[2]);
-        /* primal field (same terms as gfo_mag_get_dm) */
-        double d[4], d2[3][2][3], dd[4], dd2[3][2][3];
-        stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, 0., x, y, z, d, d2);
-        stencil(c, dms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, 0., x, y, z, dd, dd2);
+        /* primal field (same terms as gfo_mag_get_dm).  The stencil is
+         * linear in the field for given A and D, so its tangent is the
+         * stencil of dm; its surface part (the DMI slope s = D/(2A)) is
+         * linear in s, ds/dA = -s/A, and the exchange carries it as A s,
+         * which does not depend on A */
+        double d[4], d2[3][2][3], dd[4], dd2[3][2][3], dn[4], d2n[3][2][3];
+        stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, m->iDMI, x, y, z, d, d2);
+        stencil(c, dms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, m->iDMI, x, y, z, dd, dd2);
+        stencil(c, c->ms, c->surfYZ, c->surfXZ, c->surfXY, 1, m->Aex, 0., x, y, z, dn, d2n);
         double hx[3], dhx[3];
         for (int a = 0; a < 3; a++) {
             hx[a] = 2. / MU0 / Ms * (c->Aij[0][i] * d2[0][0][a] + c->Aij[1][i] * d2[0][1][a] + c->Aij[2][i] * d2[1][0][a]
                                     + c->Aij[3][i] * d2[1][1][a] + c->Aij[4][i] * d2[2][0][a] + c->Aij[5][i] * d2[2][1][a]);
             dhx[a] = 2. / MU0 / Ms * (c->Aij[0][i] * dd2[0][0][a] + c->Aij[1][i] * dd2[0][1][a] + c->Aij[2][i] * dd2[1][0][a]
                                      + c->Aij[3][i] * dd2[1][1][a] + c->Aij[4][i] * dd2[2][0][a] + c->Aij[5][i] * dd2[2][1][a]);
-            dhx[a] += hx[a] / m->Aex * dpar[1];
+            /* the faces between cells, linear in A */
+            const double hxn = 2. / MU0 / Ms * (c->Aij[0][i] * d2n[0][0][a] + c->Aij[1][i] * d2n[0][1][a]
+                                               + c->Aij[2][i] * d2n[1][0][a] + c->Aij[3][i] * d2n[1][1][a]
+                                               + c->Aij[4][i] * d2n[2][0][a] + c->Aij[5][i] * d2n[2][1][a]);
+            dhx[a] += hxn / m->Aex * dpar[1];
         }
+        /* the DMI field (magnetic_solver.cpp, as gfo_mag_get_dm) and its tangent */
+        const double kd = 2. * m->iDMI / MU0 / Ms;
+        double ddA[4];
+#if GFO_PLANT == 9
+        for (int q = 0; q < 4; q++) ddA[q] = dd[q];
+#else
+        for (int q = 0; q < 4; q++) ddA[q] = dd[q] - (d[q] - dn[q]) / m->Aex * dpar[1];
+#endif
+        const double hdm[3] = {kd * d[2], kd * d[3], -kd * (d[0] + d[1])};
+        const double dhdm[3] = {kd * ddA[2], kd * ddA[3], -kd * (ddA[0] + ddA[1])};
         double hac[3] = {0., 0., 0.}, dhac[3] = {0., 0., 0.};
         double X = mc[0], Y = mc[1], Z = mc[2], dX = dc[0], dY = dc[1], dZ = dc[2];
         if (mat->anisotropy_type == 1) {
@@ -2876,8 +2899,8 @@ static void tl_stage(gfo_ctx* c, int stage, const gf_material* mat, double* cons
         crt2glb(c, dhec[0], dhec[1], dhec[2], &dhe[0], &dhe[1], &dhe[2]);
         double H[3], dH[3];
         for (int a = 0; a < 3; a++) {
-            H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + Hext[a] + c->Hnu[a][i];
-            dH[a] = dha[a] + dhe[a] + dhx[a];
+            H[a] = c->H[a][i] + ha[a] + he[a] + hx[a] + hdm[a] + Hext[a] + c->Hnu[a][i];
+            dH[a] = dha[a] + dhe[a] + dhx[a] + dhdm[a];
             if (ms_on) dH[a] += dHd[a][i];
             if (c->flags & GF_FLAG_HDRIVE) {
                 H[a] += c->Hdrive[a][i] * drive_s;
@@ -2906,7 +2929,6 @@ int gfo_tl_probe_loss(gfo_ctx* c, int material, const int lo[3], const int hi[3]
     if (material < 1 || material > c->nmat || !c->mat[material - 1].if_FM) return -1;
     if (c->if_AFM_all || c->if_FE_all || c->if_elasto || c->if_em) return -1;
     const gf_material* mat = &c->mat[material - 1];
-    if (mat->iDMI != 0.) return -1;
     int64_t n = c->n;
     double *dm[3], *dms[3], *dk[4][3], *dHd[3], *dhd[3];
     for (int a = 0; a < 3; a++) {

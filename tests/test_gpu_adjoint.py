@@ -35,21 +35,33 @@ def rot():
     return np.array([[1, 0, 0], [0, c, -s], [0, s, c]]) @ np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
 
 
-def problem(shape=(16, 16, 8), alpha=0.02, aex=3.7e-12, b1=3.5e5, rotate=True, demag=True, anis=1):
+def dmi_cells(shape):
+    """Vacuum cells that give the magnet x and y surfaces away from the probe."""
+    nx, ny, _ = shape
+    cell = np.ones(shape, np.uint32)
+    cell[nx - 2, ny // 2:ny // 2 + 2, :] = 0
+    cell[nx // 2 + 1, 0, 1:] = 0
+    return cell
+
+
+def problem(shape=(16, 16, 8), alpha=0.02, aex=3.7e-12, b1=3.5e5, rotate=True, demag=True, anis=1, dmi=0.0):
+    """With DMI: free x ends and vacuum cells, so that its surface slope acts."""
     mat = Material.make(if_FM=1, anisotropy_type=anis, Ms=1.4e5, gyro=GYRO, FM_damping=alpha, K1=-2e4, K2=3e3,
-                        Aex=aex, B1=b1, B2=-2e5)
+                        Aex=aex, B1=b1, B2=-2e5, iDMI=dmi)
     flags = FLAG_HDRIVE | FLAG_ELASTO_ON_PM | (FLAG_MAGNETOSTATICS if demag else 0)
     if rotate:
         flags |= FLAG_ORIENT_ROTATE
-    cell = np.ones(shape, np.uint32)
+    cell = dmi_cells(shape) if dmi else np.ones(shape, np.uint32)
     return Problem(shape=shape, spacing=(2e-9,) * 3, materials=[mat], material_cell=cell, dt=2e-14,
-                   periodic=(True, True, False), flags=flags, Hext_stat=(0, 0, 0.1 / MU0), Hext_altr=(2e3, 0, 0),
+                   periodic=(not dmi, True, False), flags=flags, Hext_stat=(0, 0, 0.1 / MU0), Hext_altr=(2e3, 0, 0),
                    H_altr_freq=(2e10, 0, 0), Hdrive_freq=FREQ, Tg2c=rot() if rotate else None)
 
 
-def inputs(shape, seed=21):
+def inputs(shape, seed=21, dmi=False):
     rng = np.random.default_rng(seed)
     m0 = workloads.tilted_m(shape, seed, theta_max=0.3)
+    if dmi:
+        m0[:, dmi_cells(shape) == 0] = 0.0
     hd = np.zeros((3,) + shape)
     hd[0, :, :, -1] = rng.uniform(0.5, 1.0, shape[:2]) * 4e4
     diag = rng.standard_normal((3,) + shape) * 1e-3
@@ -77,18 +89,22 @@ def probe(shape):
     return (nx // 4, ny // 4, 0), (nx // 4 + 3, ny // 4 + 3, 1)
 
 
-@pytest.mark.parametrize("nadv,rotate,demag", [(1, True, True), (3, False, True), (5, True, False)])
-def test_stage_vjp_against_oracle_differences(nadv, rotate, demag):
+@pytest.mark.parametrize("nadv,rotate,demag,dmi", [(1, True, True, 0.0), (3, False, True, 0.0), (5, True, False, 0.0),
+                                                   (2, True, True, 1.2e-3), (1, False, False, -8e-4)])
+def test_stage_vjp_against_oracle_differences(nadv, rotate, demag, dmi):
     """<ybar, v> = <kbar, dk/dy v> for the stage increment k = f(y), the
     right side from central differences of the oracle's k (a polynomial of
     low degree in y, so the difference is exact up to rounding), at the
-    stage-1 time after nadv time steps."""
+    stage-1 time after nadv time steps; with interfacial DMI and its
+    surface slopes at free x ends and vacuum cells as well."""
     from oracle.oracle import FIELD_K1
-    say("case stage vjp", nadv, rotate, demag)
-    prob = problem(shape=(8, 8, 4), rotate=rotate, demag=demag)
-    f = inputs(prob.shape)
+    say("case stage vjp", nadv, rotate, demag, dmi)
+    prob = problem(shape=(8, 8, 4), rotate=rotate, demag=demag, dmi=dmi)
+    f = inputs(prob.shape, dmi=bool(dmi))
     rng = np.random.default_rng(40 + nadv)
     y = f[FIELD_M] + 0.05 * rng.standard_normal((3,) + prob.shape)
+    if dmi:
+        y[:, dmi_cells(prob.shape) == 0] = 0.0
     kbar = rng.standard_normal((3,) + prob.shape)
     v = rng.standard_normal((3,) + prob.shape)
     g = gpu(prob, f)
@@ -122,9 +138,11 @@ def test_stage_vjp_against_oracle_differences(nadv, rotate, demag):
         setattr(dn, name, getattr(base, name) - h)
         fd = float((kbar * (k_of(y, up) - k_of(y, dn)) / (2 * h)).sum())
         np.testing.assert_allclose(grad[idx], fd, rtol=1e-6, err_msg=name)
-    # drive profile: k is linear in it
-    o_up = {**f, FIELD_HDRIVE: f[FIELD_HDRIVE] + v}
-    o_dn = {**f, FIELD_HDRIVE: f[FIELD_HDRIVE] - v}
+    # drive profile: k is linear in it, so a large step (1e3 A/m per unit
+    # of v) is exact and keeps the difference far above rounding
+    sv = 1e3
+    o_up = {**f, FIELD_HDRIVE: f[FIELD_HDRIVE] + sv * v}
+    o_dn = {**f, FIELD_HDRIVE: f[FIELD_HDRIVE] - sv * v}
 
     def k_drive(ff):
         o = oracle(prob, {**ff, FIELD_M: y})
@@ -137,7 +155,7 @@ def test_stage_vjp_against_oracle_differences(nadv, rotate, demag):
         o.close()
         return k
 
-    fd = float((kbar * (k_drive(o_up) - k_drive(o_dn)) / 2).sum())
+    fd = float((kbar * (k_drive(o_up) - k_drive(o_dn)) / (2 * sv)).sum())
     np.testing.assert_allclose(float((ghd * v).sum()), fd, rtol=1e-7)
     g.close()
 
@@ -249,10 +267,11 @@ def run_adjoint(prob, fields, nsteps, K=0):
     return g, L, grad, ghd
 
 
-def test_gradients_match_tangent_linear_reference():
-    say("case adjoint vs tangent linear 16x16x8, 30 steps, K=7")
-    prob = problem()
-    f = inputs(prob.shape)
+@pytest.mark.parametrize("dmi", [0.0, 1.2e-3])
+def test_gradients_match_tangent_linear_reference(dmi):
+    say("case adjoint vs tangent linear 16x16x8, 30 steps, K=7, DMI", dmi)
+    prob = problem(dmi=dmi)
+    f = inputs(prob.shape, dmi=bool(dmi))
     N = 30
     g, L, grad, ghd = run_adjoint(prob, f, N, K=7)
     lo, hi = probe(prob.shape)
@@ -372,6 +391,51 @@ def test_gradients_match_gpu_finite_differences():
     g.close()
 
 
+def test_dmi_sweep_against_gpu_differences():
+    """Interfacial DMI with its surface slopes (free x ends, vacuum cells):
+    the FP64 sweep's alpha and Aex gradients and the drive gradient of two
+    top-layer cells against central differences of the GPU forward loss,
+    16x16x8, 40 steps."""
+    say("case adjoint with DMI vs gpu finite differences")
+    shape = (16, 16, 8)
+    prob = problem(shape=shape, dmi=1.2e-3)
+    f = inputs(shape, dmi=True)
+    N = 40
+    lo, hi = probe(shape)
+    g = gpu(prob, f)
+    L0 = g.adjoint_forward(1, lo, hi, FREQ, N, resident_mib=64)
+    grad, ghd = g.adjoint_backward()
+
+    def loss(p=None, hd=None):
+        if p is not None:
+            g.set_material(1, p)
+        g.upload(FIELD_M, f[FIELD_M])
+        g.upload(FIELD_HDRIVE, f[FIELD_HDRIVE] if hd is None else hd)
+        g.time = 0.0
+        return g.adjoint_forward(1, lo, hi, FREQ, N)
+
+    base = clone(prob.materials[0])
+    for idx, name in ((0, "FM_damping"), (1, "Aex")):
+        h = 1e-5 * getattr(base, name)
+        up, dn = clone(base), clone(base)
+        setattr(up, name, getattr(base, name) + h)
+        setattr(dn, name, getattr(base, name) - h)
+        fd = (loss(up) - loss(dn)) / (2 * h)
+        loss(base)
+        say("  ", name, "adjoint", grad[idx], "fd", fd)
+        np.testing.assert_allclose(grad[idx], fd, rtol=1e-6)
+    for i, j in ((3, 5), (13, 9)):
+        h = 2.0 * f[FIELD_HDRIVE][0, i, j, -1]
+        up, dn = f[FIELD_HDRIVE].copy(), f[FIELD_HDRIVE].copy()
+        up[0, i, j, -1] += h
+        dn[0, i, j, -1] -= h
+        fd = (loss(hd=up) - loss(hd=dn)) / (2 * h)
+        say("  drive cell", i, j, "adjoint", ghd[0, i, j, -1], "fd", fd)
+        np.testing.assert_allclose(ghd[0, i, j, -1], fd, rtol=1e-6)
+    assert L0 > 0
+    g.close()
+
+
 def test_backward_restores_the_forward_final_state():
     say("case state restore")
     prob = problem(shape=(8, 8, 4))
@@ -412,12 +476,14 @@ def test_adjoint_rejections():
     with pytest.raises(GoFerroError, match="ferromagnet-only"):
         a.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2)
     a.close()
-    pd = problem(shape=(4, 4, 4))
+    # DMI: the FP64 sweep takes it, the FP32 variant refuses it
+    pd = problem(shape=(4, 4, 4), rotate=False)
     mat = pd.materials[0]
     mat.iDMI = 1e-3
     d = gpu(pd, f)
+    d.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2)
     with pytest.raises(GoFerroError, match="DMI"):
-        d.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2)
+        d.adjoint_forward(1, (0, 0, 0), (1, 1, 1), FREQ, 2, precision=32)
     d.close()
     # the elastostatic solver recomputes the static strain from m after
     # every step, which the sweep would not differentiate: refused, in FP64
