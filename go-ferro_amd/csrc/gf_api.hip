@@ -1984,17 +1984,11 @@ gf_status gf_em_stage(gf_ctx* c, int stage) {
         }
         if (write) jf_comp = c->jf_comp;
     }
-    // stage 4: the planes the Mur walls read, before the combine
-    // (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
-    if (!st && stage == 4 && !c->em_periodic_all) {
-        EmMiscArgs h{};
-        for (int i = 0; i < 3; i++) { h.src[i] = c->Eb[i]; h.t1[i] = c->Et1[i]; h.per[i] = c->per[i]; }
-        h.nx = nx; h.ny = ny; h.nz = nz;
-        timer_begin(c, GF_K_OTHER, &ev);
-        st = hip_check(c, launch_em_history(h, c->stream), "EM history");
-        timer_end(c, GF_K_OTHER, ev);
-    }
     EmArgs a{};
+    // stage 4: the E update keeps the planes the Mur walls read before the
+    // combine (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
+    const bool hist = stage == 4 && !c->em_periodic_all;
+    for (int i = 0; i < 3; i++) a.t1[i] = hist ? c->Et1[i] : nullptr;
     a.code = c->d_code; a.mats = c->d_mats;
     for (int i = 0; i < 3; i++) {
         a.es[i] = es[i]; a.hs[i] = hs[i];

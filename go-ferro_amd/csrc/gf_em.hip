@@ -2,15 +2,16 @@
 //
 // One stage of EMdynamic_system (EMdynamic_solver.cpp,
 // EMdynamic_AbsorbingBoundary.cpp) as a handful of launches:
-//   history   stage 4 only: E on the boundary planes before the combine
-//             (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56)
 //   eh        Ampere per edge with the four-cell means of permittivity,
 //             conductivity, Jf, Jp and dp (get_dE_RK*, :337-2237), fused
 //             with the RK update (update_DE_RK*, :3588-3777); and in the
 //             same launch Faraday per face with dM of the two cells
 //             (get_dH_RK*, :2239-3475), fused with update_DH_RK*
 //             (:3477-3586).  The free current of the box is evaluated where
-//             an edge reads it (update_Jf_input*, :3885-4315).
+//             an edge reads it (update_Jf_input*, :3885-4315).  At stage 4
+//             each edge on a plane a Mur wall reads keeps its base value
+//             before the combine (transfer_pointer, EMdynamic_Absorbing
+//             Boundary.cpp:3-56).
 //   jp        Drude/Lorentz currents (get_dJp_RK*, update_Jp*, :4318-4524)
 //   plane     hard plane source (update_planeEM*, :3829-3883)
 //   wall x3   first-order Mur or PEC walls, x then y then z (:58-874)
@@ -195,33 +196,6 @@ __global__ void __launch_bounds__(256) em_wall_kernel(EmWallArgs a) {
     }
 }
 
-// Stage 4: E on the planes the Mur walls read, before the combine.
-__global__ void __launch_bounds__(256) em_history_kernel(EmMiscArgs a) {
-    const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t nEx = nx * (ny + 1) * (nz + 1), nEy = (nx + 1) * ny * (nz + 1), nEz = (nx + 1) * (ny + 1) * nz;
-    int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= nEx + nEy + nEz) return;
-    auto near = [](int64_t v, int64_t n) { return v <= 1 || v >= n - 1; };
-    int comp;
-    bool need;
-    if (e < nEx) {
-        comp = 0;
-        const int64_t j = (e / (nz + 1)) % (ny + 1), k = e % (nz + 1);
-        need = (!a.per[1] && near(j, ny)) || (!a.per[2] && near(k, nz));
-    } else if (e < nEx + nEy) {
-        comp = 1;
-        e -= nEx;
-        const int64_t i = e / (ny * (nz + 1)), k = e % (nz + 1);
-        need = (!a.per[0] && near(i, nx)) || (!a.per[2] && near(k, nz));
-    } else {
-        comp = 2;
-        e -= nEx + nEy;
-        const int64_t i = e / ((ny + 1) * nz), j = (e / nz) % (ny + 1);
-        need = (!a.per[0] && near(i, nx)) || (!a.per[1] && near(j, ny));
-    }
-    if (need) a.t1[comp][e] = a.src[comp][e];
-}
-
 __global__ void __launch_bounds__(256) em_plane_kernel(EmMiscArgs a) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -322,12 +296,6 @@ hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s) {
     else if (a.axis == 1) n = std::max(a.nx * (a.nz + 1), (a.nx + 1) * a.nz);
     else n = std::max(a.nx * (a.ny + 1), (a.nx + 1) * a.ny);
     hipLaunchKernelGGL(em_wall_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s) {
-    const int64_t n = a.nx * (a.ny + 1) * (a.nz + 1) + (a.nx + 1) * a.ny * (a.nz + 1) + (a.nx + 1) * (a.ny + 1) * a.nz;
-    hipLaunchKernelGGL(em_history_kernel, dim3(blocks_for(n)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -46,14 +46,22 @@ __device__ __forceinline__ double edge_increment(const EmArgs& a, int comp, cons
                                                  const int64_t cj[4], const int64_t ck[4], double curl,
                                                  double e_store) {
     double Jf = 0., Jp = 0., Ji = 0., dP = 0., er = 0., cond = 0., pc = 0.;
+    // the codes and the q store of the four cells are read together, before
+    // any code is known (every index is in bounds)
+    uint32_t tq[4];
+    double qv[4] = {0., 0., 0., 0.};
+    for (int q = 0; q < 4; q++) {
+        tq[q] = a.code[cell[q]] & kMatMask;
+        if (a.q[0]) qv[q] = a.q[comp][cell[q]];
+    }
     for (int q = 0; q < 4; q++) {
         const int64_t id = cell[q];
-        const uint32_t t = a.code[id] & kMatMask;
+        const uint32_t t = tq[q];
         if (!t) {
             er = er + 1.;
         } else {
             const DevMaterial& M = a.mats[t - 1];
-            if (a.q[0]) dP = dP + a.q[comp][id] * a.dt;
+            if (a.q[0]) dP = dP + qv[q] * a.dt;
             if (M.is_fe) pc = pc + 1.;
             er = er + M.em_er;
             cond = cond + M.em_cond;
@@ -82,9 +90,12 @@ __device__ __forceinline__ void em_edge(const EmArgs& a, int64_t e) {
     auto HZ = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * (nz + 1) + k; };
     int comp;
     double k_ = 0.;
+    bool hist = false;   // the edge lies on a plane a Mur wall reads (two next to each non-periodic wall)
+    auto near = [](int64_t v, int64_t n) { return v <= 1 || v >= n - 1; };
     if (e < nEx) {
         comp = 0;
         const int64_t i = e / ((ny + 1) * (nz + 1)), j = (e / (nz + 1)) % (ny + 1), k = e % (nz + 1);
+        hist = (!a.per[1] && near(j, ny)) || (!a.per[2] && near(k, nz));
         const bool wj = j == 0 || j == ny, wk = k == 0 || k == nz;
         if (!(wj && !a.per[1]) && !(wk && !a.per[2])) {
             const int64_t y1 = wj ? ny - 1 : j - 1, y3 = wj ? 0 : j, z1 = wk ? nz - 1 : k - 1, z2 = wk ? 0 : k;
@@ -98,6 +109,7 @@ __device__ __forceinline__ void em_edge(const EmArgs& a, int64_t e) {
         comp = 1;
         e -= nEx;
         const int64_t i = e / (ny * (nz + 1)), j = (e / (nz + 1)) % ny, k = e % (nz + 1);
+        hist = (!a.per[0] && near(i, nx)) || (!a.per[2] && near(k, nz));
         const bool wi = i == 0 || i == nx, wk = k == 0 || k == nz;
         if (!(wi && !a.per[0]) && !(wk && !a.per[2])) {
             const int64_t x1 = wi ? nx - 1 : i - 1, x3 = wi ? 0 : i, z1 = wk ? nz - 1 : k - 1, z2 = wk ? 0 : k;
@@ -111,6 +123,7 @@ __device__ __forceinline__ void em_edge(const EmArgs& a, int64_t e) {
         comp = 2;
         e -= nEx + nEy;
         const int64_t i = e / ((ny + 1) * nz), j = (e / nz) % (ny + 1), k = e % nz;
+        hist = (!a.per[0] && near(i, nx)) || (!a.per[1] && near(j, ny));
         const bool wi = i == 0 || i == nx, wj = j == 0 || j == ny;
         if (!(wi && !a.per[0]) && !(wj && !a.per[1])) {
             const int64_t x1 = wi ? nx - 1 : i - 1, x3 = wi ? 0 : i, y1 = wj ? ny - 1 : j - 1, y2 = wj ? 0 : j;
@@ -121,6 +134,10 @@ __device__ __forceinline__ void em_edge(const EmArgs& a, int64_t e) {
             k_ = edge_increment(a, 2, cell, ci, cj, ck, dHydx - dHxdy, a.es[2][e]);
         }
     }
+    // stage 4: the base value before the combine overwrites it, for the
+    // Mur walls (transfer_pointer, EMdynamic_AbsorbingBoundary.cpp:3-56);
+    // each edge's own thread reads and then writes its base
+    if (a.t1[0] && hist) a.t1[comp][e] = a.eb[comp][e];
     rk_state(a.stage, e, k_, a.eb[comp], a.eacc[comp], a.eo[comp]);
 }
 

@@ -120,6 +120,7 @@ struct EmArgs {
     const double* dma1[3];
     const double* dma2[3];
     const double* jishe[2];     // inverse spin Hall current Jx, Jy, null when off
+    double* t1[3];              // stage 4 with Mur walls: E of the planes the walls read, kept before the combine
     int64_t nx, ny, nz;
     double dx, dy, dz, dt;
     int32_t per[3];
@@ -789,7 +790,6 @@ struct SpinSaveArgs {
 hipError_t launch_spin_save(const SpinSaveArgs& a, hipStream_t s);
 hipError_t launch_fe_flexo(const FeFlexoArgs& a, hipStream_t s);
 hipError_t launch_em_wall(const EmWallArgs& a, hipStream_t s);
-hipError_t launch_em_history(const EmMiscArgs& a, hipStream_t s);
 hipError_t launch_em_plane(const EmMiscArgs& a, hipStream_t s);
 hipError_t launch_em_pec(const EmMiscArgs& a, hipStream_t s);
 
