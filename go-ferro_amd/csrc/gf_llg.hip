@@ -55,6 +55,69 @@ __device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z,
     for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd_nt(a.acc[c], id) : 0.;
 }
 
+// A double moved one lane up (lane i gets lane i - 1) or down (lane i
+// gets lane i + 1) across the 64-lane wave, with DPP wave shifts.
+__device__ __forceinline__ double lane_up(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_down(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ uint32_t lane_up(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint32_t lane_down(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xf, 0xf, false));
+}
+
+// fm_load for the fused stage, whose cells run along z in consecutive
+// lanes (cell w of the workgroup at lane w % 64, k = w % nz): a cell's z
+// neighbours are the store values of the lanes next to it, except at a
+// column end (k = 0 or nz - 1, where the neighbour wraps or the face is a
+// surface) or a wave edge, where they are loaded.  Every lane of the wave
+// that holds a cell runs the lane shifts; a lane next to one without a
+// cell is at a column end.
+template <class I>
+__device__ __forceinline__ void fm_load_fused(const StageArgs& a, I id, I x, I y, I z, FmIn& in) {
+    const I nx = static_cast<I>(a.nx), ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
+    in.own = ldu(a.code, id);
+    for (int c = 0; c < 3; c++) in.c[c] = ldd(a.store[c], id);
+    const I nb[4] = {((x == 0 ? nx - 1 : x - 1) * ny + y) * nz + z, ((x == nx - 1 ? 0 : x + 1) * ny + y) * nz + z,
+                     (x * ny + (y == 0 ? ny - 1 : y - 1)) * nz + z, (x * ny + (y == ny - 1 ? 0 : y + 1)) * nz + z};
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        for (int c = 0; c < 3; c++) in.v[f][c] = ldd(a.store[c], nb[f]);
+        in.cd[f] = ldu(a.code, nb[f]);
+    }
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const bool gm = lane == 0 || z == 0, gp = lane == 63 || z == nz - 1;
+    if (gm) {
+        const I zm = (x * ny + y) * nz + (z == 0 ? nz - 1 : z - 1);
+        for (int c = 0; c < 3; c++) in.v[4][c] = ldd(a.store[c], zm);
+        in.cd[4] = ldu(a.code, zm);
+    }
+    if (gp) {
+        const I zp = (x * ny + y) * nz + (z == nz - 1 ? 0 : z + 1);
+        for (int c = 0; c < 3; c++) in.v[5][c] = ldd(a.store[c], zp);
+        in.cd[5] = ldu(a.code, zp);
+    }
+    const bool same = a.base[0] == a.store[0] && a.base[1] == a.store[1] && a.base[2] == a.store[2];
+    for (int c = 0; c < 3; c++) in.m[c] = same ? in.c[c] : ldd_nt(a.base[c], id);
+    for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd_nt(a.acc[c], id) : 0.;
+    for (int c = 0; c < 3; c++) {
+        const double dn = lane_up(in.c[c]), up = lane_down(in.c[c]);
+        if (!gm) in.v[4][c] = dn;
+        if (!gp) in.v[5][c] = up;
+    }
+    const uint32_t cdn = lane_up(in.own), cup = lane_down(in.own);
+    if (!gm) in.cd[4] = cdn;
+    if (!gp) in.cd[5] = cup;
+}
+
 // RK update of one component (see the file comment) from its preloaded m
 // and accumulator; returns the value written to the next store (m itself
 // at stage 4): the RK value, or pv where the cell's value is prescribed.
@@ -310,7 +373,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
         double* h2 = reinterpret_cast<double*>(cbuf + fpad(2 * pairs * nz + pos)) + part;
         const I id = (static_cast<I>(x) * static_cast<I>(ny) + static_cast<I>(y)) * static_cast<I>(nz) + static_cast<I>(k);
         FmIn in;
-        fm_load<I>(a, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), in);
+        fm_load_fused<I>(a, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), in);
         double mx = 0., my = 0., mz = 0.;
         if (in.own & kIsFM) {
             double n0, n1, n2;
