@@ -30,31 +30,6 @@ constexpr int kSetupChunk = 4;
 static_assert(kLeanMats * sizeof(DevMaterial) / sizeof(double) <= kMatWordsPerThread * 256,
               "the lean stage's material records must fit the set-up loads of 256 threads");
 
-// Every global input of one FM cell of a stage except the static field:
-// loaded together, before the cell's own code is known, so that one
-// memory round trip feeds the cell (all indices are in bounds whatever the
-// code says).
-struct FmIn {
-    uint32_t own;
-    double c[3];        // stage store at the cell
-    double v[6][3];     // stage store at the six face neighbours (gather6)
-    uint32_t cd[6];     // their codes
-    double m[3];        // m at the start of the step
-    double acc[3];      // RK accumulator (read at stages 2-4)
-};
-
-template <class I>
-__device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z, FmIn& in) {
-    in.own = ldu(a.code, id);
-    const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
-    for (int c = 0; c < 3; c++) in.c[c] = ldd(a.store[c], id);
-    gather6<I>(sp, a.code, x, y, z, static_cast<I>(a.nx), static_cast<I>(a.ny), static_cast<I>(a.nz), in.v, in.cd);
-    // at stage 1 the stage store is m itself: read it once
-    const bool same = a.base[0] == a.store[0] && a.base[1] == a.store[1] && a.base[2] == a.store[2];
-    for (int c = 0; c < 3; c++) in.m[c] = same ? in.c[c] : ldd_nt(a.base[c], id);
-    for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd_nt(a.acc[c], id) : 0.;
-}
-
 // A double moved one lane up (lane i gets lane i - 1) or down (lane i
 // gets lane i + 1) across the 64-lane wave, with DPP wave shifts.
 __device__ __forceinline__ double lane_up(double v) {
@@ -74,48 +49,81 @@ __device__ __forceinline__ uint32_t lane_down(uint32_t v) {
     return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xf, 0xf, false));
 }
 
-// fm_load for the fused stage, whose cells run along z in consecutive
-// lanes (cell w of the workgroup at lane w % 64, k = w % nz): a cell's z
-// neighbours are the store values of the lanes next to it, except at a
-// column end (k = 0 or nz - 1, where the neighbour wraps or the face is a
-// surface) or a wave edge, where they are loaded.  Every lane of the wave
-// that holds a cell runs the lane shifts; a lane next to one without a
-// cell is at a column end.
+// gather6 for launches whose active lanes hold consecutive cells along z
+// (lane i of a wave at cell id0 + i, and a lane without a cell only past
+// the grid's last cell): a cell's z neighbours are then the store values
+// and codes of the lanes next to it, moved with DPP wave shifts, except at
+// a column end (k = 0 or nz - 1, where the neighbour wraps or the face is a
+// surface) or a wave edge, where they are loaded.  A lane next to one
+// without a cell is at a column end.  Two parts: the loads, and after the
+// cell's other loads have been issued, the lane shifts of its own store c
+// and code own.
 template <class I>
-__device__ __forceinline__ void fm_load_fused(const StageArgs& a, I id, I x, I y, I z, FmIn& in) {
-    const I nx = static_cast<I>(a.nx), ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
-    in.own = ldu(a.code, id);
-    for (int c = 0; c < 3; c++) in.c[c] = ldd(a.store[c], id);
+__device__ __forceinline__ void gather6_lanes_load(const P3 s3, const uint32_t* code, I x, I y, I z, I nx, I ny,
+                                                   I nz, double (&v)[6][3], uint32_t (&cd)[6]) {
     const I nb[4] = {((x == 0 ? nx - 1 : x - 1) * ny + y) * nz + z, ((x == nx - 1 ? 0 : x + 1) * ny + y) * nz + z,
                      (x * ny + (y == 0 ? ny - 1 : y - 1)) * nz + z, (x * ny + (y == ny - 1 ? 0 : y + 1)) * nz + z};
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        for (int c = 0; c < 3; c++) in.v[f][c] = ldd(a.store[c], nb[f]);
-        in.cd[f] = ldu(a.code, nb[f]);
+        for (int q = 0; q < 3; q++) v[f][q] = ldd(s3.p[q], nb[f]);
+        cd[f] = ldu(code, nb[f]);
     }
     const int lane = static_cast<int>(threadIdx.x & 63u);
     const bool gm = lane == 0 || z == 0, gp = lane == 63 || z == nz - 1;
     if (gm) {
         const I zm = (x * ny + y) * nz + (z == 0 ? nz - 1 : z - 1);
-        for (int c = 0; c < 3; c++) in.v[4][c] = ldd(a.store[c], zm);
-        in.cd[4] = ldu(a.code, zm);
+        for (int q = 0; q < 3; q++) v[4][q] = ldd(s3.p[q], zm);
+        cd[4] = ldu(code, zm);
     }
     if (gp) {
         const I zp = (x * ny + y) * nz + (z == nz - 1 ? 0 : z + 1);
-        for (int c = 0; c < 3; c++) in.v[5][c] = ldd(a.store[c], zp);
-        in.cd[5] = ldu(a.code, zp);
+        for (int q = 0; q < 3; q++) v[5][q] = ldd(s3.p[q], zp);
+        cd[5] = ldu(code, zp);
     }
+}
+
+template <class I>
+__device__ __forceinline__ void gather6_lanes_shift(I z, I nz, const double (&c)[3], uint32_t own, double (&v)[6][3],
+                                                    uint32_t (&cd)[6]) {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const bool gm = lane == 0 || z == 0, gp = lane == 63 || z == nz - 1;
+    for (int q = 0; q < 3; q++) {
+        const double dn = lane_up(c[q]), up = lane_down(c[q]);
+        if (!gm) v[4][q] = dn;
+        if (!gp) v[5][q] = up;
+    }
+    const uint32_t cdn = lane_up(own), cup = lane_down(own);
+    if (!gm) cd[4] = cdn;
+    if (!gp) cd[5] = cup;
+}
+
+// Every global input of one FM cell of a stage except the static field:
+// loaded together, before the cell's own code is known, so that one
+// memory round trip feeds the cell (all indices are in bounds whatever the
+// code says).
+struct FmIn {
+    uint32_t own;
+    double c[3];        // stage store at the cell
+    double v[6][3];     // stage store at the six face neighbours (gather6)
+    uint32_t cd[6];     // their codes
+    double m[3];        // m at the start of the step
+    double acc[3];      // RK accumulator (read at stages 2-4)
+};
+
+// kLanes: the launch's lanes hold consecutive cells (see gather6_lanes).
+template <class I, bool kLanes = false>
+__device__ __forceinline__ void fm_load(const StageArgs& a, I id, I x, I y, I z, FmIn& in) {
+    in.own = ldu(a.code, id);
+    const P3 sp = {{a.store[0], a.store[1], a.store[2]}};
+    for (int c = 0; c < 3; c++) in.c[c] = ldd(a.store[c], id);
+    const I nx = static_cast<I>(a.nx), ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
+    if (kLanes) gather6_lanes_load<I>(sp, a.code, x, y, z, nx, ny, nz, in.v, in.cd);
+    else gather6<I>(sp, a.code, x, y, z, nx, ny, nz, in.v, in.cd);
+    // at stage 1 the stage store is m itself: read it once
     const bool same = a.base[0] == a.store[0] && a.base[1] == a.store[1] && a.base[2] == a.store[2];
     for (int c = 0; c < 3; c++) in.m[c] = same ? in.c[c] : ldd_nt(a.base[c], id);
     for (int c = 0; c < 3; c++) in.acc[c] = a.stage > 1 ? ldd_nt(a.acc[c], id) : 0.;
-    for (int c = 0; c < 3; c++) {
-        const double dn = lane_up(in.c[c]), up = lane_down(in.c[c]);
-        if (!gm) in.v[4][c] = dn;
-        if (!gp) in.v[5][c] = up;
-    }
-    const uint32_t cdn = lane_up(in.own), cup = lane_down(in.own);
-    if (!gm) in.cd[4] = cdn;
-    if (!gp) in.cd[5] = cup;
+    if (kLanes) gather6_lanes_shift<I>(z, nz, in.c, in.own, in.v, in.cd);
 }
 
 // RK update of one component (see the file comment) from its preloaded m
@@ -227,13 +235,13 @@ __device__ __forceinline__ void fm_cell(const StageArgs& a, MT* mats, const FmIn
 
 // One FM cell of the unfused stage: coordinates from the linear index,
 // magnetostatic field from Hstat.
-template <class I>
+template <class I, bool kLanes = false>
 __device__ __forceinline__ void fm_point(const StageArgs& a, I id) {
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
     const I x = col / ny, y = col - x * ny;
     FmIn in;
-    fm_load<I>(a, id, x, y, z, in);
+    fm_load<I, kLanes>(a, id, x, y, z, in);
     // a null Hstat is zero (no statics, nothing uploaded)
     const bool hs = a.hstat[0] != nullptr;
     const double hs0 = hs ? ldd(a.hstat[0], id) : 0., hs1 = hs ? ldd(a.hstat[1], id) : 0.;
@@ -248,7 +256,7 @@ __global__ void __launch_bounds__(256) mag_stage_kernel(StageArgs a) {
     const I n = static_cast<I>(a.nx * a.ny * a.nz);
     const I id = static_cast<I>(blockIdx.x) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
     if (id >= n) return;
-    fm_point<I>(a, id);
+    fm_point<I, true>(a, id);
 }
 
 // The same over a list of the FM cells: a layered magnet fills whole
@@ -373,7 +381,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
         double* h2 = reinterpret_cast<double*>(cbuf + fpad(2 * pairs * nz + pos)) + part;
         const I id = (static_cast<I>(x) * static_cast<I>(ny) + static_cast<I>(y)) * static_cast<I>(nz) + static_cast<I>(k);
         FmIn in;
-        fm_load_fused<I>(a, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), in);
+        fm_load<I, true>(a, id, static_cast<I>(x), static_cast<I>(y), static_cast<I>(k), in);
         double mx = 0., my = 0., mz = 0.;
         if (in.own & kIsFM) {
             double n0, n1, n2;
@@ -419,7 +427,7 @@ __global__ void __launch_bounds__(256) mag_stage_fused_kernel(StageArgs a, Fused
 // input of the sublattice is loaded before the cell's code is known (all
 // indices are in bounds), as for the FM cell, so one memory round trip
 // feeds the cell.
-template <class I>
+template <class I, bool kLanes = false>
 __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) {
     const I ny = static_cast<I>(a.ny), nz = static_cast<I>(a.nz);
     const I col = id / nz, z = id - col * nz;
@@ -434,7 +442,8 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) 
     const double mgx = ldd(st.p[0], id), mgy = ldd(st.p[1], id), mgz = ldd(st.p[2], id);
     double v[6][3];
     uint32_t cd[6];
-    gather6<I>(st, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, v, cd);
+    if (kLanes) gather6_lanes_load<I>(st, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, v, cd);
+    else gather6<I>(st, a.code, x, y, z, static_cast<I>(a.nx), ny, nz, v, cd);
     const double o0 = ldd(other.p[0], id), o1 = ldd(other.p[1], id), o2 = ldd(other.p[2], id);
     // a null Hstat is zero (no statics, nothing uploaded)
     const bool hs = a.hstat[0] != nullptr;
@@ -447,6 +456,10 @@ __device__ __forceinline__ void afm_point(const AfmStageArgs& a, I id, int sub) 
     mb[1] = same ? mgy : ldd_nt(base[1], id);
     mb[2] = same ? mgz : ldd_nt(base[2], id);
     for (int c = 0; c < 3; c++) ac[c] = a.stage > 1 ? ldd_nt(acc[c], id) : 0.;
+    if (kLanes) {
+        const double c3[3] = {mgx, mgy, mgz};
+        gather6_lanes_shift<I>(z, nz, c3, own, v, cd);
+    }
     if (!(own & kIsAFM)) return;
     GMat& M = mat_of(a.mats, own);
     double mx = mgx, my = mgy, mz = mgz;
@@ -508,7 +521,7 @@ __global__ void __launch_bounds__(256) afm_stage_kernel(AfmStageArgs a) {
     const I n = static_cast<I>(a.nx * a.ny * a.nz);
     const I id = static_cast<I>(blk) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
     if (id >= n) return;
-    afm_point<I>(a, id, sub);
+    afm_point<I, true>(a, id, sub);
 }
 
 // Batched stages: one launch covers every instance of a batch.  The stage
@@ -528,7 +541,7 @@ __global__ void __launch_bounds__(256) mag_batch_kernel(const StageArgs* __restr
         const I first = static_cast<I>(b - L.block0[inst]) * chunk + static_cast<I>(threadIdx.x);
         for (int j = 0; j < L.cpt; j++) {
             const I id = first + static_cast<I>(j) * static_cast<I>(blockDim.x);
-            if (id < n) fm_point<I>(a, id);
+            if (id < n) fm_point<I, true>(a, id);
         }
     }
 }
@@ -549,7 +562,7 @@ __global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __re
         const AfmStageArgs& a = items[4 * inst + stage - 1];
         const I n = static_cast<I>(a.nx * a.ny * a.nz);
         const I id = static_cast<I>(b - L.block0[inst]) * static_cast<I>(blockDim.x) + static_cast<I>(threadIdx.x);
-        if (id < n) afm_point<I>(a, id, static_cast<int>((w >> 3) & 1u));
+        if (id < n) afm_point<I, true>(a, id, static_cast<int>((w >> 3) & 1u));
         return;
     }
     for (uint32_t w = blockIdx.x; w < work; w += gridDim.x) {
@@ -563,7 +576,7 @@ __global__ void __launch_bounds__(256) afm_batch_kernel(const AfmStageArgs* __re
         const I first = static_cast<I>(b - L.block0[inst]) * chunk + static_cast<I>(threadIdx.x);
         for (int j = 0; j < L.cpt; j++) {
             const I id = first + static_cast<I>(j) * static_cast<I>(blockDim.x);
-            if (id < n) afm_point<I>(a, id, sub);
+            if (id < n) afm_point<I>(a, id, sub);   // lane gathers cost this loop a wave of occupancy
         }
     }
 }
