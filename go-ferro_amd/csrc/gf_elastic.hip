@@ -70,22 +70,55 @@ __device__ __forceinline__ void cell_ijk(int64_t id, int64_t ny, int64_t nz, int
 // differences are multiplied by the reciprocal spacings (within an ulp of
 // the reference's quotients); the upper face of a cell and the lower face
 // of the cell above it are the same expression, hence the same value.
+// The cell's face gradients from the u store.  Every load is issued before
+// any code is known (all indices are in bounds), so that one memory round
+// trip feeds the cell; the launches that call it run consecutive cells in
+// consecutive lanes (one cell per thread, none past the grid's last cell),
+// so the z neighbours' values and codes come from the adjacent lanes (DPP
+// wave shifts) except at a column end or a wave edge, where they are
+// loaded.
 __device__ __forceinline__ void cell_grad(const ElStrainArgs& a, int64_t id, int64_t i, int64_t j, int64_t k,
                                           double (&g)[6][3]) {
     const int64_t nx = a.nx, ny = a.ny, nz = a.nz;
-    const bool me = solid(a.code, id);
     const int64_t nb[6] = {((i == 0 ? nx - 1 : i - 1) * ny + j) * nz + k, ((i == nx - 1 ? 0 : i + 1) * ny + j) * nz + k,
                            (i * ny + (j == 0 ? ny - 1 : j - 1)) * nz + k, (i * ny + (j == ny - 1 ? 0 : j + 1)) * nz + k,
                            (i * ny + j) * nz + (k == 0 ? nz - 1 : k - 1), (i * ny + j) * nz + (k == nz - 1 ? 0 : k + 1)};
+    const uint32_t own = a.code[id];
+    double u[3], un[6][3];
+    uint32_t cn[6];
+    for (int c = 0; c < 3; c++) u[c] = a.us[c][id];
+    for (int f = 0; f < 4; f++) {
+        for (int c = 0; c < 3; c++) un[f][c] = a.us[c][nb[f]];
+        cn[f] = a.code[nb[f]];
+    }
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const bool gm = lane == 0 || k == 0, gp = lane == 63 || k == nz - 1;
+    if (gm) {
+        for (int c = 0; c < 3; c++) un[4][c] = a.us[c][nb[4]];
+        cn[4] = a.code[nb[4]];
+    }
+    if (gp) {
+        for (int c = 0; c < 3; c++) un[5][c] = a.us[c][nb[5]];
+        cn[5] = a.code[nb[5]];
+    }
+    for (int c = 0; c < 3; c++) {
+        const double dn = lane_up(u[c]), up = lane_down(u[c]);
+        if (!gm) un[4][c] = dn;
+        if (!gp) un[5][c] = up;
+    }
+    const uint32_t cdn = lane_up(own), cup = lane_down(own);
+    if (!gm) cn[4] = cdn;
+    if (!gp) cn[5] = cup;
+    const bool me = (own & kMatMask) != 0u;
     const bool wall[6] = {i == 0 && !a.per[0], i == nx - 1 && !a.per[0], j == 0 && !a.per[1],
                           j == ny - 1 && !a.per[1], k == 0 && !a.per[2], k == nz - 1 && !a.per[2]};
     const double ih[3] = {a.idx, a.idy, a.idz};
     for (int f = 0; f < 6; f++) {
-        const bool fr = wall[f] || !(me && solid(a.code, nb[f]));
+        const bool fr = wall[f] || !(me && (cn[f] & kMatMask) != 0u);
         for (int c = 0; c < 3; c++) {
             if (fr) g[f][c] = 0.;
-            else if (f & 1) g[f][c] = (a.us[c][nb[f]] - a.us[c][id]) * ih[f >> 1];
-            else g[f][c] = (a.us[c][id] - a.us[c][nb[f]]) * ih[f >> 1];
+            else if (f & 1) g[f][c] = (un[f][c] - u[c]) * ih[f >> 1];
+            else g[f][c] = (u[c] - un[f][c]) * ih[f >> 1];
         }
     }
 }

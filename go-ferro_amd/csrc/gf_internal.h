@@ -99,6 +99,25 @@ constexpr double kCLight = 2.99792458e8;      // EMdynamic_system.h:14
 // Yee grids: E on edges, Ex nx(ny+1)(nz+1), Ey (nx+1)ny(nz+1), Ez
 // (nx+1)(ny+1)nz; H on faces, Hx (nx+1)ny nz, Hy nx(ny+1)nz, Hz nx ny(nz+1);
 // each z fastest in its own dimensions.
+// A double moved one lane up (lane i gets lane i - 1) or down (lane i
+// gets lane i + 1) across the 64-lane wave, with DPP wave shifts.
+__device__ __forceinline__ double lane_up(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_down(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ uint32_t lane_up(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint32_t lane_down(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xf, 0xf, false));
+}
+
 struct EmArgs {
     const uint32_t* code;
     const DevMaterial* mats;

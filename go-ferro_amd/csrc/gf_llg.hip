@@ -30,25 +30,6 @@ constexpr int kSetupChunk = 4;
 static_assert(kLeanMats * sizeof(DevMaterial) / sizeof(double) <= kMatWordsPerThread * 256,
               "the lean stage's material records must fit the set-up loads of 256 threads");
 
-// A double moved one lane up (lane i gets lane i - 1) or down (lane i
-// gets lane i + 1) across the 64-lane wave, with DPP wave shifts.
-__device__ __forceinline__ double lane_up(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double lane_down(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ uint32_t lane_up(uint32_t v) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xf, 0xf, false));
-}
-__device__ __forceinline__ uint32_t lane_down(uint32_t v) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xf, 0xf, false));
-}
-
 // gather6 for launches whose active lanes hold consecutive cells along z
 // (lane i of a wave at cell id0 + i, and a lane without a cell only past
 // the grid's last cell): a cell's z neighbours are then the store values
