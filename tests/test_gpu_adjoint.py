@@ -160,13 +160,16 @@ def test_stage_vjp_against_oracle_differences(nadv, rotate, demag, dmi):
     g.close()
 
 
-def two_material_problem(shape=(8, 8, 4), rotate=True, demag=True):
-    """Two ferromagnets (different Ms, Aex, damping, B1, anisotropy) in
-    stripes along x, a non-magnetic pocket, the same drives as problem()."""
+def two_material_problem(shape=(8, 8, 4), rotate=True, demag=True, dmi=False):
+    """Two ferromagnets (different Ms, Aex, damping, B1, anisotropy and,
+    with dmi, DMI of opposite signs) in stripes along x, a non-magnetic
+    pocket, the same drives as problem()."""
     prob = problem(shape=shape, rotate=rotate, demag=demag)
     a = prob.materials[0]
     b = Material.make(if_FM=1, anisotropy_type=2, Ms=8e5, gyro=GYRO, FM_damping=0.05, K1=4e4, K2=0.0, Aex=1.1e-11,
                       B1=-6e5, B2=3e5)
+    if dmi:
+        a.iDMI, b.iDMI = 1.2e-3, -7e-4
     vac = Material.make()
     cell = np.ones(shape, np.uint32)
     cell[shape[0] // 2:, :, :] = 2
@@ -175,14 +178,18 @@ def two_material_problem(shape=(8, 8, 4), rotate=True, demag=True):
     return dataclasses.replace(prob, materials=[a, b, vac], material_cell=cell)
 
 
-@pytest.mark.parametrize("material,rotate,demag", [(1, True, True), (2, False, True), (2, True, False)])
-def test_stage_vjp_two_materials(material, rotate, demag):
+@pytest.mark.parametrize("material,rotate,demag,dmi", [(1, True, True, False), (2, False, True, False),
+                                                      (2, True, False, False), (1, True, True, True),
+                                                      (2, False, False, True)])
+def test_stage_vjp_two_materials(material, rotate, demag, dmi):
     """The stage VJP with two ferromagnets: <ybar, v> against the oracle's
     central differences, and the gradients of each material's alpha, Aex
-    (its own faces and the harmonic-mean interface) and B1."""
+    (its own faces and the harmonic-mean interface) and B1; with DMI of
+    opposite signs in the two, the neighbours' DMI terms across the
+    interface carry each neighbour's own D."""
     from oracle.oracle import FIELD_K1
-    say("case stage vjp two materials", material, rotate, demag)
-    prob = two_material_problem(rotate=rotate, demag=demag)
+    say("case stage vjp two materials", material, rotate, demag, dmi)
+    prob = two_material_problem(rotate=rotate, demag=demag, dmi=dmi)
     f = inputs(prob.shape)
     magnet = np.asarray(prob.material_cell) < 3
     f[FIELD_M] = np.where(magnet[None], f[FIELD_M], 0.0)

@@ -272,6 +272,40 @@ def test_elastostatics_beside_elastodynamics_restart():
     c.close()
 
 
+def test_afm_elastostatics_beside_elastodynamics():
+    """An antiferromagnet whose static strain is solved once at start-up and
+    stepped with elastodynamics acting back: three steps against the oracle,
+    the static strain kept."""
+    from goferro import FIELD_U, FIELD_V, FLAG_ELASTO_BACKACTION
+    say("case AFM elastostatics beside elastodynamics")
+    prob, m1, m2 = workloads.afm_block(shape=(8, 8, 8))
+    mat = prob.materials[0]
+    set_elastic(mat, 2.5e11, 1.3e11, 0.8e11, 6800.0)
+    mat.lamda100_AFM1, mat.lamda100_AFM2, mat.lamda111_AFM1, mat.lamda111_AFM2 = -2e-5, -1e-5, 1e-5, 3e-5
+    mat.B1_AFM1 = mat.B1_AFM2 = 6e5
+    mat.B2_AFM1 = mat.B2_AFM2 = 4e5
+    prob.flags |= FLAG_ELASTOSTATIC | FLAG_ELASTODYNAMICS | FLAG_ELASTO_BACKACTION | FLAG_ELASTO_ON_PM
+    rng = np.random.default_rng(53)
+    f = {FIELD_M_AFM1: m1, FIELD_M_AFM2: m2, FIELD_U: 1e-12 * rng.standard_normal((3,) + prob.shape),
+         FIELD_V: 0.5 * rng.standard_normal((3,) + prob.shape)}
+    g, o = pair(prob, f)
+    g.demag()
+    o.demag()
+    assert rel(strain(g), strain(o)) <= 1e-12
+    s0 = strain(g)
+    g.step(3)
+    o.step(3)
+    for fid in (FIELD_M_AFM1, FIELD_M_AFM2):
+        assert np.abs(g.download(fid) - o.get(fid)).max() <= 1e-12, fid
+    for fid in (FIELD_U, FIELD_V):
+        d = rel(g.download(fid), o.get(fid))
+        say("  rel", fid, d)
+        assert d <= 1e-12, fid
+    assert np.array_equal(strain(g), s0)
+    g.close()
+    o.close()
+
+
 def test_elastostatic_rejections():
     say("case elastostatic rejections")
     from goferro import FLAG_FLEXO
