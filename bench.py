@@ -305,7 +305,9 @@ def bench_adjoint(shape, nsteps: int) -> dict:
                           "(checkpointed forward + reverse sweep) / plain forward, the whole gradient's cost",
             "forward_cell_steps_per_s": n * nsteps / t_plain, "loss": L,
             "grad": {"alpha": grad[0], "Aex": grad[1], "B1": grad[2]},
-            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5), "resident": resident, "fp32": fp32}
+            "grad_drive_norm": float((ghd ** 2).sum() ** 0.5),
+            "gradient_over_forward_resident": resident.get("gradient_over_forward"),
+            "resident": resident, "fp32": fp32}
 
 
 def bench_coupled(which: str, steps: int) -> dict:
