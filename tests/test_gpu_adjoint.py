@@ -3,7 +3,7 @@ product against central differences of the oracle's stage increment, the
 reverse-sweep gradients against the oracle's tangent-linear reference
 (itself pinned by finite differences on the CPU) and against central finite
 differences of the GPU forward, checkpoint invariance, state restoration and
-the torch.autograd binding.  The bar is 1e-6 relative (the north-star
+the torch.autograd binding.  The bar is 1e-6 relative (the stated
 gradient agreement); most checks land far below it.
 """
 import dataclasses

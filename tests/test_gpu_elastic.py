@@ -2,7 +2,7 @@
 through the C ABI, against the CPU oracle on the same seeded inputs.
 
 Displacement and velocity are compared relative to their peak (1e-12 per
-stage, 1e-9 after 1000 steps); m absolutely (|m| ~ 1).
+stage, 1e-9 after 1000 steps, the parity bounds); m absolutely (|m| ~ 1).
 Every case prints its name and shape first so a failing GPU step names
 itself.
 """
@@ -155,7 +155,7 @@ def test_magnetoelastic_stages(rotate):
 
 
 def test_magnetoelastic_1000_steps():
-    """The north-star bounds after 1000 coupled steps through gf_step."""
+    """The parity bounds after 1000 coupled steps through gf_step."""
     from oracle.oracle import Oracle
     prob, m = magnetoelastic(shape=(8, 8, 8), layers=3)
     u, v = random_uv(prob.shape, 7)

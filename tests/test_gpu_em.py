@@ -3,7 +3,7 @@ and current sources, Drude currents) and its couplings to p and m, through
 the C ABI, against the CPU oracle on the same seeded inputs.
 
 E and H are compared relative to their peak, 1e-12 per stage and 1e-9
-after 1000 steps; p and m likewise.  Every case prints
+after 1000 steps (the parity bounds); p and m likewise.  Every case prints
 its name first so a failing GPU step names itself.
 """
 import sys
@@ -194,7 +194,7 @@ def test_afm_em_stages():
 
 
 def test_fe_em_1000_steps():
-    """North-star bounds after 1000 coupled steps through gf_step."""
+    """The parity bounds after 1000 coupled steps through gf_step."""
     from oracle.oracle import Oracle
     prob, p, q = fe_film(shape=(2, 2, 64))
     say("case fe+em 1000 steps", prob.shape)

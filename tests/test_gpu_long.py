@@ -1,7 +1,7 @@
 """1000 coupled steps through gf_step for the couplings that act inside the
 stages (spin pumping into the electrodynamics, flexoelectricity with
 elastodynamics, the elastostatics acting on the magnets), against the
-oracle at the north-star bound of 1e-9 of each field's peak.
+oracle at the parity bound of 1e-9 of each field's peak.
 """
 import sys
 

@@ -1,6 +1,6 @@
 """GPU parity: the gfx950 library against the CPU oracle, through the C ABI.
 
-Tolerances are the north-star ones: 1e-12 per stage/step on m (|m| ~ 1, so
+Tolerances are the parity bounds: 1e-12 per stage/step on m (|m| ~ 1, so
 absolute), 1e-9 on m after 1000 steps; the magnetostatic field is compared
 relative to its peak (1e-12).  Every case prints its name and shape first so
 a failing GPU step names itself.
@@ -232,7 +232,7 @@ def test_fe_and_fm_step_together():
 
 
 def test_fm_film_1000_steps():
-    """The north-star bound after 1000 steps, at a size the oracle runs in seconds."""
+    """The parity bound after 1000 steps, at a size the oracle runs in seconds."""
     from oracle.oracle import Oracle
     shape = (16, 16, 8)
     say("case 1000 steps", shape)
