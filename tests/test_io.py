@@ -464,3 +464,52 @@ def test_state_info_rejects_foreign_files(tmp_path):
     (tmp_path / "x.bin").write_bytes(b"not an image" * 20)
     with pytest.raises(GoFerroError, match="not a goferro state image"):
         gio.state_info(tmp_path / "x.bin")
+
+
+class _CallLog(_RecordingContext):
+    """A Context stand-in for run(): records the order of uploads, statics
+    and steps (the deck runner's host logic only)."""
+
+    def __init__(self, problem):
+        super().__init__(problem.shape)
+        self.calls = []
+        self.time = 0.0
+
+    def upload(self, field, value):
+        super().upload(field, value)
+        self.calls.append(("upload", field))
+
+    def demag(self):
+        self.calls.append(("demag",))
+
+    def step(self, n):
+        self.calls.append(("step", n))
+
+    def prescribe_m(self):
+        self.calls.append(("prescribe",))
+
+
+@pytest.mark.parametrize("with_input", [False, True])
+def test_run_orders_the_start_up_static_strain(tmp_path, monkeypatch, with_input):
+    """With elastodynamics and if_elastostatic, run() calls gf_demag (which
+    solves the static strain once) after load_inputs has uploaded any static
+    strain read (main.cpp:78-80, 148-158): the upload comes first, so the
+    library skips the solve; without the input the first gf_demag solves."""
+    from goferro import FIELD_STRAIN_T0_DIAG
+    edits = [("if_elastodynamics if_elasto_backaction_from_pORm if_elasto_on_pORm", "true true true"),
+             ("if_elastostatic if_elastostatic_1D", "true false")]
+    if with_input:
+        edits.append(("if_input_uandv", "false true false"))
+    d = deck_copy(tmp_path, edits=edits)
+    if with_input:
+        gio.write_cells(d / "strain_stat.in", np.full((6, 8, 8, 4), 1e-5))
+    made = []
+    monkeypatch.setattr(gio, "Context", lambda prob: made.append(_CallLog(prob)) or made[-1])
+    gio.run(d, steps=2, out_dir=tmp_path / "out")
+    calls = made[0].calls
+    first_demag = calls.index(("demag",))
+    uploads = [i for i, c in enumerate(calls) if c == ("upload", FIELD_STRAIN_T0_DIAG)]
+    assert bool(uploads) == with_input
+    if with_input:
+        assert uploads[0] < first_demag
+    assert any(c[0] == "step" for c in calls[first_demag:])
